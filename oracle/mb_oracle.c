@@ -1,0 +1,732 @@
+/*
+ * mb_oracle.c -- CPU restatement of minibwa 0.7-r421's seeding + extension hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Linked by tests/, smoke() and
+ * bench.py's cpu_baseline leg; the product never links it.
+ *
+ * What is restated, and what pins it (each pin is a test in tests/test_oracle.py):
+ *   FM-index load/rank/extend  bwt.c:176-250,648-683   rank: brute-force count over the decoded
+ *                                                      BWT; the `bench -b 2a` checksum on chrM
+ *   SA locate                  bwt.c:469-541           brute-force suffix sort of tiny texts;
+ *                                                      `bench -b sa` / `-b msa` checksums on chrM
+ *   10-mer cache               bwt.c:552-591           equals stepwise extension of every 10-mer
+ *   SMEM + two-pass seeding    bwt.c:253-305, seed.c:24-54
+ *                                                      brute-force (l,c)-SMEM definition on tiny
+ *                                                      references; the compiled reference
+ *   ksw_extd2 / ksw_extz2      ksw2_extd2_sse.c:27-403, ksw2_extz2_sse.c:16-296, ksw2.h:124-197
+ *                                                      textbook Gotoh global score; the compiled
+ *                                                      reference on fuzzed jobs (all flag variants)
+ *
+ * The DP restatement is scalar but keeps the SSE code's observable semantics: int8 wrap-around
+ * arithmetic, the 16-cell-aligned band (cells outside the exact band are computed and kept), the
+ * byte layout of the scratch block (stale scores, reads past the target into the reversed query),
+ * the 4-lane argmax tie-break of the exact-max pass, and the rotated backtrack matrix.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "mb_oracle.h"
+
+#define M56 ((1ULL << 56) - 1)
+
+/*******************
+ * FM-index access *
+ *******************/
+
+static uint64_t orc_data_len(uint64_t len) /* bwt.c:52-58 */
+{
+	uint64_t nb = (len + 127) / 128;
+	return nb * 4 + (nb + 1) * 4;
+}
+
+orc_bwt_t *orc_bwt_load(const char *fn) /* bwt.c:648-683 */
+{
+	FILE *fp;
+	char magic[4];
+	uint64_t hdr[5];
+	orc_bwt_t *b;
+	if ((fp = fopen(fn, "rb")) == 0) return 0;
+	if (fread(magic, 1, 4, fp) != 4 || memcmp(magic, "MBW\2", 4) != 0) { fclose(fp); return 0; }
+	b = (orc_bwt_t*)calloc(1, sizeof(*b));
+	if (fread(&b->sa_bit, 4, 1, fp) != 1 || fread(hdr, 8, 5, fp) != 5) goto fail;
+	b->primary = hdr[0];
+	b->L2[0] = 0;
+	memcpy(&b->L2[1], &hdr[1], 32);
+	b->seq_len = b->L2[4];
+	b->data_len = orc_data_len(b->seq_len);
+	b->data = (uint64_t*)malloc(b->data_len * 8);
+	if (fread(b->data, 8, b->data_len, fp) != b->data_len) goto fail;
+	if (fread(&b->n_sa, 8, 1, fp) != 1) goto fail;
+	if (b->sa_bit != (uint32_t)-1 && b->n_sa > 0) {
+		if (b->n_sa != ((b->seq_len + (1ULL << b->sa_bit)) >> b->sa_bit)) goto fail;
+		b->sa = (uint64_t*)malloc(b->n_sa * 8);
+		if (fread(b->sa, 8, b->n_sa, fp) != b->n_sa) goto fail;
+	}
+	fclose(fp);
+	return b;
+fail:
+	fclose(fp);
+	orc_bwt_destroy(b);
+	return 0;
+}
+
+void orc_bwt_destroy(orc_bwt_t *b)
+{
+	if (b == 0) return;
+	free(b->data); free(b->sa); free(b->pre); free(b);
+}
+
+/* number of symbol c among the first n (0..32) 2-bit symbols of w */
+static inline uint64_t sym_count(uint64_t w, int c, int n)
+{
+	static const uint64_t pat[4] = { 0, 0x5555555555555555ULL, 0xAAAAAAAAAAAAAAAAULL, ~0ULL };
+	uint64_t e = ~(w ^ pat[c]);
+	e = e & (e >> 1) & 0x5555555555555555ULL;
+	if (n < 32) e &= (1ULL << (2 * n)) - 1;
+	return (uint64_t)__builtin_popcountll(e);
+}
+
+/* occurrences of each base among $-removed BWT symbols [0, j] (inclusive) -- bwt.c:160-193 */
+static void occ_incl(const orc_bwt_t *b, uint64_t j, uint64_t cnt[4])
+{
+	const uint64_t *p = b->data + (j >> 7 << 3);
+	int c, in = (int)(j & 127), half = in >= 64, n = (in & 63) + 1;
+	for (c = 0; c < 4; ++c) {
+		uint64_t v = (p[c] & M56) + (half ? p[c] >> 56 : 0);
+		const uint64_t *w = p + 4 + (half ? 2 : 0);
+		v += n > 32 ? sym_count(w[0], c, 32) + sym_count(w[1], c, n - 32) : sym_count(w[0], c, n);
+		cnt[c] = v;
+	}
+}
+
+void orc_rank1a(const orc_bwt_t *b, uint64_t k, uint64_t cnt[4]) /* bwt.c:176-193 */
+{
+	if (k == 0) { memset(cnt, 0, 32); return; }
+	--k;
+	k -= (k >= b->primary);
+	occ_incl(b, k, cnt);
+}
+
+static inline int bwt_b0(const orc_bwt_t *b, uint64_t k) /* bwt.c:119 */
+{
+	return b->data[(k >> 7 << 3) + 4 + ((k & 127) >> 5)] >> ((k & 31) << 1) & 3;
+}
+
+void orc_rank2a(const orc_bwt_t *b, uint64_t k, uint64_t l, uint64_t ck[4], uint64_t cl[4]) /* bwt.c:195-231 */
+{
+	orc_rank1a(b, k, ck);
+	/* bwt.c:205-209: the l==k+1 shortcut adds the symbol at $-removed index k-(k>primary).  For
+	 * k==primary that row holds '$', so the shortcut differs from rank1a(l): keep its result. */
+	if (l == k + 1 && k != 0 && k == b->primary) {
+		memcpy(cl, ck, 32);
+		++cl[bwt_b0(b, k)];
+	} else orc_rank1a(b, l, cl);
+}
+
+void orc_extend(const orc_bwt_t *b, const orc_sai_t *ik, orc_sai_t ok[4], int is_back) /* bwt.c:237-250 */
+{
+	uint64_t tk[4], tl[4];
+	int i, f = !is_back, g = is_back;
+	orc_rank2a(b, ik->x[f], ik->x[f] + ik->size, tk, tl);
+	for (i = 0; i < 4; ++i) {
+		ok[i].x[f] = b->L2[i] + 1 + tk[i];
+		ok[i].size = tl[i] - tk[i];
+		ok[i].info = 0;
+	}
+	ok[3].x[g] = ik->x[g] + (ik->x[f] <= b->primary && ik->x[f] + ik->size > b->primary);
+	ok[2].x[g] = ok[3].x[g] + ok[3].size;
+	ok[1].x[g] = ok[2].x[g] + ok[2].size;
+	ok[0].x[g] = ok[1].x[g] + ok[1].size;
+}
+
+static void set_intv(const orc_bwt_t *b, int c, orc_sai_t *ik) /* bwt.h:75-81 */
+{
+	ik->x[0] = b->L2[c] + 1;
+	ik->x[1] = b->L2[3 - c] + 1;
+	ik->size = b->L2[c + 1] - b->L2[c];
+	ik->info = 0;
+}
+
+void orc_bwt_cache(orc_bwt_t *b, int32_t len) /* bwt.c:552-591: interval of every len-mer */
+{
+	uint64_t z, n = 1ULL << (2 * len);
+	free(b->pre);
+	b->pre_len = len;
+	b->pre = (orc_sai_t*)calloc(n, sizeof(orc_sai_t));
+	for (z = 0; z < n; ++z) { /* key: last base in the top two bits, first base in the lowest */
+		orc_sai_t p, ok[4];
+		int d;
+		set_intv(b, (int)(z >> (2 * (len - 1)) & 3), &p);
+		for (d = len - 2; d >= 0; --d) {
+			orc_extend(b, &p, ok, 1);
+			p = ok[z >> (2 * d) & 3];
+		}
+		b->pre[z] = p;
+	}
+}
+
+/********
+ * SMEM *
+ ********/
+
+static int64_t smem_back(const orc_bwt_t *f, const uint8_t *q, int64_t st, int64_t pos, int64_t min_occ, orc_sai_t *p) /* bwt.c:253-277 */
+{
+	int64_t i = pos - 1;
+	orc_sai_t ok[4];
+	if (f->pre && pos - st >= f->pre_len) {
+		uint64_t z = 0, l = 0;
+		for (i = pos; l < (uint64_t)f->pre_len; --i, ++l)
+			z = z << 2 | q[i];
+		*p = f->pre[z];
+	} else p->size = 0;
+	if (p->size < (uint64_t)min_occ) {
+		set_intv(f, q[pos], p);
+		i = pos - 1;
+	}
+	for (; i >= st; --i) {
+		int c = q[i];
+		if (c > 3) break;
+		orc_extend(f, p, ok, 1);
+		if (ok[c].size < (uint64_t)min_occ) break;
+		*p = ok[c];
+	}
+	return i;
+}
+
+int64_t orc_smem(const orc_bwt_t *f, uint32_t len, const uint8_t *q, int64_t x, int64_t min_len, int64_t min_occ, orc_sai_t *p) /* bwt.c:280-305 */
+{
+	int64_t i, j, xn;
+	orc_sai_t ik, ok[4];
+	p->size = ik.size = 0;
+	if ((int64_t)len - x < min_len) return len;
+	for (i = x, xn = -1; i < x + min_len; ++i)
+		if (q[i] > 3) xn = i;
+	if (xn >= 0) return xn + 1;
+	i = smem_back(f, q, x, x + min_len - 1, min_occ, &ik);
+	if (i >= x) return i + 1;
+	for (j = x + min_len; j < (int64_t)len; ++j) {
+		int c = 3 - q[j];
+		if (q[j] > 3) break;
+		orc_extend(f, &ik, ok, 0);
+		if (ok[c].size < (uint64_t)min_occ) break;
+		ik = ok[c];
+	}
+	*p = ik;
+	p->info = (uint64_t)x << 32 | (uint64_t)j;
+	if (j == (int64_t)len) return len;
+	i = q[j] > 3 ? j : smem_back(f, q, x + 1, j, min_occ, &ik);
+	return i + 1;
+}
+
+/* seed.c:24-54; returns the number of intervals (written up to cap) */
+int64_t orc_seed_intv(const orc_bwt_t *b, int32_t len, const uint8_t *seq, int32_t min_len, int32_t max_sub_occ, orc_sai_t *out, int64_t cap)
+{
+	int64_t x = 0, n = 0, n0, i;
+	orc_sai_t p;
+	if (len <= 0) return 0;
+	do {
+		x = orc_smem(b, len, seq, x, min_len, 1, &p);
+		if (p.size > 0) { if (n < cap) out[n] = p; ++n; }
+	} while (x < len);
+	n0 = n < cap ? n : cap;
+	for (i = 0; i < n0; ++i) {
+		uint32_t st = out[i].info >> 32, en = (uint32_t)out[i].info;
+		int32_t sub;
+		if (en - st < (uint32_t)min_len * 2 || out[i].size > (uint64_t)max_sub_occ) continue;
+		x = st;
+		sub = (int32_t)(en - st) / 2 > min_len ? (int32_t)(en - st) / 2 : min_len;
+		do {
+			x = orc_smem(b, en, seq, x, sub, out[i].size + 1, &p);
+			if (p.size > out[i].size) { if (n < cap) out[n] = p; ++n; }
+		} while (x < en);
+	}
+	return n;
+}
+
+/******
+ * SA *
+ ******/
+
+static inline uint64_t inv_psi(const orc_bwt_t *b, uint64_t k) /* bwt.c:469-475 */
+{
+	uint64_t x = k - (k > b->primary), cnt[4];
+	int c;
+	if (k == b->primary) return 0;
+	c = bwt_b0(b, x);
+	orc_rank1a(b, k, cnt);
+	return b->L2[c] + 1 + cnt[c];
+}
+
+uint64_t orc_sa(const orc_bwt_t *b, uint64_t k) /* bwt.c:499-509 */
+{
+	uint64_t s = 0, mask = (1ULL << b->sa_bit) - 1;
+	while (k & mask) { ++s; k = inv_psi(b, k); }
+	return s + b->sa[k >> b->sa_bit];
+}
+
+void orc_sa_batch(const orc_bwt_t *b, int64_t n, uint64_t *x) /* bwt.c:511-541: same values, in place */
+{
+	int64_t i;
+	for (i = 0; i < n; ++i) x[i] = orc_sa(b, x[i]);
+}
+
+/*************
+ * ksw2 DP   *
+ *************/
+
+#define NEG_INF (-0x40000000)
+
+typedef struct { int n, cap, overflow; uint32_t *a; } cig_t;
+
+static void cig_push(cig_t *c, uint32_t op, int len) /* ksw2.h:124-134 */
+{
+	if (c->n > 0 && (c->a[c->n - 1] & 0xf) == op) { c->a[c->n - 1] += (uint32_t)len << 4; return; }
+	if (c->n < c->cap) c->a[c->n] = (uint32_t)len << 4 | op;
+	else c->overflow = 1;
+	c->n++;
+}
+
+/* ksw2.h:140-172 with is_rot=1, min_intron_len=0 */
+static void backtrack(cig_t *c, int is_rev, const uint8_t *p, const int *off, const int *off_end, size_t n_col, int i0, int j0)
+{
+	int i = i0, j = j0, state = 0, k;
+	while (i >= 0 && j >= 0) {
+		int r = i + j, force = -1;
+		uint32_t tmp;
+		if (i < off[r]) force = 2;
+		if (i > off_end[r]) force = 1;
+		tmp = force < 0 ? p[(size_t)r * n_col + i - off[r]] : 0;
+		if (state == 0) state = tmp & 7;
+		else if (!(tmp >> (state + 2) & 1)) state = 0;
+		if (state == 0) state = tmp & 7;
+		if (force >= 0) state = force;
+		if (state == 0) cig_push(c, 0, 1), --i, --j;
+		else if (state == 1 || state == 3) cig_push(c, 2, 1), --i;
+		else cig_push(c, 1, 1), --j;
+	}
+	if (i >= 0) cig_push(c, 2, i + 1);
+	if (j >= 0) cig_push(c, 1, j + 1);
+	if (!is_rev && c->n <= c->cap)
+		for (k = 0; k < c->n >> 1; ++k) {
+			uint32_t t = c->a[k]; c->a[k] = c->a[c->n - 1 - k]; c->a[c->n - 1 - k] = t;
+		}
+}
+
+static void reset_ez(orc_ez_t *ez) /* ksw2.h:174-179 */
+{
+	ez->max_q = ez->max_t = ez->mqe_t = ez->mte_q = -1;
+	ez->max = 0, ez->score = ez->mqe = ez->mte = NEG_INF;
+	ez->n_cigar = 0, ez->zdropped = 0, ez->reach_end = 0;
+}
+
+static int apply_zdrop(orc_ez_t *ez, int32_t H, int r, int t, int zdrop, int e) /* ksw2.h:181-197, is_rot=1 */
+{
+	if (H > ez->max) {
+		ez->max = H & 0x7fffffff, ez->max_t = t, ez->max_q = r - t;
+	} else if (t >= ez->max_t && r - t >= ez->max_q) {
+		int tl = t - ez->max_t, ql = (r - t) - ez->max_q, l = tl > ql ? tl - ql : ql - tl;
+		if (zdrop >= 0 && ez->max - H > zdrop + l * e) { ez->zdropped = 1; return 1; }
+	}
+	return 0;
+}
+
+static inline int8_t i8max(int8_t a, int8_t b) { return a > b ? a : b; }
+static inline int8_t i8min(int8_t a, int8_t b) { return a < b ? a : b; }
+static inline uint8_t u8max(uint8_t a, uint8_t b) { return a > b ? a : b; }
+static inline uint8_t u8min(uint8_t a, uint8_t b) { return a < b ? a : b; }
+
+/* exact-max pass of ksw2_extd2_sse.c:318-351 / ksw2_extz2_sse.c:219-253: the argmax follows the
+ * 4-lane SSE reduction (H[en0] seeded first; lanes t=st0+i mod 4 keep their first strict max;
+ * lanes merged in order 0..3; then the scalar tail [en1,en0)). `sub` is subtracted per cell. */
+static void exact_max(int32_t *H, const uint8_t *vb, const uint8_t *ub, int is_signed, int st0, int en0, int sub, int32_t *max_H_, int32_t *max_t_)
+{
+	int32_t max_H, max_t, HH[4], tt[4], t, i, en1 = st0 + (en0 - st0) / 4 * 4;
+#define VAL(arr, k) (is_signed ? (int32_t)(int8_t)(arr)[k] : (int32_t)(arr)[k])
+	max_H = H[en0] = en0 > 0 ? H[en0 - 1] + VAL(ub, en0) - sub : H[en0] + VAL(vb, en0) - sub;
+	max_t = en0;
+	for (i = 0; i < 4; ++i) HH[i] = max_H, tt[i] = max_t;
+	for (t = st0; t < en1; t += 4)
+		for (i = 0; i < 4; ++i) {
+			H[t + i] += VAL(vb, t + i) - sub;
+			if (H[t + i] > HH[i]) HH[i] = H[t + i], tt[i] = t;
+		}
+	for (i = 0; i < 4; ++i)
+		if (max_H < HH[i]) max_H = HH[i], max_t = tt[i] + i;
+	for (; t < en0; ++t) {
+		H[t] += VAL(vb, t) - sub;
+		if (H[t] > max_H) max_H = H[t], max_t = t;
+	}
+#undef VAL
+	*max_H_ = max_H, *max_t_ = max_t;
+}
+
+int64_t orc_dp_cells(int qlen, int tlen, int w)
+{
+	int64_t n = 0;
+	int r;
+	if (w < 0) w = tlen > qlen ? tlen : qlen;
+	for (r = 0; r < qlen + tlen - 1; ++r) {
+		int st = 0, en = tlen - 1;
+		if (st < r - qlen + 1) st = r - qlen + 1;
+		if (en > r) en = r;
+		if (st < (r - w + 1) >> 1) st = (r - w + 1) >> 1;
+		if (en > (r + w) >> 1) en = (r + w) >> 1;
+		if (st > en) break;
+		n += en - st + 1;
+	}
+	return n;
+}
+
+/* ksw2_extd2_sse.c:27-403.  Returns the number of cells of the exact band processed. */
+int orc_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m, const int8_t *mat,
+              int8_t q, int8_t e, int8_t q2, int8_t e2, int w, int zdrop, int end_bonus, int flag,
+              orc_ez_t *ez, uint32_t *cigar, int cigar_cap)
+{
+	int r, t, n_col_, tlen_, qlen_, last_st, last_en, last_max_H = 0, max_sc, min_sc, long_thres, long_diff, cells = 0;
+	int with_cigar = !(flag & 0x01), approx_max = !!(flag & 0x08), right = !!(flag & 0x02), generic = !!(flag & 0x04);
+	int32_t *H = 0, H0 = 0, last_H0_t = 0;
+	int8_t qe, qe2, sc_mch, sc_mis, sc_N, m1;
+	uint8_t *mem, *sf, *qr, *p = 0;
+	int8_t *u, *v, *x, *y, *x2, *y2, *s;
+	int *off = 0, *off_end = 0;
+	cig_t cg = { 0, cigar_cap, 0, cigar };
+
+	reset_ez(ez);
+	if (m <= 1 || qlen <= 0 || tlen <= 0) return 0;
+	if (q2 + e2 < q + e) { int8_t tq = q; q = q2; q2 = tq; tq = e; e = e2; e2 = tq; }
+	qe = (int8_t)(q + e), qe2 = (int8_t)(q2 + e2);
+	sc_mch = mat[0], sc_mis = mat[1];
+	sc_N = mat[m * m - 1] == 0 ? (int8_t)-e2 : mat[m * m - 1];
+	m1 = (int8_t)(m - 1);
+	if (w < 0) w = tlen > qlen ? tlen : qlen;
+	tlen_ = (tlen + 15) / 16;
+	n_col_ = qlen < tlen ? qlen : tlen;
+	n_col_ = ((n_col_ < w + 1 ? n_col_ : w + 1) + 15) / 16 + 1;
+	qlen_ = (qlen + 15) / 16;
+	for (t = 1, max_sc = mat[0], min_sc = mat[1]; t < m * m; ++t) {
+		max_sc = max_sc > mat[t] ? max_sc : mat[t];
+		min_sc = min_sc < mat[t] ? min_sc : mat[t];
+	}
+	if (-min_sc > 2 * (q + e)) return 0;
+	long_thres = e != e2 ? (q2 - q) / (e - e2) - 1 : 0;
+	if (q2 + e2 + long_thres * e2 > q + e + long_thres * e) ++long_thres;
+	long_diff = long_thres * (e - e2) - (q2 - q) - e2;
+
+	/* one block laid out like the reference: u v x y x2 y2 s sf (tlen_*16 each) + qr */
+	mem = (uint8_t*)calloc((size_t)(tlen_ * 8 + qlen_ + 1) * 16, 1);
+	u = (int8_t*)mem; v = u + tlen_ * 16; x = v + tlen_ * 16; y = x + tlen_ * 16;
+	x2 = y + tlen_ * 16; y2 = x2 + tlen_ * 16; s = y2 + tlen_ * 16;
+	sf = (uint8_t*)(s + tlen_ * 16); qr = sf + tlen_ * 16;
+	memset(u, -q - e, (size_t)tlen_ * 16 * 4);
+	memset(x2, -q2 - e2, (size_t)tlen_ * 16 * 2);
+	if (!approx_max) {
+		H = (int32_t*)malloc((size_t)tlen_ * 16 * 4);
+		for (t = 0; t < tlen_ * 16; ++t) H[t] = NEG_INF;
+	}
+	if (with_cigar) {
+		p = (uint8_t*)malloc((size_t)(qlen + tlen - 1) * n_col_ * 16 + 16);
+		off = (int*)malloc((size_t)(qlen + tlen - 1) * sizeof(int) * 2);
+		off_end = off + qlen + tlen - 1;
+	}
+	for (t = 0; t < qlen; ++t) qr[t] = query[qlen - 1 - t];
+	memcpy(sf, target, tlen);
+
+	for (r = 0, last_st = last_en = -1; r < qlen + tlen - 1; ++r) {
+		int st = 0, en = tlen - 1, st0, en0, i;
+		int8_t x1, x21, v1;
+		const uint8_t *qrr = qr + (qlen - 1 - r);
+		if (st < r - qlen + 1) st = r - qlen + 1;
+		if (en > r) en = r;
+		if (st < (r - w + 1) >> 1) st = (r - w + 1) >> 1;
+		if (en > (r + w) >> 1) en = (r + w) >> 1;
+		if (st > en) { ez->zdropped = 1; break; }
+		st0 = st, en0 = en;
+		cells += en0 - st0 + 1;
+		st = st / 16 * 16, en = (en + 16) / 16 * 16 - 1;
+		if (st > 0) {
+			if (st - 1 >= last_st && st - 1 <= last_en) x1 = x[st - 1], x21 = x2[st - 1], v1 = v[st - 1];
+			else x1 = -q - e, x21 = -q2 - e2, v1 = -q - e;
+		} else {
+			x1 = -q - e, x21 = -q2 - e2;
+			v1 = r == 0 ? -q - e : r < long_thres ? -e : r == long_thres ? long_diff : -e2;
+		}
+		if (en >= r) {
+			y[r] = -q - e, y2[r] = -q2 - e2;
+			u[r] = r == 0 ? -q - e : r < long_thres ? -e : r == long_thres ? long_diff : -e2;
+		}
+		/* scores: 16-byte groups from st0, may run past en0 (and past s[] into sf[]) */
+		if (!generic) {
+			for (t = st0; t <= en0; t += 16) {
+				int8_t tmp[16];
+				for (i = 0; i < 16; ++i) {
+					int8_t a = (int8_t)sf[t + i], b = (int8_t)qrr[t + i];
+					tmp[i] = (a == m1 || b == m1) ? sc_N : a == b ? sc_mch : sc_mis;
+				}
+				memcpy(s + t, tmp, 16);
+			}
+		} else for (t = st0; t <= en0; ++t) s[t] = mat[sf[t] * m + qrr[t]];
+		{ /* core: every cell of the aligned range [st,en] from the previous anti-diagonal */
+			int n = en - st + 1;
+			int8_t *ox = (int8_t*)malloc(n * 6), *ov = ox + n, *ox2 = ov + n, *ou = ox2 + n, *oy = ou + n, *oy2 = oy + n;
+			uint8_t *pr = with_cigar ? p + (size_t)r * n_col_ * 16 : 0;
+			if (with_cigar) off[r] = st, off_end[r] = en;
+			for (i = 0; i < n; ++i) {
+				int tt = st + i;
+				ox[i] = i == 0 ? x1 : x[tt - 1];
+				ov[i] = i == 0 ? v1 : v[tt - 1];
+				ox2[i] = i == 0 ? x21 : x2[tt - 1];
+				ou[i] = u[tt]; oy[i] = y[tt]; oy2[i] = y2[tt];
+			}
+			for (i = 0; i < n; ++i) {
+				int tt = st + i;
+				int8_t z = s[tt], vt1 = ov[i], ut = ou[i], tmp, d = 0;
+				int8_t a = (int8_t)(ox[i] + vt1), b = (int8_t)(oy[i] + ut);
+				int8_t a2 = (int8_t)(ox2[i] + vt1), b2 = (int8_t)(oy2[i] + ut);
+				if (!with_cigar) {
+					z = i8max(z, a); z = i8max(z, b); z = i8max(z, a2); z = i8max(z, b2);
+				} else if (!right) {
+					d = a > z ? 1 : 0;      z = i8max(z, a);
+					d = b > z ? 2 : d;      z = i8max(z, b);
+					d = a2 > z ? 3 : d;     z = i8max(z, a2);
+					d = b2 > z ? 4 : d;     z = i8max(z, b2);
+				} else {
+					d = z > a ? 0 : 1;      z = i8max(z, a);
+					d = z > b ? d : 2;      z = i8max(z, b);
+					d = z > a2 ? d : 3;     z = i8max(z, a2);
+					d = z > b2 ? d : 4;     z = i8max(z, b2);
+				}
+				z = i8min(z, sc_mch);
+				u[tt] = (int8_t)(z - vt1);
+				v[tt] = (int8_t)(z - ut);
+				tmp = (int8_t)(z - q);  a = (int8_t)(a - tmp); b = (int8_t)(b - tmp);
+				tmp = (int8_t)(z - q2); a2 = (int8_t)(a2 - tmp); b2 = (int8_t)(b2 - tmp);
+				if (!with_cigar || !right) {
+					x[tt]  = (int8_t)((a > 0 ? a : 0) - qe);
+					y[tt]  = (int8_t)((b > 0 ? b : 0) - qe);
+					x2[tt] = (int8_t)((a2 > 0 ? a2 : 0) - qe2);
+					y2[tt] = (int8_t)((b2 > 0 ? b2 : 0) - qe2);
+					if (with_cigar) d |= (a > 0 ? 0x08 : 0) | (b > 0 ? 0x10 : 0) | (a2 > 0 ? 0x20 : 0) | (b2 > 0 ? 0x40 : 0);
+				} else {
+					x[tt]  = (int8_t)((a >= 0 ? a : 0) - qe);
+					y[tt]  = (int8_t)((b >= 0 ? b : 0) - qe);
+					x2[tt] = (int8_t)((a2 >= 0 ? a2 : 0) - qe2);
+					y2[tt] = (int8_t)((b2 >= 0 ? b2 : 0) - qe2);
+					d |= (a >= 0 ? 0x08 : 0) | (b >= 0 ? 0x10 : 0) | (a2 >= 0 ? 0x20 : 0) | (b2 >= 0 ? 0x40 : 0);
+				}
+				if (with_cigar) pr[i] = (uint8_t)d;
+			}
+			free(ox);
+		}
+		if (!approx_max) {
+			int32_t max_H, max_t;
+			if (r > 0) exact_max(H, (uint8_t*)v, (uint8_t*)u, 1, st0, en0, 0, &max_H, &max_t);
+			else H[0] = v[0] - qe, max_H = H[0], max_t = 0;
+			if (en0 == tlen - 1 && H[en0] > ez->mte) ez->mte = H[en0], ez->mte_q = r - en0;
+			if (r - st0 == qlen - 1 && H[st0] > ez->mqe) ez->mqe = H[st0], ez->mqe_t = st0;
+			if (apply_zdrop(ez, max_H, r, max_t, zdrop, e2)) break;
+			if (r == qlen + tlen - 2 && en0 == tlen - 1) ez->score = H[tlen - 1];
+			if (flag & 0x40) { /* ksw2_extd2_sse.c:361-367 */
+				int32_t rH = last_max_H > max_H ? last_max_H : max_H;
+				int32_t rq = qlen - (r - st0), rt = tlen - en0;
+				int32_t rm = rq >= tlen - st0 ? tlen - st0 : rt >= qlen - (r - en0) ? qlen - (r - en0) : tlen + qlen - 1 - r;
+				if (rH + rm * max_sc + end_bonus < ez->max) break;
+			}
+			last_max_H = max_H;
+		} else {
+			if (r > 0) {
+				if (last_H0_t >= st0 && last_H0_t <= en0 && last_H0_t + 1 >= st0 && last_H0_t + 1 <= en0) {
+					int32_t d0 = v[last_H0_t], d1 = u[last_H0_t + 1];
+					if (d0 > d1) H0 += d0;
+					else H0 += d1, ++last_H0_t;
+				} else if (last_H0_t >= st0 && last_H0_t <= en0) {
+					H0 += v[last_H0_t];
+				} else {
+					++last_H0_t, H0 += u[last_H0_t];
+				}
+			} else H0 = v[0] - qe, last_H0_t = 0;
+			if ((flag & 0x10) && apply_zdrop(ez, H0, r, last_H0_t, zdrop, e2)) break;
+			if (r == qlen + tlen - 2 && en0 == tlen - 1) ez->score = H0;
+		}
+		last_st = st, last_en = en;
+	}
+	if (with_cigar) {
+		int rev = !!(flag & 0x80);
+		if (!ez->zdropped && !(flag & 0x40)) {
+			backtrack(&cg, rev, p, off, off_end, (size_t)n_col_ * 16, tlen - 1, qlen - 1);
+		} else if ((flag & 0x40) && ez->mqe + end_bonus > ez->max) {
+			ez->reach_end = 1;
+			backtrack(&cg, rev, p, off, off_end, (size_t)n_col_ * 16, ez->mqe_t, qlen - 1);
+		} else if (ez->max_t >= 0 && ez->max_q >= 0) {
+			backtrack(&cg, rev, p, off, off_end, (size_t)n_col_ * 16, ez->max_t, ez->max_q);
+		}
+		ez->n_cigar = cg.n;
+	}
+	free(mem); free(H); free(p); free(off);
+	return cg.overflow ? -1 : cells;
+}
+
+/* ksw2_extz2_sse.c:16-296 (SSE4.1 code paths).  Returns the number of exact-band cells processed. */
+int orc_extz2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m, const int8_t *mat,
+              int8_t q, int8_t e, int w, int zdrop, int end_bonus, int flag,
+              orc_ez_t *ez, uint32_t *cigar, int cigar_cap)
+{
+	int r, t, n_col_, tlen_, qlen_, last_st, last_en, max_sc, min_sc, qe = q + e, cells = 0;
+	int with_cigar = !(flag & 0x01), approx_max = !!(flag & 0x08), right = !!(flag & 0x02), generic = !!(flag & 0x04);
+	int32_t *H = 0, H0 = 0, last_H0_t = 0;
+	int8_t qe2_, sc_mch, sc_mis, sc_N, m1, max_sc_;
+	uint8_t *mem, *sf, *qr, *p = 0, *u, *v, *x, *y, *s;
+	int *off = 0, *off_end = 0;
+	cig_t cg = { 0, cigar_cap, 0, cigar };
+
+	reset_ez(ez);
+	if (m <= 0 || qlen <= 0 || tlen <= 0) return 0;
+	qe2_ = (int8_t)((q + e) * 2);
+	sc_mch = mat[0], sc_mis = mat[1];
+	sc_N = mat[m * m - 1] == 0 ? (int8_t)-e : mat[m * m - 1];
+	m1 = (int8_t)(m - 1);
+	max_sc_ = (int8_t)(mat[0] + (q + e) * 2);
+	if (w < 0) w = tlen > qlen ? tlen : qlen;
+	tlen_ = (tlen + 15) / 16;
+	n_col_ = qlen < tlen ? qlen : tlen;
+	n_col_ = ((n_col_ < w + 1 ? n_col_ : w + 1) + 15) / 16 + 1;
+	qlen_ = (qlen + 15) / 16;
+	for (t = 1, max_sc = mat[0], min_sc = mat[1]; t < m * m; ++t) {
+		max_sc = max_sc > mat[t] ? max_sc : mat[t];
+		min_sc = min_sc < mat[t] ? min_sc : mat[t];
+	}
+	if (-min_sc > 2 * (q + e)) return 0;
+
+	mem = (uint8_t*)calloc((size_t)(tlen_ * 6 + qlen_ + 1) * 16, 1);
+	u = mem; v = u + tlen_ * 16; x = v + tlen_ * 16; y = x + tlen_ * 16; s = y + tlen_ * 16;
+	sf = s + tlen_ * 16; qr = sf + tlen_ * 16;
+	if (!approx_max) {
+		H = (int32_t*)malloc((size_t)tlen_ * 16 * 4);
+		for (t = 0; t < tlen_ * 16; ++t) H[t] = NEG_INF;
+	}
+	if (with_cigar) {
+		p = (uint8_t*)malloc((size_t)(qlen + tlen - 1) * n_col_ * 16 + 16);
+		off = (int*)malloc((size_t)(qlen + tlen - 1) * sizeof(int) * 2);
+		off_end = off + qlen + tlen - 1;
+	}
+	for (t = 0; t < qlen; ++t) qr[t] = query[qlen - 1 - t];
+	memcpy(sf, target, tlen);
+
+	for (r = 0, last_st = last_en = -1; r < qlen + tlen - 1; ++r) {
+		int st = 0, en = tlen - 1, st0, en0, i;
+		int8_t x1, v1;
+		const uint8_t *qrr = qr + (qlen - 1 - r);
+		if (st < r - qlen + 1) st = r - qlen + 1;
+		if (en > r) en = r;
+		if (st < (r - w + 1) >> 1) st = (r - w + 1) >> 1;
+		if (en > (r + w) >> 1) en = (r + w) >> 1;
+		if (st > en) { ez->zdropped = 1; break; }
+		st0 = st, en0 = en;
+		cells += en0 - st0 + 1;
+		st = st / 16 * 16, en = (en + 16) / 16 * 16 - 1;
+		if (st > 0) {
+			if (st - 1 >= last_st && st - 1 <= last_en) x1 = (int8_t)x[st - 1], v1 = (int8_t)v[st - 1];
+			else x1 = v1 = 0;
+		} else x1 = 0, v1 = r ? q : 0;
+		if (en >= r) y[r] = 0, u[r] = r ? (uint8_t)q : 0;
+		if (!generic) {
+			for (t = st0; t <= en0; t += 16) {
+				int8_t tmp[16];
+				for (i = 0; i < 16; ++i) {
+					int8_t a = (int8_t)sf[t + i], b = (int8_t)qrr[t + i];
+					tmp[i] = (a == m1 || b == m1) ? sc_N : a == b ? sc_mch : sc_mis;
+				}
+				memcpy(s + t, tmp, 16);
+			}
+		} else for (t = st0; t <= en0; ++t) s[t] = (uint8_t)mat[sf[t] * m + qrr[t]];
+		{
+			int n = en - st + 1;
+			uint8_t *ox = (uint8_t*)malloc(n * 4), *ov = ox + n, *ou = ov + n, *oy = ou + n;
+			uint8_t *pr = with_cigar ? p + (size_t)r * n_col_ * 16 : 0;
+			if (with_cigar) off[r] = st, off_end[r] = en;
+			for (i = 0; i < n; ++i) {
+				int tt = st + i;
+				/* x1/v1 enter through _mm_cvtsi32_si128 of an int8: a negative value also sets
+				 * bytes 1..3 of the first 16-cell group (ksw2_extz2_sse.c:138-139) */
+				if (i == 0) ox[i] = (uint8_t)x1, ov[i] = (uint8_t)v1;
+				else {
+					ox[i] = x[tt - 1], ov[i] = v[tt - 1];
+					if (i < 4 && x1 < 0) ox[i] = 0xff;
+					if (i < 4 && v1 < 0) ov[i] = 0xff;
+				}
+				ou[i] = u[tt], oy[i] = y[tt];
+			}
+			for (i = 0; i < n; ++i) {
+				int tt = st + i;
+				uint8_t vt1 = ov[i], ut = ou[i], d = 0;
+				int8_t z = (int8_t)(s[tt] + (uint8_t)qe2_), a = (int8_t)(ox[i] + vt1), b = (int8_t)(oy[i] + ut), zz;
+				uint8_t zu;
+				if (!with_cigar) {
+					z = i8max(z, a);
+				} else if (!right) {
+					d = a > z ? 1 : 0;
+					z = i8max(z, a);
+					d = b > z ? 2 : d;
+				} else {
+					d = z > a ? 0 : 1;
+					z = i8max(z, a);
+					d = z > b ? d : 2;
+				}
+				zu = u8max((uint8_t)z, (uint8_t)b);
+				zu = u8min(zu, (uint8_t)max_sc_);
+				u[tt] = (uint8_t)(zu - vt1);
+				v[tt] = (uint8_t)(zu - ut);
+				zz = (int8_t)(zu - (uint8_t)q);
+				a = (int8_t)(a - zz); b = (int8_t)(b - zz);
+				if (!with_cigar) {
+					x[tt] = (uint8_t)i8max(a, 0); y[tt] = (uint8_t)i8max(b, 0);
+				} else if (!right) {
+					x[tt] = a > 0 ? (uint8_t)a : 0; d |= a > 0 ? 0x08 : 0;
+					y[tt] = b > 0 ? (uint8_t)b : 0; d |= b > 0 ? 0x10 : 0;
+				} else {
+					x[tt] = 0 > a ? 0 : (uint8_t)a; d |= 0 > a ? 0 : 0x08;
+					y[tt] = 0 > b ? 0 : (uint8_t)b; d |= 0 > b ? 0 : 0x10;
+				}
+				if (with_cigar) pr[i] = d;
+			}
+			free(ox);
+		}
+		if (!approx_max) {
+			int32_t max_H, max_t;
+			if (r > 0) exact_max(H, v, u, 0, st0, en0, qe, &max_H, &max_t);
+			else H[0] = v[0] - qe - qe, max_H = H[0], max_t = 0;
+			if (en0 == tlen - 1 && H[en0] > ez->mte) ez->mte = H[en0], ez->mte_q = r - en0;
+			if (r - st0 == qlen - 1 && H[st0] > ez->mqe) ez->mqe = H[st0], ez->mqe_t = st0;
+			if (apply_zdrop(ez, max_H, r, max_t, zdrop, e)) break;
+			if (r == qlen + tlen - 2 && en0 == tlen - 1) ez->score = H[tlen - 1];
+		} else {
+			if (r > 0) {
+				if (last_H0_t >= st0 && last_H0_t <= en0 && last_H0_t + 1 >= st0 && last_H0_t + 1 <= en0) {
+					int32_t d0 = v[last_H0_t] - qe, d1 = u[last_H0_t + 1] - qe;
+					if (d0 > d1) H0 += d0;
+					else H0 += d1, ++last_H0_t;
+				} else if (last_H0_t >= st0 && last_H0_t <= en0) {
+					H0 += v[last_H0_t] - qe;
+				} else {
+					++last_H0_t, H0 += u[last_H0_t] - qe;
+				}
+				if ((flag & 0x10) && apply_zdrop(ez, H0, r, last_H0_t, zdrop, e)) break;
+			} else H0 = v[0] - qe - qe, last_H0_t = 0;
+			if (r == qlen + tlen - 2 && en0 == tlen - 1) ez->score = H0;
+		}
+		last_st = st, last_en = en;
+	}
+	if (with_cigar) {
+		int rev = !!(flag & 0x80);
+		if (!ez->zdropped && !(flag & 0x40)) {
+			backtrack(&cg, rev, p, off, off_end, (size_t)n_col_ * 16, tlen - 1, qlen - 1);
+		} else if ((flag & 0x40) && ez->mqe + end_bonus > ez->max) {
+			ez->reach_end = 1;
+			backtrack(&cg, rev, p, off, off_end, (size_t)n_col_ * 16, ez->mqe_t, qlen - 1);
+		} else if (ez->max_t >= 0 && ez->max_q >= 0) {
+			backtrack(&cg, rev, p, off, off_end, (size_t)n_col_ * 16, ez->max_t, ez->max_q);
+		}
+		ez->n_cigar = cg.n;
+	}
+	free(mem); free(H); free(p); free(off);
+	return cg.overflow ? -1 : cells;
+}

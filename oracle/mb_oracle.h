@@ -1,0 +1,56 @@
+/*
+ * mb_oracle.h -- CPU restatement of minibwa 0.7-r421's seeding + extension hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP library is compared against; it is
+ * linked only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by
+ * the product (minibwa_amd/libmbg.so).  Every function cites the reference file:line it restates.
+ */
+#ifndef MB_ORACLE_H
+#define MB_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { uint64_t x[2], size, info; } orc_sai_t;   /* bwt.h:9-12 */
+
+typedef struct {                                            /* bwt.h:14-30 (the fields the path reads) */
+	uint64_t primary, L2[5], seq_len, data_len;
+	uint64_t *data;
+	uint32_t sa_bit;
+	uint64_t n_sa;
+	uint64_t *sa;
+	int32_t pre_len;
+	orc_sai_t *pre;
+} orc_bwt_t;
+
+typedef struct {                                            /* ksw2.h:36-45, bitfields widened */
+	int32_t max, zdropped;
+	int32_t max_q, max_t, mqe, mqe_t, mte, mte_q, score, reach_end;
+	int32_t n_cigar;
+} orc_ez_t;
+
+orc_bwt_t *orc_bwt_load(const char *fn);
+void orc_bwt_destroy(orc_bwt_t *b);
+void orc_bwt_cache(orc_bwt_t *b, int32_t len);
+void orc_rank1a(const orc_bwt_t *b, uint64_t k, uint64_t cnt[4]);
+void orc_rank2a(const orc_bwt_t *b, uint64_t k, uint64_t l, uint64_t ck[4], uint64_t cl[4]);
+void orc_extend(const orc_bwt_t *b, const orc_sai_t *ik, orc_sai_t ok[4], int is_back);
+int64_t orc_smem(const orc_bwt_t *b, uint32_t len, const uint8_t *q, int64_t x, int64_t min_len, int64_t min_occ, orc_sai_t *p);
+int64_t orc_seed_intv(const orc_bwt_t *b, int32_t len, const uint8_t *seq, int32_t min_len, int32_t max_sub_occ, orc_sai_t *out, int64_t cap);
+uint64_t orc_sa(const orc_bwt_t *b, uint64_t k);
+void orc_sa_batch(const orc_bwt_t *b, int64_t n, uint64_t *x);
+
+int orc_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m, const int8_t *mat,
+              int8_t q, int8_t e, int8_t q2, int8_t e2, int w, int zdrop, int end_bonus, int flag,
+              orc_ez_t *ez, uint32_t *cigar, int cigar_cap);
+int orc_extz2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m, const int8_t *mat,
+              int8_t q, int8_t e, int w, int zdrop, int end_bonus, int flag,
+              orc_ez_t *ez, uint32_t *cigar, int cigar_cap);
+int64_t orc_dp_cells(int qlen, int tlen, int w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
