@@ -1,0 +1,154 @@
+/*
+ * mbg.h -- C ABI of libmbg.so, the MI355X (gfx950) seeding + extension library for minibwa.
+ *
+ * Plain pointers and sizes only.  Host-pointer entry points are drop-in batch replacements for
+ * the reference's hot-path functions (minibwa 0.7-r421); *_dev entry points take device pointers
+ * and a hipStream_t (passed as void*) for callers that keep inputs resident in HBM.
+ *
+ * Every entry point returns MBG_OK (0) or a negative MBG_E* code; mbg_strerror() names it.
+ * A context serialises its own calls (one mutex), so one context may be shared by threads.
+ */
+#ifndef MBG_H
+#define MBG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBG_OK            0
+#define MBG_EARG        (-1)   /* invalid argument */
+#define MBG_EHIP        (-2)   /* HIP runtime error (detail in mbg_last_hip_error) */
+#define MBG_ENOMEM      (-3)   /* device or host allocation failed */
+#define MBG_EIO         (-4)   /* index file missing or unreadable */
+#define MBG_EFORMAT     (-5)   /* bad magic / inconsistent index file */
+#define MBG_ENOIDX      (-6)   /* no index uploaded to the context */
+#define MBG_EUNSUPPORTED (-7)  /* job outside the kernels' supported range */
+#define MBG_EOVERFLOW   (-8)   /* caller-provided output capacity too small */
+
+typedef struct mbg_ctx_s mbg_ctx_t;
+
+/* SA interval, layout-identical to mb_sai_t (bwt.h:9-12): x[0] forward row, x[1] reverse-complement
+ * row, size, info = qs<<32 | qe (half-open query range). */
+typedef struct { uint64_t x[2]; uint64_t size, info; } mbg_sai_t;
+
+/* One seeding output record of the device entry point: the interval and the read it belongs to. */
+typedef struct { mbg_sai_t p; uint32_t rid, pass; } mbg_seed_rec_t;
+
+/* FM-index in the reference's in-memory layout (mb_bwt_t, bwt.h:14-30; 64-byte occ blocks,
+ * bwt.c:60-65).  Host pointers; mbg_bwt_upload copies them to HBM. */
+typedef struct {
+	uint64_t primary;       /* row of '$' */
+	uint64_t L2[5];         /* cumulative base counts, L2[0]=0 */
+	uint64_t seq_len;       /* BWT length without '$' */
+	const uint64_t *data;   /* occ blocks + packed BWT, data_len = ((n+127)/128)*8 + 4 words */
+	uint32_t sa_bit;        /* SA sampled at rows with row % (1<<sa_bit) == 0 */
+	uint64_t n_sa;
+	const uint64_t *sa;     /* sampled SA, sa[0] = UINT64_MAX */
+} mbg_bwt_desc_t;
+
+/* 2-bit reference (l2b_t, l2bit.h:20-32). */
+typedef struct {
+	uint64_t tot_len, n_ctg;
+	const uint64_t *ctg_len;     /* n_ctg contig lengths; offsets are their prefix sums */
+	uint64_t n_pac;
+	const uint64_t *pac;         /* 32 bases per word, base i at bits 2*(i&31) of pac[i>>5] */
+	uint64_t n_ambi;
+	const uint64_t *ambi;        /* n_ambi sorted [st,en) pairs of N runs, concatenated coordinates */
+} mbg_l2b_desc_t;
+
+typedef struct {
+	uint64_t seq_len, primary, tot_len, n_ctg;
+	int32_t pre_len, sa_bit;
+	uint64_t device_bytes;       /* HBM held by the resident index */
+} mbg_idx_info_t;
+
+/* One DP job (the arguments of ksw_extd2_sse / ksw_extz2_sse, ksw2.h:73-80). Sequences are bytes
+ * 0..m-1 stored in the batch's concatenated query/target buffers at qoff/toff. */
+typedef struct {
+	int64_t qoff, toff;
+	int32_t qlen, tlen;
+	int8_t q, e, q2, e2;          /* q2/e2 ignored by extz2 */
+	int32_t w, zdrop, end_bonus, flag;
+	int64_t cigar_off;            /* offset (in uint32) of this job's CIGAR slots, capacity qlen+tlen */
+} mbg_dp_job_t;
+
+/* ksw_extz_t (ksw2.h:36-45) fields, plus the number of exact-band cells the kernel processed. */
+typedef struct {
+	int32_t max, zdropped, max_q, max_t, mqe, mqe_t, mte, mte_q, score, reach_end;
+	int32_t n_cigar;
+	int32_t cells;
+} mbg_dp_result_t;
+
+const char *mbg_strerror(int err);
+const char *mbg_last_hip_error(void);
+
+/* Context on HIP device `device`.  Allocates a stream and grow-only scratch pools. */
+int mbg_init(int device, mbg_ctx_t **ctx);
+void mbg_destroy(mbg_ctx_t *ctx);
+
+/* Index residency.  Replaces mb_bwt_load + mb_bwt_cache (bwt.c:585-591,648-683; map-algo.c:19-42)
+ * and l2b_load (l2bit.c:275-330).  pre_len is the k-mer interval cache length (10 in the
+ * reference, 0 for none); the cache is computed on the device. */
+int mbg_idx_load(mbg_ctx_t *ctx, const char *prefix, int32_t pre_len);
+int mbg_bwt_upload(mbg_ctx_t *ctx, const mbg_bwt_desc_t *bwt, int32_t pre_len);
+int mbg_l2b_upload(mbg_ctx_t *ctx, const mbg_l2b_desc_t *l2b);
+int mbg_idx_info(mbg_ctx_t *ctx, mbg_idx_info_t *info);
+/* Device address of the k-mer cache (4^pre_len mbg_sai_t), for checking it against the host's. */
+int mbg_pre_copy(mbg_ctx_t *ctx, mbg_sai_t *out, uint64_t n);
+
+/* Occurrence rank of all four bases at rows k[i] and l[i]: mb_bwt_rank2a (bwt.c:195-231). */
+int mbg_rank2a(mbg_ctx_t *ctx, int64_t n, const uint64_t *k, const uint64_t *l, uint64_t *cntk, uint64_t *cntl);
+/* Bidirectional extension of ik[i] by all four bases: mb_bwt_extend (bwt.c:237-250); ok has 4n. */
+int mbg_extend(mbg_ctx_t *ctx, int64_t n, const mbg_sai_t *ik, const uint8_t *is_back, mbg_sai_t *ok);
+
+/* Two-pass SMEM seeding of n_seq nt4-encoded reads: mb_seed_intv_batch (seed.c:56-103).
+ * On return *out is a malloc'd array (caller free()s it) holding read i's intervals at
+ * [off[i], off[i+1]); off has n_seq+1 entries.  Within a read the order is unspecified, as in
+ * the reference (the caller sorts them, seed.c:109-138). */
+int mbg_seed_batch(mbg_ctx_t *ctx, int32_t n_seq, const int32_t *len, const uint8_t *const *seq,
+                   int32_t min_len, int32_t max_sub_occ, mbg_sai_t **out, int64_t *off);
+
+/* In-place, order-preserving SA locate of n rows: mb_bwt_sa_batch (bwt.c:511-541). */
+int mbg_sa_batch(mbg_ctx_t *ctx, int64_t n, uint64_t *x);
+
+/* Batched banded DP: ksw_extd2_sse (ksw2_extd2_sse.c:27-403) / ksw_extz2_sse
+ * (ksw2_extz2_sse.c:16-296), one result and one CIGAR per job.  m and mat (m*m) are shared by the
+ * batch.  CIGARs are written at cigar + jobs[i].cigar_off, oriented as the reference leaves them. */
+int mbg_extd2_batch(mbg_ctx_t *ctx, int32_t n_jobs, const mbg_dp_job_t *jobs, const uint8_t *qseq, int64_t qbytes,
+                    const uint8_t *tseq, int64_t tbytes, int8_t m, const int8_t *mat,
+                    mbg_dp_result_t *res, uint32_t *cigar, int64_t cigar_cap);
+int mbg_extz2_batch(mbg_ctx_t *ctx, int32_t n_jobs, const mbg_dp_job_t *jobs, const uint8_t *qseq, int64_t qbytes,
+                    const uint8_t *tseq, int64_t tbytes, int8_t m, const int8_t *mat,
+                    mbg_dp_result_t *res, uint32_t *cigar, int64_t cigar_cap);
+
+/* Target extraction l2b_getseq (l2bit.c:52-77) for n windows [st[i],en[i]) of contig tid[i]
+ * (clipped to the contig, N runs restored as 4), written at out + out_off[i].  len[i] receives
+ * the clipped length (or -1 for a bad tid). */
+int mbg_getseq(mbg_ctx_t *ctx, int64_t n, const int64_t *tid, const int64_t *st, const int64_t *en,
+               const int64_t *out_off, uint8_t *out, int64_t out_bytes, int64_t *len);
+
+/* ---- device-resident entry points (all pointers are device addresses) ---------------------- */
+
+/* Seeding over reads already in HBM: seq bytes at seq+off[i], length len[i].  Records go to
+ * out[0..cap); *n_out (device int64) receives the total count (> cap means overflow: records past
+ * cap were dropped and MBG_EOVERFLOW is returned).  Synchronises the stream before returning. */
+int mbg_seed_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_seq, const int64_t *off, const int32_t *len,
+                       const uint8_t *seq, int32_t min_len, int32_t max_sub_occ,
+                       mbg_seed_rec_t *out, int64_t cap, int64_t *n_out_host);
+/* Same as mbg_seed_batch_dev but asynchronous: *n_out_dev is a device int64 the caller reads
+ * later; pass-2 task generation happens on the device. */
+int mbg_seed_batch_dev_async(mbg_ctx_t *ctx, void *stream, int32_t n_seq, const int64_t *off, const int32_t *len,
+                             const uint8_t *seq, int32_t min_len, int32_t max_sub_occ,
+                             mbg_seed_rec_t *out, int64_t cap, int64_t *n_out_dev);
+int mbg_sa_batch_dev(mbg_ctx_t *ctx, void *stream, int64_t n, uint64_t *x);
+int mbg_extd2_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_jobs, const mbg_dp_job_t *jobs,
+                        const uint8_t *qseq, const uint8_t *tseq, int8_t m, const int8_t *mat_host,
+                        mbg_dp_result_t *res, uint32_t *cigar, int32_t max_qlen, int32_t max_tlen, int32_t max_w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

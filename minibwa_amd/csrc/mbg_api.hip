@@ -1,0 +1,630 @@
+// mbg_api.hip -- the C ABI of libmbg.so (declared in include/mbg.h).
+//
+// Host side of the library: device context, index residency (the reference's .mbw/.l2b files
+// uploaded byte for byte), grow-only scratch pools and the batch entry points.  No torch, no
+// allocation inside kernels; every HIP call is checked and mapped to an MBG_E* code.
+#include <hip/hip_runtime.h>
+#include <mutex>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include "mbg.h"
+#include "mbg_internal.h"
+
+using namespace mbg;
+
+namespace {
+
+thread_local std::string g_hip_err;
+
+struct DBuf {
+	void *p = nullptr;
+	size_t n = 0;
+};
+
+} // namespace
+
+struct mbg_ctx_s {
+	int device = 0;
+	int n_cu = 0;
+	hipStream_t stream = nullptr;
+	std::mutex mu;
+	// resident index
+	bool has_bwt = false, has_l2b = false;
+	uint64_t *d_data = nullptr, *d_sa = nullptr;
+	Sai *d_pre = nullptr;
+	Idx ix{};
+	uint64_t *d_pac = nullptr, *d_ambi = nullptr, *d_ctg_off = nullptr, *d_ctg_len = nullptr;
+	L2b lb{};
+	uint64_t idx_bytes = 0;
+	// scratch pools
+	DBuf b_in[6], b_out[3], b_ctr, b_task, b_slab;
+	DBuf h_pinned;
+};
+
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { g_hip_err = std::string(#x) + ": " + hipGetErrorString(e_); return MBG_EHIP; } } while (0)
+
+static int dgrow(DBuf &b, size_t bytes)
+{
+	if (bytes == 0) bytes = 16;
+	if (b.n >= bytes) return MBG_OK;
+	if (b.p) { hipFree(b.p); b.p = nullptr; b.n = 0; }
+	size_t n = bytes + bytes / 4 + 256;
+	hipError_t e = hipMalloc(&b.p, n);
+	if (e != hipSuccess) { g_hip_err = std::string("hipMalloc: ") + hipGetErrorString(e); b.p = nullptr; return MBG_ENOMEM; }
+	b.n = n;
+	return MBG_OK;
+}
+
+static void dfree(DBuf &b)
+{
+	if (b.p) hipFree(b.p);
+	b.p = nullptr, b.n = 0;
+}
+
+#define TRY(x) do { int r_ = (x); if (r_ != MBG_OK) return r_; } while (0)
+
+extern "C" {
+
+const char *mbg_strerror(int err)
+{
+	switch (err) {
+	case MBG_OK: return "ok";
+	case MBG_EARG: return "invalid argument";
+	case MBG_EHIP: return "HIP runtime error";
+	case MBG_ENOMEM: return "out of memory";
+	case MBG_EIO: return "cannot read index file";
+	case MBG_EFORMAT: return "bad index file format";
+	case MBG_ENOIDX: return "no index resident in the context";
+	case MBG_EUNSUPPORTED: return "job outside the supported range";
+	case MBG_EOVERFLOW: return "output capacity exceeded";
+	default: return "unknown error";
+	}
+}
+
+const char *mbg_last_hip_error(void) { return g_hip_err.c_str(); }
+
+int mbg_init(int device, mbg_ctx_t **out)
+{
+	if (out == nullptr) return MBG_EARG;
+	*out = nullptr;
+	int n_dev = 0;
+	HIPCHK(hipGetDeviceCount(&n_dev));
+	if (device < 0 || device >= n_dev) return MBG_EARG;
+	HIPCHK(hipSetDevice(device));
+	mbg_ctx_t *c = new mbg_ctx_t();
+	c->device = device;
+	hipDeviceProp_t prop;
+	hipError_t e = hipGetDeviceProperties(&prop, device);
+	if (e == hipSuccess) c->n_cu = prop.multiProcessorCount;
+	if (c->n_cu <= 0) c->n_cu = 256;
+	e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+	if (e != hipSuccess) { g_hip_err = hipGetErrorString(e); delete c; return MBG_EHIP; }
+	*out = c;
+	return MBG_OK;
+}
+
+static void free_bwt(mbg_ctx_t *c)
+{
+	if (c->d_data) hipFree(c->d_data);
+	if (c->d_sa) hipFree(c->d_sa);
+	if (c->d_pre) hipFree(c->d_pre);
+	c->d_data = c->d_sa = nullptr, c->d_pre = nullptr;
+	c->has_bwt = false;
+}
+
+static void free_l2b(mbg_ctx_t *c)
+{
+	if (c->d_pac) hipFree(c->d_pac);
+	if (c->d_ambi) hipFree(c->d_ambi);
+	if (c->d_ctg_off) hipFree(c->d_ctg_off);
+	if (c->d_ctg_len) hipFree(c->d_ctg_len);
+	c->d_pac = c->d_ambi = c->d_ctg_off = c->d_ctg_len = nullptr;
+	c->has_l2b = false;
+}
+
+void mbg_destroy(mbg_ctx_t *c)
+{
+	if (c == nullptr) return;
+	hipSetDevice(c->device);
+	if (c->stream) hipStreamSynchronize(c->stream);
+	free_bwt(c);
+	free_l2b(c);
+	for (auto &b : c->b_in) dfree(b);
+	for (auto &b : c->b_out) dfree(b);
+	dfree(c->b_ctr); dfree(c->b_task); dfree(c->b_slab);
+	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
+	if (c->stream) hipStreamDestroy(c->stream);
+	delete c;
+}
+
+static uint64_t bwt_data_len(uint64_t len) /* bwt.c:52-58 */
+{
+	const uint64_t nb = (len + 127) / 128;
+	return nb * 4 + (nb + 1) * 4;
+}
+
+static int upload(void **dst, const void *src, size_t bytes, hipStream_t st)
+{
+	hipError_t e = hipMalloc(dst, bytes ? bytes : 16);
+	if (e != hipSuccess) { g_hip_err = hipGetErrorString(e); *dst = nullptr; return MBG_ENOMEM; }
+	if (bytes) {
+		e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, st);
+		if (e != hipSuccess) { g_hip_err = hipGetErrorString(e); return MBG_EHIP; }
+	}
+	return MBG_OK;
+}
+
+static int bwt_upload_locked(mbg_ctx_t *c, const mbg_bwt_desc_t *d, int32_t pre_len)
+{
+	if (d == nullptr || d->data == nullptr || pre_len < 0 || pre_len > 13) return MBG_EARG;
+	if (d->L2[4] != d->seq_len || d->primary > d->seq_len) return MBG_EFORMAT;
+	if (d->sa_bit > 30 || d->n_sa != ((d->seq_len + (1ULL << d->sa_bit)) >> d->sa_bit) || d->sa == nullptr) return MBG_EFORMAT;
+	free_bwt(c);
+	const uint64_t dl = bwt_data_len(d->seq_len);
+	TRY(upload((void**)&c->d_data, d->data, dl * 8, c->stream));
+	TRY(upload((void**)&c->d_sa, d->sa, d->n_sa * 8, c->stream));
+	Idx &ix = c->ix;
+	ix.data = c->d_data; ix.data_len = dl;
+	ix.sa = c->d_sa; ix.sa_bit = d->sa_bit;
+	ix.primary = d->primary; ix.seq_len = d->seq_len;
+	for (int i = 0; i < 5; ++i) ix.L2[i] = d->L2[i];
+	ix.pre = nullptr; ix.pre_len = 0;
+	c->idx_bytes = dl * 8 + d->n_sa * 8;
+	if (pre_len > 0) {
+		const size_t n = (size_t)1 << (2 * pre_len);
+		hipError_t e = hipMalloc((void**)&c->d_pre, n * sizeof(Sai));
+		if (e != hipSuccess) { g_hip_err = hipGetErrorString(e); return MBG_ENOMEM; }
+		HIPCHK(launch_kmer_cache(ix, c->d_pre, pre_len, c->stream));
+		ix.pre = c->d_pre; ix.pre_len = pre_len;
+		c->idx_bytes += n * sizeof(Sai);
+	}
+	HIPCHK(hipStreamSynchronize(c->stream));
+	c->has_bwt = true;
+	return MBG_OK;
+}
+
+int mbg_bwt_upload(mbg_ctx_t *c, const mbg_bwt_desc_t *d, int32_t pre_len)
+{
+	if (c == nullptr) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	HIPCHK(hipSetDevice(c->device));
+	return bwt_upload_locked(c, d, pre_len);
+}
+
+static int l2b_upload_locked(mbg_ctx_t *c, const mbg_l2b_desc_t *d)
+{
+	if (d == nullptr || (d->n_ctg && d->ctg_len == nullptr) || (d->n_pac && d->pac == nullptr)) return MBG_EARG;
+	std::vector<uint64_t> off(d->n_ctg + 1, 0);
+	for (uint64_t i = 0; i < d->n_ctg; ++i) off[i + 1] = off[i] + d->ctg_len[i];
+	if (off[d->n_ctg] != d->tot_len || d->n_pac * 32 < d->tot_len) return MBG_EFORMAT;
+	free_l2b(c);
+	TRY(upload((void**)&c->d_pac, d->pac, d->n_pac * 8, c->stream));
+	TRY(upload((void**)&c->d_ambi, d->ambi, d->n_ambi * 16, c->stream));
+	TRY(upload((void**)&c->d_ctg_off, off.data(), d->n_ctg * 8, c->stream));
+	TRY(upload((void**)&c->d_ctg_len, d->ctg_len, d->n_ctg * 8, c->stream));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	L2b &lb = c->lb;
+	lb.pac = c->d_pac; lb.ambi = c->d_ambi; lb.ctg_off = c->d_ctg_off; lb.ctg_len = c->d_ctg_len;
+	lb.n_ctg = d->n_ctg; lb.n_ambi = d->n_ambi; lb.tot_len = d->tot_len;
+	c->idx_bytes += d->n_pac * 8 + d->n_ambi * 16 + d->n_ctg * 16;
+	c->has_l2b = true;
+	return MBG_OK;
+}
+
+int mbg_l2b_upload(mbg_ctx_t *c, const mbg_l2b_desc_t *d)
+{
+	if (c == nullptr) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	HIPCHK(hipSetDevice(c->device));
+	return l2b_upload_locked(c, d);
+}
+
+static bool rd(FILE *fp, void *p, size_t sz, size_t n) { return n == 0 || fread(p, sz, n, fp) == n; }
+
+int mbg_idx_load(mbg_ctx_t *c, const char *prefix, int32_t pre_len)
+{
+	if (c == nullptr || prefix == nullptr) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	HIPCHK(hipSetDevice(c->device));
+	std::string fn = std::string(prefix) + ".mbw";
+	FILE *fp = fopen(fn.c_str(), "rb");
+	if (fp == nullptr) return MBG_EIO;
+	// .mbw: magic, sa_bit, primary, L2[1..4], data, n_sa, sa (bwt.c:631-646)
+	char magic[4];
+	uint32_t sa_bit = 0;
+	uint64_t hdr[5], n_sa = 0;
+	if (!rd(fp, magic, 1, 4) || memcmp(magic, "MBW\2", 4) != 0 || !rd(fp, &sa_bit, 4, 1) || !rd(fp, hdr, 8, 5)) { fclose(fp); return MBG_EFORMAT; }
+	mbg_bwt_desc_t d;
+	d.primary = hdr[0];
+	d.L2[0] = 0;
+	memcpy(&d.L2[1], &hdr[1], 32);
+	d.seq_len = d.L2[4];
+	d.sa_bit = sa_bit;
+	const uint64_t dl = bwt_data_len(d.seq_len);
+	std::vector<uint64_t> data(dl), sa;
+	if (!rd(fp, data.data(), 8, dl) || !rd(fp, &n_sa, 8, 1) || sa_bit > 30 || n_sa != ((d.seq_len + (1ULL << sa_bit)) >> sa_bit)) { fclose(fp); return MBG_EFORMAT; }
+	sa.resize(n_sa);
+	if (!rd(fp, sa.data(), 8, n_sa)) { fclose(fp); return MBG_EFORMAT; }
+	fclose(fp);
+	d.data = data.data(); d.n_sa = n_sa; d.sa = sa.data();
+	TRY(bwt_upload_locked(c, &d, pre_len));
+	// .l2b (l2bit.c:243-330)
+	fn = std::string(prefix) + ".l2b";
+	fp = fopen(fn.c_str(), "rb");
+	if (fp == nullptr) return MBG_EIO;
+	uint32_t dummy;
+	uint64_t h[7];
+	if (!rd(fp, magic, 1, 4) || memcmp(magic, "L2B\1", 4) != 0 || !rd(fp, &dummy, 4, 1) || !rd(fp, h, 8, 7)) { fclose(fp); return MBG_EFORMAT; }
+	mbg_l2b_desc_t l;
+	l.n_ctg = h[0]; l.tot_len = h[1]; l.n_ambi = h[2];
+	const uint64_t n_mask = h[3];
+	l.n_pac = h[6];
+	std::vector<uint64_t> clen(l.n_ctg), ambi(l.n_ambi * 2), mask(n_mask * 2), pac(l.n_pac);
+	if (!rd(fp, clen.data(), 8, l.n_ctg) || !rd(fp, ambi.data(), 16, l.n_ambi) || !rd(fp, mask.data(), 16, n_mask) || !rd(fp, pac.data(), 8, l.n_pac)) { fclose(fp); return MBG_EFORMAT; }
+	fclose(fp);
+	l.ctg_len = clen.data(); l.ambi = ambi.data(); l.pac = pac.data();
+	return l2b_upload_locked(c, &l);
+}
+
+int mbg_idx_info(mbg_ctx_t *c, mbg_idx_info_t *info)
+{
+	if (c == nullptr || info == nullptr) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt) return MBG_ENOIDX;
+	info->seq_len = c->ix.seq_len; info->primary = c->ix.primary;
+	info->tot_len = c->has_l2b ? c->lb.tot_len : 0; info->n_ctg = c->has_l2b ? c->lb.n_ctg : 0;
+	info->pre_len = c->ix.pre_len; info->sa_bit = (int32_t)c->ix.sa_bit;
+	info->device_bytes = c->idx_bytes;
+	return MBG_OK;
+}
+
+int mbg_pre_copy(mbg_ctx_t *c, mbg_sai_t *out, uint64_t n)
+{
+	if (c == nullptr || out == nullptr) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt || c->ix.pre_len == 0) return MBG_ENOIDX;
+	if (n != (1ULL << (2 * c->ix.pre_len))) return MBG_EARG;
+	HIPCHK(hipSetDevice(c->device));
+	HIPCHK(hipMemcpy(out, c->d_pre, n * sizeof(Sai), hipMemcpyDeviceToHost));
+	return MBG_OK;
+}
+
+int mbg_rank2a(mbg_ctx_t *c, int64_t n, const uint64_t *k, const uint64_t *l, uint64_t *ck, uint64_t *cl)
+{
+	if (c == nullptr || n < 0 || (n > 0 && (!k || !l || !ck || !cl))) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt) return MBG_ENOIDX;
+	if (n == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	for (int64_t i = 0; i < n; ++i)
+		if (k[i] > l[i] || l[i] > c->ix.seq_len + 1) return MBG_EARG;
+	TRY(dgrow(c->b_in[0], n * 8)); TRY(dgrow(c->b_in[1], n * 8)); TRY(dgrow(c->b_out[0], n * 32)); TRY(dgrow(c->b_out[1], n * 32));
+	HIPCHK(hipMemcpyAsync(c->b_in[0].p, k, n * 8, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[1].p, l, n * 8, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(launch_rank2a(c->ix, n, (uint64_t*)c->b_in[0].p, (uint64_t*)c->b_in[1].p, (uint64_t*)c->b_out[0].p, (uint64_t*)c->b_out[1].p, c->stream));
+	HIPCHK(hipMemcpyAsync(ck, c->b_out[0].p, n * 32, hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipMemcpyAsync(cl, c->b_out[1].p, n * 32, hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	return MBG_OK;
+}
+
+int mbg_extend(mbg_ctx_t *c, int64_t n, const mbg_sai_t *ik, const uint8_t *is_back, mbg_sai_t *ok)
+{
+	if (c == nullptr || n < 0 || (n > 0 && (!ik || !is_back || !ok))) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt) return MBG_ENOIDX;
+	if (n == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	for (int64_t i = 0; i < n; ++i) {
+		const uint64_t xf = is_back[i] ? ik[i].x[0] : ik[i].x[1];
+		if (xf > c->ix.seq_len + 1 || ik[i].size > c->ix.seq_len + 1 - xf) return MBG_EARG;
+	}
+	TRY(dgrow(c->b_in[0], n * 32)); TRY(dgrow(c->b_in[1], n)); TRY(dgrow(c->b_out[0], n * 128));
+	HIPCHK(hipMemcpyAsync(c->b_in[0].p, ik, n * 32, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[1].p, is_back, n, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(launch_extend(c->ix, n, (const Sai*)c->b_in[0].p, (const uint8_t*)c->b_in[1].p, (Sai*)c->b_out[0].p, c->stream));
+	HIPCHK(hipMemcpyAsync(ok, c->b_out[0].p, n * 128, hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	return MBG_OK;
+}
+
+static unsigned persistent_grid(mbg_ctx_t *c, int64_t n_items, int per_block)
+{
+	int64_t want = (n_items + per_block - 1) / per_block;
+	int64_t cap = (int64_t)c->n_cu * 8;          // 8 blocks of 256 lanes per CU keep the memory pipes busy
+	if (want > cap) want = cap;
+	if (want < 1) want = 1;
+	return (unsigned)want;
+}
+
+// seeding over device-resident reads; counters live in c->b_ctr: [0]=next(u32) [1]=n_tasks2(u32) [2..3]=n_out(i64)
+static int seed_dev_locked(mbg_ctx_t *c, hipStream_t st, int32_t n_seq, const int64_t *off, const int32_t *len, const uint8_t *seq,
+                           int32_t min_len, int32_t max_sub_occ, mbg_seed_rec_t *out, int64_t cap, int64_t **n_out_dev)
+{
+	TRY(dgrow(c->b_ctr, 64));
+	TRY(dgrow(c->b_task, (size_t)(cap > 0 ? cap : 1) * sizeof(SeedTask)));
+	uint32_t *ctr = (uint32_t*)c->b_ctr.p;
+	int64_t *n_out = (int64_t*)(ctr + 2);
+	HIPCHK(hipMemsetAsync(ctr, 0, 16, st));
+	SeedArgs a{};
+	a.seq = seq; a.off = off; a.len = len;
+	a.tasks = nullptr; a.n_tasks = (uint32_t)n_seq;
+	a.min_len = min_len;
+	a.next = ctr;
+	a.out = (SeedRec*)out; a.n_out = n_out; a.cap = cap;
+	if (n_seq > 0) {
+		HIPCHK(launch_smem(c->ix, a, 1, persistent_grid(c, n_seq, 256), st));
+		// pass 2: tasks from the pass-1 records, then a second persistent launch
+		HIPCHK(launch_subtask((const SeedRec*)out, n_out, cap, min_len, max_sub_occ, (SeedTask*)c->b_task.p, ctr + 1, cap,
+		                      (unsigned)c->n_cu * 4, st));
+		HIPCHK(hipMemsetAsync(ctr, 0, 4, st));
+		a.tasks = (const SeedTask*)c->b_task.p;
+		a.n_tasks = 0xffffffffu;              // read from ctr[1] on the device
+		a.n_tasks_dev = ctr + 1;
+		a.task_cap = cap;
+		HIPCHK(launch_smem(c->ix, a, 2, persistent_grid(c, n_seq, 256), st));
+	}
+	*n_out_dev = n_out;
+	return MBG_OK;
+}
+
+int mbg_seed_batch_dev_async(mbg_ctx_t *c, void *stream, int32_t n_seq, const int64_t *off, const int32_t *len, const uint8_t *seq,
+                             int32_t min_len, int32_t max_sub_occ, mbg_seed_rec_t *out, int64_t cap, int64_t *n_out_dev)
+{
+	if (c == nullptr || n_seq < 0 || min_len < 1 || cap < 0 || (n_seq > 0 && (!off || !len || !seq || !out)) || n_out_dev == nullptr) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt) return MBG_ENOIDX;
+	HIPCHK(hipSetDevice(c->device));
+	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	int64_t *nd = nullptr;
+	TRY(seed_dev_locked(c, st, n_seq, off, len, seq, min_len, max_sub_occ, out, cap, &nd));
+	HIPCHK(hipMemcpyAsync(n_out_dev, nd, 8, hipMemcpyDeviceToDevice, st));
+	return MBG_OK;
+}
+
+int mbg_seed_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_seq, const int64_t *off, const int32_t *len, const uint8_t *seq,
+                       int32_t min_len, int32_t max_sub_occ, mbg_seed_rec_t *out, int64_t cap, int64_t *n_out_host)
+{
+	if (c == nullptr || n_seq < 0 || min_len < 1 || cap < 0 || (n_seq > 0 && (!off || !len || !seq || !out)) || n_out_host == nullptr) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt) return MBG_ENOIDX;
+	HIPCHK(hipSetDevice(c->device));
+	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	int64_t *nd = nullptr;
+	TRY(seed_dev_locked(c, st, n_seq, off, len, seq, min_len, max_sub_occ, out, cap, &nd));
+	HIPCHK(hipMemcpyAsync(n_out_host, nd, 8, hipMemcpyDeviceToHost, st));
+	HIPCHK(hipStreamSynchronize(st));
+	return *n_out_host > cap ? MBG_EOVERFLOW : MBG_OK;
+}
+
+int mbg_seed_batch(mbg_ctx_t *c, int32_t n_seq, const int32_t *len, const uint8_t *const *seq, int32_t min_len, int32_t max_sub_occ,
+                   mbg_sai_t **out, int64_t *offs)
+{
+	if (c == nullptr || n_seq < 0 || min_len < 1 || out == nullptr || offs == nullptr || (n_seq > 0 && (!len || !seq))) return MBG_EARG;
+	*out = nullptr;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt) return MBG_ENOIDX;
+	HIPCHK(hipSetDevice(c->device));
+	std::vector<int64_t> off(n_seq + 1, 0);
+	for (int32_t i = 0; i < n_seq; ++i) {
+		if (len[i] < 0 || (len[i] > 0 && seq[i] == nullptr)) return MBG_EARG;
+		off[i + 1] = off[i] + len[i];
+	}
+	const int64_t tot = off[n_seq];
+	std::vector<uint8_t> buf(tot > 0 ? tot : 1);
+	for (int32_t i = 0; i < n_seq; ++i) if (len[i] > 0) memcpy(buf.data() + off[i], seq[i], len[i]);
+	TRY(dgrow(c->b_in[0], tot + 16)); TRY(dgrow(c->b_in[1], (n_seq + 1) * 8)); TRY(dgrow(c->b_in[2], (n_seq + 1) * 4));
+	HIPCHK(hipMemcpyAsync(c->b_in[0].p, buf.data(), tot, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[1].p, off.data(), (n_seq + 1) * 8, hipMemcpyHostToDevice, c->stream));
+	if (n_seq > 0) HIPCHK(hipMemcpyAsync(c->b_in[2].p, len, n_seq * 4, hipMemcpyHostToDevice, c->stream));
+	int64_t cap = 8 * (int64_t)n_seq + tot / 4 + 1024, n_out = 0;
+	for (;;) {
+		TRY(dgrow(c->b_out[0], cap * sizeof(SeedRec)));
+		int64_t *nd = nullptr;
+		TRY(seed_dev_locked(c, c->stream, n_seq, (const int64_t*)c->b_in[1].p, (const int32_t*)c->b_in[2].p, (const uint8_t*)c->b_in[0].p,
+		                    min_len, max_sub_occ, (mbg_seed_rec_t*)c->b_out[0].p, cap, &nd));
+		HIPCHK(hipMemcpyAsync(&n_out, nd, 8, hipMemcpyDeviceToHost, c->stream));
+		HIPCHK(hipStreamSynchronize(c->stream));
+		if (n_out <= cap) break;
+		cap = n_out * 2;
+	}
+	std::vector<SeedRec> recs(n_out);
+	if (n_out) HIPCHK(hipMemcpy(recs.data(), c->b_out[0].p, n_out * sizeof(SeedRec), hipMemcpyDeviceToHost));
+	// group by read: pass-1 records first, each group in device order
+	std::vector<int64_t> cnt(n_seq + 1, 0);
+	for (const SeedRec &r : recs) ++cnt[r.rid + 1];
+	for (int32_t i = 0; i < n_seq; ++i) cnt[i + 1] += cnt[i];
+	for (int32_t i = 0; i <= n_seq; ++i) offs[i] = cnt[i];
+	mbg_sai_t *o = (mbg_sai_t*)malloc((n_out > 0 ? n_out : 1) * sizeof(mbg_sai_t));
+	if (o == nullptr) return MBG_ENOMEM;
+	std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+	for (int pass = 1; pass <= 2; ++pass)
+		for (const SeedRec &r : recs) {
+			if ((int)r.pass != pass) continue;
+			mbg_sai_t &s = o[pos[r.rid]++];
+			s.x[0] = r.x0; s.x[1] = r.x1; s.size = r.size; s.info = r.info;
+		}
+	*out = o;
+	return MBG_OK;
+}
+
+static int sa_dev_locked(mbg_ctx_t *c, hipStream_t st, int64_t n, uint64_t *x)
+{
+	if (n <= 0) return MBG_OK;
+	if (n > 0xffffffffLL) return MBG_EUNSUPPORTED;
+	TRY(dgrow(c->b_ctr, 64));
+	uint32_t *next = (uint32_t*)c->b_ctr.p + 4;
+	HIPCHK(hipMemsetAsync(next, 0, 4, st));
+	HIPCHK(launch_sa(c->ix, n, x, next, persistent_grid(c, n, 256), st));
+	return MBG_OK;
+}
+
+int mbg_sa_batch_dev(mbg_ctx_t *c, void *stream, int64_t n, uint64_t *x)
+{
+	if (c == nullptr || n < 0 || (n > 0 && x == nullptr)) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt) return MBG_ENOIDX;
+	HIPCHK(hipSetDevice(c->device));
+	return sa_dev_locked(c, stream ? (hipStream_t)stream : c->stream, n, x);
+}
+
+int mbg_sa_batch(mbg_ctx_t *c, int64_t n, uint64_t *x)
+{
+	if (c == nullptr || n < 0 || (n > 0 && x == nullptr)) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt) return MBG_ENOIDX;
+	if (n == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	for (int64_t i = 0; i < n; ++i) if (x[i] > c->ix.seq_len) return MBG_EARG;
+	TRY(dgrow(c->b_in[3], n * 8));
+	HIPCHK(hipMemcpyAsync(c->b_in[3].p, x, n * 8, hipMemcpyHostToDevice, c->stream));
+	TRY(sa_dev_locked(c, c->stream, n, (uint64_t*)c->b_in[3].p));
+	HIPCHK(hipMemcpyAsync(x, c->b_in[3].p, n * 8, hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	return MBG_OK;
+}
+
+// ---- DP --------------------------------------------------------------------------------------
+
+static const int kMaxCap = 4096;        // circular row buffer; 12*cap bytes of LDS per wave
+
+static int dp_plan(int32_t max_qlen, int32_t max_tlen, int32_t max_w, int *cap_out, int64_t *slab_out)
+{
+	int w, n_col_;
+	dp_geom(max_qlen, max_tlen, max_w, &w, &n_col_);
+	const int nb = n_col_ * 16;
+	int cap = 64;
+	while (cap < nb + 17) cap <<= 1;
+	if (cap > kMaxCap) return MBG_EUNSUPPORTED;
+	*cap_out = cap;
+	int64_t slab = ((int64_t)max_qlen + max_tlen) * nb + 256;
+	*slab_out = (slab + 255) & ~(int64_t)255;
+	return MBG_OK;
+}
+
+static int dp_dev_locked(mbg_ctx_t *c, hipStream_t st, bool dual, int32_t n_jobs, const DpJob *jobs, const uint8_t *qseq, const uint8_t *tseq,
+                         int8_t m, const int8_t *mat, DpRes *res, uint32_t *cigar, int cap, int64_t slab)
+{
+	if (n_jobs <= 0) return MBG_OK;
+	// resident blocks: enough to fill the GPU, bounded by 2 GiB of backtrack slabs
+	int64_t g = (int64_t)c->n_cu * 16;
+	if (g > n_jobs) g = n_jobs;
+	const int64_t max_by_mem = ((int64_t)2 << 30) / slab;
+	if (g > max_by_mem) g = max_by_mem;
+	if (g < 1) g = 1;
+	TRY(dgrow(c->b_slab, (size_t)g * slab));
+	DpArgs a{};
+	a.jobs = jobs; a.n_jobs = n_jobs; a.qseq = qseq; a.tseq = tseq; a.res = res; a.cigar = cigar;
+	a.pbuf = (uint8_t*)c->b_slab.p; a.slab = slab; a.cap = cap; a.m = m;
+	for (int i = 0; i < 32; ++i) a.mat[i] = i < m * m ? mat[i] : 0;
+	HIPCHK(launch_dp(dual, a, (unsigned)g, st));
+	return MBG_OK;
+}
+
+static int dp_batch(mbg_ctx_t *c, bool dual, int32_t n_jobs, const mbg_dp_job_t *jobs, const uint8_t *qseq, int64_t qbytes,
+                    const uint8_t *tseq, int64_t tbytes, int8_t m, const int8_t *mat, mbg_dp_result_t *res, uint32_t *cigar, int64_t cigar_cap)
+{
+	if (c == nullptr || n_jobs < 0 || m < 0 || m > 5 || (n_jobs > 0 && (!jobs || !res || !mat))) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (n_jobs == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	int32_t mq = 1, mt = 1, mw = 1;
+	bool any_cigar = false;
+	for (int32_t i = 0; i < n_jobs; ++i) {
+		const mbg_dp_job_t &j = jobs[i];
+		if (j.qlen < 0 || j.tlen < 0 || j.qoff < 0 || j.toff < 0 || j.qoff + j.qlen > qbytes || j.toff + j.tlen > tbytes) return MBG_EARG;
+		if (!(j.flag & 0x01)) {
+			any_cigar = true;
+			if (cigar == nullptr || j.cigar_off < 0 || j.cigar_off + j.qlen + j.tlen > cigar_cap) return MBG_EARG;
+		}
+		int w, ncol;
+		dp_geom(j.qlen, j.tlen, j.w, &w, &ncol);
+		mq = j.qlen > mq ? j.qlen : mq;
+		mt = j.tlen > mt ? j.tlen : mt;
+		mw = w > mw ? w : mw;
+	}
+	int cap;
+	int64_t slab;
+	TRY(dp_plan(mq, mt, mw, &cap, &slab));
+	if (!any_cigar) slab = 256;
+	TRY(dgrow(c->b_in[0], qbytes + 16)); TRY(dgrow(c->b_in[1], tbytes + 16)); TRY(dgrow(c->b_in[4], (size_t)n_jobs * sizeof(DpJob)));
+	TRY(dgrow(c->b_out[0], (size_t)n_jobs * sizeof(DpRes)));
+	if (any_cigar) TRY(dgrow(c->b_out[1], (size_t)cigar_cap * 4));
+	HIPCHK(hipMemcpyAsync(c->b_in[0].p, qseq, qbytes, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[1].p, tseq, tbytes, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[4].p, jobs, (size_t)n_jobs * sizeof(DpJob), hipMemcpyHostToDevice, c->stream));
+	TRY(dp_dev_locked(c, c->stream, dual, n_jobs, (const DpJob*)c->b_in[4].p, (const uint8_t*)c->b_in[0].p, (const uint8_t*)c->b_in[1].p,
+	                  m, mat, (DpRes*)c->b_out[0].p, any_cigar ? (uint32_t*)c->b_out[1].p : nullptr, cap, slab));
+	HIPCHK(hipMemcpyAsync(res, c->b_out[0].p, (size_t)n_jobs * sizeof(DpRes), hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	if (any_cigar) {
+		std::vector<uint32_t> tmp(cigar_cap);
+		HIPCHK(hipMemcpy(tmp.data(), c->b_out[1].p, (size_t)cigar_cap * 4, hipMemcpyDeviceToHost));
+		for (int32_t i = 0; i < n_jobs; ++i)
+			if (!(jobs[i].flag & 0x01) && res[i].n_cigar > 0)
+				memcpy(cigar + jobs[i].cigar_off, tmp.data() + jobs[i].cigar_off, (size_t)res[i].n_cigar * 4);
+	}
+	return MBG_OK;
+}
+
+int mbg_extd2_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_dp_job_t *jobs, const uint8_t *qseq, int64_t qbytes, const uint8_t *tseq, int64_t tbytes,
+                    int8_t m, const int8_t *mat, mbg_dp_result_t *res, uint32_t *cigar, int64_t cigar_cap)
+{
+	return dp_batch(c, true, n_jobs, jobs, qseq, qbytes, tseq, tbytes, m, mat, res, cigar, cigar_cap);
+}
+
+int mbg_extz2_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_dp_job_t *jobs, const uint8_t *qseq, int64_t qbytes, const uint8_t *tseq, int64_t tbytes,
+                    int8_t m, const int8_t *mat, mbg_dp_result_t *res, uint32_t *cigar, int64_t cigar_cap)
+{
+	return dp_batch(c, false, n_jobs, jobs, qseq, qbytes, tseq, tbytes, m, mat, res, cigar, cigar_cap);
+}
+
+int mbg_extd2_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_jobs, const mbg_dp_job_t *jobs, const uint8_t *qseq, const uint8_t *tseq,
+                        int8_t m, const int8_t *mat_host, mbg_dp_result_t *res, uint32_t *cigar, int32_t max_qlen, int32_t max_tlen, int32_t max_w)
+{
+	if (c == nullptr || n_jobs < 0 || m < 0 || m > 5 || mat_host == nullptr || max_qlen < 0 || max_tlen < 0) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (n_jobs == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	int cap;
+	int64_t slab;
+	TRY(dp_plan(max_qlen, max_tlen, max_w, &cap, &slab));
+	return dp_dev_locked(c, stream ? (hipStream_t)stream : c->stream, true, n_jobs, (const DpJob*)jobs, qseq, tseq, m, mat_host,
+	                     (DpRes*)res, cigar, cap, slab);
+}
+
+int mbg_getseq(mbg_ctx_t *c, int64_t n, const int64_t *tid, const int64_t *st, const int64_t *en, const int64_t *out_off, uint8_t *out,
+               int64_t out_bytes, int64_t *len)
+{
+	if (c == nullptr || n < 0 || (n > 0 && (!tid || !st || !en || !out_off || !out || !len))) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_l2b) return MBG_ENOIDX;
+	if (n == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	for (int64_t i = 0; i < n; ++i) {
+		if (tid[i] < 0 || (uint64_t)tid[i] >= c->lb.n_ctg) continue;
+		int64_t s = st[i] < 0 ? 0 : st[i], e = en[i];
+		if (e - s > 0 && (out_off[i] < 0 || out_off[i] + (e - s) > out_bytes)) {
+			// clipped length may be shorter; the caller must size for the unclipped window
+			return MBG_EARG;
+		}
+	}
+	TRY(dgrow(c->b_in[0], n * 8)); TRY(dgrow(c->b_in[1], n * 8)); TRY(dgrow(c->b_in[2], n * 8)); TRY(dgrow(c->b_in[3], n * 8));
+	TRY(dgrow(c->b_out[0], out_bytes + 16)); TRY(dgrow(c->b_out[1], n * 8));
+	HIPCHK(hipMemcpyAsync(c->b_in[0].p, tid, n * 8, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[1].p, st, n * 8, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[2].p, en, n * 8, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[3].p, out_off, n * 8, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_out[0].p, out, out_bytes, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(launch_getseq(c->lb, n, (const int64_t*)c->b_in[0].p, (const int64_t*)c->b_in[1].p, (const int64_t*)c->b_in[2].p,
+	                     (const int64_t*)c->b_in[3].p, (uint8_t*)c->b_out[0].p, (int64_t*)c->b_out[1].p, c->stream));
+	HIPCHK(hipMemcpyAsync(out, c->b_out[0].p, out_bytes, hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipMemcpyAsync(len, c->b_out[1].p, n * 8, hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	return MBG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,185 @@
+// mbg_dev.h -- device-side FM-index primitives for gfx950 (rank, extend, LF), shared by the kernels.
+//
+// The index stays in the reference's byte layout in HBM (bwt.c:60-65): one 64-byte block per 128
+// BWT rows = 4 x u64 cumulative counts (low 56 bits: count before the block; high 8 bits: count in
+// the block's first 64 symbols) + 4 x u64 packed 2-bit symbols (symbol i at bits 2*(i&31) of word
+// i>>5).  A block is fetched as four 16-byte loads; counts within it use popcount on 2-bit lanes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mbg {
+
+struct Sai { uint64_t x0, x1, size, info; };
+
+struct Idx {
+	const uint64_t *data;   // occ blocks
+	uint64_t data_len;      // in u64 words
+	const uint64_t *sa;     // sampled SA
+	const Sai *pre;         // k-mer interval cache (nullptr if pre_len == 0)
+	uint64_t primary, seq_len;
+	uint64_t L2[5];
+	uint32_t sa_bit;
+	int32_t pre_len;
+};
+
+constexpr uint64_t kM56 = (1ULL << 56) - 1;
+
+__device__ __forceinline__ void load_block(const uint64_t *data, uint64_t blk, uint64_t b[8])
+{
+	const ulonglong2 *p = reinterpret_cast<const ulonglong2*>(data + (blk << 3));
+	ulonglong2 a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3];
+	b[0] = a0.x; b[1] = a0.y; b[2] = a1.x; b[3] = a1.y;
+	b[4] = a2.x; b[5] = a2.y; b[6] = a3.x; b[7] = a3.y;
+}
+
+// number of symbol-c 2-bit lanes among the first n (0..32) lanes of w
+__device__ __forceinline__ uint32_t sym_cnt(uint64_t w, uint64_t pat, int n)
+{
+	uint64_t e = ~(w ^ pat);
+	e &= (e >> 1) & 0x5555555555555555ULL;
+	uint64_t m = n >= 32 ? ~0ULL : ((1ULL << (2 * n)) - 1);
+	return (uint32_t)__popcll(e & m);
+}
+
+// occurrences of each base among $-removed symbols [blk*128, blk*128 + in] plus everything
+// before the block (bwt.c:160-193: the 56/8-bit counts and the half-block start)
+__device__ __forceinline__ void occ_incl(const uint64_t b[8], uint32_t in, uint64_t cnt[4])
+{
+	const int half = in >= 64;
+	const int n = (int)(in & 63) + 1;
+	const uint64_t w0 = half ? b[6] : b[4], w1 = half ? b[7] : b[5];
+	const int n0 = n > 32 ? 32 : n, n1 = n > 32 ? n - 32 : 0;
+	const uint32_t c1 = sym_cnt(w0, 0x5555555555555555ULL, n0) + sym_cnt(w1, 0x5555555555555555ULL, n1);
+	const uint32_t c2 = sym_cnt(w0, 0xAAAAAAAAAAAAAAAAULL, n0) + sym_cnt(w1, 0xAAAAAAAAAAAAAAAAULL, n1);
+	const uint32_t c3 = sym_cnt(w0, ~0ULL, n0) + sym_cnt(w1, ~0ULL, n1);
+	const uint32_t c0 = (uint32_t)n - c1 - c2 - c3;
+	cnt[0] = (b[0] & kM56) + (half ? (b[0] >> 56) : 0) + c0;
+	cnt[1] = (b[1] & kM56) + (half ? (b[1] >> 56) : 0) + c1;
+	cnt[2] = (b[2] & kM56) + (half ? (b[2] >> 56) : 0) + c2;
+	cnt[3] = (b[3] & kM56) + (half ? (b[3] >> 56) : 0) + c3;
+}
+
+// symbol at $-removed index k (bwt.c:119); reads past the array give 0
+__device__ __forceinline__ int bwt_b0(const Idx &ix, uint64_t k)
+{
+	uint64_t wi = (k >> 7 << 3) + 4 + ((k & 127) >> 5);
+	if (wi >= ix.data_len) return 0;
+	return (int)(ix.data[wi] >> ((k & 31) << 1) & 3);
+}
+
+// mb_bwt_rank2a (bwt.c:195-231).  One block fetch when k-1 and l-1 share a block.
+__device__ __forceinline__ void rank2a(const Idx &ix, uint64_t k, uint64_t l, uint64_t ck[4], uint64_t cl[4])
+{
+	uint64_t b[8];
+	uint64_t blk = ~0ULL;
+	if (k == 0) {
+		ck[0] = ck[1] = ck[2] = ck[3] = 0;
+	} else {
+		uint64_t j = k - 1;
+		j -= (j >= ix.primary);
+		blk = j >> 7;
+		load_block(ix.data, blk, b);
+		occ_incl(b, (uint32_t)(j & 127), ck);
+	}
+	// bwt.c:205-209: with l==k+1 the reference adds the symbol at $-removed index k-(k>primary);
+	// at k==primary that row is '$' and the shortcut's count differs from rank1a(l) -- keep it.
+	if (l == k + 1 && k != 0 && k == ix.primary) {
+		cl[0] = ck[0]; cl[1] = ck[1]; cl[2] = ck[2]; cl[3] = ck[3];
+		cl[bwt_b0(ix, k)] += 1;
+		return;
+	}
+	if (l == 0) {
+		cl[0] = cl[1] = cl[2] = cl[3] = 0;
+	} else {
+		uint64_t j = l - 1;
+		j -= (j >= ix.primary);
+		if ((j >> 7) != blk) load_block(ix.data, j >> 7, b);
+		occ_incl(b, (uint32_t)(j & 127), cl);
+	}
+}
+
+// mb_bwt_set_intv (bwt.h:75-81)
+__device__ __forceinline__ Sai set_intv(const Idx &ix, int c)
+{
+	Sai p;
+	p.x0 = ix.L2[c] + 1;
+	p.x1 = ix.L2[3 - c] + 1;
+	p.size = ix.L2[c + 1] - ix.L2[c];
+	p.info = 0;
+	return p;
+}
+
+// mb_bwt_extend (bwt.c:237-250) restricted to the one base c the caller keeps.
+__device__ __forceinline__ Sai extend1(const Idx &ix, const Sai &ik, int is_back, int c)
+{
+	const uint64_t xf = is_back ? ik.x0 : ik.x1;   // x[!is_back]
+	const uint64_t xg = is_back ? ik.x1 : ik.x0;   // x[is_back]
+	uint64_t tk[4], tl[4];
+	rank2a(ix, xf, xf + ik.size, tk, tl);
+	uint64_t acc = xg + (uint64_t)(xf <= ix.primary && xf + ik.size > ix.primary);
+	#pragma unroll
+	for (int b = 3; b > 0; --b)
+		if (b > c) acc += tl[b] - tk[b];
+	Sai o;
+	const uint64_t nf = ix.L2[c] + 1 + tk[c];
+	o.x0 = is_back ? nf : acc;
+	o.x1 = is_back ? acc : nf;
+	o.size = tl[c] - tk[c];
+	o.info = 0;
+	return o;
+}
+
+// all four extensions (mb_bwt_extend)
+__device__ __forceinline__ void extend4(const Idx &ix, const Sai &ik, int is_back, Sai ok[4])
+{
+	const uint64_t xf = is_back ? ik.x0 : ik.x1;
+	const uint64_t xg = is_back ? ik.x1 : ik.x0;
+	uint64_t tk[4], tl[4];
+	rank2a(ix, xf, xf + ik.size, tk, tl);
+	uint64_t acc = xg + (uint64_t)(xf <= ix.primary && xf + ik.size > ix.primary);
+	for (int c = 3; c >= 0; --c) {
+		const uint64_t nf = ix.L2[c] + 1 + tk[c];
+		ok[c].x0 = is_back ? nf : acc;
+		ok[c].x1 = is_back ? acc : nf;
+		ok[c].size = tl[c] - tk[c];
+		ok[c].info = 0;
+		acc += ok[c].size;
+	}
+}
+
+// bwt_invPsi (bwt.c:469-475): LF of row k with one block fetch.  The count of c among rows [0,k)
+// (excluding '$') equals the count among $-removed symbols [0, x) with x = k - (k > primary), and
+// the symbol c = B0(x) lives in the same block.
+__device__ __forceinline__ uint64_t lf(const Idx &ix, uint64_t k)
+{
+	if (k == ix.primary) return 0;
+	const uint64_t x = k - (k > ix.primary);
+	uint64_t b[8];
+	load_block(ix.data, x >> 7, b);
+	const uint32_t in = (uint32_t)(x & 127);
+	const int c = (int)(b[4 + (in >> 5)] >> ((in & 31) << 1) & 3);
+	const int half = in >= 64;
+	const int n = (int)(in & 63);                 // symbols before x within the half
+	const uint64_t pat = c == 0 ? 0 : c == 1 ? 0x5555555555555555ULL : c == 2 ? 0xAAAAAAAAAAAAAAAAULL : ~0ULL;
+	const uint64_t w0 = half ? b[6] : b[4], w1 = half ? b[7] : b[5];
+	const int n0 = n > 32 ? 32 : n, n1 = n > 32 ? n - 32 : 0;
+	const uint64_t cnt = (b[c] & kM56) + (half ? (b[c] >> 56) : 0) + sym_cnt(w0, pat, n0) + sym_cnt(w1, pat, n1);
+	return ix.L2[c] + 1 + cnt;
+}
+
+// wave-aggregated fetch of the next work item from a global counter
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t *counter, bool want)
+{
+	const uint64_t mask = __ballot(want);
+	if (!want) return 0;
+	const int leader = __ffsll((unsigned long long)mask) - 1;
+	const uint32_t lane = __lane_id();
+	uint32_t base = 0;
+	if (lane == (uint32_t)leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+	base = __shfl(base, leader);
+	const uint64_t lt = lane == 0 ? 0 : (mask & ((~0ULL) >> (64 - lane)));
+	return base + (uint32_t)__popcll(lt);
+}
+
+} // namespace mbg

@@ -1,0 +1,74 @@
+// mbg_internal.h -- types shared by the kernels and the C-ABI layer of libmbg.so
+#pragma once
+#include "mbg_dev.h"
+
+namespace mbg {
+
+// one seeding output record; layout-identical to mbg_seed_rec_t
+struct SeedRec { uint64_t x0, x1, size, info; uint32_t rid, pass; };
+struct SeedTask { uint32_t rid; int32_t st, en, min_len, min_occ; };
+
+struct SeedArgs {
+	const uint8_t *seq;     // concatenated nt4 reads
+	const int64_t *off;     // read i at seq + off[i]
+	const int32_t *len;
+	const SeedTask *tasks;  // pass 2 only
+	uint32_t n_tasks;
+	const uint32_t *n_tasks_dev;  // pass 2: task count produced on the device (clamped to task_cap)
+	int64_t task_cap;
+	int32_t min_len;
+	uint32_t *next;         // task counter (zeroed before launch)
+	SeedRec *out;
+	int64_t *n_out;         // record counter
+	int64_t cap;
+};
+
+struct L2b {
+	const uint64_t *pac;
+	const uint64_t *ambi;   // [st,en) pairs
+	const uint64_t *ctg_off, *ctg_len;
+	uint64_t n_ctg, n_ambi, tot_len;
+};
+
+// DP batch arguments (kernels in mbg_dp.hip)
+struct DpJob { int64_t qoff, toff; int32_t qlen, tlen; int8_t q, e, q2, e2; int32_t w, zdrop, end_bonus, flag; int64_t cigar_off; };
+struct DpRes { int32_t max, zdropped, max_q, max_t, mqe, mqe_t, mte, mte_q, score, reach_end, n_cigar, cells; };
+static_assert(sizeof(DpJob) == 56, "DpJob must match mbg_dp_job_t");
+static_assert(sizeof(DpRes) == 48, "DpRes must match mbg_dp_result_t");
+static_assert(sizeof(SeedRec) == 40, "SeedRec must match mbg_seed_rec_t");
+
+struct DpArgs {
+	const DpJob *jobs;
+	int32_t n_jobs;
+	const uint8_t *qseq, *tseq;
+	DpRes *res;
+	uint32_t *cigar;
+	uint8_t *pbuf;          // backtrack slabs, one per resident block
+	int64_t slab;           // bytes per slab
+	int32_t cap;            // circular-buffer length (power of two)
+	int8_t m;
+	int8_t mat[32];
+};
+
+hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st);
+hipError_t launch_rank2a(const Idx &ix, int64_t n, const uint64_t *k, const uint64_t *l, uint64_t *ck, uint64_t *cl, hipStream_t st);
+hipError_t launch_extend(const Idx &ix, int64_t n, const Sai *ik, const uint8_t *is_back, Sai *ok, hipStream_t st);
+hipError_t launch_smem(const Idx &ix, const SeedArgs &a, uint32_t pass, unsigned grid, hipStream_t st);
+hipError_t launch_subtask(const SeedRec *out, const int64_t *n_out, int64_t cap, int32_t min_len, int32_t max_sub_occ,
+                          SeedTask *tasks, uint32_t *n_tasks, int64_t task_cap, unsigned grid, hipStream_t st);
+hipError_t launch_sa(const Idx &ix, int64_t n, uint64_t *x, uint32_t *next, unsigned grid, hipStream_t st);
+hipError_t launch_getseq(const L2b &lb, int64_t n, const int64_t *tid, const int64_t *st_, const int64_t *en_, const int64_t *out_off,
+                         uint8_t *out, int64_t *len, hipStream_t st);
+hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st);
+
+// geometry of one DP job (ksw2_extd2_sse.c:82-86): the band width after `w<0` handling and the
+// number of 16-cell columns per anti-diagonal row
+__host__ __device__ static inline void dp_geom(int qlen, int tlen, int w, int *w_out, int *n_col_)
+{
+	if (w < 0) w = tlen > qlen ? tlen : qlen;
+	int n = qlen < tlen ? qlen : tlen;
+	n = ((n < w + 1 ? n : w + 1) + 15) / 16 + 1;
+	*w_out = w, *n_col_ = n;
+}
+
+} // namespace mbg

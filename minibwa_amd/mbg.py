@@ -1,0 +1,295 @@
+"""Python host mirror of the reference's hot-path interface, bound to libmbg.so (include/mbg.h).
+
+The names follow the reference functions each call replaces:
+
+==========================  =======================================================
+``Context.seed_intv_batch``  ``mb_seed_intv_batch`` (seed.c:56-103)
+``Context.bwt_sa_batch``     ``mb_bwt_sa_batch`` (bwt.c:511-541)
+``Context.bwt_rank2a``       ``mb_bwt_rank2a`` (bwt.c:195-231)
+``Context.bwt_extend``       ``mb_bwt_extend`` (bwt.c:237-250)
+``Context.ksw_extd2``        ``ksw_extd2_sse`` (ksw2_extd2_sse.c:27-403), batched
+``Context.ksw_extz2``        ``ksw_extz2_sse`` (ksw2_extz2_sse.c:16-296), batched
+``Context.l2b_getseq``       ``l2b_getseq`` (l2bit.c:52-77), batched
+``Context.idx_load``         ``mb_idx_load`` (map-algo.c:19-42) minus chaining state
+==========================  =======================================================
+
+There is no CPU fallback: constructing a Context without the built library or without a GPU
+raises.  Errors from the library raise ``MbgError`` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBMBG = os.path.join(HERE, "libmbg.so")
+
+MBG_EOVERFLOW = -8
+
+
+class MbgError(RuntimeError):
+    pass
+
+
+class Sai(C.Structure):
+    _fields_ = [("x0", C.c_uint64), ("x1", C.c_uint64), ("size", C.c_uint64), ("info", C.c_uint64)]
+
+
+class BwtDesc(C.Structure):
+    _fields_ = [("primary", C.c_uint64), ("L2", C.c_uint64 * 5), ("seq_len", C.c_uint64),
+                ("data", C.POINTER(C.c_uint64)), ("sa_bit", C.c_uint32), ("n_sa", C.c_uint64),
+                ("sa", C.POINTER(C.c_uint64))]
+
+
+class L2bDesc(C.Structure):
+    _fields_ = [("tot_len", C.c_uint64), ("n_ctg", C.c_uint64), ("ctg_len", C.POINTER(C.c_uint64)),
+                ("n_pac", C.c_uint64), ("pac", C.POINTER(C.c_uint64)), ("n_ambi", C.c_uint64),
+                ("ambi", C.POINTER(C.c_uint64))]
+
+
+class IdxInfo(C.Structure):
+    _fields_ = [("seq_len", C.c_uint64), ("primary", C.c_uint64), ("tot_len", C.c_uint64), ("n_ctg", C.c_uint64),
+                ("pre_len", C.c_int32), ("sa_bit", C.c_int32), ("device_bytes", C.c_uint64)]
+
+
+DP_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"),
+                         ("q", "i1"), ("e", "i1"), ("q2", "i1"), ("e2", "i1"),
+                         ("w", "<i4"), ("zdrop", "<i4"), ("end_bonus", "<i4"), ("flag", "<i4"),
+                         ("pad", "<i4"), ("cigar_off", "<i8")])
+assert DP_JOB_DTYPE.itemsize == 56
+DP_RES_FIELDS = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "score", "reach_end",
+                 "n_cigar", "cells")
+DP_RES_DTYPE = np.dtype([(f, "<i4") for f in DP_RES_FIELDS])
+SEED_REC_DTYPE = np.dtype([("x0", "<u8"), ("x1", "<u8"), ("size", "<u8"), ("info", "<u8"), ("rid", "<u4"),
+                           ("pass", "<u4")])
+
+_P = C.c_void_p
+_LIB = None
+
+
+def lib():
+    """Load libmbg.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBMBG):
+            raise MbgError(f"{LIBMBG} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIBMBG)
+        L.mbg_strerror.restype = C.c_char_p
+        L.mbg_strerror.argtypes = [C.c_int]
+        L.mbg_last_hip_error.restype = C.c_char_p
+        L.mbg_init.argtypes = [C.c_int, C.POINTER(_P)]
+        L.mbg_destroy.argtypes = [_P]
+        L.mbg_idx_load.argtypes = [_P, C.c_char_p, C.c_int32]
+        L.mbg_bwt_upload.argtypes = [_P, C.POINTER(BwtDesc), C.c_int32]
+        L.mbg_l2b_upload.argtypes = [_P, C.POINTER(L2bDesc)]
+        L.mbg_idx_info.argtypes = [_P, C.POINTER(IdxInfo)]
+        L.mbg_pre_copy.argtypes = [_P, _P, C.c_uint64]
+        L.mbg_rank2a.argtypes = [_P, C.c_int64, _P, _P, _P, _P]
+        L.mbg_extend.argtypes = [_P, C.c_int64, _P, _P, _P]
+        L.mbg_seed_batch.argtypes = [_P, C.c_int32, _P, _P, C.c_int32, C.c_int32, C.POINTER(_P), _P]
+        L.mbg_sa_batch.argtypes = [_P, C.c_int64, _P]
+        for fn in (L.mbg_extd2_batch, L.mbg_extz2_batch):
+            fn.argtypes = [_P, C.c_int32, _P, _P, C.c_int64, _P, C.c_int64, C.c_int8, _P, _P, _P, C.c_int64]
+        L.mbg_getseq.argtypes = [_P, C.c_int64, _P, _P, _P, _P, _P, C.c_int64, _P]
+        L.mbg_seed_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, C.c_int64,
+                                         C.POINTER(C.c_int64)]
+        L.mbg_seed_batch_dev_async.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, C.c_int64, _P]
+        L.mbg_sa_batch_dev.argtypes = [_P, _P, C.c_int64, _P]
+        L.mbg_extd2_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, _P, _P, C.c_int32, C.c_int32,
+                                          C.c_int32]
+        _LIB = L
+    return _LIB
+
+
+EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mbg_idx_load", "mbg_bwt_upload",
+            "mbg_l2b_upload", "mbg_idx_info", "mbg_pre_copy", "mbg_rank2a", "mbg_extend", "mbg_seed_batch",
+            "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
+            "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev")
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        L = lib()
+        msg = L.mbg_strerror(rc).decode()
+        if rc == -2:
+            msg += ": " + L.mbg_last_hip_error().decode()
+        raise MbgError(f"{what}: {msg} ({rc})")
+
+
+def pack_seqs(seqs):
+    """Concatenate nt4 reads -> (bytes, int64 offsets[n+1], int32 lengths[n])."""
+    lens = np.array([len(s) for s in seqs], np.int32)
+    off = np.zeros(len(seqs) + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    buf = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if len(seqs) else np.zeros(0, np.uint8)
+    return np.ascontiguousarray(buf), off, lens
+
+
+@dataclass
+class DpBatchResult:
+    res: np.ndarray       # DP_RES_DTYPE per job
+    cigar: np.ndarray     # uint32 slots; job i's CIGAR at cigar_off[i] : cigar_off[i] + res[i].n_cigar
+    cigar_off: np.ndarray
+
+    def cigar_of(self, i):
+        o = int(self.cigar_off[i])
+        return [int(c) for c in self.cigar[o:o + int(self.res["n_cigar"][i])]]
+
+
+class Context:
+    """A device context holding one resident index (HBM) and the library's scratch pools."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        check(self.L.mbg_init(device, C.byref(h)), "mbg_init")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mbg_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- index -------------------------------------------------------------------------------
+    def idx_load(self, prefix: str, pre_len: int = 10):
+        check(self.L.mbg_idx_load(self.h, prefix.encode(), pre_len), f"mbg_idx_load({prefix})")
+        return self.info()
+
+    def bwt_upload(self, primary, L2, seq_len, data: np.ndarray, sa_bit, sa: np.ndarray, pre_len=10):
+        data = np.ascontiguousarray(data, np.uint64)
+        sa = np.ascontiguousarray(sa, np.uint64)
+        d = BwtDesc(int(primary), (C.c_uint64 * 5)(*[int(v) for v in L2]), int(seq_len),
+                    data.ctypes.data_as(C.POINTER(C.c_uint64)), int(sa_bit), len(sa), sa.ctypes.data_as(C.POINTER(C.c_uint64)))
+        check(self.L.mbg_bwt_upload(self.h, C.byref(d), pre_len), "mbg_bwt_upload")
+
+    def l2b_upload(self, ctg_len, pac: np.ndarray, ambi: np.ndarray | None = None):
+        ctg_len = np.ascontiguousarray(ctg_len, np.uint64)
+        pac = np.ascontiguousarray(pac, np.uint64)
+        ambi = np.zeros((0, 2), np.uint64) if ambi is None else np.ascontiguousarray(ambi, np.uint64)
+        d = L2bDesc(int(ctg_len.sum()), len(ctg_len), ctg_len.ctypes.data_as(C.POINTER(C.c_uint64)), len(pac),
+                    pac.ctypes.data_as(C.POINTER(C.c_uint64)), len(ambi), ambi.ctypes.data_as(C.POINTER(C.c_uint64)))
+        check(self.L.mbg_l2b_upload(self.h, C.byref(d)), "mbg_l2b_upload")
+
+    def info(self) -> dict:
+        i = IdxInfo()
+        check(self.L.mbg_idx_info(self.h, C.byref(i)), "mbg_idx_info")
+        return {f: getattr(i, f) for f, _ in IdxInfo._fields_}
+
+    def pre_cache(self) -> np.ndarray:
+        n = 1 << (2 * self.info()["pre_len"])
+        out = np.zeros((n, 4), np.uint64)
+        check(self.L.mbg_pre_copy(self.h, _ptr(out), n), "mbg_pre_copy")
+        return out
+
+    # ---- FM-index primitives -----------------------------------------------------------------
+    def bwt_rank2a(self, k, l):
+        k = np.ascontiguousarray(k, np.uint64)
+        l = np.ascontiguousarray(l, np.uint64)
+        ck = np.zeros((len(k), 4), np.uint64)
+        cl = np.zeros((len(k), 4), np.uint64)
+        check(self.L.mbg_rank2a(self.h, len(k), _ptr(k), _ptr(l), _ptr(ck), _ptr(cl)), "mbg_rank2a")
+        return ck, cl
+
+    def bwt_extend(self, ik: np.ndarray, is_back: np.ndarray):
+        ik = np.ascontiguousarray(ik, np.uint64).reshape(-1, 4)
+        ib = np.ascontiguousarray(is_back, np.uint8)
+        ok = np.zeros((len(ik), 4, 4), np.uint64)
+        check(self.L.mbg_extend(self.h, len(ik), _ptr(ik), _ptr(ib), _ptr(ok)), "mbg_extend")
+        return ok
+
+    def seed_intv_batch(self, seqs, min_len=19, max_sub_occ=10):
+        """Per read, an (n,4) uint64 array of (x0, x1, size, info); order within a read unspecified."""
+        n = len(seqs)
+        lens = np.array([len(s) for s in seqs], np.int32)
+        keep = [np.ascontiguousarray(s, np.uint8) for s in seqs]
+        ptrs = (C.c_void_p * max(n, 1))(*[s.ctypes.data if len(s) else None for s in keep])
+        out = C.c_void_p()
+        off = np.zeros(n + 1, np.int64)
+        check(self.L.mbg_seed_batch(self.h, n, _ptr(lens), ptrs, min_len, max_sub_occ, C.byref(out), _ptr(off)),
+              "mbg_seed_batch")
+        tot = int(off[-1])
+        res = []
+        try:
+            flat = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_uint64)), shape=(max(tot, 1) * 4,))
+            flat = flat[: tot * 4].reshape(tot, 4).copy()
+        finally:
+            C.CDLL(None).free(out)
+        for i in range(n):
+            res.append(flat[off[i]:off[i + 1]])
+        return res
+
+    def bwt_sa_batch(self, x):
+        x = np.ascontiguousarray(x, np.uint64).copy()
+        if len(x):
+            check(self.L.mbg_sa_batch(self.h, len(x), _ptr(x)), "mbg_sa_batch")
+        return x
+
+    def l2b_getseq(self, tid, st, en):
+        tid = np.ascontiguousarray(tid, np.int64)
+        st = np.ascontiguousarray(st, np.int64)
+        en = np.ascontiguousarray(en, np.int64)
+        span = np.maximum(en - np.maximum(st, 0), 0)
+        off = np.zeros(len(tid), np.int64)
+        if len(tid) > 1:
+            np.cumsum(span[:-1], out=off[1:])
+        out = np.zeros(int(span.sum()) + 1, np.uint8)
+        ln = np.zeros(len(tid), np.int64)
+        check(self.L.mbg_getseq(self.h, len(tid), _ptr(tid), _ptr(st), _ptr(en), _ptr(off), _ptr(out), len(out),
+                                _ptr(ln)), "mbg_getseq")
+        return [out[off[i]:off[i] + max(ln[i], 0)].copy() if ln[i] >= 0 else None for i in range(len(tid))]
+
+    # ---- DP ----------------------------------------------------------------------------------
+    def _dp(self, fn, name, queries, targets, params, mat, m=5):
+        n = len(queries)
+        qbuf, qoff, qlen = pack_seqs(queries)
+        tbuf, toff, tlen = pack_seqs(targets)
+        jobs = np.zeros(n, DP_JOB_DTYPE)
+        jobs["qoff"] = qoff[:-1]
+        jobs["toff"] = toff[:-1]
+        jobs["qlen"] = qlen
+        jobs["tlen"] = tlen
+        for f in ("q", "e", "q2", "e2", "w", "zdrop", "end_bonus", "flag"):
+            jobs[f] = params[f]
+        span = qlen.astype(np.int64) + tlen
+        coff = np.zeros(n, np.int64)
+        if n > 1:
+            np.cumsum(span[:-1], out=coff[1:])
+        jobs["cigar_off"] = coff
+        res = np.zeros(n, DP_RES_DTYPE)
+        cap = int(span.sum()) + 1
+        cig = np.zeros(cap, np.uint32)
+        mat = np.ascontiguousarray(mat, np.int8)
+        qb = qbuf if len(qbuf) else np.zeros(1, np.uint8)
+        tb = tbuf if len(tbuf) else np.zeros(1, np.uint8)
+        check(fn(self.h, n, _ptr(jobs), _ptr(qb), len(qbuf), _ptr(tb), len(tbuf), m, _ptr(mat), _ptr(res), _ptr(cig),
+                 cap), name)
+        return DpBatchResult(res, cig, coff)
+
+    def ksw_extd2(self, queries, targets, params, mat, m=5) -> DpBatchResult:
+        """params: dict of per-job arrays/scalars q, e, q2, e2, w, zdrop, end_bonus, flag."""
+        return self._dp(self.L.mbg_extd2_batch, "mbg_extd2_batch", queries, targets, params, mat, m)
+
+    def ksw_extz2(self, queries, targets, params, mat, m=5) -> DpBatchResult:
+        p = dict(params)
+        p.setdefault("q2", 0)
+        p.setdefault("e2", 0)
+        return self._dp(self.L.mbg_extz2_batch, "mbg_extz2_batch", queries, targets, p, mat, m)

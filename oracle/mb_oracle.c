@@ -112,15 +112,44 @@ static inline int bwt_b0(const orc_bwt_t *b, uint64_t k) /* bwt.c:119 */
 	return b->data[(k >> 7 << 3) + 4 + ((k & 127) >> 5)] >> ((k & 31) << 1) & 3;
 }
 
+/* counts of each base among positions [0, o] of the 16-symbol group g (0..7) of block p */
+static void grp_count(const uint64_t *p, int g, int o, uint64_t cnt[4])
+{
+	uint64_t w = p[4 + (g >> 1)] >> ((g & 1) * 32);
+	int c;
+	w &= 0xffffffffULL;
+	for (c = 0; c < 4; ++c) cnt[c] += sym_count(w, c, o + 1);
+}
+
 void orc_rank2a(const orc_bwt_t *b, uint64_t k, uint64_t l, uint64_t ck[4], uint64_t cl[4]) /* bwt.c:195-231 */
 {
-	orc_rank1a(b, k, ck);
-	/* bwt.c:205-209: the l==k+1 shortcut adds the symbol at $-removed index k-(k>primary).  For
-	 * k==primary that row holds '$', so the shortcut differs from rank1a(l): keep its result. */
-	if (l == k + 1 && k != 0 && k == b->primary) {
+	uint64_t k1 = k - 1, l1 = l - 1;
+	k1 -= (k1 >= b->primary);
+	l1 -= (l1 >= b->primary);
+	if (k1 >> 7 != l1 >> 7 || k == 0 || l == 0) {
+		orc_rank1a(b, k, ck);
+		orc_rank1a(b, l, cl);
+	} else if (l - k == 1) {
+		/* bwt.c:205-209: adds the symbol at $-removed index k-(k>primary); at k==primary that row
+		 * holds '$', so this differs from rank1a(l) */
+		orc_rank1a(b, k, ck);
 		memcpy(cl, ck, 32);
-		++cl[bwt_b0(b, k)];
-	} else orc_rank1a(b, l, cl);
+		++cl[bwt_b0(b, k - (k > b->primary))];
+	} else {
+		/* bwt.c:210-230, one block.  Counts advance in 16-symbol groups from the group of k1; for
+		 * l1 < k1 (never on the mapping path, which ranks x and x+size) the l1 count is taken inside
+		 * k1's group, as the reference does. */
+		const uint64_t *p = b->data + (k1 >> 7 << 3);
+		int c, half = (k1 & 127) >= 64, g, gk = (int)((k1 & 127) >> 4), gl = (int)((l1 & 127) >> 4);
+		uint64_t base[4], x[4] = {0, 0, 0, 0}, y[4];
+		for (c = 0; c < 4; ++c) base[c] = (p[c] & M56) + (half ? p[c] >> 56 : 0);
+		for (g = half ? 4 : 0; g < gk; ++g) grp_count(p, g, 15, x);
+		memcpy(y, x, 32);
+		grp_count(p, gk, (int)(k1 & 15), x);
+		for (g = gk; g < gl; ++g) grp_count(p, g, 15, y);
+		grp_count(p, g, (int)(l1 & 15), y);
+		for (c = 0; c < 4; ++c) ck[c] = base[c] + x[c], cl[c] = base[c] + y[c];
+	}
 }
 
 void orc_extend(const orc_bwt_t *b, const orc_sai_t *ik, orc_sai_t ok[4], int is_back) /* bwt.c:237-250 */
@@ -384,7 +413,7 @@ int orc_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, i
 {
 	int r, t, n_col_, tlen_, qlen_, last_st, last_en, last_max_H = 0, max_sc, min_sc, long_thres, long_diff, cells = 0;
 	int with_cigar = !(flag & 0x01), approx_max = !!(flag & 0x08), right = !!(flag & 0x02), generic = !!(flag & 0x04);
-	int32_t *H = 0, H0 = 0, last_H0_t = 0;
+	int32_t *H = 0, H0 = 0, last_H0_t = 0, qe0 = q + e; /* scalar q+e is taken before the swap (ksw2_extd2_sse.c:60) */
 	int8_t qe, qe2, sc_mch, sc_mis, sc_N, m1;
 	uint8_t *mem, *sf, *qr, *p = 0;
 	int8_t *u, *v, *x, *y, *x2, *y2, *s;
@@ -520,7 +549,7 @@ int orc_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, i
 		if (!approx_max) {
 			int32_t max_H, max_t;
 			if (r > 0) exact_max(H, (uint8_t*)v, (uint8_t*)u, 1, st0, en0, 0, &max_H, &max_t);
-			else H[0] = v[0] - qe, max_H = H[0], max_t = 0;
+			else H[0] = v[0] - qe0, max_H = H[0], max_t = 0;
 			if (en0 == tlen - 1 && H[en0] > ez->mte) ez->mte = H[en0], ez->mte_q = r - en0;
 			if (r - st0 == qlen - 1 && H[st0] > ez->mqe) ez->mqe = H[st0], ez->mqe_t = st0;
 			if (apply_zdrop(ez, max_H, r, max_t, zdrop, e2)) break;
@@ -543,7 +572,7 @@ int orc_extd2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, i
 				} else {
 					++last_H0_t, H0 += u[last_H0_t];
 				}
-			} else H0 = v[0] - qe, last_H0_t = 0;
+			} else H0 = v[0] - qe0, last_H0_t = 0;
 			if ((flag & 0x10) && apply_zdrop(ez, H0, r, last_H0_t, zdrop, e2)) break;
 			if (r == qlen + tlen - 2 && en0 == tlen - 1) ez->score = H0;
 		}
@@ -729,4 +758,36 @@ int orc_extz2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, i
 	}
 	free(mem); free(H); free(p); free(off);
 	return cg.overflow ? -1 : cells;
+}
+
+/* main.c:164-200 (`minibwa bench`): checksum over n splitmix64-drawn queries; type 0 = rank2a,
+ * 1 = sa, 2 = batched sa over [k, k+intv) */
+static inline uint64_t orc_sm64(uint64_t *x)
+{
+	uint64_t z = ((*x) += 0x9e3779b97f4a7c15ULL);
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+	return z ^ (z >> 31);
+}
+
+uint64_t orc_bench(const orc_bwt_t *b, int type, int64_t n, int intv)
+{
+	uint64_t x = 11, cs = 1;
+	int64_t i;
+	for (i = 0; i < n; ++i) {
+		if (type == 0) {
+			uint64_t k = orc_sm64(&x) % b->seq_len, l = orc_sm64(&x) % b->seq_len, ck[4], cl[4];
+			orc_rank2a(b, k, l, ck, cl);
+			cs = cs * ck[1] + cl[0];
+		} else if (type == 1) {
+			uint64_t k = orc_sm64(&x) % b->seq_len;
+			cs = cs * 0xbf58476d1ce4e5b9ULL ^ orc_sa(b, k);
+		} else {
+			uint64_t j, xr = 0, k = orc_sm64(&x) % b->seq_len;
+			uint64_t l = k + intv < b->seq_len ? k + intv : b->seq_len;
+			for (j = k; j < l; ++j) xr ^= orc_sa(b, j);
+			cs = cs * 0xbf58476d1ce4e5b9ULL ^ xr;
+		}
+	}
+	return cs;
 }

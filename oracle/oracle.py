@@ -1,0 +1,336 @@
+"""ctypes bindings to the CPU checkers (TEST INFRASTRUCTURE ONLY).
+
+Two libraries, both built by ``oracle/Makefile``:
+
+* ``oracle/liboracle.so`` -- the C restatement in ``oracle/mb_oracle.c`` (the oracle);
+* ``oracle/_ref/libmbref.so`` -- the reference's own sources compiled where they lie (present
+  wherever ``oracle/_ref`` was built or copied; absent otherwise).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBORACLE = os.path.join(HERE, "liboracle.so")
+LIBREF = os.path.join(HERE, "_ref", "libmbref.so")
+REF_CLI = os.path.join(HERE, "_ref", "minibwa")
+
+
+class OrcSai(C.Structure):
+    _fields_ = [("x0", C.c_uint64), ("x1", C.c_uint64), ("size", C.c_uint64), ("info", C.c_uint64)]
+
+
+class OrcBwt(C.Structure):
+    _fields_ = [("primary", C.c_uint64), ("L2", C.c_uint64 * 5), ("seq_len", C.c_uint64),
+                ("data_len", C.c_uint64), ("data", C.POINTER(C.c_uint64)), ("sa_bit", C.c_uint32),
+                ("n_sa", C.c_uint64), ("sa", C.POINTER(C.c_uint64)), ("pre_len", C.c_int32),
+                ("pre", C.POINTER(OrcSai))]
+
+
+class OrcEz(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte",
+                                         "mte_q", "score", "reach_end", "n_cigar")]
+
+
+EZ_FIELDS = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "score", "reach_end")
+
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C")
+_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C")
+
+
+def _load_oracle():
+    if not os.path.exists(LIBORACLE):
+        raise RuntimeError(f"{LIBORACLE} is not built (make -C oracle)")
+    L = C.CDLL(LIBORACLE)
+    L.orc_bwt_load.restype = C.POINTER(OrcBwt)
+    L.orc_bwt_load.argtypes = [C.c_char_p]
+    L.orc_bwt_destroy.argtypes = [C.POINTER(OrcBwt)]
+    L.orc_bwt_cache.argtypes = [C.POINTER(OrcBwt), C.c_int32]
+    L.orc_rank2a.argtypes = [C.POINTER(OrcBwt), C.c_uint64, C.c_uint64, _u64p, _u64p]
+    L.orc_extend.argtypes = [C.POINTER(OrcBwt), C.POINTER(OrcSai), C.POINTER(OrcSai), C.c_int]
+    L.orc_seed_intv.restype = C.c_int64
+    L.orc_seed_intv.argtypes = [C.POINTER(OrcBwt), C.c_int32, _u8p, C.c_int32, C.c_int32, C.POINTER(OrcSai), C.c_int64]
+    L.orc_sa.restype = C.c_uint64
+    L.orc_sa.argtypes = [C.POINTER(OrcBwt), C.c_uint64]
+    L.orc_sa_batch.argtypes = [C.POINTER(OrcBwt), C.c_int64, _u64p]
+    dp_common = [C.c_int, _u8p, C.c_int, _u8p, C.c_int8, _i8p]
+    L.orc_extd2.restype = C.c_int
+    L.orc_extd2.argtypes = dp_common + [C.c_int8] * 4 + [C.c_int] * 4 + [C.POINTER(OrcEz), _u32p, C.c_int]
+    L.orc_extz2.restype = C.c_int
+    L.orc_extz2.argtypes = dp_common + [C.c_int8] * 2 + [C.c_int] * 4 + [C.POINTER(OrcEz), _u32p, C.c_int]
+    L.orc_bench.restype = C.c_uint64
+    L.orc_bench.argtypes = [C.POINTER(OrcBwt), C.c_int, C.c_int64, C.c_int]
+    L.orc_dp_cells.restype = C.c_int64
+    L.orc_dp_cells.argtypes = [C.c_int, C.c_int, C.c_int]
+    return L
+
+
+_ORC = None
+
+
+def orc():
+    global _ORC
+    if _ORC is None:
+        _ORC = _load_oracle()
+    return _ORC
+
+
+class OracleIndex:
+    """An .mbw loaded by the restatement (bwt.c:648-683) plus its 10-mer cache (map-algo.c:36)."""
+
+    def __init__(self, mbw_path: str, pre_len: int = 10):
+        self.L = orc()
+        self.b = self.L.orc_bwt_load(mbw_path.encode())
+        if not self.b:
+            raise IOError(f"cannot load {mbw_path}")
+        if pre_len > 0:
+            self.L.orc_bwt_cache(self.b, pre_len)
+        bb = self.b.contents
+        self.primary = bb.primary
+        self.seq_len = bb.seq_len
+        self.L2 = list(bb.L2)
+
+    def close(self):
+        if self.b:
+            self.L.orc_bwt_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rank2a(self, k, l):
+        ck = np.zeros(4, np.uint64)
+        cl = np.zeros(4, np.uint64)
+        self.L.orc_rank2a(self.b, int(k), int(l), ck, cl)
+        return ck, cl
+
+    def extend(self, ik, is_back):
+        i = OrcSai(*[int(v) for v in ik])
+        ok = (OrcSai * 4)()
+        self.L.orc_extend(self.b, C.byref(i), ok, int(is_back))
+        return [(o.x0, o.x1, o.size) for o in ok]
+
+    def seed(self, seq: np.ndarray, min_len=19, max_sub_occ=10):
+        """seed.c:24-54 -> array (n,4) uint64 of (x0, x1, size, info)."""
+        seq = np.ascontiguousarray(seq, np.uint8)
+        cap = 4 * len(seq) + 64
+        buf = (OrcSai * cap)()
+        n = self.L.orc_seed_intv(self.b, len(seq), seq, min_len, max_sub_occ, buf, cap)
+        assert n <= cap
+        out = np.frombuffer(buf, dtype=np.uint64).reshape(cap, 4)[:n].copy()
+        return out
+
+    def sa(self, rows):
+        x = np.ascontiguousarray(rows, np.uint64).copy()
+        self.L.orc_sa_batch(self.b, len(x), x)
+        return x
+
+    def bench_checksum(self, kind: str, n=1000000, intv=20) -> str:
+        """`minibwa bench -b 2a|sa|msa` checksum (main.c:164-200) as lower-case hex."""
+        t = {"2a": 0, "sa": 1, "msa": 2}[kind]
+        return "%x" % self.L.orc_bench(self.b, t, n, intv)
+
+
+def _dp_result(ez, cig, ret):
+    d = {f: int(getattr(ez, f)) for f in EZ_FIELDS}
+    d["cigar"] = [int(c) for c in cig[: ez.n_cigar]]
+    d["cells"] = int(ret)
+    return d
+
+
+def extd2(query, target, mat, q, e, q2, e2, w, zdrop, end_bonus, flag, m=5):
+    query = np.ascontiguousarray(query, np.uint8)
+    target = np.ascontiguousarray(target, np.uint8)
+    mat = np.ascontiguousarray(mat, np.int8)
+    ez = OrcEz()
+    cap = len(query) + len(target) + 8
+    cig = np.zeros(cap, np.uint32)
+    ret = orc().orc_extd2(len(query), query, len(target), target, m, mat, q, e, q2, e2, w, zdrop, end_bonus,
+                          flag, C.byref(ez), cig, cap)
+    return _dp_result(ez, cig, ret)
+
+
+def extz2(query, target, mat, q, e, w, zdrop, end_bonus, flag, m=5):
+    query = np.ascontiguousarray(query, np.uint8)
+    target = np.ascontiguousarray(target, np.uint8)
+    mat = np.ascontiguousarray(mat, np.int8)
+    ez = OrcEz()
+    cap = len(query) + len(target) + 8
+    cig = np.zeros(cap, np.uint32)
+    ret = orc().orc_extz2(len(query), query, len(target), target, m, mat, q, e, w, zdrop, end_bonus, flag,
+                          C.byref(ez), cig, cap)
+    return _dp_result(ez, cig, ret)
+
+
+def gen_nt4_mat(a=2, b=8, b_ts=0, b_ambi=1, mt=0):
+    """ksw2.h:199-219."""
+    m = 5
+    a = abs(a)
+    b = -abs(b)
+    b_ambi = -abs(b_ambi)
+    mat = np.zeros(25, np.int8)
+    for i in range(m - 1):
+        for j in range(m - 1):
+            mat[i * m + j] = a if i == j else b
+        mat[i * m + m - 1] = b_ambi
+    for j in range(m):
+        mat[(m - 1) * m + j] = b_ambi
+    if b_ts != 0 and b_ts != b:
+        bt = -abs(b_ts)
+        mat[0 * m + 2] = mat[1 * m + 3] = mat[2 * m + 0] = mat[3 * m + 1] = bt
+    if mt == 1:
+        mat[1 * m + 3] = a
+    elif mt == 2:
+        mat[2 * m + 0] = a
+    return mat
+
+
+# ---------------------------------------------------------------------------------------------
+# The compiled reference (oracle/_ref/libmbref.so)
+# ---------------------------------------------------------------------------------------------
+
+class RefSaiV(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("m", C.c_size_t), ("a", C.POINTER(OrcSai))]
+
+
+class RefExtz(C.Structure):
+    _fields_ = [("maxz", C.c_uint32), ("max_q", C.c_int), ("max_t", C.c_int), ("mqe", C.c_int),
+                ("mqe_t", C.c_int), ("mte", C.c_int), ("mte_q", C.c_int), ("score", C.c_int),
+                ("m_cigar", C.c_int), ("n_cigar", C.c_int), ("reach_end", C.c_int),
+                ("cigar", C.POINTER(C.c_uint32))]
+
+
+def have_ref() -> bool:
+    return os.path.exists(LIBREF)
+
+
+_REF = None
+_LIBC = None
+
+
+def ref():
+    global _REF, _LIBC
+    if _REF is None:
+        if not have_ref():
+            raise RuntimeError(f"{LIBREF} is not built (make -C oracle ref)")
+        L = C.CDLL(LIBREF)
+        L.mb_bwt_load.restype = C.c_void_p
+        L.mb_bwt_load.argtypes = [C.c_char_p]
+        L.mb_bwt_destroy.argtypes = [C.c_void_p]
+        L.mb_bwt_cache.argtypes = [C.c_void_p, C.c_int32]
+        L.mb_bwt_rank2a.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, _u64p, _u64p]
+        L.mb_bwt_extend.argtypes = [C.c_void_p, C.POINTER(OrcSai), C.POINTER(OrcSai), C.c_int]
+        L.mb_bwt_sa.restype = C.c_uint64
+        L.mb_bwt_sa.argtypes = [C.c_void_p, C.c_uint64]
+        L.mb_bwt_sa_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, _u64p]
+        L.mb_seed_intv_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
+                                         C.POINTER(C.POINTER(C.c_uint8)), C.c_int32, C.c_int32, C.POINTER(RefSaiV)]
+        L.mb_seed_intv.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, _u8p, C.c_int32, C.c_int32, C.POINTER(RefSaiV)]
+        dp_common = [C.c_void_p, C.c_int, _u8p, C.c_int, _u8p, C.c_int8, _i8p]
+        L.ksw_extd2_sse.argtypes = dp_common + [C.c_int8] * 4 + [C.c_int] * 4 + [C.POINTER(RefExtz)]
+        L.ksw_extz2_sse.argtypes = dp_common + [C.c_int8] * 2 + [C.c_int] * 4 + [C.POINTER(RefExtz)]
+        _LIBC = C.CDLL(None)
+        _LIBC.free.argtypes = [C.c_void_p]
+        _REF = L
+    return _REF
+
+
+class RefIndex:
+    """An .mbw loaded by the reference's own mb_bwt_load + mb_bwt_cache."""
+
+    def __init__(self, mbw_path: str, pre_len: int = 10):
+        self.L = ref()
+        self.b = self.L.mb_bwt_load(mbw_path.encode())
+        if not self.b:
+            raise IOError(f"cannot load {mbw_path}")
+        if pre_len > 0:
+            self.L.mb_bwt_cache(self.b, pre_len)
+
+    def close(self):
+        if self.b:
+            self.L.mb_bwt_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rank2a(self, k, l):
+        ck = np.zeros(4, np.uint64)
+        cl = np.zeros(4, np.uint64)
+        self.L.mb_bwt_rank2a(self.b, int(k), int(l), ck, cl)
+        return ck, cl
+
+    def extend(self, ik, is_back):
+        i = OrcSai(*[int(v) for v in ik])
+        ok = (OrcSai * 4)()
+        self.L.mb_bwt_extend(self.b, C.byref(i), ok, int(is_back))
+        return [(o.x0, o.x1, o.size) for o in ok]
+
+    def sa(self, rows):
+        x = np.ascontiguousarray(rows, np.uint64).copy()
+        if len(x):
+            self.L.mb_bwt_sa_batch(None, self.b, len(x), x)
+        return x
+
+    def seed_batch(self, seqs, min_len=19, max_sub_occ=10):
+        """seed.c:56-103 over a list of uint8 arrays -> list of (n,4) uint64 arrays."""
+        n = len(seqs)
+        lens = (C.c_int32 * n)(*[len(s) for s in seqs])
+        keep = [np.ascontiguousarray(s, np.uint8) for s in seqs]
+        ptrs = (C.POINTER(C.c_uint8) * n)(*[s.ctypes.data_as(C.POINTER(C.c_uint8)) for s in keep])
+        v = (RefSaiV * n)()
+        self.L.mb_seed_intv_batch(None, self.b, n, lens, ptrs, min_len, max_sub_occ, v)
+        out = []
+        for i in range(n):
+            k = v[i].n
+            if k:
+                a = np.ctypeslib.as_array(C.cast(v[i].a, C.POINTER(C.c_uint64)), shape=(k * 4,))
+                out.append(a.reshape(k, 4).copy())
+            else:
+                out.append(np.zeros((0, 4), np.uint64))
+            if v[i].a:
+                _LIBC.free(C.cast(v[i].a, C.c_void_p))
+        return out
+
+
+def _ref_dp(fn, args):
+    ez = RefExtz()
+    fn(None, *args, C.byref(ez))
+    d = {"max": ez.maxz & 0x7fffffff, "zdropped": ez.maxz >> 31}
+    for f in EZ_FIELDS[2:]:
+        d[f] = int(getattr(ez, f))
+    d["cigar"] = [int(ez.cigar[i]) for i in range(ez.n_cigar)] if ez.n_cigar > 0 else []
+    if ez.cigar:
+        _LIBC.free(C.cast(ez.cigar, C.c_void_p))
+    return d
+
+
+def ref_extd2(query, target, mat, q, e, q2, e2, w, zdrop, end_bonus, flag, m=5):
+    L = ref()
+    query = np.ascontiguousarray(query, np.uint8)
+    target = np.ascontiguousarray(target, np.uint8)
+    mat = np.ascontiguousarray(mat, np.int8)
+    return _ref_dp(L.ksw_extd2_sse, (len(query), query, len(target), target, m, mat, q, e, q2, e2, w, zdrop,
+                                     end_bonus, flag))
+
+
+def ref_extz2(query, target, mat, q, e, w, zdrop, end_bonus, flag, m=5):
+    L = ref()
+    query = np.ascontiguousarray(query, np.uint8)
+    target = np.ascontiguousarray(target, np.uint8)
+    mat = np.ascontiguousarray(mat, np.int8)
+    return _ref_dp(L.ksw_extz2_sse, (len(query), query, len(target), target, m, mat, q, e, w, zdrop, end_bonus,
+                                     flag))

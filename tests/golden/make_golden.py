@@ -1,0 +1,73 @@
+"""Write the committed golden fixtures (run in a container where oracle/_ref is built).
+
+Everything here comes from running the compiled reference (oracle/_ref/minibwa, built from the
+reference's own sources by oracle/Makefile):
+
+* ``chrM.mbw`` / ``chrM.l2b``   -- ``minibwa index`` of the reference's own test/chrM-human.fa.gz
+* ``cfg0.fa`` / ``cfg0.mbw`` / ``cfg0.l2b`` -- the 16,569 bp seeded uniform reference (splitmix64
+  seed 11) and its ``minibwa index``
+* ``golden.json`` -- known answers: ``minibwa bench -b 2a|sa|msa`` checksums on chrM, md5 of the
+  index files, and md5 of the SAM body (minus @PG) of ``minibwa map`` on the chrM read pairs and on
+  the config-0 read pairs (10,000 pairs from minibwa_amd.synth with seed 11).
+
+Usage: python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+REF_CLI = os.path.join(ROOT, "oracle", "_ref", "minibwa")
+
+
+def md5(path):
+    return hashlib.md5(open(path, "rb").read()).hexdigest()
+
+
+def sam_body_md5(sam_text):
+    body = "".join(l + "\n" for l in sam_text.split("\n") if l and not l.startswith("@PG"))
+    return hashlib.md5(body.encode()).hexdigest()
+
+
+def run(cmd, **kw):
+    return subprocess.run(cmd, check=True, capture_output=True, text=True, **kw)
+
+
+def main():
+    from minibwa_amd import synth
+    out = {}
+    pre = os.path.join(HERE, "chrM")
+    run([REF_CLI, "index", os.path.join(HERE, "chrM-human.fa.gz"), pre])
+    out["chrM.mbw.md5"] = md5(pre + ".mbw")
+    out["chrM.l2b.md5"] = md5(pre + ".l2b")
+    for b in ("2a", "sa", "msa"):
+        r = run([REF_CLI, "bench", "-b", b, "-n", "1000000", pre + ".mbw"])
+        cs = [l for l in r.stderr.split("\n") if l.startswith("checksum")][0].split("=")[1].strip()
+        out[f"bench.{b}.chrM"] = cs
+    sam = run([REF_CLI, "map", pre, os.path.join(HERE, "chrM-read_1.fa.gz"), os.path.join(HERE, "chrM-read_2.fa.gz")]).stdout
+    out["chrM.sam.body.md5"] = sam_body_md5(sam)
+
+    cfg0 = os.path.join(HERE, "cfg0")
+    ref = synth.reference(16569, seed=11, rep_frac=0.0)
+    synth.write_fasta(cfg0 + ".fa", ref, n_ctg=1, prefix="cfg0_")
+    run([REF_CLI, "index", cfg0 + ".fa", cfg0])
+    out["cfg0.mbw.md5"] = md5(cfg0 + ".mbw")
+    with tempfile.TemporaryDirectory() as td:
+        r1, r2 = synth.read_pairs(ref, 10000, seed=11)
+        f1, f2 = os.path.join(td, "r1.fq"), os.path.join(td, "r2.fq")
+        synth.write_pairs_fastq(f1, f2, r1, r2, prefix="c0_")
+        sam = run([REF_CLI, "map", cfg0, f1, f2]).stdout
+        out["cfg0.sam.body.md5"] = sam_body_md5(sam)
+        out["cfg0.sam.n_lines"] = sum(1 for l in sam.split("\n") if l and not l.startswith("@"))
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print(json.dumps(out, indent=1, sort_keys=True))
+
+
+if __name__ == "__main__":
+    main()

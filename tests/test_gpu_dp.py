@@ -1,0 +1,120 @@
+"""GPU parity of the banded DP kernels against ksw_extd2_sse / ksw_extz2_sse.
+
+Jobs are fuzzed over the flag variants the reference's callers use (0x00, 0x02 right-aligned gaps,
+0x08 approximate max, 0x40/0x42/0xc2 extension-only, 0x80 reversed CIGAR, 0x01 score only, 0x04
+generic scoring, 0x10 approximate z-drop), band widths, z-drop and end-bonus values, and the gap
+parameter sets including the q/q2 swap case.  Every ksw_extz_t field and the CIGAR must be equal.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FLAGS = [0x00, 0x01, 0x02, 0x08, 0x09, 0x40, 0x42, 0xc2, 0x80, 0x82, 0x48, 0x18, 0x04, 0x44, 0xc6]
+GAPS = [(12, 2, 24, 1), (12, 2, 23, 1), (4, 2, 24, 1), (24, 1, 12, 2), (6, 1, 6, 1)]
+FIELDS = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "score", "reach_end")
+
+
+def _mutate(rng, q, sub, indel):
+    out = []
+    for c in q:
+        r = rng.random()
+        if r < indel / 2:
+            continue
+        if r < indel:
+            out.append(int(rng.integers(0, 4)))
+            out.append(int(c))
+            continue
+        if rng.random() < sub:
+            c = (c + rng.integers(1, 4)) % 4
+        out.append(int(c))
+    return np.array(out, np.uint8)
+
+
+def fuzz_jobs(n, seed, qmax=300):
+    rng = np.random.default_rng(seed)
+    Q, T, P = [], [], []
+    for it in range(n):
+        ql = int(rng.integers(1, qmax))
+        q = rng.integers(0, 4, ql).astype(np.uint8)
+        if rng.random() < 0.05:
+            q[rng.integers(0, ql)] = 4
+        t = _mutate(rng, q, rng.choice([0.0, 0.02, 0.1, 0.3]), rng.choice([0.0, 0.005, 0.05]))
+        t = np.concatenate([t, rng.integers(0, 4, int(rng.integers(0, 60))).astype(np.uint8)])
+        if rng.random() < 0.3:
+            t = np.concatenate([rng.integers(0, 4, int(rng.integers(0, 60))).astype(np.uint8), t])
+        if len(t) == 0:
+            t = np.array([1], np.uint8)
+        qq, ee, q2, e2 = GAPS[it % len(GAPS)]
+        P.append(dict(flag=int(rng.choice(FLAGS)), w=int(rng.choice([-1, 1, 5, 17, 51, 101, 161])),
+                      zdrop=int(rng.choice([-1, 40, 172, 400])), end_bonus=int(rng.choice([-1, 0, 10])),
+                      q=qq, e=ee, q2=q2, e2=e2))
+        Q.append(q)
+        T.append(t)
+    return Q, T, P
+
+
+def _params(P):
+    return {k: np.array([p[k] for p in P]) for k in P[0]}
+
+
+def _check(out, Q, T, P, refn, dual):
+    from oracle import oracle as O
+    mat = O.gen_nt4_mat(2, 8, 0, 1)
+    for i in range(len(Q)):
+        p = P[i]
+        if dual:
+            r = O.ref_extd2(Q[i], T[i], mat, p["q"], p["e"], p["q2"], p["e2"], p["w"], p["zdrop"], p["end_bonus"], p["flag"])
+        else:
+            r = O.ref_extz2(Q[i], T[i], mat, p["q"], p["e"], p["w"], p["zdrop"], p["end_bonus"], p["flag"])
+        g = {f: int(out.res[f][i]) for f in FIELDS}
+        g["cigar"] = out.cigar_of(i)
+        rr = {f: r[f] for f in FIELDS}
+        rr["cigar"] = r["cigar"]
+        assert g == rr, (refn, i, len(Q[i]), len(T[i]), p)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_extd2_fuzz(gpu_ctx, seed):
+    from oracle import oracle as O
+    Q, T, P = fuzz_jobs(600, seed)
+    out = gpu_ctx.ksw_extd2(Q, T, _params(P), O.gen_nt4_mat(2, 8, 0, 1))
+    _check(out, Q, T, P, "extd2", True)
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_extz2_fuzz(gpu_ctx, seed):
+    from oracle import oracle as O
+    Q, T, P = fuzz_jobs(600, seed)
+    out = gpu_ctx.ksw_extz2(Q, T, _params(P), O.gen_nt4_mat(2, 8, 0, 1))
+    _check(out, Q, T, P, "extz2", False)
+
+
+def test_extd2_config2_shape(gpu_ctx):
+    """Config-2 jobs: q 100-300, 2% sub / 0.5% indel, 50 bp flank, w=101, q/e/q2/e2 = 12/2/24/1."""
+    from oracle import oracle as O
+    from minibwa_amd import synth
+    qb, qo, ql, tb, to, tl = synth.dp_jobs(400, seed=21)
+    Q = [qb[qo[i]:qo[i + 1]] for i in range(len(ql))]
+    T = [tb[to[i]:to[i + 1]] for i in range(len(tl))]
+    for flag, zd, eb in ((0x40, 172, 10), (0x00, -1, 0)):
+        P = [dict(flag=flag, w=101, zdrop=zd, end_bonus=eb, q=12, e=2, q2=24, e2=1)] * len(Q)
+        out = gpu_ctx.ksw_extd2(Q, T, _params(P), O.gen_nt4_mat(2, 8, 0, 1))
+        _check(out, Q, T, P, "cfg2", True)
+        assert int(out.res["cells"].sum()) > 0
+
+
+def test_dp_generic_matrix(gpu_ctx):
+    """KSW_EZ_GENERIC_SC with a transition-penalty matrix (ksw_gen_nt4_mat b_ts) and a C->T matrix."""
+    from oracle import oracle as O
+    Q, T, P = fuzz_jobs(200, 11, qmax=120)
+    for p in P:
+        p["flag"] |= 0x04
+    for mat in (O.gen_nt4_mat(2, 8, 4, 1), O.gen_nt4_mat(2, 8, 0, 1, mt=1)):
+        out = gpu_ctx.ksw_extd2(Q, T, _params(P), mat)
+        from oracle import oracle as OO
+        for i in range(len(Q)):
+            p = P[i]
+            r = OO.ref_extd2(Q[i], T[i], mat, p["q"], p["e"], p["q2"], p["e2"], p["w"], p["zdrop"], p["end_bonus"], p["flag"])
+            g = {f: int(out.res[f][i]) for f in FIELDS}
+            assert g == {f: r[f] for f in FIELDS} and out.cigar_of(i) == r["cigar"], i

@@ -1,0 +1,146 @@
+"""GPU parity of the FM-index kernels (rank/extend, k-mer cache, SA locate, two-pass seeding).
+
+Each test calls libmbg.so through its C ABI and compares with the compiled reference
+(oracle/_ref/libmbref.so: mb_bwt_rank2a, mb_bwt_extend, mb_bwt_sa_batch, mb_seed_intv_batch) and
+with the C restatement (oracle/liboracle.so).  Bit-exact integers throughout.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx_chrm(gpu_ctx, chrm_prefix):
+    gpu_ctx.idx_load(chrm_prefix, 10)
+    return gpu_ctx
+
+
+@pytest.fixture(scope="module")
+def ref_chrm(chrm_prefix):
+    from oracle import oracle as O
+    return O.RefIndex(chrm_prefix + ".mbw")
+
+
+@pytest.fixture(scope="module")
+def orc_chrm(chrm_prefix):
+    from oracle import oracle as O
+    return O.OracleIndex(chrm_prefix + ".mbw")
+
+
+def _edge_rows(n, primary):
+    e = [0, 1, 2, n - 1, n, n + 1, primary - 1, primary, primary + 1, primary + 2]
+    return sorted(set(v for v in e if 0 <= v <= n + 1))
+
+
+def test_rank2a(ctx_chrm, ref_chrm):
+    info = ctx_chrm.info()
+    n, primary = info["seq_len"], info["primary"]
+    rng = np.random.default_rng(7)
+    k = rng.integers(0, n + 1, 20000).astype(np.uint64)
+    l = np.minimum(k + rng.integers(0, 300, 20000).astype(np.uint64), n + 1)
+    # l == k+1 at k == primary exercises the reference's shortcut (bwt.c:205-209)
+    ed = _edge_rows(n, primary)
+    ek = np.array([a for a in ed for b in ed if a <= b], np.uint64)
+    el = np.array([b for a in ed for b in ed if a <= b], np.uint64)
+    k = np.concatenate([k, ek, [primary]]).astype(np.uint64)
+    l = np.concatenate([l, el, [primary + 1]]).astype(np.uint64)
+    gk, gl = ctx_chrm.bwt_rank2a(k, l)
+    for i in range(len(k)):
+        rk, rl = ref_chrm.rank2a(k[i], l[i])
+        assert np.array_equal(gk[i], rk) and np.array_equal(gl[i], rl), (i, k[i], l[i])
+
+
+def test_extend(ctx_chrm, ref_chrm):
+    rng = np.random.default_rng(8)
+    info = ctx_chrm.info()
+    n = info["seq_len"]
+    ik = []
+    for _ in range(5000):
+        a = int(rng.integers(0, n + 1))
+        s = int(rng.integers(0, min(200, n + 1 - a) + 1))
+        b = int(rng.integers(0, n + 1 - s + 1))
+        ik.append((a, b, s, 0))
+    ik = np.array(ik, np.uint64)
+    isb = rng.integers(0, 2, len(ik)).astype(np.uint8)
+    ok = ctx_chrm.bwt_extend(ik, isb)
+    for i in range(len(ik)):
+        # is_back=1 ranks x[0]; is_back=0 ranks x[1]: give the reference the same interval
+        r = ref_chrm.extend(ik[i], int(isb[i]))
+        for c in range(4):
+            assert tuple(int(v) for v in ok[i, c, :3]) == tuple(int(v) for v in r[c]), (i, c)
+
+
+def test_kmer_cache(ctx_chrm, orc_chrm):
+    g = ctx_chrm.pre_cache()
+    import ctypes as C
+    o = np.ctypeslib.as_array(C.cast(orc_chrm.b.contents.pre, C.POINTER(C.c_uint64)), shape=(4 << 20,)).reshape(-1, 4)
+    assert np.array_equal(g[:, :3], o[:, :3])
+
+
+def test_sa_all_rows(ctx_chrm, ref_chrm):
+    n = ctx_chrm.info()["seq_len"]
+    rows = np.arange(n + 1, dtype=np.uint64)
+    g = ctx_chrm.bwt_sa_batch(rows)
+    r = ref_chrm.sa(rows)
+    assert np.array_equal(g, r)
+    # the SA is a permutation of 0..n-1 plus the '$' row wrapped to 2^64-1+steps
+    ok = g[g < n]
+    assert len(np.unique(ok)) == len(ok)
+
+
+def test_sa_random_order(ctx_chrm, ref_chrm):
+    n = ctx_chrm.info()["seq_len"]
+    rng = np.random.default_rng(9)
+    rows = rng.integers(0, n + 1, 100000).astype(np.uint64)
+    assert np.array_equal(ctx_chrm.bwt_sa_batch(rows), ref_chrm.sa(rows))
+
+
+def _canon(a):
+    a = np.asarray(a, np.uint64).reshape(-1, 4)
+    if len(a) == 0:
+        return a
+    return a[np.lexsort((a[:, 3], a[:, 2], a[:, 1], a[:, 0]))]
+
+
+def test_seed_chrm_reads(ctx_chrm, ref_chrm, chrm_reads):
+    g = ctx_chrm.seed_intv_batch(chrm_reads, 19, 10)
+    r = ref_chrm.seed_batch(chrm_reads, 19, 10)
+    n_int = 0
+    for i in range(len(chrm_reads)):
+        assert np.array_equal(_canon(g[i]), _canon(r[i])), i
+        n_int += len(g[i])
+    assert n_int > len(chrm_reads)
+
+
+@pytest.mark.parametrize("min_len,max_sub_occ", [(19, 10), (11, 3), (10, 10), (25, 50)])
+def test_seed_params_and_edges(ctx_chrm, ref_chrm, chrm_reads, min_len, max_sub_occ):
+    rng = np.random.default_rng(min_len)
+    reads = [r.copy() for r in chrm_reads[:300]]
+    for r in reads[::7]:                       # N bases (code 4) force the stage-1 skips
+        r[rng.integers(0, len(r), 3)] = 4
+    reads += [np.zeros(0, np.uint8), np.array([1, 2, 3], np.uint8), rng.integers(0, 4, 1000).astype(np.uint8)]
+    g = ctx_chrm.seed_intv_batch(reads, min_len, max_sub_occ)
+    r = ref_chrm.seed_batch(reads, min_len, max_sub_occ)
+    for i in range(len(reads)):
+        assert np.array_equal(_canon(g[i]), _canon(r[i])), i
+
+
+def test_seed_cfg0(gpu_ctx):
+    """Config-0 reference: 2,000 read pairs from the seeded generator."""
+    from oracle import oracle as O
+    from minibwa_amd import synth
+    pre = os.path.join(GOLDEN, "cfg0")
+    gpu_ctx.idx_load(pre, 10)
+    ref = synth.reference(16569, seed=11, rep_frac=0.0)
+    r1, r2 = synth.read_pairs(ref, 2000, seed=11)
+    reads = list(r1) + list(r2)
+    ri = O.RefIndex(pre + ".mbw")
+    g = gpu_ctx.seed_intv_batch(reads)
+    r = ri.seed_batch(reads)
+    for i in range(len(reads)):
+        assert np.array_equal(_canon(g[i]), _canon(r[i])), i
