@@ -1,0 +1,224 @@
+"""Benchmark of the seeding + extension hot path on one MI355X.
+
+Workload (chromosome-scale seeding + extension): a 100 Mbp seeded synthetic reference (uniform ACGT
+plus 5% repeats: copies of 300-6,000 bp segments at 2% divergence; 4 contigs of 25 Mbp) and 1M read
+pairs 2x150 bp (fragment ~ Normal(400, 50), 1% substitutions, 0.1% indels with geometric length of
+mean 2), splitmix64 seed 11.  The index is built on the device before timing and stays resident in
+HBM, as do the reads.
+
+One step = one pass of the hot path over the 2M reads: two-pass SMEM seeding, SA locate of every
+interval's anchor rows, left/right extension jobs from each read's longest SMEM with the reference's
+ungapped fast path, and the banded dual-affine DP of the jobs that remain.  value = read pairs per
+second of that pass.
+
+cpu_baseline: the compiled reference's own functions (mb_seed_intv_batch, mb_anchor -- which
+includes mb_bwt_sa_batch -- and ksw_extd2_sse on the same extension jobs) timed on the host's cores
+over a sample of the same reads (oracle/_ref/cpu_hotpath).
+
+Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "aligned read pairs/s, 2x150bp vs 3.1Gbp synthetic ref (GPU vs CPU); p50/p99 lat"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ref-len", type=int, default=100_000_000)
+    ap.add_argument("--pairs", type=int, default=1_000_000)
+    ap.add_argument("--cpu-pairs", type=int, default=300_000, help="read pairs in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workdir", default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "Reads shard independently across GPUs with a replicated index; see DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from minibwa_amd import index as fmidx
+    from minibwa_amd import mbg, synth
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    ref = synth.reference(args.ref_len, seed=11, rep_frac=0.05, rep_min=300, rep_max=6000, rep_div=0.02)
+    log(f"reference {len(ref)} bp in {time.time() - t0:.1f}s")
+    work = args.workdir or tempfile.mkdtemp(prefix="mbg_bench_")
+    os.makedirs(work, exist_ok=True)
+    prefix = os.path.join(work, "ref")
+    t0 = time.time()
+    ix = fmidx.build(ref, prefix, n_ctg=4, ctg_prefix="chr", device="cuda")
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log(f"FM-index of {ix['seq_len']} rows built on the GPU in {time.time() - t0:.1f}s")
+    ctx = mbg.Context(0)
+    ctx.bwt_upload(ix["primary"], ix["L2"], ix["seq_len"], ix["data"], ix["sa_bit"], ix["sa"], pre_len=10)
+    ctx.l2b_upload(ix["ctg_len"], ix["pac"])
+    info = ctx.info()
+    log(f"index resident: {info['device_bytes'] / 1e9:.2f} GB")
+
+    t0 = time.time()
+    r1, r2 = synth.read_pairs(ref, args.pairs, seed=11, rlen=150, frag_mean=400, frag_sd=50, sub=0.01, indel=0.001)
+    rl = 150
+    reads = np.empty((2 * args.pairs, rl), np.uint8)
+    reads[0::2] = r1
+    reads[1::2] = r2
+    n_reads = reads.shape[0]
+    d_seq = torch.from_numpy(reads.reshape(-1)).to(dev)
+    d_off = torch.arange(0, n_reads + 1, dtype=torch.int64, device=dev) * rl
+    d_len = torch.full((n_reads,), rl, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    log(f"{args.pairs} pairs generated and resident in {time.time() - t0:.1f}s")
+    opt = mbg.adaptive_opt(rl)
+
+    def step(stats=False):
+        return ctx.map_step_dev(n_reads, d_off.data_ptr(), d_len.data_ptr(), d_seq.data_ptr(), rl, opt, collect_stats=stats)
+
+    # algorithmic work of one step (counters on; not timed)
+    st_stats, _ = step(stats=True)
+    work_stats = st_stats.as_dict()
+    log("step work:", json.dumps({k: work_stats[k] for k in ("n_rec", "n_rows", "n_anchored", "n_fast", "n_jobs", "cells",
+                                                            "occ_blocks", "kmer_lookups", "extend_steps", "lf_steps")}))
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    per = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st, out = step()
+        per.append(st.as_dict())
+        log(f"step {i}: {st.ms_total:.2f} ms device")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = args.steps * args.pairs / el
+
+    # roofline of the dominant kernel (seeding pass 1): algorithmic bytes = 64 B per occ block
+    # fetched + 32 B per k-mer cache lookup, over the kernel's HIP-event duration
+    ms_seed1 = float(np.mean([p["ms_seed1"] for p in per]))
+    bytes_seed1 = 64 * work_stats["occ_blocks"][0] + 32 * work_stats["kmer_lookups"][0]
+    achieved = bytes_seed1 / (ms_seed1 * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "smem_pass1_cfg1_pmc.json")
+    if os.path.exists(pmc):
+        try:
+            traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    phases = {k: round(float(np.mean([p[k] for p in per])), 3) for k in ("ms_seed1", "ms_seed2", "ms_rows", "ms_sa", "ms_jobs", "ms_dp", "ms_total")}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ctx, out, per[-1], reads, prefix, args, work)
+
+    res = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "read pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1000, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (FM-index ranks) + int8/int32 (DP)",
+        "data": "synthetic (seeded splitmix64; index built on the device)",
+        "config": {"workload": "seed+locate+extend pass, 1M pairs 2x150 bp vs 100 Mbp synthetic reference (5% repeats)",
+                   "ref_bp": args.ref_len, "pairs": args.pairs, "read_len": rl, "contigs": 4,
+                   "min_len": 19, "max_occ": 250, "ksw_w": opt.ksw_w, "zdrop": opt.zdrop},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "smem_kernel<1> (SMEM pass 1)", "algorithmic_bytes_per_launch": int(bytes_seed1),
+                     "avg_launch_ms": round(ms_seed1, 4)},
+        "cpu_baseline": cpu,
+        "phases_ms": phases,
+        "step_work": {k: work_stats[k] for k in ("n_rec", "n_rows", "n_anchored", "n_fast", "n_jobs", "cells", "lf_steps")},
+    }
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+def cpu_baseline(ctx, out, st, reads, prefix, args, work):
+    """Time the compiled reference's hot-path functions on a sample of the same reads."""
+    from minibwa_amd import mbg
+    exe = os.path.join(ROOT, "oracle", "_ref", "cpu_hotpath")
+    if not os.path.exists(exe):
+        log("cpu_baseline skipped: oracle/_ref/cpu_hotpath not built")
+        return None
+    n_s = min(args.cpu_pairs, args.pairs) * 2
+    rl = reads.shape[1]
+    rb = os.path.join(work, "cpu_reads.bin")
+    with open(rb, "wb") as f:
+        f.write(np.int64(n_s).tobytes())
+        f.write(np.full(n_s, rl, np.int32).tobytes())
+        f.write(np.ascontiguousarray(reads[:n_s]).tobytes())
+    n_jobs = st["n_jobs"]
+    jobs = ctx.to_host(out.jobs, mbg.DP_JOB_DTYPE, n_jobs)
+    meta = ctx.to_host(out.meta, mbg.JOB_META_DTYPE, n_jobs)
+    keep = np.nonzero(meta["rid"] < n_s)[0]
+    jobs = jobs[keep]
+    qb = ctx.to_host(out.qbuf, np.uint8, st["q_bytes"])
+    tb = ctx.to_host(out.tbuf, np.uint8, st["t_bytes"])
+    # repack the sample's jobs with their own sequence buffers
+    qparts, tparts, qo, to = [], [], 0, 0
+    j2 = jobs.copy()
+    for i in range(len(jobs)):
+        q = qb[jobs["qoff"][i]: jobs["qoff"][i] + jobs["qlen"][i]]
+        t = tb[jobs["toff"][i]: jobs["toff"][i] + jobs["tlen"][i]]
+        j2["qoff"][i], j2["toff"][i] = qo, to
+        qparts.append(q), tparts.append(t)
+        qo += len(q)
+        to += len(t)
+    qcat = np.concatenate(qparts) if qparts else np.zeros(0, np.uint8)
+    tcat = np.concatenate(tparts) if tparts else np.zeros(0, np.uint8)
+    jb = os.path.join(work, "cpu_jobs.bin")
+    with open(jb, "wb") as f:
+        f.write(np.int64(len(j2)).tobytes())
+        f.write(np.asarray(mbg.nt4_mat(), np.int8).tobytes())
+        f.write(j2.tobytes())
+        f.write(np.int64(len(qcat)).tobytes()); f.write(qcat.tobytes())
+        f.write(np.int64(len(tcat)).tobytes()); f.write(tcat.tobytes())
+    cores = host_cores()
+    log(f"cpu_baseline: {n_s // 2} pairs, {len(j2)} DP jobs, {cores} threads")
+    r = subprocess.run([exe, prefix, rb, jb, str(cores)], capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log("cpu_baseline failed:", r.stderr[-500:])
+        return None
+    d = json.loads(r.stdout.strip().split("\n")[-1])
+    tot = d["seed_anchor_s"] + d["dp_s"]
+    return {"value": round((n_s // 2) / tot, 1), "unit": "read pairs/s", "cores": cores, "kind": "reference",
+            "sample": f"first {n_s // 2} pairs of the workload: mb_seed_intv_batch + mb_anchor (incl. mb_bwt_sa_batch) "
+                      f"in 48-read sub-batches, then ksw_extd2_sse on the sample's {len(j2)} extension jobs; "
+                      f"{cores} threads, wall {tot:.2f}s",
+            "seed_anchor_s": d["seed_anchor_s"], "dp_s": d["dp_s"]}
+
+
+if __name__ == "__main__":
+    main()

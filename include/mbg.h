@@ -148,6 +148,53 @@ int mbg_extd2_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_jobs, const mbg_
                         const uint8_t *qseq, const uint8_t *tseq, int8_t m, const int8_t *mat_host,
                         mbg_dp_result_t *res, uint32_t *cigar, int32_t max_qlen, int32_t max_tlen, int32_t max_w);
 
+/* Synchronous copy of `bytes` from a device address returned by this library to host memory. */
+int mbg_copy_to_host(mbg_ctx_t *ctx, void *dst, const void *src_dev, size_t bytes);
+
+/* ---- seed -> locate -> extend step over device-resident reads --------------------------------- */
+
+/* Options of one step; defaults are minibwa's (options.c:5-46) with the adaptive band/z-drop for the
+ * batch's read length (options.c:90-116): ksw_w = (int)(bw*1.5+1) (align.c:564), zdrop = zdrop*a. */
+typedef struct {
+	int32_t min_len, max_sub_occ, max_occ;
+	int32_t a, b, q, e, q2, e2, end_bonus;
+	int32_t ksw_w, zdrop;
+	int8_t mat[25];                /* ksw_gen_nt4_mat(a, b, b_ts, b_ambi) */
+} mbg_map_opt_t;
+
+/* Per extension job: which read, which end (0 left, 1 right), strand, contig and anchor position. */
+typedef struct { uint32_t rid; int8_t side, rev; int16_t pad; int32_t tid; int64_t tpos; } mbg_job_meta_t;
+
+typedef struct {
+	int64_t n_reads, n_rec, n_rows, n_anchored, n_fast, n_jobs;
+	int64_t q_bytes, t_bytes, cells;
+	/* algorithmic work, filled when collect_stats != 0: 64-byte occ blocks fetched and 32-byte
+	 * k-mer cache lookups by seeding pass 1 / pass 2, extension steps, LF steps of SA locate */
+	uint64_t occ_blocks[2], kmer_lookups[2], extend_steps[2], lf_steps;
+	/* device time per phase from HIP events on the step's stream (ms) */
+	float ms_seed1, ms_seed2, ms_rows, ms_sa, ms_jobs, ms_dp, ms_total;
+} mbg_map_stats_t;
+
+/* Device addresses owned by the context, valid until its next call. */
+typedef struct {
+	const mbg_seed_rec_t *recs;
+	const uint64_t *pos;           /* located SA positions, n_rows */
+	const uint32_t *pos_rec;       /* record index of each position */
+	const mbg_dp_job_t *jobs;
+	const mbg_job_meta_t *meta;
+	const mbg_dp_result_t *res;
+	const uint32_t *cigar;
+	const uint8_t *qbuf, *tbuf;
+} mbg_map_out_t;
+
+/* One pass of the hot path over n_reads reads in HBM (seq+off[i], len[i] <= max_len): two-pass
+ * seeding, SA locate of every interval's anchor rows (seed.c:304-315), and for each read the left
+ * and right extension from its longest SMEM (align.c mb_align1/mb_align_pair rules; ungapped fast
+ * path first, ksw_extd2 -- or ksw_extz2 when q==q2 and e==e2 -- for the rest).  Synchronises the
+ * stream; grows the context's pools and repeats the step if a pool overflowed. */
+int mbg_map_step_dev(mbg_ctx_t *ctx, void *stream, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
+                     int32_t max_len, const mbg_map_opt_t *opt, int collect_stats, mbg_map_stats_t *stats, mbg_map_out_t *out);
+
 #ifdef __cplusplus
 }
 #endif

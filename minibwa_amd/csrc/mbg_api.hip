@@ -42,6 +42,10 @@ struct mbg_ctx_s {
 	// scratch pools
 	DBuf b_in[6], b_out[3], b_ctr, b_task, b_slab;
 	DBuf h_pinned;
+	// seed -> locate -> extend step
+	DBuf p_recs, p_rows, p_rowrec, p_first, p_best, p_jobs, p_meta, p_qb, p_tb, p_res, p_cig, p_ctr;
+	int64_t cap_rec = 0, cap_rows = 0;
+	hipEvent_t ev[8] = {};
 };
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { g_hip_err = std::string(#x) + ": " + hipGetErrorString(e_); return MBG_EHIP; } } while (0)
@@ -135,6 +139,9 @@ void mbg_destroy(mbg_ctx_t *c)
 	for (auto &b : c->b_in) dfree(b);
 	for (auto &b : c->b_out) dfree(b);
 	dfree(c->b_ctr); dfree(c->b_task); dfree(c->b_slab);
+	for (DBuf *b : {&c->p_recs, &c->p_rows, &c->p_rowrec, &c->p_first, &c->p_best, &c->p_jobs, &c->p_meta, &c->p_qb, &c->p_tb,
+	                &c->p_res, &c->p_cig, &c->p_ctr}) dfree(*b);
+	for (hipEvent_t &e : c->ev) if (e) hipEventDestroy(e);
 	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
 	if (c->stream) hipStreamDestroy(c->stream);
 	delete c;
@@ -458,7 +465,7 @@ static int sa_dev_locked(mbg_ctx_t *c, hipStream_t st, int64_t n, uint64_t *x)
 	TRY(dgrow(c->b_ctr, 64));
 	uint32_t *next = (uint32_t*)c->b_ctr.p + 4;
 	HIPCHK(hipMemsetAsync(next, 0, 4, st));
-	HIPCHK(launch_sa(c->ix, n, x, next, persistent_grid(c, n, 256), st));
+	HIPCHK(launch_sa(c->ix, n, nullptr, x, next, nullptr, persistent_grid(c, n, 256), st));
 	return MBG_OK;
 }
 
@@ -625,6 +632,150 @@ int mbg_getseq(mbg_ctx_t *c, int64_t n, const int64_t *tid, const int64_t *st, c
 	HIPCHK(hipMemcpyAsync(len, c->b_out[1].p, n * 8, hipMemcpyDeviceToHost, c->stream));
 	HIPCHK(hipStreamSynchronize(c->stream));
 	return MBG_OK;
+}
+
+int mbg_copy_to_host(mbg_ctx_t *c, void *dst, const void *src, size_t bytes)
+{
+	if (c == nullptr || (bytes > 0 && (dst == nullptr || src == nullptr))) return MBG_EARG;
+	if (bytes == 0) return MBG_OK;
+	std::lock_guard<std::mutex> lk(c->mu);
+	HIPCHK(hipSetDevice(c->device));
+	HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+	return MBG_OK;
+}
+
+// ---- seed -> locate -> extend step --------------------------------------------------------------
+
+static_assert(sizeof(mbg_job_meta_t) == sizeof(JobMeta), "job meta layout");
+static_assert(sizeof(mbg_map_opt_t) == 12 * 4 + 28, "map opt layout");
+
+int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
+                     int32_t max_len, const mbg_map_opt_t *opt, int collect_stats, mbg_map_stats_t *stats, mbg_map_out_t *out)
+{
+	if (c == nullptr || opt == nullptr || stats == nullptr || n_reads < 0 || max_len < 0 || (n_reads > 0 && (!off || !len || !seq)))
+		return MBG_EARG;
+	if (opt->min_len < 1 || opt->max_occ < 1 || opt->e <= 0 || opt->e2 <= 0) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt || !c->has_l2b) return MBG_ENOIDX;
+	HIPCHK(hipSetDevice(c->device));
+	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	for (hipEvent_t &e : c->ev) if (!e) HIPCHK(hipEventCreate(&e));
+	memset(stats, 0, sizeof(*stats));
+	stats->n_reads = n_reads;
+	MapOpt mo{};
+	mo.a = opt->a; mo.b = opt->b; mo.q = opt->q; mo.e = opt->e; mo.q2 = opt->q2; mo.e2 = opt->e2; mo.end_bonus = opt->end_bonus;
+	mo.ksw_w = opt->ksw_w; mo.zdrop = opt->zdrop;
+	for (int i = 0; i < 25; ++i) mo.mat[i] = opt->mat[i];
+	const bool dual = !(opt->q == opt->q2 && opt->e == opt->e2);
+	// job sizes: query <= max_len, target <= l + extra (align.c:590-595)
+	const int64_t l_extra = ((int64_t)max_len * opt->a + opt->end_bonus - opt->q) > 0 ? ((int64_t)max_len * opt->a + opt->end_bonus - opt->q) / opt->e : 0;
+	const int32_t max_tlen = (int32_t)(max_len + l_extra + 1);
+	int cap_dp;
+	int64_t slab;
+	TRY(dp_plan(max_len > 0 ? max_len : 1, max_tlen, opt->ksw_w, &cap_dp, &slab));
+	const int64_t job_cap = 2 * (int64_t)n_reads + 16;
+	if (c->cap_rec < 8 * (int64_t)n_reads + 1024) c->cap_rec = 8 * (int64_t)n_reads + 1024;
+	if (c->cap_rows < 16 * (int64_t)n_reads + 1024) c->cap_rows = 16 * (int64_t)n_reads + 1024;
+	for (int attempt = 0; attempt < 4; ++attempt) {
+		const int64_t rc = c->cap_rec, rw = c->cap_rows;
+		TRY(dgrow(c->p_recs, rc * sizeof(SeedRec)));
+		TRY(dgrow(c->p_rows, rw * 8)); TRY(dgrow(c->p_rowrec, rw * 4)); TRY(dgrow(c->p_first, rc * 4));
+		TRY(dgrow(c->p_best, (size_t)n_reads * 8 + 8));
+		TRY(dgrow(c->p_jobs, job_cap * sizeof(DpJob))); TRY(dgrow(c->p_meta, job_cap * sizeof(JobMeta)));
+		TRY(dgrow(c->p_qb, (size_t)n_reads * max_len + 16)); TRY(dgrow(c->p_tb, (size_t)job_cap * max_tlen + 16));
+		TRY(dgrow(c->p_res, job_cap * sizeof(DpRes)));
+		TRY(dgrow(c->p_cig, (size_t)job_cap * (max_len + max_tlen) * 4 + 16));
+		TRY(dgrow(c->p_ctr, 256));
+		unsigned long long *pc = (unsigned long long*)c->p_ctr.p;  // [0..5] job sink, [6] rows, [8..14] work stats
+		HIPCHK(hipMemsetAsync(pc, 0, 256, st));
+		HIPCHK(hipMemsetAsync(c->p_best.p, 0, (size_t)n_reads * 8 + 8, st));
+		HIPCHK(hipEventRecord(c->ev[0], st));
+		// seeding (two launches of the persistent state machine)
+		TRY(dgrow(c->b_ctr, 64));
+		TRY(dgrow(c->b_task, (size_t)rc * sizeof(SeedTask)));
+		uint32_t *ctr = (uint32_t*)c->b_ctr.p;
+		int64_t *n_rec = (int64_t*)(ctr + 2);
+		HIPCHK(hipMemsetAsync(ctr, 0, 32, st));
+		SeedArgs a{};
+		a.seq = seq; a.off = off; a.len = len; a.n_tasks = (uint32_t)n_reads; a.min_len = opt->min_len;
+		a.next = ctr; a.out = (SeedRec*)c->p_recs.p; a.n_out = n_rec; a.cap = rc;
+		a.stats = collect_stats ? pc + 8 : nullptr;
+		if (n_reads > 0) HIPCHK(launch_smem(c->ix, a, 1, persistent_grid(c, n_reads, 256), st));
+		HIPCHK(hipEventRecord(c->ev[1], st));
+		if (n_reads > 0) {
+			HIPCHK(launch_subtask(a.out, n_rec, rc, opt->min_len, opt->max_sub_occ, (SeedTask*)c->b_task.p, ctr + 1, rc,
+			                      (unsigned)c->n_cu * 4, st));
+			HIPCHK(hipMemsetAsync(ctr, 0, 4, st));
+			SeedArgs a2 = a;
+			a2.tasks = (const SeedTask*)c->b_task.p; a2.n_tasks = 0xffffffffu; a2.n_tasks_dev = ctr + 1; a2.task_cap = rc;
+			a2.stats = collect_stats ? pc + 11 : nullptr;
+			HIPCHK(launch_smem(c->ix, a2, 2, persistent_grid(c, n_reads, 256), st));
+		}
+		HIPCHK(hipEventRecord(c->ev[2], st));
+		// anchor rows and SA locate
+		HIPCHK(launch_anchor_rows(a.out, n_rec, rc, opt->max_occ, (uint64_t*)c->p_rows.p, (uint32_t*)c->p_rowrec.p, (uint32_t*)c->p_first.p,
+		                          pc + 6, rw, (unsigned)c->n_cu * 8, st));
+		HIPCHK(launch_best_seed(a.out, n_rec, rc, (unsigned long long*)c->p_best.p, (unsigned)c->n_cu * 8, st));
+		HIPCHK(hipEventRecord(c->ev[3], st));
+		HIPCHK(hipMemsetAsync(ctr + 4, 0, 4, st));
+		HIPCHK(launch_sa(c->ix, rw, (const int64_t*)(pc + 6), (uint64_t*)c->p_rows.p, ctr + 4, collect_stats ? pc + 14 : nullptr,
+		                 persistent_grid(c, 4 * (int64_t)n_reads + 1, 256), st));
+		HIPCHK(hipEventRecord(c->ev[4], st));
+		// extension jobs and DP
+		HIPCHK(launch_ext_jobs(c->lb, mo, n_reads, off, len, seq, a.out, (const uint32_t*)c->p_first.p, (const uint64_t*)c->p_rows.p,
+		                       (const unsigned long long*)c->p_best.p, (DpJob*)c->p_jobs.p, (JobMeta*)c->p_meta.p, (uint8_t*)c->p_qb.p,
+		                       (uint8_t*)c->p_tb.p, job_cap, (int64_t)n_reads * max_len, job_cap * max_tlen, pc, st));
+		HIPCHK(hipEventRecord(c->ev[5], st));
+		{
+			int64_t g = (int64_t)c->n_cu * 16;
+			if (g > job_cap) g = job_cap;
+			const int64_t max_by_mem = ((int64_t)2 << 30) / slab;
+			if (g > max_by_mem) g = max_by_mem;
+			TRY(dgrow(c->b_slab, (size_t)g * slab));
+			DpArgs d{};
+			d.jobs = (const DpJob*)c->p_jobs.p; d.n_jobs = (int32_t)job_cap; d.n_jobs_dev = pc;
+			d.qseq = (const uint8_t*)c->p_qb.p; d.tseq = (const uint8_t*)c->p_tb.p;
+			d.res = (DpRes*)c->p_res.p; d.cigar = (uint32_t*)c->p_cig.p;
+			d.pbuf = (uint8_t*)c->b_slab.p; d.slab = slab; d.cap = cap_dp; d.m = 5;
+			for (int i = 0; i < 32; ++i) d.mat[i] = i < 25 ? opt->mat[i] : 0;
+			HIPCHK(launch_dp(dual, d, (unsigned)g, st));
+		}
+		HIPCHK(hipEventRecord(c->ev[6], st));
+		unsigned long long hc[16];
+		int64_t hn_rec = 0;
+		HIPCHK(hipMemcpyAsync(hc, pc, sizeof(hc), hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(&hn_rec, n_rec, 8, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipStreamSynchronize(st));
+		if (hn_rec > rc || (int64_t)hc[6] > rw) { // a pool overflowed: grow it and run the step again
+			if (hn_rec > rc) c->cap_rec = hn_rec * 2;
+			if ((int64_t)hc[6] > rw) c->cap_rows = (int64_t)hc[6] * 2;
+			continue;
+		}
+		stats->n_rec = hn_rec; stats->n_rows = (int64_t)hc[6]; stats->n_jobs = (int64_t)hc[0]; stats->n_fast = (int64_t)hc[3];
+		stats->n_anchored = (int64_t)hc[4]; stats->q_bytes = (int64_t)hc[1]; stats->t_bytes = (int64_t)hc[2];
+		stats->occ_blocks[0] = hc[8]; stats->kmer_lookups[0] = hc[9]; stats->extend_steps[0] = hc[10];
+		stats->occ_blocks[1] = hc[11]; stats->kmer_lookups[1] = hc[12]; stats->extend_steps[1] = hc[13]; stats->lf_steps = hc[14];
+		float ms;
+		HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); stats->ms_seed1 = ms;
+		HIPCHK(hipEventElapsedTime(&ms, c->ev[1], c->ev[2])); stats->ms_seed2 = ms;
+		HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3])); stats->ms_rows = ms;
+		HIPCHK(hipEventElapsedTime(&ms, c->ev[3], c->ev[4])); stats->ms_sa = ms;
+		HIPCHK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5])); stats->ms_jobs = ms;
+		HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6])); stats->ms_dp = ms;
+		HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[6])); stats->ms_total = ms;
+		if (collect_stats && stats->n_jobs > 0) {
+			std::vector<DpRes> res(stats->n_jobs);
+			HIPCHK(hipMemcpy(res.data(), c->p_res.p, stats->n_jobs * sizeof(DpRes), hipMemcpyDeviceToHost));
+			for (const DpRes &r : res) stats->cells += r.cells;
+		}
+		if (out) {
+			out->recs = (const mbg_seed_rec_t*)c->p_recs.p; out->pos = (const uint64_t*)c->p_rows.p; out->pos_rec = (const uint32_t*)c->p_rowrec.p;
+			out->jobs = (const mbg_dp_job_t*)c->p_jobs.p; out->meta = (const mbg_job_meta_t*)c->p_meta.p; out->res = (const mbg_dp_result_t*)c->p_res.p;
+			out->cigar = (const uint32_t*)c->p_cig.p; out->qbuf = (const uint8_t*)c->p_qb.p; out->tbuf = (const uint8_t*)c->p_tb.p;
+		}
+		return MBG_OK;
+	}
+	return MBG_EOVERFLOW;
 }
 
 } // extern "C"

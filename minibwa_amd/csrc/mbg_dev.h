@@ -68,9 +68,11 @@ __device__ __forceinline__ int bwt_b0(const Idx &ix, uint64_t k)
 	return (int)(ix.data[wi] >> ((k & 31) << 1) & 3);
 }
 
-// mb_bwt_rank2a (bwt.c:195-231).  One block fetch when k-1 and l-1 share a block.
-__device__ __forceinline__ void rank2a(const Idx &ix, uint64_t k, uint64_t l, uint64_t ck[4], uint64_t cl[4])
+// mb_bwt_rank2a (bwt.c:195-231) for k <= l.  One block fetch when k-1 and l-1 share a block;
+// returns the number of 64-byte blocks fetched.
+__device__ __forceinline__ int rank2a(const Idx &ix, uint64_t k, uint64_t l, uint64_t ck[4], uint64_t cl[4])
 {
+	int nblk = 0;
 	uint64_t b[8];
 	uint64_t blk = ~0ULL;
 	if (k == 0) {
@@ -80,6 +82,7 @@ __device__ __forceinline__ void rank2a(const Idx &ix, uint64_t k, uint64_t l, ui
 		j -= (j >= ix.primary);
 		blk = j >> 7;
 		load_block(ix.data, blk, b);
+		++nblk;
 		occ_incl(b, (uint32_t)(j & 127), ck);
 	}
 	// bwt.c:205-209: with l==k+1 the reference adds the symbol at $-removed index k-(k>primary);
@@ -87,16 +90,17 @@ __device__ __forceinline__ void rank2a(const Idx &ix, uint64_t k, uint64_t l, ui
 	if (l == k + 1 && k != 0 && k == ix.primary) {
 		cl[0] = ck[0]; cl[1] = ck[1]; cl[2] = ck[2]; cl[3] = ck[3];
 		cl[bwt_b0(ix, k)] += 1;
-		return;
+		return nblk;
 	}
 	if (l == 0) {
 		cl[0] = cl[1] = cl[2] = cl[3] = 0;
 	} else {
 		uint64_t j = l - 1;
 		j -= (j >= ix.primary);
-		if ((j >> 7) != blk) load_block(ix.data, j >> 7, b);
+		if ((j >> 7) != blk) load_block(ix.data, j >> 7, b), ++nblk;
 		occ_incl(b, (uint32_t)(j & 127), cl);
 	}
+	return nblk;
 }
 
 // mb_bwt_set_intv (bwt.h:75-81)
@@ -111,12 +115,13 @@ __device__ __forceinline__ Sai set_intv(const Idx &ix, int c)
 }
 
 // mb_bwt_extend (bwt.c:237-250) restricted to the one base c the caller keeps.
-__device__ __forceinline__ Sai extend1(const Idx &ix, const Sai &ik, int is_back, int c)
+__device__ __forceinline__ Sai extend1(const Idx &ix, const Sai &ik, int is_back, int c, int *nblk = nullptr)
 {
 	const uint64_t xf = is_back ? ik.x0 : ik.x1;   // x[!is_back]
 	const uint64_t xg = is_back ? ik.x1 : ik.x0;   // x[is_back]
 	uint64_t tk[4], tl[4];
-	rank2a(ix, xf, xf + ik.size, tk, tl);
+	const int nb = rank2a(ix, xf, xf + ik.size, tk, tl);
+	if (nblk) *nblk += nb;
 	uint64_t acc = xg + (uint64_t)(xf <= ix.primary && xf + ik.size > ix.primary);
 	#pragma unroll
 	for (int b = 3; b > 0; --b)

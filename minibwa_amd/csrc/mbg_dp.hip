@@ -422,7 +422,9 @@ __global__ void __launch_bounds__(64) dp_kernel(DpArgs a)
 	if (threadIdx.x < 32) mat[threadIdx.x] = a.mat[threadIdx.x];
 	__syncthreads();
 	uint8_t *slab = a.pbuf + (size_t)blockIdx.x * a.slab;
-	for (int j = blockIdx.x; j < a.n_jobs; j += gridDim.x)
+	int n = a.n_jobs;
+	if (a.n_jobs_dev) { const unsigned long long nd = *a.n_jobs_dev; n = nd < (unsigned long long)n ? (int)nd : n; }
+	for (int j = blockIdx.x; j < n; j += gridDim.x)
 		run_job<DUAL>(a, j, lds, mat, slab);
 }
 

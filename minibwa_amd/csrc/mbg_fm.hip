@@ -110,10 +110,13 @@ __device__ __forceinline__ int control(const Idx &ix, const SeedArgs &a, Lane &s
 	}
 }
 
-__global__ void __launch_bounds__(256) smem_kernel(Idx ix, SeedArgs a, uint32_t pass)
+template <int PASS>
+__global__ void __launch_bounds__(256) smem_kernel(Idx ix, SeedArgs a)
 {
+	const uint32_t pass = PASS;
 	Lane s;
 	int act = ACT_DONE;
+	int nblk = 0, nlook = 0, next_ = 0;
 	uint32_t n_tasks = a.n_tasks;
 	if (a.n_tasks_dev) {
 		const int64_t nt = (int64_t)*a.n_tasks_dev;
@@ -122,7 +125,14 @@ __global__ void __launch_bounds__(256) smem_kernel(Idx ix, SeedArgs a, uint32_t 
 	for (;;) {
 		const uint32_t t = wave_fetch(a.next, act == ACT_DONE);
 		if (act == ACT_DONE) {
-			if (t >= n_tasks) break;
+			if (t >= n_tasks) {
+				if (a.stats) {
+					atomicAdd(&a.stats[0], (unsigned long long)nblk);
+					atomicAdd(&a.stats[1], (unsigned long long)nlook);
+					atomicAdd(&a.stats[2], (unsigned long long)next_);
+				}
+				break;
+			}
 			if (pass == 1) {
 				s.rid = t;
 				s.q = a.seq + a.off[t];
@@ -141,6 +151,7 @@ __global__ void __launch_bounds__(256) smem_kernel(Idx ix, SeedArgs a, uint32_t 
 		if (act == ACT_DONE) continue;
 		if (act == ACT_LOOKUP) {                        // stages 2/5 (bwt.c:401-413)
 			s.p = ix.pre[s.kmer];
+			++nlook;
 			if (s.p.size < s.min_occ) {
 				s.i += ix.pre_len;
 				s.p = set_intv(ix, s.q[s.i]);
@@ -148,11 +159,13 @@ __global__ void __launch_bounds__(256) smem_kernel(Idx ix, SeedArgs a, uint32_t 
 			}
 			++s.stage;
 		} else if (act == ACT_BACK) {                   // stages 3/6: se_one_step_back (bwt.c:339-354)
-			const Sai o = extend1(ix, s.p, 1, s.q[s.i]);
+			const Sai o = extend1(ix, s.p, 1, s.q[s.i], &nblk);
+			++next_;
 			if (o.size < s.min_occ) { s.x = s.i + 1; s.stage = 1; }
 			else { s.p = o; --s.i; }
 		} else {                                        // stage 4 forward step (bwt.c:421-453)
-			const Sai o = extend1(ix, s.p, 0, 3 - (int)s.q[s.i]);
+			const Sai o = extend1(ix, s.p, 0, 3 - (int)s.q[s.i], &nblk);
+			++next_;
 			if (o.size >= s.min_occ) { s.p = o; ++s.i; }
 			else {
 				emit(a, s, pass);
@@ -195,12 +208,14 @@ __global__ void __launch_bounds__(256) subtask_kernel(const SeedRec *out, const 
 // SA locate (mb_bwt_sa / mb_bwt_sa_batch, bwt.c:499-541): LF-walk each row to the nearest sampled
 // row; lanes refill from a counter because walk lengths are geometric (mean 2^sa_bit).
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) sa_kernel(Idx ix, int64_t n, uint64_t *x, uint32_t *next)
+__global__ void __launch_bounds__(256) sa_kernel(Idx ix, int64_t n, const int64_t *n_dev, uint64_t *x, uint32_t *next,
+                                                unsigned long long *stats)
 {
 	const uint64_t mask = (1ULL << ix.sa_bit) - 1;
+	if (n_dev) { const int64_t nd = *n_dev; n = nd < n ? nd : n; }
 	bool have = false;
 	uint32_t idx = 0;
-	uint64_t k = 0, steps = 0;
+	uint64_t k = 0, steps = 0, tot = 0;
 	for (;;) {
 		const uint32_t t = wave_fetch(next, !have);
 		if (!have) {
@@ -209,12 +224,14 @@ __global__ void __launch_bounds__(256) sa_kernel(Idx ix, int64_t n, uint64_t *x,
 		}
 		if ((k & mask) == 0) {
 			x[idx] = steps + ix.sa[k >> ix.sa_bit];
+			tot += steps;
 			have = false;
 			continue;
 		}
 		k = lf(ix, k);
 		++steps;
 	}
+	if (stats) atomicAdd(stats, (unsigned long long)tot);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -282,7 +299,8 @@ hipError_t launch_extend(const Idx &ix, int64_t n, const Sai *ik, const uint8_t 
 
 hipError_t launch_smem(const Idx &ix, const SeedArgs &a, uint32_t pass, unsigned grid, hipStream_t st)
 {
-	hipLaunchKernelGGL(smem_kernel, dim3(grid), dim3(256), 0, st, ix, a, pass);
+	if (pass == 1) hipLaunchKernelGGL(smem_kernel<1>, dim3(grid), dim3(256), 0, st, ix, a);
+	else hipLaunchKernelGGL(smem_kernel<2>, dim3(grid), dim3(256), 0, st, ix, a);
 	return hipGetLastError();
 }
 
@@ -293,10 +311,11 @@ hipError_t launch_subtask(const SeedRec *out, const int64_t *n_out, int64_t cap,
 	return hipGetLastError();
 }
 
-hipError_t launch_sa(const Idx &ix, int64_t n, uint64_t *x, uint32_t *next, unsigned grid, hipStream_t st)
+hipError_t launch_sa(const Idx &ix, int64_t n, const int64_t *n_dev, uint64_t *x, uint32_t *next, unsigned long long *stats,
+                     unsigned grid, hipStream_t st)
 {
 	if (n <= 0) return hipSuccess;
-	hipLaunchKernelGGL(sa_kernel, dim3(grid), dim3(256), 0, st, ix, n, x, next);
+	hipLaunchKernelGGL(sa_kernel, dim3(grid), dim3(256), 0, st, ix, n, n_dev, x, next, stats);
 	return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@ struct SeedArgs {
 	SeedRec *out;
 	int64_t *n_out;         // record counter
 	int64_t cap;
+	unsigned long long *stats;  // optional: [0] occ blocks fetched, [1] k-mer cache lookups, [2] extend steps
 };
 
 struct L2b {
@@ -29,6 +30,14 @@ struct L2b {
 	const uint64_t *ctg_off, *ctg_len;
 	uint64_t n_ctg, n_ambi, tot_len;
 };
+
+// seed -> locate -> extend step (mbg_pipe.hip)
+struct MapOpt {
+	int32_t a, b, q, e, q2, e2, end_bonus;
+	int32_t ksw_w, zdrop;      // band (int)(bw*1.5+1) and zdrop*a for the batch's read length
+	int8_t mat[28];
+};
+struct JobMeta { uint32_t rid; int8_t side, rev; int16_t pad; int32_t tid; int64_t tpos; };
 
 // DP batch arguments (kernels in mbg_dp.hip)
 struct DpJob { int64_t qoff, toff; int32_t qlen, tlen; int8_t q, e, q2, e2; int32_t w, zdrop, end_bonus, flag; int64_t cigar_off; };
@@ -40,6 +49,7 @@ static_assert(sizeof(SeedRec) == 40, "SeedRec must match mbg_seed_rec_t");
 struct DpArgs {
 	const DpJob *jobs;
 	int32_t n_jobs;
+	const unsigned long long *n_jobs_dev;   // when set: job count produced on the device (clamped to n_jobs)
 	const uint8_t *qseq, *tseq;
 	DpRes *res;
 	uint32_t *cigar;
@@ -56,10 +66,18 @@ hipError_t launch_extend(const Idx &ix, int64_t n, const Sai *ik, const uint8_t 
 hipError_t launch_smem(const Idx &ix, const SeedArgs &a, uint32_t pass, unsigned grid, hipStream_t st);
 hipError_t launch_subtask(const SeedRec *out, const int64_t *n_out, int64_t cap, int32_t min_len, int32_t max_sub_occ,
                           SeedTask *tasks, uint32_t *n_tasks, int64_t task_cap, unsigned grid, hipStream_t st);
-hipError_t launch_sa(const Idx &ix, int64_t n, uint64_t *x, uint32_t *next, unsigned grid, hipStream_t st);
+hipError_t launch_sa(const Idx &ix, int64_t n, const int64_t *n_dev, uint64_t *x, uint32_t *next, unsigned long long *stats,
+                     unsigned grid, hipStream_t st);
 hipError_t launch_getseq(const L2b &lb, int64_t n, const int64_t *tid, const int64_t *st_, const int64_t *en_, const int64_t *out_off,
                          uint8_t *out, int64_t *len, hipStream_t st);
 hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st);
+hipError_t launch_anchor_rows(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, uint64_t *rows, uint32_t *row_rec,
+                              uint32_t *first_slot, unsigned long long *n_rows, int64_t row_cap, unsigned grid, hipStream_t st);
+hipError_t launch_best_seed(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, unsigned long long *best, unsigned grid, hipStream_t st);
+hipError_t launch_ext_jobs(const L2b &lb, const MapOpt &o, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
+                           const SeedRec *recs, const uint32_t *first_slot, const uint64_t *pos, const unsigned long long *best,
+                           DpJob *jobs, JobMeta *meta, uint8_t *qbuf, uint8_t *tbuf, int64_t job_cap, int64_t qcap, int64_t tcap,
+                           unsigned long long *ctr, hipStream_t st);
 
 // geometry of one DP job (ksw2_extd2_sse.c:82-86): the band width after `w<0` handling and the
 // number of 16-cell columns per anti-diagonal row

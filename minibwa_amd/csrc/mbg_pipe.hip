@@ -1,0 +1,244 @@
+// mbg_pipe.hip -- device-resident seed -> locate -> extend pass over a batch of reads.
+//
+// After two-pass seeding the step samples anchor rows, locates them, and builds one extension job
+// per read end from the read's longest SMEM, with the window and fast-path rules the reference's
+// short-read aligner applies to a chain:
+//   * SA rows per interval (seed.c:304-315): all rows when size <= max_occ, else max_occ rows with
+//     step (size - j) / (max_occ - n);
+//   * position -> contig/strand (l2b_intv2cid, l2bit.c:9-34);
+//   * left/right windows (mb_align1 short-read mode, align.c:588-596, 640-653, 732-742): the query
+//     end beyond the anchor plus `l` target bases, l += (l*a + end_bonus - q) / e when positive;
+//   * mb_align_pair (align.c:329-378): ungapped extension first (accepted with at most 3
+//     mismatches when it reaches the end within end_bonus of its maximum), otherwise a ksw job
+//     with the band capped by max_bw_from_mm when the shorter side is < 100.
+// Only jobs that fail the fast path reach the DP kernel, as in the reference.
+#include "mbg_internal.h"
+
+namespace mbg {
+
+// SA rows of each interval (mb_anchor's sampling); first_slot[rec] = slot of its first row
+__global__ void __launch_bounds__(256) anchor_rows_kernel(const SeedRec *recs, const int64_t *n_rec_dev, int64_t rec_cap, int32_t max_occ,
+                                                         uint64_t *rows, uint32_t *row_rec, uint32_t *first_slot,
+                                                         unsigned long long *n_rows, int64_t row_cap)
+{
+	int64_t n = *n_rec_dev;
+	if (n > rec_cap) n = rec_cap;
+	for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+		const SeedRec s = recs[r];
+		int64_t cnt;
+		if ((int64_t)s.size <= max_occ) cnt = (int64_t)s.size;
+		else {
+			cnt = 0;
+			for (int64_t j = 0, k = 0; j < (int64_t)s.size && k < max_occ; ++k) {
+				int64_t step = ((int64_t)s.size - j) / (max_occ - k);
+				if (step < 1) step = 1;
+				j += step, ++cnt;
+			}
+		}
+		const unsigned long long base = atomicAdd(n_rows, (unsigned long long)cnt);
+		first_slot[r] = cnt > 0 && (int64_t)base < row_cap ? (uint32_t)base : 0xffffffffu;
+		if ((int64_t)(base + cnt) > row_cap) continue;
+		if ((int64_t)s.size <= max_occ) {
+			for (int64_t j = 0; j < cnt; ++j) rows[base + j] = s.x0 + j, row_rec[base + j] = (uint32_t)r;
+		} else {
+			int64_t o = 0;
+			for (int64_t j = 0, k = 0; j < (int64_t)s.size && k < max_occ; ++k) {
+				int64_t step = ((int64_t)s.size - j) / (max_occ - k);
+				if (step < 1) step = 1;
+				rows[base + o] = s.x0 + j, row_rec[base + o] = (uint32_t)r, ++o;
+				j += step;
+			}
+		}
+	}
+}
+
+// longest pass-1 SMEM of each read (ties: smallest query start, then first record)
+__global__ void __launch_bounds__(256) best_seed_kernel(const SeedRec *recs, const int64_t *n_rec_dev, int64_t rec_cap, unsigned long long *best)
+{
+	int64_t n = *n_rec_dev;
+	if (n > rec_cap) n = rec_cap;
+	for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+		const SeedRec s = recs[r];
+		if (s.pass != 1 || s.size == 0) continue;
+		const uint32_t qs = (uint32_t)(s.info >> 32), qe = (uint32_t)s.info;
+		const uint64_t len = qe - qs;
+		const uint64_t key = (len < 0xffff ? len : 0xffff) << 48 | (uint64_t)(0xffffu - (qs < 0xffff ? qs : 0xffff)) << 32 | (uint64_t)(0xffffffffu - (uint32_t)r);
+		atomicMax(&best[s.rid], (unsigned long long)key);
+	}
+}
+
+struct JobSink {
+	DpJob *jobs;
+	JobMeta *meta;
+	uint8_t *qbuf, *tbuf;
+	int64_t job_cap, qcap, tcap;
+	unsigned long long *ctr;   // [0] jobs [1] qbytes [2] tbytes [3] fast-path accepts [4] anchored reads [5] cigar slots
+};
+
+// target bases of concatenated coordinates [gst, gen) read forward or reversed; N runs read as 4
+// (l2b_getseq + l2b_getambi, l2bit.c:52-121)
+struct Window {
+	const L2b *lb;
+	int64_t gst, gen;
+	bool reverse;
+	int64_t a0, a1;            // N runs overlapping the window
+	__device__ void init(const L2b &l, int64_t s, int64_t e, bool rev)
+	{
+		lb = &l, gst = s, gen = e, reverse = rev, a0 = a1 = 0;
+		if (l.n_ambi == 0 || e <= s) return;
+		int64_t lo = 0, hi = (int64_t)l.n_ambi;
+		while (lo < hi) { int64_t mid = (lo + hi) / 2; if ((int64_t)l.ambi[mid * 2 + 1] > s) hi = mid; else lo = mid + 1; }
+		a0 = lo, a1 = lo;
+		while (a1 < (int64_t)l.n_ambi && (int64_t)l.ambi[a1 * 2] < e) ++a1;
+	}
+	__device__ uint8_t operator()(int64_t i) const
+	{
+		const int64_t p = reverse ? gen - 1 - i : gst + i;
+		for (int64_t a = a0; a < a1; ++a)
+			if ((int64_t)lb->ambi[a * 2] <= p && p < (int64_t)lb->ambi[a * 2 + 1]) return 4;
+		return (uint8_t)(lb->pac[p >> 5] >> ((p & 31) << 1) & 3);
+	}
+};
+
+// the read as aligned: forward, or reverse-complemented with N kept (align.c:902-903), then a
+// [start, start+n) slice, read forward or reversed
+struct QView {
+	const uint8_t *r;
+	int32_t qlen, start;
+	bool rc, reverse;
+	int32_t n;
+	__device__ uint8_t operator()(int32_t i) const
+	{
+		const int32_t j = start + (reverse ? n - 1 - i : i);      // index on the oriented read
+		if (!rc) return r[j];
+		const uint8_t c = r[qlen - 1 - j];
+		return c < 4 ? (uint8_t)(3 - c) : (uint8_t)4;
+	}
+};
+
+__device__ __forceinline__ int max_bw_from_mm(const MapOpt &o, int mm) // align.c:321-327
+{
+	const int x = mm * (o.a + o.b);
+	int max2 = 0, max1 = 0;
+	if (x >= o.q + o.e) max1 = (x - o.q + o.e - 1) / o.e;
+	if (x >= o.q2 + o.e2) max2 = (x - o.q2 + o.e2 - 1) / o.e2;
+	return max1 > max2 ? max1 : max2;
+}
+
+// one extension (mb_align_pair with KSW_EZ_EXTZ_ONLY, align.c:329-378): fast path or a DP job
+__device__ void emit_ext(const MapOpt &o, const JobSink &js, const QView &q, const Window &t, int tlen, int flag,
+                         uint32_t rid, int side, int rev, int64_t tid, int64_t tpos)
+{
+	const int qlen = q.n;
+	int w = o.ksw_w, n_mm = -1;
+	if (tlen >= qlen) {
+		int32_t score = 0, mx = 0;
+		for (int j = 0; j < qlen; ++j) {
+			const uint8_t tj = t(j), qj = q(j);
+			const int32_t sc = o.mat[tj * 5 + qj];
+			score += sc;
+			n_mm += (tj > 3 || qj > 3 || sc < 0);
+			if (mx < score) mx = score;
+		}
+		if (n_mm <= 2 && score + o.end_bonus >= mx) { atomicAdd(&js.ctr[3], 1ULL); return; }
+	}
+	if (n_mm >= 0 && (qlen < tlen ? qlen : tlen) < 100) {
+		const int mbw = max_bw_from_mm(o, n_mm);
+		if (w > mbw + 4) w = mbw + 4;
+	}
+	const unsigned long long j = atomicAdd(&js.ctr[0], 1ULL);
+	const unsigned long long qo = atomicAdd(&js.ctr[1], (unsigned long long)qlen);
+	const unsigned long long to = atomicAdd(&js.ctr[2], (unsigned long long)tlen);
+	const unsigned long long co = atomicAdd(&js.ctr[5], (unsigned long long)(qlen + tlen));
+	if ((int64_t)j >= js.job_cap || (int64_t)(qo + qlen) > js.qcap || (int64_t)(to + tlen) > js.tcap) return;
+	for (int i = 0; i < qlen; ++i) js.qbuf[qo + i] = q(i);
+	for (int i = 0; i < tlen; ++i) js.tbuf[to + i] = t(i);
+	DpJob d;
+	d.qoff = (int64_t)qo; d.toff = (int64_t)to; d.qlen = qlen; d.tlen = tlen;
+	d.q = (int8_t)o.q; d.e = (int8_t)o.e; d.q2 = (int8_t)o.q2; d.e2 = (int8_t)o.e2;
+	d.w = w; d.zdrop = o.zdrop; d.end_bonus = o.end_bonus; d.flag = flag; d.cigar_off = (int64_t)co;
+	js.jobs[j] = d;
+	JobMeta m;
+	m.rid = rid; m.side = (int8_t)side; m.rev = (int8_t)rev; m.pad = 0; m.tid = (int32_t)tid; m.tpos = tpos;
+	js.meta[j] = m;
+}
+
+// per read: anchor = longest SMEM at its first located row; left and right extension jobs
+__global__ void __launch_bounds__(128) ext_jobs_kernel(L2b lb, MapOpt o, int32_t n_reads, const int64_t *off, const int32_t *len,
+                                                      const uint8_t *seq, const SeedRec *recs, const uint32_t *first_slot,
+                                                      const uint64_t *pos, const unsigned long long *best, JobSink js)
+{
+	const int32_t rid = blockIdx.x * blockDim.x + threadIdx.x;
+	if (rid >= n_reads) return;
+	const uint64_t key = best[rid];
+	if (key == 0) return;
+	const uint32_t rec = 0xffffffffu - (uint32_t)key;
+	const SeedRec s = recs[rec];
+	const uint32_t fs = first_slot[rec];
+	if (fs == 0xffffffffu) return;
+	const int32_t qlen = len[rid];
+	const int32_t qs0 = (int32_t)(s.info >> 32), qe0 = (int32_t)s.info, alen = qe0 - qs0;
+	// l2b_intv2cid (l2bit.c:9-34)
+	const uint64_t st = pos[fs], en = st + (uint64_t)alen, tot = lb.tot_len;
+	if (en > tot * 2 || (st < tot && tot < en)) return;
+	const int rev = st >= tot;
+	const int64_t sp = (int64_t)(st < tot ? st : tot * 2 - en);
+	int64_t lo = 0, hi = (int64_t)lb.n_ctg, tid = -1;
+	while (lo < hi) {
+		const int64_t mid = (lo + hi) / 2;
+		const int64_t co = (int64_t)lb.ctg_off[mid], cl = (int64_t)lb.ctg_len[mid];
+		if (co <= sp && sp < co + cl) { tid = sp + alen <= co + cl ? mid : -1; break; }
+		else if (sp < co) hi = mid; else lo = mid + 1;
+	}
+	if (tid < 0) return;
+	atomicAdd(&js.ctr[4], 1ULL);
+	const int64_t coff = (int64_t)lb.ctg_off[tid], clen = (int64_t)lb.ctg_len[tid];
+	const int64_t ts = sp - coff, te = ts + alen;
+	const int32_t qs = rev ? qlen - qe0 : qs0, qe = rev ? qlen - qs0 : qe0;   // on the oriented read
+	const uint8_t *r = seq + off[rid];
+	if (qs > 0 && ts > 0) { // left: reversed query prefix against the reversed target before ts
+		int64_t l = qs;
+		l += l * o.a + o.end_bonus > o.q ? (l * o.a + o.end_bonus - o.q) / o.e : 0;
+		const int64_t ts0 = ts - l > 0 ? ts - l : 0;
+		QView q{r, qlen, 0, rev != 0, true, qs};
+		Window t;
+		t.init(lb, coff + ts0, coff + ts, true);
+		emit_ext(o, js, q, t, (int)(ts - ts0), 0x40 | 0x02 | 0x80, (uint32_t)rid, 0, rev, tid, ts);
+	}
+	if (qe < qlen && te < clen) { // right extension
+		int64_t l = qlen - qe;
+		l += l * o.a + o.end_bonus > o.q ? (l * o.a + o.end_bonus - o.q) / o.e : 0;
+		const int64_t te0 = te + l < clen ? te + l : clen;
+		QView q{r, qlen, qe, rev != 0, false, qlen - qe};
+		Window t;
+		t.init(lb, coff + te, coff + te0, false);
+		emit_ext(o, js, q, t, (int)(te0 - te), 0x40, (uint32_t)rid, 1, rev, tid, te);
+	}
+}
+
+hipError_t launch_anchor_rows(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, uint64_t *rows, uint32_t *row_rec,
+                              uint32_t *first_slot, unsigned long long *n_rows, int64_t row_cap, unsigned grid, hipStream_t st)
+{
+	hipLaunchKernelGGL(anchor_rows_kernel, dim3(grid), dim3(256), 0, st, recs, n_rec, rec_cap, max_occ, rows, row_rec, first_slot, n_rows, row_cap);
+	return hipGetLastError();
+}
+
+hipError_t launch_best_seed(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, unsigned long long *best, unsigned grid, hipStream_t st)
+{
+	hipLaunchKernelGGL(best_seed_kernel, dim3(grid), dim3(256), 0, st, recs, n_rec, rec_cap, best);
+	return hipGetLastError();
+}
+
+hipError_t launch_ext_jobs(const L2b &lb, const MapOpt &o, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
+                           const SeedRec *recs, const uint32_t *first_slot, const uint64_t *pos, const unsigned long long *best,
+                           DpJob *jobs, JobMeta *meta, uint8_t *qbuf, uint8_t *tbuf, int64_t job_cap, int64_t qcap, int64_t tcap,
+                           unsigned long long *ctr, hipStream_t st)
+{
+	if (n_reads <= 0) return hipSuccess;
+	JobSink js{jobs, meta, qbuf, tbuf, job_cap, qcap, tcap, ctr};
+	hipLaunchKernelGGL(ext_jobs_kernel, dim3((unsigned)((n_reads + 127) / 128)), dim3(128), 0, st, lb, o, n_reads, off, len, seq, recs,
+	                   first_slot, pos, best, js);
+	return hipGetLastError();
+}
+
+} // namespace mbg

@@ -55,6 +55,67 @@ class IdxInfo(C.Structure):
                 ("pre_len", C.c_int32), ("sa_bit", C.c_int32), ("device_bytes", C.c_uint64)]
 
 
+class MapOpt(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("min_len", "max_sub_occ", "max_occ", "a", "b", "q", "e", "q2", "e2",
+                                         "end_bonus", "ksw_w", "zdrop")] + [("mat", C.c_int8 * 25)]
+
+
+class MapStats(C.Structure):
+    _fields_ = ([(n, C.c_int64) for n in ("n_reads", "n_rec", "n_rows", "n_anchored", "n_fast", "n_jobs", "q_bytes",
+                                          "t_bytes", "cells")] +
+                [("occ_blocks", C.c_uint64 * 2), ("kmer_lookups", C.c_uint64 * 2), ("extend_steps", C.c_uint64 * 2),
+                 ("lf_steps", C.c_uint64)] +
+                [(n, C.c_float) for n in ("ms_seed1", "ms_seed2", "ms_rows", "ms_sa", "ms_jobs", "ms_dp", "ms_total")])
+
+    def as_dict(self):
+        d = {}
+        for n, t in self._fields_:
+            v = getattr(self, n)
+            d[n] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
+class MapOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("recs", "pos", "pos_rec", "jobs", "meta", "res", "cigar", "qbuf", "tbuf")]
+
+
+def adaptive_opt(read_len: int, **over) -> MapOpt:
+    """minibwa's defaults (options.c:5-88, preset "adap") with the length-adaptive band and z-drop
+    (mb_opt_adap, options.c:90-116) turned into ksw arguments (align.c:564, 374-376)."""
+    import math
+    o = dict(min_len=19, max_sub_occ=10, max_occ=250, a=2, b=8, q=12, e=2, q2=23, e2=1, end_bonus=10, bw=100,
+             zdrop=80, b_ts=0, b_ambi=1)
+    o.update(over)
+    a_ = -math.log(0.5) / (2000 - 100)
+    b_ = math.exp(-a_ * ((read_len if read_len > 100 else 100) - 100))
+    bw = int(500 - (500 - o["bw"]) * b_ + .499) if o["bw"] < 500 else o["bw"]
+    zd = int(400 - (400 - o["zdrop"]) * b_ + .499) if o["zdrop"] < 400 else o["zdrop"]
+    m = MapOpt()
+    for k in ("min_len", "max_sub_occ", "max_occ", "a", "b", "q", "e", "q2", "e2", "end_bonus"):
+        setattr(m, k, int(o[k]))
+    m.ksw_w = int(bw * 1.5 + 1.)
+    m.zdrop = zd * o["a"]
+    mat = nt4_mat(o["a"], o["b"], o["b_ts"], o["b_ambi"])
+    for i in range(25):
+        m.mat[i] = int(mat[i])
+    return m
+
+
+def nt4_mat(a=2, b=8, b_ts=0, b_ambi=1):
+    """ksw_gen_nt4_mat (ksw2.h:199-219) without the methylation cases."""
+    a, b, b_ambi = abs(a), -abs(b), -abs(b_ambi)
+    mat = np.zeros(25, np.int8)
+    for i in range(4):
+        for j in range(4):
+            mat[i * 5 + j] = a if i == j else b
+        mat[i * 5 + 4] = b_ambi
+    mat[20:25] = b_ambi
+    if b_ts != 0 and b_ts != b:
+        bt = -abs(b_ts)
+        mat[2] = mat[8] = mat[10] = mat[16] = bt
+    return mat
+
+
 DP_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"),
                          ("q", "i1"), ("e", "i1"), ("q2", "i1"), ("e2", "i1"),
                          ("w", "<i4"), ("zdrop", "<i4"), ("end_bonus", "<i4"), ("flag", "<i4"),
@@ -98,6 +159,9 @@ def lib():
                                          C.POINTER(C.c_int64)]
         L.mbg_seed_batch_dev_async.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, C.c_int64, _P]
         L.mbg_sa_batch_dev.argtypes = [_P, _P, C.c_int64, _P]
+        L.mbg_copy_to_host.argtypes = [_P, _P, _P, C.c_size_t]
+        L.mbg_map_step_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.POINTER(MapOpt), C.c_int,
+                                       C.POINTER(MapStats), C.POINTER(MapOut)]
         L.mbg_extd2_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, _P, _P, C.c_int32, C.c_int32,
                                           C.c_int32]
         _LIB = L
@@ -107,7 +171,8 @@ def lib():
 EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mbg_idx_load", "mbg_bwt_upload",
             "mbg_l2b_upload", "mbg_idx_info", "mbg_pre_copy", "mbg_rank2a", "mbg_extend", "mbg_seed_batch",
             "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
-            "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev")
+            "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host")
+JOB_META_DTYPE = np.dtype([("rid", "<u4"), ("side", "i1"), ("rev", "i1"), ("pad", "<i2"), ("tid", "<i4"), ("tpos", "<i8")])
 
 
 def _ptr(a: np.ndarray):
@@ -256,6 +321,23 @@ class Context:
         check(self.L.mbg_getseq(self.h, len(tid), _ptr(tid), _ptr(st), _ptr(en), _ptr(off), _ptr(out), len(out),
                                 _ptr(ln)), "mbg_getseq")
         return [out[off[i]:off[i] + max(ln[i], 0)].copy() if ln[i] >= 0 else None for i in range(len(tid))]
+
+    # ---- seed -> locate -> extend step over device-resident reads ------------------------------
+    def map_step_dev(self, n_reads, off_ptr, len_ptr, seq_ptr, max_len, opt: MapOpt, collect_stats=False, stream=None):
+        """One pass of the hot path; pointers are device addresses (e.g. torch tensor data_ptr()).
+        Returns (MapStats, MapOut of device addresses valid until the next call)."""
+        st = MapStats()
+        out = MapOut()
+        check(self.L.mbg_map_step_dev(self.h, stream, n_reads, off_ptr, len_ptr, seq_ptr, max_len, C.byref(opt),
+                                      int(collect_stats), C.byref(st), C.byref(out)), "mbg_map_step_dev")
+        return st, out
+
+    def to_host(self, dev_addr, dtype, count):
+        """Copy `count` elements of `dtype` from a device address returned by the library."""
+        a = np.empty(int(count), dtype)
+        if a.nbytes:
+            check(self.L.mbg_copy_to_host(self.h, _ptr(a), C.c_void_p(dev_addr), a.nbytes), "mbg_copy_to_host")
+        return a
 
     # ---- DP ----------------------------------------------------------------------------------
     def _dp(self, fn, name, queries, targets, params, mat, m=5):

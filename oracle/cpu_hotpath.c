@@ -1,0 +1,166 @@
+/*
+ * cpu_hotpath.c -- times the reference's own hot-path functions on the host cores (the CPU
+ * baseline of bench.py).  TEST/BASELINE INFRASTRUCTURE: built into oracle/_ref/ against the
+ * compiled reference, never part of libmbg.
+ *
+ *   cpu_hotpath <idx_prefix> <reads.bin> <jobs.bin> <n_threads>
+ *
+ * Phase 1, per sub-batch of 48 reads (sb_seq=24 fragments, map-main.c:163-174) on a work-stealing
+ * counter like kt_for: mb_seed_intv_batch (seed.c:56-103), then mb_anchor per read (seed.c:268-339:
+ * interval sort/dedup, SA locate through mb_bwt_sa_batch, contig mapping, anchor sort/dedup).
+ * Phase 2: ksw_extd2_sse (ksw2_extd2_sse.c:27) -- or ksw_extz2_sse when q==q2 and e==e2 -- on
+ * every extension job in jobs.bin (the jobs the GPU pass emitted for the same reads).
+ * Prints one JSON line with the wall time of each phase.
+ *
+ * reads.bin: int64 n, int32 len[n], uint8 nt4 bases (concatenated)
+ * jobs.bin:  int64 n, int8 mat[25], n x 56-byte mbg_dp_job_t, int64 nq, nq query bytes, int64 nt, nt target bytes
+ */
+#include <pthread.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+#include "mbpriv.h"
+#include "ksw2.h"
+
+typedef struct {
+	int64_t qoff, toff;
+	int32_t qlen, tlen;
+	int8_t q, e, q2, e2;
+	int32_t w, zdrop, end_bonus, flag;
+	int64_t cigar_off;
+} job_t;
+
+static double now(void) { struct timeval tv; gettimeofday(&tv, 0); return tv.tv_sec + tv.tv_usec * 1e-6; }
+
+typedef struct {
+	mb_idx_t *idx;
+	int64_t n_reads, *off;
+	int32_t *len;
+	uint8_t *seq;
+	int64_t n_jobs;
+	job_t *jobs;
+	uint8_t *qb, *tb;
+	int8_t mat[25];
+	volatile int64_t next;
+	int64_t n_anchor[256];
+	int64_t n_cells[256];
+} shared_t;
+
+typedef struct { shared_t *s; int tid; } targ_t;
+
+static void *seed_worker(void *a_)
+{
+	targ_t *a = (targ_t*)a_;
+	shared_t *s = a->s;
+	mb_sai_v v[48];
+	mb_anchor_v av = {0, 0, 0};
+	uint8_t *sq[48];
+	int32_t ln[48];
+	memset(v, 0, sizeof(v));
+	for (;;) {
+		int64_t b = __sync_fetch_and_add(&s->next, 48), e, i;
+		if (b >= s->n_reads) break;
+		e = b + 48 < s->n_reads ? b + 48 : s->n_reads;
+		for (i = b; i < e; ++i) sq[i - b] = s->seq + s->off[i], ln[i - b] = s->len[i];
+		mb_seed_intv_batch(0, s->idx->bwt, (int32_t)(e - b), ln, sq, 19, 10, v);
+		for (i = b; i < e; ++i) {
+			mb_anchor(0, s->idx, &v[i - b], 19, ln[i - b], sq[i - b], L2B_METH_NONE, 250, &av);
+			s->n_anchor[a->tid] += av.n;
+		}
+	}
+	for (int i = 0; i < 48; ++i) free(v[i].a);
+	free(av.a);
+	return 0;
+}
+
+static void *dp_worker(void *a_)
+{
+	targ_t *a = (targ_t*)a_;
+	shared_t *s = a->s;
+	ksw_extz_t ez;
+	memset(&ez, 0, sizeof(ez));
+	for (;;) {
+		int64_t b = __sync_fetch_and_add(&s->next, 64), e, i;
+		if (b >= s->n_jobs) break;
+		e = b + 64 < s->n_jobs ? b + 64 : s->n_jobs;
+		for (i = b; i < e; ++i) {
+			const job_t *j = &s->jobs[i];
+			if (j->q == j->q2 && j->e == j->e2)
+				ksw_extz2_sse(0, j->qlen, s->qb + j->qoff, j->tlen, s->tb + j->toff, 5, s->mat, j->q, j->e, j->w, j->zdrop, j->end_bonus, j->flag, &ez);
+			else
+				ksw_extd2_sse(0, j->qlen, s->qb + j->qoff, j->tlen, s->tb + j->toff, 5, s->mat, j->q, j->e, j->q2, j->e2, j->w, j->zdrop, j->end_bonus, j->flag, &ez);
+			s->n_cells[a->tid] += ez.max; /* consume the result */
+		}
+	}
+	free(ez.cigar);
+	return 0;
+}
+
+static void run(shared_t *s, int nt, void *(*fn)(void*))
+{
+	pthread_t th[256];
+	targ_t ta[256];
+	int i;
+	s->next = 0;
+	for (i = 0; i < nt; ++i) ta[i].s = s, ta[i].tid = i, pthread_create(&th[i], 0, fn, &ta[i]);
+	for (i = 0; i < nt; ++i) pthread_join(th[i], 0);
+}
+
+static void *slurp(const char *fn, size_t *len)
+{
+	FILE *fp = fopen(fn, "rb");
+	void *p;
+	if (!fp) return 0;
+	fseek(fp, 0, SEEK_END);
+	*len = (size_t)ftell(fp);
+	fseek(fp, 0, SEEK_SET);
+	p = malloc(*len + 1);
+	if (fread(p, 1, *len, fp) != *len) { fclose(fp); free(p); return 0; }
+	fclose(fp);
+	return p;
+}
+
+int main(int argc, char *argv[])
+{
+	shared_t s;
+	size_t rl, jl;
+	uint8_t *r, *jb, *p;
+	int nt, i;
+	double t0, t_seed, t_dp;
+	int64_t n_anchor = 0;
+	if (argc < 5) { fprintf(stderr, "usage: cpu_hotpath <idx_prefix> <reads.bin> <jobs.bin> <n_threads>\n"); return 1; }
+	memset(&s, 0, sizeof(s));
+	nt = atoi(argv[4]);
+	if (nt < 1) nt = 1;
+	if (nt > 256) nt = 256;
+	s.idx = mb_idx_load(argv[1], 0);
+	if (!s.idx) { fprintf(stderr, "cannot load index %s\n", argv[1]); return 1; }
+	r = (uint8_t*)slurp(argv[2], &rl);
+	jb = (uint8_t*)slurp(argv[3], &jl);
+	if (!r || !jb) { fprintf(stderr, "cannot read inputs\n"); return 1; }
+	memcpy(&s.n_reads, r, 8);
+	s.len = (int32_t*)(r + 8);
+	s.seq = r + 8 + 4 * s.n_reads;
+	s.off = (int64_t*)malloc((s.n_reads + 1) * 8);
+	s.off[0] = 0;
+	for (i = 0; i < s.n_reads; ++i) s.off[i + 1] = s.off[i] + s.len[i];
+	p = jb;
+	memcpy(&s.n_jobs, p, 8); p += 8;
+	memcpy(s.mat, p, 25); p += 25;
+	s.jobs = (job_t*)malloc(s.n_jobs * sizeof(job_t) + 1);
+	memcpy(s.jobs, p, s.n_jobs * sizeof(job_t)); p += s.n_jobs * sizeof(job_t);
+	{ int64_t n; memcpy(&n, p, 8); p += 8; s.qb = p; p += n; memcpy(&n, p, 8); p += 8; s.tb = p; }
+	t0 = now();
+	run(&s, nt, seed_worker);
+	t_seed = now() - t0;
+	t0 = now();
+	run(&s, nt, dp_worker);
+	t_dp = now() - t0;
+	for (i = 0; i < nt; ++i) n_anchor += s.n_anchor[i];
+	printf("{\"threads\": %d, \"n_reads\": %ld, \"n_jobs\": %ld, \"n_anchors\": %ld, \"seed_anchor_s\": %.6f, \"dp_s\": %.6f}\n",
+	       nt, (long)s.n_reads, (long)s.n_jobs, (long)n_anchor, t_seed, t_dp);
+	mb_idx_destroy(s.idx);
+	return 0;
+}
