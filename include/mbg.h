@@ -148,7 +148,11 @@ int mbg_extd2_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_jobs, const mbg_
                         const uint8_t *qseq, const uint8_t *tseq, int8_t m, const int8_t *mat_host,
                         mbg_dp_result_t *res, uint32_t *cigar, int32_t max_qlen, int32_t max_tlen, int32_t max_w);
 
-/* Synchronous copy of `bytes` from a device address returned by this library to host memory. */
+/* Device memory for callers that keep inputs in HBM without another runtime: allocate, free,
+ * and synchronous copies in either direction. */
+int mbg_dev_alloc(mbg_ctx_t *ctx, size_t bytes, void **ptr);
+int mbg_dev_free(mbg_ctx_t *ctx, void *ptr);
+int mbg_copy_to_device(mbg_ctx_t *ctx, void *dst_dev, const void *src, size_t bytes);
 int mbg_copy_to_host(mbg_ctx_t *ctx, void *dst, const void *src_dev, size_t bytes);
 
 /* ---- seed -> locate -> extend step over device-resident reads --------------------------------- */

@@ -634,6 +634,38 @@ int mbg_getseq(mbg_ctx_t *c, int64_t n, const int64_t *tid, const int64_t *st, c
 	return MBG_OK;
 }
 
+int mbg_dev_alloc(mbg_ctx_t *c, size_t bytes, void **ptr)
+{
+	if (c == nullptr || ptr == nullptr) return MBG_EARG;
+	*ptr = nullptr;
+	std::lock_guard<std::mutex> lk(c->mu);
+	HIPCHK(hipSetDevice(c->device));
+	hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+	if (e != hipSuccess) { g_hip_err = hipGetErrorString(e); *ptr = nullptr; return MBG_ENOMEM; }
+	return MBG_OK;
+}
+
+int mbg_dev_free(mbg_ctx_t *c, void *ptr)
+{
+	if (c == nullptr) return MBG_EARG;
+	if (ptr == nullptr) return MBG_OK;
+	std::lock_guard<std::mutex> lk(c->mu);
+	HIPCHK(hipSetDevice(c->device));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	HIPCHK(hipFree(ptr));
+	return MBG_OK;
+}
+
+int mbg_copy_to_device(mbg_ctx_t *c, void *dst, const void *src, size_t bytes)
+{
+	if (c == nullptr || (bytes > 0 && (dst == nullptr || src == nullptr))) return MBG_EARG;
+	if (bytes == 0) return MBG_OK;
+	std::lock_guard<std::mutex> lk(c->mu);
+	HIPCHK(hipSetDevice(c->device));
+	HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+	return MBG_OK;
+}
+
 int mbg_copy_to_host(mbg_ctx_t *c, void *dst, const void *src, size_t bytes)
 {
 	if (c == nullptr || (bytes > 0 && (dst == nullptr || src == nullptr))) return MBG_EARG;

@@ -160,6 +160,9 @@ def lib():
         L.mbg_seed_batch_dev_async.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, C.c_int64, _P]
         L.mbg_sa_batch_dev.argtypes = [_P, _P, C.c_int64, _P]
         L.mbg_copy_to_host.argtypes = [_P, _P, _P, C.c_size_t]
+        L.mbg_copy_to_device.argtypes = [_P, _P, _P, C.c_size_t]
+        L.mbg_dev_alloc.argtypes = [_P, C.c_size_t, C.POINTER(_P)]
+        L.mbg_dev_free.argtypes = [_P, _P]
         L.mbg_map_step_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.POINTER(MapOpt), C.c_int,
                                        C.POINTER(MapStats), C.POINTER(MapOut)]
         L.mbg_extd2_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, _P, _P, C.c_int32, C.c_int32,
@@ -171,7 +174,8 @@ def lib():
 EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mbg_idx_load", "mbg_bwt_upload",
             "mbg_l2b_upload", "mbg_idx_info", "mbg_pre_copy", "mbg_rank2a", "mbg_extend", "mbg_seed_batch",
             "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
-            "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host")
+            "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
+            "mbg_dev_free")
 JOB_META_DTYPE = np.dtype([("rid", "<u4"), ("side", "i1"), ("rev", "i1"), ("pad", "<i2"), ("tid", "<i4"), ("tpos", "<i8")])
 
 
@@ -206,6 +210,21 @@ class DpBatchResult:
     def cigar_of(self, i):
         o = int(self.cigar_off[i])
         return [int(c) for c in self.cigar[o:o + int(self.res["n_cigar"][i])]]
+
+
+class DeviceArray:
+    """A device allocation made through libmbg (no other GPU runtime in the process)."""
+
+    def __init__(self, ctx, ptr, dtype, shape):
+        self.ctx, self.ptr, self.dtype, self.shape = ctx, ptr, np.dtype(dtype), tuple(shape)
+
+    def numpy(self):
+        return self.ctx.to_host(self.ptr, self.dtype, int(np.prod(self.shape))).reshape(self.shape)
+
+    def free(self):
+        if self.ptr and getattr(self.ctx, "h", None):
+            check(self.ctx.L.mbg_dev_free(self.ctx.h, C.c_void_p(self.ptr)), "mbg_dev_free")
+        self.ptr = None
 
 
 class Context:
@@ -331,6 +350,16 @@ class Context:
         check(self.L.mbg_map_step_dev(self.h, stream, n_reads, off_ptr, len_ptr, seq_ptr, max_len, C.byref(opt),
                                       int(collect_stats), C.byref(st), C.byref(out)), "mbg_map_step_dev")
         return st, out
+
+    def to_device(self, a: np.ndarray) -> "DeviceArray":
+        """Copy a host array into a new device allocation owned by the returned DeviceArray."""
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        check(self.L.mbg_dev_alloc(self.h, max(a.nbytes, 1), C.byref(p)), "mbg_dev_alloc")
+        d = DeviceArray(self, p.value, a.dtype, a.shape)
+        if a.nbytes:
+            check(self.L.mbg_copy_to_device(self.h, C.c_void_p(p.value), _ptr(a), a.nbytes), "mbg_copy_to_device")
+        return d
 
     def to_host(self, dev_addr, dtype, count):
         """Copy `count` elements of `dtype` from a device address returned by the library."""

@@ -98,7 +98,6 @@ def expected_jobs(reads, recs, pos_of_rec, ref, ctg_len, opt):
 
 
 def test_map_step_cfg0(gpu_ctx):
-    import torch
     from minibwa_amd import mbg, synth
     from oracle import oracle as O
     pre = os.path.join(GOLDEN, "cfg0")
@@ -108,13 +107,11 @@ def test_map_step_cfg0(gpu_ctx):
     reads = [x for pair in zip(r1, r2) for x in pair]
     n = len(reads)
     rl = 150
-    dev = torch.device("cuda:0")
-    d_seq = torch.from_numpy(np.concatenate(reads)).to(dev)
-    d_off = torch.arange(0, n + 1, dtype=torch.int64, device=dev) * rl
-    d_len = torch.full((n,), rl, dtype=torch.int32, device=dev)
+    d_seq = gpu_ctx.to_device(np.concatenate(reads))
+    d_off = gpu_ctx.to_device(np.arange(0, n + 1, dtype=np.int64) * rl)
+    d_len = gpu_ctx.to_device(np.full(n, rl, np.int32))
     opt = mbg.adaptive_opt(rl)
-    st, out = gpu_ctx.map_step_dev(n, d_off.data_ptr(), d_len.data_ptr(), d_seq.data_ptr(), rl, opt, collect_stats=True)
-    torch.cuda.synchronize()
+    st, out = gpu_ctx.map_step_dev(n, d_off.ptr, d_len.ptr, d_seq.ptr, rl, opt, collect_stats=True)
     recs = gpu_ctx.to_host(out.recs, mbg.SEED_REC_DTYPE, st.n_rec)
     # 1. seeding == mb_seed_intv_batch (as sets per read)
     ri = O.RefIndex(pre + ".mbw")
