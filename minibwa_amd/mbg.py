@@ -176,7 +176,9 @@ EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mb
             "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
             "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
             "mbg_dev_free")
-JOB_META_DTYPE = np.dtype([("rid", "<u4"), ("side", "i1"), ("rev", "i1"), ("pad", "<i2"), ("tid", "<i4"), ("tpos", "<i8")])
+JOB_META_DTYPE = np.dtype([("rid", "<u4"), ("side", "i1"), ("rev", "i1"), ("pad", "<i2"), ("tid", "<i4"), ("tpos", "<i8")],
+                          align=True)
+assert JOB_META_DTYPE.itemsize == 24
 
 
 def _ptr(a: np.ndarray):
