@@ -53,9 +53,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ref-len", type=int, default=100_000_000)
     ap.add_argument("--pairs", type=int, default=1_000_000)
-    ap.add_argument("--cpu-pairs", type=int, default=300_000, help="read pairs in the CPU-baseline sample")
+    ap.add_argument("--cpu-pairs", type=int, default=1_000_000, help="read pairs in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workdir", default=None)
+    ap.add_argument("--calib", type=int, default=0, metavar="MiB",
+                    help="only run the plain copy kernel over MiB mebibytes (HBM byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -63,6 +65,16 @@ def main():
         sys.exit(2)
 
     from minibwa_amd import mbg, synth
+
+    if args.calib:
+        ctx = mbg.Context(0)
+        n = args.calib << 20
+        a = ctx.to_device(np.zeros(n, np.uint8))
+        b = ctx.to_device(np.zeros(n, np.uint8))
+        for _ in range(3):
+            mbg.check(ctx.L.mbg_stream_copy(ctx.h, None, mbg.C.c_void_p(b.ptr), mbg.C.c_void_p(a.ptr), n), "mbg_stream_copy")
+        print(json.dumps({"calib_copy_bytes": n, "launches": 3}))
+        return
 
     t0 = time.time()
     ref = synth.reference(args.ref_len, seed=11, rep_frac=0.05, rep_min=300, rep_max=6000, rep_div=0.02)

@@ -155,6 +155,10 @@ int mbg_dev_free(mbg_ctx_t *ctx, void *ptr);
 int mbg_copy_to_device(mbg_ctx_t *ctx, void *dst_dev, const void *src, size_t bytes);
 int mbg_copy_to_host(mbg_ctx_t *ctx, void *dst, const void *src_dev, size_t bytes);
 
+/* A plain streaming copy kernel (16 B per lane per iteration), used to calibrate HBM byte
+ * counters: its FETCH/WRITE counters are the reference for `bytes` read and written. */
+int mbg_stream_copy(mbg_ctx_t *ctx, void *stream, void *dst_dev, const void *src_dev, size_t bytes);
+
 /* ---- seed -> locate -> extend step over device-resident reads --------------------------------- */
 
 /* Options of one step; defaults are minibwa's (options.c:5-46) with the adaptive band/z-drop for the

@@ -676,6 +676,17 @@ int mbg_copy_to_host(mbg_ctx_t *c, void *dst, const void *src, size_t bytes)
 	return MBG_OK;
 }
 
+int mbg_stream_copy(mbg_ctx_t *c, void *stream, void *dst, const void *src, size_t bytes)
+{
+	if (c == nullptr || dst == nullptr || src == nullptr || (bytes & 15) != 0) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	HIPCHK(hipSetDevice(c->device));
+	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	HIPCHK(launch_stream_copy(dst, src, (int64_t)(bytes / 16), (unsigned)c->n_cu * 16, st));
+	HIPCHK(hipStreamSynchronize(st));
+	return MBG_OK;
+}
+
 // ---- seed -> locate -> extend step --------------------------------------------------------------
 
 static_assert(sizeof(mbg_job_meta_t) == sizeof(JobMeta), "job meta layout");

@@ -271,6 +271,19 @@ __global__ void __launch_bounds__(64) getseq_kernel(L2b lb, int64_t n, const int
 	}
 }
 
+// plain streaming copy (HBM byte-counter calibration)
+__global__ void __launch_bounds__(256) stream_copy_kernel(uint4 *dst, const uint4 *src, int64_t n16)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+		dst[i] = src[i];
+}
+
+hipError_t launch_stream_copy(void *dst, const void *src, int64_t n16, unsigned grid, hipStream_t st)
+{
+	hipLaunchKernelGGL(stream_copy_kernel, dim3(grid), dim3(256), 0, st, (uint4*)dst, (const uint4*)src, n16);
+	return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------

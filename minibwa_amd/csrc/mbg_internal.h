@@ -71,6 +71,7 @@ hipError_t launch_sa(const Idx &ix, int64_t n, const int64_t *n_dev, uint64_t *x
 hipError_t launch_getseq(const L2b &lb, int64_t n, const int64_t *tid, const int64_t *st_, const int64_t *en_, const int64_t *out_off,
                          uint8_t *out, int64_t *len, hipStream_t st);
 hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st);
+hipError_t launch_stream_copy(void *dst, const void *src, int64_t n16, unsigned grid, hipStream_t st);
 hipError_t launch_anchor_rows(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, uint64_t *rows, uint32_t *row_rec,
                               uint32_t *first_slot, unsigned long long *n_rows, int64_t row_cap, unsigned grid, hipStream_t st);
 hipError_t launch_best_seed(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, unsigned long long *best, unsigned grid, hipStream_t st);
