@@ -159,6 +159,11 @@ int mbg_copy_to_host(mbg_ctx_t *ctx, void *dst, const void *src_dev, size_t byte
  * counters: its FETCH/WRITE counters are the reference for `bytes` read and written. */
 int mbg_stream_copy(mbg_ctx_t *ctx, void *stream, void *dst_dev, const void *src_dev, size_t bytes);
 
+/* Random-access ceiling of the resident occ table: n_chains independent lanes each fetch `steps`
+ * 64-byte blocks in sequence, the next block index depending on the bytes just loaded (the access
+ * pattern of one rank per extension step).  Reports the kernel time from HIP events. */
+int mbg_gather_bench(mbg_ctx_t *ctx, int64_t n_chains, int32_t steps, int32_t block_threads, float *ms, uint64_t *checksum);
+
 /* ---- seed -> locate -> extend step over device-resident reads --------------------------------- */
 
 /* Options of one step; defaults are minibwa's (options.c:5-46) with the adaptive band/z-drop for the

@@ -44,6 +44,8 @@ struct mbg_ctx_s {
 	DBuf h_pinned;
 	// seed -> locate -> extend step
 	DBuf p_recs, p_rows, p_rowrec, p_first, p_best, p_jobs, p_meta, p_qb, p_tb, p_res, p_cig, p_ctr;
+	DBuf s_slots1, s_slots2, s_cnt, s_off, s_spill, s_tmp, r_cnt, r_off;
+	int64_t cap_task = 0, cap_spill = 0;
 	int64_t cap_rec = 0, cap_rows = 0;
 	hipEvent_t ev[8] = {};
 };
@@ -140,7 +142,8 @@ void mbg_destroy(mbg_ctx_t *c)
 	for (auto &b : c->b_out) dfree(b);
 	dfree(c->b_ctr); dfree(c->b_task); dfree(c->b_slab);
 	for (DBuf *b : {&c->p_recs, &c->p_rows, &c->p_rowrec, &c->p_first, &c->p_best, &c->p_jobs, &c->p_meta, &c->p_qb, &c->p_tb,
-	                &c->p_res, &c->p_cig, &c->p_ctr}) dfree(*b);
+	                &c->p_res, &c->p_cig, &c->p_ctr, &c->s_slots1, &c->s_slots2, &c->s_cnt, &c->s_off, &c->s_spill, &c->s_tmp,
+	                &c->r_cnt, &c->r_off}) dfree(*b);
 	for (hipEvent_t &e : c->ev) if (e) hipEventDestroy(e);
 	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
 	if (c->stream) hipStreamDestroy(c->stream);
@@ -347,35 +350,78 @@ static unsigned persistent_grid(mbg_ctx_t *c, int64_t n_items, int per_block)
 	return (unsigned)want;
 }
 
-// seeding over device-resident reads; counters live in c->b_ctr: [0]=next(u32) [1]=n_tasks2(u32) [2..3]=n_out(i64)
-static int seed_dev_locked(mbg_ctx_t *c, hipStream_t st, int32_t n_seq, const int64_t *off, const int32_t *len, const uint8_t *seq,
-                           int32_t min_len, int32_t max_sub_occ, mbg_seed_rec_t *out, int64_t cap, int64_t **n_out_dev)
+// Two-pass seeding of device-resident reads into c->p_recs (flat; c->cap_rec entries) with the
+// record count at *n_out_dev.  Per-task output slots (kSlot1 per read, kSlot2 per pass-2 task) are
+// compacted by an exclusive scan, so no output position needs an atomic except overflow spills.
+// Counters in c->b_ctr: u32 [0] task counter, [1] pass-2 task count, i64 [1] record count,
+// u64 [3] spill count.  ev (optional) receives events after pass 1 and after pass 2.
+static const int32_t kSlot1 = 16, kSlot2 = 8;
+
+struct SeedCaps { int64_t rec, task, spill; };
+
+static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
+                     int32_t min_len, int32_t max_sub_occ, unsigned long long *stats, int64_t **n_out_dev, hipEvent_t *ev)
 {
+	if (c->cap_rec < 8 * (int64_t)n_reads + 1024) c->cap_rec = 8 * (int64_t)n_reads + 1024;
+	if (c->cap_task < 2 * (int64_t)n_reads + 1024) c->cap_task = 2 * (int64_t)n_reads + 1024;
+	if (c->cap_spill < (int64_t)n_reads + 4096) c->cap_spill = (int64_t)n_reads + 4096;
+	const int64_t rc = c->cap_rec, tc = c->cap_task, sc = c->cap_spill, n_cnt = (int64_t)n_reads + tc;
 	TRY(dgrow(c->b_ctr, 64));
-	TRY(dgrow(c->b_task, (size_t)(cap > 0 ? cap : 1) * sizeof(SeedTask)));
+	TRY(dgrow(c->p_recs, rc * sizeof(SeedRec)));
+	TRY(dgrow(c->s_slots1, (size_t)n_reads * kSlot1 * sizeof(SeedRec) + 64));
+	TRY(dgrow(c->s_slots2, (size_t)tc * kSlot2 * sizeof(SeedRec)));
+	TRY(dgrow(c->s_cnt, (size_t)n_cnt * 4)); TRY(dgrow(c->s_off, (size_t)n_cnt * 4));
+	TRY(dgrow(c->s_spill, (size_t)sc * sizeof(SeedRec)));
+	TRY(dgrow(c->b_task, (size_t)tc * sizeof(SeedTask)));
+	size_t tmp_bytes = 0;
+	HIPCHK(scan_u32(nullptr, &tmp_bytes, nullptr, nullptr, n_cnt, st));
+	TRY(dgrow(c->s_tmp, tmp_bytes));
 	uint32_t *ctr = (uint32_t*)c->b_ctr.p;
 	int64_t *n_out = (int64_t*)(ctr + 2);
-	HIPCHK(hipMemsetAsync(ctr, 0, 16, st));
+	unsigned long long *n_spill = (unsigned long long*)(ctr + 6);
+	uint32_t *cnt = (uint32_t*)c->s_cnt.p;
+	HIPCHK(hipMemsetAsync(ctr, 0, 64, st));
+	HIPCHK(hipMemsetAsync(cnt, 0, (size_t)n_cnt * 4, st));
 	SeedArgs a{};
-	a.seq = seq; a.off = off; a.len = len;
-	a.tasks = nullptr; a.n_tasks = (uint32_t)n_seq;
-	a.min_len = min_len;
-	a.next = ctr;
-	a.out = (SeedRec*)out; a.n_out = n_out; a.cap = cap;
-	if (n_seq > 0) {
-		HIPCHK(launch_smem(c->ix, a, 1, persistent_grid(c, n_seq, 256), st));
-		// pass 2: tasks from the pass-1 records, then a second persistent launch
-		HIPCHK(launch_subtask((const SeedRec*)out, n_out, cap, min_len, max_sub_occ, (SeedTask*)c->b_task.p, ctr + 1, cap,
-		                      (unsigned)c->n_cu * 4, st));
+	a.seq = seq; a.off = off; a.len = len; a.n_tasks = (uint32_t)n_reads; a.min_len = min_len; a.next = ctr;
+	a.slots = (SeedRec*)c->s_slots1.p; a.slot_cnt = cnt; a.kslot = kSlot1;
+	a.spill = (SeedRec*)c->s_spill.p; a.n_spill = n_spill; a.spill_cap = sc;
+	a.stats = stats;
+	if (n_reads > 0) HIPCHK(launch_smem(c->ix, a, 1, persistent_grid(c, n_reads, 256), st));
+	if (ev) HIPCHK(hipEventRecord(ev[0], st));
+	if (n_reads > 0) {
+		HIPCHK(launch_subtask((const SeedRec*)c->s_slots1.p, cnt, kSlot1, n_reads, (const SeedRec*)c->s_spill.p, n_spill, sc,
+		                      min_len, max_sub_occ, (SeedTask*)c->b_task.p, ctr + 1, tc, st));
 		HIPCHK(hipMemsetAsync(ctr, 0, 4, st));
-		a.tasks = (const SeedTask*)c->b_task.p;
-		a.n_tasks = 0xffffffffu;              // read from ctr[1] on the device
-		a.n_tasks_dev = ctr + 1;
-		a.task_cap = cap;
-		HIPCHK(launch_smem(c->ix, a, 2, persistent_grid(c, n_seq, 256), st));
+		SeedArgs a2 = a;
+		a2.tasks = (const SeedTask*)c->b_task.p; a2.n_tasks = 0xffffffffu; a2.n_tasks_dev = ctr + 1; a2.task_cap = tc;
+		a2.slots = (SeedRec*)c->s_slots2.p; a2.slot_cnt = cnt + n_reads; a2.kslot = kSlot2;
+		a2.stats = stats ? stats + 3 : nullptr;
+		HIPCHK(launch_smem(c->ix, a2, 2, persistent_grid(c, n_reads, 256), st));
 	}
+	HIPCHK(scan_u32(c->s_tmp.p, &tmp_bytes, cnt, (uint32_t*)c->s_off.p, n_cnt, st));
+	HIPCHK(launch_compact((const SeedRec*)c->s_slots1.p, (const SeedRec*)c->s_slots2.p, kSlot1, kSlot2, n_reads, n_cnt, cnt,
+	                      (const uint32_t*)c->s_off.p, (const SeedRec*)c->s_spill.p, n_spill, sc, (SeedRec*)c->p_recs.p, rc, n_out, st));
+	if (ev) HIPCHK(hipEventRecord(ev[1], st));
 	*n_out_dev = n_out;
 	return MBG_OK;
+}
+
+// after the stream has synchronised: grow any seeding pool that overflowed; true if one did
+static bool seed_overflowed(mbg_ctx_t *c, int64_t *n_rec_host)
+{
+	uint32_t h[8];
+	if (hipMemcpy(h, c->b_ctr.p, 32, hipMemcpyDeviceToHost) != hipSuccess) return false;
+	int64_t n_out;
+	uint64_t n_spill;
+	memcpy(&n_out, h + 2, 8);
+	memcpy(&n_spill, h + 6, 8);
+	bool ov = false;
+	if ((int64_t)h[1] > c->cap_task) c->cap_task = (int64_t)h[1] * 2, ov = true;
+	if ((int64_t)n_spill > c->cap_spill) c->cap_spill = (int64_t)n_spill * 2, ov = true;
+	if (n_out > c->cap_rec) c->cap_rec = n_out * 2, ov = true;
+	if (n_rec_host) *n_rec_host = n_out;
+	return ov;
 }
 
 int mbg_seed_batch_dev_async(mbg_ctx_t *c, void *stream, int32_t n_seq, const int64_t *off, const int32_t *len, const uint8_t *seq,
@@ -387,7 +433,9 @@ int mbg_seed_batch_dev_async(mbg_ctx_t *c, void *stream, int32_t n_seq, const in
 	HIPCHK(hipSetDevice(c->device));
 	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
 	int64_t *nd = nullptr;
-	TRY(seed_dev_locked(c, st, n_seq, off, len, seq, min_len, max_sub_occ, out, cap, &nd));
+	TRY(seed_core(c, st, n_seq, off, len, seq, min_len, max_sub_occ, nullptr, &nd, nullptr));
+	const int64_t m = cap < c->cap_rec ? cap : c->cap_rec;
+	if (m > 0) HIPCHK(hipMemcpyAsync(out, c->p_recs.p, (size_t)m * sizeof(SeedRec), hipMemcpyDeviceToDevice, st));
 	HIPCHK(hipMemcpyAsync(n_out_dev, nd, 8, hipMemcpyDeviceToDevice, st));
 	return MBG_OK;
 }
@@ -400,11 +448,16 @@ int mbg_seed_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_seq, const int64_t 
 	if (!c->has_bwt) return MBG_ENOIDX;
 	HIPCHK(hipSetDevice(c->device));
 	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-	int64_t *nd = nullptr;
-	TRY(seed_dev_locked(c, st, n_seq, off, len, seq, min_len, max_sub_occ, out, cap, &nd));
-	HIPCHK(hipMemcpyAsync(n_out_host, nd, 8, hipMemcpyDeviceToHost, st));
-	HIPCHK(hipStreamSynchronize(st));
-	return *n_out_host > cap ? MBG_EOVERFLOW : MBG_OK;
+	for (int attempt = 0; attempt < 4; ++attempt) {
+		int64_t *nd = nullptr;
+		TRY(seed_core(c, st, n_seq, off, len, seq, min_len, max_sub_occ, nullptr, &nd, nullptr));
+		HIPCHK(hipStreamSynchronize(st));
+		if (seed_overflowed(c, n_out_host)) continue;
+		const int64_t m = cap < *n_out_host ? cap : *n_out_host;
+		if (m > 0) HIPCHK(hipMemcpy(out, c->p_recs.p, (size_t)m * sizeof(SeedRec), hipMemcpyDeviceToDevice));
+		return *n_out_host > cap ? MBG_EOVERFLOW : MBG_OK;
+	}
+	return MBG_EOVERFLOW;
 }
 
 int mbg_seed_batch(mbg_ctx_t *c, int32_t n_seq, const int32_t *len, const uint8_t *const *seq, int32_t min_len, int32_t max_sub_occ,
@@ -427,19 +480,18 @@ int mbg_seed_batch(mbg_ctx_t *c, int32_t n_seq, const int32_t *len, const uint8_
 	HIPCHK(hipMemcpyAsync(c->b_in[0].p, buf.data(), tot, hipMemcpyHostToDevice, c->stream));
 	HIPCHK(hipMemcpyAsync(c->b_in[1].p, off.data(), (n_seq + 1) * 8, hipMemcpyHostToDevice, c->stream));
 	if (n_seq > 0) HIPCHK(hipMemcpyAsync(c->b_in[2].p, len, n_seq * 4, hipMemcpyHostToDevice, c->stream));
-	int64_t cap = 8 * (int64_t)n_seq + tot / 4 + 1024, n_out = 0;
-	for (;;) {
-		TRY(dgrow(c->b_out[0], cap * sizeof(SeedRec)));
+	int64_t n_out = 0;
+	bool ok = false;
+	for (int attempt = 0; attempt < 4 && !ok; ++attempt) {
 		int64_t *nd = nullptr;
-		TRY(seed_dev_locked(c, c->stream, n_seq, (const int64_t*)c->b_in[1].p, (const int32_t*)c->b_in[2].p, (const uint8_t*)c->b_in[0].p,
-		                    min_len, max_sub_occ, (mbg_seed_rec_t*)c->b_out[0].p, cap, &nd));
-		HIPCHK(hipMemcpyAsync(&n_out, nd, 8, hipMemcpyDeviceToHost, c->stream));
+		TRY(seed_core(c, c->stream, n_seq, (const int64_t*)c->b_in[1].p, (const int32_t*)c->b_in[2].p, (const uint8_t*)c->b_in[0].p,
+		              min_len, max_sub_occ, nullptr, &nd, nullptr));
 		HIPCHK(hipStreamSynchronize(c->stream));
-		if (n_out <= cap) break;
-		cap = n_out * 2;
+		ok = !seed_overflowed(c, &n_out);
 	}
+	if (!ok) return MBG_EOVERFLOW;
 	std::vector<SeedRec> recs(n_out);
-	if (n_out) HIPCHK(hipMemcpy(recs.data(), c->b_out[0].p, n_out * sizeof(SeedRec), hipMemcpyDeviceToHost));
+	if (n_out) HIPCHK(hipMemcpy(recs.data(), c->p_recs.p, n_out * sizeof(SeedRec), hipMemcpyDeviceToHost));
 	// group by read: pass-1 records first, each group in device order
 	std::vector<int64_t> cnt(n_seq + 1, 0);
 	for (const SeedRec &r : recs) ++cnt[r.rid + 1];
@@ -687,6 +739,27 @@ int mbg_stream_copy(mbg_ctx_t *c, void *stream, void *dst, const void *src, size
 	return MBG_OK;
 }
 
+int mbg_gather_bench(mbg_ctx_t *c, int64_t n_chains, int32_t steps, int32_t bs, float *ms, uint64_t *checksum)
+{
+	if (c == nullptr || n_chains <= 0 || steps <= 0 || ms == nullptr || (bs != 64 && bs != 128 && bs != 256 && bs != 512 && bs != 1024)) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt) return MBG_ENOIDX;
+	HIPCHK(hipSetDevice(c->device));
+	for (hipEvent_t &e : c->ev) if (!e) HIPCHK(hipEventCreate(&e));
+	TRY(dgrow(c->b_ctr, 64));
+	unsigned long long *sum = (unsigned long long*)c->b_ctr.p + 6;
+	HIPCHK(hipMemsetAsync(sum, 0, 8, c->stream));
+	HIPCHK(hipEventRecord(c->ev[0], c->stream));
+	HIPCHK(launch_gather_chain(c->ix, n_chains, steps, bs, sum, c->stream));
+	HIPCHK(hipEventRecord(c->ev[1], c->stream));
+	unsigned long long h = 0;
+	HIPCHK(hipMemcpyAsync(&h, sum, 8, hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	HIPCHK(hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+	if (checksum) *checksum = h;
+	return MBG_OK;
+}
+
 // ---- seed -> locate -> extend step --------------------------------------------------------------
 
 static_assert(sizeof(mbg_job_meta_t) == sizeof(JobMeta), "job meta layout");
@@ -717,12 +790,8 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 	int64_t slab;
 	TRY(dp_plan(max_len > 0 ? max_len : 1, max_tlen, opt->ksw_w, &cap_dp, &slab));
 	const int64_t job_cap = 2 * (int64_t)n_reads + 16;
-	if (c->cap_rec < 8 * (int64_t)n_reads + 1024) c->cap_rec = 8 * (int64_t)n_reads + 1024;
 	if (c->cap_rows < 16 * (int64_t)n_reads + 1024) c->cap_rows = 16 * (int64_t)n_reads + 1024;
 	for (int attempt = 0; attempt < 4; ++attempt) {
-		const int64_t rc = c->cap_rec, rw = c->cap_rows;
-		TRY(dgrow(c->p_recs, rc * sizeof(SeedRec)));
-		TRY(dgrow(c->p_rows, rw * 8)); TRY(dgrow(c->p_rowrec, rw * 4)); TRY(dgrow(c->p_first, rc * 4));
 		TRY(dgrow(c->p_best, (size_t)n_reads * 8 + 8));
 		TRY(dgrow(c->p_jobs, job_cap * sizeof(DpJob))); TRY(dgrow(c->p_meta, job_cap * sizeof(JobMeta)));
 		TRY(dgrow(c->p_qb, (size_t)n_reads * max_len + 16)); TRY(dgrow(c->p_tb, (size_t)job_cap * max_tlen + 16));
@@ -733,39 +802,32 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 		HIPCHK(hipMemsetAsync(pc, 0, 256, st));
 		HIPCHK(hipMemsetAsync(c->p_best.p, 0, (size_t)n_reads * 8 + 8, st));
 		HIPCHK(hipEventRecord(c->ev[0], st));
-		// seeding (two launches of the persistent state machine)
-		TRY(dgrow(c->b_ctr, 64));
-		TRY(dgrow(c->b_task, (size_t)rc * sizeof(SeedTask)));
-		uint32_t *ctr = (uint32_t*)c->b_ctr.p;
-		int64_t *n_rec = (int64_t*)(ctr + 2);
-		HIPCHK(hipMemsetAsync(ctr, 0, 32, st));
-		SeedArgs a{};
-		a.seq = seq; a.off = off; a.len = len; a.n_tasks = (uint32_t)n_reads; a.min_len = opt->min_len;
-		a.next = ctr; a.out = (SeedRec*)c->p_recs.p; a.n_out = n_rec; a.cap = rc;
-		a.stats = collect_stats ? pc + 8 : nullptr;
-		if (n_reads > 0) HIPCHK(launch_smem(c->ix, a, 1, persistent_grid(c, n_reads, 256), st));
-		HIPCHK(hipEventRecord(c->ev[1], st));
-		if (n_reads > 0) {
-			HIPCHK(launch_subtask(a.out, n_rec, rc, opt->min_len, opt->max_sub_occ, (SeedTask*)c->b_task.p, ctr + 1, rc,
-			                      (unsigned)c->n_cu * 4, st));
-			HIPCHK(hipMemsetAsync(ctr, 0, 4, st));
-			SeedArgs a2 = a;
-			a2.tasks = (const SeedTask*)c->b_task.p; a2.n_tasks = 0xffffffffu; a2.n_tasks_dev = ctr + 1; a2.task_cap = rc;
-			a2.stats = collect_stats ? pc + 11 : nullptr;
-			HIPCHK(launch_smem(c->ix, a2, 2, persistent_grid(c, n_reads, 256), st));
-		}
-		HIPCHK(hipEventRecord(c->ev[2], st));
-		// anchor rows and SA locate
-		HIPCHK(launch_anchor_rows(a.out, n_rec, rc, opt->max_occ, (uint64_t*)c->p_rows.p, (uint32_t*)c->p_rowrec.p, (uint32_t*)c->p_first.p,
-		                          pc + 6, rw, (unsigned)c->n_cu * 8, st));
-		HIPCHK(launch_best_seed(a.out, n_rec, rc, (unsigned long long*)c->p_best.p, (unsigned)c->n_cu * 8, st));
+		// seeding: two launches of the persistent state machine, then compaction
+		int64_t *n_rec = nullptr;
+		TRY(seed_core(c, st, n_reads, off, len, seq, opt->min_len, opt->max_sub_occ, collect_stats ? pc + 8 : nullptr, &n_rec, c->ev + 1));
+		const int64_t rc = c->cap_rec, rw = c->cap_rows;
+		TRY(dgrow(c->p_rows, rw * 8)); TRY(dgrow(c->p_rowrec, rw * 4)); TRY(dgrow(c->p_first, rc * 4));
+		TRY(dgrow(c->r_cnt, rc * 4)); TRY(dgrow(c->r_off, rc * 4));
+		size_t tmp_bytes = 0;
+		HIPCHK(scan_u32(nullptr, &tmp_bytes, nullptr, nullptr, rc, st));
+		TRY(dgrow(c->s_tmp, tmp_bytes));
+		// anchor rows at scanned offsets, SA locate
+		const SeedRec *recs = (const SeedRec*)c->p_recs.p;
+		HIPCHK(launch_row_counts(recs, n_rec, rc, opt->max_occ, (uint32_t*)c->r_cnt.p, st));
+		HIPCHK(scan_u32(c->s_tmp.p, &tmp_bytes, (const uint32_t*)c->r_cnt.p, (uint32_t*)c->r_off.p, rc, st));
+		HIPCHK(launch_anchor_rows(recs, n_rec, rc, opt->max_occ, (const uint32_t*)c->r_cnt.p, (const uint32_t*)c->r_off.p,
+		                          (uint64_t*)c->p_rows.p, (uint32_t*)c->p_rowrec.p, (uint32_t*)c->p_first.p, pc + 6, rw, st));
+		HIPCHK(launch_best_seed(recs, n_rec, rc, (unsigned long long*)c->p_best.p, (unsigned)c->n_cu * 8, st));
 		HIPCHK(hipEventRecord(c->ev[3], st));
+		uint32_t *ctr = (uint32_t*)c->b_ctr.p;
 		HIPCHK(hipMemsetAsync(ctr + 4, 0, 4, st));
 		HIPCHK(launch_sa(c->ix, rw, (const int64_t*)(pc + 6), (uint64_t*)c->p_rows.p, ctr + 4, collect_stats ? pc + 14 : nullptr,
 		                 persistent_grid(c, 4 * (int64_t)n_reads + 1, 256), st));
 		HIPCHK(hipEventRecord(c->ev[4], st));
+		const SeedRec *recs_c = recs;
+		(void)recs_c;
 		// extension jobs and DP
-		HIPCHK(launch_ext_jobs(c->lb, mo, n_reads, off, len, seq, a.out, (const uint32_t*)c->p_first.p, (const uint64_t*)c->p_rows.p,
+		HIPCHK(launch_ext_jobs(c->lb, mo, n_reads, off, len, seq, recs, (const uint32_t*)c->p_first.p, (const uint64_t*)c->p_rows.p,
 		                       (const unsigned long long*)c->p_best.p, (DpJob*)c->p_jobs.p, (JobMeta*)c->p_meta.p, (uint8_t*)c->p_qb.p,
 		                       (uint8_t*)c->p_tb.p, job_cap, (int64_t)n_reads * max_len, job_cap * max_tlen, pc, st));
 		HIPCHK(hipEventRecord(c->ev[5], st));
@@ -785,15 +847,12 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 		}
 		HIPCHK(hipEventRecord(c->ev[6], st));
 		unsigned long long hc[16];
-		int64_t hn_rec = 0;
 		HIPCHK(hipMemcpyAsync(hc, pc, sizeof(hc), hipMemcpyDeviceToHost, st));
-		HIPCHK(hipMemcpyAsync(&hn_rec, n_rec, 8, hipMemcpyDeviceToHost, st));
 		HIPCHK(hipStreamSynchronize(st));
-		if (hn_rec > rc || (int64_t)hc[6] > rw) { // a pool overflowed: grow it and run the step again
-			if (hn_rec > rc) c->cap_rec = hn_rec * 2;
-			if ((int64_t)hc[6] > rw) c->cap_rows = (int64_t)hc[6] * 2;
-			continue;
-		}
+		int64_t hn_rec = 0;
+		bool ov = seed_overflowed(c, &hn_rec);
+		if ((int64_t)hc[6] > rw) c->cap_rows = (int64_t)hc[6] * 2, ov = true;
+		if (ov) continue;  // a pool overflowed: it has grown; run the step again
 		stats->n_rec = hn_rec; stats->n_rows = (int64_t)hc[6]; stats->n_jobs = (int64_t)hc[0]; stats->n_fast = (int64_t)hc[3];
 		stats->n_anchored = (int64_t)hc[4]; stats->q_bytes = (int64_t)hc[1]; stats->t_bytes = (int64_t)hc[2];
 		stats->occ_blocks[0] = hc[8]; stats->kmer_lookups[0] = hc[9]; stats->extend_steps[0] = hc[10];

@@ -25,6 +25,9 @@ struct Idx {
 
 constexpr uint64_t kM56 = (1ULL << 56) - 1;
 
+template <typename T>
+__device__ __forceinline__ T pick4(int c, T a0, T a1, T a2, T a3) { return c == 0 ? a0 : c == 1 ? a1 : c == 2 ? a2 : a3; }
+
 __device__ __forceinline__ void load_block(const uint64_t *data, uint64_t blk, uint64_t b[8])
 {
 	const ulonglong2 *p = reinterpret_cast<const ulonglong2*>(data + (blk << 3));
@@ -42,9 +45,12 @@ __device__ __forceinline__ uint32_t sym_cnt(uint64_t w, uint64_t pat, int n)
 	return (uint32_t)__popcll(e & m);
 }
 
+// four per-base counts as scalars (arrays indexed at run time would go through scratch memory)
+struct C4 { uint64_t a, c, g, t; };
+
 // occurrences of each base among $-removed symbols [blk*128, blk*128 + in] plus everything
 // before the block (bwt.c:160-193: the 56/8-bit counts and the half-block start)
-__device__ __forceinline__ void occ_incl(const uint64_t b[8], uint32_t in, uint64_t cnt[4])
+__device__ __forceinline__ C4 occ_incl(const uint64_t b[8], uint32_t in)
 {
 	const int half = in >= 64;
 	const int n = (int)(in & 63) + 1;
@@ -54,10 +60,12 @@ __device__ __forceinline__ void occ_incl(const uint64_t b[8], uint32_t in, uint6
 	const uint32_t c2 = sym_cnt(w0, 0xAAAAAAAAAAAAAAAAULL, n0) + sym_cnt(w1, 0xAAAAAAAAAAAAAAAAULL, n1);
 	const uint32_t c3 = sym_cnt(w0, ~0ULL, n0) + sym_cnt(w1, ~0ULL, n1);
 	const uint32_t c0 = (uint32_t)n - c1 - c2 - c3;
-	cnt[0] = (b[0] & kM56) + (half ? (b[0] >> 56) : 0) + c0;
-	cnt[1] = (b[1] & kM56) + (half ? (b[1] >> 56) : 0) + c1;
-	cnt[2] = (b[2] & kM56) + (half ? (b[2] >> 56) : 0) + c2;
-	cnt[3] = (b[3] & kM56) + (half ? (b[3] >> 56) : 0) + c3;
+	C4 r;
+	r.a = (b[0] & kM56) + (half ? (b[0] >> 56) : 0) + c0;
+	r.c = (b[1] & kM56) + (half ? (b[1] >> 56) : 0) + c1;
+	r.g = (b[2] & kM56) + (half ? (b[2] >> 56) : 0) + c2;
+	r.t = (b[3] & kM56) + (half ? (b[3] >> 56) : 0) + c3;
+	return r;
 }
 
 // symbol at $-removed index k (bwt.c:119); reads past the array give 0
@@ -70,67 +78,75 @@ __device__ __forceinline__ int bwt_b0(const Idx &ix, uint64_t k)
 
 // mb_bwt_rank2a (bwt.c:195-231) for k <= l.  One block fetch when k-1 and l-1 share a block;
 // returns the number of 64-byte blocks fetched.
-__device__ __forceinline__ int rank2a(const Idx &ix, uint64_t k, uint64_t l, uint64_t ck[4], uint64_t cl[4])
+__device__ __forceinline__ int rank2a(const Idx &ix, uint64_t k, uint64_t l, C4 &ck, C4 &cl)
 {
 	int nblk = 0;
 	uint64_t b[8];
 	uint64_t blk = ~0ULL;
-	if (k == 0) {
-		ck[0] = ck[1] = ck[2] = ck[3] = 0;
-	} else {
+	ck = C4{0, 0, 0, 0};
+	if (k != 0) {
 		uint64_t j = k - 1;
 		j -= (j >= ix.primary);
 		blk = j >> 7;
 		load_block(ix.data, blk, b);
 		++nblk;
-		occ_incl(b, (uint32_t)(j & 127), ck);
+		ck = occ_incl(b, (uint32_t)(j & 127));
 	}
 	// bwt.c:205-209: with l==k+1 the reference adds the symbol at $-removed index k-(k>primary);
 	// at k==primary that row is '$' and the shortcut's count differs from rank1a(l) -- keep it.
 	if (l == k + 1 && k != 0 && k == ix.primary) {
-		cl[0] = ck[0]; cl[1] = ck[1]; cl[2] = ck[2]; cl[3] = ck[3];
-		cl[bwt_b0(ix, k)] += 1;
+		const int b0 = bwt_b0(ix, k);
+		cl = C4{ck.a + (b0 == 0), ck.c + (b0 == 1), ck.g + (b0 == 2), ck.t + (b0 == 3)};
 		return nblk;
 	}
-	if (l == 0) {
-		cl[0] = cl[1] = cl[2] = cl[3] = 0;
-	} else {
+	cl = C4{0, 0, 0, 0};
+	if (l != 0) {
 		uint64_t j = l - 1;
 		j -= (j >= ix.primary);
 		if ((j >> 7) != blk) load_block(ix.data, j >> 7, b), ++nblk;
-		occ_incl(b, (uint32_t)(j & 127), cl);
+		cl = occ_incl(b, (uint32_t)(j & 127));
 	}
 	return nblk;
+}
+
+// array form for the probe kernels
+__device__ __forceinline__ int rank2a(const Idx &ix, uint64_t k, uint64_t l, uint64_t ck[4], uint64_t cl[4])
+{
+	C4 a, b;
+	const int n = rank2a(ix, k, l, a, b);
+	ck[0] = a.a; ck[1] = a.c; ck[2] = a.g; ck[3] = a.t;
+	cl[0] = b.a; cl[1] = b.c; cl[2] = b.g; cl[3] = b.t;
+	return n;
 }
 
 // mb_bwt_set_intv (bwt.h:75-81)
 __device__ __forceinline__ Sai set_intv(const Idx &ix, int c)
 {
 	Sai p;
-	p.x0 = ix.L2[c] + 1;
-	p.x1 = ix.L2[3 - c] + 1;
-	p.size = ix.L2[c + 1] - ix.L2[c];
+	p.x0 = pick4(c, ix.L2[0], ix.L2[1], ix.L2[2], ix.L2[3]) + 1;
+	p.x1 = pick4(c, ix.L2[3], ix.L2[2], ix.L2[1], ix.L2[0]) + 1;
+	p.size = pick4(c, ix.L2[1] - ix.L2[0], ix.L2[2] - ix.L2[1], ix.L2[3] - ix.L2[2], ix.L2[4] - ix.L2[3]);
 	p.info = 0;
 	return p;
 }
 
-// mb_bwt_extend (bwt.c:237-250) restricted to the one base c the caller keeps.
+// mb_bwt_extend (bwt.c:237-250) restricted to the one base c the caller keeps (selects instead of
+// indexing, so nothing is addressed through scratch memory).
 __device__ __forceinline__ Sai extend1(const Idx &ix, const Sai &ik, int is_back, int c, int *nblk = nullptr)
 {
 	const uint64_t xf = is_back ? ik.x0 : ik.x1;   // x[!is_back]
 	const uint64_t xg = is_back ? ik.x1 : ik.x0;   // x[is_back]
-	uint64_t tk[4], tl[4];
+	C4 tk, tl;
 	const int nb = rank2a(ix, xf, xf + ik.size, tk, tl);
 	if (nblk) *nblk += nb;
-	uint64_t acc = xg + (uint64_t)(xf <= ix.primary && xf + ik.size > ix.primary);
-	#pragma unroll
-	for (int b = 3; b > 0; --b)
-		if (b > c) acc += tl[b] - tk[b];
+	const uint64_t s0 = tl.a - tk.a, s1 = tl.c - tk.c, s2 = tl.g - tk.g, s3 = tl.t - tk.t;
+	const uint64_t acc = xg + (uint64_t)(xf <= ix.primary && xf + ik.size > ix.primary)
+	                   + (c < 3 ? s3 : 0) + (c < 2 ? s2 : 0) + (c < 1 ? s1 : 0);
 	Sai o;
-	const uint64_t nf = ix.L2[c] + 1 + tk[c];
+	const uint64_t nf = pick4(c, ix.L2[0], ix.L2[1], ix.L2[2], ix.L2[3]) + 1 + pick4(c, tk.a, tk.c, tk.g, tk.t);
 	o.x0 = is_back ? nf : acc;
 	o.x1 = is_back ? acc : nf;
-	o.size = tl[c] - tk[c];
+	o.size = pick4(c, s0, s1, s2, s3);
 	o.info = 0;
 	return o;
 }
@@ -163,28 +179,62 @@ __device__ __forceinline__ uint64_t lf(const Idx &ix, uint64_t k)
 	uint64_t b[8];
 	load_block(ix.data, x >> 7, b);
 	const uint32_t in = (uint32_t)(x & 127);
-	const int c = (int)(b[4 + (in >> 5)] >> ((in & 31) << 1) & 3);
+	const int c = (int)(pick4((int)(in >> 5), b[4], b[5], b[6], b[7]) >> ((in & 31) << 1) & 3);
 	const int half = in >= 64;
 	const int n = (int)(in & 63);                 // symbols before x within the half
 	const uint64_t pat = c == 0 ? 0 : c == 1 ? 0x5555555555555555ULL : c == 2 ? 0xAAAAAAAAAAAAAAAAULL : ~0ULL;
 	const uint64_t w0 = half ? b[6] : b[4], w1 = half ? b[7] : b[5];
 	const int n0 = n > 32 ? 32 : n, n1 = n > 32 ? n - 32 : 0;
-	const uint64_t cnt = (b[c] & kM56) + (half ? (b[c] >> 56) : 0) + sym_cnt(w0, pat, n0) + sym_cnt(w1, pat, n1);
-	return ix.L2[c] + 1 + cnt;
+	const uint64_t bc = pick4(c, b[0], b[1], b[2], b[3]);
+	const uint64_t cnt = (bc & kM56) + (half ? (bc >> 56) : 0) + sym_cnt(w0, pat, n0) + sym_cnt(w1, pat, n1);
+	return pick4(c, ix.L2[0], ix.L2[1], ix.L2[2], ix.L2[3]) + 1 + cnt;
 }
 
-// wave-aggregated fetch of the next work item from a global counter
-__device__ __forceinline__ uint32_t wave_fetch(uint32_t *counter, bool want)
+// exclusive prefix sum across the 64 lanes of a converged wave; *total = the wave's sum
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t *total)
 {
-	const uint64_t mask = __ballot(want);
-	if (!want) return 0;
-	const int leader = __ffsll((unsigned long long)mask) - 1;
 	const uint32_t lane = __lane_id();
-	uint32_t base = 0;
-	if (lane == (uint32_t)leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-	base = __shfl(base, leader);
-	const uint64_t lt = lane == 0 ? 0 : (mask & ((~0ULL) >> (64 - lane)));
-	return base + (uint32_t)__popcll(lt);
+	uint32_t x = v;
+	#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint32_t y = __shfl_up(x, o);
+		if (lane >= (uint32_t)o) x += y;
+	}
+	*total = __shfl(x, 63);
+	return x - v;
 }
+
+// Work distribution with one global atomic per kChunk items per wave.  The wave holds a chunk
+// [next, end) of item indices (the same values in every live lane) and hands them to its lanes in
+// lane order; only when the chunk runs out does its first live lane take a new chunk from the
+// global counter.  Indices past the caller's item count mean "no more work".
+struct WaveQueue {
+	static constexpr uint32_t kChunk = 64;
+	uint32_t next = 0, end = 0;
+	__device__ __forceinline__ uint32_t fetch(uint32_t *counter, bool want)
+	{
+		const uint64_t mask = __ballot(want);
+		if (mask == 0) return 0;
+		const uint32_t lane = __lane_id();
+		const uint32_t k = (uint32_t)__popcll(mask);
+		const uint32_t r = (uint32_t)__popcll(mask & ((1ULL << lane) - 1));
+		const uint32_t avail = end - next;
+		uint32_t t;
+		if (k <= avail) {
+			t = next + r;
+			next += k;
+		} else {
+			const uint64_t live = __ballot(1);
+			const int leader = __ffsll((unsigned long long)live) - 1;
+			uint32_t base = 0;
+			if ((int)lane == leader) base = atomicAdd(counter, kChunk);
+			base = __shfl(base, leader);
+			t = r < avail ? next + r : base + (r - avail);
+			next = base + (k - avail);
+			end = base + kChunk;
+		}
+		return t;
+	}
+};
 
 } // namespace mbg

@@ -46,48 +46,79 @@ __global__ void __launch_bounds__(256) extend_kernel(Idx ix, int64_t n, const Sa
 
 // ---------------------------------------------------------------------------------------------
 // SMEM state machine
+//
+// Each lane copies its task's bases into LDS as 2-bit codes when the window is N-free and at most
+// kQWords*16 long, so the per-step base lookups stay off the memory path; other windows read their
+// bases from global memory.  Backward and forward steps share one extension call site, so a wave
+// whose lanes sit in different stages still issues one rank per loop trip.
 // ---------------------------------------------------------------------------------------------
 enum : int { ACT_DONE = 0, ACT_LOOKUP = 1, ACT_BACK = 2, ACT_FWD = 3 };
+constexpr int kSmemBlock = 256;
+constexpr int kQWords = 24;            // up to 384 bases per lane in LDS
 
 struct Lane {
-	const uint8_t *q;
+	const uint8_t *q;                  // task bases in global memory (absolute positions)
+	uint32_t task, nemit;              // output slot owner and records emitted so far
+	int32_t qbase;                     // absolute position of LDS base 0
 	int32_t en, min_len, x, i, stage;
 	uint32_t kmer, rid;
 	uint64_t min_occ;
-	Sai p;
+	uint64_t px0, px1, psize;          // current interval
+	bool gq;                           // bases read from global memory
 };
 
-__device__ __forceinline__ void emit(const SeedArgs &a, const Lane &s, uint32_t pass)
+__device__ __forceinline__ int qget(const Lane &s, const uint32_t *qs, int i)
 {
-	const unsigned long long slot = atomicAdd((unsigned long long*)a.n_out, 1ULL);
-	if ((int64_t)slot < a.cap) {
-		SeedRec r;
-		r.x0 = s.p.x0; r.x1 = s.p.x1; r.size = s.p.size;
-		r.info = (uint64_t)(uint32_t)s.x << 32 | (uint64_t)(uint32_t)s.i;
-		r.rid = s.rid; r.pass = pass;
-		a.out[slot] = r;
+	if (s.gq) return s.q[i];
+	const int k = i - s.qbase;
+	return (int)(qs[(k >> 4) * kSmemBlock + threadIdx.x] >> ((k & 15) << 1) & 3);
+}
+
+__device__ __forceinline__ void set_p(Lane &s, const Sai &p) { s.px0 = p.x0; s.px1 = p.x1; s.psize = p.size; }
+
+// append one record to the task's slots (the spill list past kslot)
+__device__ __forceinline__ void emit(const SeedArgs &a, Lane &s, uint32_t pass)
+{
+	SeedRec r;
+	r.x0 = s.px0; r.x1 = s.px1; r.size = s.psize;
+	r.info = (uint64_t)(uint32_t)s.x << 32 | (uint64_t)(uint32_t)s.i;
+	r.rid = s.rid; r.pass = pass;
+	if (s.nemit < (uint32_t)a.kslot) {
+		a.slots[(uint64_t)s.task * a.kslot + s.nemit] = r;
+	} else {
+		const unsigned long long k = atomicAdd(a.n_spill, 1ULL);
+		if ((int64_t)k < a.spill_cap) a.spill[k] = r;
 	}
+	++s.nemit;
+}
+
+__device__ __forceinline__ uint32_t kmer_at(const Lane &s, const uint32_t *qs, int from, int len)
+{
+	uint32_t z = 0;
+	for (int l = 0; l < len; ++l) z = z << 2 | (uint32_t)qget(s, qs, from - l);
+	return z;
 }
 
 // Cheap transitions until the lane needs memory (bwt.c:381-459 minus the rank/lookup steps).
-__device__ __forceinline__ int control(const Idx &ix, const SeedArgs &a, Lane &s, uint32_t pass)
+__device__ __forceinline__ int control(const Idx &ix, const SeedArgs &a, Lane &s, const uint32_t *qs, uint32_t pass)
 {
 	for (;;) {
 		if (s.stage == 1) {
 			if (s.en - s.x < s.min_len) return ACT_DONE;
-			int xn = -1;
-			for (int i = s.x; i < s.x + s.min_len; ++i)
-				if (s.q[i] > 3) xn = i;
-			if (xn >= 0) { s.x = xn + 1; continue; }
+			if (s.gq) {                                  // only global windows can hold N
+				int xn = -1;
+				for (int i = s.x; i < s.x + s.min_len; ++i)
+					if (s.q[i] > 3) xn = i;
+				if (xn >= 0) { s.x = xn + 1; continue; }
+			}
 			s.i = s.x + s.min_len - 1;
 			if (ix.pre_len > 0 && s.min_len >= ix.pre_len) {
-				uint32_t z = 0;
-				for (int l = 0; l < ix.pre_len; ++l, --s.i) z = z << 2 | s.q[s.i];
-				s.kmer = z;
+				s.kmer = kmer_at(s, qs, s.i, ix.pre_len);
+				s.i -= ix.pre_len;
 				s.stage = 2;
 				return ACT_LOOKUP;
 			}
-			s.p = set_intv(ix, s.q[s.i]);
+			set_p(s, set_intv(ix, qget(s, qs, s.i)));
 			--s.i;
 			s.stage = 3;
 			continue;
@@ -96,7 +127,7 @@ __device__ __forceinline__ int control(const Idx &ix, const SeedArgs &a, Lane &s
 			return ACT_BACK;
 		} else if (s.stage == 4) {
 			if (s.i == s.en) { emit(a, s, pass); return ACT_DONE; }
-			if (s.q[s.i] < 4) return ACT_FWD;
+			if (qget(s, qs, s.i) < 4) return ACT_FWD;
 			emit(a, s, pass);                           // N: save, skip it, back to stage 1
 			s.x = s.i + 1;
 			s.stage = 1;
@@ -110,20 +141,46 @@ __device__ __forceinline__ int control(const Idx &ix, const SeedArgs &a, Lane &s
 	}
 }
 
-template <int PASS>
-__global__ void __launch_bounds__(256) smem_kernel(Idx ix, SeedArgs a)
+// copy the task window [st, en) into this lane's LDS slots (2-bit) unless it holds N or is long
+__device__ __forceinline__ void stage_query(Lane &s, uint32_t *qs, int st)
 {
+	const int n = s.en - st;
+	s.qbase = st;
+	s.gq = n > kQWords * 16;
+	if (s.gq) return;
+	const uint8_t *src = s.q + st;
+	bool has_n = false;
+	for (int w = 0; w < (n + 15) >> 4; ++w) {
+		uint32_t word = 0;
+		#pragma unroll
+		for (int j = 0; j < 16; ++j) {
+			const int k = (w << 4) + j;
+			const uint32_t c = k < n ? src[k] : 0;
+			has_n |= c > 3;
+			word |= (c & 3) << (j << 1);
+		}
+		qs[w * kSmemBlock + threadIdx.x] = word;
+	}
+	s.gq = has_n;
+}
+
+template <int PASS>
+__global__ void __launch_bounds__(kSmemBlock) smem_kernel(Idx ix, SeedArgs a)
+{
+	__shared__ uint32_t qs[kQWords * kSmemBlock];
 	const uint32_t pass = PASS;
 	Lane s;
 	int act = ACT_DONE;
+	bool live = false;
 	int nblk = 0, nlook = 0, next_ = 0;
 	uint32_t n_tasks = a.n_tasks;
 	if (a.n_tasks_dev) {
 		const int64_t nt = (int64_t)*a.n_tasks_dev;
 		n_tasks = (uint32_t)(nt < a.task_cap ? nt : a.task_cap);
 	}
+	WaveQueue wq;
 	for (;;) {
-		const uint32_t t = wave_fetch(a.next, act == ACT_DONE);
+		const uint32_t t = wq.fetch(a.next, act == ACT_DONE);
 		if (act == ACT_DONE) {
 			if (t >= n_tasks) {
 				if (a.stats) {
@@ -133,75 +190,139 @@ __global__ void __launch_bounds__(256) smem_kernel(Idx ix, SeedArgs a)
 				}
 				break;
 			}
+			int st;
 			if (pass == 1) {
 				s.rid = t;
 				s.q = a.seq + a.off[t];
-				s.x = 0; s.en = a.len[t];
+				st = 0; s.en = a.len[t];
 				s.min_len = a.min_len; s.min_occ = 1;
 			} else {
 				const SeedTask tk = a.tasks[t];
 				s.rid = tk.rid;
 				s.q = a.seq + a.off[tk.rid];
-				s.x = tk.st; s.en = tk.en;
+				st = tk.st; s.en = tk.en;
 				s.min_len = tk.min_len; s.min_occ = (uint64_t)(int64_t)tk.min_occ;
 			}
+			s.task = t;
+			s.nemit = 0;
+			s.x = st;
 			s.stage = 1;
+			live = true;
+			stage_query(s, qs, st);
 		}
-		act = control(ix, a, s, pass);
-		if (act == ACT_DONE) continue;
+		act = control(ix, a, s, qs, pass);
+		if (act == ACT_DONE) {
+			if (live) a.slot_cnt[s.task] = s.nemit < (uint32_t)a.kslot ? s.nemit : (uint32_t)a.kslot;
+			live = false;
+			continue;
+		}
 		if (act == ACT_LOOKUP) {                        // stages 2/5 (bwt.c:401-413)
-			s.p = ix.pre[s.kmer];
+			const Sai p = ix.pre[s.kmer];
 			++nlook;
-			if (s.p.size < s.min_occ) {
+			if (p.size < s.min_occ) {
 				s.i += ix.pre_len;
-				s.p = set_intv(ix, s.q[s.i]);
+				set_p(s, set_intv(ix, qget(s, qs, s.i)));
 				--s.i;
-			}
+			} else set_p(s, p);
 			++s.stage;
-		} else if (act == ACT_BACK) {                   // stages 3/6: se_one_step_back (bwt.c:339-354)
-			const Sai o = extend1(ix, s.p, 1, s.q[s.i], &nblk);
-			++next_;
+			continue;
+		}
+		// one extension per trip: backward for stages 3/6 (se_one_step_back, bwt.c:339-354),
+		// forward for stage 4 (bwt.c:421-453)
+		const int back = act == ACT_BACK;
+		const int qc = qget(s, qs, s.i);
+		Sai cur;
+		cur.x0 = s.px0; cur.x1 = s.px1; cur.size = s.psize; cur.info = 0;
+		const Sai o = extend1(ix, cur, back, back ? qc : 3 - qc, &nblk);
+		++next_;
+		if (back) {
 			if (o.size < s.min_occ) { s.x = s.i + 1; s.stage = 1; }
-			else { s.p = o; --s.i; }
-		} else {                                        // stage 4 forward step (bwt.c:421-453)
-			const Sai o = extend1(ix, s.p, 0, 3 - (int)s.q[s.i], &nblk);
-			++next_;
-			if (o.size >= s.min_occ) { s.p = o; ++s.i; }
-			else {
-				emit(a, s, pass);
-				if (ix.pre_len > 0 && s.i - s.x - 1 >= ix.pre_len) {
-					uint32_t z = 0;
-					for (int l = 0; l < ix.pre_len; ++l, --s.i) z = z << 2 | s.q[s.i];
-					s.kmer = z;
-					s.stage = 5;
-				} else {
-					s.p = set_intv(ix, s.q[s.i]);
-					--s.i;
-					s.stage = 6;
-				}
+			else { set_p(s, o); --s.i; }
+		} else if (o.size >= s.min_occ) {
+			set_p(s, o); ++s.i;
+		} else {
+			emit(a, s, pass);
+			if (ix.pre_len > 0 && s.i - s.x - 1 >= ix.pre_len) {
+				s.kmer = kmer_at(s, qs, s.i, ix.pre_len);
+				s.i -= ix.pre_len;
+				s.stage = 5;
+			} else {
+				set_p(s, set_intv(ix, qget(s, qs, s.i)));
+				--s.i;
+				s.stage = 6;
 			}
 		}
 	}
 }
 
-// pass-2 tasks (seed.c:80-98): every pass-1 SMEM with length >= 2*min_len and size <= max_sub_occ
-__global__ void __launch_bounds__(256) subtask_kernel(const SeedRec *out, const int64_t *n_out, int64_t cap, int32_t min_len,
-                                                     int32_t max_sub_occ, SeedTask *tasks, uint32_t *n_tasks, int64_t task_cap)
+// pass-2 tasks (seed.c:80-98): every pass-1 SMEM with length >= 2*min_len and size <= max_sub_occ.
+__device__ __forceinline__ bool subtask_ok(const SeedRec &r, int32_t min_len, int32_t max_sub_occ)
 {
-	const int64_t n = *n_out < cap ? *n_out : cap;
-	for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-		const SeedRec r = out[j];
-		const uint32_t st = (uint32_t)(r.info >> 32), en = (uint32_t)r.info;
-		if (en - st < (uint32_t)(min_len * 2) || r.size > (uint64_t)(int64_t)max_sub_occ) continue;
-		const uint32_t k = atomicAdd(n_tasks, 1u);
-		if ((int64_t)k >= task_cap) continue;
-		SeedTask t;
-		t.rid = r.rid;
-		t.st = (int32_t)st; t.en = (int32_t)en;
-		t.min_len = (int32_t)(en - st) / 2 > min_len ? (int32_t)(en - st) / 2 : min_len;
-		t.min_occ = (int32_t)r.size + 1;
-		tasks[k] = t;
+	const uint32_t st = (uint32_t)(r.info >> 32), en = (uint32_t)r.info;
+	return !(en - st < (uint32_t)(min_len * 2) || r.size > (uint64_t)(int64_t)max_sub_occ);
+}
+
+__device__ __forceinline__ SeedTask make_subtask(const SeedRec &r, int32_t min_len)
+{
+	const uint32_t st = (uint32_t)(r.info >> 32), en = (uint32_t)r.info;
+	SeedTask t;
+	t.rid = r.rid;
+	t.st = (int32_t)st; t.en = (int32_t)en;
+	t.min_len = (int32_t)(en - st) / 2 > min_len ? (int32_t)(en - st) / 2 : min_len;
+	t.min_occ = (int32_t)r.size + 1;
+	return t;
+}
+
+// one thread per read over its pass-1 slots; one atomic per wave for the task slots
+__global__ void __launch_bounds__(256) subtask_kernel(const SeedRec *slots, const uint32_t *slot_cnt, int32_t kslot, int32_t n_reads,
+                                                     const SeedRec *spill, const unsigned long long *n_spill, int64_t spill_cap,
+                                                     int32_t min_len, int32_t max_sub_occ, SeedTask *tasks, uint32_t *n_tasks, int64_t task_cap)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t c = i < n_reads ? slot_cnt[i] : 0;
+	uint32_t nq = 0;
+	for (uint32_t j = 0; j < c; ++j) nq += subtask_ok(slots[(uint64_t)i * kslot + j], min_len, max_sub_occ);
+	uint32_t wtot;
+	const uint32_t ex = wave_excl_scan(nq, &wtot);
+	uint32_t base = 0;
+	if (__lane_id() == 0 && wtot) base = atomicAdd(n_tasks, wtot);
+	base = __shfl(base, 0) + ex;
+	for (uint32_t j = 0, k = 0; j < c; ++j) {
+		const SeedRec r = slots[(uint64_t)i * kslot + j];
+		if (!subtask_ok(r, min_len, max_sub_occ)) continue;
+		if ((int64_t)(base + k) < task_cap) tasks[base + k] = make_subtask(r, min_len);
+		++k;
 	}
+	int64_t ns = (int64_t)*n_spill;
+	if (ns > spill_cap) ns = spill_cap;
+	for (int64_t k = i; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
+		const SeedRec r = spill[k];
+		if (r.pass != 1 || !subtask_ok(r, min_len, max_sub_occ)) continue;
+		const uint32_t t = atomicAdd(n_tasks, 1u);
+		if ((int64_t)t < task_cap) tasks[t] = make_subtask(r, min_len);
+	}
+}
+
+// records of the two slot arrays (cnt/off over n1 + n2 tasks, exclusive offsets) and the spill list
+// into one flat array; *n_out = total
+__global__ void __launch_bounds__(256) compact_kernel(const SeedRec *slots1, const SeedRec *slots2, int32_t k1, int32_t k2, int64_t n1,
+                                                     int64_t n_cnt, const uint32_t *cnt, const uint32_t *off, const SeedRec *spill,
+                                                     const unsigned long long *n_spill, int64_t spill_cap, SeedRec *out, int64_t cap,
+                                                     int64_t *n_out)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const uint64_t total = n_cnt > 0 ? (uint64_t)off[n_cnt - 1] + cnt[n_cnt - 1] : 0;
+	if (i < n_cnt) {
+		const uint32_t c = cnt[i], o = off[i];
+		const SeedRec *src = i < n1 ? slots1 + (uint64_t)i * k1 : slots2 + (uint64_t)(i - n1) * k2;
+		for (uint32_t j = 0; j < c; ++j)
+			if ((int64_t)(o + j) < cap) out[o + j] = src[j];
+	}
+	int64_t ns = (int64_t)*n_spill;
+	if (ns > spill_cap) ns = spill_cap;
+	for (int64_t k = i; k < ns; k += (int64_t)gridDim.x * blockDim.x)
+		if ((int64_t)(total + k) < cap) out[total + k] = spill[k];
+	if (i == 0) *n_out = (int64_t)(total + (uint64_t)ns);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -216,8 +337,9 @@ __global__ void __launch_bounds__(256) sa_kernel(Idx ix, int64_t n, const int64_
 	bool have = false;
 	uint32_t idx = 0;
 	uint64_t k = 0, steps = 0, tot = 0;
+	WaveQueue wq;
 	for (;;) {
-		const uint32_t t = wave_fetch(next, !have);
+		const uint32_t t = wq.fetch(next, !have);
 		if (!have) {
 			if ((int64_t)t >= n) break;
 			idx = t; k = x[t]; steps = 0; have = true;
@@ -271,6 +393,28 @@ __global__ void __launch_bounds__(64) getseq_kernel(L2b lb, int64_t n, const int
 	}
 }
 
+// dependent random 64-byte block fetches over the occ table (random-access ceiling)
+__global__ void gather_chain_kernel(Idx ix, int64_t n_chains, int32_t steps, unsigned long long *sum)
+{
+	const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (id >= n_chains) return;
+	const uint64_t nblk = (ix.seq_len + 127) >> 7;
+	uint64_t x = ((uint64_t)id * 0x9e3779b97f4a7c15ULL) % nblk, acc = 0;
+	for (int s = 0; s < steps; ++s) {
+		uint64_t b[8];
+		load_block(ix.data, x, b);
+		acc += b[0] ^ b[5];
+		x = ((x ^ (b[1] + b[6])) * 0xbf58476d1ce4e5b9ULL + (uint64_t)s) % nblk;
+	}
+	atomicAdd(sum, (unsigned long long)acc);
+}
+
+hipError_t launch_gather_chain(const Idx &ix, int64_t n_chains, int32_t steps, int32_t bs, unsigned long long *sum, hipStream_t st)
+{
+	hipLaunchKernelGGL(gather_chain_kernel, dim3((unsigned)((n_chains + bs - 1) / bs)), dim3(bs), 0, st, ix, n_chains, steps, sum);
+	return hipGetLastError();
+}
+
 // plain streaming copy (HBM byte-counter calibration)
 __global__ void __launch_bounds__(256) stream_copy_kernel(uint4 *dst, const uint4 *src, int64_t n16)
 {
@@ -317,10 +461,23 @@ hipError_t launch_smem(const Idx &ix, const SeedArgs &a, uint32_t pass, unsigned
 	return hipGetLastError();
 }
 
-hipError_t launch_subtask(const SeedRec *out, const int64_t *n_out, int64_t cap, int32_t min_len, int32_t max_sub_occ,
-                          SeedTask *tasks, uint32_t *n_tasks, int64_t task_cap, unsigned grid, hipStream_t st)
+hipError_t launch_subtask(const SeedRec *slots, const uint32_t *slot_cnt, int32_t kslot, int32_t n_reads, const SeedRec *spill,
+                          const unsigned long long *n_spill, int64_t spill_cap, int32_t min_len, int32_t max_sub_occ,
+                          SeedTask *tasks, uint32_t *n_tasks, int64_t task_cap, hipStream_t st)
 {
-	hipLaunchKernelGGL(subtask_kernel, dim3(grid), dim3(256), 0, st, out, n_out, cap, min_len, max_sub_occ, tasks, n_tasks, task_cap);
+	const unsigned grid = (unsigned)((n_reads + 255) / 256) + 1;
+	hipLaunchKernelGGL(subtask_kernel, dim3(grid), dim3(256), 0, st, slots, slot_cnt, kslot, n_reads, spill, n_spill, spill_cap,
+	                   min_len, max_sub_occ, tasks, n_tasks, task_cap);
+	return hipGetLastError();
+}
+
+hipError_t launch_compact(const SeedRec *slots1, const SeedRec *slots2, int32_t k1, int32_t k2, int64_t n1, int64_t n_cnt,
+                          const uint32_t *cnt, const uint32_t *off, const SeedRec *spill, const unsigned long long *n_spill,
+                          int64_t spill_cap, SeedRec *out, int64_t cap, int64_t *n_out, hipStream_t st)
+{
+	const unsigned grid = (unsigned)((n_cnt + 255) / 256) + 1;
+	hipLaunchKernelGGL(compact_kernel, dim3(grid), dim3(256), 0, st, slots1, slots2, k1, k2, n1, n_cnt, cnt, off, spill, n_spill,
+	                   spill_cap, out, cap, n_out);
 	return hipGetLastError();
 }
 

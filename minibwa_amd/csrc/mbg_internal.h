@@ -18,9 +18,14 @@ struct SeedArgs {
 	int64_t task_cap;
 	int32_t min_len;
 	uint32_t *next;         // task counter (zeroed before launch)
-	SeedRec *out;
-	int64_t *n_out;         // record counter
-	int64_t cap;
+	// output: task t's records at slots[t*kslot ...], slot_cnt[t] of them; beyond kslot they go to
+	// the spill list (one atomic each)
+	SeedRec *slots;
+	uint32_t *slot_cnt;
+	int32_t kslot;
+	SeedRec *spill;
+	unsigned long long *n_spill;
+	int64_t spill_cap;
 	unsigned long long *stats;  // optional: [0] occ blocks fetched, [1] k-mer cache lookups, [2] extend steps
 };
 
@@ -64,16 +69,24 @@ hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st);
 hipError_t launch_rank2a(const Idx &ix, int64_t n, const uint64_t *k, const uint64_t *l, uint64_t *ck, uint64_t *cl, hipStream_t st);
 hipError_t launch_extend(const Idx &ix, int64_t n, const Sai *ik, const uint8_t *is_back, Sai *ok, hipStream_t st);
 hipError_t launch_smem(const Idx &ix, const SeedArgs &a, uint32_t pass, unsigned grid, hipStream_t st);
-hipError_t launch_subtask(const SeedRec *out, const int64_t *n_out, int64_t cap, int32_t min_len, int32_t max_sub_occ,
-                          SeedTask *tasks, uint32_t *n_tasks, int64_t task_cap, unsigned grid, hipStream_t st);
+hipError_t launch_subtask(const SeedRec *slots, const uint32_t *slot_cnt, int32_t kslot, int32_t n_reads, const SeedRec *spill,
+                          const unsigned long long *n_spill, int64_t spill_cap, int32_t min_len, int32_t max_sub_occ,
+                          SeedTask *tasks, uint32_t *n_tasks, int64_t task_cap, hipStream_t st);
+hipError_t launch_compact(const SeedRec *slots1, const SeedRec *slots2, int32_t k1, int32_t k2, int64_t n1, int64_t n_cnt,
+                          const uint32_t *cnt, const uint32_t *off, const SeedRec *spill, const unsigned long long *n_spill,
+                          int64_t spill_cap, SeedRec *out, int64_t cap, int64_t *n_out, hipStream_t st);
+hipError_t scan_u32(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_t *out, int64_t n, hipStream_t st);
 hipError_t launch_sa(const Idx &ix, int64_t n, const int64_t *n_dev, uint64_t *x, uint32_t *next, unsigned long long *stats,
                      unsigned grid, hipStream_t st);
 hipError_t launch_getseq(const L2b &lb, int64_t n, const int64_t *tid, const int64_t *st_, const int64_t *en_, const int64_t *out_off,
                          uint8_t *out, int64_t *len, hipStream_t st);
 hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st);
+hipError_t launch_gather_chain(const Idx &ix, int64_t n_chains, int32_t steps, int32_t bs, unsigned long long *sum, hipStream_t st);
 hipError_t launch_stream_copy(void *dst, const void *src, int64_t n16, unsigned grid, hipStream_t st);
-hipError_t launch_anchor_rows(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, uint64_t *rows, uint32_t *row_rec,
-                              uint32_t *first_slot, unsigned long long *n_rows, int64_t row_cap, unsigned grid, hipStream_t st);
+hipError_t launch_row_counts(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, uint32_t *cnt, hipStream_t st);
+hipError_t launch_anchor_rows(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, const uint32_t *cnt,
+                              const uint32_t *off, uint64_t *rows, uint32_t *row_rec, uint32_t *first_slot,
+                              unsigned long long *n_rows, int64_t row_cap, hipStream_t st);
 hipError_t launch_best_seed(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, unsigned long long *best, unsigned grid, hipStream_t st);
 hipError_t launch_ext_jobs(const L2b &lb, const MapOpt &o, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
                            const SeedRec *recs, const uint32_t *first_slot, const uint64_t *pos, const unsigned long long *best,

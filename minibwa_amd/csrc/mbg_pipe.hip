@@ -16,30 +16,43 @@
 
 namespace mbg {
 
-// SA rows of each interval (mb_anchor's sampling); first_slot[rec] = slot of its first row
-__global__ void __launch_bounds__(256) anchor_rows_kernel(const SeedRec *recs, const int64_t *n_rec_dev, int64_t rec_cap, int32_t max_occ,
-                                                         uint64_t *rows, uint32_t *row_rec, uint32_t *first_slot,
-                                                         unsigned long long *n_rows, int64_t row_cap)
+// number of SA rows mb_anchor locates for an interval of `size` rows (seed.c:304-315)
+__device__ __forceinline__ int64_t n_anchor_rows(uint64_t size, int32_t max_occ)
+{
+	if ((int64_t)size <= max_occ) return (int64_t)size;
+	int64_t cnt = 0;
+	for (int64_t j = 0, k = 0; j < (int64_t)size && k < max_occ; ++k) {
+		int64_t step = ((int64_t)size - j) / (max_occ - k);
+		if (step < 1) step = 1;
+		j += step, ++cnt;
+	}
+	return cnt;
+}
+
+// rows per record (zero past the record count), the input of the exclusive scan
+__global__ void __launch_bounds__(256) row_counts_kernel(const SeedRec *recs, const int64_t *n_rec_dev, int64_t rec_cap, int32_t max_occ, uint32_t *cnt)
 {
 	int64_t n = *n_rec_dev;
 	if (n > rec_cap) n = rec_cap;
+	for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rec_cap; r += (int64_t)gridDim.x * blockDim.x)
+		cnt[r] = r < n ? (uint32_t)n_anchor_rows(recs[r].size, max_occ) : 0;
+}
+
+// SA rows of each interval at its scanned offset; first_slot[rec] = slot of its first row
+__global__ void __launch_bounds__(256) anchor_rows_kernel(const SeedRec *recs, const int64_t *n_rec_dev, int64_t rec_cap, int32_t max_occ,
+                                                         const uint32_t *cnt, const uint32_t *off, uint64_t *rows, uint32_t *row_rec,
+                                                         uint32_t *first_slot, unsigned long long *n_rows, int64_t row_cap)
+{
+	int64_t n = *n_rec_dev;
+	if (n > rec_cap) n = rec_cap;
+	if (blockIdx.x == 0 && threadIdx.x == 0) *n_rows = rec_cap > 0 ? (unsigned long long)off[rec_cap - 1] + cnt[rec_cap - 1] : 0;
 	for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
 		const SeedRec s = recs[r];
-		int64_t cnt;
-		if ((int64_t)s.size <= max_occ) cnt = (int64_t)s.size;
-		else {
-			cnt = 0;
-			for (int64_t j = 0, k = 0; j < (int64_t)s.size && k < max_occ; ++k) {
-				int64_t step = ((int64_t)s.size - j) / (max_occ - k);
-				if (step < 1) step = 1;
-				j += step, ++cnt;
-			}
-		}
-		const unsigned long long base = atomicAdd(n_rows, (unsigned long long)cnt);
-		first_slot[r] = cnt > 0 && (int64_t)base < row_cap ? (uint32_t)base : 0xffffffffu;
-		if ((int64_t)(base + cnt) > row_cap) continue;
+		const int64_t c = cnt[r], base = off[r];
+		first_slot[r] = c > 0 && base < row_cap ? (uint32_t)base : 0xffffffffu;
+		if (base + c > row_cap) continue;
 		if ((int64_t)s.size <= max_occ) {
-			for (int64_t j = 0; j < cnt; ++j) rows[base + j] = s.x0 + j, row_rec[base + j] = (uint32_t)r;
+			for (int64_t j = 0; j < c; ++j) rows[base + j] = s.x0 + j, row_rec[base + j] = (uint32_t)r;
 		} else {
 			int64_t o = 0;
 			for (int64_t j = 0, k = 0; j < (int64_t)s.size && k < max_occ; ++k) {
@@ -216,10 +229,18 @@ __global__ void __launch_bounds__(128) ext_jobs_kernel(L2b lb, MapOpt o, int32_t
 	}
 }
 
-hipError_t launch_anchor_rows(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, uint64_t *rows, uint32_t *row_rec,
-                              uint32_t *first_slot, unsigned long long *n_rows, int64_t row_cap, unsigned grid, hipStream_t st)
+hipError_t launch_row_counts(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, uint32_t *cnt, hipStream_t st)
 {
-	hipLaunchKernelGGL(anchor_rows_kernel, dim3(grid), dim3(256), 0, st, recs, n_rec, rec_cap, max_occ, rows, row_rec, first_slot, n_rows, row_cap);
+	hipLaunchKernelGGL(row_counts_kernel, dim3(2048), dim3(256), 0, st, recs, n_rec, rec_cap, max_occ, cnt);
+	return hipGetLastError();
+}
+
+hipError_t launch_anchor_rows(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, const uint32_t *cnt,
+                              const uint32_t *off, uint64_t *rows, uint32_t *row_rec, uint32_t *first_slot,
+                              unsigned long long *n_rows, int64_t row_cap, hipStream_t st)
+{
+	hipLaunchKernelGGL(anchor_rows_kernel, dim3(2048), dim3(256), 0, st, recs, n_rec, rec_cap, max_occ, cnt, off, rows, row_rec,
+	                   first_slot, n_rows, row_cap);
 	return hipGetLastError();
 }
 

@@ -164,6 +164,7 @@ def lib():
         L.mbg_dev_alloc.argtypes = [_P, C.c_size_t, C.POINTER(_P)]
         L.mbg_dev_free.argtypes = [_P, _P]
         L.mbg_stream_copy.argtypes = [_P, _P, _P, _P, C.c_size_t]
+        L.mbg_gather_bench.argtypes = [_P, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint64)]
         L.mbg_map_step_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.POINTER(MapOpt), C.c_int,
                                        C.POINTER(MapStats), C.POINTER(MapOut)]
         L.mbg_extd2_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, _P, _P, C.c_int32, C.c_int32,
@@ -176,7 +177,7 @@ EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mb
             "mbg_l2b_upload", "mbg_idx_info", "mbg_pre_copy", "mbg_rank2a", "mbg_extend", "mbg_seed_batch",
             "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
             "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
-            "mbg_dev_free", "mbg_stream_copy")
+            "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench")
 JOB_META_DTYPE = np.dtype([("rid", "<u4"), ("side", "i1"), ("rev", "i1"), ("pad", "<i2"), ("tid", "<i4"), ("tpos", "<i8")],
                           align=True)
 assert JOB_META_DTYPE.itemsize == 24
@@ -226,7 +227,7 @@ class DeviceArray:
 
     def free(self):
         if self.ptr and getattr(self.ctx, "h", None):
-            check(self.ctx.L.mbg_dev_free(self.ctx.h, C.c_void_p(self.ptr)), "mbg_dev_free", "mbg_stream_copy")
+            check(self.ctx.L.mbg_dev_free(self.ctx.h, C.c_void_p(self.ptr)), "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench")
         self.ptr = None
 
 
@@ -363,6 +364,13 @@ class Context:
         if a.nbytes:
             check(self.L.mbg_copy_to_device(self.h, C.c_void_p(p.value), _ptr(a), a.nbytes), "mbg_copy_to_device")
         return d
+
+    def gather_bench(self, n_chains, steps, block_threads=256):
+        """(ms, checksum) of the dependent random 64-byte gather over the resident occ table."""
+        ms = C.c_float()
+        cs = C.c_uint64()
+        check(self.L.mbg_gather_bench(self.h, n_chains, steps, block_threads, C.byref(ms), C.byref(cs)), "mbg_gather_bench")
+        return ms.value, cs.value
 
     def to_host(self, dev_addr, dtype, count):
         """Copy `count` elements of `dtype` from a device address returned by the library."""
