@@ -568,8 +568,8 @@ static int dp_dev_locked(mbg_ctx_t *c, hipStream_t st, bool dual, int32_t n_jobs
                          int8_t m, const int8_t *mat, DpRes *res, uint32_t *cigar, int cap, int64_t slab)
 {
 	if (n_jobs <= 0) return MBG_OK;
-	// resident blocks: enough to fill the GPU, bounded by 2 GiB of backtrack slabs
-	int64_t g = (int64_t)c->n_cu * 16;
+	// resident blocks (one wave each): enough to fill the GPU, bounded by 2 GiB of backtrack slabs
+	int64_t g = (int64_t)c->n_cu * 32;
 	if (g > n_jobs) g = n_jobs;
 	const int64_t max_by_mem = ((int64_t)2 << 30) / slab;
 	if (g > max_by_mem) g = max_by_mem;
@@ -832,7 +832,7 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 		                       (uint8_t*)c->p_tb.p, job_cap, (int64_t)n_reads * max_len, job_cap * max_tlen, pc, st));
 		HIPCHK(hipEventRecord(c->ev[5], st));
 		{
-			int64_t g = (int64_t)c->n_cu * 16;
+			int64_t g = (int64_t)c->n_cu * 32;
 			if (g > job_cap) g = job_cap;
 			const int64_t max_by_mem = ((int64_t)2 << 30) / slab;
 			if (g > max_by_mem) g = max_by_mem;

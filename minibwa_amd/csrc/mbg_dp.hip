@@ -23,14 +23,30 @@ namespace mbg {
 
 constexpr int32_t kNegInf = -0x40000000;
 
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v)
+// Cross-lane steps through DPP (VALU data-parallel primitives) rather than LDS permutes: the row
+// recurrence is latency-bound, and a DPP move costs a few cycles where ds_bpermute costs an LDS
+// round trip.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
 {
-	#pragma unroll
-	for (int o = 32; o > 0; o >>= 1) {
-		const uint64_t w = __shfl_xor(v, o);
-		v = w > v ? w : v;
-	}
-	return v;
+	return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+// value of lane-1 (lane 0 receives 0)
+__device__ __forceinline__ int wave_shr1(int v) { return (int)dpp_u32<0x138>((uint32_t)v); }
+
+// unsigned max over the 64 lanes, returned as a wave-uniform value (rows of 16 first, then the
+// row broadcasts fold rows 0-1 and 2-3, then lanes 31 -> 63)
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+	uint32_t m;
+	m = dpp_u32<0x111>(v); v = m > v ? m : v;   // row_shr:1
+	m = dpp_u32<0x112>(v); v = m > v ? m : v;   // row_shr:2
+	m = dpp_u32<0x114>(v); v = m > v ? m : v;   // row_shr:4
+	m = dpp_u32<0x118>(v); v = m > v ? m : v;   // row_shr:8
+	m = dpp_u32<0x142, 0xa>(v); v = m > v ? m : v;   // row_bcast:15
+	m = dpp_u32<0x143, 0xc>(v); v = m > v ? m : v;   // row_bcast:31
+	return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 __device__ __forceinline__ int8_t smax8(int8_t a, int8_t b) { return a > b ? a : b; }
@@ -38,16 +54,18 @@ __device__ __forceinline__ int8_t smin8(int8_t a, int8_t b) { return a < b ? a :
 
 struct DpState { int32_t max, zdropped, max_q, max_t, mqe, mqe_t, mte, mte_q, score, reach_end; };
 
-// ksw_apply_zdrop (ksw2.h:181-197), rotated coordinates
+// ksw_apply_zdrop (ksw2.h:181-197), rotated coordinates; written as selects so the state stays in
+// registers
 __device__ __forceinline__ bool apply_zdrop(DpState &z, int32_t H, int r, int t, int zdrop, int e)
 {
-	if (H > z.max) {
-		z.max = H & 0x7fffffff, z.max_t = t, z.max_q = r - t;
-	} else if (t >= z.max_t && r - t >= z.max_q) {
-		const int tl = t - z.max_t, ql = (r - t) - z.max_q, l = tl > ql ? tl - ql : ql - tl;
-		if (zdrop >= 0 && z.max - H > zdrop + l * e) { z.zdropped = 1; return true; }
-	}
-	return false;
+	const bool up = H > z.max;
+	const int tl = t - z.max_t, ql = (r - t) - z.max_q, l = tl > ql ? tl - ql : ql - tl;
+	const bool drop = !up && t >= z.max_t && r - t >= z.max_q && zdrop >= 0 && z.max - H > zdrop + l * e;
+	z.max = up ? (H & 0x7fffffff) : z.max;
+	z.max_t = up ? t : z.max_t;
+	z.max_q = up ? r - t : z.max_q;
+	z.zdropped = drop ? 1 : z.zdropped;
+	return drop;
 }
 
 template <bool DUAL>
@@ -176,16 +194,16 @@ __device__ void run_job(const DpArgs &a, int jid, uint8_t *lds, const int8_t *ma
 				ox = X[sl]; ov = V[sl]; ou = U[sl]; oy = Y[sl]; zs = S[sl];
 				if (DUAL) { ox2 = X2[sl]; oy2 = Y2[sl]; }
 			}
-			int8_t xt1 = (int8_t)__shfl_up((int)ox, 1), vt1 = (int8_t)__shfl_up((int)ov, 1);
-			int8_t x2t1 = DUAL ? (int8_t)__shfl_up((int)ox2, 1) : 0;
+			int8_t xt1 = (int8_t)wave_shr1((int)ox), vt1 = (int8_t)wave_shr1((int)ov);
+			int8_t x2t1 = DUAL ? (int8_t)wave_shr1((int)ox2) : 0;
 			if (lane == 0) { xt1 = cx; vt1 = cv; x2t1 = cx2; }
 			if (!DUAL && base == st && lane >= 1 && lane <= 3) {
 				// _mm_cvtsi32_si128 of a negative int8 also sets bytes 1..3 (ksw2_extz2_sse.c:138-139)
 				if (x1 < 0) xt1 = (int8_t)0xff;
 				if (v1 < 0) vt1 = (int8_t)0xff;
 			}
-			cx = (int8_t)__shfl((int)ox, 63); cv = (int8_t)__shfl((int)ov, 63);
-			if (DUAL) cx2 = (int8_t)__shfl((int)ox2, 63);
+			cx = (int8_t)__builtin_amdgcn_readlane((int)ox, 63); cv = (int8_t)__builtin_amdgcn_readlane((int)ov, 63);
+			if (DUAL) cx2 = (int8_t)__builtin_amdgcn_readlane((int)ox2, 63);
 			if (!valid) continue;
 			uint8_t d = 0;
 			if (DUAL) {
@@ -259,42 +277,45 @@ __device__ void run_job(const DpArgs &a, int jid, uint8_t *lds, const int8_t *ma
 
 		bool stop = false;
 		if (!approx_max) { // exact max over the 32-bit H row
-			int32_t max_H, max_t;
+			int32_t max_H, max_t, Hen0, Hst0;
 			if (r > 0) {
 				const int en1 = st0 + ((en0 - st0) >> 2 << 2);
-				uint64_t best = 0;
+				uint32_t bh = 0, bp = 0;         // this lane's best (H ^ sign bit, priority)
+				int32_t h_first = 0;
 				for (int t = st0 + lane; t < en0; t += 64) {
 					const int sl = t & CM;
 					const int32_t h = H[sl] + (DUAL ? (int32_t)V[sl] : (int32_t)(uint8_t)V[sl] - qe);
 					H[sl] = h;
+					if (t == st0) h_first = h;
 					const uint32_t pri = t < en1
 						? (0x80000000u | ((uint32_t)(3 - ((t - st0) & 3)) << 28) | (0x0FFFFFFFu - (uint32_t)((t - st0) >> 2)))
 						: (0x7FFFFFFFu - (uint32_t)(t - en1));
-					const uint64_t key = (uint64_t)((uint32_t)h ^ 0x80000000u) << 32 | pri;
-					best = key > best ? key : best;
+					const uint32_t hk = (uint32_t)h ^ 0x80000000u;
+					if (hk > bh || (hk == bh && pri > bp)) bh = hk, bp = pri;
 				}
+				int32_t h_en0 = 0;
 				if (lane == 0) {
 					const int sl = en0 & CM;
-					int32_t h;
-					if (DUAL) h = en0 > 0 ? Hprev + (int32_t)U[sl] : Hprev + (int32_t)V[sl];
-					else h = en0 > 0 ? Hprev + (int32_t)(uint8_t)U[sl] - qe : Hprev + (int32_t)(uint8_t)V[sl] - qe;
-					H[sl] = h;
-					const uint64_t key = (uint64_t)((uint32_t)h ^ 0x80000000u) << 32 | 0xFFFFFFFFu;
-					best = key > best ? key : best;
+					if (DUAL) h_en0 = en0 > 0 ? Hprev + (int32_t)U[sl] : Hprev + (int32_t)V[sl];
+					else h_en0 = en0 > 0 ? Hprev + (int32_t)(uint8_t)U[sl] - qe : Hprev + (int32_t)(uint8_t)V[sl] - qe;
+					H[sl] = h_en0;
+					const uint32_t hk = (uint32_t)h_en0 ^ 0x80000000u;
+					if (hk > bh || (hk == bh && 0xFFFFFFFFu > bp)) bh = hk, bp = 0xFFFFFFFFu;
 				}
-				best = wave_max_u64(best);
-				max_H = (int32_t)((uint32_t)(best >> 32) ^ 0x80000000u);
-				const uint32_t pri = (uint32_t)best;
+				const uint32_t mh = wave_max_u32(bh);
+				const uint32_t pri = wave_max_u32(bh == mh ? bp : 0u);
+				max_H = (int32_t)(mh ^ 0x80000000u);
 				if (pri == 0xFFFFFFFFu) max_t = en0;
 				else if (pri & 0x80000000u) max_t = st0 + (int)(0x0FFFFFFFu - (pri & 0x0FFFFFFFu)) * 4 + (3 - (int)((pri >> 28) & 3));
 				else max_t = en1 + (int)(0x7FFFFFFFu - pri);
+				Hen0 = __builtin_amdgcn_readlane(h_en0, 0);
+				Hst0 = en0 > st0 ? __builtin_amdgcn_readlane(h_first, 0) : Hen0;
 			} else {
 				max_H = DUAL ? (int32_t)V[0] - qe0 : (int32_t)(uint8_t)V[0] - qe - qe;
 				max_t = 0;
 				if (lane == 0) H[0] = max_H;
+				Hen0 = Hst0 = max_H;
 			}
-			__syncthreads();
-			const int32_t Hen0 = H[en0 & CM], Hst0 = H[st0 & CM];
 			if (en0 == tlen - 1 && Hen0 > z.mte) z.mte = Hen0, z.mte_q = r - en0;
 			if (r - st0 == qlen - 1 && Hst0 > z.mqe) z.mqe = Hst0, z.mqe_t = st0;
 			if (apply_zdrop(z, max_H, r, max_t, zdrop, DUAL ? e2 : e)) stop = true;
