@@ -1,0 +1,232 @@
+"""Pins of the oracle (oracle/mb_oracle.c) that do not depend on the reference binary.
+
+* the reference's own known answers: ``minibwa bench -b 2a|sa|msa`` checksums on the chrM index
+  (main.c:164-200), recorded in tests/golden/golden.json;
+* brute force on tiny indexed texts: Occ counts over the explicit BWT, the suffix array, the
+  bidirectional intervals of ``mb_bwt_extend`` and the two-pass SMEM seeding, all from a sorted
+  suffix list (tests/brute.py);
+* a textbook dual-affine (Gotoh) DP for the global score, the exact maximum, mqe/mte and the
+  score of the returned CIGAR.
+
+The oracle is test infrastructure: these tests check it, so that the GPU parity tests (which
+compare the HIP library against it) rest on something other than a retyped formula.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests import brute
+from tests.conftest import GOLDEN
+
+O = pytest.importorskip("oracle.oracle")
+U64MAX = (1 << 64) - 1
+
+
+@pytest.fixture(scope="module")
+def tiny(tmp_path_factory):
+    """Three tiny indexed texts: uniform, repeat-rich, and an all-repeat one (many multi-hit
+    intervals and pass-2 seeds)."""
+    from minibwa_amd import index, synth
+    out = []
+    d = tmp_path_factory.mktemp("tiny")
+    specs = [(120, 3, 0.0), (300, 5, 0.5), (200, 7, 0.9)]
+    for L, seed, rep in specs:
+        ref = synth.reference(L, seed=seed, rep_frac=rep, rep_min=20, rep_max=60, rep_div=0.03)
+        pre = str(d / f"t{L}_{seed}")
+        index.build(ref, pre, device="cpu")
+        out.append((ref, brute.TinyText(ref), O.OracleIndex(pre + ".mbw", pre_len=10)))
+    yield out
+    for _, _, ix in out:
+        ix.close()
+
+
+def test_known_answer_checksums(chrm_prefix):
+    """minibwa bench checksums of the reference binary on its own chrM data (main.c:164-200)."""
+    gold = json.load(open(os.path.join(GOLDEN, "golden.json")))
+    ix = O.OracleIndex(chrm_prefix + ".mbw", pre_len=0)
+    try:
+        for kind in ("2a", "sa", "msa"):
+            assert ix.bench_checksum(kind) == gold[f"bench.{kind}.chrM"], kind
+    finally:
+        ix.close()
+
+
+def test_index_header_matches_text(tiny):
+    for ref, tt, ix in tiny:
+        assert ix.seq_len == tt.n
+        assert ix.primary == tt.primary
+        cnt = [tt.B.count(c) for c in brute.ALPHA]
+        assert ix.L2 == [0, cnt[0], cnt[0] + cnt[1], cnt[0] + cnt[1] + cnt[2], sum(cnt)]
+
+
+def test_rank2a_bruteforce(tiny):
+    """Occ(c,k), Occ(c,l) for every 0 <= k <= l <= n+1.  The one deviation from the plain count is
+    the reference's l-k==1 shortcut at k == primary (bwt.c:205-209): it adds the $-removed BWT
+    symbol at index k although row k holds '$'."""
+    for ref, tt, ix in tiny:
+        n = tt.n
+        occ = [tt.occ(k) for k in range(n + 2)]
+        for k in range(n + 2):
+            for l in range(k, n + 2):
+                ck, cl = ix.rank2a(k, l)
+                exp_l = list(occ[l])
+                if k == tt.primary and l == k + 1 and k > 0:
+                    exp_l = list(occ[k])
+                    exp_l[int(tt.B[k])] += 1
+                assert list(ck) == occ[k], (k, l)
+                assert list(cl) == exp_l, (k, l)
+
+
+def test_sa_bruteforce(tiny):
+    """Every row's text position; row 0 (the '$' suffix) reads sa[0] = 2^64-1 (bwt.c:496)."""
+    for ref, tt, ix in tiny:
+        rows = np.arange(tt.n + 1, dtype=np.uint64)
+        got = ix.sa(rows)
+        assert int(got[0]) == U64MAX
+        assert [int(v) for v in got[1:]] == tt.sa[1:]
+
+
+def test_extend_bruteforce(tiny):
+    """Backward and forward one-base extension of intervals of text substrings and of absent
+    patterns (bwt.c:237-250): each child equals the brute-force interval of the extended pattern,
+    except where the parent is the single row `primary` (see brute.quirk_prefix): then the child for
+    the $-removed BWT symbol at `primary` gains one phantom occurrence, and the children sorted
+    before it in the other strand's order shift by one."""
+    rng = np.random.default_rng(5)
+    n_quirk = 0
+    for ref, tt, ix in tiny:
+        pats = [tt.T[:k] for k in range(1, 6)] + [brute.revcomp(tt.T[:k]) for k in range(1, 6)]
+        for _ in range(150):
+            ln = int(rng.integers(1, 9))
+            if rng.random() < 0.8:
+                st = int(rng.integers(0, tt.n - ln + 1))
+                pats.append(tt.T[st:st + ln])
+            else:
+                pats.append("".join(brute.ALPHA[int(c)] for c in rng.integers(0, 4, ln)))
+        for p in pats:
+            x0, x1, size = tt.interval(p)
+            back = ix.extend((x0, x1, size, 0), 1)
+            fwd = ix.extend((x0, x1, size, 0), 0)
+            cb = int(tt.B[tt.primary]) if tt.primary < len(tt.B) else -1
+            qb, qf = brute.quirk_prefix(tt, p), brute.quirk_prefix(tt, brute.revcomp(p))
+            n_quirk += qb + qf
+            for c in range(4):
+                e0, e1, es = tt.interval(brute.ALPHA[c] + p)
+                if qb and tt.primary > 0:
+                    es += c == cb
+                    e1 += c < cb
+                assert back[c] == (e0, e1, es), (p, c)
+                e0, e1, es = tt.interval(p + brute.ALPHA[3 - c])   # forward child c appends 3-c
+                if qf and tt.primary > 0:
+                    es += c == cb
+                    e0 += c < cb
+                assert fwd[c] == (e0, e1, es), (p, c)
+    assert n_quirk > 0
+
+
+def _reads_for(ref, rng, n):
+    reads = []
+    L = len(ref)
+    for k in range(n):
+        ln = int(rng.integers(20, 100))
+        st = int(rng.integers(0, max(1, L - ln)))
+        r = np.array(ref[st:st + ln], np.uint8)
+        if rng.random() < 0.5:
+            r = (3 - r[::-1]).astype(np.uint8)
+        for _ in range(int(rng.integers(0, 4))):
+            r[int(rng.integers(0, len(r)))] = int(rng.integers(0, 4))
+        if k % 7 == 3:
+            r[int(rng.integers(0, len(r)))] = 4                     # an ambiguous base
+        if k % 11 == 5:
+            r = rng.integers(0, 4, len(r)).astype(np.uint8)         # unrelated read
+        reads.append(r)
+    return reads
+
+
+@pytest.mark.parametrize("min_len,max_sub_occ", [(19, 10), (11, 10), (8, 4), (12, 50)])
+def test_seed_intv_bruteforce(tiny, min_len, max_sub_occ):
+    """Two-pass seeding (seed.c:24-54): the oracle's interval list (with the 10-mer cache when the
+    window allows it) equals the list the serial algorithm yields with brute-force counts."""
+    rng = np.random.default_rng(100 + min_len)
+    n_multi = n_cmp = n_skip = 0
+    for ref, tt, ix in tiny:
+        reads = _reads_for(ref, rng, 40)
+        L = len(ref)
+        reads.append(np.array(ref[: min(60, L)], np.uint8))                 # starts at text position 0
+        reads.append((3 - np.array(ref[: min(60, L)], np.uint8)[::-1]).astype(np.uint8))
+        for r in reads:
+            flags = {}
+            exp = brute.seed_intv(tt, brute.seq_str(r), min_len, max_sub_occ, flags)
+            if flags.get("quirk"):
+                n_skip += 1      # touches the phantom occurrence; covered against the reference binary
+                continue
+            got = ix.seed(r, min_len, max_sub_occ)
+            got_t = [(int(a), int(b), int(s), int(i) >> 32, int(i) & 0xffffffff) for a, b, s, i in got]
+            assert got_t == exp
+            n_multi += sum(1 for h in exp if h[2] > 1)
+            n_cmp += 1
+    assert n_multi > 0 and n_cmp > 10 * max(1, n_skip)
+
+
+def _dp_case(rng, qmax=36):
+    ql = int(rng.integers(1, qmax))
+    qs = rng.integers(0, 4, ql).astype(np.uint8)
+    t = list(qs)
+    for _ in range(int(rng.integers(0, 6))):
+        kind = rng.integers(0, 3)
+        p = int(rng.integers(0, len(t) + 1))
+        if kind == 0 and t:
+            t[min(p, len(t) - 1)] = int(rng.integers(0, 4))
+        elif kind == 1:
+            t[p:p] = list(rng.integers(0, 4, int(rng.integers(1, 6))))
+        elif len(t) > 1:
+            del t[p:p + int(rng.integers(1, 5))]
+    if rng.random() < 0.2 and t:
+        t[int(rng.integers(0, len(t)))] = 4
+    if rng.random() < 0.2:
+        qs[int(rng.integers(0, ql))] = 4
+    if not t:
+        t = [0]
+    return qs, np.array(t, np.uint8)
+
+
+@pytest.mark.parametrize("dual", [True, False])
+def test_dp_textbook(dual):
+    """Global mode (flag 0), z-drop off, band wider than both sequences: score, exact max,
+    mqe/mte equal the textbook DP, and the CIGAR spans both sequences with exactly that score."""
+    rng = np.random.default_rng(7 if dual else 8)
+    params = [(2, 8, 12, 2, 23, 1), (1, 4, 6, 1, 14, 0), (2, 4, 4, 2, 13, 1), (1, 2, 2, 1, 5, 0)]
+    for it in range(160):
+        a, b, q, e, q2, e2 = params[it % len(params)]
+        if not dual:
+            q2, e2 = q, e
+        mat = O.gen_nt4_mat(a, b)
+        qs, ts = _dp_case(rng)
+        w = max(len(qs), len(ts)) + 1
+        if dual:
+            r = O.extd2(qs, ts, mat, q, e, q2, e2, w, -1, 0, 0)
+        else:
+            r = O.extz2(qs, ts, mat, q, e, w, -1, 0, 0)
+        H = brute.gotoh(qs, ts, mat, q, e, q2, e2)
+        ql, tl = len(qs), len(ts)
+        assert r["cells"] == ql * tl, it
+        assert r["score"] == H[ql - 1][tl - 1], (it, qs, ts)
+        assert r["max"] == max(0, max(max(row) for row in H)), it
+        assert r["mqe"] == max(H[ql - 1]), it
+        assert r["mte"] == max(H[i][tl - 1] for i in range(ql)), it
+        sc, ci, cj = brute.cigar_score(r["cigar"], qs, ts, mat, q, e, q2, e2)
+        assert (ci, cj) == (ql, tl), it
+        assert sc == r["score"], (it, list(r["cigar"]))
+
+
+def test_dp_cells_band_count():
+    """The cells the oracle reports are the cells (i, j) of the query x target rectangle with
+    |i - j| <= w: the anti-diagonal band of ksw2_extd2_sse.c:130-139 in matrix coordinates.  (With
+    w = 0 anti-diagonal 1 holds no cell and the loop stops there, after cell (0, 0).)"""
+    L = O.orc()
+    assert L.orc_dp_cells(40, 41, 0) == 1
+    for ql, tl, w in [(1, 1, 1), (10, 30, 5), (150, 311, 161), (300, 400, 101), (57, 20, 3), (40, 41, 1)]:
+        i, j = np.meshgrid(np.arange(ql), np.arange(tl), indexing="ij")
+        assert L.orc_dp_cells(ql, tl, w) == int(np.count_nonzero(np.abs(i - j) <= w)), (ql, tl, w)

@@ -1,0 +1,131 @@
+"""The oracle against the reference compiled from its own sources (oracle/_ref, built by
+``make -C oracle ref``).  Skipped where oracle/_ref is absent.
+
+These cover what only the reference binary can pin: CIGAR tie-breaking, approximate max, z-drop,
+EXTZ_ONLY early stop, exact-max argmax ties, mqe/mte under early stop (ksw2_extd2_sse.c,
+ksw2_extz2_sse.c), the rank2a quirks on a real index, the 10-mer-cache revert path and the batched
+seeding order-free equality (seed.c:56-103).
+"""
+import numpy as np
+import pytest
+
+O = pytest.importorskip("oracle.oracle")
+
+pytestmark = pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built (make -C oracle ref)")
+
+FLAGS = [0x00, 0x40, 0xc0, 0xc2, 0x42, 0x08, 0x48, 0x18, 0x58, 0x01, 0x41, 0x04, 0x44, 0x80, 0x02, 0x10]
+
+
+def _pair(rng, qmax=300):
+    ql = int(rng.integers(1, qmax))
+    qs = rng.integers(0, 4, ql).astype(np.uint8)
+    t = list(qs)
+    for _ in range(int(rng.integers(0, max(2, ql // 15)))):
+        p = int(rng.integers(0, len(t) + 1))
+        k = rng.integers(0, 4)
+        if k <= 1 and t:
+            t[min(p, len(t) - 1)] = int(rng.integers(0, 5 if rng.random() < 0.05 else 4))
+        elif k == 2:
+            t[p:p] = list(rng.integers(0, 4, int(rng.integers(1, 40 if rng.random() < 0.1 else 6))))
+        else:
+            del t[p:p + int(rng.integers(1, 40 if rng.random() < 0.1 else 6))]
+    fl = int(rng.integers(0, 80))
+    t = t + list(rng.integers(0, 4, fl))
+    if rng.random() < 0.1:
+        t = list(rng.integers(0, 4, int(rng.integers(1, qmax))))      # unrelated: z-drop territory
+    if not t:
+        t = [1]
+    return qs, np.array(t, np.uint8)
+
+
+@pytest.mark.parametrize("dual", [True, False])
+def test_dp_fuzz_vs_reference(dual):
+    rng = np.random.default_rng(31 if dual else 32)
+    mats = [O.gen_nt4_mat(2, 8), O.gen_nt4_mat(1, 4), O.gen_nt4_mat(2, 4, 1, 1)]
+    gaps = [(12, 2, 23, 1), (4, 2, 24, 1), (6, 1, 6, 1), (12, 2, 24, 1), (8, 3, 30, 0)]
+    n_zd = 0
+    for it in range(500):
+        qs, ts = _pair(rng)
+        mat = mats[it % len(mats)]
+        q, e, q2, e2 = gaps[it % len(gaps)]
+        flag = FLAGS[it % len(FLAGS)]
+        w = int(rng.choice([1, 5, 16, 33, 101, 161, 400]))
+        zdrop = int(rng.choice([-1, 20, 100, 172]))
+        eb = int(rng.choice([0, 10]))
+        if dual:
+            a = O.extd2(qs, ts, mat, q, e, q2, e2, w, zdrop, eb, flag)
+            b = O.ref_extd2(qs, ts, mat, q, e, q2, e2, w, zdrop, eb, flag)
+        else:
+            a = O.extz2(qs, ts, mat, q, e, w, zdrop, eb, flag)
+            b = O.ref_extz2(qs, ts, mat, q, e, w, zdrop, eb, flag)
+        for f in O.EZ_FIELDS:
+            assert a[f] == b[f], (it, f, flag, w, zdrop, len(qs), len(ts))
+        assert a["cigar"] == b["cigar"], (it, flag)
+        n_zd += a["zdropped"]
+    assert n_zd > 0
+
+
+@pytest.fixture(scope="module")
+def chrm_pair(chrm_prefix):
+    a = O.OracleIndex(chrm_prefix + ".mbw", pre_len=10)
+    b = O.RefIndex(chrm_prefix + ".mbw", pre_len=10)
+    yield a, b
+    a.close()
+    b.close()
+
+
+def test_rank2a_vs_reference(chrm_pair):
+    """Random and adversarial (k, l): same 128-row block, l-k==1, around primary, block edges,
+    and k > l inside one block (a path `minibwa bench -b 2a` takes)."""
+    a, b = chrm_pair
+    n = int(a.seq_len)
+    p = int(a.primary)
+    rng = np.random.default_rng(3)
+    ks = list(rng.integers(0, n + 2, 4000))
+    pairs = [(int(k), int(min(n + 1, k + rng.integers(0, 300)))) for k in ks]
+    pairs += [(p - 1, p), (p, p + 1), (p + 1, p + 2), (p, p), (p - 1, p + 1), (0, 0), (0, 1), (n, n + 1),
+              (n + 1, n + 1)]
+    pairs += [(k, k + 1) for k in range(p - 130, p + 130)]
+    pairs += [(128 * j + d, 128 * j + d2) for j in range(1, 20) for d in (-1, 0, 1, 63, 64, 65) for d2 in (0, 1, 64)]
+    pairs += [(int(k), int(k) - int(rng.integers(1, 60))) for k in rng.integers(200, n, 500)]
+    for k, l in pairs:
+        if k < 0 or l < 0 or k > n + 1 or l > n + 1:
+            continue
+        ck, cl = a.rank2a(k, l)
+        rk, rl = b.rank2a(k, l)
+        assert list(ck) == list(rk) and list(cl) == list(rl), (k, l)
+
+
+def test_sa_all_rows_vs_reference(chrm_pair):
+    a, b = chrm_pair
+    rows = np.arange(int(a.seq_len) + 1, dtype=np.uint64)
+    assert np.array_equal(a.sa(rows), b.sa(rows))
+
+
+@pytest.mark.parametrize("min_len,max_sub_occ", [(19, 10), (15, 30), (25, 3), (11, 10)])
+def test_seeding_vs_reference(chrm_pair, chrm_reads, min_len, max_sub_occ):
+    """Every chrM read: the oracle's serial two-pass seeding equals the reference's batched state
+    machine as a multiset (order is free, seed.c:104)."""
+    a, b = chrm_pair
+    reads = chrm_reads[:1200]
+    ref_out = b.seed_batch(reads, min_len, max_sub_occ)
+    n_int = 0
+    for r, rv in zip(reads, ref_out):
+        ov = a.seed(r, min_len, max_sub_occ)
+        assert sorted(map(tuple, ov.tolist())) == sorted(map(tuple, rv.tolist()))
+        n_int += len(ov)
+    assert n_int > len(reads)
+
+
+def test_extend_vs_reference(chrm_pair, chrm_reads):
+    """Both directions of mb_bwt_extend from every seed interval of 300 reads and from the single
+    rows around primary."""
+    a, b = chrm_pair
+    ivs = []
+    for r in chrm_reads[:300]:
+        ivs += [tuple(int(v) for v in x) for x in a.seed(r)]
+    p = int(a.primary)
+    ivs += [(k, k, 1, 0) for k in range(p - 3, p + 4)] + [(p, p, 2, 0), (0, 0, 1, 0)]
+    for iv in ivs:
+        for d in (0, 1):
+            assert a.extend(iv, d) == b.extend(iv, d), (iv, d)
