@@ -9,7 +9,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN
+from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
