@@ -1,0 +1,155 @@
+/*
+ * mbg_wrap.c -- binds the reference's four hot-path hooks to libmbg without touching its sources.
+ *
+ * Linked into the unmodified minibwa objects with
+ *   -Wl,--wrap=mb_seed_intv_batch -Wl,--wrap=mb_bwt_sa_batch -Wl,--wrap=ksw_extd2_sse -Wl,--wrap=ksw_extz2_sse
+ * every call site of those functions (map-main.c:75, map-algo.c:733, seed.c:180, align.c:374-376,
+ * pe.c:287) reaches the __wrap_ definitions below, which run the work on the GPU through the C ABI
+ * of include/mbg.h and hand results back in the reference's own containers:
+ *
+ *   mb_seed_intv_batch (mbpriv.h:57, seed.c:56-103)   -> mbg_seed_batch
+ *   mb_bwt_sa_batch    (bwt.h:73, bwt.c:511-541)       -> mbg_sa_batch
+ *   ksw_extd2_sse      (ksw2.h:79-80)                  -> mbg_extd2_batch (one job)
+ *   ksw_extz2_sse      (ksw2.h:73-74)                  -> mbg_extz2_batch (one job)
+ *
+ * The index is uploaded once, on the first hooked call, from the mb_bwt_t the caller passes (the
+ * k-mer cache is recomputed on the device with the caller's pre_len).  MBG_DEVICE selects the HIP
+ * device (default 0).  Any library error aborts, as the reference does on internal failures.
+ *
+ * Containers follow the reference's allocation rules: seed intervals are appended with kalloc's
+ * krealloc on the caller's km (Kgrow, kalloc.h:34-41), CIGAR storage is grown with the allocator
+ * ksw2.h selects for this build (ksw2.h:113-121) and ksw_extz_t is filled field for field
+ * (ksw2.h:36-45).
+ */
+#include <pthread.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "kalloc.h"
+#include "bwt.h"
+#include "mbpriv.h"
+#include "ksw2.h"
+#include "mbg.h"
+
+static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
+static mbg_ctx_t *g_ctx;
+static const mb_bwt_t *g_bwt;
+
+static void die(const char *what, int rc)
+{
+	fprintf(stderr, "[mbg_wrap] %s failed: %s (%s)\n", what, mbg_strerror(rc), mbg_last_hip_error());
+	abort();
+}
+
+static mbg_ctx_t *ctx_for(const mb_bwt_t *bwt)
+{
+	pthread_mutex_lock(&g_lock);
+	if (g_ctx == 0) {
+		const char *d = getenv("MBG_DEVICE");
+		int rc = mbg_init(d ? atoi(d) : 0, &g_ctx);
+		if (rc != MBG_OK) die("mbg_init", rc);
+	}
+	if (bwt && bwt != g_bwt) {
+		mbg_bwt_desc_t dsc;
+		int rc;
+		dsc.primary = bwt->primary;
+		memcpy(dsc.L2, bwt->L2, sizeof(dsc.L2));
+		dsc.seq_len = bwt->seq_len;
+		dsc.data = bwt->data;
+		dsc.sa_bit = bwt->sa_bit;
+		dsc.n_sa = bwt->n_sa;
+		dsc.sa = bwt->sa;
+		rc = mbg_bwt_upload(g_ctx, &dsc, bwt->pre ? (int32_t)bwt->pre_len : 0);
+		if (rc != MBG_OK) die("mbg_bwt_upload", rc);
+		g_bwt = bwt;
+	}
+	pthread_mutex_unlock(&g_lock);
+	return g_ctx;
+}
+
+void __wrap_mb_seed_intv_batch(void *km, const mb_bwt_t *bwt, int32_t n_seq, const int32_t *len, uint8_t *const *seq,
+                               int32_t min_len, int32_t max_sub_occ, mb_sai_v *v)
+{
+	mbg_ctx_t *c = ctx_for(bwt);
+	mbg_sai_t *out = 0;
+	int64_t *off;
+	int32_t i;
+	int rc;
+	if (n_seq <= 0) return;
+	off = (int64_t*)malloc(sizeof(int64_t) * (n_seq + 1));
+	rc = mbg_seed_batch(c, n_seq, len, (const uint8_t *const*)seq, min_len, max_sub_occ, &out, off);
+	if (rc != MBG_OK) die("mbg_seed_batch", rc);
+	for (i = 0; i < n_seq; ++i) {
+		int64_t k, n = off[i + 1] - off[i];
+		v[i].n = 0;
+		if ((size_t)n > v[i].m) {
+			v[i].m = (size_t)n + ((size_t)n >> 1) + 16;
+			v[i].a = (mb_sai_t*)(krealloc)(km, v[i].a, v[i].m * sizeof(mb_sai_t));
+		}
+		for (k = 0; k < n; ++k) {
+			const mbg_sai_t *s = &out[off[i] + k];
+			mb_sai_t *d = &v[i].a[v[i].n++];
+			d->x[0] = s->x[0], d->x[1] = s->x[1], d->size = s->size, d->info = s->info;
+		}
+	}
+	free(out);
+	free(off);
+}
+
+void __wrap_mb_bwt_sa_batch(void *km, const mb_bwt_t *bwt, int64_t n, uint64_t *x)
+{
+	int rc;
+	(void)km;
+	if (n <= 0) return;
+	rc = mbg_sa_batch(ctx_for(bwt), n, x);
+	if (rc != MBG_OK) die("mbg_sa_batch", rc);
+}
+
+static void dp_one(int dual, void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m,
+                   const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2, int w, int zdrop, int end_bonus,
+                   int flag, ksw_extz_t *ez)
+{
+	mbg_ctx_t *c = ctx_for(0);
+	mbg_dp_job_t j;
+	mbg_dp_result_t r;
+	uint32_t *cig;
+	int64_t cap;
+	int rc;
+	memset(&j, 0, sizeof(j));
+	j.qlen = qlen > 0 ? qlen : 0;      /* the reference returns the reset state for qlen/tlen <= 0 */
+	j.tlen = tlen > 0 ? tlen : 0;
+	j.q = q, j.e = e, j.q2 = q2, j.e2 = e2;
+	j.w = w, j.zdrop = zdrop, j.end_bonus = end_bonus, j.flag = flag;
+	cap = (int64_t)j.qlen + j.tlen + 1;
+	cig = (uint32_t*)malloc(cap * sizeof(uint32_t));
+	rc = dual ? mbg_extd2_batch(c, 1, &j, query, j.qlen, target, j.tlen, m, mat, &r, cig, cap)
+	          : mbg_extz2_batch(c, 1, &j, query, j.qlen, target, j.tlen, m, mat, &r, cig, cap);
+	if (rc != MBG_OK) die(dual ? "mbg_extd2_batch" : "mbg_extz2_batch", rc);
+	ez->max = (uint32_t)r.max & 0x7fffffff, ez->zdropped = r.zdropped ? 1 : 0;
+	ez->max_q = r.max_q, ez->max_t = r.max_t;
+	ez->mqe = r.mqe, ez->mqe_t = r.mqe_t;
+	ez->mte = r.mte, ez->mte_q = r.mte_q;
+	ez->score = r.score, ez->reach_end = r.reach_end;
+	ez->n_cigar = r.n_cigar;
+	if (r.n_cigar > ez->m_cigar) {
+		ez->m_cigar = r.n_cigar;
+		ez->cigar = (uint32_t*)krealloc(km, ez->cigar, (size_t)ez->m_cigar * 4);
+	}
+	if (r.n_cigar > 0) memcpy(ez->cigar, cig, (size_t)r.n_cigar * 4);
+	free(cig);
+}
+
+void __wrap_ksw_extd2_sse(void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m,
+                          const int8_t *mat, int8_t q, int8_t e, int8_t q2, int8_t e2, int w, int zdrop, int end_bonus,
+                          int flag, ksw_extz_t *ez)
+{
+	dp_one(1, km, qlen, query, tlen, target, m, mat, q, e, q2, e2, w, zdrop, end_bonus, flag, ez);
+}
+
+void __wrap_ksw_extz2_sse(void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m,
+                          const int8_t *mat, int8_t q, int8_t e, int w, int zdrop, int end_bonus, int flag,
+                          ksw_extz_t *ez)
+{
+	dp_one(0, km, qlen, query, tlen, target, m, mat, q, e, q, e, w, zdrop, end_bonus, flag, ez);
+}

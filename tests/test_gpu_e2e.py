@@ -1,0 +1,60 @@
+"""Drop-in parity end to end: the reference CLI with its four hot-path hooks bound to libmbg
+(oracle/_ref/minibwa-mbg, linked by ``make -C oracle hooked`` from the reference's unmodified
+objects plus minibwa_amd/integration/mbg_wrap.c) must write the same SAM body as the reference
+binary.  The expected md5s were recorded from the reference binary (tests/golden/make_golden.py).
+
+Seeding (mb_seed_intv_batch), SA locate (mb_bwt_sa_batch) and every banded DP call
+(ksw_extd2_sse / ksw_extz2_sse) run on the GPU; everything else is the reference's own code.
+"""
+import hashlib
+import json
+import os
+import subprocess
+
+import pytest
+
+from tests.conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+
+HOOKED = os.path.join(ROOT, "oracle", "_ref", "minibwa-mbg")
+
+
+def _sam_body_md5(text):
+    body = "".join(l + "\n" for l in text.split("\n") if l and not l.startswith("@PG"))
+    return hashlib.md5(body.encode()).hexdigest()
+
+
+def _run(args, timeout=300):
+    print("[e2e]", " ".join(args), flush=True)
+    r = subprocess.run([HOOKED] + args, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r.stdout
+
+
+@pytest.fixture(scope="module")
+def gold():
+    if not os.path.exists(HOOKED):
+        pytest.skip("oracle/_ref/minibwa-mbg not built (make -C oracle hooked)")
+    return json.load(open(os.path.join(GOLDEN, "golden.json")))
+
+
+def test_chrm_pairs_sam_identical(gold):
+    """The reference's own test data: 1,000 chrM read pairs (test/chrM-read_{1,2}.fa.gz)."""
+    sam = _run(["map", os.path.join(GOLDEN, "chrM"), os.path.join(GOLDEN, "chrM-read_1.fa.gz"),
+                os.path.join(GOLDEN, "chrM-read_2.fa.gz")])
+    assert _sam_body_md5(sam) == gold["chrM.sam.body.md5"]
+
+
+def test_cfg0_pairs_sam_identical(gold, tmp_path):
+    """10,000 seeded 2x150 pairs on the 16,569 bp seeded reference."""
+    import numpy as np
+    from minibwa_amd import synth
+    ref = synth.reference(16569, seed=11, rep_frac=0.0)
+    r1, r2 = synth.read_pairs(ref, 10000, seed=11)
+    f1, f2 = str(tmp_path / "r1.fq"), str(tmp_path / "r2.fq")
+    synth.write_pairs_fastq(f1, f2, r1, r2, prefix="c0_")
+    sam = _run(["map", os.path.join(GOLDEN, "cfg0"), f1, f2])
+    n = sum(1 for l in sam.split("\n") if l and not l.startswith("@"))
+    assert n == gold["cfg0.sam.n_lines"]
+    assert _sam_body_md5(sam) == gold["cfg0.sam.body.md5"]
