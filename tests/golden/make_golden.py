@@ -8,7 +8,9 @@ reference's own sources by oracle/Makefile):
   seed 11) and its ``minibwa index``
 * ``golden.json`` -- known answers: ``minibwa bench -b 2a|sa|msa`` checksums on chrM, md5 of the
   index files, and md5 of the SAM body (minus @PG) of ``minibwa map`` on the chrM read pairs and on
-  the config-0 read pairs (10,000 pairs from minibwa_amd.synth with seed 11).
+  the config-0 read pairs (10,000 pairs from minibwa_amd.synth with seed 11), and md5 of the
+  output of the reference's batch-API example (api-test/ex-batch.c, mb_map_batch) on the chrM
+  read-1 file and on the config-0 read-1 file.
 
 Usage: python tests/golden/make_golden.py
 """
@@ -23,6 +25,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 REF_CLI = os.path.join(ROOT, "oracle", "_ref", "minibwa")
+REF_BATCH = os.path.join(ROOT, "oracle", "_ref", "mbmap-batch")   # api-test/ex-batch.c
 
 
 def md5(path):
@@ -51,6 +54,8 @@ def main():
         out[f"bench.{b}.chrM"] = cs
     sam = run([REF_CLI, "map", pre, os.path.join(HERE, "chrM-read_1.fa.gz"), os.path.join(HERE, "chrM-read_2.fa.gz")]).stdout
     out["chrM.sam.body.md5"] = sam_body_md5(sam)
+    r = run([REF_BATCH, pre, os.path.join(HERE, "chrM-read_1.fa.gz")]).stdout
+    out["chrM.batch.md5"] = hashlib.md5(r.encode()).hexdigest()
 
     cfg0 = os.path.join(HERE, "cfg0")
     ref = synth.reference(16569, seed=11, rep_frac=0.0)
@@ -64,6 +69,8 @@ def main():
         sam = run([REF_CLI, "map", cfg0, f1, f2]).stdout
         out["cfg0.sam.body.md5"] = sam_body_md5(sam)
         out["cfg0.sam.n_lines"] = sum(1 for l in sam.split("\n") if l and not l.startswith("@"))
+        r = run([REF_BATCH, cfg0, f1]).stdout
+        out["cfg0.batch.md5"] = hashlib.md5(r.encode()).hexdigest()
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print(json.dumps(out, indent=1, sort_keys=True))

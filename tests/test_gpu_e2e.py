@@ -18,6 +18,7 @@ from tests.conftest import GOLDEN, ROOT
 pytestmark = pytest.mark.gpu
 
 HOOKED = os.path.join(ROOT, "oracle", "_ref", "minibwa-mbg")
+HOOKED_BATCH = os.path.join(ROOT, "oracle", "_ref", "mbmap-batch-mbg")   # api-test/ex-batch.c
 
 
 def _sam_body_md5(text):
@@ -25,9 +26,9 @@ def _sam_body_md5(text):
     return hashlib.md5(body.encode()).hexdigest()
 
 
-def _run(args, timeout=300):
-    print("[e2e]", " ".join(args), flush=True)
-    r = subprocess.run([HOOKED] + args, capture_output=True, text=True, timeout=timeout)
+def _run(args, timeout=300, exe=HOOKED):
+    print("[e2e]", exe, " ".join(args), flush=True)
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=timeout)
     assert r.returncode == 0, r.stderr[-2000:]
     return r.stdout
 
@@ -44,6 +45,23 @@ def test_chrm_pairs_sam_identical(gold):
     sam = _run(["map", os.path.join(GOLDEN, "chrM"), os.path.join(GOLDEN, "chrM-read_1.fa.gz"),
                 os.path.join(GOLDEN, "chrM-read_2.fa.gz")])
     assert _sam_body_md5(sam) == gold["chrM.sam.body.md5"]
+
+
+def test_batch_api_identical(gold, tmp_path):
+    """The public batch API (mb_map_batch, minibwa.h:127-170) as the reference's own example drives
+    it (api-test/ex-batch.c:43-77), hooks on the GPU: identical hit lines and CIGARs."""
+    if not os.path.exists(HOOKED_BATCH):
+        pytest.skip("oracle/_ref/mbmap-batch-mbg not built")
+    import hashlib as H
+    out = _run([os.path.join(GOLDEN, "chrM"), os.path.join(GOLDEN, "chrM-read_1.fa.gz")], exe=HOOKED_BATCH)
+    assert H.md5(out.encode()).hexdigest() == gold["chrM.batch.md5"]
+    from minibwa_amd import synth
+    ref = synth.reference(16569, seed=11, rep_frac=0.0)
+    r1, r2 = synth.read_pairs(ref, 10000, seed=11)
+    f1, f2 = str(tmp_path / "r1.fq"), str(tmp_path / "r2.fq")
+    synth.write_pairs_fastq(f1, f2, r1, r2, prefix="c0_")
+    out = _run([os.path.join(GOLDEN, "cfg0"), f1], exe=HOOKED_BATCH)
+    assert H.md5(out.encode()).hexdigest() == gold["cfg0.batch.md5"]
 
 
 def test_cfg0_pairs_sam_identical(gold, tmp_path):
