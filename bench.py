@@ -134,10 +134,13 @@ def main():
     bytes_seed1 = 64 * work_stats["occ_blocks"][0] + 32 * work_stats["kmer_lookups"][0]
     achieved = bytes_seed1 / (ms_seed1 * 1e-3) / 1e9
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "smem_pass1_cfg1_pmc.json")
+    # HBM bytes per launch of the same kernel from the committed FETCH_SIZE/WRITE_SIZE profile of this
+    # workload (each counter scaled by what it reads for a plain copy of known size)
+    pmc = os.path.join(ROOT, "profiles", "step_cfg1_pmc.json")
     if os.path.exists(pmc):
         try:
-            traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
+            ks = json.load(open(pmc))["kernels"]
+            traffic = [v for k, v in ks.items() if "smem_kernel<1>" in k][0]["hbm_bytes_per_launch"]
         except Exception:
             traffic = None
     phases = {k: round(float(np.mean([p[k] for p in per])), 3) for k in ("ms_seed1", "ms_seed2", "ms_rows", "ms_sa", "ms_jobs", "ms_dp", "ms_total")}
