@@ -1,22 +1,21 @@
 // mbg_dp.hip -- banded dual-affine (ksw_extd2) and single-affine (ksw_extz2) extension DP for gfx950.
 //
-// One 64-lane wavefront per job; each anti-diagonal r is computed by the lanes in parallel (cell t
-// of the row on lane (t - st) % 64), the t-1 neighbour of the previous row arrives by a lane
-// shuffle, and the row's state lives in LDS.  The results must equal ksw_extd2_sse
-// (ksw2_extd2_sse.c:27-403) and ksw_extz2_sse (ksw2_extz2_sse.c:16-296) bit for bit, so the kernel
-// keeps their observable semantics rather than a textbook recurrence:
+// One 64-lane wavefront per job; each anti-diagonal r is computed by the lanes in parallel and the
+// t-1 neighbour of the previous row arrives by a DPP lane shift.  The results must equal
+// ksw_extd2_sse (ksw2_extd2_sse.c:27-403) and ksw_extz2_sse (ksw2_extz2_sse.c:16-296) bit for bit,
+// so the kernel keeps their observable semantics rather than a textbook recurrence:
 //   * int8 difference arithmetic with wrap-around, and the unsigned/signed max mix of extz2;
 //   * every row covers the 16-aligned range [st0/16*16, (en0+16)/16*16-1]: cells outside the exact
 //     band are computed, kept and later read, exactly as the 16-lane SSE blocks do;
 //   * the score row is written in 16-cell groups starting at st0 (so it runs past en0 with the
 //     target's zero padding) and cells it does not reach keep their stale scores;
-//   * the per-index state lives in a circular LDS buffer of `cap` bytes per array; an index is set
-//     to the reference's initial value when it first enters the window (cap > row width + 16, so
-//     no live index is overwritten);
+//   * an index of the row state takes the reference's initial value when it first enters the
+//     window [.., en+15] (memset/calloc/NEG_INF of the reference's arrays);
 //   * the exact-max argmax reproduces the SSE reduction order (H[en0] first, four lanes with
 //     t = st0 mod 4 classes keeping their first strict maximum, merged in lane order, then the
-//     scalar tail), encoded as one 64-bit key per cell and a wave max-reduction;
+//     scalar tail), encoded as one priority key per cell and wave max-reductions;
 //   * the backtrack walks the rotated matrix (ksw2.h:140-172) from LDS-staged chunks of rows.
+//
 #include "mbg_internal.h"
 
 namespace mbg {
@@ -34,6 +33,12 @@ __device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
 
 // value of lane-1 (lane 0 receives 0)
 __device__ __forceinline__ int wave_shr1(int v) { return (int)dpp_u32<0x138>((uint32_t)v); }
+
+// value of lane-1; lane 0 receives `fill`
+__device__ __forceinline__ int wave_shr1_fill(int v, int fill)
+{
+	return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xf, 0xf, false);
+}
 
 // unsigned max over the 64 lanes, returned as a wave-uniform value (rows of 16 first, then the
 // row broadcasts fold rows 0-1 and 2-3, then lanes 31 -> 63)
@@ -68,123 +73,443 @@ __device__ __forceinline__ bool apply_zdrop(DpState &z, int32_t H, int r, int t,
 	return drop;
 }
 
+// per-job parameters (ksw2_extd2_sse.c:60-97 / ksw2_extz2_sse.c:41-74)
 template <bool DUAL>
-__device__ void run_job(const DpArgs &a, int jid, uint8_t *lds, const int8_t *mat, uint8_t *slab)
+struct JobP {
+	int qlen, tlen, m, flag, zdrop, end_bonus;
+	int8_t q, e, q2, e2;                 // after extd2's swap that makes (q,e) the cheaper opening
+	int qe0;                             // extd2's scalar q+e, taken before the swap
+	int qe;                              // extz2's scalar q+e
+	int8_t qe8, qe28, sc_mch, sc_mis, m1, sc_N, qe2x, max_sc_, ini, ini2;
+	int w, n_col_, nb, tl_hi, max_sc, long_thres, long_diff;
+	bool skip, with_cigar, approx_max, right, generic;
+};
+
+template <bool DUAL>
+__device__ __forceinline__ JobP<DUAL> job_params(const DpJob &jb, int m, const int8_t *mat)
+{
+	JobP<DUAL> P;
+	P.qlen = jb.qlen, P.tlen = jb.tlen, P.m = m, P.flag = jb.flag, P.zdrop = jb.zdrop, P.end_bonus = jb.end_bonus;
+	int8_t q = jb.q, e = jb.e, q2 = jb.q2, e2 = jb.e2;
+	P.with_cigar = !(P.flag & 0x01), P.approx_max = !!(P.flag & 0x08), P.right = !!(P.flag & 0x02);
+	P.generic = !!(P.flag & 0x04);
+	P.skip = DUAL ? (m <= 1 || P.qlen <= 0 || P.tlen <= 0) : (m <= 0 || P.qlen <= 0 || P.tlen <= 0);
+	P.qe0 = q + e;
+	if (DUAL && q2 + e2 < q + e) { int8_t t = q; q = q2; q2 = t; t = e; e = e2; e2 = t; }
+	P.q = q, P.e = e, P.q2 = q2, P.e2 = e2;
+	P.qe8 = (int8_t)(q + e), P.qe28 = (int8_t)(q2 + e2);
+	P.qe = q + e;
+	P.sc_mch = mat[0], P.sc_mis = mat[1], P.m1 = (int8_t)(m - 1);
+	P.sc_N = 0, P.qe2x = 0, P.max_sc_ = 0;
+	P.w = jb.w, P.n_col_ = 1, P.max_sc = 0, P.long_thres = 0, P.long_diff = 0;
+	if (!P.skip) {
+		P.sc_N = DUAL ? (mat[m * m - 1] == 0 ? (int8_t)-e2 : mat[m * m - 1]) : (mat[m * m - 1] == 0 ? (int8_t)-e : mat[m * m - 1]);
+		P.qe2x = (int8_t)((q + e) * 2);
+		P.max_sc_ = (int8_t)(mat[0] + (q + e) * 2);
+		dp_geom(P.qlen, P.tlen, P.w, &P.w, &P.n_col_);
+		int max_sc = mat[0], min_sc = mat[1];
+		for (int t = 1; t < m * m; ++t) {
+			max_sc = max_sc > mat[t] ? max_sc : mat[t];
+			min_sc = min_sc < mat[t] ? min_sc : mat[t];
+		}
+		P.max_sc = max_sc;
+		if (-min_sc > 2 * (q + e)) P.skip = true;
+		if (DUAL) {
+			P.long_thres = e != e2 ? (q2 - q) / (e - e2) - 1 : 0;
+			if (q2 + e2 + P.long_thres * e2 > q + e + P.long_thres * e) ++P.long_thres;
+			P.long_diff = P.long_thres * (e - e2) - (q2 - q) - e2;
+		}
+	}
+	P.nb = P.n_col_ * 16;
+	P.ini = DUAL ? (int8_t)(-q - e) : 0, P.ini2 = DUAL ? (int8_t)(-q2 - e2) : 0;
+	P.tl_hi = (P.tlen + 15) / 16 * 16 - 1;
+	return P;
+}
+
+// band of anti-diagonal r (ksw2_extd2_sse.c:130-139); false when it is empty
+__device__ __forceinline__ bool row_range(int r, int qlen, int tlen, int w, int &st0, int &en0)
+{
+	int st = 0, en = tlen - 1;
+	if (st < r - qlen + 1) st = r - qlen + 1;
+	if (en > r) en = r;
+	if (st < (r - w + 1) >> 1) st = (r - w + 1) >> 1;
+	if (en > (r + w) >> 1) en = (r + w) >> 1;
+	st0 = st, en0 = en;
+	return st <= en;
+}
+
+// values of cell st-1 when the previous row did not hold it (ksw2_extd2_sse.c:141-155 /
+// ksw2_extz2_sse.c:110-115)
+template <bool DUAL>
+__device__ __forceinline__ void row_boundary(const JobP<DUAL> &P, int r, int st, int8_t &x1, int8_t &v1, int8_t &x21)
+{
+	if (DUAL) {
+		x1 = (int8_t)(-P.q - P.e), x21 = (int8_t)(-P.q2 - P.e2);
+		if (st > 0) v1 = (int8_t)(-P.q - P.e);
+		else v1 = (int8_t)(r == 0 ? -P.q - P.e : r < P.long_thres ? -P.e : r == P.long_thres ? P.long_diff : -P.e2);
+	} else {
+		x1 = 0, x21 = 0;
+		v1 = st > 0 ? 0 : (int8_t)(r ? P.q : 0);
+	}
+}
+
+// the values index r takes when the row reaches it (ksw2_extd2_sse.c:156-161)
+template <bool DUAL>
+__device__ __forceinline__ void row_yu(const JobP<DUAL> &P, int r, int8_t &y, int8_t &u, int8_t &y2)
+{
+	if (DUAL) {
+		y = (int8_t)(-P.q - P.e), y2 = (int8_t)(-P.q2 - P.e2);
+		u = (int8_t)(r == 0 ? -P.q - P.e : r < P.long_thres ? -P.e : r == P.long_thres ? P.long_diff : -P.e2);
+	} else {
+		y = 0, y2 = 0, u = (int8_t)(r ? P.q : 0);
+	}
+}
+
+// one cell of the difference recurrence (ksw2_extd2_sse.c:186-265 / ksw2_extz2_sse.c:139-229);
+// returns the backtrack byte
+template <bool DUAL>
+__device__ __forceinline__ uint8_t dp_cell(const JobP<DUAL> &P, int8_t xt1, int8_t vt1, int8_t x2t1, int8_t ou, int8_t oy,
+                                           int8_t oy2, int8_t zs, int8_t &nU, int8_t &nV, int8_t &nX, int8_t &nY,
+                                           int8_t &nX2, int8_t &nY2)
+{
+	uint8_t d = 0;
+	if (DUAL) {
+		int8_t zz = zs;
+		int8_t A = (int8_t)(xt1 + vt1), B = (int8_t)(oy + ou), A2 = (int8_t)(x2t1 + vt1), B2 = (int8_t)(oy2 + ou);
+		if (!P.with_cigar) {
+			zz = smax8(zz, A); zz = smax8(zz, B); zz = smax8(zz, A2); zz = smax8(zz, B2);
+		} else if (!P.right) {
+			d = A > zz ? 1 : 0;   zz = smax8(zz, A);
+			d = B > zz ? 2 : d;   zz = smax8(zz, B);
+			d = A2 > zz ? 3 : d;  zz = smax8(zz, A2);
+			d = B2 > zz ? 4 : d;  zz = smax8(zz, B2);
+		} else {
+			d = zz > A ? 0 : 1;   zz = smax8(zz, A);
+			d = zz > B ? d : 2;   zz = smax8(zz, B);
+			d = zz > A2 ? d : 3;  zz = smax8(zz, A2);
+			d = zz > B2 ? d : 4;  zz = smax8(zz, B2);
+		}
+		zz = smin8(zz, P.sc_mch);
+		nU = (int8_t)(zz - vt1);
+		nV = (int8_t)(zz - ou);
+		int8_t tmp = (int8_t)(zz - P.q);
+		A = (int8_t)(A - tmp); B = (int8_t)(B - tmp);
+		tmp = (int8_t)(zz - P.q2);
+		A2 = (int8_t)(A2 - tmp); B2 = (int8_t)(B2 - tmp);
+		if (!P.with_cigar || !P.right) {
+			nX  = (int8_t)((A > 0 ? A : 0) - P.qe8);
+			nY  = (int8_t)((B > 0 ? B : 0) - P.qe8);
+			nX2 = (int8_t)((A2 > 0 ? A2 : 0) - P.qe28);
+			nY2 = (int8_t)((B2 > 0 ? B2 : 0) - P.qe28);
+			d |= (A > 0 ? 0x08 : 0) | (B > 0 ? 0x10 : 0) | (A2 > 0 ? 0x20 : 0) | (B2 > 0 ? 0x40 : 0);
+		} else {
+			nX  = (int8_t)((A >= 0 ? A : 0) - P.qe8);
+			nY  = (int8_t)((B >= 0 ? B : 0) - P.qe8);
+			nX2 = (int8_t)((A2 >= 0 ? A2 : 0) - P.qe28);
+			nY2 = (int8_t)((B2 >= 0 ? B2 : 0) - P.qe28);
+			d |= (A >= 0 ? 0x08 : 0) | (B >= 0 ? 0x10 : 0) | (A2 >= 0 ? 0x20 : 0) | (B2 >= 0 ? 0x40 : 0);
+		}
+	} else {
+		const uint8_t uvt1 = (uint8_t)vt1, uut = (uint8_t)ou;
+		int8_t zz = (int8_t)((uint8_t)zs + (uint8_t)P.qe2x);
+		int8_t A = (int8_t)((uint8_t)xt1 + uvt1), B = (int8_t)((uint8_t)oy + uut);
+		if (!P.with_cigar) {
+			zz = smax8(zz, A);
+		} else if (!P.right) {
+			d = A > zz ? 1 : 0;
+			zz = smax8(zz, A);
+			d = B > zz ? 2 : d;
+		} else {
+			d = zz > A ? 0 : 1;
+			zz = smax8(zz, A);
+			d = zz > B ? d : 2;
+		}
+		uint8_t zu = (uint8_t)zz > (uint8_t)B ? (uint8_t)zz : (uint8_t)B;
+		zu = zu < (uint8_t)P.max_sc_ ? zu : (uint8_t)P.max_sc_;
+		nU = (int8_t)(uint8_t)(zu - uvt1);
+		nV = (int8_t)(uint8_t)(zu - uut);
+		const int8_t zq = (int8_t)(uint8_t)(zu - (uint8_t)P.q);
+		A = (int8_t)(A - zq); B = (int8_t)(B - zq);
+		nX2 = 0, nY2 = 0;
+		if (!P.with_cigar || !P.right) {
+			nX = A > 0 ? A : 0; nY = B > 0 ? B : 0;
+			d |= (A > 0 ? 0x08 : 0) | (B > 0 ? 0x10 : 0);
+		} else {
+			nX = A >= 0 ? A : 0; nY = B >= 0 ? B : 0;
+			d |= (A >= 0 ? 0x08 : 0) | (B >= 0 ? 0x10 : 0);
+		}
+	}
+	return d;
+}
+
+// H increment carried by a new v (cells t < en0) or u (cell en0) value
+template <bool DUAL>
+__device__ __forceinline__ int32_t h_term(const JobP<DUAL> &P, int8_t x)
+{
+	return DUAL ? (int32_t)x : (int32_t)(uint8_t)x - P.qe;
+}
+
+// priority of cell t in the SSE argmax order: cells of the 4-lane body rank by lane class
+// (t - st0) % 4 first, then by position; the scalar tail ranks below them, earliest first; H[en0]
+// (key 0xffffffff) ranks above everything
+__device__ __forceinline__ uint32_t argmax_key(int t, int st0, int en1)
+{
+	return t < en1 ? (0x80000000u | ((uint32_t)(3 - ((t - st0) & 3)) << 28) | (0x0FFFFFFFu - (uint32_t)((t - st0) >> 2)))
+	               : (0x7FFFFFFFu - (uint32_t)(t - en1));
+}
+
+// the same order packed with the score for rows of at most 247 cells and |H| < 2^23:
+// (H + 2^23) << 8 | rank, rank 255 for en0, 254 - (class*63 + position) for body cells, 2 - offset
+// for the (at most 3) tail cells
+__device__ __forceinline__ uint32_t argmax_key8(int32_t h, int t, int st0, int en1)
+{
+	const int d = t - st0;
+	const uint32_t rank = t < en1 ? (uint32_t)(254 - ((d & 3) * 63 + (d >> 2))) : (uint32_t)(2 - (t - en1));
+	return ((uint32_t)(h + (1 << 23)) << 8) | rank;
+}
+
+__device__ __forceinline__ void argmax_decode8(uint32_t k, int st0, int en0, int en1, int32_t &max_H, int &max_t)
+{
+	max_H = (int32_t)(k >> 8) - (1 << 23);
+	const int rank = (int)(k & 255u);
+	if (rank == 255) max_t = en0;
+	else if (rank >= 3) { const int v = 254 - rank; max_t = st0 + (v % 63) * 4 + v / 63; }
+	else max_t = en1 + (2 - rank);
+}
+
+__device__ __forceinline__ void argmax_take(uint32_t &bh, uint32_t &bp, int32_t h, uint32_t key)
+{
+	const uint32_t hk = (uint32_t)h ^ 0x80000000u;
+	if (hk > bh || (hk == bh && key > bp)) bh = hk, bp = key;
+}
+
+__device__ __forceinline__ void argmax_reduce(uint32_t bh, uint32_t bp, int st0, int en0, int en1, int32_t &max_H, int &max_t)
+{
+	const uint32_t mh = wave_max_u32(bh);
+	const uint32_t pri = wave_max_u32(bh == mh ? bp : 0u);
+	max_H = (int32_t)(mh ^ 0x80000000u);
+	if (pri == 0xFFFFFFFFu) max_t = en0;
+	else if (pri & 0x80000000u) max_t = st0 + (int)(0x0FFFFFFFu - (pri & 0x0FFFFFFFu)) * 4 + (3 - (int)((pri >> 28) & 3));
+	else max_t = en1 + (int)(0x7FFFFFFFu - pri);
+}
+
+// end of an exact-max row (ksw2_extd2_sse.c:345-367); true when the DP stops
+template <bool DUAL>
+__device__ __forceinline__ bool finish_exact(const JobP<DUAL> &P, DpState &z, int r, int st0, int en0, int32_t max_H, int max_t,
+                                             int32_t Hen0, int32_t Hst0, int32_t &last_max_H)
+{
+	if (en0 == P.tlen - 1 && Hen0 > z.mte) z.mte = Hen0, z.mte_q = r - en0;
+	if (r - st0 == P.qlen - 1 && Hst0 > z.mqe) z.mqe = Hst0, z.mqe_t = st0;
+	if (apply_zdrop(z, max_H, r, max_t, P.zdrop, DUAL ? P.e2 : P.e)) return true;
+	bool stop = false;
+	if (r == P.qlen + P.tlen - 2 && en0 == P.tlen - 1) z.score = Hen0;
+	if (DUAL && (P.flag & 0x40)) { // ksw2_extd2_sse.c:361-367
+		const int32_t rH = last_max_H > max_H ? last_max_H : max_H;
+		const int32_t rq = P.qlen - (r - st0), rt = P.tlen - en0;
+		const int32_t rm = rq >= P.tlen - st0 ? P.tlen - st0 : rt >= P.qlen - (r - en0) ? P.qlen - (r - en0) : P.tlen + P.qlen - 1 - r;
+		if (rH + rm * P.max_sc + P.end_bonus < z.max) stop = true;
+	}
+	last_max_H = max_H;
+	return stop;
+}
+
+// approximate max (ksw2_extd2_sse.c:369-385 / ksw2_extz2_sse.c:262-278): H0 follows one path
+// along the rows; rdV(t)/rdU(t) read the row's new v/u at index t
+template <bool DUAL, class RV, class RU>
+__device__ __forceinline__ bool finish_approx(const JobP<DUAL> &P, DpState &z, int r, int st0, int en0, int32_t &H0, int &last_H0_t,
+                                              RV &&rdV, RU &&rdU)
+{
+	bool stop = false;
+	if (r > 0) {
+		const bool in0 = last_H0_t >= st0 && last_H0_t <= en0, in1 = last_H0_t + 1 >= st0 && last_H0_t + 1 <= en0;
+		if (in0 && in1) {
+			const int32_t d0 = h_term<DUAL>(P, rdV(last_H0_t));
+			const int32_t d1 = h_term<DUAL>(P, rdU(last_H0_t + 1));
+			if (d0 > d1) H0 += d0;
+			else H0 += d1, ++last_H0_t;
+		} else if (in0) {
+			H0 += h_term<DUAL>(P, rdV(last_H0_t));
+		} else {
+			++last_H0_t;
+			H0 += h_term<DUAL>(P, rdU(last_H0_t));
+		}
+		if (!DUAL && (P.flag & 0x10) && apply_zdrop(z, H0, r, last_H0_t, P.zdrop, P.e)) stop = true;
+	} else {
+		const int8_t v0 = rdV(0);
+		H0 = DUAL ? (int32_t)v0 - P.qe0 : (int32_t)(uint8_t)v0 - P.qe - P.qe;
+		last_H0_t = 0;
+	}
+	if (!stop && DUAL && (P.flag & 0x10) && apply_zdrop(z, H0, r, last_H0_t, P.zdrop, P.e2)) stop = true;
+	if (!stop && r == P.qlen + P.tlen - 2 && en0 == P.tlen - 1) z.score = H0;
+	return stop;
+}
+
+// backtrack (ksw2_extd2_sse.c:391-402, ksw2.h:140-172) over the rows kept in `slab`, staged through
+// the first 12*cap bytes of LDS; returns n_cigar
+template <bool DUAL>
+__device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const DpJob &jb, const DpArgs &a, uint8_t *lds, const uint8_t *slab)
+{
+	const int lane = threadIdx.x;
+	const int qlen = P.qlen, tlen = P.tlen, w = P.w, nb = P.nb;
+	uint32_t *cg = a.cigar + jb.cigar_off;
+	int n_cigar = 0;
+	int i0 = -1, j0 = -1;
+	if (!P.skip && P.with_cigar) {
+		if (!z.zdropped && !(P.flag & 0x40)) i0 = tlen - 1, j0 = qlen - 1;
+		else if ((P.flag & 0x40) && z.mqe + P.end_bonus > z.max) z.reach_end = 1, i0 = z.mqe_t, j0 = qlen - 1;
+		else if (z.max_t >= 0 && z.max_q >= 0) i0 = z.max_t, j0 = z.max_q;
+	}
+	if (i0 < 0 && j0 < 0) return 0;
+	uint8_t *stg = lds;
+	const int K = (12 * a.cap) / nb;
+	int chunk_lo = 0x7fffffff;
+	int i = i0, j = j0, state = 0, last_op = -1;
+	uint32_t cur = 0;
+	while (i >= 0 && j >= 0) {
+		const int r = i + j;
+		int s0, e0;
+		row_range(r, qlen, tlen, w, s0, e0);
+		const int off = s0 >> 4 << 4, off_end = ((e0 + 16) >> 4 << 4) - 1;
+		int force = -1;
+		if (i < off) force = 2;
+		if (i > off_end) force = 1;
+		uint32_t tmp = 0;
+		if (force < 0) {
+			if (r < chunk_lo) {
+				const int lo = r - K + 1 > 0 ? r - K + 1 : 0;
+				const int bytes = (r - lo + 1) * nb;
+				__syncthreads();
+				for (int o = lane * 16; o < bytes; o += 64 * 16)
+					*(uint4*)(stg + o) = *(const uint4*)(slab + (size_t)lo * nb + o);
+				__syncthreads();
+				chunk_lo = lo;
+			}
+			tmp = stg[(r - chunk_lo) * nb + i - off];
+		}
+		if (state == 0) state = tmp & 7;
+		else if (!(tmp >> (state + 2) & 1)) state = 0;
+		if (state == 0) state = tmp & 7;
+		if (force >= 0) state = force;
+		int op;
+		if (state == 0) op = 0, --i, --j;
+		else if (state == 1 || state == 3) op = 2, --i;
+		else op = 1, --j;
+		if (op == last_op) cur += 16;
+		else {
+			if (last_op >= 0 && lane == 0) cg[n_cigar - 1] = cur;
+			++n_cigar, last_op = op, cur = 16u | (uint32_t)op;
+		}
+	}
+	if (i >= 0) {
+		if (last_op == 2) cur += (uint32_t)(i + 1) << 4;
+		else { if (last_op >= 0 && lane == 0) cg[n_cigar - 1] = cur; ++n_cigar, last_op = 2, cur = (uint32_t)(i + 1) << 4 | 2; }
+	}
+	if (j >= 0) {
+		if (last_op == 1) cur += (uint32_t)(j + 1) << 4;
+		else { if (last_op >= 0 && lane == 0) cg[n_cigar - 1] = cur; ++n_cigar, last_op = 1, cur = (uint32_t)(j + 1) << 4 | 1; }
+	}
+	if (last_op >= 0 && lane == 0) cg[n_cigar - 1] = cur;
+	if (!(P.flag & 0x80)) {
+		__syncthreads();
+		for (int k = lane; k < n_cigar / 2; k += 64) {
+			const uint32_t t0 = cg[k], t1 = cg[n_cigar - 1 - k];
+			cg[k] = t1, cg[n_cigar - 1 - k] = t0;
+		}
+	}
+	return n_cigar;
+}
+
+__device__ __forceinline__ void init_state(DpState &z)
+{
+	z.max_q = z.max_t = z.mqe_t = z.mte_q = -1;
+	z.max = 0, z.score = z.mqe = z.mte = kNegInf;
+	z.zdropped = 0, z.reach_end = 0;
+}
+
+__device__ __forceinline__ void write_result(const DpArgs &a, int jid, const DpState &z, int n_cigar, int cells)
+{
+	if (threadIdx.x == 0) {
+		DpRes o;
+		o.max = z.max, o.zdropped = z.zdropped, o.max_q = z.max_q, o.max_t = z.max_t;
+		o.mqe = z.mqe, o.mqe_t = z.mqe_t, o.mte = z.mte, o.mte_q = z.mte_q;
+		o.score = z.score, o.reach_end = z.reach_end, o.n_cigar = n_cigar, o.cells = cells;
+		a.res[jid] = o;
+	}
+}
+
+// ---- LDS schedule ------------------------------------------------------------------------------
+
+// STAGED: the job's query and target are copied into LDS after the 12*cap state region first
+template <bool DUAL, bool STAGED>
+__device__ __forceinline__ void run_job_lds(const DpArgs &a, int jid, const DpJob &jb, const JobP<DUAL> P, uint8_t *lds, const int8_t *mat,
+                            uint8_t *slab)
 {
 	const int lane = threadIdx.x;
 	const int CAP = a.cap, CM = CAP - 1;
 	int8_t *U = (int8_t*)lds, *V = U + CAP, *X = V + CAP, *Y = X + CAP, *X2 = Y + CAP, *Y2 = X2 + CAP, *S = Y2 + CAP;
 	int32_t *H = (int32_t*)(lds + 8 * CAP);
-	const DpJob jb = a.jobs[jid];
-	const uint8_t *Q = a.qseq + jb.qoff, *T = a.tseq + jb.toff;
-	const int qlen = jb.qlen, tlen = jb.tlen, m = a.m, flag = jb.flag, zdrop = jb.zdrop, end_bonus = jb.end_bonus;
-	int8_t q = jb.q, e = jb.e, q2 = jb.q2, e2 = jb.e2;
-	DpState z;
-	z.max_q = z.max_t = z.mqe_t = z.mte_q = -1;
-	z.max = 0, z.score = z.mqe = z.mte = kNegInf;
-	z.zdropped = 0, z.reach_end = 0;
-	int cells = 0, n_cigar = 0;
-	const bool with_cigar = !(flag & 0x01), approx_max = !!(flag & 0x08), right = !!(flag & 0x02), generic = !!(flag & 0x04);
-	bool skip = DUAL ? (m <= 1 || qlen <= 0 || tlen <= 0) : (m <= 0 || qlen <= 0 || tlen <= 0);
-
-	// parameters (ksw2_extd2_sse.c:60-97 / ksw2_extz2_sse.c:41-74)
-	const int qe0 = q + e;                       // extd2's scalar q+e is taken before the swap
-	if (DUAL && q2 + e2 < q + e) { int8_t t = q; q = q2; q2 = t; t = e; e = e2; e2 = t; }
-	const int8_t qe8 = (int8_t)(q + e), qe28 = (int8_t)(q2 + e2);
-	const int qe = q + e;                        // extz2's scalar q+e
-	const int8_t sc_mch = mat[0], sc_mis = mat[1], m1 = (int8_t)(m - 1);
-	int8_t sc_N = 0, qe2x = 0, max_sc_ = 0;
-	int w = jb.w, tlen_ = (tlen + 15) / 16, n_col_ = 1, max_sc = 0, min_sc = 0, long_thres = 0, long_diff = 0;
-	if (!skip) {
-		sc_N = DUAL ? (mat[m * m - 1] == 0 ? (int8_t)-e2 : mat[m * m - 1]) : (mat[m * m - 1] == 0 ? (int8_t)-e : mat[m * m - 1]);
-		qe2x = (int8_t)((q + e) * 2);
-		max_sc_ = (int8_t)(mat[0] + (q + e) * 2);
-		dp_geom(qlen, tlen, w, &w, &n_col_);
-		max_sc = mat[0], min_sc = mat[1];
-		for (int t = 1; t < m * m; ++t) {
-			max_sc = max_sc > mat[t] ? max_sc : mat[t];
-			min_sc = min_sc < mat[t] ? min_sc : mat[t];
-		}
-		if (-min_sc > 2 * (q + e)) skip = true;
-		if (DUAL) {
-			long_thres = e != e2 ? (q2 - q) / (e - e2) - 1 : 0;
-			if (q2 + e2 + long_thres * e2 > q + e + long_thres * e) ++long_thres;
-			long_diff = long_thres * (e - e2) - (q2 - q) - e2;
-		}
+	const uint8_t *Qg = a.qseq + jb.qoff, *Tg = a.tseq + jb.toff;
+	const int qlen = P.qlen, tlen = P.tlen;
+	uint8_t *Qs = lds + 12 * CAP, *Ts = Qs + ((qlen + 15) & ~15);
+	if (STAGED && !P.skip) {
+		for (int i = lane; i < qlen; i += 64) Qs[i] = Qg[i];
+		for (int i = lane; i < tlen; i += 64) Ts[i] = Tg[i];
+		__syncthreads();
 	}
-	const int nb = n_col_ * 16;
-	const int8_t ini = DUAL ? (int8_t)(-q - e) : 0, ini2 = DUAL ? (int8_t)(-q2 - e2) : 0;
-	const int tl_hi = tlen_ * 16 - 1;
-
+	const uint8_t *Q = STAGED ? Qs : Qg, *T = STAGED ? Ts : Tg;
+	const bool hsmall = qlen + tlen <= 60000;   // |H| < 2^23: H and the argmax rank fit one 32-bit key
+	DpState z;
+	init_state(z);
+	int cells = 0;
 	int hw = -1, last_st = -1, last_en = -1, last_max_H = 0, last_H0_t = 0;
 	int32_t H0 = 0;
-	for (int r = 0; !skip && r < qlen + tlen - 1; ++r) {
-		int st = 0, en = tlen - 1;
-		if (st < r - qlen + 1) st = r - qlen + 1;
-		if (en > r) en = r;
-		if (st < (r - w + 1) >> 1) st = (r - w + 1) >> 1;
-		if (en > (r + w) >> 1) en = (r + w) >> 1;
-		if (st > en) { z.zdropped = 1; break; }
-		const int st0 = st, en0 = en;
+	for (int r = 0; !P.skip && r < qlen + tlen - 1; ++r) {
+		int st0, en0;
+		if (!row_range(r, qlen, tlen, P.w, st0, en0)) { z.zdropped = 1; break; }
 		cells += en0 - st0 + 1;
-		st = st >> 4 << 4;
-		en = ((en + 16) >> 4 << 4) - 1;
-		// indices entering the window take the initial values (memset/calloc/NEG_INF)
+		const int st = st0 >> 4 << 4, en = ((en0 + 16) >> 4 << 4) - 1;
 		int hi = en + 15;
-		if (hi > tl_hi) hi = tl_hi;
+		if (hi > P.tl_hi) hi = P.tl_hi;
 		if (hi > hw) {
 			for (int idx = hw + 1 + lane; idx <= hi; idx += 64) {
 				const int sl = idx & CM;
-				U[sl] = ini; V[sl] = ini; X[sl] = ini; Y[sl] = ini;
-				if (DUAL) { X2[sl] = ini2; Y2[sl] = ini2; }
+				U[sl] = P.ini; V[sl] = P.ini; X[sl] = P.ini; Y[sl] = P.ini;
+				if (DUAL) { X2[sl] = P.ini2; Y2[sl] = P.ini2; }
 				S[sl] = 0;
 				H[sl] = kNegInf;
 			}
 			hw = hi;
 			__syncthreads();
 		}
-		// boundary values of cell st-1 (ksw2_extd2_sse.c:141-155 / ksw2_extz2_sse.c:110-115)
-		int8_t x1, x21 = 0, v1;
-		if (st > 0) {
-			if (st - 1 >= last_st && st - 1 <= last_en) {
-				const int sl = (st - 1) & CM;
-				x1 = X[sl], v1 = V[sl];
-				if (DUAL) x21 = X2[sl];
-			} else if (DUAL) {
-				x1 = (int8_t)(-q - e), x21 = (int8_t)(-q2 - e2), v1 = (int8_t)(-q - e);
-			} else x1 = v1 = 0;
-		} else if (DUAL) {
-			x1 = (int8_t)(-q - e), x21 = (int8_t)(-q2 - e2);
-			v1 = (int8_t)(r == 0 ? -q - e : r < long_thres ? -e : r == long_thres ? long_diff : -e2);
-		} else {
-			x1 = 0, v1 = r ? q : 0;
-		}
+		int8_t x1, x21, v1;
+		if (st > 0 && st - 1 >= last_st && st - 1 <= last_en) {
+			const int sl = (st - 1) & CM;
+			x1 = X[sl], v1 = V[sl], x21 = DUAL ? X2[sl] : 0;
+		} else row_boundary<DUAL>(P, r, st, x1, v1, x21);
 		if (en >= r && lane == 0) {
+			int8_t y, u, y2;
+			row_yu<DUAL>(P, r, y, u, y2);
 			const int sl = r & CM;
-			if (DUAL) {
-				Y[sl] = (int8_t)(-q - e), Y2[sl] = (int8_t)(-q2 - e2);
-				U[sl] = (int8_t)(r == 0 ? -q - e : r < long_thres ? -e : r == long_thres ? long_diff : -e2);
-			} else {
-				Y[sl] = 0, U[sl] = (int8_t)(r ? q : 0);
-			}
+			Y[sl] = y, U[sl] = u;
+			if (DUAL) Y2[sl] = y2;
 		}
 		// scores (loop fission in the reference): 16-cell groups from st0, padding past tlen reads 0
-		if (!generic) {
+		if (!P.generic) {
 			int shi = st0 + ((en0 - st0) >> 4 << 4) + 15;
-			if (shi > tl_hi) shi = tl_hi;
+			if (shi > P.tl_hi) shi = P.tl_hi;
 			for (int t = st0 + lane; t <= shi; t += 64) {
 				const int8_t ta = t < tlen ? (int8_t)T[t] : 0, qa = t <= r ? (int8_t)Q[r - t] : 0;
-				S[t & CM] = (ta == m1 || qa == m1) ? sc_N : ta == qa ? sc_mch : sc_mis;
+				S[t & CM] = (ta == P.m1 || qa == P.m1) ? P.sc_N : ta == qa ? P.sc_mch : P.sc_mis;
 			}
 		} else {
 			for (int t = st0 + lane; t <= en0; t += 64)
-				S[t & CM] = mat[T[t] * m + Q[r - t]];
+				S[t & CM] = mat[T[t] * P.m + Q[r - t]];
 		}
 		__syncthreads();
 
 		// core: every cell of [st, en]
 		const int32_t Hprev = en0 > 0 ? H[(en0 - 1) & CM] : H[en0 & CM];
-		uint8_t *prow = slab + (size_t)r * nb;
+		uint8_t *prow = slab + (size_t)r * P.nb;
 		int8_t cx = x1, cv = v1, cx2 = x21;
 		for (int base = st; base <= en; base += 64) {
 			const int t = base + lane, sl = t & CM;
@@ -194,9 +519,8 @@ __device__ void run_job(const DpArgs &a, int jid, uint8_t *lds, const int8_t *ma
 				ox = X[sl]; ov = V[sl]; ou = U[sl]; oy = Y[sl]; zs = S[sl];
 				if (DUAL) { ox2 = X2[sl]; oy2 = Y2[sl]; }
 			}
-			int8_t xt1 = (int8_t)wave_shr1((int)ox), vt1 = (int8_t)wave_shr1((int)ov);
-			int8_t x2t1 = DUAL ? (int8_t)wave_shr1((int)ox2) : 0;
-			if (lane == 0) { xt1 = cx; vt1 = cv; x2t1 = cx2; }
+			int8_t xt1 = (int8_t)wave_shr1_fill((int)ox, cx), vt1 = (int8_t)wave_shr1_fill((int)ov, cv);
+			int8_t x2t1 = DUAL ? (int8_t)wave_shr1_fill((int)ox2, cx2) : 0;
 			if (!DUAL && base == st && lane >= 1 && lane <= 3) {
 				// _mm_cvtsi32_si128 of a negative int8 also sets bytes 1..3 (ksw2_extz2_sse.c:138-139)
 				if (x1 < 0) xt1 = (int8_t)0xff;
@@ -205,233 +529,80 @@ __device__ void run_job(const DpArgs &a, int jid, uint8_t *lds, const int8_t *ma
 			cx = (int8_t)__builtin_amdgcn_readlane((int)ox, 63); cv = (int8_t)__builtin_amdgcn_readlane((int)ov, 63);
 			if (DUAL) cx2 = (int8_t)__builtin_amdgcn_readlane((int)ox2, 63);
 			if (!valid) continue;
-			uint8_t d = 0;
-			if (DUAL) {
-				int8_t zz = zs;
-				int8_t A = (int8_t)(xt1 + vt1), B = (int8_t)(oy + ou), A2 = (int8_t)(x2t1 + vt1), B2 = (int8_t)(oy2 + ou);
-				if (!with_cigar) {
-					zz = smax8(zz, A); zz = smax8(zz, B); zz = smax8(zz, A2); zz = smax8(zz, B2);
-				} else if (!right) {
-					d = A > zz ? 1 : 0;   zz = smax8(zz, A);
-					d = B > zz ? 2 : d;   zz = smax8(zz, B);
-					d = A2 > zz ? 3 : d;  zz = smax8(zz, A2);
-					d = B2 > zz ? 4 : d;  zz = smax8(zz, B2);
-				} else {
-					d = zz > A ? 0 : 1;   zz = smax8(zz, A);
-					d = zz > B ? d : 2;   zz = smax8(zz, B);
-					d = zz > A2 ? d : 3;  zz = smax8(zz, A2);
-					d = zz > B2 ? d : 4;  zz = smax8(zz, B2);
-				}
-				zz = smin8(zz, sc_mch);
-				U[sl] = (int8_t)(zz - vt1);
-				V[sl] = (int8_t)(zz - ou);
-				int8_t tmp = (int8_t)(zz - q);
-				A = (int8_t)(A - tmp); B = (int8_t)(B - tmp);
-				tmp = (int8_t)(zz - q2);
-				A2 = (int8_t)(A2 - tmp); B2 = (int8_t)(B2 - tmp);
-				if (!with_cigar || !right) {
-					X[sl]  = (int8_t)((A > 0 ? A : 0) - qe8);
-					Y[sl]  = (int8_t)((B > 0 ? B : 0) - qe8);
-					X2[sl] = (int8_t)((A2 > 0 ? A2 : 0) - qe28);
-					Y2[sl] = (int8_t)((B2 > 0 ? B2 : 0) - qe28);
-					d |= (A > 0 ? 0x08 : 0) | (B > 0 ? 0x10 : 0) | (A2 > 0 ? 0x20 : 0) | (B2 > 0 ? 0x40 : 0);
-				} else {
-					X[sl]  = (int8_t)((A >= 0 ? A : 0) - qe8);
-					Y[sl]  = (int8_t)((B >= 0 ? B : 0) - qe8);
-					X2[sl] = (int8_t)((A2 >= 0 ? A2 : 0) - qe28);
-					Y2[sl] = (int8_t)((B2 >= 0 ? B2 : 0) - qe28);
-					d |= (A >= 0 ? 0x08 : 0) | (B >= 0 ? 0x10 : 0) | (A2 >= 0 ? 0x20 : 0) | (B2 >= 0 ? 0x40 : 0);
-				}
-			} else {
-				const uint8_t uvt1 = (uint8_t)vt1, uut = (uint8_t)ou;
-				int8_t zz = (int8_t)((uint8_t)zs + (uint8_t)qe2x);
-				int8_t A = (int8_t)((uint8_t)xt1 + uvt1), B = (int8_t)((uint8_t)oy + uut);
-				if (!with_cigar) {
-					zz = smax8(zz, A);
-				} else if (!right) {
-					d = A > zz ? 1 : 0;
-					zz = smax8(zz, A);
-					d = B > zz ? 2 : d;
-				} else {
-					d = zz > A ? 0 : 1;
-					zz = smax8(zz, A);
-					d = zz > B ? d : 2;
-				}
-				uint8_t zu = (uint8_t)zz > (uint8_t)B ? (uint8_t)zz : (uint8_t)B;
-				zu = zu < (uint8_t)max_sc_ ? zu : (uint8_t)max_sc_;
-				U[sl] = (int8_t)(uint8_t)(zu - uvt1);
-				V[sl] = (int8_t)(uint8_t)(zu - uut);
-				const int8_t zq = (int8_t)(uint8_t)(zu - (uint8_t)q);
-				A = (int8_t)(A - zq); B = (int8_t)(B - zq);
-				if (!with_cigar || !right) {
-					X[sl] = A > 0 ? A : 0; Y[sl] = B > 0 ? B : 0;
-					d |= (A > 0 ? 0x08 : 0) | (B > 0 ? 0x10 : 0);
-				} else {
-					X[sl] = A >= 0 ? A : 0; Y[sl] = B >= 0 ? B : 0;
-					d |= (A >= 0 ? 0x08 : 0) | (B >= 0 ? 0x10 : 0);
-				}
-			}
-			if (with_cigar) prow[t - st] = d;
+			int8_t nU, nV, nX, nY, nX2, nY2;
+			const uint8_t d = dp_cell<DUAL>(P, xt1, vt1, x2t1, ou, oy, oy2, zs, nU, nV, nX, nY, nX2, nY2);
+			U[sl] = nU; V[sl] = nV; X[sl] = nX; Y[sl] = nY;
+			if (DUAL) { X2[sl] = nX2; Y2[sl] = nY2; }
+			if (P.with_cigar) prow[t - st] = d;
 		}
 		__syncthreads();
 
 		bool stop = false;
-		if (!approx_max) { // exact max over the 32-bit H row
-			int32_t max_H, max_t, Hen0, Hst0;
-			if (r > 0) {
+		if (!P.approx_max) { // exact max over the 32-bit H row
+			int32_t max_H, Hen0, Hst0;
+			int max_t;
+			if (r > 0 && hsmall && en0 - st0 <= 246) {
+				// one reduction over (H + 2^23) << 8 | rank
 				const int en1 = st0 + ((en0 - st0) >> 2 << 2);
-				uint32_t bh = 0, bp = 0;         // this lane's best (H ^ sign bit, priority)
+				uint32_t bk = 0;
 				int32_t h_first = 0;
 				for (int t = st0 + lane; t < en0; t += 64) {
 					const int sl = t & CM;
-					const int32_t h = H[sl] + (DUAL ? (int32_t)V[sl] : (int32_t)(uint8_t)V[sl] - qe);
+					const int32_t h = H[sl] + h_term<DUAL>(P, V[sl]);
 					H[sl] = h;
 					if (t == st0) h_first = h;
-					const uint32_t pri = t < en1
-						? (0x80000000u | ((uint32_t)(3 - ((t - st0) & 3)) << 28) | (0x0FFFFFFFu - (uint32_t)((t - st0) >> 2)))
-						: (0x7FFFFFFFu - (uint32_t)(t - en1));
-					const uint32_t hk = (uint32_t)h ^ 0x80000000u;
-					if (hk > bh || (hk == bh && pri > bp)) bh = hk, bp = pri;
+					const uint32_t k = argmax_key8(h, t, st0, en1);
+					bk = k > bk ? k : bk;
 				}
 				int32_t h_en0 = 0;
 				if (lane == 0) {
 					const int sl = en0 & CM;
-					if (DUAL) h_en0 = en0 > 0 ? Hprev + (int32_t)U[sl] : Hprev + (int32_t)V[sl];
-					else h_en0 = en0 > 0 ? Hprev + (int32_t)(uint8_t)U[sl] - qe : Hprev + (int32_t)(uint8_t)V[sl] - qe;
+					h_en0 = Hprev + h_term<DUAL>(P, en0 > 0 ? U[sl] : V[sl]);
 					H[sl] = h_en0;
-					const uint32_t hk = (uint32_t)h_en0 ^ 0x80000000u;
-					if (hk > bh || (hk == bh && 0xFFFFFFFFu > bp)) bh = hk, bp = 0xFFFFFFFFu;
+					const uint32_t k = ((uint32_t)(h_en0 + (1 << 23)) << 8) | 255u;
+					bk = k > bk ? k : bk;
 				}
-				const uint32_t mh = wave_max_u32(bh);
-				const uint32_t pri = wave_max_u32(bh == mh ? bp : 0u);
-				max_H = (int32_t)(mh ^ 0x80000000u);
-				if (pri == 0xFFFFFFFFu) max_t = en0;
-				else if (pri & 0x80000000u) max_t = st0 + (int)(0x0FFFFFFFu - (pri & 0x0FFFFFFFu)) * 4 + (3 - (int)((pri >> 28) & 3));
-				else max_t = en1 + (int)(0x7FFFFFFFu - pri);
+				argmax_decode8(wave_max_u32(bk), st0, en0, en1, max_H, max_t);
+				Hen0 = __builtin_amdgcn_readlane(h_en0, 0);
+				Hst0 = en0 > st0 ? __builtin_amdgcn_readlane(h_first, 0) : Hen0;
+			} else 			if (r > 0) {
+				const int en1 = st0 + ((en0 - st0) >> 2 << 2);
+				uint32_t bh = 0, bp = 0;
+				int32_t h_first = 0;
+				for (int t = st0 + lane; t < en0; t += 64) {
+					const int sl = t & CM;
+					const int32_t h = H[sl] + h_term<DUAL>(P, V[sl]);
+					H[sl] = h;
+					if (t == st0) h_first = h;
+					argmax_take(bh, bp, h, argmax_key(t, st0, en1));
+				}
+				int32_t h_en0 = 0;
+				if (lane == 0) {
+					const int sl = en0 & CM;
+					h_en0 = Hprev + h_term<DUAL>(P, en0 > 0 ? U[sl] : V[sl]);
+					H[sl] = h_en0;
+					argmax_take(bh, bp, h_en0, 0xFFFFFFFFu);
+				}
+				argmax_reduce(bh, bp, st0, en0, en1, max_H, max_t);
 				Hen0 = __builtin_amdgcn_readlane(h_en0, 0);
 				Hst0 = en0 > st0 ? __builtin_amdgcn_readlane(h_first, 0) : Hen0;
 			} else {
-				max_H = DUAL ? (int32_t)V[0] - qe0 : (int32_t)(uint8_t)V[0] - qe - qe;
+				max_H = DUAL ? (int32_t)V[0] - P.qe0 : (int32_t)(uint8_t)V[0] - P.qe - P.qe;
 				max_t = 0;
 				if (lane == 0) H[0] = max_H;
 				Hen0 = Hst0 = max_H;
 			}
-			if (en0 == tlen - 1 && Hen0 > z.mte) z.mte = Hen0, z.mte_q = r - en0;
-			if (r - st0 == qlen - 1 && Hst0 > z.mqe) z.mqe = Hst0, z.mqe_t = st0;
-			if (apply_zdrop(z, max_H, r, max_t, zdrop, DUAL ? e2 : e)) stop = true;
-			else {
-				if (r == qlen + tlen - 2 && en0 == tlen - 1) z.score = Hen0;
-				if (DUAL && (flag & 0x40)) { // ksw2_extd2_sse.c:361-367
-					const int32_t rH = last_max_H > max_H ? last_max_H : max_H;
-					const int32_t rq = qlen - (r - st0), rt = tlen - en0;
-					const int32_t rm = rq >= tlen - st0 ? tlen - st0 : rt >= qlen - (r - en0) ? qlen - (r - en0) : tlen + qlen - 1 - r;
-					if (rH + rm * max_sc + end_bonus < z.max) stop = true;
-				}
-				last_max_H = max_H;
-			}
-		} else { // approximate max (ksw2_extd2_sse.c:369-385 / ksw2_extz2_sse.c:262-278)
-			if (r > 0) {
-				const bool in0 = last_H0_t >= st0 && last_H0_t <= en0, in1 = last_H0_t + 1 >= st0 && last_H0_t + 1 <= en0;
-				if (in0 && in1) {
-					const int32_t d0 = DUAL ? (int32_t)V[last_H0_t & CM] : (int32_t)(uint8_t)V[last_H0_t & CM] - qe;
-					const int32_t d1 = DUAL ? (int32_t)U[(last_H0_t + 1) & CM] : (int32_t)(uint8_t)U[(last_H0_t + 1) & CM] - qe;
-					if (d0 > d1) H0 += d0;
-					else H0 += d1, ++last_H0_t;
-				} else if (in0) {
-					H0 += DUAL ? (int32_t)V[last_H0_t & CM] : (int32_t)(uint8_t)V[last_H0_t & CM] - qe;
-				} else {
-					++last_H0_t;
-					H0 += DUAL ? (int32_t)U[last_H0_t & CM] : (int32_t)(uint8_t)U[last_H0_t & CM] - qe;
-				}
-				if (!DUAL && (flag & 0x10) && apply_zdrop(z, H0, r, last_H0_t, zdrop, e)) stop = true;
-			} else {
-				H0 = DUAL ? (int32_t)V[0] - qe0 : (int32_t)(uint8_t)V[0] - qe - qe;
-				last_H0_t = 0;
-			}
-			if (!stop && DUAL && (flag & 0x10) && apply_zdrop(z, H0, r, last_H0_t, zdrop, e2)) stop = true;
-			if (!stop && r == qlen + tlen - 2 && en0 == tlen - 1) z.score = H0;
+			stop = finish_exact<DUAL>(P, z, r, st0, en0, max_H, max_t, Hen0, Hst0, last_max_H);
+		} else {
+			stop = finish_approx<DUAL>(P, z, r, st0, en0, H0, last_H0_t, [&](int t) { return V[t & CM]; },
+			                           [&](int t) { return U[t & CM]; });
 		}
 		__syncthreads();
 		if (stop) break;
 		last_st = st, last_en = en;
 	}
-
-	// backtrack (ksw2_extd2_sse.c:391-402, ksw2.h:140-172)
-	uint32_t *cg = a.cigar + jb.cigar_off;
-	int i0 = -1, j0 = -1;
-	if (!skip && with_cigar) {
-		if (!z.zdropped && !(flag & 0x40)) i0 = tlen - 1, j0 = qlen - 1;
-		else if ((flag & 0x40) && z.mqe + end_bonus > z.max) z.reach_end = 1, i0 = z.mqe_t, j0 = qlen - 1;
-		else if (z.max_t >= 0 && z.max_q >= 0) i0 = z.max_t, j0 = z.max_q;
-	}
-	if (i0 >= 0 || j0 >= 0) {
-		uint8_t *stg = lds;
-		const int K = (12 * CAP) / nb;
-		int chunk_lo = 0x7fffffff;
-		int i = i0, j = j0, state = 0, last_op = -1;
-		uint32_t cur = 0;
-		while (i >= 0 && j >= 0) {
-			const int r = i + j;
-			int s0 = 0, e0 = tlen - 1;
-			if (s0 < r - qlen + 1) s0 = r - qlen + 1;
-			if (e0 > r) e0 = r;
-			if (s0 < (r - w + 1) >> 1) s0 = (r - w + 1) >> 1;
-			if (e0 > (r + w) >> 1) e0 = (r + w) >> 1;
-			const int off = s0 >> 4 << 4, off_end = ((e0 + 16) >> 4 << 4) - 1;
-			int force = -1;
-			if (i < off) force = 2;
-			if (i > off_end) force = 1;
-			uint32_t tmp = 0;
-			if (force < 0) {
-				if (r < chunk_lo) {
-					const int lo = r - K + 1 > 0 ? r - K + 1 : 0;
-					const int bytes = (r - lo + 1) * nb;
-					__syncthreads();
-					for (int o = lane * 16; o < bytes; o += 64 * 16)
-						*(uint4*)(stg + o) = *(const uint4*)(slab + (size_t)lo * nb + o);
-					__syncthreads();
-					chunk_lo = lo;
-				}
-				tmp = stg[(r - chunk_lo) * nb + i - off];
-			}
-			if (state == 0) state = tmp & 7;
-			else if (!(tmp >> (state + 2) & 1)) state = 0;
-			if (state == 0) state = tmp & 7;
-			if (force >= 0) state = force;
-			int op;
-			if (state == 0) op = 0, --i, --j;
-			else if (state == 1 || state == 3) op = 2, --i;
-			else op = 1, --j;
-			if (op == last_op) cur += 16;
-			else {
-				if (last_op >= 0 && lane == 0) cg[n_cigar - 1] = cur;
-				++n_cigar, last_op = op, cur = 16u | (uint32_t)op;
-			}
-		}
-		if (i >= 0) {
-			if (last_op == 2) cur += (uint32_t)(i + 1) << 4;
-			else { if (last_op >= 0 && lane == 0) cg[n_cigar - 1] = cur; ++n_cigar, last_op = 2, cur = (uint32_t)(i + 1) << 4 | 2; }
-		}
-		if (j >= 0) {
-			if (last_op == 1) cur += (uint32_t)(j + 1) << 4;
-			else { if (last_op >= 0 && lane == 0) cg[n_cigar - 1] = cur; ++n_cigar, last_op = 1, cur = (uint32_t)(j + 1) << 4 | 1; }
-		}
-		if (last_op >= 0 && lane == 0) cg[n_cigar - 1] = cur;
-		if (!(flag & 0x80)) {
-			__syncthreads();
-			for (int k = lane; k < n_cigar / 2; k += 64) {
-				const uint32_t t0 = cg[k], t1 = cg[n_cigar - 1 - k];
-				cg[k] = t1, cg[n_cigar - 1 - k] = t0;
-			}
-		}
-	}
-	if (lane == 0) {
-		DpRes o;
-		o.max = z.max, o.zdropped = z.zdropped, o.max_q = z.max_q, o.max_t = z.max_t;
-		o.mqe = z.mqe, o.mqe_t = z.mqe_t, o.mte = z.mte, o.mte_q = z.mte_q;
-		o.score = z.score, o.reach_end = z.reach_end, o.n_cigar = n_cigar, o.cells = cells;
-		a.res[jid] = o;
-	}
+	const int n_cigar = backtrack<DUAL>(P, z, jb, a, lds, slab);
+	write_result(a, jid, z, n_cigar, cells);
 	__syncthreads();
 }
 
@@ -445,13 +616,24 @@ __global__ void __launch_bounds__(64) dp_kernel(DpArgs a)
 	uint8_t *slab = a.pbuf + (size_t)blockIdx.x * a.slab;
 	int n = a.n_jobs;
 	if (a.n_jobs_dev) { const unsigned long long nd = *a.n_jobs_dev; n = nd < (unsigned long long)n ? (int)nd : n; }
-	for (int j = blockIdx.x; j < n; j += gridDim.x)
-		run_job<DUAL>(a, j, lds, mat, slab);
+	for (int j = blockIdx.x;;) {
+		if (a.next) { // one job at a time from the shared counter
+			int k = 0;
+			if (threadIdx.x == 0) k = (int)atomicAdd(a.next, 1u);
+			j = __builtin_amdgcn_readfirstlane(k);
+		}
+		if (j >= n) break;
+		const DpJob jb = a.jobs[j];
+		const JobP<DUAL> P = job_params<DUAL>(jb, a.m, mat);
+		if (((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= a.qt_cap) run_job_lds<DUAL, true>(a, j, jb, P, lds, mat, slab);
+		else run_job_lds<DUAL, false>(a, j, jb, P, lds, mat, slab);
+		if (!a.next) j += gridDim.x;
+	}
 }
 
 hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st)
 {
-	const size_t shm = (size_t)12 * a.cap;
+	const size_t shm = (size_t)12 * a.cap + (size_t)a.qt_cap;
 	if (dual) hipLaunchKernelGGL(dp_kernel<true>, dim3(grid), dim3(64), shm, st, a);
 	else hipLaunchKernelGGL(dp_kernel<false>, dim3(grid), dim3(64), shm, st, a);
 	return hipGetLastError();

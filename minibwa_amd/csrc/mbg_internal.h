@@ -61,6 +61,8 @@ struct DpArgs {
 	uint8_t *pbuf;          // backtrack slabs, one per resident block
 	int64_t slab;           // bytes per slab
 	int32_t cap;            // circular-buffer length (power of two)
+	int32_t qt_cap;         // LDS bytes after the 12*cap region for staging a job's query and target
+	unsigned int *next;     // when set: zeroed job counter, jobs are taken one at a time
 	int8_t m;
 	int8_t mat[32];
 };
