@@ -54,10 +54,16 @@ def main():
     ap.add_argument("--ref-len", type=int, default=100_000_000)
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--cpu-pairs", type=int, default=1_000_000, help="read pairs in the CPU-baseline sample")
+    ap.add_argument("--cpu-jobs", type=int, default=200_000, help="jobs in the --dp-batch CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--calib", type=int, default=0, metavar="MiB",
                     help="only run the plain copy kernel over MiB mebibytes (HBM byte-counter calibration)")
+    ap.add_argument("--gather", action="store_true",
+                    help="after building the index, only measure the dependent random 64-B gather ceiling of its occ table")
+    ap.add_argument("--dp-batch", type=int, default=0, metavar="N",
+                    help="only run the extension-kernel batch: N independent (query, target) jobs, query 100-300 bp, "
+                         "2%% substitutions, 0.5%% indels, 50 bp flanks, band 101, gaps 12/2 and 24/1; reports GCUPS")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -74,6 +80,10 @@ def main():
         for _ in range(3):
             mbg.check(ctx.L.mbg_stream_copy(ctx.h, None, mbg.C.c_void_p(b.ptr), mbg.C.c_void_p(a.ptr), n), "mbg_stream_copy")
         print(json.dumps({"calib_copy_bytes": n, "launches": 3}))
+        return
+
+    if args.dp_batch:
+        dp_batch(args)
         return
 
     t0 = time.time()
@@ -95,6 +105,18 @@ def main():
     ctx.idx_load(prefix, 10)
     info = ctx.info()
     log(f"index resident: {info['device_bytes'] / 1e9:.2f} GB")
+    if args.gather:
+        rows = []
+        for chains in (262144, 1048576, 4194304):
+            ctx.gather_bench(chains, 64, 256)
+            ms, _ = ctx.gather_bench(chains, 64, 256)
+            tx = chains * 64
+            rows.append({"chains": chains, "steps": 64, "ms": round(ms, 3), "G_tx_per_s": round(tx / ms / 1e6, 2),
+                         "GB_per_s_64B": round(tx * 64 / ms / 1e6, 1)})
+        print(json.dumps({"gather_ceiling": rows, "occ_table_bytes": int((info["seq_len"] + 127) // 128 * 64),
+                          "ref_bp": args.ref_len}), flush=True)
+        ctx.close()
+        return
 
     t0 = time.time()
     r1, r2 = synth.read_pairs(ref, args.pairs, seed=11, rlen=150, frag_mean=400, frag_sd=50, sub=0.01, indel=0.001)
@@ -175,6 +197,77 @@ def main():
     }
     print(json.dumps(res), flush=True)
     ctx.close()
+
+
+def dp_batch(args):
+    """Extension-kernel batch: one launch of mbg_extd2_batch_dev over N resident jobs per mode, timed by
+    the host around the launch and its completion; cells are the kernel's own band-cell counts."""
+    from minibwa_amd import mbg, synth
+    ctx = mbg.Context(0)
+    mat = np.asarray(mbg.nt4_mat(2, 8, 0, 1), np.int8)
+    out = {"metric": "extension GCUPS, ksw_extd2 band 101, 1M jobs q 100-300 bp", "n_jobs": args.dp_batch, "modes": []}
+    for flag, zdrop, eb, two in ((0x40, 172, 10, False), (0x00, -1, 0, True)):
+        qb, qoff, ql, tb, toff, tl = synth.dp_jobs(args.dp_batch, seed=11, two_sided=two)
+        n = len(ql)
+        jobs = np.zeros(n, mbg.DP_JOB_DTYPE)
+        jobs["qoff"], jobs["toff"], jobs["qlen"], jobs["tlen"] = qoff[:-1], toff[:-1], ql, tl
+        jobs["q"], jobs["e"], jobs["q2"], jobs["e2"] = 12, 2, 24, 1
+        jobs["w"], jobs["zdrop"], jobs["end_bonus"], jobs["flag"] = 101, zdrop, eb, flag
+        span = ql.astype(np.int64) + tl
+        jobs["cigar_off"] = np.concatenate([[0], np.cumsum(span)[:-1]])
+        d_jobs, d_q, d_t = ctx.to_device(jobs), ctx.to_device(qb), ctx.to_device(tb)
+        d_res = ctx.to_device(np.zeros(n, mbg.DP_RES_DTYPE))
+        d_cig = ctx.to_device(np.zeros(int(span.sum()), np.uint32))
+        d_mat = mat
+        times = []
+        for it in range(3):
+            t0 = time.perf_counter()
+            mbg.check(ctx.L.mbg_extd2_batch_dev(ctx.h, None, n, mbg.C.c_void_p(d_jobs.ptr), mbg.C.c_void_p(d_q.ptr),
+                                                mbg.C.c_void_p(d_t.ptr), 5, mbg._ptr(d_mat), mbg.C.c_void_p(d_res.ptr),
+                                                mbg.C.c_void_p(d_cig.ptr), int(ql.max()), int(tl.max()), 101),
+                      "mbg_extd2_batch_dev")
+            res = ctx.to_host(d_res.ptr, mbg.DP_RES_DTYPE, n)      # waits for the kernel
+            times.append(time.perf_counter() - t0)
+        cells = int(res["cells"].astype(np.int64).sum())
+        best = min(times)
+        m = {"flag": flag, "zdrop": zdrop, "end_bonus": eb, "two_sided_flanks": two, "cells": cells,
+             "s": round(best, 4), "GCUPS": round(cells / best / 1e9, 2)}
+        cpu = dp_cpu_sample(jobs, qb, tb, res, mat, min(n, args.cpu_jobs))
+        if cpu:
+            m["cpu_baseline"] = cpu
+        log("dp-batch", json.dumps(m))
+        out["modes"].append(m)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def dp_cpu_sample(jobs, qb, tb, res, mat, n_s):
+    """ksw_extd2_sse of the compiled reference on the first n_s jobs (oracle/_ref/cpu_hotpath)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "cpu_hotpath")
+    if not os.path.exists(exe) or n_s <= 0:
+        return None
+    work = tempfile.mkdtemp(prefix="mbg_dp_")
+    rb, jb = os.path.join(work, "reads.bin"), os.path.join(work, "jobs.bin")
+    with open(rb, "wb") as f:
+        f.write(np.int64(0).tobytes())
+    j = jobs[:n_s]
+    qend = int(j["qoff"][-1] + j["qlen"][-1])
+    tend = int(j["toff"][-1] + j["tlen"][-1])
+    with open(jb, "wb") as f:
+        f.write(np.int64(n_s).tobytes())
+        f.write(np.asarray(mat, np.int8).tobytes())
+        f.write(j.tobytes())
+        f.write(np.int64(qend).tobytes()); f.write(qb[:qend].tobytes())
+        f.write(np.int64(tend).tobytes()); f.write(tb[:tend].tobytes())
+    cores = host_cores()
+    r = subprocess.run([exe, "-", rb, jb, str(cores)], capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log("dp cpu sample failed:", r.stderr[-300:])
+        return None
+    d = json.loads(r.stdout.strip().split("\n")[-1])
+    cells = int(res["cells"][:n_s].astype(np.int64).sum())
+    return {"value": round(cells / d["dp_s"] / 1e9, 3), "unit": "GCUPS", "cores": cores, "kind": "reference",
+            "sample": f"ksw_extd2_sse on the first {n_s} jobs, {cores} threads, {d['dp_s']:.2f}s"}
 
 
 def cpu_baseline(ctx, out, st, reads, prefix, args, work):

@@ -149,7 +149,9 @@ int mbg_extd2_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_jobs, const mbg_
                         mbg_dp_result_t *res, uint32_t *cigar, int32_t max_qlen, int32_t max_tlen, int32_t max_w);
 
 /* Device memory for callers that keep inputs in HBM without another runtime: allocate, free,
- * and synchronous copies in either direction. */
+ * and synchronous copies in either direction.  A copy first waits for the work queued on the
+ * context's own stream (entry points called with stream == NULL); work the caller queued on a
+ * stream of its own is the caller's to wait for. */
 int mbg_dev_alloc(mbg_ctx_t *ctx, size_t bytes, void **ptr);
 int mbg_dev_free(mbg_ctx_t *ctx, void *ptr);
 int mbg_copy_to_device(mbg_ctx_t *ctx, void *dst_dev, const void *src, size_t bytes);

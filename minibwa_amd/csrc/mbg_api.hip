@@ -723,6 +723,7 @@ int mbg_copy_to_device(mbg_ctx_t *c, void *dst, const void *src, size_t bytes)
 	if (bytes == 0) return MBG_OK;
 	std::lock_guard<std::mutex> lk(c->mu);
 	HIPCHK(hipSetDevice(c->device));
+	HIPCHK(hipStreamSynchronize(c->stream));     // ordered after the context stream's queued work
 	HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
 	return MBG_OK;
 }
@@ -733,6 +734,7 @@ int mbg_copy_to_host(mbg_ctx_t *c, void *dst, const void *src, size_t bytes)
 	if (bytes == 0) return MBG_OK;
 	std::lock_guard<std::mutex> lk(c->mu);
 	HIPCHK(hipSetDevice(c->device));
+	HIPCHK(hipStreamSynchronize(c->stream));     // ordered after the context stream's queued work
 	HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
 	return MBG_OK;
 }

@@ -5,6 +5,7 @@
  *
  *   cpu_hotpath <idx_prefix> <reads.bin> <jobs.bin> <n_threads>
  *
+ * An idx_prefix of "-" skips the index and phase 1 (DP-only timing; reads.bin may hold 0 reads).
  * Phase 1, per sub-batch of 48 reads (sb_seq=24 fragments, map-main.c:163-174) on a work-stealing
  * counter like kt_for: mb_seed_intv_batch (seed.c:56-103), then mb_anchor per read (seed.c:268-339:
  * interval sort/dedup, SA locate through mb_bwt_sa_batch, contig mapping, anchor sort/dedup).
@@ -135,8 +136,10 @@ int main(int argc, char *argv[])
 	nt = atoi(argv[4]);
 	if (nt < 1) nt = 1;
 	if (nt > 256) nt = 256;
-	s.idx = mb_idx_load(argv[1], 0);
-	if (!s.idx) { fprintf(stderr, "cannot load index %s\n", argv[1]); return 1; }
+	if (strcmp(argv[1], "-") != 0) {
+		s.idx = mb_idx_load(argv[1], 0);
+		if (!s.idx) { fprintf(stderr, "cannot load index %s\n", argv[1]); return 1; }
+	}
 	r = (uint8_t*)slurp(argv[2], &rl);
 	jb = (uint8_t*)slurp(argv[3], &jl);
 	if (!r || !jb) { fprintf(stderr, "cannot read inputs\n"); return 1; }
@@ -153,7 +156,7 @@ int main(int argc, char *argv[])
 	memcpy(s.jobs, p, s.n_jobs * sizeof(job_t)); p += s.n_jobs * sizeof(job_t);
 	{ int64_t n; memcpy(&n, p, 8); p += 8; s.qb = p; p += n; memcpy(&n, p, 8); p += 8; s.tb = p; }
 	t0 = now();
-	run(&s, nt, seed_worker);
+	if (s.idx) run(&s, nt, seed_worker);
 	t_seed = now() - t0;
 	t0 = now();
 	run(&s, nt, dp_worker);
@@ -161,6 +164,6 @@ int main(int argc, char *argv[])
 	for (i = 0; i < nt; ++i) n_anchor += s.n_anchor[i];
 	printf("{\"threads\": %d, \"n_reads\": %ld, \"n_jobs\": %ld, \"n_anchors\": %ld, \"seed_anchor_s\": %.6f, \"dp_s\": %.6f}\n",
 	       nt, (long)s.n_reads, (long)s.n_jobs, (long)n_anchor, t_seed, t_dp);
-	mb_idx_destroy(s.idx);
+	if (s.idx) mb_idx_destroy(s.idx);
 	return 0;
 }
