@@ -118,3 +118,34 @@ def test_dp_generic_matrix(gpu_ctx):
             r = OO.ref_extd2(Q[i], T[i], mat, p["q"], p["e"], p["q2"], p["e2"], p["w"], p["zdrop"], p["end_bonus"], p["flag"])
             g = {f: int(out.res[f][i]) for f in FIELDS}
             assert g == {f: r[f] for f in FIELDS} and out.cigar_of(i) == r["cigar"], i
+
+
+def test_extd2_device_entry_point(gpu_ctx):
+    """mbg_extd2_batch_dev on device-resident jobs (launch on the context stream, results read back
+    with mbg_copy_to_host, which must wait for that stream) equals the reference per job."""
+    from oracle import oracle as O
+    from minibwa_amd import mbg, synth
+    qb, qo, ql, tb, to, tl = synth.dp_jobs(3000, seed=23)
+    n = len(ql)
+    jobs = np.zeros(n, mbg.DP_JOB_DTYPE)
+    jobs["qoff"], jobs["toff"], jobs["qlen"], jobs["tlen"] = qo[:-1], to[:-1], ql, tl
+    jobs["q"], jobs["e"], jobs["q2"], jobs["e2"] = 12, 2, 24, 1
+    jobs["w"], jobs["zdrop"], jobs["end_bonus"], jobs["flag"] = 101, 172, 10, 0x40
+    span = ql.astype(np.int64) + tl
+    coff = np.concatenate([[0], np.cumsum(span)[:-1]])
+    jobs["cigar_off"] = coff
+    mat = np.asarray(O.gen_nt4_mat(2, 8, 0, 1), np.int8)
+    d_jobs, d_q, d_t = gpu_ctx.to_device(jobs), gpu_ctx.to_device(qb), gpu_ctx.to_device(tb)
+    d_res = gpu_ctx.to_device(np.zeros(n, mbg.DP_RES_DTYPE))
+    d_cig = gpu_ctx.to_device(np.zeros(int(span.sum()), np.uint32))
+    mbg.check(gpu_ctx.L.mbg_extd2_batch_dev(gpu_ctx.h, None, n, mbg.C.c_void_p(d_jobs.ptr), mbg.C.c_void_p(d_q.ptr),
+                                            mbg.C.c_void_p(d_t.ptr), 5, mbg._ptr(mat), mbg.C.c_void_p(d_res.ptr),
+                                            mbg.C.c_void_p(d_cig.ptr), int(ql.max()), int(tl.max()), 101),
+              "mbg_extd2_batch_dev")
+    res = gpu_ctx.to_host(d_res.ptr, mbg.DP_RES_DTYPE, n)
+    cig = gpu_ctx.to_host(d_cig.ptr, np.uint32, int(span.sum()))
+    for i in range(0, n, 7):
+        r = O.ref_extd2(qb[qo[i]:qo[i + 1]], tb[to[i]:to[i + 1]], mat, 12, 2, 24, 1, 101, 172, 10, 0x40)
+        assert {f: int(res[f][i]) for f in FIELDS} == {f: r[f] for f in FIELDS}, i
+        assert [int(c) for c in cig[coff[i]:coff[i] + res["n_cigar"][i]]] == r["cigar"], i
+    assert (res["cells"] > 0).all()
