@@ -91,13 +91,13 @@ struct JobSink {
 // target bases of concatenated coordinates [gst, gen) read forward or reversed; N runs read as 4
 // (l2b_getseq + l2b_getambi, l2bit.c:52-121)
 struct Window {
-	const L2b *lb;
-	int64_t gst, gen;
-	bool reverse;
-	int64_t a0, a1;            // N runs overlapping the window
+	const uint64_t *pac = nullptr, *ambi = nullptr;
+	int64_t gst = 0, gen = 0;
+	bool reverse = false;
+	int64_t a0 = 0, a1 = 0;    // N runs overlapping the window
 	__device__ void init(const L2b &l, int64_t s, int64_t e, bool rev)
 	{
-		lb = &l, gst = s, gen = e, reverse = rev, a0 = a1 = 0;
+		pac = l.pac, ambi = l.ambi, gst = s, gen = e, reverse = rev, a0 = a1 = 0;
 		if (l.n_ambi == 0 || e <= s) return;
 		int64_t lo = 0, hi = (int64_t)l.n_ambi;
 		while (lo < hi) { int64_t mid = (lo + hi) / 2; if ((int64_t)l.ambi[mid * 2 + 1] > s) hi = mid; else lo = mid + 1; }
@@ -108,8 +108,8 @@ struct Window {
 	{
 		const int64_t p = reverse ? gen - 1 - i : gst + i;
 		for (int64_t a = a0; a < a1; ++a)
-			if ((int64_t)lb->ambi[a * 2] <= p && p < (int64_t)lb->ambi[a * 2 + 1]) return 4;
-		return (uint8_t)(lb->pac[p >> 5] >> ((p & 31) << 1) & 3);
+			if ((int64_t)ambi[a * 2] <= p && p < (int64_t)ambi[a * 2 + 1]) return 4;
+		return (uint8_t)(pac[p >> 5] >> ((p & 31) << 1) & 3);
 	}
 };
 
@@ -138,95 +138,147 @@ __device__ __forceinline__ int max_bw_from_mm(const MapOpt &o, int mm) // align.
 	return max1 > max2 ? max1 : max2;
 }
 
-// one extension (mb_align_pair with KSW_EZ_EXTZ_ONLY, align.c:329-378): fast path or a DP job
-__device__ void emit_ext(const MapOpt &o, const JobSink &js, const QView &q, const Window &t, int tlen, int flag,
-                         uint32_t rid, int side, int rev, int64_t tid, int64_t tpos)
+// one extension (mb_align_pair with KSW_EZ_EXTZ_ONLY, align.c:329-378): the ungapped fast path,
+// else the band (capped by max_bw_from_mm when the shorter side is < 100) of a DP job
+struct ExtPlan {
+	bool job = false, fast = false;
+	int qlen = 0, tlen = 0, w = 0, flag = 0, side = 0;
+	int64_t tpos = 0;
+	QView q{};
+	Window t{};
+};
+
+__device__ __forceinline__ void plan_ext(const MapOpt &o, ExtPlan &p)
 {
-	const int qlen = q.n;
+	const int qlen = p.q.n, tlen = p.tlen;
 	int w = o.ksw_w, n_mm = -1;
+	p.qlen = qlen;
 	if (tlen >= qlen) {
 		int32_t score = 0, mx = 0;
 		for (int j = 0; j < qlen; ++j) {
-			const uint8_t tj = t(j), qj = q(j);
+			const uint8_t tj = p.t(j), qj = p.q(j);
 			const int32_t sc = o.mat[tj * 5 + qj];
 			score += sc;
 			n_mm += (tj > 3 || qj > 3 || sc < 0);
 			if (mx < score) mx = score;
 		}
-		if (n_mm <= 2 && score + o.end_bonus >= mx) { atomicAdd(&js.ctr[3], 1ULL); return; }
+		if (n_mm <= 2 && score + o.end_bonus >= mx) { p.fast = true; return; }
 	}
 	if (n_mm >= 0 && (qlen < tlen ? qlen : tlen) < 100) {
 		const int mbw = max_bw_from_mm(o, n_mm);
 		if (w > mbw + 4) w = mbw + 4;
 	}
-	const unsigned long long j = atomicAdd(&js.ctr[0], 1ULL);
-	const unsigned long long qo = atomicAdd(&js.ctr[1], (unsigned long long)qlen);
-	const unsigned long long to = atomicAdd(&js.ctr[2], (unsigned long long)tlen);
-	const unsigned long long co = atomicAdd(&js.ctr[5], (unsigned long long)(qlen + tlen));
-	if ((int64_t)j >= js.job_cap || (int64_t)(qo + qlen) > js.qcap || (int64_t)(to + tlen) > js.tcap) return;
-	for (int i = 0; i < qlen; ++i) js.qbuf[qo + i] = q(i);
-	for (int i = 0; i < tlen; ++i) js.tbuf[to + i] = t(i);
+	p.w = w;
+	p.job = true;
+}
+
+__device__ __forceinline__ void write_job(const MapOpt &o, const JobSink &js, const ExtPlan &p, uint64_t j, uint64_t qo, uint64_t to,
+                                          uint64_t co, uint32_t rid, int rev, int64_t tid)
+{
+	if ((int64_t)j >= js.job_cap || (int64_t)(qo + p.qlen) > js.qcap || (int64_t)(to + p.tlen) > js.tcap) return;
+	for (int i = 0; i < p.qlen; ++i) js.qbuf[qo + i] = p.q(i);
+	for (int i = 0; i < p.tlen; ++i) js.tbuf[to + i] = p.t(i);
 	DpJob d;
-	d.qoff = (int64_t)qo; d.toff = (int64_t)to; d.qlen = qlen; d.tlen = tlen;
+	d.qoff = (int64_t)qo; d.toff = (int64_t)to; d.qlen = p.qlen; d.tlen = p.tlen;
 	d.q = (int8_t)o.q; d.e = (int8_t)o.e; d.q2 = (int8_t)o.q2; d.e2 = (int8_t)o.e2;
-	d.w = w; d.zdrop = o.zdrop; d.end_bonus = o.end_bonus; d.flag = flag; d.cigar_off = (int64_t)co;
+	d.w = p.w; d.zdrop = o.zdrop; d.end_bonus = o.end_bonus; d.flag = p.flag; d.cigar_off = (int64_t)co;
 	js.jobs[j] = d;
 	JobMeta m;
-	m.rid = rid; m.side = (int8_t)side; m.rev = (int8_t)rev; m.pad = 0; m.tid = (int32_t)tid; m.tpos = tpos;
+	m.rid = rid; m.side = (int8_t)p.side; m.rev = (int8_t)rev; m.pad = 0; m.tid = (int32_t)tid; m.tpos = p.tpos;
 	js.meta[j] = m;
 }
 
-// per read: anchor = longest SMEM at its first located row; left and right extension jobs
+__device__ __forceinline__ uint64_t wave_bcast_u64(uint64_t v)
+{
+	const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, 0), hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), 0);
+	return (uint64_t)hi << 32 | lo;
+}
+
+// per read: anchor = longest SMEM at its first located row; left and right extension jobs.  Output
+// positions come from wave prefix sums and one atomic per counter per wave.
 __global__ void __launch_bounds__(128) ext_jobs_kernel(L2b lb, MapOpt o, int32_t n_reads, const int64_t *off, const int32_t *len,
                                                       const uint8_t *seq, const SeedRec *recs, const uint32_t *first_slot,
                                                       const uint64_t *pos, const unsigned long long *best, JobSink js)
 {
 	const int32_t rid = blockIdx.x * blockDim.x + threadIdx.x;
-	if (rid >= n_reads) return;
-	const uint64_t key = best[rid];
-	if (key == 0) return;
-	const uint32_t rec = 0xffffffffu - (uint32_t)key;
-	const SeedRec s = recs[rec];
-	const uint32_t fs = first_slot[rec];
-	if (fs == 0xffffffffu) return;
-	const int32_t qlen = len[rid];
-	const int32_t qs0 = (int32_t)(s.info >> 32), qe0 = (int32_t)s.info, alen = qe0 - qs0;
-	// l2b_intv2cid (l2bit.c:9-34)
-	const uint64_t st = pos[fs], en = st + (uint64_t)alen, tot = lb.tot_len;
-	if (en > tot * 2 || (st < tot && tot < en)) return;
-	const int rev = st >= tot;
-	const int64_t sp = (int64_t)(st < tot ? st : tot * 2 - en);
-	int64_t lo = 0, hi = (int64_t)lb.n_ctg, tid = -1;
-	while (lo < hi) {
-		const int64_t mid = (lo + hi) / 2;
-		const int64_t co = (int64_t)lb.ctg_off[mid], cl = (int64_t)lb.ctg_len[mid];
-		if (co <= sp && sp < co + cl) { tid = sp + alen <= co + cl ? mid : -1; break; }
-		else if (sp < co) hi = mid; else lo = mid + 1;
+	ExtPlan L, R;
+	bool anchored = false;
+	int rev = 0;
+	int64_t tid = -1;
+	if (rid < n_reads) {
+		const uint64_t key = best[rid];
+		const uint32_t rec = 0xffffffffu - (uint32_t)key;
+		const uint32_t fs = key ? first_slot[rec] : 0xffffffffu;
+		if (fs != 0xffffffffu) {
+			const SeedRec s = recs[rec];
+			const int32_t qlen = len[rid];
+			const int32_t qs0 = (int32_t)(s.info >> 32), qe0 = (int32_t)s.info, alen = qe0 - qs0;
+			// l2b_intv2cid (l2bit.c:9-34)
+			const uint64_t st = pos[fs], en = st + (uint64_t)alen, tot = lb.tot_len;
+			if (!(en > tot * 2 || (st < tot && tot < en))) {
+				rev = st >= tot;
+				const int64_t sp = (int64_t)(st < tot ? st : tot * 2 - en);
+				int64_t lo = 0, hi = (int64_t)lb.n_ctg;
+				while (lo < hi) {
+					const int64_t mid = (lo + hi) / 2;
+					const int64_t co = (int64_t)lb.ctg_off[mid], cl = (int64_t)lb.ctg_len[mid];
+					if (co <= sp && sp < co + cl) { tid = sp + alen <= co + cl ? mid : -1; break; }
+					else if (sp < co) hi = mid; else lo = mid + 1;
+				}
+				if (tid >= 0) {
+					anchored = true;
+					const int64_t coff = (int64_t)lb.ctg_off[tid], clen = (int64_t)lb.ctg_len[tid];
+					const int64_t ts = sp - coff, te = ts + alen;
+					const int32_t qs = rev ? qlen - qe0 : qs0, qe = rev ? qlen - qs0 : qe0;   // on the oriented read
+					const uint8_t *r = seq + off[rid];
+					if (qs > 0 && ts > 0) { // left: reversed query prefix against the reversed target before ts
+						int64_t l = qs;
+						l += l * o.a + o.end_bonus > o.q ? (l * o.a + o.end_bonus - o.q) / o.e : 0;
+						const int64_t ts0 = ts - l > 0 ? ts - l : 0;
+						L.q = QView{r, qlen, 0, rev != 0, true, qs};
+						L.t.init(lb, coff + ts0, coff + ts, true);
+						L.tlen = (int)(ts - ts0), L.flag = 0x40 | 0x02 | 0x80, L.side = 0, L.tpos = ts;
+						plan_ext(o, L);
+					}
+					if (qe < qlen && te < clen) { // right extension
+						int64_t l = qlen - qe;
+						l += l * o.a + o.end_bonus > o.q ? (l * o.a + o.end_bonus - o.q) / o.e : 0;
+						const int64_t te0 = te + l < clen ? te + l : clen;
+						R.q = QView{r, qlen, qe, rev != 0, false, qlen - qe};
+						R.t.init(lb, coff + te, coff + te0, false);
+						R.tlen = (int)(te0 - te), R.flag = 0x40, R.side = 1, R.tpos = te;
+						plan_ext(o, R);
+					}
+				}
+			}
+		}
 	}
-	if (tid < 0) return;
-	atomicAdd(&js.ctr[4], 1ULL);
-	const int64_t coff = (int64_t)lb.ctg_off[tid], clen = (int64_t)lb.ctg_len[tid];
-	const int64_t ts = sp - coff, te = ts + alen;
-	const int32_t qs = rev ? qlen - qe0 : qs0, qe = rev ? qlen - qs0 : qe0;   // on the oriented read
-	const uint8_t *r = seq + off[rid];
-	if (qs > 0 && ts > 0) { // left: reversed query prefix against the reversed target before ts
-		int64_t l = qs;
-		l += l * o.a + o.end_bonus > o.q ? (l * o.a + o.end_bonus - o.q) / o.e : 0;
-		const int64_t ts0 = ts - l > 0 ? ts - l : 0;
-		QView q{r, qlen, 0, rev != 0, true, qs};
-		Window t;
-		t.init(lb, coff + ts0, coff + ts, true);
-		emit_ext(o, js, q, t, (int)(ts - ts0), 0x40 | 0x02 | 0x80, (uint32_t)rid, 0, rev, tid, ts);
+	// wave-aggregated output positions
+	const uint32_t nj = (uint32_t)L.job + (uint32_t)R.job;
+	const uint32_t nq = (L.job ? L.qlen : 0) + (R.job ? R.qlen : 0), nt = (L.job ? L.tlen : 0) + (R.job ? R.tlen : 0);
+	const uint32_t nstat = ((uint32_t)L.fast + (uint32_t)R.fast) | (uint32_t)anchored << 16;
+	uint32_t tj, tq, tt, ts;
+	const uint32_t ej = wave_excl_scan(nj, &tj), eq = wave_excl_scan(nq, &tq), et = wave_excl_scan(nt, &tt);
+	wave_excl_scan(nstat, &ts);
+	uint64_t bj = 0, bq = 0, bt = 0, bc = 0;
+	if (__lane_id() == 0) {
+		if (tj) {
+			bj = atomicAdd(&js.ctr[0], (unsigned long long)tj);
+			bq = atomicAdd(&js.ctr[1], (unsigned long long)tq);
+			bt = atomicAdd(&js.ctr[2], (unsigned long long)tt);
+			bc = atomicAdd(&js.ctr[5], (unsigned long long)tq + tt);
+		}
+		if (ts & 0xffffu) atomicAdd(&js.ctr[3], (unsigned long long)(ts & 0xffffu));
+		if (ts >> 16) atomicAdd(&js.ctr[4], (unsigned long long)(ts >> 16));
 	}
-	if (qe < qlen && te < clen) { // right extension
-		int64_t l = qlen - qe;
-		l += l * o.a + o.end_bonus > o.q ? (l * o.a + o.end_bonus - o.q) / o.e : 0;
-		const int64_t te0 = te + l < clen ? te + l : clen;
-		QView q{r, qlen, qe, rev != 0, false, qlen - qe};
-		Window t;
-		t.init(lb, coff + te, coff + te0, false);
-		emit_ext(o, js, q, t, (int)(te0 - te), 0x40, (uint32_t)rid, 1, rev, tid, te);
+	if (tj == 0) return;
+	bj = wave_bcast_u64(bj), bq = wave_bcast_u64(bq), bt = wave_bcast_u64(bt), bc = wave_bcast_u64(bc);
+	uint64_t j = bj + ej, qo = bq + eq, to = bt + et, co = bc + eq + et;
+	if (L.job) {
+		write_job(o, js, L, j, qo, to, co, (uint32_t)rid, rev, tid);
+		++j, qo += L.qlen, to += L.tlen, co += L.qlen + L.tlen;
 	}
+	if (R.job) write_job(o, js, R, j, qo, to, co, (uint32_t)rid, rev, tid);
 }
 
 hipError_t launch_row_counts(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, uint32_t *cnt, hipStream_t st)
