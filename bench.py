@@ -1,10 +1,11 @@
 """Benchmark of the seeding + extension hot path on one MI355X.
 
-Workload (chromosome-scale seeding + extension): a 100 Mbp seeded synthetic reference (uniform ACGT
-plus 5% repeats: copies of 300-6,000 bp segments at 2% divergence; 4 contigs of 25 Mbp) and 1M read
-pairs 2x150 bp (fragment ~ Normal(400, 50), 1% substitutions, 0.1% indels with geometric length of
-mean 2), splitmix64 seed 11.  The index is built on the device before timing and stays resident in
-HBM, as do the reads.
+Workload (2x150 bp pairs against a 3.1 Gbp reference): a 3.1 Gbp seeded synthetic reference
+(uniform ACGT plus 8% repeats: copies of 300-6,000 bp segments at 2% divergence; 24 contigs of
+129 Mbp) and 1M read pairs 2x150 bp (fragment ~ Normal(400, 50), 1% substitutions, 0.1% indels with
+geometric length of mean 2), splitmix64 seed 11.  The index (6.2e9 BWT rows, about 7 GB) is built on
+the device before timing and stays resident in HBM, as do the reads.  --ref-len 100000000
+--contigs 4 --rep-frac 0.05 gives the 100 Mbp chromosome-scale workload.
 
 One step = one pass of the hot path over the 2M reads: two-pass SMEM seeding, SA locate of every
 interval's anchor rows, left/right extension jobs from each read's longest SMEM with the reference's
@@ -51,7 +52,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--ref-len", type=int, default=100_000_000)
+    ap.add_argument("--ref-len", type=int, default=3_100_000_000)
+    ap.add_argument("--contigs", type=int, default=24)
+    ap.add_argument("--rep-frac", type=float, default=0.08)
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--cpu-pairs", type=int, default=1_000_000, help="read pairs in the CPU-baseline sample")
     ap.add_argument("--cpu-jobs", type=int, default=200_000, help="jobs in the --dp-batch CPU-baseline sample")
@@ -87,7 +90,7 @@ def main():
         return
 
     t0 = time.time()
-    ref = synth.reference(args.ref_len, seed=11, rep_frac=0.05, rep_min=300, rep_max=6000, rep_div=0.02)
+    ref = synth.reference(args.ref_len, seed=11, rep_frac=args.rep_frac, rep_min=300, rep_max=6000, rep_div=0.02)
     log(f"reference {len(ref)} bp in {time.time() - t0:.1f}s")
     work = args.workdir or tempfile.mkdtemp(prefix="mbg_bench_")
     os.makedirs(work, exist_ok=True)
@@ -95,8 +98,8 @@ def main():
     t0 = time.time()
     # the index is built by a child process (torch suffix sort on the GPU); this process only runs libmbg
     r = subprocess.run([sys.executable, "-m", "minibwa_amd.index", "--len", str(args.ref_len), "--seed", "11",
-                        "--rep-frac", "0.05", "--rep-min", "300", "--rep-max", "6000", "--rep-div", "0.02",
-                        "--contigs", "4", "--device", "cuda", "--out", prefix], cwd=ROOT, capture_output=True, text=True)
+                        "--rep-frac", str(args.rep_frac), "--rep-min", "300", "--rep-max", "6000", "--rep-div", "0.02",
+                        "--contigs", str(args.contigs), "--device", "cuda", "--out", prefix], cwd=ROOT, capture_output=True, text=True)
     if r.returncode != 0:
         log("index build failed:", r.stderr[-2000:])
         sys.exit(1)
@@ -158,7 +161,8 @@ def main():
     traffic = None
     # HBM bytes per launch of the same kernel from the committed FETCH_SIZE/WRITE_SIZE profile of this
     # workload (each counter scaled by what it reads for a plain copy of known size)
-    pmc = os.path.join(ROOT, "profiles", "step_cfg1_pmc.json")
+    tag = "cfg1" if args.ref_len == 100_000_000 else "3g" if args.ref_len == 3_100_000_000 else None
+    pmc = os.path.join(ROOT, "profiles", f"step_{tag}_pmc.json") if tag else ""
     if os.path.exists(pmc):
         try:
             ks = json.load(open(pmc))["kernels"]
@@ -184,8 +188,9 @@ def main():
         "vs_baseline": None,
         "dtype": "u64 (FM-index ranks) + int8/int32 (DP)",
         "data": "synthetic (seeded splitmix64; index built on the device)",
-        "config": {"workload": "seed+locate+extend pass, 1M pairs 2x150 bp vs 100 Mbp synthetic reference (5% repeats)",
-                   "ref_bp": args.ref_len, "pairs": args.pairs, "read_len": rl, "contigs": 4,
+        "config": {"workload": f"seed+locate+extend pass, {args.pairs} pairs 2x150 bp vs {args.ref_len / 1e9:g} Gbp synthetic "
+                               f"reference ({args.rep_frac:.0%} repeats)",
+                   "ref_bp": args.ref_len, "pairs": args.pairs, "read_len": rl, "contigs": args.contigs,
                    "min_len": 19, "max_occ": 250, "ksw_w": opt.ksw_w, "zdrop": opt.zdrop},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -191,3 +191,22 @@ def test_synth_deterministic():
     r1, r2 = synth.read_pairs(a, 50, seed=3)
     s1, s2 = synth.read_pairs(a, 50, seed=3)
     assert np.array_equal(r1, s1) and np.array_equal(r2, s2) and r1.shape == (50, 150)
+
+
+@pytest.mark.parametrize("L,rep,n_ctg,chunk", [(32, 0.0, 1, 10), (63, 0.0, 1, 20), (96, 0.0, 1, 50), (97, 0.0, 1, 50),
+                                               (3000, 0.5, 2, 700), (200_000, 0.3, 3, 40_000)])
+def test_index_builder_matches_reference_binary(tmp_path, L, rep, n_ctg, chunk):
+    """The chunked suffix sort (several key-range chunks, repeat-rich texts, last occ block with at
+    most 64 symbols) writes the same .mbw/.l2b bytes as `minibwa index` of the compiled reference."""
+    O = pytest.importorskip("oracle.oracle")
+    if not os.path.exists(O.REF_CLI):
+        pytest.skip("oracle/_ref/minibwa not built")
+    from minibwa_amd import index, synth
+    ref = synth.reference(L, seed=5, rep_frac=rep, rep_min=20 if L < 10000 else 300, rep_max=200 if L < 10000 else 6000,
+                          rep_div=0.02)
+    pre = str(tmp_path / "g")
+    index.build(ref, pre, n_ctg=n_ctg, device="cpu", max_chunk=chunk)
+    synth.write_fasta(str(tmp_path / "r.fa"), ref, n_ctg=n_ctg, prefix="chr")
+    subprocess.run([O.REF_CLI, "index", str(tmp_path / "r.fa"), str(tmp_path / "r")], check=True, capture_output=True)
+    for ext in (".mbw", ".l2b"):
+        assert open(pre + ext, "rb").read() == open(str(tmp_path / "r") + ext, "rb").read(), ext
