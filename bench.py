@@ -62,6 +62,9 @@ def main():
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--calib", type=int, default=0, metavar="MiB",
                     help="only run the plain copy kernel over MiB mebibytes (HBM byte-counter calibration)")
+    ap.add_argument("--serve-qps", type=float, default=4000.0,
+                    help="request rate of the serving phase (Poisson arrivals)")
+    ap.add_argument("--serve-requests", type=int, default=4000, help="requests in the serving phase (0: skip it)")
     ap.add_argument("--gather", action="store_true",
                     help="after building the index, only measure the dependent random 64-B gather ceiling of its occ table")
     ap.add_argument("--dp-batch", type=int, default=0, metavar="N",
@@ -174,6 +177,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ctx, out, per[-1], reads, prefix, args, work)
+    latency = serve(ctx, ref, opt, args) if args.serve_requests > 0 else None
 
     res = {
         "metric": METRIC,
@@ -197,11 +201,140 @@ def main():
                      "kernel": "smem_kernel<1> (SMEM pass 1)", "algorithmic_bytes_per_launch": int(bytes_seed1),
                      "avg_launch_ms": round(ms_seed1, 4)},
         "cpu_baseline": cpu,
+        "latency": latency,
         "phases_ms": phases,
         "step_work": {k: work_stats[k] for k in ("n_rec", "n_rows", "n_anchored", "n_fast", "n_jobs", "cells", "lf_steps")},
     }
     print(json.dumps(res), flush=True)
     ctx.close()
+
+
+def serve(ctx0, ref, opt, args):
+    """Served mapping: requests arrive as a Poisson process at args.serve_qps; a request is 1-10,000
+    read pairs (log-uniform) of the workload's 2x150 bp model, or (5% of requests) one 10 kb read
+    with 0.1% substitutions and 0.05% indels.  Three contexts share the resident index (two for pair
+    requests, one for long reads), each on its own stream.  A free context takes every request of its
+    class that has arrived, in arrival order, up to 65,536 pairs or 64 long reads, so requests that
+    arrive while a batch is on the GPU are admitted by the next free context and overlap with it.
+    Latency = completion of the request's step (results resident in HBM) - its arrival; the request's
+    reads cross PCIe inside it."""
+    import threading
+    from minibwa_amd import mbg, synth
+    rng = np.random.default_rng(29)
+    n_req = args.serve_requests
+    arrive = np.cumsum(rng.exponential(1.0 / args.serve_qps, n_req))
+    size = np.exp(rng.uniform(0.0, np.log(10000.0), n_req)).astype(np.int64)
+    long_req = rng.random(n_req) < 0.05
+    size[long_req] = 1
+    n_pairs = int(size[~long_req].sum())
+    r1, r2 = synth.read_pairs(ref, n_pairs, seed=31)
+    pairs = np.empty((2 * n_pairs, 150), np.uint8)
+    pairs[0::2], pairs[1::2] = r1, r2
+    del r1, r2
+    n_long = int(long_req.sum())
+    lseq, loff, llen = synth.single_reads(ref, n_long, seed=37, lmin=10000, lmax=10000, sub=0.001, indel=0.0005)
+    assert (llen == 10000).all()
+    first_read = np.zeros(n_req + 1, np.int64)                 # start of each pair request in `pairs` (reads)
+    first_read[1:] = np.cumsum(np.where(long_req, 0, 2 * size))
+    long_idx = np.cumsum(long_req) - 1
+    cap_pairs, cap_long = 65536, 64
+    opt_long = mbg.adaptive_opt(10000)
+    queues = {False: np.nonzero(~long_req)[0], True: np.nonzero(long_req)[0]}
+
+    class Worker:
+        def __init__(self, ctx, is_long):
+            self.ctx, self.is_long = ctx, is_long
+            n = cap_long * 10000 if is_long else 2 * cap_pairs * 150
+            self.seq = ctx.to_device(np.zeros(n, np.uint8))
+            if is_long:
+                self.off = ctx.to_device(np.arange(cap_long + 1, dtype=np.int64) * 10000)
+                self.len = ctx.to_device(np.full(cap_long, 10000, np.int32))
+            else:
+                self.off = ctx.to_device(np.arange(2 * cap_pairs + 1, dtype=np.int64) * 150)
+                self.len = ctx.to_device(np.full(2 * cap_pairs, 150, np.int32))
+
+        def step(self, picked):
+            if self.is_long:
+                seq = np.concatenate([lseq[loff[int(long_idx[k])]:loff[int(long_idx[k]) + 1]] for k in picked])
+                self.ctx.write_device(self.seq.ptr, seq)
+                self.ctx.map_step_dev(len(picked), self.off.ptr, self.len.ptr, self.seq.ptr, 10000, opt_long)
+            else:
+                reads = pairs[np.concatenate([np.arange(first_read[k], first_read[k + 1]) for k in picked])]
+                self.ctx.write_device(self.seq.ptr, reads)
+                self.ctx.map_step_dev(len(reads), self.off.ptr, self.len.ptr, self.seq.ptr, 150, opt)
+
+    extra = [mbg.Context(0), mbg.Context(0)]
+    for c in extra:
+        c.idx_attach(ctx0)
+    workers = [Worker(ctx0, False), Worker(extra[0], False), Worker(extra[1], True)]
+    for w in workers:                                          # grow each context's pools before the clock
+        q = queues[w.is_long]
+        if len(q):
+            w.step(list(q[:cap_long]) if w.is_long else [int(k) for k in q[:1]])
+    lock = threading.Lock()
+    head = {False: 0, True: 0}
+    done = np.zeros(n_req)
+    steps = {False: [], True: []}
+    err = []
+    t0 = time.perf_counter() + 0.05
+
+    def run(w):
+        q = queues[w.is_long]
+        try:
+            while True:
+                with lock:
+                    h = head[w.is_long]
+                    if h >= len(q):
+                        return
+                    now = time.perf_counter() - t0
+                    wait = arrive[q[h]] - now
+                    if wait <= 0:
+                        picked, tot = [int(q[h])], int(size[q[h]])
+                        e = h + 1
+                        while e < len(q) and arrive[q[e]] <= now:
+                            if w.is_long and len(picked) >= cap_long:
+                                break
+                            if not w.is_long and tot + size[q[e]] > cap_pairs:
+                                break
+                            picked.append(int(q[e]))
+                            tot += int(size[q[e]])
+                            e += 1
+                        head[w.is_long] = e
+                if wait > 0:
+                    time.sleep(min(wait, 0.0005))
+                    continue
+                ts = time.perf_counter()
+                w.step(picked)
+                t = time.perf_counter()
+                done[picked] = t - t0
+                steps[w.is_long].append((len(picked), tot, (t - ts) * 1e3))
+        except Exception as ex:  # surfaced after the join
+            err.append(ex)
+
+    th = [threading.Thread(target=run, args=(w,)) for w in workers]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for c in extra:
+        c.close()
+    if err:
+        raise err[0]
+    lat = (done - arrive) * 1e3
+    span = done.max() - arrive[0]
+    q = lambda x, p: round(float(np.percentile(x, p)), 3) if len(x) else None
+    sp, sl = np.array(steps[False]).reshape(-1, 3), np.array(steps[True]).reshape(-1, 3)
+    res = {"qps": args.serve_qps, "requests": n_req, "pairs": n_pairs, "long_reads": n_long,
+           "p50_ms": q(lat, 50), "p99_ms": q(lat, 99),
+           "p50_ms_pairs": q(lat[~long_req], 50), "p99_ms_pairs": q(lat[~long_req], 99),
+           "p50_ms_long": q(lat[long_req], 50), "p99_ms_long": q(lat[long_req], 99),
+           "served_pairs_per_s": round(n_pairs / span, 1), "offered_pairs_per_s": round(n_pairs / arrive[-1], 1),
+           "pair_batches": len(sp), "mean_batch_pairs": round(float(sp[:, 1].mean()), 1) if len(sp) else None,
+           "mean_pair_step_ms": round(float(sp[:, 2].mean()), 3) if len(sp) else None,
+           "long_batches": len(sl), "mean_long_step_ms": round(float(sl[:, 2].mean()), 3) if len(sl) else None,
+           "contexts": len(workers), "note": "latency = arrival -> step complete, H2D of the request's reads included"}
+    log("serving:", json.dumps(res))
+    return res
 
 
 def dp_batch(args):

@@ -96,6 +96,11 @@ int mbg_idx_load(mbg_ctx_t *ctx, const char *prefix, int32_t pre_len);
 int mbg_bwt_upload(mbg_ctx_t *ctx, const mbg_bwt_desc_t *bwt, int32_t pre_len);
 int mbg_l2b_upload(mbg_ctx_t *ctx, const mbg_l2b_desc_t *l2b);
 int mbg_idx_info(mbg_ctx_t *ctx, mbg_idx_info_t *info);
+/* Use the index resident in src (same device) instead of a copy of one's own, so several contexts
+ * (each with its own stream and pools) serve from one index in HBM.  src keeps ownership: it must
+ * outlive ctx and must not load another index while ctx uses it.  No counterpart in the reference,
+ * whose index is shared by reference between threads (map-algo.c, mb_map_batch). */
+int mbg_idx_attach(mbg_ctx_t *ctx, mbg_ctx_t *src);
 /* Device address of the k-mer cache (4^pre_len mbg_sai_t), for checking it against the host's. */
 int mbg_pre_copy(mbg_ctx_t *ctx, mbg_sai_t *out, uint64_t n);
 

@@ -291,6 +291,29 @@ int mbg_idx_info(mbg_ctx_t *c, mbg_idx_info_t *info)
 	return MBG_OK;
 }
 
+int mbg_idx_attach(mbg_ctx_t *c, mbg_ctx_t *src)
+{
+	if (c == nullptr || src == nullptr || c == src) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	Idx ix;
+	L2b lb;
+	bool has_bwt, has_l2b;
+	{
+		std::lock_guard<std::mutex> ls(src->mu);
+		if (!src->has_bwt) return MBG_ENOIDX;
+		if (src->device != c->device) return MBG_EARG;
+		ix = src->ix, lb = src->lb, has_bwt = src->has_bwt, has_l2b = src->has_l2b;
+	}
+	HIPCHK(hipSetDevice(c->device));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	free_bwt(c);
+	free_l2b(c);
+	// the device arrays stay owned by src (c->d_* remain null, so c never frees them)
+	c->ix = ix, c->lb = lb, c->has_bwt = has_bwt, c->has_l2b = has_l2b;
+	c->idx_bytes = 0;
+	return MBG_OK;
+}
+
 int mbg_pre_copy(mbg_ctx_t *c, mbg_sai_t *out, uint64_t n)
 {
 	if (c == nullptr || out == nullptr) return MBG_EARG;

@@ -147,6 +147,7 @@ def lib():
         L.mbg_bwt_upload.argtypes = [_P, C.POINTER(BwtDesc), C.c_int32]
         L.mbg_l2b_upload.argtypes = [_P, C.POINTER(L2bDesc)]
         L.mbg_idx_info.argtypes = [_P, C.POINTER(IdxInfo)]
+        L.mbg_idx_attach.argtypes = [_P, _P]
         L.mbg_pre_copy.argtypes = [_P, _P, C.c_uint64]
         L.mbg_rank2a.argtypes = [_P, C.c_int64, _P, _P, _P, _P]
         L.mbg_extend.argtypes = [_P, C.c_int64, _P, _P, _P]
@@ -177,7 +178,7 @@ EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mb
             "mbg_l2b_upload", "mbg_idx_info", "mbg_pre_copy", "mbg_rank2a", "mbg_extend", "mbg_seed_batch",
             "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
             "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
-            "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench")
+            "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench", "mbg_idx_attach")
 JOB_META_DTYPE = np.dtype([("rid", "<u4"), ("side", "i1"), ("rev", "i1"), ("pad", "<i2"), ("tid", "<i4"), ("tpos", "<i8")],
                           align=True)
 assert JOB_META_DTYPE.itemsize == 24
@@ -261,6 +262,10 @@ class Context:
     def idx_load(self, prefix: str, pre_len: int = 10):
         check(self.L.mbg_idx_load(self.h, prefix.encode(), pre_len), f"mbg_idx_load({prefix})")
         return self.info()
+
+    def idx_attach(self, src: "Context"):
+        """Serve from the index resident in `src` (which must outlive this context)."""
+        check(self.L.mbg_idx_attach(self.h, src.h), "mbg_idx_attach")
 
     def bwt_upload(self, primary, L2, seq_len, data: np.ndarray, sa_bit, sa: np.ndarray, pre_len=10):
         data = np.ascontiguousarray(data, np.uint64)
@@ -364,6 +369,25 @@ class Context:
         if a.nbytes:
             check(self.L.mbg_copy_to_device(self.h, C.c_void_p(p.value), _ptr(a), a.nbytes), "mbg_copy_to_device")
         return d
+
+    def write_device(self, dev_addr, a: np.ndarray):
+        """Copy a host array into existing device memory at dev_addr."""
+        a = np.ascontiguousarray(a)
+        if a.nbytes:
+            check(self.L.mbg_copy_to_device(self.h, C.c_void_p(dev_addr), _ptr(a), a.nbytes), "mbg_copy_to_device")
+
+    def step_results(self, st: MapStats, out: MapOut) -> dict:
+        """Host copies of one step's outputs: seed records, located rows (and their records), the
+        extension jobs with their metadata, DP results and CIGAR slots."""
+        n_jobs = int(st.n_jobs)
+        r = {"recs": self.to_host(out.recs, SEED_REC_DTYPE, st.n_rec),
+             "pos": self.to_host(out.pos, np.uint64, st.n_rows),
+             "pos_rec": self.to_host(out.pos_rec, np.uint32, st.n_rows),
+             "jobs": self.to_host(out.jobs, DP_JOB_DTYPE, n_jobs),
+             "meta": self.to_host(out.meta, JOB_META_DTYPE, n_jobs),
+             "res": self.to_host(out.res, DP_RES_DTYPE, n_jobs)}
+        r["cigar"] = self.to_host(out.cigar, np.uint32, int(st.q_bytes) + int(st.t_bytes))
+        return r
 
     def gather_bench(self, n_chains, steps, block_threads=256):
         """(ms, checksum) of the dependent random 64-byte gather over the resident occ table."""
