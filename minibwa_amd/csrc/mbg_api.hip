@@ -44,7 +44,7 @@ struct mbg_ctx_s {
 	DBuf h_pinned;
 	// seed -> locate -> extend step
 	DBuf p_recs, p_rows, p_rowrec, p_first, p_best, p_jobs, p_meta, p_qb, p_tb, p_res, p_cig, p_ctr;
-	DBuf s_slots1, s_slots2, s_cnt, s_off, s_spill, s_tmp, r_cnt, r_off;
+	DBuf s_slots1, s_slots2, s_cnt, s_off, s_spill, s_tmp, r_cnt, r_off, s_pack, s_qflag;
 	int64_t cap_task = 0, cap_spill = 0;
 	int64_t cap_rec = 0, cap_rows = 0;
 	hipEvent_t ev[8] = {};
@@ -143,7 +143,7 @@ void mbg_destroy(mbg_ctx_t *c)
 	dfree(c->b_ctr); dfree(c->b_task); dfree(c->b_slab);
 	for (DBuf *b : {&c->p_recs, &c->p_rows, &c->p_rowrec, &c->p_first, &c->p_best, &c->p_jobs, &c->p_meta, &c->p_qb, &c->p_tb,
 	                &c->p_res, &c->p_cig, &c->p_ctr, &c->s_slots1, &c->s_slots2, &c->s_cnt, &c->s_off, &c->s_spill, &c->s_tmp,
-	                &c->r_cnt, &c->r_off}) dfree(*b);
+	                &c->r_cnt, &c->r_off, &c->s_pack, &c->s_qflag}) dfree(*b);
 	for (hipEvent_t &e : c->ev) if (e) hipEventDestroy(e);
 	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
 	if (c->stream) hipStreamDestroy(c->stream);
@@ -397,6 +397,8 @@ static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_
 	TRY(dgrow(c->s_cnt, (size_t)n_cnt * 4)); TRY(dgrow(c->s_off, (size_t)n_cnt * 4));
 	TRY(dgrow(c->s_spill, (size_t)sc * sizeof(SeedRec)));
 	TRY(dgrow(c->b_task, (size_t)tc * sizeof(SeedTask)));
+	TRY(dgrow(c->s_pack, (size_t)n_reads * kPackWords * 4 + 64));
+	TRY(dgrow(c->s_qflag, (size_t)n_reads * 4 + 64));
 	size_t tmp_bytes = 0;
 	HIPCHK(scan_u32(nullptr, &tmp_bytes, nullptr, nullptr, n_cnt, st));
 	TRY(dgrow(c->s_tmp, tmp_bytes));
@@ -406,8 +408,11 @@ static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_
 	uint32_t *cnt = (uint32_t*)c->s_cnt.p;
 	HIPCHK(hipMemsetAsync(ctr, 0, 64, st));
 	HIPCHK(hipMemsetAsync(cnt, 0, (size_t)n_cnt * 4, st));
+	HIPCHK(hipMemsetAsync(c->s_qflag.p, 0, (size_t)n_reads * 4 + 4, st));
+	HIPCHK(launch_pack_reads(seq, off, len, n_reads, (uint32_t*)c->s_pack.p, (uint32_t*)c->s_qflag.p, st));
 	SeedArgs a{};
 	a.seq = seq; a.off = off; a.len = len; a.n_tasks = (uint32_t)n_reads; a.min_len = min_len; a.next = ctr;
+	a.packed = (const uint32_t*)c->s_pack.p; a.qflag = (const uint32_t*)c->s_qflag.p;
 	a.slots = (SeedRec*)c->s_slots1.p; a.slot_cnt = cnt; a.kslot = kSlot1;
 	a.spill = (SeedRec*)c->s_spill.p; a.n_spill = n_spill; a.spill_cap = sc;
 	a.stats = stats;

@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(256) extend_kernel(Idx ix, int64_t n, const Sa
 // ---------------------------------------------------------------------------------------------
 enum : int { ACT_DONE = 0, ACT_LOOKUP = 1, ACT_BACK = 2, ACT_FWD = 3 };
 constexpr int kSmemBlock = 256;
-constexpr int kQWords = 24;            // up to 384 bases per lane in LDS
+constexpr int kQWords = kPackWords;    // up to 384 bases per lane in LDS
 
 struct Lane {
 	const uint8_t *q;                  // task bases in global memory (absolute positions)
@@ -141,27 +141,46 @@ __device__ __forceinline__ int control(const Idx &ix, const SeedArgs &a, Lane &s
 	}
 }
 
-// copy the task window [st, en) into this lane's LDS slots (2-bit) unless it holds N or is long
-__device__ __forceinline__ void stage_query(Lane &s, uint32_t *qs, int st)
+// Pack every read into 2-bit words once per batch, one thread per (read, 16-base word), so that a
+// lane starting a task copies a few vector loads into LDS instead of re-reading its bases byte by
+// byte.  qflag (zeroed before) marks reads the packed form cannot hold: an N, or more than 384 bases.
+__global__ void __launch_bounds__(256) pack_reads_kernel(const uint8_t *seq, const int64_t *off, const int32_t *len, int32_t n_reads,
+                                                        uint32_t *packed, uint32_t *qflag)
 {
-	const int n = s.en - st;
-	s.qbase = st;
-	s.gq = n > kQWords * 16;
-	if (s.gq) return;
-	const uint8_t *src = s.q + st;
-	bool has_n = false;
-	for (int w = 0; w < (n + 15) >> 4; ++w) {
-		uint32_t word = 0;
-		#pragma unroll
-		for (int j = 0; j < 16; ++j) {
-			const int k = (w << 4) + j;
-			const uint32_t c = k < n ? src[k] : 0;
-			has_n |= c > 3;
-			word |= (c & 3) << (j << 1);
-		}
-		qs[w * kSmemBlock + threadIdx.x] = word;
+	const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int64_t r = g / kPackWords;
+	const int w = (int)(g - r * kPackWords);
+	if (r >= n_reads) return;
+	const int32_t n = len[r];
+	if (n > kPackWords * 16) {
+		if (w == 0) atomicOr(&qflag[r], 1u);
+		return;
 	}
-	s.gq = has_n;
+	const uint8_t *src = seq + off[r] + w * 16;
+	const int m = n - w * 16;
+	uint32_t word = 0;
+	bool has_n = false;
+	#pragma unroll
+	for (int j = 0; j < 16; ++j) {
+		const uint32_t c = j < m ? src[j] : 0;
+		has_n |= c > 3;
+		word |= (c & 3) << (j << 1);
+	}
+	packed[r * kPackWords + w] = word;
+	if (has_n) atomicOr(&qflag[r], 1u);
+}
+
+// this lane's read into its LDS slots (the whole read, base 0), or the global path for reads with
+// an N or longer than the LDS holds
+__device__ __forceinline__ void stage_query(const SeedArgs &a, Lane &s, uint32_t *qs, int32_t rlen)
+{
+	s.qbase = 0;
+	s.gq = a.packed == nullptr || a.qflag[s.rid] != 0;
+	if (s.gq) return;
+	const uint32_t *src = a.packed + (uint64_t)s.rid * kPackWords;
+	const int nw = (rlen + 15) >> 4;
+	#pragma unroll 1
+	for (int w = 0; w < nw; ++w) qs[w * kSmemBlock + threadIdx.x] = src[w];
 }
 
 template <int PASS>
@@ -191,15 +210,17 @@ __global__ void __launch_bounds__(kSmemBlock) smem_kernel(Idx ix, SeedArgs a)
 				break;
 			}
 			int st;
+			int32_t rlen;
 			if (pass == 1) {
 				s.rid = t;
 				s.q = a.seq + a.off[t];
-				st = 0; s.en = a.len[t];
+				st = 0; s.en = rlen = a.len[t];
 				s.min_len = a.min_len; s.min_occ = 1;
 			} else {
 				const SeedTask tk = a.tasks[t];
 				s.rid = tk.rid;
 				s.q = a.seq + a.off[tk.rid];
+				rlen = a.len[tk.rid];
 				st = tk.st; s.en = tk.en;
 				s.min_len = tk.min_len; s.min_occ = (uint64_t)(int64_t)tk.min_occ;
 			}
@@ -208,7 +229,7 @@ __global__ void __launch_bounds__(kSmemBlock) smem_kernel(Idx ix, SeedArgs a)
 			s.x = st;
 			s.stage = 1;
 			live = true;
-			stage_query(s, qs, st);
+			stage_query(a, s, qs, rlen);
 		}
 		act = control(ix, a, s, qs, pass);
 		if (act == ACT_DONE) {
@@ -432,6 +453,15 @@ hipError_t launch_stream_copy(void *dst, const void *src, int64_t n16, unsigned 
 // launchers
 // ---------------------------------------------------------------------------------------------
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_pack_reads(const uint8_t *seq, const int64_t *off, const int32_t *len, int32_t n_reads, uint32_t *packed,
+                             uint32_t *qflag, hipStream_t st)
+{
+	const int64_t n = (int64_t)n_reads * kPackWords;
+	if (n <= 0) return hipSuccess;
+	hipLaunchKernelGGL(pack_reads_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, seq, off, len, n_reads, packed, qflag);
+	return hipGetLastError();
+}
 
 hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st)
 {

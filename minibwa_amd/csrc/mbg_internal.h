@@ -12,6 +12,8 @@ struct SeedArgs {
 	const uint8_t *seq;     // concatenated nt4 reads
 	const int64_t *off;     // read i at seq + off[i]
 	const int32_t *len;
+	const uint32_t *packed; // read i's bases as 2-bit codes at packed + i*kPackWords (16 a word), or null
+	const uint32_t *qflag;  // read i: nonzero when packed[] does not hold it (an N, or longer than 384)
 	const SeedTask *tasks;  // pass 2 only
 	uint32_t n_tasks;
 	const uint32_t *n_tasks_dev;  // pass 2: task count produced on the device (clamped to task_cap)
@@ -28,6 +30,8 @@ struct SeedArgs {
 	int64_t spill_cap;
 	unsigned long long *stats;  // optional: [0] occ blocks fetched, [1] k-mer cache lookups, [2] extend steps
 };
+
+constexpr int kPackWords = 24;          // 2-bit words per packed read: reads up to 384 bases
 
 struct L2b {
 	const uint64_t *pac;
@@ -82,6 +86,8 @@ hipError_t launch_sa(const Idx &ix, int64_t n, const int64_t *n_dev, uint64_t *x
                      unsigned grid, hipStream_t st);
 hipError_t launch_getseq(const L2b &lb, int64_t n, const int64_t *tid, const int64_t *st_, const int64_t *en_, const int64_t *out_off,
                          uint8_t *out, int64_t *len, hipStream_t st);
+hipError_t launch_pack_reads(const uint8_t *seq, const int64_t *off, const int32_t *len, int32_t n_reads, uint32_t *packed,
+                             uint32_t *qflag, hipStream_t st);
 hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st);
 hipError_t launch_gather_chain(const Idx &ix, int64_t n_chains, int32_t steps, int32_t bs, unsigned long long *sum, hipStream_t st);
 hipError_t launch_stream_copy(void *dst, const void *src, int64_t n16, unsigned grid, hipStream_t st);
