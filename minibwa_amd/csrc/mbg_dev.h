@@ -130,23 +130,83 @@ __device__ __forceinline__ Sai set_intv(const Idx &ix, int c)
 	return p;
 }
 
+// Occurrences of base c and of all bases greater than c among $-removed symbols
+// [blk*128, blk*128 + in] plus everything before the block: the two sums one child of an extension
+// needs (its own rank, and the sizes of the children ordered before it on the other strand).
+struct CG { uint64_t c, gt; };
+
+__device__ __forceinline__ CG occ_cg(const uint64_t b[8], uint32_t in, int c)
+{
+	constexpr uint64_t K5 = 0x5555555555555555ULL;
+	const int half = in >= 64;
+	const int n = (int)(in & 63) + 1;
+	const uint64_t w0 = half ? b[6] : b[4], w1 = half ? b[7] : b[5];
+	const int n0 = n > 32 ? 32 : n, n1 = n > 32 ? n - 32 : 0;
+	const uint64_t m0 = n0 >= 32 ? ~0ULL : ((1ULL << (2 * n0)) - 1), m1 = n1 >= 32 ? ~0ULL : ((1ULL << (2 * n1)) - 1);
+	const uint64_t pat = (uint64_t)c * K5;
+	uint64_t e0 = ~(w0 ^ pat), e1 = ~(w1 ^ pat);
+	e0 &= e0 >> 1, e1 &= e1 >> 1;
+	const uint64_t h0 = w0 >> 1 & K5, l0 = w0 & K5, h1 = w1 >> 1 & K5, l1 = w1 & K5;
+	const uint64_t g0 = c == 0 ? (h0 | l0) : c == 1 ? h0 : c == 2 ? (h0 & l0) : 0;
+	const uint64_t g1 = c == 0 ? (h1 | l1) : c == 1 ? h1 : c == 2 ? (h1 & l1) : 0;
+	const uint32_t nc = (uint32_t)(__popcll(e0 & K5 & m0) + __popcll(e1 & K5 & m1));
+	const uint32_t ng = (uint32_t)(__popcll(g0 & m0) + __popcll(g1 & m1));
+	const uint64_t p1 = (b[1] & kM56) + (half ? (b[1] >> 56) : 0), p2 = (b[2] & kM56) + (half ? (b[2] >> 56) : 0);
+	const uint64_t p3 = (b[3] & kM56) + (half ? (b[3] >> 56) : 0);
+	const uint64_t p0 = (b[0] & kM56) + (half ? (b[0] >> 56) : 0);
+	CG r;
+	r.c = pick4(c, p0, p1, p2, p3) + nc;
+	r.gt = (c == 0 ? p1 + p2 + p3 : c == 1 ? p2 + p3 : c == 2 ? p3 : 0) + ng;
+	return r;
+}
+
+// rank2a restricted to (base c, bases > c); the same block fetches and the same l-k==1 quirk at
+// k == primary (bwt.c:205-209) as rank2a above
+__device__ __forceinline__ int rank_cg(const Idx &ix, uint64_t k, uint64_t l, int c, CG &gk, CG &gl)
+{
+	int nblk = 0;
+	uint64_t b[8];
+	uint64_t blk = ~0ULL;
+	gk = CG{0, 0};
+	if (k != 0) {
+		uint64_t j = k - 1;
+		j -= (j >= ix.primary);
+		blk = j >> 7;
+		load_block(ix.data, blk, b);
+		++nblk;
+		gk = occ_cg(b, (uint32_t)(j & 127), c);
+	}
+	if (l == k + 1 && k != 0 && k == ix.primary) {
+		const int b0 = bwt_b0(ix, k);
+		gl = CG{gk.c + (b0 == c), gk.gt + (b0 > c)};
+		return nblk;
+	}
+	gl = CG{0, 0};
+	if (l != 0) {
+		uint64_t j = l - 1;
+		j -= (j >= ix.primary);
+		if ((j >> 7) != blk) load_block(ix.data, j >> 7, b), ++nblk;
+		gl = occ_cg(b, (uint32_t)(j & 127), c);
+	}
+	return nblk;
+}
+
 // mb_bwt_extend (bwt.c:237-250) restricted to the one base c the caller keeps (selects instead of
-// indexing, so nothing is addressed through scratch memory).
+// indexing, so nothing is addressed through scratch memory).  The cumulative x[is_back] of child c
+// is x[is_back] + [$ in range] + the sizes of the children > c, i.e. the rank difference of "> c".
 __device__ __forceinline__ Sai extend1(const Idx &ix, const Sai &ik, int is_back, int c, int *nblk = nullptr)
 {
 	const uint64_t xf = is_back ? ik.x0 : ik.x1;   // x[!is_back]
 	const uint64_t xg = is_back ? ik.x1 : ik.x0;   // x[is_back]
-	C4 tk, tl;
-	const int nb = rank2a(ix, xf, xf + ik.size, tk, tl);
+	CG tk, tl;
+	const int nb = rank_cg(ix, xf, xf + ik.size, c, tk, tl);
 	if (nblk) *nblk += nb;
-	const uint64_t s0 = tl.a - tk.a, s1 = tl.c - tk.c, s2 = tl.g - tk.g, s3 = tl.t - tk.t;
-	const uint64_t acc = xg + (uint64_t)(xf <= ix.primary && xf + ik.size > ix.primary)
-	                   + (c < 3 ? s3 : 0) + (c < 2 ? s2 : 0) + (c < 1 ? s1 : 0);
+	const uint64_t acc = xg + (uint64_t)(xf <= ix.primary && xf + ik.size > ix.primary) + (tl.gt - tk.gt);
 	Sai o;
-	const uint64_t nf = pick4(c, ix.L2[0], ix.L2[1], ix.L2[2], ix.L2[3]) + 1 + pick4(c, tk.a, tk.c, tk.g, tk.t);
+	const uint64_t nf = pick4(c, ix.L2[0], ix.L2[1], ix.L2[2], ix.L2[3]) + 1 + tk.c;
 	o.x0 = is_back ? nf : acc;
 	o.x1 = is_back ? acc : nf;
-	o.size = pick4(c, s0, s1, s2, s3);
+	o.size = tl.c - tk.c;
 	o.info = 0;
 	return o;
 }
