@@ -92,11 +92,21 @@ __device__ __forceinline__ void emit(const SeedArgs &a, Lane &s, uint32_t pass)
 	++s.nemit;
 }
 
+// the cache key of bases [from-len+1, from]: base `from` in the top two bits, base from-len+1 in the
+// lowest (bwt.c:259-261,392-393) -- exactly the little-endian 2-bit field the LDS words hold for those
+// bases, so from LDS it is one extraction from at most two words.
 __device__ __forceinline__ uint32_t kmer_at(const Lane &s, const uint32_t *qs, int from, int len)
 {
-	uint32_t z = 0;
-	for (int l = 0; l < len; ++l) z = z << 2 | (uint32_t)qget(s, qs, from - l);
-	return z;
+	if (s.gq || len > 16) {
+		uint32_t z = 0;
+		for (int l = 0; l < len; ++l) z = z << 2 | (uint32_t)qget(s, qs, from - l);
+		return z;
+	}
+	const int lo = from - len + 1 - s.qbase;                   // first base, >= 0
+	const int w = lo >> 4, sh = (lo & 15) << 1;
+	const uint64_t pair = (uint64_t)qs[w * kSmemBlock + threadIdx.x] |
+	                      (sh + 2 * len > 32 ? (uint64_t)qs[(w + 1) * kSmemBlock + threadIdx.x] << 32 : 0);
+	return (uint32_t)(pair >> sh) & (uint32_t)((1ULL << (2 * len)) - 1);
 }
 
 // Cheap transitions until the lane needs memory (bwt.c:381-459 minus the rank/lookup steps).
