@@ -144,3 +144,37 @@ def test_seed_cfg0(gpu_ctx):
     r = ri.seed_batch(reads)
     for i in range(len(reads)):
         assert np.array_equal(_canon(g[i]), _canon(r[i])), i
+
+
+@pytest.mark.parametrize("pre_len", [0, 12, 14])
+def test_seed_cache_length_invariant(chrm_prefix, ref_chrm, chrm_reads, pre_len):
+    """Seeding with no k-mer cache or a longer one than the reference's 10 gives the reference's
+    intervals: a cached interval is the state stepwise extension reaches, and one below min_occ
+    falls back to stepwise (bwt.c:403-406).  On chrM most 12- and 14-mers are absent, so the
+    fallback is taken often; pass 2 (min_occ = size+1) exercises it with larger thresholds."""
+    from minibwa_amd import mbg
+    reads = chrm_reads[:600]
+    with mbg.Context(0) as c:
+        c.idx_load(chrm_prefix, pre_len)
+        for min_len, mso in ((19, 10), (13, 30)):
+            g = c.seed_intv_batch(reads, min_len, mso)
+            r = ref_chrm.seed_batch(reads, min_len, mso)
+            for i in range(len(reads)):
+                assert np.array_equal(_canon(g[i]), _canon(r[i])), (pre_len, min_len, i)
+
+
+def test_seed_cfg0_cache14(gpu_ctx):
+    """Config-0 reads against its index with a 14-mer cache."""
+    from oracle import oracle as O
+    from minibwa_amd import mbg, synth
+    pre = os.path.join(GOLDEN, "cfg0")
+    ref = synth.reference(16569, seed=11, rep_frac=0.0)
+    r1, r2 = synth.read_pairs(ref, 1000, seed=13)
+    reads = list(r1) + list(r2)
+    ri = O.RefIndex(pre + ".mbw")
+    with mbg.Context(0) as c:
+        c.idx_load(pre, 14)
+        g = c.seed_intv_batch(reads)
+    r = ri.seed_batch(reads)
+    for i in range(len(reads)):
+        assert np.array_equal(_canon(g[i]), _canon(r[i])), i
