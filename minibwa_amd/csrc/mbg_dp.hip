@@ -242,6 +242,69 @@ __device__ __forceinline__ uint8_t dp_cell(const JobP<DUAL> &P, int8_t xt1, int8
 	return d;
 }
 
+// the cell of dp_cell for jobs with a CIGAR, the alignment side fixed at compile time (RIGHT: the
+// KSW_EZ_RIGHT tie rule, ksw2_extd2_sse.c:233-265) and both branches written as selects
+template <bool DUAL, bool RIGHT>
+__device__ __forceinline__ uint8_t dp_cell_cg(const JobP<DUAL> &P, int8_t xt1, int8_t vt1, int8_t x2t1, int8_t ou, int8_t oy,
+                                              int8_t oy2, int8_t zs, int8_t &nU, int8_t &nV, int8_t &nX, int8_t &nY,
+                                              int8_t &nX2, int8_t &nY2)
+{
+	uint8_t d;
+	if (DUAL) {
+		int8_t zz = zs;
+		int8_t A = (int8_t)(xt1 + vt1), B = (int8_t)(oy + ou), A2 = (int8_t)(x2t1 + vt1), B2 = (int8_t)(oy2 + ou);
+		if (!RIGHT) {
+			d = A > zz ? 1 : 0;   zz = smax8(zz, A);
+			d = B > zz ? 2 : d;   zz = smax8(zz, B);
+			d = A2 > zz ? 3 : d;  zz = smax8(zz, A2);
+			d = B2 > zz ? 4 : d;  zz = smax8(zz, B2);
+		} else {
+			d = zz > A ? 0 : 1;   zz = smax8(zz, A);
+			d = zz > B ? d : 2;   zz = smax8(zz, B);
+			d = zz > A2 ? d : 3;  zz = smax8(zz, A2);
+			d = zz > B2 ? d : 4;  zz = smax8(zz, B2);
+		}
+		zz = smin8(zz, P.sc_mch);
+		nU = (int8_t)(zz - vt1);
+		nV = (int8_t)(zz - ou);
+		int8_t tmp = (int8_t)(zz - P.q);
+		A = (int8_t)(A - tmp); B = (int8_t)(B - tmp);
+		tmp = (int8_t)(zz - P.q2);
+		A2 = (int8_t)(A2 - tmp); B2 = (int8_t)(B2 - tmp);
+		const bool pA = RIGHT ? A >= 0 : A > 0, pB = RIGHT ? B >= 0 : B > 0;
+		const bool pA2 = RIGHT ? A2 >= 0 : A2 > 0, pB2 = RIGHT ? B2 >= 0 : B2 > 0;
+		nX  = (int8_t)((pA ? A : 0) - P.qe8);
+		nY  = (int8_t)((pB ? B : 0) - P.qe8);
+		nX2 = (int8_t)((pA2 ? A2 : 0) - P.qe28);
+		nY2 = (int8_t)((pB2 ? B2 : 0) - P.qe28);
+		d |= (pA ? 0x08 : 0) | (pB ? 0x10 : 0) | (pA2 ? 0x20 : 0) | (pB2 ? 0x40 : 0);
+	} else {
+		const uint8_t uvt1 = (uint8_t)vt1, uut = (uint8_t)ou;
+		int8_t zz = (int8_t)((uint8_t)zs + (uint8_t)P.qe2x);
+		int8_t A = (int8_t)((uint8_t)xt1 + uvt1), B = (int8_t)((uint8_t)oy + uut);
+		if (!RIGHT) {
+			d = A > zz ? 1 : 0;
+			zz = smax8(zz, A);
+			d = B > zz ? 2 : d;
+		} else {
+			d = zz > A ? 0 : 1;
+			zz = smax8(zz, A);
+			d = zz > B ? d : 2;
+		}
+		uint8_t zu = (uint8_t)zz > (uint8_t)B ? (uint8_t)zz : (uint8_t)B;
+		zu = zu < (uint8_t)P.max_sc_ ? zu : (uint8_t)P.max_sc_;
+		nU = (int8_t)(uint8_t)(zu - uvt1);
+		nV = (int8_t)(uint8_t)(zu - uut);
+		const int8_t zq = (int8_t)(uint8_t)(zu - (uint8_t)P.q);
+		A = (int8_t)(A - zq); B = (int8_t)(B - zq);
+		nX2 = 0, nY2 = 0;
+		const bool pA = RIGHT ? A >= 0 : A > 0, pB = RIGHT ? B >= 0 : B > 0;
+		nX = pA ? A : 0; nY = pB ? B : 0;
+		d |= (pA ? 0x08 : 0) | (pB ? 0x10 : 0);
+	}
+	return d;
+}
+
 // H increment carried by a new v (cells t < en0) or u (cell en0) value
 template <bool DUAL>
 __device__ __forceinline__ int32_t h_term(const JobP<DUAL> &P, int8_t x)
@@ -437,12 +500,18 @@ __device__ __forceinline__ void write_result(const DpArgs &a, int jid, const DpS
 	}
 }
 
-// ---- LDS schedule ------------------------------------------------------------------------------
+// ---- the row loop --------------------------------------------------------------------------------
+//
+// One pass per 64-cell block of [st, hi], hi = min(en + 15, padded tlen - 1): indices entering the
+// window take their initial values, the score row is refreshed on [st0, fresh_hi], the cells of
+// [st, en] are computed and the exact-max H row and argmax keys updated, all with per-lane selects and
+// one predicated store, so a row costs one trip per block instead of separate passes with barriers.
+// A single wave owns the job, so its LDS reads see its earlier writes in program order.
 
 // STAGED: the job's query and target are copied into LDS after the 12*cap state region first
 template <bool DUAL, bool STAGED>
 __device__ __forceinline__ void run_job_lds(const DpArgs &a, int jid, const DpJob &jb, const JobP<DUAL> P, uint8_t *lds, const int8_t *mat,
-                            uint8_t *slab)
+                                            uint8_t *slab)
 {
 	const int lane = threadIdx.x;
 	const int CAP = a.cap, CM = CAP - 1;
@@ -470,54 +539,49 @@ __device__ __forceinline__ void run_job_lds(const DpArgs &a, int jid, const DpJo
 		const int st = st0 >> 4 << 4, en = ((en0 + 16) >> 4 << 4) - 1;
 		int hi = en + 15;
 		if (hi > P.tl_hi) hi = P.tl_hi;
-		if (hi > hw) {
-			for (int idx = hw + 1 + lane; idx <= hi; idx += 64) {
-				const int sl = idx & CM;
-				U[sl] = P.ini; V[sl] = P.ini; X[sl] = P.ini; Y[sl] = P.ini;
-				if (DUAL) { X2[sl] = P.ini2; Y2[sl] = P.ini2; }
-				S[sl] = 0;
-				H[sl] = kNegInf;
-			}
-			hw = hi;
-			__syncthreads();
-		}
-		int8_t x1, x21, v1;
+		const int hw0 = hw;                          // indices above hw0 enter the window in this row
+		if (hi > hw) hw = hi;
+		// cell st-1 of the previous row, or the boundary constants (ksw2_extd2_sse.c:141-155)
+		int8_t x1, v1, x21;
 		if (st > 0 && st - 1 >= last_st && st - 1 <= last_en) {
 			const int sl = (st - 1) & CM;
 			x1 = X[sl], v1 = V[sl], x21 = DUAL ? X2[sl] : 0;
 		} else row_boundary<DUAL>(P, r, st, x1, v1, x21);
-		if (en >= r && lane == 0) {
-			int8_t y, u, y2;
-			row_yu<DUAL>(P, r, y, u, y2);
-			const int sl = r & CM;
-			Y[sl] = y, U[sl] = u;
-			if (DUAL) Y2[sl] = y2;
-		}
-		// scores (loop fission in the reference): 16-cell groups from st0, padding past tlen reads 0
-		if (!P.generic) {
-			int shi = st0 + ((en0 - st0) >> 4 << 4) + 15;
-			if (shi > P.tl_hi) shi = P.tl_hi;
-			for (int t = st0 + lane; t <= shi; t += 64) {
-				const int8_t ta = t < tlen ? (int8_t)T[t] : 0, qa = t <= r ? (int8_t)Q[r - t] : 0;
-				S[t & CM] = (ta == P.m1 || qa == P.m1) ? P.sc_N : ta == qa ? P.sc_mch : P.sc_mis;
-			}
-		} else {
-			for (int t = st0 + lane; t <= en0; t += 64)
-				S[t & CM] = mat[T[t] * P.m + Q[r - t]];
-		}
-		__syncthreads();
-
-		// core: every cell of [st, en]
-		const int32_t Hprev = en0 > 0 ? H[(en0 - 1) & CM] : H[en0 & CM];
+		const int ep = en0 > 0 ? en0 - 1 : en0;
+		const int32_t Hprev = ep > hw0 ? kNegInf : H[ep & CM];   // H[en0-1] before this row's update
+		int8_t yr, ur, y2r;
+		row_yu<DUAL>(P, r, yr, ur, y2r);
+		const bool set_r = en >= r;
+		int shi = st0 + ((en0 - st0) >> 4 << 4) + 15;
+		if (shi > P.tl_hi) shi = P.tl_hi;
+		const int fresh_hi = P.generic ? en0 : shi;  // the score row is rewritten on [st0, fresh_hi]
+		const int en1 = st0 + ((en0 - st0) >> 2 << 2);
+		const bool exact = !P.approx_max && r > 0;
+		const bool packed = exact && hsmall && en0 - st0 <= 246;
+		uint32_t bh = 0, bp = 0;                      // best key (packed) or (H, priority) of this lane
+		int32_t h_en0 = 0, h_st0 = 0;
 		uint8_t *prow = slab + (size_t)r * P.nb;
-		int8_t cx = x1, cv = v1, cx2 = x21;
-		for (int base = st; base <= en; base += 64) {
+		int cx = x1, cv = v1, cx2 = x21;
+		for (int base = st; base <= hi; base += 64) {
 			const int t = base + lane, sl = t & CM;
-			const bool valid = t <= en;
-			int8_t ox = 0, ov = 0, ox2 = 0, ou = 0, oy = 0, oy2 = 0, zs = 0;
-			if (valid) {
-				ox = X[sl]; ov = V[sl]; ou = U[sl]; oy = Y[sl]; zs = S[sl];
-				if (DUAL) { ox2 = X2[sl]; oy2 = Y2[sl]; }
+			const bool entering = t > hw0;
+			int8_t ox = X[sl], ov = V[sl], ou = U[sl], oy = Y[sl], zs = S[sl];
+			int8_t ox2 = DUAL ? X2[sl] : 0, oy2 = DUAL ? Y2[sl] : 0;
+			int32_t oh = H[sl];
+			if (entering) {
+				ox = ov = ou = oy = P.ini;
+				ox2 = oy2 = P.ini2;
+				zs = 0;
+				oh = kNegInf;
+			}
+			if (set_r && t == r) { oy = yr; ou = ur; oy2 = y2r; }
+			if (t >= st0 && t <= fresh_hi) {
+				if (!P.generic) {
+					const int8_t ta = t < tlen ? (int8_t)T[t] : 0, qa = t <= r ? (int8_t)Q[r - t] : 0;
+					zs = (ta == P.m1 || qa == P.m1) ? P.sc_N : ta == qa ? P.sc_mch : P.sc_mis;
+				} else {
+					zs = mat[T[t] * P.m + Q[r - t]];
+				}
 			}
 			int8_t xt1 = (int8_t)wave_shr1_fill((int)ox, cx), vt1 = (int8_t)wave_shr1_fill((int)ov, cv);
 			int8_t x2t1 = DUAL ? (int8_t)wave_shr1_fill((int)ox2, cx2) : 0;
@@ -526,66 +590,43 @@ __device__ __forceinline__ void run_job_lds(const DpArgs &a, int jid, const DpJo
 				if (x1 < 0) xt1 = (int8_t)0xff;
 				if (v1 < 0) vt1 = (int8_t)0xff;
 			}
-			cx = (int8_t)__builtin_amdgcn_readlane((int)ox, 63); cv = (int8_t)__builtin_amdgcn_readlane((int)ov, 63);
-			if (DUAL) cx2 = (int8_t)__builtin_amdgcn_readlane((int)ox2, 63);
-			if (!valid) continue;
+			cx = __builtin_amdgcn_readlane((int)ox, 63);
+			cv = __builtin_amdgcn_readlane((int)ov, 63);
+			if (DUAL) cx2 = __builtin_amdgcn_readlane((int)ox2, 63);
+			const bool core = t <= en;
 			int8_t nU, nV, nX, nY, nX2, nY2;
 			const uint8_t d = dp_cell<DUAL>(P, xt1, vt1, x2t1, ou, oy, oy2, zs, nU, nV, nX, nY, nX2, nY2);
-			U[sl] = nU; V[sl] = nV; X[sl] = nX; Y[sl] = nY;
-			if (DUAL) { X2[sl] = nX2; Y2[sl] = nY2; }
-			if (P.with_cigar) prow[t - st] = d;
+			int32_t nh = oh;
+			if (exact && t >= st0 && t <= en0) {
+				nh = t < en0 ? oh + h_term<DUAL>(P, nV) : Hprev + h_term<DUAL>(P, en0 > 0 ? nU : nV);
+				if (packed) {
+					const uint32_t k = t < en0 ? argmax_key8(nh, t, st0, en1) : (((uint32_t)(nh + (1 << 23)) << 8) | 255u);
+					bh = k > bh ? k : bh;
+				} else {
+					argmax_take(bh, bp, nh, t < en0 ? argmax_key(t, st0, en1) : 0xFFFFFFFFu);
+				}
+				if (t == en0) h_en0 = nh;
+				if (t == st0) h_st0 = nh;
+			}
+			if (t <= hi) {
+				U[sl] = core ? nU : ou; V[sl] = core ? nV : ov; X[sl] = core ? nX : ox; Y[sl] = core ? nY : oy;
+				if (DUAL) { X2[sl] = core ? nX2 : ox2; Y2[sl] = core ? nY2 : oy2; }
+				S[sl] = zs;
+				H[sl] = nh;
+				if (core && P.with_cigar) prow[t - st] = d;
+			}
 		}
-		__syncthreads();
+		__builtin_amdgcn_wave_barrier();
 
 		bool stop = false;
 		if (!P.approx_max) { // exact max over the 32-bit H row
 			int32_t max_H, Hen0, Hst0;
 			int max_t;
-			if (r > 0 && hsmall && en0 - st0 <= 246) {
-				// one reduction over (H + 2^23) << 8 | rank
-				const int en1 = st0 + ((en0 - st0) >> 2 << 2);
-				uint32_t bk = 0;
-				int32_t h_first = 0;
-				for (int t = st0 + lane; t < en0; t += 64) {
-					const int sl = t & CM;
-					const int32_t h = H[sl] + h_term<DUAL>(P, V[sl]);
-					H[sl] = h;
-					if (t == st0) h_first = h;
-					const uint32_t k = argmax_key8(h, t, st0, en1);
-					bk = k > bk ? k : bk;
-				}
-				int32_t h_en0 = 0;
-				if (lane == 0) {
-					const int sl = en0 & CM;
-					h_en0 = Hprev + h_term<DUAL>(P, en0 > 0 ? U[sl] : V[sl]);
-					H[sl] = h_en0;
-					const uint32_t k = ((uint32_t)(h_en0 + (1 << 23)) << 8) | 255u;
-					bk = k > bk ? k : bk;
-				}
-				argmax_decode8(wave_max_u32(bk), st0, en0, en1, max_H, max_t);
-				Hen0 = __builtin_amdgcn_readlane(h_en0, 0);
-				Hst0 = en0 > st0 ? __builtin_amdgcn_readlane(h_first, 0) : Hen0;
-			} else 			if (r > 0) {
-				const int en1 = st0 + ((en0 - st0) >> 2 << 2);
-				uint32_t bh = 0, bp = 0;
-				int32_t h_first = 0;
-				for (int t = st0 + lane; t < en0; t += 64) {
-					const int sl = t & CM;
-					const int32_t h = H[sl] + h_term<DUAL>(P, V[sl]);
-					H[sl] = h;
-					if (t == st0) h_first = h;
-					argmax_take(bh, bp, h, argmax_key(t, st0, en1));
-				}
-				int32_t h_en0 = 0;
-				if (lane == 0) {
-					const int sl = en0 & CM;
-					h_en0 = Hprev + h_term<DUAL>(P, en0 > 0 ? U[sl] : V[sl]);
-					H[sl] = h_en0;
-					argmax_take(bh, bp, h_en0, 0xFFFFFFFFu);
-				}
-				argmax_reduce(bh, bp, st0, en0, en1, max_H, max_t);
-				Hen0 = __builtin_amdgcn_readlane(h_en0, 0);
-				Hst0 = en0 > st0 ? __builtin_amdgcn_readlane(h_first, 0) : Hen0;
+			if (r > 0) {
+				if (packed) argmax_decode8(wave_max_u32(bh), st0, en0, en1, max_H, max_t);
+				else argmax_reduce(bh, bp, st0, en0, en1, max_H, max_t);
+				Hen0 = __builtin_amdgcn_readlane(h_en0, (en0 - st) & 63);
+				Hst0 = __builtin_amdgcn_readlane(h_st0, (st0 - st) & 63);
 			} else {
 				max_H = DUAL ? (int32_t)V[0] - P.qe0 : (int32_t)(uint8_t)V[0] - P.qe - P.qe;
 				max_t = 0;
@@ -597,10 +638,129 @@ __device__ __forceinline__ void run_job_lds(const DpArgs &a, int jid, const DpJo
 			stop = finish_approx<DUAL>(P, z, r, st0, en0, H0, last_H0_t, [&](int t) { return V[t & CM]; },
 			                           [&](int t) { return U[t & CM]; });
 		}
-		__syncthreads();
+		__builtin_amdgcn_wave_barrier();
 		if (stop) break;
 		last_st = st, last_en = en;
 	}
+	__syncthreads();
+	const int n_cigar = backtrack<DUAL>(P, z, jb, a, lds, slab);
+	write_result(a, jid, z, n_cigar, cells);
+	__syncthreads();
+}
+
+// The same row loop specialised for the jobs the extension step emits: a CIGAR, the exact max with
+// the packed argmax key, the standard nt4 scoring, staged sequences, and the gap tie rule (RIGHT)
+// fixed at compile time.  Every per-lane condition is a select, so a block trip has no divergent
+// branch but the store guard.  run_job_lds is the reference for its semantics.
+template <bool DUAL, bool RIGHT>
+__device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJob &jb, const JobP<DUAL> P, uint8_t *lds, uint8_t *slab)
+{
+	const int lane = threadIdx.x;
+	const int CAP = a.cap, CM = CAP - 1;
+	int8_t *U = (int8_t*)lds, *V = U + CAP, *X = V + CAP, *Y = X + CAP, *X2 = Y + CAP, *Y2 = X2 + CAP, *S = Y2 + CAP;
+	int32_t *H = (int32_t*)(lds + 8 * CAP);
+	const int qlen = P.qlen, tlen = P.tlen;
+	uint8_t *Qs = lds + 12 * CAP, *Ts = Qs + ((qlen + 15) & ~15);
+	{
+		const uint8_t *Qg = a.qseq + jb.qoff, *Tg = a.tseq + jb.toff;
+		for (int i = lane; i < qlen; i += 64) Qs[i] = Qg[i];
+		for (int i = lane; i < tlen; i += 64) Ts[i] = Tg[i];
+		__syncthreads();
+	}
+	DpState z;
+	init_state(z);
+	int cells = 0;
+	int hw = -1, last_st = -1, last_en = -1, last_max_H = 0;
+	for (int r = 0; r < qlen + tlen - 1; ++r) {
+		int st0, en0;
+		if (!row_range(r, qlen, tlen, P.w, st0, en0)) { z.zdropped = 1; break; }
+		cells += en0 - st0 + 1;
+		const int st = st0 >> 4 << 4, en = ((en0 + 16) >> 4 << 4) - 1;
+		int hi = en + 15;
+		if (hi > P.tl_hi) hi = P.tl_hi;
+		const int hw0 = hw;
+		if (hi > hw) hw = hi;
+		int8_t x1, v1, x21;
+		if (st > 0 && st - 1 >= last_st && st - 1 <= last_en) {
+			const int sl = (st - 1) & CM;
+			x1 = X[sl], v1 = V[sl], x21 = DUAL ? X2[sl] : 0;
+		} else row_boundary<DUAL>(P, r, st, x1, v1, x21);
+		const int ep = en0 > 0 ? en0 - 1 : en0;
+		const int32_t Hprev = ep > hw0 ? kNegInf : H[ep & CM];
+		int8_t yr, ur, y2r;
+		row_yu<DUAL>(P, r, yr, ur, y2r);
+		const int rr = en >= r ? r : -1;              // the index that takes (yr, ur, y2r), if any
+		int shi = st0 + ((en0 - st0) >> 4 << 4) + 15;
+		if (shi > P.tl_hi) shi = P.tl_hi;
+		const int en1 = st0 + ((en0 - st0) >> 2 << 2);
+		const bool exact = r > 0;
+		uint32_t bk = 0;
+		int32_t h_en0 = 0, h_st0 = 0;
+		uint8_t *prow = slab + (size_t)r * P.nb;
+		int cx = x1, cv = v1, cx2 = x21;
+		for (int base = st; base <= hi; base += 64) {
+			const int t = base + lane, sl = t & CM;
+			const bool entering = t > hw0;
+			int8_t ox = X[sl], ov = V[sl], ou = U[sl], oy = Y[sl], zs = S[sl];
+			int8_t ox2 = DUAL ? X2[sl] : 0, oy2 = DUAL ? Y2[sl] : 0;
+			int32_t oh = H[sl];
+			const int ti = t < tlen ? t : tlen - 1, qi = r - t > 0 ? r - t : 0;
+			const int8_t ta0 = (int8_t)Ts[ti], qa0 = (int8_t)Qs[qi < qlen ? qi : qlen - 1];
+			ox = entering ? P.ini : ox; ov = entering ? P.ini : ov; ou = entering ? P.ini : ou; oy = entering ? P.ini : oy;
+			ox2 = entering ? P.ini2 : ox2; oy2 = entering ? P.ini2 : oy2;
+			zs = entering ? (int8_t)0 : zs;
+			oh = entering ? kNegInf : oh;
+			oy = t == rr ? yr : oy; ou = t == rr ? ur : ou; oy2 = t == rr ? y2r : oy2;
+			const int8_t ta = t < tlen ? ta0 : 0, qa = t <= r ? qa0 : 0;
+			const int8_t fs = (ta == P.m1 || qa == P.m1) ? P.sc_N : ta == qa ? P.sc_mch : P.sc_mis;
+			zs = (t >= st0 && t <= shi) ? fs : zs;
+			int8_t xt1 = (int8_t)wave_shr1_fill((int)ox, cx), vt1 = (int8_t)wave_shr1_fill((int)ov, cv);
+			int8_t x2t1 = DUAL ? (int8_t)wave_shr1_fill((int)ox2, cx2) : 0;
+			if (!DUAL) {
+				// _mm_cvtsi32_si128 of a negative int8 also sets bytes 1..3 (ksw2_extz2_sse.c:138-139)
+				const bool qk = base == st && lane >= 1 && lane <= 3;
+				xt1 = qk && x1 < 0 ? (int8_t)0xff : xt1;
+				vt1 = qk && v1 < 0 ? (int8_t)0xff : vt1;
+			}
+			cx = __builtin_amdgcn_readlane((int)ox, 63);
+			cv = __builtin_amdgcn_readlane((int)ov, 63);
+			if (DUAL) cx2 = __builtin_amdgcn_readlane((int)ox2, 63);
+			const bool core = t <= en;
+			int8_t nU, nV, nX, nY, nX2, nY2;
+			const uint8_t d = dp_cell_cg<DUAL, RIGHT>(P, xt1, vt1, x2t1, ou, oy, oy2, zs, nU, nV, nX, nY, nX2, nY2);
+			const bool band = exact && t >= st0 && t <= en0;
+			const int32_t hc = t < en0 ? oh + h_term<DUAL>(P, nV) : Hprev + h_term<DUAL>(P, en0 > 0 ? nU : nV);
+			const int32_t nh = band ? hc : oh;
+			const uint32_t key = t < en0 ? argmax_key8(hc, t, st0, en1) : (((uint32_t)(hc + (1 << 23)) << 8) | 255u);
+			bk = band && key > bk ? key : bk;
+			h_en0 = t == en0 ? nh : h_en0;
+			h_st0 = t == st0 ? nh : h_st0;
+			if (t <= hi) {
+				U[sl] = core ? nU : ou; V[sl] = core ? nV : ov; X[sl] = core ? nX : ox; Y[sl] = core ? nY : oy;
+				if (DUAL) { X2[sl] = core ? nX2 : ox2; Y2[sl] = core ? nY2 : oy2; }
+				S[sl] = zs;
+				H[sl] = nh;
+				if (core) prow[t - st] = d;
+			}
+		}
+		__builtin_amdgcn_wave_barrier();
+		int32_t max_H, Hen0, Hst0;
+		int max_t;
+		if (r > 0) {
+			argmax_decode8(wave_max_u32(bk), st0, en0, en1, max_H, max_t);
+			Hen0 = __builtin_amdgcn_readlane(h_en0, (en0 - st) & 63);
+			Hst0 = __builtin_amdgcn_readlane(h_st0, (st0 - st) & 63);
+		} else {
+			max_H = DUAL ? (int32_t)V[0] - P.qe0 : (int32_t)(uint8_t)V[0] - P.qe - P.qe;
+			max_t = 0;
+			if (lane == 0) H[0] = max_H;
+			Hen0 = Hst0 = max_H;
+		}
+		__builtin_amdgcn_wave_barrier();
+		if (finish_exact<DUAL>(P, z, r, st0, en0, max_H, max_t, Hen0, Hst0, last_max_H)) break;
+		last_st = st, last_en = en;
+	}
+	__syncthreads();
 	const int n_cigar = backtrack<DUAL>(P, z, jb, a, lds, slab);
 	write_result(a, jid, z, n_cigar, cells);
 	__syncthreads();
@@ -625,7 +785,13 @@ __global__ void __launch_bounds__(64) dp_kernel(DpArgs a)
 		if (j >= n) break;
 		const DpJob jb = a.jobs[j];
 		const JobP<DUAL> P = job_params<DUAL>(jb, a.m, mat);
-		if (((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= a.qt_cap) run_job_lds<DUAL, true>(a, j, jb, P, lds, mat, slab);
+		const bool staged = ((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= a.qt_cap;
+		// the extension step's jobs: CIGAR, exact max, standard scoring, packed argmax keys
+		const bool fast = staged && !P.skip && P.with_cigar && !P.approx_max && !P.generic && P.nb <= 240 &&
+		                  P.qlen + P.tlen <= 60000;
+		if (fast && P.right) run_job_fast<DUAL, true>(a, j, jb, P, lds, slab);
+		else if (fast) run_job_fast<DUAL, false>(a, j, jb, P, lds, slab);
+		else if (staged) run_job_lds<DUAL, true>(a, j, jb, P, lds, mat, slab);
 		else run_job_lds<DUAL, false>(a, j, jb, P, lds, mat, slab);
 		if (!a.next) j += gridDim.x;
 	}
