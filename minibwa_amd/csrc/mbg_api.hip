@@ -40,7 +40,7 @@ struct mbg_ctx_s {
 	L2b lb{};
 	uint64_t idx_bytes = 0;
 	// scratch pools
-	DBuf b_in[6], b_out[3], b_ctr, b_task, b_slab;
+	DBuf b_in[6], b_out[3], b_ctr, b_task, b_slab, b_gstate;
 	DBuf h_pinned;
 	// seed -> locate -> extend step
 	DBuf p_recs, p_rows, p_rowrec, p_first, p_best, p_jobs, p_meta, p_qb, p_tb, p_res, p_cig, p_ctr;
@@ -140,7 +140,7 @@ void mbg_destroy(mbg_ctx_t *c)
 	free_l2b(c);
 	for (auto &b : c->b_in) dfree(b);
 	for (auto &b : c->b_out) dfree(b);
-	dfree(c->b_ctr); dfree(c->b_task); dfree(c->b_slab);
+	dfree(c->b_ctr); dfree(c->b_task); dfree(c->b_slab); dfree(c->b_gstate);
 	for (DBuf *b : {&c->p_recs, &c->p_rows, &c->p_rowrec, &c->p_first, &c->p_best, &c->p_jobs, &c->p_meta, &c->p_qb, &c->p_tb,
 	                &c->p_res, &c->p_cig, &c->p_ctr, &c->s_slots1, &c->s_slots2, &c->s_cnt, &c->s_off, &c->s_spill, &c->s_tmp,
 	                &c->r_cnt, &c->r_off, &c->s_pack, &c->s_qflag}) dfree(*b);
@@ -377,8 +377,8 @@ static unsigned persistent_grid(mbg_ctx_t *c, int64_t n_items, int per_block)
 // record count at *n_out_dev.  Per-task output slots (kSlot1 per read, kSlot2 per pass-2 task) are
 // compacted by an exclusive scan, so no output position needs an atomic except overflow spills.
 // Counters in c->b_ctr: u32 [0] task counter, [1] pass-2 task count, i64 [1] record count,
-// u64 [3] spill count (u32 [4] is the SA row counter, u64 [6] the gather checksum, u32 [14] the DP
-// job counter).  ev (optional) receives events after pass 1 and after pass 2.
+// u64 [3] spill count (u32 [4] is the SA row counter, u64 [6] the gather checksum, u32 [14] and [15] the
+// narrow and wide DP job counters).  ev (optional) receives events after pass 1 and after pass 2.
 static const int32_t kSlot1 = 16, kSlot2 = 8;
 
 struct SeedCaps { int64_t rec, task, spill; };
@@ -577,7 +577,8 @@ int mbg_sa_batch(mbg_ctx_t *c, int64_t n, uint64_t *x)
 
 // ---- DP --------------------------------------------------------------------------------------
 
-static const int kMaxCap = 4096;        // circular row buffer; 12*cap bytes of LDS per wave
+static const int kMaxCap = 4096;        // circular row buffer in LDS: 12*cap bytes per wave
+static const int kMaxGcap = 1 << 22;    // circular row buffer in device memory, for wider bands
 
 static const int kMaxQt = 16384;        // LDS bytes for staging a job's query + target
 
@@ -588,13 +589,26 @@ static int dp_plan(int32_t max_qlen, int32_t max_tlen, int32_t max_w, int *cap_o
 	const int nb = n_col_ * 16;
 	int cap = 64;
 	while (cap < nb + 17) cap <<= 1;
-	if (cap > kMaxCap) return MBG_EUNSUPPORTED;
-	*cap_out = cap;
+	if (cap > kMaxGcap) return MBG_EUNSUPPORTED;
+	*cap_out = cap;                      // capped at kMaxCap by dp_state; wider jobs keep their rows in HBM
 	int64_t slab = ((int64_t)max_qlen + max_tlen) * nb + 256;
 	*slab_out = (slab + 255) & ~(int64_t)255;
 	// jobs whose query and target fit are staged in LDS and run on the register schedule
 	const int64_t qt = (((int64_t)max_qlen + 15) & ~15) + (((int64_t)max_tlen + 15) & ~15);
 	*qt_out = qt <= kMaxQt ? (int)qt : 0;
+	return MBG_OK;
+}
+
+// ring lengths for a launch of `g` blocks whose widest band needs `cap`: jobs up to kMaxCap run on
+// the LDS ring, wider ones on a per-block ring of 12*cap bytes in device memory
+static int dp_state(mbg_ctx_t *c, int64_t g, int cap, DpArgs &a)
+{
+	a.cap = cap < kMaxCap ? cap : kMaxCap;
+	a.gcap = 0, a.gstate = nullptr;
+	if (cap > kMaxCap) {
+		TRY(dgrow(c->b_gstate, (size_t)g * 12 * cap));
+		a.gcap = cap, a.gstate = (uint8_t*)c->b_gstate.p;
+	}
 	return MBG_OK;
 }
 
@@ -611,10 +625,11 @@ static int dp_dev_locked(mbg_ctx_t *c, hipStream_t st, bool dual, int32_t n_jobs
 	TRY(dgrow(c->b_slab, (size_t)g * slab));
 	DpArgs a{};
 	a.jobs = jobs; a.n_jobs = n_jobs; a.qseq = qseq; a.tseq = tseq; a.res = res; a.cigar = cigar;
-	a.pbuf = (uint8_t*)c->b_slab.p; a.slab = slab; a.cap = cap; a.qt_cap = qt; a.m = m;
+	a.pbuf = (uint8_t*)c->b_slab.p; a.slab = slab; a.qt_cap = qt; a.m = m;
+	TRY(dp_state(c, g, cap, a));
 	TRY(dgrow(c->b_ctr, 64));
 	a.next = (unsigned int*)c->b_ctr.p + 14;
-	HIPCHK(hipMemsetAsync(a.next, 0, 4, st));
+	HIPCHK(hipMemsetAsync(a.next, 0, 8, st));          // narrow and wide job counters
 	for (int i = 0; i < 32; ++i) a.mat[i] = i < m * m ? mat[i] : 0;
 	HIPCHK(launch_dp(dual, a, (unsigned)g, st));
 	return MBG_OK;
@@ -875,14 +890,16 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 			if (g > job_cap) g = job_cap;
 			const int64_t max_by_mem = ((int64_t)2 << 30) / slab;
 			if (g > max_by_mem) g = max_by_mem;
+			if (g < 1) g = 1;
 			TRY(dgrow(c->b_slab, (size_t)g * slab));
 			DpArgs d{};
 			d.jobs = (const DpJob*)c->p_jobs.p; d.n_jobs = (int32_t)job_cap; d.n_jobs_dev = pc;
 			d.qseq = (const uint8_t*)c->p_qb.p; d.tseq = (const uint8_t*)c->p_tb.p;
 			d.res = (DpRes*)c->p_res.p; d.cigar = (uint32_t*)c->p_cig.p;
-			d.pbuf = (uint8_t*)c->b_slab.p; d.slab = slab; d.cap = cap_dp; d.qt_cap = qt_dp; d.m = 5;
+			d.pbuf = (uint8_t*)c->b_slab.p; d.slab = slab; d.qt_cap = qt_dp; d.m = 5;
+			TRY(dp_state(c, g, cap_dp, d));
 			d.next = (unsigned int*)c->b_ctr.p + 14;
-			HIPCHK(hipMemsetAsync(d.next, 0, 4, st));
+			HIPCHK(hipMemsetAsync(d.next, 0, 8, st));
 			for (int i = 0; i < 32; ++i) d.mat[i] = i < 25 ? opt->mat[i] : 0;
 			HIPCHK(launch_dp(dual, d, (unsigned)g, st));
 		}

@@ -408,8 +408,8 @@ __device__ __forceinline__ bool finish_approx(const JobP<DUAL> &P, DpState &z, i
 }
 
 // backtrack (ksw2_extd2_sse.c:391-402, ksw2.h:140-172) over the rows kept in `slab`, staged through
-// the first 12*cap bytes of LDS; returns n_cigar
-template <bool DUAL>
+// the first 12*cap bytes of LDS, or read from the slab itself (DIRECT); returns n_cigar
+template <bool DUAL, bool DIRECT = false>
 __device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const DpJob &jb, const DpArgs &a, uint8_t *lds, const uint8_t *slab)
 {
 	const int lane = threadIdx.x;
@@ -437,7 +437,9 @@ __device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const 
 		if (i < off) force = 2;
 		if (i > off_end) force = 1;
 		uint32_t tmp = 0;
-		if (force < 0) {
+		if (force < 0 && DIRECT) {
+			tmp = slab[(size_t)r * nb + i - off];
+		} else if (force < 0) {
 			if (r < chunk_lo) {
 				const int lo = r - K + 1 > 0 ? r - K + 1 : 0;
 				const int bytes = (r - lo + 1) * nb;
@@ -508,13 +510,15 @@ __device__ __forceinline__ void write_result(const DpArgs &a, int jid, const DpS
 // one predicated store, so a row costs one trip per block instead of separate passes with barriers.
 // A single wave owns the job, so its LDS reads see its earlier writes in program order.
 
-// STAGED: the job's query and target are copied into LDS after the 12*cap state region first
-template <bool DUAL, bool STAGED>
+// STAGED: the job's query and target are copied into LDS after the 12*cap state region first.
+// GLOBAL: `lds` is the block's 12*CAP-byte state region in device memory, for bands wider than the
+// LDS ring; a workgroup-scope fence ends each row so the next row's loads see its stores.
+template <bool DUAL, bool STAGED, bool GLOBAL = false>
 __device__ __forceinline__ void run_job_lds(const DpArgs &a, int jid, const DpJob &jb, const JobP<DUAL> P, uint8_t *lds, const int8_t *mat,
-                                            uint8_t *slab)
+                                            uint8_t *slab, const int CAP)
 {
 	const int lane = threadIdx.x;
-	const int CAP = a.cap, CM = CAP - 1;
+	const int CM = CAP - 1;
 	int8_t *U = (int8_t*)lds, *V = U + CAP, *X = V + CAP, *Y = X + CAP, *X2 = Y + CAP, *Y2 = X2 + CAP, *S = Y2 + CAP;
 	int32_t *H = (int32_t*)(lds + 8 * CAP);
 	const uint8_t *Qg = a.qseq + jb.qoff, *Tg = a.tseq + jb.toff;
@@ -616,6 +620,7 @@ __device__ __forceinline__ void run_job_lds(const DpArgs &a, int jid, const DpJo
 				if (core && P.with_cigar) prow[t - st] = d;
 			}
 		}
+		if (GLOBAL) __threadfence_block();
 		__builtin_amdgcn_wave_barrier();
 
 		bool stop = false;
@@ -638,12 +643,13 @@ __device__ __forceinline__ void run_job_lds(const DpArgs &a, int jid, const DpJo
 			stop = finish_approx<DUAL>(P, z, r, st0, en0, H0, last_H0_t, [&](int t) { return V[t & CM]; },
 			                           [&](int t) { return U[t & CM]; });
 		}
+		if (GLOBAL) __threadfence_block();
 		__builtin_amdgcn_wave_barrier();
 		if (stop) break;
 		last_st = st, last_en = en;
 	}
 	__syncthreads();
-	const int n_cigar = backtrack<DUAL>(P, z, jb, a, lds, slab);
+	const int n_cigar = backtrack<DUAL, GLOBAL>(P, z, jb, a, lds, slab);
 	write_result(a, jid, z, n_cigar, cells);
 	__syncthreads();
 }
@@ -766,7 +772,10 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 	__syncthreads();
 }
 
-template <bool DUAL>
+// WIDE = false: the jobs whose band fits the LDS ring (nb + 17 <= cap), taken from the shared counter;
+// WIDE = true: the others, on a 12*gcap-byte ring in this block's device-memory region, taken from
+// the second counter.  Keeping the two in separate kernels leaves the narrow kernel's registers alone.
+template <bool DUAL, bool WIDE>
 __global__ void __launch_bounds__(64) dp_kernel(DpArgs a)
 {
 	extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
@@ -776,32 +785,42 @@ __global__ void __launch_bounds__(64) dp_kernel(DpArgs a)
 	uint8_t *slab = a.pbuf + (size_t)blockIdx.x * a.slab;
 	int n = a.n_jobs;
 	if (a.n_jobs_dev) { const unsigned long long nd = *a.n_jobs_dev; n = nd < (unsigned long long)n ? (int)nd : n; }
+	unsigned int *next = a.next && WIDE ? a.next + 1 : a.next;
 	for (int j = blockIdx.x;;) {
-		if (a.next) { // one job at a time from the shared counter
+		if (next) { // one job at a time from the shared counter
 			int k = 0;
-			if (threadIdx.x == 0) k = (int)atomicAdd(a.next, 1u);
+			if (threadIdx.x == 0) k = (int)atomicAdd(next, 1u);
 			j = __builtin_amdgcn_readfirstlane(k);
 		}
 		if (j >= n) break;
 		const DpJob jb = a.jobs[j];
 		const JobP<DUAL> P = job_params<DUAL>(jb, a.m, mat);
-		const bool staged = ((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= a.qt_cap;
-		// the extension step's jobs: CIGAR, exact max, standard scoring, packed argmax keys
-		const bool fast = staged && !P.skip && P.with_cigar && !P.approx_max && !P.generic && P.nb <= 240 &&
-		                  P.qlen + P.tlen <= 60000;
-		if (fast && P.right) run_job_fast<DUAL, true>(a, j, jb, P, lds, slab);
-		else if (fast) run_job_fast<DUAL, false>(a, j, jb, P, lds, slab);
-		else if (staged) run_job_lds<DUAL, true>(a, j, jb, P, lds, mat, slab);
-		else run_job_lds<DUAL, false>(a, j, jb, P, lds, mat, slab);
-		if (!a.next) j += gridDim.x;
+		const bool wide = P.nb + 17 > a.cap;
+		if (WIDE && wide) {
+			run_job_lds<DUAL, false, true>(a, j, jb, P, a.gstate + (size_t)blockIdx.x * 12 * a.gcap, mat, slab, a.gcap);
+		} else if (!WIDE && !wide) {
+			const bool staged = ((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= a.qt_cap;
+			// the extension step's jobs: CIGAR, exact max, standard scoring, packed argmax keys
+			const bool fast = staged && !P.skip && P.with_cigar && !P.approx_max && !P.generic && P.nb <= 240 &&
+			                  P.qlen + P.tlen <= 60000;
+			if (fast && P.right) run_job_fast<DUAL, true>(a, j, jb, P, lds, slab);
+			else if (fast) run_job_fast<DUAL, false>(a, j, jb, P, lds, slab);
+			else if (staged) run_job_lds<DUAL, true>(a, j, jb, P, lds, mat, slab, a.cap);
+			else run_job_lds<DUAL, false>(a, j, jb, P, lds, mat, slab, a.cap);
+		}
+		if (!next) j += gridDim.x;
 	}
 }
 
 hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st)
 {
 	const size_t shm = (size_t)12 * a.cap + (size_t)a.qt_cap;
-	if (dual) hipLaunchKernelGGL(dp_kernel<true>, dim3(grid), dim3(64), shm, st, a);
-	else hipLaunchKernelGGL(dp_kernel<false>, dim3(grid), dim3(64), shm, st, a);
+	if (dual) hipLaunchKernelGGL((dp_kernel<true, false>), dim3(grid), dim3(64), shm, st, a);
+	else hipLaunchKernelGGL((dp_kernel<false, false>), dim3(grid), dim3(64), shm, st, a);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess || a.gcap == 0) return e;
+	if (dual) hipLaunchKernelGGL((dp_kernel<true, true>), dim3(grid), dim3(64), 0, st, a);
+	else hipLaunchKernelGGL((dp_kernel<false, true>), dim3(grid), dim3(64), 0, st, a);
 	return hipGetLastError();
 }
 

@@ -66,6 +66,8 @@ struct DpArgs {
 	int64_t slab;           // bytes per slab
 	int32_t cap;            // circular-buffer length (power of two)
 	int32_t qt_cap;         // LDS bytes after the 12*cap region for staging a job's query and target
+	uint8_t *gstate;        // when gcap > 0: one 12*gcap-byte row-state region per resident block
+	int32_t gcap;           // ring length (power of two) for jobs whose band needs more than cap - 17
 	unsigned int *next;     // when set: zeroed job counter, jobs are taken one at a time
 	int8_t m;
 	int8_t mat[32];

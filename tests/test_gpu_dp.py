@@ -149,3 +149,28 @@ def test_extd2_device_entry_point(gpu_ctx):
         assert {f: int(res[f][i]) for f in FIELDS} == {f: r[f] for f in FIELDS}, i
         assert [int(c) for c in cig[coff[i]:coff[i] + res["n_cigar"][i]]] == r["cigar"], i
     assert (res["cells"] > 0).all()
+
+
+@pytest.mark.parametrize("dual", [True, False])
+def test_dp_wide_band(gpu_ctx, dual):
+    """Bands wider than the 4096-cell LDS ring (kb-scale gap fills, w >= 4100) run on a per-block ring
+    in device memory; mixed in one launch with narrow jobs, every field and CIGAR equals the reference."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(31 if dual else 32)
+    Q, T, P = [], [], []
+    specs = [(5200, 0x00, 4200, -1, -1), (4600, 0x40, -1, 400, 10), (6000, 0x02, 6000, -1, 0), (4400, 0x08, 4400, 172, -1),
+             (5000, 0x80, 5200, -1, 0), (4800, 0x01, 4800, -1, 0), (4300, 0x42, 4300, 100, 10), (5500, 0x04, 5500, -1, 0)]
+    for k, (ql, flag, w, zd, eb) in enumerate(specs):
+        q = rng.integers(0, 4, ql).astype(np.uint8)
+        t = _mutate(rng, q, 0.03 if k != 6 else 0.3, 0.01)
+        t = np.concatenate([t, rng.integers(0, 4, 80).astype(np.uint8)])
+        qq, ee, q2, e2 = GAPS[k % len(GAPS)]
+        Q.append(q), T.append(t)
+        P.append(dict(flag=flag, w=w, zdrop=zd, end_bonus=eb, q=qq, e=ee, q2=q2, e2=e2))
+    nq, nt, nP = fuzz_jobs(40, 33)            # narrow jobs in the same launch
+    for i in range(len(nq)):
+        Q.insert(2 * i % len(Q), nq[i]), T.insert(2 * i % len(T), nt[i]), P.insert(2 * i % len(P), nP[i])
+    mat = O.gen_nt4_mat(2, 8, 0, 1)
+    out = (gpu_ctx.ksw_extd2 if dual else gpu_ctx.ksw_extz2)(Q, T, _params(P), mat)
+    _check(out, Q, T, P, "wide", dual)
+    assert int(out.res["cells"].max()) > 4096 * 4096
