@@ -40,7 +40,7 @@ struct mbg_ctx_s {
 	L2b lb{};
 	uint64_t idx_bytes = 0;
 	// scratch pools
-	DBuf b_in[6], b_out[3], b_ctr, b_task, b_slab, b_gstate;
+	DBuf b_in[6], b_out[3], b_ctr, b_task, b_slab, b_gstate, b_defer;
 	DBuf h_pinned;
 	// seed -> locate -> extend step
 	DBuf p_recs, p_rows, p_rowrec, p_first, p_best, p_jobs, p_meta, p_qb, p_tb, p_res, p_cig, p_ctr;
@@ -140,7 +140,7 @@ void mbg_destroy(mbg_ctx_t *c)
 	free_l2b(c);
 	for (auto &b : c->b_in) dfree(b);
 	for (auto &b : c->b_out) dfree(b);
-	dfree(c->b_ctr); dfree(c->b_task); dfree(c->b_slab); dfree(c->b_gstate);
+	dfree(c->b_ctr); dfree(c->b_task); dfree(c->b_slab); dfree(c->b_gstate); dfree(c->b_defer);
 	for (DBuf *b : {&c->p_recs, &c->p_rows, &c->p_rowrec, &c->p_first, &c->p_best, &c->p_jobs, &c->p_meta, &c->p_qb, &c->p_tb,
 	                &c->p_res, &c->p_cig, &c->p_ctr, &c->s_slots1, &c->s_slots2, &c->s_cnt, &c->s_off, &c->s_spill, &c->s_tmp,
 	                &c->r_cnt, &c->r_off, &c->s_pack, &c->s_qflag}) dfree(*b);
@@ -377,8 +377,8 @@ static unsigned persistent_grid(mbg_ctx_t *c, int64_t n_items, int per_block)
 // record count at *n_out_dev.  Per-task output slots (kSlot1 per read, kSlot2 per pass-2 task) are
 // compacted by an exclusive scan, so no output position needs an atomic except overflow spills.
 // Counters in c->b_ctr: u32 [0] task counter, [1] pass-2 task count, i64 [1] record count,
-// u64 [3] spill count (u32 [4] is the SA row counter, u64 [6] the gather checksum, u32 [14] and [15] the
-// narrow and wide DP job counters).  ev (optional) receives events after pass 1 and after pass 2.
+// u64 [3] spill count (u32 [4] is the SA row counter, u64 [6] the gather checksum, u32 [14..17] the
+// DP job counters).  ev (optional) receives events after pass 1 and after pass 2.
 static const int32_t kSlot1 = 16, kSlot2 = 8;
 
 struct SeedCaps { int64_t rec, task, spill; };
@@ -390,7 +390,7 @@ static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_
 	if (c->cap_task < 2 * (int64_t)n_reads + 1024) c->cap_task = 2 * (int64_t)n_reads + 1024;
 	if (c->cap_spill < (int64_t)n_reads + 4096) c->cap_spill = (int64_t)n_reads + 4096;
 	const int64_t rc = c->cap_rec, tc = c->cap_task, sc = c->cap_spill, n_cnt = (int64_t)n_reads + tc;
-	TRY(dgrow(c->b_ctr, 64));
+	TRY(dgrow(c->b_ctr, 128));
 	TRY(dgrow(c->p_recs, rc * sizeof(SeedRec)));
 	TRY(dgrow(c->s_slots1, (size_t)n_reads * kSlot1 * sizeof(SeedRec) + 64));
 	TRY(dgrow(c->s_slots2, (size_t)tc * kSlot2 * sizeof(SeedRec)));
@@ -543,7 +543,7 @@ static int sa_dev_locked(mbg_ctx_t *c, hipStream_t st, int64_t n, uint64_t *x)
 {
 	if (n <= 0) return MBG_OK;
 	if (n > 0xffffffffLL) return MBG_EUNSUPPORTED;
-	TRY(dgrow(c->b_ctr, 64));
+	TRY(dgrow(c->b_ctr, 128));
 	uint32_t *next = (uint32_t*)c->b_ctr.p + 4;
 	HIPCHK(hipMemsetAsync(next, 0, 4, st));
 	HIPCHK(launch_sa(c->ix, n, nullptr, x, next, nullptr, persistent_grid(c, n, 256), st));
@@ -627,10 +627,12 @@ static int dp_dev_locked(mbg_ctx_t *c, hipStream_t st, bool dual, int32_t n_jobs
 	a.jobs = jobs; a.n_jobs = n_jobs; a.qseq = qseq; a.tseq = tseq; a.res = res; a.cigar = cigar;
 	a.pbuf = (uint8_t*)c->b_slab.p; a.slab = slab; a.qt_cap = qt; a.m = m;
 	TRY(dp_state(c, g, cap, a));
-	TRY(dgrow(c->b_ctr, 64));
+	TRY(dgrow(c->b_ctr, 128));
 	a.next = (unsigned int*)c->b_ctr.p + 14;
-	HIPCHK(hipMemsetAsync(a.next, 0, 8, st));          // narrow and wide job counters
-	for (int i = 0; i < 32; ++i) a.mat[i] = i < m * m ? mat[i] : 0;
+	HIPCHK(hipMemsetAsync(a.next, 0, 16, st));
+	TRY(dgrow(c->b_defer, (size_t)n_jobs * 4));
+	a.defer = (int32_t*)c->b_defer.p;
+	dp_set_matrix(a, m, mat);
 	HIPCHK(launch_dp(dual, a, (unsigned)g, st));
 	return MBG_OK;
 }
@@ -800,7 +802,7 @@ int mbg_gather_bench(mbg_ctx_t *c, int64_t n_chains, int32_t steps, int32_t bs, 
 	if (!c->has_bwt) return MBG_ENOIDX;
 	HIPCHK(hipSetDevice(c->device));
 	for (hipEvent_t &e : c->ev) if (!e) HIPCHK(hipEventCreate(&e));
-	TRY(dgrow(c->b_ctr, 64));
+	TRY(dgrow(c->b_ctr, 128));
 	unsigned long long *sum = (unsigned long long*)c->b_ctr.p + 6;
 	HIPCHK(hipMemsetAsync(sum, 0, 8, c->stream));
 	HIPCHK(hipEventRecord(c->ev[0], c->stream));
@@ -899,8 +901,10 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 			d.pbuf = (uint8_t*)c->b_slab.p; d.slab = slab; d.qt_cap = qt_dp; d.m = 5;
 			TRY(dp_state(c, g, cap_dp, d));
 			d.next = (unsigned int*)c->b_ctr.p + 14;
-			HIPCHK(hipMemsetAsync(d.next, 0, 8, st));
-			for (int i = 0; i < 32; ++i) d.mat[i] = i < 25 ? opt->mat[i] : 0;
+			HIPCHK(hipMemsetAsync(d.next, 0, 16, st));
+			TRY(dgrow(c->b_defer, (size_t)job_cap * 4));
+			d.defer = (int32_t*)c->b_defer.p;
+			dp_set_matrix(d, 5, opt->mat);
 			HIPCHK(launch_dp(dual, d, (unsigned)g, st));
 		}
 		HIPCHK(hipEventRecord(c->ev[6], st));

@@ -85,9 +85,12 @@ struct JobP {
 	bool skip, with_cigar, approx_max, right, generic;
 };
 
+// the scoring matrix enters through scalars the host derived from it (entries 0, 1 and m*m-1, the
+// max and the min), so that every job parameter stays wave-uniform
 template <bool DUAL>
-__device__ __forceinline__ JobP<DUAL> job_params(const DpJob &jb, int m, const int8_t *mat)
+__device__ __forceinline__ JobP<DUAL> job_params(const DpJob &jb, const DpArgs &a)
 {
+	const int m = a.m;
 	JobP<DUAL> P;
 	P.qlen = jb.qlen, P.tlen = jb.tlen, P.m = m, P.flag = jb.flag, P.zdrop = jb.zdrop, P.end_bonus = jb.end_bonus;
 	int8_t q = jb.q, e = jb.e, q2 = jb.q2, e2 = jb.e2;
@@ -99,21 +102,16 @@ __device__ __forceinline__ JobP<DUAL> job_params(const DpJob &jb, int m, const i
 	P.q = q, P.e = e, P.q2 = q2, P.e2 = e2;
 	P.qe8 = (int8_t)(q + e), P.qe28 = (int8_t)(q2 + e2);
 	P.qe = q + e;
-	P.sc_mch = mat[0], P.sc_mis = mat[1], P.m1 = (int8_t)(m - 1);
+	P.sc_mch = a.mat[0], P.sc_mis = a.mat[1], P.m1 = (int8_t)(m - 1);
 	P.sc_N = 0, P.qe2x = 0, P.max_sc_ = 0;
 	P.w = jb.w, P.n_col_ = 1, P.max_sc = 0, P.long_thres = 0, P.long_diff = 0;
 	if (!P.skip) {
-		P.sc_N = DUAL ? (mat[m * m - 1] == 0 ? (int8_t)-e2 : mat[m * m - 1]) : (mat[m * m - 1] == 0 ? (int8_t)-e : mat[m * m - 1]);
+		P.sc_N = DUAL ? (a.mat_last == 0 ? (int8_t)-e2 : (int8_t)a.mat_last) : (a.mat_last == 0 ? (int8_t)-e : (int8_t)a.mat_last);
 		P.qe2x = (int8_t)((q + e) * 2);
-		P.max_sc_ = (int8_t)(mat[0] + (q + e) * 2);
+		P.max_sc_ = (int8_t)(a.mat[0] + (q + e) * 2);
 		dp_geom(P.qlen, P.tlen, P.w, &P.w, &P.n_col_);
-		int max_sc = mat[0], min_sc = mat[1];
-		for (int t = 1; t < m * m; ++t) {
-			max_sc = max_sc > mat[t] ? max_sc : mat[t];
-			min_sc = min_sc < mat[t] ? min_sc : mat[t];
-		}
-		P.max_sc = max_sc;
-		if (-min_sc > 2 * (q + e)) P.skip = true;
+		P.max_sc = a.mat_max;
+		if (-a.mat_min > 2 * (q + e)) P.skip = true;
 		if (DUAL) {
 			P.long_thres = e != e2 ? (q2 - q) / (e - e2) - 1 : 0;
 			if (q2 + e2 + P.long_thres * e2 > q + e + P.long_thres * e) ++P.long_thres;
@@ -408,9 +406,10 @@ __device__ __forceinline__ bool finish_approx(const JobP<DUAL> &P, DpState &z, i
 }
 
 // backtrack (ksw2_extd2_sse.c:391-402, ksw2.h:140-172) over the rows kept in `slab`, staged through
-// the first 12*cap bytes of LDS, or read from the slab itself (DIRECT); returns n_cigar
+// the first `stage` bytes of LDS, or read from the slab itself (DIRECT); returns n_cigar
 template <bool DUAL, bool DIRECT = false>
-__device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const DpJob &jb, const DpArgs &a, uint8_t *lds, const uint8_t *slab)
+__device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const DpJob &jb, const DpArgs &a, uint8_t *lds, const uint8_t *slab,
+                                         int stage)
 {
 	const int lane = threadIdx.x;
 	const int qlen = P.qlen, tlen = P.tlen, w = P.w, nb = P.nb;
@@ -424,7 +423,7 @@ __device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const 
 	}
 	if (i0 < 0 && j0 < 0) return 0;
 	uint8_t *stg = lds;
-	const int K = (12 * a.cap) / nb;
+	const int K = stage / nb;
 	int chunk_lo = 0x7fffffff;
 	int i = i0, j = j0, state = 0, last_op = -1;
 	uint32_t cur = 0;
@@ -438,7 +437,7 @@ __device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const 
 		if (i > off_end) force = 1;
 		uint32_t tmp = 0;
 		if (force < 0 && DIRECT) {
-			tmp = slab[(size_t)r * nb + i - off];
+			tmp = (uint32_t)__builtin_amdgcn_readfirstlane(slab[(size_t)r * nb + i - off]);
 		} else if (force < 0) {
 			if (r < chunk_lo) {
 				const int lo = r - K + 1 > 0 ? r - K + 1 : 0;
@@ -449,7 +448,9 @@ __device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const 
 				__syncthreads();
 				chunk_lo = lo;
 			}
-			tmp = stg[(r - chunk_lo) * nb + i - off];
+			// every lane reads the same byte: marked uniform, so the walk (and the barriers of the
+			// staging above) stays in uniform control flow
+			tmp = (uint32_t)__builtin_amdgcn_readfirstlane(stg[(r - chunk_lo) * nb + i - off]);
 		}
 		if (state == 0) state = tmp & 7;
 		else if (!(tmp >> (state + 2) & 1)) state = 0;
@@ -649,77 +650,110 @@ __device__ __forceinline__ void run_job_lds(const DpArgs &a, int jid, const DpJo
 		last_st = st, last_en = en;
 	}
 	__syncthreads();
-	const int n_cigar = backtrack<DUAL, GLOBAL>(P, z, jb, a, lds, slab);
+	const int n_cigar = backtrack<DUAL, GLOBAL>(P, z, jb, a, lds, slab, 12 * CAP);
 	write_result(a, jid, z, n_cigar, cells);
 	__syncthreads();
 }
 
-// The same row loop specialised for the jobs the extension step emits: a CIGAR, the exact max with
-// the packed argmax key, the standard nt4 scoring, staged sequences, and the gap tie rule (RIGHT)
-// fixed at compile time.  Every per-lane condition is a select, so a block trip has no divergent
-// branch but the store guard.  run_job_lds is the reference for its semantics.
+// ---- the extension step's jobs ----------------------------------------------------------------
+//
+// Jobs with a CIGAR, the exact max, the standard nt4 scoring and a band of at most 224 cells run in
+// their own kernel on a ring of kFastCap slots whose array offsets are compile-time constants.
+// Instead of testing every lane for indices entering the window, the ring starts with the initial
+// values (ksw2_extd2_sse.c:101-111) and the slots of the 16 indices that leave the window when st
+// moves up are reset at once: the window [st-1, end] spans at most nb + 16 < kFastCap indices, so a
+// slot is reset before its next index arrives.  The target and the reversed query are staged in LDS
+// with zero padding (the bytes ksw2's unaligned score loads see past the sequences), a block trip
+// ends at max(en, shi) instead of en + 15, and the stores are predicated instead of selected.
+// run_job_lds is the reference for the semantics.
+
+constexpr int kFastCap = 256, kFastMask = kFastCap - 1;
+
+// a value every lane read from the same LDS address, marked uniform so that the row loop's control
+// flow stays scalar
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+constexpr int kFastMaxNb = kFastCap - 32;
+constexpr int kFastState = 12 * kFastCap;     // U V X Y X2 Y2 S (int8) and H (int32)
+
+template <bool DUAL>
+__device__ __forceinline__ bool fast_job(const JobP<DUAL> &P, int qt_cap)
+{
+	return !P.skip && P.with_cigar && !P.approx_max && !P.generic && P.nb <= kFastMaxNb && P.qlen + P.tlen <= 60000 &&
+	       ((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= qt_cap;
+}
+
+template <bool DUAL>
+__device__ __forceinline__ void ring_reset(uint8_t *lds, int slot, int8_t ini, int8_t ini2)
+{
+	int8_t *R = (int8_t*)lds;
+	R[slot] = ini, R[kFastCap + slot] = ini, R[2 * kFastCap + slot] = ini, R[3 * kFastCap + slot] = ini;
+	if (DUAL) R[4 * kFastCap + slot] = ini2, R[5 * kFastCap + slot] = ini2;
+	R[6 * kFastCap + slot] = 0;
+	((int32_t*)(lds + 8 * kFastCap))[slot] = kNegInf;
+}
+
 template <bool DUAL, bool RIGHT>
-__device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJob &jb, const JobP<DUAL> P, uint8_t *lds, uint8_t *slab)
+__device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJob &jb, const JobP<DUAL> &P, uint8_t *lds, uint8_t *slab)
 {
 	const int lane = threadIdx.x;
-	const int CAP = a.cap, CM = CAP - 1;
-	int8_t *U = (int8_t*)lds, *V = U + CAP, *X = V + CAP, *Y = X + CAP, *X2 = Y + CAP, *Y2 = X2 + CAP, *S = Y2 + CAP;
-	int32_t *H = (int32_t*)(lds + 8 * CAP);
-	const int qlen = P.qlen, tlen = P.tlen;
-	uint8_t *Qs = lds + 12 * CAP, *Ts = Qs + ((qlen + 15) & ~15);
+	int8_t *U = (int8_t*)lds, *V = U + kFastCap, *X = V + kFastCap, *Y = X + kFastCap, *X2 = Y + kFastCap, *Y2 = X2 + kFastCap;
+	int8_t *S = Y2 + kFastCap;
+	int32_t *H = (int32_t*)(lds + 8 * kFastCap);
+	const int qlen = P.qlen, tlen = P.tlen, tpad = P.tl_hi + 1;
+	uint8_t *Ts = lds + kFastState, *Qr = Ts + tpad + 16;
 	{
 		const uint8_t *Qg = a.qseq + jb.qoff, *Tg = a.tseq + jb.toff;
-		for (int i = lane; i < qlen; i += 64) Qs[i] = Qg[i];
-		for (int i = lane; i < tlen; i += 64) Ts[i] = Tg[i];
+		for (int i = lane; i < tpad + 16; i += 64) Ts[i] = i < tlen ? Tg[i] : 0;
+		for (int i = lane; i < qlen + 16; i += 64) Qr[i] = i < qlen ? Qg[qlen - 1 - i] : 0;
+		for (int i = lane; i < kFastCap; i += 64) ring_reset<DUAL>(lds, i, P.ini, P.ini2);
 		__syncthreads();
 	}
 	DpState z;
 	init_state(z);
 	int cells = 0;
-	int hw = -1, last_st = -1, last_en = -1, last_max_H = 0;
+	int last_st = -1, last_en = -1, last_max_H = 0, prev_st = 0;
 	for (int r = 0; r < qlen + tlen - 1; ++r) {
 		int st0, en0;
 		if (!row_range(r, qlen, tlen, P.w, st0, en0)) { z.zdropped = 1; break; }
 		cells += en0 - st0 + 1;
 		const int st = st0 >> 4 << 4, en = ((en0 + 16) >> 4 << 4) - 1;
-		int hi = en + 15;
-		if (hi > P.tl_hi) hi = P.tl_hi;
-		const int hw0 = hw;
-		if (hi > hw) hw = hi;
+		if (st != prev_st) { // indices [prev_st-1, st-1) leave the window
+			for (int p = prev_st - 1 + lane; p < st - 1; p += 64)
+				if (p >= 0) ring_reset<DUAL>(lds, p & kFastMask, P.ini, P.ini2);
+			prev_st = st;
+		}
 		int8_t x1, v1, x21;
 		if (st > 0 && st - 1 >= last_st && st - 1 <= last_en) {
-			const int sl = (st - 1) & CM;
-			x1 = X[sl], v1 = V[sl], x21 = DUAL ? X2[sl] : 0;
+			const int sl = (st - 1) & kFastMask;
+			x1 = (int8_t)uni(X[sl]), v1 = (int8_t)uni(V[sl]), x21 = DUAL ? (int8_t)uni(X2[sl]) : 0;
 		} else row_boundary<DUAL>(P, r, st, x1, v1, x21);
 		const int ep = en0 > 0 ? en0 - 1 : en0;
-		const int32_t Hprev = ep > hw0 ? kNegInf : H[ep & CM];
-		int8_t yr, ur, y2r;
-		row_yu<DUAL>(P, r, yr, ur, y2r);
-		const int rr = en >= r ? r : -1;              // the index that takes (yr, ur, y2r), if any
+		const int32_t Hprev = uni(H[ep & kFastMask]);  // H[en0-1] before this row's update
+		if (en >= r && lane == 0) {                  // ksw2_extd2_sse.c:156-161
+			int8_t yr, ur, y2r;
+			row_yu<DUAL>(P, r, yr, ur, y2r);
+			Y[r & kFastMask] = yr, U[r & kFastMask] = ur;
+			if (DUAL) Y2[r & kFastMask] = y2r;
+		}
+		__builtin_amdgcn_wave_barrier();
 		int shi = st0 + ((en0 - st0) >> 4 << 4) + 15;
 		if (shi > P.tl_hi) shi = P.tl_hi;
+		const int end = en > shi ? en : shi;         // the last index this row writes
 		const int en1 = st0 + ((en0 - st0) >> 2 << 2);
 		const bool exact = r > 0;
 		uint32_t bk = 0;
-		int32_t h_en0 = 0, h_st0 = 0;
-		uint8_t *prow = slab + (size_t)r * P.nb;
+		uint8_t *prow = slab + (size_t)r * P.nb - st;
+		const uint8_t *qrow = Qr + (qlen - 1 - r);   // qrow[t] = query[r - t] for t <= r, 0 up to r + 15
 		int cx = x1, cv = v1, cx2 = x21;
-		for (int base = st; base <= hi; base += 64) {
-			const int t = base + lane, sl = t & CM;
-			const bool entering = t > hw0;
+		for (int base = st; base <= end; base += 64) {
+			const int t = base + lane, sl = t & kFastMask;
 			int8_t ox = X[sl], ov = V[sl], ou = U[sl], oy = Y[sl], zs = S[sl];
 			int8_t ox2 = DUAL ? X2[sl] : 0, oy2 = DUAL ? Y2[sl] : 0;
-			int32_t oh = H[sl];
-			const int ti = t < tlen ? t : tlen - 1, qi = r - t > 0 ? r - t : 0;
-			const int8_t ta0 = (int8_t)Ts[ti], qa0 = (int8_t)Qs[qi < qlen ? qi : qlen - 1];
-			ox = entering ? P.ini : ox; ov = entering ? P.ini : ov; ou = entering ? P.ini : ou; oy = entering ? P.ini : oy;
-			ox2 = entering ? P.ini2 : ox2; oy2 = entering ? P.ini2 : oy2;
-			zs = entering ? (int8_t)0 : zs;
-			oh = entering ? kNegInf : oh;
-			oy = t == rr ? yr : oy; ou = t == rr ? ur : ou; oy2 = t == rr ? y2r : oy2;
-			const int8_t ta = t < tlen ? ta0 : 0, qa = t <= r ? qa0 : 0;
+			const int32_t oh = H[sl];
+			const int tc = t < end ? t : end;
+			const int8_t ta = (int8_t)Ts[tc], qa = (int8_t)qrow[tc];
 			const int8_t fs = (ta == P.m1 || qa == P.m1) ? P.sc_N : ta == qa ? P.sc_mch : P.sc_mis;
-			zs = (t >= st0 && t <= shi) ? fs : zs;
+			zs = (unsigned)(t - st0) <= (unsigned)(shi - st0) ? fs : zs;
 			int8_t xt1 = (int8_t)wave_shr1_fill((int)ox, cx), vt1 = (int8_t)wave_shr1_fill((int)ov, cv);
 			int8_t x2t1 = DUAL ? (int8_t)wave_shr1_fill((int)ox2, cx2) : 0;
 			if (!DUAL) {
@@ -731,22 +765,20 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 			cx = __builtin_amdgcn_readlane((int)ox, 63);
 			cv = __builtin_amdgcn_readlane((int)ov, 63);
 			if (DUAL) cx2 = __builtin_amdgcn_readlane((int)ox2, 63);
-			const bool core = t <= en;
 			int8_t nU, nV, nX, nY, nX2, nY2;
 			const uint8_t d = dp_cell_cg<DUAL, RIGHT>(P, xt1, vt1, x2t1, ou, oy, oy2, zs, nU, nV, nX, nY, nX2, nY2);
 			const bool band = exact && t >= st0 && t <= en0;
 			const int32_t hc = t < en0 ? oh + h_term<DUAL>(P, nV) : Hprev + h_term<DUAL>(P, en0 > 0 ? nU : nV);
-			const int32_t nh = band ? hc : oh;
 			const uint32_t key = t < en0 ? argmax_key8(hc, t, st0, en1) : (((uint32_t)(hc + (1 << 23)) << 8) | 255u);
 			bk = band && key > bk ? key : bk;
-			h_en0 = t == en0 ? nh : h_en0;
-			h_st0 = t == st0 ? nh : h_st0;
-			if (t <= hi) {
-				U[sl] = core ? nU : ou; V[sl] = core ? nV : ov; X[sl] = core ? nX : ox; Y[sl] = core ? nY : oy;
-				if (DUAL) { X2[sl] = core ? nX2 : ox2; Y2[sl] = core ? nY2 : oy2; }
+			if (t <= end) {
 				S[sl] = zs;
-				H[sl] = nh;
-				if (core) prow[t - st] = d;
+				if (t <= en) {
+					U[sl] = nU, V[sl] = nV, X[sl] = nX, Y[sl] = nY;
+					if (DUAL) X2[sl] = nX2, Y2[sl] = nY2;
+					prow[t] = d;
+				}
+				if (band) H[sl] = hc;
 			}
 		}
 		__builtin_amdgcn_wave_barrier();
@@ -754,10 +786,9 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 		int max_t;
 		if (r > 0) {
 			argmax_decode8(wave_max_u32(bk), st0, en0, en1, max_H, max_t);
-			Hen0 = __builtin_amdgcn_readlane(h_en0, (en0 - st) & 63);
-			Hst0 = __builtin_amdgcn_readlane(h_st0, (st0 - st) & 63);
+			Hen0 = uni(H[en0 & kFastMask]), Hst0 = uni(H[st0 & kFastMask]);
 		} else {
-			max_H = DUAL ? (int32_t)V[0] - P.qe0 : (int32_t)(uint8_t)V[0] - P.qe - P.qe;
+			max_H = DUAL ? uni(V[0]) - P.qe0 : uni((uint8_t)V[0]) - P.qe - P.qe;
 			max_t = 0;
 			if (lane == 0) H[0] = max_H;
 			Hen0 = Hst0 = max_H;
@@ -767,14 +798,44 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 		last_st = st, last_en = en;
 	}
 	__syncthreads();
-	const int n_cigar = backtrack<DUAL>(P, z, jb, a, lds, slab);
+	const int n_cigar = backtrack<DUAL>(P, z, jb, a, lds, slab, kFastState + a.qt_cap + 32);
 	write_result(a, jid, z, n_cigar, cells);
 	__syncthreads();
 }
 
-// WIDE = false: the jobs whose band fits the LDS ring (nb + 17 <= cap), taken from the shared counter;
-// WIDE = true: the others, on a 12*gcap-byte ring in this block's device-memory region, taken from
-// the second counter.  Keeping the two in separate kernels leaves the narrow kernel's registers alone.
+// the next index from a shared counter, one per wave: lane 0 takes it, every lane receives it
+__device__ __forceinline__ int wave_take(unsigned int *ctr)
+{
+	int k = 0;
+	if (threadIdx.x == 0) k = (int)atomicAdd(ctr, 1u);
+	return __builtin_amdgcn_readfirstlane(k);
+}
+
+// job counters in a.next: [0] taken by dp_fast_kernel, [1] jobs it deferred (listed in a.defer),
+// [2] and [3] deferred jobs taken by dp_kernel<DUAL, false> and dp_kernel<DUAL, true>.  The job
+// loops take the next index in the loop condition and have no other exit, so that the lane-0
+// regions inside the body cannot become part of the loop's exit.
+template <bool DUAL>
+__global__ void __launch_bounds__(64) dp_fast_kernel(DpArgs a)
+{
+	extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+	uint8_t *slab = a.pbuf + (size_t)blockIdx.x * a.slab;
+	int n = a.n_jobs;
+	if (a.n_jobs_dev) { const unsigned long long nd = *a.n_jobs_dev; n = nd < (unsigned long long)n ? (int)nd : n; }
+	n = __builtin_amdgcn_readfirstlane(n);
+	for (int j = wave_take(a.next); j < n; j = wave_take(a.next)) {
+		const DpJob jb = a.jobs[j];
+		const JobP<DUAL> P = job_params<DUAL>(jb, a);
+		if (!fast_job<DUAL>(P, a.qt_cap)) {
+			const int pos = wave_take(a.next + 1);
+			if (threadIdx.x == 0) a.defer[pos] = j;
+		} else if (P.right) run_job_fast<DUAL, true>(a, j, jb, P, lds, slab);
+		else run_job_fast<DUAL, false>(a, j, jb, P, lds, slab);
+	}
+}
+
+// the deferred jobs: WIDE = false takes those whose band fits the LDS ring (nb + 17 <= cap),
+// WIDE = true the others, on a 12*gcap-byte ring in this block's device-memory region
 template <bool DUAL, bool WIDE>
 __global__ void __launch_bounds__(64) dp_kernel(DpArgs a)
 {
@@ -783,41 +844,36 @@ __global__ void __launch_bounds__(64) dp_kernel(DpArgs a)
 	if (threadIdx.x < 32) mat[threadIdx.x] = a.mat[threadIdx.x];
 	__syncthreads();
 	uint8_t *slab = a.pbuf + (size_t)blockIdx.x * a.slab;
-	int n = a.n_jobs;
-	if (a.n_jobs_dev) { const unsigned long long nd = *a.n_jobs_dev; n = nd < (unsigned long long)n ? (int)nd : n; }
-	unsigned int *next = a.next && WIDE ? a.next + 1 : a.next;
-	for (int j = blockIdx.x;;) {
-		if (next) { // one job at a time from the shared counter
-			int k = 0;
-			if (threadIdx.x == 0) k = (int)atomicAdd(next, 1u);
-			j = __builtin_amdgcn_readfirstlane(k);
-		}
-		if (j >= n) break;
+	// written by dp_fast_kernel; read once, as a wave-uniform value (a divergent count would turn
+	// the job loop, and the barriers and lane-0 stores inside it, into divergent control flow)
+	const int n = __builtin_amdgcn_readfirstlane((int)*(volatile unsigned int*)(a.next + 1));
+	if (n == 0) return;
+	unsigned int *next = a.next + (WIDE ? 3 : 2);
+	for (int i = wave_take(next); i < n; i = wave_take(next)) {
+		const int j = __builtin_amdgcn_readfirstlane(a.defer[i]);
 		const DpJob jb = a.jobs[j];
-		const JobP<DUAL> P = job_params<DUAL>(jb, a.m, mat);
+		const JobP<DUAL> P = job_params<DUAL>(jb, a);
 		const bool wide = P.nb + 17 > a.cap;
 		if (WIDE && wide) {
 			run_job_lds<DUAL, false, true>(a, j, jb, P, a.gstate + (size_t)blockIdx.x * 12 * a.gcap, mat, slab, a.gcap);
 		} else if (!WIDE && !wide) {
-			const bool staged = ((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= a.qt_cap;
-			// the extension step's jobs: CIGAR, exact max, standard scoring, packed argmax keys
-			const bool fast = staged && !P.skip && P.with_cigar && !P.approx_max && !P.generic && P.nb <= 240 &&
-			                  P.qlen + P.tlen <= 60000;
-			if (fast && P.right) run_job_fast<DUAL, true>(a, j, jb, P, lds, slab);
-			else if (fast) run_job_fast<DUAL, false>(a, j, jb, P, lds, slab);
-			else if (staged) run_job_lds<DUAL, true>(a, j, jb, P, lds, mat, slab, a.cap);
+			if (((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= a.qt_cap) run_job_lds<DUAL, true>(a, j, jb, P, lds, mat, slab, a.cap);
 			else run_job_lds<DUAL, false>(a, j, jb, P, lds, mat, slab, a.cap);
 		}
-		if (!next) j += gridDim.x;
 	}
 }
 
 hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st)
 {
+	const size_t fshm = (size_t)kFastState + (size_t)a.qt_cap + 32;
+	if (dual) hipLaunchKernelGGL(dp_fast_kernel<true>, dim3(grid), dim3(64), fshm, st, a);
+	else hipLaunchKernelGGL(dp_fast_kernel<false>, dim3(grid), dim3(64), fshm, st, a);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess) return e;
 	const size_t shm = (size_t)12 * a.cap + (size_t)a.qt_cap;
 	if (dual) hipLaunchKernelGGL((dp_kernel<true, false>), dim3(grid), dim3(64), shm, st, a);
 	else hipLaunchKernelGGL((dp_kernel<false, false>), dim3(grid), dim3(64), shm, st, a);
-	hipError_t e = hipGetLastError();
+	e = hipGetLastError();
 	if (e != hipSuccess || a.gcap == 0) return e;
 	if (dual) hipLaunchKernelGGL((dp_kernel<true, true>), dim3(grid), dim3(64), 0, st, a);
 	else hipLaunchKernelGGL((dp_kernel<false, true>), dim3(grid), dim3(64), 0, st, a);

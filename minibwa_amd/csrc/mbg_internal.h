@@ -68,10 +68,26 @@ struct DpArgs {
 	int32_t qt_cap;         // LDS bytes after the 12*cap region for staging a job's query and target
 	uint8_t *gstate;        // when gcap > 0: one 12*gcap-byte row-state region per resident block
 	int32_t gcap;           // ring length (power of two) for jobs whose band needs more than cap - 17
-	unsigned int *next;     // when set: zeroed job counter, jobs are taken one at a time
+	unsigned int *next;     // four zeroed job counters (jobs are taken one at a time)
+	int32_t *defer;         // n_jobs entries: jobs the fast kernel leaves to the general ones
+	int32_t mat_last, mat_max, mat_min;  // mat[m*m-1] and the extremes (ksw2_extd2_sse.c:77,87-90), as
+	                                     // dwords so that they load as scalars
 	int8_t m;
 	int8_t mat[32];
 };
+
+// fills DpArgs' matrix fields from an m x m scoring matrix
+inline void dp_set_matrix(DpArgs &a, int8_t m, const int8_t *mat)
+{
+	a.m = m;
+	for (int i = 0; i < 32; ++i) a.mat[i] = i < m * m ? mat[i] : 0;
+	a.mat_last = m > 0 ? a.mat[m * m - 1] : 0;
+	a.mat_max = a.mat[0], a.mat_min = a.mat[1];     // entries past m*m read as 0
+	for (int t = 1; t < m * m; ++t) {
+		a.mat_max = a.mat_max > a.mat[t] ? a.mat_max : a.mat[t];
+		a.mat_min = a.mat_min < a.mat[t] ? a.mat_min : a.mat[t];
+	}
+}
 
 hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st);
 hipError_t launch_rank2a(const Idx &ix, int64_t n, const uint64_t *k, const uint64_t *l, uint64_t *ck, uint64_t *cl, hipStream_t st);

@@ -174,3 +174,15 @@ def test_dp_wide_band(gpu_ctx, dual):
     out = (gpu_ctx.ksw_extd2 if dual else gpu_ctx.ksw_extz2)(Q, T, _params(P), mat)
     _check(out, Q, T, P, "wide", dual)
     assert int(out.res["cells"].max()) > 4096 * 4096
+
+
+@pytest.mark.parametrize("dual", [True, False])
+def test_dp_many_jobs_per_wave(gpu_ctx, dual):
+    """More jobs than resident waves (16384 > the 32 waves per CU the launch keeps), mixing the jobs
+    the fast kernel runs with those it defers to the general kernel (score-only, approximate max,
+    generic scoring), so that waves run fast jobs after deferring others."""
+    from oracle import oracle as O
+    Q, T, P = fuzz_jobs(16384, 41, qmax=48)
+    mat = O.gen_nt4_mat(2, 8, 0, 1)
+    out = (gpu_ctx.ksw_extd2 if dual else gpu_ctx.ksw_extz2)(Q, T, _params(P), mat)
+    _check(out, Q, T, P, "many", dual)
