@@ -700,6 +700,8 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 	int8_t *S = Y2 + kFastCap;
 	int32_t *H = (int32_t*)(lds + 8 * kFastCap);
 	const int qlen = P.qlen, tlen = P.tlen, tpad = P.tl_hi + 1;
+	// scalar copies: a select between members of P would become a select of their addresses
+	const int8_t m1 = P.m1, sc_N = P.sc_N, sc_mch = P.sc_mch, sc_mis = P.sc_mis;
 	uint8_t *Ts = lds + kFastState, *Qr = Ts + tpad + 16;
 	{
 		const uint8_t *Qg = a.qseq + jb.qoff, *Tg = a.tseq + jb.toff;
@@ -742,7 +744,7 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 		const int en1 = st0 + ((en0 - st0) >> 2 << 2);
 		const bool exact = r > 0;
 		uint32_t bk = 0;
-		uint8_t *prow = slab + (size_t)r * P.nb - st;
+		uint8_t *prow = slab + (size_t)r * P.nb;        // cell t at prow[t - st]
 		const uint8_t *qrow = Qr + (qlen - 1 - r);   // qrow[t] = query[r - t] for t <= r, 0 up to r + 15
 		int cx = x1, cv = v1, cx2 = x21;
 		for (int base = st; base <= end; base += 64) {
@@ -752,7 +754,7 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 			const int32_t oh = H[sl];
 			const int tc = t < end ? t : end;
 			const int8_t ta = (int8_t)Ts[tc], qa = (int8_t)qrow[tc];
-			const int8_t fs = (ta == P.m1 || qa == P.m1) ? P.sc_N : ta == qa ? P.sc_mch : P.sc_mis;
+			const int8_t fs = ((ta == m1) | (qa == m1)) ? sc_N : ta == qa ? sc_mch : sc_mis;
 			zs = (unsigned)(t - st0) <= (unsigned)(shi - st0) ? fs : zs;
 			int8_t xt1 = (int8_t)wave_shr1_fill((int)ox, cx), vt1 = (int8_t)wave_shr1_fill((int)ov, cv);
 			int8_t x2t1 = DUAL ? (int8_t)wave_shr1_fill((int)ox2, cx2) : 0;
@@ -769,14 +771,17 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 			const uint8_t d = dp_cell_cg<DUAL, RIGHT>(P, xt1, vt1, x2t1, ou, oy, oy2, zs, nU, nV, nX, nY, nX2, nY2);
 			const bool band = exact && t >= st0 && t <= en0;
 			const int32_t hc = t < en0 ? oh + h_term<DUAL>(P, nV) : Hprev + h_term<DUAL>(P, en0 > 0 ? nU : nV);
-			const uint32_t key = t < en0 ? argmax_key8(hc, t, st0, en1) : (((uint32_t)(hc + (1 << 23)) << 8) | 255u);
+			// argmax_key8 without branches: 254 - ((d&3)*63 + (d>>2)) == 254 - ((d + 251*(d&3)) >> 2)
+			const int dd = t - st0;
+			const uint32_t rank = t >= en0 ? 255u : t < en1 ? (uint32_t)(254 - ((dd + 251 * (dd & 3)) >> 2)) : (uint32_t)(2 + en1 - t);
+			const uint32_t key = ((uint32_t)(hc + (1 << 23)) << 8) | rank;
 			bk = band && key > bk ? key : bk;
 			if (t <= end) {
 				S[sl] = zs;
 				if (t <= en) {
 					U[sl] = nU, V[sl] = nV, X[sl] = nX, Y[sl] = nY;
 					if (DUAL) X2[sl] = nX2, Y2[sl] = nY2;
-					prow[t] = d;
+					prow[(uint32_t)(t - st)] = d;
 				}
 				if (band) H[sl] = hc;
 			}
