@@ -675,10 +675,20 @@ __device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_read
 constexpr int kFastMaxNb = kFastCap - 32;
 constexpr int kFastState = 12 * kFastCap;     // U V X Y X2 Y2 S (int8) and H (int32)
 
+// the argmax key of the fast kernel: (H + 2^22) << 9 | rank, rank 511 for en0, 508 - (class << 6 |
+// position) for body cells (class (t - st0) % 4, position (t - st0) / 4, at most 61), 2 - offset
+// for the tail cells; |H| < 2^22 holds for qlen + tlen <= 30000
+__device__ __forceinline__ void argmax_decode9(uint32_t k, int st0, int en0, int en1, int32_t &max_H, int &max_t)
+{
+	max_H = (int32_t)(k >> 9) - (1 << 22);
+	const int rank = (int)(k & 511u), v = 508 - rank;
+	max_t = rank == 511 ? en0 : rank >= 255 ? st0 + ((v & 63) << 2) + (v >> 6) : en1 + (2 - rank);
+}
+
 template <bool DUAL>
 __device__ __forceinline__ bool fast_job(const JobP<DUAL> &P, int qt_cap)
 {
-	return !P.skip && P.with_cigar && !P.approx_max && !P.generic && P.nb <= kFastMaxNb && P.qlen + P.tlen <= 60000 &&
+	return !P.skip && P.with_cigar && !P.approx_max && !P.generic && P.nb <= kFastMaxNb && P.qlen + P.tlen <= 30000 &&
 	       ((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= qt_cap;
 }
 
@@ -771,26 +781,25 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 			const uint8_t d = dp_cell_cg<DUAL, RIGHT>(P, xt1, vt1, x2t1, ou, oy, oy2, zs, nU, nV, nX, nY, nX2, nY2);
 			const bool band = exact && t >= st0 && t <= en0;
 			const int32_t hc = t < en0 ? oh + h_term<DUAL>(P, nV) : Hprev + h_term<DUAL>(P, en0 > 0 ? nU : nV);
-			// argmax_key8 without branches: 254 - ((d&3)*63 + (d>>2)) == 254 - ((d + 251*(d&3)) >> 2)
 			const int dd = t - st0;
-			const uint32_t rank = t >= en0 ? 255u : t < en1 ? (uint32_t)(254 - ((dd + 251 * (dd & 3)) >> 2)) : (uint32_t)(2 + en1 - t);
-			const uint32_t key = ((uint32_t)(hc + (1 << 23)) << 8) | rank;
+			const uint32_t rank = t >= en0 ? 511u : t < en1 ? (uint32_t)(508 - (((dd & 3) << 6) | (dd >> 2))) : (uint32_t)(2 + en1 - t);
+			const uint32_t key = ((uint32_t)(hc + (1 << 22)) << 9) | rank;
 			bk = band && key > bk ? key : bk;
-			if (t <= end) {
-				S[sl] = zs;
-				if (t <= en) {
-					U[sl] = nU, V[sl] = nV, X[sl] = nX, Y[sl] = nY;
-					if (DUAL) X2[sl] = nX2, Y2[sl] = nY2;
-					prow[(uint32_t)(t - st)] = d;
-				}
-				if (band) H[sl] = hc;
-			}
+			// every lane stores: lanes past en write back what they loaded (the ring never maps two
+			// indices of one trip to a slot), and the CIGAR bytes past en land beyond the row's
+			// [st, en], which the next rows overwrite and the backtrack never reads
+			const bool core = t <= en;
+			S[sl] = zs;
+			U[sl] = core ? nU : ou, V[sl] = core ? nV : ov, X[sl] = core ? nX : ox, Y[sl] = core ? nY : oy;
+			if (DUAL) X2[sl] = core ? nX2 : ox2, Y2[sl] = core ? nY2 : oy2;
+			prow[(uint32_t)(t - st)] = d;
+			H[sl] = band ? hc : oh;
 		}
 		__builtin_amdgcn_wave_barrier();
 		int32_t max_H, Hen0, Hst0;
 		int max_t;
 		if (r > 0) {
-			argmax_decode8(wave_max_u32(bk), st0, en0, en1, max_H, max_t);
+			argmax_decode9(wave_max_u32(bk), st0, en0, en1, max_H, max_t);
 			Hen0 = uni(H[en0 & kFastMask]), Hst0 = uni(H[st0 & kFastMask]);
 		} else {
 			max_H = DUAL ? uni(V[0]) - P.qe0 : uni((uint8_t)V[0]) - P.qe - P.qe;
