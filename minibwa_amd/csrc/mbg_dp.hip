@@ -796,19 +796,36 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 			H[sl] = band ? hc : oh;
 		}
 		__builtin_amdgcn_wave_barrier();
-		int32_t max_H, Hen0, Hst0;
+		int32_t max_H;
 		int max_t;
 		if (r > 0) {
 			argmax_decode9(wave_max_u32(bk), st0, en0, en1, max_H, max_t);
-			Hen0 = uni(H[en0 & kFastMask]), Hst0 = uni(H[st0 & kFastMask]);
 		} else {
 			max_H = DUAL ? uni(V[0]) - P.qe0 : uni((uint8_t)V[0]) - P.qe - P.qe;
 			max_t = 0;
 			if (lane == 0) H[0] = max_H;
-			Hen0 = Hst0 = max_H;
 		}
 		__builtin_amdgcn_wave_barrier();
-		if (finish_exact<DUAL>(P, z, r, st0, en0, max_H, max_t, Hen0, Hst0, last_max_H)) break;
+		// finish_exact with H[en0] and H[st0] read only in the rows that use them
+		const bool at_tend = en0 == tlen - 1;
+		int32_t Hen0 = max_H;
+		if (at_tend) {
+			if (r > 0) Hen0 = uni(H[en0 & kFastMask]);
+			if (Hen0 > z.mte) z.mte = Hen0, z.mte_q = r - en0;
+		}
+		if (r - st0 == qlen - 1) {
+			const int32_t Hst0 = r > 0 ? uni(H[st0 & kFastMask]) : max_H;
+			if (Hst0 > z.mqe) z.mqe = Hst0, z.mqe_t = st0;
+		}
+		if (apply_zdrop(z, max_H, r, max_t, P.zdrop, DUAL ? P.e2 : P.e)) break;
+		if (at_tend && r == qlen + tlen - 2) z.score = Hen0;
+		if (DUAL && (P.flag & 0x40)) { // ksw2_extd2_sse.c:361-367
+			const int32_t rH = last_max_H > max_H ? last_max_H : max_H;
+			const int32_t rq = qlen - (r - st0), rt = tlen - en0;
+			const int32_t rm = rq >= tlen - st0 ? tlen - st0 : rt >= qlen - (r - en0) ? qlen - (r - en0) : tlen + qlen - 1 - r;
+			if (rH + rm * P.max_sc + P.end_bonus < z.max) break;
+		}
+		last_max_H = max_H;
 		last_st = st, last_en = en;
 	}
 	__syncthreads();
