@@ -111,6 +111,24 @@ struct Window {
 			if ((int64_t)ambi[a * 2] <= p && p < (int64_t)ambi[a * 2 + 1]) return 4;
 		return (uint8_t)(pac[p >> 5] >> ((p & 31) << 1) & 3);
 	}
+	__device__ bool plain() const { return a0 == a1; }   // no N run overlaps the window
+};
+
+// the bases of a Window without N runs in order, one 64-bit pac load per 32 bases
+struct WinCursor {
+	const uint64_t *pac;
+	int64_t p, wi = -1;
+	int step;
+	uint64_t w = 0;
+	__device__ explicit WinCursor(const Window &t) : pac(t.pac), p(t.reverse ? t.gen - 1 : t.gst), step(t.reverse ? -1 : 1) {}
+	__device__ uint8_t next()
+	{
+		const int64_t k = p >> 5;
+		if (k != wi) wi = k, w = pac[k];
+		const uint8_t c = (uint8_t)(w >> ((p & 31) << 1) & 3);
+		p += step;
+		return c;
+	}
 };
 
 // the read as aligned: forward, or reverse-complemented with N kept (align.c:902-903), then a
@@ -148,19 +166,32 @@ struct ExtPlan {
 	Window t{};
 };
 
-__device__ __forceinline__ void plan_ext(const MapOpt &o, ExtPlan &p)
+// ungapped score of the extension: n_mm (starting at -1, as in align.c:337), final score and max
+template <class TB>
+__device__ __forceinline__ void ungapped(const int8_t *mat, const QView &q, int qlen, TB &&tb, int &n_mm, int32_t &score, int32_t &mx)
+{
+	for (int j = 0; j < qlen; ++j) {
+		const uint8_t tj = tb(j), qj = q(j);
+		const int32_t sc = mat[tj * 5 + qj];
+		score += sc;
+		n_mm += (tj > 3 || qj > 3 || sc < 0);
+		if (mx < score) mx = score;
+	}
+}
+
+// mat: the 5x5 scoring matrix staged in LDS
+__device__ __forceinline__ void plan_ext(const MapOpt &o, const int8_t *mat, ExtPlan &p)
 {
 	const int qlen = p.q.n, tlen = p.tlen;
 	int w = o.ksw_w, n_mm = -1;
 	p.qlen = qlen;
 	if (tlen >= qlen) {
 		int32_t score = 0, mx = 0;
-		for (int j = 0; j < qlen; ++j) {
-			const uint8_t tj = p.t(j), qj = p.q(j);
-			const int32_t sc = o.mat[tj * 5 + qj];
-			score += sc;
-			n_mm += (tj > 3 || qj > 3 || sc < 0);
-			if (mx < score) mx = score;
+		if (p.t.plain()) {
+			WinCursor c(p.t);
+			ungapped(mat, p.q, qlen, [&](int) { return c.next(); }, n_mm, score, mx);
+		} else {
+			ungapped(mat, p.q, qlen, [&](int j) { return p.t(j); }, n_mm, score, mx);
 		}
 		if (n_mm <= 2 && score + o.end_bonus >= mx) { p.fast = true; return; }
 	}
@@ -177,7 +208,12 @@ __device__ __forceinline__ void write_job(const MapOpt &o, const JobSink &js, co
 {
 	if ((int64_t)j >= js.job_cap || (int64_t)(qo + p.qlen) > js.qcap || (int64_t)(to + p.tlen) > js.tcap) return;
 	for (int i = 0; i < p.qlen; ++i) js.qbuf[qo + i] = p.q(i);
-	for (int i = 0; i < p.tlen; ++i) js.tbuf[to + i] = p.t(i);
+	if (p.t.plain()) {
+		WinCursor c(p.t);
+		for (int i = 0; i < p.tlen; ++i) js.tbuf[to + i] = c.next();
+	} else {
+		for (int i = 0; i < p.tlen; ++i) js.tbuf[to + i] = p.t(i);
+	}
 	DpJob d;
 	d.qoff = (int64_t)qo; d.toff = (int64_t)to; d.qlen = p.qlen; d.tlen = p.tlen;
 	d.q = (int8_t)o.q; d.e = (int8_t)o.e; d.q2 = (int8_t)o.q2; d.e2 = (int8_t)o.e2;
@@ -200,6 +236,9 @@ __global__ void __launch_bounds__(128) ext_jobs_kernel(L2b lb, MapOpt o, int32_t
                                                       const uint8_t *seq, const SeedRec *recs, const uint32_t *first_slot,
                                                       const uint64_t *pos, const unsigned long long *best, JobSink js)
 {
+	__shared__ int8_t mat[28];
+	if (threadIdx.x < 28) mat[threadIdx.x] = o.mat[threadIdx.x];
+	__syncthreads();
 	const int32_t rid = blockIdx.x * blockDim.x + threadIdx.x;
 	ExtPlan L, R;
 	bool anchored = false;
@@ -238,7 +277,7 @@ __global__ void __launch_bounds__(128) ext_jobs_kernel(L2b lb, MapOpt o, int32_t
 						L.q = QView{r, qlen, 0, rev != 0, true, qs};
 						L.t.init(lb, coff + ts0, coff + ts, true);
 						L.tlen = (int)(ts - ts0), L.flag = 0x40 | 0x02 | 0x80, L.side = 0, L.tpos = ts;
-						plan_ext(o, L);
+						plan_ext(o, mat, L);
 					}
 					if (qe < qlen && te < clen) { // right extension
 						int64_t l = qlen - qe;
@@ -247,7 +286,7 @@ __global__ void __launch_bounds__(128) ext_jobs_kernel(L2b lb, MapOpt o, int32_t
 						R.q = QView{r, qlen, qe, rev != 0, false, qlen - qe};
 						R.t.init(lb, coff + te, coff + te0, false);
 						R.tlen = (int)(te0 - te), R.flag = 0x40, R.side = 1, R.tpos = te;
-						plan_ext(o, R);
+						plan_ext(o, mat, R);
 					}
 				}
 			}
