@@ -54,7 +54,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ref-len", type=int, default=3_100_000_000)
     ap.add_argument("--contigs", type=int, default=24)
-    ap.add_argument("--pre-len", type=int, default=14,
+    ap.add_argument("--pre-len", type=int, default=15,
                     help="k-mer interval cache length on the device (the reference uses 10; results do not depend on it)")
     ap.add_argument("--rep-frac", type=float, default=0.08)
     ap.add_argument("--pairs", type=int, default=1_000_000)

@@ -169,7 +169,7 @@ static int upload(void **dst, const void *src, size_t bytes, hipStream_t st)
 
 static int bwt_upload_locked(mbg_ctx_t *c, const mbg_bwt_desc_t *d, int32_t pre_len)
 {
-	if (d == nullptr || d->data == nullptr || pre_len < 0 || pre_len > 14) return MBG_EARG;
+	if (d == nullptr || d->data == nullptr || pre_len < 0 || pre_len > 16) return MBG_EARG;
 	if (d->L2[4] != d->seq_len || d->primary > d->seq_len) return MBG_EFORMAT;
 	if (d->sa_bit > 30 || d->n_sa != ((d->seq_len + (1ULL << d->sa_bit)) >> d->sa_bit) || d->sa == nullptr) return MBG_EFORMAT;
 	free_bwt(c);

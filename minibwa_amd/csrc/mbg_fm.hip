@@ -17,13 +17,13 @@ namespace mbg {
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) kmer_cache_kernel(Idx ix, Sai *pre, int len, uint64_t n)
 {
-	const uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (z >= n) return;
-	Sai p = set_intv(ix, (int)(z >> (2 * (len - 1)) & 3));
-	for (int d = len - 2; d >= 0; --d)
-		p = extend1(ix, p, 1, (int)(z >> (2 * d) & 3));
-	p.info = 0;
-	pre[z] = p;
+	for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < n; z += (uint64_t)gridDim.x * blockDim.x) {
+		Sai p = set_intv(ix, (int)(z >> (2 * (len - 1)) & 3));
+		for (int d = len - 2; d >= 0; --d)
+			p = extend1(ix, p, 1, (int)(z >> (2 * d) & 3));
+		p.info = 0;
+		pre[z] = p;
+	}
 }
 
 __global__ void __launch_bounds__(256) rank2a_kernel(Idx ix, int64_t n, const uint64_t *k, const uint64_t *l, uint64_t *ck, uint64_t *cl)
@@ -476,7 +476,9 @@ hipError_t launch_pack_reads(const uint8_t *seq, const int64_t *off, const int32
 hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st)
 {
 	const uint64_t n = 1ULL << (2 * len);
-	hipLaunchKernelGGL(kmer_cache_kernel, dim3(blocks_for((int64_t)n, 256)), dim3(256), 0, st, ix, pre, len, n);
+	// a dispatch holds fewer than 2^32 work-items: larger tables are covered by a grid-stride loop
+	const int64_t nb = blocks_for((int64_t)n, 256);
+	hipLaunchKernelGGL(kmer_cache_kernel, dim3((unsigned)(nb < (1 << 22) ? nb : (1 << 22))), dim3(256), 0, st, ix, pre, len, n);
 	return hipGetLastError();
 }
 

@@ -146,12 +146,13 @@ def test_seed_cfg0(gpu_ctx):
         assert np.array_equal(_canon(g[i]), _canon(r[i])), i
 
 
-@pytest.mark.parametrize("pre_len", [0, 12, 14])
+@pytest.mark.parametrize("pre_len", [0, 12, 14, 15])
 def test_seed_cache_length_invariant(chrm_prefix, ref_chrm, chrm_reads, pre_len):
     """Seeding with no k-mer cache or a longer one than the reference's 10 gives the reference's
     intervals: a cached interval is the state stepwise extension reaches, and one below min_occ
-    falls back to stepwise (bwt.c:403-406).  On chrM most 12- and 14-mers are absent, so the
-    fallback is taken often; pass 2 (min_occ = size+1) exercises it with larger thresholds."""
+    falls back to stepwise (bwt.c:403-406).  On chrM most 12- to 15-mers are absent, so the
+    fallback is taken often; pass 2 (min_occ = size+1) exercises it with larger thresholds, and
+    min_len 13 < 14, 15 runs the searches that start without a lookup."""
     from minibwa_amd import mbg
     reads = chrm_reads[:600]
     with mbg.Context(0) as c:
