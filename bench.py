@@ -110,9 +110,10 @@ def main():
         sys.exit(1)
     log(r.stdout.strip(), f"(wall {time.time() - t0:.1f}s)")
     ctx = mbg.Context(0)
+    t0 = time.time()
     ctx.idx_load(prefix, args.pre_len)
     info = ctx.info()
-    log(f"index resident: {info['device_bytes'] / 1e9:.2f} GB")
+    log(f"index resident: {info['device_bytes'] / 1e9:.2f} GB (load, k-mer cache and SA grid in {time.time() - t0:.1f}s)")
     if args.gather:
         rows = []
         for chains in (262144, 1048576, 4194304):

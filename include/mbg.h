@@ -91,10 +91,13 @@ void mbg_destroy(mbg_ctx_t *ctx);
 
 /* Index residency.  Replaces mb_bwt_load + mb_bwt_cache (bwt.c:585-591,648-683; map-algo.c:19-42)
  * and l2b_load (l2bit.c:275-330).  pre_len is the k-mer interval cache length, 0..16 (10 in the
- * reference, 0 for none; 4^pre_len x 32 bytes of HBM: 8.6 GB at 14, 34 GB at 15, 137 GB at 16), computed on the device.  Seeding results do
- * not depend on it: an interval only shrinks as its pattern grows, so a cached k-mer interval is
- * the state stepwise extension would reach, and a too-small one falls back to stepwise
- * (bwt.c:403-406).  A longer cache skips more dependent steps. */
+ * reference, 0 for none; 4^pre_len x 32 bytes of HBM: 8.6 GB at 14, 34 GB at 15, 137 GB at 16),
+ * computed on the device.  Seeding results do not depend on it: an interval only shrinks as its
+ * pattern grows, so a cached k-mer interval is the state stepwise extension would reach, and a
+ * too-small one falls back to stepwise (bwt.c:403-406).  A longer cache skips more dependent
+ * steps.  The load also builds the suffix array of every 4th row (rows/4 x 8 bytes, by the same
+ * walk mb_bwt_sa_batch takes), which shortens locates without changing their results; it is
+ * skipped when the device cannot hold it. */
 int mbg_idx_load(mbg_ctx_t *ctx, const char *prefix, int32_t pre_len);
 int mbg_bwt_upload(mbg_ctx_t *ctx, const mbg_bwt_desc_t *bwt, int32_t pre_len);
 int mbg_l2b_upload(mbg_ctx_t *ctx, const mbg_l2b_desc_t *l2b);

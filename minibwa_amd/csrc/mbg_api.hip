@@ -33,7 +33,7 @@ struct mbg_ctx_s {
 	std::mutex mu;
 	// resident index
 	bool has_bwt = false, has_l2b = false;
-	uint64_t *d_data = nullptr, *d_sa = nullptr;
+	uint64_t *d_data = nullptr, *d_sa = nullptr, *d_sa_fine = nullptr;
 	Sai *d_pre = nullptr;
 	Idx ix{};
 	uint64_t *d_pac = nullptr, *d_ambi = nullptr, *d_ctg_off = nullptr, *d_ctg_len = nullptr;
@@ -116,8 +116,9 @@ static void free_bwt(mbg_ctx_t *c)
 {
 	if (c->d_data) hipFree(c->d_data);
 	if (c->d_sa) hipFree(c->d_sa);
+	if (c->d_sa_fine) hipFree(c->d_sa_fine);
 	if (c->d_pre) hipFree(c->d_pre);
-	c->d_data = c->d_sa = nullptr, c->d_pre = nullptr;
+	c->d_data = c->d_sa = c->d_sa_fine = nullptr, c->d_pre = nullptr;
 	c->has_bwt = false;
 }
 
@@ -167,6 +168,8 @@ static int upload(void **dst, const void *src, size_t bytes, hipStream_t st)
 	return MBG_OK;
 }
 
+static const uint32_t kFineBit = 2;     // rows between entries of the finer SA grid: 4
+
 static int bwt_upload_locked(mbg_ctx_t *c, const mbg_bwt_desc_t *d, int32_t pre_len)
 {
 	if (d == nullptr || d->data == nullptr || pre_len < 0 || pre_len > 16) return MBG_EARG;
@@ -182,7 +185,22 @@ static int bwt_upload_locked(mbg_ctx_t *c, const mbg_bwt_desc_t *d, int32_t pre_
 	ix.primary = d->primary; ix.seq_len = d->seq_len;
 	for (int i = 0; i < 5; ++i) ix.L2[i] = d->L2[i];
 	ix.pre = nullptr; ix.pre_len = 0;
+	ix.sa_fine = nullptr; ix.fine_bit = 0;
 	c->idx_bytes = dl * 8 + d->n_sa * 8;
+	// SA of every 4th row (rows/4 x 8 bytes), so that a locate walks about 3 LF steps instead of
+	// about 2^sa_bit - 1; skipped when the device cannot hold it
+	if (d->sa_bit > kFineBit) {
+		const uint64_t n_fine = (d->seq_len >> kFineBit) + 1;
+		if (hipMalloc((void**)&c->d_sa_fine, n_fine * 8) == hipSuccess) {
+			ix.fine_bit = kFineBit;
+			HIPCHK(launch_sa_fine(ix, c->d_sa_fine, n_fine, c->stream));
+			ix.sa_fine = c->d_sa_fine;
+			c->idx_bytes += n_fine * 8;
+		} else {
+			(void)hipGetLastError();
+			c->d_sa_fine = nullptr;
+		}
+	}
 	if (pre_len > 0) {
 		const size_t n = (size_t)1 << (2 * pre_len);
 		hipError_t e = hipMalloc((void**)&c->d_pre, n * sizeof(Sai));

@@ -16,10 +16,11 @@ struct Idx {
 	const uint64_t *data;   // occ blocks
 	uint64_t data_len;      // in u64 words
 	const uint64_t *sa;     // sampled SA
+	const uint64_t *sa_fine;   // SA of every 2^fine_bit-th row, from the same walk (nullptr if absent)
 	const Sai *pre;         // k-mer interval cache (nullptr if pre_len == 0)
 	uint64_t primary, seq_len;
 	uint64_t L2[5];
-	uint32_t sa_bit;
+	uint32_t sa_bit, fine_bit;
 	int32_t pre_len;
 };
 

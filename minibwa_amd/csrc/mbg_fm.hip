@@ -360,10 +360,25 @@ __global__ void __launch_bounds__(256) compact_kernel(const SeedRec *slots1, con
 // SA locate (mb_bwt_sa / mb_bwt_sa_batch, bwt.c:499-541): LF-walk each row to the nearest sampled
 // row; lanes refill from a counter because walk lengths are geometric (mean 2^sa_bit).
 // ---------------------------------------------------------------------------------------------
+// SA of rows i << fine_bit: the walk of mb_bwt_sa_batch (bwt.c:511-541) from each, so that a walk
+// from any row may stop at the first row of that finer grid: the result is steps + fine value, the
+// same number the walk to a sampled row gives
+__global__ void __launch_bounds__(256) sa_fine_kernel(Idx ix, uint64_t *fine, uint64_t n_fine)
+{
+	const uint64_t mask = (1ULL << ix.sa_bit) - 1;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_fine; i += (uint64_t)gridDim.x * blockDim.x) {
+		uint64_t k = i << ix.fine_bit, steps = 0;
+		while ((k & mask) != 0) k = lf(ix, k), ++steps;
+		fine[i] = steps + ix.sa[k >> ix.sa_bit];
+	}
+}
+
 __global__ void __launch_bounds__(256) sa_kernel(Idx ix, int64_t n, const int64_t *n_dev, uint64_t *x, uint32_t *next,
                                                 unsigned long long *stats)
 {
-	const uint64_t mask = (1ULL << ix.sa_bit) - 1;
+	const uint32_t bit = ix.sa_fine ? ix.fine_bit : ix.sa_bit;
+	const uint64_t *samp = ix.sa_fine ? ix.sa_fine : ix.sa;
+	const uint64_t mask = (1ULL << bit) - 1;
 	if (n_dev) { const int64_t nd = *n_dev; n = nd < n ? nd : n; }
 	bool have = false;
 	uint32_t idx = 0;
@@ -376,7 +391,7 @@ __global__ void __launch_bounds__(256) sa_kernel(Idx ix, int64_t n, const int64_
 			idx = t; k = x[t]; steps = 0; have = true;
 		}
 		if ((k & mask) == 0) {
-			x[idx] = steps + ix.sa[k >> ix.sa_bit];
+			x[idx] = steps + samp[k >> bit];
 			tot += steps;
 			have = false;
 			continue;
@@ -479,6 +494,13 @@ hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st)
 	// a dispatch holds fewer than 2^32 work-items: larger tables are covered by a grid-stride loop
 	const int64_t nb = blocks_for((int64_t)n, 256);
 	hipLaunchKernelGGL(kmer_cache_kernel, dim3((unsigned)(nb < (1 << 22) ? nb : (1 << 22))), dim3(256), 0, st, ix, pre, len, n);
+	return hipGetLastError();
+}
+
+hipError_t launch_sa_fine(const Idx &ix, uint64_t *fine, uint64_t n_fine, hipStream_t st)
+{
+	const int64_t nb = blocks_for((int64_t)n_fine, 256);
+	hipLaunchKernelGGL(sa_fine_kernel, dim3((unsigned)(nb < (1 << 22) ? nb : (1 << 22))), dim3(256), 0, st, ix, fine, n_fine);
 	return hipGetLastError();
 }
 

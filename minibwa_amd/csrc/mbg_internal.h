@@ -90,6 +90,7 @@ inline void dp_set_matrix(DpArgs &a, int8_t m, const int8_t *mat)
 }
 
 hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st);
+hipError_t launch_sa_fine(const Idx &ix, uint64_t *fine, uint64_t n_fine, hipStream_t st);
 hipError_t launch_rank2a(const Idx &ix, int64_t n, const uint64_t *k, const uint64_t *l, uint64_t *ck, uint64_t *cl, hipStream_t st);
 hipError_t launch_extend(const Idx &ix, int64_t n, const Sai *ik, const uint8_t *is_back, Sai *ok, hipStream_t st);
 hipError_t launch_smem(const Idx &ix, const SeedArgs &a, uint32_t pass, unsigned grid, hipStream_t st);
