@@ -395,7 +395,7 @@ static unsigned persistent_grid(mbg_ctx_t *c, int64_t n_items, int per_block)
 // record count at *n_out_dev.  Per-task output slots (kSlot1 per read, kSlot2 per pass-2 task) are
 // compacted by an exclusive scan, so no output position needs an atomic except overflow spills.
 // Counters in c->b_ctr: u32 [0] task counter, [1] pass-2 task count, i64 [1] record count,
-// u64 [3] spill count (u32 [4] is the SA row counter, u64 [6] the gather checksum, u32 [14..17] the
+// u64 [3] spill count (u32 [4] is the SA row counter, u64 [6] the gather checksum, u32 [14..18] the
 // DP job counters).  ev (optional) receives events after pass 1 and after pass 2.
 static const int32_t kSlot1 = 16, kSlot2 = 8;
 
@@ -600,7 +600,7 @@ static const int kMaxGcap = 1 << 22;    // circular row buffer in device memory,
 
 static const int kMaxQt = 16384;        // LDS bytes for staging a job's query + target
 
-static int dp_plan(int32_t max_qlen, int32_t max_tlen, int32_t max_w, int *cap_out, int64_t *slab_out, int *qt_out)
+static int dp_plan(int32_t max_qlen, int32_t max_tlen, int32_t max_w, int *cap_out, int64_t *slab_out, int *qt_out, int *tcap_out)
 {
 	int w, n_col_;
 	dp_geom(max_qlen, max_tlen, max_w, &w, &n_col_);
@@ -614,13 +614,15 @@ static int dp_plan(int32_t max_qlen, int32_t max_tlen, int32_t max_w, int *cap_o
 	// jobs whose query and target fit are staged in LDS and run on the register schedule
 	const int64_t qt = (((int64_t)max_qlen + 15) & ~15) + (((int64_t)max_tlen + 15) & ~15);
 	*qt_out = qt <= kMaxQt ? (int)qt : 0;
+	*tcap_out = dp_team_cap(nb);         // bands over the fast kernel's ring run on a team of waves
 	return MBG_OK;
 }
 
 // ring lengths for a launch of `g` blocks whose widest band needs `cap`: jobs up to kMaxCap run on
 // the LDS ring, wider ones on a per-block ring of 12*cap bytes in device memory
-static int dp_state(mbg_ctx_t *c, int64_t g, int cap, DpArgs &a)
+static int dp_state(mbg_ctx_t *c, int64_t g, int cap, int tcap, DpArgs &a)
 {
+	a.tcap = tcap;
 	a.cap = cap < kMaxCap ? cap : kMaxCap;
 	a.gcap = 0, a.gstate = nullptr;
 	if (cap > kMaxCap) {
@@ -631,7 +633,7 @@ static int dp_state(mbg_ctx_t *c, int64_t g, int cap, DpArgs &a)
 }
 
 static int dp_dev_locked(mbg_ctx_t *c, hipStream_t st, bool dual, int32_t n_jobs, const DpJob *jobs, const uint8_t *qseq, const uint8_t *tseq,
-                         int8_t m, const int8_t *mat, DpRes *res, uint32_t *cigar, int cap, int64_t slab, int qt)
+                         int8_t m, const int8_t *mat, DpRes *res, uint32_t *cigar, int cap, int64_t slab, int qt, int tcap)
 {
 	if (n_jobs <= 0) return MBG_OK;
 	// resident blocks (one wave each): enough to fill the GPU, bounded by 2 GiB of backtrack slabs
@@ -644,10 +646,10 @@ static int dp_dev_locked(mbg_ctx_t *c, hipStream_t st, bool dual, int32_t n_jobs
 	DpArgs a{};
 	a.jobs = jobs; a.n_jobs = n_jobs; a.qseq = qseq; a.tseq = tseq; a.res = res; a.cigar = cigar;
 	a.pbuf = (uint8_t*)c->b_slab.p; a.slab = slab; a.qt_cap = qt; a.m = m;
-	TRY(dp_state(c, g, cap, a));
+	TRY(dp_state(c, g, cap, tcap, a));
 	TRY(dgrow(c->b_ctr, 128));
 	a.next = (unsigned int*)c->b_ctr.p + 14;
-	HIPCHK(hipMemsetAsync(a.next, 0, 16, st));
+	HIPCHK(hipMemsetAsync(a.next, 0, 20, st));
 	TRY(dgrow(c->b_defer, (size_t)n_jobs * 4));
 	a.defer = (int32_t*)c->b_defer.p;
 	dp_set_matrix(a, m, mat);
@@ -677,9 +679,9 @@ static int dp_batch(mbg_ctx_t *c, bool dual, int32_t n_jobs, const mbg_dp_job_t 
 		mt = j.tlen > mt ? j.tlen : mt;
 		mw = w > mw ? w : mw;
 	}
-	int cap, qt;
+	int cap, qt, tcap;
 	int64_t slab;
-	TRY(dp_plan(mq, mt, mw, &cap, &slab, &qt));
+	TRY(dp_plan(mq, mt, mw, &cap, &slab, &qt, &tcap));
 	if (!any_cigar) slab = 256;
 	TRY(dgrow(c->b_in[0], qbytes + 16)); TRY(dgrow(c->b_in[1], tbytes + 16)); TRY(dgrow(c->b_in[4], (size_t)n_jobs * sizeof(DpJob)));
 	TRY(dgrow(c->b_out[0], (size_t)n_jobs * sizeof(DpRes)));
@@ -688,7 +690,7 @@ static int dp_batch(mbg_ctx_t *c, bool dual, int32_t n_jobs, const mbg_dp_job_t 
 	HIPCHK(hipMemcpyAsync(c->b_in[1].p, tseq, tbytes, hipMemcpyHostToDevice, c->stream));
 	HIPCHK(hipMemcpyAsync(c->b_in[4].p, jobs, (size_t)n_jobs * sizeof(DpJob), hipMemcpyHostToDevice, c->stream));
 	TRY(dp_dev_locked(c, c->stream, dual, n_jobs, (const DpJob*)c->b_in[4].p, (const uint8_t*)c->b_in[0].p, (const uint8_t*)c->b_in[1].p,
-	                  m, mat, (DpRes*)c->b_out[0].p, any_cigar ? (uint32_t*)c->b_out[1].p : nullptr, cap, slab, qt));
+	                  m, mat, (DpRes*)c->b_out[0].p, any_cigar ? (uint32_t*)c->b_out[1].p : nullptr, cap, slab, qt, tcap));
 	HIPCHK(hipMemcpyAsync(res, c->b_out[0].p, (size_t)n_jobs * sizeof(DpRes), hipMemcpyDeviceToHost, c->stream));
 	HIPCHK(hipStreamSynchronize(c->stream));
 	if (any_cigar) {
@@ -720,11 +722,11 @@ int mbg_extd2_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_jobs, const mbg_dp
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (n_jobs == 0) return MBG_OK;
 	HIPCHK(hipSetDevice(c->device));
-	int cap, qt;
+	int cap, qt, tcap;
 	int64_t slab;
-	TRY(dp_plan(max_qlen, max_tlen, max_w, &cap, &slab, &qt));
+	TRY(dp_plan(max_qlen, max_tlen, max_w, &cap, &slab, &qt, &tcap));
 	return dp_dev_locked(c, stream ? (hipStream_t)stream : c->stream, true, n_jobs, (const DpJob*)jobs, qseq, tseq, m, mat_host,
-	                     (DpRes*)res, cigar, cap, slab, qt);
+	                     (DpRes*)res, cigar, cap, slab, qt, tcap);
 }
 
 int mbg_getseq(mbg_ctx_t *c, int64_t n, const int64_t *tid, const int64_t *st, const int64_t *en, const int64_t *out_off, uint8_t *out,
@@ -860,9 +862,9 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 	// job sizes: query <= max_len, target <= l + extra (align.c:590-595)
 	const int64_t l_extra = ((int64_t)max_len * opt->a + opt->end_bonus - opt->q) > 0 ? ((int64_t)max_len * opt->a + opt->end_bonus - opt->q) / opt->e : 0;
 	const int32_t max_tlen = (int32_t)(max_len + l_extra + 1);
-	int cap_dp, qt_dp;
+	int cap_dp, qt_dp, tcap_dp;
 	int64_t slab;
-	TRY(dp_plan(max_len > 0 ? max_len : 1, max_tlen, opt->ksw_w, &cap_dp, &slab, &qt_dp));
+	TRY(dp_plan(max_len > 0 ? max_len : 1, max_tlen, opt->ksw_w, &cap_dp, &slab, &qt_dp, &tcap_dp));
 	const int64_t job_cap = 2 * (int64_t)n_reads + 16;
 	if (c->cap_rows < 16 * (int64_t)n_reads + 1024) c->cap_rows = 16 * (int64_t)n_reads + 1024;
 	for (int attempt = 0; attempt < 4; ++attempt) {
@@ -917,9 +919,9 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 			d.qseq = (const uint8_t*)c->p_qb.p; d.tseq = (const uint8_t*)c->p_tb.p;
 			d.res = (DpRes*)c->p_res.p; d.cigar = (uint32_t*)c->p_cig.p;
 			d.pbuf = (uint8_t*)c->b_slab.p; d.slab = slab; d.qt_cap = qt_dp; d.m = 5;
-			TRY(dp_state(c, g, cap_dp, d));
+			TRY(dp_state(c, g, cap_dp, tcap_dp, d));
 			d.next = (unsigned int*)c->b_ctr.p + 14;
-			HIPCHK(hipMemsetAsync(d.next, 0, 16, st));
+			HIPCHK(hipMemsetAsync(d.next, 0, 20, st));
 			TRY(dgrow(c->b_defer, (size_t)job_cap * 4));
 			d.defer = (int32_t*)c->b_defer.p;
 			dp_set_matrix(d, 5, opt->mat);

@@ -406,8 +406,9 @@ __device__ __forceinline__ bool finish_approx(const JobP<DUAL> &P, DpState &z, i
 }
 
 // backtrack (ksw2_extd2_sse.c:391-402, ksw2.h:140-172) over the rows kept in `slab`, staged through
-// the first `stage` bytes of LDS, or read from the slab itself (DIRECT); returns n_cigar
-template <bool DUAL, bool DIRECT = false>
+// the first `stage` bytes of LDS, or read from the slab itself (DIRECT), by a workgroup of NT
+// threads that all walk the same path; returns n_cigar
+template <bool DUAL, bool DIRECT = false, int NT = 64>
 __device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const DpJob &jb, const DpArgs &a, uint8_t *lds, const uint8_t *slab,
                                          int stage)
 {
@@ -443,7 +444,7 @@ __device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const 
 				const int lo = r - K + 1 > 0 ? r - K + 1 : 0;
 				const int bytes = (r - lo + 1) * nb;
 				__syncthreads();
-				for (int o = lane * 16; o < bytes; o += 64 * 16)
+				for (int o = lane * 16; o < bytes; o += NT * 16)
 					*(uint4*)(stg + o) = *(const uint4*)(slab + (size_t)lo * nb + o);
 				__syncthreads();
 				chunk_lo = lo;
@@ -477,7 +478,7 @@ __device__ __forceinline__ int backtrack(const JobP<DUAL> &P, DpState &z, const 
 	if (last_op >= 0 && lane == 0) cg[n_cigar - 1] = cur;
 	if (!(P.flag & 0x80)) {
 		__syncthreads();
-		for (int k = lane; k < n_cigar / 2; k += 64) {
+		for (int k = lane; k < n_cigar / 2; k += NT) {
 			const uint32_t t0 = cg[k], t1 = cg[n_cigar - 1 - k];
 			cg[k] = t1, cg[n_cigar - 1 - k] = t0;
 		}
@@ -834,6 +835,195 @@ __device__ __forceinline__ void run_job_fast(const DpArgs &a, int jid, const DpJ
 	__syncthreads();
 }
 
+// ---- wide bands: a team of waves per job ----------------------------------------------------------
+//
+// Jobs of the fast kernel's kind whose band is wider than its ring (long-read gap fills, bands of
+// hundreds of cells) run on kTeamWaves waves of one workgroup.  The row state is double-buffered by
+// row parity: row r reads buffer (r-1)&1 and writes buffer r&1, so the waves of a row never wait on
+// each other and one barrier ends each row.  A lane reads its t-1 neighbours from the old buffer
+// instead of a DPP shift.  Each row also carries the 32 indices past its end into the new buffer:
+// the next row reads at most that far beyond, and no index past end + 15 has ever held computed
+// values.  Every wave computes the row's finish from the same LDS values, so control flow stays
+// uniform across the workgroup.  Slots of indices that leave the window are reset at the start of a
+// row; they lie below st0 of the previous row, the lowest index a wave still finishing that row reads.
+// run_job_lds is the reference for the semantics.
+
+constexpr int kTeamWaves = 8, kTeamThreads = 64 * kTeamWaves;
+
+template <bool DUAL>
+__device__ __forceinline__ bool team_job(const JobP<DUAL> &P, int tcap)
+{
+	return tcap > 0 && !P.skip && P.with_cigar && !P.approx_max && !P.generic && P.nb > kFastMaxNb && P.nb + 144 <= tcap &&
+	       P.qlen + P.tlen <= 30000;
+}
+
+template <bool DUAL>
+__device__ __forceinline__ void team_reset(uint8_t *buf, int TC, int slot, int8_t ini, int8_t ini2)
+{
+	int8_t *R = (int8_t*)buf;
+	R[slot] = ini, R[TC + slot] = ini, R[2 * TC + slot] = ini, R[3 * TC + slot] = ini;
+	if (DUAL) R[4 * TC + slot] = ini2, R[5 * TC + slot] = ini2;
+	R[6 * TC + slot] = 0;
+	((int32_t*)(buf + 8 * TC))[slot] = kNegInf;
+}
+
+template <bool DUAL, bool RIGHT>
+__device__ __forceinline__ void run_job_team(const DpArgs &a, int jid, const DpJob &jb, const JobP<DUAL> &P, uint8_t *lds, uint8_t *slab)
+{
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int TC = a.tcap, TM = TC - 1;
+	uint32_t *keys = (uint32_t*)(lds + 24 * TC);                // [2][kTeamWaves]
+	const int qlen = P.qlen, tlen = P.tlen;
+	const uint8_t *Qg = a.qseq + jb.qoff, *Tg = a.tseq + jb.toff;
+	const int8_t m1 = P.m1, sc_N = P.sc_N, sc_mch = P.sc_mch, sc_mis = P.sc_mis;
+	for (int i = tid; i < TC; i += kTeamThreads) {
+		team_reset<DUAL>(lds, TC, i, P.ini, P.ini2);
+		team_reset<DUAL>(lds + 12 * TC, TC, i, P.ini, P.ini2);
+	}
+	__syncthreads();
+	DpState z;
+	init_state(z);
+	int cells = 0;
+	int last_st = -1, last_en = -1, last_max_H = 0, prev_st = 0;
+	for (int r = 0; r < qlen + tlen - 1; ++r) {
+		// buffer offsets computed from the LDS base keep the accesses in LDS instructions
+		uint8_t *bo = lds + ((r + 1) & 1) * 12 * TC, *bn = lds + (r & 1) * 12 * TC;
+		const int8_t *Uo = (const int8_t*)bo, *Vo = Uo + TC, *Xo = Vo + TC, *Yo = Xo + TC, *X2o = Yo + TC, *Y2o = X2o + TC;
+		const int8_t *So = Y2o + TC;
+		const int32_t *Ho = (const int32_t*)(bo + 8 * TC);
+		int8_t *Un = (int8_t*)bn, *Vn = Un + TC, *Xn = Vn + TC, *Yn = Xn + TC, *X2n = Yn + TC, *Y2n = X2n + TC, *Sn = Y2n + TC;
+		int32_t *Hn = (int32_t*)(bn + 8 * TC);
+		int st0, en0;
+		if (!row_range(r, qlen, tlen, P.w, st0, en0)) { z.zdropped = 1; break; }
+		cells += en0 - st0 + 1;
+		const int st = st0 >> 4 << 4, en = ((en0 + 16) >> 4 << 4) - 1;
+		if (st != prev_st) { // indices [prev_st-1, st-1) leave the window
+			for (int p = prev_st - 1 + tid; p < st - 1; p += kTeamThreads)
+				if (p >= 0) team_reset<DUAL>(lds, TC, p & TM, P.ini, P.ini2), team_reset<DUAL>(lds + 12 * TC, TC, p & TM, P.ini, P.ini2);
+			prev_st = st;
+		}
+		int8_t x1, v1, x21;
+		if (st > 0 && st - 1 >= last_st && st - 1 <= last_en) {
+			const int sl = (st - 1) & TM;
+			x1 = (int8_t)uni(Xo[sl]), v1 = (int8_t)uni(Vo[sl]), x21 = DUAL ? (int8_t)uni(X2o[sl]) : 0;
+		} else row_boundary<DUAL>(P, r, st, x1, v1, x21);
+		const int ep = en0 > 0 ? en0 - 1 : en0;
+		const int32_t Hprev = uni(Ho[ep & TM]);
+		int8_t yr, ur, y2r;
+		row_yu<DUAL>(P, r, yr, ur, y2r);
+		const int rr = en >= r ? r : -1;
+		int shi = st0 + ((en0 - st0) >> 4 << 4) + 15;
+		if (shi > P.tl_hi) shi = P.tl_hi;
+		const int end = en > shi ? en : shi, cend = end + 32;
+		const int en1 = st0 + ((en0 - st0) >> 2 << 2);
+		const bool exact = r > 0;
+		uint32_t bk = 0;
+		uint8_t *prow = slab + (size_t)r * P.nb;
+		for (int base = st + 64 * wave; base <= cend; base += 64 * kTeamWaves) {
+			const int t = base + lane, sl = t & TM, sm = (t - 1) & TM;
+			int8_t ox = Xo[sl], ov = Vo[sl], ou = Uo[sl], oy = Yo[sl], zs = So[sl];
+			int8_t ox2 = DUAL ? X2o[sl] : 0, oy2 = DUAL ? Y2o[sl] : 0;
+			const int32_t oh = Ho[sl];
+			const bool first = t == st;
+			int8_t xt1 = first ? x1 : Xo[sm], vt1 = first ? v1 : Vo[sm];
+			int8_t x2t1 = DUAL ? (first ? x21 : X2o[sm]) : 0;
+			if (!DUAL) {
+				// _mm_cvtsi32_si128 of a negative int8 also sets bytes 1..3 (ksw2_extz2_sse.c:138-139)
+				const bool qk = t > st && t <= st + 3;
+				xt1 = qk && x1 < 0 ? (int8_t)0xff : xt1;
+				vt1 = qk && v1 < 0 ? (int8_t)0xff : vt1;
+			}
+			oy = t == rr ? yr : oy, ou = t == rr ? ur : ou, oy2 = t == rr ? y2r : oy2;
+			const int tc = t < tlen - 1 ? t : tlen - 1, qi = r - t;
+			const int8_t ta0 = (int8_t)Tg[tc > 0 ? tc : 0], qa0 = (int8_t)Qg[qi < 0 ? 0 : qi < qlen ? qi : qlen - 1];
+			const int8_t ta = t < tlen ? ta0 : 0, qa = qi >= 0 ? qa0 : 0;
+			const int8_t fs = ((ta == m1) | (qa == m1)) ? sc_N : ta == qa ? sc_mch : sc_mis;
+			zs = (unsigned)(t - st0) <= (unsigned)(shi - st0) ? fs : zs;
+			int8_t nU, nV, nX, nY, nX2, nY2;
+			const uint8_t d = dp_cell_cg<DUAL, RIGHT>(P, xt1, vt1, x2t1, ou, oy, oy2, zs, nU, nV, nX, nY, nX2, nY2);
+			const bool band = exact && t >= st0 && t <= en0;
+			const int32_t hc = t < en0 ? oh + h_term<DUAL>(P, nV) : Hprev + h_term<DUAL>(P, en0 > 0 ? nU : nV);
+			const int dd = t - st0;
+			const uint32_t rank = t >= en0 ? 511u : t < en1 ? (uint32_t)(508 - (((dd & 3) << 6) | (dd >> 2))) : (uint32_t)(2 + en1 - t);
+			const uint32_t key = ((uint32_t)(hc + (1 << 22)) << 9) | rank;
+			bk = band && key > bk ? key : bk;
+			// row 0 keeps H[0] = v - (q+e) (ksw2_extd2_sse.c:339-343)
+			const int32_t h0 = DUAL ? (int32_t)nV - P.qe0 : (int32_t)(uint8_t)nV - P.qe - P.qe;
+			if (t <= cend) {
+				const bool core = t <= en;
+				Sn[sl] = zs;
+				Un[sl] = core ? nU : ou, Vn[sl] = core ? nV : ov, Xn[sl] = core ? nX : ox, Yn[sl] = core ? nY : oy;
+				if (DUAL) X2n[sl] = core ? nX2 : ox2, Y2n[sl] = core ? nY2 : oy2;
+				Hn[sl] = band ? hc : r == 0 && t == 0 ? h0 : oh;
+				prow[(uint32_t)(t - st)] = d;
+			}
+		}
+		const uint32_t wk = wave_max_u32(bk);
+		if (lane == 0) keys[(r & 1) * kTeamWaves + wave] = wk;
+		__syncthreads();
+		int32_t max_H;
+		int max_t;
+		if (r > 0) {
+			const uint32_t k = wave_max_u32(lane < kTeamWaves ? keys[(r & 1) * kTeamWaves + lane] : 0u);
+			argmax_decode9(k, st0, en0, en1, max_H, max_t);
+		} else {
+			max_H = uni(Hn[0]);
+			max_t = 0;
+		}
+		const bool at_tend = en0 == tlen - 1;
+		int32_t Hen0 = max_H;
+		if (at_tend) {
+			if (r > 0) Hen0 = uni(Hn[en0 & TM]);
+			if (Hen0 > z.mte) z.mte = Hen0, z.mte_q = r - en0;
+		}
+		if (r - st0 == qlen - 1) {
+			const int32_t Hst0 = r > 0 ? uni(Hn[st0 & TM]) : max_H;
+			if (Hst0 > z.mqe) z.mqe = Hst0, z.mqe_t = st0;
+		}
+		if (apply_zdrop(z, max_H, r, max_t, P.zdrop, DUAL ? P.e2 : P.e)) break;
+		if (at_tend && r == qlen + tlen - 2) z.score = Hen0;
+		if (DUAL && (P.flag & 0x40)) { // ksw2_extd2_sse.c:361-367
+			const int32_t rH = last_max_H > max_H ? last_max_H : max_H;
+			const int32_t rq = qlen - (r - st0), rt = tlen - en0;
+			const int32_t rm = rq >= tlen - st0 ? tlen - st0 : rt >= qlen - (r - en0) ? qlen - (r - en0) : tlen + qlen - 1 - r;
+			if (rH + rm * P.max_sc + P.end_bonus < z.max) break;
+		}
+		last_max_H = max_H;
+		last_st = st, last_en = en;
+	}
+	__syncthreads();
+	const int n_cigar = backtrack<DUAL, false, kTeamThreads>(P, z, jb, a, lds, slab, 24 * TC);
+	write_result(a, jid, z, n_cigar, cells);
+	__syncthreads();
+}
+
+// the deferred jobs that team_job takes, one workgroup per job; the index comes through LDS
+template <bool DUAL>
+__global__ void __launch_bounds__(kTeamThreads) dp_team_kernel(DpArgs a)
+{
+	extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+	__shared__ int s_take;
+	const int n = __builtin_amdgcn_readfirstlane((int)*(volatile unsigned int*)(a.next + 1));
+	if (n == 0) return;
+	uint8_t *slab = a.pbuf + (size_t)blockIdx.x * a.slab;
+	auto take = [&]() {
+		if (threadIdx.x == 0) s_take = (int)atomicAdd(a.next + 4, 1u);
+		__syncthreads();
+		const int v = __builtin_amdgcn_readfirstlane(s_take);
+		__syncthreads();
+		return v;
+	};
+	for (int i = take(); i < n; i = take()) {
+		const int j = __builtin_amdgcn_readfirstlane(a.defer[i]);
+		const DpJob jb = a.jobs[j];
+		const JobP<DUAL> P = job_params<DUAL>(jb, a);
+		if (team_job<DUAL>(P, a.tcap)) {
+			if (P.right) run_job_team<DUAL, true>(a, j, jb, P, lds, slab);
+			else run_job_team<DUAL, false>(a, j, jb, P, lds, slab);
+		}
+	}
+}
+
 // the next index from a shared counter, one per wave: lane 0 takes it, every lane receives it
 __device__ __forceinline__ int wave_take(unsigned int *ctr)
 {
@@ -843,7 +1033,8 @@ __device__ __forceinline__ int wave_take(unsigned int *ctr)
 }
 
 // job counters in a.next: [0] taken by dp_fast_kernel, [1] jobs it deferred (listed in a.defer),
-// [2] and [3] deferred jobs taken by dp_kernel<DUAL, false> and dp_kernel<DUAL, true>.  The job
+// [2], [3] and [4] deferred jobs taken by dp_kernel<DUAL, false>, dp_kernel<DUAL, true> and
+// dp_team_kernel.  The job
 // loops take the next index in the loop condition and have no other exit, so that the lane-0
 // regions inside the body cannot become part of the loop's exit.
 template <bool DUAL>
@@ -887,11 +1078,19 @@ __global__ void __launch_bounds__(64) dp_kernel(DpArgs a)
 		const bool wide = P.nb + 17 > a.cap;
 		if (WIDE && wide) {
 			run_job_lds<DUAL, false, true>(a, j, jb, P, a.gstate + (size_t)blockIdx.x * 12 * a.gcap, mat, slab, a.gcap);
-		} else if (!WIDE && !wide) {
+		} else if (!WIDE && !wide && !team_job<DUAL>(P, a.tcap)) {
 			if (((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= a.qt_cap) run_job_lds<DUAL, true>(a, j, jb, P, lds, mat, slab, a.cap);
 			else run_job_lds<DUAL, false>(a, j, jb, P, lds, mat, slab, a.cap);
 		}
 	}
+}
+
+int dp_team_cap(int nb_max)
+{
+	if (nb_max <= kFastMaxNb) return 0;
+	int c = 256;
+	while (c < nb_max + 144 && c < 2048) c <<= 1;   // 24*2048 bytes of LDS: within 64 KB per workgroup
+	return c;
 }
 
 hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st)
@@ -901,6 +1100,15 @@ hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st)
 	else hipLaunchKernelGGL(dp_fast_kernel<false>, dim3(grid), dim3(64), fshm, st, a);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess) return e;
+	if (a.tcap > 0) {
+		const size_t tshm = (size_t)24 * a.tcap + 8 * kTeamWaves;
+		// one workgroup per team job (a slab each: grid <= the slabs allocated), at most 4096
+		const unsigned tgrid = grid < 4096 ? grid : 4096;
+		if (dual) hipLaunchKernelGGL(dp_team_kernel<true>, dim3(tgrid), dim3(kTeamThreads), tshm, st, a);
+		else hipLaunchKernelGGL(dp_team_kernel<false>, dim3(tgrid), dim3(kTeamThreads), tshm, st, a);
+		e = hipGetLastError();
+		if (e != hipSuccess) return e;
+	}
 	const size_t shm = (size_t)12 * a.cap + (size_t)a.qt_cap;
 	if (dual) hipLaunchKernelGGL((dp_kernel<true, false>), dim3(grid), dim3(64), shm, st, a);
 	else hipLaunchKernelGGL((dp_kernel<false, false>), dim3(grid), dim3(64), shm, st, a);

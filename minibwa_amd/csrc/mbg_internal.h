@@ -68,7 +68,8 @@ struct DpArgs {
 	int32_t qt_cap;         // LDS bytes after the 12*cap region for staging a job's query and target
 	uint8_t *gstate;        // when gcap > 0: one 12*gcap-byte row-state region per resident block
 	int32_t gcap;           // ring length (power of two) for jobs whose band needs more than cap - 17
-	unsigned int *next;     // four zeroed job counters (jobs are taken one at a time)
+	int32_t tcap;           // ring length of dp_team_kernel (0: no job needs it)
+	unsigned int *next;     // five zeroed job counters (jobs are taken one at a time)
 	int32_t *defer;         // n_jobs entries: jobs the fast kernel leaves to the general ones
 	int32_t mat_last, mat_max, mat_min;  // mat[m*m-1] and the extremes (ksw2_extd2_sse.c:77,87-90), as
 	                                     // dwords so that they load as scalars
@@ -108,6 +109,8 @@ hipError_t launch_getseq(const L2b &lb, int64_t n, const int64_t *tid, const int
 hipError_t launch_pack_reads(const uint8_t *seq, const int64_t *off, const int32_t *len, int32_t n_reads, uint32_t *packed,
                              uint32_t *qflag, hipStream_t st);
 hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st);
+// ring length of dp_team_kernel for a launch whose widest band is nb_max cells (0: not needed)
+int dp_team_cap(int nb_max);
 hipError_t launch_gather_chain(const Idx &ix, int64_t n_chains, int32_t steps, int32_t bs, unsigned long long *sum, hipStream_t st);
 hipError_t launch_stream_copy(void *dst, const void *src, int64_t n16, unsigned grid, hipStream_t st);
 hipError_t launch_row_counts(const SeedRec *recs, const int64_t *n_rec, int64_t rec_cap, int32_t max_occ, uint32_t *cnt, hipStream_t st);

@@ -186,3 +186,33 @@ def test_dp_many_jobs_per_wave(gpu_ctx, dual):
     mat = O.gen_nt4_mat(2, 8, 0, 1)
     out = (gpu_ctx.ksw_extd2 if dual else gpu_ctx.ksw_extz2)(Q, T, _params(P), mat)
     _check(out, Q, T, P, "many", dual)
+
+
+@pytest.mark.parametrize("dual", [True, False])
+def test_dp_team_bands(gpu_ctx, dual):
+    """Bands of 240-1900 cells (long-read gap fills, w in the hundreds) run on a team of waves per job
+    with a double-buffered row state; mixed with narrow and deferred jobs in one launch, every field
+    and CIGAR equals the reference."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(51 if dual else 52)
+    Q, T, P = [], [], []
+    flags = [0x00, 0x02, 0x40, 0x42, 0x80, 0xc2, 0x40, 0x00]
+    for k in range(16):
+        ql = int(rng.integers(600, 4000))
+        q = rng.integers(0, 4, ql).astype(np.uint8)
+        if k % 5 == 0:
+            q[rng.integers(0, ql, 3)] = 4
+        t = _mutate(rng, q, 0.02 if k % 3 else 0.15, 0.01)
+        t = np.concatenate([rng.integers(0, 4, int(rng.integers(0, 300))).astype(np.uint8), t,
+                            rng.integers(0, 4, int(rng.integers(0, 500))).astype(np.uint8)])
+        qq, ee, q2, e2 = GAPS[k % len(GAPS)]
+        w = int(rng.choice([230, 301, 501, 751, 900]))
+        Q.append(q), T.append(t)
+        P.append(dict(flag=flags[k % len(flags)], w=w, zdrop=int(rng.choice([-1, 400, 200])), end_bonus=int(rng.choice([-1, 0, 10])),
+                      q=qq, e=ee, q2=q2, e2=e2))
+    nq, nt, nP = fuzz_jobs(60, 53)              # narrow and deferred jobs in the same launch
+    for i in range(len(nq)):
+        Q.insert(2 * i % len(Q), nq[i]), T.insert(2 * i % len(T), nt[i]), P.insert(2 * i % len(P), nP[i])
+    mat = O.gen_nt4_mat(2, 8, 0, 1)
+    out = (gpu_ctx.ksw_extd2 if dual else gpu_ctx.ksw_extz2)(Q, T, _params(P), mat)
+    _check(out, Q, T, P, "team", dual)
