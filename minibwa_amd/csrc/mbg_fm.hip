@@ -304,29 +304,48 @@ __device__ __forceinline__ SeedTask make_subtask(const SeedRec &r, int32_t min_l
 	return t;
 }
 
-// one thread per read over its pass-1 slots; one atomic per wave for the task slots
+// one wave per 64 reads: over the kslot steps, lane l takes slot (step*64 + l) % kslot of read
+// (step*64 + l) / kslot, so the wave reads its reads' slot arrays contiguously.  A counting pass and
+// one atomic give the wave a contiguous range of task positions in (read, slot) order: the tasks of
+// neighbouring reads stay together, which pass 2's FM-index accesses profit from.
+__device__ __forceinline__ bool subtask_at(const SeedRec *slots, const uint32_t *slot_cnt, int32_t kslot, int32_t n_reads, int64_t r0,
+                                           int64_t g, int32_t min_len, int32_t max_sub_occ, SeedTask &t)
+{
+	const int64_t rd = r0 + g / kslot;
+	const int32_t sj = (int32_t)(g % kslot);
+	if (rd >= n_reads || sj >= (int32_t)slot_cnt[rd]) return false;
+	const SeedRec r = slots[rd * kslot + sj];
+	if (!subtask_ok(r, min_len, max_sub_occ)) return false;
+	t = make_subtask(r, min_len);
+	return true;
+}
+
 __global__ void __launch_bounds__(256) subtask_kernel(const SeedRec *slots, const uint32_t *slot_cnt, int32_t kslot, int32_t n_reads,
                                                      const SeedRec *spill, const unsigned long long *n_spill, int64_t spill_cap,
                                                      int32_t min_len, int32_t max_sub_occ, SeedTask *tasks, uint32_t *n_tasks, int64_t task_cap)
 {
-	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	const uint32_t c = i < n_reads ? slot_cnt[i] : 0;
-	uint32_t nq = 0;
-	for (uint32_t j = 0; j < c; ++j) nq += subtask_ok(slots[(uint64_t)i * kslot + j], min_len, max_sub_occ);
-	uint32_t wtot;
-	const uint32_t ex = wave_excl_scan(nq, &wtot);
+	const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int lane = threadIdx.x & 63;
+	const int64_t r0 = (tid >> 6) << 6;                   // first read of this wave
+	uint32_t tot = 0;
+	for (int64_t g = lane; g < (int64_t)64 * kslot; g += 64) {
+		SeedTask t;
+		tot += (uint32_t)__popcll(__ballot(subtask_at(slots, slot_cnt, kslot, n_reads, r0, g, min_len, max_sub_occ, t)));
+	}
 	uint32_t base = 0;
-	if (__lane_id() == 0 && wtot) base = atomicAdd(n_tasks, wtot);
-	base = __shfl(base, 0) + ex;
-	for (uint32_t j = 0, k = 0; j < c; ++j) {
-		const SeedRec r = slots[(uint64_t)i * kslot + j];
-		if (!subtask_ok(r, min_len, max_sub_occ)) continue;
-		if ((int64_t)(base + k) < task_cap) tasks[base + k] = make_subtask(r, min_len);
-		++k;
+	if (lane == 0 && tot) base = atomicAdd(n_tasks, tot);
+	base = (uint32_t)__shfl((int)base, 0);
+	for (int64_t g = lane; tot && g < (int64_t)64 * kslot; g += 64) {
+		SeedTask t;
+		const bool q = subtask_at(slots, slot_cnt, kslot, n_reads, r0, g, min_len, max_sub_occ, t);
+		const uint64_t m = __ballot(q);
+		const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+		if (q && (int64_t)pos < task_cap) tasks[pos] = t;
+		base += (uint32_t)__popcll(m);
 	}
 	int64_t ns = (int64_t)*n_spill;
 	if (ns > spill_cap) ns = spill_cap;
-	for (int64_t k = i; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
+	for (int64_t k = tid; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
 		const SeedRec r = spill[k];
 		if (r.pass != 1 || !subtask_ok(r, min_len, max_sub_occ)) continue;
 		const uint32_t t = atomicAdd(n_tasks, 1u);
