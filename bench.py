@@ -21,6 +21,7 @@ Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2]
 import argparse
 import json
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -72,6 +73,9 @@ def main():
     ap.add_argument("--dp-batch", type=int, default=0, metavar="N",
                     help="only run the extension-kernel batch: N independent (query, target) jobs, query 100-300 bp, "
                          "2%% substitutions, 0.5%% indels, 50 bp flanks, band 101, gaps 12/2 and 24/1; reports GCUPS")
+    ap.add_argument("--ll-batch", type=int, default=0, metavar="N",
+                    help="only run the mate-rescue local SW batch (ksw_ll, u8 core): N jobs of a 150 bp read against "
+                         "a 550 bp window holding a 1%%-diverged copy of it in 80%% of the jobs; reports GCUPS")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -92,6 +96,9 @@ def main():
 
     if args.dp_batch:
         dp_batch(args)
+        return
+    if args.ll_batch:
+        ll_batch(args)
         return
 
     t0 = time.time()
@@ -380,6 +387,73 @@ def dp_batch(args):
         out["modes"].append(m)
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def ll_batch(args):
+    """Mate-rescue local SW batch: one mbg_ll_batch_dev call over N resident jobs, timed by the host
+    around the call (which waits for the kernel); cells = qlen x target rows processed."""
+    from minibwa_amd import mbg, synth
+    ctx = mbg.Context(0)
+    mat = np.asarray(mbg.nt4_mat(1, 4, 0, 1), np.int8)    # pe.c:258-262 with -A2 -B8: match 1, mismatch -4
+    gapo, gape, xtra = 6, 1, mbg.KSW_LL_SUBO | 19          # ceil(12/2), ceil(2/2), SUBO | min_len
+    t0 = time.time()
+    qb, qoff, ql, tb, toff, tl = synth.ll_jobs(args.ll_batch, seed=11)
+    n = len(ql)
+    jobs = np.zeros(n, mbg.LL_JOB_DTYPE)
+    jobs["qoff"], jobs["toff"], jobs["qlen"], jobs["tlen"], jobs["size"] = qoff[:-1], toff[:-1], ql, tl, 1
+    log(f"{n} ll jobs generated in {time.time() - t0:.1f}s")
+    d_jobs, d_q, d_t = ctx.to_device(jobs), ctx.to_device(qb), ctx.to_device(tb)
+    d_res = ctx.to_device(np.zeros(n, mbg.LL_RES_DTYPE))
+    rows = int(tl.astype(np.int64).sum())
+    times = []
+    for it in range(3):
+        t0 = time.perf_counter()
+        mbg.check(ctx.L.mbg_ll_batch_dev(ctx.h, None, n, mbg.C.c_void_p(d_jobs.ptr), mbg.C.c_void_p(d_q.ptr),
+                                         mbg.C.c_void_p(d_t.ptr), 5, mbg._ptr(mat), gapo, gape, xtra,
+                                         mbg.C.c_void_p(d_res.ptr), int(ql.max()), rows), "mbg_ll_batch_dev")
+        times.append(time.perf_counter() - t0)
+    res = ctx.to_host(d_res.ptr, mbg.LL_RES_DTYPE, n)
+    assert (res["score"] >= 0).all()
+    cells = int((ql.astype(np.int64) * tl).sum())   # scores stay below 251: every row is processed
+    best = min(times)
+    out = {"metric": "mate-rescue local SW GCUPS (ksw_ll u8 core, 150 bp reads x 550 bp windows)",
+           "value": round(cells / best / 1e9, 2), "unit": "GCUPS", "n_jobs": n, "cells": cells, "s": round(best, 4),
+           "jobs_per_s": round(n / best), "rescued_frac": round(float((res["score"] >= 19).mean()), 4),
+           "with_second_best": int((res["score2"] >= 0).sum())}
+    cpu = ll_cpu_sample(jobs, qb, tb, mat, gapo, gape, xtra, min(n, args.cpu_jobs))
+    if cpu:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def ll_cpu_sample(jobs, qb, tb, mat, gapo, gape, xtra, n_s):
+    """ksw_ll_qinit + ksw_ll_u8_core of the compiled reference on the first n_s jobs (oracle/_ref/cpu_hotpath ll)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "cpu_hotpath")
+    if not os.path.exists(exe) or n_s <= 0:
+        return None
+    work = tempfile.mkdtemp(prefix="mbg_ll_")
+    jb = os.path.join(work, "ll_jobs.bin")
+    j = jobs[:n_s]
+    qend = int(j["qoff"][-1] + j["qlen"][-1])
+    tend = int(j["toff"][-1] + j["tlen"][-1])
+    with open(jb, "wb") as f:
+        f.write(np.int64(n_s).tobytes())
+        f.write(np.asarray(mat, np.int8).tobytes())
+        f.write(np.array([gapo, gape, xtra], np.int32).tobytes())
+        f.write(j.tobytes())
+        f.write(np.int64(qend).tobytes()); f.write(qb[:qend].tobytes())
+        f.write(np.int64(tend).tobytes()); f.write(tb[:tend].tobytes())
+    cores = host_cores()
+    r = subprocess.run([exe, "ll", jb, str(cores)], capture_output=True, text=True, timeout=600)
+    shutil.rmtree(work, ignore_errors=True)
+    if r.returncode != 0:
+        log("ll cpu sample failed:", r.stderr[-300:])
+        return None
+    d = json.loads(r.stdout.strip().split("\n")[-1])
+    cells = int((j["qlen"].astype(np.int64) * j["tlen"]).sum())
+    return {"value": round(cells / d["ll_s"] / 1e9, 3), "unit": "GCUPS", "cores": cores, "kind": "reference",
+            "sample": f"ksw_ll_qinit + ksw_ll_u8_core on the first {n_s} jobs, {cores} threads, {d['ll_s']:.2f}s"}
 
 
 def dp_cpu_sample(jobs, qb, tb, res, mat, n_s):

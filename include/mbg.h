@@ -135,6 +135,29 @@ int mbg_extz2_batch(mbg_ctx_t *ctx, int32_t n_jobs, const mbg_dp_job_t *jobs, co
                     const uint8_t *tseq, int64_t tbytes, int8_t m, const int8_t *mat,
                     mbg_dp_result_t *res, uint32_t *cigar, int64_t cigar_cap);
 
+/* One ksw_ll job: query qseq[qoff, qoff+qlen), target tseq[toff, toff+tlen), bytes 0..m-1.  size
+ * selects the core as ksw_ll_qinit's size does: 1 = ksw_ll_u8_core, 2 = ksw_ll_i16_core. */
+typedef struct {
+	int64_t qoff, toff;
+	int32_t qlen, tlen;
+	int32_t size, pad;
+} mbg_ll_job_t;
+
+/* ksw_llrst_t (ksw2.h:47-51): best score, its target and query end, second best and its end. */
+typedef struct { int32_t score, te, qe, score2, te2; } mbg_ll_result_t;
+
+/* Batched striped local Smith-Waterman as mate rescue runs it (pe.c:260-272): per job,
+ * ksw_ll_qinit (ksw2_ll_sse.c:49-96) then ksw_ll_u8_core or ksw_ll_i16_core (ksw2_ll_sse.c:121-333;
+ * declared ksw2.h:101-103), results bit-identical to them.  m, mat (m*m), gapo, gape and xtra
+ * (KSW_LL_SUBO / KSW_LL_STOP | score) are shared by the batch.  Supported range: gapo >= 1 (with
+ * gapo == 0 the reference's lazy-F exit depends on its SSE lane order), gapo + gape <= 255,
+ * qlen <= 1024, a matrix with a positive entry; u8 jobs need a negative entry and
+ * max - min <= 255, i16 jobs max(mat) * min(qlen, tlen) <= 32767 (no saturation).  Outside it:
+ * MBG_EUNSUPPORTED, and no job runs. */
+int mbg_ll_batch(mbg_ctx_t *ctx, int32_t n_jobs, const mbg_ll_job_t *jobs, const uint8_t *qseq, int64_t qbytes,
+                 const uint8_t *tseq, int64_t tbytes, int8_t m, const int8_t *mat, int32_t gapo, int32_t gape, int32_t xtra,
+                 mbg_ll_result_t *res);
+
 /* Target extraction l2b_getseq (l2bit.c:52-77) for n windows [st[i],en[i]) of contig tid[i]
  * (clipped to the contig, N runs restored as 4), written at out + out_off[i].  len[i] receives
  * the clipped length (or -1 for a bad tid). */
@@ -155,6 +178,14 @@ int mbg_seed_batch_dev_async(mbg_ctx_t *ctx, void *stream, int32_t n_seq, const 
                              const uint8_t *seq, int32_t min_len, int32_t max_sub_occ,
                              mbg_seed_rec_t *out, int64_t cap, int64_t *n_out_dev);
 int mbg_sa_batch_dev(mbg_ctx_t *ctx, void *stream, int64_t n, uint64_t *x);
+/* mbg_ll_batch over jobs, sequences and results in HBM.  Every job is run with the registers of
+ * max_qlen (<= 1024); rows_total bounds the sum of the jobs' tlen (the per-row maxima lists).  A job
+ * outside the supported range (qlen > max_qlen, an i16 job that could saturate) gets score -1 and
+ * its other fields -1; the matrix must suit the u8 core (a negative entry, max - min <= 255).
+ * Synchronises the stream. */
+int mbg_ll_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_jobs, const mbg_ll_job_t *jobs, const uint8_t *qseq,
+                     const uint8_t *tseq, int8_t m, const int8_t *mat_host, int32_t gapo, int32_t gape, int32_t xtra,
+                     mbg_ll_result_t *res, int32_t max_qlen, int64_t rows_total);
 int mbg_extd2_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_jobs, const mbg_dp_job_t *jobs,
                         const uint8_t *qseq, const uint8_t *tseq, int8_t m, const int8_t *mat_host,
                         mbg_dp_result_t *res, uint32_t *cigar, int32_t max_qlen, int32_t max_tlen, int32_t max_w);

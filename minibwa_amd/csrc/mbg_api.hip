@@ -715,6 +715,122 @@ int mbg_extz2_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_dp_job_t *jobs, cons
 	return dp_batch(c, false, n_jobs, jobs, qseq, qbytes, tseq, tbytes, m, mat, res, cigar, cigar_cap);
 }
 
+// batch-wide ksw_ll parameters: the matrix's u8 bias and largest entry as ksw_ll_qinit computes them
+// (ksw2_ll_sse.c:67-74, uint8 arithmetic), the score table, the SUBO/STOP thresholds
+static int ll_params(int8_t m, const int8_t *mat, int32_t gapo, int32_t gape, int32_t xtra, LlArgs &a, int8_t sc[40], int *mat_min)
+{
+	if (m < 1 || m > 5 || mat == nullptr) return MBG_EARG;
+	if (gapo < 1 || gape < 0 || gapo + gape > 255) return MBG_EUNSUPPORTED;
+	uint8_t sh = 127, md = 0;
+	for (int k = 0; k < m * m; ++k) {
+		if (mat[k] < (int8_t)sh) sh = (uint8_t)mat[k];
+		if (mat[k] > (int8_t)md) md = (uint8_t)mat[k];
+	}
+	a.shift = (uint8_t)(256 - sh), a.qmax = md, *mat_min = (int8_t)sh;
+	if (a.qmax <= 0) return MBG_EUNSUPPORTED;      // the second-best window divides by it
+	for (int t = 0; t < 5; ++t)                    // sc[t*8 + c]; c = 7 scores padding
+		for (int q = 0; q < 8; ++q) sc[t * 8 + q] = t < m && q < m ? mat[t * m + q] : -1;
+	a.gapo = gapo, a.gape = gape, a.m = m;
+	a.minsc = (xtra & 0x40000) ? xtra & 0xffff : 0x10000;    // KSW_LL_SUBO, KSW_LL_STOP (ksw2.h:21-22)
+	a.endsc = (xtra & 0x20000) ? xtra & 0xffff : 0x10000;
+	return MBG_OK;
+}
+
+int mbg_ll_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_ll_job_t *jobs, const uint8_t *qseq, int64_t qbytes, const uint8_t *tseq,
+                 int64_t tbytes, int8_t m, const int8_t *mat, int32_t gapo, int32_t gape, int32_t xtra, mbg_ll_result_t *res)
+{
+	if (c == nullptr || n_jobs < 0 || (n_jobs > 0 && (!jobs || !res))) return MBG_EARG;
+	LlArgs a{};
+	int8_t sc[40];
+	int mat_min;
+	TRY(ll_params(m, mat, gapo, gape, xtra, a, sc, &mat_min));
+	const int shift = a.shift, qmax = a.qmax;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (n_jobs == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	// jobs grouped by registers per lane; each gets its own span of the per-row maxima list
+	std::vector<int32_t> order(n_jobs);
+	std::vector<int64_t> boff(n_jobs);
+	int64_t cnt[17] = {0}, nb = 0;
+	std::vector<int8_t> slot(n_jobs);
+	for (int32_t i = 0; i < n_jobs; ++i) {
+		const mbg_ll_job_t &j = jobs[i];
+		if (j.qlen < 0 || j.tlen < 0 || j.qoff < 0 || j.toff < 0 || j.qoff + j.qlen > qbytes || j.toff + j.tlen > tbytes) return MBG_EARG;
+		const int k = ll_slots(j.qlen, j.size);
+		if (k < 0) return MBG_EUNSUPPORTED;
+		if (j.size <= 1) {
+			// u8 scores must fit the biased byte (padding scores -1 + shift)
+			if (mat_min >= 0 || qmax + shift > 255) return MBG_EUNSUPPORTED;
+		} else if ((int64_t)qmax * (j.qlen < j.tlen ? j.qlen : j.tlen) > 32767) {
+			return MBG_EUNSUPPORTED;               // the i16 core would saturate
+		}
+		slot[i] = (int8_t)k, ++cnt[k];
+		boff[i] = nb, nb += j.tlen;
+	}
+	int64_t first[17], at[17];
+	for (int k = 0, s = 0; k <= 16; ++k) first[k] = at[k] = s, s += (int)cnt[k];
+	for (int32_t i = 0; i < n_jobs; ++i) order[at[slot[i]]++] = i;
+	TRY(dgrow(c->b_in[0], qbytes + 16)); TRY(dgrow(c->b_in[1], tbytes + 16)); TRY(dgrow(c->b_in[4], (size_t)n_jobs * sizeof(LlJob)));
+	TRY(dgrow(c->b_in[2], (size_t)n_jobs * 4)); TRY(dgrow(c->b_in[3], (size_t)n_jobs * 8 + 64));
+	TRY(dgrow(c->b_out[0], (size_t)n_jobs * sizeof(LlRes))); TRY(dgrow(c->b_out[1], (size_t)(nb + 1) * 8));
+	HIPCHK(hipMemcpyAsync(c->b_in[0].p, qseq, qbytes, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[1].p, tseq, tbytes, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[4].p, jobs, (size_t)n_jobs * sizeof(LlJob), hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[2].p, order.data(), (size_t)n_jobs * 4, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[3].p, boff.data(), (size_t)n_jobs * 8, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync((char*)c->b_in[3].p + (size_t)n_jobs * 8, sc, sizeof(sc), hipMemcpyHostToDevice, c->stream));
+	a.jobs = (const LlJob*)c->b_in[4].p;
+	a.qseq = (const uint8_t*)c->b_in[0].p, a.tseq = (const uint8_t*)c->b_in[1].p;
+	a.res = (LlRes*)c->b_out[0].p;
+	a.bbuf = (uint64_t*)c->b_out[1].p, a.boff = (const int64_t*)c->b_in[3].p;
+	a.sc = (const int8_t*)c->b_in[3].p + (size_t)n_jobs * 8;
+	a.rows_cap = nb, a.check = false;
+	for (int k = 1; k <= 16; ++k) {
+		if (cnt[k] == 0) continue;
+		a.order = (const int32_t*)c->b_in[2].p + first[k];
+		a.n = (int32_t)cnt[k];
+		HIPCHK(launch_ll(a, k, c->stream));
+	}
+	HIPCHK(hipMemcpyAsync(res, c->b_out[0].p, (size_t)n_jobs * sizeof(LlRes), hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	return MBG_OK;
+}
+
+int mbg_ll_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_jobs, const mbg_ll_job_t *jobs, const uint8_t *qseq, const uint8_t *tseq,
+                     int8_t m, const int8_t *mat_host, int32_t gapo, int32_t gape, int32_t xtra, mbg_ll_result_t *res,
+                     int32_t max_qlen, int64_t rows_total)
+{
+	if (c == nullptr || n_jobs < 0 || max_qlen < 0 || rows_total < 0 || (n_jobs > 0 && (!jobs || !qseq || !tseq || !res)))
+		return MBG_EARG;
+	LlArgs a{};
+	int8_t sc[40];
+	int mat_min;
+	TRY(ll_params(m, mat_host, gapo, gape, xtra, a, sc, &mat_min));
+	// the widest padding (16 positions a vector) sets the registers per lane for every job
+	const int k = ll_slots(max_qlen, 1);
+	if (k < 0) return MBG_EUNSUPPORTED;
+	if (mat_min >= 0 || a.qmax + a.shift > 255) return MBG_EUNSUPPORTED;   // the u8 core's biased bytes
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (n_jobs == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	size_t tmp_bytes = 0;
+	HIPCHK(scan_ll_rows(nullptr, &tmp_bytes, nullptr, nullptr, n_jobs, st));
+	TRY(dgrow(c->s_tmp, tmp_bytes + 16));
+	TRY(dgrow(c->b_in[3], (size_t)n_jobs * 8 + 64));
+	TRY(dgrow(c->b_out[1], (size_t)(rows_total + 1) * 8));
+	HIPCHK(hipMemcpyAsync((char*)c->b_in[3].p + (size_t)n_jobs * 8, sc, sizeof(sc), hipMemcpyHostToDevice, st));
+	HIPCHK(scan_ll_rows(c->s_tmp.p, &tmp_bytes, (const LlJob*)jobs, (int64_t*)c->b_in[3].p, n_jobs, st));
+	a.jobs = (const LlJob*)jobs, a.qseq = qseq, a.tseq = tseq, a.res = (LlRes*)res;
+	a.order = nullptr, a.n = n_jobs;
+	a.bbuf = (uint64_t*)c->b_out[1].p, a.boff = (const int64_t*)c->b_in[3].p;
+	a.sc = (const int8_t*)c->b_in[3].p + (size_t)n_jobs * 8;
+	a.rows_cap = rows_total, a.check = true;
+	HIPCHK(launch_ll(a, k, st));
+	HIPCHK(hipStreamSynchronize(st));
+	return MBG_OK;
+}
+
 int mbg_extd2_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_jobs, const mbg_dp_job_t *jobs, const uint8_t *qseq, const uint8_t *tseq,
                         int8_t m, const int8_t *mat_host, mbg_dp_result_t *res, uint32_t *cigar, int32_t max_qlen, int32_t max_tlen, int32_t max_w)
 {

@@ -90,6 +90,33 @@ inline void dp_set_matrix(DpArgs &a, int8_t m, const int8_t *mat)
 	}
 }
 
+// ksw_ll batch (mbg_ll.hip); layouts of mbg_ll_job_t / mbg_ll_result_t
+struct LlJob { int64_t qoff, toff; int32_t qlen, tlen, size, pad; };
+struct LlRes { int32_t score, te, qe, score2, te2; };
+static_assert(sizeof(LlJob) == 32, "LlJob must match mbg_ll_job_t");
+static_assert(sizeof(LlRes) == 20, "LlRes must match mbg_ll_result_t");
+
+struct LlArgs {
+	const LlJob *jobs;
+	const int32_t *order;   // the launch's job indices (null: jobs 0..n-1)
+	int32_t n;              // entries in order
+	const uint8_t *qseq, *tseq;
+	LlRes *res;
+	uint64_t *bbuf;         // per-row maxima list of job j at bbuf + boff[j] (tlen entries)
+	const int64_t *boff;
+	int32_t gapo, gape, minsc, endsc;
+	int32_t shift, qmax;    // u8 bias of the matrix (ksw2_ll_sse.c:67-74) and its largest entry
+	int32_t m;
+	const int8_t *sc;       // scores sc[t*8 + c] for target base t, query base c; c = 7: padding (-1)
+	int64_t rows_cap;       // entries in bbuf; a job whose span would pass it is refused
+	bool check;             // refuse jobs outside the launch's range on the device (score -1)
+};
+constexpr int kLlMaxQlen = 1024;        // query positions a wave holds in registers (16 a lane)
+// slots per lane (1, 2, 3, 4, 6, 8, 12 or 16) for a job of qlen bases and `size` (1: u8, 2: i16)
+int ll_slots(int qlen, int size);
+hipError_t launch_ll(const LlArgs &a, int slots, hipStream_t st);
+hipError_t scan_ll_rows(void *temp, size_t *temp_bytes, const LlJob *jobs, int64_t *out, int64_t n, hipStream_t st);
+
 hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st);
 hipError_t launch_sa_fine(const Idx &ix, uint64_t *fine, uint64_t n_fine, hipStream_t st);
 hipError_t launch_rank2a(const Idx &ix, int64_t n, const uint64_t *k, const uint64_t *l, uint64_t *ck, uint64_t *cl, hipStream_t st);

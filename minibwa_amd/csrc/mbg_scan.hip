@@ -11,4 +11,17 @@ hipError_t scan_u32(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_t
 	return hipcub::DeviceScan::ExclusiveSum(temp, *temp_bytes, in, out, (int)n, st);
 }
 
+namespace {
+struct LlRows {
+	__host__ __device__ int64_t operator()(const LlJob &j) const { return j.tlen > 0 ? j.tlen : 0; }
+};
+} // namespace
+
+// exclusive prefix sum of the jobs' target lengths: each job's span of the per-row maxima list
+hipError_t scan_ll_rows(void *temp, size_t *temp_bytes, const LlJob *jobs, int64_t *out, int64_t n, hipStream_t st)
+{
+	hipcub::TransformInputIterator<int64_t, LlRows, const LlJob*> in(jobs, LlRows());
+	return hipcub::DeviceScan::ExclusiveSum(temp, *temp_bytes, in, out, (int)n, st);
+}
+
 } // namespace mbg

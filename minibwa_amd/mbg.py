@@ -170,6 +170,10 @@ def lib():
                                        C.POINTER(MapStats), C.POINTER(MapOut)]
         L.mbg_extd2_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, _P, _P, C.c_int32, C.c_int32,
                                           C.c_int32]
+        L.mbg_ll_batch.argtypes = [_P, C.c_int32, _P, _P, C.c_int64, _P, C.c_int64, C.c_int8, _P, C.c_int32, C.c_int32,
+                                   C.c_int32, _P]
+        L.mbg_ll_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, C.c_int32, C.c_int32, C.c_int32, _P,
+                                       C.c_int32, C.c_int64]
         _LIB = L
     return _LIB
 
@@ -178,7 +182,14 @@ EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mb
             "mbg_l2b_upload", "mbg_idx_info", "mbg_pre_copy", "mbg_rank2a", "mbg_extend", "mbg_seed_batch",
             "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
             "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
-            "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench", "mbg_idx_attach")
+            "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench", "mbg_idx_attach", "mbg_ll_batch",
+            "mbg_ll_batch_dev")
+LL_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"), ("size", "<i4"),
+                         ("pad", "<i4")], align=True)
+LL_RES_DTYPE = np.dtype([(f, "<i4") for f in ("score", "te", "qe", "score2", "te2")])
+assert LL_JOB_DTYPE.itemsize == 32 and LL_RES_DTYPE.itemsize == 20
+KSW_LL_STOP = 0x20000
+KSW_LL_SUBO = 0x40000
 JOB_META_DTYPE = np.dtype([("rid", "<u4"), ("side", "i1"), ("rev", "i1"), ("pad", "<i2"), ("tid", "<i4"), ("tpos", "<i8")],
                           align=True)
 assert JOB_META_DTYPE.itemsize == 24
@@ -433,6 +444,26 @@ class Context:
     def ksw_extd2(self, queries, targets, params, mat, m=5) -> DpBatchResult:
         """params: dict of per-job arrays/scalars q, e, q2, e2, w, zdrop, end_bonus, flag."""
         return self._dp(self.L.mbg_extd2_batch, "mbg_extd2_batch", queries, targets, params, mat, m)
+
+    def ksw_ll(self, queries, targets, sizes, mat, gapo, gape, xtra, m=5) -> np.ndarray:
+        """ksw_ll_qinit + ksw_ll_{u8,i16}_core over a batch (ksw2_ll_sse.c:49-333); sizes is per job
+        (1 = u8 core, 2 = i16 core) or one value.  Returns an LL_RES_DTYPE array."""
+        n = len(queries)
+        qbuf, qoff, qlen = pack_seqs(queries)
+        tbuf, toff, tlen = pack_seqs(targets)
+        jobs = np.zeros(n, LL_JOB_DTYPE)
+        jobs["qoff"] = qoff[:-1]
+        jobs["toff"] = toff[:-1]
+        jobs["qlen"] = qlen
+        jobs["tlen"] = tlen
+        jobs["size"] = sizes
+        res = np.zeros(n, LL_RES_DTYPE)
+        mat = np.ascontiguousarray(mat, np.int8)
+        qb = qbuf if len(qbuf) else np.zeros(1, np.uint8)
+        tb = tbuf if len(tbuf) else np.zeros(1, np.uint8)
+        check(self.L.mbg_ll_batch(self.h, n, _ptr(jobs), _ptr(qb), len(qbuf), _ptr(tb), len(tbuf), m, _ptr(mat), gapo,
+                                  gape, xtra, _ptr(res)), "mbg_ll_batch")
+        return res
 
     def ksw_extz2(self, queries, targets, params, mat, m=5) -> DpBatchResult:
         p = dict(params)

@@ -130,3 +130,21 @@ def ref_to_l2b_arrays(ref: np.ndarray, n_ctg=1):
     sh = (np.arange(32, dtype=np.uint64) * 2)
     pac = (padded.reshape(n_pac, 32) << sh).sum(axis=1, dtype=np.uint64)
     return ctg_len, pac
+
+
+def ll_jobs(n, seed=11, read_len=150, window=550, sub=0.01, frac_in=0.8):
+    """Mate-rescue shaped ksw_ll jobs: a read of read_len bases against a window of `window` bases
+    (insert-size range + read length), holding a copy of the read with `sub` substitutions in a
+    fraction frac_in of the jobs -> (qbuf, qoff, qlen, tbuf, toff, tlen)."""
+    rng = np.random.default_rng(seed)
+    q = rng.integers(0, 4, (n, read_len), dtype=np.uint8)
+    t = rng.integers(0, 4, (n, window), dtype=np.uint8)
+    has = np.nonzero(rng.random(n) < frac_in)[0]
+    st = rng.integers(0, window - read_len + 1, len(has))
+    cp = q[has].copy()
+    mut = rng.random(cp.shape) < sub
+    cp[mut] = (cp[mut] + rng.integers(1, 4, int(mut.sum()), dtype=np.uint8)) % 4
+    t[has[:, None], st[:, None] + np.arange(read_len)] = cp
+    qoff = np.arange(n + 1, dtype=np.int64) * read_len
+    toff = np.arange(n + 1, dtype=np.int64) * window
+    return (q.reshape(-1), qoff, np.full(n, read_len, np.int32), t.reshape(-1), toff, np.full(n, window, np.int32))

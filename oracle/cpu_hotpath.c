@@ -15,6 +15,12 @@
  *
  * reads.bin: int64 n, int32 len[n], uint8 nt4 bases (concatenated)
  * jobs.bin:  int64 n, int8 mat[25], n x 56-byte mbg_dp_job_t, int64 nq, nq query bytes, int64 nt, nt target bytes
+ *
+ *   cpu_hotpath ll <jobs.bin> <n_threads>
+ *
+ * Mate-rescue local SW: ksw_ll_qinit + ksw_ll_u8_core / ksw_ll_i16_core (ksw2_ll_sse.c:49-333) per
+ * job, as pe.c:268-272 calls them.  jobs.bin: int64 n, int8 mat[25], int32 gapo, gape, xtra,
+ * n x 32-byte mbg_ll_job_t, int64 nq, nq query bytes, int64 nt, nt target bytes.
  */
 #include <pthread.h>
 #include <stdint.h>
@@ -99,6 +105,67 @@ static void *dp_worker(void *a_)
 	return 0;
 }
 
+typedef struct { int64_t qoff, toff; int32_t qlen, tlen, size, pad; } ll_job_t;
+typedef struct {
+	int64_t n_jobs;
+	ll_job_t *jobs;
+	uint8_t *qb, *tb;
+	int8_t mat[25];
+	int32_t gapo, gape, xtra;
+	volatile int64_t next;
+	int64_t sum[256];
+} ll_shared_t;
+typedef struct { ll_shared_t *s; int tid; } ll_targ_t;
+
+static void *ll_worker(void *a_)
+{
+	ll_targ_t *a = (ll_targ_t*)a_;
+	ll_shared_t *s = a->s;
+	for (;;) {
+		int64_t b = __sync_fetch_and_add(&s->next, 64), e, i;
+		if (b >= s->n_jobs) break;
+		e = b + 64 < s->n_jobs ? b + 64 : s->n_jobs;
+		for (i = b; i < e; ++i) {
+			const ll_job_t *j = &s->jobs[i];
+			void *qp = ksw_ll_qinit(0, j->size, j->qlen, s->qb + j->qoff, 5, s->mat);
+			ksw_llrst_t r = j->size <= 1 ? ksw_ll_u8_core(qp, j->tlen, s->tb + j->toff, s->gapo, s->gape, s->xtra)
+			                             : ksw_ll_i16_core(qp, j->tlen, s->tb + j->toff, s->gapo, s->gape, s->xtra);
+			free(qp);
+			s->sum[a->tid] += r.score + r.te + r.qe + r.score2; /* consume the result */
+		}
+	}
+	return 0;
+}
+
+static void *slurp(const char *fn, size_t *len);
+
+static int ll_main(const char *fn, int nt)
+{
+	ll_shared_t s;
+	pthread_t th[256];
+	ll_targ_t ta[256];
+	size_t jl;
+	uint8_t *p = (uint8_t*)slurp(fn, &jl);
+	int64_t n, sum = 0;
+	double t0, t;
+	int i;
+	if (!p) { fprintf(stderr, "cannot read %s\n", fn); return 1; }
+	memset(&s, 0, sizeof(s));
+	memcpy(&s.n_jobs, p, 8); p += 8;
+	memcpy(s.mat, p, 25); p += 25;
+	memcpy(&s.gapo, p, 4); memcpy(&s.gape, p + 4, 4); memcpy(&s.xtra, p + 8, 4); p += 12;
+	s.jobs = (ll_job_t*)malloc(s.n_jobs * sizeof(ll_job_t) + 1);
+	memcpy(s.jobs, p, s.n_jobs * sizeof(ll_job_t)); p += s.n_jobs * sizeof(ll_job_t);
+	memcpy(&n, p, 8); p += 8; s.qb = p; p += n; memcpy(&n, p, 8); p += 8; s.tb = p;
+	t0 = now();
+	for (i = 0; i < nt; ++i) ta[i].s = &s, ta[i].tid = i, pthread_create(&th[i], 0, ll_worker, &ta[i]);
+	for (i = 0; i < nt; ++i) pthread_join(th[i], 0);
+	t = now() - t0;
+	for (i = 0; i < nt; ++i) sum += s.sum[i];
+	printf("{\"threads\": %d, \"n_jobs\": %ld, \"ll_s\": %.6f, \"checksum\": %ld}\n", nt, (long)s.n_jobs, t, (long)sum);
+	return 0;
+}
+
 static void run(shared_t *s, int nt, void *(*fn)(void*))
 {
 	pthread_t th[256];
@@ -131,6 +198,10 @@ int main(int argc, char *argv[])
 	int nt, i;
 	double t0, t_seed, t_dp;
 	int64_t n_anchor = 0;
+	if (argc == 4 && strcmp(argv[1], "ll") == 0) {
+		nt = atoi(argv[3]);
+		return ll_main(argv[2], nt < 1 ? 1 : nt > 256 ? 256 : nt);
+	}
 	if (argc < 5) { fprintf(stderr, "usage: cpu_hotpath <idx_prefix> <reads.bin> <jobs.bin> <n_threads>\n"); return 1; }
 	memset(&s, 0, sizeof(s));
 	nt = atoi(argv[4]);

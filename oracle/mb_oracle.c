@@ -16,6 +16,9 @@
  *   ksw_extd2 / ksw_extz2      ksw2_extd2_sse.c:27-403, ksw2_extz2_sse.c:16-296, ksw2.h:124-197
  *                                                      textbook Gotoh global score; the compiled
  *                                                      reference on fuzzed jobs (all flag variants)
+ *   ksw_ll (u8 / i16 cores)    ksw2_ll_sse.c:49-333     textbook Gotoh local score (brute force on
+ *                                                      small jobs); the compiled reference on fuzzed
+ *                                                      jobs (both sizes, SUBO/STOP, saturation)
  *
  * The DP restatement is scalar but keeps the SSE code's observable semantics: int8 wrap-around
  * arithmetic, the 16-cell-aligned band (cells outside the exact band are computed and kept), the
@@ -758,6 +761,141 @@ int orc_extz2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, i
 	}
 	free(mem); free(H); free(p); free(off);
 	return cg.overflow ? -1 : cells;
+}
+
+/* ksw_ll_qinit + ksw_ll_u8_core / ksw_ll_i16_core (ksw2_ll_sse.c:49-333): local Smith-Waterman
+ * with affine gaps, score and end positions only, as the mate-rescue step calls it (pe.c:260-272).
+ *
+ * Restated over the query in plain left-to-right order instead of the SSE striped layout; the
+ * observable effects of the striping are kept:
+ *   - the query is padded to p*slen positions (p = 16 bytes or 8 words per vector,
+ *     slen = ceil(qlen/p)); padded positions score -1 (ksw2_ll_sse.c:82,90) and take part in the
+ *     row maxima;
+ *   - the first pass carries F only inside each stripe of slen consecutive positions (the
+ *     vector lanes, :149-172); E for the next row is taken from that first-pass H (:162-166);
+ *   - the lazy-F loop (:174-185) makes H the fixed point of the full-column F chain, sourced from
+ *     the first-pass H; here that chain is one sequential pass over the row;
+ *   - the row maximum is taken over the first-pass H (:159,187); qe is the smallest position
+ *     holding the maximum of the post-lazy-F row kept at te (:200-204);
+ *   - u8 arithmetic saturates (adds/subs with the matrix biased by shift, :154-170) and the
+ *     scan stops once gmax+shift reaches 255 (:197), reporting 255 without qe or a second best;
+ *   - the second best walks the b array of per-row maxima (:188-193), merged only into an entry
+ *     whose stored row is the previous row, outside te +- ceil(score / max(mat)) (:205-213).
+ * Returns 0, or -1 for arguments not restated: m outside 1..16, or gapo < 1.  With gapo == 0 the
+ * lazy-F loop's early exit (f <= H - (gapo+gape) after f -= gape) also fires on a tie, before
+ * the fixed point, at a point set by the SSE lane order; for gapo >= 1 it fires only once no
+ * lane can change, so the fixed point above is what the reference computes. */
+static inline int ll_sat_u8(int x) { return x < 0 ? 0 : x > 255 ? 255 : x; }
+static inline int ll_sat_i16(int x) { return x < -32768 ? -32768 : x > 32767 ? 32767 : x; }
+
+int orc_ll(int size, int qlen, const uint8_t *query, int m, const int8_t *mat, int tlen, const uint8_t *target,
+           int gapo, int gape, int xtra, orc_llrst_t *r)
+{
+	int p, slen, n, i, k, te = -1, gmax = 0, minsc, endsc, u8, shift, qmax, gapoe_, gape_, nb = 0, mb = 0;
+	int *Hp, *Hc, *Hd, *E, *Hmax, *sc;
+	uint64_t *b = 0;
+	uint8_t sh, md;
+
+	r->score = 0, r->te = r->qe = r->score2 = r->te2 = -1;
+	if (m < 1 || m > 16 || gapo < 1) return -1;
+	u8 = size <= 1;
+	p = u8 ? 16 : 8;
+	slen = (qlen + p - 1) / p;
+	n = slen * p;
+	for (k = 0, sh = 127, md = 0; k < m * m; ++k) {     /* :67-74 in uint8 arithmetic */
+		if (mat[k] < (int8_t)sh) sh = (uint8_t)mat[k];
+		if (mat[k] > (int8_t)md) md = (uint8_t)mat[k];
+	}
+	qmax = md;
+	shift = (uint8_t)(256 - sh);
+	minsc = (xtra & 0x40000) ? xtra & 0xffff : 0x10000;
+	endsc = (xtra & 0x20000) ? xtra & 0xffff : 0x10000;
+	/* the SSE code broadcasts gapo+gape and gape into bytes or words */
+	gapoe_ = u8 ? (uint8_t)(gapo + gape) : (uint16_t)(gapo + gape);
+	gape_ = u8 ? (uint8_t)gape : (uint16_t)gape;
+
+	sc = (int*)malloc(sizeof(int) * ((size_t)m * n + 1));
+	Hp = (int*)calloc((size_t)n + 1, sizeof(int));      /* previous row, post-lazy-F */
+	Hc = (int*)calloc((size_t)n + 1, sizeof(int));      /* current row, post-lazy-F */
+	Hd = (int*)calloc((size_t)n + 1, sizeof(int));      /* current row, first pass */
+	E = (int*)calloc((size_t)n + 1, sizeof(int));
+	Hmax = (int*)calloc((size_t)n + 1, sizeof(int));
+	for (i = 0; i < m; ++i)                             /* query profile, :77-94 */
+		for (k = 0; k < n; ++k) {
+			int s = k >= qlen ? -1 : mat[i * m + query[k]];
+			sc[i * n + k] = u8 ? (uint8_t)(int8_t)(s + shift) : s;
+		}
+	for (i = 0; i < tlen; ++i) {
+		int f = 0, imax = 0, *t;
+		const int *S = sc + (size_t)target[i] * n;
+		for (k = 0; k < n; ++k) {
+			int h, e = E[k], d = k == 0 ? 0 : Hp[k - 1];
+			if (k % slen == 0) f = 0;                   /* a new vector lane starts its own F */
+			if (u8) h = ll_sat_u8(ll_sat_u8(d + S[k]) - shift);
+			else h = ll_sat_i16(d + S[k]);
+			h = h > e ? h : e;
+			h = h > f ? h : f;
+			Hd[k] = h;
+			imax = imax > h ? imax : h;
+			if (u8) {
+				e = ll_sat_u8(e - gape_), f = ll_sat_u8(f - gape_);
+				E[k] = e > ll_sat_u8(h - gapoe_) ? e : ll_sat_u8(h - gapoe_);
+				f = f > ll_sat_u8(h - gapoe_) ? f : ll_sat_u8(h - gapoe_);
+			} else {                                    /* _mm_subs_epu16 on words, then signed max */
+				int a1, a2;
+				a1 =(int16_t)(uint16_t)((uint16_t)e > (uint16_t)gape_ ? (uint16_t)e - (uint16_t)gape_ : 0);
+				a2 = (int16_t)(uint16_t)((uint16_t)h > (uint16_t)gapoe_ ? (uint16_t)h - (uint16_t)gapoe_ : 0);
+				E[k] = a1 > a2 ? a1 : a2;
+				a1 = (int16_t)(uint16_t)((uint16_t)f > (uint16_t)gape_ ? (uint16_t)f - (uint16_t)gape_ : 0);
+				f = a1 > a2 ? a1 : a2;
+			}
+		}
+		for (k = 0, f = 0; k < n; ++k) {               /* the lazy-F fixed point */
+			int h = Hd[k] > f ? Hd[k] : f, a;
+			Hc[k] = h;
+			if (u8) {
+				a = ll_sat_u8(Hd[k] - gapoe_);
+				f = ll_sat_u8(f - gape_);
+			} else {
+				a = (int16_t)(uint16_t)((uint16_t)Hd[k] > (uint16_t)gapoe_ ? (uint16_t)Hd[k] - (uint16_t)gapoe_ : 0);
+				f = (int16_t)(uint16_t)((uint16_t)f > (uint16_t)gape_ ? (uint16_t)f - (uint16_t)gape_ : 0);
+			}
+			f = f > a ? f : a;
+		}
+		if (imax >= minsc) {                            /* :188-193 */
+			if (nb == 0 || (int32_t)b[nb - 1] + 1 != i) {
+				if (nb == mb) {
+					mb = mb ? mb * 2 : 16;
+					b = (uint64_t*)realloc(b, sizeof(uint64_t) * mb);
+				}
+				b[nb++] = (uint64_t)imax << 32 | (uint32_t)i;
+			} else if ((int)(b[nb - 1] >> 32) < imax) b[nb - 1] = (uint64_t)imax << 32 | (uint32_t)i;
+		}
+		if (imax > gmax) {
+			gmax = imax, te = i;
+			memcpy(Hmax, Hc, sizeof(int) * n);
+			if ((u8 && gmax + shift >= 255) || gmax >= endsc) break;
+		}
+		t = Hp, Hp = Hc, Hc = t;
+	}
+	r->score = u8 && gmax + shift >= 255 ? 255 : gmax;
+	r->te = te;
+	if (!u8 || r->score != 255) {
+		int best = -1, lo, hi;
+		for (k = 0; k < n; ++k)                         /* smallest position holding the maximum */
+			if (Hmax[k] > best) best = Hmax[k], r->qe = k;
+		if (b) {
+			int L = (r->score + qmax - 1) / qmax;
+			lo = te - L, hi = te + L;
+			for (i = 0; i < nb; ++i) {
+				int e = (int32_t)b[i];
+				if ((e < lo || e > hi) && (int)(b[i] >> 32) > r->score2)
+					r->score2 = (int)(b[i] >> 32), r->te2 = e;
+			}
+		}
+	}
+	free(b); free(sc); free(Hp); free(Hc); free(Hd); free(E); free(Hmax);
+	return 0;
 }
 
 /* main.c:164-200 (`minibwa bench`): checksum over n splitmix64-drawn queries; type 0 = rank2a,

@@ -49,6 +49,10 @@ int orc_extz2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, i
               int8_t q, int8_t e, int w, int zdrop, int end_bonus, int flag,
               orc_ez_t *ez, uint32_t *cigar, int cigar_cap);
 int64_t orc_dp_cells(int qlen, int tlen, int w);
+
+typedef struct { int32_t score, te, qe, score2, te2; } orc_llrst_t;   /* ksw_llrst_t, ksw2.h:47-51 */
+int orc_ll(int size, int qlen, const uint8_t *query, int m, const int8_t *mat, int tlen, const uint8_t *target,
+           int gapo, int gape, int xtra, orc_llrst_t *r);
 uint64_t orc_bench(const orc_bwt_t *b, int type, int64_t n, int intv);
 
 #ifdef __cplusplus

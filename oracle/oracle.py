@@ -37,6 +37,15 @@ class OrcEz(C.Structure):
                                          "mte_q", "score", "reach_end", "n_cigar")]
 
 
+class LlRst(C.Structure):
+    """ksw_llrst_t (ksw2.h:47-51)."""
+    _fields_ = [(n, C.c_int32) for n in ("score", "te", "qe", "score2", "te2")]
+
+
+LL_FIELDS = ("score", "te", "qe", "score2", "te2")
+KSW_LL_STOP = 0x20000
+KSW_LL_SUBO = 0x40000
+
 EZ_FIELDS = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "score", "reach_end")
 
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C")
@@ -69,6 +78,9 @@ def _load_oracle():
     L.orc_bench.argtypes = [C.POINTER(OrcBwt), C.c_int, C.c_int64, C.c_int]
     L.orc_dp_cells.restype = C.c_int64
     L.orc_dp_cells.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.orc_ll.restype = C.c_int
+    L.orc_ll.argtypes = [C.c_int, C.c_int, _u8p, C.c_int, _i8p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int,
+                         C.POINTER(LlRst)]
     return L
 
 
@@ -239,6 +251,11 @@ def ref():
         dp_common = [C.c_void_p, C.c_int, _u8p, C.c_int, _u8p, C.c_int8, _i8p]
         L.ksw_extd2_sse.argtypes = dp_common + [C.c_int8] * 4 + [C.c_int] * 4 + [C.POINTER(RefExtz)]
         L.ksw_extz2_sse.argtypes = dp_common + [C.c_int8] * 2 + [C.c_int] * 4 + [C.POINTER(RefExtz)]
+        L.ksw_ll_qinit.restype = C.c_void_p
+        L.ksw_ll_qinit.argtypes = [C.c_void_p, C.c_int, C.c_int, _u8p, C.c_int, _i8p]
+        for fn in (L.ksw_ll_u8_core, L.ksw_ll_i16_core):
+            fn.restype = LlRst
+            fn.argtypes = [C.c_void_p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int]
         _LIBC = C.CDLL(None)
         _LIBC.free.argtypes = [C.c_void_p]
         _REF = L
@@ -334,3 +351,28 @@ def ref_extz2(query, target, mat, q, e, w, zdrop, end_bonus, flag, m=5):
     mat = np.ascontiguousarray(mat, np.int8)
     return _ref_dp(L.ksw_extz2_sse, (len(query), query, len(target), target, m, mat, q, e, w, zdrop, end_bonus,
                                      flag))
+
+
+def _ll_args(query, target, mat):
+    return (np.ascontiguousarray(query, np.uint8), np.ascontiguousarray(target, np.uint8),
+            np.ascontiguousarray(mat, np.int8))
+
+
+def orc_ll(query, target, mat, size, gapo, gape, xtra, m=5):
+    """The C restatement of ksw_ll_qinit + ksw_ll_{u8,i16}_core -> dict of ksw_llrst_t fields."""
+    query, target, mat = _ll_args(query, target, mat)
+    r = LlRst()
+    if orc().orc_ll(size, len(query), query, m, mat, len(target), target, gapo, gape, xtra, C.byref(r)) != 0:
+        raise ValueError("orc_ll: bad arguments")
+    return {f: int(getattr(r, f)) for f in LL_FIELDS}
+
+
+def ref_ll(query, target, mat, size, gapo, gape, xtra, m=5):
+    """The compiled reference: ksw_ll_qinit (ksw2_ll_sse.c:49-96) then the u8 or i16 core."""
+    L = ref()
+    query, target, mat = _ll_args(query, target, mat)
+    q = L.ksw_ll_qinit(None, size, len(query), query, m, mat)
+    core = L.ksw_ll_u8_core if size <= 1 else L.ksw_ll_i16_core
+    r = core(q, len(target), target, gapo, gape, xtra)
+    _LIBC.free(C.c_void_p(q))
+    return {f: int(getattr(r, f)) for f in LL_FIELDS}

@@ -182,3 +182,29 @@ def cigar_score(cigar, query, target, mat, q, e, q2, e2, m=5):
         else:
             raise ValueError(f"unexpected CIGAR op {op}")
     return sc, i, j
+
+
+def sw_local_rows(query, target, mat, gapo, gape, n_pos, id_adjacent, m=5):
+    """Textbook local Smith-Waterman with affine gaps (a gap of k bases costs gapo + k*gape) over
+    the query padded to n_pos positions, each padded position scoring -1 against every base.
+    id_adjacent: whether a deletion (E, consuming target) may open right after an insertion (F,
+    consuming query).  Returns the H matrix as a list of rows, one per target base."""
+    ql = len(query)
+    sc = [[(int(mat[t * m + int(query[k])]) if k < ql else -1) for k in range(n_pos)] for t in range(m)]
+    Hp = [0] * n_pos
+    E = [0] * n_pos
+    rows = []
+    for i in range(len(target)):
+        s = sc[min(int(target[i]), m - 1)]
+        H = [0] * n_pos
+        f = 0
+        for k in range(n_pos):
+            d = (Hp[k - 1] if k else 0) + s[k]
+            hn = max(0, d, E[k])                # paths not ending in an insertion
+            h = max(hn, f)
+            H[k] = h
+            E[k] = max(0, E[k] - gape, (h if id_adjacent else hn) - gapo - gape)
+            f = max(0, f - gape, h - gapo - gape)
+        rows.append(H)
+        Hp = H
+    return rows

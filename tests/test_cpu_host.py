@@ -75,6 +75,8 @@ def test_header_layouts_match_python(tmp_path):
         "mbg_bwt_desc_t": ["primary", "L2", "seq_len", "data", "sa_bit", "n_sa", "sa"],
         "mbg_l2b_desc_t": ["tot_len", "n_ctg", "ctg_len", "n_pac", "pac", "n_ambi", "ambi"],
         "mbg_idx_info_t": ["seq_len", "primary", "tot_len", "n_ctg", "pre_len", "sa_bit", "device_bytes"],
+        "mbg_ll_job_t": ["qoff", "toff", "qlen", "tlen", "size", "pad"],
+        "mbg_ll_result_t": ["score", "te", "qe", "score2", "te2"],
     }
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "mbg.h"', "int main(void) {"]
     for s, fields in checks.items():
@@ -103,6 +105,8 @@ def test_header_layouts_match_python(tmp_path):
     np_layout("mbg_dp_job_t", mbg.DP_JOB_DTYPE)
     np_layout("mbg_dp_result_t", mbg.DP_RES_DTYPE)
     np_layout("mbg_job_meta_t", mbg.JOB_META_DTYPE)
+    np_layout("mbg_ll_job_t", mbg.LL_JOB_DTYPE)
+    np_layout("mbg_ll_result_t", mbg.LL_RES_DTYPE)
     sr = mbg.SEED_REC_DTYPE
     assert got[("mbg_seed_rec_t", "sizeof")] == sr.itemsize
     assert got[("mbg_seed_rec_t", "rid")] == sr.fields["rid"][1] and got[("mbg_seed_rec_t", "pass")] == sr.fields["pass"][1]

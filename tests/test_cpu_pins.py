@@ -230,3 +230,83 @@ def test_dp_cells_band_count():
     for ql, tl, w in [(1, 1, 1), (10, 30, 5), (150, 311, 161), (300, 400, 101), (57, 20, 3), (40, 41, 1)]:
         i, j = np.meshgrid(np.arange(ql), np.arange(tl), indexing="ij")
         assert L.orc_dp_cells(ql, tl, w) == int(np.count_nonzero(np.abs(i - j) <= w)), (ql, tl, w)
+
+
+def _ll_expect(rows, gmax_cap, minsc, qmax):
+    """ksw_llrst_t from the H rows of a local DP under ksw2_ll_sse.c:186-213's bookkeeping (per-row
+    maxima, the run-merged b array, the second best outside te +- ceil(score/qmax))."""
+    gmax, te, qe, b = 0, -1, 0, []
+    for i, H in enumerate(rows):
+        imax = max(H)
+        if imax >= minsc:
+            if not b or b[-1][1] + 1 != i:
+                b.append([imax, i])
+            elif b[-1][0] < imax:
+                b[-1] = [imax, i]
+        if imax > gmax:
+            gmax, te = imax, i
+            qe = H.index(imax)
+    assert gmax < gmax_cap
+    s2, te2 = -1, -1
+    if b:
+        L = (gmax + qmax - 1) // qmax
+        for v, r in b:
+            if (r < te - L or r > te + L) and v > s2:
+                s2, te2 = v, r
+    return {"score": gmax, "te": te, "qe": qe, "score2": s2, "te2": te2}
+
+
+@pytest.mark.parametrize("size", [1, 2])
+def test_ll_textbook(size):
+    """ksw_ll against textbook local Smith-Waterman on small jobs.  The reference takes E of the
+    next row from the first-pass H, which holds F only within a stripe, so its H lies between the
+    textbook H that never opens a deletion right after an insertion and the one that may; where the
+    two agree on every row maximum (and on the row of the best score) every field is fixed."""
+    rng = np.random.default_rng(41 + size)
+    params = [(1, 4, 6, 1), (2, 8, 12, 2), (1, 2, 3, 1), (1, 3, 1, 1), (2, 3, 4, 0)]
+    n_cmp = 0
+    for it in range(400):
+        a, b_mm, gapo, gape = params[it % len(params)]
+        mat = O.gen_nt4_mat(a, b_mm, 0, 1)
+        ql, tl = int(rng.integers(1, 30)), int(rng.integers(0, 50))
+        qs = rng.integers(0, 4, ql).astype(np.uint8)
+        ts = rng.integers(0, 4, tl).astype(np.uint8)
+        if tl > 4 and ql > 2 and rng.random() < 0.8:   # a mutated copy of a query stretch
+            st, n = int(rng.integers(0, tl - 2)), int(rng.integers(2, ql + 1))
+            cp = qs[int(rng.integers(0, ql - n + 1)):][:n].copy()
+            cp[rng.random(n) < 0.1] = rng.integers(0, 4)
+            cp = np.delete(cp, np.nonzero(rng.random(n) < 0.05)[0])
+            ts[st:st + len(cp)] = cp[:tl - st]
+        if rng.random() < 0.1:
+            ts[rng.random(tl) < 0.1] = 4
+        p = 16 if size == 1 else 8
+        n_pos = (ql + p - 1) // p * p
+        xtra = int(rng.choice([0, O.KSW_LL_SUBO | 3, O.KSW_LL_SUBO | 0, O.KSW_LL_SUBO | 8]))
+        lo = brute.sw_local_rows(qs, ts, mat, gapo, gape, n_pos, False)
+        hi = brute.sw_local_rows(qs, ts, mat, gapo, gape, n_pos, True)
+        r = O.orc_ll(qs, ts, mat, size, gapo, gape, xtra)
+        gl = max([max(h) for h in lo], default=0)
+        gh = max([max(h) for h in hi], default=0)
+        assert gl <= r["score"] <= gh, it
+        if [max(h) for h in lo] != [max(h) for h in hi]:
+            continue
+        minsc = xtra & 0xffff if xtra & O.KSW_LL_SUBO else 0x10000
+        e = _ll_expect(lo, 200, minsc, int(mat.max()))
+        if e["te"] >= 0 and lo[e["te"]] != hi[e["te"]]:
+            e["qe"] = r["qe"]
+        assert r == e, (it, r, e)
+        n_cmp += 1
+    assert n_cmp > 300
+
+
+def test_ll_u8_saturation():
+    """The u8 core stops at the first row whose maximum reaches 255 - shift and reports 255 with
+    no query end or second best (ksw2_ll_sse.c:197-199); the i16 core keeps the exact score."""
+    mat = O.gen_nt4_mat(1, 4, 0, 1)          # shift = 4: stops once a row reaches 251
+    rng = np.random.default_rng(5)
+    qs = rng.integers(0, 4, 300).astype(np.uint8)
+    ts = np.concatenate([rng.integers(0, 4, 37).astype(np.uint8), qs, rng.integers(0, 4, 20).astype(np.uint8)])
+    r = O.orc_ll(qs, ts, mat, 1, 6, 1, O.KSW_LL_SUBO | 19)
+    assert r == {"score": 255, "te": 37 + 250, "qe": -1, "score2": -1, "te2": -1}
+    r = O.orc_ll(qs, ts, mat, 2, 6, 1, O.KSW_LL_SUBO | 19)
+    assert (r["score"], r["te"], r["qe"]) == (300, 37 + 299, 299)

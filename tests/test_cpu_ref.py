@@ -129,3 +129,46 @@ def test_extend_vs_reference(chrm_pair, chrm_reads):
     for iv in ivs:
         for d in (0, 1):
             assert a.extend(iv, d) == b.extend(iv, d), (iv, d)
+
+
+def _ll_case(rng, qmax, tmax):
+    ql = int(rng.integers(1, qmax))
+    tl = int(rng.integers(0, tmax))
+    q = rng.integers(0, 4, ql).astype(np.uint8)
+    if rng.random() < 0.3:
+        q[rng.random(ql) < 0.03] = 4
+    t = rng.integers(0, 4, tl).astype(np.uint8)
+    if tl and rng.random() < 0.8:                    # one or two mutated copies of the query
+        for _ in range(1 + (rng.random() < 0.3)):
+            st = int(rng.integers(0, max(1, tl - ql // 2)))
+            cp = q.copy()
+            mut = rng.random(ql) < rng.choice([0.0, 0.02, 0.1])
+            cp[mut] = rng.integers(0, 4, int(mut.sum()))
+            n = min(ql, tl - st)
+            t[st:st + n] = cp[:n]
+    if rng.random() < 0.2:
+        t[rng.random(tl) < 0.02] = 4
+    return q, t
+
+
+@pytest.mark.parametrize("size", [1, 2])
+def test_ll_fuzz_vs_reference(size):
+    """orc_ll against ksw_ll_qinit + ksw_ll_u8_core / ksw_ll_i16_core: score, te, qe, score2 and te2
+    over matrices (with transition and methylation variants), gap pairs with gapo >= 1, SUBO
+    thresholds, STOP scores and u8 saturation."""
+    rng = np.random.default_rng(61 + size)
+    n_sat = n_sub = 0
+    for it in range(1500):
+        q, t = _ll_case(rng, 300, 700)
+        mat = O.gen_nt4_mat(1, int(rng.choice([4, 2, 3, 6])), int(rng.choice([0, 0, 2])), 1, int(rng.choice([0, 0, 1, 2])))
+        if rng.random() < 0.2:
+            mat = O.gen_nt4_mat(int(rng.integers(1, 4)), int(rng.integers(1, 6)), 0, 1)
+        gapo, gape = int(rng.choice([6, 3, 1, 2, 10])), int(rng.choice([1, 2, 1, 0, 3]))
+        xtra = int(rng.choice([0, O.KSW_LL_SUBO | 19, O.KSW_LL_SUBO | int(rng.integers(0, 60)),
+                               O.KSW_LL_STOP | int(rng.integers(1, 200)), O.KSW_LL_SUBO | O.KSW_LL_STOP | 30]))
+        a = O.orc_ll(q, t, mat, size, gapo, gape, xtra)
+        b = O.ref_ll(q, t, mat, size, gapo, gape, xtra)
+        assert a == b, (it, len(q), len(t), gapo, gape, hex(xtra), a, b)
+        n_sat += a["score"] == 255
+        n_sub += a["score2"] >= 0
+    assert n_sub > 100 and (size == 2 or n_sat > 20)
