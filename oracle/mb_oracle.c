@@ -19,6 +19,8 @@
  *   ksw_ll (u8 / i16 cores)    ksw2_ll_sse.c:49-333     textbook Gotoh local score (brute force on
  *                                                      small jobs); the compiled reference on fuzzed
  *                                                      jobs (both sizes, SUBO/STOP, saturation)
+ *   mate rescue                pe.c:209-312,342-349     mb_ungap: brute-force vote count; mb_ungap and
+ *     (ungap, matesw_align)                            mb_matesw_align of the compiled reference
  *
  * The DP restatement is scalar but keeps the SSE code's observable semantics: int8 wrap-around
  * arithmetic, the 16-cell-aligned band (cells outside the exact band are computed and kept), the
@@ -895,6 +897,120 @@ int orc_ll(int size, int qlen, const uint8_t *query, int m, const int8_t *mat, i
 		}
 	}
 	free(b); free(sc); free(Hp); free(Hc); free(Hd); free(E); free(Hmax);
+	return 0;
+}
+
+/* ksw_gen_nt4_mat (ksw2.h:199-219): 5x5 scores for A,C,G,T,N; mt 1 = C->T, 2 = G->A conversion */
+static void gen_nt4_mat(int8_t *mat, int a, int b, int b_ts, int b_ambi, int mt)
+{
+	int i, j;
+	a = a < 0 ? -a : a, b = b > 0 ? -b : b, b_ambi = b_ambi > 0 ? -b_ambi : b_ambi;
+	for (i = 0; i < 4; ++i) {
+		for (j = 0; j < 4; ++j) mat[i * 5 + j] = (int8_t)(i == j ? a : b);
+		mat[i * 5 + 4] = (int8_t)b_ambi;
+	}
+	for (j = 0; j < 5; ++j) mat[20 + j] = (int8_t)b_ambi;
+	if (b_ts != 0 && b_ts != b) {
+		b_ts = b_ts > 0 ? -b_ts : b_ts;
+		mat[2] = mat[8] = mat[10] = mat[16] = (int8_t)b_ts;
+	}
+	if (mt == 1) mat[8] = (int8_t)a;
+	else if (mt == 2) mat[10] = (int8_t)a;
+}
+
+/* mb_ungap (pe.c:209-247): the offset (target position - query position) of most k-mer hits, each
+ * target k-mer voting for the diagonal of the LAST query position where the same k-mer ends.
+ * Bases go through the methylation conversion first (mt 1: C->T, 2: G->A); an N restarts the
+ * k-mer.  Returns the top vote count; *max_i its first diagonal (-1 when no vote), *n_good the
+ * diagonals with more than half of it, *n_kmer the distinct query k-mers.  kmer <= 8. */
+int32_t orc_ungap(int32_t qlen, const uint8_t *qseq, int32_t tlen, const uint8_t *tseq, int32_t kmer, int32_t mt,
+                  int32_t *max_i, int32_t *n_good, int32_t *n_kmer)
+{
+	static const uint8_t conv[3][4] = { {0, 1, 2, 3}, {0, 3, 2, 3}, {0, 1, 0, 3} };
+	const uint8_t *cv = conv[mt == 1 ? 1 : mt == 2 ? 2 : 0];
+	uint32_t mask = (1U << 2 * kmer) - 1, x = 0;
+	int32_t i, l, best = 0, *a;
+	uint16_t *h;
+	*max_i = -1, *n_good = *n_kmer = 0;
+	if (qlen >= 65535) return 0;
+	a = (int32_t*)calloc((size_t)tlen + 1, sizeof(int32_t));
+	h = (uint16_t*)calloc((size_t)mask + 1, sizeof(uint16_t));
+	for (i = l = 0; i < qlen; ++i) {
+		if (qseq[i] > 3) { x = 0, l = 0; continue; }
+		x = (x << 2 | cv[qseq[i]]) & mask;
+		if (++l < kmer) continue;
+		*n_kmer += h[x] == 0;
+		h[x] = (uint16_t)i;                         /* the last occurrence wins */
+	}
+	for (i = l = 0, x = 0; i < tlen; ++i) {
+		if (tseq[i] > 3) { x = 0, l = 0; continue; }
+		x = (x << 2 | cv[tseq[i]]) & mask;
+		if (++l >= kmer && h[x] > 0 && i >= h[x]) ++a[i - h[x]];
+	}
+	for (i = 0; i < tlen; ++i)
+		if (a[i] > best) best = a[i], *max_i = i;
+	for (i = 0; i < tlen; ++i) *n_good += a[i] > best >> 1;
+	free(a); free(h);
+	return best;
+}
+
+/* mb_matesw_align (pe.c:249-312) up to the hit it fills, before mb_append_cigar/mb_update_extra:
+ * ksw_ll with the match-1 matrix and gaps divided by a (u8 core when the best possible score stays
+ * below 255 - mismatch), then ksw_extz2 of the reversed prefixes ending at (qe, te) with
+ * EXTZ_ONLY|RIGHT|REV_CIGAR (and GENERIC_SC under a methylation conversion).
+ * p = {a, b, b_ts, b_ambi, q, e, bw, zdrop, end_bonus, min_dp_max, min_len}.  Returns 0. */
+int orc_matesw_align(const int32_t p[11], int32_t qlen, const uint8_t *qseq, int32_t tlen, const uint8_t *tseq, int32_t min_sc,
+                     int32_t mt, orc_rescue_t *r, uint32_t *cigar, int cigar_cap)
+{
+	const int a = p[0], b = p[1], b_ts = p[2], b_ambi = p[3], q = p[4], e = p[5], bw = p[6], zdrop = p[7], eb = p[8];
+	const int min_dp_max = p[9], min_len = p[10];
+	int32_t max_sc = qlen < tlen ? qlen : tlen, b_mm = (b + a - 1) / a, sz, i;
+	int8_t mat[25];
+	memset(r, 0, sizeof(*r));
+	r->ll.score = r->ll.te = r->ll.qe = r->ll.score2 = r->ll.te2 = -1;
+	if (max_sc >= 32767) return 0;
+	gen_nt4_mat(mat, 1, b_mm, (b_ts + a - 1) / a, (b_ambi + a - 1) / a, mt);
+	sz = max_sc < 255 - b_mm ? 1 : 2;
+	orc_ll(sz, qlen, qseq, 5, mat, tlen, tseq, (q + a - 1) / a, (e + a - 1) / a, 0x40000 | min_len, &r->ll);
+	r->ran_ll = 1;
+	if (r->ll.score >= min_dp_max && r->ll.score >= min_sc) {
+		int32_t qe = r->ll.qe + 1, te = r->ll.te + 1, flag = 0x40 | 0x02 | 0x80 | (mt ? 0x04 : 0);
+		uint8_t *rq = (uint8_t*)malloc((size_t)qe + 1), *rt = (uint8_t*)malloc((size_t)te + 1);
+		for (i = 0; i < qe; ++i) rq[i] = qseq[qe - 1 - i];
+		for (i = 0; i < te; ++i) rt[i] = tseq[te - 1 - i];
+		gen_nt4_mat(mat, a, b, b_ts, b_ambi, mt);
+		orc_extz2(qe, rq, te, rt, 5, mat, (int8_t)q, (int8_t)e, bw, zdrop, eb, flag, &r->ez, cigar, cigar_cap);
+		r->ran_dp = 1;
+		free(rq); free(rt);
+		if (r->ez.n_cigar > 0 && r->ez.max >= min_dp_max * a) {
+			r->rescued = 1, r->qe = qe, r->te = te;
+			r->ts = te - (r->ez.reach_end ? r->ez.mqe_t + 1 : r->ez.max_t + 1);
+			r->qs = qe - (r->ez.reach_end ? qe : r->ez.max_q + 1);
+			r->dp_max2 = (int32_t)((double)r->ez.max / r->ll.score * r->ll.score2 + .499);
+			if (r->dp_max2 < 0) r->dp_max2 = 0;
+		}
+	}
+	return 0;
+}
+
+/* one rescue candidate of mb_matesw_core (pe.c:342-349), window-relative: mb_ungap over the whole
+ * window, the narrowing to [max_i - qlen/2, +2*qlen) when the vote is clear, and mb_matesw_align on
+ * the (narrowed) window when the vote passes.  ug = {max_ug, max_i, n_good, n_kmer}, win2 = {ts2, te2}. */
+int orc_matesw_cand(const int32_t p[11], int32_t qlen, const uint8_t *qseq, int32_t wlen, const uint8_t *win, int32_t min_sc,
+                    int32_t mt, int32_t ug[4], int32_t win2[2], int32_t *aligned, orc_rescue_t *r, uint32_t *cigar, int cigar_cap)
+{
+	int32_t ts2 = 0, te2 = wlen;
+	ug[0] = orc_ungap(qlen, qseq, wlen, win, 7, mt, &ug[1], &ug[2], &ug[3]);
+	if (ug[0] >= 10 && ug[0] >= qlen >> 1 && ug[2] == 1) {
+		ts2 = ug[1] - qlen / 2, te2 = ts2 + qlen * 2;
+		if (ts2 < 0) ts2 = 0;
+		if (te2 > wlen) te2 = wlen;
+	}
+	win2[0] = ts2, win2[1] = te2;
+	*aligned = ug[0] >= 10 || ug[0] >= ug[3] * 0.33;
+	memset(r, 0, sizeof(*r));
+	r->ll.score = r->ll.te = r->ll.qe = r->ll.score2 = r->ll.te2 = -1;
+	if (*aligned) orc_matesw_align(p, qlen, qseq, te2 - ts2, win + ts2, min_sc, mt, r, cigar, cigar_cap);
 	return 0;
 }
 

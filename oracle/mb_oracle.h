@@ -53,6 +53,19 @@ int64_t orc_dp_cells(int qlen, int tlen, int w);
 typedef struct { int32_t score, te, qe, score2, te2; } orc_llrst_t;   /* ksw_llrst_t, ksw2.h:47-51 */
 int orc_ll(int size, int qlen, const uint8_t *query, int m, const int8_t *mat, int tlen, const uint8_t *target,
            int gapo, int gape, int xtra, orc_llrst_t *r);
+
+typedef struct {               /* mb_matesw_align's results before mb_update_extra */
+	int32_t ran_ll, ran_dp, rescued;
+	orc_llrst_t ll;
+	orc_ez_t ez;
+	int32_t qs, qe, ts, te, dp_max2;    /* window-relative */
+} orc_rescue_t;
+int32_t orc_ungap(int32_t qlen, const uint8_t *qseq, int32_t tlen, const uint8_t *tseq, int32_t kmer, int32_t mt,
+                  int32_t *max_i, int32_t *n_good, int32_t *n_kmer);
+int orc_matesw_align(const int32_t p[11], int32_t qlen, const uint8_t *qseq, int32_t tlen, const uint8_t *tseq, int32_t min_sc,
+                     int32_t mt, orc_rescue_t *r, uint32_t *cigar, int cigar_cap);
+int orc_matesw_cand(const int32_t p[11], int32_t qlen, const uint8_t *qseq, int32_t wlen, const uint8_t *win, int32_t min_sc,
+                    int32_t mt, int32_t ug[4], int32_t win2[2], int32_t *aligned, orc_rescue_t *r, uint32_t *cigar, int cigar_cap);
 uint64_t orc_bench(const orc_bwt_t *b, int type, int64_t n, int intv);
 
 #ifdef __cplusplus

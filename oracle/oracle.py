@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBORACLE = os.path.join(HERE, "liboracle.so")
 LIBREF = os.path.join(HERE, "_ref", "libmbref.so")
+LIBREF_PE = os.path.join(HERE, "_ref", "libmbref_pe.so")
 REF_CLI = os.path.join(HERE, "_ref", "minibwa")
 
 
@@ -43,6 +44,27 @@ class LlRst(C.Structure):
 
 
 LL_FIELDS = ("score", "te", "qe", "score2", "te2")
+
+
+class OrcRescue(C.Structure):
+    _fields_ = [("ran_ll", C.c_int32), ("ran_dp", C.c_int32), ("rescued", C.c_int32), ("ll", LlRst), ("ez", OrcEz),
+                ("qs", C.c_int32), ("qe", C.c_int32), ("ts", C.c_int32), ("te", C.c_int32), ("dp_max2", C.c_int32)]
+
+
+class MbxHit(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("rescued", "qs", "qe", "score", "score0", "subsc", "dp_max", "dp_max2",
+                                         "dp_score", "dp_max0", "mlen", "blen", "n_ambi", "n_cigar")] + \
+               [("ts", C.c_int64), ("te", C.c_int64)]
+
+
+HIT_FIELDS = ("rescued", "qs", "qe", "score", "score0", "subsc", "dp_max", "dp_max2", "dp_score", "dp_max0", "mlen",
+              "blen", "n_ambi", "n_cigar", "ts", "te")
+EZ11 = ("max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "score", "reach_end", "n_cigar")
+
+
+def rescue_params(a=2, b=8, b_ts=0, b_ambi=1, q=12, e=2, bw=100, zdrop=100, end_bonus=5, min_dp_max=30, min_len=19):
+    """The mb_opt_t fields mate rescue reads (pe.c:249-312), as the int32[11] the rescue functions take."""
+    return np.array([a, b, b_ts, b_ambi, q, e, bw, zdrop, end_bonus, min_dp_max, min_len], np.int32)
 KSW_LL_STOP = 0x20000
 KSW_LL_SUBO = 0x40000
 
@@ -52,6 +74,7 @@ _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C")
 _i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C")
 
 
 def _load_oracle():
@@ -78,6 +101,12 @@ def _load_oracle():
     L.orc_bench.argtypes = [C.POINTER(OrcBwt), C.c_int, C.c_int64, C.c_int]
     L.orc_dp_cells.restype = C.c_int64
     L.orc_dp_cells.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.orc_ungap.restype = C.c_int32
+    L.orc_ungap.argtypes = [C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32] + [C.POINTER(C.c_int32)] * 3
+    L.orc_matesw_align.argtypes = [_i32p, C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, C.POINTER(OrcRescue),
+                                   _u32p, C.c_int]
+    L.orc_matesw_cand.argtypes = [_i32p, C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, _i32p, _i32p,
+                                  C.POINTER(C.c_int32), C.POINTER(OrcRescue), _u32p, C.c_int]
     L.orc_ll.restype = C.c_int
     L.orc_ll.argtypes = [C.c_int, C.c_int, _u8p, C.c_int, _i8p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int,
                          C.POINTER(LlRst)]
@@ -376,3 +405,101 @@ def ref_ll(query, target, mat, size, gapo, gape, xtra, m=5):
     r = core(q, len(target), target, gapo, gape, xtra)
     _LIBC.free(C.c_void_p(q))
     return {f: int(getattr(r, f)) for f in LL_FIELDS}
+
+
+def _rescue_dict(r, cig):
+    ez = {f: int(getattr(r.ez, f)) for f in EZ11}
+    d = {"ran_ll": r.ran_ll, "ran_dp": r.ran_dp, "rescued": r.rescued, "ll": {f: int(getattr(r.ll, f)) for f in LL_FIELDS},
+         "ez": ez, "qs": r.qs, "qe": r.qe, "ts": r.ts, "te": r.te, "dp_max2": r.dp_max2,
+         "cigar": [int(c) for c in cig[:max(0, ez["n_cigar"])]] if r.ran_dp else []}
+    return d
+
+
+def orc_ungap(query, target, mt=0, kmer=7):
+    """mb_ungap restated -> (max, max_i, n_good, n_kmer)."""
+    q = np.ascontiguousarray(query, np.uint8)
+    t = np.ascontiguousarray(target, np.uint8)
+    a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+    m = orc().orc_ungap(len(q), q, len(t), t, kmer, mt, C.byref(a), C.byref(b), C.byref(c))
+    return int(m), a.value, b.value, c.value
+
+
+def orc_matesw_cand(p, query, window, min_sc, mt=0):
+    """One mate-rescue candidate restated (pe.c:342-349 + mb_matesw_align before mb_update_extra)."""
+    q = np.ascontiguousarray(query, np.uint8)
+    w = np.ascontiguousarray(window, np.uint8)
+    ug = np.zeros(4, np.int32)
+    w2 = np.zeros(2, np.int32)
+    al = C.c_int32()
+    r = OrcRescue()
+    cig = np.zeros(len(q) + len(w) + 8, np.uint32)
+    orc().orc_matesw_cand(np.ascontiguousarray(p, np.int32), len(q), q, len(w), w, min_sc, mt, ug, w2, C.byref(al),
+                          C.byref(r), cig, len(cig))
+    d = _rescue_dict(r, cig)
+    d.update(ug=tuple(int(x) for x in ug), win2=(int(w2[0]), int(w2[1])), aligned=al.value)
+    return d
+
+
+_REFPE = None
+
+
+def have_ref_pe():
+    return os.path.exists(LIBREF_PE)
+
+
+def ref_pe():
+    global _REFPE
+    if _REFPE is None:
+        if not have_ref_pe():
+            raise RuntimeError(f"{LIBREF_PE} is not built (make -C oracle ref)")
+        L = C.CDLL(LIBREF_PE)
+        L.mbx_ungap.restype = C.c_int32
+        L.mbx_ungap.argtypes = [C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, _i32p]
+        L.mbx_matesw_align.argtypes = [_i32p, C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, C.POINTER(MbxHit),
+                                       _u32p, C.c_int32, _i32p, _u32p, C.c_int32]
+        L.mbx_finish.argtypes = [_i32p, C.c_int32, _u8p, _u8p, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32,
+                                 C.c_int32, C.c_int32, C.c_int32, C.c_int32, _u32p, C.POINTER(MbxHit), _u32p, C.c_int32]
+        _REFPE = L
+    return _REFPE
+
+
+def ref_ungap(query, target, mt=0, kmer=7):
+    """The reference's mb_ungap (pe.c:209-247) -> (max, max_i, n_good, n_kmer)."""
+    q = np.ascontiguousarray(query, np.uint8)
+    t = np.ascontiguousarray(target, np.uint8)
+    out = np.zeros(3, np.int32)
+    m = ref_pe().mbx_ungap(len(q), q, len(t), t, kmer, mt, out)
+    return int(m), int(out[0]), int(out[1]), int(out[2])
+
+
+def _hit_dict(h, cig):
+    d = {f: int(getattr(h, f)) for f in HIT_FIELDS}
+    d["cigar"] = [int(c) for c in cig[:d["n_cigar"]]]
+    return d
+
+
+def ref_matesw_align(p, query, window, min_sc, mt=0):
+    """The reference's mb_matesw_align (pe.c:249-312) -> (hit after mb_update_extra, ez fields, ez CIGAR)."""
+    q = np.ascontiguousarray(query, np.uint8).copy()
+    w = np.ascontiguousarray(window, np.uint8).copy()
+    h = MbxHit()
+    cap = len(q) + len(w) + 8
+    cig, ezc = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+    ez = np.zeros(11, np.int32)
+    ref_pe().mbx_matesw_align(np.ascontiguousarray(p, np.int32), len(q), q, len(w), w, min_sc, mt, C.byref(h), cig, cap,
+                              ez, ezc, cap)
+    ezd = {f: int(ez[i]) for i, f in enumerate(EZ11)}
+    return _hit_dict(h, cig), ezd, [int(c) for c in ezc[:max(0, ezd["n_cigar"])]]
+
+
+def ref_finish(p, query, window, mt, qs, qe, ts, te, ez_max, score, score2, dp_max2, cigar):
+    """A rescued hit built from pre-update fields, then the reference's mb_append_cigar + mb_update_extra."""
+    q = np.ascontiguousarray(query, np.uint8)
+    w = np.ascontiguousarray(window, np.uint8)
+    c0 = np.ascontiguousarray(cigar, np.uint32) if len(cigar) else np.zeros(1, np.uint32)
+    h = MbxHit()
+    cap = len(q) + len(w) + 8
+    cig = np.zeros(cap, np.uint32)
+    ref_pe().mbx_finish(np.ascontiguousarray(p, np.int32), mt, q, w, qs, qe, ts, te, ez_max, score, score2, dp_max2,
+                        len(cigar), c0, C.byref(h), cig, cap)
+    return _hit_dict(h, cig)

@@ -208,3 +208,28 @@ def sw_local_rows(query, target, mat, gapo, gape, n_pos, id_adjacent, m=5):
         rows.append(H)
         Hp = H
     return rows
+
+
+def ungap_votes(query, target, mt=0, k=7):
+    """mb_ungap's vote by its definition: every target position j ending a k-mer without N votes for
+    diagonal j - i, where i is the last query position ending the same (methylation-converted) k-mer,
+    when i <= j.  Returns (votes per diagonal as a dict, number of distinct query k-mers)."""
+    conv = {0: (0, 1, 2, 3), 1: (0, 3, 2, 3), 2: (0, 1, 0, 3)}[mt]
+
+    def kmers(s):
+        out = {}
+        for j in range(k - 1, len(s)):
+            w = s[j - k + 1:j + 1]
+            if all(int(c) < 4 for c in w):
+                out[j] = tuple(conv[int(c)] for c in w)
+        return out
+
+    last = {}
+    for i, km in sorted(kmers(query).items()):
+        last[km] = i
+    votes = {}
+    for j, km in kmers(target).items():
+        i = last.get(km)
+        if i is not None and j >= i:
+            votes[j - i] = votes.get(j - i, 0) + 1
+    return votes, len(last)

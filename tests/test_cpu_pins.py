@@ -310,3 +310,31 @@ def test_ll_u8_saturation():
     assert r == {"score": 255, "te": 37 + 250, "qe": -1, "score2": -1, "te2": -1}
     r = O.orc_ll(qs, ts, mat, 2, 6, 1, O.KSW_LL_SUBO | 19)
     assert (r["score"], r["te"], r["qe"]) == (300, 37 + 299, 299)
+
+
+@pytest.mark.parametrize("mt", [0, 1, 2])
+def test_ungap_bruteforce(mt):
+    """mb_ungap (pe.c:209-247) against its definition: top vote, its first diagonal, the diagonals with
+    more than half of it, and the distinct query 7-mers."""
+    rng = np.random.default_rng(71 + mt)
+    for it in range(300):
+        ql = int(rng.integers(1, 120))
+        q = rng.integers(0, 4, ql).astype(np.uint8)
+        if it % 5 == 0:
+            q = rng.integers(0, 2, ql).astype(np.uint8)          # repetitive: k-mers recur
+        if rng.random() < 0.3:
+            q[rng.random(ql) < 0.05] = 4
+        tl = int(rng.integers(0, 300))
+        t = rng.integers(0, 4, tl).astype(np.uint8)
+        for _ in range(int(rng.integers(0, 3))):
+            if tl > 10:
+                st = int(rng.integers(0, tl - 5))
+                n = min(ql, tl - st)
+                t[st:st + n] = q[:n]
+        votes, n_kmer = brute.ungap_votes(q, t, mt)
+        best = max(votes.values(), default=0)
+        first = min((d for d, v in votes.items() if v == best), default=-1) if best > 0 else -1
+        n_good = sum(1 for v in votes.values() if v > best >> 1)
+        if best == 0:
+            n_good = 0
+        assert O.orc_ungap(q, t, mt) == (best, first, n_good, n_kmer), it

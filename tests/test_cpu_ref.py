@@ -172,3 +172,65 @@ def test_ll_fuzz_vs_reference(size):
         n_sat += a["score"] == 255
         n_sub += a["score2"] >= 0
     assert n_sub > 100 and (size == 2 or n_sat > 20)
+
+
+def rescue_case(rng, qmin=20, qmax=300):
+    """A mate-rescue candidate: a read and a window longer than it holding a mutated copy (with indels)
+    of the read most of the time."""
+    ql = int(rng.integers(qmin, qmax))
+    q = rng.integers(0, 4, ql).astype(np.uint8)
+    if rng.random() < 0.2:
+        q[rng.random(ql) < 0.02] = 4
+    wl = ql + int(rng.integers(1, 600))
+    w = rng.integers(0, 4, wl).astype(np.uint8)
+    if rng.random() < 0.8:
+        t = list(q)
+        for _ in range(int(rng.integers(0, 4))):
+            p = int(rng.integers(0, len(t)))
+            if rng.random() < 0.5:
+                del t[p:p + int(rng.integers(1, 6))]
+            else:
+                t[p:p] = list(rng.integers(0, 4, int(rng.integers(1, 6))))
+        t = np.array(t, np.uint8)
+        sub = rng.random(len(t)) < rng.choice([0.0, 0.01, 0.05, 0.15])
+        t[sub] = rng.integers(0, 4, int(sub.sum()))
+        st = int(rng.integers(0, max(1, wl - len(t) // 2)))
+        n = min(len(t), wl - st)
+        w[st:st + n] = t[:n]
+    if rng.random() < 0.1:
+        w[rng.random(wl) < 0.02] = 4
+    return q, w
+
+
+RESCUE_PARAMS = [O.rescue_params(), O.rescue_params(a=1, b=4, q=6, e=1, bw=50, zdrop=60, end_bonus=0, min_dp_max=20),
+                 O.rescue_params(b_ts=4, min_dp_max=40, bw=30), O.rescue_params(a=2, b=6, q=8, e=3, zdrop=400, end_bonus=10,
+                                                                          min_len=25)]
+
+
+@pytest.mark.parametrize("mt", [0, 1, 2])
+def test_rescue_vs_reference(mt):
+    """The oracle's rescue candidate against the reference's own mb_ungap and mb_matesw_align
+    (ref_pe_shim.c): the vote, the extension DP's ksw_extz_t and CIGAR, and the rescued hit after the
+    reference's mb_append_cigar + mb_update_extra applied to the oracle's pre-update fields."""
+    rng = np.random.default_rng(81 + mt)
+    n_res = n_narrow = 0
+    for it in range(400):
+        q, w = rescue_case(rng)
+        p = RESCUE_PARAMS[it % len(RESCUE_PARAMS)]
+        min_sc = int(rng.choice([0, 20, 40]))
+        d = O.orc_matesw_cand(p, q, w, min_sc, mt)
+        assert d["ug"] == O.ref_ungap(q, w, mt), it
+        if not d["aligned"]:
+            continue
+        ts2, te2 = d["win2"]
+        n_narrow += (ts2, te2) != (0, len(w))
+        h, ez, ezc = O.ref_matesw_align(p, q, w[ts2:te2], min_sc, mt)
+        if d["ran_dp"]:
+            assert d["ez"] == ez and d["cigar"] == ezc, it
+        assert h["rescued"] == d["rescued"], it
+        if d["rescued"]:
+            n_res += 1
+            f = O.ref_finish(p, q, w[ts2:te2], mt, d["qs"], d["qe"], d["ts"], d["te"], d["ez"]["max"], d["ll"]["score"],
+                             d["ll"]["score2"], d["dp_max2"], d["cigar"])
+            assert f == h, it
+    assert n_res > 100 and n_narrow > 50
