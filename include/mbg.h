@@ -158,6 +158,47 @@ int mbg_ll_batch(mbg_ctx_t *ctx, int32_t n_jobs, const mbg_ll_job_t *jobs, const
                  const uint8_t *tseq, int64_t tbytes, int8_t m, const int8_t *mat, int32_t gapo, int32_t gape, int32_t xtra,
                  mbg_ll_result_t *res);
 
+/* One mate-rescue candidate of mb_matesw_core (pe.c:319-378): the mate's bases as they would align
+ * (reverse-complemented for a reverse-strand candidate, pe.c:345) at qseq[qoff, qoff+qlen), and
+ * the window [ts, te) on contig tid where the insert-size model puts it (pe.c:338-339; clipped to
+ * the contig here as pe.c:340-341 does). */
+typedef struct {
+	int64_t qoff;
+	int32_t qlen, tid;
+	int64_t ts, te;
+	int32_t min_sc, pad;          /* mb_matesw's min_sc, already divided by a */
+	int64_t cigar_off;            /* CIGAR slots at cigar + cigar_off, capacity qlen + te - ts */
+} mbg_matesw_job_t;
+
+/* The mb_opt_t fields mate rescue reads (pe.c:249-312); mt is the methylation conversion of the
+ * batch (0 none, 1 C->T, 2 G->A) after l2b_meth_rev for reverse-strand candidates. */
+typedef struct {
+	int32_t a, b, b_ts, b_ambi, q, e, bw, zdrop, end_bonus, min_dp_max, min_len, mt;
+} mbg_matesw_opt_t;
+
+/* Per candidate: the vote, the window mb_matesw_align saw, and its results before mb_update_extra. */
+typedef struct {
+	int32_t max_ug, max_i, n_good, n_kmer;   /* mb_ungap over the window (pe.c:209-247) */
+	int32_t ts2, te2;                        /* aligned window, relative to the clipped ts (pe.c:343-348) */
+	int32_t aligned;                         /* the vote passed: mb_matesw_align ran (pe.c:349) */
+	int32_t rescued;                         /* a hit was made (pe.c:293) */
+	mbg_ll_result_t ll;                      /* its ksw_ll; all -1 when it did not run */
+	mbg_dp_result_t ez;                      /* its ksw_extz2 on the reversed prefixes; zero when not run */
+	int32_t qs, qe, ts, te;                  /* the hit, relative to the aligned window (pe.c:295-296) */
+	int32_t dp_max2;                         /* pe.c:298-299 */
+	int32_t ran_dp;                          /* the extension DP ran (pe.c:273) */
+} mbg_matesw_result_t;
+
+/* Batched mate rescue on the resident reference: per candidate, l2b_getseq of the window, mb_ungap,
+ * the narrowing to a clear diagonal and mb_matesw_align's ksw_ll + ksw_extz2 (pe.c:342-349,
+ * 249-300), results equal to the reference's.  The host keeps the rest of mb_matesw_core
+ * (candidate windows, orientation, mb_append_cigar + mb_update_extra on rescued hits).  A window
+ * not longer than the read after clipping is no candidate (aligned 0).  Supported: qlen <= 1024,
+ * windows <= 16384 bases, q/a >= 1 (ksw_ll); else MBG_EUNSUPPORTED.  The extension CIGAR of
+ * candidate i is at cigar + jobs[i].cigar_off (ez.n_cigar entries, as ksw_extz2 leaves it). */
+int mbg_matesw_batch(mbg_ctx_t *ctx, int32_t n_jobs, const mbg_matesw_job_t *jobs, const uint8_t *qseq, int64_t qbytes,
+                     const mbg_matesw_opt_t *opt, mbg_matesw_result_t *res, uint32_t *cigar, int64_t cigar_cap);
+
 /* Target extraction l2b_getseq (l2bit.c:52-77) for n windows [st[i],en[i]) of contig tid[i]
  * (clipped to the contig, N runs restored as 4), written at out + out_off[i].  len[i] receives
  * the clipped length (or -1 for a bad tid). */

@@ -45,6 +45,7 @@ struct mbg_ctx_s {
 	// seed -> locate -> extend step
 	DBuf p_recs, p_rows, p_rowrec, p_first, p_best, p_jobs, p_meta, p_qb, p_tb, p_res, p_cig, p_ctr;
 	DBuf s_slots1, s_slots2, s_cnt, s_off, s_spill, s_tmp, r_cnt, r_off, s_pack, s_qflag;
+	DBuf m_buf[12];         // mate rescue
 	int64_t cap_task = 0, cap_spill = 0;
 	int64_t cap_rec = 0, cap_rows = 0;
 	hipEvent_t ev[8] = {};
@@ -145,6 +146,7 @@ void mbg_destroy(mbg_ctx_t *c)
 	for (DBuf *b : {&c->p_recs, &c->p_rows, &c->p_rowrec, &c->p_first, &c->p_best, &c->p_jobs, &c->p_meta, &c->p_qb, &c->p_tb,
 	                &c->p_res, &c->p_cig, &c->p_ctr, &c->s_slots1, &c->s_slots2, &c->s_cnt, &c->s_off, &c->s_spill, &c->s_tmp,
 	                &c->r_cnt, &c->r_off, &c->s_pack, &c->s_qflag}) dfree(*b);
+	for (auto &b : c->m_buf) dfree(b);
 	for (hipEvent_t &e : c->ev) if (e) hipEventDestroy(e);
 	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
 	if (c->stream) hipStreamDestroy(c->stream);
@@ -715,6 +717,23 @@ int mbg_extz2_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_dp_job_t *jobs, cons
 	return dp_batch(c, false, n_jobs, jobs, qseq, qbytes, tseq, tbytes, m, mat, res, cigar, cigar_cap);
 }
 
+// ksw_gen_nt4_mat (ksw2.h:199-219): 5x5 scores for A,C,G,T,N; mt 1 = C->T, 2 = G->A conversion
+static void gen_nt4_mat(int8_t *mat, int a, int b, int b_ts, int b_ambi, int mt)
+{
+	a = a < 0 ? -a : a, b = b > 0 ? -b : b, b_ambi = b_ambi > 0 ? -b_ambi : b_ambi;
+	for (int i = 0; i < 4; ++i) {
+		for (int j = 0; j < 4; ++j) mat[i * 5 + j] = (int8_t)(i == j ? a : b);
+		mat[i * 5 + 4] = (int8_t)b_ambi;
+	}
+	for (int j = 0; j < 5; ++j) mat[20 + j] = (int8_t)b_ambi;
+	if (b_ts != 0 && b_ts != b) {
+		b_ts = b_ts > 0 ? -b_ts : b_ts;
+		mat[0 * 5 + 2] = mat[1 * 5 + 3] = mat[2 * 5 + 0] = mat[3 * 5 + 1] = (int8_t)b_ts;
+	}
+	if (mt == 1) mat[1 * 5 + 3] = (int8_t)a;
+	else if (mt == 2) mat[2 * 5 + 0] = (int8_t)a;
+}
+
 // batch-wide ksw_ll parameters: the matrix's u8 bias and largest entry as ksw_ll_qinit computes them
 // (ksw2_ll_sse.c:67-74, uint8 arithmetic), the score table, the SUBO/STOP thresholds
 static int ll_params(int8_t m, const int8_t *mat, int32_t gapo, int32_t gape, int32_t xtra, LlArgs &a, int8_t sc[40], int *mat_min)
@@ -793,6 +812,98 @@ int mbg_ll_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_ll_job_t *jobs, const u
 	}
 	HIPCHK(hipMemcpyAsync(res, c->b_out[0].p, (size_t)n_jobs * sizeof(LlRes), hipMemcpyDeviceToHost, c->stream));
 	HIPCHK(hipStreamSynchronize(c->stream));
+	return MBG_OK;
+}
+
+int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, const uint8_t *qseq, int64_t qbytes,
+                     const mbg_matesw_opt_t *opt, mbg_matesw_result_t *res, uint32_t *cigar, int64_t cigar_cap)
+{
+	if (c == nullptr || n < 0 || opt == nullptr || (n > 0 && (!jobs || !qseq || !res || !cigar))) return MBG_EARG;
+	const mbg_matesw_opt_t &o = *opt;
+	if (o.a <= 0 || o.mt < 0 || o.mt > 2) return MBG_EARG;
+	// mb_matesw_align's ksw_ll parameters (pe.c:253-266): scores divided by a, rounded up
+	const int b_mm = (o.b + o.a - 1) / o.a, gapo = (o.q + o.a - 1) / o.a, gape = (o.e + o.a - 1) / o.a;
+	int8_t mat_ll[25], mat_dp[25];
+	gen_nt4_mat(mat_ll, 1, b_mm, (o.b_ts + o.a - 1) / o.a, (o.b_ambi + o.a - 1) / o.a, o.mt);
+	gen_nt4_mat(mat_dp, o.a, o.b, o.b_ts, o.b_ambi, o.mt);
+	LlArgs la{};
+	int8_t sc[40];
+	int mat_min;
+	TRY(ll_params(5, mat_ll, gapo, gape, 0x40000 | o.min_len, la, sc, &mat_min));
+	if (mat_min >= 0 || la.qmax + la.shift > 255) return MBG_EUNSUPPORTED;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_l2b) return MBG_ENOIDX;
+	if (n == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	int32_t max_q = 1, max_w = 1;
+	std::vector<int64_t> tid(n), st(n), en(n), woff(n);
+	int64_t wtot = 0;
+	for (int32_t i = 0; i < n; ++i) {
+		const mbg_matesw_job_t &j = jobs[i];
+		const int64_t ts = j.ts < 0 ? 0 : j.ts, w = j.te > ts ? j.te - ts : 0;   // an empty window is no candidate
+		if (j.qlen < 0 || j.qoff < 0 || j.qoff + j.qlen > qbytes || j.tid < 0 || (uint64_t)j.tid >= c->lb.n_ctg) return MBG_EARG;
+		if (j.cigar_off < 0 || j.cigar_off + j.qlen + w > cigar_cap) return MBG_EARG;
+		if (j.qlen > kLlMaxQlen || w > kMsMaxWindow) return MBG_EUNSUPPORTED;
+		tid[i] = j.tid, st[i] = ts, en[i] = j.te, woff[i] = wtot, wtot += w;
+		max_q = j.qlen > max_q ? j.qlen : max_q;
+		max_w = w > max_w ? (int32_t)w : max_w;
+	}
+	int tab = 128;
+	while (tab < 2 * max_q) tab <<= 1;
+	// device buffers: 0 jobs, 1 reads, 2 windows, 3 tid/st/en/woff/wlen, 4 reversed reads, 5 reversed
+	// windows, 6 ksw_ll jobs, 7 ksw_ll results, 8 extension jobs, 9 extension results, 10 results,
+	// 11 CIGARs
+	DBuf *m = c->m_buf;
+	TRY(dgrow(m[0], (size_t)n * sizeof(MsJob))); TRY(dgrow(m[1], qbytes + 16)); TRY(dgrow(m[2], wtot + 16));
+	TRY(dgrow(m[3], (size_t)n * 40)); TRY(dgrow(m[4], qbytes + 16)); TRY(dgrow(m[5], wtot + 16));
+	TRY(dgrow(m[6], (size_t)n * sizeof(LlJob))); TRY(dgrow(m[7], (size_t)n * sizeof(LlRes)));
+	TRY(dgrow(m[8], (size_t)n * sizeof(DpJob))); TRY(dgrow(m[9], (size_t)n * sizeof(DpRes)));
+	TRY(dgrow(m[10], (size_t)n * sizeof(MsRes))); TRY(dgrow(m[11], (size_t)cigar_cap * 4));
+	int64_t *d_tid = (int64_t*)m[3].p, *d_st = d_tid + n, *d_en = d_st + n, *d_woff = d_en + n, *d_wlen = d_woff + n;
+	hipStream_t s = c->stream;
+	HIPCHK(hipMemcpyAsync(m[0].p, jobs, (size_t)n * sizeof(MsJob), hipMemcpyHostToDevice, s));
+	HIPCHK(hipMemcpyAsync(m[1].p, qseq, qbytes, hipMemcpyHostToDevice, s));
+	HIPCHK(hipMemcpyAsync(d_tid, tid.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+	HIPCHK(hipMemcpyAsync(d_st, st.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+	HIPCHK(hipMemcpyAsync(d_en, en.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+	HIPCHK(hipMemcpyAsync(d_woff, woff.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+	HIPCHK(launch_getseq(c->lb, n, d_tid, d_st, d_en, d_woff, (uint8_t*)m[2].p, d_wlen, s));
+	MsArgs a{};
+	a.jobs = (const MsJob*)m[0].p, a.n = n, a.qseq = (const uint8_t*)m[1].p, a.win = (const uint8_t*)m[2].p;
+	a.woff = d_woff, a.wlen = d_wlen, a.rq = (uint8_t*)m[4].p, a.rt = (uint8_t*)m[5].p;
+	a.ll_jobs = (LlJob*)m[6].p, a.ll_res = (const LlRes*)m[7].p, a.dp_jobs = (DpJob*)m[8].p, a.dp_res = (const DpRes*)m[9].p;
+	a.res = (MsRes*)m[10].p, a.tab_size = tab;
+	a.mt = o.mt, a.a = o.a, a.b_mm = b_mm, a.q = o.q, a.e = o.e, a.bw = o.bw, a.zdrop = o.zdrop, a.end_bonus = o.end_bonus;
+	a.min_dp_max = o.min_dp_max;
+	HIPCHK(launch_ungap(a, max_w, s));
+	// ksw_ll over the (narrowed) windows: per-job spans of the per-row maxima list from a scan
+	size_t tmp_bytes = 0;
+	HIPCHK(scan_ll_rows(nullptr, &tmp_bytes, nullptr, nullptr, n, s));
+	TRY(dgrow(c->s_tmp, tmp_bytes + 16));
+	TRY(dgrow(c->b_in[3], (size_t)n * 8 + 64));
+	TRY(dgrow(c->b_out[1], (size_t)(wtot + 1) * 8));
+	HIPCHK(hipMemcpyAsync((char*)c->b_in[3].p + (size_t)n * 8, sc, sizeof(sc), hipMemcpyHostToDevice, s));
+	HIPCHK(scan_ll_rows(c->s_tmp.p, &tmp_bytes, (const LlJob*)m[6].p, (int64_t*)c->b_in[3].p, n, s));
+	la.jobs = (const LlJob*)m[6].p, la.qseq = (const uint8_t*)m[1].p, la.tseq = (const uint8_t*)m[2].p, la.res = (LlRes*)m[7].p;
+	la.order = nullptr, la.n = n, la.bbuf = (uint64_t*)c->b_out[1].p, la.boff = (const int64_t*)c->b_in[3].p;
+	la.sc = (const int8_t*)c->b_in[3].p + (size_t)n * 8, la.rows_cap = wtot, la.check = true;
+	HIPCHK(launch_ll(la, ll_slots(max_q, 1), s));
+	// the extension DP of the rescued candidates (pe.c:274-287)
+	HIPCHK(launch_rescue_dp(a, s));
+	int cap, qt, tcap;
+	int64_t slab;
+	TRY(dp_plan(max_q, max_w, o.bw, &cap, &slab, &qt, &tcap));
+	TRY(dp_dev_locked(c, s, false, n, (const DpJob*)m[8].p, (const uint8_t*)m[4].p, (const uint8_t*)m[5].p, 5, mat_dp,
+	                  (DpRes*)m[9].p, (uint32_t*)m[11].p, cap, slab, qt, tcap));
+	HIPCHK(launch_rescue_finish(a, s));
+	HIPCHK(hipMemcpyAsync(res, m[10].p, (size_t)n * sizeof(MsRes), hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	std::vector<uint32_t> tmp;
+	for (int32_t i = 0; i < n; ++i) {
+		if (!res[i].ran_dp || res[i].ez.n_cigar <= 0) continue;
+		HIPCHK(hipMemcpy(cigar + jobs[i].cigar_off, (const uint32_t*)m[11].p + jobs[i].cigar_off, (size_t)res[i].ez.n_cigar * 4,
+		                 hipMemcpyDeviceToHost));
+	}
 	return MBG_OK;
 }
 

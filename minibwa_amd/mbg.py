@@ -172,6 +172,7 @@ def lib():
                                           C.c_int32]
         L.mbg_ll_batch.argtypes = [_P, C.c_int32, _P, _P, C.c_int64, _P, C.c_int64, C.c_int8, _P, C.c_int32, C.c_int32,
                                    C.c_int32, _P]
+        L.mbg_matesw_batch.argtypes = [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(MateswOpt), _P, _P, C.c_int64]
         L.mbg_ll_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, C.c_int32, C.c_int32, C.c_int32, _P,
                                        C.c_int32, C.c_int64]
         _LIB = L
@@ -183,13 +184,26 @@ EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mb
             "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
             "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
             "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench", "mbg_idx_attach", "mbg_ll_batch",
-            "mbg_ll_batch_dev")
+            "mbg_ll_batch_dev", "mbg_matesw_batch")
 LL_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"), ("size", "<i4"),
                          ("pad", "<i4")], align=True)
 LL_RES_DTYPE = np.dtype([(f, "<i4") for f in ("score", "te", "qe", "score2", "te2")])
 assert LL_JOB_DTYPE.itemsize == 32 and LL_RES_DTYPE.itemsize == 20
 KSW_LL_STOP = 0x20000
 KSW_LL_SUBO = 0x40000
+MATESW_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("qlen", "<i4"), ("tid", "<i4"), ("ts", "<i8"), ("te", "<i8"),
+                             ("min_sc", "<i4"), ("pad", "<i4"), ("cigar_off", "<i8")])
+MATESW_RES_DTYPE = np.dtype([(f, "<i4") for f in ("max_ug", "max_i", "n_good", "n_kmer", "ts2", "te2", "aligned",
+                                                    "rescued")] +
+                            [("ll", LL_RES_DTYPE), ("ez", DP_RES_DTYPE)] +
+                            [(f, "<i4") for f in ("qs", "qe", "ts", "te", "dp_max2", "ran_dp")])
+assert MATESW_JOB_DTYPE.itemsize == 48 and MATESW_RES_DTYPE.itemsize == 124
+
+
+class MateswOpt(C.Structure):
+    """mbg_matesw_opt_t: the mb_opt_t fields mate rescue reads, and the batch's methylation mode."""
+    _fields_ = [(f, C.c_int32) for f in ("a", "b", "b_ts", "b_ambi", "q", "e", "bw", "zdrop", "end_bonus", "min_dp_max",
+                                         "min_len", "mt")]
 JOB_META_DTYPE = np.dtype([("rid", "<u4"), ("side", "i1"), ("rev", "i1"), ("pad", "<i2"), ("tid", "<i4"), ("tpos", "<i8")],
                           align=True)
 assert JOB_META_DTYPE.itemsize == 24
@@ -464,6 +478,27 @@ class Context:
         check(self.L.mbg_ll_batch(self.h, n, _ptr(jobs), _ptr(qb), len(qbuf), _ptr(tb), len(tbuf), m, _ptr(mat), gapo,
                                   gape, xtra, _ptr(res)), "mbg_ll_batch")
         return res
+
+    def matesw(self, reads, tid, ts, te, min_sc, opt: MateswOpt):
+        """Mate rescue of candidate windows [ts, te) on contigs tid for reads (one per candidate, as they
+        would align) -> (MATESW_RES_DTYPE array, list of extension CIGARs)."""
+        n = len(reads)
+        qbuf, qoff, qlen = pack_seqs(reads)
+        jobs = np.zeros(n, MATESW_JOB_DTYPE)
+        jobs["qoff"], jobs["qlen"], jobs["tid"], jobs["ts"], jobs["te"], jobs["min_sc"] = qoff[:-1], qlen, tid, ts, te, min_sc
+        span = qlen.astype(np.int64) + np.maximum(np.asarray(te, np.int64) - np.maximum(np.asarray(ts, np.int64), 0), 0)
+        coff = np.zeros(n, np.int64)
+        if n > 1:
+            np.cumsum(span[:-1], out=coff[1:])
+        jobs["cigar_off"] = coff
+        cap = int(span.sum()) + 1
+        cig = np.zeros(cap, np.uint32)
+        res = np.zeros(n, MATESW_RES_DTYPE)
+        qb = qbuf if len(qbuf) else np.zeros(1, np.uint8)
+        check(self.L.mbg_matesw_batch(self.h, n, _ptr(jobs), _ptr(qb), len(qbuf), C.byref(opt), _ptr(res), _ptr(cig), cap),
+              "mbg_matesw_batch")
+        cigars = [cig[coff[i]:coff[i] + res["ez"]["n_cigar"][i]].tolist() if res["ran_dp"][i] else [] for i in range(n)]
+        return res, cigars
 
     def ksw_extz2(self, queries, targets, params, mat, m=5) -> DpBatchResult:
         p = dict(params)

@@ -77,6 +77,10 @@ def test_header_layouts_match_python(tmp_path):
         "mbg_idx_info_t": ["seq_len", "primary", "tot_len", "n_ctg", "pre_len", "sa_bit", "device_bytes"],
         "mbg_ll_job_t": ["qoff", "toff", "qlen", "tlen", "size", "pad"],
         "mbg_ll_result_t": ["score", "te", "qe", "score2", "te2"],
+        "mbg_matesw_job_t": ["qoff", "qlen", "tid", "ts", "te", "min_sc", "pad", "cigar_off"],
+        "mbg_matesw_opt_t": ["a", "mt"],
+        "mbg_matesw_result_t": ["max_ug", "max_i", "n_good", "n_kmer", "ts2", "te2", "aligned", "rescued", "ll", "ez", "qs",
+                                "qe", "ts", "te", "dp_max2", "ran_dp"],
     }
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "mbg.h"', "int main(void) {"]
     for s, fields in checks.items():
@@ -107,6 +111,9 @@ def test_header_layouts_match_python(tmp_path):
     np_layout("mbg_job_meta_t", mbg.JOB_META_DTYPE)
     np_layout("mbg_ll_job_t", mbg.LL_JOB_DTYPE)
     np_layout("mbg_ll_result_t", mbg.LL_RES_DTYPE)
+    np_layout("mbg_matesw_job_t", mbg.MATESW_JOB_DTYPE)
+    np_layout("mbg_matesw_result_t", mbg.MATESW_RES_DTYPE)
+    assert got[("mbg_matesw_opt_t", "sizeof")] == C.sizeof(mbg.MateswOpt)
     sr = mbg.SEED_REC_DTYPE
     assert got[("mbg_seed_rec_t", "sizeof")] == sr.itemsize
     assert got[("mbg_seed_rec_t", "rid")] == sr.fields["rid"][1] and got[("mbg_seed_rec_t", "pass")] == sr.fields["pass"][1]
