@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--ll-batch", type=int, default=0, metavar="N",
                     help="only run the mate-rescue local SW batch (ksw_ll, u8 core): N jobs of a 150 bp read against "
                          "a 550 bp window holding a 1%%-diverged copy of it in 80%% of the jobs; reports GCUPS")
+    ap.add_argument("--rescue-batch", type=int, default=0, metavar="N",
+                    help="only run the mate-rescue batch: N candidates of a 150 bp mate against a 550 bp window of a "
+                         "100 Mbp reference, the mate a 1%%-diverged copy in 80%% of them; reports candidates/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -99,6 +102,9 @@ def main():
         return
     if args.ll_batch:
         ll_batch(args)
+        return
+    if args.rescue_batch:
+        rescue_batch(args)
         return
 
     t0 = time.time()
@@ -425,6 +431,75 @@ def ll_batch(args):
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def rescue_batch(args):
+    """Mate-rescue batch: mbg_matesw_batch over N candidates on a resident 100 Mbp reference (l2b only);
+    the value is candidates per second of device time (HIP events around the phases, window extraction
+    to packed CIGARs); the host-inclusive rate (PCIe both ways) is reported beside it."""
+    from minibwa_amd import mbg, synth
+    ctx = mbg.Context(0)
+    t0 = time.time()
+    ref = synth.reference(100_000_000, seed=11)
+    ctg_len, pac = synth.ref_to_l2b_arrays(ref, 1)
+    ctx.l2b_upload(ctg_len, pac)
+    reads, ts, te = synth.rescue_cands(ref, args.rescue_batch, seed=11)
+    n = len(reads)
+    log(f"{n} rescue candidates on a 100 Mbp reference in {time.time() - t0:.1f}s")
+    # short-read preset (options.c:56-67) and defaults: -A2 -B8 -O12 -E2, band 100, z-drop 80, end bonus 10
+    o = dict(a=2, b=8, b_ts=0, b_ambi=1, q=12, e=2, bw=100, zdrop=80, end_bonus=10, min_dp_max=30, min_len=19)
+    opt = mbg.MateswOpt(**o, mt=0)
+    min_sc = np.full(n, 20, np.int32)
+    runs = []
+    for it in range(3):
+        st = mbg.MateswStats()
+        t1 = time.perf_counter()
+        res, _ = ctx.matesw(reads, np.zeros(n, np.int32), ts, te, min_sc, opt, st)
+        runs.append((st.ms_total, time.perf_counter() - t1, st.as_dict()))
+    best = min(runs, key=lambda r: r[0])
+    st = best[2]
+    out = {"metric": "mate-rescue candidates/s (mb_ungap + ksw_ll + ksw_extz2, 150 bp mates x 550 bp windows)",
+           "value": round(n / (best[0] / 1e3)), "unit": "candidates/s", "n": n, "ms_device": round(best[0], 3),
+           "host_inclusive_per_s": round(n / best[1]), "phases": st,
+           "ll_GCUPS": round(st["ll_cells"] / (st["ms_ll"] / 1e3) / 1e9, 1),
+           "rescued_frac": round(float(res["rescued"].mean()), 4)}
+    cpu = rescue_cpu_sample(reads, ts, te, ref, o, min(n, args.cpu_jobs))
+    if cpu:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def rescue_cpu_sample(reads, ts, te, ref, o, n_s):
+    """The reference's rescue step (mb_ungap + mb_matesw_align, oracle/_ref/cpu_rescue) on the first n_s
+    candidates, windows cut on the host."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "cpu_rescue")
+    if not os.path.exists(exe) or n_s <= 0:
+        return None
+    rl = reads.shape[1]
+    wl = int(te[0] - ts[0])
+    win = np.asarray(ref, np.uint8)[ts[:n_s, None] + np.arange(wl)]
+    c = np.zeros(n_s, [("qoff", "<i8"), ("woff", "<i8"), ("qlen", "<i4"), ("wlen", "<i4"), ("min_sc", "<i4"), ("pad", "<i4")])
+    c["qoff"], c["woff"], c["qlen"], c["wlen"], c["min_sc"] = np.arange(n_s) * rl, np.arange(n_s) * wl, rl, wl, 20
+    work = tempfile.mkdtemp(prefix="mbg_rescue_")
+    fn = os.path.join(work, "cands.bin")
+    p = [o[k] for k in ("a", "b", "b_ts", "b_ambi", "q", "e", "bw", "zdrop", "end_bonus", "min_dp_max", "min_len")]
+    with open(fn, "wb") as f:
+        f.write(np.array(p + [0], np.int32).tobytes())
+        f.write(np.int64(n_s).tobytes())
+        f.write(c.tobytes())
+        f.write(np.int64(n_s * rl).tobytes()); f.write(np.ascontiguousarray(reads[:n_s]).tobytes())
+        f.write(np.int64(n_s * wl).tobytes()); f.write(win.tobytes())
+    cores = host_cores()
+    r = subprocess.run([exe, fn, str(cores)], capture_output=True, text=True, timeout=600)
+    shutil.rmtree(work, ignore_errors=True)
+    if r.returncode != 0:
+        log("rescue cpu sample failed:", r.stderr[-300:])
+        return None
+    d = json.loads(r.stdout.strip().split("\n")[-1])
+    return {"value": round(n_s / d["rescue_s"]), "unit": "candidates/s", "cores": cores, "kind": "reference",
+            "sample": f"mb_ungap + mb_matesw_align on the first {n_s} candidates, {cores} threads, {d['rescue_s']:.2f}s, "
+                      f"{d['rescued']} rescued"}
 
 
 def ll_cpu_sample(jobs, qb, tb, mat, gapo, gape, xtra, n_s):

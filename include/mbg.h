@@ -195,9 +195,17 @@ typedef struct {
  * (candidate windows, orientation, mb_append_cigar + mb_update_extra on rescued hits).  A window
  * not longer than the read after clipping is no candidate (aligned 0).  Supported: qlen <= 1024,
  * windows <= 16384 bases, q/a >= 1 (ksw_ll); else MBG_EUNSUPPORTED.  The extension CIGAR of
- * candidate i is at cigar + jobs[i].cigar_off (ez.n_cigar entries, as ksw_extz2 leaves it). */
+ * candidate i is at cigar + jobs[i].cigar_off (ez.n_cigar entries, as ksw_extz2 leaves it).  stats
+ * (NULL: not wanted) receives counts and the device time of each phase. */
+typedef struct {
+	int64_t n_aligned, n_rescued;
+	int64_t ll_cells;             /* ksw_ll: read length x window rows scanned */
+	int64_t dp_cells;             /* extension DP: band cells */
+	float ms_window, ms_ungap, ms_ll, ms_dp, ms_finish, ms_total;   /* device time per phase */
+} mbg_matesw_stats_t;
 int mbg_matesw_batch(mbg_ctx_t *ctx, int32_t n_jobs, const mbg_matesw_job_t *jobs, const uint8_t *qseq, int64_t qbytes,
-                     const mbg_matesw_opt_t *opt, mbg_matesw_result_t *res, uint32_t *cigar, int64_t cigar_cap);
+                     const mbg_matesw_opt_t *opt, mbg_matesw_result_t *res, uint32_t *cigar, int64_t cigar_cap,
+                     mbg_matesw_stats_t *stats);
 
 /* Target extraction l2b_getseq (l2bit.c:52-77) for n windows [st[i],en[i]) of contig tid[i]
  * (clipped to the contig, N runs restored as 4), written at out + out_off[i].  len[i] receives

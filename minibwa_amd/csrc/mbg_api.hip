@@ -816,7 +816,8 @@ int mbg_ll_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_ll_job_t *jobs, const u
 }
 
 int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, const uint8_t *qseq, int64_t qbytes,
-                     const mbg_matesw_opt_t *opt, mbg_matesw_result_t *res, uint32_t *cigar, int64_t cigar_cap)
+                     const mbg_matesw_opt_t *opt, mbg_matesw_result_t *res, uint32_t *cigar, int64_t cigar_cap,
+                     mbg_matesw_stats_t *stats)
 {
 	if (c == nullptr || n < 0 || opt == nullptr || (n > 0 && (!jobs || !qseq || !res || !cigar))) return MBG_EARG;
 	const mbg_matesw_opt_t &o = *opt;
@@ -867,7 +868,10 @@ int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, cons
 	HIPCHK(hipMemcpyAsync(d_st, st.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
 	HIPCHK(hipMemcpyAsync(d_en, en.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
 	HIPCHK(hipMemcpyAsync(d_woff, woff.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+	for (hipEvent_t &e : c->ev) if (!e) HIPCHK(hipEventCreate(&e));
+	HIPCHK(hipEventRecord(c->ev[0], s));
 	HIPCHK(launch_getseq(c->lb, n, d_tid, d_st, d_en, d_woff, (uint8_t*)m[2].p, d_wlen, s));
+	HIPCHK(hipEventRecord(c->ev[1], s));
 	MsArgs a{};
 	a.jobs = (const MsJob*)m[0].p, a.n = n, a.qseq = (const uint8_t*)m[1].p, a.win = (const uint8_t*)m[2].p;
 	a.woff = d_woff, a.wlen = d_wlen, a.rq = (uint8_t*)m[4].p, a.rt = (uint8_t*)m[5].p;
@@ -875,7 +879,10 @@ int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, cons
 	a.res = (MsRes*)m[10].p, a.tab_size = tab;
 	a.mt = o.mt, a.a = o.a, a.b_mm = b_mm, a.q = o.q, a.e = o.e, a.bw = o.bw, a.zdrop = o.zdrop, a.end_bonus = o.end_bonus;
 	a.min_dp_max = o.min_dp_max;
+	TRY(dgrow(c->r_cnt, (size_t)n * 4 + 16)); TRY(dgrow(c->r_off, (size_t)n * 4 + 16));
+	a.cigar = (const uint32_t*)m[11].p, a.ccnt = (uint32_t*)c->r_cnt.p;
 	HIPCHK(launch_ungap(a, max_w, s));
+	HIPCHK(hipEventRecord(c->ev[2], s));
 	// ksw_ll over the (narrowed) windows: per-job spans of the per-row maxima list from a scan
 	size_t tmp_bytes = 0;
 	HIPCHK(scan_ll_rows(nullptr, &tmp_bytes, nullptr, nullptr, n, s));
@@ -888,6 +895,7 @@ int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, cons
 	la.order = nullptr, la.n = n, la.bbuf = (uint64_t*)c->b_out[1].p, la.boff = (const int64_t*)c->b_in[3].p;
 	la.sc = (const int8_t*)c->b_in[3].p + (size_t)n * 8, la.rows_cap = wtot, la.check = true;
 	HIPCHK(launch_ll(la, ll_slots(max_q, 1), s));
+	HIPCHK(hipEventRecord(c->ev[3], s));
 	// the extension DP of the rescued candidates (pe.c:274-287)
 	HIPCHK(launch_rescue_dp(a, s));
 	int cap, qt, tcap;
@@ -895,14 +903,39 @@ int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, cons
 	TRY(dp_plan(max_q, max_w, o.bw, &cap, &slab, &qt, &tcap));
 	TRY(dp_dev_locked(c, s, false, n, (const DpJob*)m[8].p, (const uint8_t*)m[4].p, (const uint8_t*)m[5].p, 5, mat_dp,
 	                  (DpRes*)m[9].p, (uint32_t*)m[11].p, cap, slab, qt, tcap));
+	HIPCHK(hipEventRecord(c->ev[4], s));
 	HIPCHK(launch_rescue_finish(a, s));
+	// CIGARs packed at the prefix sums of their lengths, so that only their operations cross PCIe
+	size_t scan_bytes = 0;
+	HIPCHK(scan_u32(nullptr, &scan_bytes, nullptr, nullptr, n, s));
+	TRY(dgrow(c->s_tmp, scan_bytes + 16));
+	HIPCHK(scan_u32(c->s_tmp.p, &scan_bytes, a.ccnt, (uint32_t*)c->r_off.p, n, s));
+	TRY(dgrow(c->b_out[2], (size_t)cigar_cap * 4));
+	HIPCHK(launch_rescue_cigar(a, (const uint32_t*)c->r_off.p, (uint32_t*)c->b_out[2].p, s));
+	HIPCHK(hipEventRecord(c->ev[5], s));
 	HIPCHK(hipMemcpyAsync(res, m[10].p, (size_t)n * sizeof(MsRes), hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
-	std::vector<uint32_t> tmp;
+	int64_t n_ops = 0;
+	for (int32_t i = 0; i < n; ++i) n_ops += res[i].ran_dp && res[i].ez.n_cigar > 0 ? res[i].ez.n_cigar : 0;
+	std::vector<uint32_t> packed((size_t)n_ops + 1);
+	if (n_ops > 0) HIPCHK(hipMemcpy(packed.data(), c->b_out[2].p, (size_t)n_ops * 4, hipMemcpyDeviceToHost));
+	int64_t at = 0;
 	for (int32_t i = 0; i < n; ++i) {
 		if (!res[i].ran_dp || res[i].ez.n_cigar <= 0) continue;
-		HIPCHK(hipMemcpy(cigar + jobs[i].cigar_off, (const uint32_t*)m[11].p + jobs[i].cigar_off, (size_t)res[i].ez.n_cigar * 4,
-		                 hipMemcpyDeviceToHost));
+		memcpy(cigar + jobs[i].cigar_off, packed.data() + at, (size_t)res[i].ez.n_cigar * 4);
+		at += res[i].ez.n_cigar;
+	}
+	if (stats) {
+		memset(stats, 0, sizeof(*stats));
+		for (int32_t i = 0; i < n; ++i) {
+			const mbg_matesw_result_t &r = res[i];
+			stats->n_aligned += r.aligned, stats->n_rescued += r.rescued;
+			if (r.ll.score >= 0) stats->ll_cells += (int64_t)jobs[i].qlen * (r.te2 - r.ts2);
+			if (r.ran_dp) stats->dp_cells += r.ez.cells;
+		}
+		float *ms[] = {&stats->ms_window, &stats->ms_ungap, &stats->ms_ll, &stats->ms_dp, &stats->ms_finish};
+		for (int k = 0; k < 5; ++k) HIPCHK(hipEventElapsedTime(ms[k], c->ev[k], c->ev[k + 1]));
+		HIPCHK(hipEventElapsedTime(&stats->ms_total, c->ev[0], c->ev[5]));
 	}
 	return MBG_OK;
 }

@@ -140,6 +140,8 @@ struct MsArgs {
 	DpJob *dp_jobs;
 	const DpRes *dp_res;
 	MsRes *res;
+	const uint32_t *cigar;  // the extension DP's CIGAR slots (at each job's cigar_off)
+	uint32_t *ccnt;         // per candidate: CIGAR operations to bring back
 	int32_t tab_size;       // k-mer table slots of the vote (a power of two >= 2 * the longest read)
 	int32_t mt, a, b_mm, q, e, bw, zdrop, end_bonus, min_dp_max;
 };
@@ -147,6 +149,7 @@ constexpr int kMsMaxWindow = 16384;    // rescue window bases: 16-bit vote count
 hipError_t launch_ungap(const MsArgs &a, int max_wlen, hipStream_t st);
 hipError_t launch_rescue_dp(const MsArgs &a, hipStream_t st);
 hipError_t launch_rescue_finish(const MsArgs &a, hipStream_t st);
+hipError_t launch_rescue_cigar(const MsArgs &a, const uint32_t *coff, uint32_t *packed, hipStream_t st);
 
 hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st);
 hipError_t launch_sa_fine(const Idx &ix, uint64_t *fine, uint64_t n_fine, hipStream_t st);

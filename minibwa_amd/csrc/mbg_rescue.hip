@@ -151,7 +151,8 @@ __global__ __launch_bounds__(64) void rescue_dp_kernel(MsArgs a)
 	}
 }
 
-// one thread per candidate: the hit fields before mb_update_extra (pe.c:288-300)
+// one thread per candidate: the hit fields before mb_update_extra (pe.c:288-300), and the number of
+// CIGAR operations to bring back
 __global__ void rescue_finish_kernel(MsArgs a)
 {
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -161,6 +162,7 @@ __global__ void rescue_finish_kernel(MsArgs a)
 	const LlRes ll = lj.qlen >= 0 ? a.ll_res[c] : LlRes{-1, -1, -1, -1, -1};
 	const DpRes ez = o.ran_dp ? a.dp_res[c] : DpRes{};
 	o.ll = ll, o.ez = ez;
+	a.ccnt[c] = o.ran_dp && ez.n_cigar > 0 ? (uint32_t)ez.n_cigar : 0u;
 	o.rescued = o.ran_dp && ez.n_cigar > 0 && ez.max >= a.min_dp_max * a.a;
 	o.qs = o.qe = o.ts = o.te = o.dp_max2 = 0;
 	if (o.rescued) {
@@ -171,6 +173,23 @@ __global__ void rescue_finish_kernel(MsArgs a)
 		const int m2 = (int)((double)ez.max / ll.score * ll.score2 + .499);
 		o.dp_max2 = m2 < 0 ? 0 : m2;
 	}
+}
+
+// one thread per candidate: its CIGAR moved from the DP's slots to a packed list at ccnt's prefix sum
+__global__ void rescue_cigar_kernel(MsArgs a, const uint32_t *coff, uint32_t *packed)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= a.n) return;
+	const uint32_t n = a.ccnt[c];
+	const uint32_t *src = a.cigar + a.jobs[c].cigar_off;
+	for (uint32_t i = 0; i < n; ++i) packed[coff[c] + i] = src[i];
+}
+
+hipError_t launch_rescue_cigar(const MsArgs &a, const uint32_t *coff, uint32_t *packed, hipStream_t st)
+{
+	if (a.n <= 0) return hipSuccess;
+	hipLaunchKernelGGL(rescue_cigar_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a, coff, packed);
+	return hipGetLastError();
 }
 
 hipError_t launch_ungap(const MsArgs &a, int max_wlen, hipStream_t st)

@@ -1,17 +1,21 @@
 /*
- * mbg_wrap.c -- binds the reference's four hot-path hooks to libmbg without touching its sources.
+ * mbg_wrap.c -- binds the reference's hot-path hooks to libmbg without touching its sources.
  *
  * Linked into the unmodified minibwa objects with
  *   -Wl,--wrap=mb_seed_intv_batch -Wl,--wrap=mb_bwt_sa_batch -Wl,--wrap=ksw_extd2_sse -Wl,--wrap=ksw_extz2_sse
+ *   -Wl,--wrap=ksw_ll_qinit -Wl,--wrap=ksw_ll_u8_core -Wl,--wrap=ksw_ll_i16_core
  * every call site of those functions (map-main.c:75, map-algo.c:733, seed.c:180, align.c:374-376,
- * pe.c:287) reaches the __wrap_ definitions below, which run the work on the GPU through the C ABI
- * of include/mbg.h and hand results back in the reference's own containers:
+ * pe.c:267-272,287) reaches the __wrap_ definitions below, which run the work on the GPU through the
+ * C ABI of include/mbg.h and hand results back in the reference's own containers:
  *
  *   mb_seed_intv_batch (mbpriv.h:57, seed.c:56-103)   -> mbg_seed_batch
  *   mb_bwt_sa_batch    (bwt.h:73, bwt.c:511-541)       -> mbg_sa_batch
  *   ksw_extd2_sse      (ksw2.h:79-80)                  -> mbg_extd2_batch (one job)
  *   ksw_extz2_sse      (ksw2.h:73-74)                  -> mbg_extz2_batch (one job)
+ *   ksw_ll_qinit + ksw_ll_u8_core / ksw_ll_i16_core (ksw2.h:101-103)
+ *                                                      -> mbg_ll_batch (one job)
  *
+ * MBG_WRAP_STATS=1 prints the number of calls each hook took, at exit, to stderr.
  * The index is uploaded once, on the first hooked call, from the mb_bwt_t the caller passes (the
  * k-mer cache is recomputed on the device with the caller's pre_len).  MBG_DEVICE selects the HIP
  * device (default 0).  Any library error aborts, as the reference does on internal failures.
@@ -35,6 +39,20 @@
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 static mbg_ctx_t *g_ctx;
 static const mb_bwt_t *g_bwt;
+static long g_calls[5];                    /* seed, sa, extd2, extz2, ll */
+
+static void report(void)
+{
+	fprintf(stderr, "[mbg_wrap] calls: seed %ld sa %ld extd2 %ld extz2 %ld ll %ld\n", g_calls[0], g_calls[1], g_calls[2],
+	        g_calls[3], g_calls[4]);
+}
+
+static void count(int k)
+{
+	pthread_mutex_lock(&g_lock);
+	++g_calls[k];
+	pthread_mutex_unlock(&g_lock);
+}
 
 static void die(const char *what, int rc)
 {
@@ -46,9 +64,10 @@ static mbg_ctx_t *ctx_for(const mb_bwt_t *bwt)
 {
 	pthread_mutex_lock(&g_lock);
 	if (g_ctx == 0) {
-		const char *d = getenv("MBG_DEVICE");
+		const char *d = getenv("MBG_DEVICE"), *st = getenv("MBG_WRAP_STATS");
 		int rc = mbg_init(d ? atoi(d) : 0, &g_ctx);
 		if (rc != MBG_OK) die("mbg_init", rc);
+		if (st && atoi(st)) atexit(report);
 	}
 	if (bwt && bwt != g_bwt) {
 		mbg_bwt_desc_t dsc;
@@ -77,6 +96,7 @@ void __wrap_mb_seed_intv_batch(void *km, const mb_bwt_t *bwt, int32_t n_seq, con
 	int32_t i;
 	int rc;
 	if (n_seq <= 0) return;
+	count(0);
 	off = (int64_t*)malloc(sizeof(int64_t) * (n_seq + 1));
 	rc = mbg_seed_batch(c, n_seq, len, (const uint8_t *const*)seq, min_len, max_sub_occ, &out, off);
 	if (rc != MBG_OK) die("mbg_seed_batch", rc);
@@ -102,6 +122,7 @@ void __wrap_mb_bwt_sa_batch(void *km, const mb_bwt_t *bwt, int64_t n, uint64_t *
 	int rc;
 	(void)km;
 	if (n <= 0) return;
+	count(1);
 	rc = mbg_sa_batch(ctx_for(bwt), n, x);
 	if (rc != MBG_OK) die("mbg_sa_batch", rc);
 }
@@ -116,6 +137,7 @@ static void dp_one(int dual, void *km, int qlen, const uint8_t *query, int tlen,
 	uint32_t *cig;
 	int64_t cap;
 	int rc;
+	count(dual ? 2 : 3);
 	memset(&j, 0, sizeof(j));
 	j.qlen = qlen > 0 ? qlen : 0;      /* the reference returns the reset state for qlen/tlen <= 0 */
 	j.tlen = tlen > 0 ? tlen : 0;
@@ -152,4 +174,47 @@ void __wrap_ksw_extz2_sse(void *km, int qlen, const uint8_t *query, int tlen, co
                           ksw_extz_t *ez)
 {
 	dp_one(0, km, qlen, query, tlen, target, m, mat, q, e, q, e, w, zdrop, end_bonus, flag, ez);
+}
+
+/* ksw_ll_qinit's query profile is kept as the query and matrix themselves; the core call runs the
+ * job.  The record comes from the caller's km, so the caller's kfree (pe.c:281) releases it. */
+typedef struct { int32_t qlen, m; int8_t mat[25]; uint8_t q[]; } ll_query_t;
+
+void *__wrap_ksw_ll_qinit(void *km, int size, int qlen, const uint8_t *query, int m, const int8_t *mat)
+{
+	ll_query_t *q = (ll_query_t*)kmalloc(km, sizeof(ll_query_t) + (qlen > 0 ? qlen : 0) + 1);
+	(void)size;                        /* the core the caller picks names the size */
+	if (m < 1 || m > 5) die("ksw_ll_qinit (alphabet size)", MBG_EUNSUPPORTED);
+	q->qlen = qlen > 0 ? qlen : 0, q->m = m;
+	memcpy(q->mat, mat, (size_t)m * m);
+	if (q->qlen) memcpy(q->q, query, q->qlen);
+	return q;
+}
+
+static ksw_llrst_t ll_one(void *q_, int tlen, const uint8_t *target, int gapo, int gape, int xtra, int size)
+{
+	const ll_query_t *q = (const ll_query_t*)q_;
+	mbg_ll_job_t j;
+	mbg_ll_result_t r;
+	ksw_llrst_t o;
+	uint8_t dummy = 0;
+	int rc;
+	count(4);
+	memset(&j, 0, sizeof(j));
+	j.qlen = q->qlen, j.tlen = tlen > 0 ? tlen : 0, j.size = size;
+	rc = mbg_ll_batch(ctx_for(0), 1, &j, q->qlen ? q->q : &dummy, q->qlen, j.tlen ? target : &dummy, j.tlen, (int8_t)q->m,
+	                  q->mat, gapo, gape, xtra, &r);
+	if (rc != MBG_OK) die("mbg_ll_batch", rc);
+	o.score = r.score, o.te = r.te, o.qe = r.qe, o.score2 = r.score2, o.te2 = r.te2;
+	return o;
+}
+
+ksw_llrst_t __wrap_ksw_ll_u8_core(void *q, int tlen, const uint8_t *target, int gapo, int gape, int xtra)
+{
+	return ll_one(q, tlen, target, gapo, gape, xtra, 1);
+}
+
+ksw_llrst_t __wrap_ksw_ll_i16_core(void *q, int tlen, const uint8_t *target, int gapo, int gape, int xtra)
+{
+	return ll_one(q, tlen, target, gapo, gape, xtra, 2);
 }

@@ -172,7 +172,8 @@ def lib():
                                           C.c_int32]
         L.mbg_ll_batch.argtypes = [_P, C.c_int32, _P, _P, C.c_int64, _P, C.c_int64, C.c_int8, _P, C.c_int32, C.c_int32,
                                    C.c_int32, _P]
-        L.mbg_matesw_batch.argtypes = [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(MateswOpt), _P, _P, C.c_int64]
+        L.mbg_matesw_batch.argtypes = [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(MateswOpt), _P, _P, C.c_int64,
+                                       C.POINTER(MateswStats)]
         L.mbg_ll_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, C.c_int32, C.c_int32, C.c_int32, _P,
                                        C.c_int32, C.c_int64]
         _LIB = L
@@ -200,6 +201,14 @@ MATESW_RES_DTYPE = np.dtype([(f, "<i4") for f in ("max_ug", "max_i", "n_good", "
 assert MATESW_JOB_DTYPE.itemsize == 48 and MATESW_RES_DTYPE.itemsize == 124
 
 
+class MateswStats(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in ("n_aligned", "n_rescued", "ll_cells", "dp_cells")] + \
+               [(f, C.c_float) for f in ("ms_window", "ms_ungap", "ms_ll", "ms_dp", "ms_finish", "ms_total")]
+
+    def as_dict(self):
+        return {f: (round(getattr(self, f), 3) if f.startswith("ms") else int(getattr(self, f))) for f, _ in self._fields_}
+
+
 class MateswOpt(C.Structure):
     """mbg_matesw_opt_t: the mb_opt_t fields mate rescue reads, and the batch's methylation mode."""
     _fields_ = [(f, C.c_int32) for f in ("a", "b", "b_ts", "b_ambi", "q", "e", "bw", "zdrop", "end_bonus", "min_dp_max",
@@ -223,7 +232,12 @@ def check(rc: int, what: str = ""):
 
 
 def pack_seqs(seqs):
-    """Concatenate nt4 reads -> (bytes, int64 offsets[n+1], int32 lengths[n])."""
+    """Concatenate nt4 reads (a list, or a 2-D array of equal-length ones) -> (bytes, int64 offsets[n+1],
+    int32 lengths[n])."""
+    if isinstance(seqs, np.ndarray) and seqs.ndim == 2:
+        n, rl = seqs.shape
+        return (np.ascontiguousarray(seqs, np.uint8).reshape(-1), np.arange(n + 1, dtype=np.int64) * rl,
+                np.full(n, rl, np.int32))
     lens = np.array([len(s) for s in seqs], np.int32)
     off = np.zeros(len(seqs) + 1, np.int64)
     np.cumsum(lens, out=off[1:])
@@ -479,7 +493,7 @@ class Context:
                                   gape, xtra, _ptr(res)), "mbg_ll_batch")
         return res
 
-    def matesw(self, reads, tid, ts, te, min_sc, opt: MateswOpt):
+    def matesw(self, reads, tid, ts, te, min_sc, opt: MateswOpt, stats: "MateswStats | None" = None):
         """Mate rescue of candidate windows [ts, te) on contigs tid for reads (one per candidate, as they
         would align) -> (MATESW_RES_DTYPE array, list of extension CIGARs)."""
         n = len(reads)
@@ -495,8 +509,8 @@ class Context:
         cig = np.zeros(cap, np.uint32)
         res = np.zeros(n, MATESW_RES_DTYPE)
         qb = qbuf if len(qbuf) else np.zeros(1, np.uint8)
-        check(self.L.mbg_matesw_batch(self.h, n, _ptr(jobs), _ptr(qb), len(qbuf), C.byref(opt), _ptr(res), _ptr(cig), cap),
-              "mbg_matesw_batch")
+        check(self.L.mbg_matesw_batch(self.h, n, _ptr(jobs), _ptr(qb), len(qbuf), C.byref(opt), _ptr(res), _ptr(cig), cap,
+                                      C.byref(stats) if stats is not None else None), "mbg_matesw_batch")
         cigars = [cig[coff[i]:coff[i] + res["ez"]["n_cigar"][i]].tolist() if res["ran_dp"][i] else [] for i in range(n)]
         return res, cigars
 

@@ -148,3 +148,19 @@ def ll_jobs(n, seed=11, read_len=150, window=550, sub=0.01, frac_in=0.8):
     qoff = np.arange(n + 1, dtype=np.int64) * read_len
     toff = np.arange(n + 1, dtype=np.int64) * window
     return (q.reshape(-1), qoff, np.full(n, read_len, np.int32), t.reshape(-1), toff, np.full(n, window, np.int32))
+
+
+def rescue_cands(ref, n, seed=11, read_len=150, window=550, sub=0.01, frac_in=0.8):
+    """Mate-rescue candidates on one contig: a read_len-base mate (a copy of the reference with `sub`
+    substitutions in a fraction frac_in of them, unrelated sequence otherwise) and a window of
+    `window` bases around where the insert-size model puts it -> (reads [n, read_len], ts, te)."""
+    rng = np.random.default_rng(seed)
+    ref = np.asarray(ref, np.uint8)
+    p = rng.integers(window, len(ref) - 2 * window, n)
+    reads = ref[p[:, None] + np.arange(read_len)]
+    mut = rng.random(reads.shape) < sub
+    reads[mut] = (reads[mut] + rng.integers(1, 4, int(mut.sum()), dtype=np.uint8)) % 4
+    other = rng.random(n) >= frac_in
+    reads[other] = rng.integers(0, 4, (int(other.sum()), read_len), dtype=np.uint8)
+    ts = p - rng.integers(0, window - read_len + 1, n)
+    return reads, ts.astype(np.int64), (ts + window).astype(np.int64)

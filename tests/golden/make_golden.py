@@ -10,7 +10,8 @@ reference's own sources by oracle/Makefile):
   index files, and md5 of the SAM body (minus @PG) of ``minibwa map`` on the chrM read pairs and on
   the config-0 read pairs (10,000 pairs from minibwa_amd.synth with seed 11), and md5 of the
   output of the reference's batch-API example (api-test/ex-batch.c, mb_map_batch) on the chrM
-  read-1 file and on the config-0 read-1 file.
+  read-1 file and on the config-0 read-1 file; the SAM body md5 of a mate-rescue-heavy config-0
+  set (read 2 with 8% substitutions) and its number of ksw_ll calls (``--dbg-aln-pe``).
 
 Usage: python tests/golden/make_golden.py
 """
@@ -71,6 +72,14 @@ def main():
         out["cfg0.sam.n_lines"] = sum(1 for l in sam.split("\n") if l and not l.startswith("@"))
         r = run([REF_BATCH, cfg0, f1]).stdout
         out["cfg0.batch.md5"] = hashlib.md5(r.encode()).hexdigest()
+        # a mate-rescue-heavy set: read 2 of every pair with 8% substitutions, so that hundreds of
+        # mates are found by mb_matesw (pe.c:319-378) rather than by seeding
+        _, r2 = synth.read_pairs(ref, 10000, seed=11, sub=0.08)
+        f2 = os.path.join(td, "r2_rescue.fq")
+        synth.write_pairs_fastq(f1, f2, r1, r2, prefix="c0_")
+        p = run([REF_CLI, "map", "--dbg-aln-pe", cfg0, f1, f2])
+        out["rescue.sam.body.md5"] = sam_body_md5(p.stdout)
+        out["rescue.n_ll_calls"] = p.stderr.count("===>")
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print(json.dumps(out, indent=1, sort_keys=True))

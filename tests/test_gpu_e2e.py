@@ -3,8 +3,9 @@
 objects plus minibwa_amd/integration/mbg_wrap.c) must write the same SAM body as the reference
 binary.  The expected md5s were recorded from the reference binary (tests/golden/make_golden.py).
 
-Seeding (mb_seed_intv_batch), SA locate (mb_bwt_sa_batch) and every banded DP call
-(ksw_extd2_sse / ksw_extz2_sse) run on the GPU; everything else is the reference's own code.
+Seeding (mb_seed_intv_batch), SA locate (mb_bwt_sa_batch), every banded DP call
+(ksw_extd2_sse / ksw_extz2_sse) and every mate-rescue local SW (ksw_ll_qinit + ksw_ll_u8_core /
+ksw_ll_i16_core) run on the GPU; everything else is the reference's own code.
 """
 import hashlib
 import json
@@ -26,11 +27,12 @@ def _sam_body_md5(text):
     return hashlib.md5(body.encode()).hexdigest()
 
 
-def _run(args, timeout=300, exe=HOOKED):
+def _run(args, timeout=300, exe=HOOKED, stats=False):
     print("[e2e]", exe, " ".join(args), flush=True)
-    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=timeout)
+    env = dict(os.environ, MBG_WRAP_STATS="1") if stats else None
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=timeout, env=env)
     assert r.returncode == 0, r.stderr[-2000:]
-    return r.stdout
+    return (r.stdout, r.stderr) if stats else r.stdout
 
 
 @pytest.fixture(scope="module")
@@ -76,3 +78,21 @@ def test_cfg0_pairs_sam_identical(gold, tmp_path):
     n = sum(1 for l in sam.split("\n") if l and not l.startswith("@"))
     assert n == gold["cfg0.sam.n_lines"]
     assert _sam_body_md5(sam) == gold["cfg0.sam.body.md5"]
+
+
+def test_rescue_pairs_sam_identical(gold, tmp_path):
+    """A mate-rescue-heavy set (read 2 of each pair with 8% substitutions): hundreds of mates are
+    found by mb_matesw (pe.c:319-378), whose ksw_ll and extension DP both run on the GPU through the
+    hooks.  The SAM body and the number of ksw_ll calls equal the reference binary's."""
+    from minibwa_amd import synth
+    ref = synth.reference(16569, seed=11, rep_frac=0.0)
+    r1, _ = synth.read_pairs(ref, 10000, seed=11)
+    _, r2 = synth.read_pairs(ref, 10000, seed=11, sub=0.08)
+    f1, f2 = str(tmp_path / "r1.fq"), str(tmp_path / "r2_rescue.fq")
+    synth.write_pairs_fastq(f1, f2, r1, r2, prefix="c0_")
+    sam, err = _run(["map", os.path.join(GOLDEN, "cfg0"), f1, f2], stats=True)
+    calls = [l for l in err.split("\n") if l.startswith("[mbg_wrap] calls:")]
+    assert calls, err[-500:]
+    n_ll = int(calls[-1].split()[-1])
+    assert n_ll == gold["rescue.n_ll_calls"] > 500
+    assert _sam_body_md5(sam) == gold["rescue.sam.body.md5"]
