@@ -207,6 +207,25 @@ int mbg_matesw_batch(mbg_ctx_t *ctx, int32_t n_jobs, const mbg_matesw_job_t *job
                      const mbg_matesw_opt_t *opt, mbg_matesw_result_t *res, uint32_t *cigar, int64_t cigar_cap,
                      mbg_matesw_stats_t *stats);
 
+/* One hit to finish: its CIGAR and the bases it aligns. */
+typedef struct {
+	int64_t qoff, toff;           /* the hit's first query and target bases: qseq + qoff, tseq + toff */
+	int32_t qs, qe;               /* in; out: a dropped leading insertion moves qs (qe when rev) */
+	int64_t ts, te;               /* in; out: a dropped leading deletion moves ts */
+	int32_t rev, n_cigar;         /* mb_hit_t.rev; n_cigar in and out */
+	int64_t cigar_off;            /* the CIGAR at cigar + cigar_off, edited in place */
+	int32_t blen, mlen, n_ambi, dp_max;   /* out: mb_hit_t.blen/mlen, mb_extra_t.n_ambi, dp_max (= dp_max0) */
+} mbg_extra_job_t;
+
+/* Batched mb_fix_cigar (align.c:70-142) + the scores of mb_update_extra (align.c:219-258; decl
+ * mbpriv.h:98): indels left-aligned, adjacent I/D runs merged, zero-length runs dropped, a leading
+ * gap removed, then blen, mlen, the ambiguous-base count and the best prefix score, with gaps at
+ * q+e (log_gap 0) or q+e*mb_log2(1+len) (log_gap 1).  mat is the 5x5 matrix.  The =/X, cs and MD
+ * strings stay with the host.  Results equal the reference's. */
+int mbg_update_extra_batch(mbg_ctx_t *ctx, int64_t n, mbg_extra_job_t *jobs, const uint8_t *qseq, int64_t qbytes,
+                           const uint8_t *tseq, int64_t tbytes, const int8_t *mat, int8_t q, int8_t e, int32_t log_gap,
+                           uint32_t *cigar, int64_t cigar_cap);
+
 /* Target extraction l2b_getseq (l2bit.c:52-77) for n windows [st[i],en[i]) of contig tid[i]
  * (clipped to the contig, N runs restored as 4), written at out + out_off[i].  len[i] receives
  * the clipped length (or -1 for a bad tid). */

@@ -11,11 +11,15 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIP_SRC = ["csrc/mbg_api.hip", "csrc/mbg_fm.hip", "csrc/mbg_dp.hip", "csrc/mbg_pipe.hip", "csrc/mbg_scan.hip",
-           "csrc/mbg_ll.hip", "csrc/mbg_rescue.hip"]
+           "csrc/mbg_ll.hip", "csrc/mbg_rescue.hip",
+           "csrc/mbg_extra.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wno-unused-value",
              # packed fp32 VALU ops stay off in every HIP object of this library
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # the few floating-point results (mate rescue's dp_max2, mb_update_extra's log-gap
+             # scores) must round like the host code they reproduce: no fused multiply-add
+             "-ffp-contract=off"]
 
 
 def _run(cmd, cwd):

@@ -755,6 +755,45 @@ static int ll_params(int8_t m, const int8_t *mat, int32_t gapo, int32_t gape, in
 	return MBG_OK;
 }
 
+int mbg_update_extra_batch(mbg_ctx_t *c, int64_t n, mbg_extra_job_t *jobs, const uint8_t *qseq, int64_t qbytes, const uint8_t *tseq,
+                           int64_t tbytes, const int8_t *mat, int8_t q, int8_t e, int32_t log_gap, uint32_t *cigar, int64_t cigar_cap)
+{
+	if (c == nullptr || n < 0 || (n > 0 && (!jobs || !qseq || !tseq || !mat || !cigar))) return MBG_EARG;
+	int64_t cig_end = 0;
+	for (int64_t i = 0; i < n; ++i) {
+		const mbg_extra_job_t &j = jobs[i];
+		int64_t ql = 0, tl = 0;
+		if (j.n_cigar < 0 || j.cigar_off < 0 || j.cigar_off + j.n_cigar > cigar_cap || j.qoff < 0 || j.toff < 0) return MBG_EARG;
+		for (int32_t k = 0; k < j.n_cigar; ++k) {      // the bases the CIGAR covers must lie in the buffers
+			const uint32_t op = cigar[j.cigar_off + k] & 0xf, len = cigar[j.cigar_off + k] >> 4;
+			if (op > 3) return MBG_EUNSUPPORTED;         // M, I, D and N only: the DP's operations
+			if (op == 0 || op == 1) ql += len;
+			if (op == 0 || op == 2 || op == 3) tl += len;
+		}
+		if (j.qoff + ql > qbytes || j.toff + tl > tbytes) return MBG_EARG;
+		cig_end = j.cigar_off + j.n_cigar > cig_end ? j.cigar_off + j.n_cigar : cig_end;
+	}
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (n == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	TRY(dgrow(c->b_in[0], qbytes + 16)); TRY(dgrow(c->b_in[1], tbytes + 16)); TRY(dgrow(c->b_in[4], (size_t)n * sizeof(ExtraJob)));
+	TRY(dgrow(c->b_out[1], (size_t)cig_end * 4 + 16));
+	HIPCHK(hipMemcpyAsync(c->b_in[0].p, qseq, qbytes, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[1].p, tseq, tbytes, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[4].p, jobs, (size_t)n * sizeof(ExtraJob), hipMemcpyHostToDevice, c->stream));
+	if (cig_end > 0) HIPCHK(hipMemcpyAsync(c->b_out[1].p, cigar, (size_t)cig_end * 4, hipMemcpyHostToDevice, c->stream));
+	ExtraArgs a{};
+	a.jobs = (ExtraJob*)c->b_in[4].p, a.n = n, a.qseq = (const uint8_t*)c->b_in[0].p, a.tseq = (const uint8_t*)c->b_in[1].p;
+	a.cigar = (uint32_t*)c->b_out[1].p;
+	for (int k = 0; k < 28; ++k) a.mat[k] = k < 25 ? mat[k] : 0;
+	a.q = q, a.e = e, a.log_gap = log_gap != 0;
+	HIPCHK(launch_extra(a, c->stream));
+	HIPCHK(hipMemcpyAsync(jobs, c->b_in[4].p, (size_t)n * sizeof(ExtraJob), hipMemcpyDeviceToHost, c->stream));
+	if (cig_end > 0) HIPCHK(hipMemcpyAsync(cigar, c->b_out[1].p, (size_t)cig_end * 4, hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	return MBG_OK;
+}
+
 int mbg_ll_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_ll_job_t *jobs, const uint8_t *qseq, int64_t qbytes, const uint8_t *tseq,
                  int64_t tbytes, int8_t m, const int8_t *mat, int32_t gapo, int32_t gape, int32_t xtra, mbg_ll_result_t *res)
 {

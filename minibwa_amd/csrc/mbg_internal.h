@@ -151,6 +151,26 @@ hipError_t launch_rescue_dp(const MsArgs &a, hipStream_t st);
 hipError_t launch_rescue_finish(const MsArgs &a, hipStream_t st);
 hipError_t launch_rescue_cigar(const MsArgs &a, const uint32_t *coff, uint32_t *packed, hipStream_t st);
 
+// hit finishing (mbg_extra.hip); layout of mbg_extra_job_t
+struct ExtraJob {
+	int64_t qoff, toff;
+	int32_t qs, qe;
+	int64_t ts, te;
+	int32_t rev, n_cigar;
+	int64_t cigar_off;
+	int32_t blen, mlen, n_ambi, dp_max;
+};
+static_assert(sizeof(ExtraJob) == 72, "ExtraJob must match mbg_extra_job_t");
+struct ExtraArgs {
+	ExtraJob *jobs;
+	int64_t n;
+	const uint8_t *qseq, *tseq;
+	uint32_t *cigar;
+	int8_t mat[28];
+	int32_t q, e, log_gap;
+};
+hipError_t launch_extra(const ExtraArgs &a, hipStream_t st);
+
 hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st);
 hipError_t launch_sa_fine(const Idx &ix, uint64_t *fine, uint64_t n_fine, hipStream_t st);
 hipError_t launch_rank2a(const Idx &ix, int64_t n, const uint64_t *k, const uint64_t *l, uint64_t *ck, uint64_t *cl, hipStream_t st);

@@ -155,6 +155,8 @@ __global__ __launch_bounds__(64) void rescue_dp_kernel(MsArgs a)
 // CIGAR operations to bring back
 __global__ void rescue_finish_kernel(MsArgs a)
 {
+	// dp_max2's product and sum rounded one at a time, as the host code computes them
+#pragma clang fp contract(off)
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c >= a.n) return;
 	MsRes &o = a.res[c];

@@ -174,6 +174,8 @@ def lib():
                                    C.c_int32, _P]
         L.mbg_matesw_batch.argtypes = [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(MateswOpt), _P, _P, C.c_int64,
                                        C.POINTER(MateswStats)]
+        L.mbg_update_extra_batch.argtypes = [_P, C.c_int64, _P, _P, C.c_int64, _P, C.c_int64, _P, C.c_int8, C.c_int8,
+                                             C.c_int32, _P, C.c_int64]
         L.mbg_ll_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, C.c_int32, C.c_int32, C.c_int32, _P,
                                        C.c_int32, C.c_int64]
         _LIB = L
@@ -185,7 +187,7 @@ EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mb
             "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
             "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
             "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench", "mbg_idx_attach", "mbg_ll_batch",
-            "mbg_ll_batch_dev", "mbg_matesw_batch")
+            "mbg_ll_batch_dev", "mbg_matesw_batch", "mbg_update_extra_batch")
 LL_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"), ("size", "<i4"),
                          ("pad", "<i4")], align=True)
 LL_RES_DTYPE = np.dtype([(f, "<i4") for f in ("score", "te", "qe", "score2", "te2")])
@@ -199,6 +201,12 @@ MATESW_RES_DTYPE = np.dtype([(f, "<i4") for f in ("max_ug", "max_i", "n_good", "
                             [("ll", LL_RES_DTYPE), ("ez", DP_RES_DTYPE)] +
                             [(f, "<i4") for f in ("qs", "qe", "ts", "te", "dp_max2", "ran_dp")])
 assert MATESW_JOB_DTYPE.itemsize == 48 and MATESW_RES_DTYPE.itemsize == 124
+
+
+EXTRA_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qs", "<i4"), ("qe", "<i4"), ("ts", "<i8"), ("te", "<i8"),
+                            ("rev", "<i4"), ("n_cigar", "<i4"), ("cigar_off", "<i8"), ("blen", "<i4"), ("mlen", "<i4"),
+                            ("n_ambi", "<i4"), ("dp_max", "<i4")])
+assert EXTRA_JOB_DTYPE.itemsize == 72
 
 
 class MateswStats(C.Structure):
@@ -492,6 +500,31 @@ class Context:
         check(self.L.mbg_ll_batch(self.h, n, _ptr(jobs), _ptr(qb), len(qbuf), _ptr(tb), len(tbuf), m, _ptr(mat), gapo,
                                   gape, xtra, _ptr(res)), "mbg_ll_batch")
         return res
+
+    def update_extra(self, hits, cigars, qseqs, tseqs, mat, q, e, log_gap):
+        """mb_fix_cigar + mb_update_extra's scores (align.c:70-258) for hits = list of dict(qs, qe, ts, te,
+        rev) with their CIGARs and the query/target bases from each hit's first base.  Returns
+        (EXTRA_JOB_DTYPE array, fixed CIGARs)."""
+        n = len(hits)
+        qbuf, qoff, _ = pack_seqs(qseqs)
+        tbuf, toff, _ = pack_seqs(tseqs)
+        jobs = np.zeros(n, EXTRA_JOB_DTYPE)
+        for f in ("qs", "qe", "ts", "te", "rev"):
+            jobs[f] = [h.get(f, 0) for h in hits]
+        jobs["qoff"], jobs["toff"] = qoff[:-1], toff[:-1]
+        jobs["n_cigar"] = [len(c) for c in cigars]
+        coff = np.zeros(n + 1, np.int64)
+        np.cumsum(jobs["n_cigar"], out=coff[1:])
+        jobs["cigar_off"] = coff[:-1]
+        cig = np.zeros(int(coff[-1]) + 1, np.uint32)
+        for i, c in enumerate(cigars):
+            cig[coff[i]:coff[i + 1]] = c
+        mat = np.ascontiguousarray(mat, np.int8)
+        qb = qbuf if len(qbuf) else np.zeros(1, np.uint8)
+        tb = tbuf if len(tbuf) else np.zeros(1, np.uint8)
+        check(self.L.mbg_update_extra_batch(self.h, n, _ptr(jobs), _ptr(qb), len(qbuf), _ptr(tb), len(tbuf), _ptr(mat), q, e,
+                                            log_gap, _ptr(cig), len(cig)), "mbg_update_extra_batch")
+        return jobs, [cig[coff[i]:coff[i] + jobs["n_cigar"][i]].tolist() for i in range(n)]
 
     def matesw(self, reads, tid, ts, te, min_sc, opt: MateswOpt, stats: "MateswStats | None" = None):
         """Mate rescue of candidate windows [ts, te) on contigs tid for reads (one per candidate, as they

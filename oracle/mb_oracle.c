@@ -21,6 +21,9 @@
  *                                                      jobs (both sizes, SUBO/STOP, saturation)
  *   mate rescue                pe.c:209-312,342-349     mb_ungap: brute-force vote count; mb_ungap and
  *     (ungap, matesw_align)                            mb_matesw_align of the compiled reference
+ *   mb_fix_cigar + mb_update_extra  align.c:70-258      the compiled reference's mb_update_extra on
+ *                                                      hits from the DP (both gap models); the
+ *                                                      score of the fixed CIGAR (invariant)
  *
  * The DP restatement is scalar but keeps the SSE code's observable semantics: int8 wrap-around
  * arithmetic, the 16-cell-aligned band (cells outside the exact band are computed and kept), the
@@ -1012,6 +1015,125 @@ int orc_matesw_cand(const int32_t p[11], int32_t qlen, const uint8_t *qseq, int3
 	r->ll.score = r->ll.te = r->ll.qe = r->ll.score2 = r->ll.te2 = -1;
 	if (*aligned) orc_matesw_align(p, qlen, qseq, te2 - ts2, win + ts2, min_sc, mt, r, cigar, cigar_cap);
 	return 0;
+}
+
+/* mb_log2 (mbpriv.h:105-113): the float bit-pattern approximation of log2 */
+static float orc_log2(float x)
+{
+	union { float f; uint32_t i; } z = { x };
+	float l = (float)(((z.i >> 23) & 255u) - 128u);
+	z.i &= ~(255u << 23);
+	z.i += 127u << 23;
+	l += (-0.34484843f * z.f + 2.02466578f) * z.f - 0.67487759f;
+	return l;
+}
+
+/* mb_fix_cigar (align.c:70-142) and mb_update_extra (align.c:219-258) up to the scores it sets,
+ * without the =/X, cs and MD strings.  qseq and tseq point at the hit's first bases (qs, ts), as
+ * the reference's callers pass them.  The CIGAR is edited in place:
+ *   - an indel between two M runs moves left while the bases before it equal its last ones;
+ *   - a run of adjacent I and D operations becomes one I and one D;
+ *   - zero-length operations are squeezed out and equal neighbours merged;
+ *   - a leading I or D is dropped, moving qs (qe for a reverse hit) or ts.
+ * Then blen, mlen and the ambiguous-base count, and dp_max, the best score of any prefix of the
+ * alignment (never below 0), with gaps costing q+e, or q+e*log2(1+len) when log_gap. */
+void orc_update_extra(orc_extra_t *r, uint32_t *cigar, const uint8_t *qseq, const uint8_t *tseq, const int8_t *mat, int8_t q, int8_t e,
+                      int log_gap)
+{
+	int32_t toff = 0, qoff = 0, to_shrink = 0, qshift = 0, tshift = 0;
+	uint32_t k, n = (uint32_t)r->n_cigar;
+	double sc = 0.0, best = 0.0;
+	if (n == 0) return;
+	if (n > 1) {
+		for (k = 0; k < n; ++k) {
+			uint32_t op = cigar[k] & 0xf, len = cigar[k] >> 4;
+			if (len == 0) to_shrink = 1;
+			if (op == 0) {
+				toff += len, qoff += len;
+			} else if (op == 1 || op == 2) {
+				if (k > 0 && k < n - 1 && (cigar[k - 1] & 0xf) == 0 && (cigar[k + 1] & 0xf) == 0) {
+					int l, prev = (int)(cigar[k - 1] >> 4);
+					const uint8_t *sq = op == 1 ? qseq : tseq;
+					int off = op == 1 ? qoff : toff;
+					for (l = 0; l < prev; ++l)
+						if (sq[off - 1 - l] != sq[off + (int)len - 1 - l]) break;
+					if (l > 0) cigar[k - 1] -= (uint32_t)l << 4, cigar[k + 1] += (uint32_t)l << 4, qoff -= l, toff -= l;
+					if (l == prev) to_shrink = 1;
+				}
+				if (op == 1) qoff += len;
+				else toff += len;
+			} else if (op == 3) {
+				toff += len;
+			}
+		}
+		for (k = 0; k + 2 < n; ++k) {                  /* a run like 5I6D7I becomes 12I6D */
+			if ((cigar[k] & 0xf) > 0 && (cigar[k] & 0xf) + (cigar[k + 1] & 0xf) == 3) {
+				uint32_t l, cnt[16] = {0};
+				for (l = k; l < n; ++l) {
+					uint32_t op = cigar[l] & 0xf;
+					if (op == 1 || op == 2 || cigar[l] >> 4 == 0) cnt[op] += cigar[l] >> 4;
+					else break;
+				}
+				if (cnt[1] > 0 && cnt[2] > 0 && l - k > 2) {
+					cigar[k] = cnt[1] << 4 | 1;
+					cigar[k + 1] = cnt[2] << 4 | 2;
+					for (k += 2; k < l; ++k) cigar[k] &= 0xf;
+					to_shrink = 1;
+				}
+				k = l;
+			}
+		}
+		if (to_shrink) {
+			uint32_t l = 0;
+			for (k = 0; k < n; ++k)
+				if (cigar[k] >> 4 != 0) cigar[l++] = cigar[k];
+			n = l;
+			for (k = l = 0; k < n; ++k)
+				if (k == n - 1 || (cigar[k] & 0xf) != (cigar[k + 1] & 0xf)) cigar[l++] = cigar[k];
+				else cigar[k + 1] += cigar[k] >> 4 << 4;
+			n = l;
+		}
+		if ((cigar[0] & 0xf) == 1 || (cigar[0] & 0xf) == 2) {
+			int32_t l = (int32_t)(cigar[0] >> 4);
+			if ((cigar[0] & 0xf) == 1) {
+				if (r->rev) r->qe -= l;
+				else r->qs += l;
+				qshift = l;
+			} else r->ts += l, tshift = l;
+			--n;
+			memmove(cigar, cigar + 1, n * 4);
+		}
+	}
+	qseq += qshift, tseq += tshift;
+	r->n_cigar = (int32_t)n;
+	r->blen = r->mlen = r->n_ambi = 0;
+	toff = qoff = 0;
+	for (k = 0; k < n; ++k) {
+		uint32_t op = cigar[k] & 0xf, len = cigar[k] >> 4, l;
+		int na = 0, nd = 0;
+		if (op == 0) {
+			for (l = 0; l < len; ++l) {
+				int cq = qseq[qoff + l], ct = tseq[toff + l];
+				if (ct > 3 || cq > 3) ++na;
+				else if (ct != cq) nd += mat[ct * 5 + cq] < 0;
+				sc += mat[ct * 5 + cq];
+				if (sc < 0) sc = 0;
+				else best = best > sc ? best : sc;
+			}
+			r->blen += (int32_t)len - na, r->mlen += (int32_t)len - (na + nd), r->n_ambi += na;
+			toff += len, qoff += len;
+		} else if (op == 1 || op == 2) {
+			const uint8_t *sq = op == 1 ? qseq + qoff : tseq + toff;
+			for (l = 0; l < len; ++l) na += sq[l] > 3;
+			r->blen += (int32_t)len - na, r->n_ambi += na;
+			if (log_gap) sc -= q + (double)e * orc_log2((float)(1.0 + len));
+			else sc -= q + e;
+			if (sc < 0) sc = 0;
+			if (op == 1) qoff += len;
+			else toff += len;
+		}
+	}
+	r->dp_max = (int32_t)(best + .499);
 }
 
 /* main.c:164-200 (`minibwa bench`): checksum over n splitmix64-drawn queries; type 0 = rank2a,

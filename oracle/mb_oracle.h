@@ -64,6 +64,13 @@ int32_t orc_ungap(int32_t qlen, const uint8_t *qseq, int32_t tlen, const uint8_t
                   int32_t *max_i, int32_t *n_good, int32_t *n_kmer);
 int orc_matesw_align(const int32_t p[11], int32_t qlen, const uint8_t *qseq, int32_t tlen, const uint8_t *tseq, int32_t min_sc,
                      int32_t mt, orc_rescue_t *r, uint32_t *cigar, int cigar_cap);
+typedef struct {               /* the mb_hit_t / mb_extra_t fields mb_fix_cigar and mb_update_extra set */
+	int32_t qs, qe, rev, n_cigar;
+	int64_t ts, te;
+	int32_t blen, mlen, n_ambi, dp_max;
+} orc_extra_t;
+void orc_update_extra(orc_extra_t *r, uint32_t *cigar, const uint8_t *qseq, const uint8_t *tseq, const int8_t *mat, int8_t q, int8_t e,
+                      int log_gap);
 int orc_matesw_cand(const int32_t p[11], int32_t qlen, const uint8_t *qseq, int32_t wlen, const uint8_t *win, int32_t min_sc,
                     int32_t mt, int32_t ug[4], int32_t win2[2], int32_t *aligned, orc_rescue_t *r, uint32_t *cigar, int cigar_cap);
 uint64_t orc_bench(const orc_bwt_t *b, int type, int64_t n, int intv);

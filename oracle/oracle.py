@@ -51,6 +51,14 @@ class OrcRescue(C.Structure):
                 ("qs", C.c_int32), ("qe", C.c_int32), ("ts", C.c_int32), ("te", C.c_int32), ("dp_max2", C.c_int32)]
 
 
+class OrcExtra(C.Structure):
+    _fields_ = [("qs", C.c_int32), ("qe", C.c_int32), ("rev", C.c_int32), ("n_cigar", C.c_int32), ("ts", C.c_int64),
+                ("te", C.c_int64), ("blen", C.c_int32), ("mlen", C.c_int32), ("n_ambi", C.c_int32), ("dp_max", C.c_int32)]
+
+
+EXTRA_FIELDS = ("qs", "qe", "ts", "te", "n_cigar", "blen", "mlen", "n_ambi", "dp_max")
+
+
 class MbxHit(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("rescued", "qs", "qe", "score", "score0", "subsc", "dp_max", "dp_max2",
                                          "dp_score", "dp_max0", "mlen", "blen", "n_ambi", "n_cigar")] + \
@@ -107,6 +115,7 @@ def _load_oracle():
                                    _u32p, C.c_int]
     L.orc_matesw_cand.argtypes = [_i32p, C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, _i32p, _i32p,
                                   C.POINTER(C.c_int32), C.POINTER(OrcRescue), _u32p, C.c_int]
+    L.orc_update_extra.argtypes = [C.POINTER(OrcExtra), _u32p, _u8p, _u8p, _i8p, C.c_int8, C.c_int8, C.c_int]
     L.orc_ll.restype = C.c_int
     L.orc_ll.argtypes = [C.c_int, C.c_int, _u8p, C.c_int, _i8p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int,
                          C.POINTER(LlRst)]
@@ -457,6 +466,8 @@ def ref_pe():
         L.mbx_ungap.argtypes = [C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, _i32p]
         L.mbx_matesw_align.argtypes = [_i32p, C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, C.POINTER(MbxHit),
                                        _u32p, C.c_int32, _i32p, _u32p, C.c_int32]
+        L.mbx_update_extra.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32, _u32p, _u8p, _u8p,
+                                       _i8p, C.c_int8, C.c_int8, C.c_int32, C.POINTER(MbxHit), _u32p, C.c_int32]
         L.mbx_finish.argtypes = [_i32p, C.c_int32, _u8p, _u8p, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32,
                                  C.c_int32, C.c_int32, C.c_int32, C.c_int32, _u32p, C.POINTER(MbxHit), _u32p, C.c_int32]
         _REFPE = L
@@ -503,3 +514,27 @@ def ref_finish(p, query, window, mt, qs, qe, ts, te, ez_max, score, score2, dp_m
     ref_pe().mbx_finish(np.ascontiguousarray(p, np.int32), mt, q, w, qs, qe, ts, te, ez_max, score, score2, dp_max2,
                         len(cigar), c0, C.byref(h), cig, cap)
     return _hit_dict(h, cig)
+
+
+def orc_update_extra(hit, cigar, qseq, tseq, mat, q, e, log_gap):
+    """mb_fix_cigar + mb_update_extra restated; hit = dict(qs, qe, ts, te, rev); qseq/tseq start at the
+    hit's first bases.  Returns (dict of EXTRA_FIELDS, fixed CIGAR)."""
+    r = OrcExtra(hit["qs"], hit["qe"], hit.get("rev", 0), len(cigar), hit["ts"], hit["te"], 0, 0, 0, 0)
+    c = np.array(list(cigar) + [0], np.uint32)
+    orc().orc_update_extra(C.byref(r), c, np.ascontiguousarray(qseq, np.uint8), np.ascontiguousarray(tseq, np.uint8),
+                           np.ascontiguousarray(mat, np.int8), q, e, log_gap)
+    return {f: int(getattr(r, f)) for f in EXTRA_FIELDS}, [int(x) for x in c[:r.n_cigar]]
+
+
+def ref_update_extra(hit, cigar, qseq, tseq, mat, q, e, log_gap):
+    """The reference's mb_update_extra (with mb_fix_cigar) -> (dict of EXTRA_FIELDS, fixed CIGAR)."""
+    h = MbxHit()
+    c0 = np.array(list(cigar) + [0], np.uint32)
+    cap = len(cigar) + 8
+    out = np.zeros(cap, np.uint32)
+    ref_pe().mbx_update_extra(hit["qs"], hit["qe"], hit["ts"], hit["te"], hit.get("rev", 0), len(cigar), c0,
+                              np.ascontiguousarray(qseq, np.uint8), np.ascontiguousarray(tseq, np.uint8),
+                              np.ascontiguousarray(mat, np.int8), q, e, log_gap, C.byref(h), out, cap)
+    d = {f: int(getattr(h, f)) for f in ("qs", "qe", "ts", "te", "n_cigar", "blen", "mlen", "dp_max")}
+    d["n_ambi"] = int(h.n_ambi)
+    return d, [int(x) for x in out[:h.n_cigar]]

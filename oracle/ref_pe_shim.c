@@ -79,3 +79,18 @@ void mbx_finish(const int32_t p[11], int32_t mt, const uint8_t *qseq, const uint
 	mbx_out(&h, r, cigar, cap);
 	free(h.p);
 }
+
+/* the reference's mb_update_extra (align.c:219-283, with mb_fix_cigar) on one hit, no cs/MD/=X
+ * output: qseq and tseq point at the hit's first bases, as its callers pass them */
+void mbx_update_extra(int32_t qs, int32_t qe, int64_t ts, int64_t te, int32_t rev, int32_t n_cigar, const uint32_t *cigar0,
+                      const uint8_t *qseq, const uint8_t *tseq, const int8_t *mat, int8_t q, int8_t e, int32_t log_gap,
+                      mbx_hit_t *r, uint32_t *cigar, int32_t cap)
+{
+	mb_hit_t h;
+	memset(&h, 0, sizeof(h));
+	mb_append_cigar(&h, n_cigar, cigar0);
+	h.qs = qs, h.qe = qe, h.ts = ts, h.te = te, h.rev = rev;
+	mb_update_extra(0, &h, qseq, tseq, mat, q, e, 0, log_gap);
+	mbx_out(&h, r, cigar, cap);
+	free(h.p);
+}

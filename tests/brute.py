@@ -233,3 +233,30 @@ def ungap_votes(query, target, mt=0, k=7):
         if i is not None and j >= i:
             votes[j - i] = votes.get(j - i, 0) + 1
     return votes, len(last)
+
+
+def random_alignment(rng, n_ops=None, lowcomp=False):
+    """A CIGAR of M/I/D runs (adjacent gaps and zero-length runs included) and query/target bases
+    that it aligns, with substitutions and Ns in the M runs: (cigar, qseq, tseq)."""
+    alpha = rng.integers(0, 4, 2) if lowcomp else None
+    ops = []
+    for _ in range(n_ops or int(rng.integers(1, 9))):
+        op = int(rng.choice([0, 0, 1, 2]))
+        ln = int(rng.integers(0 if rng.random() < 0.05 else 1, 12))
+        ops.append((op, ln))
+    q, t = [], []
+
+    def base():
+        return int(rng.choice(alpha)) if lowcomp else int(rng.integers(0, 4))
+    for op, ln in ops:
+        for _ in range(ln):
+            if op == 0:
+                b = base()
+                q.append(b)
+                r = rng.random()
+                t.append(int(rng.integers(0, 4)) if r < 0.05 else 4 if r < 0.07 else b)
+            elif op == 1:
+                q.append(base())
+            else:
+                t.append(base())
+    return [ln << 4 | op for op, ln in ops], np.array(q, np.uint8), np.array(t, np.uint8)

@@ -338,3 +338,38 @@ def test_ungap_bruteforce(mt):
         if best == 0:
             n_good = 0
         assert O.orc_ungap(q, t, mt) == (best, first, n_good, n_kmer), it
+
+
+def test_update_extra_invariants():
+    """mb_fix_cigar + mb_update_extra's invariants, independent of the reference: the fixed CIGAR
+    spans the hit's new query and target ranges, the M-run bases still pair as before (a slid indel
+    only moves between equal bases), and dp_max is the best clamped prefix score of the fixed CIGAR
+    computed here from its definition."""
+    rng = np.random.default_rng(17)
+    mat = O.gen_nt4_mat(2, 8, 0, 1)
+    for it in range(800):
+        cig, qs, ts = brute.random_alignment(rng, lowcomp=rng.random() < 0.6)
+        ql = sum(c >> 4 for c in cig if c & 0xf in (0, 1))
+        tl = sum(c >> 4 for c in cig if c & 0xf in (0, 2))
+        hit = dict(qs=0, qe=ql, ts=0, te=tl, rev=0)
+        d, fc = O.orc_update_extra(hit, cig, qs, ts, mat, 12, 2, 0)
+        assert sum(c >> 4 for c in fc if c & 0xf in (0, 1)) == d["qe"] - d["qs"], it
+        assert sum(c >> 4 for c in fc if c & 0xf in (0, 2)) == d["te"] - d["ts"], it
+        q0, t0, sc, best, mm, blen = d["qs"], d["ts"], 0, 0, 0, 0
+        for c in fc:
+            op, ln = c & 0xf, c >> 4
+            assert ln > 0, it
+            if op == 0:
+                for k in range(ln):
+                    a, b = int(qs[q0 + k]), int(ts[t0 + k])
+                    sc = max(0, sc + int(mat[b * 5 + a]))
+                    best = max(best, sc)
+                    if a < 4 and b < 4:
+                        blen += 1
+                        mm += a == b
+                q0, t0 = q0 + ln, t0 + ln
+            else:
+                sc = max(0, sc - 14)
+                blen += int(sum(1 for k in range(ln) if (qs[q0 + k] if op == 1 else ts[t0 + k]) < 4))
+                q0, t0 = (q0 + ln, t0) if op == 1 else (q0, t0 + ln)
+        assert d["dp_max"] == best and d["blen"] == blen and d["mlen"] == mm, (it, fc, d)

@@ -234,3 +234,32 @@ def test_rescue_vs_reference(mt):
                              d["ll"]["score2"], d["dp_max2"], d["cigar"])
             assert f == h, it
     assert n_res > 100 and n_narrow > 50
+
+
+@pytest.mark.parametrize("log_gap", [0, 1])
+def test_update_extra_vs_reference(log_gap):
+    """mb_fix_cigar + mb_update_extra (align.c:70-258): the oracle against the reference's own
+    function on hits from the extension DP and on random CIGARs (adjacent I/D runs, zero-length runs,
+    leading gaps, low-complexity bases that make indels slide, Ns), both strands and gap models."""
+    from tests import brute
+    rng = np.random.default_rng(91 + log_gap)
+    mats = [O.gen_nt4_mat(2, 8, 0, 1), O.gen_nt4_mat(1, 4, 2, 1)]
+    n_fixed = 0
+    for it in range(1500):
+        mat = mats[it % 2]
+        if it % 3 == 0:
+            qs, ts = _pair(rng, 200)
+            d = O.extz2(qs, ts, mat, 12, 2, 101, -1, 0, 0)
+            cig = d["cigar"]
+            if not cig:
+                continue
+        else:
+            cig, qs, ts = brute.random_alignment(rng, lowcomp=rng.random() < 0.5)
+        ql = sum(c >> 4 for c in cig if c & 0xf in (0, 1))
+        tl = sum(c >> 4 for c in cig if c & 0xf in (0, 2))
+        hit = dict(qs=0, qe=ql, ts=0, te=tl, rev=int(rng.random() < 0.5))
+        a, ac = O.orc_update_extra(hit, cig, qs[:ql], ts[:tl], mat, 12, 2, log_gap)
+        b, bc = O.ref_update_extra(hit, cig, qs[:ql], ts[:tl], mat, 12, 2, log_gap)
+        assert a == b and ac == bc, (it, cig, a, b, ac, bc)
+        n_fixed += ac != list(cig)
+    assert n_fixed > 200
