@@ -184,18 +184,23 @@ typedef struct {
 	int32_t rescued;                         /* a hit was made (pe.c:293) */
 	mbg_ll_result_t ll;                      /* its ksw_ll; all -1 when it did not run */
 	mbg_dp_result_t ez;                      /* its ksw_extz2 on the reversed prefixes; zero when not run */
-	int32_t qs, qe, ts, te;                  /* the hit, relative to the aligned window (pe.c:295-296) */
+	int32_t qs, qe, ts, te;                  /* the rescued hit as mb_update_extra leaves it, relative to the
+	                                            aligned window (pe.c:295-296,309) */
 	int32_t dp_max2;                         /* pe.c:298-299 */
 	int32_t ran_dp;                          /* the extension DP ran (pe.c:273) */
+	int32_t n_cigar;                         /* the rescued hit's CIGAR after mb_fix_cigar */
+	int32_t blen, mlen, n_ambi, dp_max;      /* mb_update_extra's (dp_max0 = dp_max; dp_score = ez.max) */
 } mbg_matesw_result_t;
 
 /* Batched mate rescue on the resident reference: per candidate, l2b_getseq of the window, mb_ungap,
- * the narrowing to a clear diagonal and mb_matesw_align's ksw_ll + ksw_extz2 (pe.c:342-349,
- * 249-300), results equal to the reference's.  The host keeps the rest of mb_matesw_core
- * (candidate windows, orientation, mb_append_cigar + mb_update_extra on rescued hits).  A window
+ * the narrowing to a clear diagonal and mb_matesw_align (pe.c:342-349, 249-312): ksw_ll, ksw_extz2
+ * of the reversed prefixes, and for a rescued hit mb_fix_cigar + mb_update_extra's scores.  Results
+ * equal the reference's.  The host keeps the rest of mb_matesw_core (candidate windows,
+ * orientation, the hit's coordinates on the contig) and the cs/MD strings.  A window
  * not longer than the read after clipping is no candidate (aligned 0).  Supported: qlen <= 1024,
- * windows <= 16384 bases, q/a >= 1 (ksw_ll); else MBG_EUNSUPPORTED.  The extension CIGAR of
- * candidate i is at cigar + jobs[i].cigar_off (ez.n_cigar entries, as ksw_extz2 leaves it).  stats
+ * windows <= 16384 bases, q/a >= 1 (ksw_ll); else MBG_EUNSUPPORTED.  Candidate i's CIGAR is at
+ * cigar + jobs[i].cigar_off: the rescued hit's (n_cigar entries), else the extension DP's as
+ * ksw_extz2 leaves it (ez.n_cigar entries).  stats
  * (NULL: not wanted) receives counts and the device time of each phase. */
 typedef struct {
 	int64_t n_aligned, n_rescued;

@@ -919,7 +919,8 @@ int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, cons
 	a.mt = o.mt, a.a = o.a, a.b_mm = b_mm, a.q = o.q, a.e = o.e, a.bw = o.bw, a.zdrop = o.zdrop, a.end_bonus = o.end_bonus;
 	a.min_dp_max = o.min_dp_max;
 	TRY(dgrow(c->r_cnt, (size_t)n * 4 + 16)); TRY(dgrow(c->r_off, (size_t)n * 4 + 16));
-	a.cigar = (const uint32_t*)m[11].p, a.ccnt = (uint32_t*)c->r_cnt.p;
+	TRY(dgrow(c->b_in[5], (size_t)n * sizeof(ExtraJob)));
+	a.cigar = (const uint32_t*)m[11].p, a.ccnt = (uint32_t*)c->r_cnt.p, a.extra = (ExtraJob*)c->b_in[5].p;
 	HIPCHK(launch_ungap(a, max_w, s));
 	HIPCHK(hipEventRecord(c->ev[2], s));
 	// ksw_ll over the (narrowed) windows: per-job spans of the per-row maxima list from a scan
@@ -944,6 +945,15 @@ int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, cons
 	                  (DpRes*)m[9].p, (uint32_t*)m[11].p, cap, slab, qt, tcap));
 	HIPCHK(hipEventRecord(c->ev[4], s));
 	HIPCHK(launch_rescue_finish(a, s));
+	// mb_fix_cigar + mb_update_extra on the rescued hits (pe.c:309), CIGARs edited in their slots
+	{
+		ExtraArgs x{};
+		x.jobs = a.extra, x.n = n, x.qseq = (const uint8_t*)m[1].p, x.tseq = (const uint8_t*)m[2].p, x.cigar = (uint32_t*)m[11].p;
+		for (int k = 0; k < 28; ++k) x.mat[k] = k < 25 ? mat_dp[k] : 0;
+		x.q = o.q, x.e = o.e, x.log_gap = 0;
+		HIPCHK(launch_extra(x, s));
+		HIPCHK(launch_rescue_final(a, s));
+	}
 	// CIGARs packed at the prefix sums of their lengths, so that only their operations cross PCIe
 	size_t scan_bytes = 0;
 	HIPCHK(scan_u32(nullptr, &scan_bytes, nullptr, nullptr, n, s));
@@ -954,15 +964,21 @@ int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, cons
 	HIPCHK(hipEventRecord(c->ev[5], s));
 	HIPCHK(hipMemcpyAsync(res, m[10].p, (size_t)n * sizeof(MsRes), hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
+	// the CIGAR brought back: the fixed one of a rescued hit, else the extension DP's
+	auto n_back = [&](int32_t i) -> int32_t {
+		const mbg_matesw_result_t &r = res[i];
+		return r.rescued ? r.n_cigar : r.ran_dp && r.ez.n_cigar > 0 ? r.ez.n_cigar : 0;
+	};
 	int64_t n_ops = 0;
-	for (int32_t i = 0; i < n; ++i) n_ops += res[i].ran_dp && res[i].ez.n_cigar > 0 ? res[i].ez.n_cigar : 0;
+	for (int32_t i = 0; i < n; ++i) n_ops += n_back(i);
 	std::vector<uint32_t> packed((size_t)n_ops + 1);
 	if (n_ops > 0) HIPCHK(hipMemcpy(packed.data(), c->b_out[2].p, (size_t)n_ops * 4, hipMemcpyDeviceToHost));
 	int64_t at = 0;
 	for (int32_t i = 0; i < n; ++i) {
-		if (!res[i].ran_dp || res[i].ez.n_cigar <= 0) continue;
-		memcpy(cigar + jobs[i].cigar_off, packed.data() + at, (size_t)res[i].ez.n_cigar * 4);
-		at += res[i].ez.n_cigar;
+		const int32_t k = n_back(i);
+		if (k <= 0) continue;
+		memcpy(cigar + jobs[i].cigar_off, packed.data() + at, (size_t)k * 4);
+		at += k;
 	}
 	if (stats) {
 		memset(stats, 0, sizeof(*stats));

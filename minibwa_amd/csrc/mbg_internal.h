@@ -117,40 +117,6 @@ int ll_slots(int qlen, int size);
 hipError_t launch_ll(const LlArgs &a, int slots, hipStream_t st);
 hipError_t scan_ll_rows(void *temp, size_t *temp_bytes, const LlJob *jobs, int64_t *out, int64_t n, hipStream_t st);
 
-// mate rescue (mbg_rescue.hip); layouts of mbg_matesw_job_t / mbg_matesw_result_t
-struct MsJob { int64_t qoff; int32_t qlen, tid; int64_t ts, te; int32_t min_sc, pad; int64_t cigar_off; };
-struct MsRes {
-	int32_t max_ug, max_i, n_good, n_kmer, ts2, te2, aligned, rescued;
-	LlRes ll;
-	DpRes ez;
-	int32_t qs, qe, ts, te, dp_max2, ran_dp;
-};
-static_assert(sizeof(MsJob) == 48, "MsJob must match mbg_matesw_job_t");
-static_assert(sizeof(MsRes) == 124, "MsRes must match mbg_matesw_result_t");
-
-struct MsArgs {
-	const MsJob *jobs;
-	int32_t n;
-	const uint8_t *qseq;
-	const uint8_t *win;     // candidate windows: c's at win + woff[c], wlen[c] bases (clipped to its contig)
-	const int64_t *woff, *wlen;
-	uint8_t *rq, *rt;       // reversed prefixes for the extension DP, laid out as qseq and win
-	LlJob *ll_jobs;
-	const LlRes *ll_res;
-	DpJob *dp_jobs;
-	const DpRes *dp_res;
-	MsRes *res;
-	const uint32_t *cigar;  // the extension DP's CIGAR slots (at each job's cigar_off)
-	uint32_t *ccnt;         // per candidate: CIGAR operations to bring back
-	int32_t tab_size;       // k-mer table slots of the vote (a power of two >= 2 * the longest read)
-	int32_t mt, a, b_mm, q, e, bw, zdrop, end_bonus, min_dp_max;
-};
-constexpr int kMsMaxWindow = 16384;    // rescue window bases: 16-bit vote counters, 32 KB of LDS
-hipError_t launch_ungap(const MsArgs &a, int max_wlen, hipStream_t st);
-hipError_t launch_rescue_dp(const MsArgs &a, hipStream_t st);
-hipError_t launch_rescue_finish(const MsArgs &a, hipStream_t st);
-hipError_t launch_rescue_cigar(const MsArgs &a, const uint32_t *coff, uint32_t *packed, hipStream_t st);
-
 // hit finishing (mbg_extra.hip); layout of mbg_extra_job_t
 struct ExtraJob {
 	int64_t qoff, toff;
@@ -170,6 +136,44 @@ struct ExtraArgs {
 	int32_t q, e, log_gap;
 };
 hipError_t launch_extra(const ExtraArgs &a, hipStream_t st);
+
+// mate rescue (mbg_rescue.hip); layouts of mbg_matesw_job_t / mbg_matesw_result_t
+struct MsJob { int64_t qoff; int32_t qlen, tid; int64_t ts, te; int32_t min_sc, pad; int64_t cigar_off; };
+struct MsRes {
+	int32_t max_ug, max_i, n_good, n_kmer, ts2, te2, aligned, rescued;
+	LlRes ll;
+	DpRes ez;
+	int32_t qs, qe, ts, te, dp_max2, ran_dp;
+	int32_t n_cigar, blen, mlen, n_ambi, dp_max;
+};
+static_assert(sizeof(MsJob) == 48, "MsJob must match mbg_matesw_job_t");
+static_assert(sizeof(MsRes) == 144, "MsRes must match mbg_matesw_result_t");
+
+struct MsArgs {
+	const MsJob *jobs;
+	int32_t n;
+	const uint8_t *qseq;
+	const uint8_t *win;     // candidate windows: c's at win + woff[c], wlen[c] bases (clipped to its contig)
+	const int64_t *woff, *wlen;
+	uint8_t *rq, *rt;       // reversed prefixes for the extension DP, laid out as qseq and win
+	LlJob *ll_jobs;
+	const LlRes *ll_res;
+	DpJob *dp_jobs;
+	const DpRes *dp_res;
+	MsRes *res;
+	const uint32_t *cigar;  // the extension DP's CIGAR slots (at each job's cigar_off)
+	uint32_t *ccnt;         // per candidate: CIGAR operations to bring back
+	ExtraJob *extra;        // per candidate: the rescued hit's mb_fix_cigar + mb_update_extra job
+	int32_t tab_size;       // k-mer table slots of the vote (a power of two >= 2 * the longest read)
+	int32_t mt, a, b_mm, q, e, bw, zdrop, end_bonus, min_dp_max;
+};
+constexpr int kMsMaxWindow = 16384;    // rescue window bases: 16-bit vote counters, 32 KB of LDS
+hipError_t launch_ungap(const MsArgs &a, int max_wlen, hipStream_t st);
+hipError_t launch_rescue_dp(const MsArgs &a, hipStream_t st);
+hipError_t launch_rescue_finish(const MsArgs &a, hipStream_t st);
+hipError_t launch_rescue_cigar(const MsArgs &a, const uint32_t *coff, uint32_t *packed, hipStream_t st);
+hipError_t launch_rescue_final(const MsArgs &a, hipStream_t st);
+
 
 hipError_t launch_kmer_cache(const Idx &ix, Sai *pre, int len, hipStream_t st);
 hipError_t launch_sa_fine(const Idx &ix, uint64_t *fine, uint64_t n_fine, hipStream_t st);

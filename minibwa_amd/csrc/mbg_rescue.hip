@@ -167,6 +167,9 @@ __global__ void rescue_finish_kernel(MsArgs a)
 	a.ccnt[c] = o.ran_dp && ez.n_cigar > 0 ? (uint32_t)ez.n_cigar : 0u;
 	o.rescued = o.ran_dp && ez.n_cigar > 0 && ez.max >= a.min_dp_max * a.a;
 	o.qs = o.qe = o.ts = o.te = o.dp_max2 = 0;
+	o.n_cigar = o.blen = o.mlen = o.n_ambi = o.dp_max = 0;
+	ExtraJob x{};
+	x.cigar_off = a.jobs[c].cigar_off;
 	if (o.rescued) {
 		const int qe = ll.qe + 1, te = ll.te + 1;
 		o.qe = qe, o.te = te;
@@ -174,7 +177,31 @@ __global__ void rescue_finish_kernel(MsArgs a)
 		o.qs = qe - (ez.reach_end ? qe : ez.max_q + 1);
 		const int m2 = (int)((double)ez.max / ll.score * ll.score2 + .499);
 		o.dp_max2 = m2 < 0 ? 0 : m2;
+		// mb_update_extra on the hit (pe.c:309): the read from qs, the aligned window from ts
+		x.qoff = a.jobs[c].qoff + o.qs, x.toff = lj.toff + o.ts;
+		x.qs = o.qs, x.qe = o.qe, x.ts = o.ts, x.te = o.te, x.rev = 0, x.n_cigar = ez.n_cigar;
 	}
+	a.extra[c] = x;
+}
+
+// one thread per candidate: the hit fields mb_update_extra left, and the CIGAR length to bring back
+__global__ void rescue_final_kernel(MsArgs a)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= a.n) return;
+	MsRes &o = a.res[c];
+	if (!o.rescued) return;
+	const ExtraJob x = a.extra[c];
+	o.qs = x.qs, o.qe = x.qe, o.ts = (int32_t)x.ts, o.te = (int32_t)x.te;
+	o.n_cigar = x.n_cigar, o.blen = x.blen, o.mlen = x.mlen, o.n_ambi = x.n_ambi, o.dp_max = x.dp_max;
+	a.ccnt[c] = (uint32_t)x.n_cigar;
+}
+
+hipError_t launch_rescue_final(const MsArgs &a, hipStream_t st)
+{
+	if (a.n <= 0) return hipSuccess;
+	hipLaunchKernelGGL(rescue_final_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a);
+	return hipGetLastError();
 }
 
 // one thread per candidate: its CIGAR moved from the DP's slots to a packed list at ccnt's prefix sum

@@ -199,8 +199,9 @@ MATESW_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("qlen", "<i4"), ("tid", "<i4"), (
 MATESW_RES_DTYPE = np.dtype([(f, "<i4") for f in ("max_ug", "max_i", "n_good", "n_kmer", "ts2", "te2", "aligned",
                                                     "rescued")] +
                             [("ll", LL_RES_DTYPE), ("ez", DP_RES_DTYPE)] +
-                            [(f, "<i4") for f in ("qs", "qe", "ts", "te", "dp_max2", "ran_dp")])
-assert MATESW_JOB_DTYPE.itemsize == 48 and MATESW_RES_DTYPE.itemsize == 124
+                            [(f, "<i4") for f in ("qs", "qe", "ts", "te", "dp_max2", "ran_dp", "n_cigar", "blen", "mlen",
+                                                  "n_ambi", "dp_max")])
+assert MATESW_JOB_DTYPE.itemsize == 48 and MATESW_RES_DTYPE.itemsize == 144
 
 
 EXTRA_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qs", "<i4"), ("qe", "<i4"), ("ts", "<i8"), ("te", "<i8"),
@@ -544,7 +545,8 @@ class Context:
         qb = qbuf if len(qbuf) else np.zeros(1, np.uint8)
         check(self.L.mbg_matesw_batch(self.h, n, _ptr(jobs), _ptr(qb), len(qbuf), C.byref(opt), _ptr(res), _ptr(cig), cap,
                                       C.byref(stats) if stats is not None else None), "mbg_matesw_batch")
-        cigars = [cig[coff[i]:coff[i] + res["ez"]["n_cigar"][i]].tolist() if res["ran_dp"][i] else [] for i in range(n)]
+        nb = np.where(res["rescued"] != 0, res["n_cigar"], np.where(res["ran_dp"] != 0, np.maximum(res["ez"]["n_cigar"], 0), 0))
+        cigars = [cig[coff[i]:coff[i] + nb[i]].tolist() for i in range(n)]
         return res, cigars
 
     def ksw_extz2(self, queries, targets, params, mat, m=5) -> DpBatchResult:

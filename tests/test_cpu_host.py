@@ -82,7 +82,7 @@ def test_header_layouts_match_python(tmp_path):
         "mbg_extra_job_t": ["qoff", "toff", "qs", "qe", "ts", "te", "rev", "n_cigar", "cigar_off", "blen", "mlen", "n_ambi",
                             "dp_max"],
         "mbg_matesw_result_t": ["max_ug", "max_i", "n_good", "n_kmer", "ts2", "te2", "aligned", "rescued", "ll", "ez", "qs",
-                                "qe", "ts", "te", "dp_max2", "ran_dp"],
+                                "qe", "ts", "te", "dp_max2", "ran_dp", "n_cigar", "blen", "mlen", "n_ambi", "dp_max"],
     }
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "mbg.h"', "int main(void) {"]
     for s, fields in checks.items():

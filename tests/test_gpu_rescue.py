@@ -1,6 +1,6 @@
 """GPU parity of mbg_matesw_batch against the oracle's mate-rescue candidate: mb_ungap's vote, the
-window narrowing and mb_matesw_align before mb_update_extra (pe.c:209-312,342-349), which the CPU
-tests pin against the reference's own static functions.  Windows come from the chrM index resident
+window narrowing and mb_matesw_align with its mb_fix_cigar + mb_update_extra (pe.c:209-312,342-349),
+which the CPU tests pin against the reference's own functions.  Windows come from the chrM index resident
 in HBM (l2b_getseq on the device); reads are mutated chrM segments (substitutions, indels, Ns) and
 unrelated sequence, over the methylation modes and several option sets.
 """
@@ -66,6 +66,7 @@ def _candidates(rng, ref, n):
 
 
 def _compare(res, cigars, reads, ts, te, ref, p, mt, min_sc):
+    mat_dp = O.gen_nt4_mat(int(p[0]), int(p[1]), int(p[2]), int(p[3]), mt)
     n_res = 0
     for i, r in enumerate(reads):
         s, e = max(0, int(ts[i])), min(len(ref), int(te[i]))
@@ -80,11 +81,19 @@ def _compare(res, cigars, reads, ts, te, ref, p, mt, min_sc):
         assert {f: int(got["ll"][f]) for f in O.LL_FIELDS} == d["ll"], ctx
         assert int(got["ran_dp"]) == d["ran_dp"], ctx
         assert {f: int(got["ez"][f]) for f in EZ} == d["ez"], ctx
-        assert cigars[i] == d["cigar"], ctx
         assert int(got["rescued"]) == d["rescued"], ctx
-        assert tuple(int(got[f]) for f in ("qs", "qe", "ts", "te", "dp_max2")) == \
-            (d["qs"], d["qe"], d["ts"], d["te"], d["dp_max2"]), ctx
-        n_res += d["rescued"]
+        assert int(got["dp_max2"]) == d["dp_max2"], ctx
+        if not d["rescued"]:
+            assert cigars[i] == d["cigar"], ctx
+            continue
+        # the rescued hit after mb_fix_cigar + mb_update_extra (pe.c:309), from the oracle
+        ts2 = d["win2"][0]
+        hit = dict(qs=d["qs"], qe=d["qe"], ts=d["ts"], te=d["te"], rev=0)
+        x, xc = O.orc_update_extra(hit, d["cigar"], r[d["qs"]:], ref[s + ts2 + d["ts"]:e], mat_dp, int(p[4]), int(p[5]), 0)
+        assert cigars[i] == xc, ctx
+        assert {f: int(got[f]) for f in ("qs", "qe", "ts", "te", "n_cigar", "blen", "mlen", "n_ambi", "dp_max")} == \
+            {f: x[f] for f in ("qs", "qe", "ts", "te", "n_cigar", "blen", "mlen", "n_ambi", "dp_max")}, ctx
+        n_res += 1
     return n_res
 
 
