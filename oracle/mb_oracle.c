@@ -21,6 +21,8 @@
  *                                                      jobs (both sizes, SUBO/STOP, saturation)
  *   mate rescue                pe.c:209-312,342-349     mb_ungap: brute-force vote count; mb_ungap and
  *     (ungap, matesw_align)                            mb_matesw_align of the compiled reference
+ *   mb_anchor (no methylation) seed.c:109-339          the compiled reference's mb_anchor on chrM
+ *                                                      and repeat-rich indexes (klib sort ties)
  *   mb_fix_cigar + mb_update_extra  align.c:70-258      the compiled reference's mb_update_extra on
  *                                                      hits from the DP (both gap models); the
  *                                                      score of the fixed CIGAR (invariant)
@@ -1134,6 +1136,184 @@ void orc_update_extra(orc_extra_t *r, uint32_t *cigar, const uint8_t *qseq, cons
 		}
 	}
 	r->dp_max = (int32_t)(best + .499);
+}
+
+/* klib's radix sort (ksort.h:108-161 as seed.c:9-18 instantiates it): insertion sort up to 64
+ * elements, else an in-place most-significant-byte-first radix sort, 8 bits a pass, whose cycle
+ * permutation is not stable; buckets of 2-64 elements finish by insertion sort.  Equal keys end up
+ * in the order this procedure leaves them, which anchor ties depend on.  key(i) reads element i. */
+typedef struct { int64_t b, e; } rs_bucket_t;
+
+static void rs_insert(void *a, size_t sz, int64_t beg, int64_t end, uint64_t (*key)(const void*))
+{
+	char tmp[64], *base = (char*)a;
+	int64_t i, j;
+	for (i = beg + 1; i < end; ++i)
+		if (key(base + i * sz) < key(base + (i - 1) * sz)) {
+			memcpy(tmp, base + i * sz, sz);
+			for (j = i; j > beg && key(tmp) < key(base + (j - 1) * sz); --j) memcpy(base + j * sz, base + (j - 1) * sz, sz);
+			memcpy(base + j * sz, tmp, sz);
+		}
+}
+
+static void rs_msd(void *a, size_t sz, int64_t beg, int64_t end, int s, uint64_t (*key)(const void*))
+{
+	char tmp[64], swp[64], *base = (char*)a;
+	rs_bucket_t b[256];
+	int64_t i, k, l;
+	for (k = 0; k < 256; ++k) b[k].b = b[k].e = beg;
+	for (i = beg; i < end; ++i) ++b[key(base + i * sz) >> s & 255].e;
+	for (k = 1; k < 256; ++k) b[k].e += b[k - 1].e - beg, b[k].b = b[k - 1].e;
+	for (k = 0; k < 256;) {                               /* the in-place cycle permutation */
+		if (b[k].b != b[k].e) {
+			if ((l = (int64_t)(key(base + b[k].b * sz) >> s & 255)) != k) {
+				memcpy(tmp, base + b[k].b * sz, sz);
+				do {
+					memcpy(swp, tmp, sz);
+					memcpy(tmp, base + b[l].b * sz, sz);
+					memcpy(base + b[l].b++ * sz, swp, sz);
+					l = (int64_t)(key(tmp) >> s & 255);
+				} while (l != k);
+				memcpy(base + b[k].b++ * sz, tmp, sz);
+			} else ++b[k].b;
+		} else ++k;
+	}
+	for (b[0].b = beg, k = 1; k < 256; ++k) b[k].b = b[k - 1].e;
+	if (s) {
+		s = s > 8 ? s - 8 : 0;
+		for (k = 0; k < 256; ++k)
+			if (b[k].e - b[k].b > 64) rs_msd(a, sz, b[k].b, b[k].e, s, key);
+			else if (b[k].e - b[k].b > 1) rs_insert(a, sz, b[k].b, b[k].e, key);
+	}
+}
+
+static void rs_sort(void *a, size_t sz, int64_t n, uint64_t (*key)(const void*))
+{
+	if (n <= 64) rs_insert(a, sz, 0, n, key);
+	else rs_msd(a, sz, 0, n, 56, key);
+}
+
+static uint64_t key_x0(const void *p) { return ((const orc_sai_t*)p)->x[0]; }
+static uint64_t key_size(const void *p) { return ((const orc_sai_t*)p)->size; }
+static uint64_t key_info(const void *p) { return ((const orc_sai_t*)p)->info; }
+static uint64_t key_tpos(const void *p) { return (uint64_t)((const orc_anchor_t*)p)->tpos; }
+
+/* mb_seed_sort_dedup (seed.c:109-138): by x[0]; a group of equal x[0] by size, reversed, and runs of
+ * equal size by info; then one of each (x[0], size, info).  Returns the new count. */
+static int64_t seed_sort_dedup(orc_sai_t *u, int64_t n)
+{
+	int64_t i, i0, j, k, k0;
+	if (n <= 1) return n;
+	rs_sort(u, sizeof(*u), n, key_x0);
+	for (i = 1, i0 = 0; i <= n; ++i) {
+		if (i == n || u[i].x[0] != u[i0].x[0]) {
+			if (i - i0 > 1) {
+				rs_sort(u + i0, sizeof(*u), i - i0, key_size);
+				for (k = 0; k < (i - i0) / 2; ++k) {
+					orc_sai_t t = u[i0 + k]; u[i0 + k] = u[i - 1 - k]; u[i - 1 - k] = t;
+				}
+				for (k = i0 + 1, k0 = i0; k <= i; ++k)
+					if (k == i || u[k0].size != u[k].size) {
+						if (k - k0 > 1) rs_sort(u + k0, sizeof(*u), k - k0, key_info);
+						k0 = k;
+					}
+			}
+			i0 = i;
+		}
+	}
+	for (i = 1, j = 0; i < n; ++i)
+		if (!(u[i].x[0] == u[j].x[0] && u[i].size == u[j].size && u[i].info == u[j].info)) u[++j] = u[i];
+	return j + 1;
+}
+
+/* l2b_pos2cid + l2b_intv2cid (l2bit.c:9-34): the contig of [st,en) on the forward+reverse text */
+static int64_t intv2cid(int64_t n_ctg, const uint64_t *ctg_off, const uint64_t *ctg_len, uint64_t tot_len, uint64_t st, uint64_t en,
+                        int64_t *cst, int *rev)
+{
+	int64_t s, len = (int64_t)(en - st), lo = 0, hi = n_ctg, mid;
+	if (en > tot_len * 2) return -3;
+	if (st < tot_len && tot_len < en) return -2;
+	*rev = st >= tot_len;
+	s = st < tot_len ? (int64_t)st : (int64_t)(tot_len * 2 - en);
+	while (lo < hi) {
+		mid = (lo + hi) / 2;
+		if ((int64_t)ctg_off[mid] <= s && s < (int64_t)(ctg_off[mid] + ctg_len[mid])) {
+			*cst = s - (int64_t)ctg_off[mid];
+			return s + len <= (int64_t)(ctg_off[mid] + ctg_len[mid]) ? mid : -1;
+		} else if (s < (int64_t)ctg_off[mid]) hi = mid;
+		else lo = mid + 1;
+	}
+	return -1;
+}
+
+/* mb_anchor (seed.c:268-339) without a methylation conversion: the read's intervals sorted and
+ * deduplicated, SA positions of every interval of size <= max_occ or of max_occ evenly stepped rows
+ * of a larger one (seed.c:304-315), each position of each interval sharing (x[0], size) mapped to its
+ * contig and strand (seed.c:176-210), the anchors radix-sorted by concatenated tpos, tpos made
+ * contig-local, and anchors contained in another with the same diagonal dropped (seed.c:140-166).
+ * u is edited in place.  Returns the number of anchors written to v (cap entries at most; -1 when
+ * more would be needed). */
+int64_t orc_anchor(const orc_bwt_t *b, int64_t n_ctg, const uint64_t *ctg_off, const uint64_t *ctg_len, orc_sai_t *u, int64_t n_u,
+                   int32_t qlen, int32_t max_occ, orc_anchor_t *v, int64_t cap)
+{
+	int64_t i, i0, j, k, n = 0, ng = 0;
+	const uint64_t tot = ctg_off[n_ctg - 1] + ctg_len[n_ctg - 1];
+	n_u = seed_sort_dedup(u, n_u);
+	for (i = 1, i0 = 0; i <= n_u; ++i) {
+		if (i < n_u && u[i].x[0] == u[i0].x[0] && u[i].size == u[i0].size) continue;
+		/* one group: intervals [i0, i) share the rows to locate */
+		{
+			const orc_sai_t *q = &u[i0];
+			int64_t jj, cnt = 0;
+			for (jj = 0; jj < (int64_t)q->size && cnt < max_occ; ++cnt) {
+				const uint64_t row = q->x[0] + jj, pos = orc_sa(b, row);
+				if ((int64_t)q->size <= max_occ) ++jj;
+				else {
+					int64_t step = ((int64_t)q->size - jj) / (max_occ - cnt);
+					jj += step < 1 ? 1 : step;
+				}
+				for (j = i0; j < i; ++j) {
+					const int32_t qs = (int32_t)(u[j].info >> 32), qe = (int32_t)u[j].info, len = qe - qs;
+					int64_t cst = 0, tid;
+					int rev = 0;
+					tid = intv2cid(n_ctg, ctg_off, ctg_len, tot, pos, pos + len, &cst, &rev);
+					if (tid < 0) continue;
+					if (n >= cap) return -1;
+					memset(&v[n], 0, sizeof(v[n]));
+					v[n].sid = (int32_t)(tid << 1 | rev);
+					v[n].len = len;
+					v[n].qpos = rev ? qlen - 1 - qs : qs + len - 1;
+					v[n].tpos = (int64_t)(ctg_off[tid] * 2 + ctg_len[tid] * rev) + cst + len - 1;
+					++n;
+				}
+			}
+		}
+		++ng;
+		i0 = i;
+	}
+	rs_sort(v, sizeof(*v), n, key_tpos);
+	for (i = 0; i < n; ++i) v[i].tpos -= (int64_t)(ctg_off[v[i].sid >> 1] * 2 + ctg_len[v[i].sid >> 1] * (v[i].sid & 1));
+	for (i = 1; i < n; ++i) {                              /* mb_anchor_dedup, seed.c:140-166 */
+		orc_anchor_t *ai = &v[i];
+		const int64_t tsi = ai->tpos + 1 - ai->len;
+		const int32_t qsi = ai->qpos + 1 - ai->len;
+		for (j = i - 1, k = 0; j >= 0 && k < 100; --j, ++k) {
+			orc_anchor_t *aj = &v[j];
+			int64_t tsj;
+			int32_t qsj;
+			if (aj->sid != ai->sid || aj->tpos < tsi) break;
+			tsj = aj->tpos + 1 - aj->len, qsj = aj->qpos + 1 - aj->len;
+			if (tsj >= tsi) {
+				if (tsj - tsi == qsj - qsi) aj->flt = 1;
+			} else if (ai->tpos == aj->tpos) {
+				if (ai->qpos == aj->qpos) ai->flt = 1;
+			}
+		}
+	}
+	for (i = j = 0; i < n; ++i)
+		if (!v[i].flt) v[j++] = v[i];
+	(void)ng;
+	return j;
 }
 
 /* main.c:164-200 (`minibwa bench`): checksum over n splitmix64-drawn queries; type 0 = rank2a,

@@ -64,6 +64,14 @@ int32_t orc_ungap(int32_t qlen, const uint8_t *qseq, int32_t tlen, const uint8_t
                   int32_t *max_i, int32_t *n_good, int32_t *n_kmer);
 int orc_matesw_align(const int32_t p[11], int32_t qlen, const uint8_t *qseq, int32_t tlen, const uint8_t *tseq, int32_t min_sc,
                      int32_t mt, orc_rescue_t *r, uint32_t *cigar, int cigar_cap);
+typedef struct {               /* mb_anchor_t (mbpriv.h:26-32) */
+	int32_t sid, len, qpos;
+	uint32_t flag:31, flt:1;
+	int64_t tpos;
+} orc_anchor_t;
+int64_t orc_anchor(const orc_bwt_t *b, int64_t n_ctg, const uint64_t *ctg_off, const uint64_t *ctg_len, orc_sai_t *u, int64_t n_u,
+                   int32_t qlen, int32_t max_occ, orc_anchor_t *v, int64_t cap);
+
 typedef struct {               /* the mb_hit_t / mb_extra_t fields mb_fix_cigar and mb_update_extra set */
 	int32_t qs, qe, rev, n_cigar;
 	int64_t ts, te;

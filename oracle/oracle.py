@@ -51,6 +51,14 @@ class OrcRescue(C.Structure):
                 ("qs", C.c_int32), ("qe", C.c_int32), ("ts", C.c_int32), ("te", C.c_int32), ("dp_max2", C.c_int32)]
 
 
+class Anchor(C.Structure):
+    """mb_anchor_t (mbpriv.h:26-32)."""
+    _fields_ = [("sid", C.c_int32), ("len", C.c_int32), ("qpos", C.c_int32), ("flag", C.c_uint32), ("tpos", C.c_int64)]
+
+
+ANCHOR_DTYPE = np.dtype([("sid", "<i4"), ("len", "<i4"), ("qpos", "<i4"), ("flag", "<u4"), ("tpos", "<i8")])
+
+
 class OrcExtra(C.Structure):
     _fields_ = [("qs", C.c_int32), ("qe", C.c_int32), ("rev", C.c_int32), ("n_cigar", C.c_int32), ("ts", C.c_int64),
                 ("te", C.c_int64), ("blen", C.c_int32), ("mlen", C.c_int32), ("n_ambi", C.c_int32), ("dp_max", C.c_int32)]
@@ -115,6 +123,9 @@ def _load_oracle():
                                    _u32p, C.c_int]
     L.orc_matesw_cand.argtypes = [_i32p, C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, _i32p, _i32p,
                                   C.POINTER(C.c_int32), C.POINTER(OrcRescue), _u32p, C.c_int]
+    L.orc_anchor.restype = C.c_int64
+    L.orc_anchor.argtypes = [C.POINTER(OrcBwt), C.c_int64, _u64p, _u64p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                             C.c_void_p, C.c_int64]
     L.orc_update_extra.argtypes = [C.POINTER(OrcExtra), _u32p, _u8p, _u8p, _i8p, C.c_int8, C.c_int8, C.c_int]
     L.orc_ll.restype = C.c_int
     L.orc_ll.argtypes = [C.c_int, C.c_int, _u8p, C.c_int, _i8p, C.c_int, _u8p, C.c_int, C.c_int, C.c_int,
@@ -466,6 +477,11 @@ def ref_pe():
         L.mbx_ungap.argtypes = [C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, _i32p]
         L.mbx_matesw_align.argtypes = [_i32p, C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, C.POINTER(MbxHit),
                                        _u32p, C.c_int32, _i32p, _u32p, C.c_int32]
+        L.mb_idx_load.restype = C.c_void_p
+        L.mb_idx_load.argtypes = [C.c_char_p, C.c_int]
+        L.mb_idx_destroy.argtypes = [C.c_void_p]
+        L.mbx_anchor.restype = C.c_int64
+        L.mbx_anchor.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]
         L.mbx_update_extra.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32, _u32p, _u8p, _u8p,
                                        _i8p, C.c_int8, C.c_int8, C.c_int32, C.POINTER(MbxHit), _u32p, C.c_int32]
         L.mbx_finish.argtypes = [_i32p, C.c_int32, _u8p, _u8p, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32,
@@ -538,3 +554,48 @@ def ref_update_extra(hit, cigar, qseq, tseq, mat, q, e, log_gap):
     d = {f: int(getattr(h, f)) for f in ("qs", "qe", "ts", "te", "n_cigar", "blen", "mlen", "dp_max")}
     d["n_ambi"] = int(h.n_ambi)
     return d, [int(x) for x in out[:h.n_cigar]]
+
+
+def _sai_array(intervals):
+    a = np.ascontiguousarray(np.asarray(intervals, np.uint64).reshape(-1, 4))
+    return a
+
+
+def orc_anchor(index, ctg_len, intervals, qlen, max_occ=250):
+    """mb_anchor restated (no methylation) on one read's intervals ((n,4) uint64: x0, x1, size, info) with
+    an OracleIndex -> ANCHOR_DTYPE array."""
+    u = _sai_array(intervals).copy()
+    ctg_len = np.ascontiguousarray(ctg_len, np.uint64)
+    ctg_off = np.zeros(len(ctg_len), np.uint64)
+    if len(ctg_len) > 1:
+        np.cumsum(ctg_len[:-1], out=ctg_off[1:])
+    cap = int(np.minimum(u[:, 2], max_occ).sum()) + 1 if len(u) else 1
+    out = np.zeros(cap, ANCHOR_DTYPE)
+    n = orc().orc_anchor(index.b, len(ctg_len), ctg_off, ctg_len, u.ctypes.data_as(C.c_void_p), len(u), qlen, max_occ,
+                         out.ctypes.data_as(C.c_void_p), cap)
+    if n < 0:
+        raise RuntimeError("orc_anchor: capacity")
+    return out[:n]
+
+
+class RefMapIndex:
+    """A prefix loaded by the reference's mb_idx_load (minibwa.h), for mb_anchor."""
+
+    def __init__(self, prefix):
+        self.L = ref_pe()
+        self.p = self.L.mb_idx_load(prefix.encode(), 0)
+        if not self.p:
+            raise IOError(prefix)
+
+    def anchor(self, intervals, qlen, max_occ=250):
+        u = _sai_array(intervals)
+        cap = int(np.minimum(u[:, 2], max_occ).sum()) + 1 if len(u) else 1
+        out = np.zeros(cap, ANCHOR_DTYPE)
+        n = self.L.mbx_anchor(C.c_void_p(self.p), u.ctypes.data_as(C.c_void_p), len(u), qlen, max_occ,
+                              out.ctypes.data_as(C.c_void_p), cap)
+        return out[:min(n, cap)]
+
+    def close(self):
+        if self.p:
+            self.L.mb_idx_destroy(C.c_void_p(self.p))
+            self.p = None

@@ -94,3 +94,20 @@ void mbx_update_extra(int32_t qs, int32_t qe, int64_t ts, int64_t te, int32_t re
 	mbx_out(&h, r, cigar, cap);
 	free(h.p);
 }
+
+/* the reference's mb_anchor (seed.c:268-339) on one read's intervals, no methylation conversion;
+ * returns the number of anchors (at most cap copied to out) */
+int64_t mbx_anchor(const mb_idx_t *idx, const mb_sai_t *u0, int64_t n_u, int32_t qlen, int32_t max_occ, mb_anchor_t *out, int64_t cap)
+{
+	mb_sai_v u;
+	mb_anchor_v v = {0, 0, 0};
+	int64_t n;
+	u.n = u.m = (size_t)n_u;
+	u.a = (mb_sai_t*)malloc(sizeof(mb_sai_t) * (n_u + 1));
+	memcpy(u.a, u0, sizeof(mb_sai_t) * n_u);
+	mb_anchor(0, idx, &u, 19, qlen, 0, L2B_METH_NONE, max_occ, &v);
+	n = v.n;
+	memcpy(out, v.a, sizeof(mb_anchor_t) * (size_t)(n < cap ? n : cap));
+	free(u.a); free(v.a);
+	return n;
+}

@@ -263,3 +263,54 @@ def test_update_extra_vs_reference(log_gap):
         assert a == b and ac == bc, (it, cig, a, b, ac, bc)
         n_fixed += ac != list(cig)
     assert n_fixed > 200
+
+
+@pytest.fixture(scope="module")
+def repeat_index(tmp_path_factory):
+    """A repeat-rich 4-contig reference (30% of it diverged copies) indexed by the reference CLI: reads
+    get intervals of large size, so anchor lists run past 64 entries and klib's MSD radix pass runs."""
+    from minibwa_amd import synth
+    if not O.have_ref_pe():
+        pytest.skip("oracle/_ref/libmbref_pe.so not built")
+    d = tmp_path_factory.mktemp("rep")
+    rng = np.random.default_rng(5)
+    ref = synth.reference(300_000, seed=5, rep_frac=0.1, rep_min=300, rep_max=3000, rep_div=0.01)
+    elem = rng.integers(0, 4, 400).astype(np.uint8)          # a 400 bp element in 200 near-identical copies
+    parts = []
+    for c in range(200):
+        cp = elem.copy()
+        cp[rng.random(400) < 0.003] = rng.integers(0, 4)
+        parts += [ref[c * 1500:(c + 1) * 1500], cp]
+    ref = np.concatenate(parts).astype(np.uint8)
+    synth.write_fasta(str(d / "rep.fa"), ref, n_ctg=4, prefix="rc")
+    import subprocess
+    subprocess.run([O.REF_CLI, "index", str(d / "rep.fa"), str(d / "rep")], check=True, capture_output=True)
+    per = (len(ref) + 3) // 4
+    ctg_len = np.array([min(per, len(ref) - c * per) for c in range(4)], np.uint64)
+    return str(d / "rep"), ref, ctg_len
+
+
+def test_anchor_vs_reference(repeat_index):
+    """The oracle's mb_anchor against the reference's own (seed.c:268-339) on the reference's seeding
+    of short and long reads: anchors, their order (klib's radix sort ties) and the dedup."""
+    from minibwa_amd import synth
+    prefix, ref, ctg_len = repeat_index
+    ri = O.RefIndex(prefix + ".mbw", pre_len=10)
+    oi = O.OracleIndex(prefix + ".mbw", pre_len=10)
+    rm = O.RefMapIndex(prefix)
+    rng = np.random.default_rng(3)
+    r1, r2 = synth.read_pairs(ref, 300, seed=9)
+    reads = [r for r in r1] + [r for r in r2]
+    lr, lo, ll = synth.single_reads(ref, 20, seed=4, lmin=2000, lmax=8000, sub=0.04)
+    reads += [lr[lo[i]:lo[i] + ll[i]] for i in range(len(ll))]
+    ivs = ri.seed_batch(reads, 19, 10)
+    n_big = n_many = 0
+    for i, r in enumerate(reads):
+        for max_occ in (250, 20):
+            a = O.orc_anchor(oi, ctg_len, ivs[i], len(r), max_occ)
+            b = rm.anchor(ivs[i], len(r), max_occ)
+            assert len(a) == len(b) and (a == b).all(), (i, max_occ, len(a), len(b))
+            n_big += len(a) > 64
+        n_many += len(ivs[i]) > 64
+    rm.close()
+    assert n_big > 20 and n_many > 5
