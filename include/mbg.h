@@ -231,6 +231,25 @@ int mbg_update_extra_batch(mbg_ctx_t *ctx, int64_t n, mbg_extra_job_t *jobs, con
                            const uint8_t *tseq, int64_t tbytes, const int8_t *mat, int8_t q, int8_t e, int32_t log_gap,
                            uint32_t *cigar, int64_t cigar_cap);
 
+/* mb_anchor_t (mbpriv.h:26-32): sid = contig<<1 | reverse strand; len bases ending at query position
+ * qpos (on the anchor's strand) and at contig position tpos; flag is 0 on output (its top bit is
+ * mb_anchor_t::flt). */
+typedef struct { int32_t sid, len, qpos; uint32_t flag; int64_t tpos; } mbg_anchor_t;
+
+/* Anchors of n_reads reads from their SMEM intervals: mb_anchor (seed.c:268-339; decl mbpriv.h:58)
+ * for reads without a methylation conversion (mt L2B_METH_NONE): mb_seed_sort_dedup
+ * (seed.c:109-138), the SA of every row of an interval group of size <= max_occ or of max_occ evenly
+ * stepped rows (seed.c:296-318), the contig and strand of each hit (l2b_intv2cid, l2bit.c:23-34),
+ * klib's radix sort by concatenated position (seed.c:331) and mb_anchor_dedup (seed.c:140-166).
+ * Read r's intervals are intv[intv_off[r] .. intv_off[r+1]) (intv_off[0] = 0); they come back sorted
+ * and deduplicated as the reference leaves its mb_sai_v, n_intv[r] of them (n_intv may be null).  Its
+ * anchors are written to anc[anc_off[r] ...] (anc_off[0] = 0), n_anc[r] of them, in the reference's
+ * order; anc_off[r+1] - anc_off[r] must be at least the sum of min(size, max_occ) over read r's
+ * intervals (the reference's preallocation, seed.c:292-293).  The FM-index and the l2b must be
+ * resident.  Results equal the reference's. */
+int mbg_anchor_batch(mbg_ctx_t *ctx, int32_t n_reads, mbg_sai_t *intv, const int64_t *intv_off, const int32_t *qlen,
+                     int32_t max_occ, mbg_anchor_t *anc, const int64_t *anc_off, int64_t *n_anc, int64_t *n_intv);
+
 /* Target extraction l2b_getseq (l2bit.c:52-77) for n windows [st[i],en[i]) of contig tid[i]
  * (clipped to the contig, N runs restored as 4), written at out + out_off[i].  len[i] receives
  * the clipped length (or -1 for a bad tid). */
@@ -251,6 +270,13 @@ int mbg_seed_batch_dev_async(mbg_ctx_t *ctx, void *stream, int32_t n_seq, const 
                              const uint8_t *seq, int32_t min_len, int32_t max_sub_occ,
                              mbg_seed_rec_t *out, int64_t cap, int64_t *n_out_dev);
 int mbg_sa_batch_dev(mbg_ctx_t *ctx, void *stream, int64_t n, uint64_t *x);
+/* mbg_anchor_batch over buffers in HBM: intv_off, anc_off (n_reads+1 each, host-computed as there,
+ * anc_off[n_reads] = n_slots), qlen, n_anc and n_intv (n_reads each, both required) and rows, a
+ * workspace of n_slots uint64.  Every row x[0] + size - 1 of every interval must be a row of the
+ * resident index.  A read whose slots do not suffice gets n_anc -1.  Asynchronous on the stream. */
+int mbg_anchor_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_reads, int64_t n_slots, mbg_sai_t *intv,
+                         const int64_t *intv_off, const int32_t *qlen, int32_t max_occ, uint64_t *rows, mbg_anchor_t *anc,
+                         const int64_t *anc_off, int64_t *n_anc, int64_t *n_intv);
 /* mbg_ll_batch over jobs, sequences and results in HBM.  Every job is run with the registers of
  * max_qlen (<= 1024); rows_total bounds the sum of the jobs' tlen (the per-row maxima lists).  A job
  * outside the supported range (qlen > max_qlen, an i16 job that could saturate) gets score -1 and

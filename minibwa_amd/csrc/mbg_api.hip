@@ -46,6 +46,7 @@ struct mbg_ctx_s {
 	DBuf p_recs, p_rows, p_rowrec, p_first, p_best, p_jobs, p_meta, p_qb, p_tb, p_res, p_cig, p_ctr;
 	DBuf s_slots1, s_slots2, s_cnt, s_off, s_spill, s_tmp, r_cnt, r_off, s_pack, s_qflag;
 	DBuf m_buf[12];         // mate rescue
+	DBuf a_buf[8];          // anchors
 	int64_t cap_task = 0, cap_spill = 0;
 	int64_t cap_rec = 0, cap_rows = 0;
 	hipEvent_t ev[8] = {};
@@ -147,6 +148,7 @@ void mbg_destroy(mbg_ctx_t *c)
 	                &c->p_res, &c->p_cig, &c->p_ctr, &c->s_slots1, &c->s_slots2, &c->s_cnt, &c->s_off, &c->s_spill, &c->s_tmp,
 	                &c->r_cnt, &c->r_off, &c->s_pack, &c->s_qflag}) dfree(*b);
 	for (auto &b : c->m_buf) dfree(b);
+	for (auto &b : c->a_buf) dfree(b);
 	for (hipEvent_t &e : c->ev) if (e) hipEventDestroy(e);
 	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
 	if (c->stream) hipStreamDestroy(c->stream);
@@ -791,6 +793,77 @@ int mbg_update_extra_batch(mbg_ctx_t *c, int64_t n, mbg_extra_job_t *jobs, const
 	HIPCHK(hipMemcpyAsync(jobs, c->b_in[4].p, (size_t)n * sizeof(ExtraJob), hipMemcpyDeviceToHost, c->stream));
 	if (cig_end > 0) HIPCHK(hipMemcpyAsync(cigar, c->b_out[1].p, (size_t)cig_end * 4, hipMemcpyDeviceToHost, c->stream));
 	HIPCHK(hipStreamSynchronize(c->stream));
+	return MBG_OK;
+}
+
+// ---- anchors ----------------------------------------------------------------------------------
+
+static int anchor_dev_locked(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, int64_t n_slots, Sai *intv, const int64_t *uoff,
+                             const int32_t *qlen, int32_t max_occ, uint64_t *rows, AncRec *anc, const int64_t *aoff, int64_t *nanc,
+                             int64_t *nu)
+{
+	AncArgs a{};
+	a.n_reads = n_reads, a.max_occ = max_occ, a.u = intv, a.uoff = uoff, a.qlen = qlen, a.aoff = aoff;
+	a.rows = rows, a.anc = anc, a.nu = nu, a.nanc = nanc, a.lb = c->lb;
+	HIPCHK(launch_anc_rows(a, st));
+	TRY(sa_dev_locked(c, st, n_slots, rows));
+	HIPCHK(launch_anc_build(a, st));
+	return MBG_OK;
+}
+
+int mbg_anchor_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, int64_t n_slots, mbg_sai_t *intv, const int64_t *intv_off,
+                         const int32_t *qlen, int32_t max_occ, uint64_t *rows, mbg_anchor_t *anc, const int64_t *anc_off, int64_t *n_anc,
+                         int64_t *n_intv)
+{
+	if (c == nullptr || n_reads < 0 || n_slots < 0 || max_occ < 0) return MBG_EARG;
+	if (n_reads > 0 && (!intv_off || !qlen || !anc_off || !n_anc || !n_intv || (n_slots > 0 && (!rows || !anc)))) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt || !c->has_l2b) return MBG_ENOIDX;
+	if (n_reads == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	return anchor_dev_locked(c, stream ? (hipStream_t)stream : c->stream, n_reads, n_slots, (Sai*)intv, intv_off, qlen, max_occ,
+	                         rows, (AncRec*)anc, anc_off, n_anc, n_intv);
+}
+
+int mbg_anchor_batch(mbg_ctx_t *c, int32_t n_reads, mbg_sai_t *intv, const int64_t *intv_off, const int32_t *qlen, int32_t max_occ,
+                     mbg_anchor_t *anc, const int64_t *anc_off, int64_t *n_anc, int64_t *n_intv)
+{
+	if (c == nullptr || n_reads < 0 || max_occ < 0) return MBG_EARG;
+	if (n_reads > 0 && (!intv_off || !qlen || !anc_off || !n_anc || intv_off[0] != 0 || anc_off[0] != 0)) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt || !c->has_l2b) return MBG_ENOIDX;
+	if (n_reads == 0) return MBG_OK;
+	const int64_t n_u = intv_off[n_reads], n_slots = anc_off[n_reads];
+	if ((n_u > 0 && !intv) || (n_slots > 0 && !anc)) return MBG_EARG;
+	for (int32_t r = 0; r < n_reads; ++r) {          // offsets, rows and the slots the reference preallocates
+		if (intv_off[r + 1] < intv_off[r] || anc_off[r + 1] < anc_off[r] || qlen[r] < 0) return MBG_EARG;
+		int64_t need = 0;
+		for (int64_t i = intv_off[r]; i < intv_off[r + 1]; ++i) {
+			const mbg_sai_t &u = intv[i];
+			if (u.size > 0 && (u.x[0] > c->ix.seq_len || u.size - 1 > c->ix.seq_len - u.x[0])) return MBG_EARG;
+			need += u.size < (uint64_t)max_occ ? (int64_t)u.size : max_occ;
+		}
+		if (need > anc_off[r + 1] - anc_off[r]) return MBG_EARG;
+	}
+	HIPCHK(hipSetDevice(c->device));
+	hipStream_t st = c->stream;
+	DBuf *b = c->a_buf;
+	TRY(dgrow(b[0], (size_t)n_u * sizeof(Sai))); TRY(dgrow(b[1], (size_t)(n_reads + 1) * 8)); TRY(dgrow(b[2], (size_t)n_reads * 4));
+	TRY(dgrow(b[3], (size_t)(n_reads + 1) * 8)); TRY(dgrow(b[4], (size_t)n_slots * 8)); TRY(dgrow(b[5], (size_t)n_slots * sizeof(AncRec)));
+	TRY(dgrow(b[6], (size_t)n_reads * 8)); TRY(dgrow(b[7], (size_t)n_reads * 8));
+	if (n_u > 0) HIPCHK(hipMemcpyAsync(b[0].p, intv, (size_t)n_u * sizeof(Sai), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b[1].p, intv_off, (size_t)(n_reads + 1) * 8, hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b[2].p, qlen, (size_t)n_reads * 4, hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b[3].p, anc_off, (size_t)(n_reads + 1) * 8, hipMemcpyHostToDevice, st));
+	TRY(anchor_dev_locked(c, st, n_reads, n_slots, (Sai*)b[0].p, (const int64_t*)b[1].p, (const int32_t*)b[2].p, max_occ,
+	                      (uint64_t*)b[4].p, (AncRec*)b[5].p, (const int64_t*)b[3].p, (int64_t*)b[6].p, (int64_t*)b[7].p));
+	HIPCHK(hipMemcpyAsync(n_anc, b[6].p, (size_t)n_reads * 8, hipMemcpyDeviceToHost, st));
+	if (n_intv) HIPCHK(hipMemcpyAsync(n_intv, b[7].p, (size_t)n_reads * 8, hipMemcpyDeviceToHost, st));
+	if (n_u > 0) HIPCHK(hipMemcpyAsync(intv, b[0].p, (size_t)n_u * sizeof(Sai), hipMemcpyDeviceToHost, st));
+	if (n_slots > 0) HIPCHK(hipMemcpyAsync(anc, b[5].p, (size_t)n_slots * sizeof(AncRec), hipMemcpyDeviceToHost, st));
+	HIPCHK(hipStreamSynchronize(st));
+	for (int32_t r = 0; r < n_reads; ++r)
+		if (n_anc[r] < 0) return MBG_EOVERFLOW;
 	return MBG_OK;
 }
 

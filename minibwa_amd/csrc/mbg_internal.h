@@ -167,6 +167,23 @@ struct MsArgs {
 	int32_t tab_size;       // k-mer table slots of the vote (a power of two >= 2 * the longest read)
 	int32_t mt, a, b_mm, q, e, bw, zdrop, end_bonus, min_dp_max;
 };
+// mb_anchor (kernels in mbg_anchor.hip); AncRec is layout-identical to mbg_anchor_t / mb_anchor_t
+struct AncRec { int32_t sid, len, qpos; uint32_t flag; int64_t tpos; };
+static_assert(sizeof(AncRec) == 24, "AncRec must match mbg_anchor_t");
+struct AncArgs {
+	int32_t n_reads, max_occ;
+	Sai *u;                  // read r's intervals at u[uoff[r] .. uoff[r+1]), sorted and deduplicated in place
+	const int64_t *uoff;
+	const int32_t *qlen;
+	const int64_t *aoff;     // read r's row and anchor slots: [aoff[r], aoff[r+1])
+	uint64_t *rows;          // rows to locate, then their SA positions
+	AncRec *anc;
+	int64_t *nu, *nanc;      // per read: intervals after dedup, anchors; -1 when the slots do not suffice
+	L2b lb;
+};
+hipError_t launch_anc_rows(const AncArgs &a, hipStream_t st);
+hipError_t launch_anc_build(const AncArgs &a, hipStream_t st);
+
 constexpr int kMsMaxWindow = 16384;    // rescue window bases: 16-bit vote counters, 32 KB of LDS
 hipError_t launch_ungap(const MsArgs &a, int max_wlen, hipStream_t st);
 hipError_t launch_rescue_dp(const MsArgs &a, hipStream_t st);

@@ -174,6 +174,8 @@ def lib():
                                    C.c_int32, _P]
         L.mbg_matesw_batch.argtypes = [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(MateswOpt), _P, _P, C.c_int64,
                                        C.POINTER(MateswStats)]
+        L.mbg_anchor_batch.argtypes = [_P, C.c_int32, _P, _P, _P, C.c_int32, _P, _P, _P, _P]
+        L.mbg_anchor_batch_dev.argtypes = [_P, _P, C.c_int32, C.c_int64, _P, _P, _P, C.c_int32, _P, _P, _P, _P, _P]
         L.mbg_update_extra_batch.argtypes = [_P, C.c_int64, _P, _P, C.c_int64, _P, C.c_int64, _P, C.c_int8, C.c_int8,
                                              C.c_int32, _P, C.c_int64]
         L.mbg_ll_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, C.c_int32, C.c_int32, C.c_int32, _P,
@@ -187,7 +189,8 @@ EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mb
             "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
             "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
             "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench", "mbg_idx_attach", "mbg_ll_batch",
-            "mbg_ll_batch_dev", "mbg_matesw_batch", "mbg_update_extra_batch")
+            "mbg_ll_batch_dev", "mbg_matesw_batch", "mbg_update_extra_batch",
+            "mbg_anchor_batch", "mbg_anchor_batch_dev")
 LL_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"), ("size", "<i4"),
                          ("pad", "<i4")], align=True)
 LL_RES_DTYPE = np.dtype([(f, "<i4") for f in ("score", "te", "qe", "score2", "te2")])
@@ -208,6 +211,17 @@ EXTRA_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qs", "<i4"), ("q
                             ("rev", "<i4"), ("n_cigar", "<i4"), ("cigar_off", "<i8"), ("blen", "<i4"), ("mlen", "<i4"),
                             ("n_ambi", "<i4"), ("dp_max", "<i4")])
 assert EXTRA_JOB_DTYPE.itemsize == 72
+
+
+# mbg_anchor_t = mb_anchor_t (mbpriv.h:26-32)
+ANCHOR_DTYPE = np.dtype([("sid", "<i4"), ("len", "<i4"), ("qpos", "<i4"), ("flag", "<u4"), ("tpos", "<i8")])
+assert ANCHOR_DTYPE.itemsize == 24
+
+
+def anchor_slots(intervals, max_occ):
+    """Per read, the anchor slots mb_anchor preallocates (seed.c:292-293): the sum of min(size, max_occ)."""
+    m = max(int(max_occ), 0)
+    return np.array([int(np.minimum(np.asarray(u, np.uint64).reshape(-1, 4)[:, 2], m).sum()) for u in intervals], np.int64)
 
 
 class MateswStats(C.Structure):
@@ -276,7 +290,7 @@ class DeviceArray:
 
     def free(self):
         if self.ptr and getattr(self.ctx, "h", None):
-            check(self.ctx.L.mbg_dev_free(self.ctx.h, C.c_void_p(self.ptr)), "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench")
+            check(self.ctx.L.mbg_dev_free(self.ctx.h, C.c_void_p(self.ptr)), "mbg_dev_free")
         self.ptr = None
 
 
@@ -526,6 +540,26 @@ class Context:
         check(self.L.mbg_update_extra_batch(self.h, n, _ptr(jobs), _ptr(qb), len(qbuf), _ptr(tb), len(tbuf), _ptr(mat), q, e,
                                             log_gap, _ptr(cig), len(cig)), "mbg_update_extra_batch")
         return jobs, [cig[coff[i]:coff[i] + jobs["n_cigar"][i]].tolist() for i in range(n)]
+
+    def anchor(self, intervals, qlens, max_occ=250):
+        """mb_anchor (seed.c:268-339, no methylation conversion) of each read's SMEM intervals ((n,4)
+        uint64: x0, x1, size, info) -> (per read an ANCHOR_DTYPE array, per read its intervals sorted and
+        deduplicated as the reference leaves them)."""
+        n = len(intervals)
+        us = [np.asarray(u, np.uint64).reshape(-1, 4) for u in intervals]
+        uoff = np.zeros(n + 1, np.int64)
+        np.cumsum([len(u) for u in us], out=uoff[1:])
+        u = np.ascontiguousarray(np.concatenate(us) if n else np.zeros((0, 4), np.uint64), np.uint64)
+        aoff = np.zeros(n + 1, np.int64)
+        np.cumsum(anchor_slots(us, max_occ), out=aoff[1:])
+        anc = np.zeros(max(int(aoff[-1]), 1), ANCHOR_DTYPE)
+        nanc = np.zeros(max(n, 1), np.int64)
+        nu = np.zeros(max(n, 1), np.int64)
+        ql = np.ascontiguousarray(qlens, np.int32)
+        check(self.L.mbg_anchor_batch(self.h, n, _ptr(u), _ptr(uoff), _ptr(ql), max_occ, _ptr(anc), _ptr(aoff), _ptr(nanc),
+                                      _ptr(nu)), "mbg_anchor_batch")
+        return ([anc[aoff[i]:aoff[i] + nanc[i]].copy() for i in range(n)],
+                [u[uoff[i]:uoff[i] + nu[i]].copy() for i in range(n)])
 
     def matesw(self, reads, tid, ts, te, min_sc, opt: MateswOpt, stats: "MateswStats | None" = None):
         """Mate rescue of candidate windows [ts, te) on contigs tid for reads (one per candidate, as they

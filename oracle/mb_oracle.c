@@ -1269,7 +1269,7 @@ int64_t orc_anchor(const orc_bwt_t *b, int64_t n_ctg, const uint64_t *ctg_off, c
 				const uint64_t row = q->x[0] + jj, pos = orc_sa(b, row);
 				if ((int64_t)q->size <= max_occ) ++jj;
 				else {
-					int64_t step = ((int64_t)q->size - jj) / (max_occ - cnt);
+					int32_t step = (int32_t)((q->size - (uint64_t)jj) / (uint64_t)(max_occ - cnt));   /* int32 in seed.c:310 */
 					jj += step < 1 ? 1 : step;
 				}
 				for (j = i0; j < i; ++j) {

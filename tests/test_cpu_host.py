@@ -79,6 +79,7 @@ def test_header_layouts_match_python(tmp_path):
         "mbg_ll_result_t": ["score", "te", "qe", "score2", "te2"],
         "mbg_matesw_job_t": ["qoff", "qlen", "tid", "ts", "te", "min_sc", "pad", "cigar_off"],
         "mbg_matesw_opt_t": ["a", "mt"],
+        "mbg_anchor_t": ["sid", "len", "qpos", "flag", "tpos"],
         "mbg_extra_job_t": ["qoff", "toff", "qs", "qe", "ts", "te", "rev", "n_cigar", "cigar_off", "blen", "mlen", "n_ambi",
                             "dp_max"],
         "mbg_matesw_result_t": ["max_ug", "max_i", "n_good", "n_kmer", "ts2", "te2", "aligned", "rescued", "ll", "ez", "qs",
@@ -116,6 +117,7 @@ def test_header_layouts_match_python(tmp_path):
     np_layout("mbg_matesw_job_t", mbg.MATESW_JOB_DTYPE)
     np_layout("mbg_extra_job_t", mbg.EXTRA_JOB_DTYPE)
     np_layout("mbg_matesw_result_t", mbg.MATESW_RES_DTYPE)
+    np_layout("mbg_anchor_t", mbg.ANCHOR_DTYPE)
     assert got[("mbg_matesw_opt_t", "sizeof")] == C.sizeof(mbg.MateswOpt)
     sr = mbg.SEED_REC_DTYPE
     assert got[("mbg_seed_rec_t", "sizeof")] == sr.itemsize

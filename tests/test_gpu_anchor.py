@@ -1,0 +1,126 @@
+"""GPU parity of mbg_anchor_batch against the oracle's mb_anchor (seed.c:268-339), which the CPU tests pin
+against the reference's own function: anchors, their order (klib's radix sort leaves equal positions in
+an order of its own) and the dedup, on a repeat-rich 4-contig index (lists of more than 64 intervals and
+of more than 64 anchors, intervals of more than max_occ rows) and on chrM with its golden reads.  The
+intervals come back sorted and deduplicated; their order is the closed form of mb_seed_sort_dedup:
+by x[0], then size descending, then info.
+"""
+import numpy as np
+import pytest
+
+from minibwa_amd import mbg
+
+O = pytest.importorskip("oracle.oracle")
+
+pytestmark = pytest.mark.gpu
+
+
+def repeat_reference(seed=5):
+    """300 kb with 10% diverged segment copies, and a 400 bp element in 320 near-identical copies."""
+    from minibwa_amd import synth
+    rng = np.random.default_rng(seed)
+    ref = synth.reference(300_000, seed=seed, rep_frac=0.1, rep_min=300, rep_max=3000, rep_div=0.01)
+    elem = rng.integers(0, 4, 400).astype(np.uint8)
+    parts = []
+    for c in range(320):
+        cp = elem.copy()
+        cp[rng.random(400) < 0.001] = rng.integers(0, 4)
+        parts += [ref[c * 900:(c + 1) * 900], cp]
+    return np.concatenate(parts).astype(np.uint8)
+
+
+@pytest.fixture(scope="module")
+def rep(tmp_path_factory):
+    from minibwa_amd import index
+    ref = repeat_reference()
+    pre = str(tmp_path_factory.mktemp("anc") / "rep")
+    parts = index.build(ref, pre, n_ctg=4, ctg_prefix="rc", device="cpu")
+    ctx = mbg.Context(0)
+    ctx.idx_load(pre, 10)
+    oi = O.OracleIndex(pre + ".mbw", pre_len=10)
+    yield ctx, oi, ref, parts["ctg_len"]
+    oi.close()
+    ctx.close()
+
+
+def _reads(ref):
+    from minibwa_amd import synth
+    r1, r2 = synth.read_pairs(ref, 400, seed=9)
+    reads = [r for r in r1] + [r for r in r2]
+    lr, lo, ll = synth.single_reads(ref, 24, seed=4, lmin=2000, lmax=8000, sub=0.04)
+    return reads + [lr[lo[i]:lo[i] + ll[i]] for i in range(len(ll))]
+
+
+def _sort_dedup_closed_form(u):
+    u = np.asarray(u, np.uint64).reshape(-1, 4)
+    if len(u) == 0:
+        return u
+    o = np.lexsort((u[:, 3], np.iinfo(np.uint64).max - u[:, 2], u[:, 0]))
+    u = u[o]
+    keep = np.ones(len(u), bool)
+    keep[1:] = (u[1:, 0] != u[:-1, 0]) | (u[1:, 2] != u[:-1, 2]) | (u[1:, 3] != u[:-1, 3])
+    return u[keep]
+
+
+def _check(ctx, oi, ctg_len, reads, ivs, max_occ):
+    anc, us = ctx.anchor(ivs, [len(r) for r in reads], max_occ)
+    n_big = 0
+    for i, r in enumerate(reads):
+        want = O.orc_anchor(oi, ctg_len, ivs[i], len(r), max_occ)
+        got = anc[i]
+        assert len(got) == len(want) and (got == want).all(), (i, max_occ, len(got), len(want))
+        assert (us[i] == _sort_dedup_closed_form(ivs[i])).all(), (i, max_occ)
+        n_big += len(got) > 64
+    return n_big
+
+
+@pytest.mark.parametrize("max_occ", [250, 20, 3, 1])
+def test_anchor_repeat_index(rep, max_occ):
+    ctx, oi, ref, ctg_len = rep
+    reads = _reads(ref)
+    ivs = ctx.seed_intv_batch(reads, 19, 10)
+    assert sum(len(u) > 64 for u in ivs) > 5
+    assert sum(int((np.asarray(u)[:, 2] > max_occ).sum()) for u in ivs if len(u)) > 50
+    n_big = _check(ctx, oi, ctg_len, reads, ivs, max_occ)
+    if max_occ >= 20:
+        assert n_big > 20
+
+
+def test_anchor_chrm(gpu_ctx, chrm_prefix, chrm_reads):
+    gpu_ctx.idx_load(chrm_prefix, 10)
+    oi = O.OracleIndex(chrm_prefix + ".mbw", pre_len=10)
+    reads = chrm_reads[:2000]
+    ivs = gpu_ctx.seed_intv_batch(reads, 19, 10)
+    ctg_len = np.array([gpu_ctx.info()["tot_len"]], np.uint64)
+    _check(gpu_ctx, oi, ctg_len, reads, ivs, 250)
+    oi.close()
+
+
+def test_anchor_edges(rep):
+    """Reads without intervals, an empty batch, max_occ 0, shuffled and duplicated interval lists."""
+    ctx, oi, ref, ctg_len = rep
+    reads = _reads(ref)[:60]
+    ivs = ctx.seed_intv_batch(reads, 19, 10)
+    rng = np.random.default_rng(1)
+    mixed = []
+    for u in ivs:
+        u = np.asarray(u, np.uint64).reshape(-1, 4)
+        if len(u):
+            u = np.concatenate([u, u[rng.integers(0, len(u), len(u) // 2)]])[rng.permutation(len(u) + len(u) // 2)]
+        mixed.append(u)
+    mixed[3] = np.zeros((0, 4), np.uint64)
+    _check(ctx, oi, ctg_len, reads, mixed, 250)
+    anc, us = ctx.anchor(mixed, [len(r) for r in reads], 0)
+    assert all(len(a) == 0 for a in anc)
+    anc, us = ctx.anchor([], [], 250)
+    assert anc == [] and us == []
+
+
+def test_anchor_errors(rep):
+    ctx, oi, ref, ctg_len = rep
+    seq_len = ctx.info()["seq_len"]
+    bad = np.array([[seq_len - 2, 0, 5, (3 << 32) | 40]], np.uint64)        # rows past the index
+    with pytest.raises(mbg.MbgError, match="invalid"):
+        ctx.anchor([bad], [100], 250)
+    with pytest.raises(mbg.MbgError, match="invalid"):
+        ctx.anchor([np.zeros((0, 4), np.uint64)], [100], -1)
