@@ -3,9 +3,9 @@
  *
  * Linked into the unmodified minibwa objects with
  *   -Wl,--wrap=mb_seed_intv_batch -Wl,--wrap=mb_bwt_sa_batch -Wl,--wrap=ksw_extd2_sse -Wl,--wrap=ksw_extz2_sse
- *   -Wl,--wrap=ksw_ll_qinit -Wl,--wrap=ksw_ll_u8_core -Wl,--wrap=ksw_ll_i16_core
+ *   -Wl,--wrap=ksw_ll_qinit -Wl,--wrap=ksw_ll_u8_core -Wl,--wrap=ksw_ll_i16_core -Wl,--wrap=mb_anchor
  * every call site of those functions (map-main.c:75, map-algo.c:733, seed.c:180, align.c:374-376,
- * pe.c:267-272,287) reaches the __wrap_ definitions below, which run the work on the GPU through the
+ * pe.c:267-272,287, map-algo.c:610) reaches the __wrap_ definitions below, which run the work on the GPU through the
  * C ABI of include/mbg.h and hand results back in the reference's own containers:
  *
  *   mb_seed_intv_batch (mbpriv.h:57, seed.c:56-103)   -> mbg_seed_batch
@@ -14,16 +14,21 @@
  *   ksw_extz2_sse      (ksw2.h:73-74)                  -> mbg_extz2_batch (one job)
  *   ksw_ll_qinit + ksw_ll_u8_core / ksw_ll_i16_core (ksw2.h:101-103)
  *                                                      -> mbg_ll_batch (one job)
+ *   mb_anchor          (mbpriv.h:58, seed.c:268-339)   -> mbg_anchor_batch (one read; reads with a
+ *                                                         methylation conversion keep the reference's
+ *                                                         own, which then locates through mbg_sa_batch)
  *
  * MBG_WRAP_STATS=1 prints the number of calls each hook took, at exit, to stderr.
  * The index is uploaded once, on the first hooked call, from the mb_bwt_t the caller passes (the
  * k-mer cache is recomputed on the device with the caller's pre_len).  MBG_DEVICE selects the HIP
  * device (default 0).  Any library error aborts, as the reference does on internal failures.
  *
- * Containers follow the reference's allocation rules: seed intervals are appended with kalloc's
- * krealloc on the caller's km (Kgrow, kalloc.h:34-41), CIGAR storage is grown with the allocator
- * ksw2.h selects for this build (ksw2.h:113-121) and ksw_extz_t is filled field for field
- * (ksw2.h:36-45).
+ * Containers follow the reference's allocation rules: seed intervals and anchors are grown with
+ * kalloc's krealloc on the caller's km (Kgrow, kalloc.h:34-41; map-algo.c and lchain.c release them
+ * with kalloc's kfree), CIGAR storage and the ksw_ll query record with the allocator ksw2.h selects
+ * for this build (ksw2.h:113-121: plain malloc/realloc/free unless HAVE_KALLOC, as align.c and pe.c
+ * see it), and ksw_extz_t is filled field for field (ksw2.h:36-45).  ksw2.h's macros shadow
+ * kalloc's names in this file, so kalloc's own functions are called as (krealloc)(...).
  */
 #include <pthread.h>
 #include <stdint.h>
@@ -39,12 +44,13 @@
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 static mbg_ctx_t *g_ctx;
 static const mb_bwt_t *g_bwt;
-static long g_calls[5];                    /* seed, sa, extd2, extz2, ll */
+static const l2b_t *g_l2b;
+static long g_calls[6];                    /* seed, sa, extd2, extz2, ll, anchor */
 
 static void report(void)
 {
-	fprintf(stderr, "[mbg_wrap] calls: seed %ld sa %ld extd2 %ld extz2 %ld ll %ld\n", g_calls[0], g_calls[1], g_calls[2],
-	        g_calls[3], g_calls[4]);
+	fprintf(stderr, "[mbg_wrap] calls: seed %ld sa %ld anchor %ld extd2 %ld extz2 %ld ll %ld\n", g_calls[0], g_calls[1],
+	        g_calls[5], g_calls[2], g_calls[3], g_calls[4]);
 }
 
 static void count(int k)
@@ -217,4 +223,53 @@ ksw_llrst_t __wrap_ksw_ll_u8_core(void *q, int tlen, const uint8_t *target, int 
 ksw_llrst_t __wrap_ksw_ll_i16_core(void *q, int tlen, const uint8_t *target, int gapo, int gape, int xtra)
 {
 	return ll_one(q, tlen, target, gapo, gape, xtra, 2);
+}
+
+/* mb_anchor: the read's intervals and anchors cross the ABI as they are (mb_sai_t = mbg_sai_t,
+ * mb_anchor_t = mbg_anchor_t, whose flag word holds flag:31 and flt:1); the anchor array is grown on
+ * the caller's km to the reference's preallocation (seed.c:292-293).  The contig table goes to the
+ * device with the first call on an index. */
+double __real_mb_anchor(void *km, const mb_idx_t *idx, mb_sai_v *u, int32_t min_len, int32_t qlen, const uint8_t *qseq,
+                        l2b_meth_t mt, int32_t max_occ, mb_anchor_v *v);
+
+double __wrap_mb_anchor(void *km, const mb_idx_t *idx, mb_sai_v *u, int32_t min_len, int32_t qlen, const uint8_t *qseq,
+                        l2b_meth_t mt, int32_t max_occ, mb_anchor_v *v)
+{
+	mbg_ctx_t *c;
+	int64_t i, slots = 0, uoff[2], aoff[2], n_anc = 0, n_intv = 0;
+	int rc;
+	typedef char sai_layout[sizeof(mb_sai_t) == sizeof(mbg_sai_t) ? 1 : -1];
+	typedef char anchor_layout[sizeof(mb_anchor_t) == sizeof(mbg_anchor_t) ? 1 : -1];
+	(void)sizeof(sai_layout), (void)sizeof(anchor_layout);
+	if (mt != L2B_METH_NONE) return __real_mb_anchor(km, idx, u, min_len, qlen, qseq, mt, max_occ, v);
+	v->n = 0;
+	if (u->n == 0) return 1.0;
+	count(5);
+	c = ctx_for(idx->bwt);
+	pthread_mutex_lock(&g_lock);
+	if (idx->l2b != g_l2b) {
+		const l2b_t *l = idx->l2b;
+		uint64_t *len = (uint64_t*)malloc(sizeof(uint64_t) * (l->n_ctg ? l->n_ctg : 1));
+		mbg_l2b_desc_t d;
+		for (i = 0; i < (int64_t)l->n_ctg; ++i) len[i] = l->ctg[i].len;
+		d.tot_len = l->tot_len, d.n_ctg = l->n_ctg, d.ctg_len = len;
+		d.n_pac = l->n_pac, d.pac = l->pac, d.n_ambi = l->n_ambi, d.ambi = (const uint64_t*)l->ambi;
+		rc = mbg_l2b_upload(c, &d);
+		free(len);
+		if (rc != MBG_OK) die("mbg_l2b_upload", rc);
+		g_l2b = l;
+	}
+	pthread_mutex_unlock(&g_lock);
+	for (i = 0; i < (int64_t)u->n; ++i)
+		slots += u->a[i].size < (uint64_t)max_occ ? (int64_t)u->a[i].size : max_occ;
+	if (slots > v->m) {
+		v->m = slots;
+		v->a = (mb_anchor_t*)(krealloc)(km, v->a, (size_t)v->m * sizeof(mb_anchor_t));   /* kalloc's, not ksw2.h's macro */
+	}
+	uoff[0] = 0, uoff[1] = (int64_t)u->n, aoff[0] = 0, aoff[1] = slots;
+	rc = mbg_anchor_batch(c, 1, (mbg_sai_t*)u->a, uoff, &qlen, max_occ, (mbg_anchor_t*)v->a, aoff, &n_anc, &n_intv);
+	if (rc != MBG_OK) die("mbg_anchor_batch", rc);
+	u->n = (size_t)n_intv;
+	v->n = n_anc;
+	return 1.0;
 }

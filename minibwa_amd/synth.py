@@ -48,6 +48,21 @@ def reference(length, seed=11, rep_frac=0.05, rep_min=300, rep_max=6000, rep_div
     return out
 
 
+def repeat_reference(seed=5, n_copies=320, spacing=900, elem_len=400, elem_div=0.001):
+    """A repeat-rich reference: 300 kb of synth.reference (10% diverged segment copies) cut into
+    n_copies stretches of `spacing` bases, each followed by a copy of one random elem_len-base element
+    with elem_div substitutions -- reads on it get intervals of several hundred rows."""
+    rng = np.random.default_rng(seed)
+    base = reference(max(300_000, n_copies * spacing), seed=seed, rep_frac=0.1, rep_min=300, rep_max=3000, rep_div=0.01)
+    elem = rng.integers(0, 4, elem_len).astype(np.uint8)
+    parts = []
+    for c in range(n_copies):
+        cp = elem.copy()
+        cp[rng.random(elem_len) < elem_div] = rng.integers(0, 4)
+        parts += [base[c * spacing:(c + 1) * spacing], cp]
+    return np.concatenate(parts).astype(np.uint8)
+
+
 def read_pairs(ref, n_pairs, seed=11, rlen=150, frag_mean=400.0, frag_sd=50.0, sub=0.01, indel=0.001, truth=False):
     """(r1, r2) as (n_pairs, rlen) uint8 arrays; with truth=True also (n_pairs, 2) int64."""
     ref = np.ascontiguousarray(ref, np.uint8)

@@ -11,7 +11,9 @@ reference's own sources by oracle/Makefile):
   the config-0 read pairs (10,000 pairs from minibwa_amd.synth with seed 11), and md5 of the
   output of the reference's batch-API example (api-test/ex-batch.c, mb_map_batch) on the chrM
   read-1 file and on the config-0 read-1 file; the SAM body md5 of a mate-rescue-heavy config-0
-  set (read 2 with 8% substitutions) and its number of ksw_ll calls (``--dbg-aln-pe``).
+  set (read 2 with 8% substitutions) and its number of ksw_ll calls (``--dbg-aln-pe``); the SAM body
+  md5 of ``minibwa map`` on the repeat-rich reference of minibwa_amd.synth.repeat_reference (4
+  contigs, indexed in a temporary directory) for 4,000 2x150 pairs and for 40 noisy long reads.
 
 Usage: python tests/golden/make_golden.py
 """
@@ -80,6 +82,20 @@ def main():
         p = run([REF_CLI, "map", "--dbg-aln-pe", cfg0, f1, f2])
         out["rescue.sam.body.md5"] = sam_body_md5(p.stdout)
         out["rescue.n_ll_calls"] = p.stderr.count("===>")
+        # the repeat-rich reference: intervals of hundreds of rows, anchor lists past 64 entries
+        rep = os.path.join(td, "rep")
+        ref = synth.repeat_reference()
+        synth.write_fasta(rep + ".fa", ref, n_ctg=4, prefix="rc")
+        run([REF_CLI, "index", rep + ".fa", rep])
+        out["repeat.mbw.md5"] = md5(rep + ".mbw")
+        r1, r2 = synth.read_pairs(ref, 4000, seed=9)
+        f1, f2 = os.path.join(td, "rep1.fq"), os.path.join(td, "rep2.fq")
+        synth.write_pairs_fastq(f1, f2, r1, r2, prefix="rp_")
+        out["repeat.sam.body.md5"] = sam_body_md5(run([REF_CLI, "map", rep, f1, f2]).stdout)
+        seq, off, ln = synth.single_reads(ref, 40, seed=4, lmin=2000, lmax=8000, sub=0.04)
+        fl = os.path.join(td, "long.fq")
+        synth.write_fastq(fl, seq, off, ln, prefix="lr_")
+        out["repeat.long.sam.body.md5"] = sam_body_md5(run([REF_CLI, "map", rep, fl]).stdout)
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print(json.dumps(out, indent=1, sort_keys=True))

@@ -273,15 +273,7 @@ def repeat_index(tmp_path_factory):
     if not O.have_ref_pe():
         pytest.skip("oracle/_ref/libmbref_pe.so not built")
     d = tmp_path_factory.mktemp("rep")
-    rng = np.random.default_rng(5)
-    ref = synth.reference(300_000, seed=5, rep_frac=0.1, rep_min=300, rep_max=3000, rep_div=0.01)
-    elem = rng.integers(0, 4, 400).astype(np.uint8)          # a 400 bp element in 200 near-identical copies
-    parts = []
-    for c in range(200):
-        cp = elem.copy()
-        cp[rng.random(400) < 0.003] = rng.integers(0, 4)
-        parts += [ref[c * 1500:(c + 1) * 1500], cp]
-    ref = np.concatenate(parts).astype(np.uint8)
+    ref = synth.repeat_reference()
     synth.write_fasta(str(d / "rep.fa"), ref, n_ctg=4, prefix="rc")
     import subprocess
     subprocess.run([O.REF_CLI, "index", str(d / "rep.fa"), str(d / "rep")], check=True, capture_output=True)
@@ -306,7 +298,7 @@ def test_anchor_vs_reference(repeat_index):
     ivs = ri.seed_batch(reads, 19, 10)
     n_big = n_many = 0
     for i, r in enumerate(reads):
-        for max_occ in (250, 20):
+        for max_occ in (250, 20, 1):
             a = O.orc_anchor(oi, ctg_len, ivs[i], len(r), max_occ)
             b = rm.anchor(ivs[i], len(r), max_occ)
             assert len(a) == len(b) and (a == b).all(), (i, max_occ, len(a), len(b))

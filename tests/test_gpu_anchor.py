@@ -15,24 +15,10 @@ O = pytest.importorskip("oracle.oracle")
 pytestmark = pytest.mark.gpu
 
 
-def repeat_reference(seed=5):
-    """300 kb with 10% diverged segment copies, and a 400 bp element in 320 near-identical copies."""
-    from minibwa_amd import synth
-    rng = np.random.default_rng(seed)
-    ref = synth.reference(300_000, seed=seed, rep_frac=0.1, rep_min=300, rep_max=3000, rep_div=0.01)
-    elem = rng.integers(0, 4, 400).astype(np.uint8)
-    parts = []
-    for c in range(320):
-        cp = elem.copy()
-        cp[rng.random(400) < 0.001] = rng.integers(0, 4)
-        parts += [ref[c * 900:(c + 1) * 900], cp]
-    return np.concatenate(parts).astype(np.uint8)
-
-
 @pytest.fixture(scope="module")
 def rep(tmp_path_factory):
-    from minibwa_amd import index
-    ref = repeat_reference()
+    from minibwa_amd import index, synth
+    ref = synth.repeat_reference()
     pre = str(tmp_path_factory.mktemp("anc") / "rep")
     parts = index.build(ref, pre, n_ctg=4, ctg_prefix="rc", device="cpu")
     ctx = mbg.Context(0)

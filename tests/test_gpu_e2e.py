@@ -3,9 +3,10 @@
 objects plus minibwa_amd/integration/mbg_wrap.c) must write the same SAM body as the reference
 binary.  The expected md5s were recorded from the reference binary (tests/golden/make_golden.py).
 
-Seeding (mb_seed_intv_batch), SA locate (mb_bwt_sa_batch), every banded DP call
-(ksw_extd2_sse / ksw_extz2_sse) and every mate-rescue local SW (ksw_ll_qinit + ksw_ll_u8_core /
-ksw_ll_i16_core) run on the GPU; everything else is the reference's own code.
+Seeding (mb_seed_intv_batch), anchors (mb_anchor, whose SA locate runs inside it), SA locate
+(mb_bwt_sa_batch), every banded DP call (ksw_extd2_sse / ksw_extz2_sse) and every mate-rescue local
+SW (ksw_ll_qinit + ksw_ll_u8_core / ksw_ll_i16_core) run on the GPU; everything else is the
+reference's own code.
 """
 import hashlib
 import json
@@ -33,6 +34,14 @@ def _run(args, timeout=300, exe=HOOKED, stats=False):
     r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=timeout, env=env)
     assert r.returncode == 0, r.stderr[-2000:]
     return (r.stdout, r.stderr) if stats else r.stdout
+
+
+def _calls(err):
+    """The hook call counts the wrapper prints at exit under MBG_WRAP_STATS=1."""
+    lines = [l for l in err.split("\n") if l.startswith("[mbg_wrap] calls:")]
+    assert lines, err[-500:]
+    w = lines[-1].split()[2:]
+    return {w[i]: int(w[i + 1]) for i in range(0, len(w), 2)}
 
 
 @pytest.fixture(scope="module")
@@ -91,8 +100,45 @@ def test_rescue_pairs_sam_identical(gold, tmp_path):
     f1, f2 = str(tmp_path / "r1.fq"), str(tmp_path / "r2_rescue.fq")
     synth.write_pairs_fastq(f1, f2, r1, r2, prefix="c0_")
     sam, err = _run(["map", os.path.join(GOLDEN, "cfg0"), f1, f2], stats=True)
-    calls = [l for l in err.split("\n") if l.startswith("[mbg_wrap] calls:")]
-    assert calls, err[-500:]
-    n_ll = int(calls[-1].split()[-1])
+    n_ll = _calls(err)["ll"]
     assert n_ll == gold["rescue.n_ll_calls"] > 500
     assert _sam_body_md5(sam) == gold["rescue.sam.body.md5"]
+
+
+@pytest.fixture(scope="module")
+def repeat_prefix(tmp_path_factory, gold):
+    """The repeat-rich 4-contig reference, indexed by minibwa_amd.index (byte-identical to the
+    reference's indexer: the .mbw md5 is the reference binary's)."""
+    from minibwa_amd import index, synth
+    pre = str(tmp_path_factory.mktemp("rep") / "rep")
+    ref = synth.repeat_reference()
+    index.build(ref, pre, n_ctg=4, ctg_prefix="rc", device="cpu")
+    with open(pre + ".mbw", "rb") as f:
+        assert hashlib.md5(f.read()).hexdigest() == gold["repeat.mbw.md5"]
+    return pre, ref
+
+
+def test_repeat_pairs_sam_identical(repeat_prefix, gold, tmp_path):
+    """4,000 2x150 pairs on the repeat-rich reference: intervals of hundreds of rows, sampled to
+    max_occ, and anchor lists past 64 entries go through mb_anchor on the GPU."""
+    from minibwa_amd import synth
+    pre, ref = repeat_prefix
+    r1, r2 = synth.read_pairs(ref, 4000, seed=9)
+    f1, f2 = str(tmp_path / "rep1.fq"), str(tmp_path / "rep2.fq")
+    synth.write_pairs_fastq(f1, f2, r1, r2, prefix="rp_")
+    sam, err = _run(["map", pre, f1, f2], stats=True)
+    c = _calls(err)
+    assert c["anchor"] > 7000 and c["sa"] == 0
+    assert _sam_body_md5(sam) == gold["repeat.sam.body.md5"]
+
+
+def test_repeat_long_reads_sam_identical(repeat_prefix, gold, tmp_path):
+    """40 long reads (2-8 kb, 4% substitutions) on the repeat-rich reference."""
+    from minibwa_amd import synth
+    pre, ref = repeat_prefix
+    seq, off, ln = synth.single_reads(ref, 40, seed=4, lmin=2000, lmax=8000, sub=0.04)
+    fl = str(tmp_path / "long.fq")
+    synth.write_fastq(fl, seq, off, ln, prefix="lr_")
+    sam, err = _run(["map", pre, fl], stats=True)
+    assert _calls(err)["anchor"] == 40
+    assert _sam_body_md5(sam) == gold["repeat.long.sam.body.md5"]
