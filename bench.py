@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--rescue-batch", type=int, default=0, metavar="N",
                     help="only run the mate-rescue batch: N candidates of a 150 bp mate against a 550 bp window of a "
                          "100 Mbp reference, the mate a 1%%-diverged copy in 80%% of them; reports candidates/s")
+    ap.add_argument("--anchor-batch", action="store_true",
+                    help="only run mb_anchor over the seeding output of the read pairs (default 1M pairs of 2x150 bp "
+                         "on the 3.1 Gbp reference, max_occ 250); reports reads/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -152,6 +155,10 @@ def main():
     d_len = ctx.to_device(np.full(n_reads, rl, np.int32))
     log(f"{args.pairs} pairs generated and resident in {time.time() - t0:.1f}s")
     opt = mbg.adaptive_opt(rl)
+    if args.anchor_batch:
+        anchor_batch(ctx, reads, prefix, args)
+        ctx.close()
+        return
 
     def step(stats=False):
         return ctx.map_step_dev(n_reads, d_off.ptr, d_len.ptr, d_seq.ptr, rl, opt, collect_stats=stats)
@@ -468,6 +475,77 @@ def rescue_batch(args):
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def anchor_batch(ctx, reads, prefix, args):
+    """mb_anchor batch: mbg_anchor_batch_dev over the SMEM intervals of every read (from mbg_seed_batch on
+    the same resident index), inputs resident in HBM; the value is reads per second of host wall time
+    around the call and its completion.  The interval array is restored from a pristine copy before each
+    run (the call sorts it in place)."""
+    from minibwa_amd import mbg
+    n, rl = reads.shape
+    t0 = time.time()
+    flat, off = ctx.seed_intv_flat(reads, 19, 10)
+    log(f"{n} reads seeded: {len(flat)} intervals in {time.time() - t0:.1f}s")
+    max_occ = 250
+    qlen = np.full(n, rl, np.int32)
+    cs = np.zeros(len(flat) + 1, np.int64)
+    np.cumsum(np.minimum(flat[:, 2], max_occ).astype(np.int64), out=cs[1:])
+    aoff = np.ascontiguousarray(cs[off])
+    n_slots = int(aoff[-1])
+    d_u0, d_u = ctx.to_device(flat), ctx.to_device(flat)
+    d_off, d_ql, d_aoff = ctx.to_device(off), ctx.to_device(qlen), ctx.to_device(aoff)
+    d_rows = ctx.to_device(np.zeros(n_slots, np.uint64))
+    d_anc = ctx.to_device(np.zeros(n_slots, mbg.ANCHOR_DTYPE))
+    d_nanc, d_nu = ctx.to_device(np.zeros(n, np.int64)), ctx.to_device(np.zeros(n, np.int64))
+    P = mbg.C.c_void_p
+    times = []
+    for it in range(4):
+        mbg.check(ctx.L.mbg_stream_copy(ctx.h, None, P(d_u.ptr), P(d_u0.ptr), flat.nbytes), "mbg_stream_copy")
+        t1 = time.perf_counter()
+        mbg.check(ctx.L.mbg_anchor_batch_dev(ctx.h, None, n, n_slots, P(d_u.ptr), P(d_off.ptr), P(d_ql.ptr), max_occ,
+                                             P(d_rows.ptr), P(d_anc.ptr), P(d_aoff.ptr), P(d_nanc.ptr), P(d_nu.ptr)),
+                  "mbg_anchor_batch_dev")
+        ctx.to_host(d_nanc.ptr, np.int64, 1)          # waits for the context stream
+        times.append(time.perf_counter() - t1)
+        log(f"anchor run {it}: {times[-1] * 1e3:.2f} ms")
+    nanc = ctx.to_host(d_nanc.ptr, np.int64, n)
+    nu = ctx.to_host(d_nu.ptr, np.int64, n)
+    assert (nanc >= 0).all()
+    best = min(times[1:])
+    out = {"metric": "mb_anchor reads/s (interval sort/dedup, SA locate, contig mapping, anchor sort/dedup)",
+           "value": round(n / best), "unit": "reads/s", "n_reads": n, "ms": round(best * 1e3, 3),
+           "n_intervals": len(flat), "n_intervals_dedup": int(nu.sum()), "n_slots": n_slots, "n_anchors": int(nanc.sum()),
+           "ref_bp": args.ref_len, "max_occ": max_occ}
+    if not args.no_cpu:
+        cpu = anchor_cpu_sample(prefix, flat, off, qlen, max_occ, min(n, args.cpu_pairs))
+        if cpu:
+            out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+
+
+def anchor_cpu_sample(prefix, flat, off, qlen, max_occ, n_s):
+    """The reference's mb_anchor (oracle/_ref/cpu_hotpath anchor) on the first n_s reads' intervals."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "cpu_hotpath")
+    if not os.path.exists(exe) or n_s <= 0:
+        return None
+    work = tempfile.mkdtemp(prefix="mbg_anchor_")
+    fn = os.path.join(work, "intv.bin")
+    with open(fn, "wb") as f:
+        f.write(np.int64(n_s).tobytes())
+        f.write(np.ascontiguousarray(qlen[:n_s]).tobytes())
+        f.write(np.ascontiguousarray(off[:n_s + 1]).tobytes())
+        f.write(np.ascontiguousarray(flat[:off[n_s]]).tobytes())
+    cores = host_cores()
+    r = subprocess.run([exe, "anchor", prefix, fn, str(max_occ), str(cores)], capture_output=True, text=True, timeout=900)
+    shutil.rmtree(work, ignore_errors=True)
+    if r.returncode != 0:
+        log("anchor cpu sample failed:", r.stderr[-300:])
+        return None
+    d = json.loads(r.stdout.strip().split("\n")[-1])
+    return {"value": round(n_s / d["anchor_s"]), "unit": "reads/s", "cores": cores, "kind": "reference",
+            "sample": f"mb_anchor on the first {n_s} reads' intervals, {cores} threads, {d['anchor_s']:.2f}s, "
+                      f"{d['n_anchors']} anchors"}
 
 
 def rescue_cpu_sample(reads, ts, te, ref, o, n_s):

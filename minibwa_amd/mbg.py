@@ -392,6 +392,25 @@ class Context:
             res.append(flat[off[i]:off[i + 1]])
         return res
 
+    def seed_intv_flat(self, reads: np.ndarray, min_len=19, max_sub_occ=10):
+        """seed_intv_batch over equal-length reads (a 2-D uint8 array) -> ((m,4) uint64 intervals, int64
+        offsets[n+1]): read i's intervals are rows off[i]..off[i+1]."""
+        reads = np.ascontiguousarray(reads, np.uint8)
+        n, rl = reads.shape
+        lens = np.full(n, rl, np.int32)
+        addrs = np.uint64(reads.ctypes.data) + np.arange(n, dtype=np.uint64) * np.uint64(rl)   # uint8_t *const seq[n]
+        out = C.c_void_p()
+        off = np.zeros(n + 1, np.int64)
+        check(self.L.mbg_seed_batch(self.h, n, _ptr(lens), _ptr(addrs), min_len, max_sub_occ, C.byref(out), _ptr(off)),
+              "mbg_seed_batch")
+        tot = int(off[-1])
+        try:
+            flat = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_uint64)), shape=(max(tot, 1) * 4,))
+            flat = flat[: tot * 4].reshape(tot, 4).copy()
+        finally:
+            C.CDLL(None).free(out)
+        return flat, off
+
     def bwt_sa_batch(self, x):
         x = np.ascontiguousarray(x, np.uint64).copy()
         if len(x):

@@ -21,6 +21,12 @@
  * Mate-rescue local SW: ksw_ll_qinit + ksw_ll_u8_core / ksw_ll_i16_core (ksw2_ll_sse.c:49-333) per
  * job, as pe.c:268-272 calls them.  jobs.bin: int64 n, int8 mat[25], int32 gapo, gape, xtra,
  * n x 32-byte mbg_ll_job_t, int64 nq, nq query bytes, int64 nt, nt target bytes.
+ *
+ *   cpu_hotpath anchor <idx_prefix> <intv.bin> <max_occ> <n_threads>
+ *
+ * mb_anchor alone (seed.c:268-339, no methylation conversion) per read on given SMEM intervals, as
+ * map-algo.c:610 calls it.  intv.bin: int64 n, int32 qlen[n], int64 off[n+1], off[n] x 32-byte
+ * mb_sai_t.  Prints the wall time and the anchor count.
  */
 #include <pthread.h>
 #include <stdint.h>
@@ -190,6 +196,73 @@ static void *slurp(const char *fn, size_t *len)
 	return p;
 }
 
+typedef struct {
+	mb_idx_t *idx;
+	int64_t n, *off;
+	int32_t *qlen, max_occ;
+	const mb_sai_t *u;
+	volatile int64_t next;
+	int64_t n_anchor[256];
+} anc_shared_t;
+
+typedef struct { anc_shared_t *s; int tid; } anc_targ_t;
+
+static void *anchor_worker(void *a_)
+{
+	anc_targ_t *a = (anc_targ_t*)a_;
+	anc_shared_t *s = a->s;
+	mb_sai_v v = {0, 0, 0};
+	mb_anchor_v av = {0, 0, 0};
+	for (;;) {
+		int64_t b = __sync_fetch_and_add(&s->next, 64), e, i;
+		if (b >= s->n) break;
+		e = b + 64 < s->n ? b + 64 : s->n;
+		for (i = b; i < e; ++i) {
+			size_t k = (size_t)(s->off[i + 1] - s->off[i]);
+			if (k > v.m) v.m = k, v.a = (mb_sai_t*)realloc(v.a, k * sizeof(mb_sai_t));
+			memcpy(v.a, s->u + s->off[i], k * sizeof(mb_sai_t));     /* mb_anchor sorts them in place */
+			v.n = k;
+			if (k == 0) continue;                                      /* map-algo.c:595-598 */
+			mb_anchor(0, s->idx, &v, 19, s->qlen[i], 0, L2B_METH_NONE, s->max_occ, &av);
+			s->n_anchor[a->tid] += av.n;
+		}
+	}
+	free(v.a);
+	free(av.a);
+	return 0;
+}
+
+static int anchor_main(const char *prefix, const char *fn, int max_occ, int nt)
+{
+	anc_shared_t s;
+	pthread_t th[256];
+	anc_targ_t ta[256];
+	size_t len;
+	uint8_t *p = (uint8_t*)slurp(fn, &len);
+	int64_t sum = 0;
+	double t0, t;
+	int i;
+	if (!p) { fprintf(stderr, "cannot read %s\n", fn); return 1; }
+	memset(&s, 0, sizeof(s));
+	s.idx = mb_idx_load(prefix, 0);
+	if (!s.idx) { fprintf(stderr, "cannot load index %s\n", prefix); return 1; }
+	memcpy(&s.n, p, 8); p += 8;
+	s.qlen = (int32_t*)p; p += 4 * s.n;
+	s.off = (int64_t*)malloc((s.n + 1) * 8);
+	memcpy(s.off, p, (s.n + 1) * 8); p += (s.n + 1) * 8;
+	s.u = (const mb_sai_t*)p;
+	s.max_occ = max_occ;
+	t0 = now();
+	for (i = 0; i < nt; ++i) ta[i].s = &s, ta[i].tid = i, pthread_create(&th[i], 0, anchor_worker, &ta[i]);
+	for (i = 0; i < nt; ++i) pthread_join(th[i], 0);
+	t = now() - t0;
+	for (i = 0; i < nt; ++i) sum += s.n_anchor[i];
+	printf("{\"threads\": %d, \"n_reads\": %ld, \"n_intv\": %ld, \"anchor_s\": %.6f, \"n_anchors\": %ld}\n", nt, (long)s.n,
+	       (long)s.off[s.n], t, (long)sum);
+	mb_idx_destroy(s.idx);
+	return 0;
+}
+
 int main(int argc, char *argv[])
 {
 	shared_t s;
@@ -198,6 +271,10 @@ int main(int argc, char *argv[])
 	int nt, i;
 	double t0, t_seed, t_dp;
 	int64_t n_anchor = 0;
+	if (argc == 6 && strcmp(argv[1], "anchor") == 0) {
+		nt = atoi(argv[5]);
+		return anchor_main(argv[2], argv[3], atoi(argv[4]), nt < 1 ? 1 : nt > 256 ? 256 : nt);
+	}
 	if (argc == 4 && strcmp(argv[1], "ll") == 0) {
 		nt = atoi(argv[3]);
 		return ll_main(argv[2], nt < 1 ? 1 : nt > 256 ? 256 : nt);
