@@ -250,6 +250,25 @@ typedef struct { int32_t sid, len, qpos; uint32_t flag; int64_t tpos; } mbg_anch
 int mbg_anchor_batch(mbg_ctx_t *ctx, int32_t n_reads, mbg_sai_t *intv, const int64_t *intv_off, const int32_t *qlen,
                      int32_t max_occ, mbg_anchor_t *anc, const int64_t *anc_off, int64_t *n_anc, int64_t *n_intv);
 
+/* The parameters of mb_lchain_dp (lchain.c:145; decl mbpriv.h:62-63): max_dist_x/max_dist_y (raised
+ * to bw when smaller), bw, max_skip, max_iter, min_sc, and pen_gap = chn_pen_gap, which the mapper
+ * sets to chain_gap_scale * .01 * min_len (map-algo.c:607). */
+typedef struct {
+	int32_t max_dist_x, max_dist_y, bw, max_skip, max_iter, min_sc;
+	float pen_gap;
+	int32_t pad;
+} mbg_chain_opt_t;
+
+/* Chaining of n_reads reads' anchors: mb_lchain_dp (lchain.c:145-240) -- the DP with its skip and
+ * iteration limits, the backtrack (lchain.c:44-92) and the compaction (lchain.c:94-123).  Read r's
+ * anchors are anc[anc_off[r] .. anc_off[r] + n_anc[r]) in mb_anchor's order (anc_off has n_reads+1
+ * entries, anc_off[0] = 0, n_anc[r] <= anc_off[r+1] - anc_off[r]).  Its chains (score<<32 | anchor
+ * count, ordered by the first anchor's concatenated position) go to u[anc_off[r] ...], n_u[r] of them,
+ * and their anchors, first to last, to out[anc_off[r] ...], n_out[r] of them.  Anchors' contigs must
+ * be those of the resident l2b.  Results equal the reference's. */
+int mbg_lchain_batch(mbg_ctx_t *ctx, int32_t n_reads, const mbg_anchor_t *anc, const int64_t *anc_off, const int64_t *n_anc,
+                     const mbg_chain_opt_t *opt, uint64_t *u, int32_t *n_u, mbg_anchor_t *out, int64_t *n_out);
+
 /* Target extraction l2b_getseq (l2bit.c:52-77) for n windows [st[i],en[i]) of contig tid[i]
  * (clipped to the contig, N runs restored as 4), written at out + out_off[i].  len[i] receives
  * the clipped length (or -1 for a bad tid). */
@@ -277,6 +296,13 @@ int mbg_sa_batch_dev(mbg_ctx_t *ctx, void *stream, int64_t n, uint64_t *x);
 int mbg_anchor_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_reads, int64_t n_slots, mbg_sai_t *intv,
                          const int64_t *intv_off, const int32_t *qlen, int32_t max_occ, uint64_t *rows, mbg_anchor_t *anc,
                          const int64_t *anc_off, int64_t *n_anc, int64_t *n_intv);
+/* Workspace bytes mbg_lchain_batch_dev needs for n_slots anchor slots. */
+#define MBG_LCHAIN_WORK_BYTES(n_slots) ((size_t)(n_slots) * 72 + 2048)
+/* mbg_lchain_batch over buffers in HBM (anc_off[n_reads] = n_slots) with a caller's workspace of
+ * work_bytes >= MBG_LCHAIN_WORK_BYTES(n_slots).  Asynchronous on the stream. */
+int mbg_lchain_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_reads, int64_t n_slots, const mbg_anchor_t *anc,
+                         const int64_t *anc_off, const int64_t *n_anc, const mbg_chain_opt_t *opt, void *work, size_t work_bytes,
+                         uint64_t *u, int32_t *n_u, mbg_anchor_t *out, int64_t *n_out);
 /* mbg_ll_batch over jobs, sequences and results in HBM.  Every job is run with the registers of
  * max_qlen (<= 1024); rows_total bounds the sum of the jobs' tlen (the per-row maxima lists).  A job
  * outside the supported range (qlen > max_qlen, an i16 job that could saturate) gets score -1 and

@@ -1,5 +1,6 @@
-// mbg_anchor.hip -- mb_anchor on gfx950 (seed.c:109-339, no methylation conversion): a read's SMEM
-// intervals become contig-local anchors, in the reference's order.
+// mbg_anchor.hip -- mb_anchor (seed.c:109-339, no methylation conversion) and mb_lchain_dp
+// (lchain.c:25-240) on gfx950: a read's SMEM intervals become contig-local anchors, and its anchors
+// chains, in the reference's order.
 //
 //   anc_rows_kernel      one thread per read: mb_seed_sort_dedup, then the rows to locate for each
 //                        group of intervals sharing (x[0], size): all of a group of size <= max_occ,
@@ -8,8 +9,10 @@
 //   anc_build_kernel     one thread per read: each position of each interval of its group mapped to a
 //                        contig and strand (seed.c:176-210), the anchors sorted by concatenated tpos,
 //                        made contig-local, deduplicated (seed.c:140-166) and compacted
+//   lchain_kernel        one thread per read: the chaining DP with its skip heuristics, the backtrack
+//                        and the compaction of chained anchors (lchain.c:145-240)
 //
-// Both sorts are klib's radix sort (ksort.h:108-161): insertion sort up to 64 elements, else an
+// Every sort is klib's radix sort (ksort.h:108-161): insertion sort up to 64 elements, else an
 // in-place most-significant-byte-first radix sort whose cycle permutation is not stable, so anchors
 // with equal keys keep exactly the order that procedure leaves; it is restated here byte by byte in
 // place of recursion (rs_sort below).
@@ -223,6 +226,153 @@ __global__ void __launch_bounds__(64) anc_build_kernel(AncArgs a)
 	for (int64_t i = 0; i < na; ++i)
 		if (!(v[i].flag >> 31)) v[j++] = v[i];
 	a.nanc[r] = j;
+}
+
+// ---- chaining: mb_lchain_dp (lchain.c:145-240), one thread per read ------------------------------
+
+struct KeyU128 { __device__ uint64_t operator()(const U128 &v) const { return v.x; } };
+
+// comput_sc (lchain.c:125-143): the score of chaining aj -> ai, or INT32_MIN; the gap penalty is
+// float arithmetic rounded one operation at a time, then truncated
+__device__ int32_t chain_sc(const AncRec &ai, const AncRec &aj, int32_t max_dist_x, int32_t max_dist_y, int32_t bw, float pen_gap)
+{
+#pragma clang fp contract(off)
+	const int64_t dq = (int64_t)ai.qpos - aj.qpos;
+	if (dq <= 0 || dq > max_dist_y + ai.len) return INT32_MIN;
+	if (ai.sid != aj.sid) return INT32_MIN;
+	const int64_t dr = ai.tpos - aj.tpos;
+	if (dr <= 0 || dq > max_dist_x + ai.len) return INT32_MIN;
+	const int64_t dd = dr > dq ? dr - dq : dq - dr;
+	if (dd > bw) return INT32_MIN;
+	const int64_t dg = dr < dq ? dr : dq;
+	int64_t sc = ai.len < dg ? ai.len : dg;
+	if (dd) {
+		const float lin = pen_gap * (float)dd;
+		const float half = .5f * mb_log2_dev((float)(dd + 1));
+		sc -= (int)(lin + half);
+	}
+	return (int32_t)sc;
+}
+
+// mb_chain_bk_end (lchain.c:25-42)
+__device__ int64_t chain_bk_end(int32_t max_drop, const U128 *z, const int32_t *f, const int64_t *p, int32_t *t, int64_t k)
+{
+	int64_t i = (int64_t)z[k].y, end_i = -1, max_i = i;
+	int32_t max_s = 0;
+	if (i < 0 || t[i] != 0) return i;
+	do {
+		t[i] = 2;
+		end_i = i = p[i];
+		const int32_t s = i < 0 ? (int32_t)z[k].x : (int32_t)z[k].x - f[i];
+		if (s > max_s) max_s = s, max_i = i;
+		else if (max_s - s > max_drop) break;
+	} while (i >= 0 && t[i] == 0);
+	for (i = (int64_t)z[k].y; i >= 0 && i != end_i; i = p[i]) t[i] = 0;
+	return max_i;
+}
+
+__global__ void __launch_bounds__(64) lchain_kernel(ChainArgs a)
+{
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= a.n_reads) return;
+	const int64_t o = a.aoff[r], n = a.nanc[r];
+	const AncRec *an = a.anc + o;
+	int32_t *f = a.f + o, *t = a.t + o, *v = a.v + o;
+	int64_t *p = a.p + o;
+	U128 *z = a.z + o;
+	uint64_t *u = a.ut + o;
+	int32_t n_u = 0;
+	int64_t n_v = 0;
+	const int32_t bw = a.bw, max_drop = a.bw;
+	const int32_t mdx = a.max_dist_x < bw ? bw : a.max_dist_x, mdy = a.max_dist_y < bw ? bw : a.max_dist_y;
+	a.n_u[r] = 0, a.n_b[r] = 0;
+	if (n <= 0) return;
+	for (int64_t i = 0; i < n; ++i) t[i] = 0;
+	// the DP (lchain.c:170-222)
+	int64_t max_ii = -1;
+	for (int64_t i = 0; i < n; ++i) {
+		const AncRec ai = an[i];
+		int64_t max_j = -1, j;
+		int32_t max_f = ai.len, n_skip = 0;
+		for (j = i - 1; j >= 0 && j >= i - a.max_iter; --j) {
+			const AncRec aj = an[j];
+			if (ai.tpos - aj.tpos >= (int64_t)mdx + ai.len) break;
+			int32_t sc = chain_sc(ai, aj, mdx, mdy, bw, a.pen_gap);
+			if (sc == INT32_MIN) continue;
+			sc += f[j];
+			if (sc > max_f) {
+				max_f = sc, max_j = j;
+				if (n_skip > 0) --n_skip;
+			} else if (t[j] == (int32_t)i) {
+				if (++n_skip > a.max_skip) break;
+			}
+			if (p[j] >= 0) t[p[j]] = (int32_t)i;
+		}
+		const int64_t end_j = j;
+		if (max_ii < 0 || ai.tpos - an[max_ii].tpos > (int64_t)mdx + ai.len) {
+			int32_t mx = INT32_MIN;
+			max_ii = -1;
+			for (j = i - 1; j >= end_j && j >= 0; --j)
+				if (mx < f[j]) mx = f[j], max_ii = j;
+		}
+		if (max_ii >= 0 && max_ii < end_j) {
+			const int32_t tmp = chain_sc(ai, an[max_ii], mdx, mdy, bw, a.pen_gap);
+			if (tmp != INT32_MIN && max_f < tmp + f[max_ii]) max_f = tmp + f[max_ii], max_j = max_ii;
+		}
+		f[i] = max_f, p[i] = max_j;
+		if (max_ii < 0 || (ai.tpos - an[max_ii].tpos <= (int64_t)mdx + ai.len && f[max_ii] < f[i])) max_ii = i;
+	}
+	// the backtrack (lchain.c:44-92), in one pass: the reference's counting pass leaves the same state
+	int32_t bb[256], be[256];
+	int64_t n_z = 0;
+	for (int64_t i = 0; i < n; ++i)
+		if (f[i] >= a.min_sc) z[n_z].x = (uint64_t)(int64_t)f[i], z[n_z].y = (uint64_t)i, ++n_z;
+	if (n_z == 0) return;
+	rs_sort(z, n_z, KeyU128(), bb, be);
+	for (int64_t i = 0; i < n; ++i) t[i] = 0;
+	for (int64_t k = n_z - 1; k >= 0; --k) {
+		if (t[z[k].y] != 0) continue;
+		const int64_t n_v0 = n_v, end_i = chain_bk_end(max_drop, z, f, p, t, k);
+		int64_t i = (int64_t)z[k].y;
+		for (; i != end_i; i = p[i]) v[n_v++] = (int32_t)i, t[i] = 1;
+		const int32_t sc = i < 0 ? (int32_t)z[k].x : (int32_t)z[k].x - f[i];
+		if (sc >= a.min_sc && n_v > n_v0) u[n_u++] = (uint64_t)(uint32_t)sc << 32 | (uint64_t)(n_v - n_v0);
+		else n_v = n_v0;
+	}
+	if (n_u == 0) return;
+	// compact_a (lchain.c:94-123): each chain first anchor first, chains by their first anchor's
+	// concatenated tpos
+	AncRec *b0 = a.bb + o;
+	U128 *w = z;                                   // z is done with
+	for (int64_t i = 0, k = 0; i < n_u; ++i) {
+		const int64_t k0 = k, ni = (int32_t)u[i];
+		for (int64_t j = 0; j < ni; ++j) b0[k++] = an[v[k0 + (ni - j - 1)]];
+	}
+	for (int64_t i = 0, k = 0; i < n_u; ++i) {
+		const int64_t c = b0[k].sid >> 1;
+		w[i].x = (uint64_t)(b0[k].tpos + (int64_t)(a.ctg_off[c] * 2 + a.ctg_len[c] * (uint64_t)(b0[k].sid & 1)));
+		w[i].y = (uint64_t)k << 32 | (uint64_t)i;
+		k += (int32_t)u[i];
+	}
+	rs_sort(w, n_u, KeyU128(), bb, be);
+	uint64_t *uo = a.u + o;
+	AncRec *bo = a.b + o;
+	int64_t k = 0;
+	for (int64_t i = 0; i < n_u; ++i) {
+		const int32_t jj = (int32_t)w[i].y, m = (int32_t)u[jj];
+		uo[i] = u[jj];
+		const AncRec *src = b0 + (w[i].y >> 32);
+		for (int32_t q = 0; q < m; ++q) bo[k + q] = src[q];
+		k += m;
+	}
+	a.n_u[r] = n_u, a.n_b[r] = k;
+}
+
+hipError_t launch_lchain(const ChainArgs &a, hipStream_t st)
+{
+	if (a.n_reads <= 0) return hipSuccess;
+	hipLaunchKernelGGL(lchain_kernel, dim3((unsigned)((a.n_reads + 63) / 64)), dim3(64), 0, st, a);
+	return hipGetLastError();
 }
 
 hipError_t launch_anc_rows(const AncArgs &a, hipStream_t st)

@@ -47,6 +47,7 @@ struct mbg_ctx_s {
 	DBuf s_slots1, s_slots2, s_cnt, s_off, s_spill, s_tmp, r_cnt, r_off, s_pack, s_qflag;
 	DBuf m_buf[12];         // mate rescue
 	DBuf a_buf[8];          // anchors
+	DBuf c_buf[8];          // chains
 	int64_t cap_task = 0, cap_spill = 0;
 	int64_t cap_rec = 0, cap_rows = 0;
 	hipEvent_t ev[8] = {};
@@ -149,6 +150,7 @@ void mbg_destroy(mbg_ctx_t *c)
 	                &c->r_cnt, &c->r_off, &c->s_pack, &c->s_qflag}) dfree(*b);
 	for (auto &b : c->m_buf) dfree(b);
 	for (auto &b : c->a_buf) dfree(b);
+	for (auto &b : c->c_buf) dfree(b);
 	for (hipEvent_t &e : c->ev) if (e) hipEventDestroy(e);
 	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
 	if (c->stream) hipStreamDestroy(c->stream);
@@ -864,6 +866,78 @@ int mbg_anchor_batch(mbg_ctx_t *c, int32_t n_reads, mbg_sai_t *intv, const int64
 	HIPCHK(hipStreamSynchronize(st));
 	for (int32_t r = 0; r < n_reads; ++r)
 		if (n_anc[r] < 0) return MBG_EOVERFLOW;
+	return MBG_OK;
+}
+
+static int lchain_dev_locked(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, int64_t n_slots, const AncRec *anc, const int64_t *aoff,
+                             const int64_t *nanc, const mbg_chain_opt_t &o, void *work, size_t work_bytes, uint64_t *u, int32_t *n_u,
+                             AncRec *out, int64_t *n_out)
+{
+	if (work_bytes < MBG_LCHAIN_WORK_BYTES(n_slots)) return MBG_EARG;
+	ChainArgs a{};
+	a.n_reads = n_reads, a.anc = anc, a.aoff = aoff, a.nanc = nanc;
+	a.max_dist_x = o.max_dist_x, a.max_dist_y = o.max_dist_y, a.bw = o.bw, a.max_skip = o.max_skip, a.max_iter = o.max_iter;
+	a.min_sc = o.min_sc, a.pen_gap = o.pen_gap;
+	a.ctg_off = c->lb.ctg_off, a.ctg_len = c->lb.ctg_len;
+	char *w = (char*)work;                          // regions of n_slots elements, 256-byte aligned
+	auto take = [&](size_t elem) { char *p = w; w += ((size_t)n_slots * elem + 255) / 256 * 256; return (void*)p; };
+	w = (char*)(((uintptr_t)w + 255) / 256 * 256);
+	a.p = (int64_t*)take(8), a.z = (U128*)take(16), a.ut = (uint64_t*)take(8), a.bb = (AncRec*)take(sizeof(AncRec));
+	a.f = (int32_t*)take(4), a.t = (int32_t*)take(4), a.v = (int32_t*)take(4);
+	a.u = u, a.n_u = n_u, a.b = out, a.n_b = n_out;
+	HIPCHK(launch_lchain(a, st));
+	return MBG_OK;
+}
+
+int mbg_lchain_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, int64_t n_slots, const mbg_anchor_t *anc, const int64_t *anc_off,
+                         const int64_t *n_anc, const mbg_chain_opt_t *opt, void *work, size_t work_bytes, uint64_t *u, int32_t *n_u,
+                         mbg_anchor_t *out, int64_t *n_out)
+{
+	if (c == nullptr || n_reads < 0 || n_slots < 0 || opt == nullptr) return MBG_EARG;
+	if (n_reads > 0 && (!anc_off || !n_anc || !n_u || !n_out || (n_slots > 0 && (!anc || !work || !u || !out)))) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_l2b) return MBG_ENOIDX;
+	if (n_reads == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	return lchain_dev_locked(c, stream ? (hipStream_t)stream : c->stream, n_reads, n_slots, (const AncRec*)anc, anc_off, n_anc, *opt,
+	                         work, work_bytes, u, n_u, (AncRec*)out, n_out);
+}
+
+int mbg_lchain_batch(mbg_ctx_t *c, int32_t n_reads, const mbg_anchor_t *anc, const int64_t *anc_off, const int64_t *n_anc,
+                     const mbg_chain_opt_t *opt, uint64_t *u, int32_t *n_u, mbg_anchor_t *out, int64_t *n_out)
+{
+	if (c == nullptr || n_reads < 0 || opt == nullptr) return MBG_EARG;
+	if (n_reads > 0 && (!anc_off || !n_anc || !n_u || !n_out || anc_off[0] != 0)) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_l2b) return MBG_ENOIDX;
+	if (n_reads == 0) return MBG_OK;
+	const int64_t n_slots = anc_off[n_reads];
+	if (n_slots > 0 && (!anc || !u || !out)) return MBG_EARG;
+	for (int32_t r = 0; r < n_reads; ++r) {          // offsets, counts, and contigs the resident l2b holds
+		if (anc_off[r + 1] < anc_off[r] || n_anc[r] < 0 || n_anc[r] > anc_off[r + 1] - anc_off[r] || n_anc[r] > INT32_MAX)
+			return MBG_EARG;
+		for (int64_t i = anc_off[r]; i < anc_off[r] + n_anc[r]; ++i)
+			if (anc[i].sid < 0 || (uint64_t)(anc[i].sid >> 1) >= c->lb.n_ctg) return MBG_EARG;
+	}
+	HIPCHK(hipSetDevice(c->device));
+	hipStream_t st = c->stream;
+	DBuf *b = c->c_buf;
+	const size_t wb = MBG_LCHAIN_WORK_BYTES(n_slots);
+	TRY(dgrow(b[0], (size_t)n_slots * sizeof(AncRec))); TRY(dgrow(b[1], (size_t)(n_reads + 1) * 8)); TRY(dgrow(b[2], (size_t)n_reads * 8));
+	TRY(dgrow(b[3], wb)); TRY(dgrow(b[4], (size_t)n_slots * 8)); TRY(dgrow(b[5], (size_t)n_reads * 4));
+	TRY(dgrow(b[6], (size_t)n_slots * sizeof(AncRec))); TRY(dgrow(b[7], (size_t)n_reads * 8));
+	if (n_slots > 0) HIPCHK(hipMemcpyAsync(b[0].p, anc, (size_t)n_slots * sizeof(AncRec), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b[1].p, anc_off, (size_t)(n_reads + 1) * 8, hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(b[2].p, n_anc, (size_t)n_reads * 8, hipMemcpyHostToDevice, st));
+	TRY(lchain_dev_locked(c, st, n_reads, n_slots, (const AncRec*)b[0].p, (const int64_t*)b[1].p, (const int64_t*)b[2].p, *opt, b[3].p,
+	                      b[3].n, (uint64_t*)b[4].p, (int32_t*)b[5].p, (AncRec*)b[6].p, (int64_t*)b[7].p));
+	HIPCHK(hipMemcpyAsync(n_u, b[5].p, (size_t)n_reads * 4, hipMemcpyDeviceToHost, st));
+	HIPCHK(hipMemcpyAsync(n_out, b[7].p, (size_t)n_reads * 8, hipMemcpyDeviceToHost, st));
+	if (n_slots > 0) {
+		HIPCHK(hipMemcpyAsync(u, b[4].p, (size_t)n_slots * 8, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(out, b[6].p, (size_t)n_slots * sizeof(AncRec), hipMemcpyDeviceToHost, st));
+	}
+	HIPCHK(hipStreamSynchronize(st));
 	return MBG_OK;
 }
 

@@ -6,21 +6,6 @@
 
 namespace mbg {
 
-// mb_log2 (mbpriv.h:105-113), with the float operations kept separate as the host computes them
-__device__ __forceinline__ float mb_log2_dev(float x)
-{
-#pragma clang fp contract(off)
-	uint32_t i = __float_as_uint(x);
-	float l = (float)(((i >> 23) & 255u) - 128u);
-	i &= ~(255u << 23);
-	i += 127u << 23;
-	const float f = __uint_as_float(i);
-	const float a = -0.34484843f * f;
-	const float b = a + 2.02466578f;
-	const float c = b * f;
-	l += c - 0.67487759f;
-	return l;
-}
 
 __global__ void __launch_bounds__(128) extra_kernel(ExtraArgs a)
 {

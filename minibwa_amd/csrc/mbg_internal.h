@@ -167,7 +167,23 @@ struct MsArgs {
 	int32_t tab_size;       // k-mer table slots of the vote (a power of two >= 2 * the longest read)
 	int32_t mt, a, b_mm, q, e, bw, zdrop, end_bonus, min_dp_max;
 };
-// mb_anchor (kernels in mbg_anchor.hip); AncRec is layout-identical to mbg_anchor_t / mb_anchor_t
+// mb_log2 (mbpriv.h:105-113), with the float operations kept separate as the host computes them
+__device__ __forceinline__ float mb_log2_dev(float x)
+{
+#pragma clang fp contract(off)
+	uint32_t i = __float_as_uint(x);
+	float l = (float)(((i >> 23) & 255u) - 128u);
+	i &= ~(255u << 23);
+	i += 127u << 23;
+	const float f = __uint_as_float(i);
+	const float a = -0.34484843f * f;
+	const float b = a + 2.02466578f;
+	const float c = b * f;
+	l += c - 0.67487759f;
+	return l;
+}
+
+// mb_anchor and mb_lchain_dp (kernels in mbg_anchor.hip); AncRec is layout-identical to mbg_anchor_t / mb_anchor_t
 struct AncRec { int32_t sid, len, qpos; uint32_t flag; int64_t tpos; };
 static_assert(sizeof(AncRec) == 24, "AncRec must match mbg_anchor_t");
 struct AncArgs {
@@ -183,6 +199,27 @@ struct AncArgs {
 };
 hipError_t launch_anc_rows(const AncArgs &a, hipStream_t st);
 hipError_t launch_anc_build(const AncArgs &a, hipStream_t st);
+struct U128 { uint64_t x, y; };           // mb128_t (mbpriv.h:42)
+struct ChainArgs {
+	int32_t n_reads;
+	const AncRec *anc;       // read r's anchors at anc[aoff[r] .. aoff[r] + nanc[r]), mb_anchor's order
+	const int64_t *aoff, *nanc;
+	int32_t max_dist_x, max_dist_y, bw, max_skip, max_iter, min_sc;
+	float pen_gap;
+	const uint64_t *ctg_off, *ctg_len;
+	// per-read workspace at offset aoff[r] of each array
+	int32_t *f, *t, *v;
+	int64_t *p;
+	U128 *z;
+	uint64_t *ut;
+	AncRec *bb;
+	// output at offset aoff[r]: n_u[r] chains (score<<32 | count), n_b[r] chained anchors
+	uint64_t *u;
+	int32_t *n_u;
+	AncRec *b;
+	int64_t *n_b;
+};
+hipError_t launch_lchain(const ChainArgs &a, hipStream_t st);
 
 constexpr int kMsMaxWindow = 16384;    // rescue window bases: 16-bit vote counters, 32 KB of LDS
 hipError_t launch_ungap(const MsArgs &a, int max_wlen, hipStream_t st);

@@ -176,6 +176,9 @@ def lib():
                                        C.POINTER(MateswStats)]
         L.mbg_anchor_batch.argtypes = [_P, C.c_int32, _P, _P, _P, C.c_int32, _P, _P, _P, _P]
         L.mbg_anchor_batch_dev.argtypes = [_P, _P, C.c_int32, C.c_int64, _P, _P, _P, C.c_int32, _P, _P, _P, _P, _P]
+        L.mbg_lchain_batch.argtypes = [_P, C.c_int32, _P, _P, _P, C.POINTER(ChainOpt), _P, _P, _P, _P]
+        L.mbg_lchain_batch_dev.argtypes = [_P, _P, C.c_int32, C.c_int64, _P, _P, _P, C.POINTER(ChainOpt), _P, C.c_size_t, _P,
+                                           _P, _P, _P]
         L.mbg_update_extra_batch.argtypes = [_P, C.c_int64, _P, _P, C.c_int64, _P, C.c_int64, _P, C.c_int8, C.c_int8,
                                              C.c_int32, _P, C.c_int64]
         L.mbg_ll_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, C.c_int32, C.c_int32, C.c_int32, _P,
@@ -190,7 +193,7 @@ EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mb
             "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
             "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench", "mbg_idx_attach", "mbg_ll_batch",
             "mbg_ll_batch_dev", "mbg_matesw_batch", "mbg_update_extra_batch",
-            "mbg_anchor_batch", "mbg_anchor_batch_dev")
+            "mbg_anchor_batch", "mbg_anchor_batch_dev", "mbg_lchain_batch", "mbg_lchain_batch_dev")
 LL_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"), ("size", "<i4"),
                          ("pad", "<i4")], align=True)
 LL_RES_DTYPE = np.dtype([(f, "<i4") for f in ("score", "te", "qe", "score2", "te2")])
@@ -222,6 +225,17 @@ def anchor_slots(intervals, max_occ):
     """Per read, the anchor slots mb_anchor preallocates (seed.c:292-293): the sum of min(size, max_occ)."""
     m = max(int(max_occ), 0)
     return np.array([int(np.minimum(np.asarray(u, np.uint64).reshape(-1, 4)[:, 2], m).sum()) for u in intervals], np.int64)
+
+
+class ChainOpt(C.Structure):
+    """mbg_chain_opt_t: mb_lchain_dp's parameters (lchain.c:145)."""
+    _fields_ = [(f, C.c_int32) for f in ("max_dist_x", "max_dist_y", "bw", "max_skip", "max_iter", "min_sc")] + \
+               [("pen_gap", C.c_float), ("pad", C.c_int32)]
+
+
+def lchain_work_bytes(n_slots):
+    """MBG_LCHAIN_WORK_BYTES (include/mbg.h)."""
+    return int(n_slots) * 72 + 2048
 
 
 class MateswStats(C.Structure):
@@ -579,6 +593,25 @@ class Context:
                                       _ptr(nu)), "mbg_anchor_batch")
         return ([anc[aoff[i]:aoff[i] + nanc[i]].copy() for i in range(n)],
                 [u[uoff[i]:uoff[i] + nu[i]].copy() for i in range(n)])
+
+    def lchain(self, anchors, opt: ChainOpt):
+        """mb_lchain_dp (lchain.c:145-240) of each read's anchors (ANCHOR_DTYPE arrays in mb_anchor's order)
+        -> per read (uint64 chains score<<32 | count, chained anchors ANCHOR_DTYPE)."""
+        n = len(anchors)
+        cnt = np.array([len(a) for a in anchors], np.int64)
+        off = np.zeros(n + 1, np.int64)
+        np.cumsum(cnt, out=off[1:])
+        anc = np.ascontiguousarray(np.concatenate(anchors) if n else np.zeros(0, ANCHOR_DTYPE), ANCHOR_DTYPE)
+        if not len(anc):
+            anc = np.zeros(1, ANCHOR_DTYPE)
+        u = np.zeros(max(int(off[-1]), 1), np.uint64)
+        out = np.zeros(max(int(off[-1]), 1), ANCHOR_DTYPE)
+        n_u = np.zeros(max(n, 1), np.int32)
+        n_out = np.zeros(max(n, 1), np.int64)
+        cn = cnt if n else np.zeros(1, np.int64)
+        check(self.L.mbg_lchain_batch(self.h, n, _ptr(anc), _ptr(off), _ptr(cn), C.byref(opt), _ptr(u), _ptr(n_u), _ptr(out),
+                                      _ptr(n_out)), "mbg_lchain_batch")
+        return [(u[off[i]:off[i] + n_u[i]].copy(), out[off[i]:off[i] + n_out[i]].copy()) for i in range(n)]
 
     def matesw(self, reads, tid, ts, te, min_sc, opt: MateswOpt, stats: "MateswStats | None" = None):
         """Mate rescue of candidate windows [ts, te) on contigs tid for reads (one per candidate, as they

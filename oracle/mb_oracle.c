@@ -1316,6 +1316,154 @@ int64_t orc_anchor(const orc_bwt_t *b, int64_t n_ctg, const uint64_t *ctg_off, c
 	return j;
 }
 
+/* ---- chaining: mb_lchain_dp (lchain.c:145-240) ---------------------------------------------- */
+
+typedef struct { uint64_t x, y; } orc_u128_t;   /* mb128_t (mbpriv.h:42) */
+static uint64_t key_u128x(const void *p) { return ((const orc_u128_t*)p)->x; }
+
+/* comput_sc (lchain.c:125-143): the score of chaining aj -> ai, or INT32_MIN when they cannot chain */
+static int32_t chain_sc(const orc_anchor_t *ai, const orc_anchor_t *aj, int32_t max_dist_x, int32_t max_dist_y, int32_t bw,
+                        float pen_gap)
+{
+	const int64_t dq = (int64_t)ai->qpos - aj->qpos;
+	int64_t dr, dd, dg, sc;
+	if (dq <= 0 || dq > max_dist_y + ai->len) return INT32_MIN;
+	if (ai->sid != aj->sid) return INT32_MIN;
+	dr = ai->tpos - aj->tpos;
+	if (dr <= 0 || dq > max_dist_x + ai->len) return INT32_MIN;
+	dd = dr > dq ? dr - dq : dq - dr;
+	if (dd > bw) return INT32_MIN;
+	dg = dr < dq ? dr : dq;
+	sc = ai->len < dg ? ai->len : dg;
+	if (dd) {
+		const float lin = pen_gap * (float)dd;
+		const float lg = orc_log2((float)(dd + 1));
+		const float half = .5f * lg;
+		const float tot = lin + half;
+		sc -= (int)tot;
+	}
+	return (int32_t)sc;
+}
+
+/* mb_chain_bk_end (lchain.c:25-42): where the backtrack from z[k] stops: at a visited anchor, or where
+ * the chain's score since the best cut point has dropped by more than max_drop */
+static int64_t chain_bk_end(int32_t max_drop, const orc_u128_t *z, const int32_t *f, const int64_t *p, int32_t *t, int64_t k)
+{
+	int64_t i = (int64_t)z[k].y, end_i = -1, max_i = i;
+	int32_t max_s = 0;
+	if (i < 0 || t[i] != 0) return i;
+	do {
+		int32_t s;
+		t[i] = 2;
+		end_i = i = p[i];
+		s = i < 0 ? (int32_t)z[k].x : (int32_t)z[k].x - f[i];
+		if (s > max_s) max_s = s, max_i = i;
+		else if (max_s - s > max_drop) break;
+	} while (i >= 0 && t[i] == 0);
+	for (i = (int64_t)z[k].y; i >= 0 && i != end_i; i = p[i]) t[i] = 0;
+	return max_i;
+}
+
+/* mb_lchain_dp without its allocator: a[0..n) in mb_anchor's order (edited); chains to u (score<<32 |
+ * anchor count, n_u of them, in the order of their first anchor's concatenated tpos) and their anchors,
+ * first to last, to b.  Returns n_u (0: no chain; -1: out of memory).  *n_b receives the anchor count. */
+int32_t orc_lchain(int64_t n_ctg, const uint64_t *ctg_off, const uint64_t *ctg_len, int32_t max_dist_x, int32_t max_dist_y,
+                   int32_t bw, int32_t max_skip, int32_t max_iter, int32_t min_sc, float pen_gap, int64_t n, const orc_anchor_t *a,
+                   uint64_t *u_out, orc_anchor_t *b_out, int64_t *n_b)
+{
+	int32_t *f, *t, *v, n_u = 0, max_drop = bw;
+	int64_t *p, i, j, k, max_ii, n_z, n_v = 0;
+	orc_u128_t *z, *w;
+	uint64_t *u;
+	(void)n_ctg;
+	*n_b = 0;
+	if (n <= 0) return 0;
+	if (max_dist_x < bw) max_dist_x = bw;
+	if (max_dist_y < bw) max_dist_y = bw;
+	f = (int32_t*)malloc(n * 4), t = (int32_t*)calloc(n, 4), v = (int32_t*)malloc(n * 4), p = (int64_t*)malloc(n * 8);
+	z = (orc_u128_t*)malloc(n * sizeof(orc_u128_t)), u = (uint64_t*)malloc(n * 8);
+	if (!f || !t || !v || !p || !z || !u) return -1;
+	/* the DP (lchain.c:170-222) */
+	for (i = 0, max_ii = -1; i < n; ++i) {
+		int64_t max_j = -1, end_j;
+		int32_t max_f = a[i].len, n_skip = 0;
+		for (j = i - 1; j >= 0 && j >= i - max_iter; --j) {
+			int32_t sc;
+			if (a[i].tpos - a[j].tpos >= (int64_t)max_dist_x + a[i].len) break;
+			sc = chain_sc(&a[i], &a[j], max_dist_x, max_dist_y, bw, pen_gap);
+			if (sc == INT32_MIN) continue;
+			sc += f[j];
+			if (sc > max_f) {
+				max_f = sc, max_j = j;
+				if (n_skip > 0) --n_skip;
+			} else if (t[j] == (int32_t)i) {
+				if (++n_skip > max_skip) break;
+			}
+			if (p[j] >= 0) t[p[j]] = (int32_t)i;
+		}
+		end_j = j;
+		if (max_ii < 0 || a[i].tpos - a[max_ii].tpos > (int64_t)max_dist_x + a[i].len) {
+			int32_t mx = INT32_MIN;
+			max_ii = -1;
+			for (j = i - 1; j >= end_j && j >= 0; --j)
+				if (mx < f[j]) mx = f[j], max_ii = j;
+		}
+		if (max_ii >= 0 && max_ii < end_j) {
+			const int32_t tmp = chain_sc(&a[i], &a[max_ii], max_dist_x, max_dist_y, bw, pen_gap);
+			if (tmp != INT32_MIN && max_f < tmp + f[max_ii]) max_f = tmp + f[max_ii], max_j = max_ii;
+		}
+		f[i] = max_f, p[i] = max_j;
+		v[i] = max_j >= 0 && v[max_j] > max_f ? v[max_j] : max_f;
+		if (max_ii < 0 || (a[i].tpos - a[max_ii].tpos <= (int64_t)max_dist_x + a[i].len && f[max_ii] < f[i])) max_ii = i;
+	}
+	/* the backtrack (lchain.c:44-92): ends by score, klib's radix order among equal scores; one pass
+	 * (the reference counts first, then fills) */
+	for (i = 0, n_z = 0; i < n; ++i)
+		if (f[i] >= min_sc) z[n_z].x = (uint64_t)(int64_t)f[i], z[n_z++].y = (uint64_t)i;
+	if (n_z > 0) {
+		rs_sort(z, sizeof(*z), n_z, key_u128x);
+		memset(t, 0, n * 4);
+		for (k = n_z - 1; k >= 0; --k) {
+			if (t[z[k].y] == 0) {
+				const int64_t n_v0 = n_v, end_i = chain_bk_end(max_drop, z, f, p, t, k);
+				int32_t sc;
+				for (i = (int64_t)z[k].y; i != end_i; i = p[i]) v[n_v++] = (int32_t)i, t[i] = 1;
+				sc = i < 0 ? (int32_t)z[k].x : (int32_t)z[k].x - f[i];
+				if (sc >= min_sc && n_v > n_v0) u[n_u++] = (uint64_t)sc << 32 | (uint64_t)(n_v - n_v0);
+				else n_v = n_v0;
+			}
+		}
+	}
+	/* compact_a (lchain.c:94-123): chains first anchor first, then ordered by the first anchor's
+	 * concatenated tpos (klib's radix order among equal ones) */
+	if (n_u > 0) {
+		orc_anchor_t *bb = (orc_anchor_t*)malloc(n_v * sizeof(orc_anchor_t));
+		w = (orc_u128_t*)malloc(n_u * sizeof(orc_u128_t));
+		if (!bb || !w) return -1;
+		for (i = 0, k = 0; i < n_u; ++i) {
+			const int64_t k0 = k, ni = (int32_t)u[i];
+			for (j = 0; j < ni; ++j) bb[k++] = a[v[k0 + (ni - j - 1)]];
+		}
+		for (i = 0, k = 0; i < n_u; ++i) {
+			const int64_t c = bb[k].sid >> 1;
+			w[i].x = (uint64_t)(bb[k].tpos + (int64_t)(ctg_off[c] * 2 + ctg_len[c] * (uint64_t)(bb[k].sid & 1)));
+			w[i].y = (uint64_t)k << 32 | (uint64_t)i;
+			k += (int32_t)u[i];
+		}
+		rs_sort(w, sizeof(*w), n_u, key_u128x);
+		for (i = 0, k = 0; i < n_u; ++i) {
+			const int32_t jj = (int32_t)w[i].y, m = (int32_t)u[jj];
+			u_out[i] = u[jj];
+			memcpy(&b_out[k], &bb[w[i].y >> 32], m * sizeof(orc_anchor_t));
+			k += m;
+		}
+		*n_b = k;
+		free(bb); free(w);
+	}
+	free(f); free(t); free(v); free(p); free(z); free(u);
+	return n_u;
+}
+
 /* main.c:164-200 (`minibwa bench`): checksum over n splitmix64-drawn queries; type 0 = rank2a,
  * 1 = sa, 2 = batched sa over [k, k+intv) */
 static inline uint64_t orc_sm64(uint64_t *x)

@@ -71,6 +71,9 @@ typedef struct {               /* mb_anchor_t (mbpriv.h:26-32) */
 } orc_anchor_t;
 int64_t orc_anchor(const orc_bwt_t *b, int64_t n_ctg, const uint64_t *ctg_off, const uint64_t *ctg_len, orc_sai_t *u, int64_t n_u,
                    int32_t qlen, int32_t max_occ, orc_anchor_t *v, int64_t cap);
+int32_t orc_lchain(int64_t n_ctg, const uint64_t *ctg_off, const uint64_t *ctg_len, int32_t max_dist_x, int32_t max_dist_y,
+                   int32_t bw, int32_t max_skip, int32_t max_iter, int32_t min_sc, float pen_gap, int64_t n, const orc_anchor_t *a,
+                   uint64_t *u_out, orc_anchor_t *b_out, int64_t *n_b);
 
 typedef struct {               /* the mb_hit_t / mb_extra_t fields mb_fix_cigar and mb_update_extra set */
 	int32_t qs, qe, rev, n_cigar;

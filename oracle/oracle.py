@@ -124,6 +124,9 @@ def _load_oracle():
     L.orc_matesw_cand.argtypes = [_i32p, C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, _i32p, _i32p,
                                   C.POINTER(C.c_int32), C.POINTER(OrcRescue), _u32p, C.c_int]
     L.orc_anchor.restype = C.c_int64
+    L.orc_lchain.restype = C.c_int32
+    L.orc_lchain.argtypes = [C.c_int64, _u64p, _u64p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                             C.c_float, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
     L.orc_anchor.argtypes = [C.POINTER(OrcBwt), C.c_int64, _u64p, _u64p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
                              C.c_void_p, C.c_int64]
     L.orc_update_extra.argtypes = [C.POINTER(OrcExtra), _u32p, _u8p, _u8p, _i8p, C.c_int8, C.c_int8, C.c_int]
@@ -481,6 +484,9 @@ def ref_pe():
         L.mb_idx_load.argtypes = [C.c_char_p, C.c_int]
         L.mb_idx_destroy.argtypes = [C.c_void_p]
         L.mbx_anchor.restype = C.c_int64
+        L.mbx_lchain.restype = C.c_int32
+        L.mbx_lchain.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                 C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
         L.mbx_anchor.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]
         L.mbx_update_extra.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32, _u32p, _u8p, _u8p,
                                        _i8p, C.c_int8, C.c_int8, C.c_int32, C.POINTER(MbxHit), _u32p, C.c_int32]
@@ -578,6 +584,45 @@ def orc_anchor(index, ctg_len, intervals, qlen, max_occ=250):
     return out[:n]
 
 
+# mb_lchain_dp's parameters (lchain.c:145): max_dist_x, max_dist_y, bw, max_skip, max_iter, min_sc and the
+# gap scale chn_pen_gap = chain_gap_scale * .01 * min_len (map-algo.c:607); short-read preset defaults
+CHAIN_SR = dict(max_dist_x=100, max_dist_y=100, bw=100, max_skip=25, max_iter=5000, min_sc=25, pen_gap=0.8 * .01 * 19)
+
+
+# option sets of the chaining tests: the short-read preset, the long-read preset (options.c:70-82), the
+# re-chaining band (bw_long, map-algo.c:618-632), and tight skip/iteration limits that reach the breaks
+CHAIN_PARAM_SETS = [
+    CHAIN_SR,
+    dict(max_dist_x=5000, max_dist_y=5000, bw=500, max_skip=25, max_iter=5000, min_sc=40, pen_gap=0.8 * .01 * 19),
+    dict(max_dist_x=20000, max_dist_y=5000, bw=20000, max_skip=25, max_iter=5000, min_sc=40, pen_gap=0.8 * .01 * 19),
+    dict(max_dist_x=400, max_dist_y=300, bw=60, max_skip=1, max_iter=12, min_sc=19, pen_gap=0.05),
+    dict(max_dist_x=50, max_dist_y=50, bw=10, max_skip=0, max_iter=3, min_sc=1, pen_gap=0.5),
+]
+
+
+def _chain_args(p):
+    return (int(p["max_dist_x"]), int(p["max_dist_y"]), int(p["bw"]), int(p["max_skip"]), int(p["max_iter"]), int(p["min_sc"]),
+            float(p["pen_gap"]))
+
+
+def orc_lchain(ctg_len, anchors, params=CHAIN_SR):
+    """mb_lchain_dp restated on one read's anchors (ANCHOR_DTYPE, mb_anchor's order) -> (u: uint64 score<<32 |
+    count per chain, chained anchors ANCHOR_DTYPE)."""
+    a = np.ascontiguousarray(anchors, ANCHOR_DTYPE)
+    ctg_len = np.ascontiguousarray(ctg_len, np.uint64)
+    ctg_off = np.zeros(len(ctg_len), np.uint64)
+    if len(ctg_len) > 1:
+        np.cumsum(ctg_len[:-1], out=ctg_off[1:])
+    u = np.zeros(max(len(a), 1), np.uint64)
+    b = np.zeros(max(len(a), 1), ANCHOR_DTYPE)
+    nb = C.c_int64(0)
+    n_u = orc().orc_lchain(len(ctg_len), ctg_off, ctg_len, *_chain_args(params), len(a), a.ctypes.data_as(C.c_void_p),
+                           u.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p), C.byref(nb))
+    if n_u < 0:
+        raise MemoryError("orc_lchain")
+    return u[:n_u].copy(), b[:nb.value].copy()
+
+
 class RefMapIndex:
     """A prefix loaded by the reference's mb_idx_load (minibwa.h), for mb_anchor."""
 
@@ -594,6 +639,16 @@ class RefMapIndex:
         n = self.L.mbx_anchor(C.c_void_p(self.p), u.ctypes.data_as(C.c_void_p), len(u), qlen, max_occ,
                               out.ctypes.data_as(C.c_void_p), cap)
         return out[:min(n, cap)]
+
+    def lchain(self, anchors, params=CHAIN_SR):
+        """The reference's own mb_lchain_dp on one read's anchors -> (u, chained anchors)."""
+        a = np.ascontiguousarray(anchors, ANCHOR_DTYPE)
+        u = np.zeros(max(len(a), 1), np.uint64)
+        b = np.zeros(max(len(a), 1), ANCHOR_DTYPE)
+        nb = C.c_int64(0)
+        n_u = self.L.mbx_lchain(C.c_void_p(self.p), *_chain_args(params), len(a), a.ctypes.data_as(C.c_void_p),
+                                u.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p), C.byref(nb))
+        return u[:n_u].copy(), b[:nb.value].copy()
 
     def close(self):
         if self.p:

@@ -111,3 +111,22 @@ int64_t mbx_anchor(const mb_idx_t *idx, const mb_sai_t *u0, int64_t n_u, int32_t
 	free(u.a); free(v.a);
 	return n;
 }
+
+/* mb_lchain_dp (lchain.c:145-240) on a copy of a[0..n) (the reference frees its input): the chains
+ * (score<<32 | count) to u_out and their anchors to b_out; returns n_u, *n_b the anchor count. */
+int32_t mbx_lchain(const mb_idx_t *idx, int32_t max_dist_x, int32_t max_dist_y, int32_t bw, int32_t max_skip, int32_t max_iter,
+                   int32_t min_sc, float pen_gap, int64_t n, const mb_anchor_t *a, uint64_t *u_out, mb_anchor_t *b_out,
+                   int64_t *n_b)
+{
+	mb_anchor_t *a2 = (mb_anchor_t*)malloc(sizeof(mb_anchor_t) * (n + 1)), *b;
+	uint64_t *u = 0;
+	int32_t n_u = 0, i;
+	int64_t k = 0;
+	memcpy(a2, a, sizeof(mb_anchor_t) * n);
+	b = mb_lchain_dp(0, idx->l2b, max_dist_x, max_dist_y, bw, max_skip, max_iter, min_sc, pen_gap, n, a2, &n_u, &u);
+	for (i = 0; i < n_u; ++i) u_out[i] = u[i], k += (int32_t)u[i];
+	if (b) memcpy(b_out, b, sizeof(mb_anchor_t) * k);
+	*n_b = k;
+	free(b); free(u);
+	return n_u;
+}

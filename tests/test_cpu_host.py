@@ -80,6 +80,7 @@ def test_header_layouts_match_python(tmp_path):
         "mbg_matesw_job_t": ["qoff", "qlen", "tid", "ts", "te", "min_sc", "pad", "cigar_off"],
         "mbg_matesw_opt_t": ["a", "mt"],
         "mbg_anchor_t": ["sid", "len", "qpos", "flag", "tpos"],
+        "mbg_chain_opt_t": ["max_dist_x", "max_dist_y", "bw", "max_skip", "max_iter", "min_sc", "pen_gap", "pad"],
         "mbg_extra_job_t": ["qoff", "toff", "qs", "qe", "ts", "te", "rev", "n_cigar", "cigar_off", "blen", "mlen", "n_ambi",
                             "dp_max"],
         "mbg_matesw_result_t": ["max_ug", "max_i", "n_good", "n_kmer", "ts2", "te2", "aligned", "rescued", "ll", "ez", "qs",
@@ -118,6 +119,9 @@ def test_header_layouts_match_python(tmp_path):
     np_layout("mbg_extra_job_t", mbg.EXTRA_JOB_DTYPE)
     np_layout("mbg_matesw_result_t", mbg.MATESW_RES_DTYPE)
     np_layout("mbg_anchor_t", mbg.ANCHOR_DTYPE)
+    assert got[("mbg_chain_opt_t", "sizeof")] == C.sizeof(mbg.ChainOpt)
+    for f, _ in mbg.ChainOpt._fields_:
+        assert got[("mbg_chain_opt_t", f)] == getattr(mbg.ChainOpt, f).offset, f
     assert got[("mbg_matesw_opt_t", "sizeof")] == C.sizeof(mbg.MateswOpt)
     sr = mbg.SEED_REC_DTYPE
     assert got[("mbg_seed_rec_t", "sizeof")] == sr.itemsize

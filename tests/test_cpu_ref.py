@@ -306,3 +306,63 @@ def test_anchor_vs_reference(repeat_index):
         n_many += len(ivs[i]) > 64
     rm.close()
     assert n_big > 20 and n_many > 5
+
+
+CHAIN_PARAMS = O.CHAIN_PARAM_SETS if O is not None else []
+
+
+def test_lchain_vs_reference(repeat_index):
+    """The oracle's mb_lchain_dp against the reference's own (lchain.c:145-240) on the reference's anchors
+    of short and long reads: chain scores and counts, their order (klib's radix sort of equal scores and
+    of equal start positions) and the chained anchors, over option sets that reach the skip and max_iter
+    breaks and the max_ii rescue."""
+    from minibwa_amd import synth
+    prefix, ref, ctg_len = repeat_index
+    ri = O.RefIndex(prefix + ".mbw", pre_len=10)
+    rm = O.RefMapIndex(prefix)
+    r1, r2 = synth.read_pairs(ref, 200, seed=19)
+    reads = [r for r in r1] + [r for r in r2]
+    lr, lo, ll = synth.single_reads(ref, 16, seed=6, lmin=2000, lmax=8000, sub=0.04)
+    reads += [lr[lo[i]:lo[i] + ll[i]] for i in range(len(ll))]
+    ivs = ri.seed_batch(reads, 19, 10)
+    n_multi = n_long = 0
+    for i, r in enumerate(reads):
+        anc = rm.anchor(ivs[i], len(r), 250)
+        for p in CHAIN_PARAMS:
+            ua, ba = O.orc_lchain(ctg_len, anc, p)
+            ub, bb = rm.lchain(anc, p)
+            assert len(ua) == len(ub) and (ua == ub).all(), (i, p, len(ua), len(ub))
+            assert len(ba) == len(bb) and (ba == bb).all(), (i, p)
+            n_multi += len(ua) > 1
+        n_long += len(anc) > 1000
+    rm.close()
+    assert n_multi > 200 and n_long >= 4
+
+
+def test_lchain_invariants(repeat_index):
+    """What a chain is, independent of the reference: anchors strictly increasing in qpos and tpos on one
+    strand of one contig, each chain's anchor count in u, the score at least min_sc and at most the sum of
+    its anchors' lengths."""
+    from minibwa_amd import synth
+    prefix, ref, ctg_len = repeat_index
+    ri = O.RefIndex(prefix + ".mbw", pre_len=10)
+    oi = O.OracleIndex(prefix + ".mbw", pre_len=10)
+    lr, lo, ll = synth.single_reads(ref, 12, seed=8, lmin=1000, lmax=6000, sub=0.03)
+    reads = [lr[lo[i]:lo[i] + ll[i]] for i in range(len(ll))]
+    ivs = ri.seed_batch(reads, 19, 10)
+    n_chains = 0
+    for i, r in enumerate(reads):
+        anc = O.orc_anchor(oi, ctg_len, ivs[i], len(r), 250)
+        for p in CHAIN_PARAMS[:2]:
+            u, b = O.orc_lchain(ctg_len, anc, p)
+            k = 0
+            for c in u:
+                sc, m = int(c >> 32), int(c & 0xffffffff)
+                ch = b[k:k + m]
+                k += m
+                assert m >= 1 and (ch["sid"] == ch["sid"][0]).all()
+                assert (np.diff(ch["qpos"]) > 0).all() and (np.diff(ch["tpos"]) > 0).all()
+                assert p["min_sc"] <= sc <= int(ch["len"].sum())
+                n_chains += 1
+            assert k == len(b)
+    assert n_chains > 20
