@@ -4,8 +4,9 @@
  * Linked into the unmodified minibwa objects with
  *   -Wl,--wrap=mb_seed_intv_batch -Wl,--wrap=mb_bwt_sa_batch -Wl,--wrap=ksw_extd2_sse -Wl,--wrap=ksw_extz2_sse
  *   -Wl,--wrap=ksw_ll_qinit -Wl,--wrap=ksw_ll_u8_core -Wl,--wrap=ksw_ll_i16_core -Wl,--wrap=mb_anchor
+ *   -Wl,--wrap=mb_lchain_dp
  * every call site of those functions (map-main.c:75, map-algo.c:733, seed.c:180, align.c:374-376,
- * pe.c:267-272,287, map-algo.c:610) reaches the __wrap_ definitions below, which run the work on the GPU through the
+ * pe.c:267-272,287, map-algo.c:610,614,634) reaches the __wrap_ definitions below, which run the work on the GPU through the
  * C ABI of include/mbg.h and hand results back in the reference's own containers:
  *
  *   mb_seed_intv_batch (mbpriv.h:57, seed.c:56-103)   -> mbg_seed_batch
@@ -17,6 +18,7 @@
  *   mb_anchor          (mbpriv.h:58, seed.c:268-339)   -> mbg_anchor_batch (one read; reads with a
  *                                                         methylation conversion keep the reference's
  *                                                         own, which then locates through mbg_sa_batch)
+ *   mb_lchain_dp       (mbpriv.h:62-63, lchain.c:145-240) -> mbg_lchain_batch (one read)
  *
  * MBG_WRAP_STATS=1 prints the number of calls each hook took, at exit, to stderr.
  * The index is uploaded once, on the first hooked call, from the mb_bwt_t the caller passes (the
@@ -45,12 +47,12 @@ static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 static mbg_ctx_t *g_ctx;
 static const mb_bwt_t *g_bwt;
 static const l2b_t *g_l2b;
-static long g_calls[6];                    /* seed, sa, extd2, extz2, ll, anchor */
+static long g_calls[7];                    /* seed, sa, extd2, extz2, ll, anchor, chain */
 
 static void report(void)
 {
-	fprintf(stderr, "[mbg_wrap] calls: seed %ld sa %ld anchor %ld extd2 %ld extz2 %ld ll %ld\n", g_calls[0], g_calls[1],
-	        g_calls[5], g_calls[2], g_calls[3], g_calls[4]);
+	fprintf(stderr, "[mbg_wrap] calls: seed %ld sa %ld anchor %ld chain %ld extd2 %ld extz2 %ld ll %ld\n", g_calls[0],
+	        g_calls[1], g_calls[5], g_calls[6], g_calls[2], g_calls[3], g_calls[4]);
 }
 
 static void count(int k)
@@ -225,6 +227,26 @@ ksw_llrst_t __wrap_ksw_ll_i16_core(void *q, int tlen, const uint8_t *target, int
 	return ll_one(q, tlen, target, gapo, gape, xtra, 2);
 }
 
+/* the contig table goes to the device once per l2b */
+static void l2b_for(mbg_ctx_t *c, const l2b_t *l)
+{
+	pthread_mutex_lock(&g_lock);
+	if (l != g_l2b) {
+		uint64_t *len = (uint64_t*)malloc(sizeof(uint64_t) * (l->n_ctg ? l->n_ctg : 1));
+		mbg_l2b_desc_t d;
+		uint64_t i;
+		int rc;
+		for (i = 0; i < l->n_ctg; ++i) len[i] = l->ctg[i].len;
+		d.tot_len = l->tot_len, d.n_ctg = l->n_ctg, d.ctg_len = len;
+		d.n_pac = l->n_pac, d.pac = l->pac, d.n_ambi = l->n_ambi, d.ambi = (const uint64_t*)l->ambi;
+		rc = mbg_l2b_upload(c, &d);
+		free(len);
+		if (rc != MBG_OK) die("mbg_l2b_upload", rc);
+		g_l2b = l;
+	}
+	pthread_mutex_unlock(&g_lock);
+}
+
 /* mb_anchor: the read's intervals and anchors cross the ABI as they are (mb_sai_t = mbg_sai_t,
  * mb_anchor_t = mbg_anchor_t, whose flag word holds flag:31 and flt:1); the anchor array is grown on
  * the caller's km to the reference's preallocation (seed.c:292-293).  The contig table goes to the
@@ -246,20 +268,7 @@ double __wrap_mb_anchor(void *km, const mb_idx_t *idx, mb_sai_v *u, int32_t min_
 	if (u->n == 0) return 1.0;
 	count(5);
 	c = ctx_for(idx->bwt);
-	pthread_mutex_lock(&g_lock);
-	if (idx->l2b != g_l2b) {
-		const l2b_t *l = idx->l2b;
-		uint64_t *len = (uint64_t*)malloc(sizeof(uint64_t) * (l->n_ctg ? l->n_ctg : 1));
-		mbg_l2b_desc_t d;
-		for (i = 0; i < (int64_t)l->n_ctg; ++i) len[i] = l->ctg[i].len;
-		d.tot_len = l->tot_len, d.n_ctg = l->n_ctg, d.ctg_len = len;
-		d.n_pac = l->n_pac, d.pac = l->pac, d.n_ambi = l->n_ambi, d.ambi = (const uint64_t*)l->ambi;
-		rc = mbg_l2b_upload(c, &d);
-		free(len);
-		if (rc != MBG_OK) die("mbg_l2b_upload", rc);
-		g_l2b = l;
-	}
-	pthread_mutex_unlock(&g_lock);
+	l2b_for(c, idx->l2b);
 	for (i = 0; i < (int64_t)u->n; ++i)
 		slots += u->a[i].size < (uint64_t)max_occ ? (int64_t)u->a[i].size : max_occ;
 	if (slots > v->m) {
@@ -272,4 +281,43 @@ double __wrap_mb_anchor(void *km, const mb_idx_t *idx, mb_sai_v *u, int32_t min_
 	u->n = (size_t)n_intv;
 	v->n = n_anc;
 	return 1.0;
+}
+
+/* mb_lchain_dp: the input anchors are released and the chained anchors and chain list come from the
+ * caller's km, with kalloc's own functions, as lchain.c does; no chain leaves both outputs empty. */
+mb_anchor_t *__wrap_mb_lchain_dp(void *km, const l2b_t *l2b, int max_dist_x, int max_dist_y, int bw, int max_skip, int max_iter,
+                                 int min_sc, float chn_pen_gap, int64_t n, mb_anchor_t *a, int *n_u_, uint64_t **_u)
+{
+	mbg_ctx_t *c;
+	mbg_chain_opt_t o;
+	int64_t off[2], n_out = 0;
+	int32_t n_u = 0;
+	uint64_t *u;
+	mb_anchor_t *b;
+	int rc;
+	if (_u) *_u = 0, *n_u_ = 0;
+	if (n == 0 || a == 0) {
+		(kfree)(km, a);
+		return 0;
+	}
+	if (n > INT32_MAX) die("mb_lchain_dp (more than 2^31 anchors)", MBG_EUNSUPPORTED);
+	count(6);
+	c = ctx_for(0);
+	l2b_for(c, l2b);
+	memset(&o, 0, sizeof(o));
+	o.max_dist_x = max_dist_x, o.max_dist_y = max_dist_y, o.bw = bw, o.max_skip = max_skip, o.max_iter = max_iter;
+	o.min_sc = min_sc, o.pen_gap = chn_pen_gap;
+	u = (uint64_t*)(kmalloc)(km, (size_t)n * sizeof(uint64_t));
+	b = (mb_anchor_t*)(kmalloc)(km, (size_t)n * sizeof(mb_anchor_t));
+	off[0] = 0, off[1] = n;
+	rc = mbg_lchain_batch(c, 1, (const mbg_anchor_t*)a, off, &n, &o, u, &n_u, (mbg_anchor_t*)b, &n_out);
+	if (rc != MBG_OK) die("mbg_lchain_batch", rc);
+	(kfree)(km, a);
+	if (n_u == 0) {
+		(kfree)(km, u); (kfree)(km, b);
+		return 0;
+	}
+	if (_u) *_u = u, *n_u_ = n_u;
+	else (kfree)(km, u);
+	return b;
 }

@@ -3,8 +3,8 @@
 objects plus minibwa_amd/integration/mbg_wrap.c) must write the same SAM body as the reference
 binary.  The expected md5s were recorded from the reference binary (tests/golden/make_golden.py).
 
-Seeding (mb_seed_intv_batch), anchors (mb_anchor, whose SA locate runs inside it), SA locate
-(mb_bwt_sa_batch), every banded DP call (ksw_extd2_sse / ksw_extz2_sse) and every mate-rescue local
+Seeding (mb_seed_intv_batch), anchors (mb_anchor, whose SA locate runs inside it), chaining
+(mb_lchain_dp), SA locate (mb_bwt_sa_batch), every banded DP call (ksw_extd2_sse / ksw_extz2_sse) and every mate-rescue local
 SW (ksw_ll_qinit + ksw_ll_u8_core / ksw_ll_i16_core) run on the GPU; everything else is the
 reference's own code.
 """
@@ -128,7 +128,7 @@ def test_repeat_pairs_sam_identical(repeat_prefix, gold, tmp_path):
     synth.write_pairs_fastq(f1, f2, r1, r2, prefix="rp_")
     sam, err = _run(["map", pre, f1, f2], stats=True)
     c = _calls(err)
-    assert c["anchor"] > 7000 and c["sa"] == 0
+    assert c["anchor"] > 7000 and c["chain"] > 7000 and c["sa"] == 0
     assert _sam_body_md5(sam) == gold["repeat.sam.body.md5"]
 
 
@@ -140,5 +140,6 @@ def test_repeat_long_reads_sam_identical(repeat_prefix, gold, tmp_path):
     fl = str(tmp_path / "long.fq")
     synth.write_fastq(fl, seq, off, ln, prefix="lr_")
     sam, err = _run(["map", pre, fl], stats=True)
-    assert _calls(err)["anchor"] == 40
+    c = _calls(err)
+    assert c["anchor"] == 40 and c["chain"] >= 40
     assert _sam_body_md5(sam) == gold["repeat.long.sam.body.md5"]
