@@ -80,8 +80,9 @@ def main():
                     help="only run the mate-rescue batch: N candidates of a 150 bp mate against a 550 bp window of a "
                          "100 Mbp reference, the mate a 1%%-diverged copy in 80%% of them; reports candidates/s")
     ap.add_argument("--anchor-batch", action="store_true",
-                    help="only run mb_anchor over the seeding output of the read pairs (default 1M pairs of 2x150 bp "
-                         "on the 3.1 Gbp reference, max_occ 250); reports reads/s")
+                    help="only run mb_anchor and then mb_lchain_dp over the seeding output of the read pairs (default 1M "
+                         "pairs of 2x150 bp on the 3.1 Gbp reference, max_occ 250, the mapper's chaining options for "
+                         "150 bp); reports reads/s of each")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -478,10 +479,11 @@ def rescue_batch(args):
 
 
 def anchor_batch(ctx, reads, prefix, args):
-    """mb_anchor batch: mbg_anchor_batch_dev over the SMEM intervals of every read (from mbg_seed_batch on
-    the same resident index), inputs resident in HBM; the value is reads per second of host wall time
-    around the call and its completion.  The interval array is restored from a pristine copy before each
-    run (the call sorts it in place)."""
+    """mb_anchor then mb_lchain_dp batch: mbg_anchor_batch_dev over the SMEM intervals of every read (from
+    mbg_seed_batch on the same resident index), then mbg_lchain_batch_dev over the anchors it left in HBM;
+    the value is reads per second of host wall time around the anchor call and its completion, and the
+    chaining rate is reported beside it.  The interval array is restored from a pristine copy before each
+    anchor run (the call sorts it in place)."""
     from minibwa_amd import mbg
     n, rl = reads.shape
     t0 = time.time()
@@ -513,19 +515,42 @@ def anchor_batch(ctx, reads, prefix, args):
     nu = ctx.to_host(d_nu.ptr, np.int64, n)
     assert (nanc >= 0).all()
     best = min(times[1:])
+    # chaining the resident anchors with the mapper's options for this read length
+    copt = mbg.chain_opt(rl)
+    wb = mbg.lchain_work_bytes(n_slots)
+    d_work = ctx.to_device(np.zeros(wb, np.uint8))
+    d_u = ctx.to_device(np.zeros(n_slots, np.uint64))
+    d_nu_c = ctx.to_device(np.zeros(n, np.int32))
+    d_cb = ctx.to_device(np.zeros(n_slots, mbg.ANCHOR_DTYPE))
+    d_ncb = ctx.to_device(np.zeros(n, np.int64))
+    ctimes = []
+    for it in range(4):
+        t1 = time.perf_counter()
+        mbg.check(ctx.L.mbg_lchain_batch_dev(ctx.h, None, n, n_slots, P(d_anc.ptr), P(d_aoff.ptr), P(d_nanc.ptr),
+                                             mbg.C.byref(copt), P(d_work.ptr), wb, P(d_u.ptr), P(d_nu_c.ptr), P(d_cb.ptr),
+                                             P(d_ncb.ptr)), "mbg_lchain_batch_dev")
+        ctx.to_host(d_ncb.ptr, np.int64, 1)
+        ctimes.append(time.perf_counter() - t1)
+        log(f"chain run {it}: {ctimes[-1] * 1e3:.2f} ms")
+    n_chains = int(ctx.to_host(d_nu_c.ptr, np.int32, n).astype(np.int64).sum())
+    cbest = min(ctimes[1:])
     out = {"metric": "mb_anchor reads/s (interval sort/dedup, SA locate, contig mapping, anchor sort/dedup)",
            "value": round(n / best), "unit": "reads/s", "n_reads": n, "ms": round(best * 1e3, 3),
            "n_intervals": len(flat), "n_intervals_dedup": int(nu.sum()), "n_slots": n_slots, "n_anchors": int(nanc.sum()),
-           "ref_bp": args.ref_len, "max_occ": max_occ}
+           "ref_bp": args.ref_len, "max_occ": max_occ,
+           "chain": {"value": round(n / cbest), "unit": "reads/s", "ms": round(cbest * 1e3, 3), "n_chains": n_chains,
+                     "opt": {f: getattr(copt, f) for f, _ in mbg.ChainOpt._fields_ if f != "pad"}}}
     if not args.no_cpu:
-        cpu = anchor_cpu_sample(prefix, flat, off, qlen, max_occ, min(n, args.cpu_pairs))
+        cpu = anchor_cpu_sample(prefix, flat, off, qlen, max_occ, copt, min(n, args.cpu_pairs))
         if cpu:
-            out["cpu_baseline"] = cpu
+            out["cpu_baseline"], out["chain"]["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 
-def anchor_cpu_sample(prefix, flat, off, qlen, max_occ, n_s):
-    """The reference's mb_anchor (oracle/_ref/cpu_hotpath anchor) on the first n_s reads' intervals."""
+def anchor_cpu_sample(prefix, flat, off, qlen, max_occ, copt, n_s):
+    """The reference's mb_anchor and mb_lchain_dp (oracle/_ref/cpu_hotpath anchor) on the first n_s reads'
+    intervals -> (anchor baseline, chain baseline); each rate is n_s over that function's thread-seconds
+    divided by the thread count."""
     exe = os.path.join(ROOT, "oracle", "_ref", "cpu_hotpath")
     if not os.path.exists(exe) or n_s <= 0:
         return None
@@ -533,6 +558,9 @@ def anchor_cpu_sample(prefix, flat, off, qlen, max_occ, n_s):
     fn = os.path.join(work, "intv.bin")
     with open(fn, "wb") as f:
         f.write(np.int64(n_s).tobytes())
+        f.write(np.array([copt.max_dist_x, copt.max_dist_y, copt.bw, copt.max_skip, copt.max_iter, copt.min_sc],
+                         np.int32).tobytes())
+        f.write(np.float32(copt.pen_gap).tobytes())
         f.write(np.ascontiguousarray(qlen[:n_s]).tobytes())
         f.write(np.ascontiguousarray(off[:n_s + 1]).tobytes())
         f.write(np.ascontiguousarray(flat[:off[n_s]]).tobytes())
@@ -543,9 +571,13 @@ def anchor_cpu_sample(prefix, flat, off, qlen, max_occ, n_s):
         log("anchor cpu sample failed:", r.stderr[-300:])
         return None
     d = json.loads(r.stdout.strip().split("\n")[-1])
-    return {"value": round(n_s / d["anchor_s"]), "unit": "reads/s", "cores": cores, "kind": "reference",
-            "sample": f"mb_anchor on the first {n_s} reads' intervals, {cores} threads, {d['anchor_s']:.2f}s, "
-                      f"{d['n_anchors']} anchors"}
+    ta, tc = d["anchor_thread_s"] / cores, d["chain_thread_s"] / cores
+    return ({"value": round(n_s / ta), "unit": "reads/s", "cores": cores, "kind": "reference",
+             "sample": f"mb_anchor on the first {n_s} reads' intervals, {cores} threads, {d['anchor_thread_s']:.2f} "
+                       f"thread-s ({d['wall_s']:.2f}s wall with chaining), {d['n_anchors']} anchors"},
+            {"value": round(n_s / tc), "unit": "reads/s", "cores": cores, "kind": "reference",
+             "sample": f"mb_lchain_dp on the same reads' anchors, {cores} threads, {d['chain_thread_s']:.2f} thread-s, "
+                       f"{d['n_chains']} chains"})
 
 
 def rescue_cpu_sample(reads, ts, te, ref, o, n_s):

@@ -101,6 +101,26 @@ def adaptive_opt(read_len: int, **over) -> MapOpt:
     return m
 
 
+def chain_opt(read_len: int, rechain: bool = False) -> "ChainOpt":
+    """mb_lchain_dp's arguments for a read of read_len bases as mb_map_sai passes them (map-algo.c:607-616,
+    634): minibwa's defaults (preset "adap": bw 100, max_gap 100, min_chain_score 25, max_chain_skip 25,
+    max_chain_iter 5000, chain_gap_scale 0.8, min_len 19, bw_long 20000) through mb_opt_adap
+    (options.c:90-116); rechain=True gives the re-chaining band bw_long."""
+    import math
+    a_ = -math.log(0.5) / (2000 - 100)
+    b_ = math.exp(-a_ * ((read_len if read_len > 100 else 100) - 100))
+    max_gap = int(5000 - (5000 - 100) * b_ + .499)
+    if max_gap > read_len:
+        max_gap = read_len
+    bw = int(500 - (500 - 100) * b_ + .499)
+    bw_long = min(20000, read_len * 5)
+    bw_long = max(bw_long, bw)
+    min_sc = int(40 - (40 - 25) * b_ + .499)
+    pen = C.c_float(0.8).value * .01 * 19               # chain_gap_scale (float) * .01 * min_len, in double
+    return ChainOpt(max_dist_x=max_gap, max_dist_y=max_gap, bw=bw_long if rechain else bw, max_skip=25, max_iter=5000,
+                    min_sc=min_sc, pen_gap=pen)
+
+
 def nt4_mat(a=2, b=8, b_ts=0, b_ambi=1):
     """ksw_gen_nt4_mat (ksw2.h:199-219) without the methylation cases."""
     a, b, b_ambi = abs(a), -abs(b), -abs(b_ambi)

@@ -24,9 +24,11 @@
  *
  *   cpu_hotpath anchor <idx_prefix> <intv.bin> <max_occ> <n_threads>
  *
- * mb_anchor alone (seed.c:268-339, no methylation conversion) per read on given SMEM intervals, as
- * map-algo.c:610 calls it.  intv.bin: int64 n, int32 qlen[n], int64 off[n+1], off[n] x 32-byte
- * mb_sai_t.  Prints the wall time and the anchor count.
+ * mb_anchor (seed.c:268-339, no methylation conversion) per read on given SMEM intervals, as
+ * map-algo.c:610 calls it, then mb_lchain_dp (lchain.c:145-240) on its anchors as map-algo.c:616 does.
+ * intv.bin: int64 n, int32 max_dist_x, max_dist_y, bw, max_skip, max_iter, min_sc, float pen_gap,
+ * int32 qlen[n], int64 off[n+1], off[n] x 32-byte mb_sai_t.  Prints the wall time of the loop, the
+ * thread-seconds spent in each of the two functions, and the anchor and chain counts.
  */
 #include <pthread.h>
 #include <stdint.h>
@@ -201,8 +203,11 @@ typedef struct {
 	int64_t n, *off;
 	int32_t *qlen, max_occ;
 	const mb_sai_t *u;
+	int32_t cp[6];
+	float pen_gap;
 	volatile int64_t next;
-	int64_t n_anchor[256];
+	int64_t n_anchor[256], n_chain[256];
+	double t_anchor[256], t_chain[256];
 } anc_shared_t;
 
 typedef struct { anc_shared_t *s; int tid; } anc_targ_t;
@@ -223,8 +228,21 @@ static void *anchor_worker(void *a_)
 			memcpy(v.a, s->u + s->off[i], k * sizeof(mb_sai_t));     /* mb_anchor sorts them in place */
 			v.n = k;
 			if (k == 0) continue;                                      /* map-algo.c:595-598 */
-			mb_anchor(0, s->idx, &v, 19, s->qlen[i], 0, L2B_METH_NONE, s->max_occ, &av);
-			s->n_anchor[a->tid] += av.n;
+			{
+				double t0 = now(), t1;
+				mb_anchor_t *b;
+				uint64_t *w = 0;
+				int n_u = 0;
+				mb_anchor(0, s->idx, &v, 19, s->qlen[i], 0, L2B_METH_NONE, s->max_occ, &av);
+				s->n_anchor[a->tid] += av.n;
+				t1 = now();
+				b = mb_lchain_dp(0, s->idx->l2b, s->cp[0], s->cp[1], s->cp[2], s->cp[3], s->cp[4], s->cp[5], s->pen_gap, av.n,
+				                 av.a, &n_u, &w);                       /* frees av.a */
+				av.a = 0, av.n = av.m = 0;
+				s->t_chain[a->tid] += now() - t1, s->t_anchor[a->tid] += t1 - t0;
+				s->n_chain[a->tid] += n_u;
+				free(b); free(w);
+			}
 		}
 	}
 	free(v.a);
@@ -239,14 +257,16 @@ static int anchor_main(const char *prefix, const char *fn, int max_occ, int nt)
 	anc_targ_t ta[256];
 	size_t len;
 	uint8_t *p = (uint8_t*)slurp(fn, &len);
-	int64_t sum = 0;
-	double t0, t;
+	int64_t sum = 0, n_chain = 0;
+	double t0, t, t_a = 0, t_c = 0;
 	int i;
 	if (!p) { fprintf(stderr, "cannot read %s\n", fn); return 1; }
 	memset(&s, 0, sizeof(s));
 	s.idx = mb_idx_load(prefix, 0);
 	if (!s.idx) { fprintf(stderr, "cannot load index %s\n", prefix); return 1; }
 	memcpy(&s.n, p, 8); p += 8;
+	memcpy(s.cp, p, 24); p += 24;
+	memcpy(&s.pen_gap, p, 4); p += 4;
 	s.qlen = (int32_t*)p; p += 4 * s.n;
 	s.off = (int64_t*)malloc((s.n + 1) * 8);
 	memcpy(s.off, p, (s.n + 1) * 8); p += (s.n + 1) * 8;
@@ -256,9 +276,10 @@ static int anchor_main(const char *prefix, const char *fn, int max_occ, int nt)
 	for (i = 0; i < nt; ++i) ta[i].s = &s, ta[i].tid = i, pthread_create(&th[i], 0, anchor_worker, &ta[i]);
 	for (i = 0; i < nt; ++i) pthread_join(th[i], 0);
 	t = now() - t0;
-	for (i = 0; i < nt; ++i) sum += s.n_anchor[i];
-	printf("{\"threads\": %d, \"n_reads\": %ld, \"n_intv\": %ld, \"anchor_s\": %.6f, \"n_anchors\": %ld}\n", nt, (long)s.n,
-	       (long)s.off[s.n], t, (long)sum);
+	for (i = 0; i < nt; ++i) sum += s.n_anchor[i], n_chain += s.n_chain[i], t_a += s.t_anchor[i], t_c += s.t_chain[i];
+	printf("{\"threads\": %d, \"n_reads\": %ld, \"n_intv\": %ld, \"wall_s\": %.6f, \"anchor_thread_s\": %.6f, "
+	       "\"chain_thread_s\": %.6f, \"n_anchors\": %ld, \"n_chains\": %ld}\n", nt, (long)s.n, (long)s.off[s.n], t, t_a, t_c,
+	       (long)sum, (long)n_chain);
 	mb_idx_destroy(s.idx);
 	return 0;
 }

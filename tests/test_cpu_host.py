@@ -187,6 +187,13 @@ def test_adaptive_options_vs_reference():
         assert m.zdrop == oa.zdrop * oa.a, ln
         for f in ("min_len", "max_sub_occ", "max_occ", "a", "b", "q", "e", "q2", "e2", "end_bonus"):
             assert getattr(m, f) == getattr(oa, f), (ln, f)
+        # mb_lchain_dp's arguments (map-algo.c:607-616, 634)
+        pen = C.c_float(oa.chain_gap_scale * .01 * oa.min_len).value
+        for rechain, bw in ((False, oa.bw), (True, oa.bw_long)):
+            co = mbg.chain_opt(ln, rechain)
+            assert (co.max_dist_x, co.max_dist_y, co.bw, co.max_skip, co.max_iter, co.min_sc) == \
+                (oa.max_gap, oa.max_gap, bw, oa.max_chain_skip, oa.max_chain_iter, oa.min_chain_score), (ln, rechain)
+            assert co.pen_gap == pen, ln
 
 
 def test_index_builder_matches_reference_indexer(tmp_path):
