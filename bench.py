@@ -83,6 +83,9 @@ def main():
                     help="only run mb_anchor and then mb_lchain_dp over the seeding output of the read pairs (default 1M "
                          "pairs of 2x150 bp on the 3.1 Gbp reference, max_occ 250, the mapper's chaining options for "
                          "150 bp); reports reads/s of each")
+    ap.add_argument("--fastq-batch", type=int, default=0, metavar="N",
+                    help="only run the FASTQ reader (host): N 150 bp reads in one file, plain and gzip, in the CLI's "
+                         "100M-base chunks; reports records/s beside the reference's mb_bseq_read")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -101,6 +104,9 @@ def main():
         print(json.dumps({"calib_copy_bytes": n, "launches": 3}))
         return
 
+    if args.fastq_batch:
+        fastq_batch(args)
+        return
     if args.dp_batch:
         dp_batch(args)
         return
@@ -476,6 +482,60 @@ def rescue_batch(args):
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def fastq_batch(args):
+    """The FASTQ reader at the edge of the path (host code, no device): mbg_fq_read over one file of N
+    150 bp reads (the synthetic pairs of the default workload, interleaved), in the mini-batches the CLI
+    reads (100M bases, frag mode, min_cnt 40000: map-main.c:114-125), from the page cache; plain and
+    gzip.  The value is records/s of the plain file; the reference's mb_bseq_read
+    (oracle/_ref/cpu_hotpath bseq) on the same files is timed beside it."""
+    from minibwa_amd import mbg, synth
+    n = args.fastq_batch
+    ref = synth.reference(10_000_000, seed=11, rep_frac=0.0)
+    r1, r2 = synth.read_pairs(ref, (n + 1) // 2, seed=11, rlen=150)
+    reads = np.empty((2 * r1.shape[0], 150), np.uint8)
+    reads[0::2], reads[1::2] = r1, r2
+    reads = reads[:n]
+    work = tempfile.mkdtemp(prefix="mbg_fq_")
+    fq = os.path.join(work, "reads.fq")
+    synth.write_fastq(fq, reads.reshape(-1), np.arange(n + 1, dtype=np.int64) * 150, np.full(n, 150, np.int32), prefix="r")
+    gz = fq + ".gz"
+    with open(gz, "wb") as f:
+        subprocess.run(["gzip", "-1", "-c", fq], stdout=f, check=True)
+    cores = host_cores()
+    chunk = 100_000_000
+    out = {"metric": "FASTQ records/s (kseq parsing, U->T, nt4 codes), 150 bp reads, 100M-base chunks", "n_records": n,
+           "file_bytes": os.path.getsize(fq), "gz_bytes": os.path.getsize(gz), "threads": cores}
+    for tag, path in (("plain", fq), ("gz", gz)):
+        for _ in range(2):                                  # the first pass warms the page cache
+            r = mbg.FastxReader(path, cores)
+            ch = mbg.FqChunk()
+            t0 = time.perf_counter()
+            n_rec = n_bases = 0
+            while True:
+                mbg.check(r.L.mbg_fq_read(r.h, chunk, 1, 0, 1, 40000, chunk, r.buf, mbg.C.byref(ch)), "mbg_fq_read")
+                if ch.n == 0:
+                    break
+                n_rec += ch.n
+                n_bases += int(ch.seq_off[ch.n])
+            t = time.perf_counter() - t0
+            r.close()
+        assert n_rec == n and n_bases == 150 * n
+        d = {"records_per_s": round(n / t), "MB_per_s": round(os.path.getsize(path) / t / 1e6, 1), "s": round(t, 3)}
+        exe = os.path.join(ROOT, "oracle", "_ref", "cpu_hotpath")
+        if not args.no_cpu and os.path.exists(exe):
+            rr = subprocess.run([exe, "bseq", path, str(chunk)], capture_output=True, text=True, timeout=900)
+            if rr.returncode == 0:
+                c = json.loads(rr.stdout.strip().split("\n")[-1])
+                d["cpu_baseline"] = {"value": round(c["records"] / c["bseq_s"]), "unit": "records/s", "cores": 1,
+                                     "kind": "reference", "sample": f"mb_bseq_read over the same file, {c['bseq_s']:.2f}s"}
+        out[tag] = d
+    out["value"], out["unit"] = out["plain"]["records_per_s"], "records/s"
+    if "cpu_baseline" in out["plain"]:
+        out["cpu_baseline"] = out["plain"]["cpu_baseline"]
+    shutil.rmtree(work, ignore_errors=True)
+    print(json.dumps(out), flush=True)
 
 
 def anchor_batch(ctx, reads, prefix, args):

@@ -269,6 +269,41 @@ typedef struct {
 int mbg_lchain_batch(mbg_ctx_t *ctx, int32_t n_reads, const mbg_anchor_t *anc, const int64_t *anc_off, const int64_t *n_anc,
                      const mbg_chain_opt_t *opt, uint64_t *u, int32_t *n_u, mbg_anchor_t *out, int64_t *n_out);
 
+/* ---- reads at the edge: FASTA/FASTQ (host code) ------------------------------------------------ */
+
+/* A reader of one FASTA/FASTQ file, plain or gzip ("-" or NULL: standard input): mb_bseq_open
+ * (bseq.c:41-52).  A read-ahead thread decompresses while the caller parses; n_threads (1-64) encode
+ * each chunk's bases. */
+typedef struct mbg_fq_s mbg_fq_t;
+/* Storage of one chunk, reused from call to call (the views below point into it). */
+typedef struct mbg_fq_buf_s mbg_fq_buf_t;
+/* One chunk: record i's name is name[name_off[i] .. name_off[i+1]), its comment comment[com_off[i] ..
+ * com_off[i+1]) (empty when not kept), its bases seq[seq_off[i] .. seq_off[i+1]) (U/u turned into T/t as
+ * kseq2bseq does, bseq.c:64-78), their nt4 codes (kom_nt4_table, kommon.c:162-179) at the same
+ * offsets of nt4, and its quality at the same offsets of qual when has_qual[i].  status: 0, or -2 when
+ * a record with a missing or short quality string ended the chunk (kseq_read's -2; the reference
+ * warns and continues), or MBG_EIO on a decompression error. */
+typedef struct {
+	int32_t n, status;
+	const int64_t *name_off, *com_off, *seq_off;
+	const char *name, *comment, *seq, *qual;
+	const uint8_t *has_qual, *nt4;
+} mbg_fq_chunk_t;
+int mbg_fq_open(const char *path, int32_t n_threads, mbg_fq_t **fq);
+void mbg_fq_close(mbg_fq_t *fq);                                     /* mb_bseq_close (bseq.c:54-59) */
+int mbg_fq_buf_new(mbg_fq_buf_t **buf);
+void mbg_fq_buf_free(mbg_fq_buf_t *buf);
+/* The next chunk: mb_bseq_read (bseq.c:85-131) -- records until chunk_size bases and min_cnt records,
+ * or max_chunk_size bases; in frag_mode the records after that whose names equal the last one's
+ * (mb_qname_same, bseq.h:26-39) join it, and the first that does not opens the next chunk.  n = 0 at
+ * the end of the input. */
+int mbg_fq_read(mbg_fq_t *fq, int64_t chunk_size, int32_t with_qual, int32_t with_comment, int32_t frag_mode, int32_t min_cnt,
+                int64_t max_chunk_size, mbg_fq_buf_t *buf, mbg_fq_chunk_t *chunk);
+/* mb_bseq_read_frag (bseq.c:133-161): one record from each of n_fp files per step, interleaved, until
+ * chunk_size bases or until a file ends. */
+int mbg_fq_read_frag(int32_t n_fp, mbg_fq_t **fq, int64_t chunk_size, int32_t with_qual, int32_t with_comment, mbg_fq_buf_t *buf,
+                     mbg_fq_chunk_t *chunk);
+
 /* Target extraction l2b_getseq (l2bit.c:52-77) for n windows [st[i],en[i]) of contig tid[i]
  * (clipped to the contig, N runs restored as 4), written at out + out_off[i].  len[i] receives
  * the clipped length (or -1 for a bad tid). */

@@ -199,6 +199,15 @@ def lib():
         L.mbg_lchain_batch.argtypes = [_P, C.c_int32, _P, _P, _P, C.POINTER(ChainOpt), _P, _P, _P, _P]
         L.mbg_lchain_batch_dev.argtypes = [_P, _P, C.c_int32, C.c_int64, _P, _P, _P, C.POINTER(ChainOpt), _P, C.c_size_t, _P,
                                            _P, _P, _P]
+        L.mbg_fq_open.argtypes = [C.c_char_p, C.c_int32, C.POINTER(_P)]
+        L.mbg_fq_close.argtypes = [_P]
+        L.mbg_fq_close.restype = None
+        L.mbg_fq_buf_new.argtypes = [C.POINTER(_P)]
+        L.mbg_fq_buf_free.argtypes = [_P]
+        L.mbg_fq_buf_free.restype = None
+        L.mbg_fq_read.argtypes = [_P, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int64, _P,
+                                  C.POINTER(FqChunk)]
+        L.mbg_fq_read_frag.argtypes = [C.c_int32, C.POINTER(_P), C.c_int64, C.c_int32, C.c_int32, _P, C.POINTER(FqChunk)]
         L.mbg_update_extra_batch.argtypes = [_P, C.c_int64, _P, _P, C.c_int64, _P, C.c_int64, _P, C.c_int8, C.c_int8,
                                              C.c_int32, _P, C.c_int64]
         L.mbg_ll_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int8, _P, C.c_int32, C.c_int32, C.c_int32, _P,
@@ -213,7 +222,8 @@ EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mb
             "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
             "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench", "mbg_idx_attach", "mbg_ll_batch",
             "mbg_ll_batch_dev", "mbg_matesw_batch", "mbg_update_extra_batch",
-            "mbg_anchor_batch", "mbg_anchor_batch_dev", "mbg_lchain_batch", "mbg_lchain_batch_dev")
+            "mbg_anchor_batch", "mbg_anchor_batch_dev", "mbg_lchain_batch", "mbg_lchain_batch_dev", "mbg_fq_open",
+            "mbg_fq_close", "mbg_fq_buf_new", "mbg_fq_buf_free", "mbg_fq_read", "mbg_fq_read_frag")
 LL_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"), ("size", "<i4"),
                          ("pad", "<i4")], align=True)
 LL_RES_DTYPE = np.dtype([(f, "<i4") for f in ("score", "te", "qe", "score2", "te2")])
@@ -245,6 +255,78 @@ def anchor_slots(intervals, max_occ):
     """Per read, the anchor slots mb_anchor preallocates (seed.c:292-293): the sum of min(size, max_occ)."""
     m = max(int(max_occ), 0)
     return np.array([int(np.minimum(np.asarray(u, np.uint64).reshape(-1, 4)[:, 2], m).sum()) for u in intervals], np.int64)
+
+
+class FqChunk(C.Structure):
+    """mbg_fq_chunk_t: views into a chunk's flat arrays."""
+    _P8 = C.POINTER(C.c_uint8)
+    _fields_ = [("n", C.c_int32), ("status", C.c_int32)] + \
+               [(f, C.POINTER(C.c_int64)) for f in ("name_off", "com_off", "seq_off")] + \
+               [(f, C.POINTER(C.c_char)) for f in ("name", "comment", "seq", "qual")] + \
+               [("has_qual", C.POINTER(C.c_uint8)), ("nt4", C.POINTER(C.c_uint8))]
+
+
+class FastxReader:
+    """A FASTA/FASTQ file read in mb_bseq_read's chunks (bseq.c:85-161) by the library's reader; host only,
+    no device needed."""
+
+    def __init__(self, path, n_threads=4):
+        self.L = lib()
+        h = C.c_void_p()
+        check(self.L.mbg_fq_open(path.encode() if path else None, n_threads, C.byref(h)), "mbg_fq_open")
+        self.h = h
+        b = C.c_void_p()
+        check(self.L.mbg_fq_buf_new(C.byref(b)), "mbg_fq_buf_new")
+        self.buf = b
+
+    @staticmethod
+    def _records(ch):
+        n = ch.n
+        if n == 0:
+            return [], np.zeros(0, np.uint8), ch.status
+
+        def arr(p, k, dt=np.int64):
+            return np.ctypeslib.as_array(p, shape=(k,)).astype(dt, copy=True) if k else np.zeros(0, dt)
+        no, co, so = arr(ch.name_off, n + 1), arr(ch.com_off, n + 1), arr(ch.seq_off, n + 1)
+        names, coms = C.string_at(ch.name, int(no[-1])), C.string_at(ch.comment, int(co[-1]))
+        seqs = C.string_at(ch.seq, int(so[-1]))
+        quals = C.string_at(ch.qual, int(so[-1]))
+        hq = arr(ch.has_qual, n, np.uint8)
+        nt4 = np.frombuffer(C.string_at(ch.nt4, int(so[-1])), np.uint8).copy()
+        recs = []
+        for i in range(n):
+            c = coms[co[i]:co[i + 1]]
+            recs.append((names[no[i]:no[i + 1]], seqs[so[i]:so[i + 1]], quals[so[i]:so[i + 1]] if hq[i] else None,
+                         c if c else None))
+        return recs, nt4, ch.status
+
+    def read(self, chunk_size, with_qual=True, with_comment=True, frag_mode=False, min_cnt=0, max_chunk_size=0):
+        """-> ([(name, seq, qual|None, comment|None) bytes], nt4 codes of the chunk's bases, status)."""
+        ch = FqChunk()
+        check(self.L.mbg_fq_read(self.h, chunk_size, int(with_qual), int(with_comment), int(frag_mode), min_cnt,
+                                 max_chunk_size, self.buf, C.byref(ch)), "mbg_fq_read")
+        return self._records(ch)
+
+    @staticmethod
+    def read_frag(readers, chunk_size, with_qual=True, with_comment=True):
+        L = readers[0].L
+        hs = (C.c_void_p * len(readers))(*[r.h for r in readers])
+        ch = FqChunk()
+        check(L.mbg_fq_read_frag(len(readers), hs, chunk_size, int(with_qual), int(with_comment), readers[0].buf,
+                                 C.byref(ch)), "mbg_fq_read_frag")
+        return FastxReader._records(ch)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mbg_fq_close(self.h)
+            self.L.mbg_fq_buf_free(self.buf)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ChainOpt(C.Structure):

@@ -29,6 +29,12 @@
  * intv.bin: int64 n, int32 max_dist_x, max_dist_y, bw, max_skip, max_iter, min_sc, float pen_gap,
  * int32 qlen[n], int64 off[n+1], off[n] x 32-byte mb_sai_t.  Prints the wall time of the loop, the
  * thread-seconds spent in each of the two functions, and the anchor and chain counts.
+ *
+ *   cpu_hotpath bseq <reads.fq[.gz]> <chunk_size>
+ *
+ * The reference's reader: mb_bseq_read (bseq.c:85-131) in chunks of chunk_size bases, frag mode and
+ * min_cnt 40000 as map-main.c:114-125 calls it for one interleaved file, until the input ends.
+ * Prints the wall time, the records and the bases.
  */
 #include <pthread.h>
 #include <stdint.h>
@@ -37,6 +43,7 @@
 #include <string.h>
 #include <sys/time.h>
 #include "mbpriv.h"
+#include "bseq.h"
 #include "ksw2.h"
 
 typedef struct {
@@ -284,6 +291,29 @@ static int anchor_main(const char *prefix, const char *fn, int max_occ, int nt)
 	return 0;
 }
 
+static int bseq_main(const char *fn, int64_t chunk)
+{
+	mb_bseq_file_t *fp = mb_bseq_open(fn);
+	int64_t n_rec = 0, n_bases = 0;
+	double t0;
+	if (!fp) { fprintf(stderr, "cannot open %s\n", fn); return 1; }
+	t0 = now();
+	for (;;) {
+		int n = 0, i;
+		mb_bseq1_t *s = mb_bseq_read(fp, chunk, 1, 0, 1, 40000, chunk, &n);
+		if (n == 0) { free(s); break; }
+		for (i = 0; i < n; ++i) {
+			n_bases += s[i].l_seq;
+			free(s[i].name); free(s[i].seq); free(s[i].qual); free(s[i].comment);
+		}
+		n_rec += n;
+		free(s);
+	}
+	printf("{\"bseq_s\": %.6f, \"records\": %ld, \"bases\": %ld}\n", now() - t0, (long)n_rec, (long)n_bases);
+	mb_bseq_close(fp);
+	return 0;
+}
+
 int main(int argc, char *argv[])
 {
 	shared_t s;
@@ -292,6 +322,7 @@ int main(int argc, char *argv[])
 	int nt, i;
 	double t0, t_seed, t_dp;
 	int64_t n_anchor = 0;
+	if (argc == 4 && strcmp(argv[1], "bseq") == 0) return bseq_main(argv[2], atoll(argv[3]));
 	if (argc == 6 && strcmp(argv[1], "anchor") == 0) {
 		nt = atoi(argv[5]);
 		return anchor_main(argv[2], argv[3], atoi(argv[4]), nt < 1 ? 1 : nt > 256 ? 256 : nt);

@@ -623,6 +623,66 @@ def orc_lchain(ctg_len, anchors, params=CHAIN_SR):
     return u[:n_u].copy(), b[:nb.value].copy()
 
 
+class _Bseq1(C.Structure):       # mb_bseq1_t (bseq.h:14-18)
+    _fields_ = [("l_seq", C.c_uint64), ("id", C.c_uint64), ("name", C.c_void_p), ("seq", C.c_void_p), ("qual", C.c_void_p),
+                ("comment", C.c_void_p)]
+
+
+class RefBseq:
+    """The reference's own mb_bseq_open / mb_bseq_read / mb_bseq_read_frag (bseq.c:41-161), compiled in
+    oracle/_ref/libmbref.so."""
+
+    def __init__(self, path):
+        L = ref()
+        L.mb_bseq_open.restype = C.c_void_p
+        L.mb_bseq_open.argtypes = [C.c_char_p]
+        L.mb_bseq_close.argtypes = [C.c_void_p]
+        L.mb_bseq_read.restype = C.POINTER(_Bseq1)
+        L.mb_bseq_read.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_int)]
+        L.mb_bseq_read_frag.restype = C.POINTER(_Bseq1)
+        L.mb_bseq_read_frag.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        self.L = L
+        self.p = L.mb_bseq_open(path.encode())
+        if not self.p:
+            raise IOError(path)
+
+    @staticmethod
+    def _take(a, n):
+        free = C.CDLL(None).free
+        free.argtypes = [C.c_void_p]
+        out = []
+        for i in range(n):
+            r = a[i]
+            get = (lambda p: C.string_at(p) if p else None)
+            out.append((C.string_at(r.name), C.string_at(r.seq, r.l_seq), C.string_at(r.qual, r.l_seq) if r.qual else None,
+                        get(r.comment)))
+            for f in (r.name, r.seq, r.qual, r.comment):
+                if f:
+                    free(f)
+        if a:
+            free(C.cast(a, C.c_void_p))
+        return out
+
+    def read(self, chunk_size, with_qual=True, with_comment=True, frag_mode=False, min_cnt=0, max_chunk_size=0):
+        n = C.c_int(0)
+        a = self.L.mb_bseq_read(self.p, chunk_size, int(with_qual), int(with_comment), int(frag_mode), min_cnt, max_chunk_size,
+                                C.byref(n))
+        return self._take(a, n.value)
+
+    @staticmethod
+    def read_frag(readers, chunk_size, with_qual=True, with_comment=True):
+        L = readers[0].L
+        ps = (C.c_void_p * len(readers))(*[r.p for r in readers])
+        n = C.c_int(0)
+        a = L.mb_bseq_read_frag(len(readers), ps, chunk_size, int(with_qual), int(with_comment), C.byref(n))
+        return RefBseq._take(a, n.value)
+
+    def close(self):
+        if self.p:
+            self.L.mb_bseq_close(self.p)
+            self.p = None
+
+
 class RefMapIndex:
     """A prefix loaded by the reference's mb_idx_load (minibwa.h), for mb_anchor."""
 
