@@ -114,13 +114,14 @@ public:
 		if (at_end()) { tried_ = true; return -1; }
 		return (uint8_t)cur_[pos_++];
 	}
-	// bytes up to the delimiter (consumed; *dret = it, or 0 at the end); sep_line: '\n' and one
-	// trailing '\r' of the whole field dropped; else any isspace() byte.  Returns -1 when the stream
-	// was already at its end (the field untouched apart from a reset when !append), else the length.
-	int64_t until(bool sep_line, std::string &str, int *dret, bool append)
+	// bytes up to the delimiter (consumed; *dret = it, or 0 at the end) appended to the field
+	// v[st ..); sep_line: '\n', and one trailing '\r' of the whole field dropped; else any isspace()
+	// byte.  Returns -1 when the stream was already at its end (the field untouched apart from a reset
+	// when !append), else the field's length.
+	int64_t until(bool sep_line, std::vector<char> &v, size_t st, int *dret, bool append)
 	{
 		if (dret) *dret = 0;
-		if (!append) str.clear();
+		if (!append) v.resize(st);
 		if (at_end()) {
 			if (end_seen()) return -1;
 			tried_ = true;
@@ -133,7 +134,7 @@ public:
 				for (const char *p = b; p < e; ++p)
 					if (*p == ' ' || (*p >= '\t' && *p <= '\r')) { hit = p; break; }
 			const char *stop = hit ? hit : e;
-			str.append(b, (size_t)(stop - b));
+			v.insert(v.end(), b, stop);
 			pos_ = (size_t)(stop - cur_.data());
 			if (hit) {
 				if (dret) *dret = (uint8_t)*hit;
@@ -141,8 +142,8 @@ public:
 				break;
 			}
 		}
-		if (sep_line && str.size() > 1 && str.back() == '\r') str.pop_back();
-		return (int64_t)str.size();
+		if (sep_line && v.size() - st > 1 && v.back() == '\r') v.pop_back();
+		return (int64_t)(v.size() - st);
 	}
 
 private:
@@ -164,39 +165,42 @@ private:
 	bool eof_ = false, tried_ = false;
 };
 
+// the four fields of records, each appended at the end of its own array
 struct Rec {
-	std::string name, comment, seq, qual;
+	std::vector<char> name, comment, seq, qual;
+	void clear() { name.clear(), comment.clear(), seq.clear(), qual.clear(); }
 };
 
-// one record as kseq_read returns it: >= 0 the sequence length, -1 end of input, -2 a quality
-// string missing or of another length
+// one record as kseq_read returns it, its fields appended to r's arrays (from their sizes at the
+// call): >= 0 the sequence length, -1 end of input, -2 a quality string missing or of another length
 class Parser {
 public:
 	explicit Parser(Stream *s) : s_(s) {}
 	int64_t read(Rec &r)
 	{
 		int c;
+		const size_t n0 = r.name.size(), c0 = r.comment.size(), s0 = r.seq.size(), q0 = r.qual.size();
 		if (last_ == 0) {                                          // find the next header
 			while ((c = s_->getc()) != -1 && c != '>' && c != '@') {}
 			if (c == -1) return -1;
 			last_ = c;
 		}
-		r.comment.clear(), r.seq.clear(), r.qual.clear();
-		if (s_->until(false, r.name, &c, false) < 0) return -1;
-		if (c != '\n') s_->until(true, r.comment, nullptr, false);
+		if (s_->until(false, r.name, n0, &c, false) < 0) return -1;
+		if (c != '\n') s_->until(true, r.comment, c0, nullptr, false);
 		while ((c = s_->getc()) != -1 && c != '>' && c != '+' && c != '@') {
 			if (c == '\n') continue;
 			r.seq.push_back((char)c);
-			s_->until(true, r.seq, nullptr, true);
+			s_->until(true, r.seq, s0, nullptr, true);
 		}
+		const size_t sl = r.seq.size() - s0;
 		if (c == '>' || c == '@') last_ = c;
-		if (c != '+') return (int64_t)r.seq.size();
+		if (c != '+') return (int64_t)sl;
 		while ((c = s_->getc()) != -1 && c != '\n') {}
 		if (c == -1) return -2;
-		while (s_->until(true, r.qual, nullptr, true) >= 0 && r.qual.size() < r.seq.size()) {}
+		while (s_->until(true, r.qual, q0, nullptr, true) >= 0 && r.qual.size() - q0 < sl) {}
 		last_ = 0;
-		if (r.seq.size() != r.qual.size()) return -2;
-		return (int64_t)r.seq.size();
+		if (sl != r.qual.size() - q0) return -2;
+		return (int64_t)sl;
 	}
 
 private:
@@ -204,14 +208,13 @@ private:
 	int last_ = 0;
 };
 
-bool qname_same(const std::string &a, const std::string &b)      // mb_qname_same (bseq.h:26-39)
+bool qname_same(const char *a, size_t la, const char *b, size_t lb)   // mb_qname_same (bseq.h:26-39)
 {
-	auto len = [](const std::string &s) {
-		const size_t l = s.size();
+	auto len = [](const char *s, size_t l) {
 		return l >= 3 && s[l - 1] >= '0' && s[l - 1] <= '9' && s[l - 2] == '/' ? l - 2 : l;
 	};
-	const size_t la = len(a), lb = len(b);
-	return la == lb && a.compare(0, la, b, 0, lb) == 0;
+	la = len(a, la), lb = len(b, lb);
+	return la == lb && memcmp(a, b, la) == 0;
 }
 
 } // namespace
@@ -223,33 +226,52 @@ struct mbg_fq_s {
 	int32_t n_threads = 1;
 	bool have_pending = false;                  // the record frag mode read past the chunk (bseq.c:113)
 	int pend_q = 0, pend_c = 0;                 // ... converted with the flags of the call that read it
-	Rec pending, tmp;
+	Rec pending;
 };
 
+// A chunk: committed records, then possibly one record parsed past them (its fields at the ends of
+// the arrays).  qual runs parallel to seq: a record without a kept quality holds zeros there.
 struct mbg_fq_buf_s {
 	std::vector<int64_t> name_off, com_off, seq_off;
-	std::vector<char> name, comment, seq, qual;
+	Rec f;
 	std::vector<uint8_t> has_qual, nt4;
 	void clear()
 	{
 		name_off.assign(1, 0), com_off.assign(1, 0), seq_off.assign(1, 0);
-		name.clear(), comment.clear(), seq.clear(), qual.clear(), has_qual.clear(), nt4.clear();
-	}
-	// kseq2bseq (bseq.c:64-78): the quality kept when asked for and non-empty, the comment likewise
-	void push(const Rec &r, int with_qual, int with_comment)
-	{
-		name.insert(name.end(), r.name.begin(), r.name.end());
-		name_off.push_back((int64_t)name.size());
-		if (with_comment && !r.comment.empty()) comment.insert(comment.end(), r.comment.begin(), r.comment.end());
-		com_off.push_back((int64_t)comment.size());
-		seq.insert(seq.end(), r.seq.begin(), r.seq.end());
-		const bool q = with_qual && !r.qual.empty();
-		has_qual.push_back(q ? 1 : 0);
-		if (q) qual.insert(qual.end(), r.qual.begin(), r.qual.end());
-		else qual.resize(seq.size(), 0);
-		seq_off.push_back((int64_t)seq.size());
+		f.clear(), has_qual.clear(), nt4.clear();
 	}
 	int32_t n() const { return (int32_t)has_qual.size(); }
+	// kseq2bseq (bseq.c:64-78) of the record parsed past the committed ones: the quality kept when
+	// asked for and non-empty, the comment likewise
+	void commit(int with_qual, int with_comment)
+	{
+		const size_t s0 = (size_t)seq_off.back(), sl = f.seq.size() - s0;
+		if (!(with_comment && f.comment.size() > (size_t)com_off.back())) f.comment.resize((size_t)com_off.back());
+		const bool q = with_qual && f.qual.size() > s0;
+		if (!q) f.qual.assign(f.qual.begin(), f.qual.begin() + (ptrdiff_t)s0), f.qual.resize(s0 + sl, 0);
+		has_qual.push_back(q ? 1 : 0);
+		name_off.push_back((int64_t)f.name.size()), com_off.push_back((int64_t)f.comment.size());
+		seq_off.push_back((int64_t)f.seq.size());
+	}
+	void drop()                                 // forget the record parsed past the committed ones
+	{
+		f.name.resize((size_t)name_off.back()), f.comment.resize((size_t)com_off.back());
+		f.seq.resize((size_t)seq_off.back()), f.qual.resize((size_t)seq_off.back());
+	}
+	void take(Rec &r) const                     // a copy of the record parsed past the committed ones
+	{
+		r.name.assign(f.name.begin() + name_off.back(), f.name.end());
+		r.comment.assign(f.comment.begin() + com_off.back(), f.comment.end());
+		r.seq.assign(f.seq.begin() + seq_off.back(), f.seq.end());
+		r.qual.assign(f.qual.begin() + seq_off.back(), f.qual.end());
+	}
+	void put(const Rec &r)                      // r as the record parsed past the committed ones
+	{
+		f.name.insert(f.name.end(), r.name.begin(), r.name.end());
+		f.comment.insert(f.comment.end(), r.comment.begin(), r.comment.end());
+		f.seq.insert(f.seq.end(), r.seq.begin(), r.seq.end());
+		f.qual.insert(f.qual.end(), r.qual.begin(), r.qual.end());
+	}
 };
 
 namespace {
@@ -258,11 +280,11 @@ namespace {
 void encode(mbg_fq_buf_s &b, int n_threads)
 {
 	const uint8_t *t = nt4_table();
-	const int64_t n = (int64_t)b.seq.size();
+	const int64_t n = (int64_t)b.f.seq.size();
 	b.nt4.resize((size_t)n);
 	auto work = [&](int64_t lo, int64_t hi) {
 		for (int64_t i = lo; i < hi; ++i) {
-			char &c = b.seq[(size_t)i];
+			char &c = b.f.seq[(size_t)i];
 			if (c == 'u' || c == 'U') --c;
 			b.nt4[(size_t)i] = t[(uint8_t)c];
 		}
@@ -279,7 +301,7 @@ void view(const mbg_fq_buf_s &b, int32_t status, mbg_fq_chunk_t *out)
 	out->n = b.n();
 	out->status = status;
 	out->name_off = b.name_off.data(), out->com_off = b.com_off.data(), out->seq_off = b.seq_off.data();
-	out->name = b.name.data(), out->comment = b.comment.data(), out->seq = b.seq.data(), out->qual = b.qual.data();
+	out->name = b.f.name.data(), out->comment = b.f.comment.data(), out->seq = b.f.seq.data(), out->qual = b.f.qual.data();
 	out->has_qual = b.has_qual.data(), out->nt4 = b.nt4.data();
 }
 
@@ -329,28 +351,31 @@ int mbg_fq_read(mbg_fq_t *q, int64_t chunk_size, int32_t with_qual, int32_t with
 	b->clear();
 	int64_t size = 0, ret;
 	if (q->have_pending) {
-		b->push(q->pending, q->pend_q, q->pend_c);
+		b->put(q->pending);
+		b->commit(q->pend_q, q->pend_c);
 		size = (int64_t)q->pending.seq.size();
 		q->have_pending = false;
 	}
 	if (max_chunk_size < chunk_size) max_chunk_size = chunk_size;
-	std::string last;                                               // the name of the chunk's last record
-	while ((ret = q->ps->read(q->tmp)) >= 0) {
-		b->push(q->tmp, with_qual, with_comment);
-		size += (int64_t)q->tmp.seq.size();
+	while ((ret = q->ps->read(b->f)) >= 0) {
+		b->commit(with_qual, with_comment);
+		size += ret;
 		bool stop;
 		if (chunk_size <= 0 || max_chunk_size <= 0) stop = true;
 		else if (size >= max_chunk_size) stop = true;
 		else stop = size >= chunk_size && b->n() >= min_cnt;
 		if (stop) {
-			if (frag_mode && q->tmp.seq.size() < kPairCheck) {
-				last = q->tmp.name;
-				while ((ret = q->ps->read(q->pending)) >= 0) {
-					size += (int64_t)q->pending.seq.size();
-					if (qname_same(q->pending.name, last)) {
-						b->push(q->pending, with_qual, with_comment);
-						last = q->pending.name;
+			if (frag_mode && (uint64_t)ret < kPairCheck) {
+				while ((ret = q->ps->read(b->f)) >= 0) {
+					size += ret;
+					const int64_t *no = b->name_off.data();
+					const int32_t k = b->n();
+					const char *nm = b->f.name.data();
+					if (qname_same(nm + no[k], b->f.name.size() - (size_t)no[k], nm + no[k - 1], (size_t)(no[k] - no[k - 1]))) {
+						b->commit(with_qual, with_comment);
 					} else {
+						b->take(q->pending);
+						b->drop();
 						q->have_pending = true, q->pend_q = with_qual, q->pend_c = with_comment;
 						break;
 					}
@@ -359,6 +384,7 @@ int mbg_fq_read(mbg_fq_t *q, int64_t chunk_size, int32_t with_qual, int32_t with
 			break;
 		}
 	}
+	b->drop();                                                      // a record cut by an error or the end
 	encode(*b, q->n_threads);
 	view(*b, ret < -1 ? (int32_t)ret : (q->src->failed() ? MBG_EIO : 0), out);
 	return MBG_OK;
@@ -374,11 +400,14 @@ int mbg_fq_read_frag(int32_t n_fp, mbg_fq_t **q, int64_t chunk_size, int32_t wit
 	std::vector<Rec> recs((size_t)n_fp);
 	for (;;) {                                                      // bseq.c:135-158: one record of each file per step
 		int n_read = 0;
-		for (int32_t i = 0; i < n_fp; ++i)
+		for (int32_t i = 0; i < n_fp; ++i) {
+			recs[(size_t)i].clear();
 			if (q[i]->ps->read(recs[(size_t)i]) >= 0) ++n_read;
+		}
 		if (n_read < n_fp) break;
 		for (int32_t i = 0; i < n_fp; ++i) {
-			b->push(recs[(size_t)i], with_qual, with_comment);
+			b->put(recs[(size_t)i]);
+			b->commit(with_qual, with_comment);
 			size += (int64_t)recs[(size_t)i].seq.size();
 		}
 		if (size >= chunk_size) break;
