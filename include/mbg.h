@@ -262,7 +262,8 @@ typedef struct {
 /* Chaining of n_reads reads' anchors: mb_lchain_dp (lchain.c:145-240) -- the DP with its skip and
  * iteration limits, the backtrack (lchain.c:44-92) and the compaction (lchain.c:94-123).  Read r's
  * anchors are anc[anc_off[r] .. anc_off[r] + n_anc[r]) in mb_anchor's order (anc_off has n_reads+1
- * entries, anc_off[0] = 0, n_anc[r] <= anc_off[r+1] - anc_off[r]).  Its chains (score<<32 | anchor
+ * entries, anc_off[0] = 0, n_anc[r] <= anc_off[r+1] - anc_off[r], n_anc[r] < 2^31 as lchain.c's int32
+ * indices require).  Its chains (score<<32 | anchor
  * count, ordered by the first anchor's concatenated position) go to u[anc_off[r] ...], n_u[r] of them,
  * and their anchors, first to last, to out[anc_off[r] ...], n_out[r] of them.  Anchors' contigs must
  * be those of the resident l2b.  Results equal the reference's. */
@@ -273,7 +274,7 @@ int mbg_lchain_batch(mbg_ctx_t *ctx, int32_t n_reads, const mbg_anchor_t *anc, c
 
 /* A reader of one FASTA/FASTQ file, plain or gzip ("-" or NULL: standard input): mb_bseq_open
  * (bseq.c:41-52).  A read-ahead thread decompresses while the caller parses; n_threads (1-64) encode
- * each chunk's bases. */
+ * each chunk's bases.  One reader is used by one thread at a time. */
 typedef struct mbg_fq_s mbg_fq_t;
 /* Storage of one chunk, reused from call to call (the views below point into it). */
 typedef struct mbg_fq_buf_s mbg_fq_buf_t;
