@@ -146,6 +146,15 @@ public:
 		return (int64_t)(v.size() - st);
 	}
 
+	// the unread bytes of the current block (fetching the next block when it is used up)
+	bool span(const char **b, const char **e)
+	{
+		if (at_end()) return false;
+		*b = cur_.data() + pos_, *e = cur_.data() + cur_.size();
+		return true;
+	}
+	void skip(size_t n) { pos_ += n; }
+
 private:
 	bool at_end()
 	{
@@ -185,6 +194,8 @@ public:
 			if (c == -1) return -1;
 			last_ = c;
 		}
+		const int64_t fl = fast(r);
+		if (fl >= 0) return fl;
 		if (s_->until(false, r.name, n0, &c, false) < 0) return -1;
 		if (c != '\n') s_->until(true, r.comment, c0, nullptr, false);
 		while ((c = s_->getc()) != -1 && c != '>' && c != '+' && c != '@') {
@@ -204,6 +215,38 @@ public:
 	}
 
 private:
+	// A FASTQ record whose four lines lie whole in the current block, with a one-line sequence and a
+	// one-line quality of the same length and no '\r' at a line end, read at memchr speed.  For such a
+	// record the general path below takes the same fields and leaves the stream at the same byte with
+	// the same state; anything else (returns -1, nothing consumed) goes to the general path.
+	int64_t fast(Rec &r)
+	{
+		const char *b, *e;
+		if (!s_->span(&b, &e)) return -1;
+		const char *l1 = (const char*)memchr(b, '\n', (size_t)(e - b));
+		if (!l1) return -1;
+		const char *sq = l1 + 1;
+		if (sq >= e || *sq == '>' || *sq == '+' || *sq == '@' || *sq == '\n') return -1;
+		const char *l2 = (const char*)memchr(sq, '\n', (size_t)(e - sq));
+		if (!l2 || l2[-1] == '\r' || l2 + 1 >= e || l2[1] != '+') return -1;
+		const char *l3 = (const char*)memchr(l2 + 1, '\n', (size_t)(e - l2 - 1));
+		if (!l3) return -1;
+		const char *qs = l3 + 1;
+		const size_t sl = (size_t)(l2 - sq);
+		if ((size_t)(e - qs) <= sl || qs[sl] != '\n' || (sl > 0 && qs[sl - 1] == '\r')) return -1;
+		if (memchr(qs, '\n', sl)) return -1;                     // the quality is one line
+		const char *nm = b;
+		while (nm < l1 && !(*nm == ' ' || (*nm >= '\t' && *nm <= '\r'))) ++nm;
+		if (nm < l1 && *nm != '\n' && l1[-1] == '\r') return -1;  // a comment ending in '\r'
+		if (nm == l1 && l1 > b && l1[-1] == '\r') return -1;
+		r.name.insert(r.name.end(), b, nm);
+		if (nm < l1) r.comment.insert(r.comment.end(), nm + 1, l1);
+		r.seq.insert(r.seq.end(), sq, l2);
+		r.qual.insert(r.qual.end(), qs, qs + sl);
+		s_->skip((size_t)(qs + sl + 1 - b));
+		last_ = 0;
+		return (int64_t)sl;
+	}
 	Stream *s_;
 	int last_ = 0;
 };
