@@ -134,3 +134,43 @@ def test_fuzz(tmp_path, seed):
 def test_reader_errors(tmp_path):
     with pytest.raises(mbg.MbgError):
         mbg.FastxReader(str(tmp_path / "missing.fq"))
+
+
+def _mixed_records(rng, n):
+    """Standard 4-line FASTQ records mixed with ones the block-level path must leave to the general
+    one: CRLF, multi-line sequence or quality, comments, '+name' lines, '@' leading a quality, empty
+    sequences, blank lines, lowercase and U, FASTA records."""
+    out = []
+    for i in range(n):
+        L = int(rng.integers(0, 300)) if rng.random() < 0.1 else int(rng.integers(20, 200))
+        seq = bytes(rng.choice(np.frombuffer(b"ACGTNacgtU", np.uint8), L))
+        qual = bytes(rng.choice(np.frombuffer(b"!#5?@IJ+>", np.uint8), L))
+        kind = rng.integers(0, 10)
+        name = b"m%d" % i + (b"/1" if i % 2 == 0 else b"/2")
+        if kind <= 4:
+            out.append(b"@" + name + b"\n" + seq + b"\n+\n" + qual + b"\n")
+        elif kind == 5:
+            out.append(b"@" + name + b" c=1\tx\r\n" + seq + b"\r\n+\r\n" + qual + b"\r\n")
+        elif kind == 6 and L > 4:
+            h = L // 2
+            out.append(b"@" + name + b"\n" + seq[:h] + b"\n" + seq[h:] + b"\n+" + name + b"\n" + qual[:h] + b"\n" + qual[h:] + b"\n")
+        elif kind == 7:
+            out.append(b">" + name + b" fa\n" + seq + b"\n\n")
+        elif kind == 8:
+            out.append(b"@" + name + b"\n" + seq + b"\n+\n@" + qual[1:] + b"\n" if L else b"@" + name + b"\n\n+\n\n")
+        else:
+            out.append(b"@" + name + b"\n" + seq + b"\n+\n" + qual + b"\n\n")
+    return b"".join(out)
+
+
+@pytest.mark.parametrize("gz", [False, True])
+def test_mixed_records_across_blocks(tmp_path, gz):
+    """Several MB of mixed records, so the block-level path and the general path alternate and records
+    straddle the reader's 1 MiB block boundaries."""
+    rng = np.random.default_rng(17)
+    data = _mixed_records(rng, 40000)
+    assert len(data) > 3 << 20
+    f = str(tmp_path / ("mixed.fq" + (".gz" if gz else "")))
+    _write(f, data, gz)
+    assert _compare(f, PARAMS) > 39000
+    assert _compare(f, [dict(chunk_size=2_000_000, frag_mode=True, min_cnt=100)]) > 39000
