@@ -83,6 +83,8 @@ def main():
                     help="only run mb_anchor and then mb_lchain_dp over the seeding output of the read pairs (default 1M "
                          "pairs of 2x150 bp on the 3.1 Gbp reference, max_occ 250, the mapper's chaining options for "
                          "150 bp); reports reads/s of each")
+    ap.add_argument("--long", type=int, default=0, metavar="N",
+                    help="with --anchor-batch: N single 10 kb reads (0.1%% substitutions) instead of the read pairs")
     ap.add_argument("--fastq-batch", type=int, default=0, metavar="N",
                     help="only run the FASTQ reader (host): N 150 bp reads in one file, plain and gzip, in the CLI's "
                          "100M-base chunks; reports records/s beside the reference's mb_bseq_read")
@@ -163,6 +165,9 @@ def main():
     log(f"{args.pairs} pairs generated and resident in {time.time() - t0:.1f}s")
     opt = mbg.adaptive_opt(rl)
     if args.anchor_batch:
+        if args.long:
+            seq, _, _ = synth.single_reads(ref, args.long, seed=13, lmin=10000, lmax=10000, sub=0.001, indel=0.0005)
+            reads = seq.reshape(args.long, 10000)
         anchor_batch(ctx, reads, prefix, args)
         ctx.close()
         return
