@@ -136,7 +136,13 @@ __global__ void __launch_bounds__(64) anc_rows_kernel(AncArgs a)
 		if (i < n && u[i].x0 == u[i0].x0 && u[i].size == u[i0].size) continue;
 		const uint64_t size = u[i0].size;
 		const int64_t ns = n_located(size, a.max_occ);
-		if (m + ns > cap) { a.nu[r] = -1; return; }
+		if (m + ns > cap) {
+			// every slot still gets a row: a call that reuses the workspace holds located values there
+			// (row 0 locates to 2^64-1, the '$' suffix), and the SA walk must never take one for a row
+			for (int64_t k = 0; k < cap; ++k) rows[k] = 0;
+			a.nu[r] = -1;
+			return;
+		}
 		if ((int64_t)size <= a.max_occ)
 			for (int64_t j = 0; j < ns; ++j) rows[m++] = u[i0].x0 + (uint64_t)j;
 		else

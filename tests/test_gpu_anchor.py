@@ -110,3 +110,45 @@ def test_anchor_errors(rep):
         ctx.anchor([bad], [100], 250)
     with pytest.raises(mbg.MbgError, match="invalid"):
         ctx.anchor([np.zeros((0, 4), np.uint64)], [100], -1)
+
+
+def test_anchor_dev_short_slots_twice(rep):
+    """mbg_anchor_batch_dev twice on one workspace, one read given a slot fewer than it needs: that read
+    gets n_anc -1 on both calls (its rows are all rewritten, so the second call's SA walk never sees a
+    value the first call located there), every other read the host entry point's anchors."""
+    import ctypes as C
+    ctx, oi, ref, ctg_len = rep
+    P = C.c_void_p
+    reads = _reads(ref)[:300]
+    ivs = ctx.seed_intv_batch(reads, 19, 10)
+    n = len(reads)
+    want, _ = ctx.anchor(ivs, [len(r) for r in reads], 250)
+    us = [np.asarray(u, np.uint64).reshape(-1, 4) for u in ivs]
+    uoff = np.zeros(n + 1, np.int64)
+    np.cumsum([len(u) for u in us], out=uoff[1:])
+    flat = np.ascontiguousarray(np.concatenate(us))
+    slots = mbg.anchor_slots(us, 250)
+
+    def need(u):                                   # rows located once per group of equal (x[0], size)
+        g = np.unique(u[:, [0, 2]], axis=0) if len(u) else np.zeros((0, 2), np.uint64)
+        return int(np.minimum(g[:, 1], 250).sum())
+    short = next(i for i in range(n) if need(us[i]) >= 3)
+    slots[short] = need(us[short]) - 1
+    aoff = np.zeros(n + 1, np.int64)
+    np.cumsum(slots, out=aoff[1:])
+    ns = int(aoff[-1])
+    d_u0, d_u = ctx.to_device(flat), ctx.to_device(flat)
+    d_uoff, d_ql, d_aoff = ctx.to_device(uoff), ctx.to_device(np.array([len(r) for r in reads], np.int32)), ctx.to_device(aoff)
+    d_rows, d_anc = ctx.to_device(np.zeros(ns, np.uint64)), ctx.to_device(np.zeros(ns, mbg.ANCHOR_DTYPE))
+    d_nanc, d_nu = ctx.to_device(np.zeros(n, np.int64)), ctx.to_device(np.zeros(n, np.int64))
+    for call in range(2):
+        mbg.check(ctx.L.mbg_stream_copy(ctx.h, None, P(d_u.ptr), P(d_u0.ptr), flat.nbytes), "mbg_stream_copy")
+        mbg.check(ctx.L.mbg_anchor_batch_dev(ctx.h, None, n, ns, P(d_u.ptr), P(d_uoff.ptr), P(d_ql.ptr), 250, P(d_rows.ptr),
+                                             P(d_anc.ptr), P(d_aoff.ptr), P(d_nanc.ptr), P(d_nu.ptr)), "mbg_anchor_batch_dev")
+        nanc, anc = d_nanc.numpy(), d_anc.numpy()
+        assert nanc[short] == -1, call
+        for i in range(n):
+            if i != short:
+                assert nanc[i] == len(want[i]) and (anc[aoff[i]:aoff[i] + nanc[i]] == want[i]).all(), (call, i)
+    for d in (d_u0, d_u, d_uoff, d_ql, d_aoff, d_rows, d_anc, d_nanc, d_nu):
+        d.free()
