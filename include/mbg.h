@@ -270,6 +270,47 @@ typedef struct {
 int mbg_lchain_batch(mbg_ctx_t *ctx, int32_t n_reads, const mbg_anchor_t *anc, const int64_t *anc_off, const int64_t *n_anc,
                      const mbg_chain_opt_t *opt, uint64_t *u, int32_t *n_u, mbg_anchor_t *out, int64_t *n_out);
 
+/* ---- the mapper's front pass: seeding -> anchors -> chains of one mini-batch -------------------- */
+
+/* Options of mbg_map_front: seeding's min_len and max_sub_occ (mb_seed_intv_batch), mb_anchor's
+ * max_occ, and n_copt sets of mb_lchain_dp parameters, read r using copt[copt_idx[r]] (the mapper
+ * derives them from mb_opt_adap for each read length, map-algo.c:606-612).  no_chain (may be NULL):
+ * reads r with no_chain[r] != 0 are seeded only (methylation-converted reads, whose anchors the
+ * reference builds with a conversion-aware mb_anchor). */
+typedef struct {
+	int32_t min_len, max_sub_occ, max_occ, n_copt;
+	const mbg_chain_opt_t *copt;
+	const int32_t *copt_idx;
+	const uint8_t *no_chain;
+} mbg_front_opt_t;
+
+/* Results of mbg_map_front, in host memory the context owns (pinned), valid until its next
+ * mbg_map_front call.  Read r:
+ *   intervals  intv[intv_off[r] .. + n_intv[r]): mb_seed_intv_batch's set for the read, sorted and
+ *              deduplicated as mb_seed_sort_dedup leaves it (seed.c:109-138);
+ *   anchors    anc[anc_off[r] .. + n_anc[r]): mb_anchor's result in its order (n_anc -1 for no_chain);
+ *   chains     u[anc_off[r] .. + n_u[r]) and chn[anc_off[r] .. + n_chn[r]): mb_lchain_dp's chain list
+ *              (score<<32 | anchor count, by first anchor's position) and chained anchors. */
+typedef struct {
+	int64_t n_rec, n_slots;
+	const int64_t *intv_off, *n_intv;
+	const mbg_sai_t *intv;
+	const int64_t *anc_off, *n_anc;
+	const mbg_anchor_t *anc;
+	const int32_t *n_u;
+	const uint64_t *u;
+	const int64_t *n_chn;
+	const mbg_anchor_t *chn;
+	float ms_seed, ms_group, ms_anchor, ms_chain, ms_copy, ms_total;   /* device time per phase */
+} mbg_front_t;
+
+/* The front of mb_map_sai (map-algo.c:588-631) for n_reads reads whose nt4 bases are at
+ * seq + off[r], len[r] of them (host memory): two-pass seeding (mb_seed_intv_batch, seed.c:56-103),
+ * mb_anchor (seed.c:268-339) and the first mb_lchain_dp (lchain.c:145-231), in one device pass with
+ * no host step between them.  Results equal the reference's. */
+int mbg_map_front(mbg_ctx_t *ctx, int32_t n_reads, const uint8_t *seq, const int64_t *off, const int32_t *len,
+                  const mbg_front_opt_t *opt, mbg_front_t *out);
+
 /* ---- reads at the edge: FASTA/FASTQ (host code) ------------------------------------------------ */
 
 /* A reader of one FASTA/FASTQ file, plain or gzip ("-" or NULL: standard input): mb_bseq_open
@@ -359,6 +400,10 @@ int mbg_dev_alloc(mbg_ctx_t *ctx, size_t bytes, void **ptr);
 int mbg_dev_free(mbg_ctx_t *ctx, void *ptr);
 int mbg_copy_to_device(mbg_ctx_t *ctx, void *dst_dev, const void *src, size_t bytes);
 int mbg_copy_to_host(mbg_ctx_t *ctx, void *dst, const void *src_dev, size_t bytes);
+/* Page-locked host memory, for staging buffers that the host-pointer entry points copy from and to
+ * at full PCIe rate. */
+int mbg_host_alloc(mbg_ctx_t *ctx, size_t bytes, void **ptr);
+int mbg_host_free(mbg_ctx_t *ctx, void *ptr);
 
 /* A plain streaming copy kernel (16 B per lane per iteration), used to calibrate HBM byte
  * counters: its FETCH/WRITE counters are the reference for `bytes` read and written. */

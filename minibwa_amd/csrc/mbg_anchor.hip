@@ -289,8 +289,13 @@ __global__ void __launch_bounds__(64) lchain_kernel(ChainArgs a)
 	uint64_t *u = a.ut + o;
 	int32_t n_u = 0;
 	int64_t n_v = 0;
-	const int32_t bw = a.bw, max_drop = a.bw;
-	const int32_t mdx = a.max_dist_x < bw ? bw : a.max_dist_x, mdy = a.max_dist_y < bw ? bw : a.max_dist_y;
+	ChainOpt co;
+	if (a.opts) co = a.opts[a.opt_idx[r]];
+	else co.max_dist_x = a.max_dist_x, co.max_dist_y = a.max_dist_y, co.bw = a.bw, co.max_skip = a.max_skip, co.max_iter = a.max_iter,
+	     co.min_sc = a.min_sc, co.pen_gap = a.pen_gap;
+	const int32_t bw = co.bw, max_drop = co.bw, max_iter = co.max_iter, max_skip = co.max_skip, min_sc = co.min_sc;
+	const float pen_gap = co.pen_gap;
+	const int32_t mdx = co.max_dist_x < bw ? bw : co.max_dist_x, mdy = co.max_dist_y < bw ? bw : co.max_dist_y;
 	a.n_u[r] = 0, a.n_b[r] = 0;
 	if (n <= 0) return;
 	for (int64_t i = 0; i < n; ++i) t[i] = 0;
@@ -300,17 +305,17 @@ __global__ void __launch_bounds__(64) lchain_kernel(ChainArgs a)
 		const AncRec ai = an[i];
 		int64_t max_j = -1, j;
 		int32_t max_f = ai.len, n_skip = 0;
-		for (j = i - 1; j >= 0 && j >= i - a.max_iter; --j) {
+		for (j = i - 1; j >= 0 && j >= i - max_iter; --j) {
 			const AncRec aj = an[j];
 			if (ai.tpos - aj.tpos >= (int64_t)mdx + ai.len) break;
-			int32_t sc = chain_sc(ai, aj, mdx, mdy, bw, a.pen_gap);
+			int32_t sc = chain_sc(ai, aj, mdx, mdy, bw, pen_gap);
 			if (sc == INT32_MIN) continue;
 			sc += f[j];
 			if (sc > max_f) {
 				max_f = sc, max_j = j;
 				if (n_skip > 0) --n_skip;
 			} else if (t[j] == (int32_t)i) {
-				if (++n_skip > a.max_skip) break;
+				if (++n_skip > max_skip) break;
 			}
 			if (p[j] >= 0) t[p[j]] = (int32_t)i;
 		}
@@ -322,7 +327,7 @@ __global__ void __launch_bounds__(64) lchain_kernel(ChainArgs a)
 				if (mx < f[j]) mx = f[j], max_ii = j;
 		}
 		if (max_ii >= 0 && max_ii < end_j) {
-			const int32_t tmp = chain_sc(ai, an[max_ii], mdx, mdy, bw, a.pen_gap);
+			const int32_t tmp = chain_sc(ai, an[max_ii], mdx, mdy, bw, pen_gap);
 			if (tmp != INT32_MIN && max_f < tmp + f[max_ii]) max_f = tmp + f[max_ii], max_j = max_ii;
 		}
 		f[i] = max_f, p[i] = max_j;
@@ -332,7 +337,7 @@ __global__ void __launch_bounds__(64) lchain_kernel(ChainArgs a)
 	int32_t bb[256], be[256];
 	int64_t n_z = 0;
 	for (int64_t i = 0; i < n; ++i)
-		if (f[i] >= a.min_sc) z[n_z].x = (uint64_t)(int64_t)f[i], z[n_z].y = (uint64_t)i, ++n_z;
+		if (f[i] >= min_sc) z[n_z].x = (uint64_t)(int64_t)f[i], z[n_z].y = (uint64_t)i, ++n_z;
 	if (n_z == 0) return;
 	rs_sort(z, n_z, KeyU128(), bb, be);
 	for (int64_t i = 0; i < n; ++i) t[i] = 0;
@@ -342,7 +347,7 @@ __global__ void __launch_bounds__(64) lchain_kernel(ChainArgs a)
 		int64_t i = (int64_t)z[k].y;
 		for (; i != end_i; i = p[i]) v[n_v++] = (int32_t)i, t[i] = 1;
 		const int32_t sc = i < 0 ? (int32_t)z[k].x : (int32_t)z[k].x - f[i];
-		if (sc >= a.min_sc && n_v > n_v0) u[n_u++] = (uint64_t)(uint32_t)sc << 32 | (uint64_t)(n_v - n_v0);
+		if (sc >= min_sc && n_v > n_v0) u[n_u++] = (uint64_t)(uint32_t)sc << 32 | (uint64_t)(n_v - n_v0);
 		else n_v = n_v0;
 	}
 	if (n_u == 0) return;

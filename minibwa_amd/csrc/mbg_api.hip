@@ -48,6 +48,9 @@ struct mbg_ctx_s {
 	DBuf m_buf[12];         // mate rescue
 	DBuf a_buf[8];          // anchors
 	DBuf c_buf[8];          // chains
+	DBuf f_buf[20];         // the mapper's front pass
+	DBuf h_front[10];       // its results in pinned host memory
+	hipEvent_t fev[6] = {};
 	int64_t cap_task = 0, cap_spill = 0;
 	int64_t cap_rec = 0, cap_rows = 0;
 	hipEvent_t ev[8] = {};
@@ -71,6 +74,19 @@ static void dfree(DBuf &b)
 {
 	if (b.p) hipFree(b.p);
 	b.p = nullptr, b.n = 0;
+}
+
+// grow-only pinned host buffer
+static int hgrow(DBuf &b, size_t bytes)
+{
+	if (bytes == 0) bytes = 16;
+	if (b.n >= bytes) return MBG_OK;
+	if (b.p) { hipHostFree(b.p); b.p = nullptr; b.n = 0; }
+	size_t n = bytes + bytes / 4 + 256;
+	hipError_t e = hipHostMalloc(&b.p, n, 0);
+	if (e != hipSuccess) { g_hip_err = std::string("hipHostMalloc: ") + hipGetErrorString(e); b.p = nullptr; return MBG_ENOMEM; }
+	b.n = n;
+	return MBG_OK;
 }
 
 #define TRY(x) do { int r_ = (x); if (r_ != MBG_OK) return r_; } while (0)
@@ -151,7 +167,10 @@ void mbg_destroy(mbg_ctx_t *c)
 	for (auto &b : c->m_buf) dfree(b);
 	for (auto &b : c->a_buf) dfree(b);
 	for (auto &b : c->c_buf) dfree(b);
+	for (auto &b : c->f_buf) dfree(b);
+	for (auto &b : c->h_front) if (b.p) hipHostFree(b.p);
 	for (hipEvent_t &e : c->ev) if (e) hipEventDestroy(e);
+	for (hipEvent_t &e : c->fev) if (e) hipEventDestroy(e);
 	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
 	if (c->stream) hipStreamDestroy(c->stream);
 	delete c;
@@ -699,12 +718,30 @@ static int dp_batch(mbg_ctx_t *c, bool dual, int32_t n_jobs, const mbg_dp_job_t 
 	                  m, mat, (DpRes*)c->b_out[0].p, any_cigar ? (uint32_t*)c->b_out[1].p : nullptr, cap, slab, qt, tcap));
 	HIPCHK(hipMemcpyAsync(res, c->b_out[0].p, (size_t)n_jobs * sizeof(DpRes), hipMemcpyDeviceToHost, c->stream));
 	HIPCHK(hipStreamSynchronize(c->stream));
-	if (any_cigar) {
-		std::vector<uint32_t> tmp(cigar_cap);
-		HIPCHK(hipMemcpy(tmp.data(), c->b_out[1].p, (size_t)cigar_cap * 4, hipMemcpyDeviceToHost));
-		for (int32_t i = 0; i < n_jobs; ++i)
-			if (!(jobs[i].flag & 0x01) && res[i].n_cigar > 0)
-				memcpy(cigar + jobs[i].cigar_off, tmp.data() + jobs[i].cigar_off, (size_t)res[i].n_cigar * 4);
+	if (any_cigar) {           // only the operations written cross PCIe: packed on the device at host-computed offsets
+		std::vector<int64_t> poff(n_jobs + 1);
+		int64_t tot = 0;
+		for (int32_t i = 0; i < n_jobs; ++i) {
+			poff[i] = tot;
+			if (!(jobs[i].flag & 0x01)) {
+				if (res[i].n_cigar < 0 || res[i].n_cigar > jobs[i].qlen + jobs[i].tlen) return MBG_EHIP;
+				tot += res[i].n_cigar;
+			}
+		}
+		poff[n_jobs] = tot;
+		if (tot > 0) {
+			TRY(dgrow(c->b_in[5], (size_t)(n_jobs + 1) * 8)); TRY(dgrow(c->b_out[2], (size_t)tot * 4));
+			TRY(hgrow(c->h_pinned, (size_t)tot * 4));
+			HIPCHK(hipMemcpyAsync(c->b_in[5].p, poff.data(), (size_t)(n_jobs + 1) * 8, hipMemcpyHostToDevice, c->stream));
+			HIPCHK(launch_cigar_pack((const DpJob*)c->b_in[4].p, (const DpRes*)c->b_out[0].p, n_jobs, (const int64_t*)c->b_in[5].p,
+			                         (const uint32_t*)c->b_out[1].p, (uint32_t*)c->b_out[2].p, c->stream));
+			HIPCHK(hipMemcpyAsync(c->h_pinned.p, c->b_out[2].p, (size_t)tot * 4, hipMemcpyDeviceToHost, c->stream));
+			HIPCHK(hipStreamSynchronize(c->stream));
+			const uint32_t *pk = (const uint32_t*)c->h_pinned.p;
+			for (int32_t i = 0; i < n_jobs; ++i)
+				if (!(jobs[i].flag & 0x01) && res[i].n_cigar > 0)
+					memcpy(cigar + jobs[i].cigar_off, pk + poff[i], (size_t)res[i].n_cigar * 4);
+		}
 	}
 	return MBG_OK;
 }
@@ -871,10 +908,11 @@ int mbg_anchor_batch(mbg_ctx_t *c, int32_t n_reads, mbg_sai_t *intv, const int64
 
 static int lchain_dev_locked(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, int64_t n_slots, const AncRec *anc, const int64_t *aoff,
                              const int64_t *nanc, const mbg_chain_opt_t &o, void *work, size_t work_bytes, uint64_t *u, int32_t *n_u,
-                             AncRec *out, int64_t *n_out)
+                             AncRec *out, int64_t *n_out, const ChainOpt *opts = nullptr, const int32_t *opt_idx = nullptr)
 {
 	if (work_bytes < MBG_LCHAIN_WORK_BYTES(n_slots)) return MBG_EARG;
 	ChainArgs a{};
+	a.opts = opts, a.opt_idx = opt_idx;
 	a.n_reads = n_reads, a.anc = anc, a.aoff = aoff, a.nanc = nanc;
 	a.max_dist_x = o.max_dist_x, a.max_dist_y = o.max_dist_y, a.bw = o.bw, a.max_skip = o.max_skip, a.max_iter = o.max_iter;
 	a.min_sc = o.min_sc, a.pen_gap = o.pen_gap;
@@ -938,6 +976,135 @@ int mbg_lchain_batch(mbg_ctx_t *c, int32_t n_reads, const mbg_anchor_t *anc, con
 		HIPCHK(hipMemcpyAsync(out, b[6].p, (size_t)n_slots * sizeof(AncRec), hipMemcpyDeviceToHost, st));
 	}
 	HIPCHK(hipStreamSynchronize(st));
+	return MBG_OK;
+}
+
+// ---- the mapper's front pass ---------------------------------------------------------------------
+
+static_assert(sizeof(mbg_chain_opt_t) == sizeof(ChainOpt), "chain option layout");
+
+int mbg_map_front(mbg_ctx_t *c, int32_t n, const uint8_t *seq, const int64_t *off, const int32_t *len, const mbg_front_opt_t *o,
+                  mbg_front_t *out)
+{
+	if (c == nullptr || o == nullptr || out == nullptr || n < 0 || o->min_len < 1 || o->max_occ < 1 || o->n_copt < 0) return MBG_EARG;
+	if (n > 0 && (!seq || !off || !len || !o->copt || !o->copt_idx || o->n_copt < 1)) return MBG_EARG;
+	for (int32_t r = 0; r < n; ++r)
+		if (len[r] < 0 || off[r] < 0 || off[r + 1] < off[r] + len[r] || o->copt_idx[r] < 0 || o->copt_idx[r] >= o->n_copt) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt || !c->has_l2b) return MBG_ENOIDX;
+	memset(out, 0, sizeof(*out));
+	HIPCHK(hipSetDevice(c->device));
+	hipStream_t st = c->stream;
+	for (hipEvent_t &e : c->fev) if (!e) HIPCHK(hipEventCreate(&e));
+	DBuf *f = c->f_buf, *h = c->h_front;
+	const int64_t tot = n > 0 ? off[n] : 0;
+	// inputs
+	TRY(dgrow(f[0], tot + 16)); TRY(dgrow(f[1], (size_t)(n + 1) * 8)); TRY(dgrow(f[2], (size_t)(n + 1) * 4));
+	TRY(dgrow(f[3], (size_t)(n + 1) * 4)); TRY(dgrow(f[4], (size_t)o->n_copt * sizeof(ChainOpt) + 16));
+	HIPCHK(hipEventRecord(c->fev[0], st));
+	if (n > 0) {
+		HIPCHK(hipMemcpyAsync(f[0].p, seq, tot, hipMemcpyHostToDevice, st));
+		HIPCHK(hipMemcpyAsync(f[1].p, off, (size_t)(n + 1) * 8, hipMemcpyHostToDevice, st));
+		HIPCHK(hipMemcpyAsync(f[2].p, len, (size_t)n * 4, hipMemcpyHostToDevice, st));
+		HIPCHK(hipMemcpyAsync(f[3].p, o->copt_idx, (size_t)n * 4, hipMemcpyHostToDevice, st));
+		HIPCHK(hipMemcpyAsync(f[4].p, o->copt, (size_t)o->n_copt * sizeof(ChainOpt), hipMemcpyHostToDevice, st));
+	}
+	// 1. seeding (grows its pools and repeats on overflow)
+	int64_t n_rec = 0;
+	int64_t *n_rec_dev = nullptr;
+	bool ok = false;
+	for (int attempt = 0; attempt < 4 && !ok; ++attempt) {
+		TRY(seed_core(c, st, n, (const int64_t*)f[1].p, (const int32_t*)f[2].p, (const uint8_t*)f[0].p, o->min_len, o->max_sub_occ,
+		              nullptr, &n_rec_dev, nullptr));
+		HIPCHK(hipStreamSynchronize(st));
+		ok = !seed_overflowed(c, &n_rec);
+	}
+	if (!ok) return MBG_EOVERFLOW;
+	HIPCHK(hipEventRecord(c->fev[1], st));
+	// 2. group the records by read; anchor slots per read from the raw records
+	TRY(dgrow(f[5], (size_t)(n + 1) * 4)); TRY(dgrow(f[6], (size_t)(n + 1) * 8)); TRY(dgrow(f[7], (size_t)(n + 1) * 8));
+	TRY(dgrow(f[8], (size_t)(n + 1) * 8)); TRY(dgrow(f[9], (size_t)(n + 1) * 4)); TRY(dgrow(f[10], (size_t)n_rec * sizeof(Sai) + 16));
+	uint32_t *cnt = (uint32_t*)f[5].p, *cursor = (uint32_t*)f[9].p;
+	unsigned long long *slots = (unsigned long long*)f[6].p;
+	int64_t *uoff = (int64_t*)f[7].p, *aoff = (int64_t*)f[8].p;
+	Sai *intv = (Sai*)f[10].p;
+	size_t t1 = 0, t2 = 0;
+	HIPCHK(scan_u32_i64(nullptr, &t1, nullptr, nullptr, n + 1, st));
+	HIPCHK(scan_u64_i64(nullptr, &t2, nullptr, nullptr, n + 1, st));
+	TRY(dgrow(c->s_tmp, t1 > t2 ? t1 : t2));
+	size_t tb = c->s_tmp.n;
+	HIPCHK(hipMemsetAsync(cnt, 0, (size_t)(n + 1) * 4, st));
+	HIPCHK(hipMemsetAsync(slots, 0, (size_t)(n + 1) * 8, st));
+	HIPCHK(hipMemsetAsync(cursor, 0, (size_t)(n + 1) * 4, st));
+	const unsigned gg = persistent_grid(c, n_rec, 256);
+	if (n_rec > 0) HIPCHK(launch_grp_count((const SeedRec*)c->p_recs.p, n_rec_dev, c->cap_rec, o->max_occ, cnt, slots, gg, st));
+	HIPCHK(scan_u32_i64(c->s_tmp.p, &tb, cnt, uoff, n + 1, st));
+	tb = c->s_tmp.n;
+	HIPCHK(scan_u64_i64(c->s_tmp.p, &tb, slots, aoff, n + 1, st));
+	if (n_rec > 0) HIPCHK(launch_grp_scatter((const SeedRec*)c->p_recs.p, n_rec_dev, c->cap_rec, uoff, cursor, intv, gg, st));
+	int64_t n_slots = 0;
+	HIPCHK(hipMemcpyAsync(&n_slots, aoff + n, 8, hipMemcpyDeviceToHost, st));
+	HIPCHK(hipStreamSynchronize(st));
+	HIPCHK(hipEventRecord(c->fev[2], st));
+	// 3. anchors (mb_anchor), 4. chains (mb_lchain_dp) with each read's options
+	TRY(dgrow(f[11], (size_t)n * 8 + 8)); TRY(dgrow(f[12], (size_t)n * 8 + 8)); TRY(dgrow(f[13], (size_t)n_slots * 8 + 16));
+	TRY(dgrow(f[14], (size_t)n_slots * sizeof(AncRec) + 16));
+	int64_t *nu = (int64_t*)f[11].p, *nanc = (int64_t*)f[12].p;
+	AncRec *anc = (AncRec*)f[14].p;
+	if (n > 0)
+		TRY(anchor_dev_locked(c, st, n, n_slots, intv, uoff, (const int32_t*)f[2].p, o->max_occ, (uint64_t*)f[13].p, anc, aoff, nanc, nu));
+	HIPCHK(hipEventRecord(c->fev[3], st));
+	const size_t wb = MBG_LCHAIN_WORK_BYTES(n_slots);
+	TRY(dgrow(f[15], wb)); TRY(dgrow(f[16], (size_t)n_slots * 8 + 16)); TRY(dgrow(f[17], (size_t)n * 4 + 8));
+	TRY(dgrow(f[18], (size_t)n_slots * sizeof(AncRec) + 16)); TRY(dgrow(f[19], (size_t)n * 8 + 8));
+	uint64_t *u = (uint64_t*)f[16].p;
+	int32_t *nuc = (int32_t*)f[17].p;
+	AncRec *chn = (AncRec*)f[18].p;
+	int64_t *nchn = (int64_t*)f[19].p;
+	if (n > 0) {
+		mbg_chain_opt_t none{};
+		TRY(lchain_dev_locked(c, st, n, n_slots, anc, aoff, nanc, none, f[15].p, f[15].n, u, nuc, chn, nchn, (const ChainOpt*)f[4].p,
+		                      (const int32_t*)f[3].p));
+	}
+	HIPCHK(hipEventRecord(c->fev[4], st));
+	// 5. results to pinned host memory
+	TRY(hgrow(h[0], (size_t)(n + 1) * 8)); TRY(hgrow(h[1], (size_t)n * 8 + 8)); TRY(hgrow(h[2], (size_t)n_rec * sizeof(Sai) + 16));
+	TRY(hgrow(h[3], (size_t)(n + 1) * 8)); TRY(hgrow(h[4], (size_t)n * 8 + 8)); TRY(hgrow(h[5], (size_t)n_slots * sizeof(AncRec) + 16));
+	TRY(hgrow(h[6], (size_t)n * 4 + 8)); TRY(hgrow(h[7], (size_t)n_slots * 8 + 16)); TRY(hgrow(h[8], (size_t)n * 8 + 8));
+	TRY(hgrow(h[9], (size_t)n_slots * sizeof(AncRec) + 16));
+	if (n > 0) {
+		HIPCHK(hipMemcpyAsync(h[0].p, uoff, (size_t)(n + 1) * 8, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(h[1].p, nu, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+		if (n_rec > 0) HIPCHK(hipMemcpyAsync(h[2].p, intv, (size_t)n_rec * sizeof(Sai), hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(h[3].p, aoff, (size_t)(n + 1) * 8, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(h[4].p, nanc, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(h[6].p, nuc, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(h[8].p, nchn, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+		if (n_slots > 0) {
+			HIPCHK(hipMemcpyAsync(h[5].p, anc, (size_t)n_slots * sizeof(AncRec), hipMemcpyDeviceToHost, st));
+			HIPCHK(hipMemcpyAsync(h[7].p, u, (size_t)n_slots * 8, hipMemcpyDeviceToHost, st));
+			HIPCHK(hipMemcpyAsync(h[9].p, chn, (size_t)n_slots * sizeof(AncRec), hipMemcpyDeviceToHost, st));
+		}
+	} else ((int64_t*)h[0].p)[0] = ((int64_t*)h[3].p)[0] = 0;
+	HIPCHK(hipEventRecord(c->fev[5], st));
+	HIPCHK(hipStreamSynchronize(st));
+	int64_t *hn_anc = (int64_t*)h[4].p;
+	for (int32_t r = 0; r < n; ++r) {
+		if (hn_anc[r] < 0) return MBG_EOVERFLOW;         // slots from the raw records always suffice
+		if (o->no_chain && o->no_chain[r]) hn_anc[r] = -1, ((int32_t*)h[6].p)[r] = 0, ((int64_t*)h[8].p)[r] = 0;
+	}
+	out->n_rec = n_rec, out->n_slots = n_slots;
+	out->intv_off = (const int64_t*)h[0].p, out->n_intv = (const int64_t*)h[1].p, out->intv = (const mbg_sai_t*)h[2].p;
+	out->anc_off = (const int64_t*)h[3].p, out->n_anc = hn_anc, out->anc = (const mbg_anchor_t*)h[5].p;
+	out->n_u = (const int32_t*)h[6].p, out->u = (const uint64_t*)h[7].p;
+	out->n_chn = (const int64_t*)h[8].p, out->chn = (const mbg_anchor_t*)h[9].p;
+	float ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[0], c->fev[1])); out->ms_seed = ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[1], c->fev[2])); out->ms_group = ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[2], c->fev[3])); out->ms_anchor = ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[3], c->fev[4])); out->ms_chain = ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[4], c->fev[5])); out->ms_copy = ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[0], c->fev[5])); out->ms_total = ms;
 	return MBG_OK;
 }
 
@@ -1263,6 +1430,23 @@ int mbg_copy_to_host(mbg_ctx_t *c, void *dst, const void *src, size_t bytes)
 	HIPCHK(hipSetDevice(c->device));
 	HIPCHK(hipStreamSynchronize(c->stream));     // ordered after the context stream's queued work
 	HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+	return MBG_OK;
+}
+
+int mbg_host_alloc(mbg_ctx_t *c, size_t bytes, void **ptr)
+{
+	if (c == nullptr || ptr == nullptr) return MBG_EARG;
+	*ptr = nullptr;
+	HIPCHK(hipSetDevice(c->device));
+	hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 16, 0);
+	if (e != hipSuccess) { g_hip_err = std::string("hipHostMalloc: ") + hipGetErrorString(e); *ptr = nullptr; return MBG_ENOMEM; }
+	return MBG_OK;
+}
+
+int mbg_host_free(mbg_ctx_t *c, void *ptr)
+{
+	if (c == nullptr) return MBG_EARG;
+	if (ptr) HIPCHK(hipHostFree(ptr));
 	return MBG_OK;
 }
 

@@ -200,12 +200,17 @@ struct AncArgs {
 hipError_t launch_anc_rows(const AncArgs &a, hipStream_t st);
 hipError_t launch_anc_build(const AncArgs &a, hipStream_t st);
 struct U128 { uint64_t x, y; };           // mb128_t (mbpriv.h:42)
+// one set of mb_lchain_dp's parameters; layout of mbg_chain_opt_t
+struct ChainOpt { int32_t max_dist_x, max_dist_y, bw, max_skip, max_iter, min_sc; float pen_gap; int32_t pad; };
+static_assert(sizeof(ChainOpt) == 32, "ChainOpt must match mbg_chain_opt_t");
 struct ChainArgs {
 	int32_t n_reads;
 	const AncRec *anc;       // read r's anchors at anc[aoff[r] .. aoff[r] + nanc[r]), mb_anchor's order
 	const int64_t *aoff, *nanc;
 	int32_t max_dist_x, max_dist_y, bw, max_skip, max_iter, min_sc;
 	float pen_gap;
+	const ChainOpt *opts;    // when set: read r uses opts[opt_idx[r]] instead of the fields above
+	const int32_t *opt_idx;
 	const uint64_t *ctg_off, *ctg_len;
 	// per-read workspace at offset aoff[r] of each array
 	int32_t *f, *t, *v;
@@ -241,6 +246,17 @@ hipError_t launch_compact(const SeedRec *slots1, const SeedRec *slots2, int32_t 
                           const uint32_t *cnt, const uint32_t *off, const SeedRec *spill, const unsigned long long *n_spill,
                           int64_t spill_cap, SeedRec *out, int64_t cap, int64_t *n_out, hipStream_t st);
 hipError_t scan_u32(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_t *out, int64_t n, hipStream_t st);
+// exclusive prefix sums into 64-bit offsets
+hipError_t scan_u32_i64(void *temp, size_t *temp_bytes, const uint32_t *in, int64_t *out, int64_t n, hipStream_t st);
+hipError_t scan_u64_i64(void *temp, size_t *temp_bytes, const unsigned long long *in, int64_t *out, int64_t n, hipStream_t st);
+// CIGARs of a DP batch packed back to back (mbg_front.hip); poff[i]: job i's first operation
+hipError_t launch_cigar_pack(const DpJob *jobs, const DpRes *res, int32_t n, const int64_t *poff, const uint32_t *cigar, uint32_t *packed,
+                             hipStream_t st);
+// grouping of seeding records by read (mbg_front.hip)
+hipError_t launch_grp_count(const SeedRec *recs, const int64_t *n_rec, int64_t cap, int32_t max_occ, uint32_t *cnt,
+                            unsigned long long *slots, unsigned grid, hipStream_t st);
+hipError_t launch_grp_scatter(const SeedRec *recs, const int64_t *n_rec, int64_t cap, const int64_t *uoff, uint32_t *cursor, Sai *intv,
+                              unsigned grid, hipStream_t st);
 hipError_t launch_sa(const Idx &ix, int64_t n, const int64_t *n_dev, uint64_t *x, uint32_t *next, unsigned long long *stats,
                      unsigned grid, hipStream_t st);
 hipError_t launch_getseq(const L2b &lb, int64_t n, const int64_t *tid, const int64_t *st_, const int64_t *en_, const int64_t *out_off,

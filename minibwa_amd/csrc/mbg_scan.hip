@@ -12,6 +12,25 @@ hipError_t scan_u32(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_t
 }
 
 namespace {
+struct ToI64 {
+	__host__ __device__ int64_t operator()(uint32_t x) const { return (int64_t)x; }
+	__host__ __device__ int64_t operator()(unsigned long long x) const { return (int64_t)x; }
+};
+} // namespace
+
+hipError_t scan_u32_i64(void *temp, size_t *temp_bytes, const uint32_t *in, int64_t *out, int64_t n, hipStream_t st)
+{
+	hipcub::TransformInputIterator<int64_t, ToI64, const uint32_t*> it(in, ToI64());
+	return hipcub::DeviceScan::ExclusiveSum(temp, *temp_bytes, it, out, (int)n, st);
+}
+
+hipError_t scan_u64_i64(void *temp, size_t *temp_bytes, const unsigned long long *in, int64_t *out, int64_t n, hipStream_t st)
+{
+	hipcub::TransformInputIterator<int64_t, ToI64, const unsigned long long*> it(in, ToI64());
+	return hipcub::DeviceScan::ExclusiveSum(temp, *temp_bytes, it, out, (int)n, st);
+}
+
+namespace {
 struct LlRows {
 	__host__ __device__ int64_t operator()(const LlJob &j) const { return j.tlen > 0 ? j.tlen : 0; }
 };

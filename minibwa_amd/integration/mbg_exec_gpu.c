@@ -1,0 +1,268 @@
+/*
+ * mbg_exec_gpu.c -- the batched mapper's device requests (mbg_exec.h) executed by libmbg.so on one
+ * MI355X: the front pass through mbg_map_front, each round's DP jobs through mbg_extd2_batch /
+ * mbg_extz2_batch (one call per scoring matrix), its local-SW jobs through mbg_ll_batch.  Sequences
+ * are packed into page-locked staging buffers so each round crosses PCIe in a few large copies.
+ * Any library error aborts with its message, as the reference does on internal failures.
+ *
+ * Environment: MBG_DEVICE (HIP device, default 0), MBG_PRE_LEN (k-mer interval cache length on the
+ * device; default 15 for indexes over 2^30 rows, else 12; results do not depend on it).
+ */
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "kalloc.h"
+#include "mbg_exec.h"
+
+typedef char sai_layout_check[sizeof(mb_sai_t) == sizeof(mbg_sai_t) ? 1 : -1];
+typedef char anchor_layout_check[sizeof(mb_anchor_t) == sizeof(mbg_anchor_t) ? 1 : -1];
+
+static mbg_ctx_t *g_ctx;
+static const mb_idx_t *g_idx;
+static mbx_stats_t g_st;
+
+/* staging buffers (page-locked, grow-only) */
+typedef struct { void *p; size_t n; } hbuf_t;
+static hbuf_t b_q, b_t, b_jobs, b_res, b_cig[2], b_ll_jobs, b_ll_res;   /* b_cig: extz2, extd2 (both live in one round) */
+
+static void chk(int rc, const char *what)
+{
+	if (rc != MBG_OK) {
+		fprintf(stderr, "[mbg_exec] %s failed: %s (%s)\n", what, mbg_strerror(rc), mbg_last_hip_error());
+		abort();
+	}
+}
+
+static double now_s(void)
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return ts.tv_sec + ts.tv_nsec * 1e-9;
+}
+
+static void *grow(hbuf_t *b, size_t bytes)
+{
+	if (bytes == 0) bytes = 16;
+	if (b->n < bytes) {
+		size_t n = bytes + bytes / 2 + 4096;
+		if (b->p) chk(mbg_host_free(g_ctx, b->p), "mbg_host_free");
+		chk(mbg_host_alloc(g_ctx, n, &b->p), "mbg_host_alloc");
+		b->n = n;
+	}
+	return b->p;
+}
+
+const char *mbx_name(void) { return "gpu"; }
+
+void mbx_index(const mb_idx_t *idx)
+{
+	const mb_bwt_t *bwt = idx->bwt;
+	const l2b_t *l = idx->l2b;
+	mbg_bwt_desc_t d;
+	mbg_l2b_desc_t ld;
+	uint64_t *len, i;
+	const char *s;
+	int pre;
+	if (g_ctx == 0) {
+		s = getenv("MBG_DEVICE");
+		chk(mbg_init(s ? atoi(s) : 0, &g_ctx), "mbg_init");
+	}
+	s = getenv("MBG_PRE_LEN");
+	pre = s ? atoi(s) : bwt->seq_len > (1ULL << 30) ? 15 : 12;
+	d.primary = bwt->primary;
+	memcpy(d.L2, bwt->L2, sizeof(d.L2));
+	d.seq_len = bwt->seq_len, d.data = bwt->data, d.sa_bit = bwt->sa_bit, d.n_sa = bwt->n_sa, d.sa = bwt->sa;
+	chk(mbg_bwt_upload(g_ctx, &d, pre), "mbg_bwt_upload");
+	len = (uint64_t*)malloc(sizeof(uint64_t) * (l->n_ctg ? l->n_ctg : 1));
+	for (i = 0; i < l->n_ctg; ++i) len[i] = l->ctg[i].len;
+	ld.tot_len = l->tot_len, ld.n_ctg = l->n_ctg, ld.ctg_len = len;
+	ld.n_pac = l->n_pac, ld.pac = l->pac, ld.n_ambi = l->n_ambi, ld.ambi = (const uint64_t*)l->ambi;
+	chk(mbg_l2b_upload(g_ctx, &ld), "mbg_l2b_upload");
+	free(len);
+	g_idx = idx;
+}
+
+void mbx_front(const mbx_front_in_t *in, mbx_front_t *out)
+{
+	mbg_front_opt_t o;
+	mbg_front_t r;
+	double t0 = now_s();
+	o.min_len = in->min_len, o.max_sub_occ = in->max_sub_occ, o.max_occ = in->max_occ;
+	o.n_copt = in->n_copt, o.copt = in->copt, o.copt_idx = in->copt_idx, o.no_chain = in->no_chain;
+	chk(mbg_map_front(g_ctx, in->n, in->nt4, in->off, in->len, &o, &r), "mbg_map_front");
+	out->n = in->n;
+	out->intv_off = r.intv_off, out->n_intv = r.n_intv, out->intv = (const mb_sai_t*)r.intv;
+	out->anc_off = r.anc_off, out->n_anc = r.n_anc, out->anc = (const mb_anchor_t*)r.anc;
+	out->n_u = r.n_u, out->u = r.u, out->n_chn = r.n_chn, out->chn = (const mb_anchor_t*)r.chn;
+	out->dev_ms = r.ms_total;
+	++g_st.fronts, g_st.front_reads += in->n, g_st.front_s += now_s() - t0;
+}
+
+/* the requests of one scoring matrix in one library call */
+static void dp_group(int dual, int32_t n, mbx_req_t *const *req, const int32_t *sel)
+{
+	int64_t qb = 0, tb = 0, cb = 0;
+	int32_t i;
+	uint8_t *q, *t;
+	mbg_dp_job_t *jobs;
+	mbg_dp_result_t *res;
+	uint32_t *cig;
+	for (i = 0; i < n; ++i) {
+		const mbx_req_t *r = req[sel[i]];
+		qb += r->qlen, tb += r->tlen, cb += (int64_t)r->qlen + r->tlen;
+	}
+	q = (uint8_t*)grow(&b_q, qb + 16), t = (uint8_t*)grow(&b_t, tb + 16);
+	jobs = (mbg_dp_job_t*)grow(&b_jobs, (size_t)n * sizeof(mbg_dp_job_t));
+	res = (mbg_dp_result_t*)grow(&b_res, (size_t)n * sizeof(mbg_dp_result_t));
+	cig = (uint32_t*)grow(&b_cig[dual != 0], (size_t)cb * 4 + 16);
+	for (i = 0, qb = tb = cb = 0; i < n; ++i) {
+		const mbx_req_t *r = req[sel[i]];
+		mbg_dp_job_t *j = &jobs[i];
+		j->qoff = qb, j->toff = tb, j->qlen = r->qlen, j->tlen = r->tlen;
+		j->q = r->gq, j->e = r->ge, j->q2 = r->gq2, j->e2 = r->ge2;
+		j->w = r->w, j->zdrop = r->zdrop, j->end_bonus = r->end_bonus, j->flag = r->flag;
+		j->cigar_off = cb;
+		if (r->qlen) memcpy(q + qb, r->q, r->qlen);
+		if (r->tlen) memcpy(t + tb, r->t, r->tlen);
+		qb += r->qlen, tb += r->tlen, cb += (int64_t)r->qlen + r->tlen;
+	}
+	if (dual) chk(mbg_extd2_batch(g_ctx, n, jobs, q, qb, t, tb, req[sel[0]]->m, req[sel[0]]->mat, res, cig, cb), "mbg_extd2_batch");
+	else chk(mbg_extz2_batch(g_ctx, n, jobs, q, qb, t, tb, req[sel[0]]->m, req[sel[0]]->mat, res, cig, cb), "mbg_extz2_batch");
+	for (i = 0; i < n; ++i) {
+		mbx_req_t *r = req[sel[i]];
+		r->dp = res[i];
+		r->cigar = cig + jobs[i].cigar_off;
+	}
+}
+
+/* Requests are grouped by their scoring matrix (and, for local SW, by the shared gap and xtra
+ * arguments): a library call takes one of each.  With one group the round's CIGARs stay in the
+ * staging buffer of their kind (extd2 or extz2), which only the next round overwrites; with several,
+ * each group's CIGARs are copied to a buffer sized for the whole round before the next group reuses
+ * the staging buffer. */
+static int32_t *g_sel, *g_grp;
+static int64_t g_sel_m;
+static uint32_t *g_cig_keep[2];
+static int64_t g_cig_keep_m[2];
+
+static void sel_buf(int64_t n)
+{
+	if (n > g_sel_m) {
+		g_sel_m = n + 1024;
+		g_sel = (int32_t*)realloc(g_sel, g_sel_m * sizeof(int32_t));
+		g_grp = (int32_t*)realloc(g_grp, g_sel_m * sizeof(int32_t));
+		if (g_sel == 0 || g_grp == 0) { fprintf(stderr, "[mbg_exec] out of memory\n"); abort(); }
+	}
+}
+
+void mbx_run_dp(int dual, int32_t n, mbx_req_t *const *req)
+{
+	int32_t i, g, n_grp = 0, n_sel;
+	int64_t keep = 0;
+	double t0 = now_s();
+	sel_buf(n);
+	for (i = 0; i < n; ++i) g_grp[i] = -1;
+	for (i = 0; i < n; ++i) {          /* group ids in order of first appearance */
+		int32_t j;
+		if (g_grp[i] >= 0) continue;
+		g_grp[i] = n_grp;
+		for (j = i + 1; j < n; ++j)
+			if (g_grp[j] < 0 && req[j]->m == req[i]->m && memcmp(req[j]->mat, req[i]->mat, (size_t)req[i]->m * req[i]->m) == 0)
+				g_grp[j] = n_grp;
+		++n_grp;
+	}
+	if (n_grp > 1) {
+		int64_t all = 0;
+		for (i = 0; i < n; ++i) all += (int64_t)req[i]->qlen + req[i]->tlen;
+		if (all > g_cig_keep_m[dual != 0]) {
+			free(g_cig_keep[dual != 0]);
+			g_cig_keep_m[dual != 0] = all + 1024;
+			g_cig_keep[dual != 0] = (uint32_t*)malloc(g_cig_keep_m[dual != 0] * 4);
+			if (g_cig_keep[dual != 0] == 0) { fprintf(stderr, "[mbg_exec] out of memory\n"); abort(); }
+		}
+	}
+	for (g = 0; g < n_grp; ++g) {
+		for (i = 0, n_sel = 0; i < n; ++i)
+			if (g_grp[i] == g) g_sel[n_sel++] = i;
+		dp_group(dual, n_sel, req, g_sel);
+		if (n_grp > 1)
+			for (i = 0; i < n_sel; ++i) {
+				mbx_req_t *r = req[g_sel[i]];
+				memcpy(g_cig_keep[dual != 0] + keep, r->cigar, (size_t)r->dp.n_cigar * 4);
+				r->cigar = g_cig_keep[dual != 0] + keep;
+				keep += r->dp.n_cigar;
+			}
+	}
+	++g_st.dp_calls, g_st.dp_jobs += n, g_st.dp_s += now_s() - t0;
+}
+
+void mbx_run_ll(int32_t n, mbx_req_t *const *req)
+{
+	int32_t i, n_sel, done = 0, first = 0, *sel;
+	uint8_t *taken = (uint8_t*)calloc(n, 1);
+	double t0 = now_s();
+	sel_buf(n);
+	sel = g_sel;
+	while (done < n) {
+		const mbx_req_t *k;
+		int64_t qb = 0, tb = 0;
+		uint8_t *q, *t;
+		mbg_ll_job_t *jobs;
+		mbg_ll_result_t *res;
+		while (taken[first]) ++first;
+		k = req[first];
+		for (i = first, n_sel = 0; i < n; ++i) {
+			const mbx_req_t *r = req[i];
+			if (!taken[i] && r->m == k->m && r->gapo == k->gapo && r->gape == k->gape && r->xtra == k->xtra &&
+			    memcmp(r->mat, k->mat, (size_t)k->m * k->m) == 0)
+				sel[n_sel++] = i, taken[i] = 1, qb += r->qlen, tb += r->tlen;
+		}
+		done += n_sel;
+		q = (uint8_t*)grow(&b_q, qb + 16), t = (uint8_t*)grow(&b_t, tb + 16);
+		jobs = (mbg_ll_job_t*)grow(&b_ll_jobs, (size_t)n_sel * sizeof(mbg_ll_job_t));
+		res = (mbg_ll_result_t*)grow(&b_ll_res, (size_t)n_sel * sizeof(mbg_ll_result_t));
+		for (i = 0, qb = tb = 0; i < n_sel; ++i) {
+			const mbx_req_t *r = req[sel[i]];
+			mbg_ll_job_t *j = &jobs[i];
+			memset(j, 0, sizeof(*j));
+			j->qoff = qb, j->toff = tb, j->qlen = r->qlen, j->tlen = r->tlen, j->size = r->size;
+			if (r->qlen) memcpy(q + qb, r->q, r->qlen);
+			if (r->tlen) memcpy(t + tb, r->t, r->tlen);
+			qb += r->qlen, tb += r->tlen;
+		}
+		chk(mbg_ll_batch(g_ctx, n_sel, jobs, q, qb, t, tb, k->m, k->mat, k->gapo, k->gape, k->xtra, res), "mbg_ll_batch");
+		for (i = 0; i < n_sel; ++i) req[sel[i]]->ll = res[i];
+	}
+	free(taken);
+	++g_st.ll_calls, g_st.ll_jobs += n, g_st.ll_s += now_s() - t0;
+}
+
+void mbx_sa(int64_t n, uint64_t *x)
+{
+	chk(mbg_sa_batch(g_ctx, n, x), "mbg_sa_batch");
+}
+
+/* intervals appended into the caller's containers with kalloc on its km, as mb_seed_intv_batch does
+ * (bwt.c:368-371) */
+void mbx_seed(void *km, int32_t n, const int32_t *len, uint8_t *const *seq, int32_t min_len, int32_t max_sub_occ, mb_sai_v *v)
+{
+	mbg_sai_t *out = 0;
+	int64_t *off = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
+	int32_t i;
+	chk(mbg_seed_batch(g_ctx, n, len, (const uint8_t *const*)seq, min_len, max_sub_occ, &out, off), "mbg_seed_batch");
+	for (i = 0; i < n; ++i) {
+		int64_t k, m = off[i + 1] - off[i];
+		v[i].n = 0;
+		if ((size_t)m > v[i].m) {
+			v[i].m = (size_t)m + ((size_t)m >> 1) + 16;
+			v[i].a = (mb_sai_t*)krealloc(km, v[i].a, v[i].m * sizeof(mb_sai_t));   /* kalloc's, as Kgrow */
+		}
+		for (k = 0; k < m; ++k) memcpy(&v[i].a[v[i].n++], &out[off[i] + k], sizeof(mb_sai_t));
+	}
+	free(out);
+	free(off);
+}
+
+void mbx_get_stats(mbx_stats_t *st) { *st = g_st; }

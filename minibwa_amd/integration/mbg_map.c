@@ -108,6 +108,19 @@ static void die(const char *msg)
 	abort();
 }
 
+static void report(void)
+{
+	mbs_stats_t ss;
+	mbx_stats_t xs;
+	memset(&ss, 0, sizeof(ss));
+	if (g_pool) mbs_get_stats(g_pool, &ss);
+	mbx_get_stats(&xs);
+	fprintf(stderr, "[mbg_map] summary: executor %s fronts %ld reads %ld front_s %.3f | rounds %ld yields %ld peak_coroutines %ld "
+	        "round_s %.3f run_s %.3f | dp_calls %ld dp_jobs %ld dp_s %.3f | ll_calls %ld ll_jobs %ld ll_s %.3f\n",
+	        mbx_name(), (long)xs.fronts, (long)xs.front_reads, xs.front_s, (long)ss.rounds, (long)ss.yields, (long)ss.coroutines_peak,
+	        ss.round_s, ss.task_s, (long)xs.dp_calls, (long)xs.dp_jobs, xs.dp_s, (long)xs.ll_calls, (long)xs.ll_jobs, xs.ll_s);
+}
+
 static mbs_pool_t *pool_for(int n_threads)
 {
 	int i;
@@ -120,6 +133,7 @@ static mbs_pool_t *pool_for(int n_threads)
 	}
 	if (g_pool == 0) {
 		const char *s = getenv("MBG_MAP_STATS");
+		if (s && atoi(s) && !g_stats) atexit(report);
 		g_stats = s && atoi(s);
 		g_pool = mbs_pool_init(n_threads, 256 * 1024);
 		if (g_pool == 0) die("cannot create the worker pool");

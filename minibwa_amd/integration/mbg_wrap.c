@@ -3,8 +3,8 @@
  *
  * Linked into the unmodified minibwa objects with
  *   -Wl,--wrap=mb_seed_intv_batch -Wl,--wrap=mb_bwt_sa_batch -Wl,--wrap=ksw_extd2_sse -Wl,--wrap=ksw_extz2_sse
- *   -Wl,--wrap=ksw_ll_qinit -Wl,--wrap=ksw_ll_u8_core -Wl,--wrap=ksw_ll_i16_core -Wl,--wrap=mb_anchor
- *   -Wl,--wrap=mb_lchain_dp
+ *   -Wl,--wrap=ksw_ll_qinit -Wl,--wrap=ksw_ll_u8_core -Wl,--wrap=ksw_ll_i16_core -Wl,--wrap=ksw_ll_i16
+ *   -Wl,--wrap=mb_anchor -Wl,--wrap=mb_lchain_dp
  * every call site of those functions (map-main.c:75, map-algo.c:733, seed.c:180, align.c:374-376,
  * pe.c:267-272,287, map-algo.c:610,614,634) reaches the __wrap_ definitions below, which run the work on the GPU through the
  * C ABI of include/mbg.h and hand results back in the reference's own containers:
@@ -225,6 +225,15 @@ ksw_llrst_t __wrap_ksw_ll_u8_core(void *q, int tlen, const uint8_t *target, int 
 ksw_llrst_t __wrap_ksw_ll_i16_core(void *q, int tlen, const uint8_t *target, int gapo, int gape, int xtra)
 {
 	return ll_one(q, tlen, target, gapo, gape, xtra, 2);
+}
+
+/* ksw_ll_i16 (ksw2_ll_sse.c:335-341, called by mb_align1_inv, align.c:798) reaches the i16 core inside
+ * its own file, past the hook above, with the query record __wrap_ksw_ll_qinit made: bound here too */
+int __wrap_ksw_ll_i16(void *q, int tlen, const uint8_t *target, int gapo, int gape, int *qe, int *te)
+{
+	ksw_llrst_t r = ll_one(q, tlen, target, gapo, gape, 0, 2);
+	*qe = r.qe, *te = r.te;
+	return r.score;
 }
 
 /* the contig table goes to the device once per l2b */

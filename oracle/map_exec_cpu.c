@@ -42,9 +42,9 @@ static int32_t *f_n_u;
 static mb_sai_t *f_intv;
 static mb_anchor_t *f_anc, *f_chn;
 static uint64_t *f_u;
-/* the round's CIGARs */
-static uint32_t *r_cig;
-static int64_t r_cig_m;
+/* the round's CIGARs, one buffer per kind (extz2, extd2): a round may hold both */
+static uint32_t *r_cig[2];
+static int64_t r_cig_m[2];
 
 const char *mbx_name(void) { return "cpu-reference"; }
 
@@ -121,7 +121,7 @@ void mbx_run_dp(int dual, int32_t n, mbx_req_t *const *req)
 	ksw_extz_t ez;
 	memset(&ez, 0, sizeof(ez));
 	for (i = 0; i < n; ++i) tot += (int64_t)req[i]->qlen + req[i]->tlen + 1;
-	if (tot > r_cig_m) r_cig_m = tot, r_cig = (uint32_t*)xrealloc(r_cig, tot * 4);
+	if (tot > r_cig_m[dual != 0]) r_cig_m[dual != 0] = tot, r_cig[dual != 0] = (uint32_t*)xrealloc(r_cig[dual != 0], tot * 4);
 	for (i = 0, tot = 0; i < n; ++i) {
 		mbx_req_t *q = req[i];
 		if (dual) __real_ksw_extd2_sse(0, q->qlen, q->q, q->tlen, q->t, q->m, q->mat, q->gq, q->ge, q->gq2, q->ge2, q->w, q->zdrop,
@@ -130,8 +130,8 @@ void mbx_run_dp(int dual, int32_t n, mbx_req_t *const *req)
 		q->dp.max = ez.max, q->dp.zdropped = ez.zdropped, q->dp.max_q = ez.max_q, q->dp.max_t = ez.max_t;
 		q->dp.mqe = ez.mqe, q->dp.mqe_t = ez.mqe_t, q->dp.mte = ez.mte, q->dp.mte_q = ez.mte_q;
 		q->dp.score = ez.score, q->dp.reach_end = ez.reach_end, q->dp.n_cigar = ez.n_cigar, q->dp.cells = 0;
-		memcpy(r_cig + tot, ez.cigar, (size_t)ez.n_cigar * 4);
-		q->cigar = r_cig + tot;
+		memcpy(r_cig[dual != 0] + tot, ez.cigar, (size_t)ez.n_cigar * 4);
+		q->cigar = r_cig[dual != 0] + tot;
 		tot += (int64_t)q->qlen + q->tlen + 1;
 	}
 	kfree(0, ez.cigar);
