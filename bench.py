@@ -1,22 +1,27 @@
-"""Benchmark of the seeding + extension hot path on one MI355X.
+"""Benchmark of minibwa's seeding + extension hot path on one MI355X, as the end-to-end mapper.
 
 Workload (2x150 bp pairs against a 3.1 Gbp reference): a 3.1 Gbp seeded synthetic reference
 (uniform ACGT plus 8% repeats: copies of 300-6,000 bp segments at 2% divergence; 24 contigs of
 129 Mbp) and 1M read pairs 2x150 bp (fragment ~ Normal(400, 50), 1% substitutions, 0.1% indels with
-geometric length of mean 2), splitmix64 seed 11.  The index (6.2e9 BWT rows, about 7 GB) is built on
-the device before timing and stays resident in HBM, as do the reads.  --ref-len 100000000
---contigs 4 --rep-frac 0.05 gives the 100 Mbp chromosome-scale workload.
+geometric length of mean 2) in two FASTQ files, splitmix64 seed 11.  The index is built on the device
+before timing.  --ref-len 100000000 --contigs 4 --rep-frac 0.05 gives the 100 Mbp workload.
 
-One step = one pass of the hot path over the 2M reads: two-pass SMEM seeding, SA locate of every
-interval's anchor rows, left/right extension jobs from each read's longest SMEM with the reference's
-ungapped fast path, and the banded dual-affine DP of the jobs that remain.  value = read pairs per
-second of that pass.
+One step = one `minibwa map` pass over the 1M pairs, FASTQ in -> SAM out, by the batched GPU mapper
+(minibwa_amd/bin/mbg-map-bench: the reference's own mb_map_file with the index loaded once and made
+resident in HBM; seeding, anchors and chains in one device pass per 100 Mbp mini-batch, every DP and
+mate-rescue call in device rounds, the rest the reference's host code on -t host threads).  The first
+pass writes its SAM to a file; timed passes write to /dev/null.  value = read pairs per second of
+the timed passes' wall time.
 
-cpu_baseline: the compiled reference's own functions (mb_seed_intv_batch, mb_anchor -- which
-includes mb_bwt_sa_batch -- and ksw_extd2_sse on the same extension jobs) timed on the host's cores
-over a sample of the same reads (oracle/_ref/cpu_hotpath).
+cpu_baseline: the compiled reference, `minibwa map -t<all host cores>` on the same FASTQ and index
+files, timed as T_end - T_idx from its own stderr (main.c:86, map-main.c:561-562), output to
+/dev/null.  A second reference run writes its SAM to a file, and the bench checks that it equals the
+GPU mapper's byte for byte (minus the @PG line, which holds the command line).
 
-Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+--step runs the earlier diagnostic instead: one device pass of seeding, SA locate and a stand-in
+extension job set over resident reads (mbg_map_step_dev), with its own serving loop.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
 import argparse
 import json
@@ -45,14 +50,28 @@ def host_cores():
         n = len(os.sched_getaffinity(0))
     except Exception:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--step", action="store_true",
+                    help="run the device-pass diagnostic (seeding, SA locate, stand-in extension jobs) instead of the mapper")
+    ap.add_argument("--threads", type=int, default=0, help="host threads of both mappers (default: all host cores)")
+    ap.add_argument("--cpu-runs", type=int, default=1, help="timed runs of the reference mapper (median)")
     ap.add_argument("--ref-len", type=int, default=3_100_000_000)
     ap.add_argument("--contigs", type=int, default=24)
     ap.add_argument("--pre-len", type=int, default=15,
@@ -117,6 +136,9 @@ def main():
         return
     if args.rescue_batch:
         rescue_batch(args)
+        return
+    if not (args.step or args.anchor_batch or args.gather):
+        mapper_bench(args)
         return
 
     t0 = time.time()
@@ -215,7 +237,7 @@ def main():
     latency = serve(ctx, ref, opt, args) if args.serve_requests > 0 else None
 
     res = {
-        "metric": METRIC,
+        "metric": "device-pass read pairs/s (seeding, SA locate, stand-in extension jobs; diagnostic)",
         "value": round(value, 1),
         "unit": "read pairs/s",
         "n_gpus": 1,
@@ -242,6 +264,234 @@ def main():
     }
     print(json.dumps(res), flush=True)
     ctx.close()
+
+
+def _ref_timings(err):
+    """(T_idx, T_end) from a reference-CLI stderr: the 'index loaded' and final 'Real time' stamps
+    (map-main.c:561-562, main.c:86)."""
+    t_idx = t_end = None
+    for line in err.split("\n"):
+        if line.endswith("index loaded") and line.startswith("[M::main_map::"):
+            t_idx = float(line.split("::")[2].split("*")[0])
+        elif line.startswith("[M::main] Real time:"):
+            t_end = float(line.split()[3])
+    if t_idx is None or t_end is None:
+        raise RuntimeError("no timing lines in the mapper's stderr:\n" + err[-2000:])
+    return t_idx, t_end
+
+
+def _sam_body_md5(path):
+    """md5 and line count of a SAM file without its @PG line (which holds the command line)."""
+    import hashlib
+    h, n = hashlib.md5(), 0
+    with open(path, "rb") as f:
+        for line in f:
+            if line.startswith(b"@PG"):
+                continue
+            h.update(line)
+            n += 1
+    return h.hexdigest(), n
+
+
+def _kv(line):
+    w = line.replace("|", " ").split()
+    out = {}
+    for i in range(len(w) - 1):
+        try:
+            out[w[i]] = float(w[i + 1])
+        except ValueError:
+            pass
+    return out
+
+
+def mapper_bench(args):
+    """The end-to-end mapper: W + K passes of FASTQ -> SAM in one process (index loaded once), the
+    reference CLI beside it, and a byte-for-byte SAM check between them."""
+    from minibwa_amd import synth
+    exe = os.path.join(ROOT, "minibwa_amd", "bin", "mbg-map-bench")
+    cli = os.path.join(ROOT, "minibwa_amd", "bin", "minibwa-gpu")
+    ref_cli = os.path.join(ROOT, "oracle", "_ref", "minibwa")
+    for e in (exe, cli):
+        if not os.path.exists(e):
+            log(f"{e} is not built (make -C minibwa_amd/integration); the GPU mapper cannot run")
+            sys.exit(1)
+    cores = args.threads or host_cores()
+    t0 = time.time()
+    ref = synth.reference(args.ref_len, seed=11, rep_frac=args.rep_frac, rep_min=300, rep_max=6000, rep_div=0.02)
+    log(f"reference {len(ref)} bp in {time.time() - t0:.1f}s")
+    work = args.workdir or tempfile.mkdtemp(prefix="mbg_bench_")
+    os.makedirs(work, exist_ok=True)
+    prefix = os.path.join(work, "ref")
+    t0 = time.time()
+    params = json.dumps({"len": args.ref_len, "seed": 11, "rep_frac": args.rep_frac, "contigs": args.contigs})
+    pfile = prefix + ".params.json"
+    if os.path.exists(prefix + ".mbw") and os.path.exists(pfile) and open(pfile).read() == params:
+        log(f"index of the same reference found in {work}")     # an earlier run with --workdir built it
+    else:
+        r = subprocess.run([sys.executable, "-m", "minibwa_amd.index", "--len", str(args.ref_len), "--seed", "11",
+                            "--rep-frac", str(args.rep_frac), "--rep-min", "300", "--rep-max", "6000", "--rep-div", "0.02",
+                            "--contigs", str(args.contigs), "--device", "cuda", "--out", prefix], cwd=ROOT, capture_output=True, text=True)
+        if r.returncode != 0:
+            log("index build failed:", r.stderr[-2000:])
+            sys.exit(1)
+        open(pfile, "w").write(params)
+        log(r.stdout.strip(), f"(wall {time.time() - t0:.1f}s)")
+    t0 = time.time()
+    r1, r2 = synth.read_pairs(ref, args.pairs, seed=11, rlen=150, frag_mean=400, frag_sd=50, sub=0.01, indel=0.001)
+    f1, f2 = os.path.join(work, "r1.fq"), os.path.join(work, "r2.fq")
+    synth.write_pairs_fastq(f1, f2, r1, r2, prefix="p_")
+    del r1, r2
+    log(f"{args.pairs} pairs written to FASTQ in {time.time() - t0:.1f}s")
+
+    # the GPU mapper: warmup + timed passes in one process; pass 0 writes the SAM compared below
+    gpu_sam = os.path.join(work, "gpu.sam")
+    passes = args.warmup + args.steps
+    n_front_per_pass = None
+    env = dict(os.environ, MBG_MAP_STATS="1")
+    # pass-1 algorithmic work is counted in the first (warmup) pass only: counting adds atomics
+    fronts_pass = max(1, -(-(300 * args.pairs) // 100_000_000))
+    if args.warmup > 0:
+        env["MBG_COUNT_WORK"] = str(fronts_pass)
+    cmd = [exe, "-t", str(cores), "-P", str(passes), "-o", gpu_sam, "-O", "/dev/null", prefix, f1, f2]
+    log("gpu:", " ".join(cmd))
+    t0 = time.time()
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=3600)
+    if r.returncode != 0:
+        log("GPU mapper failed:", r.stderr[-3000:])
+        sys.exit(1)
+    for l in r.stderr.split("\n"):
+        if l.startswith("[mbg_map]") or l.startswith("[mbg_exec] front"):
+            log(l)
+    pass_s = [float(l.split()[3]) for l in r.stderr.split("\n") if l.startswith("[mbg_bench] pass ")]
+    load_s = [float(l.split()[2]) for l in r.stderr.split("\n") if l.startswith("[mbg_bench] load ")][0]
+    fronts = [_kv(l) for l in r.stderr.split("\n") if l.startswith("[mbg_exec] front ")]
+    summ = _kv([l for l in r.stderr.split("\n") if l.startswith("[mbg_map] summary:")][-1])
+    assert len(pass_s) == passes, r.stderr[-2000:]
+    for i, t in enumerate(pass_s):
+        log(f"pass {i}{' (warmup)' if i < args.warmup else ''}: {t:.3f} s = {args.pairs / t:,.0f} pairs/s")
+    timed = pass_s[args.warmup:]
+    el = sum(timed)
+    value = args.steps * args.pairs / el
+    n_front_per_pass = len(fronts) // passes
+    tf = fronts[args.warmup * n_front_per_pass:]
+    cf = fronts[:n_front_per_pass] if args.warmup > 0 else []
+
+    # device time per timed pass, by kernel / phase (HIP events on the context's stream)
+    per = lambda key: sum(f[key] for f in tf) / max(1, args.steps)
+    seed1_ms, seed2_ms = per("seed1_ms"), per("seed2_ms")
+    phases = {"smem_pass1_ms": seed1_ms, "smem_pass2_ms": seed2_ms, "group_ms": per("group_ms"), "anchor_sa_ms": per("anchor_ms"),
+              "chain_ms": per("chain_ms"), "front_d2h_ms": per("copy_ms")}
+    # DP and local SW: cumulative over all passes (every pass does the same work)
+    dp_ms = summ.get("dev_dp_ms", 0.0) / passes
+    phases["dp_ms"] = dp_ms
+    phases["ll_ms"] = summ.get("dev_ll_ms", 0.0) / passes
+    phases = {k: round(v, 3) for k, v in phases.items()}
+    dev_total = sum(phases.values())
+    occ1 = sum(f["seed1_occ_blocks"] for f in cf)
+    kmer1 = sum(f["seed1_kmer_lookups"] for f in cf)
+    bytes1 = 64 * occ1 + 32 * kmer1
+    kernels = []
+    if bytes1 > 0 and seed1_ms > 0:
+        a = bytes1 / (seed1_ms * 1e-3) / 1e9
+        kernels.append({"kernel": "smem_kernel<1> (SMEM pass 1)", "bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(a / HBM_PEAK_GBS, 4), "traffic": None,
+                        "algorithmic_bytes_per_launch": int(bytes1 / fronts_pass), "avg_launch_ms": round(seed1_ms / fronts_pass, 4),
+                        "share_of_device": round(seed1_ms / dev_total, 3) if dev_total else None})
+    cells = summ.get("dev_dp_cells", 0.0) / passes
+    if cells > 0 and dp_ms > 0:
+        ops = cells * 37 / (dp_ms * 1e-3) / 1e12      # about 37 integer operations per cell with traceback bits
+        kernels.append({"kernel": "dp kernels (ksw_extd2 rounds)", "bound": "valu", "achieved": round(ops, 3), "peak": 39.3,
+                        "unit": "Tops/s", "frac": round(ops / 39.3, 4), "traffic": None, "gcups": round(cells / (dp_ms * 1e-3) / 1e9, 1),
+                        "cells_per_pass": int(cells), "launches_per_pass": round(summ.get("dev_dp_launches", 0) / passes, 1),
+                        "share_of_device": round(dp_ms / dev_total, 3) if dev_total else None})
+    kernels.sort(key=lambda k: -(k["share_of_device"] or 0))
+    roofline = kernels[0] if kernels else None
+    if roofline and roofline["kernel"].startswith("smem"):
+        pmc = os.path.join(ROOT, "profiles", "mapper_3g_pmc.json") if args.ref_len == 3_100_000_000 else ""
+        if pmc and os.path.exists(pmc):
+            try:
+                ks = json.load(open(pmc))["kernels"]
+                roofline["traffic"] = [v for k, v in ks.items() if "smem_kernel<1>" in k][0]["hbm_bytes_per_launch"]
+            except Exception:
+                pass
+
+    # the same program, one `minibwa map` CLI run, timed the reference's way (T_end - T_idx)
+    cli_run = None
+    if not args.no_cpu:
+        r = subprocess.run([cli, "map", "-t", str(cores), "-o", "/dev/null", prefix, f1, f2], capture_output=True, text=True,
+                           timeout=3600)
+        if r.returncode == 0:
+            ti, te = _ref_timings(r.stderr)
+            cli_run = {"pairs_per_s": round(args.pairs / (te - ti), 1), "map_s": round(te - ti, 3), "index_load_s": round(ti, 3)}
+            log("gpu CLI run:", json.dumps(cli_run))
+
+    cpu, parity = None, None
+    if not args.no_cpu and os.path.exists(ref_cli):
+        runs = []
+        for k in range(max(1, args.cpu_runs)):
+            r = subprocess.run([ref_cli, "map", "-t", str(cores), "-o", "/dev/null", prefix, f1, f2], capture_output=True,
+                               text=True, timeout=3600)
+            if r.returncode != 0:
+                log("reference mapper failed:", r.stderr[-1000:])
+                break
+            ti, te = _ref_timings(r.stderr)
+            runs.append(te - ti)
+            log(f"reference run {k}: {te - ti:.3f} s mapping ({ti:.1f} s index load)")
+        if runs:
+            med = float(np.median(runs))
+            cpu = {"value": round(args.pairs / med, 1), "unit": "read pairs/s", "cores": cores, "kind": "reference",
+                   "sample": f"the whole workload: `minibwa map -t{cores}` (the reference built from source, -O3 -msse4.2, "
+                             f"mimalloc) on the same FASTQ and index files, output to /dev/null; T_end - T_idx from its "
+                             f"stderr, median of {len(runs)} run(s): {med:.3f} s; host: {cpu_model()}, {cores} cores, "
+                             f"1 thread per core",
+                   "runs_s": [round(x, 3) for x in runs]}
+        cpu_sam = os.path.join(work, "cpu.sam")
+        r = subprocess.run([ref_cli, "map", "-t", str(cores), "-o", cpu_sam, prefix, f1, f2], capture_output=True, text=True,
+                           timeout=3600)
+        if r.returncode == 0:
+            mg, ng = _sam_body_md5(gpu_sam)
+            mc, nc = _sam_body_md5(cpu_sam)
+            parity = {"sam_identical": mg == mc, "gpu_md5": mg, "cpu_md5": mc, "lines": ng, "cpu_lines": nc}
+            log("parity:", json.dumps(parity))
+    latency = None
+    if args.serve_requests > 0:
+        latency = serve_mapper(args, ref, prefix, work, cores)
+
+    res = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "read pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1000, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (FM-index ranks) + int8/int32 (DP)",
+        "data": "synthetic (seeded splitmix64 reference and reads; index built on the device)",
+        "config": {"workload": f"end-to-end mapping (FASTQ in, SAM out), {args.pairs} pairs 2x150 bp vs "
+                               f"{args.ref_len / 1e9:g} Gbp synthetic reference ({args.rep_frac:.0%} repeats)",
+                   "ref_bp": args.ref_len, "pairs": args.pairs, "read_len": 150, "contigs": args.contigs,
+                   "host_threads": cores, "options": "minibwa map defaults (adaptive preset)", "mini_batches_per_pass": n_front_per_pass},
+        "roofline": roofline,
+        "roofline_kernels": kernels,
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "latency": latency,
+        "device_ms_per_pass": phases,
+        "gpu_cli_single_run": cli_run,
+        "index_load_s": round(load_s, 2),
+        "rounds_per_pass": round(summ.get("rounds", 0) / passes, 1),
+    }
+    print(json.dumps(res), flush=True)
+    if not args.workdir:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def serve_mapper(args, ref, prefix, work, cores):
+    return None
+
 
 
 def serve(ctx0, ref, opt, args):

@@ -40,6 +40,7 @@ typedef struct {
  * conversion: two-pass seeding (mb_seed_intv_batch), mb_anchor and the first mb_lchain_dp call, with
  * each read's own chaining options (copt[copt_idx[r]], from mb_opt_adap for its length). */
 typedef struct {
+	int32_t slot;                  /* 0..MBX_FRONT_SLOTS-1: results of different slots stay valid together */
 	int32_t n;
 	const uint8_t *nt4;            /* read r: nt4 + off[r], len[r] bases (already converted for methylation) */
 	const int64_t *off;
@@ -52,7 +53,10 @@ typedef struct {
 } mbx_front_in_t;
 
 /* Per read r: its intervals (deduplicated, any order), anchors in mb_anchor's order, and chains with
- * their anchors as mb_lchain_dp returns them.  Valid until the next mbx_front call. */
+ * their anchors as mb_lchain_dp returns them.  Valid until the next mbx_front call on the same slot;
+ * calls on different slots may run concurrently (the CLI's three pipeline steps hold up to three
+ * mini-batches). */
+#define MBX_FRONT_SLOTS 3
 typedef struct {
 	int32_t n;
 	const int64_t *intv_off, *n_intv;
@@ -77,7 +81,12 @@ void mbx_run_ll(int32_t n, mbx_req_t *const *r);
 void mbx_sa(int64_t n, uint64_t *x);
 void mbx_seed(void *km, int32_t n, const int32_t *len, uint8_t *const *seq, int32_t min_len, int32_t max_sub_occ, mb_sai_v *v);
 /* Accumulated device-side counters for the run summary (may be all zero). */
-typedef struct { int64_t dp_calls, dp_jobs, ll_calls, ll_jobs, fronts, front_reads; double dp_s, ll_s, front_s; } mbx_stats_t;
+typedef struct {
+	int64_t dp_calls, dp_jobs, ll_calls, ll_jobs, fronts, front_reads;
+	double dp_s, ll_s, front_s;            /* host wall time inside the executor's calls */
+	double dp_pack_s;                      /* of dp_s: packing the rounds' sequences */
+	mbg_counters_t dev;                    /* the device's own counters (zero for a CPU executor) */
+} mbx_stats_t;
 void mbx_get_stats(mbx_stats_t *st);
 
 #ifdef __cplusplus

@@ -1,12 +1,17 @@
 /*
  * mbg_exec_gpu.c -- the batched mapper's device requests (mbg_exec.h) executed by libmbg.so on one
- * MI355X: the front pass through mbg_map_front, each round's DP jobs through mbg_extd2_batch /
+ * MI355X: the front pass through mbg_map_front (on one of MBX_FRONT_SLOTS contexts that share the
+ * resident index, so that a mini-batch's front overlaps the previous one's rounds), each round's DP
+ * jobs through mbg_extd2_batch /
  * mbg_extz2_batch (one call per scoring matrix), its local-SW jobs through mbg_ll_batch.  Sequences
  * are packed into page-locked staging buffers so each round crosses PCIe in a few large copies.
  * Any library error aborts with its message, as the reference does on internal failures.
  *
  * Environment: MBG_DEVICE (HIP device, default 0), MBG_PRE_LEN (k-mer interval cache length on the
- * device; default 15 for indexes over 2^30 rows, else 12; results do not depend on it).
+ * device; default 15 for indexes over 2^30 rows, else 12; results do not depend on it),
+ * MBG_COUNT_WORK=N (count SMEM pass 1's algorithmic work during the first N front passes; counting
+ * adds atomics to the seeding kernels), MBG_MAP_STATS=1 (one stderr line per front pass with its
+ * device times).
  */
 #include <pthread.h>
 #include <stdio.h>
@@ -19,9 +24,14 @@
 typedef char sai_layout_check[sizeof(mb_sai_t) == sizeof(mbg_sai_t) ? 1 : -1];
 typedef char anchor_layout_check[sizeof(mb_anchor_t) == sizeof(mbg_anchor_t) ? 1 : -1];
 
-static mbg_ctx_t *g_ctx;
+static mbg_ctx_t *g_ctx;                        /* the index, the DP and local-SW rounds */
+static mbg_ctx_t *g_fctx[MBX_FRONT_SLOTS];      /* front passes: one context per slot, on g_ctx's index */
 static const mb_idx_t *g_idx;
 static mbx_stats_t g_st;
+static pthread_mutex_t g_st_mu = PTHREAD_MUTEX_INITIALIZER;
+static int g_front_log;          /* MBG_MAP_STATS: one line per front pass */
+static double g_pack_s;          /* host time packing DP rounds */
+static long g_count_fronts;      /* MBG_COUNT_WORK=N: count pass-1 work during the first N front passes */
 
 /* staging buffers (page-locked, grow-only) */
 typedef struct { void *p; size_t n; } hbuf_t;
@@ -69,6 +79,10 @@ void mbx_index(const mb_idx_t *idx)
 		s = getenv("MBG_DEVICE");
 		chk(mbg_init(s ? atoi(s) : 0, &g_ctx), "mbg_init");
 	}
+	s = getenv("MBG_COUNT_WORK");
+	if (s && atoi(s) > 0) g_count_fronts = atol(s);
+	s = getenv("MBG_MAP_STATS");
+	g_front_log = s && atoi(s);
 	s = getenv("MBG_PRE_LEN");
 	pre = s ? atoi(s) : bwt->seq_len > (1ULL << 30) ? 15 : 12;
 	d.primary = bwt->primary;
@@ -84,20 +98,48 @@ void mbx_index(const mb_idx_t *idx)
 	g_idx = idx;
 }
 
+static mbg_ctx_t *front_ctx(int slot)
+{
+	mbg_ctx_t *c;
+	if (slot < 0 || slot >= MBX_FRONT_SLOTS) slot = 0;
+	pthread_mutex_lock(&g_st_mu);
+	if (g_fctx[slot] == 0) {
+		const char *s = getenv("MBG_DEVICE");
+		chk(mbg_init(s ? atoi(s) : 0, &g_fctx[slot]), "mbg_init");
+		chk(mbg_idx_attach(g_fctx[slot], g_ctx), "mbg_idx_attach");
+		if (g_count_fronts > 0) chk(mbg_count_work(g_fctx[slot], 1), "mbg_count_work");
+	}
+	c = g_fctx[slot];
+	pthread_mutex_unlock(&g_st_mu);
+	return c;
+}
+
 void mbx_front(const mbx_front_in_t *in, mbx_front_t *out)
 {
 	mbg_front_opt_t o;
 	mbg_front_t r;
+	mbg_ctx_t *c = front_ctx(in->slot);
 	double t0 = now_s();
+	int k;
 	o.min_len = in->min_len, o.max_sub_occ = in->max_sub_occ, o.max_occ = in->max_occ;
 	o.n_copt = in->n_copt, o.copt = in->copt, o.copt_idx = in->copt_idx, o.no_chain = in->no_chain;
-	chk(mbg_map_front(g_ctx, in->n, in->nt4, in->off, in->len, &o, &r), "mbg_map_front");
+	chk(mbg_map_front(c, in->n, in->nt4, in->off, in->len, &o, &r), "mbg_map_front");
 	out->n = in->n;
 	out->intv_off = r.intv_off, out->n_intv = r.n_intv, out->intv = (const mb_sai_t*)r.intv;
 	out->anc_off = r.anc_off, out->n_anc = r.n_anc, out->anc = (const mb_anchor_t*)r.anc;
 	out->n_u = r.n_u, out->u = r.u, out->n_chn = r.n_chn, out->chn = (const mb_anchor_t*)r.chn;
 	out->dev_ms = r.ms_total;
+	pthread_mutex_lock(&g_st_mu);
 	++g_st.fronts, g_st.front_reads += in->n, g_st.front_s += now_s() - t0;
+	if (g_front_log)
+		fprintf(stderr, "[mbg_exec] front %ld reads %d records %ld slots %ld seed1_ms %.4f seed2_ms %.4f group_ms %.4f anchor_ms %.4f "
+		        "chain_ms %.4f copy_ms %.4f total_ms %.4f seed1_occ_blocks %lu seed1_kmer_lookups %lu\n", (long)g_st.fronts, in->n,
+		        (long)r.n_rec, (long)r.n_slots, r.ms_seed1, r.ms_seed2, r.ms_group, r.ms_anchor, r.ms_chain, r.ms_copy, r.ms_total,
+		        (unsigned long)r.seed1_occ_blocks, (unsigned long)r.seed1_kmer_lookups);
+	if (g_count_fronts > 0 && --g_count_fronts == 0)
+		for (k = 0; k < MBX_FRONT_SLOTS; ++k)
+			if (g_fctx[k]) chk(mbg_count_work(g_fctx[k], 0), "mbg_count_work");
+	pthread_mutex_unlock(&g_st_mu);
 }
 
 /* the requests of one scoring matrix in one library call */
@@ -113,21 +155,27 @@ static void dp_group(int dual, int32_t n, mbx_req_t *const *req, const int32_t *
 		const mbx_req_t *r = req[sel[i]];
 		qb += r->qlen, tb += r->tlen, cb += (int64_t)r->qlen + r->tlen;
 	}
+	double t0 = now_s();
 	q = (uint8_t*)grow(&b_q, qb + 16), t = (uint8_t*)grow(&b_t, tb + 16);
 	jobs = (mbg_dp_job_t*)grow(&b_jobs, (size_t)n * sizeof(mbg_dp_job_t));
 	res = (mbg_dp_result_t*)grow(&b_res, (size_t)n * sizeof(mbg_dp_result_t));
 	cig = (uint32_t*)grow(&b_cig[dual != 0], (size_t)cb * 4 + 16);
-	for (i = 0, qb = tb = cb = 0; i < n; ++i) {
+	for (i = 0, qb = tb = cb = 0; i < n; ++i) {          /* offsets first, so the copies can run in parallel */
 		const mbx_req_t *r = req[sel[i]];
 		mbg_dp_job_t *j = &jobs[i];
 		j->qoff = qb, j->toff = tb, j->qlen = r->qlen, j->tlen = r->tlen;
 		j->q = r->gq, j->e = r->ge, j->q2 = r->gq2, j->e2 = r->ge2;
 		j->w = r->w, j->zdrop = r->zdrop, j->end_bonus = r->end_bonus, j->flag = r->flag;
 		j->cigar_off = cb;
-		if (r->qlen) memcpy(q + qb, r->q, r->qlen);
-		if (r->tlen) memcpy(t + tb, r->t, r->tlen);
 		qb += r->qlen, tb += r->tlen, cb += (int64_t)r->qlen + r->tlen;
 	}
+#pragma omp parallel for schedule(static) if (n >= 8192)
+	for (i = 0; i < n; ++i) {
+		const mbx_req_t *r = req[sel[i]];
+		if (r->qlen) memcpy(q + jobs[i].qoff, r->q, r->qlen);
+		if (r->tlen) memcpy(t + jobs[i].toff, r->t, r->tlen);
+	}
+	g_pack_s += now_s() - t0;
 	if (dual) chk(mbg_extd2_batch(g_ctx, n, jobs, q, qb, t, tb, req[sel[0]]->m, req[sel[0]]->mat, res, cig, cb), "mbg_extd2_batch");
 	else chk(mbg_extz2_batch(g_ctx, n, jobs, q, qb, t, tb, req[sel[0]]->m, req[sel[0]]->mat, res, cig, cb), "mbg_extz2_batch");
 	for (i = 0; i < n; ++i) {
@@ -195,7 +243,9 @@ void mbx_run_dp(int dual, int32_t n, mbx_req_t *const *req)
 				keep += r->dp.n_cigar;
 			}
 	}
+	pthread_mutex_lock(&g_st_mu);
 	++g_st.dp_calls, g_st.dp_jobs += n, g_st.dp_s += now_s() - t0;
+	pthread_mutex_unlock(&g_st_mu);
 }
 
 void mbx_run_ll(int32_t n, mbx_req_t *const *req)
@@ -236,7 +286,9 @@ void mbx_run_ll(int32_t n, mbx_req_t *const *req)
 		for (i = 0; i < n_sel; ++i) req[sel[i]]->ll = res[i];
 	}
 	free(taken);
+	pthread_mutex_lock(&g_st_mu);
 	++g_st.ll_calls, g_st.ll_jobs += n, g_st.ll_s += now_s() - t0;
+	pthread_mutex_unlock(&g_st_mu);
 }
 
 void mbx_sa(int64_t n, uint64_t *x)
@@ -265,4 +317,23 @@ void mbx_seed(void *km, int32_t n, const int32_t *len, uint8_t *const *seq, int3
 	free(off);
 }
 
-void mbx_get_stats(mbx_stats_t *st) { *st = g_st; }
+void mbx_get_stats(mbx_stats_t *st)
+{
+	int k;
+	pthread_mutex_lock(&g_st_mu);
+	*st = g_st;
+	st->dp_pack_s = g_pack_s;
+	memset(&st->dev, 0, sizeof(st->dev));
+	if (g_ctx) chk(mbg_counters(g_ctx, &st->dev, 0), "mbg_counters");
+	for (k = 0; k < MBX_FRONT_SLOTS; ++k) {          /* the front passes' counters live on their contexts */
+		mbg_counters_t f;
+		if (g_fctx[k] == 0) continue;
+		chk(mbg_counters(g_fctx[k], &f, 0), "mbg_counters");
+		st->dev.fronts += f.fronts, st->dev.front_reads += f.front_reads, st->dev.front_records += f.front_records;
+		st->dev.front_slots += f.front_slots, st->dev.ms_seed1 += f.ms_seed1, st->dev.ms_seed2 += f.ms_seed2;
+		st->dev.ms_group += f.ms_group, st->dev.ms_anchor += f.ms_anchor, st->dev.ms_chain += f.ms_chain;
+		st->dev.ms_front_copy += f.ms_front_copy, st->dev.ms_front += f.ms_front;
+		st->dev.seed1_occ_blocks += f.seed1_occ_blocks, st->dev.seed1_kmer_lookups += f.seed1_kmer_lookups;
+	}
+	pthread_mutex_unlock(&g_st_mu);
+}

@@ -7,9 +7,12 @@
  * mapping step of its pipeline and binds the hook points below.
  *
  * Linked into the unmodified reference objects with -Wl,--wrap for:
- *   kt_pipeline          (kthread.h:10; only caller mb_map_file, map-main.c:292)  the mapping step of
- *                         worker_pipeline (map-main.c:176-192) is replaced by map_step() below; reading
- *                         (step 0) and output (step 2) stay the reference's
+ *   kt_pipeline          (kthread.h:10; only caller mb_map_file, map-main.c:292)  worker_pipeline's three
+ *                         steps (map-main.c:112-256) run as read_step, map_step and write_step below,
+ *                         still overlapped by the reference's kt_pipeline
+ *   mb_bseq_open         (bseq.h:20)  records each input's name so that read_step can read the same
+ *                         files with the native reader (mbg_fq_read / mbg_fq_read_frag, include/mbg.h),
+ *                         whose records equal mb_bseq_read / mb_bseq_read_frag's
  *   mb_map_batch         (minibwa.h:170, map-algo.c:703-770)  the same machinery for API callers
  *   mb_idx_load          (minibwa.h:123, map-algo.c:19-42)  the loaded index is made resident
  *   mb_anchor, mb_lchain_dp (mbpriv.h:58,62)  answered from the mini-batch's device front pass
@@ -33,9 +36,13 @@
  * Results are the reference's by construction: the same functions run on the same data, and every
  * deferred call returns what the reference's call returns (the device kernels are bit-exact).
  *
- * The layouts of pipeline_t and step_t (map-main.c:16-34, minibwa 0.7-r421) are restated below because
- * kt_pipeline hands them over as void*; map_step checks that the pipeline's fields agree with its
- * options before using them.
+ * Mini-batches are read, mapped and written as worker_pipeline does: the same chunking (mb_bseq_read(_frag)
+ * rules), the same fragments and name trimming, the same insert-size model per mini-batch, the same
+ * records in the same order.  Formatting (mb_format per record, map-main.c:207-239) runs on the worker
+ * pool in fragment ranges whose strings the output step writes in order.
+ *
+ * The layout of pipeline_t (map-main.c:16-24, minibwa 0.7-r421) is restated below because kt_pipeline
+ * hands it over as void*; the pipeline checks that its fields agree with its options before use.
  *
  * Environment: MBG_MAP_THREADS (pool size for mb_map_batch; the CLI uses -t), MBG_MAP_STATS=1 (per
  * mini-batch timing on stderr).
@@ -58,7 +65,7 @@
 #include "mbg_exec.h"
 #include "mbg_sched.h"
 
-/* ---- the reference's pipeline records (map-main.c:16-34) --------------------------------------- */
+/* ---- the reference's pipeline record (map-main.c:16-24) ----------------------------------------- */
 
 typedef struct {
 	int32_t n_fp, n_threads;
@@ -70,16 +77,6 @@ typedef struct {
 	FILE *fp_out;
 } ref_pipeline_t;
 
-typedef struct {
-	const ref_pipeline_t *p;
-	int32_t n_seq, n_frag, n_sb, n_pe;
-	int32_t *n_hit, *seg_off, *seg_cnt;
-	int32_t *sb_off, *sb_cnt;
-	mb_pestat_t pes[4];
-	mb_bseq1_t *seq;
-	mb_hit_t **hit;
-	mb_tbuf_t **tbuf;
-} ref_step_t;
 
 /* ---- global state ------------------------------------------------------------------------------ */
 
@@ -112,13 +109,21 @@ static void report(void)
 {
 	mbs_stats_t ss;
 	mbx_stats_t xs;
+	const mbg_counters_t *d = &xs.dev;
 	memset(&ss, 0, sizeof(ss));
 	if (g_pool) mbs_get_stats(g_pool, &ss);
 	mbx_get_stats(&xs);
 	fprintf(stderr, "[mbg_map] summary: executor %s fronts %ld reads %ld front_s %.3f | rounds %ld yields %ld peak_coroutines %ld "
-	        "round_s %.3f run_s %.3f | dp_calls %ld dp_jobs %ld dp_s %.3f | ll_calls %ld ll_jobs %ld ll_s %.3f\n",
+	        "round_s %.3f run_s %.3f phase_s %.3f busy_s %.3f | dp_calls %ld dp_jobs %ld dp_s %.3f dp_pack_s %.3f | ll_calls %ld ll_jobs %ld ll_s %.3f | "
+	        "dev_seed1_ms %.3f dev_seed2_ms %.3f dev_group_ms %.3f dev_anchor_ms %.3f dev_chain_ms %.3f dev_front_copy_ms %.3f "
+	        "dev_front_ms %.3f seed_records %ld anchor_slots %ld seed1_occ_blocks %lu seed1_kmer_lookups %lu dev_dp_launches %ld "
+	        "dev_dp_jobs %ld dev_dp_cells %ld dev_dp_ms %.3f dev_ll_launches %ld dev_ll_cells %ld dev_ll_ms %.3f\n",
 	        mbx_name(), (long)xs.fronts, (long)xs.front_reads, xs.front_s, (long)ss.rounds, (long)ss.yields, (long)ss.coroutines_peak,
-	        ss.round_s, ss.task_s, (long)xs.dp_calls, (long)xs.dp_jobs, xs.dp_s, (long)xs.ll_calls, (long)xs.ll_jobs, xs.ll_s);
+	        ss.round_s, ss.task_s, ss.phase_s, ss.busy_s, (long)xs.dp_calls, (long)xs.dp_jobs, xs.dp_s, xs.dp_pack_s, (long)xs.ll_calls,
+	        (long)xs.ll_jobs, xs.ll_s,
+	        d->ms_seed1, d->ms_seed2, d->ms_group, d->ms_anchor, d->ms_chain, d->ms_front_copy, d->ms_front, (long)d->front_records,
+	        (long)d->front_slots, (unsigned long)d->seed1_occ_blocks, (unsigned long)d->seed1_kmer_lookups, (long)d->dp_launches,
+	        (long)d->dp_jobs, (long)d->dp_cells, d->ms_dp, (long)d->ll_launches, (long)d->ll_cells, d->ms_ll);
 }
 
 static mbs_pool_t *pool_for(int n_threads)
@@ -168,6 +173,7 @@ typedef struct {
 	int32_t n_pair;
 	const int32_t *pair_off;         /* pair k: reads pair_off[k], pair_off[k] + 1 */
 	const mb_pestat_t *pes;
+	double t_front;
 } batch_t;
 
 /* per coroutine: the read being mapped and the anchors handed to its chaining call */
@@ -268,32 +274,46 @@ static void chain_opt_of(const mb_opt_t *opt, int32_t qlen, mbg_chain_opt_t *o)
 	o->pen_gap = (float)(oa.chain_gap_scale * .01 * oa.min_len);
 }
 
-/* The front pass and the single-end mapping of a mini-batch. */
-static void map_reads(batch_t *b, int n_threads)
+/* The front pass of a mini-batch on front context `slot`.  nt4/off (may be null): the reads' nt4 codes
+ * as the native reader left them (kom_nt4_table's), used as they are for unconverted reads. */
+static void front_reads(batch_t *b, int slot, const uint8_t *nt4, const int64_t *off)
 {
-	mbs_pool_t *pool = pool_for(n_threads);
 	int64_t i, tot = 0;
-	double t0 = now_s(), t1, t2;
+	double t0 = now_s();
 	mbx_front_in_t in;
 	int32_t last_len = -1;
+	int any_meth = 0;
 	b->off = (int64_t*)malloc((b->n_seq + 1) * sizeof(int64_t));
 	b->len = (int32_t*)malloc((b->n_seq + 1) * sizeof(int32_t));
 	b->no_chain = (uint8_t*)calloc(b->n_seq + 1, 1);
 	b->copt_idx = (int32_t*)malloc((b->n_seq + 1) * sizeof(int32_t));
-	b->copt = 0, b->n_copt = 0;
-	for (i = 0; i < b->n_seq; ++i) tot += b->qlen[i];
-	b->nt4 = (uint8_t*)malloc(tot + 1);
-	if (!b->off || !b->len || !b->no_chain || !b->copt_idx || !b->nt4) die("out of memory (mini-batch)");
+	b->copt = 0, b->n_copt = 0, b->nt4 = 0;
+	if (!b->off || !b->len || !b->no_chain || !b->copt_idx) die("out of memory (mini-batch)");
+	for (i = 0; i < b->n_seq; ++i) {
+		tot += b->qlen[i];
+		if (b->mt[i] != L2B_METH_NONE) any_meth = 1;
+	}
+	if (nt4 == 0 || any_meth) {
+		b->nt4 = (uint8_t*)malloc(tot + 1);
+		if (b->nt4 == 0) die("out of memory (mini-batch)");
+	}
 	for (i = 0, tot = 0; i < b->n_seq; ++i) {
-		int32_t l, n = b->qlen[i];
-		const uint8_t *s = (const uint8_t*)b->seq[i];
-		uint8_t *d = b->nt4 + tot;
-		b->off[i] = tot, b->len[i] = n;
-		for (l = 0; l < n; ++l) d[l] = kom_nt4_table[s[l]];
-		if (b->mt[i] != L2B_METH_NONE) {
-			l2b_meth_convert((l2b_meth_t)b->mt[i], n, d);
-			b->no_chain[i] = 1;
-		}
+		int32_t n = b->qlen[i];
+		if (b->nt4) {
+			uint8_t *d = b->nt4 + tot;
+			if (nt4) memcpy(d, nt4 + off[i], n);
+			else {
+				const uint8_t *q = (const uint8_t*)b->seq[i];
+				int32_t l;
+				for (l = 0; l < n; ++l) d[l] = kom_nt4_table[q[l]];
+			}
+			if (b->mt[i] != L2B_METH_NONE) {
+				l2b_meth_convert((l2b_meth_t)b->mt[i], n, d);
+				b->no_chain[i] = 1;
+			}
+			b->off[i] = tot;
+		} else b->off[i] = off[i];
+		b->len[i] = n;
 		tot += n;
 		/* reads of one length share their chaining options; lengths mostly repeat in runs */
 		if (n != last_len) {
@@ -312,21 +332,37 @@ static void map_reads(batch_t *b, int n_threads)
 			b->copt_idx[i] = k, last_len = n;
 		} else b->copt_idx[i] = b->copt_idx[i - 1];
 	}
-	b->off[b->n_seq] = tot;
+	b->off[b->n_seq] = b->nt4 ? tot : (b->n_seq ? off[b->n_seq - 1] + b->qlen[b->n_seq - 1] : 0);
 	memset(&in, 0, sizeof(in));
-	in.n = b->n_seq, in.nt4 = b->nt4, in.off = b->off, in.len = b->len, in.no_chain = b->no_chain;
+	in.slot = slot;
+	in.n = b->n_seq, in.nt4 = b->nt4 ? b->nt4 : nt4, in.off = b->off, in.len = b->len, in.no_chain = b->no_chain;
 	in.min_len = b->opt->min_len, in.max_sub_occ = b->opt->max_sub_occ, in.max_occ = b->opt->max_occ;
 	in.n_copt = b->n_copt, in.copt = b->copt, in.copt_idx = b->copt_idx;
 	mbx_front(&in, &b->front);
-	t1 = now_s();
+	b->t_front = now_s() - t0;
+}
+
+/* The single-end mapping of a mini-batch whose front pass has run. */
+static void se_reads(batch_t *b, int n_threads)
+{
+	mbs_pool_t *pool = pool_for(n_threads);
+	double t0 = now_s(), t1;
 	mbs_run(pool, b->n_seq, 64, se_task, round_cb, b);
-	t2 = now_s();
+	t1 = now_s();
 	if (g_stats) {
 		mbs_stats_t st;
 		mbs_get_stats(pool, &st);
-		fprintf(stderr, "[mbg_map] %d reads: front %.3f s (device %.1f ms), SE %.3f s; rounds so far %ld, peak coroutines %ld\n",
-		        b->n_seq, t1 - t0, b->front.dev_ms, t2 - t1, (long)st.rounds, (long)st.coroutines_peak);
+		fprintf(stderr, "[mbg_map] %d reads: front %.3f s (device %.1f ms), SE %.3f s; rounds so far %ld, peak coroutines %ld, "
+		        "compute phases %.3f s, thread busy %.3f s, round callbacks %.3f s (cumulative)\n",
+		        b->n_seq, b->t_front, b->front.dev_ms, t1 - t0, (long)st.rounds, (long)st.coroutines_peak, st.phase_s, st.busy_s,
+		        st.round_s);
 	}
+}
+
+static void map_reads(batch_t *b, int n_threads)
+{
+	front_reads(b, 0, 0, 0);
+	se_reads(b, n_threads);
 }
 
 static void map_free(batch_t *b)
@@ -344,41 +380,236 @@ static void pair_reads(batch_t *b)
 	if (g_stats) fprintf(stderr, "[mbg_map] %d pairs paired in %.3f s\n", b->n_pair, now_s() - t0);
 }
 
-/* ---- the CLI: kt_pipeline's mapping step --------------------------------------------------------- */
+/* ---- the CLI: kt_pipeline's three steps ---------------------------------------------------------- */
 
 void __real_kt_pipeline(int n_threads, void *(*func)(void*, int, void*), void *shared_data, int n_steps);
+mb_bseq_file_t *__real_mb_bseq_open(const char *fn);
+
+/* the file name behind each reader mb_open_bseqs opened (map-main.c:257-272), so that the mini-batches
+ * can be read by the native reader instead */
+#define MAX_INPUTS 64
+static struct { mb_bseq_file_t *fp; char *fn; } g_inputs[MAX_INPUTS];
+static int g_n_inputs;
+
+mb_bseq_file_t *__wrap_mb_bseq_open(const char *fn)
+{
+	mb_bseq_file_t *fp = __real_mb_bseq_open(fn);
+	if (fp) {
+		pthread_mutex_lock(&g_mu);
+		g_inputs[g_n_inputs % MAX_INPUTS].fp = fp;
+		free(g_inputs[g_n_inputs % MAX_INPUTS].fn);
+		g_inputs[g_n_inputs % MAX_INPUTS].fn = strdup(fn ? fn : "-");
+		++g_n_inputs;
+		pthread_mutex_unlock(&g_mu);
+	}
+	return fp;
+}
 
 static struct {
-	void *(*func)(void*, int, void*);
-	void *shared;
+	const ref_pipeline_t *p;
+	mbg_fq_t *fq[MAX_INPUTS];
+	int n_fq;
+	int64_t n_seq;                /* reads so far: each read's id (map-main.c:119) */
+	int64_t n_batch;              /* mini-batches read so far: the front slot of the next one */
 } g_pipe;
+static mbs_pool_t *g_fmt_pool;    /* formatting in the output step, beside the mapping pool */
 
-static void *map_step(ref_step_t *s)
-{
-	const ref_pipeline_t *p = s->p;
-	const mb_opt_t *opt = p->opt;
-	const mb_idx_t *idx = p->idx;
+typedef struct {
+	mbg_fq_buf_t *buf;
+	mbg_fq_chunk_t ch;
+	int32_t n_seq, n_frag, n_pe;
+	mb_bseq1_t *seq;              /* names and comments in arena (NUL-terminated); bases and qualities in the chunk */
+	char *arena;
+	int32_t *seg_off, *seg_cnt, *n_hit;
+	mb_hit_t **hit;
+	mb_pestat_t pes[4];
+	kstring_t *out;               /* the formatted records, in read order, one string per format task */
+	int32_t n_out;
+	int64_t tot_len;
+	double t_read, t_map, t_fmt;
+	/* the mapping of the mini-batch (its front pass runs in the read step) */
 	batch_t b;
-	int32_t k, j, *pair_off = 0, n_pair = 0;
+	int32_t *qlen;
+	const char **sq, **nm;
 	uint8_t *mt;
-	if (opt == 0 || idx == 0 || p->mb_size != opt->mb_size || p->n_fp < 1)
-		die("the pipeline record does not match minibwa 0.7-r421's layout");
-	mt = (uint8_t*)calloc(s->n_seq + 1, 1);
-	for (k = 0; k < s->n_frag; ++k)       /* worker_for_se_batch's conversion (map-main.c:66) */
-		for (j = 0; j < s->seg_cnt[k]; ++j)
-			mt[s->seg_off[k] + j] = !idx->is_meth ? L2B_METH_NONE : (j & 1) == 0 ? L2B_METH_C2T : L2B_METH_G2A;
-	memset(&b, 0, sizeof(b));
-	b.opt = opt, b.idx = idx, b.n_seq = s->n_seq, b.mt = mt;
-	{
-		int32_t *ql = (int32_t*)malloc((s->n_seq + 1) * sizeof(int32_t));
-		const char **sq = (const char**)malloc((s->n_seq + 1) * sizeof(char*));
-		const char **nm = (const char**)malloc((s->n_seq + 1) * sizeof(char*));
-		for (k = 0; k < s->n_seq; ++k)
-			ql[k] = (int32_t)s->seq[k].l_seq, sq[k] = s->seq[k].seq, nm[k] = s->seq[k].name;
-		b.qlen = ql, b.seq = sq, b.name = nm;
+} pbatch_t;
+
+static void die_rc(const char *what, int rc)
+{
+	fprintf(stderr, "[mbg_map] %s failed (mbg error %d)\n", what, rc);
+	abort();
+}
+
+/* step 0 (map-main.c:114-173): a mini-batch of reads, then fragments (mates sharing a name) and the
+ * trimming of "/1" "/2" name suffixes */
+static pbatch_t *read_step(const ref_pipeline_t *p)
+{
+	const mb_opt_t *opt = p->opt;
+	int with_qual = !(opt->flag & MB_F_PAF), with_comment = !!(opt->flag & MB_F_COPY_COMMENT);
+	int frag_mode = (p->n_fp > 1 || !!(opt->flag & MB_F_PE));
+	pbatch_t *s;
+	int32_t i, j;
+	int64_t an = 0;
+	double t0 = now_s();
+	int rc;
+	if (g_pipe.n_fq == 0) {          /* the first mini-batch opens the native readers on the same files */
+		pthread_mutex_lock(&g_mu);
+		for (i = 0; i < p->n_fp; ++i) {
+			const char *fn = 0;
+			for (j = 0; j < g_n_inputs && j < MAX_INPUTS; ++j)
+				if (g_inputs[j].fp == p->fp[i]) fn = g_inputs[j].fn;
+			if (fn == 0) die("an input was not opened through mb_bseq_open");
+			if ((rc = mbg_fq_open(fn, 4, &g_pipe.fq[i])) != MBG_OK) die_rc("mbg_fq_open", rc);
+		}
+		g_pipe.n_fq = p->n_fp;
+		pthread_mutex_unlock(&g_mu);
 	}
-	b.n_hit = s->n_hit, b.hit = s->hit;
-	map_reads(&b, opt->n_thread);
+	s = (pbatch_t*)calloc(1, sizeof(pbatch_t));
+	if ((rc = mbg_fq_buf_new(&s->buf)) != MBG_OK) die_rc("mbg_fq_buf_new", rc);
+	if (p->n_fp > 1) rc = mbg_fq_read_frag(p->n_fp, g_pipe.fq, p->mb_size, with_qual, with_comment, s->buf, &s->ch);
+	else rc = mbg_fq_read(g_pipe.fq[0], p->mb_size, with_qual, with_comment, frag_mode, 40000, opt->max_mb_size, s->buf, &s->ch);
+	if (rc != MBG_OK) die_rc("reading the input", rc);
+	if (s->ch.status == -2 && p->n_fp == 1) {   /* bseq.c:124-127 */
+		if (s->ch.n) {
+			int64_t l = s->ch.name_off[s->ch.n] - s->ch.name_off[s->ch.n - 1];
+			fprintf(stderr, "[WARNING]\033[1;31m failed to parse the FASTA/FASTQ record next to '%.*s'. Continue anyway.\033[0m\n",
+			        (int)l, s->ch.name + s->ch.name_off[s->ch.n - 1]);
+		} else fprintf(stderr, "[WARNING]\033[1;31m failed to parse the first FASTA/FASTQ record. Continue anyway.\033[0m\n");
+	}
+	if (s->ch.n == 0) {
+		mbg_fq_buf_free(s->buf);
+		free(s);
+		return 0;
+	}
+	s->n_seq = s->ch.n;
+	for (i = 0; i < s->n_seq; ++i)
+		an += (s->ch.name_off[i + 1] - s->ch.name_off[i]) + (s->ch.com_off[i + 1] - s->ch.com_off[i]) + 2;
+	s->arena = (char*)malloc(an + 1);
+	s->seq = (mb_bseq1_t*)calloc(s->n_seq, sizeof(mb_bseq1_t));
+	s->n_hit = (int32_t*)calloc(3 * s->n_seq + 3, sizeof(int32_t));
+	s->seg_off = s->n_hit + s->n_seq + 1, s->seg_cnt = s->seg_off + s->n_seq + 1;
+	s->hit = (mb_hit_t**)calloc(s->n_seq, sizeof(mb_hit_t*));
+	if (!s->arena || !s->seq || !s->n_hit || !s->hit) die("out of memory (mini-batch)");
+	for (i = 0, an = 0; i < s->n_seq; ++i) {
+		mb_bseq1_t *t = &s->seq[i];
+		int64_t nl = s->ch.name_off[i + 1] - s->ch.name_off[i], cl = s->ch.com_off[i + 1] - s->ch.com_off[i];
+		t->name = s->arena + an;
+		memcpy(t->name, s->ch.name + s->ch.name_off[i], nl);
+		t->name[nl] = 0, an += nl + 1;
+		if (cl > 0) {
+			t->comment = s->arena + an;
+			memcpy(t->comment, s->ch.comment + s->ch.com_off[i], cl);
+			t->comment[cl] = 0, an += cl + 1;
+		}
+		t->l_seq = s->ch.seq_off[i + 1] - s->ch.seq_off[i];
+		t->seq = (char*)s->ch.seq + s->ch.seq_off[i];
+		t->qual = s->ch.has_qual[i] ? (char*)s->ch.qual + s->ch.seq_off[i] : 0;
+		t->id = g_pipe.n_seq++;
+	}
+	for (i = 0; i < 4; ++i) s->pes[i].failed = 1;
+	for (i = 1, j = 0; i <= s->n_seq; ++i) {
+		if (i == s->n_seq || !frag_mode || !mb_qname_same(s->seq[i - 1].name, s->seq[i].name)) {
+			if (i - j > 2) die("a fragment of more than two reads");      /* the reference asserts the same */
+			s->seg_cnt[s->n_frag] = i - j;
+			s->seg_off[s->n_frag++] = j;
+			if (i - j == 2) s->n_pe++;
+			j = i;
+		}
+	}
+	if (s->n_pe > 0) {
+		for (i = 0; i < s->n_frag; ++i) {
+			int32_t j0, j1, l0, l1;
+			if (s->seg_cnt[i] != 2) continue;
+			j0 = s->seg_off[i], l0 = strlen(s->seq[j0].name);
+			j1 = s->seg_off[i] + 1, l1 = strlen(s->seq[j1].name);
+			if (l0 < 3 || l0 != l1) continue;
+			if (s->seq[j0].name[l0 - 1] != s->seq[j1].name[l1 - 1] && s->seq[j0].name[l0 - 2] == '/')
+				s->seq[j0].name[l0 - 2] = s->seq[j1].name[l1 - 2] = 0;
+		}
+	}
+	/* the front pass of this mini-batch, overlapping the previous one's mapping */
+	{
+		const mb_idx_t *idx = p->idx;
+		batch_t *b = &s->b;
+		int32_t k;
+		s->mt = (uint8_t*)calloc(s->n_seq + 1, 1);
+		s->qlen = (int32_t*)malloc((s->n_seq + 1) * sizeof(int32_t));
+		s->sq = (const char**)malloc((s->n_seq + 1) * sizeof(char*));
+		s->nm = (const char**)malloc((s->n_seq + 1) * sizeof(char*));
+		for (k = 0; k < s->n_frag; ++k)       /* worker_for_se_batch's conversion (map-main.c:66) */
+			for (j = 0; j < s->seg_cnt[k]; ++j)
+				s->mt[s->seg_off[k] + j] = !idx->is_meth ? L2B_METH_NONE : (j & 1) == 0 ? L2B_METH_C2T : L2B_METH_G2A;
+		for (k = 0; k < s->n_seq; ++k)
+			s->qlen[k] = (int32_t)s->seq[k].l_seq, s->sq[k] = s->seq[k].seq, s->nm[k] = s->seq[k].name, s->tot_len += s->seq[k].l_seq;
+		memset(b, 0, sizeof(*b));
+		b->opt = opt, b->idx = idx, b->n_seq = s->n_seq, b->mt = s->mt, b->qlen = s->qlen, b->seq = s->sq, b->name = s->nm;
+		b->n_hit = s->n_hit, b->hit = s->hit;
+		front_reads(b, (int)(g_pipe.n_batch++ % MBX_FRONT_SLOTS), s->ch.nt4, s->ch.seq_off);
+	}
+	s->t_read = now_s() - t0;
+	return s;
+}
+
+/* formatting of fragments [k*FMT_FRAGS, (k+1)*FMT_FRAGS) as the output step does (map-main.c:207-246) */
+#define FMT_FRAGS 2048
+typedef struct { pbatch_t *s; const mb_opt_t *opt; const mb_idx_t *idx; } fmt_ctx_t;
+
+static void fmt_task(void *data, int64_t k, int tid)
+{
+	const fmt_ctx_t *c = (const fmt_ctx_t*)data;
+	pbatch_t *s = c->s;
+	const mb_opt_t *opt = c->opt;
+	kstring_t *out = &s->out[k];
+	int32_t f, f1 = (int32_t)((k + 1) * FMT_FRAGS < s->n_frag ? (k + 1) * FMT_FRAGS : s->n_frag);
+	void *km = 0;                /* libc allocation (the CLI's default build, MB_F_NO_KALLOC) */
+	(void)tid;
+	for (f = (int32_t)(k * FMT_FRAGS); f < f1; ++f) {
+		int32_t i, j, seg_st = s->seg_off[f], seg_en = s->seg_off[f] + s->seg_cnt[f];
+		for (i = seg_st; i < seg_en; ++i) {
+			mb_bseq1_t *t = &s->seq[i];
+			int32_t mate_qlen = 0;
+			if (seg_en - seg_st > 1) {
+				int32_t mate_idx = i != seg_en - 1 ? i + 1 : seg_st;
+				mate_qlen = s->seq[mate_idx].l_seq;
+			}
+			if (s->n_hit[i] > 0) {
+				int32_t n_sec = 0;
+				for (j = 0; j < s->n_hit[i]; ++j) {
+					const mb_hit_t *h = &s->hit[i][j];
+					if (h->parent == h->id || n_sec < opt->out_n) {
+						if (h->parent != h->id) {
+							const mb_hit_t *q = &s->hit[i][h->parent];
+							if (q->p && h->p) {
+								if (h->p->dp_max < (double)opt->out_s * q->p->dp_max) continue;
+							} else {
+								if (h->score < (double)opt->out_s * q->score) continue;
+							}
+						}
+						mb_format(km, out, c->idx->l2b, t, seg_en - seg_st, &s->n_hit[seg_st], &s->hit[seg_st], j, opt, i - seg_st, mate_qlen);
+						n_sec += (h->parent != h->id);
+					}
+				}
+			} else if (!(opt->flag & MB_F_NO_UNMAP)) {
+				mb_format(km, out, c->idx->l2b, t, seg_en - seg_st, &s->n_hit[seg_st], &s->hit[seg_st], -1, opt, i - seg_st, mate_qlen);
+			}
+		}
+		for (i = seg_st; i < seg_en; ++i) {
+			for (j = 0; j < s->n_hit[i]; ++j) free(s->hit[i][j].p);
+			free(s->hit[i]);
+			s->hit[i] = 0;
+		}
+	}
+}
+
+/* step 1 (map-main.c:176-192): single-end mapping, the insert-size model, pairing */
+static void *map_step(pbatch_t *s)
+{
+	const ref_pipeline_t *p = g_pipe.p;
+	const mb_opt_t *opt = p->opt;
+	batch_t *b = &s->b;
+	int32_t k, *pair_off = 0, n_pair = 0;
+	double t0 = now_s();
+	se_reads(b, opt->n_thread);
 	if ((opt->flag & MB_F_PE) && s->n_frag < s->n_seq && !(opt->flag & (MB_F_NO_PAIRING | MB_F_NO_ALN))) { /* map-main.c:180-192 */
 		if ((opt->flag & MB_F_PE_PREDEF) || s->n_pe < 20) {
 			s->pes[1].failed = 0;
@@ -392,30 +623,74 @@ static void *map_step(ref_step_t *s)
 		pair_off = (int32_t*)malloc((s->n_frag + 1) * sizeof(int32_t));
 		for (k = 0; k < s->n_frag; ++k)
 			if (s->seg_cnt[k] == 2) pair_off[n_pair++] = s->seg_off[k];
-		b.n_pair = n_pair, b.pair_off = pair_off, b.pes = s->pes;
-		pair_reads(&b);
+		b->n_pair = n_pair, b->pair_off = pair_off, b->pes = s->pes;
+		pair_reads(b);
 	}
-	map_free(&b);
+	map_free(b);
 	free(pair_off);
-	free((void*)b.qlen); free((void*)b.seq); free((void*)b.name); free(mt);
+	s->t_map = now_s() - t0;
 	return s;
+}
+
+/* step 2: the formatted records in order, then the mini-batch's memory (map-main.c:242-256) */
+static void write_step(pbatch_t *s)
+{
+	const ref_pipeline_t *p = g_pipe.p;
+	int32_t k;
+	fmt_ctx_t fc;
+	double t0 = now_s(), t1;
+	s->n_out = (s->n_frag + FMT_FRAGS - 1) / FMT_FRAGS;
+	s->out = (kstring_t*)calloc(s->n_out + 1, sizeof(kstring_t));
+	fc.s = s, fc.opt = p->opt, fc.idx = p->idx;
+	mbs_run(g_fmt_pool, s->n_out, 1, fmt_task, 0, &fc);
+	t1 = now_s();
+	s->t_fmt = t1 - t0;
+	t0 = t1;
+	for (k = 0; k < s->n_out; ++k) {
+		if (s->out[k].l) fwrite(s->out[k].s, 1, s->out[k].l, p->fp_out);
+		free(s->out[k].s);
+	}
+	if (kom_verbose >= 3)
+		fprintf(stderr, "[M::%s::%.3f*%.2f] mapped %ld bp in %ld sequences\n", "worker_pipeline", kom_realtime(), kom_percent_cpu(),
+		        (long)s->tot_len, (long)s->n_seq);
+	if (g_stats)
+		fprintf(stderr, "[mbg_map] mini-batch of %d reads: read %.3f s, map %.3f s, format %.3f s, write %.3f s\n", s->n_seq, s->t_read,
+		        s->t_map, s->t_fmt, now_s() - t0);
+	free(s->out); free(s->hit); free(s->n_hit); free(s->seq); free(s->arena);
+	free(s->mt); free(s->qlen); free(s->sq); free(s->nm);
+	mbg_fq_buf_free(s->buf);
+	free(s);
 }
 
 static void *pipe_step(void *data, int step, void *in)
 {
 	(void)data;
-	if (step == 1) return map_step((ref_step_t*)in);
-	return g_pipe.func(g_pipe.shared, step, in);
+	if (step == 0) return read_step(g_pipe.p);
+	if (step == 1) return map_step((pbatch_t*)in);
+	write_step((pbatch_t*)in);
+	return 0;
 }
 
 void __wrap_kt_pipeline(int n_threads, void *(*func)(void*, int, void*), void *shared_data, int n_steps)
 {
+	const ref_pipeline_t *p = (const ref_pipeline_t*)shared_data;
+	int i;
 	if (n_steps != 3) {          /* not the mapper's three-step pipeline */
 		__real_kt_pipeline(n_threads, func, shared_data, n_steps);
 		return;
 	}
-	g_pipe.func = func, g_pipe.shared = shared_data;
+	if (p->opt == 0 || p->idx == 0 || p->mb_size != p->opt->mb_size || p->n_fp < 1 || p->n_fp > MAX_INPUTS)
+		die("the pipeline record does not match minibwa 0.7-r421's layout");
+	pool_for(p->opt->n_thread);
+	if (g_fmt_pool == 0) {       /* formatting overlaps the next mini-batch's mapping */
+		const char *e = getenv("MBG_FMT_THREADS");
+		int n = e ? atoi(e) : p->opt->n_thread;
+		g_fmt_pool = mbs_pool_init(n > 0 ? n : 1, 64 * 1024);
+	}
+	g_pipe.p = p, g_pipe.n_fq = 0, g_pipe.n_seq = p->n_seq, g_pipe.n_batch = 0;
 	__real_kt_pipeline(n_threads, pipe_step, 0, n_steps);
+	for (i = 0; i < g_pipe.n_fq; ++i) mbg_fq_close(g_pipe.fq[i]);
+	g_pipe.n_fq = 0;
 }
 
 /* ---- the public batch API (map-algo.c:703-770) ---------------------------------------------------- */

@@ -1,0 +1,126 @@
+/*
+ * mbg_map_bench.c -- the batched GPU mapper timed over several passes of the same inputs, with the
+ * index loaded (and made resident in HBM) once:
+ *
+ *   mbg-map-bench [-t threads] [-P passes] [-o first.sam] [-O rest.sam] <idx> <reads.fq> [mates.fq]
+ *
+ * Each pass is one call of the reference's mb_map_file (map-main.c:276-299) -- the function
+ * `minibwa map` runs after parsing its options and loading the index -- with minibwa's default options
+ * (mb_opt_init: the adaptive preset) and -t threads, so a pass is exactly a `minibwa map` run without
+ * the index load: FASTQ reading and mini-batching, mapping, pairing and SAM output.  The first pass
+ * writes to -o (default /dev/null), later passes to -O (default /dev/null).  The wall time of each pass
+ * is printed to stderr as "[mbg_bench] pass <i> <seconds>", the index load as "[mbg_bench] load <s>".
+ *
+ * MBG_PROF_OUT=<file>: sample the program counter of whichever thread is on a CPU every millisecond of
+ * process CPU time (setitimer ITIMER_PROF) during the passes after the first, and write the samples
+ * and /proc/self/maps to <file> at the end, for an offline flat profile of the host side.
+ */
+#define _GNU_SOURCE
+#include <signal.h>
+#include <stdatomic.h>
+#include <sys/time.h>
+#include <ucontext.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "kommon.h"
+#include "mbpriv.h"
+#include "ketopt.h"
+
+int32_t mb_map_file(const mb_opt_t *opt, const mb_idx_t *idx, int32_t n, const char **fn, const char *hdr, const char *fn_out);
+
+#define PROF_CAP (1 << 22)
+static uintptr_t *g_prof;
+static atomic_long g_nprof;
+
+static void prof_handler(int sig, siginfo_t *si, void *uc_)
+{
+	ucontext_t *uc = (ucontext_t*)uc_;
+	long k = atomic_fetch_add(&g_nprof, 1);
+	(void)sig, (void)si;
+	if (k < PROF_CAP) g_prof[k] = (uintptr_t)uc->uc_mcontext.gregs[REG_RIP];
+}
+
+static void prof_start(void)
+{
+	struct sigaction sa;
+	struct itimerval it;
+	g_prof = (uintptr_t*)calloc(PROF_CAP, sizeof(uintptr_t));
+	memset(&sa, 0, sizeof(sa));
+	sa.sa_sigaction = prof_handler;
+	sa.sa_flags = SA_SIGINFO | SA_RESTART;
+	sigaction(SIGPROF, &sa, 0);
+	it.it_interval.tv_sec = 0, it.it_interval.tv_usec = 1000;
+	it.it_value = it.it_interval;
+	setitimer(ITIMER_PROF, &it, 0);
+}
+
+static void prof_stop(const char *fn)
+{
+	struct itimerval it;
+	FILE *fp, *maps;
+	long i, n;
+	char line[4096];
+	memset(&it, 0, sizeof(it));
+	setitimer(ITIMER_PROF, &it, 0);
+	n = atomic_load(&g_nprof);
+	if (n > PROF_CAP) n = PROF_CAP;
+	if ((fp = fopen(fn, "w")) == 0) return;
+	maps = fopen("/proc/self/maps", "r");
+	while (maps && fgets(line, sizeof(line), maps)) fprintf(fp, "M %s", line);
+	if (maps) fclose(maps);
+	for (i = 0; i < n; ++i) fprintf(fp, "P %lx\n", (unsigned long)g_prof[i]);
+	fclose(fp);
+}
+
+static double now_s(void)
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return ts.tv_sec + ts.tv_nsec * 1e-9;
+}
+
+int main(int argc, char *argv[])
+{
+	ketopt_t o = KETOPT_INIT;
+	int c, i, passes = 1;
+	const char *out0 = "/dev/null", *out1 = "/dev/null";
+	mb_opt_t opt;
+	mb_idx_t *idx;
+	kstring_t hdr = {0, 0, 0};
+	double t0;
+	kom_realtime();
+	mb_opt_init(&opt);
+	while ((c = ketopt(&o, argc, argv, 1, "t:P:o:O:", 0)) >= 0) {
+		if (c == 't') opt.n_thread = atoi(o.arg);
+		else if (c == 'P') passes = atoi(o.arg);
+		else if (c == 'o') out0 = o.arg;
+		else if (c == 'O') out1 = o.arg;
+	}
+	if (argc - o.ind < 2 || passes < 1) {
+		fprintf(stderr, "Usage: mbg-map-bench [-t threads] [-P passes] [-o first.sam] [-O rest.sam] <idx> <reads.fq> [mates.fq]\n");
+		return 1;
+	}
+	t0 = now_s();
+	idx = mb_idx_load(argv[o.ind], 0);
+	if (idx == 0) {
+		fprintf(stderr, "[mbg_bench] failed to load the index '%s'\n", argv[o.ind]);
+		return 1;
+	}
+	fprintf(stderr, "[mbg_bench] load %.3f\n", now_s() - t0);
+	if (mb_fmt_sam_hdr(&hdr, idx->l2b, 0, MB_VERSION, argc, argv) < 0) return 1;
+	for (i = 0; i < passes; ++i) {
+		double t1 = now_s();
+		if (i == 1 && getenv("MBG_PROF_OUT")) prof_start();
+		if (mb_map_file(&opt, idx, argc - (o.ind + 1), (const char**)&argv[o.ind + 1], hdr.s, i == 0 ? out0 : out1) != 0) {
+			fprintf(stderr, "[mbg_bench] mapping failed\n");
+			return 1;
+		}
+		fprintf(stderr, "[mbg_bench] pass %d %.4f\n", i, now_s() - t1);
+	}
+	if (getenv("MBG_PROF_OUT") && g_prof) prof_stop(getenv("MBG_PROF_OUT"));
+	free(hdr.s);
+	mb_idx_destroy(idx);
+	return 0;
+}

@@ -47,6 +47,7 @@ struct thr_s {
 	co_t *free_co;
 	int64_t n_co;             /* coroutine objects (each with a stack) owned by this thread */
 	int64_t yields;
+	double busy_s;            /* time spent running coroutines */
 };
 
 struct mbs_pool_s {
@@ -68,7 +69,7 @@ struct mbs_pool_s {
 	int finish;
 	/* statistics */
 	int64_t rounds, co_peak;
-	double round_s, task_s;
+	double round_s, task_s, phase_s, phase_t0;
 };
 
 static __thread thr_t *tls_thr;
@@ -205,6 +206,7 @@ static void run_job(thr_t *t)
 	for (;;) {
 		int64_t i, n_old, i0;
 		co_t **old;
+		double tb = now_s();
 		/* 1. resume what the previous round suspended */
 		old = t->susp, n_old = t->n_susp;
 		t->susp = t->susp2, t->susp2 = old;
@@ -227,9 +229,11 @@ static void run_job(thr_t *t)
 			}
 		}
 		atomic_fetch_add(&p->live, t->n_susp);
+		t->busy_s += now_s() - tb;
 		/* 3. the round */
 		if (pthread_barrier_wait(&p->bar) == PTHREAD_BARRIER_SERIAL_THREAD) {
 			int64_t live = atomic_load(&p->live);
+			p->phase_s += now_s() - p->phase_t0;
 			int64_t n_co = 0;
 			int k;
 			for (k = 0; k < p->n; ++k) n_co += p->thr[k].n_susp;
@@ -243,6 +247,7 @@ static void run_job(thr_t *t)
 			}
 			p->finish = live == 0;
 			atomic_store(&p->live, 0);
+			p->phase_t0 = now_s();
 		}
 		pthread_barrier_wait(&p->bar);
 		if (p->finish) break;
@@ -328,6 +333,7 @@ void mbs_run(mbs_pool_t *p, int64_t n, int64_t chunk, mbs_task_f task, mbs_round
 	p->task = task, p->round = round, p->data = data;
 	p->finish = 0;
 	p->n_finished = 0;
+	p->phase_t0 = now_s();
 	++p->gen;
 	pthread_cond_broadcast(&p->cv_start);
 	while (p->n_finished < p->n) pthread_cond_wait(&p->cv_done, &p->mu);
@@ -354,6 +360,6 @@ void mbs_get_stats(const mbs_pool_t *p, mbs_stats_t *st)
 	int i;
 	memset(st, 0, sizeof(*st));
 	st->rounds = p->rounds, st->coroutines_peak = p->co_peak;
-	st->round_s = p->round_s, st->task_s = p->task_s;
-	for (i = 0; i < p->n; ++i) st->yields += p->thr[i].yields, st->stacks += p->thr[i].n_co;
+	st->round_s = p->round_s, st->task_s = p->task_s, st->phase_s = p->phase_s;
+	for (i = 0; i < p->n; ++i) st->yields += p->thr[i].yields, st->stacks += p->thr[i].n_co, st->busy_s += p->thr[i].busy_s;
 }

@@ -47,6 +47,8 @@ int mbs_tid(void);
 typedef struct {
 	int64_t rounds, yields, coroutines_peak, stacks;
 	double round_s, task_s;          /* wall time in round callbacks, and in mbs_run overall */
+	double phase_s;                  /* wall time of the compute phases (from a round's start to the last thread's arrival) */
+	double busy_s;                   /* thread time spent running coroutines, summed over threads */
 } mbs_stats_t;
 void mbs_get_stats(const mbs_pool_t *p, mbs_stats_t *st);
 

@@ -10,6 +10,7 @@
  *   DP      __real_ksw_extd2_sse / __real_ksw_extz2_sse (ksw2.h:73-80)
  *   ll      __real_ksw_ll_qinit + __real_ksw_ll_u8_core / __real_ksw_ll_i16_core (ksw2.h:101-103)
  */
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -35,13 +36,17 @@ ksw_llrst_t __real_ksw_ll_i16_core(void *q, int tlen, const uint8_t *target, int
 
 static const mb_idx_t *g_idx;
 static mbx_stats_t g_st;
+static pthread_mutex_t g_st_mu = PTHREAD_MUTEX_INITIALIZER;
 
-/* front results */
-static int64_t *f_intv_off, *f_n_intv, *f_anc_off, *f_n_anc, *f_n_chn;
-static int32_t *f_n_u;
-static mb_sai_t *f_intv;
-static mb_anchor_t *f_anc, *f_chn;
-static uint64_t *f_u;
+/* front results, one set per slot */
+typedef struct {
+	int64_t *intv_off, *n_intv, *anc_off, *n_anc, *n_chn;
+	int32_t *n_u;
+	mb_sai_t *intv;
+	mb_anchor_t *anc, *chn;
+	uint64_t *u;
+} fres_t;
+static fres_t g_fr[MBX_FRONT_SLOTS];
 /* the round's CIGARs, one buffer per kind (extz2, extd2): a round may hold both */
 static uint32_t *r_cig[2];
 static int64_t r_cig_m[2];
@@ -63,25 +68,26 @@ void mbx_front(const mbx_front_in_t *in, mbx_front_t *out)
 	uint8_t **seq = (uint8_t**)xrealloc(0, (n + 1) * sizeof(uint8_t*));
 	mb_sai_v *sai = (mb_sai_v*)calloc(n + 1, sizeof(mb_sai_v));
 	int64_t ti = 0, ta = 0;
+	fres_t *F = &g_fr[in->slot >= 0 && in->slot < MBX_FRONT_SLOTS ? in->slot : 0];
 	for (r = 0; r < n; ++r) seq[r] = (uint8_t*)in->nt4 + in->off[r];
 	__real_mb_seed_intv_batch(0, g_idx->bwt, n, in->len, seq, in->min_len, in->max_sub_occ, sai);
-	f_intv_off = (int64_t*)xrealloc(f_intv_off, (n + 1) * 8); f_n_intv = (int64_t*)xrealloc(f_n_intv, (n + 1) * 8);
-	f_anc_off = (int64_t*)xrealloc(f_anc_off, (n + 1) * 8); f_n_anc = (int64_t*)xrealloc(f_n_anc, (n + 1) * 8);
-	f_n_chn = (int64_t*)xrealloc(f_n_chn, (n + 1) * 8); f_n_u = (int32_t*)xrealloc(f_n_u, (n + 1) * 4);
+	F->intv_off = (int64_t*)xrealloc(F->intv_off, (n + 1) * 8); F->n_intv = (int64_t*)xrealloc(F->n_intv, (n + 1) * 8);
+	F->anc_off = (int64_t*)xrealloc(F->anc_off, (n + 1) * 8); F->n_anc = (int64_t*)xrealloc(F->n_anc, (n + 1) * 8);
+	F->n_chn = (int64_t*)xrealloc(F->n_chn, (n + 1) * 8); F->n_u = (int32_t*)xrealloc(F->n_u, (n + 1) * 4);
 	for (r = 0; r < n; ++r) {
 		int64_t k, s = 0;
 		for (k = 0; k < (int64_t)sai[r].n; ++k) s += sai[r].a[k].size < (uint64_t)in->max_occ ? (int64_t)sai[r].a[k].size : in->max_occ;
-		f_intv_off[r] = ti, ti += sai[r].n;
-		f_anc_off[r] = ta, ta += s;
+		F->intv_off[r] = ti, ti += sai[r].n;
+		F->anc_off[r] = ta, ta += s;
 	}
-	f_intv_off[n] = ti, f_anc_off[n] = ta;
-	f_intv = (mb_sai_t*)xrealloc(f_intv, ti * sizeof(mb_sai_t));
-	f_anc = (mb_anchor_t*)xrealloc(f_anc, ta * sizeof(mb_anchor_t));
-	f_chn = (mb_anchor_t*)xrealloc(f_chn, ta * sizeof(mb_anchor_t));
-	f_u = (uint64_t*)xrealloc(f_u, ta * sizeof(uint64_t));
+	F->intv_off[n] = ti, F->anc_off[n] = ta;
+	F->intv = (mb_sai_t*)xrealloc(F->intv, ti * sizeof(mb_sai_t));
+	F->anc = (mb_anchor_t*)xrealloc(F->anc, ta * sizeof(mb_anchor_t));
+	F->chn = (mb_anchor_t*)xrealloc(F->chn, ta * sizeof(mb_anchor_t));
+	F->u = (uint64_t*)xrealloc(F->u, ta * sizeof(uint64_t));
 	for (r = 0; r < n; ++r) {
 		mb_anchor_v v = {0, 0, 0};
-		f_n_anc[r] = -1, f_n_u[r] = 0, f_n_chn[r] = 0;
+		F->n_anc[r] = -1, F->n_u[r] = 0, F->n_chn[r] = 0;
 		if (!(in->no_chain && in->no_chain[r]) && sai[r].n > 0) {
 			const mbg_chain_opt_t *o = &in->copt[in->copt_idx[r]];
 			mb_anchor_t *c;
@@ -89,29 +95,31 @@ void mbx_front(const mbx_front_in_t *in, mbx_front_t *out)
 			int n_u = 0, i;
 			int64_t n_c = 0;
 			__real_mb_anchor(0, g_idx, &sai[r], in->min_len, in->len[r], seq[r], L2B_METH_NONE, in->max_occ, &v);
-			f_n_anc[r] = v.n;
-			memcpy(f_anc + f_anc_off[r], v.a, v.n * sizeof(mb_anchor_t));
+			F->n_anc[r] = v.n;
+			memcpy(F->anc + F->anc_off[r], v.a, v.n * sizeof(mb_anchor_t));
 			c = __real_mb_lchain_dp(0, g_idx->l2b, o->max_dist_x, o->max_dist_y, o->bw, o->max_skip, o->max_iter, o->min_sc, o->pen_gap,
 			                        v.n, v.a, &n_u, &u);
 			for (i = 0; i < n_u; ++i) n_c += (int32_t)u[i];
 			if (c) {
-				memcpy(f_u + f_anc_off[r], u, n_u * sizeof(uint64_t));
-				memcpy(f_chn + f_anc_off[r], c, n_c * sizeof(mb_anchor_t));
-				f_n_u[r] = n_u, f_n_chn[r] = n_c;
+				memcpy(F->u + F->anc_off[r], u, n_u * sizeof(uint64_t));
+				memcpy(F->chn + F->anc_off[r], c, n_c * sizeof(mb_anchor_t));
+				F->n_u[r] = n_u, F->n_chn[r] = n_c;
 			}
 			kfree(0, u); kfree(0, c);
-		} else if (sai[r].n == 0) f_n_anc[r] = 0;
-		f_n_intv[r] = sai[r].n;
-		memcpy(f_intv + f_intv_off[r], sai[r].a, sai[r].n * sizeof(mb_sai_t));
+		} else if (sai[r].n == 0) F->n_anc[r] = 0;
+		F->n_intv[r] = sai[r].n;
+		memcpy(F->intv + F->intv_off[r], sai[r].a, sai[r].n * sizeof(mb_sai_t));
 		kfree(0, sai[r].a);
 	}
 	free(sai); free(seq);
 	out->n = n;
-	out->intv_off = f_intv_off, out->n_intv = f_n_intv, out->intv = f_intv;
-	out->anc_off = f_anc_off, out->n_anc = f_n_anc, out->anc = f_anc;
-	out->n_u = f_n_u, out->u = f_u, out->n_chn = f_n_chn, out->chn = f_chn;
+	out->intv_off = F->intv_off, out->n_intv = F->n_intv, out->intv = F->intv;
+	out->anc_off = F->anc_off, out->n_anc = F->n_anc, out->anc = F->anc;
+	out->n_u = F->n_u, out->u = F->u, out->n_chn = F->n_chn, out->chn = F->chn;
 	out->dev_ms = 0;
+	pthread_mutex_lock(&g_st_mu);
 	++g_st.fronts, g_st.front_reads += n;
+	pthread_mutex_unlock(&g_st_mu);
 }
 
 void mbx_run_dp(int dual, int32_t n, mbx_req_t *const *req)
@@ -159,4 +167,4 @@ void mbx_seed(void *km, int32_t n, const int32_t *len, uint8_t *const *seq, int3
 	__real_mb_seed_intv_batch(km, g_idx->bwt, n, len, seq, min_len, max_sub_occ, v);
 }
 
-void mbx_get_stats(mbx_stats_t *st) { *st = g_st; }
+void mbx_get_stats(mbx_stats_t *st) { *st = g_st; memset(&st->dev, 0, sizeof(st->dev)); }
