@@ -302,6 +302,8 @@ typedef struct {
 	const int64_t *n_chn;
 	const mbg_anchor_t *chn;
 	float ms_seed, ms_group, ms_anchor, ms_chain, ms_copy, ms_total;   /* device time per phase */
+	float ms_seed1, ms_seed2;            /* SMEM pass 1 kernel; pass 2 with its task step and compaction */
+	uint64_t seed1_occ_blocks, seed1_kmer_lookups;   /* pass 1's work, while mbg_count_work is on (else 0) */
 } mbg_front_t;
 
 /* The front of mb_map_sai (map-algo.c:588-631) for n_reads reads whose nt4 bases are at
@@ -310,6 +312,24 @@ typedef struct {
  * no host step between them.  Results equal the reference's. */
 int mbg_map_front(mbg_ctx_t *ctx, int32_t n_reads, const uint8_t *seq, const int64_t *off, const int32_t *len,
                   const mbg_front_opt_t *opt, mbg_front_t *out);
+
+/* Cumulative device time (HIP events on the context's stream, in ms) and work of the context's
+ * calls since mbg_init or the last reset: mbg_map_front's phases, the DP batches (mbg_extd2_batch /
+ * mbg_extz2_batch: kernel time and the band cells processed) and the local-SW batches (mbg_ll_batch).
+ * seed1_occ_blocks / seed1_kmer_lookups (the algorithmic bytes of SMEM pass 1: 64 B per occ block
+ * fetched, 32 B per k-mer cache lookup) are counted only while mbg_count_work(ctx, 1) is on; counting
+ * adds atomics to the seeding kernels, so timed runs leave it off.  No counterpart in the reference. */
+typedef struct {
+	int64_t fronts, front_reads, front_records, front_slots;
+	double ms_seed1, ms_seed2, ms_group, ms_anchor, ms_chain, ms_front_copy, ms_front;
+	uint64_t seed1_occ_blocks, seed1_kmer_lookups;
+	int64_t dp_launches, dp_jobs, dp_cells;
+	double ms_dp;
+	int64_t ll_launches, ll_jobs, ll_cells;
+	double ms_ll;
+} mbg_counters_t;
+int mbg_counters(mbg_ctx_t *ctx, mbg_counters_t *out, int reset);
+int mbg_count_work(mbg_ctx_t *ctx, int on);
 
 /* ---- reads at the edge: FASTA/FASTQ (host code) ------------------------------------------------ */
 
@@ -366,6 +386,16 @@ int mbg_seed_batch_dev_async(mbg_ctx_t *ctx, void *stream, int32_t n_seq, const 
                              const uint8_t *seq, int32_t min_len, int32_t max_sub_occ,
                              mbg_seed_rec_t *out, int64_t cap, int64_t *n_out_dev);
 int mbg_sa_batch_dev(mbg_ctx_t *ctx, void *stream, int64_t n, uint64_t *x);
+/* mbg_sa_batch_dev without any allocation: no counterpart in the reference.  It is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The kernel's work-queue counter lives in a 256-byte array of static device memory
+ * (one per GPU, held until the process ends) instead of the context's pools; nothing is allocated,
+ * freed or set by hipMemset.  Up to MBG_SA_NOALLOC_MAX_ROWS rows (else MBG_EUNSUPPORTED).  All calls on
+ * one GPU share that array: calls that overlap in time on one GPU (two streams, or two contexts on the
+ * same device) race, so a caller serialises them.  Asynchronous on the stream, like mbg_sa_batch_dev.
+ * On the device entry points a row outside the index (above seq_len) comes back as 2^64-1. */
+#define MBG_SA_NOALLOC_MAX_ROWS ((int64_t)1 << 31)
+int mbg_sa_batch_dev_noalloc(mbg_ctx_t *ctx, void *stream, int64_t n, uint64_t *x);
 /* mbg_anchor_batch over buffers in HBM: intv_off, anc_off (n_reads+1 each, host-computed as there,
  * anc_off[n_reads] = n_slots), qlen, n_anc and n_intv (n_reads each, both required) and rows, a
  * workspace of n_slots uint64.  Every row x[0] + size - 1 of every interval must be a row of the

@@ -50,7 +50,12 @@ struct mbg_ctx_s {
 	DBuf c_buf[8];          // chains
 	DBuf f_buf[20];         // the mapper's front pass
 	DBuf h_front[10];       // its results in pinned host memory
-	hipEvent_t fev[6] = {};
+	hipEvent_t fev[8] = {};
+	// cumulative counters (mbg_counters)
+	mbg_counters_t cnt{};
+	bool count_work = false;
+	DBuf f_stats;           // pass-1 work counters while count_work is on
+	hipEvent_t dev_ev[2] = {};
 	int64_t cap_task = 0, cap_spill = 0;
 	int64_t cap_rec = 0, cap_rows = 0;
 	hipEvent_t ev[8] = {};
@@ -91,6 +96,40 @@ static int hgrow(DBuf &b, size_t bytes)
 
 #define TRY(x) do { int r_ = (x); if (r_ != MBG_OK) return r_; } while (0)
 
+// ---- the allocation-free SA locate (mbg_sa_batch_dev_noalloc) ----------------------------------
+// sa_kernel's only scratch is its work-queue counter.  For callers that cannot allocate, it lives in
+// this file-scope device array: one copy per GPU, held until the process ends, 256 bytes.  Host code
+// never passes the array's name to a kernel (on the host it is only a stand-in for the symbol): each
+// device's address comes from hipGetSymbolAddress, once per device, at the first mbg_init on it.
+static constexpr int kNoAllocWords = 64;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static uint32_t *g_noalloc_ptr[kMaxDevices];
+
+// with `device` current: resolve this device's copy of the scratch array
+static int noalloc_setup(int device)
+{
+	if (device < 0 || device >= kMaxDevices) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(g_noalloc_mu);
+	if (g_noalloc_ptr[device]) return MBG_OK;
+	void *p = nullptr;
+	hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+	if (e != hipSuccess || p == nullptr) {
+		g_hip_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+		return MBG_EHIP;
+	}
+	g_noalloc_ptr[device] = (uint32_t*)p;
+	return MBG_OK;
+}
+
+// zeroes n words; a kernel rather than hipMemsetAsync, so that the call stays valid under graph capture
+__global__ void noalloc_clear_kernel(uint32_t *p, int n)
+{
+	const int i = threadIdx.x;
+	if (i < n) p[i] = 0;
+}
+
 extern "C" {
 
 const char *mbg_strerror(int err)
@@ -127,6 +166,8 @@ int mbg_init(int device, mbg_ctx_t **out)
 	if (c->n_cu <= 0) c->n_cu = 256;
 	e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
 	if (e != hipSuccess) { g_hip_err = hipGetErrorString(e); delete c; return MBG_EHIP; }
+	int rc = noalloc_setup(device);
+	if (rc != MBG_OK) { hipStreamDestroy(c->stream); delete c; return rc; }
 	*out = c;
 	return MBG_OK;
 }
@@ -171,6 +212,8 @@ void mbg_destroy(mbg_ctx_t *c)
 	for (auto &b : c->h_front) if (b.p) hipHostFree(b.p);
 	for (hipEvent_t &e : c->ev) if (e) hipEventDestroy(e);
 	for (hipEvent_t &e : c->fev) if (e) hipEventDestroy(e);
+	for (hipEvent_t &e : c->dev_ev) if (e) hipEventDestroy(e);
+	dfree(c->f_stats);
 	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
 	if (c->stream) hipStreamDestroy(c->stream);
 	delete c;
@@ -427,7 +470,8 @@ static const int32_t kSlot1 = 16, kSlot2 = 8;
 struct SeedCaps { int64_t rec, task, spill; };
 
 static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
-                     int32_t min_len, int32_t max_sub_occ, unsigned long long *stats, int64_t **n_out_dev, hipEvent_t *ev)
+                     int32_t min_len, int32_t max_sub_occ, unsigned long long *stats, int64_t **n_out_dev, hipEvent_t *ev,
+                     hipEvent_t ev_pre = nullptr)
 {
 	if (c->cap_rec < 8 * (int64_t)n_reads + 1024) c->cap_rec = 8 * (int64_t)n_reads + 1024;
 	if (c->cap_task < 2 * (int64_t)n_reads + 1024) c->cap_task = 2 * (int64_t)n_reads + 1024;
@@ -459,6 +503,7 @@ static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_
 	a.slots = (SeedRec*)c->s_slots1.p; a.slot_cnt = cnt; a.kslot = kSlot1;
 	a.spill = (SeedRec*)c->s_spill.p; a.n_spill = n_spill; a.spill_cap = sc;
 	a.stats = stats;
+	if (ev_pre) HIPCHK(hipEventRecord(ev_pre, st));
 	if (n_reads > 0) HIPCHK(launch_smem(c->ix, a, 1, persistent_grid(c, n_reads, 256), st));
 	if (ev) HIPCHK(hipEventRecord(ev[0], st));
 	if (n_reads > 0) {
@@ -602,6 +647,27 @@ int mbg_sa_batch_dev(mbg_ctx_t *c, void *stream, int64_t n, uint64_t *x)
 	return sa_dev_locked(c, stream ? (hipStream_t)stream : c->stream, n, x);
 }
 
+int mbg_sa_batch_dev_noalloc(mbg_ctx_t *c, void *stream, int64_t n, uint64_t *x)
+{
+	if (c == nullptr || n < 0 || (n > 0 && x == nullptr)) return MBG_EARG;
+	if (n > MBG_SA_NOALLOC_MAX_ROWS) return MBG_EUNSUPPORTED;
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (!c->has_bwt) return MBG_ENOIDX;
+	if (n == 0) return MBG_OK;
+	HIPCHK(hipSetDevice(c->device));
+	uint32_t *ptr = nullptr;
+	{
+		std::lock_guard<std::mutex> g(g_noalloc_mu);
+		ptr = g_noalloc_ptr[c->device];
+	}
+	if (ptr == nullptr) return MBG_EHIP;            // mbg_init resolves it; it cannot be missing here
+	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	hipLaunchKernelGGL(noalloc_clear_kernel, dim3(1), dim3(kNoAllocWords), 0, st, ptr, 1);
+	HIPCHK(hipGetLastError());
+	HIPCHK(launch_sa(c->ix, n, nullptr, x, ptr, nullptr, persistent_grid(c, n, 256), st));
+	return MBG_OK;
+}
+
 int mbg_sa_batch(mbg_ctx_t *c, int64_t n, uint64_t *x)
 {
 	if (c == nullptr || n < 0 || (n > 0 && x == nullptr)) return MBG_EARG;
@@ -714,10 +780,19 @@ static int dp_batch(mbg_ctx_t *c, bool dual, int32_t n_jobs, const mbg_dp_job_t 
 	HIPCHK(hipMemcpyAsync(c->b_in[0].p, qseq, qbytes, hipMemcpyHostToDevice, c->stream));
 	HIPCHK(hipMemcpyAsync(c->b_in[1].p, tseq, tbytes, hipMemcpyHostToDevice, c->stream));
 	HIPCHK(hipMemcpyAsync(c->b_in[4].p, jobs, (size_t)n_jobs * sizeof(DpJob), hipMemcpyHostToDevice, c->stream));
+	for (hipEvent_t &e : c->dev_ev) if (!e) HIPCHK(hipEventCreate(&e));
+	HIPCHK(hipEventRecord(c->dev_ev[0], c->stream));
 	TRY(dp_dev_locked(c, c->stream, dual, n_jobs, (const DpJob*)c->b_in[4].p, (const uint8_t*)c->b_in[0].p, (const uint8_t*)c->b_in[1].p,
 	                  m, mat, (DpRes*)c->b_out[0].p, any_cigar ? (uint32_t*)c->b_out[1].p : nullptr, cap, slab, qt, tcap));
+	HIPCHK(hipEventRecord(c->dev_ev[1], c->stream));
 	HIPCHK(hipMemcpyAsync(res, c->b_out[0].p, (size_t)n_jobs * sizeof(DpRes), hipMemcpyDeviceToHost, c->stream));
 	HIPCHK(hipStreamSynchronize(c->stream));
+	{
+		float ms = 0;
+		HIPCHK(hipEventElapsedTime(&ms, c->dev_ev[0], c->dev_ev[1]));
+		c->cnt.ms_dp += ms, ++c->cnt.dp_launches, c->cnt.dp_jobs += n_jobs;
+		for (int32_t i = 0; i < n_jobs; ++i) c->cnt.dp_cells += res[i].cells;
+	}
 	if (any_cigar) {           // only the operations written cross PCIe: packed on the device at host-computed offsets
 		std::vector<int64_t> poff(n_jobs + 1);
 		int64_t tot = 0;
@@ -1013,14 +1088,27 @@ int mbg_map_front(mbg_ctx_t *c, int32_t n, const uint8_t *seq, const int64_t *of
 	int64_t n_rec = 0;
 	int64_t *n_rec_dev = nullptr;
 	bool ok = false;
+	unsigned long long *wstats = nullptr;
+	if (c->count_work) {
+		TRY(dgrow(c->f_stats, 64));
+		wstats = (unsigned long long*)c->f_stats.p;
+	}
+	hipEvent_t sev[2] = {c->fev[1], c->fev[7]};
 	for (int attempt = 0; attempt < 4 && !ok; ++attempt) {
+		if (wstats) HIPCHK(hipMemsetAsync(wstats, 0, 64, st));
 		TRY(seed_core(c, st, n, (const int64_t*)f[1].p, (const int32_t*)f[2].p, (const uint8_t*)f[0].p, o->min_len, o->max_sub_occ,
-		              nullptr, &n_rec_dev, nullptr));
+		              wstats, &n_rec_dev, sev, c->fev[6]));
 		HIPCHK(hipStreamSynchronize(st));
 		ok = !seed_overflowed(c, &n_rec);
 	}
 	if (!ok) return MBG_EOVERFLOW;
-	HIPCHK(hipEventRecord(c->fev[1], st));
+	uint64_t work1[2] = {0, 0};
+	if (wstats) {
+		unsigned long long w[6];
+		HIPCHK(hipMemcpy(w, wstats, sizeof(w), hipMemcpyDeviceToHost));
+		work1[0] = w[0], work1[1] = w[1];
+		c->cnt.seed1_occ_blocks += w[0], c->cnt.seed1_kmer_lookups += w[1];
+	}
 	// 2. group the records by read; anchor slots per read from the raw records
 	TRY(dgrow(f[5], (size_t)(n + 1) * 4)); TRY(dgrow(f[6], (size_t)(n + 1) * 8)); TRY(dgrow(f[7], (size_t)(n + 1) * 8));
 	TRY(dgrow(f[8], (size_t)(n + 1) * 8)); TRY(dgrow(f[9], (size_t)(n + 1) * 4)); TRY(dgrow(f[10], (size_t)n_rec * sizeof(Sai) + 16));
@@ -1043,9 +1131,9 @@ int mbg_map_front(mbg_ctx_t *c, int32_t n, const uint8_t *seq, const int64_t *of
 	HIPCHK(scan_u64_i64(c->s_tmp.p, &tb, slots, aoff, n + 1, st));
 	if (n_rec > 0) HIPCHK(launch_grp_scatter((const SeedRec*)c->p_recs.p, n_rec_dev, c->cap_rec, uoff, cursor, intv, gg, st));
 	int64_t n_slots = 0;
+	HIPCHK(hipEventRecord(c->fev[2], st));
 	HIPCHK(hipMemcpyAsync(&n_slots, aoff + n, 8, hipMemcpyDeviceToHost, st));
 	HIPCHK(hipStreamSynchronize(st));
-	HIPCHK(hipEventRecord(c->fev[2], st));
 	// 3. anchors (mb_anchor), 4. chains (mb_lchain_dp) with each read's options
 	TRY(dgrow(f[11], (size_t)n * 8 + 8)); TRY(dgrow(f[12], (size_t)n * 8 + 8)); TRY(dgrow(f[13], (size_t)n_slots * 8 + 16));
 	TRY(dgrow(f[14], (size_t)n_slots * sizeof(AncRec) + 16));
@@ -1099,12 +1187,16 @@ int mbg_map_front(mbg_ctx_t *c, int32_t n, const uint8_t *seq, const int64_t *of
 	out->n_u = (const int32_t*)h[6].p, out->u = (const uint64_t*)h[7].p;
 	out->n_chn = (const int64_t*)h[8].p, out->chn = (const mbg_anchor_t*)h[9].p;
 	float ms;
-	HIPCHK(hipEventElapsedTime(&ms, c->fev[0], c->fev[1])); out->ms_seed = ms;
-	HIPCHK(hipEventElapsedTime(&ms, c->fev[1], c->fev[2])); out->ms_group = ms;
-	HIPCHK(hipEventElapsedTime(&ms, c->fev[2], c->fev[3])); out->ms_anchor = ms;
-	HIPCHK(hipEventElapsedTime(&ms, c->fev[3], c->fev[4])); out->ms_chain = ms;
-	HIPCHK(hipEventElapsedTime(&ms, c->fev[4], c->fev[5])); out->ms_copy = ms;
-	HIPCHK(hipEventElapsedTime(&ms, c->fev[0], c->fev[5])); out->ms_total = ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[6], c->fev[1])); c->cnt.ms_seed1 += ms; out->ms_seed1 = ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[1], c->fev[7])); c->cnt.ms_seed2 += ms; out->ms_seed2 = ms;
+	out->seed1_occ_blocks = work1[0], out->seed1_kmer_lookups = work1[1];
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[0], c->fev[7])); out->ms_seed = ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[7], c->fev[2])); out->ms_group = ms; c->cnt.ms_group += ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[2], c->fev[3])); out->ms_anchor = ms; c->cnt.ms_anchor += ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[3], c->fev[4])); out->ms_chain = ms; c->cnt.ms_chain += ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[4], c->fev[5])); out->ms_copy = ms; c->cnt.ms_front_copy += ms;
+	HIPCHK(hipEventElapsedTime(&ms, c->fev[0], c->fev[5])); out->ms_total = ms; c->cnt.ms_front += ms;
+	++c->cnt.fronts, c->cnt.front_reads += n, c->cnt.front_records += n_rec, c->cnt.front_slots += n_slots;
 	return MBG_OK;
 }
 
@@ -1157,14 +1249,23 @@ int mbg_ll_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_ll_job_t *jobs, const u
 	a.bbuf = (uint64_t*)c->b_out[1].p, a.boff = (const int64_t*)c->b_in[3].p;
 	a.sc = (const int8_t*)c->b_in[3].p + (size_t)n_jobs * 8;
 	a.rows_cap = nb, a.check = false;
+	for (hipEvent_t &e : c->dev_ev) if (!e) HIPCHK(hipEventCreate(&e));
+	HIPCHK(hipEventRecord(c->dev_ev[0], c->stream));
 	for (int k = 1; k <= 16; ++k) {
 		if (cnt[k] == 0) continue;
 		a.order = (const int32_t*)c->b_in[2].p + first[k];
 		a.n = (int32_t)cnt[k];
 		HIPCHK(launch_ll(a, k, c->stream));
 	}
+	HIPCHK(hipEventRecord(c->dev_ev[1], c->stream));
 	HIPCHK(hipMemcpyAsync(res, c->b_out[0].p, (size_t)n_jobs * sizeof(LlRes), hipMemcpyDeviceToHost, c->stream));
 	HIPCHK(hipStreamSynchronize(c->stream));
+	{
+		float ms = 0;
+		HIPCHK(hipEventElapsedTime(&ms, c->dev_ev[0], c->dev_ev[1]));
+		c->cnt.ms_ll += ms, ++c->cnt.ll_launches, c->cnt.ll_jobs += n_jobs;
+		for (int32_t i = 0; i < n_jobs; ++i) c->cnt.ll_cells += (int64_t)jobs[i].qlen * jobs[i].tlen;
+	}
 	return MBG_OK;
 }
 
@@ -1430,6 +1531,23 @@ int mbg_copy_to_host(mbg_ctx_t *c, void *dst, const void *src, size_t bytes)
 	HIPCHK(hipSetDevice(c->device));
 	HIPCHK(hipStreamSynchronize(c->stream));     // ordered after the context stream's queued work
 	HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+	return MBG_OK;
+}
+
+int mbg_counters(mbg_ctx_t *c, mbg_counters_t *out, int reset)
+{
+	if (c == nullptr || out == nullptr) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	*out = c->cnt;
+	if (reset) memset(&c->cnt, 0, sizeof(c->cnt));
+	return MBG_OK;
+}
+
+int mbg_count_work(mbg_ctx_t *c, int on)
+{
+	if (c == nullptr) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	c->count_work = on != 0;
 	return MBG_OK;
 }
 

@@ -408,6 +408,11 @@ __global__ void __launch_bounds__(256) sa_kernel(Idx ix, int64_t n, const int64_
 		if (!have) {
 			if ((int64_t)t >= n) break;
 			idx = t; k = x[t]; steps = 0; have = true;
+			if (k > ix.seq_len) {          // not a row of the index: no walk, the result says so
+				x[idx] = ~0ULL;
+				have = false;
+				continue;
+			}
 		}
 		if ((k & mask) == 0) {
 			x[idx] = steps + samp[k >> bit];

@@ -180,6 +180,7 @@ def lib():
                                          C.POINTER(C.c_int64)]
         L.mbg_seed_batch_dev_async.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, C.c_int64, _P]
         L.mbg_sa_batch_dev.argtypes = [_P, _P, C.c_int64, _P]
+        L.mbg_sa_batch_dev_noalloc.argtypes = [_P, _P, C.c_int64, _P]
         L.mbg_copy_to_host.argtypes = [_P, _P, _P, C.c_size_t]
         L.mbg_copy_to_device.argtypes = [_P, _P, _P, C.c_size_t]
         L.mbg_dev_alloc.argtypes = [_P, C.c_size_t, C.POINTER(_P)]
