@@ -158,6 +158,15 @@ int mbg_init(int device, mbg_ctx_t **out)
 	HIPCHK(hipGetDeviceCount(&n_dev));
 	if (device < 0 || device >= n_dev) return MBG_EARG;
 	HIPCHK(hipSetDevice(device));
+	{   // how host threads wait for the device: MBG_HOST_WAIT=block sleeps in the driver (frees the core
+	    // for other host work), spin / yield busy-wait; unset keeps HIP's automatic choice
+		const char *w = getenv("MBG_HOST_WAIT");
+		unsigned fl = 0;
+		if (w && strcmp(w, "block") == 0) fl = hipDeviceScheduleBlockingSync;
+		else if (w && strcmp(w, "spin") == 0) fl = hipDeviceScheduleSpin;
+		else if (w && strcmp(w, "yield") == 0) fl = hipDeviceScheduleYield;
+		if (fl && hipSetDeviceFlags(fl) != hipSuccess) (void)hipGetLastError();   // refused once the device is active: the default stays
+	}
 	mbg_ctx_t *c = new mbg_ctx_t();
 	c->device = device;
 	hipDeviceProp_t prop;

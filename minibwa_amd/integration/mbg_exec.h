@@ -30,9 +30,9 @@ typedef struct {
 	int8_t mat[25];
 	int32_t w, zdrop, end_bonus, flag;     /* DP */
 	int32_t size, gapo, gape, xtra;        /* local SW: size 1 = u8 core, 2 = i16 core */
-	/* results, filled by the round */
-	mbg_dp_result_t dp;
-	const uint32_t *cigar;                 /* dp.n_cigar operations, valid until the next round */
+	uint32_t *cigar;                       /* DP: room for qlen + tlen operations, the poster's own memory */
+	/* results, filled by the batch */
+	mbg_dp_result_t dp;                    /* its CIGAR (dp.n_cigar operations) is written to cigar */
 	mbg_ll_result_t ll;
 } mbx_req_t;
 

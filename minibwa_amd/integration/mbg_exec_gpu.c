@@ -35,7 +35,7 @@ static long g_count_fronts;      /* MBG_COUNT_WORK=N: count pass-1 work during t
 
 /* staging buffers (page-locked, grow-only) */
 typedef struct { void *p; size_t n; } hbuf_t;
-static hbuf_t b_q, b_t, b_jobs, b_res, b_cig[2], b_ll_jobs, b_ll_res;   /* b_cig: extz2, extd2 (both live in one round) */
+static hbuf_t b_q, b_t, b_jobs, b_res, b_cig, b_ll_jobs, b_ll_res;
 
 static void chk(int rc, const char *what)
 {
@@ -159,41 +159,38 @@ static void dp_group(int dual, int32_t n, mbx_req_t *const *req, const int32_t *
 	q = (uint8_t*)grow(&b_q, qb + 16), t = (uint8_t*)grow(&b_t, tb + 16);
 	jobs = (mbg_dp_job_t*)grow(&b_jobs, (size_t)n * sizeof(mbg_dp_job_t));
 	res = (mbg_dp_result_t*)grow(&b_res, (size_t)n * sizeof(mbg_dp_result_t));
-	cig = (uint32_t*)grow(&b_cig[dual != 0], (size_t)cb * 4 + 16);
-	for (i = 0, qb = tb = cb = 0; i < n; ++i) {          /* offsets first, so the copies can run in parallel */
+	cig = (uint32_t*)grow(&b_cig, (size_t)cb * 4 + 16);
+	/* one thread packs: the mapper's workers keep every core busy meanwhile, and a parallel region
+	 * would wait for all of its threads to be scheduled */
+	for (i = 0, qb = tb = cb = 0; i < n; ++i) {
 		const mbx_req_t *r = req[sel[i]];
 		mbg_dp_job_t *j = &jobs[i];
 		j->qoff = qb, j->toff = tb, j->qlen = r->qlen, j->tlen = r->tlen;
 		j->q = r->gq, j->e = r->ge, j->q2 = r->gq2, j->e2 = r->ge2;
 		j->w = r->w, j->zdrop = r->zdrop, j->end_bonus = r->end_bonus, j->flag = r->flag;
 		j->cigar_off = cb;
+		if (r->qlen) memcpy(q + qb, r->q, r->qlen);
+		if (r->tlen) memcpy(t + tb, r->t, r->tlen);
 		qb += r->qlen, tb += r->tlen, cb += (int64_t)r->qlen + r->tlen;
-	}
-#pragma omp parallel for schedule(static) if (n >= 8192)
-	for (i = 0; i < n; ++i) {
-		const mbx_req_t *r = req[sel[i]];
-		if (r->qlen) memcpy(q + jobs[i].qoff, r->q, r->qlen);
-		if (r->tlen) memcpy(t + jobs[i].toff, r->t, r->tlen);
 	}
 	g_pack_s += now_s() - t0;
 	if (dual) chk(mbg_extd2_batch(g_ctx, n, jobs, q, qb, t, tb, req[sel[0]]->m, req[sel[0]]->mat, res, cig, cb), "mbg_extd2_batch");
 	else chk(mbg_extz2_batch(g_ctx, n, jobs, q, qb, t, tb, req[sel[0]]->m, req[sel[0]]->mat, res, cig, cb), "mbg_extz2_batch");
-	for (i = 0; i < n; ++i) {
+	for (i = 0; i < n; ++i) {               /* each CIGAR into its poster's buffer (room for qlen + tlen) */
 		mbx_req_t *r = req[sel[i]];
 		r->dp = res[i];
-		r->cigar = cig + jobs[i].cigar_off;
+		if (r->dp.n_cigar < 0 || r->dp.n_cigar > r->qlen + r->tlen + 1) {
+			fprintf(stderr, "[mbg_exec] a DP job returned %d CIGAR operations for %d + %d bases\n", r->dp.n_cigar, r->qlen, r->tlen);
+			abort();
+		}
+		if (r->dp.n_cigar) memcpy(r->cigar, cig + jobs[i].cigar_off, (size_t)r->dp.n_cigar * 4);
 	}
 }
 
 /* Requests are grouped by their scoring matrix (and, for local SW, by the shared gap and xtra
- * arguments): a library call takes one of each.  With one group the round's CIGARs stay in the
- * staging buffer of their kind (extd2 or extz2), which only the next round overwrites; with several,
- * each group's CIGARs are copied to a buffer sized for the whole round before the next group reuses
- * the staging buffer. */
+ * arguments): a library call takes one of each. */
 static int32_t *g_sel, *g_grp;
 static int64_t g_sel_m;
-static uint32_t *g_cig_keep[2];
-static int64_t g_cig_keep_m[2];
 
 static void sel_buf(int64_t n)
 {
@@ -208,7 +205,6 @@ static void sel_buf(int64_t n)
 void mbx_run_dp(int dual, int32_t n, mbx_req_t *const *req)
 {
 	int32_t i, g, n_grp = 0, n_sel;
-	int64_t keep = 0;
 	double t0 = now_s();
 	sel_buf(n);
 	for (i = 0; i < n; ++i) g_grp[i] = -1;
@@ -221,27 +217,14 @@ void mbx_run_dp(int dual, int32_t n, mbx_req_t *const *req)
 				g_grp[j] = n_grp;
 		++n_grp;
 	}
-	if (n_grp > 1) {
-		int64_t all = 0;
-		for (i = 0; i < n; ++i) all += (int64_t)req[i]->qlen + req[i]->tlen;
-		if (all > g_cig_keep_m[dual != 0]) {
-			free(g_cig_keep[dual != 0]);
-			g_cig_keep_m[dual != 0] = all + 1024;
-			g_cig_keep[dual != 0] = (uint32_t*)malloc(g_cig_keep_m[dual != 0] * 4);
-			if (g_cig_keep[dual != 0] == 0) { fprintf(stderr, "[mbg_exec] out of memory\n"); abort(); }
-		}
-	}
 	for (g = 0; g < n_grp; ++g) {
-		for (i = 0, n_sel = 0; i < n; ++i)
-			if (g_grp[i] == g) g_sel[n_sel++] = i;
+		if (n_grp == 1) {
+			for (i = 0; i < n; ++i) g_sel[i] = i;
+			n_sel = n;
+		} else
+			for (i = 0, n_sel = 0; i < n; ++i)
+				if (g_grp[i] == g) g_sel[n_sel++] = i;
 		dp_group(dual, n_sel, req, g_sel);
-		if (n_grp > 1)
-			for (i = 0; i < n_sel; ++i) {
-				mbx_req_t *r = req[g_sel[i]];
-				memcpy(g_cig_keep[dual != 0] + keep, r->cigar, (size_t)r->dp.n_cigar * 4);
-				r->cigar = g_cig_keep[dual != 0] + keep;
-				keep += r->dp.n_cigar;
-			}
 	}
 	pthread_mutex_lock(&g_st_mu);
 	++g_st.dp_calls, g_st.dp_jobs += n, g_st.dp_s += now_s() - t0;

@@ -87,10 +87,7 @@ static mbs_pool_t *g_pool;
 static mb_tbuf_t *g_tbuf[MAX_THREADS];
 static const mb_idx_t *g_idx;
 static int g_stats;
-
-/* requests posted by the coroutines of each pool thread during the current round */
-typedef struct { mbx_req_t **a; int64_t n, m; } req_list_t;
-static req_list_t g_posted[MAX_THREADS];
+static int64_t g_flush = 16384;   /* pending requests that start a device batch (MBG_FLUSH) */
 
 static double now_s(void)
 {
@@ -113,13 +110,14 @@ static void report(void)
 	memset(&ss, 0, sizeof(ss));
 	if (g_pool) mbs_get_stats(g_pool, &ss);
 	mbx_get_stats(&xs);
-	fprintf(stderr, "[mbg_map] summary: executor %s fronts %ld reads %ld front_s %.3f | rounds %ld yields %ld peak_coroutines %ld "
-	        "round_s %.3f run_s %.3f phase_s %.3f busy_s %.3f | dp_calls %ld dp_jobs %ld dp_s %.3f dp_pack_s %.3f | ll_calls %ld ll_jobs %ld ll_s %.3f | "
+	fprintf(stderr, "[mbg_map] summary: executor %s fronts %ld reads %ld front_s %.3f | batches %ld waits %ld peak_coroutines %ld "
+	        "batch_s %.3f run_s %.3f busy_s %.3f idle_s %.3f | dp_calls %ld dp_jobs %ld dp_s %.3f dp_pack_s %.3f | ll_calls %ld "
+	        "ll_jobs %ld ll_s %.3f | "
 	        "dev_seed1_ms %.3f dev_seed2_ms %.3f dev_group_ms %.3f dev_anchor_ms %.3f dev_chain_ms %.3f dev_front_copy_ms %.3f "
 	        "dev_front_ms %.3f seed_records %ld anchor_slots %ld seed1_occ_blocks %lu seed1_kmer_lookups %lu dev_dp_launches %ld "
 	        "dev_dp_jobs %ld dev_dp_cells %ld dev_dp_ms %.3f dev_ll_launches %ld dev_ll_cells %ld dev_ll_ms %.3f\n",
-	        mbx_name(), (long)xs.fronts, (long)xs.front_reads, xs.front_s, (long)ss.rounds, (long)ss.yields, (long)ss.coroutines_peak,
-	        ss.round_s, ss.task_s, ss.phase_s, ss.busy_s, (long)xs.dp_calls, (long)xs.dp_jobs, xs.dp_s, xs.dp_pack_s, (long)xs.ll_calls,
+	        mbx_name(), (long)xs.fronts, (long)xs.front_reads, xs.front_s, (long)ss.batches, (long)ss.waits, (long)ss.coroutines_peak,
+	        ss.batch_s, ss.task_s, ss.busy_s, ss.idle_s, (long)xs.dp_calls, (long)xs.dp_jobs, xs.dp_s, xs.dp_pack_s, (long)xs.ll_calls,
 	        (long)xs.ll_jobs, xs.ll_s,
 	        d->ms_seed1, d->ms_seed2, d->ms_group, d->ms_anchor, d->ms_chain, d->ms_front_copy, d->ms_front, (long)d->front_records,
 	        (long)d->front_slots, (unsigned long)d->seed1_occ_blocks, (unsigned long)d->seed1_kmer_lookups, (long)d->dp_launches,
@@ -140,6 +138,8 @@ static mbs_pool_t *pool_for(int n_threads)
 		const char *s = getenv("MBG_MAP_STATS");
 		if (s && atoi(s) && !g_stats) atexit(report);
 		g_stats = s && atoi(s);
+		s = getenv("MBG_FLUSH");
+		if (s && atol(s) > 0) g_flush = atol(s);
 		g_pool = mbs_pool_init(n_threads, 256 * 1024);
 		if (g_pool == 0) die("cannot create the worker pool");
 		for (i = 0; i < n_threads; ++i)
@@ -187,43 +187,29 @@ static __thread tctx_t *tls_tc;
 
 static void post_and_wait(mbx_req_t *q)
 {
-	int tid = mbs_tid();
 	tctx_t *tc = tls_tc;
-	req_list_t *l = &g_posted[tid];
-	if (l->n == l->m) {
-		l->m = l->m ? l->m * 2 : 4096;
-		l->a = (mbx_req_t**)realloc(l->a, l->m * sizeof(mbx_req_t*));
-		if (l->a == 0) die("out of memory (request list)");
-	}
-	l->a[l->n++] = q;
-	mbs_yield();
+	mbs_wait(q);
 	tls_tc = tc;
 }
 
-/* one round: every posted request of every thread, as one batch per kind */
+/* one device batch: every request the dispatcher took, one library call per kind */
 static mbx_req_t **g_round_buf;
 static int64_t g_round_m;
 
-static void round_cb(void *data, int n_threads)
+static void round_cb(void *data, void **reqs, int64_t tot)
 {
-	int64_t tot = 0, n_d2 = 0, n_z2 = 0, n_ll = 0, i;
-	int t;
+	int64_t n_d2 = 0, n_z2 = 0, n_ll = 0, i;
 	(void)data;
-	for (t = 0; t < n_threads; ++t) tot += g_posted[t].n;
 	if (tot * 3 > g_round_m) {
 		g_round_m = tot * 3 + 1024;
 		g_round_buf = (mbx_req_t**)realloc(g_round_buf, g_round_m * sizeof(mbx_req_t*));
 		if (g_round_buf == 0) die("out of memory (round)");
 	}
-	for (t = 0; t < n_threads; ++t) {
-		req_list_t *l = &g_posted[t];
-		for (i = 0; i < l->n; ++i) {
-			mbx_req_t *q = l->a[i];
-			if (q->kind == MBX_D2) g_round_buf[n_d2++] = q;
-			else if (q->kind == MBX_Z2) g_round_buf[tot + n_z2++] = q;
-			else g_round_buf[2 * tot + n_ll++] = q;
-		}
-		l->n = 0;
+	for (i = 0; i < tot; ++i) {
+		mbx_req_t *q = (mbx_req_t*)reqs[i];
+		if (q->kind == MBX_D2) g_round_buf[n_d2++] = q;
+		else if (q->kind == MBX_Z2) g_round_buf[tot + n_z2++] = q;
+		else g_round_buf[2 * tot + n_ll++] = q;
 	}
 	if (n_d2) mbx_run_dp(1, (int32_t)n_d2, g_round_buf);
 	if (n_z2) mbx_run_dp(0, (int32_t)n_z2, g_round_buf + tot);
@@ -347,15 +333,15 @@ static void se_reads(batch_t *b, int n_threads)
 {
 	mbs_pool_t *pool = pool_for(n_threads);
 	double t0 = now_s(), t1;
-	mbs_run(pool, b->n_seq, 64, se_task, round_cb, b);
+	mbs_run(pool, b->n_seq, 64, se_task, round_cb, b, g_flush);
 	t1 = now_s();
 	if (g_stats) {
 		mbs_stats_t st;
 		mbs_get_stats(pool, &st);
-		fprintf(stderr, "[mbg_map] %d reads: front %.3f s (device %.1f ms), SE %.3f s; rounds so far %ld, peak coroutines %ld, "
-		        "compute phases %.3f s, thread busy %.3f s, round callbacks %.3f s (cumulative)\n",
-		        b->n_seq, b->t_front, b->front.dev_ms, t1 - t0, (long)st.rounds, (long)st.coroutines_peak, st.phase_s, st.busy_s,
-		        st.round_s);
+		fprintf(stderr, "[mbg_map] %d reads: front %.3f s (device %.1f ms), SE %.3f s; batches so far %ld, peak coroutines %ld, "
+		        "thread busy %.3f s, idle %.3f s, batch callbacks %.3f s (cumulative)\n",
+		        b->n_seq, b->t_front, b->front.dev_ms, t1 - t0, (long)st.batches, (long)st.coroutines_peak, st.busy_s, st.idle_s,
+		        st.batch_s);
 	}
 }
 
@@ -376,7 +362,7 @@ static void pair_reads(batch_t *b)
 {
 	double t0 = now_s();
 	if (b->n_pair <= 0) return;
-	mbs_run(g_pool, b->n_pair, 64, pe_task, round_cb, b);
+	mbs_run(g_pool, b->n_pair, 64, pe_task, round_cb, b, g_flush);
 	if (g_stats) fprintf(stderr, "[mbg_map] %d pairs paired in %.3f s\n", b->n_pair, now_s() - t0);
 }
 
@@ -642,7 +628,7 @@ static void write_step(pbatch_t *s)
 	s->n_out = (s->n_frag + FMT_FRAGS - 1) / FMT_FRAGS;
 	s->out = (kstring_t*)calloc(s->n_out + 1, sizeof(kstring_t));
 	fc.s = s, fc.opt = p->opt, fc.idx = p->idx;
-	mbs_run(g_fmt_pool, s->n_out, 1, fmt_task, 0, &fc);
+	mbs_run(g_fmt_pool, s->n_out, 1, fmt_task, 0, &fc, 0);
 	t1 = now_s();
 	s->t_fmt = t1 - t0;
 	t0 = t1;
@@ -829,10 +815,18 @@ static void dp_call(int dual, void *km, int qlen, const uint8_t *query, int tlen
 	memcpy(r.mat, mat, (size_t)m * m);
 	r.gq = q, r.ge = e, r.gq2 = q2, r.ge2 = e2;
 	r.w = w, r.zdrop = zdrop, r.end_bonus = end_bonus, r.flag = flag;
+	/* the CIGAR lands in ez's own buffer, grown first to the most a job can write (ksw2.h's allocator,
+	 * as ksw2 itself grows it) */
+	if (ez->m_cigar < r.qlen + r.tlen + 1) {
+		ez->m_cigar = r.qlen + r.tlen + 1;
+		ez->cigar = (uint32_t*)krealloc(km, ez->cigar, (size_t)ez->m_cigar * 4);
+	}
+	r.cigar = ez->cigar;
 	if (solo) {
 		mbx_req_t *p = &r;
 		pthread_mutex_lock(&g_solo_mu);
 		mbx_run_dp(dual, 1, &p);
+		pthread_mutex_unlock(&g_solo_mu);
 	} else post_and_wait(&r);
 	ez->max = (uint32_t)r.dp.max & 0x7fffffff, ez->zdropped = r.dp.zdropped ? 1 : 0;
 	ez->max_q = r.dp.max_q, ez->max_t = r.dp.max_t;
@@ -840,12 +834,6 @@ static void dp_call(int dual, void *km, int qlen, const uint8_t *query, int tlen
 	ez->mte = r.dp.mte, ez->mte_q = r.dp.mte_q;
 	ez->score = r.dp.score, ez->reach_end = r.dp.reach_end;
 	ez->n_cigar = r.dp.n_cigar;
-	if (r.dp.n_cigar > ez->m_cigar) {
-		ez->m_cigar = r.dp.n_cigar;
-		ez->cigar = (uint32_t*)krealloc(km, ez->cigar, (size_t)ez->m_cigar * 4);   /* ksw2.h's allocator, as ksw2 itself */
-	}
-	if (r.dp.n_cigar > 0) memcpy(ez->cigar, r.cigar, (size_t)r.dp.n_cigar * 4);
-	if (solo) pthread_mutex_unlock(&g_solo_mu);
 }
 
 void __wrap_ksw_extd2_sse(void *km, int qlen, const uint8_t *query, int tlen, const uint8_t *target, int8_t m, const int8_t *mat,

@@ -11,12 +11,14 @@
  * writes to -o (default /dev/null), later passes to -O (default /dev/null).  The wall time of each pass
  * is printed to stderr as "[mbg_bench] pass <i> <seconds>", the index load as "[mbg_bench] load <s>".
  *
- * MBG_PROF_OUT=<file>: sample the program counter of whichever thread is on a CPU every millisecond of
- * process CPU time (setitimer ITIMER_PROF) during the passes after the first, and write the samples
- * and /proc/self/maps to <file> at the end, for an offline flat profile of the host side.
+ * MBG_PROF_OUT=<file>: sample the program counter of every thread the mapper creates (and the main
+ * thread) once per millisecond of that thread's CPU time, during the passes after the first, and write
+ * the samples and /proc/self/maps to <file> at the end, for an offline flat profile of the host side.
  */
 #define _GNU_SOURCE
+#include <pthread.h>
 #include <signal.h>
+#include <unistd.h>
 #include <stdatomic.h>
 #include <sys/time.h>
 #include <ucontext.h>
@@ -30,40 +32,75 @@
 
 int32_t mb_map_file(const mb_opt_t *opt, const mb_idx_t *idx, int32_t n, const char **fn, const char *hdr, const char *fn_out);
 
-#define PROF_CAP (1 << 22)
+#define PROF_CAP (1 << 24)
 static uintptr_t *g_prof;
 static atomic_long g_nprof;
+static atomic_int g_prof_on;
+static int g_prof_req;                   /* MBG_PROF_OUT is set: every thread gets a CPU-time timer */
 
 static void prof_handler(int sig, siginfo_t *si, void *uc_)
 {
 	ucontext_t *uc = (ucontext_t*)uc_;
-	long k = atomic_fetch_add(&g_nprof, 1);
+	long k;
 	(void)sig, (void)si;
+	if (!atomic_load(&g_prof_on)) return;
+	k = atomic_fetch_add(&g_nprof, 1);
 	if (k < PROF_CAP) g_prof[k] = (uintptr_t)uc->uc_mcontext.gregs[REG_RIP];
 }
 
-static void prof_start(void)
+/* a timer on the calling thread's own CPU clock, signalling that thread every millisecond it runs */
+static void prof_thread_timer(void)
+{
+	timer_t tm;
+	struct sigevent ev;
+	struct itimerspec it;
+	memset(&ev, 0, sizeof(ev));
+	ev.sigev_notify = SIGEV_THREAD_ID;
+	ev.sigev_signo = SIGPROF;
+	ev._sigev_un._tid = gettid();
+	if (timer_create(CLOCK_THREAD_CPUTIME_ID, &ev, &tm) != 0) return;
+	it.it_interval.tv_sec = 0, it.it_interval.tv_nsec = 1000000;
+	it.it_value = it.it_interval;
+	timer_settime(tm, 0, &it, 0);
+}
+
+typedef struct { void *(*fn)(void*); void *arg; } tramp_t;
+static void *prof_tramp(void *a)
+{
+	tramp_t t = *(tramp_t*)a;
+	free(a);
+	prof_thread_timer();
+	return t.fn(t.arg);
+}
+
+/* threads created by the mapper (kt_pipeline, kt_for, the worker pools) are linked to come here */
+int __real_pthread_create(pthread_t *th, const pthread_attr_t *attr, void *(*fn)(void*), void *arg);
+int __wrap_pthread_create(pthread_t *th, const pthread_attr_t *attr, void *(*fn)(void*), void *arg)
+{
+	tramp_t *t;
+	if (!g_prof_req || (t = (tramp_t*)malloc(sizeof(tramp_t))) == 0) return __real_pthread_create(th, attr, fn, arg);
+	t->fn = fn, t->arg = arg;
+	return __real_pthread_create(th, attr, prof_tramp, t);
+}
+
+static void prof_init(void)
 {
 	struct sigaction sa;
-	struct itimerval it;
+	g_prof_req = 1;
 	g_prof = (uintptr_t*)calloc(PROF_CAP, sizeof(uintptr_t));
 	memset(&sa, 0, sizeof(sa));
 	sa.sa_sigaction = prof_handler;
 	sa.sa_flags = SA_SIGINFO | SA_RESTART;
 	sigaction(SIGPROF, &sa, 0);
-	it.it_interval.tv_sec = 0, it.it_interval.tv_usec = 1000;
-	it.it_value = it.it_interval;
-	setitimer(ITIMER_PROF, &it, 0);
+	prof_thread_timer();
 }
 
 static void prof_stop(const char *fn)
 {
-	struct itimerval it;
 	FILE *fp, *maps;
 	long i, n;
 	char line[4096];
-	memset(&it, 0, sizeof(it));
-	setitimer(ITIMER_PROF, &it, 0);
+	atomic_store(&g_prof_on, 0);
 	n = atomic_load(&g_nprof);
 	if (n > PROF_CAP) n = PROF_CAP;
 	if ((fp = fopen(fn, "w")) == 0) return;
@@ -91,6 +128,7 @@ int main(int argc, char *argv[])
 	kstring_t hdr = {0, 0, 0};
 	double t0;
 	kom_realtime();
+	if (getenv("MBG_PROF_OUT")) prof_init();
 	mb_opt_init(&opt);
 	while ((c = ketopt(&o, argc, argv, 1, "t:P:o:O:", 0)) >= 0) {
 		if (c == 't') opt.n_thread = atoi(o.arg);
@@ -112,14 +150,14 @@ int main(int argc, char *argv[])
 	if (mb_fmt_sam_hdr(&hdr, idx->l2b, 0, MB_VERSION, argc, argv) < 0) return 1;
 	for (i = 0; i < passes; ++i) {
 		double t1 = now_s();
-		if (i == 1 && getenv("MBG_PROF_OUT")) prof_start();
+		if (i == 1 && g_prof_req) atomic_store(&g_prof_on, 1);
 		if (mb_map_file(&opt, idx, argc - (o.ind + 1), (const char**)&argv[o.ind + 1], hdr.s, i == 0 ? out0 : out1) != 0) {
 			fprintf(stderr, "[mbg_bench] mapping failed\n");
 			return 1;
 		}
 		fprintf(stderr, "[mbg_bench] pass %d %.4f\n", i, now_s() - t1);
 	}
-	if (getenv("MBG_PROF_OUT") && g_prof) prof_stop(getenv("MBG_PROF_OUT"));
+	if (g_prof_req) prof_stop(getenv("MBG_PROF_OUT"));
 	free(hdr.s);
 	mb_idx_destroy(idx);
 	return 0;

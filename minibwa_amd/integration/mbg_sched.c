@@ -1,17 +1,28 @@
 /*
- * mbg_sched.c -- coroutine worker pool with batch rounds (interface and rationale in mbg_sched.h).
+ * mbg_sched.c -- coroutine worker pool with an asynchronous batch dispatcher (interface and rationale
+ * in mbg_sched.h).
  *
- * Each pool thread keeps its own suspended coroutines and its own stacks, so a coroutine always
- * resumes on the thread that started it (the reference's per-thread kalloc pools and mimalloc's
- * thread heaps stay consistent).  A round:
- *   1. every thread resumes the coroutines it suspended in the previous round, then starts new tasks
- *      until none are left; each coroutine runs until it yields (a request posted) or finishes;
- *   2. all threads meet at a barrier; one thread runs the round callback over every posted request;
- *   3. all threads meet again; if no coroutine was suspended the run is over.
+ * Each pool thread keeps its own coroutines and stacks, so a coroutine always resumes on the thread
+ * that started it (the reference's per-thread kalloc pools and mimalloc's thread heaps stay
+ * consistent).  A worker loops over: the coroutines whose requests have finished (handed back by the
+ * dispatcher), then a chunk of new tasks; each coroutine runs until it posts a request (mbs_wait) or
+ * finishes.  With nothing to run, a worker sleeps until one of its requests finishes.
+ *
+ * The dispatcher thread starts a batch when `flush` requests are pending, when half the workers are
+ * idle and an eighth of that is pending, or when every worker is idle; it takes every posted request,
+ * runs the batch callback, and hands each coroutine back to its thread.  Requests posted while a batch
+ * is on the device wait for the next one, so host work and device work overlap.
+ *
+ * Termination: each thread counts its tasks claimed and not finished (`live`, raised before a claim).
+ * When every task has been claimed and every thread's live count is zero, no request can be pending,
+ * and the dispatcher ends the run.  The counters the workers touch per task or per request (live, the
+ * posted count) are per thread, each thread's on cache lines of its own: pool-wide atomics there were
+ * contended by every worker at once.
+ *
  * Stacks come from mmap'd chunks (MAP_NORESERVE: only touched pages take memory).  Guard pages would
- * cost two memory mappings per stack, and a round can hold hundreds of thousands of suspended
- * coroutines, beyond the kernel's mapping limit; instead the lowest bytes of each stack hold a canary
- * that is checked whenever its coroutine yields or finishes, and a damaged canary aborts.
+ * cost two memory mappings per stack, and a run can hold hundreds of thousands of suspended coroutines,
+ * beyond the kernel's mapping limit; instead the lowest bytes of each stack hold a canary that is
+ * checked whenever its coroutine suspends or finishes, and a damaged canary aborts.
  */
 #define _GNU_SOURCE
 #include <pthread.h>
@@ -37,39 +48,56 @@ typedef struct co_s {
 	struct co_s *next_free;
 } co_t;
 
+/* a growable array of (request, coroutine) pairs */
+typedef struct { void **req; co_t **co; int64_t n, m; } plist_t;
+
 struct thr_s {
+	_Alignas(128) atomic_llong live;   /* tasks this thread claimed and has not finished */
+	atomic_llong n_posted;    /* posted.n, readable without the lock */
 	mbs_pool_t *pool;
 	int tid;
 	pthread_t th;
 	void *sched_sp;           /* the thread's scheduler context while a coroutine runs */
-	co_t **susp, **susp2;     /* suspended coroutines: current round's and the one being resumed */
-	int64_t n_susp, m_susp, m_susp2;
+	pthread_mutex_t mu;       /* guards ready, posted and waiting */
+	pthread_cond_t cv;
+	co_t **ready;             /* coroutines whose requests have finished */
+	int64_t n_ready, m_ready;
+	co_t **rbuf;              /* the ready list is swapped into this one to be resumed */
+	int64_t m_rbuf;
+	plist_t posted;           /* posted requests not yet taken by the dispatcher */
+	int waiting;
 	co_t *free_co;
 	int64_t n_co;             /* coroutine objects (each with a stack) owned by this thread */
-	int64_t yields;
-	double busy_s;            /* time spent running coroutines */
+	int64_t waits;
+	double busy_s, idle_s;
+	char pad_[128];           /* keeps the next thread's counters off this thread's last line */
 };
 
 struct mbs_pool_s {
 	int n;
 	int64_t stack_bytes;
 	thr_t *thr;
-	pthread_barrier_t bar;
 	pthread_mutex_t mu, run_mu;
 	pthread_cond_t cv_start, cv_done;
 	int64_t gen;
 	int n_finished, stop;
+	pthread_t disp;
 	/* the current run */
-	int64_t n_tasks, chunk;
+	int64_t n_tasks, chunk, flush;
 	atomic_llong next;
 	mbs_task_f task;
-	mbs_round_f round;
+	mbs_batch_f batch;
 	void *data;
-	atomic_llong live;
-	int finish;
+	int64_t flush_thr;        /* a thread's posted count that pokes the dispatcher (flush / n) */
+	atomic_int n_idle;
+	atomic_int all_done;
+	pthread_mutex_t dmu;      /* dispatcher wake-ups */
+	pthread_cond_t dcv;
+	plist_t batch_l;          /* the dispatcher's batch */
+	int64_t *toff;            /* per thread: where its requests start in the batch */
 	/* statistics */
-	int64_t rounds, co_peak;
-	double round_s, task_s, phase_s, phase_t0;
+	int64_t batches, co_peak;
+	double batch_s, task_s;
 };
 
 static __thread thr_t *tls_thr;
@@ -112,6 +140,24 @@ static void die(const char *msg)
 	abort();
 }
 
+static void plist_reserve(plist_t *l, int64_t need)
+{
+	if (need <= l->m) return;
+	l->m = need + (need >> 1) + 1024;
+	l->req = (void**)realloc(l->req, (size_t)l->m * sizeof(void*));
+	l->co = (co_t**)realloc(l->co, (size_t)l->m * sizeof(co_t*));
+	if (l->req == 0 || l->co == 0) die("out of memory (request list)");
+}
+
+#define STACKS_PER_CHUNK 64
+#define CANARY_WORDS 8
+static const uint64_t kCanary = 0x6d62675f73746b21ULL;
+
+static void check_canary(const co_t *co)
+{
+	if (co->canary[0] != kCanary || co->canary[CANARY_WORDS - 1] != kCanary) die("coroutine stack overflow (canary overwritten)");
+}
+
 void mbs_co_main(co_t *co);
 void mbs_co_main(co_t *co)
 {
@@ -121,27 +167,6 @@ void mbs_co_main(co_t *co)
 	co->done = 1;
 	mbs_switch(&co->sp, t->sched_sp);
 	die("finished coroutine resumed");
-}
-
-static void push_susp(thr_t *t, co_t *co)
-{
-	if (t->n_susp == t->m_susp) {
-		t->m_susp = t->m_susp ? t->m_susp * 2 : 1024;
-		t->susp = (co_t**)realloc(t->susp, t->m_susp * sizeof(co_t*));
-		if (t->susp == 0) die("out of memory (suspended list)");
-	}
-	t->susp[t->n_susp++] = co;
-}
-
-#define STACKS_PER_CHUNK 64
-#define CANARY_WORDS 8
-static const uint64_t kCanary = 0x6d62675f73746b21ULL;
-
-static void check_canary(const co_t *co)
-{
-	int k;
-	for (k = 0; k < CANARY_WORDS; ++k)
-		if (co->canary[k] != kCanary) die("coroutine stack overflow (canary overwritten)");
 }
 
 static co_t *get_co(thr_t *t)
@@ -171,10 +196,37 @@ static co_t *get_co(thr_t *t)
 	}
 }
 
-static void put_co(thr_t *t, co_t *co)
+static void poke(mbs_pool_t *p)
 {
+	pthread_mutex_lock(&p->dmu);
+	pthread_cond_signal(&p->dcv);
+	pthread_mutex_unlock(&p->dmu);
+}
+
+static int64_t live_sum(mbs_pool_t *p)
+{
+	int64_t s = 0;
+	int k;
+	for (k = 0; k < p->n; ++k) s += atomic_load(&p->thr[k].live);
+	return s;
+}
+
+static int64_t posted_sum(mbs_pool_t *p)
+{
+	int64_t s = 0;
+	int k;
+	for (k = 0; k < p->n; ++k) s += atomic_load_explicit(&p->thr[k].n_posted, memory_order_relaxed);
+	return s;
+}
+
+static void finish(thr_t *t, co_t *co)
+{
+	mbs_pool_t *p = t->pool;
+	check_canary(co);
 	co->next_free = t->free_co;
 	t->free_co = co;
+	/* this thread's last task while nothing is left to claim: possibly the run's last */
+	if (atomic_fetch_sub(&t->live, 1) == 1 && atomic_load(&p->next) >= p->n_tasks) poke(p);
 }
 
 static void resume(thr_t *t, co_t *co)
@@ -182,7 +234,7 @@ static void resume(thr_t *t, co_t *co)
 	tls_co = co;
 	mbs_switch(&t->sched_sp, co->sp);
 	tls_co = 0;
-	check_canary(co);
+	if (co->done) finish(t, co);
 }
 
 static void start(thr_t *t, co_t *co, int64_t task)
@@ -204,54 +256,140 @@ static void run_job(thr_t *t)
 {
 	mbs_pool_t *p = t->pool;
 	for (;;) {
-		int64_t i, n_old, i0;
-		co_t **old;
-		double tb = now_s();
-		/* 1. resume what the previous round suspended */
-		old = t->susp, n_old = t->n_susp;
-		t->susp = t->susp2, t->susp2 = old;
-		{ int64_t m = t->m_susp; t->m_susp = t->m_susp2; t->m_susp2 = m; }
-		t->n_susp = 0;
-		for (i = 0; i < n_old; ++i) {
-			co_t *co = old[i];
-			resume(t, co);
-			if (co->done) put_co(t, co);
-			else push_susp(t, co);
+		double t0 = now_s();
+		int64_t i, n, i0;
+		int progressed;
+		/* 1. coroutines whose requests have finished */
+		pthread_mutex_lock(&t->mu);
+		n = t->n_ready;
+		if (n) {
+			co_t **r = t->ready;
+			int64_t m = t->m_ready;
+			t->ready = t->rbuf, t->m_ready = t->m_rbuf;
+			t->rbuf = r, t->m_rbuf = m;
+			t->n_ready = 0;
 		}
-		/* 2. start new tasks */
-		while ((i0 = atomic_fetch_add(&p->next, p->chunk)) < p->n_tasks) {
-			int64_t e = i0 + p->chunk < p->n_tasks ? i0 + p->chunk : p->n_tasks;
-			for (i = i0; i < e; ++i) {
-				co_t *co = get_co(t);
-				start(t, co, i);
-				if (co->done) put_co(t, co);
-				else push_susp(t, co);
-			}
+		pthread_mutex_unlock(&t->mu);
+		for (i = 0; i < n; ++i) resume(t, t->rbuf[i]);
+		progressed = n > 0;
+		/* 2. a chunk of new tasks; live is raised before the claim, so that the dispatcher never sees
+		 * every task claimed and none live while a claimed chunk has not started */
+		if (atomic_load(&p->next) < p->n_tasks) {
+			atomic_fetch_add(&t->live, p->chunk);
+			i0 = atomic_fetch_add(&p->next, p->chunk);
+			if (i0 < p->n_tasks) {
+				int64_t e = i0 + p->chunk < p->n_tasks ? i0 + p->chunk : p->n_tasks;
+				if (e - i0 < p->chunk) atomic_fetch_sub(&t->live, p->chunk - (e - i0));
+				for (i = i0; i < e; ++i) start(t, get_co(t), i);
+				progressed = 1;
+			} else if (atomic_fetch_sub(&t->live, p->chunk) == p->chunk) poke(p);
 		}
-		atomic_fetch_add(&p->live, t->n_susp);
-		t->busy_s += now_s() - tb;
-		/* 3. the round */
-		if (pthread_barrier_wait(&p->bar) == PTHREAD_BARRIER_SERIAL_THREAD) {
-			int64_t live = atomic_load(&p->live);
-			p->phase_s += now_s() - p->phase_t0;
-			int64_t n_co = 0;
-			int k;
-			for (k = 0; k < p->n; ++k) n_co += p->thr[k].n_susp;
-			if (n_co > p->co_peak) p->co_peak = n_co;
-			if (live > 0) {
-				double t0 = now_s();
-				if (p->round == 0) die("a task yielded but the run has no round callback");
-				p->round(p->data, p->n);
-				p->round_s += now_s() - t0;
-				++p->rounds;
-			}
-			p->finish = live == 0;
-			atomic_store(&p->live, 0);
-			p->phase_t0 = now_s();
+		t->busy_s += now_s() - t0;
+		if (progressed) continue;
+		/* 3. nothing to run: sleep until a request of this thread finishes, or the run ends */
+		if (atomic_load(&p->all_done)) break;
+		t0 = now_s();
+		pthread_mutex_lock(&t->mu);
+		if (t->n_ready == 0 && !atomic_load(&p->all_done)) {
+			t->waiting = 1;
+			atomic_fetch_add(&p->n_idle, 1);
+			pthread_mutex_unlock(&t->mu);
+			poke(p);
+			pthread_mutex_lock(&t->mu);
+			while (t->n_ready == 0 && !atomic_load(&p->all_done)) pthread_cond_wait(&t->cv, &t->mu);
+			atomic_fetch_sub(&p->n_idle, 1);
+			t->waiting = 0;
 		}
-		pthread_barrier_wait(&p->bar);
-		if (p->finish) break;
+		pthread_mutex_unlock(&t->mu);
+		t->idle_s += now_s() - t0;
 	}
+}
+
+/* the dispatcher's part of a run: batches until every task has finished */
+static void dispatch_job(mbs_pool_t *p)
+{
+	plist_t *b = &p->batch_l;
+	for (;;) {
+		int done = 0, k;
+		int64_t nb = 0, i;
+		pthread_mutex_lock(&p->dmu);
+		for (;;) {
+			int64_t pend = posted_sum(p);
+			int idle = atomic_load(&p->n_idle);
+			if (atomic_load(&p->next) >= p->n_tasks && live_sum(p) == 0) { done = 1; break; }
+			if (pend > 0 && (pend >= p->flush || idle == p->n || (idle * 2 >= p->n && pend * 8 >= p->flush))) break;
+			{
+				struct timespec ts;
+				clock_gettime(CLOCK_REALTIME, &ts);
+				ts.tv_nsec += 500000;
+				if (ts.tv_nsec >= 1000000000L) ts.tv_sec += 1, ts.tv_nsec -= 1000000000L;
+				pthread_cond_timedwait(&p->dcv, &p->dmu, &ts);
+			}
+		}
+		pthread_mutex_unlock(&p->dmu);
+		if (done) {
+			atomic_store(&p->all_done, 1);
+			for (k = 0; k < p->n; ++k) {
+				pthread_mutex_lock(&p->thr[k].mu);
+				pthread_cond_signal(&p->thr[k].cv);
+				pthread_mutex_unlock(&p->thr[k].mu);
+			}
+			return;
+		}
+		/* take every posted request */
+		for (k = 0; k < p->n; ++k) {
+			thr_t *t = &p->thr[k];
+			pthread_mutex_lock(&t->mu);
+			plist_reserve(b, nb + t->posted.n);
+			p->toff[k] = nb;
+			memcpy(b->req + nb, t->posted.req, (size_t)t->posted.n * sizeof(void*));
+			memcpy(b->co + nb, t->posted.co, (size_t)t->posted.n * sizeof(co_t*));
+			nb += t->posted.n;
+			t->posted.n = 0;
+			atomic_store_explicit(&t->n_posted, 0, memory_order_relaxed);
+			pthread_mutex_unlock(&t->mu);
+		}
+		p->toff[p->n] = nb;
+		if (nb == 0) continue;
+		{
+			int64_t live = live_sum(p);
+			double t0 = now_s();
+			if (live > p->co_peak) p->co_peak = live;
+			p->batch(p->data, b->req, nb);
+			p->batch_s += now_s() - t0;
+			++p->batches;
+		}
+		/* hand each coroutine back to its thread */
+		for (k = 0; k < p->n; ++k) {
+			thr_t *t = &p->thr[k];
+			int64_t a = p->toff[k], e = p->toff[k + 1];
+			if (a == e) continue;
+			pthread_mutex_lock(&t->mu);
+			if (t->n_ready + (e - a) > t->m_ready) {
+				t->m_ready = t->n_ready + (e - a) + 1024;
+				t->ready = (co_t**)realloc(t->ready, (size_t)t->m_ready * sizeof(co_t*));
+				if (t->ready == 0) die("out of memory (ready list)");
+			}
+			for (i = a; i < e; ++i) t->ready[t->n_ready++] = b->co[i];
+			if (t->waiting) pthread_cond_signal(&t->cv);
+			pthread_mutex_unlock(&t->mu);
+		}
+	}
+}
+
+static void wait_job(mbs_pool_t *p, int64_t *my_gen)
+{
+	pthread_mutex_lock(&p->mu);
+	while (p->gen == *my_gen && !p->stop) pthread_cond_wait(&p->cv_start, &p->mu);
+	*my_gen = p->gen;
+	pthread_mutex_unlock(&p->mu);
+}
+
+static void job_finished(mbs_pool_t *p)
+{
+	pthread_mutex_lock(&p->mu);
+	if (++p->n_finished == p->n + 1) pthread_cond_signal(&p->cv_done);
+	pthread_mutex_unlock(&p->mu);
 }
 
 static void *worker(void *arg)
@@ -261,15 +399,23 @@ static void *worker(void *arg)
 	int64_t my_gen = 0;
 	tls_thr = t;
 	for (;;) {
-		pthread_mutex_lock(&p->mu);
-		while (p->gen == my_gen && !p->stop) pthread_cond_wait(&p->cv_start, &p->mu);
-		if (p->stop) { pthread_mutex_unlock(&p->mu); break; }
-		my_gen = p->gen;
-		pthread_mutex_unlock(&p->mu);
+		wait_job(p, &my_gen);
+		if (p->stop) break;
 		run_job(t);
-		pthread_mutex_lock(&p->mu);
-		if (++p->n_finished == p->n) pthread_cond_signal(&p->cv_done);
-		pthread_mutex_unlock(&p->mu);
+		job_finished(p);
+	}
+	return 0;
+}
+
+static void *dispatcher(void *arg)
+{
+	mbs_pool_t *p = (mbs_pool_t*)arg;
+	int64_t my_gen = 0;
+	for (;;) {
+		wait_job(p, &my_gen);
+		if (p->stop) break;
+		dispatch_job(p);
+		job_finished(p);
 	}
 	return 0;
 }
@@ -286,16 +432,24 @@ mbs_pool_t *mbs_pool_init(int n_threads, int64_t stack_bytes)
 	if (p == 0) return 0;
 	p->n = n_threads;
 	p->stack_bytes = stack_bytes;
-	p->thr = (thr_t*)calloc(n_threads, sizeof(thr_t));
-	pthread_barrier_init(&p->bar, 0, n_threads);
+	p->thr = (thr_t*)aligned_alloc(128, (size_t)n_threads * sizeof(thr_t));
+	if (p->thr) memset(p->thr, 0, (size_t)n_threads * sizeof(thr_t));
+	p->toff = (int64_t*)calloc(n_threads + 1, sizeof(int64_t));
+	if (p->thr == 0 || p->toff == 0) die("out of memory (pool)");
 	pthread_mutex_init(&p->mu, 0);
 	pthread_mutex_init(&p->run_mu, 0);
+	pthread_mutex_init(&p->dmu, 0);
 	pthread_cond_init(&p->cv_start, 0);
 	pthread_cond_init(&p->cv_done, 0);
+	pthread_cond_init(&p->dcv, 0);
 	for (i = 0; i < n_threads; ++i) {
-		p->thr[i].pool = p, p->thr[i].tid = i;
-		if (pthread_create(&p->thr[i].th, 0, worker, &p->thr[i]) != 0) die("pthread_create failed");
+		thr_t *t = &p->thr[i];
+		t->pool = p, t->tid = i;
+		pthread_mutex_init(&t->mu, 0);
+		pthread_cond_init(&t->cv, 0);
+		if (pthread_create(&t->th, 0, worker, t) != 0) die("pthread_create failed");
 	}
+	if (pthread_create(&p->disp, 0, dispatcher, p) != 0) die("pthread_create failed");
 	return p;
 }
 
@@ -308,46 +462,63 @@ void mbs_pool_destroy(mbs_pool_t *p)
 	pthread_cond_broadcast(&p->cv_start);
 	pthread_mutex_unlock(&p->mu);
 	for (i = 0; i < p->n; ++i) pthread_join(p->thr[i].th, 0);
+	pthread_join(p->disp, 0);
 	for (i = 0; i < p->n; ++i) {
-		free(p->thr[i].susp);
-		free(p->thr[i].susp2);
+		thr_t *t = &p->thr[i];
+		free(t->ready); free(t->rbuf); free(t->posted.req); free(t->posted.co);
 		/* stacks and coroutine objects stay mapped: the pool lives for the process */
 	}
-	pthread_barrier_destroy(&p->bar);
+	free(p->batch_l.req); free(p->batch_l.co); free(p->toff);
 	free(p->thr);
 	free(p);
 }
 
 int mbs_pool_size(const mbs_pool_t *p) { return p->n; }
 
-void mbs_run(mbs_pool_t *p, int64_t n, int64_t chunk, mbs_task_f task, mbs_round_f round, void *data)
+void mbs_run(mbs_pool_t *p, int64_t n, int64_t chunk, mbs_task_f task, mbs_batch_f batch, void *data, int64_t flush)
 {
 	double t0;
 	if (n <= 0) return;
 	pthread_mutex_lock(&p->run_mu);
 	t0 = now_s();
 	pthread_mutex_lock(&p->mu);
-	p->n_tasks = n, p->chunk = chunk > 0 ? chunk : 1;
+	p->n_tasks = n, p->chunk = chunk > 0 ? chunk : 1, p->flush = flush > 0 ? flush : 16384;
+	p->flush_thr = p->flush / p->n > 0 ? p->flush / p->n : 1;
 	atomic_store(&p->next, 0);
-	atomic_store(&p->live, 0);
-	p->task = task, p->round = round, p->data = data;
-	p->finish = 0;
+	{
+		int k;
+		for (k = 0; k < p->n; ++k) atomic_store(&p->thr[k].live, 0), atomic_store(&p->thr[k].n_posted, 0);
+	}
+	atomic_store(&p->n_idle, 0);
+	atomic_store(&p->all_done, 0);
+	p->task = task, p->batch = batch, p->data = data;
 	p->n_finished = 0;
-	p->phase_t0 = now_s();
 	++p->gen;
 	pthread_cond_broadcast(&p->cv_start);
-	while (p->n_finished < p->n) pthread_cond_wait(&p->cv_done, &p->mu);
+	while (p->n_finished < p->n + 1) pthread_cond_wait(&p->cv_done, &p->mu);
 	pthread_mutex_unlock(&p->mu);
 	p->task_s += now_s() - t0;
 	pthread_mutex_unlock(&p->run_mu);
 }
 
-void mbs_yield(void)
+void mbs_wait(void *req)
 {
 	co_t *co = tls_co;
 	thr_t *t = tls_thr;
-	if (co == 0 || t == 0) die("mbs_yield outside a task");
-	++t->yields;
+	mbs_pool_t *p;
+	int64_t np;
+	if (co == 0 || t == 0) die("mbs_wait outside a task");
+	p = t->pool;
+	if (p->batch == 0) die("a task waits but the run has no batch callback");
+	pthread_mutex_lock(&t->mu);
+	plist_reserve(&t->posted, t->posted.n + 1);
+	t->posted.req[t->posted.n] = req, t->posted.co[t->posted.n] = co;
+	np = ++t->posted.n;
+	atomic_store_explicit(&t->n_posted, np, memory_order_relaxed);
+	pthread_mutex_unlock(&t->mu);
+	if (np == p->flush_thr) poke(p);   /* this thread's share of a full batch */
+	++t->waits;
+	check_canary(co);
 	mbs_switch(&co->sp, t->sched_sp);
 	tls_co = co;
 }
@@ -359,7 +530,8 @@ void mbs_get_stats(const mbs_pool_t *p, mbs_stats_t *st)
 {
 	int i;
 	memset(st, 0, sizeof(*st));
-	st->rounds = p->rounds, st->coroutines_peak = p->co_peak;
-	st->round_s = p->round_s, st->task_s = p->task_s, st->phase_s = p->phase_s;
-	for (i = 0; i < p->n; ++i) st->yields += p->thr[i].yields, st->stacks += p->thr[i].n_co, st->busy_s += p->thr[i].busy_s;
+	st->batches = p->batches, st->coroutines_peak = p->co_peak;
+	st->batch_s = p->batch_s, st->task_s = p->task_s;
+	for (i = 0; i < p->n; ++i)
+		st->waits += p->thr[i].waits, st->stacks += p->thr[i].n_co, st->busy_s += p->thr[i].busy_s, st->idle_s += p->thr[i].idle_s;
 }

@@ -47,9 +47,6 @@ typedef struct {
 	uint64_t *u;
 } fres_t;
 static fres_t g_fr[MBX_FRONT_SLOTS];
-/* the round's CIGARs, one buffer per kind (extz2, extd2): a round may hold both */
-static uint32_t *r_cig[2];
-static int64_t r_cig_m[2];
 
 const char *mbx_name(void) { return "cpu-reference"; }
 
@@ -125,12 +122,9 @@ void mbx_front(const mbx_front_in_t *in, mbx_front_t *out)
 void mbx_run_dp(int dual, int32_t n, mbx_req_t *const *req)
 {
 	int32_t i;
-	int64_t tot = 0;
 	ksw_extz_t ez;
 	memset(&ez, 0, sizeof(ez));
-	for (i = 0; i < n; ++i) tot += (int64_t)req[i]->qlen + req[i]->tlen + 1;
-	if (tot > r_cig_m[dual != 0]) r_cig_m[dual != 0] = tot, r_cig[dual != 0] = (uint32_t*)xrealloc(r_cig[dual != 0], tot * 4);
-	for (i = 0, tot = 0; i < n; ++i) {
+	for (i = 0; i < n; ++i) {
 		mbx_req_t *q = req[i];
 		if (dual) __real_ksw_extd2_sse(0, q->qlen, q->q, q->tlen, q->t, q->m, q->mat, q->gq, q->ge, q->gq2, q->ge2, q->w, q->zdrop,
 		                               q->end_bonus, q->flag, &ez);
@@ -138,12 +132,13 @@ void mbx_run_dp(int dual, int32_t n, mbx_req_t *const *req)
 		q->dp.max = ez.max, q->dp.zdropped = ez.zdropped, q->dp.max_q = ez.max_q, q->dp.max_t = ez.max_t;
 		q->dp.mqe = ez.mqe, q->dp.mqe_t = ez.mqe_t, q->dp.mte = ez.mte, q->dp.mte_q = ez.mte_q;
 		q->dp.score = ez.score, q->dp.reach_end = ez.reach_end, q->dp.n_cigar = ez.n_cigar, q->dp.cells = 0;
-		memcpy(r_cig[dual != 0] + tot, ez.cigar, (size_t)ez.n_cigar * 4);
-		q->cigar = r_cig[dual != 0] + tot;
-		tot += (int64_t)q->qlen + q->tlen + 1;
+		if (ez.n_cigar > q->qlen + q->tlen + 1) { fprintf(stderr, "[map_exec_cpu] CIGAR longer than its room\n"); abort(); }
+		memcpy(q->cigar, ez.cigar, (size_t)ez.n_cigar * 4);
 	}
 	kfree(0, ez.cigar);
+	pthread_mutex_lock(&g_st_mu);
 	++g_st.dp_calls, g_st.dp_jobs += n;
+	pthread_mutex_unlock(&g_st_mu);
 }
 
 void mbx_run_ll(int32_t n, mbx_req_t *const *req)
@@ -157,7 +152,9 @@ void mbx_run_ll(int32_t n, mbx_req_t *const *req)
 		kfree(0, qp);
 		q->ll.score = r.score, q->ll.te = r.te, q->ll.qe = r.qe, q->ll.score2 = r.score2, q->ll.te2 = r.te2;
 	}
+	pthread_mutex_lock(&g_st_mu);
 	++g_st.ll_calls, g_st.ll_jobs += n;
+	pthread_mutex_unlock(&g_st_mu);
 }
 
 void mbx_sa(int64_t n, uint64_t *x) { __real_mb_bwt_sa_batch(0, g_idx->bwt, n, x); }

@@ -89,8 +89,9 @@ def check_sets(cli, api, name, tmp_path_factory, threads=4):
         assert _body_md5(out) == gold["repeat.long.sam.body.md5"]
     else:
         raise KeyError(name)
-    # every read went through one front pass; the DP calls went in rounds
+    # every read went through one front pass; the DP calls went in device batches
     assert int(st["fronts"]) >= 1 and int(st["reads"]) > 0
     if name not in ("batch_chrM", "batch_cfg0"):
-        assert int(st["rounds"]) >= 1 and int(st["dp_jobs"]) >= int(st["dp_calls"])
+        assert int(st["batches"]) >= 1 and int(st["waits"]) >= int(st["batches"])
+        assert int(st["dp_jobs"]) >= int(st["dp_calls"])
     return st
