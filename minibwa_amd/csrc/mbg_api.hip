@@ -37,6 +37,7 @@ struct mbg_ctx_s {
 	Sai *d_pre = nullptr;
 	Idx ix{};
 	uint64_t *d_pac = nullptr, *d_ambi = nullptr, *d_ctg_off = nullptr, *d_ctg_len = nullptr;
+	std::vector<uint64_t> h_ctg_len;   // host copy of the contig lengths (argument checks)
 	L2b lb{};
 	uint64_t idx_bytes = 0;
 	// scratch pools
@@ -314,6 +315,7 @@ static int l2b_upload_locked(mbg_ctx_t *c, const mbg_l2b_desc_t *d)
 	L2b &lb = c->lb;
 	lb.pac = c->d_pac; lb.ambi = c->d_ambi; lb.ctg_off = c->d_ctg_off; lb.ctg_len = c->d_ctg_len;
 	lb.n_ctg = d->n_ctg; lb.n_ambi = d->n_ambi; lb.tot_len = d->tot_len;
+	c->h_ctg_len.assign(d->ctg_len, d->ctg_len + d->n_ctg);
 	c->idx_bytes += d->n_pac * 8 + d->n_ambi * 16 + d->n_ctg * 16;
 	c->has_l2b = true;
 	return MBG_OK;
@@ -393,11 +395,12 @@ int mbg_idx_attach(mbg_ctx_t *c, mbg_ctx_t *src)
 	Idx ix;
 	L2b lb;
 	bool has_bwt, has_l2b;
+	std::vector<uint64_t> clen;
 	{
 		std::lock_guard<std::mutex> ls(src->mu);
 		if (!src->has_bwt) return MBG_ENOIDX;
 		if (src->device != c->device) return MBG_EARG;
-		ix = src->ix, lb = src->lb, has_bwt = src->has_bwt, has_l2b = src->has_l2b;
+		ix = src->ix, lb = src->lb, has_bwt = src->has_bwt, has_l2b = src->has_l2b, clen = src->h_ctg_len;
 	}
 	HIPCHK(hipSetDevice(c->device));
 	HIPCHK(hipStreamSynchronize(c->stream));
@@ -405,6 +408,7 @@ int mbg_idx_attach(mbg_ctx_t *c, mbg_ctx_t *src)
 	free_l2b(c);
 	// the device arrays stay owned by src (c->d_* remain null, so c never frees them)
 	c->ix = ix, c->lb = lb, c->has_bwt = has_bwt, c->has_l2b = has_l2b;
+	c->h_ctg_len.swap(clen);
 	c->idx_bytes = 0;
 	return MBG_OK;
 }
@@ -1476,13 +1480,11 @@ int mbg_getseq(mbg_ctx_t *c, int64_t n, const int64_t *tid, const int64_t *st, c
 	if (!c->has_l2b) return MBG_ENOIDX;
 	if (n == 0) return MBG_OK;
 	HIPCHK(hipSetDevice(c->device));
-	for (int64_t i = 0; i < n; ++i) {
+	for (int64_t i = 0; i < n; ++i) {   // the clipped window must fit (the kernel clips the same way)
 		if (tid[i] < 0 || (uint64_t)tid[i] >= c->lb.n_ctg) continue;
-		int64_t s = st[i] < 0 ? 0 : st[i], e = en[i];
-		if (e - s > 0 && (out_off[i] < 0 || out_off[i] + (e - s) > out_bytes)) {
-			// clipped length may be shorter; the caller must size for the unclipped window
-			return MBG_EARG;
-		}
+		const uint64_t clen = c->h_ctg_len[tid[i]];
+		int64_t s = st[i] < 0 ? 0 : st[i], e = (uint64_t)en[i] > clen ? (int64_t)clen : en[i];
+		if (e - s > 0 && (out_off[i] < 0 || out_off[i] + (e - s) > out_bytes)) return MBG_EARG;
 	}
 	TRY(dgrow(c->b_in[0], n * 8)); TRY(dgrow(c->b_in[1], n * 8)); TRY(dgrow(c->b_in[2], n * 8)); TRY(dgrow(c->b_in[3], n * 8));
 	TRY(dgrow(c->b_out[0], out_bytes + 16)); TRY(dgrow(c->b_out[1], n * 8));

@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(64) getseq_kernel(L2b lb, int64_t n, const int
 	const uint64_t coff = lb.ctg_off[t], clen = lb.ctg_len[t];
 	int64_t st = st_[w], en = en_[w];
 	if (st < 0) st = 0;
-	if (en > (int64_t)clen) en = (int64_t)clen;
+	if ((uint64_t)en > clen) en = (int64_t)clen;   // unsigned, as l2bit.c:61 compares: a negative en clips to the end
 	uint8_t *o = out + out_off[w];
 	const int64_t gst = (int64_t)coff + st, gen = (int64_t)coff + en;
 	for (int64_t i = gst + lane; i < gen; i += 64)

@@ -63,6 +63,7 @@
 #include "ksw2.h"
 #include "mbg.h"
 #include "mbg_exec.h"
+#include "mbg_fmt.h"
 #include "mbg_sched.h"
 
 /* ---- the reference's pipeline record (map-main.c:16-24) ----------------------------------------- */
@@ -87,7 +88,9 @@ static mbs_pool_t *g_pool;
 static mb_tbuf_t *g_tbuf[MAX_THREADS];
 static const mb_idx_t *g_idx;
 static int g_stats;
-static int64_t g_flush = 16384;   /* pending requests that start a device batch (MBG_FLUSH) */
+static int64_t g_flush = 4096;    /* pending requests that start a device batch (MBG_FLUSH) */
+static int g_fmt_native;          /* output records by mbg_fmt_record (else the reference's mb_format) */
+static int g_fmt_check;           /* MBG_FMT_CHECK=1: every record by both, compared */
 
 static double now_s(void)
 {
@@ -110,13 +113,13 @@ static void report(void)
 	memset(&ss, 0, sizeof(ss));
 	if (g_pool) mbs_get_stats(g_pool, &ss);
 	mbx_get_stats(&xs);
-	fprintf(stderr, "[mbg_map] summary: executor %s fronts %ld reads %ld front_s %.3f | batches %ld waits %ld peak_coroutines %ld "
+	fprintf(stderr, "[mbg_map] summary: executor %s formatter %s fronts %ld reads %ld front_s %.3f | batches %ld waits %ld peak_coroutines %ld "
 	        "batch_s %.3f run_s %.3f busy_s %.3f idle_s %.3f | dp_calls %ld dp_jobs %ld dp_s %.3f dp_pack_s %.3f | ll_calls %ld "
 	        "ll_jobs %ld ll_s %.3f | "
 	        "dev_seed1_ms %.3f dev_seed2_ms %.3f dev_group_ms %.3f dev_anchor_ms %.3f dev_chain_ms %.3f dev_front_copy_ms %.3f "
 	        "dev_front_ms %.3f seed_records %ld anchor_slots %ld seed1_occ_blocks %lu seed1_kmer_lookups %lu dev_dp_launches %ld "
 	        "dev_dp_jobs %ld dev_dp_cells %ld dev_dp_ms %.3f dev_ll_launches %ld dev_ll_cells %ld dev_ll_ms %.3f\n",
-	        mbx_name(), (long)xs.fronts, (long)xs.front_reads, xs.front_s, (long)ss.batches, (long)ss.waits, (long)ss.coroutines_peak,
+	        mbx_name(), g_fmt_native ? "native" : "reference", (long)xs.fronts, (long)xs.front_reads, xs.front_s, (long)ss.batches, (long)ss.waits, (long)ss.coroutines_peak,
 	        ss.batch_s, ss.task_s, ss.busy_s, ss.idle_s, (long)xs.dp_calls, (long)xs.dp_jobs, xs.dp_s, xs.dp_pack_s, (long)xs.ll_calls,
 	        (long)xs.ll_jobs, xs.ll_s,
 	        d->ms_seed1, d->ms_seed2, d->ms_group, d->ms_anchor, d->ms_chain, d->ms_front_copy, d->ms_front, (long)d->front_records,
@@ -400,6 +403,26 @@ static struct {
 } g_pipe;
 static mbs_pool_t *g_fmt_pool;    /* formatting in the output step, beside the mapping pool */
 
+/* one record into out, as mb_format (format.c:355-361) writes it */
+static void fmt_one(void *km, kstring_t *out, const l2b_t *l2b, const mb_bseq1_t *t, int32_t n_seg, const int32_t *n_hit,
+                    mb_hit_t *const *hit, int32_t hit_idx, const mb_opt_t *opt, int seg_idx, int32_t mate_qlen)
+{
+	size_t l0 = out->l;
+	if (!g_fmt_native || mbg_fmt_record(out, l2b, t, n_seg, n_hit, hit, hit_idx, opt, seg_idx, mate_qlen) < 0) {
+		mb_format(km, out, l2b, t, n_seg, n_hit, hit, hit_idx, opt, seg_idx, mate_qlen);
+		return;
+	}
+	if (g_fmt_check) {
+		kstring_t r = {0, 0, 0};
+		mb_format(km, &r, l2b, t, n_seg, n_hit, hit, hit_idx, opt, seg_idx, mate_qlen);
+		if (r.l != out->l - l0 || memcmp(r.s, out->s + l0, r.l) != 0) {
+			fprintf(stderr, "[mbg_map] record of '%s' differs from mb_format:\n  ours: %.*s  ref:  %s", t->name, (int)(out->l - l0), out->s + l0, r.s);
+			abort();
+		}
+		free(r.s);
+	}
+}
+
 typedef struct {
 	mbg_fq_buf_t *buf;
 	mbg_fq_chunk_t ch;
@@ -571,12 +594,12 @@ static void fmt_task(void *data, int64_t k, int tid)
 								if (h->score < (double)opt->out_s * q->score) continue;
 							}
 						}
-						mb_format(km, out, c->idx->l2b, t, seg_en - seg_st, &s->n_hit[seg_st], &s->hit[seg_st], j, opt, i - seg_st, mate_qlen);
+						fmt_one(km, out, c->idx->l2b, t, seg_en - seg_st, &s->n_hit[seg_st], &s->hit[seg_st], j, opt, i - seg_st, mate_qlen);
 						n_sec += (h->parent != h->id);
 					}
 				}
 			} else if (!(opt->flag & MB_F_NO_UNMAP)) {
-				mb_format(km, out, c->idx->l2b, t, seg_en - seg_st, &s->n_hit[seg_st], &s->hit[seg_st], -1, opt, i - seg_st, mate_qlen);
+				fmt_one(km, out, c->idx->l2b, t, seg_en - seg_st, &s->n_hit[seg_st], &s->hit[seg_st], -1, opt, i - seg_st, mate_qlen);
 			}
 		}
 		for (i = seg_st; i < seg_en; ++i) {
@@ -672,6 +695,12 @@ void __wrap_kt_pipeline(int n_threads, void *(*func)(void*, int, void*), void *s
 		const char *e = getenv("MBG_FMT_THREADS");
 		int n = e ? atoi(e) : p->opt->n_thread;
 		g_fmt_pool = mbs_pool_init(n > 0 ? n : 1, 64 * 1024);
+	}
+	{   /* the record formatter needs the run's read group, which only mb_fmt_sam knows */
+		const char *e = getenv("MBG_FMT");
+		g_fmt_native = !(e && strcmp(e, "ref") == 0) && mbg_fmt_learn_rg(p->opt) == 0;
+		e = getenv("MBG_FMT_CHECK");
+		g_fmt_check = e && atoi(e) > 0;
 	}
 	g_pipe.p = p, g_pipe.n_fq = 0, g_pipe.n_seq = p->n_seq, g_pipe.n_batch = 0;
 	__real_kt_pipeline(n_threads, pipe_step, 0, n_steps);

@@ -534,11 +534,21 @@ class Context:
             check(self.L.mbg_sa_batch(self.h, len(x), _ptr(x)), "mbg_sa_batch")
         return x
 
-    def l2b_getseq(self, tid, st, en):
+    def l2b_getseq(self, tid, st, en, ctg_len=None, with_len=False):
+        """Windows [st,en) of contigs tid, clipped as l2b_getseq clips them; with ctg_len (the
+        index's contig lengths) each output is sized to its clipped window, which a negative en
+        (clipped to the contig end, l2bit.c:61) needs.  with_len: also l2b_getseq's return values
+        (en - st after clipping, which may be negative; -1 for a bad contig id)."""
         tid = np.ascontiguousarray(tid, np.int64)
         st = np.ascontiguousarray(st, np.int64)
         en = np.ascontiguousarray(en, np.int64)
-        span = np.maximum(en - np.maximum(st, 0), 0)
+        en_c = en
+        if ctg_len is not None:
+            cl = np.asarray(ctg_len, np.uint64)
+            ok = (tid >= 0) & (tid < len(cl))
+            L = np.where(ok, cl[np.where(ok, tid, 0)], 0).astype(np.uint64)
+            en_c = np.where(en.astype(np.uint64) > L, L.astype(np.int64), en)
+        span = np.maximum(en_c - np.maximum(st, 0), 0)
         off = np.zeros(len(tid), np.int64)
         if len(tid) > 1:
             np.cumsum(span[:-1], out=off[1:])
@@ -546,7 +556,8 @@ class Context:
         ln = np.zeros(len(tid), np.int64)
         check(self.L.mbg_getseq(self.h, len(tid), _ptr(tid), _ptr(st), _ptr(en), _ptr(off), _ptr(out), len(out),
                                 _ptr(ln)), "mbg_getseq")
-        return [out[off[i]:off[i] + max(ln[i], 0)].copy() if ln[i] >= 0 else None for i in range(len(tid))]
+        seqs = [out[off[i]:off[i] + max(ln[i], 0)].copy() if ln[i] >= 0 else None for i in range(len(tid))]
+        return (seqs, ln) if with_len else seqs
 
     # ---- seed -> locate -> extend step over device-resident reads ------------------------------
     def map_step_dev(self, n_reads, off_ptr, len_ptr, seq_ptr, max_len, opt: MapOpt, collect_stats=False, stream=None):

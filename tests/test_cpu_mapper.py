@@ -26,3 +26,78 @@ def exes():
 def test_batched_driver_matches_reference(exes, tmp_path_factory, name):
     st = check_sets(exes[0], exes[1], name, tmp_path_factory, threads=4)
     assert st["executor"] == "cpu-reference"
+
+
+REF_CLI = os.path.join(ROOT, "oracle", "_ref", "minibwa")
+
+# output options that change what mb_fmt_sam / mb_fmt_paf write (map-main.c option letters)
+OUT_OPTS = {
+    "rg": ["-R", r"@RG\tID:grp7\tSM:s1"],
+    "comment": ["-y"],
+    "supp_soft": ["-Y"],
+    "xa": ["--outn", "5", "--xa", "12", "--outs", "0.5"],
+    "no_unmap_eqx": ["-u", "--eqx"],
+    "no_pairing": ["-P"],
+    "paf": ["-f"],
+}
+
+
+def _with_comments(src, dst):
+    """The same FASTQ with a comment after each name (for -y)."""
+    with open(src) as f, open(dst, "w") as g:
+        for i, line in enumerate(f):
+            if i % 4 == 0:
+                line = line.rstrip("\n") + " BC:Z:AC%d\tXX:i:%d\n" % (i % 7, i)
+            g.write(line)
+
+
+@pytest.mark.parametrize("opt", sorted(OUT_OPTS))
+def test_output_options_match_reference(exes, tmp_path_factory, tmp_path, opt):
+    """The driver's native SAM/PAF formatter (mbg_fmt.c) on output options the golden sets leave at
+    their defaults: every record is also formatted by the reference's mb_format and compared
+    (MBG_FMT_CHECK=1 aborts on the first difference), and the whole output equals the reference
+    binary's on the same inputs."""
+    import subprocess
+    from tests.mapper_sets import _body_md5, _files
+    if not os.path.exists(REF_CLI):
+        pytest.skip("oracle/_ref/minibwa not built (make -C oracle)")
+    f = _files(tmp_path_factory)
+    r1, r2 = f["p1"], f["p2"]
+    if opt == "comment":
+        r1, r2 = str(tmp_path / "c1.fq"), str(tmp_path / "c2.fq")
+        _with_comments(f["p1"], r1)
+        _with_comments(f["p2"], r2)
+    args = ["map", "-t4"] + OUT_OPTS[opt] + [f["rep"], r1, r2]
+    ref = subprocess.run([REF_CLI] + args, capture_output=True, text=True, timeout=300)
+    assert ref.returncode == 0, ref.stderr[-2000:]
+    env = dict(os.environ, MBG_MAP_STATS="1", MBG_FMT_CHECK="1")
+    got = subprocess.run([exes[0]] + args, capture_output=True, text=True, timeout=300, env=env)
+    assert got.returncode == 0, got.stderr[-3000:]
+    assert "formatter native" in got.stderr
+    n = sum(1 for l in got.stdout.split("\n") if l and not l.startswith("@"))
+    assert n >= 4000
+    assert _body_md5(got.stdout) == _body_md5(ref.stdout)
+    if opt == "rg":
+        assert "\tRG:Z:grp7" in got.stdout
+    if opt == "xa":
+        assert "\tXA:Z:" in got.stdout or "\tn2:i:" in got.stdout
+    if opt == "comment":
+        assert "\tBC:Z:AC" in got.stdout
+
+
+def test_long_reads_paf_match_reference(exes, tmp_path_factory):
+    """Long reads in PAF (-f), the long-read workload's output, through the native formatter."""
+    import subprocess
+    from tests.mapper_sets import _body_md5, _files
+    if not os.path.exists(REF_CLI):
+        pytest.skip("oracle/_ref/minibwa not built (make -C oracle)")
+    f = _files(tmp_path_factory)
+    args = ["map", "-t4", "-f", f["rep"], f["lr"]]
+    ref = subprocess.run([REF_CLI] + args, capture_output=True, text=True, timeout=300)
+    assert ref.returncode == 0, ref.stderr[-2000:]
+    env = dict(os.environ, MBG_MAP_STATS="1", MBG_FMT_CHECK="1")
+    got = subprocess.run([exes[0]] + args, capture_output=True, text=True, timeout=300, env=env)
+    assert got.returncode == 0, got.stderr[-3000:]
+    assert "formatter native" in got.stderr
+    assert "\tcg:Z:" in got.stdout
+    assert _body_md5(got.stdout) == _body_md5(ref.stdout)
