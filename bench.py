@@ -38,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "aligned read pairs/s, 2x150bp vs 3.1Gbp synthetic ref (GPU vs CPU); p50/p99 lat"
+METRIC_LONG = "mapped long reads/s, log-uniform 100-20000 bp vs 3.1Gbp synthetic ref, PAF output (GPU vs CPU)"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -104,6 +105,9 @@ def main():
                          "150 bp); reports reads/s of each")
     ap.add_argument("--long", type=int, default=0, metavar="N",
                     help="with --anchor-batch: N single 10 kb reads (0.1%% substitutions) instead of the read pairs")
+    ap.add_argument("--long-reads", type=int, default=0, metavar="N",
+                    help="the mapper on N single-end long reads (log-uniform 100-20,000 bp, 0.1%% substitutions, "
+                         "0.05%% indels) with PAF output (`map -f`) instead of the read pairs; reports reads/s")
     ap.add_argument("--fastq-batch", type=int, default=0, metavar="N",
                     help="only run the FASTQ reader (host): N 150 bp reads in one file, plain and gzip, in the CLI's "
                          "100M-base chunks; reports records/s beside the reference's mb_bseq_read")
@@ -337,11 +341,22 @@ def mapper_bench(args):
         open(pfile, "w").write(params)
         log(r.stdout.strip(), f"(wall {time.time() - t0:.1f}s)")
     t0 = time.time()
-    r1, r2 = synth.read_pairs(ref, args.pairs, seed=11, rlen=150, frag_mean=400, frag_sd=50, sub=0.01, indel=0.001)
-    f1, f2 = os.path.join(work, "r1.fq"), os.path.join(work, "r2.fq")
-    synth.write_pairs_fastq(f1, f2, r1, r2, prefix="p_")
-    del r1, r2
-    log(f"{args.pairs} pairs written to FASTQ in {time.time() - t0:.1f}s")
+    long_mode = args.long_reads > 0
+    if long_mode:      # single-end long reads, PAF output
+        seq, off, ln = synth.single_reads(ref, args.long_reads, seed=13, lmin=100, lmax=20000, sub=0.001, indel=0.0005)
+        f1 = os.path.join(work, "long.fq")
+        synth.write_fastq(f1, seq, off, ln, prefix="l_")
+        n_units, tot_bases, inputs, fmt_opt = args.long_reads, int(ln.sum()), [f1], ["-f"]
+        del seq, off, ln
+        log(f"{n_units} long reads ({tot_bases / 1e6:.1f} Mbp) written to FASTQ in {time.time() - t0:.1f}s")
+    else:
+        r1, r2 = synth.read_pairs(ref, args.pairs, seed=11, rlen=150, frag_mean=400, frag_sd=50, sub=0.01, indel=0.001)
+        f1, f2 = os.path.join(work, "r1.fq"), os.path.join(work, "r2.fq")
+        synth.write_pairs_fastq(f1, f2, r1, r2, prefix="p_")
+        n_units, tot_bases, inputs, fmt_opt = args.pairs, 300 * args.pairs, [f1, f2], []
+        del r1, r2
+        log(f"{args.pairs} pairs written to FASTQ in {time.time() - t0:.1f}s")
+    unit = "long reads/s" if long_mode else "pairs/s"
 
     # the GPU mapper: warmup + timed passes in one process; pass 0 writes the SAM compared below
     gpu_sam = os.path.join(work, "gpu.sam")
@@ -349,10 +364,10 @@ def mapper_bench(args):
     n_front_per_pass = None
     env = dict(os.environ, MBG_MAP_STATS="1")
     # pass-1 algorithmic work is counted in the first (warmup) pass only: counting adds atomics
-    fronts_pass = max(1, -(-(300 * args.pairs) // 100_000_000))
+    fronts_pass = max(1, -(-tot_bases // 100_000_000))
     if args.warmup > 0:
         env["MBG_COUNT_WORK"] = str(fronts_pass)
-    cmd = [exe, "-t", str(cores), "-P", str(passes), "-o", gpu_sam, "-O", "/dev/null", prefix, f1, f2]
+    cmd = [exe] + fmt_opt + ["-t", str(cores), "-P", str(passes), "-o", gpu_sam, "-O", "/dev/null", prefix] + inputs
     log("gpu:", " ".join(cmd))
     t0 = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=3600)
@@ -368,10 +383,10 @@ def mapper_bench(args):
     summ = _kv([l for l in r.stderr.split("\n") if l.startswith("[mbg_map] summary:")][-1])
     assert len(pass_s) == passes, r.stderr[-2000:]
     for i, t in enumerate(pass_s):
-        log(f"pass {i}{' (warmup)' if i < args.warmup else ''}: {t:.3f} s = {args.pairs / t:,.0f} pairs/s")
+        log(f"pass {i}{' (warmup)' if i < args.warmup else ''}: {t:.3f} s = {n_units / t:,.0f} {unit}")
     timed = pass_s[args.warmup:]
     el = sum(timed)
-    value = args.steps * args.pairs / el
+    value = args.steps * n_units / el
     n_front_per_pass = len(fronts) // passes
     tf = fronts[args.warmup * n_front_per_pass:]
     cf = fronts[:n_front_per_pass] if args.warmup > 0 else []
@@ -418,19 +433,20 @@ def mapper_bench(args):
     # the same program, one `minibwa map` CLI run, timed the reference's way (T_end - T_idx)
     cli_run = None
     if not args.no_cpu:
-        r = subprocess.run([cli, "map", "-t", str(cores), "-o", "/dev/null", prefix, f1, f2], capture_output=True, text=True,
-                           timeout=3600)
+        r = subprocess.run([cli, "map"] + fmt_opt + ["-t", str(cores), "-o", "/dev/null", prefix] + inputs, capture_output=True,
+                           text=True, timeout=3600)
         if r.returncode == 0:
             ti, te = _ref_timings(r.stderr)
-            cli_run = {"pairs_per_s": round(args.pairs / (te - ti), 1), "map_s": round(te - ti, 3), "index_load_s": round(ti, 3)}
+            cli_run = {"units_per_s": round(n_units / (te - ti), 1), "unit": unit, "map_s": round(te - ti, 3),
+                       "index_load_s": round(ti, 3)}
             log("gpu CLI run:", json.dumps(cli_run))
 
     cpu, parity = None, None
     if not args.no_cpu and os.path.exists(ref_cli):
         runs = []
         for k in range(max(1, args.cpu_runs)):
-            r = subprocess.run([ref_cli, "map", "-t", str(cores), "-o", "/dev/null", prefix, f1, f2], capture_output=True,
-                               text=True, timeout=3600)
+            r = subprocess.run([ref_cli, "map"] + fmt_opt + ["-t", str(cores), "-o", "/dev/null", prefix] + inputs,
+                               capture_output=True, text=True, timeout=3600)
             if r.returncode != 0:
                 log("reference mapper failed:", r.stderr[-1000:])
                 break
@@ -439,28 +455,37 @@ def mapper_bench(args):
             log(f"reference run {k}: {te - ti:.3f} s mapping ({ti:.1f} s index load)")
         if runs:
             med = float(np.median(runs))
-            cpu = {"value": round(args.pairs / med, 1), "unit": "read pairs/s", "cores": cores, "kind": "reference",
-                   "sample": f"the whole workload: `minibwa map -t{cores}` (the reference built from source, -O3 -msse4.2, "
+            cpu = {"value": round(n_units / med, 1), "unit": "long reads/s" if long_mode else "read pairs/s", "cores": cores,
+                   "kind": "reference",
+                   "sample": f"the whole workload: `minibwa map {' '.join(fmt_opt + ['-t' + str(cores)])}` (the reference built "
+                             f"from source, -O3 -msse4.2, "
                              f"mimalloc) on the same FASTQ and index files, output to /dev/null; T_end - T_idx from its "
                              f"stderr, median of {len(runs)} run(s): {med:.3f} s; host: {cpu_model()}, {cores} cores, "
                              f"1 thread per core",
                    "runs_s": [round(x, 3) for x in runs]}
         cpu_sam = os.path.join(work, "cpu.sam")
-        r = subprocess.run([ref_cli, "map", "-t", str(cores), "-o", cpu_sam, prefix, f1, f2], capture_output=True, text=True,
-                           timeout=3600)
+        r = subprocess.run([ref_cli, "map"] + fmt_opt + ["-t", str(cores), "-o", cpu_sam, prefix] + inputs, capture_output=True,
+                           text=True, timeout=3600)
         if r.returncode == 0:
             mg, ng = _sam_body_md5(gpu_sam)
             mc, nc = _sam_body_md5(cpu_sam)
             parity = {"sam_identical": mg == mc, "gpu_md5": mg, "cpu_md5": mc, "lines": ng, "cpu_lines": nc}
             log("parity:", json.dumps(parity))
     latency = None
-    if args.serve_requests > 0:
+    if args.serve_requests > 0 and not long_mode:
         latency = serve_mapper(args, ref, prefix, work, cores)
 
+    if long_mode:
+        workload = (f"end-to-end long-read mapping (FASTQ in, PAF out: map -f), {n_units} single-end reads log-uniform "
+                    f"100-20,000 bp ({tot_bases / 1e6:.0f} Mbp) vs {args.ref_len / 1e9:g} Gbp synthetic reference "
+                    f"({args.rep_frac:.0%} repeats)")
+    else:
+        workload = (f"end-to-end mapping (FASTQ in, SAM out), {args.pairs} pairs 2x150 bp vs "
+                    f"{args.ref_len / 1e9:g} Gbp synthetic reference ({args.rep_frac:.0%} repeats)")
     res = {
-        "metric": METRIC,
+        "metric": METRIC_LONG if long_mode else METRIC,
         "value": round(value, 1),
-        "unit": "read pairs/s",
+        "unit": "long reads/s" if long_mode else "read pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -470,9 +495,9 @@ def mapper_bench(args):
         "vs_baseline": None,
         "dtype": "u64 (FM-index ranks) + int8/int32 (DP)",
         "data": "synthetic (seeded splitmix64 reference and reads; index built on the device)",
-        "config": {"workload": f"end-to-end mapping (FASTQ in, SAM out), {args.pairs} pairs 2x150 bp vs "
-                               f"{args.ref_len / 1e9:g} Gbp synthetic reference ({args.rep_frac:.0%} repeats)",
-                   "ref_bp": args.ref_len, "pairs": args.pairs, "read_len": 150, "contigs": args.contigs,
+        "config": {"workload": workload,
+                   "ref_bp": args.ref_len, "pairs": 0 if long_mode else args.pairs, "long_reads": args.long_reads,
+                   "read_bases": tot_bases, "read_len": "100-20000" if long_mode else 150, "contigs": args.contigs,
                    "host_threads": cores, "options": "minibwa map defaults (adaptive preset)", "mini_batches_per_pass": n_front_per_pass},
         "roofline": roofline,
         "roofline_kernels": kernels,
@@ -482,7 +507,7 @@ def mapper_bench(args):
         "device_ms_per_pass": phases,
         "gpu_cli_single_run": cli_run,
         "index_load_s": round(load_s, 2),
-        "rounds_per_pass": round(summ.get("rounds", 0) / passes, 1),
+        "device_batches_per_pass": round(summ.get("batches", 0) / passes, 1),
     }
     print(json.dumps(res), flush=True)
     if not args.workdir:

@@ -1312,7 +1312,9 @@ int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, cons
 		if (j.qlen < 0 || j.qoff < 0 || j.qoff + j.qlen > qbytes || j.tid < 0 || (uint64_t)j.tid >= c->lb.n_ctg) return MBG_EARG;
 		if (j.cigar_off < 0 || j.cigar_off + j.qlen + w > cigar_cap) return MBG_EARG;
 		if (j.qlen > kLlMaxQlen || w > kMsMaxWindow) return MBG_EUNSUPPORTED;
-		tid[i] = j.tid, st[i] = ts, en[i] = j.te, woff[i] = wtot, wtot += w;
+		// ts + w, not te: an end at or before the start is an empty window here, where getseq_kernel
+		// would clip a negative end to the contig end as l2b_getseq does
+		tid[i] = j.tid, st[i] = ts, en[i] = ts + w, woff[i] = wtot, wtot += w;
 		max_q = j.qlen > max_q ? j.qlen : max_q;
 		max_w = w > max_w ? (int32_t)w : max_w;
 	}

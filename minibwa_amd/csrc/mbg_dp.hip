@@ -1006,10 +1006,20 @@ __device__ __forceinline__ int wave_take(unsigned int *ctr)
 // dp_team_kernel.  The job
 // loops take the next index in the loop condition and have no other exit, so that the lane-0
 // regions inside the body cannot become part of the loop's exit.
+// With a.inline_general (the batch's widest LDS ring fits the fast kernel's LDS), the jobs the ring
+// path of dp_kernel<DUAL, false> would take run here as they come, so that the batch's two job
+// classes share one launch and its tail instead of running one launch after the other; only
+// device-memory rings and team jobs are deferred then.
 template <bool DUAL>
 __global__ void __launch_bounds__(64) dp_fast_kernel(DpArgs a)
 {
 	extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+	__shared__ int8_t mat[32];
+	const bool inl = a.inline_general != 0;
+	if (inl) {
+		if (threadIdx.x < 32) mat[threadIdx.x] = a.mat[threadIdx.x];
+		__syncthreads();
+	}
 	uint8_t *slab = a.pbuf + (size_t)blockIdx.x * a.slab;
 	int n = a.n_jobs;
 	if (a.n_jobs_dev) { const unsigned long long nd = *a.n_jobs_dev; n = nd < (unsigned long long)n ? (int)nd : n; }
@@ -1018,8 +1028,13 @@ __global__ void __launch_bounds__(64) dp_fast_kernel(DpArgs a)
 		const DpJob jb = a.jobs[j];
 		const JobP<DUAL> P = job_params<DUAL>(jb, a);
 		if (!fast_job<DUAL>(P, a.qt_cap)) {
-			const int pos = wave_take(a.next + 1);
-			if (threadIdx.x == 0) a.defer[pos] = j;
+			if (inl && P.nb + 17 <= a.cap && !team_job<DUAL>(P, a.tcap)) {    // dp_kernel<DUAL, false>'s jobs
+				if (((P.qlen + 15) & ~15) + ((P.tlen + 15) & ~15) <= a.qt_cap) run_job_lds<DUAL, true>(a, j, jb, P, lds, mat, slab, a.cap);
+				else run_job_lds<DUAL, false>(a, j, jb, P, lds, mat, slab, a.cap);
+			} else {
+				const int pos = wave_take(a.next + 1);
+				if (threadIdx.x == 0) a.defer[pos] = j;
+			}
 		} else if (P.right) run_job_fast<DUAL, true>(a, j, jb, P, lds, slab);
 		else run_job_fast<DUAL, false>(a, j, jb, P, lds, slab);
 	}
@@ -1062,8 +1077,10 @@ int dp_team_cap(int nb_max)
 	return c;
 }
 
-hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st)
+hipError_t launch_dp(bool dual, const DpArgs &a0, unsigned grid, hipStream_t st)
 {
+	DpArgs a = a0;
+	a.inline_general = 12 * a.cap <= kFastState;   // the ring path's state and staging fit the fast kernel's LDS
 	const size_t fshm = (size_t)kFastState + (size_t)a.qt_cap + 32;
 	if (dual) hipLaunchKernelGGL(dp_fast_kernel<true>, dim3(grid), dim3(64), fshm, st, a);
 	else hipLaunchKernelGGL(dp_fast_kernel<false>, dim3(grid), dim3(64), fshm, st, a);
@@ -1078,10 +1095,12 @@ hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st)
 		e = hipGetLastError();
 		if (e != hipSuccess) return e;
 	}
-	const size_t shm = (size_t)12 * a.cap + (size_t)a.qt_cap;
-	if (dual) hipLaunchKernelGGL((dp_kernel<true, false>), dim3(grid), dim3(64), shm, st, a);
-	else hipLaunchKernelGGL((dp_kernel<false, false>), dim3(grid), dim3(64), shm, st, a);
-	e = hipGetLastError();
+	if (!a.inline_general) {   // otherwise the fast kernel took these jobs itself
+		const size_t shm = (size_t)12 * a.cap + (size_t)a.qt_cap;
+		if (dual) hipLaunchKernelGGL((dp_kernel<true, false>), dim3(grid), dim3(64), shm, st, a);
+		else hipLaunchKernelGGL((dp_kernel<false, false>), dim3(grid), dim3(64), shm, st, a);
+		e = hipGetLastError();
+	}
 	if (e != hipSuccess || a.gcap == 0) return e;
 	if (dual) hipLaunchKernelGGL((dp_kernel<true, true>), dim3(grid), dim3(64), 0, st, a);
 	else hipLaunchKernelGGL((dp_kernel<false, true>), dim3(grid), dim3(64), 0, st, a);

@@ -71,6 +71,7 @@ struct DpArgs {
 	int32_t tcap;           // ring length of dp_team_kernel (0: no job needs it)
 	unsigned int *next;     // five zeroed job counters (jobs are taken one at a time)
 	int32_t *defer;         // n_jobs entries: jobs the fast kernel leaves to the general ones
+	int32_t inline_general; // the fast kernel runs the LDS-ring general path itself (set when cap fits its LDS)
 	int32_t mat_last, mat_max, mat_min;  // mat[m*m-1] and the extremes (ksw2_extd2_sse.c:77,87-90), as
 	                                     // dwords so that they load as scalars
 	int8_t m;

@@ -2,12 +2,13 @@
  * mbg_map_bench.c -- the batched GPU mapper timed over several passes of the same inputs, with the
  * index loaded (and made resident in HBM) once:
  *
- *   mbg-map-bench [-t threads] [-P passes] [-o first.sam] [-O rest.sam] <idx> <reads.fq> [mates.fq]
+ *   mbg-map-bench [-f] [-t threads] [-P passes] [-o first.sam] [-O rest.sam] <idx> <reads.fq> [mates.fq]
  *
  * Each pass is one call of the reference's mb_map_file (map-main.c:276-299) -- the function
  * `minibwa map` runs after parsing its options and loading the index -- with minibwa's default options
  * (mb_opt_init: the adaptive preset) and -t threads, so a pass is exactly a `minibwa map` run without
- * the index load: FASTQ reading and mini-batching, mapping, pairing and SAM output.  The first pass
+ * the index load: FASTQ reading and mini-batching, mapping, pairing and SAM output (PAF with -f, as
+ * `minibwa map -f`, which writes no header: map-main.c:564-569).  The first pass
  * writes to -o (default /dev/null), later passes to -O (default /dev/null).  The wall time of each pass
  * is printed to stderr as "[mbg_bench] pass <i> <seconds>", the index load as "[mbg_bench] load <s>".
  *
@@ -130,14 +131,15 @@ int main(int argc, char *argv[])
 	kom_realtime();
 	if (getenv("MBG_PROF_OUT")) prof_init();
 	mb_opt_init(&opt);
-	while ((c = ketopt(&o, argc, argv, 1, "t:P:o:O:", 0)) >= 0) {
-		if (c == 't') opt.n_thread = atoi(o.arg);
+	while ((c = ketopt(&o, argc, argv, 1, "ft:P:o:O:", 0)) >= 0) {
+		if (c == 'f') opt.flag |= MB_F_PAF;
+		else if (c == 't') opt.n_thread = atoi(o.arg);
 		else if (c == 'P') passes = atoi(o.arg);
 		else if (c == 'o') out0 = o.arg;
 		else if (c == 'O') out1 = o.arg;
 	}
 	if (argc - o.ind < 2 || passes < 1) {
-		fprintf(stderr, "Usage: mbg-map-bench [-t threads] [-P passes] [-o first.sam] [-O rest.sam] <idx> <reads.fq> [mates.fq]\n");
+		fprintf(stderr, "Usage: mbg-map-bench [-f] [-t threads] [-P passes] [-o first.sam] [-O rest.sam] <idx> <reads.fq> [mates.fq]\n");
 		return 1;
 	}
 	t0 = now_s();
@@ -147,7 +149,7 @@ int main(int argc, char *argv[])
 		return 1;
 	}
 	fprintf(stderr, "[mbg_bench] load %.3f\n", now_s() - t0);
-	if (mb_fmt_sam_hdr(&hdr, idx->l2b, 0, MB_VERSION, argc, argv) < 0) return 1;
+	if (!(opt.flag & MB_F_PAF) && mb_fmt_sam_hdr(&hdr, idx->l2b, 0, MB_VERSION, argc, argv) < 0) return 1;
 	for (i = 0; i < passes; ++i) {
 		double t1 = now_s();
 		if (i == 1 && g_prof_req) atomic_store(&g_prof_on, 1);
