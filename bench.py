@@ -87,7 +87,9 @@ def main():
                     help="only run the plain copy kernel over MiB mebibytes (HBM byte-counter calibration)")
     ap.add_argument("--serve-qps", type=float, default=4000.0,
                     help="request rate of the serving phase (Poisson arrivals)")
-    ap.add_argument("--serve-requests", type=int, default=4000, help="requests in the serving phase (0: skip it)")
+    ap.add_argument("--serve-requests", type=int, default=600, help="requests per offered rate in the serving phase (0: skip it)")
+    ap.add_argument("--serve-sweep", default="250,500,1000,1500,2000",
+                    help="offered request rates (requests/s) of the mapper's serving sweep")
     ap.add_argument("--gather", action="store_true",
                     help="after building the index, only measure the dependent random 64-B gather ceiling of its occ table")
     ap.add_argument("--dp-batch", type=int, default=0, metavar="N",
@@ -515,7 +517,39 @@ def mapper_bench(args):
 
 
 def serve_mapper(args, ref, prefix, work, cores):
-    return None
+    """Served mapping on the batched mapper (minibwa_amd/bin/mbg-serve-bench, the serving loop of
+    minibwa_amd/integration/mbg_serve.c): requests arrive as a Poisson process at each offered rate of
+    the sweep; a request is 1-10,000 read pairs (log-uniform) of the 2x150 bp model, or (5% of the
+    requests) one 10 kb read.  Batches form continuously under the device-memory budget (half the free
+    HBM), pairs with predefined insert-size statistics (-I 400,50) so that every request's records are
+    the same in any batch.  Latency = the request's records back on the host - its arrival."""
+    from minibwa_amd import serve_workload as W
+    exe = os.path.join(ROOT, "minibwa_amd", "bin", "mbg-serve-bench")
+    if not os.path.exists(exe):
+        log(f"{exe} is not built; no serving sweep")
+        return None
+    t0 = time.time()
+    pools = W.write_pools(ref, work, 400_000, 200, seed=21)
+    log(f"serving pools (400,000 pairs, 200 long reads of 10 kb) written in {time.time() - t0:.1f}s")
+    points = []
+    for k, qps in enumerate(float(x) for x in args.serve_sweep.split(",") if x):
+        sched = W.schedule(args.serve_requests, qps, seed=100 + k)
+        sp = os.path.join(work, "sched.txt")
+        W.write_schedule(sp, sched)
+        r = subprocess.run([exe, "-t", str(cores), prefix] + list(pools) + [sp], capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            log("serving run failed:", r.stderr[-2000:])
+            break
+        pt = json.loads(r.stdout.strip().split("\n")[-1])
+        pt["offered_qps_target"] = qps
+        log(f"serve @{qps:g} req/s: achieved {pt['achieved_qps']:.0f} req/s, {pt['pairs_per_s']:,.0f} pairs/s, "
+            f"p50 {pt['p50_ms']:.1f} ms, p99 {pt['p99_ms']:.1f} ms, {pt['reqs_per_batch']:.1f} req/batch")
+        points.append(pt)
+    if not points:
+        return None
+    return {"workload": "Poisson arrivals; request = 1-10,000 pairs 2x150 bp (log-uniform) or, 5% of requests, one 10 kb "
+                        "read; -I 400,50; latency = records back on the host - arrival",
+            "requests_per_point": args.serve_requests, "points": points}
 
 
 

@@ -422,6 +422,13 @@ int mbg_extd2_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_jobs, const mbg_
                         const uint8_t *qseq, const uint8_t *tseq, int8_t m, const int8_t *mat_host,
                         mbg_dp_result_t *res, uint32_t *cigar, int32_t max_qlen, int32_t max_tlen, int32_t max_w);
 
+/* Device memory accounting for admission control: *bytes receives the HBM held by the work buffers
+ * of every context of this process (they grow to the largest batch seen and are kept; the resident
+ * index is not counted), *dev_free and *dev_total the current device's free and total memory (each
+ * pointer may be null).  No counterpart in the reference, which has no device: for the serving loop's
+ * memory-gated admission. */
+int mbg_work_bytes(int64_t *bytes, int64_t *dev_free, int64_t *dev_total);
+
 /* Device memory for callers that keep inputs in HBM without another runtime: allocate, free,
  * and synchronous copies in either direction.  A copy first waits for the work queued on the
  * context's own stream (entry points called with stream == NULL); work the caller queued on a

@@ -4,6 +4,7 @@
 // uploaded byte for byte), grow-only scratch pools and the batch entry points.  No torch, no
 // allocation inside kernels; every HIP call is checked and mapped to an MBG_E* code.
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <mutex>
 #include <stdio.h>
 #include <stdlib.h>
@@ -64,21 +65,25 @@ struct mbg_ctx_s {
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { g_hip_err = std::string(#x) + ": " + hipGetErrorString(e_); return MBG_EHIP; } } while (0)
 
+// device bytes held by the grow-only work buffers of every context in the process (not the index)
+static std::atomic<int64_t> g_work_bytes{0};
+
 static int dgrow(DBuf &b, size_t bytes)
 {
 	if (bytes == 0) bytes = 16;
 	if (b.n >= bytes) return MBG_OK;
-	if (b.p) { hipFree(b.p); b.p = nullptr; b.n = 0; }
+	if (b.p) { hipFree(b.p); b.p = nullptr; g_work_bytes -= (int64_t)b.n; b.n = 0; }
 	size_t n = bytes + bytes / 4 + 256;
 	hipError_t e = hipMalloc(&b.p, n);
 	if (e != hipSuccess) { g_hip_err = std::string("hipMalloc: ") + hipGetErrorString(e); b.p = nullptr; return MBG_ENOMEM; }
 	b.n = n;
+	g_work_bytes += (int64_t)n;
 	return MBG_OK;
 }
 
 static void dfree(DBuf &b)
 {
-	if (b.p) hipFree(b.p);
+	if (b.p) hipFree(b.p), g_work_bytes -= (int64_t)b.n;
 	b.p = nullptr, b.n = 0;
 }
 
@@ -1500,6 +1505,18 @@ int mbg_getseq(mbg_ctx_t *c, int64_t n, const int64_t *tid, const int64_t *st, c
 	HIPCHK(hipMemcpyAsync(out, c->b_out[0].p, out_bytes, hipMemcpyDeviceToHost, c->stream));
 	HIPCHK(hipMemcpyAsync(len, c->b_out[1].p, n * 8, hipMemcpyDeviceToHost, c->stream));
 	HIPCHK(hipStreamSynchronize(c->stream));
+	return MBG_OK;
+}
+
+int mbg_work_bytes(int64_t *bytes, int64_t *dev_free, int64_t *dev_total)
+{
+	if (bytes) *bytes = g_work_bytes.load();
+	if (dev_free || dev_total) {
+		size_t f = 0, t = 0;
+		HIPCHK(hipMemGetInfo(&f, &t));
+		if (dev_free) *dev_free = (int64_t)f;
+		if (dev_total) *dev_total = (int64_t)t;
+	}
 	return MBG_OK;
 }
 

@@ -80,6 +80,9 @@ void mbx_run_ll(int32_t n, mbx_req_t *const *r);
 /* mb_bwt_sa_batch and mb_seed_intv_batch semantics, for calls outside the batched driver. */
 void mbx_sa(int64_t n, uint64_t *x);
 void mbx_seed(void *km, int32_t n, const int32_t *len, uint8_t *const *seq, int32_t min_len, int32_t max_sub_occ, mb_sai_v *v);
+/* Device memory: the executor's work buffers (high-water, all contexts), and the device's free and
+ * total memory; all zero for an executor without a device. */
+void mbx_mem(int64_t *work, int64_t *dev_free, int64_t *dev_total);
 /* Accumulated device-side counters for the run summary (may be all zero). */
 typedef struct {
 	int64_t dp_calls, dp_jobs, ll_calls, ll_jobs, fronts, front_reads;

@@ -320,3 +320,9 @@ void mbx_get_stats(mbx_stats_t *st)
 	}
 	pthread_mutex_unlock(&g_st_mu);
 }
+
+void mbx_mem(int64_t *work, int64_t *dev_free, int64_t *dev_total)
+{
+	*work = *dev_free = *dev_total = 0;
+	chk(mbg_work_bytes(work, dev_free, dev_total), "mbg_work_bytes");
+}

@@ -563,50 +563,53 @@ static pbatch_t *read_step(const ref_pipeline_t *p)
 #define FMT_FRAGS 2048
 typedef struct { pbatch_t *s; const mb_opt_t *opt; const mb_idx_t *idx; } fmt_ctx_t;
 
+/* One fragment (n_seg segments t[0..n_seg)) written as the output step writes it (map-main.c:213-240):
+ * each hit kept by the secondary filter (out_n, out_s), or the unmapped record; then its hits freed. */
+void mbg_map_fmt_fragment(kstring_t *out, const l2b_t *l2b, const mb_opt_t *opt, mb_bseq1_t *t, int32_t n_seg, int32_t *n_hit,
+                          mb_hit_t **hit)
+{
+	int32_t i, j;
+	void *km = 0;                /* libc allocation (the CLI's default build, MB_F_NO_KALLOC) */
+	for (i = 0; i < n_seg; ++i) {
+		int32_t mate_qlen = 0;
+		if (n_seg > 1) mate_qlen = t[i != n_seg - 1 ? i + 1 : 0].l_seq;
+		if (n_hit[i] > 0) {
+			int32_t n_sec = 0;
+			for (j = 0; j < n_hit[i]; ++j) {
+				const mb_hit_t *h = &hit[i][j];
+				if (h->parent == h->id || n_sec < opt->out_n) {
+					if (h->parent != h->id) {
+						const mb_hit_t *q = &hit[i][h->parent];
+						if (q->p && h->p) {
+							if (h->p->dp_max < (double)opt->out_s * q->p->dp_max) continue;
+						} else {
+							if (h->score < (double)opt->out_s * q->score) continue;
+						}
+					}
+					fmt_one(km, out, l2b, &t[i], n_seg, n_hit, hit, j, opt, i, mate_qlen);
+					n_sec += (h->parent != h->id);
+				}
+			}
+		} else if (!(opt->flag & MB_F_NO_UNMAP)) {
+			fmt_one(km, out, l2b, &t[i], n_seg, n_hit, hit, -1, opt, i, mate_qlen);
+		}
+	}
+	for (i = 0; i < n_seg; ++i) {
+		for (j = 0; j < n_hit[i]; ++j) free(hit[i][j].p);
+		free(hit[i]);
+		hit[i] = 0;
+	}
+}
+
 static void fmt_task(void *data, int64_t k, int tid)
 {
 	const fmt_ctx_t *c = (const fmt_ctx_t*)data;
 	pbatch_t *s = c->s;
-	const mb_opt_t *opt = c->opt;
-	kstring_t *out = &s->out[k];
 	int32_t f, f1 = (int32_t)((k + 1) * FMT_FRAGS < s->n_frag ? (k + 1) * FMT_FRAGS : s->n_frag);
-	void *km = 0;                /* libc allocation (the CLI's default build, MB_F_NO_KALLOC) */
 	(void)tid;
 	for (f = (int32_t)(k * FMT_FRAGS); f < f1; ++f) {
-		int32_t i, j, seg_st = s->seg_off[f], seg_en = s->seg_off[f] + s->seg_cnt[f];
-		for (i = seg_st; i < seg_en; ++i) {
-			mb_bseq1_t *t = &s->seq[i];
-			int32_t mate_qlen = 0;
-			if (seg_en - seg_st > 1) {
-				int32_t mate_idx = i != seg_en - 1 ? i + 1 : seg_st;
-				mate_qlen = s->seq[mate_idx].l_seq;
-			}
-			if (s->n_hit[i] > 0) {
-				int32_t n_sec = 0;
-				for (j = 0; j < s->n_hit[i]; ++j) {
-					const mb_hit_t *h = &s->hit[i][j];
-					if (h->parent == h->id || n_sec < opt->out_n) {
-						if (h->parent != h->id) {
-							const mb_hit_t *q = &s->hit[i][h->parent];
-							if (q->p && h->p) {
-								if (h->p->dp_max < (double)opt->out_s * q->p->dp_max) continue;
-							} else {
-								if (h->score < (double)opt->out_s * q->score) continue;
-							}
-						}
-						fmt_one(km, out, c->idx->l2b, t, seg_en - seg_st, &s->n_hit[seg_st], &s->hit[seg_st], j, opt, i - seg_st, mate_qlen);
-						n_sec += (h->parent != h->id);
-					}
-				}
-			} else if (!(opt->flag & MB_F_NO_UNMAP)) {
-				fmt_one(km, out, c->idx->l2b, t, seg_en - seg_st, &s->n_hit[seg_st], &s->hit[seg_st], -1, opt, i - seg_st, mate_qlen);
-			}
-		}
-		for (i = seg_st; i < seg_en; ++i) {
-			for (j = 0; j < s->n_hit[i]; ++j) free(s->hit[i][j].p);
-			free(s->hit[i]);
-			s->hit[i] = 0;
-		}
+		int32_t seg_st = s->seg_off[f];
+		mbg_map_fmt_fragment(&s->out[k], c->idx->l2b, c->opt, &s->seq[seg_st], s->seg_cnt[f], &s->n_hit[seg_st], &s->hit[seg_st]);
 	}
 }
 

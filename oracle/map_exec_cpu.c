@@ -165,3 +165,8 @@ void mbx_seed(void *km, int32_t n, const int32_t *len, uint8_t *const *seq, int3
 }
 
 void mbx_get_stats(mbx_stats_t *st) { *st = g_st; memset(&st->dev, 0, sizeof(st->dev)); }
+
+void mbx_mem(int64_t *work, int64_t *dev_free, int64_t *dev_total)
+{
+	*work = *dev_free = *dev_total = 0;
+}
