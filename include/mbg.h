@@ -6,7 +6,10 @@
  * and a hipStream_t (passed as void*) for callers that keep inputs resident in HBM.
  *
  * Every entry point returns MBG_OK (0) or a negative MBG_E* code; mbg_strerror() names it.
- * A context serialises its own calls (one mutex), so one context may be shared by threads.
+ * A context serialises its own calls (one mutex), so one context may be shared by threads.  The
+ * calls share the context's scratch on the device; a *_dev call may return with its work still
+ * running on the caller's stream, and the next call on a different stream (or on the context's own)
+ * first waits for it on the device, so calls from different streams never overlap on that scratch.
  */
 #ifndef MBG_H
 #define MBG_H
@@ -418,6 +421,10 @@ int mbg_lchain_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_reads, int64_t 
 int mbg_ll_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_jobs, const mbg_ll_job_t *jobs, const uint8_t *qseq,
                      const uint8_t *tseq, int8_t m, const int8_t *mat_host, int32_t gapo, int32_t gape, int32_t xtra,
                      mbg_ll_result_t *res, int32_t max_qlen, int64_t rows_total);
+/* mbg_extd2_batch over device-resident jobs, sequences, results and CIGAR slots, asynchronous on the
+ * stream.  The launch is planned for jobs with qlen <= max_qlen, tlen <= max_tlen and w <= max_w
+ * (backtrack slabs and row rings are sized from them); a job outside that plan is not run: its result
+ * is an empty job's with cells -1. */
 int mbg_extd2_batch_dev(mbg_ctx_t *ctx, void *stream, int32_t n_jobs, const mbg_dp_job_t *jobs,
                         const uint8_t *qseq, const uint8_t *tseq, int8_t m, const int8_t *mat_host,
                         mbg_dp_result_t *res, uint32_t *cigar, int32_t max_qlen, int32_t max_tlen, int32_t max_w);
@@ -491,7 +498,9 @@ typedef struct {
  * seeding, SA locate of every interval's anchor rows (seed.c:304-315), and for each read the left
  * and right extension from its longest SMEM (align.c mb_align1/mb_align_pair rules; ungapped fast
  * path first, ksw_extd2 -- or ksw_extz2 when q==q2 and e==e2 -- for the rest).  Synchronises the
- * stream; grows the context's pools and repeats the step if a pool overflowed. */
+ * stream; grows the context's pools and repeats the step if a pool overflowed.  A read longer than
+ * max_len overflows the job pools sized from it: its jobs run empty and the step returns
+ * MBG_EOVERFLOW (the other results stand). */
 int mbg_map_step_dev(mbg_ctx_t *ctx, void *stream, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
                      int32_t max_len, const mbg_map_opt_t *opt, int collect_stats, mbg_map_stats_t *stats, mbg_map_out_t *out);
 

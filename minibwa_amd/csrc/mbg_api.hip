@@ -61,6 +61,10 @@ struct mbg_ctx_s {
 	int64_t cap_task = 0, cap_spill = 0;
 	int64_t cap_rec = 0, cap_rows = 0;
 	hipEvent_t ev[8] = {};
+	// ordering of calls on different streams (ctx_order / ctx_mark)
+	hipEvent_t ord_ev = nullptr;
+	hipStream_t ord_st = nullptr;
+	bool ord_pending = false;
 };
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { g_hip_err = std::string(#x) + ": " + hipGetErrorString(e_); return MBG_EHIP; } } while (0)
@@ -101,6 +105,32 @@ static int hgrow(DBuf &b, size_t bytes)
 }
 
 #define TRY(x) do { int r_ = (x); if (r_ != MBG_OK) return r_; } while (0)
+
+// Every entry point of a context uses the context's scratch (counters, slabs, work buffers).  A call
+// that returns with its work still running on a caller's stream records ord_ev there (ctx_mark); the
+// next call, if it runs on another stream (the context's own included), makes that stream wait for the
+// event first (ctx_order), so that calls from threads on different streams never overlap on the
+// scratch.  Calls on one stream are ordered by the stream itself.
+static int ctx_order(mbg_ctx_t *c, hipStream_t st)
+{
+	if (c->ord_pending && c->ord_st != st) HIPCHK(hipStreamWaitEvent(st, c->ord_ev, 0));
+	return MBG_OK;
+}
+
+static int ctx_mark(mbg_ctx_t *c, hipStream_t st)
+{
+	if (c->ord_ev == nullptr) HIPCHK(hipEventCreateWithFlags(&c->ord_ev, hipEventDisableTiming));
+	HIPCHK(hipEventRecord(c->ord_ev, st));
+	c->ord_st = st, c->ord_pending = true;
+	return MBG_OK;
+}
+
+// the device current and the context's stream ordered after pending work on another stream
+static int ctx_enter(mbg_ctx_t *c)
+{
+	HIPCHK(hipSetDevice(c->device));
+	return ctx_order(c, c->stream);
+}
 
 // ---- the allocation-free SA locate (mbg_sa_batch_dev_noalloc) ----------------------------------
 // sa_kernel's only scratch is its work-queue counter.  For callers that cannot allocate, it lives in
@@ -211,7 +241,9 @@ void mbg_destroy(mbg_ctx_t *c)
 {
 	if (c == nullptr) return;
 	hipSetDevice(c->device);
+	if (c->ord_pending) hipEventSynchronize(c->ord_ev);   // work left on a caller's stream
 	if (c->stream) hipStreamSynchronize(c->stream);
+	if (c->ord_ev) hipEventDestroy(c->ord_ev);
 	free_bwt(c);
 	free_l2b(c);
 	for (auto &b : c->b_in) dfree(b);
@@ -301,7 +333,7 @@ int mbg_bwt_upload(mbg_ctx_t *c, const mbg_bwt_desc_t *d, int32_t pre_len)
 {
 	if (c == nullptr) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	return bwt_upload_locked(c, d, pre_len);
 }
 
@@ -330,7 +362,7 @@ int mbg_l2b_upload(mbg_ctx_t *c, const mbg_l2b_desc_t *d)
 {
 	if (c == nullptr) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	return l2b_upload_locked(c, d);
 }
 
@@ -340,7 +372,7 @@ int mbg_idx_load(mbg_ctx_t *c, const char *prefix, int32_t pre_len)
 {
 	if (c == nullptr || prefix == nullptr) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	std::string fn = std::string(prefix) + ".mbw";
 	FILE *fp = fopen(fn.c_str(), "rb");
 	if (fp == nullptr) return MBG_EIO;
@@ -407,7 +439,7 @@ int mbg_idx_attach(mbg_ctx_t *c, mbg_ctx_t *src)
 		if (src->device != c->device) return MBG_EARG;
 		ix = src->ix, lb = src->lb, has_bwt = src->has_bwt, has_l2b = src->has_l2b, clen = src->h_ctg_len;
 	}
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	HIPCHK(hipStreamSynchronize(c->stream));
 	free_bwt(c);
 	free_l2b(c);
@@ -424,7 +456,7 @@ int mbg_pre_copy(mbg_ctx_t *c, mbg_sai_t *out, uint64_t n)
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt || c->ix.pre_len == 0) return MBG_ENOIDX;
 	if (n != (1ULL << (2 * c->ix.pre_len))) return MBG_EARG;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	HIPCHK(hipMemcpy(out, c->d_pre, n * sizeof(Sai), hipMemcpyDeviceToHost));
 	return MBG_OK;
 }
@@ -435,7 +467,7 @@ int mbg_rank2a(mbg_ctx_t *c, int64_t n, const uint64_t *k, const uint64_t *l, ui
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt) return MBG_ENOIDX;
 	if (n == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	for (int64_t i = 0; i < n; ++i)
 		if (k[i] > l[i] || l[i] > c->ix.seq_len + 1) return MBG_EARG;
 	TRY(dgrow(c->b_in[0], n * 8)); TRY(dgrow(c->b_in[1], n * 8)); TRY(dgrow(c->b_out[0], n * 32)); TRY(dgrow(c->b_out[1], n * 32));
@@ -454,7 +486,7 @@ int mbg_extend(mbg_ctx_t *c, int64_t n, const mbg_sai_t *ik, const uint8_t *is_b
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt) return MBG_ENOIDX;
 	if (n == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	for (int64_t i = 0; i < n; ++i) {
 		const uint64_t xf = is_back[i] ? ik[i].x[0] : ik[i].x[1];
 		if (xf > c->ix.seq_len + 1 || ik[i].size > c->ix.seq_len + 1 - xf) return MBG_EARG;
@@ -565,14 +597,15 @@ int mbg_seed_batch_dev_async(mbg_ctx_t *c, void *stream, int32_t n_seq, const in
 	if (c == nullptr || n_seq < 0 || min_len < 1 || cap < 0 || (n_seq > 0 && (!off || !len || !seq || !out)) || n_out_dev == nullptr) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt) return MBG_ENOIDX;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	TRY(ctx_order(c, st));
 	int64_t *nd = nullptr;
 	TRY(seed_core(c, st, n_seq, off, len, seq, min_len, max_sub_occ, nullptr, &nd, nullptr));
 	const int64_t m = cap < c->cap_rec ? cap : c->cap_rec;
 	if (m > 0) HIPCHK(hipMemcpyAsync(out, c->p_recs.p, (size_t)m * sizeof(SeedRec), hipMemcpyDeviceToDevice, st));
 	HIPCHK(hipMemcpyAsync(n_out_dev, nd, 8, hipMemcpyDeviceToDevice, st));
-	return MBG_OK;
+	return ctx_mark(c, st);
 }
 
 int mbg_seed_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_seq, const int64_t *off, const int32_t *len, const uint8_t *seq,
@@ -581,8 +614,9 @@ int mbg_seed_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_seq, const int64_t 
 	if (c == nullptr || n_seq < 0 || min_len < 1 || cap < 0 || (n_seq > 0 && (!off || !len || !seq || !out)) || n_out_host == nullptr) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt) return MBG_ENOIDX;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	TRY(ctx_order(c, st));
 	for (int attempt = 0; attempt < 4; ++attempt) {
 		int64_t *nd = nullptr;
 		TRY(seed_core(c, st, n_seq, off, len, seq, min_len, max_sub_occ, nullptr, &nd, nullptr));
@@ -602,7 +636,7 @@ int mbg_seed_batch(mbg_ctx_t *c, int32_t n_seq, const int32_t *len, const uint8_
 	*out = nullptr;
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt) return MBG_ENOIDX;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	std::vector<int64_t> off(n_seq + 1, 0);
 	for (int32_t i = 0; i < n_seq; ++i) {
 		if (len[i] < 0 || (len[i] > 0 && seq[i] == nullptr)) return MBG_EARG;
@@ -661,8 +695,11 @@ int mbg_sa_batch_dev(mbg_ctx_t *c, void *stream, int64_t n, uint64_t *x)
 	if (c == nullptr || n < 0 || (n > 0 && x == nullptr)) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt) return MBG_ENOIDX;
-	HIPCHK(hipSetDevice(c->device));
-	return sa_dev_locked(c, stream ? (hipStream_t)stream : c->stream, n, x);
+	TRY(ctx_enter(c));
+	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	TRY(ctx_order(c, st));
+	TRY(sa_dev_locked(c, st, n, x));
+	return ctx_mark(c, st);
 }
 
 int mbg_sa_batch_dev_noalloc(mbg_ctx_t *c, void *stream, int64_t n, uint64_t *x)
@@ -672,7 +709,7 @@ int mbg_sa_batch_dev_noalloc(mbg_ctx_t *c, void *stream, int64_t n, uint64_t *x)
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt) return MBG_ENOIDX;
 	if (n == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	uint32_t *ptr = nullptr;
 	{
 		std::lock_guard<std::mutex> g(g_noalloc_mu);
@@ -680,10 +717,11 @@ int mbg_sa_batch_dev_noalloc(mbg_ctx_t *c, void *stream, int64_t n, uint64_t *x)
 	}
 	if (ptr == nullptr) return MBG_EHIP;            // mbg_init resolves it; it cannot be missing here
 	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	TRY(ctx_order(c, st));
 	hipLaunchKernelGGL(noalloc_clear_kernel, dim3(1), dim3(kNoAllocWords), 0, st, ptr, 1);
 	HIPCHK(hipGetLastError());
 	HIPCHK(launch_sa(c->ix, n, nullptr, x, ptr, nullptr, persistent_grid(c, n, 256), st));
-	return MBG_OK;
+	return ctx_mark(c, st);
 }
 
 int mbg_sa_batch(mbg_ctx_t *c, int64_t n, uint64_t *x)
@@ -692,7 +730,7 @@ int mbg_sa_batch(mbg_ctx_t *c, int64_t n, uint64_t *x)
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt) return MBG_ENOIDX;
 	if (n == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	for (int64_t i = 0; i < n; ++i) if (x[i] > c->ix.seq_len) return MBG_EARG;
 	TRY(dgrow(c->b_in[3], n * 8));
 	HIPCHK(hipMemcpyAsync(c->b_in[3].p, x, n * 8, hipMemcpyHostToDevice, c->stream));
@@ -772,7 +810,7 @@ static int dp_batch(mbg_ctx_t *c, bool dual, int32_t n_jobs, const mbg_dp_job_t 
 	if (c == nullptr || n_jobs < 0 || m < 0 || m > 5 || (n_jobs > 0 && (!jobs || !res || !mat))) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (n_jobs == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	int32_t mq = 1, mt = 1, mw = 1;
 	bool any_cigar = false;
 	for (int32_t i = 0; i < n_jobs; ++i) {
@@ -909,7 +947,7 @@ int mbg_update_extra_batch(mbg_ctx_t *c, int64_t n, mbg_extra_job_t *jobs, const
 	}
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (n == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	TRY(dgrow(c->b_in[0], qbytes + 16)); TRY(dgrow(c->b_in[1], tbytes + 16)); TRY(dgrow(c->b_in[4], (size_t)n * sizeof(ExtraJob)));
 	TRY(dgrow(c->b_out[1], (size_t)cig_end * 4 + 16));
 	HIPCHK(hipMemcpyAsync(c->b_in[0].p, qseq, qbytes, hipMemcpyHostToDevice, c->stream));
@@ -952,9 +990,11 @@ int mbg_anchor_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, int64_t n_
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt || !c->has_l2b) return MBG_ENOIDX;
 	if (n_reads == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
-	return anchor_dev_locked(c, stream ? (hipStream_t)stream : c->stream, n_reads, n_slots, (Sai*)intv, intv_off, qlen, max_occ,
-	                         rows, (AncRec*)anc, anc_off, n_anc, n_intv);
+	TRY(ctx_enter(c));
+	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	TRY(ctx_order(c, st));
+	TRY(anchor_dev_locked(c, st, n_reads, n_slots, (Sai*)intv, intv_off, qlen, max_occ, rows, (AncRec*)anc, anc_off, n_anc, n_intv));
+	return ctx_mark(c, st);
 }
 
 int mbg_anchor_batch(mbg_ctx_t *c, int32_t n_reads, mbg_sai_t *intv, const int64_t *intv_off, const int32_t *qlen, int32_t max_occ,
@@ -977,7 +1017,7 @@ int mbg_anchor_batch(mbg_ctx_t *c, int32_t n_reads, mbg_sai_t *intv, const int64
 		}
 		if (need > anc_off[r + 1] - anc_off[r]) return MBG_EARG;
 	}
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	hipStream_t st = c->stream;
 	DBuf *b = c->a_buf;
 	TRY(dgrow(b[0], (size_t)n_u * sizeof(Sai))); TRY(dgrow(b[1], (size_t)(n_reads + 1) * 8)); TRY(dgrow(b[2], (size_t)n_reads * 4));
@@ -1029,9 +1069,12 @@ int mbg_lchain_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, int64_t n_
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_l2b) return MBG_ENOIDX;
 	if (n_reads == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
-	return lchain_dev_locked(c, stream ? (hipStream_t)stream : c->stream, n_reads, n_slots, (const AncRec*)anc, anc_off, n_anc, *opt,
-	                         work, work_bytes, u, n_u, (AncRec*)out, n_out);
+	TRY(ctx_enter(c));
+	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	TRY(ctx_order(c, st));
+	TRY(lchain_dev_locked(c, st, n_reads, n_slots, (const AncRec*)anc, anc_off, n_anc, *opt, work, work_bytes, u, n_u, (AncRec*)out,
+	                      n_out));
+	return ctx_mark(c, st);
 }
 
 int mbg_lchain_batch(mbg_ctx_t *c, int32_t n_reads, const mbg_anchor_t *anc, const int64_t *anc_off, const int64_t *n_anc,
@@ -1050,7 +1093,7 @@ int mbg_lchain_batch(mbg_ctx_t *c, int32_t n_reads, const mbg_anchor_t *anc, con
 		for (int64_t i = anc_off[r]; i < anc_off[r] + n_anc[r]; ++i)
 			if (anc[i].sid < 0 || (uint64_t)(anc[i].sid >> 1) >= c->lb.n_ctg) return MBG_EARG;
 	}
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	hipStream_t st = c->stream;
 	DBuf *b = c->c_buf;
 	const size_t wb = MBG_LCHAIN_WORK_BYTES(n_slots);
@@ -1086,7 +1129,7 @@ int mbg_map_front(mbg_ctx_t *c, int32_t n, const uint8_t *seq, const int64_t *of
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt || !c->has_l2b) return MBG_ENOIDX;
 	memset(out, 0, sizeof(*out));
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	hipStream_t st = c->stream;
 	for (hipEvent_t &e : c->fev) if (!e) HIPCHK(hipEventCreate(&e));
 	DBuf *f = c->f_buf, *h = c->h_front;
@@ -1229,7 +1272,7 @@ int mbg_ll_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_ll_job_t *jobs, const u
 	const int shift = a.shift, qmax = a.qmax;
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (n_jobs == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	// jobs grouped by registers per lane; each gets its own span of the per-row maxima list
 	std::vector<int32_t> order(n_jobs);
 	std::vector<int64_t> boff(n_jobs);
@@ -1307,7 +1350,7 @@ int mbg_matesw_batch(mbg_ctx_t *c, int32_t n, const mbg_matesw_job_t *jobs, cons
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_l2b) return MBG_ENOIDX;
 	if (n == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	int32_t max_q = 1, max_w = 1;
 	std::vector<int64_t> tid(n), st(n), en(n), woff(n);
 	int64_t wtot = 0;
@@ -1446,8 +1489,9 @@ int mbg_ll_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_jobs, const mbg_ll_jo
 	if (mat_min >= 0 || a.qmax + a.shift > 255) return MBG_EUNSUPPORTED;   // the u8 core's biased bytes
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (n_jobs == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	TRY(ctx_order(c, st));
 	size_t tmp_bytes = 0;
 	HIPCHK(scan_ll_rows(nullptr, &tmp_bytes, nullptr, nullptr, n_jobs, st));
 	TRY(dgrow(c->s_tmp, tmp_bytes + 16));
@@ -1471,12 +1515,14 @@ int mbg_extd2_batch_dev(mbg_ctx_t *c, void *stream, int32_t n_jobs, const mbg_dp
 	if (c == nullptr || n_jobs < 0 || m < 0 || m > 5 || mat_host == nullptr || max_qlen < 0 || max_tlen < 0) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (n_jobs == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	int cap, qt, tcap;
 	int64_t slab;
 	TRY(dp_plan(max_qlen, max_tlen, max_w, &cap, &slab, &qt, &tcap));
-	return dp_dev_locked(c, stream ? (hipStream_t)stream : c->stream, true, n_jobs, (const DpJob*)jobs, qseq, tseq, m, mat_host,
-	                     (DpRes*)res, cigar, cap, slab, qt, tcap);
+	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	TRY(ctx_order(c, st));
+	TRY(dp_dev_locked(c, st, true, n_jobs, (const DpJob*)jobs, qseq, tseq, m, mat_host, (DpRes*)res, cigar, cap, slab, qt, tcap));
+	return ctx_mark(c, st);
 }
 
 int mbg_getseq(mbg_ctx_t *c, int64_t n, const int64_t *tid, const int64_t *st, const int64_t *en, const int64_t *out_off, uint8_t *out,
@@ -1486,7 +1532,7 @@ int mbg_getseq(mbg_ctx_t *c, int64_t n, const int64_t *tid, const int64_t *st, c
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_l2b) return MBG_ENOIDX;
 	if (n == 0) return MBG_OK;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	for (int64_t i = 0; i < n; ++i) {   // the clipped window must fit (the kernel clips the same way)
 		if (tid[i] < 0 || (uint64_t)tid[i] >= c->lb.n_ctg) continue;
 		const uint64_t clen = c->h_ctg_len[tid[i]];
@@ -1525,7 +1571,7 @@ int mbg_dev_alloc(mbg_ctx_t *c, size_t bytes, void **ptr)
 	if (c == nullptr || ptr == nullptr) return MBG_EARG;
 	*ptr = nullptr;
 	std::lock_guard<std::mutex> lk(c->mu);
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
 	if (e != hipSuccess) { g_hip_err = hipGetErrorString(e); *ptr = nullptr; return MBG_ENOMEM; }
 	return MBG_OK;
@@ -1536,7 +1582,7 @@ int mbg_dev_free(mbg_ctx_t *c, void *ptr)
 	if (c == nullptr) return MBG_EARG;
 	if (ptr == nullptr) return MBG_OK;
 	std::lock_guard<std::mutex> lk(c->mu);
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	HIPCHK(hipStreamSynchronize(c->stream));
 	HIPCHK(hipFree(ptr));
 	return MBG_OK;
@@ -1547,7 +1593,7 @@ int mbg_copy_to_device(mbg_ctx_t *c, void *dst, const void *src, size_t bytes)
 	if (c == nullptr || (bytes > 0 && (dst == nullptr || src == nullptr))) return MBG_EARG;
 	if (bytes == 0) return MBG_OK;
 	std::lock_guard<std::mutex> lk(c->mu);
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	HIPCHK(hipStreamSynchronize(c->stream));     // ordered after the context stream's queued work
 	HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
 	return MBG_OK;
@@ -1558,7 +1604,7 @@ int mbg_copy_to_host(mbg_ctx_t *c, void *dst, const void *src, size_t bytes)
 	if (c == nullptr || (bytes > 0 && (dst == nullptr || src == nullptr))) return MBG_EARG;
 	if (bytes == 0) return MBG_OK;
 	std::lock_guard<std::mutex> lk(c->mu);
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	HIPCHK(hipStreamSynchronize(c->stream));     // ordered after the context stream's queued work
 	HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
 	return MBG_OK;
@@ -1585,7 +1631,7 @@ int mbg_host_alloc(mbg_ctx_t *c, size_t bytes, void **ptr)
 {
 	if (c == nullptr || ptr == nullptr) return MBG_EARG;
 	*ptr = nullptr;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 16, 0);
 	if (e != hipSuccess) { g_hip_err = std::string("hipHostMalloc: ") + hipGetErrorString(e); *ptr = nullptr; return MBG_ENOMEM; }
 	return MBG_OK;
@@ -1602,8 +1648,9 @@ int mbg_stream_copy(mbg_ctx_t *c, void *stream, void *dst, const void *src, size
 {
 	if (c == nullptr || dst == nullptr || src == nullptr || (bytes & 15) != 0) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	TRY(ctx_order(c, st));
 	HIPCHK(launch_stream_copy(dst, src, (int64_t)(bytes / 16), (unsigned)c->n_cu * 16, st));
 	HIPCHK(hipStreamSynchronize(st));
 	return MBG_OK;
@@ -1614,7 +1661,7 @@ int mbg_gather_bench(mbg_ctx_t *c, int64_t n_chains, int32_t steps, int32_t bs, 
 	if (c == nullptr || n_chains <= 0 || steps <= 0 || ms == nullptr || (bs != 64 && bs != 128 && bs != 256 && bs != 512 && bs != 1024)) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt) return MBG_ENOIDX;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	for (hipEvent_t &e : c->ev) if (!e) HIPCHK(hipEventCreate(&e));
 	TRY(dgrow(c->b_ctr, 128));
 	unsigned long long *sum = (unsigned long long*)c->b_ctr.p + 6;
@@ -1643,8 +1690,9 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 	if (opt->min_len < 1 || opt->max_occ < 1 || opt->e <= 0 || opt->e2 <= 0) return MBG_EARG;
 	std::lock_guard<std::mutex> lk(c->mu);
 	if (!c->has_bwt || !c->has_l2b) return MBG_ENOIDX;
-	HIPCHK(hipSetDevice(c->device));
+	TRY(ctx_enter(c));
 	hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+	TRY(ctx_order(c, st));
 	for (hipEvent_t &e : c->ev) if (!e) HIPCHK(hipEventCreate(&e));
 	memset(stats, 0, sizeof(*stats));
 	stats->n_reads = n_reads;
@@ -1668,7 +1716,7 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 		TRY(dgrow(c->p_res, job_cap * sizeof(DpRes)));
 		TRY(dgrow(c->p_cig, (size_t)job_cap * (max_len + max_tlen) * 4 + 16));
 		TRY(dgrow(c->p_ctr, 256));
-		unsigned long long *pc = (unsigned long long*)c->p_ctr.p;  // [0..5] job sink, [6] rows, [8..14] work stats
+		unsigned long long *pc = (unsigned long long*)c->p_ctr.p;  // [0..5] job sink, [6] rows, [8..14] work stats, [15] jobs past the pools
 		HIPCHK(hipMemsetAsync(pc, 0, 256, st));
 		HIPCHK(hipMemsetAsync(c->p_best.p, 0, (size_t)n_reads * 8 + 8, st));
 		HIPCHK(hipEventRecord(c->ev[0], st));
@@ -1751,7 +1799,7 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 			out->jobs = (const mbg_dp_job_t*)c->p_jobs.p; out->meta = (const mbg_job_meta_t*)c->p_meta.p; out->res = (const mbg_dp_result_t*)c->p_res.p;
 			out->cigar = (const uint32_t*)c->p_cig.p; out->qbuf = (const uint8_t*)c->p_qb.p; out->tbuf = (const uint8_t*)c->p_tb.p;
 		}
-		return MBG_OK;
+		return hc[15] > 0 ? MBG_EOVERFLOW : MBG_OK;   // jobs of reads longer than max_len ran empty
 	}
 	return MBG_EOVERFLOW;
 }

@@ -52,6 +52,7 @@ struct JobP {
 	int8_t qe8, qe28, sc_mch, sc_mis, m1, sc_N, qe2x, max_sc_, ini, ini2;
 	int w, n_col_, nb, tl_hi, max_sc, long_thres, long_diff;
 	bool skip, with_cigar, approx_max, right, generic;
+	bool unplanned;                      // outside the launch's plan: not run, reported with cells -1
 };
 
 // the scoring matrix enters through scalars the host derived from it (entries 0, 1 and m*m-1, the
@@ -88,6 +89,15 @@ __device__ __forceinline__ JobP<DUAL> job_params(const DpJob &jb, const DpArgs &
 		}
 	}
 	P.nb = P.n_col_ * 16;
+	// a job the launch was not planned for (the *_dev entry points size slabs and rings from the
+	// caller's max_qlen / max_tlen / max_w) would overrun its block's backtrack slab or find no ring
+	// wide enough: it runs as an empty job and reports cells -1 instead
+	P.unplanned = false;
+	if (!P.skip) {
+		const bool ring = P.nb + 17 <= a.cap || (a.gcap > 0 && P.nb + 17 <= a.gcap) ||
+		                  (a.tcap > 0 && P.with_cigar && !P.approx_max && !P.generic && P.nb + 144 <= a.tcap && P.qlen + P.tlen <= 30000);
+		if ((int64_t)(P.qlen + P.tlen) * P.nb + 256 > a.slab || !ring) P.unplanned = P.skip = true, P.n_col_ = 1, P.nb = 16;
+	}
 	P.ini = DUAL ? (int8_t)(-q - e) : 0, P.ini2 = DUAL ? (int8_t)(-q2 - e2) : 0;
 	P.tl_hi = (P.tlen + 15) / 16 * 16 - 1;
 	return P;
@@ -621,7 +631,7 @@ __device__ __forceinline__ void run_job_lds(const DpArgs &a, int jid, const DpJo
 	}
 	__syncthreads();
 	const int n_cigar = backtrack<DUAL, GLOBAL>(P, z, jb, a, lds, slab, 12 * CAP);
-	write_result(a, jid, z, n_cigar, cells);
+	write_result(a, jid, z, n_cigar, P.unplanned ? -1 : cells);
 	__syncthreads();
 }
 

@@ -206,7 +206,20 @@ __device__ __forceinline__ void plan_ext(const MapOpt &o, const int8_t *mat, Ext
 __device__ __forceinline__ void write_job(const MapOpt &o, const JobSink &js, const ExtPlan &p, uint64_t j, uint64_t qo, uint64_t to,
                                           uint64_t co, uint32_t rid, int rev, int64_t tid)
 {
-	if ((int64_t)j >= js.job_cap || (int64_t)(qo + p.qlen) > js.qcap || (int64_t)(to + p.tlen) > js.tcap) return;
+	if ((int64_t)j >= js.job_cap || (int64_t)(qo + p.qlen) > js.qcap || (int64_t)(to + p.tlen) > js.tcap) {
+		// a read longer than the step's max_len: the job, already counted, becomes an empty one (the DP
+		// kernels skip qlen 0) and the overflow is counted, so that the step reports it
+		if ((int64_t)j < js.job_cap) {
+			DpJob d{};
+			d.cigar_off = (int64_t)co;
+			js.jobs[j] = d;
+			JobMeta m{};
+			m.rid = rid; m.side = (int8_t)p.side; m.rev = (int8_t)rev; m.tid = (int32_t)tid; m.tpos = p.tpos;
+			js.meta[j] = m;
+		}
+		atomicAdd(&js.ctr[15], 1ull);
+		return;
+	}
 	for (int i = 0; i < p.qlen; ++i) js.qbuf[qo + i] = p.q(i);
 	if (p.t.plain()) {
 		WinCursor c(p.t);

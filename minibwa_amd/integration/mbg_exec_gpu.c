@@ -179,7 +179,7 @@ static void dp_group(int dual, int32_t n, mbx_req_t *const *req, const int32_t *
 	for (i = 0; i < n; ++i) {               /* each CIGAR into its poster's buffer (room for qlen + tlen) */
 		mbx_req_t *r = req[sel[i]];
 		r->dp = res[i];
-		if (r->dp.n_cigar < 0 || r->dp.n_cigar > r->qlen + r->tlen + 1) {
+		if (r->dp.cells < 0 || r->dp.n_cigar < 0 || r->dp.n_cigar > r->qlen + r->tlen + 1) {
 			fprintf(stderr, "[mbg_exec] a DP job returned %d CIGAR operations for %d + %d bases\n", r->dp.n_cigar, r->qlen, r->tlen);
 			abort();
 		}

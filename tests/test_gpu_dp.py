@@ -151,6 +151,44 @@ def test_extd2_device_entry_point(gpu_ctx):
     assert (res["cells"] > 0).all()
 
 
+def test_extd2_device_entry_point_out_of_plan(gpu_ctx):
+    """A launch planned for shorter jobs than some it holds (max_qlen / max_tlen below theirs): the
+    jobs over the plan are not run and report cells -1, the others equal the reference; the slabs
+    sized from the plan are never overrun."""
+    from oracle import oracle as O
+    from minibwa_amd import mbg, synth
+    qb, qo, ql, tb, to, tl = synth.dp_jobs(600, seed=29)
+    n = len(ql)
+    jobs = np.zeros(n, mbg.DP_JOB_DTYPE)
+    jobs["qoff"], jobs["toff"], jobs["qlen"], jobs["tlen"] = qo[:-1], to[:-1], ql, tl
+    jobs["q"], jobs["e"], jobs["q2"], jobs["e2"] = 12, 2, 24, 1
+    jobs["w"], jobs["zdrop"], jobs["end_bonus"], jobs["flag"] = 101, 172, 10, 0x40
+    span = ql.astype(np.int64) + tl
+    coff = np.concatenate([[0], np.cumsum(span)[:-1]])
+    jobs["cigar_off"] = coff
+    mq, mt = int(np.percentile(ql, 50)), int(np.percentile(tl, 50))
+    over = (ql > mq) | (tl > mt)
+    assert 0 < over.sum() < n
+    mat = np.asarray(O.gen_nt4_mat(2, 8, 0, 1), np.int8)
+    d_jobs, d_q, d_t = gpu_ctx.to_device(jobs), gpu_ctx.to_device(qb), gpu_ctx.to_device(tb)
+    d_res = gpu_ctx.to_device(np.zeros(n, mbg.DP_RES_DTYPE))
+    d_cig = gpu_ctx.to_device(np.zeros(int(span.sum()), np.uint32))
+    mbg.check(gpu_ctx.L.mbg_extd2_batch_dev(gpu_ctx.h, None, n, mbg.C.c_void_p(d_jobs.ptr), mbg.C.c_void_p(d_q.ptr),
+                                            mbg.C.c_void_p(d_t.ptr), 5, mbg._ptr(mat), mbg.C.c_void_p(d_res.ptr),
+                                            mbg.C.c_void_p(d_cig.ptr), mq, mt, 101), "mbg_extd2_batch_dev")
+    res = gpu_ctx.to_host(d_res.ptr, mbg.DP_RES_DTYPE, n)
+    cells = res["cells"]
+    for i in range(n):
+        # over the plan: not run, unless its band still fits the slab planned from the larger sizes
+        # (a job only a little longer than planned may still fit; either way it must be exact or -1)
+        if cells[i] < 0:
+            assert over[i] and int(res["n_cigar"][i]) == 0, i
+            continue
+        r = O.ref_extd2(qb[qo[i]:qo[i + 1]], tb[to[i]:to[i + 1]], mat, 12, 2, 24, 1, 101, 172, 10, 0x40)
+        assert {f: int(res[f][i]) for f in FIELDS} == {f: r[f] for f in FIELDS}, i
+    assert (cells[~over] > 0).all() and (cells[over] < 0).sum() > 0
+
+
 @pytest.mark.parametrize("dual", [True, False])
 def test_dp_wide_band(gpu_ctx, dual):
     """Bands wider than the 4096-cell LDS ring (kb-scale gap fills, w >= 4100) run on a per-block ring
