@@ -420,17 +420,22 @@ def mapper_bench(args):
         kernels.append({"kernel": "dp kernels (ksw_extd2 rounds)", "bound": "valu", "achieved": round(ops, 3), "peak": 39.3,
                         "unit": "Tops/s", "frac": round(ops / 39.3, 4), "traffic": None, "gcups": round(cells / (dp_ms * 1e-3) / 1e9, 1),
                         "cells_per_pass": int(cells), "launches_per_pass": round(summ.get("dev_dp_launches", 0) / passes, 1),
+                        "avg_launch_ms": round(dp_ms / max(1.0, summ.get("dev_dp_launches", 0) / passes), 4),
+                        "note": "dp_fast_kernel<true/false> (ksw_extd2 / ksw_extz2 batches, the ring path inline); ~37 integer "
+                                "VALU ops per band cell with traceback bits; HIP events around each batch's launch",
                         "share_of_device": round(dp_ms / dev_total, 3) if dev_total else None})
     kernels.sort(key=lambda k: -(k["share_of_device"] or 0))
     roofline = kernels[0] if kernels else None
-    if roofline and roofline["kernel"].startswith("smem"):
-        pmc = os.path.join(ROOT, "profiles", "mapper_3g_pmc.json") if args.ref_len == 3_100_000_000 else ""
-        if pmc and os.path.exists(pmc):
-            try:
-                ks = json.load(open(pmc))["kernels"]
-                roofline["traffic"] = [v for k, v in ks.items() if "smem_kernel<1>" in k][0]["hbm_bytes_per_launch"]
-            except Exception:
-                pass
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
+    pmc = os.path.join(ROOT, "profiles", "mapper_3g_pmc.json")
+    if args.ref_len == 3_100_000_000 and not long_mode and os.path.exists(pmc):
+        ks = json.load(open(pmc))["kernels"]
+        for k in kernels:
+            key = "smem_kernel<1>" if k["kernel"].startswith("smem") else "dp_fast_kernel<true>"
+            hit = [v for n, v in ks.items() if key in n]
+            if hit:
+                k["traffic"] = hit[0]["hbm_bytes_per_launch"]
+                k["traffic_source"] = "profiles/mapper_3g_pmc.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)"
 
     # the same program, one `minibwa map` CLI run, timed the reference's way (T_end - T_idx)
     cli_run = None
