@@ -47,6 +47,13 @@ struct mbg_ctx_s {
 	// seed -> locate -> extend step
 	DBuf p_recs, p_rows, p_rowrec, p_first, p_best, p_jobs, p_meta, p_qb, p_tb, p_res, p_cig, p_ctr;
 	DBuf s_slots1, s_slots2, s_cnt, s_off, s_spill, s_tmp, r_cnt, r_off, s_pack, s_qflag;
+	// segmented pass 1 of long reads: segment counts / first tasks per read, segment tasks' reads,
+	// long-read list, handoffs, chains, joins, flags, segment and fix-up slots, fallback list, counters
+	DBuf g_nseg, g_base, g_rid, g_long, g_hand0, g_hand, g_chain, g_chn, g_conv, g_flag, g_slots, g_cnt, g_fix, g_fixcnt, g_fbl, g_ctr;
+	// off unless MBG_SEED_SEG sets a length: on high-identity long reads the pass is bound by the
+	// longest exact stretch, which every segment lane inside it walks again
+	int32_t seg_len = 0, seg_slots = 16, seg_chain = 64, fix_slots = 16;
+	int64_t cap_seg = 0;
 	DBuf m_buf[12];         // mate rescue
 	DBuf a_buf[8];          // anchors
 	DBuf c_buf[8];          // chains
@@ -205,6 +212,17 @@ int mbg_init(int device, mbg_ctx_t **out)
 	}
 	mbg_ctx_t *c = new mbg_ctx_t();
 	c->device = device;
+	{   // MBG_SEED_SEG=len[,segment slots[,chain cap[,fix-up slots]]]: pass 1 splits reads of at least 2*len
+	    // bases into segments searched in parallel (unset or 0: off); the capacities only decide how
+	    // often a read falls back to a serial search, never the result
+		const char *e = getenv("MBG_SEED_SEG");
+		if (e) {
+			int v[4] = {c->seg_len, c->seg_slots, c->seg_chain, c->fix_slots};
+			sscanf(e, "%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3]);
+			c->seg_len = v[0] >= 32 ? v[0] : 0;
+			c->seg_slots = v[1] > 0 ? v[1] : 1, c->seg_chain = v[2] > 0 ? v[2] : 1, c->fix_slots = v[3] > 0 ? v[3] : 1;
+		}
+	}
 	hipDeviceProp_t prop;
 	hipError_t e = hipGetDeviceProperties(&prop, device);
 	if (e == hipSuccess) c->n_cu = prop.multiProcessorCount;
@@ -251,7 +269,9 @@ void mbg_destroy(mbg_ctx_t *c)
 	dfree(c->b_ctr); dfree(c->b_task); dfree(c->b_slab); dfree(c->b_gstate); dfree(c->b_defer);
 	for (DBuf *b : {&c->p_recs, &c->p_rows, &c->p_rowrec, &c->p_first, &c->p_best, &c->p_jobs, &c->p_meta, &c->p_qb, &c->p_tb,
 	                &c->p_res, &c->p_cig, &c->p_ctr, &c->s_slots1, &c->s_slots2, &c->s_cnt, &c->s_off, &c->s_spill, &c->s_tmp,
-	                &c->r_cnt, &c->r_off, &c->s_pack, &c->s_qflag}) dfree(*b);
+	                &c->r_cnt, &c->r_off, &c->s_pack, &c->s_qflag, &c->g_nseg, &c->g_base, &c->g_rid, &c->g_long, &c->g_hand0,
+	                &c->g_hand, &c->g_chain, &c->g_chn, &c->g_conv, &c->g_flag, &c->g_slots, &c->g_cnt, &c->g_fix, &c->g_fixcnt,
+	                &c->g_fbl, &c->g_ctr}) dfree(*b);
 	for (auto &b : c->m_buf) dfree(b);
 	for (auto &b : c->a_buf) dfree(b);
 	for (auto &b : c->c_buf) dfree(b);
@@ -519,6 +539,85 @@ static const int32_t kSlot1 = 16, kSlot2 = 8;
 
 struct SeedCaps { int64_t rec, task, spill; };
 
+// Segmented pass 1 (reads of at least 2*seg_len bases): the serial SMEM search is a chain of starts
+// x0 = 0 -> x1 -> ..., one step each, a step emitting at most one SMEM (mb_bwt_smem, bwt.c:280-305).
+// Segment k's lane starts at k*seg_len and records its chain up to its first value past the segment
+// (its handoff); segment 0 is the read's own pass-1 task and is exact.  The fix-up walks the true
+// chain from segment 0's handoff: where it meets a segment's chain, that lane's steps from there on are
+// the true ones and the walk jumps to its handoff; where it does not, the fix-up takes the true steps
+// itself.  Records of a segment before its join are dropped.  A segment or fix-up task that runs out
+// of room flags its read, which then runs serially from segment 0's handoff.  The result is the serial
+// search's set of SMEMs exactly, whatever the segment length and capacities.
+static int seg_setup(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int32_t *len, SeedArgs &a)
+{
+	if (c->cap_seg < 1024 + n_reads / 16) c->cap_seg = 1024 + n_reads / 16;
+	const int64_t cs = c->cap_seg, nr = n_reads;
+	TRY(dgrow(c->g_nseg, (size_t)(nr + 1) * 4)); TRY(dgrow(c->g_base, (size_t)(nr + 1) * 4));
+	TRY(dgrow(c->g_hand0, (size_t)nr * 4)); TRY(dgrow(c->g_flag, (size_t)nr * 4));
+	TRY(dgrow(c->g_rid, (size_t)cs * 4)); TRY(dgrow(c->g_long, (size_t)nr * 4)); TRY(dgrow(c->g_hand, (size_t)cs * 4));
+	TRY(dgrow(c->g_chain, (size_t)cs * c->seg_chain * 4)); TRY(dgrow(c->g_chn, (size_t)cs * 4)); TRY(dgrow(c->g_conv, (size_t)cs * 4));
+	TRY(dgrow(c->g_slots, (size_t)cs * c->seg_slots * sizeof(SeedRec))); TRY(dgrow(c->g_cnt, (size_t)cs * 4));
+	TRY(dgrow(c->g_fix, (size_t)cs * c->fix_slots * sizeof(SeedRec))); TRY(dgrow(c->g_fixcnt, (size_t)cs * 4));
+	TRY(dgrow(c->g_fbl, (size_t)nr * 4)); TRY(dgrow(c->g_ctr, 64));
+	uint32_t *nseg = (uint32_t*)c->g_nseg.p, *base = (uint32_t*)c->g_base.p, *gc = (uint32_t*)c->g_ctr.p;
+	HIPCHK(hipMemsetAsync(gc, 0, 64, st));
+	HIPCHK(hipMemsetAsync(c->g_flag.p, 0, (size_t)nr * 4, st));
+	HIPCHK(hipMemsetAsync(nseg + nr, 0, 4, st));
+	HIPCHK(launch_seg_count(len, n_reads, c->seg_len, nseg, st));
+	size_t tb = 0;
+	HIPCHK(scan_u32(nullptr, &tb, nullptr, nullptr, nr + 1, st));
+	TRY(dgrow(c->s_tmp, tb > c->s_tmp.n ? tb : c->s_tmp.n));
+	HIPCHK(scan_u32(c->s_tmp.p, &tb, nseg, base, nr + 1, st));
+	// the segment total (base[n_reads]) where seed_overflowed reads it
+	HIPCHK(hipMemcpyAsync(gc + 5, base + nr, 4, hipMemcpyDeviceToDevice, st));
+	HIPCHK(launch_seg_build(len, n_reads, c->seg_len, base, cs, (uint32_t*)c->g_rid.p, (uint32_t*)c->g_long.p, gc + 3,
+	                        (uint32_t*)c->g_flag.p, st));
+	a.seg_len = c->seg_len;
+	a.segbase = base; a.seg_rid = (const uint32_t*)c->g_rid.p;
+	a.hand0 = (uint32_t*)c->g_hand0.p; a.hand = (uint32_t*)c->g_hand.p;
+	a.chain = (uint32_t*)c->g_chain.p; a.chain_n = (uint32_t*)c->g_chn.p; a.chain_cap = c->seg_chain;
+	a.conv = (uint32_t*)c->g_conv.p; a.rflag = (uint32_t*)c->g_flag.p;
+	return MBG_OK;
+}
+
+// callers that hold the read lengths on the host size the segment pools for them up front (otherwise
+// the pools grow after a batch that outran them, whose reads then ran serially)
+static void seg_hint(mbg_ctx_t *c, const int32_t *len_host, int32_t n)
+{
+	if (c->seg_len <= 0 || len_host == nullptr) return;
+	int64_t need = 0;
+	for (int32_t i = 0; i < n; ++i)
+		if (len_host[i] >= 2 * c->seg_len) need += len_host[i] / c->seg_len - 1;
+	if (need > 0 && need + 64 > c->cap_seg) c->cap_seg = need + 64;
+}
+
+// the segment launch, the fix-up, the fallback and the merge (after pass 1 over the reads)
+static int seg_run(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int32_t *len, const SeedArgs &a)
+{
+	const int64_t cs = c->cap_seg;
+	uint32_t *gc = (uint32_t*)c->g_ctr.p;
+	const unsigned grid = persistent_grid(c, cs < n_reads ? cs : n_reads, 256);
+	SeedArgs s3 = a;                       // segments k >= 1
+	s3.next = gc; s3.n_tasks = 0xffffffffu; s3.n_tasks_dev = (const uint32_t*)c->g_base.p + n_reads; s3.task_cap = cs;
+	s3.slots = (SeedRec*)c->g_slots.p; s3.slot_cnt = (uint32_t*)c->g_cnt.p; s3.kslot = c->seg_slots;
+	HIPCHK(launch_smem(c->ix, s3, 3, grid, st));
+	SeedArgs s4 = a;                       // the fix-up of each long read
+	s4.next = gc + 1; s4.n_tasks = 0xffffffffu; s4.n_tasks_dev = gc + 3; s4.task_cap = cs; s4.list = (const uint32_t*)c->g_long.p;
+	s4.slots = (SeedRec*)c->g_fix.p; s4.slot_cnt = (uint32_t*)c->g_fixcnt.p; s4.kslot = c->fix_slots;
+	HIPCHK(launch_smem(c->ix, s4, 4, grid, st));
+	HIPCHK(launch_seg_flagged((const uint32_t*)c->g_long.p, gc + 3, (const uint32_t*)c->g_flag.p, (uint32_t*)c->g_fbl.p, gc + 4, n_reads,
+	                          st));
+	SeedArgs s5 = a;                       // reads whose segment or fix-up tasks overflowed
+	s5.next = gc + 2; s5.n_tasks = 0xffffffffu; s5.n_tasks_dev = gc + 4; s5.task_cap = n_reads; s5.list = (const uint32_t*)c->g_fbl.p;
+	s5.slots = nullptr; s5.slot_cnt = nullptr; s5.kslot = 0;
+	HIPCHK(launch_smem(c->ix, s5, 5, grid, st));
+	HIPCHK(launch_seg_merge(len, c->seg_len, (const uint32_t*)c->g_long.p, gc + 3, (const uint32_t*)c->g_flag.p,
+	                        (const uint32_t*)c->g_base.p, (const SeedRec*)c->g_slots.p, (const uint32_t*)c->g_cnt.p, c->seg_slots,
+	                        (const uint32_t*)c->g_conv.p, (const SeedRec*)c->g_fix.p, (const uint32_t*)c->g_fixcnt.p, c->fix_slots,
+	                        a.spill, a.n_spill, a.spill_cap, cs, st));
+	return MBG_OK;
+}
+
 static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
                      int32_t min_len, int32_t max_sub_occ, unsigned long long *stats, int64_t **n_out_dev, hipEvent_t *ev,
                      hipEvent_t ev_pre = nullptr)
@@ -553,8 +652,11 @@ static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_
 	a.slots = (SeedRec*)c->s_slots1.p; a.slot_cnt = cnt; a.kslot = kSlot1;
 	a.spill = (SeedRec*)c->s_spill.p; a.n_spill = n_spill; a.spill_cap = sc;
 	a.stats = stats;
+	const bool seg = c->seg_len > 0 && n_reads > 0;
+	if (seg) TRY(seg_setup(c, st, n_reads, len, a));
 	if (ev_pre) HIPCHK(hipEventRecord(ev_pre, st));
 	if (n_reads > 0) HIPCHK(launch_smem(c->ix, a, 1, persistent_grid(c, n_reads, 256), st));
+	if (seg) TRY(seg_run(c, st, n_reads, len, a));
 	if (ev) HIPCHK(hipEventRecord(ev[0], st));
 	if (n_reads > 0) {
 		HIPCHK(launch_subtask((const SeedRec*)c->s_slots1.p, cnt, kSlot1, n_reads, (const SeedRec*)c->s_spill.p, n_spill, sc,
@@ -585,6 +687,10 @@ static bool seed_overflowed(mbg_ctx_t *c, int64_t *n_rec_host)
 	memcpy(&n_spill, h + 6, 8);
 	bool ov = false;
 	if ((int64_t)h[1] > c->cap_task) c->cap_task = (int64_t)h[1] * 2, ov = true;
+	if (c->seg_len > 0 && c->g_ctr.p) {     // segment tasks past the pool ran as fallbacks: exact, but slow
+		uint32_t g[8];
+		if (hipMemcpy(g, c->g_ctr.p, 32, hipMemcpyDeviceToHost) == hipSuccess && (int64_t)g[5] > c->cap_seg) c->cap_seg = (int64_t)g[5] * 2;
+	}
 	if ((int64_t)n_spill > c->cap_spill) c->cap_spill = (int64_t)n_spill * 2, ov = true;
 	if (n_out > c->cap_rec) c->cap_rec = n_out * 2, ov = true;
 	if (n_rec_host) *n_rec_host = n_out;
@@ -651,6 +757,7 @@ int mbg_seed_batch(mbg_ctx_t *c, int32_t n_seq, const int32_t *len, const uint8_
 	if (n_seq > 0) HIPCHK(hipMemcpyAsync(c->b_in[2].p, len, n_seq * 4, hipMemcpyHostToDevice, c->stream));
 	int64_t n_out = 0;
 	bool ok = false;
+	seg_hint(c, len, n_seq);
 	for (int attempt = 0; attempt < 4 && !ok; ++attempt) {
 		int64_t *nd = nullptr;
 		TRY(seed_core(c, c->stream, n_seq, (const int64_t*)c->b_in[1].p, (const int32_t*)c->b_in[2].p, (const uint8_t*)c->b_in[0].p,
@@ -1155,6 +1262,7 @@ int mbg_map_front(mbg_ctx_t *c, int32_t n, const uint8_t *seq, const int64_t *of
 		wstats = (unsigned long long*)c->f_stats.p;
 	}
 	hipEvent_t sev[2] = {c->fev[1], c->fev[7]};
+	seg_hint(c, len, n);
 	for (int attempt = 0; attempt < 4 && !ok; ++attempt) {
 		if (wstats) HIPCHK(hipMemsetAsync(wstats, 0, 64, st));
 		TRY(seed_core(c, st, n, (const int64_t*)f[1].p, (const int32_t*)f[2].p, (const uint8_t*)f[0].p, o->min_len, o->max_sub_occ,

@@ -65,7 +65,14 @@ struct Lane {
 	uint64_t min_occ;
 	uint64_t px0, px1, psize;          // current interval
 	bool gq;                           // bases read from global memory
+	int32_t stop;                      // segmented pass 1: the first chain value this task does not start from
+	int32_t k, K;                      // fix-up: the segment it is at, and the read's segment count
+	uint32_t nch;                      // segment task: chain values recorded
 };
+
+// stage of a task that gave up (its output slots are full): the read falls back to a serial task
+constexpr int kStageAbandon = 7;
+constexpr uint32_t kNoConv = 0xffffffffu;
 
 __device__ __forceinline__ int qget(const Lane &s, const uint32_t *qs, int i)
 {
@@ -76,20 +83,55 @@ __device__ __forceinline__ int qget(const Lane &s, const uint32_t *qs, int i)
 
 __device__ __forceinline__ void set_p(Lane &s, const Sai &p) { s.px0 = p.x0; s.px1 = p.x1; s.psize = p.size; }
 
-// append one record to the task's slots (the spill list past kslot)
-__device__ __forceinline__ void emit(const SeedArgs &a, Lane &s, uint32_t pass)
+// append one record to the task's slots (the spill list past kslot).  PASS 3 / 4 (segment and fix-up
+// tasks of segmented pass 1) have no spill: a full task flags its read and is abandoned.  Records of
+// every pass-1 kind carry pass 1.
+template <int PASS>
+__device__ __forceinline__ void emit(const SeedArgs &a, Lane &s)
 {
 	SeedRec r;
 	r.x0 = s.px0; r.x1 = s.px1; r.size = s.psize;
 	r.info = (uint64_t)(uint32_t)s.x << 32 | (uint64_t)(uint32_t)s.i;
-	r.rid = s.rid; r.pass = pass;
+	r.rid = s.rid; r.pass = PASS == 2 ? 2 : 1;
 	if (s.nemit < (uint32_t)a.kslot) {
 		a.slots[(uint64_t)s.task * a.kslot + s.nemit] = r;
+	} else if (PASS == 3 || PASS == 4) {
+		atomicOr(&a.rflag[s.rid], 1u);
+		s.stage = kStageAbandon;
+		return;
 	} else {
 		const unsigned long long k = atomicAdd(a.n_spill, 1ULL);
 		if ((int64_t)k < a.spill_cap) a.spill[k] = r;
 	}
 	++s.nemit;
+}
+
+// segmented pass 1, fix-up: whether x is one of segment task j's chain values (recorded ascending)
+__device__ __forceinline__ bool in_chain(const SeedArgs &a, uint32_t j, uint32_t x)
+{
+	const uint32_t *c = a.chain + (uint64_t)j * a.chain_cap;
+	int lo = 0, hi = (int)a.chain_n[j];
+	while (lo < hi) {
+		const int mid = (lo + hi) >> 1;
+		if (c[mid] < x) lo = mid + 1;
+		else hi = mid;
+	}
+	return lo < (int)a.chain_n[j] && c[lo] == x;
+}
+
+// segmented pass 1, fix-up: with x the true chain's value, skip the segments whose lanes joined the
+// true chain (x is one of their chain values: their steps from x on are true, and x moves to their
+// handoff) and those the true chain passed; false when the read has no segment left
+__device__ __forceinline__ bool fix_advance(const SeedArgs &a, Lane &s)
+{
+	while (s.k < s.K) {
+		const uint32_t j = a.segbase[s.rid] + (uint32_t)s.k - 1;
+		const int32_t stop = s.k == s.K - 1 ? s.en : (s.k + 1) * a.seg_len;
+		if (s.x >= stop) { ++s.k; continue; }                        // conv[j] stays kNoConv
+		if (in_chain(a, j, (uint32_t)s.x)) { a.conv[j] = (uint32_t)s.x; s.x = (int32_t)a.hand[j]; ++s.k; continue; }
+		return true;
+	}
+	return false;
 }
 
 // the cache key of bases [from-len+1, from]: base `from` in the top two bits, base from-len+1 in the
@@ -109,12 +151,30 @@ __device__ __forceinline__ uint32_t kmer_at(const Lane &s, const uint32_t *qs, i
 	return (uint32_t)(pair >> sh) & (uint32_t)((1ULL << (2 * len)) - 1);
 }
 
+// the end of a segmented pass-1 task's chain: where the next task (or the fix-up) takes over
+template <int PASS>
+__device__ __forceinline__ void hand_off(const SeedArgs &a, const Lane &s, int32_t x)
+{
+	if (PASS == 1 && s.stop < s.en) a.hand0[s.rid] = (uint32_t)x;
+	if (PASS == 3) a.hand[s.task] = (uint32_t)x, a.chain_n[s.task] = s.nch;
+}
+
 // Cheap transitions until the lane needs memory (bwt.c:381-459 minus the rank/lookup steps).
-__device__ __forceinline__ int control(const Idx &ix, const SeedArgs &a, Lane &s, const uint32_t *qs, uint32_t pass)
+// Segmented pass 1 (PASS 1 for segment 0, 3 for the other segments, 4 for the fix-up) adds, at
+// every new start x: the stop at the segment's end, the chain record, and the fix-up's joins.
+template <int PASS>
+__device__ __forceinline__ int control(const Idx &ix, const SeedArgs &a, Lane &s, const uint32_t *qs)
 {
 	for (;;) {
+		if (s.stage == kStageAbandon) return ACT_DONE;
 		if (s.stage == 1) {
-			if (s.en - s.x < s.min_len) return ACT_DONE;
+			if (PASS == 4 && !fix_advance(a, s)) return ACT_DONE;
+			if ((PASS == 1 || PASS == 3) && s.x >= s.stop) { hand_off<PASS>(a, s, s.x); return ACT_DONE; }
+			if (PASS == 3) {
+				if (s.nch >= (uint32_t)a.chain_cap) { atomicOr(&a.rflag[s.rid], 1u); return ACT_DONE; }
+				a.chain[(uint64_t)s.task * a.chain_cap + s.nch++] = (uint32_t)s.x;
+			}
+			if (s.en - s.x < s.min_len) { hand_off<PASS>(a, s, s.en); return ACT_DONE; }
 			if (s.gq) {                                  // only global windows can hold N
 				int xn = -1;
 				for (int i = s.x; i < s.x + s.min_len; ++i)
@@ -136,9 +196,10 @@ __device__ __forceinline__ int control(const Idx &ix, const SeedArgs &a, Lane &s
 			if (s.i < s.x) { s.i = s.x + s.min_len; s.stage = 4; continue; }
 			return ACT_BACK;
 		} else if (s.stage == 4) {
-			if (s.i == s.en) { emit(a, s, pass); return ACT_DONE; }
+			if (s.i == s.en) { emit<PASS>(a, s); hand_off<PASS>(a, s, s.en); return ACT_DONE; }
 			if (qget(s, qs, s.i) < 4) return ACT_FWD;
-			emit(a, s, pass);                           // N: save, skip it, back to stage 1
+			emit<PASS>(a, s);                           // N: save, skip it, back to stage 1
+			if (s.stage == kStageAbandon) return ACT_DONE;
 			s.x = s.i + 1;
 			s.stage = 1;
 			continue;
@@ -193,11 +254,14 @@ __device__ __forceinline__ void stage_query(const SeedArgs &a, Lane &s, uint32_t
 	for (int w = 0; w < nw; ++w) qs[w * kSmemBlock + threadIdx.x] = src[w];
 }
 
+// PASS 1: pass 1 over the reads (task t = read t; segment 0 of a long read); 2: pass 2 over the
+// device-built SMEM tasks; 3: segments k >= 1 of long reads; 4: the fix-up of each long read (task t
+// = list[t]); 5: the reads whose segment or fix-up tasks overflowed, serially from segment 0's
+// handoff, every record to the spill list.
 template <int PASS>
 __global__ void __launch_bounds__(kSmemBlock) smem_kernel(Idx ix, SeedArgs a)
 {
 	__shared__ uint32_t qs[kQWords * kSmemBlock];
-	const uint32_t pass = PASS;
 	Lane s;
 	int act = ACT_DONE;
 	bool live = false;
@@ -221,11 +285,32 @@ __global__ void __launch_bounds__(kSmemBlock) smem_kernel(Idx ix, SeedArgs a)
 			}
 			int st;
 			int32_t rlen;
-			if (pass == 1) {
+			s.stop = 0x7fffffff;
+			if (PASS == 1) {
 				s.rid = t;
 				s.q = a.seq + a.off[t];
 				st = 0; s.en = rlen = a.len[t];
 				s.min_len = a.min_len; s.min_occ = 1;
+				if (a.seg_len > 0 && rlen >= 2 * a.seg_len) s.stop = a.seg_len;
+			} else if (PASS == 3) {
+				s.rid = a.seg_rid[t];
+				s.q = a.seq + a.off[s.rid];
+				s.en = rlen = a.len[s.rid];
+				s.min_len = a.min_len; s.min_occ = 1;
+				s.K = rlen / a.seg_len;
+				s.k = (int32_t)(t - a.segbase[s.rid]) + 1;
+				st = s.k * a.seg_len;
+				s.stop = s.k == s.K - 1 ? rlen : (s.k + 1) * a.seg_len;
+				s.nch = 0;
+				a.conv[t] = kNoConv;
+			} else if (PASS == 4 || PASS == 5) {
+				s.rid = a.list[t];
+				s.q = a.seq + a.off[s.rid];
+				s.en = rlen = a.len[s.rid];
+				s.min_len = a.min_len; s.min_occ = 1;
+				s.K = rlen / a.seg_len;
+				s.k = 1;
+				st = (int32_t)a.hand0[s.rid];
 			} else {
 				const SeedTask tk = a.tasks[t];
 				s.rid = tk.rid;
@@ -237,13 +322,13 @@ __global__ void __launch_bounds__(kSmemBlock) smem_kernel(Idx ix, SeedArgs a)
 			s.task = t;
 			s.nemit = 0;
 			s.x = st;
-			s.stage = 1;
+			s.stage = PASS == 4 && a.rflag[s.rid] ? kStageAbandon : 1;
 			live = true;
 			stage_query(a, s, qs, rlen);
 		}
-		act = control(ix, a, s, qs, pass);
+		act = control<PASS>(ix, a, s, qs);
 		if (act == ACT_DONE) {
-			if (live) a.slot_cnt[s.task] = s.nemit < (uint32_t)a.kslot ? s.nemit : (uint32_t)a.kslot;
+			if (live && a.slot_cnt) a.slot_cnt[s.task] = s.nemit < (uint32_t)a.kslot ? s.nemit : (uint32_t)a.kslot;
 			live = false;
 			continue;
 		}
@@ -272,7 +357,8 @@ __global__ void __launch_bounds__(kSmemBlock) smem_kernel(Idx ix, SeedArgs a)
 		} else if (o.size >= s.min_occ) {
 			set_p(s, o); ++s.i;
 		} else {
-			emit(a, s, pass);
+			emit<PASS>(a, s);
+			if (s.stage == kStageAbandon) continue;
 			if (ix.pre_len > 0 && s.i - s.x - 1 >= ix.pre_len) {
 				s.kmer = kmer_at(s, qs, s.i, ix.pre_len);
 				s.i -= ix.pre_len;
@@ -544,8 +630,111 @@ hipError_t launch_extend(const Idx &ix, int64_t n, const Sai *ik, const uint8_t 
 
 hipError_t launch_smem(const Idx &ix, const SeedArgs &a, uint32_t pass, unsigned grid, hipStream_t st)
 {
-	if (pass == 1) hipLaunchKernelGGL(smem_kernel<1>, dim3(grid), dim3(256), 0, st, ix, a);
-	else hipLaunchKernelGGL(smem_kernel<2>, dim3(grid), dim3(256), 0, st, ix, a);
+	switch (pass) {
+	case 1: hipLaunchKernelGGL(smem_kernel<1>, dim3(grid), dim3(256), 0, st, ix, a); break;
+	case 2: hipLaunchKernelGGL(smem_kernel<2>, dim3(grid), dim3(256), 0, st, ix, a); break;
+	case 3: hipLaunchKernelGGL(smem_kernel<3>, dim3(grid), dim3(256), 0, st, ix, a); break;
+	case 4: hipLaunchKernelGGL(smem_kernel<4>, dim3(grid), dim3(256), 0, st, ix, a); break;
+	case 5: hipLaunchKernelGGL(smem_kernel<5>, dim3(grid), dim3(256), 0, st, ix, a); break;
+	default: return hipErrorInvalidValue;
+	}
+	return hipGetLastError();
+}
+
+// ---- segmented pass 1: task lists and the merge ----------------------------------------------
+
+__global__ void __launch_bounds__(256) seg_count_kernel(const int32_t *len, int32_t n_reads, int32_t seg_len, uint32_t *nseg)
+{
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= n_reads) return;
+	const int32_t l = len[r];
+	nseg[r] = l >= 2 * seg_len ? (uint32_t)(l / seg_len - 1) : 0u;
+}
+
+// per long read: its segment tasks' read ids at segbase[r]..; the read into the long-read list (room
+// for every read).  A read whose segment tasks, or whose fix-up task, fall past seg_cap is flagged:
+// it gets the serial fallback, so a full pool costs time, never results.
+__global__ void __launch_bounds__(256) seg_build_kernel(const int32_t *len, int32_t n_reads, int32_t seg_len, const uint32_t *segbase,
+                                                       int64_t seg_cap, uint32_t *seg_rid, uint32_t *longl, uint32_t *n_long,
+                                                       uint32_t *rflag)
+{
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= n_reads) return;
+	const int32_t l = len[r];
+	if (l < 2 * seg_len) return;
+	const uint32_t b = segbase[r], n = (uint32_t)(l / seg_len - 1);
+	for (uint32_t k = 0; k < n; ++k)
+		if ((int64_t)(b + k) < seg_cap) seg_rid[b + k] = (uint32_t)r;
+	const uint32_t i = atomicAdd(n_long, 1u);
+	longl[i] = (uint32_t)r;
+	if ((int64_t)(b + n) > seg_cap || (int64_t)i >= seg_cap) rflag[r] = 1u;
+}
+
+__global__ void __launch_bounds__(256) seg_flagged_kernel(const uint32_t *longl, const uint32_t *n_long, const uint32_t *rflag,
+                                                         uint32_t *fbl, uint32_t *n_fb)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= *n_long) return;
+	const uint32_t r = longl[i];
+	if (rflag[r]) fbl[atomicAdd(n_fb, 1u)] = r;
+}
+
+// one wave per long read: segment k's records from where its lane joined the true chain, and the
+// fix-up's records, appended to the spill list (a flagged read's come from its fallback task instead)
+__global__ void __launch_bounds__(64) seg_merge_kernel(const int32_t *len, int32_t seg_len, const uint32_t *longl, const uint32_t *n_long,
+                                                      const uint32_t *rflag, const uint32_t *segbase, const SeedRec *segslots,
+                                                      const uint32_t *segcnt, int32_t kseg, const uint32_t *conv, const SeedRec *fixslots,
+                                                      const uint32_t *fixcnt, int32_t kfix, SeedRec *spill, unsigned long long *n_spill,
+                                                      int64_t spill_cap)
+{
+	const uint32_t i = blockIdx.x;                 // the grid is the fix-up cap: later entries are flagged
+	if (i >= *n_long) return;
+	const uint32_t r = longl[i];
+	if (rflag[r]) return;
+	const uint32_t b = segbase[r], n = (uint32_t)(len[r] / seg_len - 1);
+	for (uint32_t j = b; j < b + n; ++j)
+		for (uint32_t e = threadIdx.x; e < segcnt[j]; e += 64) {
+			const SeedRec rec = segslots[(uint64_t)j * kseg + e];
+			if ((uint32_t)(rec.info >> 32) < conv[j]) continue;
+			const unsigned long long k = atomicAdd(n_spill, 1ULL);
+			if ((int64_t)k < spill_cap) spill[k] = rec;
+		}
+	for (uint32_t e = threadIdx.x; e < fixcnt[i]; e += 64) {
+		const unsigned long long k = atomicAdd(n_spill, 1ULL);
+		if ((int64_t)k < spill_cap) spill[k] = fixslots[(uint64_t)i * kfix + e];
+	}
+}
+
+hipError_t launch_seg_count(const int32_t *len, int32_t n_reads, int32_t seg_len, uint32_t *nseg, hipStream_t st)
+{
+	if (n_reads <= 0) return hipSuccess;
+	hipLaunchKernelGGL(seg_count_kernel, dim3(blocks_for(n_reads, 256)), dim3(256), 0, st, len, n_reads, seg_len, nseg);
+	return hipGetLastError();
+}
+
+hipError_t launch_seg_build(const int32_t *len, int32_t n_reads, int32_t seg_len, const uint32_t *segbase, int64_t seg_cap,
+                            uint32_t *seg_rid, uint32_t *longl, uint32_t *n_long, uint32_t *rflag, hipStream_t st)
+{
+	if (n_reads <= 0) return hipSuccess;
+	hipLaunchKernelGGL(seg_build_kernel, dim3(blocks_for(n_reads, 256)), dim3(256), 0, st, len, n_reads, seg_len, segbase, seg_cap,
+	                   seg_rid, longl, n_long, rflag);
+	return hipGetLastError();
+}
+
+hipError_t launch_seg_flagged(const uint32_t *longl, const uint32_t *n_long, const uint32_t *rflag, uint32_t *fbl, uint32_t *n_fb,
+                              int64_t cap, hipStream_t st)
+{
+	hipLaunchKernelGGL(seg_flagged_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, longl, n_long, rflag, fbl, n_fb);
+	return hipGetLastError();
+}
+
+hipError_t launch_seg_merge(const int32_t *len, int32_t seg_len, const uint32_t *longl, const uint32_t *n_long, const uint32_t *rflag,
+                            const uint32_t *segbase, const SeedRec *segslots, const uint32_t *segcnt, int32_t kseg,
+                            const uint32_t *conv, const SeedRec *fixslots, const uint32_t *fixcnt, int32_t kfix, SeedRec *spill,
+                            unsigned long long *n_spill, int64_t spill_cap, int64_t cap, hipStream_t st)
+{
+	hipLaunchKernelGGL(seg_merge_kernel, dim3((unsigned)cap), dim3(64), 0, st, len, seg_len, longl, n_long, rflag, segbase, segslots,
+	                   segcnt, kseg, conv, fixslots, fixcnt, kfix, spill, n_spill, spill_cap);
 	return hipGetLastError();
 }
 

@@ -29,6 +29,22 @@ struct SeedArgs {
 	unsigned long long *n_spill;
 	int64_t spill_cap;
 	unsigned long long *stats;  // optional: [0] occ blocks fetched, [1] k-mer cache lookups, [2] extend steps
+	// segmented pass 1 for reads of at least 2*seg_len bases (seg_len 0: off).  Read r's segments are
+	// [k*seg_len, (k+1)*seg_len), the last one running to the read's end (K = len / seg_len of them).
+	// Segment 0 is the read's own pass-1 task; segment k >= 1 is task segbase[r] + k - 1 of the
+	// segment launch.
+	int32_t seg_len;
+	const uint32_t *segbase;    // per read: its first segment task (exclusive scan of K - 1)
+	const uint32_t *seg_rid;    // per segment task: its read (segment k found from segbase)
+	uint32_t *hand0;            // per read: segment 0's handoff (first chain value >= seg_len)
+	uint32_t *hand;             // per segment task: its handoff (the read's end when its chain ends)
+	uint32_t *chain;            // per segment task: its chain values, chain_cap each
+	uint32_t *chain_n;
+	int32_t chain_cap;
+	uint32_t *conv;             // per segment task: where it joined the true chain (0xffffffff: never)
+	uint32_t *rflag;            // per read: 1 when a segment or fix-up task overflowed (the read falls back)
+	const uint32_t *list;       // fix-up and fallback launches: their reads
+	const uint32_t *n_list;     // their count (on the device)
 };
 
 constexpr int kPackWords = 24;          // 2-bit words per packed read: reads up to 384 bases
@@ -240,6 +256,18 @@ hipError_t launch_sa_fine(const Idx &ix, uint64_t *fine, uint64_t n_fine, hipStr
 hipError_t launch_rank2a(const Idx &ix, int64_t n, const uint64_t *k, const uint64_t *l, uint64_t *ck, uint64_t *cl, hipStream_t st);
 hipError_t launch_extend(const Idx &ix, int64_t n, const Sai *ik, const uint8_t *is_back, Sai *ok, hipStream_t st);
 hipError_t launch_smem(const Idx &ix, const SeedArgs &a, uint32_t pass, unsigned grid, hipStream_t st);
+// segmented pass 1: segment counts per read (long[r] = K - 1), the segment tasks and the long-read list
+hipError_t launch_seg_count(const int32_t *len, int32_t n_reads, int32_t seg_len, uint32_t *nseg, hipStream_t st);
+hipError_t launch_seg_build(const int32_t *len, int32_t n_reads, int32_t seg_len, const uint32_t *segbase, int64_t seg_cap,
+                            uint32_t *seg_rid, uint32_t *longl, uint32_t *n_long, uint32_t *rflag, hipStream_t st);
+// after the segment and fix-up launches: the flagged reads (fallback list) and the valid records of
+// the others appended to the spill list
+hipError_t launch_seg_flagged(const uint32_t *longl, const uint32_t *n_long, const uint32_t *rflag, uint32_t *fbl, uint32_t *n_fb,
+                              int64_t cap, hipStream_t st);
+hipError_t launch_seg_merge(const int32_t *len, int32_t seg_len, const uint32_t *longl, const uint32_t *n_long, const uint32_t *rflag,
+                            const uint32_t *segbase, const SeedRec *segslots, const uint32_t *segcnt, int32_t kseg,
+                            const uint32_t *conv, const SeedRec *fixslots, const uint32_t *fixcnt, int32_t kfix, SeedRec *spill,
+                            unsigned long long *n_spill, int64_t spill_cap, int64_t cap, hipStream_t st);
 hipError_t launch_subtask(const SeedRec *slots, const uint32_t *slot_cnt, int32_t kslot, int32_t n_reads, const SeedRec *spill,
                           const unsigned long long *n_spill, int64_t spill_cap, int32_t min_len, int32_t max_sub_occ,
                           SeedTask *tasks, uint32_t *n_tasks, int64_t task_cap, hipStream_t st);

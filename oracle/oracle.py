@@ -105,6 +105,8 @@ def _load_oracle():
     L.orc_extend.argtypes = [C.POINTER(OrcBwt), C.POINTER(OrcSai), C.POINTER(OrcSai), C.c_int]
     L.orc_seed_intv.restype = C.c_int64
     L.orc_seed_intv.argtypes = [C.POINTER(OrcBwt), C.c_int32, _u8p, C.c_int32, C.c_int32, C.POINTER(OrcSai), C.c_int64]
+    L.orc_smem.restype = C.c_int64
+    L.orc_smem.argtypes = [C.POINTER(OrcBwt), C.c_uint32, _u8p, C.c_int64, C.c_int64, C.c_int64, C.POINTER(OrcSai)]
     L.orc_sa.restype = C.c_uint64
     L.orc_sa.argtypes = [C.POINTER(OrcBwt), C.c_uint64]
     L.orc_sa_batch.argtypes = [C.POINTER(OrcBwt), C.c_int64, _u64p]
@@ -193,6 +195,12 @@ class OracleIndex:
         assert n <= cap
         out = np.frombuffer(buf, dtype=np.uint64).reshape(cap, 4)[:n].copy()
         return out
+
+    def smem_step(self, seq: np.ndarray, x: int, min_len=19, min_occ=1, length=None):
+        """One mb_bwt_smem call (bwt.c:280-305) from position x -> (next x, (x0, x1, size, info) or None)."""
+        p = OrcSai()
+        nx = self.L.orc_smem(self.b, len(seq) if length is None else length, seq, int(x), int(min_len), int(min_occ), C.byref(p))
+        return int(nx), ((p.x0, p.x1, p.size, p.info) if p.size > 0 else None)
 
     def sa(self, rows):
         x = np.ascontiguousarray(rows, np.uint64).copy()

@@ -179,3 +179,56 @@ def test_seed_cfg0_cache14(gpu_ctx):
     r = ri.seed_batch(reads)
     for i in range(len(reads)):
         assert np.array_equal(_canon(g[i]), _canon(r[i])), i
+
+
+def _long_reads(ref, rng, n):
+    """Reads of 1-16 kb from chrM (either strand) at 0.1-15% substitutions, some with N runs and some
+    with an unrelated stretch, where the segments' chains join the true chain late or not at all."""
+    out = []
+    for it in range(n):
+        L = int(rng.integers(1024, min(16000, len(ref) - 1)))
+        p = int(rng.integers(0, len(ref) - L))
+        q = ref[p:p + L].copy()
+        if it % 2:
+            q = q[::-1].copy()
+            q = np.where(q < 4, 3 - q, q).astype(np.uint8)
+        sub = rng.random(L) < rng.choice([0.001, 0.005, 0.02, 0.08, 0.15])
+        q[sub] = rng.integers(0, 4, int(sub.sum()))
+        if it % 4 == 0:
+            a = int(rng.integers(0, L - 50))
+            q[a:a + int(rng.integers(1, 50))] = 4
+        if it % 3 == 0:
+            a = int(rng.integers(0, L - 700))
+            q[a:a + 700] = rng.integers(0, 4, 700)
+        out.append(q.astype(np.uint8))
+    return out
+
+
+@pytest.mark.parametrize("seg", ["512", "100,3,8,2", "64,1,2,1", "0"])
+def test_seed_long_reads_segmented(chrm_prefix, ref_chrm, chrm_reads, seg):
+    """Long reads: pass 1 splits each read of at least 2*len bases into segments searched in
+    parallel, joined to the true chain by a fix-up, with a serial fallback for reads whose segment
+    or fix-up tasks run out of room (MBG_SEED_SEG=len,segment slots,chain cap,fix-up slots; "0"
+    is the unsegmented search).  Tiny capacities force the fallback on most reads.  Every read's
+    intervals must equal the reference's mb_seed_intv_batch, mixed with short reads in one batch."""
+    from tests.conftest import read_fasta, nt4
+    from minibwa_amd import mbg
+    _, seqs = read_fasta(os.path.join(GOLDEN, "chrM-human.fa.gz"))
+    ref = nt4(seqs[0])
+    reads = _long_reads(ref, np.random.default_rng(41), 120) + list(chrm_reads[:200])
+    old = os.environ.get("MBG_SEED_SEG")
+    os.environ["MBG_SEED_SEG"] = seg
+    try:
+        with mbg.Context(0) as c:
+            c.idx_load(chrm_prefix, 10)
+            for min_len, mso in ((19, 10), (11, 4)):
+                print("[seed-long] seg %s min_len %d" % (seg, min_len), flush=True)
+                g = c.seed_intv_batch(reads, min_len, mso)
+                r = ref_chrm.seed_batch(reads, min_len, mso)
+                for i in range(len(reads)):
+                    assert np.array_equal(_canon(g[i]), _canon(r[i])), (seg, min_len, i, len(reads[i]))
+    finally:
+        if old is None:
+            del os.environ["MBG_SEED_SEG"]
+        else:
+            os.environ["MBG_SEED_SEG"] = old
