@@ -50,9 +50,9 @@ struct mbg_ctx_s {
 	// segmented pass 1 of long reads: segment counts / first tasks per read, segment tasks' reads,
 	// long-read list, handoffs, chains, joins, flags, segment and fix-up slots, fallback list, counters
 	DBuf g_nseg, g_base, g_rid, g_long, g_hand0, g_hand, g_chain, g_chn, g_conv, g_flag, g_slots, g_cnt, g_fix, g_fixcnt, g_fbl, g_ctr;
-	// off unless MBG_SEED_SEG sets a length: on high-identity long reads the pass is bound by the
-	// longest exact stretch, which every segment lane inside it walks again
-	int32_t seg_len = 0, seg_slots = 16, seg_chain = 64, fix_slots = 16;
+	// 512 unless MBG_SEED_SEG says otherwise (0: off).  On high-identity long reads the pass stays
+	// bound by the longest exact stretch, which the lane holding it walks serially either way
+	int32_t seg_len = 512, seg_slots = 16, seg_chain = 64, fix_slots = 16;
 	int64_t cap_seg = 0;
 	DBuf m_buf[12];         // mate rescue
 	DBuf a_buf[8];          // anchors
@@ -213,8 +213,8 @@ int mbg_init(int device, mbg_ctx_t **out)
 	mbg_ctx_t *c = new mbg_ctx_t();
 	c->device = device;
 	{   // MBG_SEED_SEG=len[,segment slots[,chain cap[,fix-up slots]]]: pass 1 splits reads of at least 2*len
-	    // bases into segments searched in parallel (unset or 0: off); the capacities only decide how
-	    // often a read falls back to a serial search, never the result
+	    // bases into segments searched in parallel (0: off); the capacities only decide how often a
+	    // read falls back to a serial search, never the result
 		const char *e = getenv("MBG_SEED_SEG");
 		if (e) {
 			int v[4] = {c->seg_len, c->seg_slots, c->seg_chain, c->fix_slots};

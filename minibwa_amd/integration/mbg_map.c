@@ -177,6 +177,7 @@ typedef struct {
 	const int32_t *pair_off;         /* pair k: reads pair_off[k], pair_off[k] + 1 */
 	const mb_pestat_t *pes;
 	double t_front;
+	int front_done;               /* the front pass has run (set by the front thread, under g_front.mu) */
 } batch_t;
 
 /* per coroutine: the read being mapped and the anchors handed to its chaining call */
@@ -449,6 +450,68 @@ static void die_rc(const char *what, int rc)
 	abort();
 }
 
+/* The CLI's front passes run on a thread of their own: the read step queues its mini-batch and
+ * returns, so that reading the next mini-batch, this one's front pass (mostly device time) and the
+ * previous one's mapping overlap; the map step waits for its batch's front.  One front at a time, in
+ * mini-batch order, each on its own front slot (kt_pipeline holds at most three mini-batches, and a
+ * slot is reused only after its previous batch has been written).  MBG_FRONT_THREAD=0 runs the front
+ * inside the read step instead. */
+#define FRONT_Q 8
+static struct {
+	pthread_t th;
+	pthread_mutex_t mu;
+	pthread_cond_t cv_q, cv_done;
+	int started, stop, n, head;
+	struct { batch_t *b; int slot; const uint8_t *nt4; const int64_t *off; } q[FRONT_Q];
+} g_front = {.mu = PTHREAD_MUTEX_INITIALIZER, .cv_q = PTHREAD_COND_INITIALIZER, .cv_done = PTHREAD_COND_INITIALIZER};
+
+static void *front_main(void *arg)
+{
+	(void)arg;
+	for (;;) {
+		batch_t *b;
+		int slot;
+		const uint8_t *nt4;
+		const int64_t *off;
+		pthread_mutex_lock(&g_front.mu);
+		while (g_front.n == 0 && !g_front.stop) pthread_cond_wait(&g_front.cv_q, &g_front.mu);
+		if (g_front.n == 0) {
+			pthread_mutex_unlock(&g_front.mu);
+			break;
+		}
+		b = g_front.q[g_front.head].b, slot = g_front.q[g_front.head].slot;
+		nt4 = g_front.q[g_front.head].nt4, off = g_front.q[g_front.head].off;
+		g_front.head = (g_front.head + 1) % FRONT_Q, --g_front.n;
+		pthread_mutex_unlock(&g_front.mu);
+		front_reads(b, slot, nt4, off);
+		pthread_mutex_lock(&g_front.mu);
+		b->front_done = 1;
+		pthread_cond_broadcast(&g_front.cv_done);
+		pthread_mutex_unlock(&g_front.mu);
+	}
+	return 0;
+}
+
+static void front_queue(batch_t *b, int slot, const uint8_t *nt4, const int64_t *off)
+{
+	pthread_mutex_lock(&g_front.mu);
+	if (g_front.n == FRONT_Q) die("front queue full");
+	{
+		const int k = (g_front.head + g_front.n) % FRONT_Q;
+		g_front.q[k].b = b, g_front.q[k].slot = slot, g_front.q[k].nt4 = nt4, g_front.q[k].off = off;
+	}
+	++g_front.n;
+	pthread_cond_signal(&g_front.cv_q);
+	pthread_mutex_unlock(&g_front.mu);
+}
+
+static void front_wait(batch_t *b)
+{
+	pthread_mutex_lock(&g_front.mu);
+	while (!b->front_done) pthread_cond_wait(&g_front.cv_done, &g_front.mu);
+	pthread_mutex_unlock(&g_front.mu);
+}
+
 /* step 0 (map-main.c:114-173): a mini-batch of reads, then fragments (mates sharing a name) and the
  * trimming of "/1" "/2" name suffixes */
 static pbatch_t *read_step(const ref_pipeline_t *p)
@@ -553,7 +616,11 @@ static pbatch_t *read_step(const ref_pipeline_t *p)
 		memset(b, 0, sizeof(*b));
 		b->opt = opt, b->idx = idx, b->n_seq = s->n_seq, b->mt = s->mt, b->qlen = s->qlen, b->seq = s->sq, b->name = s->nm;
 		b->n_hit = s->n_hit, b->hit = s->hit;
-		front_reads(b, (int)(g_pipe.n_batch++ % MBX_FRONT_SLOTS), s->ch.nt4, s->ch.seq_off);
+		if (g_front.started) front_queue(b, (int)(g_pipe.n_batch++ % MBX_FRONT_SLOTS), s->ch.nt4, s->ch.seq_off);
+		else {
+			front_reads(b, (int)(g_pipe.n_batch++ % MBX_FRONT_SLOTS), s->ch.nt4, s->ch.seq_off);
+			b->front_done = 1;
+		}
 	}
 	s->t_read = now_s() - t0;
 	return s;
@@ -621,6 +688,7 @@ static void *map_step(pbatch_t *s)
 	batch_t *b = &s->b;
 	int32_t k, *pair_off = 0, n_pair = 0;
 	double t0 = now_s();
+	front_wait(b);
 	se_reads(b, opt->n_thread);
 	if ((opt->flag & MB_F_PE) && s->n_frag < s->n_seq && !(opt->flag & (MB_F_NO_PAIRING | MB_F_NO_ALN))) { /* map-main.c:180-192 */
 		if ((opt->flag & MB_F_PE_PREDEF) || s->n_pe < 20) {
@@ -706,7 +774,23 @@ void __wrap_kt_pipeline(int n_threads, void *(*func)(void*, int, void*), void *s
 		g_fmt_check = e && atoi(e) > 0;
 	}
 	g_pipe.p = p, g_pipe.n_fq = 0, g_pipe.n_seq = p->n_seq, g_pipe.n_batch = 0;
+	if (!g_front.started) {
+		const char *e = getenv("MBG_FRONT_THREAD");
+		if (!(e && atoi(e) == 0)) {
+			g_front.stop = 0;
+			if (pthread_create(&g_front.th, 0, front_main, 0) != 0) die("pthread_create failed (front thread)");
+			g_front.started = 1;
+		}
+	}
 	__real_kt_pipeline(n_threads, pipe_step, 0, n_steps);
+	if (g_front.started) {        /* every queued front has been waited for by its map step */
+		pthread_mutex_lock(&g_front.mu);
+		g_front.stop = 1;
+		pthread_cond_signal(&g_front.cv_q);
+		pthread_mutex_unlock(&g_front.mu);
+		pthread_join(g_front.th, 0);
+		g_front.started = 0;
+	}
 	for (i = 0; i < g_pipe.n_fq; ++i) mbg_fq_close(g_pipe.fq[i]);
 	g_pipe.n_fq = 0;
 }
