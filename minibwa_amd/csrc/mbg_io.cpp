@@ -5,12 +5,20 @@
 //
 //   read-ahead thread   gzread (plain files pass through zlib unchanged) into 1 MiB blocks, up to
 //                       kAhead blocks ahead of the parser, so decompression overlaps parsing
-//   parser              kseq's state machine over the blocks: a header starts at '>' or '@'; the name
-//                       ends at any isspace() byte and the comment is the rest of that line; sequence
-//                       lines follow (blank lines skipped) until a line starts with '>', '@' or '+';
-//                       after '+', quality lines are read until the quality is as long as the
-//                       sequence; a field read up to a line end loses one trailing '\r'
-//   encoder             U/u -> T/t and nt4 codes over the chunk's records, split over threads
+//   parse-ahead thread  kseq's state machine over the blocks, one thread per file, into batches of
+//                       parsed records kept up to kAheadBytes ahead of the caller: a header starts at
+//                       '>' or '@'; the name ends at any isspace() byte and the comment is the rest of
+//                       that line; sequence lines follow (blank lines skipped) until a line starts with
+//                       '>', '@' or '+'; after '+', quality lines are read until the quality is as
+//                       long as the sequence; a field read up to a line end loses one trailing '\r'
+//   chunking            the caller's thread takes parsed records one at a time under bseq.c's rules
+//                       (only lengths and names are looked at), so that what a chunk holds and where
+//                       the next one starts are exactly the serial reader's
+//   assembly            the chunk's records copied into its flat arrays, with U/u -> T/t and the nt4
+//                       codes, split over threads by bytes
+//
+// Parsing a file never depends on the chunking parameters of the calls that consume it, which is what
+// lets it run ahead: the records, in file order, are kseq_read's whatever the chunks are.
 //
 // Host code only; no GPU call.
 #include <zlib.h>
@@ -20,9 +28,11 @@
 #include <cstdint>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "mbg.h"
@@ -31,6 +41,8 @@ namespace {
 
 constexpr size_t kBlock = 1 << 20;
 constexpr int kAhead = 4;
+constexpr size_t kBatchRecs = 16384;           // parsed records per batch
+constexpr size_t kAheadBytes = (size_t)768 << 20;   // parsed bytes a file may run ahead of its caller
 constexpr uint64_t kPairCheck = 1000000;      // frag mode pairs reads shorter than this (bseq.c:29)
 constexpr uint64_t kKseqBuf = 16384;          // kseq's read size (kseq.h:242): it decides one end-of-input case
 
@@ -45,6 +57,18 @@ const uint8_t *nt4_table()
 	});
 	return t;
 }
+
+// an allocator whose resize() leaves new elements uninitialised: the chunk arrays are sized first and
+// then written in parallel, so zero-filling them would be a serial pass over every byte
+template <class T>
+struct NoInit : std::allocator<T> {
+	template <class U> struct rebind { using other = NoInit<U>; };
+	NoInit() = default;
+	template <class U> NoInit(const NoInit<U> &) {}
+	template <class U> void construct(U *p) noexcept { ::new ((void*)p) U; }
+	template <class U, class... A> void construct(U *p, A &&...a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+};
+using Bytes = std::vector<char, NoInit<char>>;
 
 // blocks of the decompressed stream, produced by a reader thread
 class Source {
@@ -118,7 +142,7 @@ public:
 	// v[st ..); sep_line: '\n', and one trailing '\r' of the whole field dropped; else any isspace()
 	// byte.  Returns -1 when the stream was already at its end (the field untouched apart from a reset
 	// when !append), else the field's length.
-	int64_t until(bool sep_line, std::vector<char> &v, size_t st, int *dret, bool append)
+	int64_t until(bool sep_line, Bytes &v, size_t st, int *dret, bool append)
 	{
 		if (dret) *dret = 0;
 		if (!append) v.resize(st);
@@ -174,9 +198,10 @@ private:
 	bool eof_ = false, tried_ = false;
 };
 
+
 // the four fields of records, each appended at the end of its own array
 struct Rec {
-	std::vector<char> name, comment, seq, qual;
+	Bytes name, comment, seq, qual;
 	void clear() { name.clear(), comment.clear(), seq.clear(), qual.clear(); }
 };
 
@@ -260,88 +285,176 @@ bool qname_same(const char *a, size_t la, const char *b, size_t lb)   // mb_qnam
 	return la == lb && memcmp(a, b, la) == 0;
 }
 
+// parsed records of one file, in file order: record i's fields are f.name[no[i] .. no[i+1]) and so on,
+// and ret[i] is what kseq_read returned for it (>= 0 its length, -1 the end, -2 a bad quality; the
+// fields of a -2 record are what was read of it)
+struct Parsed {
+	Rec f;
+	std::vector<int64_t> no{0}, co{0}, so{0}, qo{0}, ret;
+	size_t n() const { return ret.size(); }
+	size_t bytes() const { return f.name.size() + f.comment.size() + f.seq.size() + f.qual.size() + 40 * ret.size(); }
+};
+
+// the parse-ahead thread of one file, and the caller's position in what it parsed
+class Ahead {
+public:
+	explicit Ahead(Parser *ps) : ps_(ps), th_([this] { run(); }) {}
+	~Ahead()
+	{
+		{
+			std::lock_guard<std::mutex> lk(mu_);
+			stop_ = true;
+		}
+		cv_.notify_all();
+		th_.join();
+	}
+	// the next record (after the end: the end again)
+	void next(std::shared_ptr<Parsed> *b, size_t *i)
+	{
+		if (!cur_ || pos_ >= cur_->n()) {
+			if (cur_ && cur_->ret.back() == -1) {           // the end record, every time from now on
+				*b = cur_, *i = cur_->n() - 1;
+				return;
+			}
+			std::unique_lock<std::mutex> lk(mu_);
+			cv_.wait(lk, [this] { return !q_.empty(); });
+			cur_ = std::move(q_.front());
+			q_.pop_front();
+			bytes_ -= cur_->bytes();
+			cv_.notify_all();
+			pos_ = 0;
+		}
+		*b = cur_, *i = pos_++;
+	}
+
+private:
+	void run()
+	{
+		for (bool end = false; !end;) {
+			auto b = std::make_shared<Parsed>();
+			while (b->n() < kBatchRecs) {
+				const int64_t r = ps_->read(b->f);
+				b->ret.push_back(r);
+				b->no.push_back((int64_t)b->f.name.size()), b->co.push_back((int64_t)b->f.comment.size());
+				b->so.push_back((int64_t)b->f.seq.size()), b->qo.push_back((int64_t)b->f.qual.size());
+				if (r == -1) { end = true; break; }
+			}
+			std::unique_lock<std::mutex> lk(mu_);
+			cv_.wait(lk, [this] { return bytes_ < kAheadBytes || stop_; });
+			if (stop_) return;
+			bytes_ += b->bytes();
+			q_.push_back(std::move(b));
+			cv_.notify_all();
+		}
+	}
+	Parser *ps_;
+	std::mutex mu_;
+	std::condition_variable cv_;
+	std::deque<std::shared_ptr<Parsed>> q_;
+	size_t bytes_ = 0;
+	bool stop_ = false;
+	std::shared_ptr<Parsed> cur_;
+	size_t pos_ = 0;
+	std::thread th_;
+};
+
+// one record of a chunk: where it was parsed, and the flags of the call that took it (kseq2bseq,
+// bseq.c:64-78: the quality kept when asked for and non-empty, the comment likewise)
+struct Take {
+	std::shared_ptr<Parsed> b;
+	size_t i;
+	bool q, c;
+	int64_t len() const { return b->ret[i]; }
+	const char *name() const { return b->f.name.data() + b->no[i]; }
+	size_t name_len() const { return (size_t)(b->no[i + 1] - b->no[i]); }
+};
+
 } // namespace
 
 struct mbg_fq_s {
 	Source *src = nullptr;
 	Stream *st = nullptr;
 	Parser *ps = nullptr;
+	Ahead *ah = nullptr;
 	int32_t n_threads = 1;
-	bool have_pending = false;                  // the record frag mode read past the chunk (bseq.c:113)
-	int pend_q = 0, pend_c = 0;                 // ... converted with the flags of the call that read it
-	Rec pending;
+	bool have_pending = false;                  // the record frag mode took past the chunk (bseq.c:113)
+	Take pending;                               // ... kept with the flags of the call that took it
 };
 
-// A chunk: committed records, then possibly one record parsed past them (its fields at the ends of
-// the arrays).  qual runs parallel to seq: a record without a kept quality holds zeros there.
+// A chunk's flat arrays.  qual runs parallel to seq: a record without a kept quality holds zeros there.
 struct mbg_fq_buf_s {
 	std::vector<int64_t> name_off, com_off, seq_off;
 	Rec f;
-	std::vector<uint8_t> has_qual, nt4;
-	void clear()
-	{
-		name_off.assign(1, 0), com_off.assign(1, 0), seq_off.assign(1, 0);
-		f.clear(), has_qual.clear(), nt4.clear();
-	}
-	int32_t n() const { return (int32_t)has_qual.size(); }
-	// kseq2bseq (bseq.c:64-78) of the record parsed past the committed ones: the quality kept when
-	// asked for and non-empty, the comment likewise
-	void commit(int with_qual, int with_comment)
-	{
-		const size_t s0 = (size_t)seq_off.back(), sl = f.seq.size() - s0;
-		if (!(with_comment && f.comment.size() > (size_t)com_off.back())) f.comment.resize((size_t)com_off.back());
-		const bool q = with_qual && f.qual.size() > s0;
-		if (!q) f.qual.assign(f.qual.begin(), f.qual.begin() + (ptrdiff_t)s0), f.qual.resize(s0 + sl, 0);
-		has_qual.push_back(q ? 1 : 0);
-		name_off.push_back((int64_t)f.name.size()), com_off.push_back((int64_t)f.comment.size());
-		seq_off.push_back((int64_t)f.seq.size());
-	}
-	void drop()                                 // forget the record parsed past the committed ones
-	{
-		f.name.resize((size_t)name_off.back()), f.comment.resize((size_t)com_off.back());
-		f.seq.resize((size_t)seq_off.back()), f.qual.resize((size_t)seq_off.back());
-	}
-	void take(Rec &r) const                     // a copy of the record parsed past the committed ones
-	{
-		r.name.assign(f.name.begin() + name_off.back(), f.name.end());
-		r.comment.assign(f.comment.begin() + com_off.back(), f.comment.end());
-		r.seq.assign(f.seq.begin() + seq_off.back(), f.seq.end());
-		r.qual.assign(f.qual.begin() + seq_off.back(), f.qual.end());
-	}
-	void put(const Rec &r)                      // r as the record parsed past the committed ones
-	{
-		f.name.insert(f.name.end(), r.name.begin(), r.name.end());
-		f.comment.insert(f.comment.end(), r.comment.begin(), r.comment.end());
-		f.seq.insert(f.seq.end(), r.seq.begin(), r.seq.end());
-		f.qual.insert(f.qual.end(), r.qual.begin(), r.qual.end());
-	}
+	std::vector<uint8_t> has_qual;
+	std::vector<uint8_t, NoInit<uint8_t>> nt4;
 };
 
 namespace {
 
-// U -> T and the nt4 codes of every record, split over threads
-void encode(mbg_fq_buf_s &b, int n_threads)
+// the records of a chunk into b's arrays: offsets first (serial, a few adds a record), then the bytes,
+// split over threads by bytes; U/u -> T/t and the nt4 codes on the way
+void assemble(mbg_fq_buf_s &b, const std::vector<Take> &t, int n_threads)
 {
-	const uint8_t *t = nt4_table();
-	const int64_t n = (int64_t)b.f.seq.size();
-	b.nt4.resize((size_t)n);
-	auto work = [&](int64_t lo, int64_t hi) {
-		for (int64_t i = lo; i < hi; ++i) {
-			char &c = b.f.seq[(size_t)i];
-			if (c == 'u' || c == 'U') --c;
-			b.nt4[(size_t)i] = t[(uint8_t)c];
+	const size_t n = t.size();
+	b.name_off.resize(n + 1), b.com_off.resize(n + 1), b.seq_off.resize(n + 1), b.has_qual.resize(n);
+	b.name_off[0] = b.com_off[0] = b.seq_off[0] = 0;
+	for (size_t k = 0; k < n; ++k) {
+		const Parsed &p = *t[k].b;
+		const size_t i = t[k].i;
+		const int64_t sl = p.so[i + 1] - p.so[i], cl = p.co[i + 1] - p.co[i], ql = p.qo[i + 1] - p.qo[i];
+		b.name_off[k + 1] = b.name_off[k] + (p.no[i + 1] - p.no[i]);
+		b.com_off[k + 1] = b.com_off[k] + (t[k].c && cl > 0 ? cl : 0);
+		b.seq_off[k + 1] = b.seq_off[k] + sl;
+		b.has_qual[k] = t[k].q && ql > 0;
+	}
+	b.f.name.resize((size_t)b.name_off[n]), b.f.comment.resize((size_t)b.com_off[n]);
+	b.f.seq.resize((size_t)b.seq_off[n]), b.f.qual.resize((size_t)b.seq_off[n]), b.nt4.resize((size_t)b.seq_off[n]);
+	const uint8_t *tab = nt4_table();
+	auto work = [&](size_t lo, size_t hi) {
+		for (size_t k = lo; k < hi; ++k) {
+			const Parsed &p = *t[k].b;
+			const size_t i = t[k].i;
+			const int64_t sl = b.seq_off[k + 1] - b.seq_off[k];
+			memcpy(&b.f.name[(size_t)b.name_off[k]], &p.f.name[(size_t)p.no[i]], (size_t)(b.name_off[k + 1] - b.name_off[k]));
+			if (b.com_off[k + 1] > b.com_off[k])
+				memcpy(&b.f.comment[(size_t)b.com_off[k]], &p.f.comment[(size_t)p.co[i]], (size_t)(b.com_off[k + 1] - b.com_off[k]));
+			char *sq = &b.f.seq[(size_t)b.seq_off[k]];
+			uint8_t *e = &b.nt4[(size_t)b.seq_off[k]];
+			const char *src = &p.f.seq[(size_t)p.so[i]];
+			for (int64_t l = 0; l < sl; ++l) {
+				char c = src[l];
+				if (c == 'u' || c == 'U') --c;
+				sq[l] = c, e[l] = tab[(uint8_t)c];
+			}
+			if (b.has_qual[k]) memcpy(&b.f.qual[(size_t)b.seq_off[k]], &p.f.qual[(size_t)p.qo[i]], (size_t)sl);
+			else if (sl > 0) memset(&b.f.qual[(size_t)b.seq_off[k]], 0, (size_t)sl);
 		}
 	};
-	const int nt = n < (1 << 20) ? 1 : std::max(1, n_threads);
+	const int64_t tot = b.seq_off[n] + b.name_off[n];
+	const int nt = tot < (1 << 20) ? 1 : std::max(1, n_threads);
 	if (nt == 1) { work(0, n); return; }
 	std::vector<std::thread> th;
-	for (int k = 0; k < nt; ++k) th.emplace_back(work, n * k / nt, n * (k + 1) / nt);
+	size_t lo = 0;
+	for (int k = 1; k <= nt && lo < n; ++k) {                      // ranges of about tot/nt bytes
+		size_t hi = n;
+		if (k < nt) {
+			const int64_t goal = tot * k / nt;
+			size_t a = lo, z = n;                                  // the first record past goal bytes
+			while (a < z) {
+				const size_t m = (a + z) / 2;
+				if (b.seq_off[m + 1] + b.name_off[m + 1] > goal) z = m; else a = m + 1;
+			}
+			hi = std::max(lo, a);
+		}
+		th.emplace_back(work, lo, hi);
+		lo = hi;
+	}
 	for (auto &x : th) x.join();
 }
 
 void view(const mbg_fq_buf_s &b, int32_t status, mbg_fq_chunk_t *out)
 {
-	out->n = b.n();
+	out->n = (int32_t)b.has_qual.size();
 	out->status = status;
 	out->name_off = b.name_off.data(), out->com_off = b.com_off.data(), out->seq_off = b.seq_off.data();
 	out->name = b.f.name.data(), out->comment = b.f.comment.data(), out->seq = b.f.seq.data(), out->qual = b.f.qual.data();
@@ -363,6 +476,7 @@ int mbg_fq_open(const char *path, int32_t n_threads, mbg_fq_t **out)
 	q->src = new Source(f);
 	q->st = new Stream(q->src);
 	q->ps = new Parser(q->st);
+	q->ah = new Ahead(q->ps);
 	q->n_threads = n_threads < 1 ? 1 : n_threads > 64 ? 64 : n_threads;
 	*out = q;
 	return MBG_OK;
@@ -371,6 +485,7 @@ int mbg_fq_open(const char *path, int32_t n_threads, mbg_fq_t **out)
 void mbg_fq_close(mbg_fq_t *q)
 {
 	if (q == nullptr) return;
+	delete q->ah;                                                   // stops the parser before its stream goes
 	delete q->ps;
 	delete q->st;
 	delete q->src;
@@ -381,7 +496,6 @@ int mbg_fq_buf_new(mbg_fq_buf_t **out)
 {
 	if (out == nullptr) return MBG_EARG;
 	*out = new mbg_fq_buf_s;
-	(*out)->clear();
 	return MBG_OK;
 }
 
@@ -391,35 +505,37 @@ int mbg_fq_read(mbg_fq_t *q, int64_t chunk_size, int32_t with_qual, int32_t with
                 int64_t max_chunk_size, mbg_fq_buf_t *b, mbg_fq_chunk_t *out)
 {
 	if (q == nullptr || b == nullptr || out == nullptr) return MBG_EARG;
-	b->clear();
-	int64_t size = 0, ret;
+	std::vector<Take> t;
+	int64_t size = 0, ret = 0;
 	if (q->have_pending) {
-		b->put(q->pending);
-		b->commit(q->pend_q, q->pend_c);
-		size = (int64_t)q->pending.seq.size();
+		t.push_back(std::move(q->pending));
+		size = t.back().len();
 		q->have_pending = false;
 	}
 	if (max_chunk_size < chunk_size) max_chunk_size = chunk_size;
-	while ((ret = q->ps->read(b->f)) >= 0) {
-		b->commit(with_qual, with_comment);
+	for (;;) {
+		Take r{nullptr, 0, with_qual != 0, with_comment != 0};
+		q->ah->next(&r.b, &r.i);
+		if ((ret = r.len()) < 0) break;
+		t.push_back(std::move(r));
 		size += ret;
 		bool stop;
 		if (chunk_size <= 0 || max_chunk_size <= 0) stop = true;
 		else if (size >= max_chunk_size) stop = true;
-		else stop = size >= chunk_size && b->n() >= min_cnt;
+		else stop = size >= chunk_size && (int64_t)t.size() >= min_cnt;
 		if (stop) {
 			if (frag_mode && (uint64_t)ret < kPairCheck) {
-				while ((ret = q->ps->read(b->f)) >= 0) {
+				for (;;) {
+					Take r2{nullptr, 0, with_qual != 0, with_comment != 0};
+					q->ah->next(&r2.b, &r2.i);
+					if ((ret = r2.len()) < 0) break;
 					size += ret;
-					const int64_t *no = b->name_off.data();
-					const int32_t k = b->n();
-					const char *nm = b->f.name.data();
-					if (qname_same(nm + no[k], b->f.name.size() - (size_t)no[k], nm + no[k - 1], (size_t)(no[k] - no[k - 1]))) {
-						b->commit(with_qual, with_comment);
+					const Take &l = t.back();
+					if (qname_same(r2.name(), r2.name_len(), l.name(), l.name_len())) {
+						t.push_back(std::move(r2));
 					} else {
-						b->take(q->pending);
-						b->drop();
-						q->have_pending = true, q->pend_q = with_qual, q->pend_c = with_comment;
+						q->pending = std::move(r2);
+						q->have_pending = true;
 						break;
 					}
 				}
@@ -427,8 +543,7 @@ int mbg_fq_read(mbg_fq_t *q, int64_t chunk_size, int32_t with_qual, int32_t with
 			break;
 		}
 	}
-	b->drop();                                                      // a record cut by an error or the end
-	encode(*b, q->n_threads);
+	assemble(*b, t, q->n_threads);
 	view(*b, ret < -1 ? (int32_t)ret : (q->src->failed() ? MBG_EIO : 0), out);
 	return MBG_OK;
 }
@@ -438,24 +553,24 @@ int mbg_fq_read_frag(int32_t n_fp, mbg_fq_t **q, int64_t chunk_size, int32_t wit
 {
 	if (n_fp < 1 || q == nullptr || b == nullptr || out == nullptr) return MBG_EARG;
 	for (int32_t i = 0; i < n_fp; ++i) if (q[i] == nullptr) return MBG_EARG;
-	b->clear();
+	std::vector<Take> t, step((size_t)n_fp);
 	int64_t size = 0;
-	std::vector<Rec> recs((size_t)n_fp);
 	for (;;) {                                                      // bseq.c:135-158: one record of each file per step
 		int n_read = 0;
 		for (int32_t i = 0; i < n_fp; ++i) {
-			recs[(size_t)i].clear();
-			if (q[i]->ps->read(recs[(size_t)i]) >= 0) ++n_read;
+			Take &r = step[(size_t)i];
+			r.q = with_qual != 0, r.c = with_comment != 0;
+			q[i]->ah->next(&r.b, &r.i);
+			if (r.len() >= 0) ++n_read;
 		}
 		if (n_read < n_fp) break;
 		for (int32_t i = 0; i < n_fp; ++i) {
-			b->put(recs[(size_t)i]);
-			b->commit(with_qual, with_comment);
-			size += (int64_t)recs[(size_t)i].seq.size();
+			size += step[(size_t)i].len();
+			t.push_back(step[(size_t)i]);
 		}
 		if (size >= chunk_size) break;
 	}
-	encode(*b, q[0]->n_threads);
+	assemble(*b, t, q[0]->n_threads);
 	view(*b, 0, out);
 	return MBG_OK;
 }

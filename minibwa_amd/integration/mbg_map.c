@@ -50,6 +50,7 @@
 #define _GNU_SOURCE
 #include <assert.h>
 #include <pthread.h>
+#include <stdatomic.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -64,6 +65,7 @@
 #include "mbg.h"
 #include "mbg_exec.h"
 #include "mbg_fmt.h"
+#include "mbg_pe.h"
 #include "mbg_sched.h"
 
 /* ---- the reference's pipeline record (map-main.c:16-24) ----------------------------------------- */
@@ -91,6 +93,24 @@ static int g_stats;
 static int64_t g_flush = 4096;    /* pending requests that start a device batch (MBG_FLUSH) */
 static int g_fmt_native;          /* output records by mbg_fmt_record (else the reference's mb_format) */
 static int g_fmt_check;           /* MBG_FMT_CHECK=1: every record by both, compared */
+
+static double now_s(void);
+
+/* MBG_TIMELINE=1: the start and end of every pipeline stage of every mini-batch, printed after the
+ * pipeline, relative to its start */
+#define TL_CAP 4096
+static int g_tl;
+static double g_tl_t0;
+static atomic_int g_tl_n;
+static struct { double t; int64_t batch; const char *ev; } g_tl_ev[TL_CAP];
+
+static void tl_mark(int64_t batch, const char *ev)
+{
+	int k;
+	if (!g_tl) return;
+	k = atomic_fetch_add(&g_tl_n, 1);
+	if (k < TL_CAP) g_tl_ev[k].t = now_s(), g_tl_ev[k].batch = batch, g_tl_ev[k].ev = ev;
+}
 
 static double now_s(void)
 {
@@ -177,6 +197,7 @@ typedef struct {
 	const int32_t *pair_off;         /* pair k: reads pair_off[k], pair_off[k] + 1 */
 	const mb_pestat_t *pes;
 	double t_front;
+	int64_t seq_no;               /* the mini-batch's number in the run (for the timeline) */
 	int front_done;               /* the front pass has run (set by the front thread, under g_front.mu) */
 } batch_t;
 
@@ -271,6 +292,7 @@ static void front_reads(batch_t *b, int slot, const uint8_t *nt4, const int64_t 
 	int64_t i, tot = 0;
 	double t0 = now_s();
 	mbx_front_in_t in;
+	tl_mark(b->seq_no, "front+");
 	int32_t last_len = -1;
 	int any_meth = 0;
 	b->off = (int64_t*)malloc((b->n_seq + 1) * sizeof(int64_t));
@@ -330,6 +352,7 @@ static void front_reads(batch_t *b, int slot, const uint8_t *nt4, const int64_t 
 	in.n_copt = b->n_copt, in.copt = b->copt, in.copt_idx = b->copt_idx;
 	mbx_front(&in, &b->front);
 	b->t_front = now_s() - t0;
+	tl_mark(b->seq_no, "front-");
 }
 
 /* The single-end mapping of a mini-batch whose front pass has run. */
@@ -337,8 +360,10 @@ static void se_reads(batch_t *b, int n_threads)
 {
 	mbs_pool_t *pool = pool_for(n_threads);
 	double t0 = now_s(), t1;
+	tl_mark(b->seq_no, "se+");
 	mbs_run(pool, b->n_seq, 64, se_task, round_cb, b, g_flush);
 	t1 = now_s();
+	tl_mark(b->seq_no, "se-");
 	if (g_stats) {
 		mbs_stats_t st;
 		mbs_get_stats(pool, &st);
@@ -366,7 +391,9 @@ static void pair_reads(batch_t *b)
 {
 	double t0 = now_s();
 	if (b->n_pair <= 0) return;
+	tl_mark(b->seq_no, "pair+");
 	mbs_run(g_pool, b->n_pair, 64, pe_task, round_cb, b, g_flush);
+	tl_mark(b->seq_no, "pair-");
 	if (g_stats) fprintf(stderr, "[mbg_map] %d pairs paired in %.3f s\n", b->n_pair, now_s() - t0);
 }
 
@@ -436,6 +463,8 @@ typedef struct {
 	kstring_t *out;               /* the formatted records, in read order, one string per format task */
 	int32_t n_out;
 	int64_t tot_len;
+	int64_t id0;                  /* the id of the chunk's first read */
+	uint8_t *same;                /* same[i]: read i joins read i-1's fragment (the read step only) */
 	double t_read, t_map, t_fmt;
 	/* the mapping of the mini-batch (its front pass runs in the read step) */
 	batch_t b;
@@ -512,6 +541,81 @@ static void front_wait(batch_t *b)
 	pthread_mutex_unlock(&g_front.mu);
 }
 
+/* fn(ctx, lo, hi) over [0, n) split into nt ranges, on nt - 1 new threads and the caller's */
+typedef struct { void (*fn)(void*, int64_t, int64_t); void *ctx; int64_t lo, hi; } par_t;
+static void *par_main(void *a)
+{
+	par_t *p = (par_t*)a;
+	p->fn(p->ctx, p->lo, p->hi);
+	return 0;
+}
+static void par_for(int64_t n, int nt, void (*fn)(void*, int64_t, int64_t), void *ctx)
+{
+	pthread_t th[16];
+	par_t a[16];
+	int k, started[16];
+	if (nt > 16) nt = 16;
+	if (nt < 1 || n < 4096) nt = 1;
+	for (k = 0; k < nt; ++k) a[k].fn = fn, a[k].ctx = ctx, a[k].lo = n * k / nt, a[k].hi = n * (k + 1) / nt;
+	for (k = 1; k < nt; ++k) started[k] = pthread_create(&th[k], 0, par_main, &a[k]) == 0;
+	fn(ctx, a[0].lo, a[0].hi);
+	for (k = 1; k < nt; ++k) {
+		if (started[k]) pthread_join(th[k], 0);
+		else fn(ctx, a[k].lo, a[k].hi);                            /* no thread: run the range here */
+	}
+}
+
+#define READ_THREADS 4
+
+/* the records of a chunk as mb_bseq1_t: names and comments NUL-terminated in the arena, at offset
+ * name_off + com_off + 2 i for record i (room for both terminators) */
+static void rec_fill(void *ctx, int64_t lo, int64_t hi)
+{
+	pbatch_t *s = (pbatch_t*)ctx;
+	int64_t i;
+	for (i = lo; i < hi; ++i) {
+		mb_bseq1_t *t = &s->seq[i];
+		int64_t nl = s->ch.name_off[i + 1] - s->ch.name_off[i], cl = s->ch.com_off[i + 1] - s->ch.com_off[i];
+		t->name = s->arena + s->ch.name_off[i] + s->ch.com_off[i] + 2 * i;
+		memcpy(t->name, s->ch.name + s->ch.name_off[i], nl);
+		t->name[nl] = 0;
+		if (cl > 0) {
+			t->comment = t->name + nl + 1;
+			memcpy(t->comment, s->ch.comment + s->ch.com_off[i], cl);
+			t->comment[cl] = 0;
+		}
+		t->l_seq = s->ch.seq_off[i + 1] - s->ch.seq_off[i];
+		t->seq = (char*)s->ch.seq + s->ch.seq_off[i];
+		t->qual = s->ch.has_qual[i] ? (char*)s->ch.qual + s->ch.seq_off[i] : 0;
+		t->id = s->id0 + i;
+	}
+}
+
+/* same[i] (i >= 1): read i shares read i-1's name (mb_qname_same), which joins them in one fragment */
+static void rec_same(void *ctx, int64_t lo, int64_t hi)
+{
+	pbatch_t *s = (pbatch_t*)ctx;
+	int64_t i;
+	for (i = lo < 1 ? 1 : lo; i < hi; ++i)
+		s->same[i] = mb_qname_same(s->seq[i - 1].name, s->seq[i].name);
+}
+
+/* the "/1" "/2" trimming of mates' names, and the per-read arrays of the mapping */
+static void rec_trim(void *ctx, int64_t lo, int64_t hi)
+{
+	pbatch_t *s = (pbatch_t*)ctx;
+	int64_t i;
+	for (i = lo; i < hi; ++i) {
+		int32_t j0, j1, l0, l1;
+		if (s->seg_cnt[i] != 2) continue;
+		j0 = s->seg_off[i], l0 = strlen(s->seq[j0].name);
+		j1 = s->seg_off[i] + 1, l1 = strlen(s->seq[j1].name);
+		if (l0 < 3 || l0 != l1) continue;
+		if (s->seq[j0].name[l0 - 1] != s->seq[j1].name[l1 - 1] && s->seq[j0].name[l0 - 2] == '/')
+			s->seq[j0].name[l0 - 2] = s->seq[j1].name[l1 - 2] = 0;
+	}
+}
+
 /* step 0 (map-main.c:114-173): a mini-batch of reads, then fragments (mates sharing a name) and the
  * trimming of "/1" "/2" name suffixes */
 static pbatch_t *read_step(const ref_pipeline_t *p)
@@ -524,6 +628,7 @@ static pbatch_t *read_step(const ref_pipeline_t *p)
 	int64_t an = 0;
 	double t0 = now_s();
 	int rc;
+	tl_mark(g_pipe.n_batch, "read+");
 	if (g_pipe.n_fq == 0) {          /* the first mini-batch opens the native readers on the same files */
 		pthread_mutex_lock(&g_mu);
 		for (i = 0; i < p->n_fp; ++i) {
@@ -541,6 +646,7 @@ static pbatch_t *read_step(const ref_pipeline_t *p)
 	if (p->n_fp > 1) rc = mbg_fq_read_frag(p->n_fp, g_pipe.fq, p->mb_size, with_qual, with_comment, s->buf, &s->ch);
 	else rc = mbg_fq_read(g_pipe.fq[0], p->mb_size, with_qual, with_comment, frag_mode, 40000, opt->max_mb_size, s->buf, &s->ch);
 	if (rc != MBG_OK) die_rc("reading the input", rc);
+	tl_mark(g_pipe.n_batch, "parsed");
 	if (s->ch.status == -2 && p->n_fp == 1) {   /* bseq.c:124-127 */
 		if (s->ch.n) {
 			int64_t l = s->ch.name_off[s->ch.n] - s->ch.name_off[s->ch.n - 1];
@@ -554,33 +660,24 @@ static pbatch_t *read_step(const ref_pipeline_t *p)
 		return 0;
 	}
 	s->n_seq = s->ch.n;
-	for (i = 0; i < s->n_seq; ++i)
-		an += (s->ch.name_off[i + 1] - s->ch.name_off[i]) + (s->ch.com_off[i + 1] - s->ch.com_off[i]) + 2;
+	an = s->ch.name_off[s->n_seq] + s->ch.com_off[s->n_seq] + 2 * (int64_t)s->n_seq;
 	s->arena = (char*)malloc(an + 1);
 	s->seq = (mb_bseq1_t*)calloc(s->n_seq, sizeof(mb_bseq1_t));
 	s->n_hit = (int32_t*)calloc(3 * s->n_seq + 3, sizeof(int32_t));
 	s->seg_off = s->n_hit + s->n_seq + 1, s->seg_cnt = s->seg_off + s->n_seq + 1;
 	s->hit = (mb_hit_t**)calloc(s->n_seq, sizeof(mb_hit_t*));
 	if (!s->arena || !s->seq || !s->n_hit || !s->hit) die("out of memory (mini-batch)");
-	for (i = 0, an = 0; i < s->n_seq; ++i) {
-		mb_bseq1_t *t = &s->seq[i];
-		int64_t nl = s->ch.name_off[i + 1] - s->ch.name_off[i], cl = s->ch.com_off[i + 1] - s->ch.com_off[i];
-		t->name = s->arena + an;
-		memcpy(t->name, s->ch.name + s->ch.name_off[i], nl);
-		t->name[nl] = 0, an += nl + 1;
-		if (cl > 0) {
-			t->comment = s->arena + an;
-			memcpy(t->comment, s->ch.comment + s->ch.com_off[i], cl);
-			t->comment[cl] = 0, an += cl + 1;
-		}
-		t->l_seq = s->ch.seq_off[i + 1] - s->ch.seq_off[i];
-		t->seq = (char*)s->ch.seq + s->ch.seq_off[i];
-		t->qual = s->ch.has_qual[i] ? (char*)s->ch.qual + s->ch.seq_off[i] : 0;
-		t->id = g_pipe.n_seq++;
-	}
+	s->id0 = g_pipe.n_seq;
+	par_for(s->n_seq, READ_THREADS, rec_fill, s);
+	g_pipe.n_seq += s->n_seq;
 	for (i = 0; i < 4; ++i) s->pes[i].failed = 1;
+	if (frag_mode) {
+		s->same = (uint8_t*)calloc(s->n_seq + 1, 1);
+		if (s->same == 0) die("out of memory (mini-batch)");
+		par_for(s->n_seq, READ_THREADS, rec_same, s);
+	}
 	for (i = 1, j = 0; i <= s->n_seq; ++i) {
-		if (i == s->n_seq || !frag_mode || !mb_qname_same(s->seq[i - 1].name, s->seq[i].name)) {
+		if (i == s->n_seq || !frag_mode || !s->same[i]) {
 			if (i - j > 2) die("a fragment of more than two reads");      /* the reference asserts the same */
 			s->seg_cnt[s->n_frag] = i - j;
 			s->seg_off[s->n_frag++] = j;
@@ -588,17 +685,9 @@ static pbatch_t *read_step(const ref_pipeline_t *p)
 			j = i;
 		}
 	}
-	if (s->n_pe > 0) {
-		for (i = 0; i < s->n_frag; ++i) {
-			int32_t j0, j1, l0, l1;
-			if (s->seg_cnt[i] != 2) continue;
-			j0 = s->seg_off[i], l0 = strlen(s->seq[j0].name);
-			j1 = s->seg_off[i] + 1, l1 = strlen(s->seq[j1].name);
-			if (l0 < 3 || l0 != l1) continue;
-			if (s->seq[j0].name[l0 - 1] != s->seq[j1].name[l1 - 1] && s->seq[j0].name[l0 - 2] == '/')
-				s->seq[j0].name[l0 - 2] = s->seq[j1].name[l1 - 2] = 0;
-		}
-	}
+	free(s->same);
+	s->same = 0;
+	if (s->n_pe > 0) par_for(s->n_frag, READ_THREADS, rec_trim, s);
 	/* the front pass of this mini-batch, overlapping the previous one's mapping */
 	{
 		const mb_idx_t *idx = p->idx;
@@ -616,6 +705,7 @@ static pbatch_t *read_step(const ref_pipeline_t *p)
 		memset(b, 0, sizeof(*b));
 		b->opt = opt, b->idx = idx, b->n_seq = s->n_seq, b->mt = s->mt, b->qlen = s->qlen, b->seq = s->sq, b->name = s->nm;
 		b->n_hit = s->n_hit, b->hit = s->hit;
+		b->seq_no = g_pipe.n_batch;
 		if (g_front.started) front_queue(b, (int)(g_pipe.n_batch++ % MBX_FRONT_SLOTS), s->ch.nt4, s->ch.seq_off);
 		else {
 			front_reads(b, (int)(g_pipe.n_batch++ % MBX_FRONT_SLOTS), s->ch.nt4, s->ch.seq_off);
@@ -623,6 +713,7 @@ static pbatch_t *read_step(const ref_pipeline_t *p)
 		}
 	}
 	s->t_read = now_s() - t0;
+	tl_mark(s->b.seq_no, "read-");
 	return s;
 }
 
@@ -688,6 +779,7 @@ static void *map_step(pbatch_t *s)
 	batch_t *b = &s->b;
 	int32_t k, *pair_off = 0, n_pair = 0;
 	double t0 = now_s();
+	tl_mark(b->seq_no, "map+");
 	front_wait(b);
 	se_reads(b, opt->n_thread);
 	if ((opt->flag & MB_F_PE) && s->n_frag < s->n_seq && !(opt->flag & (MB_F_NO_PAIRING | MB_F_NO_ALN))) { /* map-main.c:180-192 */
@@ -696,9 +788,8 @@ static void *map_step(pbatch_t *s)
 			s->pes[1].avg = opt->pe_avg, s->pes[1].std = opt->pe_std;
 			s->pes[1].lo = opt->pe_lo, s->pes[1].hi = opt->pe_hi;
 		} else {
-			void *km = opt->flag & MB_F_NO_KALLOC ? 0 : km_init();
-			mb_pestat(km, opt, s->n_frag, s->seg_off, s->seg_cnt, s->n_hit, s->hit, s->pes);
-			if (km) km_destroy(km);
+			tl_mark(b->seq_no, "pestat+");
+			mbg_pestat(opt, s->n_frag, s->seg_off, s->seg_cnt, s->n_hit, s->hit, s->pes, opt->n_thread);
 		}
 		pair_off = (int32_t*)malloc((s->n_frag + 1) * sizeof(int32_t));
 		for (k = 0; k < s->n_frag; ++k)
@@ -709,6 +800,7 @@ static void *map_step(pbatch_t *s)
 	map_free(b);
 	free(pair_off);
 	s->t_map = now_s() - t0;
+	tl_mark(b->seq_no, "map-");
 	return s;
 }
 
@@ -722,6 +814,7 @@ static void write_step(pbatch_t *s)
 	s->n_out = (s->n_frag + FMT_FRAGS - 1) / FMT_FRAGS;
 	s->out = (kstring_t*)calloc(s->n_out + 1, sizeof(kstring_t));
 	fc.s = s, fc.opt = p->opt, fc.idx = p->idx;
+	tl_mark(s->b.seq_no, "fmt+");
 	mbs_run(g_fmt_pool, s->n_out, 1, fmt_task, 0, &fc, 0);
 	t1 = now_s();
 	s->t_fmt = t1 - t0;
@@ -730,6 +823,7 @@ static void write_step(pbatch_t *s)
 		if (s->out[k].l) fwrite(s->out[k].s, 1, s->out[k].l, p->fp_out);
 		free(s->out[k].s);
 	}
+	tl_mark(s->b.seq_no, "write-");
 	if (kom_verbose >= 3)
 		fprintf(stderr, "[M::%s::%.3f*%.2f] mapped %ld bp in %ld sequences\n", "worker_pipeline", kom_realtime(), kom_percent_cpu(),
 		        (long)s->tot_len, (long)s->n_seq);
@@ -774,6 +868,12 @@ void __wrap_kt_pipeline(int n_threads, void *(*func)(void*, int, void*), void *s
 		g_fmt_check = e && atoi(e) > 0;
 	}
 	g_pipe.p = p, g_pipe.n_fq = 0, g_pipe.n_seq = p->n_seq, g_pipe.n_batch = 0;
+	{
+		const char *e = getenv("MBG_TIMELINE");
+		g_tl = e && atoi(e) > 0;
+		g_tl_t0 = now_s();
+		atomic_store(&g_tl_n, 0);
+	}
 	if (!g_front.started) {
 		const char *e = getenv("MBG_FRONT_THREAD");
 		if (!(e && atoi(e) == 0)) {
@@ -793,6 +893,12 @@ void __wrap_kt_pipeline(int n_threads, void *(*func)(void*, int, void*), void *s
 	}
 	for (i = 0; i < g_pipe.n_fq; ++i) mbg_fq_close(g_pipe.fq[i]);
 	g_pipe.n_fq = 0;
+	if (g_tl) {
+		int n = atomic_load(&g_tl_n);
+		if (n > TL_CAP) n = TL_CAP;
+		for (i = 0; i < n; ++i)
+			fprintf(stderr, "[mbg_map] timeline %8.1f ms  batch %ld %s\n", 1e3 * (g_tl_ev[i].t - g_tl_t0), (long)g_tl_ev[i].batch, g_tl_ev[i].ev);
+	}
 }
 
 /* ---- the public batch API (map-algo.c:703-770) ---------------------------------------------------- */

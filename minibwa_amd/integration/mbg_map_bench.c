@@ -15,6 +15,8 @@
  * MBG_PROF_OUT=<file>: sample the program counter of every thread the mapper creates (and the main
  * thread) once per millisecond of that thread's CPU time, during the passes after the first, and write
  * the samples and /proc/self/maps to <file> at the end, for an offline flat profile of the host side.
+ * Each sample also holds the word at the top of the stack, which inside a leaf function (libc's
+ * memcpy and the like) is its return address, so that such time can be charged to the caller.
  */
 #define _GNU_SOURCE
 #include <pthread.h>
@@ -22,6 +24,7 @@
 #include <unistd.h>
 #include <stdatomic.h>
 #include <sys/time.h>
+#include <sys/resource.h>
 #include <ucontext.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -34,7 +37,7 @@
 int32_t mb_map_file(const mb_opt_t *opt, const mb_idx_t *idx, int32_t n, const char **fn, const char *hdr, const char *fn_out);
 
 #define PROF_CAP (1 << 24)
-static uintptr_t *g_prof;
+static uintptr_t *g_prof, *g_prof_ret;
 static atomic_long g_nprof;
 static atomic_int g_prof_on;
 static int g_prof_req;                   /* MBG_PROF_OUT is set: every thread gets a CPU-time timer */
@@ -46,7 +49,10 @@ static void prof_handler(int sig, siginfo_t *si, void *uc_)
 	(void)sig, (void)si;
 	if (!atomic_load(&g_prof_on)) return;
 	k = atomic_fetch_add(&g_nprof, 1);
-	if (k < PROF_CAP) g_prof[k] = (uintptr_t)uc->uc_mcontext.gregs[REG_RIP];
+	if (k < PROF_CAP) {              /* the PC, and the word at the top of the stack: a leaf function's return address */
+		g_prof[k] = (uintptr_t)uc->uc_mcontext.gregs[REG_RIP];
+		g_prof_ret[k] = *(const uintptr_t*)uc->uc_mcontext.gregs[REG_RSP];
+	}
 }
 
 /* a timer on the calling thread's own CPU clock, signalling that thread every millisecond it runs */
@@ -89,6 +95,7 @@ static void prof_init(void)
 	struct sigaction sa;
 	g_prof_req = 1;
 	g_prof = (uintptr_t*)calloc(PROF_CAP, sizeof(uintptr_t));
+	g_prof_ret = (uintptr_t*)calloc(PROF_CAP, sizeof(uintptr_t));
 	memset(&sa, 0, sizeof(sa));
 	sa.sa_sigaction = prof_handler;
 	sa.sa_flags = SA_SIGINFO | SA_RESTART;
@@ -108,7 +115,7 @@ static void prof_stop(const char *fn)
 	maps = fopen("/proc/self/maps", "r");
 	while (maps && fgets(line, sizeof(line), maps)) fprintf(fp, "M %s", line);
 	if (maps) fclose(maps);
-	for (i = 0; i < n; ++i) fprintf(fp, "P %lx\n", (unsigned long)g_prof[i]);
+	for (i = 0; i < n; ++i) fprintf(fp, "P %lx %lx\n", (unsigned long)g_prof[i], (unsigned long)g_prof_ret[i]);
 	fclose(fp);
 }
 
@@ -117,6 +124,14 @@ static double now_s(void)
 	struct timespec ts;
 	clock_gettime(CLOCK_MONOTONIC, &ts);
 	return ts.tv_sec + ts.tv_nsec * 1e-9;
+}
+
+/* user + system CPU time of the whole process */
+static double cpu_s(void)
+{
+	struct rusage ru;
+	getrusage(RUSAGE_SELF, &ru);
+	return ru.ru_utime.tv_sec + ru.ru_stime.tv_sec + 1e-6 * (ru.ru_utime.tv_usec + ru.ru_stime.tv_usec);
 }
 
 int main(int argc, char *argv[])
@@ -151,13 +166,13 @@ int main(int argc, char *argv[])
 	fprintf(stderr, "[mbg_bench] load %.3f\n", now_s() - t0);
 	if (!(opt.flag & MB_F_PAF) && mb_fmt_sam_hdr(&hdr, idx->l2b, 0, MB_VERSION, argc, argv) < 0) return 1;
 	for (i = 0; i < passes; ++i) {
-		double t1 = now_s();
+		double t1 = now_s(), c1 = cpu_s();
 		if (i == 1 && g_prof_req) atomic_store(&g_prof_on, 1);
 		if (mb_map_file(&opt, idx, argc - (o.ind + 1), (const char**)&argv[o.ind + 1], hdr.s, i == 0 ? out0 : out1) != 0) {
 			fprintf(stderr, "[mbg_bench] mapping failed\n");
 			return 1;
 		}
-		fprintf(stderr, "[mbg_bench] pass %d %.4f\n", i, now_s() - t1);
+		fprintf(stderr, "[mbg_bench] pass %d %.4f cpu %.3f\n", i, now_s() - t1, cpu_s() - c1);
 	}
 	if (g_prof_req) prof_stop(getenv("MBG_PROF_OUT"));
 	free(hdr.s);
