@@ -44,9 +44,18 @@ static const char kDigits2[201] =
 	"8081828384858687888990919293949596979899";
 
 /* unsigned decimal, most significant digit first */
+static const uint64_t kPow10[20] = {1ULL, 10ULL, 100ULL, 1000ULL, 10000ULL, 100000ULL, 1000000ULL, 10000000ULL, 100000000ULL,
+	1000000000ULL, 10000000000ULL, 100000000000ULL, 1000000000000ULL, 10000000000000ULL, 100000000000000ULL,
+	1000000000000000ULL, 10000000000000000ULL, 100000000000000000ULL, 1000000000000000000ULL, 10000000000000000000ULL};
+
+/* decimal digits of x written in place, two at a time from the last */
 static inline void put_u64(kstring_t *s, uint64_t x)
 {
-	char buf[24], *e = buf + sizeof(buf), *p = e;
+	int n = 1;
+	char *p;
+	while (n < 20 && x >= kPow10[n]) ++n;
+	p = s->s + s->l + n;
+	s->l += (size_t)n;
 	while (x >= 100) {
 		unsigned r = (unsigned)(x % 100);
 		x /= 100;
@@ -54,8 +63,6 @@ static inline void put_u64(kstring_t *s, uint64_t x)
 	}
 	if (x >= 10) p -= 2, memcpy(p, &kDigits2[x * 2], 2);
 	else *--p = (char)('0' + x);
-	memcpy(s->s + s->l, p, (size_t)(e - p));
-	s->l += (size_t)(e - p);
 }
 
 /* "%d": an int; the magnitude of a negative value is taken as unsigned (INT_MIN included) */

@@ -989,6 +989,29 @@ double __wrap_mb_anchor(void *km, const mb_idx_t *idx, mb_sai_v *u, int32_t min_
 
 mb_anchor_t *__real_mb_lchain_dp(void *km, const l2b_t *l2b, int max_dist_x, int max_dist_y, int bw, int max_skip, int max_iter,
                                  int min_sc, float chn_pen_gap, int64_t n, mb_anchor_t *a, int *n_u_, uint64_t **_u);
+/* The reference bases around the first anchor of each of a read's first chains, towards the caches:
+ * the read's extension reads them (l2b_getseq) a few microseconds later, and the first touch of a
+ * random spot of a multi-gigabase reference is otherwise a full memory miss on the mapping thread.
+ * u[i]'s low 32 bits count chain i's anchors, which follow each other in a. */
+static void prefetch_targets(const l2b_t *l2b, int32_t n_u, const uint64_t *u, const mb_anchor_t *a)
+{
+	int32_t i;
+	int64_t k = 0;
+	for (i = 0; i < n_u && i < 4; ++i) {
+		const mb_anchor_t *x = &a[k];
+		const int64_t tid = x->sid >> 1;
+		if (tid >= 0 && (uint64_t)tid < l2b->n_ctg) {
+			const int64_t g = l2b->ctg[tid].off + x->tpos;
+			const char *p = (const char*)l2b->pac + (g >> 2);
+			const char *lo = (const char*)l2b->pac + (l2b->ctg[tid].off >> 2), *hi = lo + ((l2b->ctg[tid].len + 3) >> 2);
+			int d;
+			for (d = -192; d <= 192; d += 64)
+				if (p + d >= lo && p + d < hi) __builtin_prefetch(p + d);
+		}
+		k += (uint32_t)u[i];
+	}
+}
+
 mb_anchor_t *__wrap_mb_lchain_dp(void *km, const l2b_t *l2b, int max_dist_x, int max_dist_y, int bw, int max_skip, int max_iter,
                                  int min_sc, float chn_pen_gap, int64_t n, mb_anchor_t *a, int *n_u_, uint64_t **_u)
 {
@@ -1011,6 +1034,7 @@ mb_anchor_t *__wrap_mb_lchain_dp(void *km, const l2b_t *l2b, int max_dist_x, int
 			c = (mb_anchor_t*)(kmalloc)(km, (size_t)n_c * sizeof(mb_anchor_t));
 			memcpy(u, f->u + f->anc_off[tc->r], (size_t)n_u * sizeof(uint64_t));
 			memcpy(c, f->chn + f->anc_off[tc->r], (size_t)n_c * sizeof(mb_anchor_t));
+			prefetch_targets(l2b, n_u, u, c);
 			if (_u) *_u = u, *n_u_ = n_u;
 			else (kfree)(km, u);
 			return c;
@@ -1037,11 +1061,14 @@ static void dp_call(int dual, void *km, int qlen, const uint8_t *query, int tlen
 	memcpy(r.mat, mat, (size_t)m * m);
 	r.gq = q, r.ge = e, r.gq2 = q2, r.ge2 = e2;
 	r.w = w, r.zdrop = zdrop, r.end_bonus = end_bonus, r.flag = flag;
-	/* the CIGAR lands in ez's own buffer, grown first to the most a job can write (ksw2.h's allocator,
-	 * as ksw2 itself grows it) */
+	/* the CIGAR lands in ez's own buffer, grown first to the most a job can write with the allocator
+	 * ksw2 grows it with (kalloc's; libc's for a null km); what it held is overwritten, so not copied */
 	if (ez->m_cigar < r.qlen + r.tlen + 1) {
-		ez->m_cigar = r.qlen + r.tlen + 1;
-		ez->cigar = (uint32_t*)krealloc(km, ez->cigar, (size_t)ez->m_cigar * 4);
+		int m = ez->m_cigar * 2 > r.qlen + r.tlen + 1 ? ez->m_cigar * 2 : r.qlen + r.tlen + 1;
+		kfree(km, ez->cigar);
+		ez->cigar = (uint32_t*)kmalloc(km, (size_t)m * 4);
+		if (ez->cigar == 0) die("out of memory (CIGAR)");
+		ez->m_cigar = m;
 	}
 	r.cigar = ez->cigar;
 	if (solo) {

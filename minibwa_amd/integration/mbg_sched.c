@@ -68,6 +68,8 @@ struct thr_s {
 	int waiting;
 	co_t *free_co;
 	int64_t n_co;             /* coroutine objects (each with a stack) owned by this thread */
+	co_t **chunks;            /* ... allocated STACKS_PER_CHUNK at a time: their canaries are checked after each run */
+	int64_t n_chunks, m_chunks;
 	int64_t waits;
 	double busy_s, idle_s;
 	char pad_[128];           /* keeps the next thread's counters off this thread's last line */
@@ -191,6 +193,12 @@ static co_t *get_co(thr_t *t)
 			objs[k].next_free = k + 1 < STACKS_PER_CHUNK ? &objs[k + 1] : 0;
 		}
 		t->n_co += STACKS_PER_CHUNK;
+		if (t->n_chunks == t->m_chunks) {
+			t->m_chunks = t->m_chunks ? t->m_chunks * 2 : 64;
+			t->chunks = (co_t**)realloc(t->chunks, (size_t)t->m_chunks * sizeof(co_t*));
+			if (t->chunks == 0) die("out of memory (coroutine stacks)");
+		}
+		t->chunks[t->n_chunks++] = objs;
 		t->free_co = objs[0].next_free;
 		return &objs[0];
 	}
@@ -219,10 +227,20 @@ static int64_t posted_sum(mbs_pool_t *p)
 	return s;
 }
 
+/* every stack of the thread: a task that overflowed its stack overwrote the words at its low end
+ * (checked once a run rather than at each switch, where the cold low end of a stack would cost a
+ * cache miss per coroutine) */
+static void check_stacks(const thr_t *t)
+{
+	int64_t i;
+	int k;
+	for (i = 0; i < t->n_chunks; ++i)
+		for (k = 0; k < STACKS_PER_CHUNK; ++k) check_canary(&t->chunks[i][k]);
+}
+
 static void finish(thr_t *t, co_t *co)
 {
 	mbs_pool_t *p = t->pool;
-	check_canary(co);
 	co->next_free = t->free_co;
 	t->free_co = co;
 	/* this thread's last task while nothing is left to claim: possibly the run's last */
@@ -402,6 +420,7 @@ static void *worker(void *arg)
 		wait_job(p, &my_gen);
 		if (p->stop) break;
 		run_job(t);
+		check_stacks(t);
 		job_finished(p);
 	}
 	return 0;
@@ -465,7 +484,7 @@ void mbs_pool_destroy(mbs_pool_t *p)
 	pthread_join(p->disp, 0);
 	for (i = 0; i < p->n; ++i) {
 		thr_t *t = &p->thr[i];
-		free(t->ready); free(t->rbuf); free(t->posted.req); free(t->posted.co);
+		free(t->ready); free(t->rbuf); free(t->posted.req); free(t->posted.co); free(t->chunks);
 		/* stacks and coroutine objects stay mapped: the pool lives for the process */
 	}
 	free(p->batch_l.req); free(p->batch_l.co); free(p->toff);
@@ -518,7 +537,6 @@ void mbs_wait(void *req)
 	pthread_mutex_unlock(&t->mu);
 	if (np == p->flush_thr) poke(p);   /* this thread's share of a full batch */
 	++t->waits;
-	check_canary(co);
 	mbs_switch(&co->sp, t->sched_sp);
 	tls_co = co;
 }
