@@ -26,10 +26,10 @@
  *              the first mb_lchain_dp with the read's adaptive options (mbg_exec.h: mbx_front);
  *   2. SE      each read runs the reference's mb_map_sai in a coroutine (mbg_sched.h).  Its mb_anchor
  *              and mb_lchain_dp calls return the front's results; each DP call posts a request and
- *              yields; when all reads have yielded or finished, the round runs every posted DP job as
- *              one device batch, and the reads resume -- so the k-th DP call of every read runs in
- *              round k, and a read's calls keep their order and their dependences (the gap-fill loop's
- *              z-drop break and second pass, align.c:689-721, see the previous call's result);
+ *              yields; the scheduler's dispatcher runs the posted jobs as device batches while the
+ *              threads go on with other reads, and each read resumes with its result -- so a read's
+ *              calls keep their order and their dependences (the gap-fill loop's z-drop break and
+ *              second pass, align.c:689-721, see the previous call's result);
  *   3. PE      the insert-size model (mb_pestat, or the predefined one) exactly as worker_pipeline
  *              chooses it (map-main.c:180-190), then mb_pair per pair in coroutines: each rescue
  *              candidate's ksw_ll and extension DP are deferred into rounds the same way.
@@ -38,8 +38,12 @@
  *
  * Mini-batches are read, mapped and written as worker_pipeline does: the same chunking (mb_bseq_read(_frag)
  * rules), the same fragments and name trimming, the same insert-size model per mini-batch, the same
- * records in the same order.  Formatting (mb_format per record, map-main.c:207-239) runs on the worker
- * pool in fragment ranges whose strings the output step writes in order.
+ * records in the same order.  Formatting (mb_format's records, map-main.c:207-239, written by
+ * mbg_fmt.c) runs on a second pool in fragment ranges whose strings the output step writes in order.
+ *
+ * Groups (mbg_map.h) are the serving loop's unit: a set of reads whose front pass has run, mapped one
+ * read or one pair per task inside an open run of the worker pool, pairing included (predefined
+ * insert-size statistics), so that reads of several groups map at the same time.
  *
  * The layout of pipeline_t (map-main.c:16-24, minibwa 0.7-r421) is restated below because kt_pipeline
  * hands it over as void*; the pipeline checks that its fields agree with its options before use.
@@ -66,6 +70,7 @@
 #include "mbg_exec.h"
 #include "mbg_fmt.h"
 #include "mbg_pe.h"
+#include "mbg_map.h"
 #include "mbg_sched.h"
 
 /* ---- the reference's pipeline record (map-main.c:16-24) ----------------------------------------- */
@@ -430,6 +435,15 @@ static struct {
 	int64_t n_batch;              /* mini-batches read so far: the front slot of the next one */
 } g_pipe;
 static mbs_pool_t *g_fmt_pool;    /* formatting in the output step, beside the mapping pool */
+
+/* the record formatter needs the run's read group, which only format.c knows: learnt from a probe record */
+void mbg_map_fmt_init(const mb_opt_t *opt)
+{
+	const char *e = getenv("MBG_FMT");
+	g_fmt_native = !(e && strcmp(e, "ref") == 0) && mbg_fmt_learn_rg(opt) == 0;
+	e = getenv("MBG_FMT_CHECK");
+	g_fmt_check = e && atoi(e) > 0;
+}
 
 /* one record into out, as mb_format (format.c:355-361) writes it */
 static void fmt_one(void *km, kstring_t *out, const l2b_t *l2b, const mb_bseq1_t *t, int32_t n_seg, const int32_t *n_hit,
@@ -861,12 +875,7 @@ void __wrap_kt_pipeline(int n_threads, void *(*func)(void*, int, void*), void *s
 		int n = e ? atoi(e) : p->opt->n_thread;
 		g_fmt_pool = mbs_pool_init(n > 0 ? n : 1, 64 * 1024);
 	}
-	{   /* the record formatter needs the run's read group, which only mb_fmt_sam knows */
-		const char *e = getenv("MBG_FMT");
-		g_fmt_native = !(e && strcmp(e, "ref") == 0) && mbg_fmt_learn_rg(p->opt) == 0;
-		e = getenv("MBG_FMT_CHECK");
-		g_fmt_check = e && atoi(e) > 0;
-	}
+	mbg_map_fmt_init(p->opt);
 	g_pipe.p = p, g_pipe.n_fq = 0, g_pipe.n_seq = p->n_seq, g_pipe.n_batch = 0;
 	{
 		const char *e = getenv("MBG_TIMELINE");
@@ -939,6 +948,75 @@ mb_hit_t **__wrap_mb_map_batch(const mb_opt_t *opt, const mb_idx_t *idx, int32_t
 	free(mt);
 	return hit;
 }
+
+/* ---- groups: reads mapped by tasks of an open run (the serving loop) ------------------------------ */
+
+struct mbg_group_s {
+	batch_t b;
+	uint8_t *mt;
+	mb_pestat_t pes[4];
+	int pair;                     /* tasks pair their two reads (mb_pair) */
+};
+
+mbg_group_t *mbg_group_new(const mb_opt_t *opt, const mb_idx_t *idx, int32_t n_seq, const int32_t *qlen, const char *const *seq,
+                           const char *const *name, int slot)
+{
+	int32_t i, is_pe = !!(opt->flag & MB_F_PE), is_meth = !!(opt->flag & MB_F_METH);
+	mbg_group_t *g = (mbg_group_t*)calloc(1, sizeof(mbg_group_t));
+	if (g == 0 || n_seq <= 0 || (is_meth && !idx->is_meth)) {
+		free(g);
+		return 0;
+	}
+	g->mt = (uint8_t*)calloc(n_seq, 1);
+	g->b.n_hit = (int32_t*)calloc(n_seq, sizeof(int32_t));
+	g->b.hit = (mb_hit_t**)calloc(n_seq, sizeof(mb_hit_t*));
+	if (!g->mt || !g->b.n_hit || !g->b.hit) die("out of memory (group)");
+	for (i = 0; i < n_seq; ++i)        /* mb_map_batch's conversion (map-algo.c:733-737) */
+		g->mt[i] = !is_meth ? L2B_METH_NONE : !is_pe || (i & 1) == 0 ? L2B_METH_C2T : L2B_METH_G2A;
+	g->b.opt = opt, g->b.idx = idx, g->b.n_seq = n_seq, g->b.qlen = qlen, g->b.seq = (const char *const*)seq;
+	g->b.name = (const char *const*)name, g->b.mt = g->mt;
+	for (i = 0; i < 4; ++i) g->pes[i].failed = 1;       /* mb_map_batch's predefined statistics (map-algo.c:755-766) */
+	g->pes[1].failed = 0;
+	g->pes[1].avg = opt->pe_avg, g->pes[1].std = opt->pe_std;
+	g->pes[1].lo = opt->pe_lo, g->pes[1].hi = opt->pe_hi;
+	g->b.pes = g->pes;
+	g->pair = is_pe && !(opt->flag & (MB_F_NO_PAIRING | MB_F_NO_ALN));
+	front_reads(&g->b, slot, 0, 0);
+	g->b.front_done = 1;
+	return g;
+}
+
+void mbg_group_map(mbg_group_t *g, int32_t r0, int32_t n)
+{
+	const int tid = mbs_tid();
+	if (tid < 0) die("mbg_group_map outside a task");
+	se_task(&g->b, r0, tid);
+	if (n == 2) {
+		se_task(&g->b, r0 + 1, tid);
+		if (g->pair) {
+			int32_t len[2];
+			char *sq[2];
+			len[0] = g->b.qlen[r0], len[1] = g->b.qlen[r0 + 1];
+			sq[0] = (char*)g->b.seq[r0], sq[1] = (char*)g->b.seq[r0 + 1];
+			mb_pair(mb_tbuf_km(g_tbuf[tid]), g->b.opt, g->b.idx->l2b, &g->b.n_hit[r0], &g->b.hit[r0], g->pes, len, sq);
+		}
+	}
+}
+
+int32_t *mbg_group_n_hit(mbg_group_t *g) { return g->b.n_hit; }
+mb_hit_t **mbg_group_hit(mbg_group_t *g) { return g->b.hit; }
+
+void mbg_group_free(mbg_group_t *g)
+{
+	if (g == 0) return;
+	map_free(&g->b);
+	free(g->b.n_hit); free(g->b.hit); free(g->mt);
+	free(g);
+}
+
+mbs_pool_t *mbg_map_pool(int n_threads) { return pool_for(n_threads); }
+void mbg_map_round(void *data, void **reqs, int64_t n) { round_cb(data, reqs, n); }
+int64_t mbg_map_flush(void) { return g_flush; }
 
 /* ---- index --------------------------------------------------------------------------------------- */
 

@@ -21,8 +21,13 @@
  *
  * Stacks come from mmap'd chunks (MAP_NORESERVE: only touched pages take memory).  Guard pages would
  * cost two memory mappings per stack, and a run can hold hundreds of thousands of suspended coroutines,
- * beyond the kernel's mapping limit; instead the lowest bytes of each stack hold a canary that is
- * checked whenever its coroutine suspends or finishes, and a damaged canary aborts.
+ * beyond the kernel's mapping limit; instead the lowest bytes of each stack hold a canary that each
+ * thread checks over all of its stacks after every run, and a damaged canary aborts.
+ *
+ * An open run (mbs_open) takes tasks while it runs: mbs_add makes more indices runnable and wakes the
+ * idle workers, and the run ends once it is closed and every added task has finished.  Chunks are
+ * claimed with a compare-and-swap that never moves past the tasks added so far, so no index is lost
+ * to a claim made before it was added.
  */
 #define _GNU_SOURCE
 #include <pthread.h>
@@ -85,7 +90,9 @@ struct mbs_pool_s {
 	int n_finished, stop;
 	pthread_t disp;
 	/* the current run */
-	int64_t n_tasks, chunk, flush;
+	atomic_llong n_tasks;     /* tasks added so far */
+	int64_t chunk, flush;
+	atomic_int open_run;      /* the run still takes tasks (mbs_add) */
 	atomic_llong next;
 	mbs_task_f task;
 	mbs_batch_f batch;
@@ -97,6 +104,7 @@ struct mbs_pool_s {
 	pthread_cond_t dcv;
 	plist_t batch_l;          /* the dispatcher's batch */
 	int64_t *toff;            /* per thread: where its requests start in the batch */
+	double run_t0;            /* start of the current run */
 	/* statistics */
 	int64_t batches, co_peak;
 	double batch_s, task_s;
@@ -244,7 +252,7 @@ static void finish(thr_t *t, co_t *co)
 	co->next_free = t->free_co;
 	t->free_co = co;
 	/* this thread's last task while nothing is left to claim: possibly the run's last */
-	if (atomic_fetch_sub(&t->live, 1) == 1 && atomic_load(&p->next) >= p->n_tasks) poke(p);
+	if (atomic_fetch_sub(&t->live, 1) == 1 && atomic_load(&p->next) >= atomic_load(&p->n_tasks)) poke(p);
 }
 
 static void resume(thr_t *t, co_t *co)
@@ -292,11 +300,17 @@ static void run_job(thr_t *t)
 		progressed = n > 0;
 		/* 2. a chunk of new tasks; live is raised before the claim, so that the dispatcher never sees
 		 * every task claimed and none live while a claimed chunk has not started */
-		if (atomic_load(&p->next) < p->n_tasks) {
+		i0 = atomic_load(&p->next);
+		if (i0 < atomic_load(&p->n_tasks)) {
+			int64_t nt = 0, e = 0;
 			atomic_fetch_add(&t->live, p->chunk);
-			i0 = atomic_fetch_add(&p->next, p->chunk);
-			if (i0 < p->n_tasks) {
-				int64_t e = i0 + p->chunk < p->n_tasks ? i0 + p->chunk : p->n_tasks;
+			for (;;) {                      /* [i0, e): at most a chunk, never past the tasks added */
+				nt = atomic_load(&p->n_tasks);
+				if (i0 >= nt) break;
+				e = i0 + p->chunk < nt ? i0 + p->chunk : nt;
+				if (atomic_compare_exchange_weak(&p->next, &i0, e)) break;
+			}
+			if (i0 < nt) {
 				if (e - i0 < p->chunk) atomic_fetch_sub(&t->live, p->chunk - (e - i0));
 				for (i = i0; i < e; ++i) start(t, get_co(t), i);
 				progressed = 1;
@@ -304,17 +318,19 @@ static void run_job(thr_t *t)
 		}
 		t->busy_s += now_s() - t0;
 		if (progressed) continue;
-		/* 3. nothing to run: sleep until a request of this thread finishes, or the run ends */
+		/* 3. nothing to run: sleep until a request of this thread finishes, tasks are added, or the run
+		 * ends (mbs_add signals each thread under its lock after adding, so no addition is missed) */
 		if (atomic_load(&p->all_done)) break;
 		t0 = now_s();
 		pthread_mutex_lock(&t->mu);
-		if (t->n_ready == 0 && !atomic_load(&p->all_done)) {
+		if (t->n_ready == 0 && !atomic_load(&p->all_done) && atomic_load(&p->next) >= atomic_load(&p->n_tasks)) {
 			t->waiting = 1;
 			atomic_fetch_add(&p->n_idle, 1);
 			pthread_mutex_unlock(&t->mu);
 			poke(p);
 			pthread_mutex_lock(&t->mu);
-			while (t->n_ready == 0 && !atomic_load(&p->all_done)) pthread_cond_wait(&t->cv, &t->mu);
+			while (t->n_ready == 0 && !atomic_load(&p->all_done) && atomic_load(&p->next) >= atomic_load(&p->n_tasks))
+				pthread_cond_wait(&t->cv, &t->mu);
 			atomic_fetch_sub(&p->n_idle, 1);
 			t->waiting = 0;
 		}
@@ -334,7 +350,7 @@ static void dispatch_job(mbs_pool_t *p)
 		for (;;) {
 			int64_t pend = posted_sum(p);
 			int idle = atomic_load(&p->n_idle);
-			if (atomic_load(&p->next) >= p->n_tasks && live_sum(p) == 0) { done = 1; break; }
+			if (!atomic_load(&p->open_run) && atomic_load(&p->next) >= atomic_load(&p->n_tasks) && live_sum(p) == 0) { done = 1; break; }
 			if (pend > 0 && (pend >= p->flush || idle == p->n || (idle * 2 >= p->n && pend * 8 >= p->flush))) break;
 			{
 				struct timespec ts;
@@ -494,30 +510,71 @@ void mbs_pool_destroy(mbs_pool_t *p)
 
 int mbs_pool_size(const mbs_pool_t *p) { return p->n; }
 
-void mbs_run(mbs_pool_t *p, int64_t n, int64_t chunk, mbs_task_f task, mbs_batch_f batch, void *data, int64_t flush)
+/* starts a run of n tasks (open: more may be added) and returns; called with run_mu held */
+static void run_start(mbs_pool_t *p, int64_t n, int open, int64_t chunk, mbs_task_f task, mbs_batch_f batch, void *data, int64_t flush)
 {
-	double t0;
-	if (n <= 0) return;
-	pthread_mutex_lock(&p->run_mu);
-	t0 = now_s();
+	int k;
+	p->run_t0 = now_s();
 	pthread_mutex_lock(&p->mu);
-	p->n_tasks = n, p->chunk = chunk > 0 ? chunk : 1, p->flush = flush > 0 ? flush : 16384;
+	atomic_store(&p->n_tasks, n);
+	atomic_store(&p->open_run, open);
+	p->chunk = chunk > 0 ? chunk : 1, p->flush = flush > 0 ? flush : 16384;
 	p->flush_thr = p->flush / p->n > 0 ? p->flush / p->n : 1;
 	atomic_store(&p->next, 0);
-	{
-		int k;
-		for (k = 0; k < p->n; ++k) atomic_store(&p->thr[k].live, 0), atomic_store(&p->thr[k].n_posted, 0);
-	}
+	for (k = 0; k < p->n; ++k) atomic_store(&p->thr[k].live, 0), atomic_store(&p->thr[k].n_posted, 0);
 	atomic_store(&p->n_idle, 0);
 	atomic_store(&p->all_done, 0);
 	p->task = task, p->batch = batch, p->data = data;
 	p->n_finished = 0;
 	++p->gen;
 	pthread_cond_broadcast(&p->cv_start);
+	pthread_mutex_unlock(&p->mu);
+}
+
+/* waits for the end of the run started by run_start, then releases run_mu */
+static void run_end(mbs_pool_t *p)
+{
+	pthread_mutex_lock(&p->mu);
 	while (p->n_finished < p->n + 1) pthread_cond_wait(&p->cv_done, &p->mu);
 	pthread_mutex_unlock(&p->mu);
-	p->task_s += now_s() - t0;
+	p->task_s += now_s() - p->run_t0;
 	pthread_mutex_unlock(&p->run_mu);
+}
+
+void mbs_run(mbs_pool_t *p, int64_t n, int64_t chunk, mbs_task_f task, mbs_batch_f batch, void *data, int64_t flush)
+{
+	if (n <= 0) return;
+	pthread_mutex_lock(&p->run_mu);
+	run_start(p, n, 0, chunk, task, batch, data, flush);
+	run_end(p);
+}
+
+void mbs_open(mbs_pool_t *p, int64_t chunk, mbs_task_f task, mbs_batch_f batch, void *data, int64_t flush)
+{
+	pthread_mutex_lock(&p->run_mu);
+	run_start(p, 0, 1, chunk, task, batch, data, flush);
+}
+
+int64_t mbs_add(mbs_pool_t *p, int64_t n)
+{
+	int k;
+	int64_t first;
+	if (n <= 0) return atomic_load(&p->n_tasks);
+	first = atomic_fetch_add(&p->n_tasks, n);
+	for (k = 0; k < p->n; ++k) {            /* under each thread's lock: a thread about to sleep sees the tasks */
+		thr_t *t = &p->thr[k];
+		pthread_mutex_lock(&t->mu);
+		pthread_cond_signal(&t->cv);
+		pthread_mutex_unlock(&t->mu);
+	}
+	return first;
+}
+
+void mbs_close(mbs_pool_t *p)
+{
+	atomic_store(&p->open_run, 0);
+	poke(p);
+	run_end(p);
 }
 
 void mbs_wait(void *req)

@@ -41,6 +41,14 @@ int mbs_pool_size(const mbs_pool_t *p);
  * of pending requests that starts a batch while workers still have work (0: a default). */
 void mbs_run(mbs_pool_t *p, int64_t n, int64_t chunk, mbs_task_f task, mbs_batch_f batch, void *data, int64_t flush);
 
+/* An open run: mbs_open starts a run of no tasks and returns; mbs_add(p, n) makes tasks [k, k + n)
+ * runnable, where k (returned) is the number of tasks added before, and may be called from any thread
+ * while the run is open (from inside a task too); mbs_close waits until every added task has finished
+ * and ends the run.  One run (mbs_run or open) per pool at a time. */
+void mbs_open(mbs_pool_t *p, int64_t chunk, mbs_task_f task, mbs_batch_f batch, void *data, int64_t flush);
+int64_t mbs_add(mbs_pool_t *p, int64_t n);
+void mbs_close(mbs_pool_t *p);
+
 /* Inside a task: post req and suspend until the batch holding it has run. */
 void mbs_wait(void *req);
 /* Whether the caller runs inside a task coroutine, and the pool thread it runs on (-1 outside). */
