@@ -87,8 +87,8 @@ def main():
                     help="only run the plain copy kernel over MiB mebibytes (HBM byte-counter calibration)")
     ap.add_argument("--serve-qps", type=float, default=4000.0,
                     help="request rate of the serving phase (Poisson arrivals)")
-    ap.add_argument("--serve-requests", type=int, default=600, help="requests per offered rate in the serving phase (0: skip it)")
-    ap.add_argument("--serve-sweep", default="250,500,1000,1500,2000",
+    ap.add_argument("--serve-requests", type=int, default=1500, help="requests per offered rate in the serving phase (0: skip it)")
+    ap.add_argument("--serve-sweep", default="250,500,1000,2000,3000",
                     help="offered request rates (requests/s) of the mapper's serving sweep")
     ap.add_argument("--gather", action="store_true",
                     help="after building the index, only measure the dependent random 64-B gather ceiling of its occ table")
@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--long-reads", type=int, default=0, metavar="N",
                     help="the mapper on N single-end long reads (log-uniform 100-20,000 bp, 0.1%% substitutions, "
                          "0.05%% indels) with PAF output (`map -f`) instead of the read pairs; reports reads/s")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner (launch shapes and batch sizes of the mapper) and write minibwa_amd/tuned.env")
     ap.add_argument("--fastq-batch", type=int, default=0, metavar="N",
                     help="only run the FASTQ reader (host): N 150 bp reads in one file, plain and gzip, in the CLI's "
                          "100M-base chunks; reports records/s beside the reference's mb_bseq_read")
@@ -133,6 +135,9 @@ def main():
 
     if args.fastq_batch:
         fastq_batch(args)
+        return
+    if args.autotune:
+        autotune(args)
         return
     if args.dp_batch:
         dp_batch(args)
@@ -310,21 +315,9 @@ def _kv(line):
     return out
 
 
-def mapper_bench(args):
-    """The end-to-end mapper: W + K passes of FASTQ -> SAM in one process (index loaded once), the
-    reference CLI beside it, and a byte-for-byte SAM check between them."""
-    from minibwa_amd import synth
-    exe = os.path.join(ROOT, "minibwa_amd", "bin", "mbg-map-bench")
-    cli = os.path.join(ROOT, "minibwa_amd", "bin", "minibwa-gpu")
-    ref_cli = os.path.join(ROOT, "oracle", "_ref", "minibwa")
-    for e in (exe, cli):
-        if not os.path.exists(e):
-            log(f"{e} is not built (make -C minibwa_amd/integration); the GPU mapper cannot run")
-            sys.exit(1)
-    cores = args.threads or host_cores()
-    t0 = time.time()
-    ref = synth.reference(args.ref_len, seed=11, rep_frac=args.rep_frac, rep_min=300, rep_max=6000, rep_div=0.02)
-    log(f"reference {len(ref)} bp in {time.time() - t0:.1f}s")
+def mapper_index(args):
+    """The workload's index (built on the device by a child process), reused from args.workdir when an
+    earlier run built the same reference there; returns (workdir, index prefix)."""
     work = args.workdir or tempfile.mkdtemp(prefix="mbg_bench_")
     os.makedirs(work, exist_ok=True)
     prefix = os.path.join(work, "ref")
@@ -342,6 +335,107 @@ def mapper_bench(args):
             sys.exit(1)
         open(pfile, "w").write(params)
         log(r.stdout.strip(), f"(wall {time.time() - t0:.1f}s)")
+    return work, prefix
+
+
+# The autotuner's knobs: (environment setting, the built-in default, candidate values, workload,
+# objective).  Objectives: "wall" the pass time, "seed" the device time of seeding + anchors (SA walks),
+# "dp" that of the DP kernels.  Every knob changes launch shapes or batch sizes only, never results.
+TUNE_KNOBS = [
+    ("MBG_GATHER_BPC", "8", ["4", "6", "8", "12", "16"], "pairs", "seed"),   # 256-lane seeding/SA blocks per CU
+    ("MBG_DP_WPC", "32", ["8", "16", "32", "64"], "pairs", "dp"),           # DP waves per CU
+    ("MBG_FLUSH", "4096", ["2048", "4096", "8192"], "pairs", "wall"),      # pending DP requests that launch a batch
+    ("MBG_PRE_LEN", "15", ["13", "14", "15"], "pairs", "seed"),             # k of the k-mer interval cache
+    ("MBG_SEED_SEG", "512", ["0", "256", "512", "1024"], "long", "seed"),   # long-read segment length (0: one lane per read)
+]
+TUNE_MARGIN = 0.03        # a value replaces the default only when it beats it by this fraction
+
+
+def autotune(args):
+    """A coordinate search over TUNE_KNOBS on the mapper itself: for each knob in turn, every candidate
+    value runs the mapper (3 passes of the workload, index loaded once; the objective is the mean of
+    the last two passes) with the values chosen so far for the knobs before it, twice, keeping the
+    better run; the value with the lowest objective is kept if it beats the default by TUNE_MARGIN
+    (run-to-run noise is a few percent).  The choices go to minibwa_amd/tuned.env, which the mapper
+    programs read at start-up (a setting in the environment wins), and every trial to the JSON line
+    on stdout."""
+    from minibwa_amd import synth
+    exe = os.path.join(ROOT, "minibwa_amd", "bin", "mbg-map-bench")
+    if not os.path.exists(exe):
+        log(f"{exe} is not built (make -C minibwa_amd/integration)")
+        sys.exit(1)
+    cores = args.threads or host_cores()
+    ref = synth.reference(args.ref_len, seed=11, rep_frac=args.rep_frac, rep_min=300, rep_max=6000, rep_div=0.02)
+    work, prefix = mapper_index(args)
+    r1, r2 = synth.read_pairs(ref, args.pairs, seed=11, rlen=150, frag_mean=400, frag_sd=50, sub=0.01, indel=0.001)
+    f1, f2 = os.path.join(work, "r1.fq"), os.path.join(work, "r2.fq")
+    synth.write_pairs_fastq(f1, f2, r1, r2, prefix="p_")
+    del r1, r2
+    n_long = 20000
+    seq, off, ln = synth.single_reads(ref, n_long, seed=13, lmin=100, lmax=20000, sub=0.001, indel=0.0005)
+    fl = os.path.join(work, "long_tune.fq")
+    synth.write_fastq(fl, seq, off, ln, prefix="l_")
+    del seq, off, ln, ref
+    inputs = {"pairs": ([f1, f2], []), "long": ([fl], ["-f"])}
+
+    def trial(settings, load):
+        env = dict(os.environ, MBG_MAP_STATS="1", MBG_TUNE="none", **settings)
+        fins, fopt = inputs[load]
+        r = subprocess.run([exe] + fopt + ["-t", str(cores), "-P", "3", "-O", "/dev/null", prefix] + fins, capture_output=True,
+                           text=True, env=env, timeout=900)
+        if r.returncode != 0:
+            log("trial failed:", settings, r.stderr[-1500:])
+            return None
+        pass_s = [float(l.split()[3]) for l in r.stderr.split("\n") if l.startswith("[mbg_bench] pass ")]
+        summ = _kv([l for l in r.stderr.split("\n") if l.startswith("[mbg_map] summary:")][-1])
+        seed_ms = (summ.get("dev_seed1_ms", 0) + summ.get("dev_seed2_ms", 0) + summ.get("dev_anchor_ms", 0)) / 3
+        return {"wall": (pass_s[1] + pass_s[2]) / 2, "seed": seed_ms, "dp": summ.get("dev_dp_ms", 0) / 3}
+
+    best, trials = {}, []
+    for key, default, values, load, obj in TUNE_KNOBS:
+        scores = {}
+        for v in values:
+            runs = [m for m in (trial(dict(best, **{key: v}), load) for _ in range(2)) if m is not None]
+            if not runs:
+                continue
+            m = min(runs, key=lambda x: x[obj])
+            scores[v] = m
+            trials.append({"knob": key, "value": v, "workload": load, "objective": obj, **{k: round(x, 4) for k, x in m.items()}})
+            log(f"tune {key}={v} ({load}): wall {m['wall']:.3f} s, seeding {m['seed']:.2f} ms, DP {m['dp']:.2f} ms")
+        if scores:
+            pick = min(scores, key=lambda v: scores[v][obj])
+            if default in scores and scores[pick][obj] > (1 - TUNE_MARGIN) * scores[default][obj]:
+                pick = default
+            best[key] = pick
+            log(f"tune {key}: {best[key]} (by {obj})")
+    out = os.path.join(ROOT, "minibwa_amd", "tuned.env")
+    with open(out, "w") as f:
+        f.write("# Written by `python bench.py --autotune` (coordinate search over launch shapes and batch sizes,\n"
+                f"# {args.pairs} pairs of 2x150 bp and {n_long} long reads on the {args.ref_len / 1e9:.1f} Gbp synthetic reference,\n"
+                f"# {cores} host threads).  Read by the mapper programs at start-up; the environment wins.\n")
+        for k, v in best.items():
+            f.write(f"{k}={v}\n")
+    log(f"wrote {out}")
+    print(json.dumps({"autotune": best, "trials": trials, "workload": {"pairs": args.pairs, "long_reads": n_long,
+                      "ref_bp": args.ref_len, "host_threads": cores}}))
+
+
+def mapper_bench(args):
+    """The end-to-end mapper: W + K passes of FASTQ -> SAM in one process (index loaded once), the
+    reference CLI beside it, and a byte-for-byte SAM check between them."""
+    from minibwa_amd import synth
+    exe = os.path.join(ROOT, "minibwa_amd", "bin", "mbg-map-bench")
+    cli = os.path.join(ROOT, "minibwa_amd", "bin", "minibwa-gpu")
+    ref_cli = os.path.join(ROOT, "oracle", "_ref", "minibwa")
+    for e in (exe, cli):
+        if not os.path.exists(e):
+            log(f"{e} is not built (make -C minibwa_amd/integration); the GPU mapper cannot run")
+            sys.exit(1)
+    cores = args.threads or host_cores()
+    t0 = time.time()
+    ref = synth.reference(args.ref_len, seed=11, rep_frac=args.rep_frac, rep_min=300, rep_max=6000, rep_div=0.02)
+    log(f"reference {len(ref)} bp in {time.time() - t0:.1f}s")
+    work, prefix = mapper_index(args)
     t0 = time.time()
     long_mode = args.long_reads > 0
     if long_mode:      # single-end long reads, PAF output
@@ -359,6 +453,13 @@ def mapper_bench(args):
         del r1, r2
         log(f"{args.pairs} pairs written to FASTQ in {time.time() - t0:.1f}s")
     unit = "long reads/s" if long_mode else "pairs/s"
+    tuned = {}
+    tpath = os.path.join(ROOT, "minibwa_amd", "tuned.env")
+    if os.path.exists(tpath):          # the autotuner's settings, which the mapper programs read at start-up
+        for l in open(tpath):
+            if l.startswith("MBG_") and "=" in l:
+                k, v = l.strip().split("=", 1)
+                tuned[k] = v
 
     # the GPU mapper: warmup + timed passes in one process; pass 0 writes the SAM compared below
     gpu_sam = os.path.join(work, "gpu.sam")
@@ -505,7 +606,8 @@ def mapper_bench(args):
         "config": {"workload": workload,
                    "ref_bp": args.ref_len, "pairs": 0 if long_mode else args.pairs, "long_reads": args.long_reads,
                    "read_bases": tot_bases, "read_len": "100-20000" if long_mode else 150, "contigs": args.contigs,
-                   "host_threads": cores, "options": "minibwa map defaults (adaptive preset)", "mini_batches_per_pass": n_front_per_pass},
+                   "host_threads": cores, "options": "minibwa map defaults (adaptive preset)", "mini_batches_per_pass": n_front_per_pass,
+                   "tuned": tuned},
         "roofline": roofline,
         "roofline_kernels": kernels,
         "cpu_baseline": cpu,
@@ -525,9 +627,11 @@ def serve_mapper(args, ref, prefix, work, cores):
     """Served mapping on the batched mapper (minibwa_amd/bin/mbg-serve-bench, the serving loop of
     minibwa_amd/integration/mbg_serve.c): requests arrive as a Poisson process at each offered rate of
     the sweep; a request is 1-10,000 read pairs (log-uniform) of the 2x150 bp model, or (5% of the
-    requests) one 10 kb read.  Batches form continuously under the device-memory budget (half the free
-    HBM), pairs with predefined insert-size statistics (-I 400,50) so that every request's records are
-    the same in any batch.  Latency = the request's records back on the host - its arrival."""
+    requests) one 10 kb read.  Requests are admitted continuously, in groups whose front passes run on
+    the device while earlier groups' reads map in the same open run of the worker pool, under the
+    device-memory budget (half the free HBM); pairs use predefined insert-size statistics (-I 400,50)
+    so that every request's records are the same whatever else is in flight.  Latency = the request's
+    records back on the host - its arrival."""
     from minibwa_amd import serve_workload as W
     exe = os.path.join(ROOT, "minibwa_amd", "bin", "mbg-serve-bench")
     if not os.path.exists(exe):

@@ -54,6 +54,10 @@ struct mbg_ctx_s {
 	// bound by the longest exact stretch, which the lane holding it walks serially either way
 	int32_t seg_len = 512, seg_slots = 16, seg_chain = 64, fix_slots = 16;
 	int64_t cap_seg = 0;
+	// launch shapes (MBG_GATHER_BPC, MBG_DP_WPC; the autotuner's knobs): 256-lane blocks per CU of the
+	// seeding and SA kernels, and DP waves per CU; every such kernel takes its work from a queue, so
+	// the shape changes only the speed
+	int32_t gather_bpc = 8, dp_wpc = 32;
 	DBuf m_buf[12];         // mate rescue
 	DBuf a_buf[8];          // anchors
 	DBuf c_buf[8];          // chains
@@ -222,6 +226,11 @@ int mbg_init(int device, mbg_ctx_t **out)
 			c->seg_len = v[0] >= 32 ? v[0] : 0;
 			c->seg_slots = v[1] > 0 ? v[1] : 1, c->seg_chain = v[2] > 0 ? v[2] : 1, c->fix_slots = v[3] > 0 ? v[3] : 1;
 		}
+	}
+	{
+		const char *g = getenv("MBG_GATHER_BPC"), *d = getenv("MBG_DP_WPC");
+		if (g && atoi(g) >= 1 && atoi(g) <= 64) c->gather_bpc = atoi(g);
+		if (d && atoi(d) >= 1 && atoi(d) <= 128) c->dp_wpc = atoi(d);
 	}
 	hipDeviceProp_t prop;
 	hipError_t e = hipGetDeviceProperties(&prop, device);
@@ -520,6 +529,17 @@ int mbg_extend(mbg_ctx_t *c, int64_t n, const mbg_sai_t *ik, const uint8_t *is_b
 	return MBG_OK;
 }
 
+// the grid of the seeding and SA kernels: n_items lanes of work in 256-lane blocks, at most
+// gather_bpc blocks per CU (their queues hand out the rest)
+static unsigned gather_grid(mbg_ctx_t *c, int64_t n_items)
+{
+	int64_t want = (n_items + 255) / 256;
+	const int64_t cap = (int64_t)c->n_cu * c->gather_bpc;
+	if (want > cap) want = cap;
+	if (want < 1) want = 1;
+	return (unsigned)want;
+}
+
 static unsigned persistent_grid(mbg_ctx_t *c, int64_t n_items, int per_block)
 {
 	int64_t want = (n_items + per_block - 1) / per_block;
@@ -596,7 +616,7 @@ static int seg_run(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int32_t 
 {
 	const int64_t cs = c->cap_seg;
 	uint32_t *gc = (uint32_t*)c->g_ctr.p;
-	const unsigned grid = persistent_grid(c, cs < n_reads ? cs : n_reads, 256);
+	const unsigned grid = gather_grid(c, cs < n_reads ? cs : n_reads);
 	SeedArgs s3 = a;                       // segments k >= 1
 	s3.next = gc; s3.n_tasks = 0xffffffffu; s3.n_tasks_dev = (const uint32_t*)c->g_base.p + n_reads; s3.task_cap = cs;
 	s3.slots = (SeedRec*)c->g_slots.p; s3.slot_cnt = (uint32_t*)c->g_cnt.p; s3.kslot = c->seg_slots;
@@ -655,7 +675,7 @@ static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_
 	const bool seg = c->seg_len > 0 && n_reads > 0;
 	if (seg) TRY(seg_setup(c, st, n_reads, len, a));
 	if (ev_pre) HIPCHK(hipEventRecord(ev_pre, st));
-	if (n_reads > 0) HIPCHK(launch_smem(c->ix, a, 1, persistent_grid(c, n_reads, 256), st));
+	if (n_reads > 0) HIPCHK(launch_smem(c->ix, a, 1, gather_grid(c, n_reads), st));
 	if (seg) TRY(seg_run(c, st, n_reads, len, a));
 	if (ev) HIPCHK(hipEventRecord(ev[0], st));
 	if (n_reads > 0) {
@@ -666,7 +686,7 @@ static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_
 		a2.tasks = (const SeedTask*)c->b_task.p; a2.n_tasks = 0xffffffffu; a2.n_tasks_dev = ctr + 1; a2.task_cap = tc;
 		a2.slots = (SeedRec*)c->s_slots2.p; a2.slot_cnt = cnt + n_reads; a2.kslot = kSlot2;
 		a2.stats = stats ? stats + 3 : nullptr;
-		HIPCHK(launch_smem(c->ix, a2, 2, persistent_grid(c, n_reads, 256), st));
+		HIPCHK(launch_smem(c->ix, a2, 2, gather_grid(c, n_reads), st));
 	}
 	HIPCHK(scan_u32(c->s_tmp.p, &tmp_bytes, cnt, (uint32_t*)c->s_off.p, n_cnt, st));
 	HIPCHK(launch_compact((const SeedRec*)c->s_slots1.p, (const SeedRec*)c->s_slots2.p, kSlot1, kSlot2, n_reads, n_cnt, cnt,
@@ -793,7 +813,7 @@ static int sa_dev_locked(mbg_ctx_t *c, hipStream_t st, int64_t n, uint64_t *x)
 	TRY(dgrow(c->b_ctr, 128));
 	uint32_t *next = (uint32_t*)c->b_ctr.p + 4;
 	HIPCHK(hipMemsetAsync(next, 0, 4, st));
-	HIPCHK(launch_sa(c->ix, n, nullptr, x, next, nullptr, persistent_grid(c, n, 256), st));
+	HIPCHK(launch_sa(c->ix, n, nullptr, x, next, nullptr, gather_grid(c, n), st));
 	return MBG_OK;
 }
 
@@ -827,7 +847,7 @@ int mbg_sa_batch_dev_noalloc(mbg_ctx_t *c, void *stream, int64_t n, uint64_t *x)
 	TRY(ctx_order(c, st));
 	hipLaunchKernelGGL(noalloc_clear_kernel, dim3(1), dim3(kNoAllocWords), 0, st, ptr, 1);
 	HIPCHK(hipGetLastError());
-	HIPCHK(launch_sa(c->ix, n, nullptr, x, ptr, nullptr, persistent_grid(c, n, 256), st));
+	HIPCHK(launch_sa(c->ix, n, nullptr, x, ptr, nullptr, gather_grid(c, n), st));
 	return ctx_mark(c, st);
 }
 
@@ -891,7 +911,7 @@ static int dp_dev_locked(mbg_ctx_t *c, hipStream_t st, bool dual, int32_t n_jobs
 {
 	if (n_jobs <= 0) return MBG_OK;
 	// resident blocks (one wave each): enough to fill the GPU, bounded by 2 GiB of backtrack slabs
-	int64_t g = (int64_t)c->n_cu * 32;
+	int64_t g = (int64_t)c->n_cu * c->dp_wpc;
 	if (g > n_jobs) g = n_jobs;
 	const int64_t max_by_mem = ((int64_t)2 << 30) / slab;
 	if (g > max_by_mem) g = max_by_mem;

@@ -60,6 +60,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 #include "kalloc.h"
 #include "bwt.h"
 #include "bseq.h"
@@ -1020,10 +1021,47 @@ int64_t mbg_map_flush(void) { return g_flush; }
 
 /* ---- index --------------------------------------------------------------------------------------- */
 
+/* The autotuner's choices (bench.py --autotune): KEY=VALUE lines of MBG_ settings in the file named by
+ * MBG_TUNE, or else in tuned.env beside the program's directory (minibwa_amd/tuned.env for the
+ * programs in minibwa_amd/bin); a setting already in the environment wins.  MBG_TUNE=none reads none.
+ * The settings change launch shapes and batch sizes, never results. */
+static void tune_load(void)
+{
+	static int done;
+	char path[4096], line[512];
+	const char *e = getenv("MBG_TUNE");
+	FILE *fp;
+	if (done) return;
+	done = 1;
+	if (e && strcmp(e, "none") == 0) return;
+	if (e && *e) snprintf(path, sizeof(path), "%s", e);
+	else {
+		ssize_t n = readlink("/proc/self/exe", path, sizeof(path) - 32);
+		char *sl;
+		if (n <= 0) return;
+		path[n] = 0;
+		if ((sl = strrchr(path, '/')) == 0) return;
+		strcpy(sl, "/../tuned.env");
+	}
+	if ((fp = fopen(path, "r")) == 0) return;
+	while (fgets(line, sizeof(line), fp)) {
+		char *eq = strchr(line, '='), *v, *nl;
+		if (line[0] == '#' || eq == 0 || strncmp(line, "MBG_", 4) != 0) continue;
+		*eq = 0, v = eq + 1;
+		if ((nl = strpbrk(v, "\r\n")) != 0) *nl = 0;
+		setenv(line, v, 0);
+	}
+	fclose(fp);
+}
+
 mb_idx_t *__real_mb_idx_load(const char *prefix, int32_t is_meth);
 mb_idx_t *__wrap_mb_idx_load(const char *prefix, int32_t is_meth)
 {
-	mb_idx_t *idx = __real_mb_idx_load(prefix, is_meth);
+	mb_idx_t *idx;
+	pthread_mutex_lock(&g_mu);
+	tune_load();
+	pthread_mutex_unlock(&g_mu);
+	idx = __real_mb_idx_load(prefix, is_meth);
 	if (idx) {
 		pthread_mutex_lock(&g_mu);
 		mbx_index(idx);

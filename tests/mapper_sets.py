@@ -45,8 +45,8 @@ def _files(tmp_path_factory):
     return f
 
 
-def _run(exe, args, threads):
-    env = dict(os.environ, MBG_MAP_STATS="1", MBG_MAP_THREADS=str(threads))
+def _run(exe, args, threads, extra_env=None):
+    env = dict(os.environ, MBG_MAP_STATS="1", MBG_MAP_THREADS=str(threads), **(extra_env or {}))
     print("[mapper]", exe, " ".join(args), flush=True)
     r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=600, env=env)
     assert r.returncode == 0, r.stderr[-3000:]
@@ -57,35 +57,35 @@ def _run(exe, args, threads):
     return r.stdout, st
 
 
-def check_sets(cli, api, name, tmp_path_factory, threads=4):
+def check_sets(cli, api, name, tmp_path_factory, threads=4, extra_env=None):
     """Runs one set through the mapper programs and checks the reference's md5; returns the run's
-    summary counters."""
+    summary counters.  extra_env: settings added to the programs' environment."""
     gold = json.load(open(os.path.join(GOLDEN, "golden.json")))
     f = _files(tmp_path_factory)
     t = ["-t%d" % threads]
     if name == "chrM":
         out, st = _run(cli, ["map"] + t + [os.path.join(GOLDEN, "chrM"), os.path.join(GOLDEN, "chrM-read_1.fa.gz"),
-                                            os.path.join(GOLDEN, "chrM-read_2.fa.gz")], threads)
+                                            os.path.join(GOLDEN, "chrM-read_2.fa.gz")], threads, extra_env)
         assert _body_md5(out) == gold["chrM.sam.body.md5"]
     elif name == "cfg0":
-        out, st = _run(cli, ["map"] + t + [os.path.join(GOLDEN, "cfg0"), f["c1"], f["c2"]], threads)
+        out, st = _run(cli, ["map"] + t + [os.path.join(GOLDEN, "cfg0"), f["c1"], f["c2"]], threads, extra_env)
         assert sum(1 for l in out.split("\n") if l and not l.startswith("@")) == gold["cfg0.sam.n_lines"]
         assert _body_md5(out) == gold["cfg0.sam.body.md5"]
     elif name == "rescue":
-        out, st = _run(cli, ["map"] + t + [os.path.join(GOLDEN, "cfg0"), f["c1"], f["r2"]], threads)
+        out, st = _run(cli, ["map"] + t + [os.path.join(GOLDEN, "cfg0"), f["c1"], f["r2"]], threads, extra_env)
         assert _body_md5(out) == gold["rescue.sam.body.md5"]
         assert int(st["ll_jobs"]) == gold["rescue.n_ll_calls"]
     elif name == "batch_chrM":
-        out, st = _run(api, [os.path.join(GOLDEN, "chrM"), os.path.join(GOLDEN, "chrM-read_1.fa.gz")], threads)
+        out, st = _run(api, [os.path.join(GOLDEN, "chrM"), os.path.join(GOLDEN, "chrM-read_1.fa.gz")], threads, extra_env)
         assert hashlib.md5(out.encode()).hexdigest() == gold["chrM.batch.md5"]
     elif name == "batch_cfg0":
-        out, st = _run(api, [os.path.join(GOLDEN, "cfg0"), f["c1"]], threads)
+        out, st = _run(api, [os.path.join(GOLDEN, "cfg0"), f["c1"]], threads, extra_env)
         assert hashlib.md5(out.encode()).hexdigest() == gold["cfg0.batch.md5"]
     elif name == "repeat":
-        out, st = _run(cli, ["map"] + t + [f["rep"], f["p1"], f["p2"]], threads)
+        out, st = _run(cli, ["map"] + t + [f["rep"], f["p1"], f["p2"]], threads, extra_env)
         assert _body_md5(out) == gold["repeat.sam.body.md5"]
     elif name == "repeat_long":
-        out, st = _run(cli, ["map"] + t + [f["rep"], f["lr"]], threads)
+        out, st = _run(cli, ["map"] + t + [f["rep"], f["lr"]], threads, extra_env)
         assert _body_md5(out) == gold["repeat.long.sam.body.md5"]
     else:
         raise KeyError(name)

@@ -101,3 +101,14 @@ def test_long_reads_paf_match_reference(exes, tmp_path_factory):
     assert "formatter native" in got.stderr
     assert "\tcg:Z:" in got.stdout
     assert _body_md5(got.stdout) == _body_md5(ref.stdout)
+
+
+def test_tuned_settings_file_changes_nothing(exes, tmp_path_factory, tmp_path):
+    """A tuned.env file (read at start-up through MBG_TUNE) with extreme batching settings leaves the
+    SAM as the reference's: the settings change how requests are batched, never results."""
+    tune = tmp_path / "tuned.env"
+    tune.write_text("# extreme batching\nMBG_FLUSH=7\n")
+    st0 = check_sets(exes[0], exes[1], "rescue", tmp_path_factory, threads=4, extra_env={"MBG_TUNE": "none"})
+    st = check_sets(exes[0], exes[1], "rescue", tmp_path_factory, threads=4, extra_env={"MBG_TUNE": str(tune)})
+    assert st["executor"] == "cpu-reference"
+    assert int(st["batches"]) > int(st0["batches"])      # the file was read: smaller device batches, more of them

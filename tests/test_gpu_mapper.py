@@ -26,3 +26,14 @@ def exes():
 def test_gpu_mapper_matches_reference(exes, tmp_path_factory, name):
     st = check_sets(exes[0], exes[1], name, tmp_path_factory, threads=8)
     assert st["executor"] == "gpu"
+
+
+@pytest.mark.parametrize("name", ["cfg0", "rescue", "repeat_long"])
+def test_gpu_mapper_tuned_settings_change_nothing(exes, tmp_path_factory, tmp_path, name):
+    """The autotuner's settings (launch shapes of the seeding, SA and DP kernels, the device batch size,
+    the long-read segment length), read from a tuned.env file at start-up, change speed only: with
+    extreme values the SAM still equals the reference's."""
+    tune = tmp_path / "tuned.env"
+    tune.write_text("# extreme launch shapes\nMBG_GATHER_BPC=1\nMBG_DP_WPC=3\nMBG_FLUSH=100\nMBG_SEED_SEG=64\n")
+    st = check_sets(exes[0], exes[1], name, tmp_path_factory, threads=8, extra_env={"MBG_TUNE": str(tune)})
+    assert st["executor"] == "gpu"
