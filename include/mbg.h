@@ -138,6 +138,37 @@ int mbg_extz2_batch(mbg_ctx_t *ctx, int32_t n_jobs, const mbg_dp_job_t *jobs, co
                     const uint8_t *tseq, int64_t tbytes, int8_t m, const int8_t *mat,
                     mbg_dp_result_t *res, uint32_t *cigar, int64_t cigar_cap);
 
+/* The mb_opt_t fields mb_align_pair reads (align.c:329-378): match a, mismatch b, transition
+ * mismatch b_ts, the two gap pairs, meth = (opt->flag & MB_F_METH) != 0, and max_sw_mat. */
+typedef struct {
+	int32_t a, b, b_ts, q, e, q2, e2, meth;
+	int64_t max_sw_mat;
+} mbg_align_opt_t;
+
+/* One mb_align_pair call: query qseq[qoff, qoff+qlen) against tseq[toff, toff+tlen) (nt4 codes
+ * 0..4), with the band w, end_bonus, zdrop (before the reference's scaling by opt->a) and ksw_flag
+ * as align.c passes them.  cigar_off: the call's CIGAR slots, capacity qlen+tlen+1. */
+typedef struct {
+	int64_t qoff, toff;
+	int32_t qlen, tlen;
+	int32_t w, end_bonus, zdrop, flag;
+	int64_t cigar_off;
+} mbg_align_job_t;
+
+/* Batched mb_align_pair (align.c:329-390, static there; its callers are mb_align1's extensions and
+ * gap fills, align.c:645-735, and mb_align1_inv, :792): the KSW_EZ_GENERIC_SC switch for a
+ * transition mismatch or methylation, the ungapped extension fast path (EXTZ_ONLY and tlen >= qlen:
+ * at most 2 mismatches and the end within end_bonus of the maximum), the ungapped global fast path
+ * (qlen == tlen: at most 3 mismatches or a score above the best gapped one), the band cap from the
+ * mismatch count when the shorter side is under 100, the max_sw_mat skip (zdropped set, no CIGAR),
+ * and the DP the rest reach: ksw_extz2 when q == q2 and e == e2, else ksw_extd2, with zdrop * a.
+ * mat is the 5x5 nt4 matrix the caller built (ksw_gen_nt4_mat, ksw2.h:199-219).  res[i] holds the
+ * ksw_extz_t fields the call leaves (cells: the DP's exact-band cells, 0 when no DP ran); its CIGAR
+ * is at cigar + jobs[i].cigar_off. */
+int mbg_align_pair_batch(mbg_ctx_t *ctx, const mbg_align_opt_t *opt, int32_t n, const mbg_align_job_t *jobs, const uint8_t *qseq,
+                         int64_t qbytes, const uint8_t *tseq, int64_t tbytes, const int8_t *mat, mbg_dp_result_t *res,
+                         uint32_t *cigar, int64_t cigar_cap);
+
 /* One ksw_ll job: query qseq[qoff, qoff+qlen), target tseq[toff, toff+tlen), bytes 0..m-1.  size
  * selects the core as ksw_ll_qinit's size does: 1 = ksw_ll_u8_core, 2 = ksw_ll_i16_core. */
 typedef struct {

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIP_SRC = ["csrc/mbg_api.hip", "csrc/mbg_fm.hip", "csrc/mbg_dp.hip", "csrc/mbg_pipe.hip", "csrc/mbg_scan.hip",
            "csrc/mbg_ll.hip", "csrc/mbg_rescue.hip",
-           "csrc/mbg_extra.hip", "csrc/mbg_anchor.hip", "csrc/mbg_front.hip", "csrc/mbg_io.cpp"]
+           "csrc/mbg_extra.hip", "csrc/mbg_align.hip", "csrc/mbg_anchor.hip", "csrc/mbg_front.hip", "csrc/mbg_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wno-unused-value",
              # packed fp32 VALU ops stay off in every HIP object of this library

@@ -58,6 +58,7 @@ struct mbg_ctx_s {
 	// seeding and SA kernels, and DP waves per CU; every such kernel takes its work from a queue, so
 	// the shape changes only the speed
 	int32_t gather_bpc = 8, dp_wpc = 32;
+	DBuf l_buf[3];          // mb_align_pair batches: DP jobs, decision kinds, decided results
 	DBuf m_buf[12];         // mate rescue
 	DBuf a_buf[8];          // anchors
 	DBuf c_buf[8];          // chains
@@ -281,6 +282,7 @@ void mbg_destroy(mbg_ctx_t *c)
 	                &c->r_cnt, &c->r_off, &c->s_pack, &c->s_qflag, &c->g_nseg, &c->g_base, &c->g_rid, &c->g_long, &c->g_hand0,
 	                &c->g_hand, &c->g_chain, &c->g_chn, &c->g_conv, &c->g_flag, &c->g_slots, &c->g_cnt, &c->g_fix, &c->g_fixcnt,
 	                &c->g_fbl, &c->g_ctr}) dfree(*b);
+	for (auto &b : c->l_buf) dfree(b);
 	for (auto &b : c->m_buf) dfree(b);
 	for (auto &b : c->a_buf) dfree(b);
 	for (auto &b : c->c_buf) dfree(b);
@@ -931,6 +933,51 @@ static int dp_dev_locked(mbg_ctx_t *c, hipStream_t st, bool dual, int32_t n_jobs
 	return MBG_OK;
 }
 
+// results of a DP batch (device res / CIGAR slots at cigar_off) back to the caller, after dev_ev[0..1]
+// bracketed its kernels on c->stream: only the CIGAR operations written cross PCIe, packed on the
+// device at host-computed offsets.  The host jobs give each job's qlen, tlen and cigar_off: either DP
+// jobs (dj; a CIGAR unless KSW_EZ_SCORE_ONLY) or mb_align_pair calls (aj; always a CIGAR slot, since
+// the fast paths write one whatever the flag).
+static int dp_readback(mbg_ctx_t *c, int32_t n_jobs, const DpJob *djobs, const mbg_dp_job_t *dj, const mbg_align_job_t *aj,
+                       const DpRes *dres, mbg_dp_result_t *res, const uint32_t *dcigar, uint32_t *cigar)
+{
+	auto has_cigar = [&](int32_t i) { return aj != nullptr || !(dj[i].flag & 0x01); };
+	auto bound = [&](int32_t i) { return aj ? aj[i].qlen + aj[i].tlen + 1 : dj[i].qlen + dj[i].tlen; };
+	auto coff = [&](int32_t i) { return aj ? aj[i].cigar_off : dj[i].cigar_off; };
+	HIPCHK(hipMemcpyAsync(res, dres, (size_t)n_jobs * sizeof(DpRes), hipMemcpyDeviceToHost, c->stream));
+	HIPCHK(hipStreamSynchronize(c->stream));
+	{
+		float ms = 0;
+		HIPCHK(hipEventElapsedTime(&ms, c->dev_ev[0], c->dev_ev[1]));
+		c->cnt.ms_dp += ms, ++c->cnt.dp_launches, c->cnt.dp_jobs += n_jobs;
+		for (int32_t i = 0; i < n_jobs; ++i) c->cnt.dp_cells += res[i].cells;
+	}
+	if (dcigar == nullptr) return MBG_OK;
+	std::vector<int64_t> poff(n_jobs + 1);
+	int64_t tot = 0;
+	for (int32_t i = 0; i < n_jobs; ++i) {
+		poff[i] = tot;
+		if (has_cigar(i)) {
+			if (res[i].n_cigar < 0 || res[i].n_cigar > bound(i)) return MBG_EHIP;
+			tot += res[i].n_cigar;
+		}
+	}
+	poff[n_jobs] = tot;
+	if (tot > 0) {
+		TRY(dgrow(c->b_in[5], (size_t)(n_jobs + 1) * 8)); TRY(dgrow(c->b_out[2], (size_t)tot * 4));
+		TRY(hgrow(c->h_pinned, (size_t)tot * 4));
+		HIPCHK(hipMemcpyAsync(c->b_in[5].p, poff.data(), (size_t)(n_jobs + 1) * 8, hipMemcpyHostToDevice, c->stream));
+		HIPCHK(launch_cigar_pack(djobs, dres, n_jobs, (const int64_t*)c->b_in[5].p, dcigar, (uint32_t*)c->b_out[2].p, c->stream));
+		HIPCHK(hipMemcpyAsync(c->h_pinned.p, c->b_out[2].p, (size_t)tot * 4, hipMemcpyDeviceToHost, c->stream));
+		HIPCHK(hipStreamSynchronize(c->stream));
+		const uint32_t *pk = (const uint32_t*)c->h_pinned.p;
+		for (int32_t i = 0; i < n_jobs; ++i)
+			if (has_cigar(i) && res[i].n_cigar > 0)
+				memcpy(cigar + coff(i), pk + poff[i], (size_t)res[i].n_cigar * 4);
+	}
+	return MBG_OK;
+}
+
 static int dp_batch(mbg_ctx_t *c, bool dual, int32_t n_jobs, const mbg_dp_job_t *jobs, const uint8_t *qseq, int64_t qbytes,
                     const uint8_t *tseq, int64_t tbytes, int8_t m, const int8_t *mat, mbg_dp_result_t *res, uint32_t *cigar, int64_t cigar_cap)
 {
@@ -968,40 +1015,8 @@ static int dp_batch(mbg_ctx_t *c, bool dual, int32_t n_jobs, const mbg_dp_job_t 
 	TRY(dp_dev_locked(c, c->stream, dual, n_jobs, (const DpJob*)c->b_in[4].p, (const uint8_t*)c->b_in[0].p, (const uint8_t*)c->b_in[1].p,
 	                  m, mat, (DpRes*)c->b_out[0].p, any_cigar ? (uint32_t*)c->b_out[1].p : nullptr, cap, slab, qt, tcap));
 	HIPCHK(hipEventRecord(c->dev_ev[1], c->stream));
-	HIPCHK(hipMemcpyAsync(res, c->b_out[0].p, (size_t)n_jobs * sizeof(DpRes), hipMemcpyDeviceToHost, c->stream));
-	HIPCHK(hipStreamSynchronize(c->stream));
-	{
-		float ms = 0;
-		HIPCHK(hipEventElapsedTime(&ms, c->dev_ev[0], c->dev_ev[1]));
-		c->cnt.ms_dp += ms, ++c->cnt.dp_launches, c->cnt.dp_jobs += n_jobs;
-		for (int32_t i = 0; i < n_jobs; ++i) c->cnt.dp_cells += res[i].cells;
-	}
-	if (any_cigar) {           // only the operations written cross PCIe: packed on the device at host-computed offsets
-		std::vector<int64_t> poff(n_jobs + 1);
-		int64_t tot = 0;
-		for (int32_t i = 0; i < n_jobs; ++i) {
-			poff[i] = tot;
-			if (!(jobs[i].flag & 0x01)) {
-				if (res[i].n_cigar < 0 || res[i].n_cigar > jobs[i].qlen + jobs[i].tlen) return MBG_EHIP;
-				tot += res[i].n_cigar;
-			}
-		}
-		poff[n_jobs] = tot;
-		if (tot > 0) {
-			TRY(dgrow(c->b_in[5], (size_t)(n_jobs + 1) * 8)); TRY(dgrow(c->b_out[2], (size_t)tot * 4));
-			TRY(hgrow(c->h_pinned, (size_t)tot * 4));
-			HIPCHK(hipMemcpyAsync(c->b_in[5].p, poff.data(), (size_t)(n_jobs + 1) * 8, hipMemcpyHostToDevice, c->stream));
-			HIPCHK(launch_cigar_pack((const DpJob*)c->b_in[4].p, (const DpRes*)c->b_out[0].p, n_jobs, (const int64_t*)c->b_in[5].p,
-			                         (const uint32_t*)c->b_out[1].p, (uint32_t*)c->b_out[2].p, c->stream));
-			HIPCHK(hipMemcpyAsync(c->h_pinned.p, c->b_out[2].p, (size_t)tot * 4, hipMemcpyDeviceToHost, c->stream));
-			HIPCHK(hipStreamSynchronize(c->stream));
-			const uint32_t *pk = (const uint32_t*)c->h_pinned.p;
-			for (int32_t i = 0; i < n_jobs; ++i)
-				if (!(jobs[i].flag & 0x01) && res[i].n_cigar > 0)
-					memcpy(cigar + jobs[i].cigar_off, pk + poff[i], (size_t)res[i].n_cigar * 4);
-		}
-	}
-	return MBG_OK;
+	return dp_readback(c, n_jobs, (const DpJob*)c->b_in[4].p, jobs, nullptr, (const DpRes*)c->b_out[0].p, res,
+	                   any_cigar ? (const uint32_t*)c->b_out[1].p : nullptr, cigar);
 }
 
 int mbg_extd2_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_dp_job_t *jobs, const uint8_t *qseq, int64_t qbytes, const uint8_t *tseq, int64_t tbytes,
@@ -1014,6 +1029,54 @@ int mbg_extz2_batch(mbg_ctx_t *c, int32_t n_jobs, const mbg_dp_job_t *jobs, cons
                     int8_t m, const int8_t *mat, mbg_dp_result_t *res, uint32_t *cigar, int64_t cigar_cap)
 {
 	return dp_batch(c, false, n_jobs, jobs, qseq, qbytes, tseq, tbytes, m, mat, res, cigar, cigar_cap);
+}
+
+int mbg_align_pair_batch(mbg_ctx_t *c, const mbg_align_opt_t *opt, int32_t n, const mbg_align_job_t *jobs, const uint8_t *qseq,
+                         int64_t qbytes, const uint8_t *tseq, int64_t tbytes, const int8_t *mat, mbg_dp_result_t *res, uint32_t *cigar,
+                         int64_t cigar_cap)
+{
+	if (c == nullptr || opt == nullptr || n < 0 || (n > 0 && (!jobs || !res || !mat || !cigar))) return MBG_EARG;
+	if (opt->e <= 0 || opt->e2 <= 0) return MBG_EARG;            // max_bw_from_mm divides by them
+	std::lock_guard<std::mutex> lk(c->mu);
+	if (n == 0) return MBG_OK;
+	TRY(ctx_enter(c));
+	int32_t mq = 1, mt = 1, mw = 1;
+	for (int32_t i = 0; i < n; ++i) {
+		const mbg_align_job_t &j = jobs[i];
+		if (j.qlen < 0 || j.tlen < 0 || j.qoff < 0 || j.toff < 0 || j.qoff + j.qlen > qbytes || j.toff + j.tlen > tbytes) return MBG_EARG;
+		if (j.cigar_off < 0 || j.cigar_off + j.qlen + j.tlen + 1 > cigar_cap) return MBG_EARG;
+		int w, ncol;
+		dp_geom(j.qlen, j.tlen, j.w, &w, &ncol);                  // the band before any cap: an upper bound
+		mq = j.qlen > mq ? j.qlen : mq;
+		mt = j.tlen > mt ? j.tlen : mt;
+		mw = w > mw ? w : mw;
+	}
+	int cap, qt, tcap;
+	int64_t slab;
+	TRY(dp_plan(mq, mt, mw, &cap, &slab, &qt, &tcap));
+	const bool dual = !(opt->q == opt->q2 && opt->e == opt->e2); // align.c:373-378
+	TRY(dgrow(c->b_in[0], qbytes + 16)); TRY(dgrow(c->b_in[1], tbytes + 16)); TRY(dgrow(c->b_in[2], (size_t)n * sizeof(AlignJob)));
+	TRY(dgrow(c->b_in[3], 32)); TRY(dgrow(c->b_in[4], (size_t)n * sizeof(DpJob)));
+	TRY(dgrow(c->l_buf[0], (size_t)n)); TRY(dgrow(c->l_buf[1], (size_t)n * sizeof(DpRes)));
+	TRY(dgrow(c->b_out[0], (size_t)n * sizeof(DpRes))); TRY(dgrow(c->b_out[1], (size_t)cigar_cap * 4));
+	HIPCHK(hipMemcpyAsync(c->b_in[0].p, qseq, qbytes, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[1].p, tseq, tbytes, hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[2].p, jobs, (size_t)n * sizeof(AlignJob), hipMemcpyHostToDevice, c->stream));
+	HIPCHK(hipMemcpyAsync(c->b_in[3].p, mat, 25, hipMemcpyHostToDevice, c->stream));
+	for (hipEvent_t &e : c->dev_ev) if (!e) HIPCHK(hipEventCreate(&e));
+	HIPCHK(hipEventRecord(c->dev_ev[0], c->stream));
+	AlignArgs a{};
+	a.jobs = (const AlignJob*)c->b_in[2].p; a.n = n; a.qseq = (const uint8_t*)c->b_in[0].p; a.tseq = (const uint8_t*)c->b_in[1].p;
+	a.mat = (const int8_t*)c->b_in[3].p;
+	a.sc_a = opt->a, a.sc_b = opt->b, a.q = opt->q, a.e = opt->e, a.q2 = opt->q2, a.e2 = opt->e2;
+	a.generic = (opt->b_ts != 0 && opt->b != opt->b_ts) || opt->meth;   // align.c:334-335
+	a.max_sw_mat = opt->max_sw_mat;
+	a.dp = (DpJob*)c->b_in[4].p; a.kind = (uint8_t*)c->l_buf[0].p; a.res = (DpRes*)c->l_buf[1].p; a.cigar = (uint32_t*)c->b_out[1].p;
+	HIPCHK(launch_align_plan(a, c->stream));
+	TRY(dp_dev_locked(c, c->stream, dual, n, a.dp, a.qseq, a.tseq, 5, mat, (DpRes*)c->b_out[0].p, a.cigar, cap, slab, qt, tcap));
+	HIPCHK(launch_align_merge(a.kind, a.res, (DpRes*)c->b_out[0].p, n, c->stream));
+	HIPCHK(hipEventRecord(c->dev_ev[1], c->stream));
+	return dp_readback(c, n, a.dp, nullptr, jobs, (const DpRes*)c->b_out[0].p, res, a.cigar, cigar);
 }
 
 // ksw_gen_nt4_mat (ksw2.h:199-219): 5x5 scores for A,C,G,T,N; mt 1 = C->T, 2 = G->A conversion

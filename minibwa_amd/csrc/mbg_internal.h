@@ -293,6 +293,24 @@ hipError_t launch_getseq(const L2b &lb, int64_t n, const int64_t *tid, const int
 hipError_t launch_pack_reads(const uint8_t *seq, const int64_t *off, const int32_t *len, int32_t n_reads, uint32_t *packed,
                              uint32_t *qflag, hipStream_t st);
 hipError_t launch_dp(bool dual, const DpArgs &a, unsigned grid, hipStream_t st);
+// mb_align_pair's decisions before the DP (mbg_align.hip): one call of align.c:329-390 per job
+struct AlignJob { int64_t qoff, toff; int32_t qlen, tlen; int32_t w, end_bonus, zdrop, flag; int64_t cigar_off; };
+static_assert(sizeof(AlignJob) == 48, "AlignJob must match mbg_align_job_t");
+struct AlignArgs {
+	const AlignJob *jobs;
+	int64_t n;
+	const uint8_t *qseq, *tseq;
+	const int8_t *mat;                   // 5x5
+	int32_t sc_a, sc_b, q, e, q2, e2;    // opt->a, b, q, e, q2, e2
+	int32_t generic;                     // KSW_EZ_GENERIC_SC for every call (transition mismatch or methylation)
+	int64_t max_sw_mat;
+	DpJob *dp;                           // out: the DP job, or an empty one (qlen 0) for a settled call
+	uint8_t *kind;                       // out: 0 DP, 1 fast path, 2 max_sw_mat skip
+	DpRes *res;                          // out: the settled calls' results
+	uint32_t *cigar;                     // out: the fast paths' CIGARs at cigar_off
+};
+hipError_t launch_align_plan(const AlignArgs &a, hipStream_t st);
+hipError_t launch_align_merge(const uint8_t *kind, const DpRes *decided, DpRes *res, int64_t n, hipStream_t st);
 // ring length of dp_team_kernel for a launch whose widest band is nb_max cells (0: not needed)
 int dp_team_cap(int nb_max);
 hipError_t launch_gather_chain(const Idx &ix, int64_t n_chains, int32_t steps, int32_t bs, unsigned long long *sum, hipStream_t st);

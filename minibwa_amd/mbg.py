@@ -55,6 +55,10 @@ class IdxInfo(C.Structure):
                 ("pre_len", C.c_int32), ("sa_bit", C.c_int32), ("device_bytes", C.c_uint64)]
 
 
+class AlignOpt(C.Structure):       # mbg_align_opt_t
+    _fields_ = [(f, C.c_int32) for f in ("a", "b", "b_ts", "q", "e", "q2", "e2", "meth")] + [("max_sw_mat", C.c_int64)]
+
+
 class MapOpt(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("min_len", "max_sub_occ", "max_occ", "a", "b", "q", "e", "q2", "e2",
                                          "end_bonus", "ksw_w", "zdrop")] + [("mat", C.c_int8 * 25)]
@@ -175,6 +179,8 @@ def lib():
         L.mbg_sa_batch.argtypes = [_P, C.c_int64, _P]
         for fn in (L.mbg_extd2_batch, L.mbg_extz2_batch):
             fn.argtypes = [_P, C.c_int32, _P, _P, C.c_int64, _P, C.c_int64, C.c_int8, _P, _P, _P, C.c_int64]
+        L.mbg_align_pair_batch.argtypes = [_P, C.POINTER(AlignOpt), C.c_int32, _P, _P, C.c_int64, _P, C.c_int64, _P, _P, _P,
+                                           C.c_int64]
         L.mbg_getseq.argtypes = [_P, C.c_int64, _P, _P, _P, _P, _P, C.c_int64, _P]
         L.mbg_seed_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, C.c_int64,
                                          C.POINTER(C.c_int64)]
@@ -219,12 +225,14 @@ def lib():
 
 EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mbg_idx_load", "mbg_bwt_upload",
             "mbg_l2b_upload", "mbg_idx_info", "mbg_pre_copy", "mbg_rank2a", "mbg_extend", "mbg_seed_batch",
-            "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_getseq", "mbg_seed_batch_dev",
+            "mbg_sa_batch", "mbg_extd2_batch", "mbg_extz2_batch", "mbg_align_pair_batch", "mbg_getseq", "mbg_seed_batch_dev",
             "mbg_seed_batch_dev_async", "mbg_sa_batch_dev", "mbg_extd2_batch_dev", "mbg_map_step_dev", "mbg_copy_to_host", "mbg_copy_to_device", "mbg_dev_alloc",
             "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench", "mbg_idx_attach", "mbg_ll_batch",
             "mbg_ll_batch_dev", "mbg_matesw_batch", "mbg_update_extra_batch",
             "mbg_anchor_batch", "mbg_anchor_batch_dev", "mbg_lchain_batch", "mbg_lchain_batch_dev", "mbg_fq_open",
             "mbg_fq_close", "mbg_fq_buf_new", "mbg_fq_buf_free", "mbg_fq_read", "mbg_fq_read_frag")
+ALIGN_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"), ("w", "<i4"),
+                            ("end_bonus", "<i4"), ("zdrop", "<i4"), ("flag", "<i4"), ("cigar_off", "<i8")], align=True)
 LL_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"), ("size", "<i4"),
                          ("pad", "<i4")], align=True)
 LL_RES_DTYPE = np.dtype([(f, "<i4") for f in ("score", "te", "qe", "score2", "te2")])
@@ -642,6 +650,35 @@ class Context:
     def ksw_extd2(self, queries, targets, params, mat, m=5) -> DpBatchResult:
         """params: dict of per-job arrays/scalars q, e, q2, e2, w, zdrop, end_bonus, flag."""
         return self._dp(self.L.mbg_extd2_batch, "mbg_extd2_batch", queries, targets, params, mat, m)
+
+    def align_pair(self, p, max_sw_mat, queries, targets, params, mat) -> DpBatchResult:
+        """mb_align_pair (align.c:329-390) over a batch: p = (a, b, b_ts, q, e, q2, e2, meth) as the
+        oracle's align_params; params: per-call arrays/scalars w, end_bonus, zdrop, flag."""
+        n = len(queries)
+        qbuf, qoff, qlen = pack_seqs(queries)
+        tbuf, toff, tlen = pack_seqs(targets)
+        jobs = np.zeros(n, ALIGN_JOB_DTYPE)
+        jobs["qoff"] = qoff[:-1]
+        jobs["toff"] = toff[:-1]
+        jobs["qlen"] = qlen
+        jobs["tlen"] = tlen
+        for f in ("w", "end_bonus", "zdrop", "flag"):
+            jobs[f] = params[f]
+        span = qlen.astype(np.int64) + tlen + 1
+        coff = np.zeros(n, np.int64)
+        if n > 1:
+            np.cumsum(span[:-1], out=coff[1:])
+        jobs["cigar_off"] = coff
+        res = np.zeros(n, DP_RES_DTYPE)
+        cap = int(span.sum()) + 1
+        cig = np.zeros(cap, np.uint32)
+        mat = np.ascontiguousarray(mat, np.int8)
+        o = AlignOpt(*[int(x) for x in p], int(max_sw_mat))
+        qb = qbuf if len(qbuf) else np.zeros(1, np.uint8)
+        tb = tbuf if len(tbuf) else np.zeros(1, np.uint8)
+        check(self.L.mbg_align_pair_batch(self.h, C.byref(o), n, _ptr(jobs), _ptr(qb), len(qbuf), _ptr(tb), len(tbuf), _ptr(mat),
+                                          _ptr(res), _ptr(cig), cap), "mbg_align_pair_batch")
+        return DpBatchResult(res, cig, coff)
 
     def ksw_ll(self, queries, targets, sizes, mat, gapo, gape, xtra, m=5) -> np.ndarray:
         """ksw_ll_qinit + ksw_ll_{u8,i16}_core over a batch (ksw2_ll_sse.c:49-333); sizes is per job

@@ -770,6 +770,69 @@ int orc_extz2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, i
 	return cg.overflow ? -1 : cells;
 }
 
+/* mb_align_pair (align.c:316-390): the generic-scoring switch (:334-335), the ungapped extension
+ * fast path for EXTZ_ONLY calls with tlen >= qlen (:336-350), the ungapped global fast path for
+ * qlen == tlen (:351-362), the band cap from the mismatch count when the shorter side is under 100
+ * (:364-368, max_bw_from_mm :321-327), the max_sw_mat skip (:370-372) and the dispatch to extz2
+ * (single affine) or extd2 (:373-378).  p = {a, b, b_ts, q, e, q2, e2, meth}.  Returns the DP's cells
+ * (0 on a fast path or a skip), -1 on CIGAR overflow. */
+int orc_align_pair(const int32_t p[8], int64_t max_sw_mat, int qlen, const uint8_t *qseq, int tlen, const uint8_t *tseq,
+                   const int8_t *mat, int w, int end_bonus, int zdrop, int flag, orc_ez_t *ez, uint32_t *cigar, int cigar_cap)
+{
+	const int32_t a = p[0], b = p[1], b_ts = p[2], q = p[3], e = p[4], q2 = p[5], e2 = p[6], meth = p[7];
+	int32_t j, n_mm = -1;
+	if ((b_ts != 0 && b != b_ts) || meth) flag |= 0x04;
+	if ((flag & 0x40) && tlen >= qlen) {
+		reset_ez(ez);
+		ez->score = ez->max = 0;
+		for (j = 0; j < qlen; ++j) {
+			const int32_t sc = mat[tseq[j] * 5 + qseq[j]];
+			ez->score += sc;
+			n_mm += tseq[j] > 3 || qseq[j] > 3 || sc < 0;
+			if (ez->max < ez->score) ez->max = ez->score, ez->max_q = ez->max_t = j;
+		}
+		if (n_mm <= 2) {
+			ez->mqe = ez->score, ez->mqe_t = qlen - 1;
+			if (ez->mqe + end_bonus >= ez->max) {
+				ez->reach_end = 1;
+				if (cigar_cap < 1) return -1;
+				cigar[0] = (uint32_t)qlen << 4, ez->n_cigar = 1;   /* ksw_push_cigar, a zero length included */
+				return 0;
+			}
+		}
+	} else if (qlen == tlen && !(flag & 0x40)) {
+		const int32_t max_gapped = (qlen - 2) * a - 2 * (q + e);
+		reset_ez(ez);
+		for (j = 0, ez->score = 0; j < qlen; ++j) {
+			const int32_t sc = mat[tseq[j] * 5 + qseq[j]];
+			ez->score += sc;
+			n_mm += tseq[j] > 3 || qseq[j] > 3 || sc < 0;
+		}
+		if (n_mm <= 3 || ez->score > max_gapped) {
+			if (cigar_cap < 1) return -1;
+			cigar[0] = (uint32_t)qlen << 4, ez->n_cigar = 1;
+			return 0;
+		}
+	}
+	if (n_mm >= 0 && (qlen < tlen ? qlen : tlen) < 100) {
+		const int32_t x = n_mm * (a + b);
+		int32_t max1 = 0, max2 = 0, mbw;
+		if (x >= q + e) max1 = (x - q + e - 1) / e;
+		if (x >= q2 + e2) max2 = (x - q2 + e2 - 1) / e2;
+		mbw = max1 > max2 ? max1 : max2;
+		if (w > mbw + 4) w = mbw + 4;
+	}
+	if (max_sw_mat > 0 && (int64_t)tlen * qlen > max_sw_mat) {
+		reset_ez(ez);
+		ez->zdropped = 1;
+		return 0;
+	}
+	if (q == q2 && e == e2)
+		return orc_extz2(qlen, qseq, tlen, tseq, 5, mat, (int8_t)q, (int8_t)e, w, zdrop * a, end_bonus, flag, ez, cigar, cigar_cap);
+	return orc_extd2(qlen, qseq, tlen, tseq, 5, mat, (int8_t)q, (int8_t)e, (int8_t)q2, (int8_t)e2, w, zdrop * a, end_bonus, flag, ez,
+	                 cigar, cigar_cap);
+}
+
 /* ksw_ll_qinit + ksw_ll_u8_core / ksw_ll_i16_core (ksw2_ll_sse.c:49-333): local Smith-Waterman
  * with affine gaps, score and end positions only, as the mate-rescue step calls it (pe.c:260-272).
  *

@@ -49,6 +49,8 @@ int orc_extz2(int qlen, const uint8_t *query, int tlen, const uint8_t *target, i
               int8_t q, int8_t e, int w, int zdrop, int end_bonus, int flag,
               orc_ez_t *ez, uint32_t *cigar, int cigar_cap);
 int64_t orc_dp_cells(int qlen, int tlen, int w);
+int orc_align_pair(const int32_t p[8], int64_t max_sw_mat, int qlen, const uint8_t *qseq, int tlen, const uint8_t *tseq,
+                   const int8_t *mat, int w, int end_bonus, int zdrop, int flag, orc_ez_t *ez, uint32_t *cigar, int cigar_cap);
 
 typedef struct { int32_t score, te, qe, score2, te2; } orc_llrst_t;   /* ksw_llrst_t, ksw2.h:47-51 */
 int orc_ll(int size, int qlen, const uint8_t *query, int m, const int8_t *mat, int tlen, const uint8_t *target,

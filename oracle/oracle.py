@@ -119,6 +119,9 @@ def _load_oracle():
     L.orc_bench.argtypes = [C.POINTER(OrcBwt), C.c_int, C.c_int64, C.c_int]
     L.orc_dp_cells.restype = C.c_int64
     L.orc_dp_cells.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.orc_align_pair.restype = C.c_int
+    L.orc_align_pair.argtypes = [_i32p, C.c_int64, C.c_int, _u8p, C.c_int, _u8p, _i8p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.POINTER(OrcEz), _u32p, C.c_int]
     L.orc_ungap.restype = C.c_int32
     L.orc_ungap.argtypes = [C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32] + [C.POINTER(C.c_int32)] * 3
     L.orc_matesw_align.argtypes = [_i32p, C.c_int32, _u8p, C.c_int32, _u8p, C.c_int32, C.c_int32, C.POINTER(OrcRescue),
@@ -241,6 +244,26 @@ def extz2(query, target, mat, q, e, w, zdrop, end_bonus, flag, m=5):
     cig = np.zeros(cap, np.uint32)
     ret = orc().orc_extz2(len(query), query, len(target), target, m, mat, q, e, w, zdrop, end_bonus, flag,
                           C.byref(ez), cig, cap)
+    return _dp_result(ez, cig, ret)
+
+
+def align_params(a=2, b=8, b_ts=0, q=4, e=2, q2=24, e2=1, meth=0):
+    """The mb_opt_t fields mb_align_pair reads (align.c:329-378), in the order orc_align_pair and
+    mbx_align_pair take them."""
+    return np.array([a, b, b_ts, q, e, q2, e2, meth], np.int32)
+
+
+def orc_align_pair(p, max_sw_mat, query, target, mat, w, end_bonus, zdrop, flag):
+    """mb_align_pair (align.c:329-390) restated: the fast paths, band cap, skip and DP dispatch."""
+    query = np.ascontiguousarray(query, np.uint8)
+    target = np.ascontiguousarray(target, np.uint8)
+    mat = np.ascontiguousarray(mat, np.int8)
+    p = np.ascontiguousarray(p, np.int32)
+    ez = OrcEz()
+    cap = len(query) + len(target) + 8
+    cig = np.zeros(cap, np.uint32)
+    ret = orc().orc_align_pair(p, max_sw_mat, len(query), query, len(target), target, mat, w, end_bonus, zdrop, flag, C.byref(ez),
+                               cig, cap)
     return _dp_result(ez, cig, ret)
 
 
@@ -498,6 +521,8 @@ def ref_pe():
         L.mbx_anchor.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]
         L.mbx_update_extra.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32, _u32p, _u8p, _u8p,
                                        _i8p, C.c_int8, C.c_int8, C.c_int32, C.POINTER(MbxHit), _u32p, C.c_int32]
+        L.mbx_align_pair.argtypes = [_i32p, C.c_int64, C.c_int32, _u8p, C.c_int32, _u8p, _i8p, C.c_int32, C.c_int32, C.c_int32,
+                                     C.c_int32, _i32p, _u32p, C.c_int32]
         L.mbx_finish.argtypes = [_i32p, C.c_int32, _u8p, _u8p, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32,
                                  C.c_int32, C.c_int32, C.c_int32, C.c_int32, _u32p, C.POINTER(MbxHit), _u32p, C.c_int32]
         _REFPE = L
@@ -511,6 +536,22 @@ def ref_ungap(query, target, mt=0, kmer=7):
     out = np.zeros(3, np.int32)
     m = ref_pe().mbx_ungap(len(q), q, len(t), t, kmer, mt, out)
     return int(m), int(out[0]), int(out[1]), int(out[2])
+
+
+def ref_align_pair(p, max_sw_mat, query, target, mat, w, end_bonus, zdrop, flag):
+    """The reference's own static mb_align_pair (align.c:329-390), through ref_align_shim.c."""
+    L = ref_pe()
+    query = np.ascontiguousarray(query, np.uint8)
+    target = np.ascontiguousarray(target, np.uint8)
+    mat = np.ascontiguousarray(mat, np.int8)
+    p = np.ascontiguousarray(p, np.int32)
+    out = np.zeros(11, np.int32)
+    cap = len(query) + len(target) + 8
+    cig = np.zeros(cap, np.uint32)
+    L.mbx_align_pair(p, max_sw_mat, len(query), query, len(target), target, mat, w, end_bonus, zdrop, flag, out, cig, cap)
+    d = {f: int(out[i]) for i, f in enumerate(EZ_FIELDS)}
+    d["cigar"] = [int(c) for c in cig[: out[10]]]
+    return d
 
 
 def _hit_dict(h, cig):

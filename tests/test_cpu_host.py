@@ -63,6 +63,8 @@ def test_header_layouts_match_python(tmp_path):
         "mbg_seed_rec_t": ["rid", "pass"],
         "mbg_dp_job_t": ["qoff", "toff", "qlen", "tlen", "q", "e", "q2", "e2", "w", "zdrop", "end_bonus", "flag",
                          "cigar_off"],
+        "mbg_align_job_t": ["qoff", "toff", "qlen", "tlen", "w", "end_bonus", "zdrop", "flag", "cigar_off"],
+        "mbg_align_opt_t": ["a", "b", "b_ts", "q", "e", "q2", "e2", "meth", "max_sw_mat"],
         "mbg_dp_result_t": ["max", "zdropped", "max_q", "max_t", "mqe", "mqe_t", "mte", "mte_q", "score",
                             "reach_end", "n_cigar", "cells"],
         "mbg_job_meta_t": ["rid", "side", "rev", "tid", "tpos"],
@@ -112,6 +114,7 @@ def test_header_layouts_match_python(tmp_path):
 
     np_layout("mbg_dp_job_t", mbg.DP_JOB_DTYPE)
     np_layout("mbg_dp_result_t", mbg.DP_RES_DTYPE)
+    np_layout("mbg_align_job_t", mbg.ALIGN_JOB_DTYPE)
     np_layout("mbg_job_meta_t", mbg.JOB_META_DTYPE)
     np_layout("mbg_ll_job_t", mbg.LL_JOB_DTYPE)
     np_layout("mbg_ll_result_t", mbg.LL_RES_DTYPE)
@@ -135,6 +138,7 @@ def test_header_layouts_match_python(tmp_path):
                 assert got[(s, f)] == getattr(cls, x).offset, (s, f)
 
     ct_layout("mbg_map_opt_t", mbg.MapOpt)
+    ct_layout("mbg_align_opt_t", mbg.AlignOpt)
     ct_layout("mbg_map_stats_t", mbg.MapStats)
     ct_layout("mbg_map_out_t", mbg.MapOut)
     ct_layout("mbg_bwt_desc_t", mbg.BwtDesc)

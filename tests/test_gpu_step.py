@@ -3,7 +3,10 @@
 Checks every stage against the compiled reference or a restatement of the reference's rules:
 seeding sets (mb_seed_intv_batch), located positions (mb_bwt_sa_batch), the anchor-row sampling
 (seed.c:304-315), the extension jobs (windows of mb_align1, fast path and band cap of
-mb_align_pair, align.c:321-372) and the DP results (ksw_extd2_sse on the emitted jobs).
+mb_align_pair, align.c:321-372) and the DP results (ksw_extd2_sse on the emitted jobs).  The step's
+fast-path and band decisions are also pinned against the reference's own mb_align_pair (static in
+align.c, exported by oracle/ref_align_shim.c): every extension the step settles without DP, the
+reference settles with one M, and every job it emits, the reference aligns to the same result.
 """
 import os
 
@@ -46,6 +49,7 @@ def expected_jobs(reads, recs, pos_of_rec, ref, ctg_len, opt):
             best[rid] = (key, i)
     jobs = {}
     fast = 0
+    fast_calls = []
     for rid, (_, i) in best.items():
         r = recs[i]
         qs0, qe0 = int(r["info"]) >> 32, int(r["info"]) & 0xffffffff
@@ -76,6 +80,7 @@ def expected_jobs(reads, recs, pos_of_rec, ref, ctg_len, opt):
                 mx = max(0, int(cs.max()))
                 if n_mm <= 2 and int(cs[-1]) + opt.end_bonus >= mx:
                     fast += 1
+                    fast_calls.append((qq.copy(), tt.copy(), flag))
                     return
             if n_mm >= 0 and min(len(qq), len(tt)) < 100:
                 x = n_mm * (opt.a + opt.b)
@@ -94,7 +99,7 @@ def expected_jobs(reads, recs, pos_of_rec, ref, ctg_len, opt):
             l += (l * opt.a + opt.end_bonus - opt.q) // opt.e if l * opt.a + opt.end_bonus > opt.q else 0
             te0 = min(te + l, clen)
             one(oq[qe:], ref[coff + te: coff + te0], 0x40, 1)
-    return jobs, fast, len(best)
+    return jobs, fast, len(best), fast_calls
 
 
 def test_map_step_cfg0(gpu_ctx):
@@ -138,7 +143,7 @@ def test_map_step_cfg0(gpu_ctx):
     meta = gpu_ctx.to_host(out.meta, mbg.JOB_META_DTYPE, st.n_jobs)
     qb = gpu_ctx.to_host(out.qbuf, np.uint8, st.q_bytes)
     tb = gpu_ctx.to_host(out.tbuf, np.uint8, st.t_bytes)
-    exp, fast, anchored = expected_jobs(reads, recs, {k: int(v) for k, v in pos_of_rec.items()}, ref, [16569], opt)
+    exp, fast, anchored, fast_calls = expected_jobs(reads, recs, {k: int(v) for k, v in pos_of_rec.items()}, ref, [16569], opt)
     assert st.n_fast == fast and st.n_anchored == anchored
     got = {}
     for j in range(st.n_jobs):
@@ -159,3 +164,18 @@ def test_map_step_cfg0(gpu_ctx):
         r = O.ref_extd2(q, t, mat, opt.q, opt.e, opt.q2, opt.e2, int(jb["w"]), opt.zdrop, opt.end_bonus, int(jb["flag"]))
         assert {f: int(res[f][j]) for f in FIELDS} == {f: r[f] for f in FIELDS}, j
         assert cig[jb["cigar_off"]:jb["cigar_off"] + res["n_cigar"][j]].tolist() == r["cigar"], j
+    # 5. the reference's own mb_align_pair, given the uncapped band and the unscaled z-drop, makes the
+    # same decisions: a single M for every settled extension, the same DP result for every job
+    if O.have_ref_pe():
+        p = O.align_params(opt.a, opt.b, 0, opt.q, opt.e, opt.q2, opt.e2, 0)
+        assert opt.zdrop % opt.a == 0
+        for q, t, flag in fast_calls:
+            r = O.ref_align_pair(p, 0, q, t, mat, opt.ksw_w, opt.end_bonus, opt.zdrop // opt.a, flag)
+            assert r["cigar"] == [len(q) << 4] and r["reach_end"] == 1
+        for j in range(st.n_jobs):
+            jb = jobs[j]
+            q, t = got[(int(meta["rid"][j]), int(meta["side"][j]))][:2]
+            r = O.ref_align_pair(p, 0, q, t, mat, opt.ksw_w, opt.end_bonus, opt.zdrop // opt.a, int(jb["flag"]))
+            assert {f: int(res[f][j]) for f in FIELDS} == {f: r[f] for f in FIELDS}, j
+            assert cig[jb["cigar_off"]:jb["cigar_off"] + res["n_cigar"][j]].tolist() == r["cigar"], j
+        assert len(fast_calls) > 50
