@@ -361,9 +361,18 @@ typedef struct {
 	double ms_dp;
 	int64_t ll_launches, ll_jobs, ll_cells;
 	double ms_ll;
+	/* launch chains run as captured HIP graphs, how many of those needed a new executable graph
+	 * (the rest updated the last one in place), and chains launched one by one (graphs off) */
+	int64_t graph_launches, graph_instantiations, direct_chains;
 } mbg_counters_t;
 int mbg_counters(mbg_ctx_t *ctx, mbg_counters_t *out, int reset);
 int mbg_count_work(mbg_ctx_t *ctx, int on);
+/* Launch chains without a host step inside (the seeding phases around SMEM pass 1; the whole
+ * seed -> locate -> extend pass of mbg_map_step_dev) run as HIP graphs captured from the stream and
+ * updated in place from call to call (on by default; MBG_GRAPH=0 in the environment of mbg_init turns
+ * it off).  Results do not depend on it.  With graphs on, mbg_map_step_dev reports only its total
+ * device time, not its phases.  No counterpart in the reference. */
+int mbg_use_graphs(mbg_ctx_t *ctx, int on);
 
 /* ---- reads at the edge: FASTA/FASTQ (host code) ------------------------------------------------ */
 

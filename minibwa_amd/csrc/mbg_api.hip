@@ -27,6 +27,9 @@ struct DBuf {
 
 } // namespace
 
+// kinds of captured launch chains (an executable graph each per context)
+enum { kGraphSeedPre, kGraphSeedPreSeg, kGraphSeedPost, kGraphStep, kGraphStepSeg, kGraphKinds };
+
 struct mbg_ctx_s {
 	int device = 0;
 	int n_cu = 0;
@@ -77,6 +80,9 @@ struct mbg_ctx_s {
 	hipEvent_t ord_ev = nullptr;
 	hipStream_t ord_st = nullptr;
 	bool ord_pending = false;
+	// launch chains captured as graphs (run_graph): one executable graph per chain kind
+	bool use_graphs = true;
+	hipGraphExec_t graph_exec[kGraphKinds] = {};
 };
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { g_hip_err = std::string(#x) + ": " + hipGetErrorString(e_); return MBG_EHIP; } } while (0)
@@ -232,6 +238,8 @@ int mbg_init(int device, mbg_ctx_t **out)
 		const char *g = getenv("MBG_GATHER_BPC"), *d = getenv("MBG_DP_WPC");
 		if (g && atoi(g) >= 1 && atoi(g) <= 64) c->gather_bpc = atoi(g);
 		if (d && atoi(d) >= 1 && atoi(d) <= 128) c->dp_wpc = atoi(d);
+		const char *gr = getenv("MBG_GRAPH");
+		if (gr && atoi(gr) == 0) c->use_graphs = false;
 	}
 	hipDeviceProp_t prop;
 	hipError_t e = hipGetDeviceProperties(&prop, device);
@@ -293,6 +301,7 @@ void mbg_destroy(mbg_ctx_t *c)
 	for (hipEvent_t &e : c->dev_ev) if (e) hipEventDestroy(e);
 	dfree(c->f_stats);
 	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
+	for (hipGraphExec_t &g : c->graph_exec) if (g) hipGraphExecDestroy(g);
 	if (c->stream) hipStreamDestroy(c->stream);
 	delete c;
 }
@@ -551,15 +560,68 @@ static unsigned persistent_grid(mbg_ctx_t *c, int64_t n_items, int per_block)
 	return (unsigned)want;
 }
 
-// Two-pass seeding of device-resident reads into c->p_recs (flat; c->cap_rec entries) with the
-// record count at *n_out_dev.  Per-task output slots (kSlot1 per read, kSlot2 per pass-2 task) are
-// compacted by an exclusive scan, so no output position needs an atomic except overflow spills.
-// Counters in c->b_ctr: u32 [0] task counter, [1] pass-2 task count, i64 [1] record count,
-// u64 [3] spill count (u32 [4] is the SA row counter, u64 [6] the gather checksum, u32 [14..18] the
-// DP job counters).  ev (optional) receives events after pass 1 and after pass 2.
 static const int32_t kSlot1 = 16, kSlot2 = 8;
 
-struct SeedCaps { int64_t rec, task, spill; };
+// ---- launch chains as HIP graphs ----------------------------------------------------------------
+//
+// A chain of launches with no allocation and no host synchronisation inside (a seeding pass; the
+// whole seed -> locate -> extend step) is captured from the stream into a graph and launched as one
+// unit.  Every call captures its chain again, since pointers and sizes change from call to call, and
+// updates the context's executable graph of that kind in place (hipGraphExecUpdate), which is cheap
+// when the chain has the same shape as the last one; a chain of another shape instantiates anew.
+// Counters and flags are cleared by a kernel (zero_async), never by a memset node.  MBG_GRAPH=0 turns
+// capture off (the launches then go to the stream one by one); results are the same either way.
+
+__global__ void zero_words_kernel(uint32_t *p, int64_t n)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// zeroes `bytes` (a multiple of 4, 4-aligned) on the stream, as a kernel
+static hipError_t zero_async(void *p, size_t bytes, hipStream_t st)
+{
+	const int64_t n = (int64_t)(bytes / 4);
+	if (n <= 0) return hipSuccess;
+	int64_t g = (n + 255) / 256;
+	if (g > 2048) g = 2048;
+	hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)g), dim3(256), 0, st, (uint32_t*)p, n);
+	return hipGetLastError();
+}
+
+extern "C++" {
+template <class F>
+static int run_graph(mbg_ctx_t *c, hipStream_t st, int kind, F &&enqueue)
+{
+	if (!c->use_graphs) {
+		++c->cnt.direct_chains;
+		return enqueue(st);
+	}
+	HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+	const int rc = enqueue(st);
+	hipGraph_t g = nullptr;
+	const hipError_t ec = hipStreamEndCapture(st, &g);
+	if (rc != MBG_OK) { if (g) hipGraphDestroy(g); return rc; }
+	if (ec != hipSuccess) { g_hip_err = std::string("hipStreamEndCapture: ") + hipGetErrorString(ec); return MBG_EHIP; }
+	hipGraphExec_t &ex = c->graph_exec[kind];
+	if (ex) {
+		hipGraphExecUpdateResult ur;
+		hipGraphNode_t bad = nullptr;
+		if (hipGraphExecUpdate(ex, g, &bad, &ur) != hipSuccess || ur != hipGraphExecUpdateSuccess) {
+			(void)hipGetLastError();
+			hipGraphExecDestroy(ex);
+			ex = nullptr;
+			++c->cnt.graph_instantiations;
+		}
+	} else ++c->cnt.graph_instantiations;
+	hipError_t e = hipSuccess;
+	if (!ex) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+	hipGraphDestroy(g);
+	if (e != hipSuccess) { ex = nullptr; g_hip_err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e); return MBG_EHIP; }
+	HIPCHK(hipGraphLaunch(ex, st));
+	++c->cnt.graph_launches;
+	return MBG_OK;
+}
+} // extern "C++"
 
 // Segmented pass 1 (reads of at least 2*seg_len bases): the serial SMEM search is a chain of starts
 // x0 = 0 -> x1 -> ..., one step each, a step emitting at most one SMEM (mb_bwt_smem, bwt.c:280-305).
@@ -570,7 +632,8 @@ struct SeedCaps { int64_t rec, task, spill; };
 // itself.  Records of a segment before its join are dropped.  A segment or fix-up task that runs out
 // of room flags its read, which then runs serially from segment 0's handoff.  The result is the serial
 // search's set of SMEMs exactly, whatever the segment length and capacities.
-static int seg_setup(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int32_t *len, SeedArgs &a)
+// pools of the segmented pass (grown before any launch, so that the launches can be captured)
+static int seg_alloc(mbg_ctx_t *c, int32_t n_reads, hipStream_t st)
 {
 	if (c->cap_seg < 1024 + n_reads / 16) c->cap_seg = 1024 + n_reads / 16;
 	const int64_t cs = c->cap_seg, nr = n_reads;
@@ -581,14 +644,21 @@ static int seg_setup(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int32_
 	TRY(dgrow(c->g_slots, (size_t)cs * c->seg_slots * sizeof(SeedRec))); TRY(dgrow(c->g_cnt, (size_t)cs * 4));
 	TRY(dgrow(c->g_fix, (size_t)cs * c->fix_slots * sizeof(SeedRec))); TRY(dgrow(c->g_fixcnt, (size_t)cs * 4));
 	TRY(dgrow(c->g_fbl, (size_t)nr * 4)); TRY(dgrow(c->g_ctr, 64));
-	uint32_t *nseg = (uint32_t*)c->g_nseg.p, *base = (uint32_t*)c->g_base.p, *gc = (uint32_t*)c->g_ctr.p;
-	HIPCHK(hipMemsetAsync(gc, 0, 64, st));
-	HIPCHK(hipMemsetAsync(c->g_flag.p, 0, (size_t)nr * 4, st));
-	HIPCHK(hipMemsetAsync(nseg + nr, 0, 4, st));
-	HIPCHK(launch_seg_count(len, n_reads, c->seg_len, nseg, st));
 	size_t tb = 0;
 	HIPCHK(scan_u32(nullptr, &tb, nullptr, nullptr, nr + 1, st));
 	TRY(dgrow(c->s_tmp, tb > c->s_tmp.n ? tb : c->s_tmp.n));
+	return MBG_OK;
+}
+
+static int seg_setup(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int32_t *len, SeedArgs &a)
+{
+	const int64_t cs = c->cap_seg, nr = n_reads;
+	uint32_t *nseg = (uint32_t*)c->g_nseg.p, *base = (uint32_t*)c->g_base.p, *gc = (uint32_t*)c->g_ctr.p;
+	HIPCHK(zero_async(gc, 64, st));
+	HIPCHK(zero_async(c->g_flag.p, (size_t)nr * 4, st));
+	HIPCHK(zero_async(nseg + nr, 4, st));
+	HIPCHK(launch_seg_count(len, n_reads, c->seg_len, nseg, st));
+	size_t tb = c->s_tmp.n;
 	HIPCHK(scan_u32(c->s_tmp.p, &tb, nseg, base, nr + 1, st));
 	// the segment total (base[n_reads]) where seed_overflowed reads it
 	HIPCHK(hipMemcpyAsync(gc + 5, base + nr, 4, hipMemcpyDeviceToDevice, st));
@@ -640,9 +710,8 @@ static int seg_run(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int32_t 
 	return MBG_OK;
 }
 
-static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
-                     int32_t min_len, int32_t max_sub_occ, unsigned long long *stats, int64_t **n_out_dev, hipEvent_t *ev,
-                     hipEvent_t ev_pre = nullptr)
+// every pool of a seeding pass, grown before its launches
+static int seed_alloc(mbg_ctx_t *c, hipStream_t st, int32_t n_reads)
 {
 	if (c->cap_rec < 8 * (int64_t)n_reads + 1024) c->cap_rec = 8 * (int64_t)n_reads + 1024;
 	if (c->cap_task < 2 * (int64_t)n_reads + 1024) c->cap_task = 2 * (int64_t)n_reads + 1024;
@@ -660,41 +729,107 @@ static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_
 	size_t tmp_bytes = 0;
 	HIPCHK(scan_u32(nullptr, &tmp_bytes, nullptr, nullptr, n_cnt, st));
 	TRY(dgrow(c->s_tmp, tmp_bytes));
+	if (c->seg_len > 0 && n_reads > 0) TRY(seg_alloc(c, n_reads, st));
+	return MBG_OK;
+}
+
+// The launches of a seeding pass over pools seed_alloc sized, in three phases with no allocation and
+// no host synchronisation, so that run_graph can capture them: before pass 1 (counters cleared, reads
+// packed, long reads' segments laid out), pass 1 (with the segment lanes, fix-ups and fallbacks of
+// long reads), and after it (pass-2 tasks, pass 2, compaction).
+struct SeedPass {
+	int32_t n_reads, min_len, max_sub_occ;
+	const int64_t *off;
+	const int32_t *len;
+	const uint8_t *seq;
+	unsigned long long *stats;
+	SeedArgs a;
+};
+
+static int seed_phase_pre(mbg_ctx_t *c, hipStream_t st, SeedPass &sp)
+{
+	const int64_t tc = c->cap_task, sc = c->cap_spill, n_cnt = (int64_t)sp.n_reads + tc;
+	uint32_t *ctr = (uint32_t*)c->b_ctr.p;
+	uint32_t *cnt = (uint32_t*)c->s_cnt.p;
+	HIPCHK(zero_async(ctr, 64, st));
+	HIPCHK(zero_async(cnt, (size_t)n_cnt * 4, st));
+	HIPCHK(zero_async(c->s_qflag.p, (size_t)sp.n_reads * 4 + 4, st));
+	HIPCHK(launch_pack_reads(sp.seq, sp.off, sp.len, sp.n_reads, (uint32_t*)c->s_pack.p, (uint32_t*)c->s_qflag.p, st));
+	SeedArgs &a = sp.a;
+	a = SeedArgs{};
+	a.seq = sp.seq; a.off = sp.off; a.len = sp.len; a.n_tasks = (uint32_t)sp.n_reads; a.min_len = sp.min_len; a.next = ctr;
+	a.packed = (const uint32_t*)c->s_pack.p; a.qflag = (const uint32_t*)c->s_qflag.p;
+	a.slots = (SeedRec*)c->s_slots1.p; a.slot_cnt = cnt; a.kslot = kSlot1;
+	a.spill = (SeedRec*)c->s_spill.p; a.n_spill = (unsigned long long*)(ctr + 6); a.spill_cap = sc;
+	a.stats = sp.stats;
+	if (c->seg_len > 0 && sp.n_reads > 0) TRY(seg_setup(c, st, sp.n_reads, sp.len, a));
+	return MBG_OK;
+}
+
+static int seed_phase_pass1(mbg_ctx_t *c, hipStream_t st, SeedPass &sp)
+{
+	if (sp.n_reads > 0) HIPCHK(launch_smem(c->ix, sp.a, 1, gather_grid(c, sp.n_reads), st));
+	if (c->seg_len > 0 && sp.n_reads > 0) TRY(seg_run(c, st, sp.n_reads, sp.len, sp.a));
+	return MBG_OK;
+}
+
+static int seed_phase_post(mbg_ctx_t *c, hipStream_t st, SeedPass &sp)
+{
+	const int64_t rc = c->cap_rec, tc = c->cap_task, sc = c->cap_spill, n_cnt = (int64_t)sp.n_reads + tc;
+	size_t tmp_bytes = c->s_tmp.n;
 	uint32_t *ctr = (uint32_t*)c->b_ctr.p;
 	int64_t *n_out = (int64_t*)(ctr + 2);
 	unsigned long long *n_spill = (unsigned long long*)(ctr + 6);
 	uint32_t *cnt = (uint32_t*)c->s_cnt.p;
-	HIPCHK(hipMemsetAsync(ctr, 0, 64, st));
-	HIPCHK(hipMemsetAsync(cnt, 0, (size_t)n_cnt * 4, st));
-	HIPCHK(hipMemsetAsync(c->s_qflag.p, 0, (size_t)n_reads * 4 + 4, st));
-	HIPCHK(launch_pack_reads(seq, off, len, n_reads, (uint32_t*)c->s_pack.p, (uint32_t*)c->s_qflag.p, st));
-	SeedArgs a{};
-	a.seq = seq; a.off = off; a.len = len; a.n_tasks = (uint32_t)n_reads; a.min_len = min_len; a.next = ctr;
-	a.packed = (const uint32_t*)c->s_pack.p; a.qflag = (const uint32_t*)c->s_qflag.p;
-	a.slots = (SeedRec*)c->s_slots1.p; a.slot_cnt = cnt; a.kslot = kSlot1;
-	a.spill = (SeedRec*)c->s_spill.p; a.n_spill = n_spill; a.spill_cap = sc;
-	a.stats = stats;
-	const bool seg = c->seg_len > 0 && n_reads > 0;
-	if (seg) TRY(seg_setup(c, st, n_reads, len, a));
-	if (ev_pre) HIPCHK(hipEventRecord(ev_pre, st));
-	if (n_reads > 0) HIPCHK(launch_smem(c->ix, a, 1, gather_grid(c, n_reads), st));
-	if (seg) TRY(seg_run(c, st, n_reads, len, a));
-	if (ev) HIPCHK(hipEventRecord(ev[0], st));
-	if (n_reads > 0) {
-		HIPCHK(launch_subtask((const SeedRec*)c->s_slots1.p, cnt, kSlot1, n_reads, (const SeedRec*)c->s_spill.p, n_spill, sc,
-		                      min_len, max_sub_occ, (SeedTask*)c->b_task.p, ctr + 1, tc, st));
-		HIPCHK(hipMemsetAsync(ctr, 0, 4, st));
-		SeedArgs a2 = a;
+	if (sp.n_reads > 0) {
+		HIPCHK(launch_subtask((const SeedRec*)c->s_slots1.p, cnt, kSlot1, sp.n_reads, (const SeedRec*)c->s_spill.p, n_spill, sc,
+		                      sp.min_len, sp.max_sub_occ, (SeedTask*)c->b_task.p, ctr + 1, tc, st));
+		HIPCHK(zero_async(ctr, 4, st));
+		SeedArgs a2 = sp.a;
 		a2.tasks = (const SeedTask*)c->b_task.p; a2.n_tasks = 0xffffffffu; a2.n_tasks_dev = ctr + 1; a2.task_cap = tc;
-		a2.slots = (SeedRec*)c->s_slots2.p; a2.slot_cnt = cnt + n_reads; a2.kslot = kSlot2;
-		a2.stats = stats ? stats + 3 : nullptr;
-		HIPCHK(launch_smem(c->ix, a2, 2, gather_grid(c, n_reads), st));
+		a2.slots = (SeedRec*)c->s_slots2.p; a2.slot_cnt = cnt + sp.n_reads; a2.kslot = kSlot2;
+		a2.stats = sp.stats ? sp.stats + 3 : nullptr;
+		HIPCHK(launch_smem(c->ix, a2, 2, gather_grid(c, sp.n_reads), st));
 	}
 	HIPCHK(scan_u32(c->s_tmp.p, &tmp_bytes, cnt, (uint32_t*)c->s_off.p, n_cnt, st));
-	HIPCHK(launch_compact((const SeedRec*)c->s_slots1.p, (const SeedRec*)c->s_slots2.p, kSlot1, kSlot2, n_reads, n_cnt, cnt,
+	HIPCHK(launch_compact((const SeedRec*)c->s_slots1.p, (const SeedRec*)c->s_slots2.p, kSlot1, kSlot2, sp.n_reads, n_cnt, cnt,
 	                      (const uint32_t*)c->s_off.p, (const SeedRec*)c->s_spill.p, n_spill, sc, (SeedRec*)c->p_recs.p, rc, n_out, st));
+	return MBG_OK;
+}
+
+// Two-pass seeding of device-resident reads into c->p_recs (flat; c->cap_rec entries) with the
+// record count at *n_out_dev.  Per-task output slots (kSlot1 per read, kSlot2 per pass-2 task) are
+// compacted by an exclusive scan, so no output position needs an atomic except overflow spills.
+// Counters in c->b_ctr: u32 [0] task counter, [1] pass-2 task count, i64 [1] record count,
+// u64 [3] spill count (u32 [4] is the SA row counter, u64 [6] the gather checksum, u32 [14..18] the
+// DP job counters).
+//  * inner: the caller is capturing the stream itself; the three phases are only enqueued.
+//  * otherwise, with graphs on, the phases before and after pass 1 run as captured graphs and pass 1
+//    between them as plain launches, so that ev_pre (before pass 1), ev[0] (after it) and ev[1] (after
+//    pass 2) time it; with graphs off every launch goes to the stream.
+static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
+                     int32_t min_len, int32_t max_sub_occ, unsigned long long *stats, int64_t **n_out_dev, hipEvent_t *ev,
+                     hipEvent_t ev_pre = nullptr, bool inner = false)
+{
+	if (!inner) TRY(seed_alloc(c, st, n_reads));
+	*n_out_dev = (int64_t*)((uint32_t*)c->b_ctr.p + 2);
+	SeedPass sp{n_reads, min_len, max_sub_occ, off, len, seq, stats, SeedArgs{}};
+	const bool seg = c->seg_len > 0 && n_reads > 0;
+	if (inner) {                // events only from a caller that is not capturing
+		TRY(seed_phase_pre(c, st, sp));
+		if (ev_pre) HIPCHK(hipEventRecord(ev_pre, st));
+		TRY(seed_phase_pass1(c, st, sp));
+		if (ev) HIPCHK(hipEventRecord(ev[0], st));
+		TRY(seed_phase_post(c, st, sp));
+		if (ev) HIPCHK(hipEventRecord(ev[1], st));
+		return MBG_OK;
+	}
+	TRY(run_graph(c, st, seg ? kGraphSeedPreSeg : kGraphSeedPre, [&](hipStream_t s) { return seed_phase_pre(c, s, sp); }));
+	if (ev_pre) HIPCHK(hipEventRecord(ev_pre, st));
+	TRY(seed_phase_pass1(c, st, sp));
+	if (ev) HIPCHK(hipEventRecord(ev[0], st));
+	TRY(run_graph(c, st, kGraphSeedPost, [&](hipStream_t s) { return seed_phase_post(c, s, sp); }));
 	if (ev) HIPCHK(hipEventRecord(ev[1], st));
-	*n_out_dev = n_out;
 	return MBG_OK;
 }
 
@@ -1810,6 +1945,14 @@ int mbg_counters(mbg_ctx_t *c, mbg_counters_t *out, int reset)
 	return MBG_OK;
 }
 
+int mbg_use_graphs(mbg_ctx_t *c, int on)
+{
+	if (c == nullptr) return MBG_EARG;
+	std::lock_guard<std::mutex> lk(c->mu);
+	c->use_graphs = on != 0;
+	return MBG_OK;
+}
+
 int mbg_count_work(mbg_ctx_t *c, int on)
 {
 	if (c == nullptr) return MBG_EARG;
@@ -1901,65 +2044,70 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 	const int64_t job_cap = 2 * (int64_t)n_reads + 16;
 	if (c->cap_rows < 16 * (int64_t)n_reads + 1024) c->cap_rows = 16 * (int64_t)n_reads + 1024;
 	for (int attempt = 0; attempt < 4; ++attempt) {
+		// every pool first: the chain below allocates nothing, so that it can run as one graph
 		TRY(dgrow(c->p_best, (size_t)n_reads * 8 + 8));
 		TRY(dgrow(c->p_jobs, job_cap * sizeof(DpJob))); TRY(dgrow(c->p_meta, job_cap * sizeof(JobMeta)));
 		TRY(dgrow(c->p_qb, (size_t)n_reads * max_len + 16)); TRY(dgrow(c->p_tb, (size_t)job_cap * max_tlen + 16));
 		TRY(dgrow(c->p_res, job_cap * sizeof(DpRes)));
 		TRY(dgrow(c->p_cig, (size_t)job_cap * (max_len + max_tlen) * 4 + 16));
 		TRY(dgrow(c->p_ctr, 256));
-		unsigned long long *pc = (unsigned long long*)c->p_ctr.p;  // [0..5] job sink, [6] rows, [8..14] work stats, [15] jobs past the pools
-		HIPCHK(hipMemsetAsync(pc, 0, 256, st));
-		HIPCHK(hipMemsetAsync(c->p_best.p, 0, (size_t)n_reads * 8 + 8, st));
-		HIPCHK(hipEventRecord(c->ev[0], st));
-		// seeding: two launches of the persistent state machine, then compaction
-		int64_t *n_rec = nullptr;
-		TRY(seed_core(c, st, n_reads, off, len, seq, opt->min_len, opt->max_sub_occ, collect_stats ? pc + 8 : nullptr, &n_rec, c->ev + 1));
+		TRY(seed_alloc(c, st, n_reads));
 		const int64_t rc = c->cap_rec, rw = c->cap_rows;
 		TRY(dgrow(c->p_rows, rw * 8)); TRY(dgrow(c->p_rowrec, rw * 4)); TRY(dgrow(c->p_first, rc * 4));
 		TRY(dgrow(c->r_cnt, rc * 4)); TRY(dgrow(c->r_off, rc * 4));
 		size_t tmp_bytes = 0;
 		HIPCHK(scan_u32(nullptr, &tmp_bytes, nullptr, nullptr, rc, st));
-		TRY(dgrow(c->s_tmp, tmp_bytes));
-		// anchor rows at scanned offsets, SA locate
-		const SeedRec *recs = (const SeedRec*)c->p_recs.p;
-		HIPCHK(launch_row_counts(recs, n_rec, rc, opt->max_occ, (uint32_t*)c->r_cnt.p, st));
-		HIPCHK(scan_u32(c->s_tmp.p, &tmp_bytes, (const uint32_t*)c->r_cnt.p, (uint32_t*)c->r_off.p, rc, st));
-		HIPCHK(launch_anchor_rows(recs, n_rec, rc, opt->max_occ, (const uint32_t*)c->r_cnt.p, (const uint32_t*)c->r_off.p,
-		                          (uint64_t*)c->p_rows.p, (uint32_t*)c->p_rowrec.p, (uint32_t*)c->p_first.p, pc + 6, rw, st));
-		HIPCHK(launch_best_seed(recs, n_rec, rc, (unsigned long long*)c->p_best.p, (unsigned)c->n_cu * 8, st));
-		HIPCHK(hipEventRecord(c->ev[3], st));
-		uint32_t *ctr = (uint32_t*)c->b_ctr.p;
-		HIPCHK(hipMemsetAsync(ctr + 4, 0, 4, st));
-		HIPCHK(launch_sa(c->ix, rw, (const int64_t*)(pc + 6), (uint64_t*)c->p_rows.p, ctr + 4, collect_stats ? pc + 14 : nullptr,
-		                 persistent_grid(c, 4 * (int64_t)n_reads + 1, 256), st));
-		HIPCHK(hipEventRecord(c->ev[4], st));
-		const SeedRec *recs_c = recs;
-		(void)recs_c;
-		// extension jobs and DP
-		HIPCHK(launch_ext_jobs(c->lb, mo, n_reads, off, len, seq, recs, (const uint32_t*)c->p_first.p, (const uint64_t*)c->p_rows.p,
-		                       (const unsigned long long*)c->p_best.p, (DpJob*)c->p_jobs.p, (JobMeta*)c->p_meta.p, (uint8_t*)c->p_qb.p,
-		                       (uint8_t*)c->p_tb.p, job_cap, (int64_t)n_reads * max_len, job_cap * max_tlen, pc, st));
-		HIPCHK(hipEventRecord(c->ev[5], st));
-		{
-			int64_t g = (int64_t)c->n_cu * 32;
-			if (g > job_cap) g = job_cap;
-			const int64_t max_by_mem = ((int64_t)2 << 30) / slab;
-			if (g > max_by_mem) g = max_by_mem;
-			if (g < 1) g = 1;
-			TRY(dgrow(c->b_slab, (size_t)g * slab));
-			DpArgs d{};
+		TRY(dgrow(c->s_tmp, tmp_bytes > c->s_tmp.n ? tmp_bytes : c->s_tmp.n));
+		int64_t g = (int64_t)c->n_cu * 32;
+		if (g > job_cap) g = job_cap;
+		const int64_t max_by_mem = ((int64_t)2 << 30) / slab;
+		if (g > max_by_mem) g = max_by_mem;
+		if (g < 1) g = 1;
+		TRY(dgrow(c->b_slab, (size_t)g * slab));
+		DpArgs d{};
+		TRY(dp_state(c, g, cap_dp, tcap_dp, d));
+		TRY(dgrow(c->b_defer, (size_t)job_cap * 4));
+		unsigned long long *pc = (unsigned long long*)c->p_ctr.p;  // [0..5] job sink, [6] rows, [8..14] work stats, [15] jobs past the pools
+		const bool timed = !c->use_graphs;                          // phase events only when the launches go one by one
+		auto chain = [&](hipStream_t s) -> int {
+			HIPCHK(zero_async(pc, 256, s));
+			HIPCHK(zero_async(c->p_best.p, (size_t)n_reads * 8 + 8, s));
+			// seeding: two launches of the persistent state machine, then compaction
+			int64_t *n_rec = nullptr;
+			TRY(seed_core(c, s, n_reads, off, len, seq, opt->min_len, opt->max_sub_occ, collect_stats ? pc + 8 : nullptr, &n_rec,
+			              timed ? c->ev + 1 : nullptr, nullptr, true));
+			size_t tb = c->s_tmp.n;
+			// anchor rows at scanned offsets, SA locate
+			const SeedRec *recs = (const SeedRec*)c->p_recs.p;
+			HIPCHK(launch_row_counts(recs, n_rec, rc, opt->max_occ, (uint32_t*)c->r_cnt.p, s));
+			HIPCHK(scan_u32(c->s_tmp.p, &tb, (const uint32_t*)c->r_cnt.p, (uint32_t*)c->r_off.p, rc, s));
+			HIPCHK(launch_anchor_rows(recs, n_rec, rc, opt->max_occ, (const uint32_t*)c->r_cnt.p, (const uint32_t*)c->r_off.p,
+			                          (uint64_t*)c->p_rows.p, (uint32_t*)c->p_rowrec.p, (uint32_t*)c->p_first.p, pc + 6, rw, s));
+			HIPCHK(launch_best_seed(recs, n_rec, rc, (unsigned long long*)c->p_best.p, (unsigned)c->n_cu * 8, s));
+			if (timed) HIPCHK(hipEventRecord(c->ev[3], s));
+			uint32_t *ctr = (uint32_t*)c->b_ctr.p;
+			HIPCHK(zero_async(ctr + 4, 4, s));
+			HIPCHK(launch_sa(c->ix, rw, (const int64_t*)(pc + 6), (uint64_t*)c->p_rows.p, ctr + 4, collect_stats ? pc + 14 : nullptr,
+			                 persistent_grid(c, 4 * (int64_t)n_reads + 1, 256), s));
+			if (timed) HIPCHK(hipEventRecord(c->ev[4], s));
+			// extension jobs and DP
+			HIPCHK(launch_ext_jobs(c->lb, mo, n_reads, off, len, seq, recs, (const uint32_t*)c->p_first.p, (const uint64_t*)c->p_rows.p,
+			                       (const unsigned long long*)c->p_best.p, (DpJob*)c->p_jobs.p, (JobMeta*)c->p_meta.p, (uint8_t*)c->p_qb.p,
+			                       (uint8_t*)c->p_tb.p, job_cap, (int64_t)n_reads * max_len, job_cap * max_tlen, pc, s));
+			if (timed) HIPCHK(hipEventRecord(c->ev[5], s));
 			d.jobs = (const DpJob*)c->p_jobs.p; d.n_jobs = (int32_t)job_cap; d.n_jobs_dev = pc;
 			d.qseq = (const uint8_t*)c->p_qb.p; d.tseq = (const uint8_t*)c->p_tb.p;
 			d.res = (DpRes*)c->p_res.p; d.cigar = (uint32_t*)c->p_cig.p;
 			d.pbuf = (uint8_t*)c->b_slab.p; d.slab = slab; d.qt_cap = qt_dp; d.m = 5;
-			TRY(dp_state(c, g, cap_dp, tcap_dp, d));
 			d.next = (unsigned int*)c->b_ctr.p + 14;
-			HIPCHK(hipMemsetAsync(d.next, 0, 20, st));
-			TRY(dgrow(c->b_defer, (size_t)job_cap * 4));
+			HIPCHK(zero_async(d.next, 20, s));
 			d.defer = (int32_t*)c->b_defer.p;
 			dp_set_matrix(d, 5, opt->mat);
-			HIPCHK(launch_dp(dual, d, (unsigned)g, st));
-		}
+			HIPCHK(launch_dp(dual, d, (unsigned)g, s));
+			return MBG_OK;
+		};
+		HIPCHK(hipEventRecord(c->ev[0], st));
+		TRY(run_graph(c, st, c->seg_len > 0 && n_reads > 0 ? kGraphStepSeg : kGraphStep, chain));
 		HIPCHK(hipEventRecord(c->ev[6], st));
 		unsigned long long hc[16];
 		HIPCHK(hipMemcpyAsync(hc, pc, sizeof(hc), hipMemcpyDeviceToHost, st));
@@ -1973,12 +2121,14 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 		stats->occ_blocks[0] = hc[8]; stats->kmer_lookups[0] = hc[9]; stats->extend_steps[0] = hc[10];
 		stats->occ_blocks[1] = hc[11]; stats->kmer_lookups[1] = hc[12]; stats->extend_steps[1] = hc[13]; stats->lf_steps = hc[14];
 		float ms;
-		HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); stats->ms_seed1 = ms;
-		HIPCHK(hipEventElapsedTime(&ms, c->ev[1], c->ev[2])); stats->ms_seed2 = ms;
-		HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3])); stats->ms_rows = ms;
-		HIPCHK(hipEventElapsedTime(&ms, c->ev[3], c->ev[4])); stats->ms_sa = ms;
-		HIPCHK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5])); stats->ms_jobs = ms;
-		HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6])); stats->ms_dp = ms;
+		if (timed) {              // the phases: only when the chain ran launch by launch (MBG_GRAPH=0)
+			HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); stats->ms_seed1 = ms;
+			HIPCHK(hipEventElapsedTime(&ms, c->ev[1], c->ev[2])); stats->ms_seed2 = ms;
+			HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3])); stats->ms_rows = ms;
+			HIPCHK(hipEventElapsedTime(&ms, c->ev[3], c->ev[4])); stats->ms_sa = ms;
+			HIPCHK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5])); stats->ms_jobs = ms;
+			HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6])); stats->ms_dp = ms;
+		}
 		HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[6])); stats->ms_total = ms;
 		if (collect_stats && stats->n_jobs > 0) {
 			std::vector<DpRes> res(stats->n_jobs);

@@ -317,6 +317,8 @@ void mbx_get_stats(mbx_stats_t *st)
 		st->dev.ms_group += f.ms_group, st->dev.ms_anchor += f.ms_anchor, st->dev.ms_chain += f.ms_chain;
 		st->dev.ms_front_copy += f.ms_front_copy, st->dev.ms_front += f.ms_front;
 		st->dev.seed1_occ_blocks += f.seed1_occ_blocks, st->dev.seed1_kmer_lookups += f.seed1_kmer_lookups;
+		st->dev.graph_launches += f.graph_launches, st->dev.graph_instantiations += f.graph_instantiations;
+		st->dev.direct_chains += f.direct_chains;
 	}
 	pthread_mutex_unlock(&g_st_mu);
 }

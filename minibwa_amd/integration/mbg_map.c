@@ -144,13 +144,15 @@ static void report(void)
 	        "ll_jobs %ld ll_s %.3f | "
 	        "dev_seed1_ms %.3f dev_seed2_ms %.3f dev_group_ms %.3f dev_anchor_ms %.3f dev_chain_ms %.3f dev_front_copy_ms %.3f "
 	        "dev_front_ms %.3f seed_records %ld anchor_slots %ld seed1_occ_blocks %lu seed1_kmer_lookups %lu dev_dp_launches %ld "
-	        "dev_dp_jobs %ld dev_dp_cells %ld dev_dp_ms %.3f dev_ll_launches %ld dev_ll_cells %ld dev_ll_ms %.3f\n",
+	        "dev_dp_jobs %ld dev_dp_cells %ld dev_dp_ms %.3f dev_ll_launches %ld dev_ll_cells %ld dev_ll_ms %.3f graph_launches %ld "
+	        "graph_instantiations %ld direct_chains %ld\n",
 	        mbx_name(), g_fmt_native ? "native" : "reference", (long)xs.fronts, (long)xs.front_reads, xs.front_s, (long)ss.batches, (long)ss.waits, (long)ss.coroutines_peak,
 	        ss.batch_s, ss.task_s, ss.busy_s, ss.idle_s, (long)xs.dp_calls, (long)xs.dp_jobs, xs.dp_s, xs.dp_pack_s, (long)xs.ll_calls,
 	        (long)xs.ll_jobs, xs.ll_s,
 	        d->ms_seed1, d->ms_seed2, d->ms_group, d->ms_anchor, d->ms_chain, d->ms_front_copy, d->ms_front, (long)d->front_records,
 	        (long)d->front_slots, (unsigned long)d->seed1_occ_blocks, (unsigned long)d->seed1_kmer_lookups, (long)d->dp_launches,
-	        (long)d->dp_jobs, (long)d->dp_cells, d->ms_dp, (long)d->ll_launches, (long)d->ll_cells, d->ms_ll);
+	        (long)d->dp_jobs, (long)d->dp_cells, d->ms_dp, (long)d->ll_launches, (long)d->ll_cells, d->ms_ll, (long)d->graph_launches,
+	        (long)d->graph_instantiations, (long)d->direct_chains);
 }
 
 static mbs_pool_t *pool_for(int n_threads)

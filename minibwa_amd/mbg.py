@@ -55,6 +55,15 @@ class IdxInfo(C.Structure):
                 ("pre_len", C.c_int32), ("sa_bit", C.c_int32), ("device_bytes", C.c_uint64)]
 
 
+class Counters(C.Structure):       # mbg_counters_t
+    _fields_ = ([(f, C.c_int64) for f in ("fronts", "front_reads", "front_records", "front_slots")] +
+                [(f, C.c_double) for f in ("ms_seed1", "ms_seed2", "ms_group", "ms_anchor", "ms_chain", "ms_front_copy", "ms_front")] +
+                [(f, C.c_uint64) for f in ("seed1_occ_blocks", "seed1_kmer_lookups")] +
+                [(f, C.c_int64) for f in ("dp_launches", "dp_jobs", "dp_cells")] + [("ms_dp", C.c_double)] +
+                [(f, C.c_int64) for f in ("ll_launches", "ll_jobs", "ll_cells")] + [("ms_ll", C.c_double)] +
+                [(f, C.c_int64) for f in ("graph_launches", "graph_instantiations", "direct_chains")])
+
+
 class AlignOpt(C.Structure):       # mbg_align_opt_t
     _fields_ = [(f, C.c_int32) for f in ("a", "b", "b_ts", "q", "e", "q2", "e2", "meth")] + [("max_sw_mat", C.c_int64)]
 
@@ -182,6 +191,9 @@ def lib():
         L.mbg_align_pair_batch.argtypes = [_P, C.POINTER(AlignOpt), C.c_int32, _P, _P, C.c_int64, _P, C.c_int64, _P, _P, _P,
                                            C.c_int64]
         L.mbg_getseq.argtypes = [_P, C.c_int64, _P, _P, _P, _P, _P, C.c_int64, _P]
+        L.mbg_use_graphs.argtypes = [_P, C.c_int]
+        L.mbg_counters.argtypes = [_P, C.POINTER(Counters), C.c_int]
+        L.mbg_count_work.argtypes = [_P, C.c_int]
         L.mbg_seed_batch_dev.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, C.c_int64,
                                          C.POINTER(C.c_int64)]
         L.mbg_seed_batch_dev_async.argtypes = [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, _P, C.c_int64, _P]
@@ -230,7 +242,8 @@ EXPORTED = ("mbg_strerror", "mbg_last_hip_error", "mbg_init", "mbg_destroy", "mb
             "mbg_dev_free", "mbg_stream_copy", "mbg_gather_bench", "mbg_idx_attach", "mbg_ll_batch",
             "mbg_ll_batch_dev", "mbg_matesw_batch", "mbg_update_extra_batch",
             "mbg_anchor_batch", "mbg_anchor_batch_dev", "mbg_lchain_batch", "mbg_lchain_batch_dev", "mbg_fq_open",
-            "mbg_fq_close", "mbg_fq_buf_new", "mbg_fq_buf_free", "mbg_fq_read", "mbg_fq_read_frag")
+            "mbg_fq_close", "mbg_fq_buf_new", "mbg_fq_buf_free", "mbg_fq_read", "mbg_fq_read_frag", "mbg_use_graphs",
+            "mbg_counters", "mbg_count_work")
 ALIGN_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"), ("w", "<i4"),
                             ("end_bonus", "<i4"), ("zdrop", "<i4"), ("flag", "<i4"), ("cigar_off", "<i8")], align=True)
 LL_JOB_DTYPE = np.dtype([("qoff", "<i8"), ("toff", "<i8"), ("qlen", "<i4"), ("tlen", "<i4"), ("size", "<i4"),
@@ -619,6 +632,15 @@ class Context:
         if a.nbytes:
             check(self.L.mbg_copy_to_host(self.h, _ptr(a), C.c_void_p(dev_addr), a.nbytes), "mbg_copy_to_host")
         return a
+
+    def use_graphs(self, on: bool):
+        """Launch chains as captured HIP graphs (on) or launch by launch (off); results are the same."""
+        check(self.L.mbg_use_graphs(self.h, 1 if on else 0), "mbg_use_graphs")
+
+    def counters(self, reset=False) -> dict:
+        c = Counters()
+        check(self.L.mbg_counters(self.h, C.byref(c), 1 if reset else 0), "mbg_counters")
+        return {f: getattr(c, f) for f, _ in Counters._fields_}
 
     # ---- DP ----------------------------------------------------------------------------------
     def _dp(self, fn, name, queries, targets, params, mat, m=5):
