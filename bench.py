@@ -215,14 +215,22 @@ def main():
                                                             "occ_blocks", "kmer_lookups", "extend_steps", "lf_steps")}))
     for _ in range(args.warmup):
         step()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st, out = step()
+        log(f"step {i}: {st.ms_total:.2f} ms device (one captured graph)")
+    el = time.perf_counter() - t0  # every step synchronises its stream before returning
+    value = args.steps * args.pairs / el
+    # the same steps launch by launch, for the phases' HIP-event times (a graph reports only its total)
+    ctx.use_graphs(False)
     per = []
     t0 = time.perf_counter()
     for i in range(args.steps):
         st, out = step()
         per.append(st.as_dict())
-        log(f"step {i}: {st.ms_total:.2f} ms device")
-    el = time.perf_counter() - t0  # every step synchronises its stream before returning
-    value = args.steps * args.pairs / el
+    el_direct = time.perf_counter() - t0
+    ctx.use_graphs(True)
+    graph_cmp = graph_overhead(ctx, n_reads, d_off, d_len, d_seq, rl, opt)
 
     # roofline of the dominant kernel (seeding pass 1): algorithmic bytes = 64 B per occ block
     # fetched + 32 B per k-mer cache lookup, over the kernel's HIP-event duration
@@ -271,10 +279,37 @@ def main():
         "cpu_baseline": cpu,
         "latency": latency,
         "phases_ms": phases,
+        "ms_per_step_direct": round(el_direct / args.steps * 1000, 3),
+        "graph_vs_direct": graph_cmp,
         "step_work": {k: work_stats[k] for k in ("n_rec", "n_rows", "n_anchored", "n_fast", "n_jobs", "cells", "lf_steps")},
     }
     print(json.dumps(res), flush=True)
     ctx.close()
+
+
+def graph_overhead(ctx, n_reads, d_off, d_len, d_seq, rl, opt, sizes=(1000, 5000, 15000, 60000), reps=20):
+    """Wall time per device pass (seed -> locate -> extend, synchronised) at serving batch sizes, as one
+    captured graph and launch by launch: the launch overhead the graph removes."""
+    rows = []
+    for pairs in sizes:
+        n = min(2 * pairs, n_reads)
+        row = {"pairs": n // 2}
+        for mode, on in (("graph", True), ("direct", False)):
+            ctx.use_graphs(on)
+            for _ in range(3):
+                ctx.map_step_dev(n, d_off.ptr, d_len.ptr, d_seq.ptr, rl, opt)
+            t0 = time.perf_counter()
+            dev = 0.0
+            for _ in range(reps):
+                st, _ = ctx.map_step_dev(n, d_off.ptr, d_len.ptr, d_seq.ptr, rl, opt)
+                dev += st.ms_total
+            row[f"ms_{mode}"] = round((time.perf_counter() - t0) / reps * 1000, 4)
+            row[f"dev_ms_{mode}"] = round(dev / reps, 4)
+        ctx.use_graphs(True)
+        row["saved_ms"] = round(row["ms_direct"] - row["ms_graph"], 4)
+        rows.append(row)
+        log("graph vs direct:", json.dumps(row))
+    return rows
 
 
 def _ref_timings(err):
@@ -481,12 +516,14 @@ def mapper_bench(args):
         if l.startswith("[mbg_map]") or l.startswith("[mbg_exec] front"):
             log(l)
     pass_s = [float(l.split()[3]) for l in r.stderr.split("\n") if l.startswith("[mbg_bench] pass ")]
+    pass_cpu = [float(l.split()[5]) for l in r.stderr.split("\n") if l.startswith("[mbg_bench] pass ") and len(l.split()) > 5]
     load_s = [float(l.split()[2]) for l in r.stderr.split("\n") if l.startswith("[mbg_bench] load ")][0]
     fronts = [_kv(l) for l in r.stderr.split("\n") if l.startswith("[mbg_exec] front ")]
     summ = _kv([l for l in r.stderr.split("\n") if l.startswith("[mbg_map] summary:")][-1])
     assert len(pass_s) == passes, r.stderr[-2000:]
     for i, t in enumerate(pass_s):
-        log(f"pass {i}{' (warmup)' if i < args.warmup else ''}: {t:.3f} s = {n_units / t:,.0f} {unit}")
+        cpu_i = f", host CPU {pass_cpu[i]:.2f} s ({pass_cpu[i] / t:.1f} cores busy)" if i < len(pass_cpu) else ""
+        log(f"pass {i}{' (warmup)' if i < args.warmup else ''}: {t:.3f} s = {n_units / t:,.0f} {unit}{cpu_i}")
     timed = pass_s[args.warmup:]
     el = sum(timed)
     value = args.steps * n_units / el
@@ -617,6 +654,7 @@ def mapper_bench(args):
         "gpu_cli_single_run": cli_run,
         "index_load_s": round(load_s, 2),
         "device_batches_per_pass": round(summ.get("batches", 0) / passes, 1),
+        "host_cpu_s_per_pass": round(sum(pass_cpu[args.warmup:]) / max(1, len(pass_cpu[args.warmup:])), 3) if pass_cpu else None,
     }
     print(json.dumps(res), flush=True)
     if not args.workdir:
@@ -656,9 +694,20 @@ def serve_mapper(args, ref, prefix, work, cores):
         points.append(pt)
     if not points:
         return None
+    # the lowest rate again with every launch chain issued launch by launch (MBG_GRAPH=0), same schedule
+    direct = None
+    qps0 = float(args.serve_sweep.split(",")[0])
+    W.write_schedule(sp, W.schedule(args.serve_requests, qps0, seed=100))
+    r = subprocess.run([exe, "-t", str(cores), prefix] + list(pools) + [sp], capture_output=True, text=True, timeout=900,
+                       env=dict(os.environ, MBG_GRAPH="0"))
+    if r.returncode == 0:
+        direct = json.loads(r.stdout.strip().split("\n")[-1])
+        direct["offered_qps_target"] = qps0
+        log(f"serve @{qps0:g} req/s without graphs: p50 {direct['p50_ms']:.1f} ms, p99 {direct['p99_ms']:.1f} ms "
+            f"(with graphs: p50 {points[0]['p50_ms']:.1f} ms, p99 {points[0]['p99_ms']:.1f} ms)")
     return {"workload": "Poisson arrivals; request = 1-10,000 pairs 2x150 bp (log-uniform) or, 5% of requests, one 10 kb "
                         "read; -I 400,50; latency = records back on the host - arrival",
-            "requests_per_point": args.serve_requests, "points": points}
+            "requests_per_point": args.serve_requests, "points": points, "lowest_rate_without_graphs": direct}
 
 
 

@@ -112,3 +112,24 @@ def test_tuned_settings_file_changes_nothing(exes, tmp_path_factory, tmp_path):
     st = check_sets(exes[0], exes[1], "rescue", tmp_path_factory, threads=4, extra_env={"MBG_TUNE": str(tune)})
     assert st["executor"] == "cpu-reference"
     assert int(st["batches"]) > int(st0["batches"])      # the file was read: smaller device batches, more of them
+
+
+def test_many_mini_batches_match_reference(exes, tmp_path_factory):
+    """Mini-batches of 300 kbp (-K 300k) over the rescue-heavy pairs: ten mini-batches, each with its
+    own insert-size estimate (map-main.c:180-190), the pipeline's read, map and write steps of
+    neighbouring mini-batches overlapping; the SAM must equal the reference binary's with the same -K."""
+    import subprocess
+    from tests.mapper_sets import _body_md5, _files
+    if not os.path.exists(REF_CLI):
+        pytest.skip("oracle/_ref/minibwa not built (make -C oracle)")
+    f = _files(tmp_path_factory)
+    args = ["map", "-t4", "-K", "300k", os.path.join(GOLDEN, "cfg0"), f["c1"], f["r2"]]
+    ref = subprocess.run([REF_CLI] + args, capture_output=True, text=True, timeout=300)
+    assert ref.returncode == 0, ref.stderr[-2000:]
+    env = dict(os.environ, MBG_MAP_STATS="1")
+    got = subprocess.run([exes[0]] + args, capture_output=True, text=True, timeout=300, env=env)
+    assert got.returncode == 0, got.stderr[-3000:]
+    pe_r = [l for l in ref.stderr.split("\n") if "mb_pestat::FR] (25, 50, 75)" in l]
+    pe_g = [l for l in got.stderr.split("\n") if "mb_pestat::FR] (25, 50, 75)" in l]
+    assert len(pe_r) >= 8 and pe_g == pe_r
+    assert _body_md5(got.stdout) == _body_md5(ref.stdout)
