@@ -21,6 +21,7 @@
 // lets it run ahead: the records, in file order, are kseq_read's whatever the chunks are.
 //
 // Host code only; no GPU call.
+#include <pthread.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -100,6 +101,7 @@ public:
 private:
 	void run()
 	{
+		pthread_setname_np(pthread_self(), "fq-read");
 		for (;;) {
 			std::vector<char> b(kBlock);
 			const int n = gzread(f_, b.data(), (unsigned)kBlock);
@@ -330,6 +332,7 @@ public:
 private:
 	void run()
 	{
+		pthread_setname_np(pthread_self(), "fq-parse");
 		for (bool end = false; !end;) {
 			auto b = std::make_shared<Parsed>();
 			while (b->n() < kBatchRecs) {

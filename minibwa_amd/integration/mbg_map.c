@@ -514,6 +514,7 @@ static struct {
 static void *front_main(void *arg)
 {
 	(void)arg;
+	pthread_setname_np(pthread_self(), "mbg-front");
 	for (;;) {
 		batch_t *b;
 		int slot;

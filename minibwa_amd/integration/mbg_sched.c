@@ -81,7 +81,7 @@ struct thr_s {
 };
 
 struct mbs_pool_s {
-	int n;
+	int n, id;
 	int64_t stack_bytes;
 	thr_t *thr;
 	pthread_mutex_t mu, run_mu;
@@ -426,11 +426,16 @@ static void job_finished(mbs_pool_t *p)
 	pthread_mutex_unlock(&p->mu);
 }
 
+static atomic_int g_pool_ids;     /* thread names: mbs<pool>w<i> for workers, mbs<pool>d for the dispatcher */
+
 static void *worker(void *arg)
 {
 	thr_t *t = (thr_t*)arg;
 	mbs_pool_t *p = t->pool;
 	int64_t my_gen = 0;
+	char nm[16];
+	snprintf(nm, sizeof(nm), "mbs%dw%d", p->id, t->tid);
+	pthread_setname_np(pthread_self(), nm);
 	tls_thr = t;
 	for (;;) {
 		wait_job(p, &my_gen);
@@ -446,6 +451,9 @@ static void *dispatcher(void *arg)
 {
 	mbs_pool_t *p = (mbs_pool_t*)arg;
 	int64_t my_gen = 0;
+	char nm[16];
+	snprintf(nm, sizeof(nm), "mbs%dd", p->id);
+	pthread_setname_np(pthread_self(), nm);
 	for (;;) {
 		wait_job(p, &my_gen);
 		if (p->stop) break;
@@ -466,6 +474,7 @@ mbs_pool_t *mbs_pool_init(int n_threads, int64_t stack_bytes)
 	p = (mbs_pool_t*)calloc(1, sizeof(mbs_pool_t));
 	if (p == 0) return 0;
 	p->n = n_threads;
+	p->id = atomic_fetch_add(&g_pool_ids, 1);
 	p->stack_bytes = stack_bytes;
 	p->thr = (thr_t*)aligned_alloc(128, (size_t)n_threads * sizeof(thr_t));
 	if (p->thr) memset(p->thr, 0, (size_t)n_threads * sizeof(thr_t));
