@@ -496,15 +496,24 @@ static void die_rc(const char *what, int rc)
 	abort();
 }
 
-/* The CLI's front passes run on a thread of their own: the read step queues its mini-batch and
- * returns, so that reading the next mini-batch, this one's front pass (mostly device time) and the
- * previous one's mapping overlap; the map step waits for its batch's front.  One front at a time, in
- * mini-batch order, each on its own front slot (kt_pipeline holds at most three mini-batches, and a
- * slot is reused only after its previous batch has been written).  MBG_FRONT_THREAD=0 runs the front
- * inside the read step instead. */
+/* The CLI's front passes run on threads of their own: the read step queues its mini-batch and
+ * returns, so that reading the next mini-batches, their front passes (mostly device time) and the
+ * previous one's mapping overlap; the map step waits for its batch's front.  Fronts are taken in
+ * mini-batch order by MBG_FRONT_THREADS threads (one fewer than the mini-batches in flight by
+ * default), each front on its own front slot and context (a slot is reused only after its previous
+ * batch has been written: the pipeline holds at most as many mini-batches as there are slots), so that
+ * fronts share the device: a long-read front is bound by the latency of its longest dependent SMEM
+ * walks, not by the device's throughput, and the next fronts' walks fill the time.
+ * MBG_FRONT_THREADS=0 (or MBG_FRONT_THREAD=0) runs the front inside the read step instead. */
 #define FRONT_Q 8
+#define FRONT_MAX_THREADS 3
+/* mini-batches in flight in the CLI's pipeline (MBG_PIPE_DEPTH; the reference keeps three, kt_pipeline's
+ * worker count, map-main.c:290-292): each holds a front slot, so at most MBX_FRONT_SLOTS.  The order of
+ * the output and each mini-batch's contents do not depend on it. */
+static int g_pipe_depth = 3;
 static struct {
-	pthread_t th;
+	pthread_t th[FRONT_MAX_THREADS];
+	int n_th;
 	pthread_mutex_t mu;
 	pthread_cond_t cv_q, cv_done;
 	int started, stop, n, head;
@@ -887,21 +896,33 @@ void __wrap_kt_pipeline(int n_threads, void *(*func)(void*, int, void*), void *s
 		g_tl_t0 = now_s();
 		atomic_store(&g_tl_n, 0);
 	}
+	{
+		const char *e = getenv("MBG_PIPE_DEPTH");
+		g_pipe_depth = e ? atoi(e) : MBX_FRONT_SLOTS;
+		if (g_pipe_depth < 3 || n_threads < 3) g_pipe_depth = 3;
+		if (g_pipe_depth > MBX_FRONT_SLOTS) g_pipe_depth = MBX_FRONT_SLOTS;
+	}
 	if (!g_front.started) {
-		const char *e = getenv("MBG_FRONT_THREAD");
-		if (!(e && atoi(e) == 0)) {
-			g_front.stop = 0;
-			if (pthread_create(&g_front.th, 0, front_main, 0) != 0) die("pthread_create failed (front thread)");
+		const char *e = getenv("MBG_FRONT_THREAD"), *e2 = getenv("MBG_FRONT_THREADS");
+		int n = e2 ? atoi(e2) : g_pipe_depth - 1;
+		if (e && atoi(e) == 0) n = 0;
+		if (n > FRONT_MAX_THREADS) n = FRONT_MAX_THREADS;
+		if (n > g_pipe_depth - 1) n = g_pipe_depth - 1;    /* fronts waiting at once: the mini-batches in flight but one */
+		if (n > 0) {
+			int k;
+			g_front.stop = 0, g_front.n_th = n;
+			for (k = 0; k < n; ++k)
+				if (pthread_create(&g_front.th[k], 0, front_main, 0) != 0) die("pthread_create failed (front thread)");
 			g_front.started = 1;
 		}
 	}
-	__real_kt_pipeline(n_threads, pipe_step, 0, n_steps);
+	__real_kt_pipeline(n_threads >= 3 ? g_pipe_depth : n_threads, pipe_step, 0, n_steps);
 	if (g_front.started) {        /* every queued front has been waited for by its map step */
 		pthread_mutex_lock(&g_front.mu);
 		g_front.stop = 1;
-		pthread_cond_signal(&g_front.cv_q);
+		pthread_cond_broadcast(&g_front.cv_q);
 		pthread_mutex_unlock(&g_front.mu);
-		pthread_join(g_front.th, 0);
+		for (i = 0; i < g_front.n_th; ++i) pthread_join(g_front.th[i], 0);
 		g_front.started = 0;
 	}
 	for (i = 0; i < g_pipe.n_fq; ++i) mbg_fq_close(g_pipe.fq[i]);
