@@ -548,10 +548,19 @@ def mapper_bench(args):
     kernels = []
     if bytes1 > 0 and seed1_ms > 0:
         a = bytes1 / (seed1_ms * 1e-3) / 1e9
-        kernels.append({"kernel": "smem_kernel<1> (SMEM pass 1)", "bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(a / HBM_PEAK_GBS, 4), "traffic": None,
-                        "algorithmic_bytes_per_launch": int(bytes1 / fronts_pass), "avg_launch_ms": round(seed1_ms / fronts_pass, 4),
-                        "share_of_device": round(seed1_ms / dev_total, 3) if dev_total else None})
+        ent = {"kernel": "smem_kernel<1> (SMEM pass 1)", "bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": round(a / HBM_PEAK_GBS, 4), "traffic": None,
+               "algorithmic_bytes_per_launch": int(bytes1 / fronts_pass), "avg_launch_ms": round(seed1_ms / fronts_pass, 4),
+               "share_of_device": round(seed1_ms / dev_total, 3) if dev_total else None}
+        # dependent random transactions (occ blocks + k-mer lookups) per second, against the gather
+        # ceiling measured on the same resident table (bench.py --gather, committed profile)
+        ent["G_tx_per_s"] = round((occ1 + kmer1) / (seed1_ms * 1e-3) / 1e9, 2)
+        gc = os.path.join(ROOT, "profiles", "gather_ceiling_3g.json")
+        if args.ref_len == 3_100_000_000 and os.path.exists(gc):
+            ceil = max(r["G_tx_per_s"] for r in json.load(open(gc))["gather_ceiling"])
+            ent["gather_ceiling_G_tx_per_s"] = ceil
+            ent["frac_of_gather_ceiling"] = round(ent["G_tx_per_s"] / ceil, 3)
+        kernels.append(ent)
     cells = summ.get("dev_dp_cells", 0.0) / passes
     if cells > 0 and dp_ms > 0:
         ops = cells * 37 / (dp_ms * 1e-3) / 1e12      # about 37 integer operations per cell with traceback bits

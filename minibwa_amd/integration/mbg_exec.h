@@ -56,7 +56,7 @@ typedef struct {
  * their anchors as mb_lchain_dp returns them.  Valid until the next mbx_front call on the same slot;
  * calls on different slots may run concurrently (the CLI's three pipeline steps hold up to three
  * mini-batches). */
-#define MBX_FRONT_SLOTS 4
+#define MBX_FRONT_SLOTS 6
 typedef struct {
 	int32_t n;
 	const int64_t *intv_off, *n_intv;

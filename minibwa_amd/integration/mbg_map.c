@@ -506,7 +506,7 @@ static void die_rc(const char *what, int rc)
  * walks, not by the device's throughput, and the next fronts' walks fill the time.
  * MBG_FRONT_THREADS=0 (or MBG_FRONT_THREAD=0) runs the front inside the read step instead. */
 #define FRONT_Q 8
-#define FRONT_MAX_THREADS 3
+#define FRONT_MAX_THREADS 5
 /* mini-batches in flight in the CLI's pipeline (MBG_PIPE_DEPTH; the reference keeps three, kt_pipeline's
  * worker count, map-main.c:290-292): each holds a front slot, so at most MBX_FRONT_SLOTS.  The order of
  * the output and each mini-batch's contents do not depend on it. */
@@ -898,7 +898,7 @@ void __wrap_kt_pipeline(int n_threads, void *(*func)(void*, int, void*), void *s
 	}
 	{
 		const char *e = getenv("MBG_PIPE_DEPTH");
-		g_pipe_depth = e ? atoi(e) : MBX_FRONT_SLOTS;
+		g_pipe_depth = e ? atoi(e) : 4;
 		if (g_pipe_depth < 3 || n_threads < 3) g_pipe_depth = 3;
 		if (g_pipe_depth > MBX_FRONT_SLOTS) g_pipe_depth = MBX_FRONT_SLOTS;
 	}
