@@ -710,11 +710,15 @@ static int seg_run(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int32_t 
 	return MBG_OK;
 }
 
-// every pool of a seeding pass, grown before its launches
-static int seed_alloc(mbg_ctx_t *c, hipStream_t st, int32_t n_reads)
+// every pool of a seeding pass, grown before its launches.  Records and pass-2 tasks are sized from the
+// reads and, when the caller knows them (bases >= 0), from their bases too: long reads give about two
+// SMEMs per kilobase, so a pool sized by reads alone overflows on a context's first long-read batch and
+// the pass runs again.
+static int seed_alloc(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, int64_t bases = -1)
 {
-	if (c->cap_rec < 8 * (int64_t)n_reads + 1024) c->cap_rec = 8 * (int64_t)n_reads + 1024;
-	if (c->cap_task < 2 * (int64_t)n_reads + 1024) c->cap_task = 2 * (int64_t)n_reads + 1024;
+	const int64_t by_bases = bases > 0 ? bases / 256 : 0;
+	if (c->cap_rec < 8 * (int64_t)n_reads + by_bases + 1024) c->cap_rec = 8 * (int64_t)n_reads + by_bases + 1024;
+	if (c->cap_task < 2 * (int64_t)n_reads + by_bases + 1024) c->cap_task = 2 * (int64_t)n_reads + by_bases + 1024;
 	if (c->cap_spill < (int64_t)n_reads + 4096) c->cap_spill = (int64_t)n_reads + 4096;
 	const int64_t rc = c->cap_rec, tc = c->cap_task, sc = c->cap_spill, n_cnt = (int64_t)n_reads + tc;
 	TRY(dgrow(c->b_ctr, 128));
@@ -809,9 +813,9 @@ static int seed_phase_post(mbg_ctx_t *c, hipStream_t st, SeedPass &sp)
 //    pass 2) time it; with graphs off every launch goes to the stream.
 static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_t *off, const int32_t *len, const uint8_t *seq,
                      int32_t min_len, int32_t max_sub_occ, unsigned long long *stats, int64_t **n_out_dev, hipEvent_t *ev,
-                     hipEvent_t ev_pre = nullptr, bool inner = false)
+                     hipEvent_t ev_pre = nullptr, bool inner = false, int64_t bases = -1)
 {
-	if (!inner) TRY(seed_alloc(c, st, n_reads));
+	if (!inner) TRY(seed_alloc(c, st, n_reads, bases));
 	*n_out_dev = (int64_t*)((uint32_t*)c->b_ctr.p + 2);
 	SeedPass sp{n_reads, min_len, max_sub_occ, off, len, seq, stats, SeedArgs{}};
 	const bool seg = c->seg_len > 0 && n_reads > 0;
@@ -1484,7 +1488,7 @@ int mbg_map_front(mbg_ctx_t *c, int32_t n, const uint8_t *seq, const int64_t *of
 	for (int attempt = 0; attempt < 4 && !ok; ++attempt) {
 		if (wstats) HIPCHK(hipMemsetAsync(wstats, 0, 64, st));
 		TRY(seed_core(c, st, n, (const int64_t*)f[1].p, (const int32_t*)f[2].p, (const uint8_t*)f[0].p, o->min_len, o->max_sub_occ,
-		              wstats, &n_rec_dev, sev, c->fev[6]));
+		              wstats, &n_rec_dev, sev, c->fev[6], false, tot));
 		HIPCHK(hipStreamSynchronize(st));
 		ok = !seed_overflowed(c, &n_rec);
 	}
