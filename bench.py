@@ -215,21 +215,15 @@ def main():
                                                             "occ_blocks", "kmer_lookups", "extend_steps", "lf_steps")}))
     for _ in range(args.warmup):
         step()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        st, out = step()
-        log(f"step {i}: {st.ms_total:.2f} ms device (one captured graph)")
-    el = time.perf_counter() - t0  # every step synchronises its stream before returning
-    value = args.steps * args.pairs / el
-    # the same steps launch by launch, for the phases' HIP-event times (a graph reports only its total)
-    ctx.use_graphs(False)
     per = []
     t0 = time.perf_counter()
     for i in range(args.steps):
         st, out = step()
         per.append(st.as_dict())
-    el_direct = time.perf_counter() - t0
-    ctx.use_graphs(True)
+        log(f"step {i}: {st.ms_total:.2f} ms device")
+    el = time.perf_counter() - t0  # every step synchronises its stream before returning
+    value = args.steps * args.pairs / el
+    # the same pass as one captured HIP graph against launch by launch, at serving batch sizes
     graph_cmp = graph_overhead(ctx, n_reads, d_off, d_len, d_seq, rl, opt)
 
     # roofline of the dominant kernel (seeding pass 1): algorithmic bytes = 64 B per occ block
@@ -279,7 +273,6 @@ def main():
         "cpu_baseline": cpu,
         "latency": latency,
         "phases_ms": phases,
-        "ms_per_step_direct": round(el_direct / args.steps * 1000, 3),
         "graph_vs_direct": graph_cmp,
         "step_work": {k: work_stats[k] for k in ("n_rec", "n_rows", "n_anchored", "n_fast", "n_jobs", "cells", "lf_steps")},
     }
@@ -305,7 +298,7 @@ def graph_overhead(ctx, n_reads, d_off, d_len, d_seq, rl, opt, sizes=(1000, 5000
                 dev += st.ms_total
             row[f"ms_{mode}"] = round((time.perf_counter() - t0) / reps * 1000, 4)
             row[f"dev_ms_{mode}"] = round(dev / reps, 4)
-        ctx.use_graphs(True)
+        ctx.use_graphs(False)
         row["saved_ms"] = round(row["ms_direct"] - row["ms_graph"], 4)
         rows.append(row)
         log("graph vs direct:", json.dumps(row))
@@ -703,20 +696,9 @@ def serve_mapper(args, ref, prefix, work, cores):
         points.append(pt)
     if not points:
         return None
-    # the lowest rate again with every launch chain issued launch by launch (MBG_GRAPH=0), same schedule
-    direct = None
-    qps0 = float(args.serve_sweep.split(",")[0])
-    W.write_schedule(sp, W.schedule(args.serve_requests, qps0, seed=100))
-    r = subprocess.run([exe, "-t", str(cores), prefix] + list(pools) + [sp], capture_output=True, text=True, timeout=900,
-                       env=dict(os.environ, MBG_GRAPH="0"))
-    if r.returncode == 0:
-        direct = json.loads(r.stdout.strip().split("\n")[-1])
-        direct["offered_qps_target"] = qps0
-        log(f"serve @{qps0:g} req/s without graphs: p50 {direct['p50_ms']:.1f} ms, p99 {direct['p99_ms']:.1f} ms "
-            f"(with graphs: p50 {points[0]['p50_ms']:.1f} ms, p99 {points[0]['p99_ms']:.1f} ms)")
     return {"workload": "Poisson arrivals; request = 1-10,000 pairs 2x150 bp (log-uniform) or, 5% of requests, one 10 kb "
                         "read; -I 400,50; latency = records back on the host - arrival",
-            "requests_per_point": args.serve_requests, "points": points, "lowest_rate_without_graphs": direct}
+            "requests_per_point": args.serve_requests, "points": points}
 
 
 

@@ -369,9 +369,11 @@ int mbg_counters(mbg_ctx_t *ctx, mbg_counters_t *out, int reset);
 int mbg_count_work(mbg_ctx_t *ctx, int on);
 /* Launch chains without a host step inside (the seeding phases around SMEM pass 1; the whole
  * seed -> locate -> extend pass of mbg_map_step_dev) run as HIP graphs captured from the stream and
- * updated in place from call to call (on by default; MBG_GRAPH=0 in the environment of mbg_init turns
- * it off).  Results do not depend on it.  With graphs on, mbg_map_step_dev reports only its total
- * device time, not its phases.  No counterpart in the reference. */
+ * updated in place from call to call when on (off by default; MBG_GRAPH=1 in the environment of
+ * mbg_init turns it on).  Results do not depend on it.  While a context captures, no other thread of
+ * the process should allocate device memory or create streams (the capture fails with MBG_EHIP).
+ * With graphs on, mbg_map_step_dev reports only its total device time, not its phases.  No
+ * counterpart in the reference. */
 int mbg_use_graphs(mbg_ctx_t *ctx, int on);
 
 /* ---- reads at the edge: FASTA/FASTQ (host code) ------------------------------------------------ */

@@ -80,9 +80,17 @@ struct mbg_ctx_s {
 	hipEvent_t ord_ev = nullptr;
 	hipStream_t ord_st = nullptr;
 	bool ord_pending = false;
-	// launch chains captured as graphs (run_graph): one executable graph per chain kind
-	bool use_graphs = true;
+	// launch chains captured as graphs (run_graph): one executable graph per chain kind; off unless
+	// MBG_GRAPH=1 or mbg_use_graphs (a capture fails when another thread of the process allocates
+	// device memory or creates a stream meanwhile, which the mapper's front and DP threads do)
+	bool use_graphs = false;
 	hipGraphExec_t graph_exec[kGraphKinds] = {};
+	// long reads' segment lanes on a side stream beside pass 1 proper (MBG_SEED_SIDE=1 turns it on: it
+	// shortens one front pass, but with several fronts and the DP already filling the device the
+	// mapper ran slower with it)
+	bool seg_side = false;
+	hipStream_t side = nullptr;
+	hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { g_hip_err = std::string(#x) + ": " + hipGetErrorString(e_); return MBG_EHIP; } } while (0)
@@ -239,7 +247,9 @@ int mbg_init(int device, mbg_ctx_t **out)
 		if (g && atoi(g) >= 1 && atoi(g) <= 64) c->gather_bpc = atoi(g);
 		if (d && atoi(d) >= 1 && atoi(d) <= 128) c->dp_wpc = atoi(d);
 		const char *gr = getenv("MBG_GRAPH");
-		if (gr && atoi(gr) == 0) c->use_graphs = false;
+		if (gr) c->use_graphs = atoi(gr) != 0;
+		const char *sd = getenv("MBG_SEED_SIDE");
+		if (sd) c->seg_side = atoi(sd) != 0;
 	}
 	hipDeviceProp_t prop;
 	hipError_t e = hipGetDeviceProperties(&prop, device);
@@ -247,8 +257,14 @@ int mbg_init(int device, mbg_ctx_t **out)
 	if (c->n_cu <= 0) c->n_cu = 256;
 	e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
 	if (e != hipSuccess) { g_hip_err = hipGetErrorString(e); delete c; return MBG_EHIP; }
+	if (c->seg_side) {   // created here, not at first use: no stream is created while another context captures
+		if ((e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) == hipSuccess &&
+		    (e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming)) == hipSuccess)
+			e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
+		if (e != hipSuccess) { g_hip_err = hipGetErrorString(e); mbg_destroy(c); return MBG_EHIP; }
+	}
 	int rc = noalloc_setup(device);
-	if (rc != MBG_OK) { hipStreamDestroy(c->stream); delete c; return rc; }
+	if (rc != MBG_OK) { mbg_destroy(c); return rc; }
 	*out = c;
 	return MBG_OK;
 }
@@ -302,6 +318,9 @@ void mbg_destroy(mbg_ctx_t *c)
 	dfree(c->f_stats);
 	if (c->h_pinned.p) hipHostFree(c->h_pinned.p);
 	for (hipGraphExec_t &g : c->graph_exec) if (g) hipGraphExecDestroy(g);
+	if (c->side) hipStreamSynchronize(c->side), hipStreamDestroy(c->side);
+	if (c->fork_ev) hipEventDestroy(c->fork_ev);
+	if (c->join_ev) hipEventDestroy(c->join_ev);
 	if (c->stream) hipStreamDestroy(c->stream);
 	delete c;
 }
@@ -569,8 +588,10 @@ static const int32_t kSlot1 = 16, kSlot2 = 8;
 // unit.  Every call captures its chain again, since pointers and sizes change from call to call, and
 // updates the context's executable graph of that kind in place (hipGraphExecUpdate), which is cheap
 // when the chain has the same shape as the last one; a chain of another shape instantiates anew.
-// Counters and flags are cleared by a kernel (zero_async), never by a memset node.  MBG_GRAPH=0 turns
-// capture off (the launches then go to the stream one by one); results are the same either way.
+// Counters and flags are cleared by a kernel (zero_async), never by a memset node.  Capture is off by
+// default (MBG_GRAPH=1 or mbg_use_graphs turns it on): on this runtime a capture failed when another
+// thread created a stream or allocated while it was open, and the graphs saved no time at any batch
+// size measured.  Results are the same either way.
 
 __global__ void zero_words_kernel(uint32_t *p, int64_t n)
 {
@@ -596,7 +617,7 @@ static int run_graph(mbg_ctx_t *c, hipStream_t st, int kind, F &&enqueue)
 		++c->cnt.direct_chains;
 		return enqueue(st);
 	}
-	HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+	HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
 	const int rc = enqueue(st);
 	hipGraph_t g = nullptr;
 	const hipError_t ec = hipStreamEndCapture(st, &g);
@@ -684,15 +705,24 @@ static void seg_hint(mbg_ctx_t *c, const int32_t *len_host, int32_t n)
 }
 
 // the segment launch, the fix-up, the fallback and the merge (after pass 1 over the reads)
-static int seg_run(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int32_t *len, const SeedArgs &a)
+static int seg_segments(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const SeedArgs &a)
+{
+	const int64_t cs = c->cap_seg;
+	uint32_t *gc = (uint32_t*)c->g_ctr.p;
+	SeedArgs s3 = a;                       // segments k >= 1
+	s3.next = gc; s3.n_tasks = 0xffffffffu; s3.n_tasks_dev = (const uint32_t*)c->g_base.p + n_reads; s3.task_cap = cs;
+	s3.slots = (SeedRec*)c->g_slots.p; s3.slot_cnt = (uint32_t*)c->g_cnt.p; s3.kslot = c->seg_slots;
+	HIPCHK(launch_smem(c->ix, s3, 3, gather_grid(c, cs < n_reads ? cs : n_reads), st));
+	return MBG_OK;
+}
+
+// segments == false: the segment launch already ran (seg_segments, on a side stream)
+static int seg_run(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int32_t *len, const SeedArgs &a, bool segments = true)
 {
 	const int64_t cs = c->cap_seg;
 	uint32_t *gc = (uint32_t*)c->g_ctr.p;
 	const unsigned grid = gather_grid(c, cs < n_reads ? cs : n_reads);
-	SeedArgs s3 = a;                       // segments k >= 1
-	s3.next = gc; s3.n_tasks = 0xffffffffu; s3.n_tasks_dev = (const uint32_t*)c->g_base.p + n_reads; s3.task_cap = cs;
-	s3.slots = (SeedRec*)c->g_slots.p; s3.slot_cnt = (uint32_t*)c->g_cnt.p; s3.kslot = c->seg_slots;
-	HIPCHK(launch_smem(c->ix, s3, 3, grid, st));
+	if (segments) TRY(seg_segments(c, st, n_reads, a));
 	SeedArgs s4 = a;                       // the fix-up of each long read
 	s4.next = gc + 1; s4.n_tasks = 0xffffffffu; s4.n_tasks_dev = gc + 3; s4.task_cap = cs; s4.list = (const uint32_t*)c->g_long.p;
 	s4.slots = (SeedRec*)c->g_fix.p; s4.slot_cnt = (uint32_t*)c->g_fixcnt.p; s4.kslot = c->fix_slots;
@@ -770,10 +800,23 @@ static int seed_phase_pre(mbg_ctx_t *c, hipStream_t st, SeedPass &sp)
 	return MBG_OK;
 }
 
-static int seed_phase_pass1(mbg_ctx_t *c, hipStream_t st, SeedPass &sp)
+// side: long reads' segments k >= 1 run on the context's side stream beside pass 1 proper (the two
+// launches share no data: pass 1 writes the reads' slots and segment 0's handoffs, the segment lanes
+// their own slots and chains), so that the two latency-bound launches overlap; the fix-up waits for both
+static int seed_phase_pass1(mbg_ctx_t *c, hipStream_t st, SeedPass &sp, bool side = false)
 {
+	const bool seg = c->seg_len > 0 && sp.n_reads > 0;
+	if (seg && side && c->side) {
+		HIPCHK(hipEventRecord(c->fork_ev, st));
+		HIPCHK(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+		TRY(seg_segments(c, c->side, sp.n_reads, sp.a));
+		HIPCHK(launch_smem(c->ix, sp.a, 1, gather_grid(c, sp.n_reads), st));
+		HIPCHK(hipEventRecord(c->join_ev, c->side));
+		HIPCHK(hipStreamWaitEvent(st, c->join_ev, 0));
+		return seg_run(c, st, sp.n_reads, sp.len, sp.a, false);
+	}
 	if (sp.n_reads > 0) HIPCHK(launch_smem(c->ix, sp.a, 1, gather_grid(c, sp.n_reads), st));
-	if (c->seg_len > 0 && sp.n_reads > 0) TRY(seg_run(c, st, sp.n_reads, sp.len, sp.a));
+	if (seg) TRY(seg_run(c, st, sp.n_reads, sp.len, sp.a));
 	return MBG_OK;
 }
 
@@ -830,7 +873,7 @@ static int seed_core(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, const int64_
 	}
 	TRY(run_graph(c, st, seg ? kGraphSeedPreSeg : kGraphSeedPre, [&](hipStream_t s) { return seed_phase_pre(c, s, sp); }));
 	if (ev_pre) HIPCHK(hipEventRecord(ev_pre, st));
-	TRY(seed_phase_pass1(c, st, sp));
+	TRY(seed_phase_pass1(c, st, sp, c->seg_side));
 	if (ev) HIPCHK(hipEventRecord(ev[0], st));
 	TRY(run_graph(c, st, kGraphSeedPost, [&](hipStream_t s) { return seed_phase_post(c, s, sp); }));
 	if (ev) HIPCHK(hipEventRecord(ev[1], st));

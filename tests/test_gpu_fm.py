@@ -204,31 +204,34 @@ def _long_reads(ref, rng, n):
     return out
 
 
-@pytest.mark.parametrize("seg", ["512", "100,3,8,2", "64,1,2,1", "0"])
-def test_seed_long_reads_segmented(chrm_prefix, ref_chrm, chrm_reads, seg):
+@pytest.mark.parametrize("seg,side", [("512", "0"), ("512", "1"), ("100,3,8,2", "0"), ("64,1,2,1", "1"), ("0", "0")])
+def test_seed_long_reads_segmented(chrm_prefix, ref_chrm, chrm_reads, seg, side):
     """Long reads: pass 1 splits each read of at least 2*len bases into segments searched in
     parallel, joined to the true chain by a fix-up, with a serial fallback for reads whose segment
     or fix-up tasks run out of room (MBG_SEED_SEG=len,segment slots,chain cap,fix-up slots; "0"
-    is the unsegmented search).  Tiny capacities force the fallback on most reads.  Every read's
-    intervals must equal the reference's mb_seed_intv_batch, mixed with short reads in one batch."""
+    is the unsegmented search).  Tiny capacities force the fallback on most reads.  MBG_SEED_SIDE=1
+    runs the segment lanes on a second stream beside pass 1 proper.  Every read's intervals must
+    equal the reference's mb_seed_intv_batch, mixed with short reads in one batch."""
     from tests.conftest import read_fasta, nt4
     from minibwa_amd import mbg
     _, seqs = read_fasta(os.path.join(GOLDEN, "chrM-human.fa.gz"))
     ref = nt4(seqs[0])
     reads = _long_reads(ref, np.random.default_rng(41), 120) + list(chrm_reads[:200])
-    old = os.environ.get("MBG_SEED_SEG")
-    os.environ["MBG_SEED_SEG"] = seg
+    env = {"MBG_SEED_SEG": seg, "MBG_SEED_SIDE": side}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         with mbg.Context(0) as c:
             c.idx_load(chrm_prefix, 10)
             for min_len, mso in ((19, 10), (11, 4)):
-                print("[seed-long] seg %s min_len %d" % (seg, min_len), flush=True)
+                print("[seed-long] seg %s side %s min_len %d" % (seg, side, min_len), flush=True)
                 g = c.seed_intv_batch(reads, min_len, mso)
                 r = ref_chrm.seed_batch(reads, min_len, mso)
                 for i in range(len(reads)):
-                    assert np.array_equal(_canon(g[i]), _canon(r[i])), (seg, min_len, i, len(reads[i]))
+                    assert np.array_equal(_canon(g[i]), _canon(r[i])), (seg, side, min_len, i, len(reads[i]))
     finally:
-        if old is None:
-            del os.environ["MBG_SEED_SEG"]
-        else:
-            os.environ["MBG_SEED_SEG"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
