@@ -749,7 +749,8 @@ static int seed_alloc(mbg_ctx_t *c, hipStream_t st, int32_t n_reads, int64_t bas
 	const int64_t by_bases = bases > 0 ? bases / 256 : 0;
 	if (c->cap_rec < 8 * (int64_t)n_reads + by_bases + 1024) c->cap_rec = 8 * (int64_t)n_reads + by_bases + 1024;
 	if (c->cap_task < 2 * (int64_t)n_reads + by_bases + 1024) c->cap_task = 2 * (int64_t)n_reads + by_bases + 1024;
-	if (c->cap_spill < (int64_t)n_reads + 4096) c->cap_spill = (int64_t)n_reads + 4096;
+	// a long read's records beyond its pass-1 slots spill: the spill list grows with the bases too
+	if (c->cap_spill < (int64_t)n_reads + by_bases + 4096) c->cap_spill = (int64_t)n_reads + by_bases + 4096;
 	const int64_t rc = c->cap_rec, tc = c->cap_task, sc = c->cap_spill, n_cnt = (int64_t)n_reads + tc;
 	TRY(dgrow(c->b_ctr, 128));
 	TRY(dgrow(c->p_recs, rc * sizeof(SeedRec)));
