@@ -6,7 +6,9 @@
 //   read-ahead thread   gzread (plain files pass through zlib unchanged) into 1 MiB blocks, up to
 //                       kAhead blocks ahead of the parser, so decompression overlaps parsing
 //   parse-ahead thread  kseq's state machine over the blocks, one thread per file, into batches of
-//                       parsed records kept up to kAheadBytes ahead of the caller: a header starts at
+//                       parsed records kept up to kAheadBytes ahead of the caller; a record whose
+//                       four lines lie whole in one block is kept as pointers into that block (the
+//                       batch holds the block), anything else is copied out: a header starts at
 //                       '>' or '@'; the name ends at any isspace() byte and the comment is the rest of
 //                       that line; sequence lines follow (blank lines skipped) until a line starts with
 //                       '>', '@' or '+'; after '+', quality lines are read until the quality is as
@@ -86,12 +88,12 @@ public:
 		gzclose(f_);
 	}
 	// the next block; empty at the end of the stream
-	std::vector<char> next()
+	Bytes next()
 	{
 		std::unique_lock<std::mutex> lk(mu_);
 		cv_.wait(lk, [this] { return !q_.empty() || done_; });
 		if (q_.empty()) return {};
-		std::vector<char> b = std::move(q_.front());
+		Bytes b = std::move(q_.front());
 		q_.pop_front();
 		cv_.notify_all();
 		return b;
@@ -103,7 +105,7 @@ private:
 	{
 		pthread_setname_np(pthread_self(), "fq-read");
 		for (;;) {
-			std::vector<char> b(kBlock);
+			Bytes b(kBlock);   // not zero-filled: gzread writes it
 			const int n = gzread(f_, b.data(), (unsigned)kBlock);
 			std::unique_lock<std::mutex> lk(mu_);
 			if (n < 0) err_ = true;
@@ -123,7 +125,7 @@ private:
 	gzFile f_;
 	std::mutex mu_;
 	std::condition_variable cv_;
-	std::deque<std::vector<char>> q_;
+	std::deque<Bytes> q_;
 	bool done_ = false, stop_ = false, err_ = false;
 	std::thread th_;
 };
@@ -138,7 +140,7 @@ public:
 	int getc()
 	{
 		if (at_end()) { tried_ = true; return -1; }
-		return (uint8_t)cur_[pos_++];
+		return (uint8_t)(*blk_)[pos_++];
 	}
 	// bytes up to the delimiter (consumed; *dret = it, or 0 at the end) appended to the field
 	// v[st ..); sep_line: '\n', and one trailing '\r' of the whole field dropped; else any isspace()
@@ -154,14 +156,14 @@ public:
 		}
 		for (;;) {
 			if (at_end()) { tried_ = true; break; }
-			const char *b = cur_.data() + pos_, *e = cur_.data() + cur_.size(), *hit = nullptr;
+			const char *b = blk_->data() + pos_, *e = blk_->data() + blk_->size(), *hit = nullptr;
 			if (sep_line) hit = (const char*)memchr(b, '\n', (size_t)(e - b));
 			else
 				for (const char *p = b; p < e; ++p)
 					if (*p == ' ' || (*p >= '\t' && *p <= '\r')) { hit = p; break; }
 			const char *stop = hit ? hit : e;
 			v.insert(v.end(), b, stop);
-			pos_ = (size_t)(stop - cur_.data());
+			pos_ = (size_t)(stop - blk_->data());
 			if (hit) {
 				if (dret) *dret = (uint8_t)*hit;
 				++pos_;
@@ -176,25 +178,27 @@ public:
 	bool span(const char **b, const char **e)
 	{
 		if (at_end()) return false;
-		*b = cur_.data() + pos_, *e = cur_.data() + cur_.size();
+		*b = blk_->data() + pos_, *e = blk_->data() + blk_->size();
 		return true;
 	}
 	void skip(size_t n) { pos_ += n; }
+	// the block span() points into (records of the fast path keep it alive)
+	const std::shared_ptr<Bytes> &block() const { return blk_; }
 
 private:
 	bool at_end()
 	{
-		if (pos_ < cur_.size()) return false;
+		if (pos_ < blk_->size()) return false;
 		if (eof_) return true;
-		cur_ = src_->next();
+		blk_ = std::make_shared<Bytes>(src_->next());
 		pos_ = 0;
-		if (cur_.empty()) eof_ = true;
-		total_ += cur_.size();
+		if (blk_->empty()) eof_ = true;
+		total_ += blk_->size();
 		return eof_;
 	}
 	bool end_seen() const { return total_ % kKseqBuf != 0 || tried_; }
 	Source *src_;
-	std::vector<char> cur_;
+	std::shared_ptr<Bytes> blk_ = std::make_shared<Bytes>();
 	size_t pos_ = 0;
 	uint64_t total_ = 0;                        // bytes of the input so far (all of it once eof_)
 	bool eof_ = false, tried_ = false;
@@ -207,12 +211,66 @@ struct Rec {
 	void clear() { name.clear(), comment.clear(), seq.clear(), qual.clear(); }
 };
 
-// one record as kseq_read returns it, its fields appended to r's arrays (from their sizes at the
-// call): >= 0 the sequence length, -1 end of input, -2 a quality string missing or of another length
+// parsed records of one file, in file order.  Record i's four fields (name, comment, sequence,
+// quality) are fld[i].p[k][0 .. fld[i].l[k]): inside one of `blocks` for a record the fast path found
+// whole in a block (no copy), else in f, where the general path appended them (offsets into f until
+// resolve() turns them into pointers, once the batch is complete and f no longer moves).  ret[i] is
+// what kseq_read returned for the record (>= 0 its length, -1 the end, -2 a bad quality; the fields
+// of a -2 record are what was read of it).
+struct Fld {
+	const char *p[4];
+	int64_t l[4];
+};
+struct Parsed {
+	Rec f;
+	std::vector<std::shared_ptr<Bytes>> blocks;
+	std::vector<Fld> fld;
+	std::vector<int64_t> ret;
+	std::vector<size_t> in_f;      // records whose fields are in f
+	size_t field_bytes = 0;
+	size_t n() const { return ret.size(); }
+	size_t bytes() const { return field_bytes + 40 * ret.size(); }
+	void resolve()
+	{
+		const char *base[4] = {f.name.data(), f.comment.data(), f.seq.data(), f.qual.data()};
+		for (size_t i : in_f)
+			for (int k = 0; k < 4; ++k) fld[i].p[k] = base[k] + (uintptr_t)fld[i].p[k];
+	}
+};
+
+// one record as kseq_read returns it, appended to b: >= 0 the sequence length, -1 end of input, -2 a
+// quality string missing or of another length
 class Parser {
 public:
 	explicit Parser(Stream *s) : s_(s) {}
-	int64_t read(Rec &r)
+	int64_t read(Parsed &b)
+	{
+		Fld fd;
+		Rec &r = b.f;
+		const size_t o[4] = {r.name.size(), r.comment.size(), r.seq.size(), r.qual.size()};
+		int64_t ret = -1;
+		int c = 0;
+		if (last_ == 0) {                                          // find the next header
+			while ((c = s_->getc()) != -1 && c != '>' && c != '@') {}
+			if (c != -1) last_ = c;
+		}
+		if (c != -1) {
+			ret = fast(b, fd);
+			if (ret >= 0) {
+				b.fld.push_back(fd);
+				return ret;
+			}
+			ret = general(r);
+		}
+		const size_t e[4] = {r.name.size(), r.comment.size(), r.seq.size(), r.qual.size()};
+		for (int k = 0; k < 4; ++k) fd.p[k] = (const char*)(uintptr_t)o[k], fd.l[k] = (int64_t)(e[k] - o[k]), b.field_bytes += e[k] - o[k];
+		b.in_f.push_back(b.fld.size());
+		b.fld.push_back(fd);
+		return ret;
+	}
+
+private:
+	int64_t general(Rec &r)
 	{
 		int c;
 		const size_t n0 = r.name.size(), c0 = r.comment.size(), s0 = r.seq.size(), q0 = r.qual.size();
@@ -221,8 +279,6 @@ public:
 			if (c == -1) return -1;
 			last_ = c;
 		}
-		const int64_t fl = fast(r);
-		if (fl >= 0) return fl;
 		if (s_->until(false, r.name, n0, &c, false) < 0) return -1;
 		if (c != '\n') s_->until(true, r.comment, c0, nullptr, false);
 		while ((c = s_->getc()) != -1 && c != '>' && c != '+' && c != '@') {
@@ -241,12 +297,11 @@ public:
 		return (int64_t)sl;
 	}
 
-private:
 	// A FASTQ record whose four lines lie whole in the current block, with a one-line sequence and a
 	// one-line quality of the same length and no '\r' at a line end, read at memchr speed.  For such a
 	// record the general path below takes the same fields and leaves the stream at the same byte with
 	// the same state; anything else (returns -1, nothing consumed) goes to the general path.
-	int64_t fast(Rec &r)
+	int64_t fast(Parsed &b_out, Fld &fd)
 	{
 		const char *b, *e;
 		if (!s_->span(&b, &e)) return -1;
@@ -266,10 +321,12 @@ private:
 		while (nm < l1 && !(*nm == ' ' || (*nm >= '\t' && *nm <= '\r'))) ++nm;
 		if (nm < l1 && *nm != '\n' && l1[-1] == '\r') return -1;  // a comment ending in '\r'
 		if (nm == l1 && l1 > b && l1[-1] == '\r') return -1;
-		r.name.insert(r.name.end(), b, nm);
-		if (nm < l1) r.comment.insert(r.comment.end(), nm + 1, l1);
-		r.seq.insert(r.seq.end(), sq, l2);
-		r.qual.insert(r.qual.end(), qs, qs + sl);
+		fd.p[0] = b, fd.l[0] = nm - b;
+		fd.p[1] = nm < l1 ? nm + 1 : l1, fd.l[1] = nm < l1 ? l1 - nm - 1 : 0;
+		fd.p[2] = sq, fd.l[2] = (int64_t)sl;
+		fd.p[3] = qs, fd.l[3] = (int64_t)sl;
+		b_out.field_bytes += (size_t)(fd.l[0] + fd.l[1] + 2 * sl);
+		if (b_out.blocks.empty() || b_out.blocks.back() != s_->block()) b_out.blocks.push_back(s_->block());
 		s_->skip((size_t)(qs + sl + 1 - b));
 		last_ = 0;
 		return (int64_t)sl;
@@ -286,16 +343,6 @@ bool qname_same(const char *a, size_t la, const char *b, size_t lb)   // mb_qnam
 	la = len(a, la), lb = len(b, lb);
 	return la == lb && memcmp(a, b, la) == 0;
 }
-
-// parsed records of one file, in file order: record i's fields are f.name[no[i] .. no[i+1]) and so on,
-// and ret[i] is what kseq_read returned for it (>= 0 its length, -1 the end, -2 a bad quality; the
-// fields of a -2 record are what was read of it)
-struct Parsed {
-	Rec f;
-	std::vector<int64_t> no{0}, co{0}, so{0}, qo{0}, ret;
-	size_t n() const { return ret.size(); }
-	size_t bytes() const { return f.name.size() + f.comment.size() + f.seq.size() + f.qual.size() + 40 * ret.size(); }
-};
 
 // the parse-ahead thread of one file, and the caller's position in what it parsed
 class Ahead {
@@ -336,12 +383,11 @@ private:
 		for (bool end = false; !end;) {
 			auto b = std::make_shared<Parsed>();
 			while (b->n() < kBatchRecs) {
-				const int64_t r = ps_->read(b->f);
+				const int64_t r = ps_->read(*b);
 				b->ret.push_back(r);
-				b->no.push_back((int64_t)b->f.name.size()), b->co.push_back((int64_t)b->f.comment.size());
-				b->so.push_back((int64_t)b->f.seq.size()), b->qo.push_back((int64_t)b->f.qual.size());
 				if (r == -1) { end = true; break; }
 			}
+			b->resolve();
 			std::unique_lock<std::mutex> lk(mu_);
 			cv_.wait(lk, [this] { return bytes_ < kAheadBytes || stop_; });
 			if (stop_) return;
@@ -368,8 +414,8 @@ struct Take {
 	size_t i;
 	bool q, c;
 	int64_t len() const { return b->ret[i]; }
-	const char *name() const { return b->f.name.data() + b->no[i]; }
-	size_t name_len() const { return (size_t)(b->no[i + 1] - b->no[i]); }
+	const char *name() const { return b->fld[i].p[0]; }
+	size_t name_len() const { return (size_t)b->fld[i].l[0]; }
 };
 
 } // namespace
@@ -402,10 +448,9 @@ void assemble(mbg_fq_buf_s &b, const std::vector<Take> &t, int n_threads)
 	b.name_off.resize(n + 1), b.com_off.resize(n + 1), b.seq_off.resize(n + 1), b.has_qual.resize(n);
 	b.name_off[0] = b.com_off[0] = b.seq_off[0] = 0;
 	for (size_t k = 0; k < n; ++k) {
-		const Parsed &p = *t[k].b;
-		const size_t i = t[k].i;
-		const int64_t sl = p.so[i + 1] - p.so[i], cl = p.co[i + 1] - p.co[i], ql = p.qo[i + 1] - p.qo[i];
-		b.name_off[k + 1] = b.name_off[k] + (p.no[i + 1] - p.no[i]);
+		const Fld &fd = t[k].b->fld[t[k].i];
+		const int64_t sl = fd.l[2], cl = fd.l[1], ql = fd.l[3];
+		b.name_off[k + 1] = b.name_off[k] + fd.l[0];
 		b.com_off[k + 1] = b.com_off[k] + (t[k].c && cl > 0 ? cl : 0);
 		b.seq_off[k + 1] = b.seq_off[k] + sl;
 		b.has_qual[k] = t[k].q && ql > 0;
@@ -415,21 +460,21 @@ void assemble(mbg_fq_buf_s &b, const std::vector<Take> &t, int n_threads)
 	const uint8_t *tab = nt4_table();
 	auto work = [&](size_t lo, size_t hi) {
 		for (size_t k = lo; k < hi; ++k) {
-			const Parsed &p = *t[k].b;
-			const size_t i = t[k].i;
+			const Fld &fd = t[k].b->fld[t[k].i];
 			const int64_t sl = b.seq_off[k + 1] - b.seq_off[k];
-			memcpy(&b.f.name[(size_t)b.name_off[k]], &p.f.name[(size_t)p.no[i]], (size_t)(b.name_off[k + 1] - b.name_off[k]));
+			if (b.name_off[k + 1] > b.name_off[k])
+				memcpy(&b.f.name[(size_t)b.name_off[k]], fd.p[0], (size_t)(b.name_off[k + 1] - b.name_off[k]));
 			if (b.com_off[k + 1] > b.com_off[k])
-				memcpy(&b.f.comment[(size_t)b.com_off[k]], &p.f.comment[(size_t)p.co[i]], (size_t)(b.com_off[k + 1] - b.com_off[k]));
-			char *sq = &b.f.seq[(size_t)b.seq_off[k]];
-			uint8_t *e = &b.nt4[(size_t)b.seq_off[k]];
-			const char *src = &p.f.seq[(size_t)p.so[i]];
+				memcpy(&b.f.comment[(size_t)b.com_off[k]], fd.p[1], (size_t)(b.com_off[k + 1] - b.com_off[k]));
+			char *sq = sl > 0 ? &b.f.seq[(size_t)b.seq_off[k]] : nullptr;
+			uint8_t *e = sl > 0 ? &b.nt4[(size_t)b.seq_off[k]] : nullptr;
+			const char *src = fd.p[2];
 			for (int64_t l = 0; l < sl; ++l) {
 				char c = src[l];
 				if (c == 'u' || c == 'U') --c;
 				sq[l] = c, e[l] = tab[(uint8_t)c];
 			}
-			if (b.has_qual[k]) memcpy(&b.f.qual[(size_t)b.seq_off[k]], &p.f.qual[(size_t)p.qo[i]], (size_t)sl);
+			if (b.has_qual[k]) memcpy(&b.f.qual[(size_t)b.seq_off[k]], fd.p[3], (size_t)sl);
 			else if (sl > 0) memset(&b.f.qual[(size_t)b.seq_off[k]], 0, (size_t)sl);
 		}
 	};
