@@ -3,8 +3,9 @@
 // written into flat arrays instead of one allocation per field, with the bases also encoded to nt4
 // (kom_nt4_table, kommon.c:162-179) for upload.
 //
-//   read-ahead thread   gzread (plain files pass through zlib unchanged) into 1 MiB blocks, up to
-//                       kAhead blocks ahead of the parser, so decompression overlaps parsing
+//   read-ahead thread   gzread into 1 MiB blocks, up to kAhead blocks ahead of the parser, so
+//                       decompression overlaps parsing.  An uncompressed regular file is mapped
+//                       instead and read in place (like any mapping, it must not shrink while read)
 //   parse-ahead thread  kseq's state machine over the blocks, one thread per file, into batches of
 //                       parsed records kept up to kAheadBytes ahead of the caller; a record whose
 //                       four lines lie whole in one block is kept as pointers into that block (the
@@ -23,7 +24,11 @@
 // lets it run ahead: the records, in file order, are kseq_read's whatever the chunks are.
 //
 // Host code only; no GPU call.
+#include <fcntl.h>
 #include <pthread.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -73,12 +78,27 @@ struct NoInit : std::allocator<T> {
 };
 using Bytes = std::vector<char, NoInit<char>>;
 
-// blocks of the decompressed stream, produced by a reader thread
+// a block of the input: bytes p[0 .. n), kept alive by `keep` (the block's own buffer, or the
+// mapping of a whole uncompressed file)
+struct Block {
+	std::shared_ptr<const char> keep;
+	const char *p = nullptr;
+	size_t n = 0;
+	const char *data() const { return p; }
+	size_t size() const { return n; }
+	bool empty() const { return n == 0; }
+	char operator[](size_t i) const { return p[i]; }
+};
+
+// blocks of the decompressed stream, produced by a reader thread; an uncompressed regular file is
+// mapped instead, and its blocks are views of the mapping (no copy, no thread)
 class Source {
 public:
 	explicit Source(gzFile f) : f_(f), th_([this] { run(); }) {}
+	Source(std::shared_ptr<const char> map, size_t size) : map_(std::move(map)), map_size_(size) {}
 	~Source()
 	{
+		if (!f_) return;
 		{
 			std::lock_guard<std::mutex> lk(mu_);
 			stop_ = true;
@@ -88,14 +108,24 @@ public:
 		gzclose(f_);
 	}
 	// the next block; empty at the end of the stream
-	Bytes next()
+	Block next()
 	{
+		if (!f_) {
+			Block b;
+			if (map_off_ < map_size_) {
+				b.p = map_.get() + map_off_, b.n = std::min(kBlock, map_size_ - map_off_), b.keep = map_;
+				map_off_ += b.n;
+			}
+			return b;
+		}
 		std::unique_lock<std::mutex> lk(mu_);
 		cv_.wait(lk, [this] { return !q_.empty() || done_; });
 		if (q_.empty()) return {};
-		Bytes b = std::move(q_.front());
+		auto own = std::make_shared<Bytes>(std::move(q_.front()));
 		q_.pop_front();
 		cv_.notify_all();
+		Block b;
+		b.p = own->data(), b.n = own->size(), b.keep = std::shared_ptr<const char>(own, own->data());
 		return b;
 	}
 	bool failed() const { return err_; }
@@ -122,13 +152,34 @@ private:
 		done_ = true;
 		cv_.notify_all();
 	}
-	gzFile f_;
+	gzFile f_ = nullptr;
+	std::shared_ptr<const char> map_;
+	size_t map_size_ = 0, map_off_ = 0;
 	std::mutex mu_;
 	std::condition_variable cv_;
 	std::deque<Bytes> q_;
 	bool done_ = false, stop_ = false, err_ = false;
 	std::thread th_;
 };
+
+// an uncompressed regular file mapped whole, or null (then the file goes through zlib)
+std::shared_ptr<const char> map_plain(const char *path, size_t *size)
+{
+	const int fd = open(path, O_RDONLY);
+	if (fd < 0) return nullptr;
+	struct stat sb;
+	unsigned char magic[2] = {0, 0};
+	void *m = MAP_FAILED;
+	if (fstat(fd, &sb) == 0 && S_ISREG(sb.st_mode) && sb.st_size > 0 && pread(fd, magic, 2, 0) == 2 &&
+	    !(magic[0] == 0x1f && magic[1] == 0x8b))
+		m = mmap(nullptr, (size_t)sb.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+	close(fd);
+	if (m == MAP_FAILED) return nullptr;
+	madvise(m, (size_t)sb.st_size, MADV_SEQUENTIAL);
+	*size = (size_t)sb.st_size;
+	const size_t len = *size;
+	return std::shared_ptr<const char>((const char*)m, [len](const char *q) { munmap((void*)q, len); });
+}
 
 // kseq's byte stream over the blocks: getc, and reads up to a delimiter.  kseq notices the end of its
 // input when a read returns fewer than 16 KiB; an input whose length is a multiple of that is noticed
@@ -140,7 +191,7 @@ public:
 	int getc()
 	{
 		if (at_end()) { tried_ = true; return -1; }
-		return (uint8_t)(*blk_)[pos_++];
+		return (uint8_t)blk_[pos_++];
 	}
 	// bytes up to the delimiter (consumed; *dret = it, or 0 at the end) appended to the field
 	// v[st ..); sep_line: '\n', and one trailing '\r' of the whole field dropped; else any isspace()
@@ -156,14 +207,14 @@ public:
 		}
 		for (;;) {
 			if (at_end()) { tried_ = true; break; }
-			const char *b = blk_->data() + pos_, *e = blk_->data() + blk_->size(), *hit = nullptr;
+			const char *b = blk_.data() + pos_, *e = blk_.data() + blk_.size(), *hit = nullptr;
 			if (sep_line) hit = (const char*)memchr(b, '\n', (size_t)(e - b));
 			else
 				for (const char *p = b; p < e; ++p)
 					if (*p == ' ' || (*p >= '\t' && *p <= '\r')) { hit = p; break; }
 			const char *stop = hit ? hit : e;
 			v.insert(v.end(), b, stop);
-			pos_ = (size_t)(stop - blk_->data());
+			pos_ = (size_t)(stop - blk_.data());
 			if (hit) {
 				if (dret) *dret = (uint8_t)*hit;
 				++pos_;
@@ -178,27 +229,27 @@ public:
 	bool span(const char **b, const char **e)
 	{
 		if (at_end()) return false;
-		*b = blk_->data() + pos_, *e = blk_->data() + blk_->size();
+		*b = blk_.data() + pos_, *e = blk_.data() + blk_.size();
 		return true;
 	}
 	void skip(size_t n) { pos_ += n; }
-	// the block span() points into (records of the fast path keep it alive)
-	const std::shared_ptr<Bytes> &block() const { return blk_; }
+	// what keeps the block span() points into alive (records of the fast path hold it)
+	const std::shared_ptr<const char> &block() const { return blk_.keep; }
 
 private:
 	bool at_end()
 	{
-		if (pos_ < blk_->size()) return false;
+		if (pos_ < blk_.size()) return false;
 		if (eof_) return true;
-		blk_ = std::make_shared<Bytes>(src_->next());
+		blk_ = src_->next();
 		pos_ = 0;
-		if (blk_->empty()) eof_ = true;
-		total_ += blk_->size();
+		if (blk_.empty()) eof_ = true;
+		total_ += blk_.size();
 		return eof_;
 	}
 	bool end_seen() const { return total_ % kKseqBuf != 0 || tried_; }
 	Source *src_;
-	std::shared_ptr<Bytes> blk_ = std::make_shared<Bytes>();
+	Block blk_;
 	size_t pos_ = 0;
 	uint64_t total_ = 0;                        // bytes of the input so far (all of it once eof_)
 	bool eof_ = false, tried_ = false;
@@ -223,7 +274,7 @@ struct Fld {
 };
 struct Parsed {
 	Rec f;
-	std::vector<std::shared_ptr<Bytes>> blocks;
+	std::vector<std::shared_ptr<const char>> blocks;
 	std::vector<Fld> fld;
 	std::vector<int64_t> ret;
 	std::vector<size_t> in_f;      // records whose fields are in f
@@ -517,11 +568,19 @@ int mbg_fq_open(const char *path, int32_t n_threads, mbg_fq_t **out)
 {
 	if (out == nullptr) return MBG_EARG;
 	*out = nullptr;
-	gzFile f = path && strcmp(path, "-") != 0 ? gzopen(path, "r") : gzdopen(0, "r");
-	if (f == nullptr) return MBG_EIO;
-	gzbuffer(f, 1 << 20);
+	const bool named = path && strcmp(path, "-") != 0;
+	size_t map_size = 0;
+	std::shared_ptr<const char> map = named ? map_plain(path, &map_size) : nullptr;
+	Source *src = nullptr;
+	if (map) src = new Source(std::move(map), map_size);
+	else {
+		gzFile f = named ? gzopen(path, "r") : gzdopen(0, "r");
+		if (f == nullptr) return MBG_EIO;
+		gzbuffer(f, 1 << 20);
+		src = new Source(f);
+	}
 	mbg_fq_s *q = new mbg_fq_s;
-	q->src = new Source(f);
+	q->src = src;
 	q->st = new Stream(q->src);
 	q->ps = new Parser(q->st);
 	q->ah = new Ahead(q->ps);
