@@ -81,8 +81,8 @@ struct mbg_ctx_s {
 	hipStream_t ord_st = nullptr;
 	bool ord_pending = false;
 	// launch chains captured as graphs (run_graph): one executable graph per chain kind; off unless
-	// MBG_GRAPH=1 or mbg_use_graphs (a capture fails when another thread of the process allocates
-	// device memory or creates a stream meanwhile, which the mapper's front and DP threads do)
+	// MBG_GRAPH=1 or mbg_use_graphs (a capture was invalidated when another thread of the process
+	// created a stream meanwhile; the mapper's other threads also allocate device memory while it runs)
 	bool use_graphs = false;
 	hipGraphExec_t graph_exec[kGraphKinds] = {};
 	// long reads' segment lanes on a side stream beside pass 1 proper (MBG_SEED_SIDE=1 turns it on: it
@@ -589,9 +589,9 @@ static const int32_t kSlot1 = 16, kSlot2 = 8;
 // updates the context's executable graph of that kind in place (hipGraphExecUpdate), which is cheap
 // when the chain has the same shape as the last one; a chain of another shape instantiates anew.
 // Counters and flags are cleared by a kernel (zero_async), never by a memset node.  Capture is off by
-// default (MBG_GRAPH=1 or mbg_use_graphs turns it on): on this runtime a capture failed when another
-// thread created a stream or allocated while it was open, and the graphs saved no time at any batch
-// size measured.  Results are the same either way.
+// default (MBG_GRAPH=1 or mbg_use_graphs turns it on): on this runtime a capture was invalidated when
+// another thread created a stream while it was open, and the graphs saved no time at any batch size
+// measured.  Results are the same either way.
 
 __global__ void zero_words_kernel(uint32_t *p, int64_t n)
 {
@@ -2169,7 +2169,7 @@ int mbg_map_step_dev(mbg_ctx_t *c, void *stream, int32_t n_reads, const int64_t 
 		stats->occ_blocks[0] = hc[8]; stats->kmer_lookups[0] = hc[9]; stats->extend_steps[0] = hc[10];
 		stats->occ_blocks[1] = hc[11]; stats->kmer_lookups[1] = hc[12]; stats->extend_steps[1] = hc[13]; stats->lf_steps = hc[14];
 		float ms;
-		if (timed) {              // the phases: only when the chain ran launch by launch (MBG_GRAPH=0)
+		if (timed) {              // the phases: only when the chain ran launch by launch (graphs off, the default)
 			HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); stats->ms_seed1 = ms;
 			HIPCHK(hipEventElapsedTime(&ms, c->ev[1], c->ev[2])); stats->ms_seed2 = ms;
 			HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3])); stats->ms_rows = ms;
