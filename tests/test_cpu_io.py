@@ -174,3 +174,24 @@ def test_mixed_records_across_blocks(tmp_path, gz):
     _write(f, data, gz)
     assert _compare(f, PARAMS) > 39000
     assert _compare(f, [dict(chunk_size=2_000_000, frag_mode=True, min_cnt=100)]) > 39000
+
+
+@pytest.mark.parametrize("gz", [False, True])
+def test_empty_and_block_sized_inputs(tmp_path, gz):
+    """An empty file, and inputs whose length is exactly kseq's 16 KiB read size or the reader's 1 MiB
+    block (uncompressed files are read in place, in views of that size), ending on a record boundary
+    and one byte into a header."""
+    rng = np.random.default_rng(5)
+    cases = [b""]
+    for size in (16384, 1 << 20, 2 << 20):
+        body = _mixed_records(rng, size // 60)[:size - 64]
+        body = body[:body.rfind(b"\n@") + 1] if b"\n@" in body else body
+        pad = size - len(body)
+        tail = b"@" + b"p" * (pad - 8) + b"\nA\n+\nI\n"    # a last record that ends the file exactly
+        assert len(body + tail) == size
+        cases.append(body + tail)
+        cases.append((body + tail)[:-1] + b"@")          # the last byte starts a header
+    for k, data in enumerate(cases):
+        f = str(tmp_path / (f"s{k}.fq" + (".gz" if gz else "")))
+        _write(f, data, gz)
+        _compare(f, PARAMS)
